@@ -1,0 +1,43 @@
+"""numpy record layouts of the C-ABI structs in include/fluidity_hip.h (byte for byte).
+
+They mirror the reference PODs: Vec3/Mat3/BlockIndex3D (native/include/fluidity/base/mac_types_3d.h:9-72),
+MacTransferParticle3D / MacFaceDepositQuanta3D / MacParticleGatherResult3D
+(native/include/fluidity/base/mac_transfer_types_3d.h:7-37), LiquidParticleSortEntry3D
+(native/include/fluidity/host/liquid_step_workspace_3d.h:14-17) and the flattened
+MacPressureCell3D / MacPressureFace3D (native/include/fluidity/host/mac_pressure_3d.h:14-51).
+"""
+import numpy as np
+
+VEC3 = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8")])
+MAT3 = np.dtype([("x_row", VEC3), ("y_row", VEC3), ("z_row", VEC3)])
+INT3 = np.dtype([("x", "i4"), ("y", "i4"), ("z", "i4")])
+GEOMETRY = np.dtype([("world_origin", VEC3), ("cell_width", "f8")])
+PARTICLE = np.dtype([("sorted_world_position", VEC3), ("trial_world_position", VEC3), ("particle_mass", "f8"),
+                     ("particle_velocity", VEC3), ("particle_velocity_slopes", MAT3)])
+QUANTIZATION = np.dtype([("mass_per_quantum", "f8"), ("pure_momentum_per_quantum", "f8"),
+                         ("declared_particle_capacity", "u8")])
+FACE_QUANTA = np.dtype([("mass_quanta_sum", "i8"), ("pure_momentum_quanta_sum", "i8"),
+                        ("stress_force_quanta_sum", "i8")])
+GATHER_RESULT = np.dtype([("new_pic_velocity", VEC3), ("grid_velocity_increment", VEC3),
+                          ("fitted_velocity_slopes", MAT3), ("velocity_gradient", MAT3)])
+SORT_ENTRY = np.dtype([("bin_index", INT3), ("pad", "u4"), ("original_ordinal", "u8")])
+PRESSURE_CELL = np.dtype([("cell_index", INT3), ("pad", "i4"), ("fluid_volume", "f8"), ("target_divergence", "f8"),
+                          ("bulk_modulus", "f8"), ("previous_pressure", "f8"), ("integrated_inward_wall_flux", "f8")])
+PRESSURE_FACE = np.dtype([("axis", "i4"), ("lattice", INT3), ("open_area", "f8"), ("inverse_density", "f8"),
+                          ("pressure_gradient_distance", "f8"), ("predicted_normal_velocity", "f8"),
+                          ("wall_normal_velocity", "f8"), ("link_kind", "i4"), ("has_prescribed_wall_flux", "i4"),
+                          ("lift", "f8"), ("prescribed_wall_flux", "f8")])
+PRESSURE_OPTIONS = np.dtype([("max_iterations", "u8"), ("cpu_residual_tolerance_scale", "f8"),
+                             ("relative_divergence_target", "f8"), ("levels", "i4"), ("smoothing_sweeps", "i4"),
+                             ("coarse_sweeps", "i4"), ("fp32_vcycle", "i4"), ("coarse_scale", "f8")])
+PRESSURE_DIAGNOSTICS = np.dtype([("iteration_count", "u8"), ("multigrid_level_count", "u8"),
+                                 ("true_residual_l2", "f8"), ("max_volume_scaled_residual", "f8"),
+                                 ("max_cell_constraint_defect", "f8"), ("rhs_l2", "f8"),
+                                 ("initial_defect_l2", "f8"), ("defect_l2", "f8"), ("defect_linf", "f8"),
+                                 ("restarts", "u8"), ("gauge_count", "u8")])
+
+SEALED, INTERNAL, DIRICHLET = 0, 1, 2
+LINK_KINDS = {"sealed": SEALED, "internal": INTERNAL, "dirichlet": DIRICHLET}
+
+assert PARTICLE.itemsize == 152 and GATHER_RESULT.itemsize == 192 and SORT_ENTRY.itemsize == 24
+assert PRESSURE_CELL.itemsize == 56 and PRESSURE_FACE.itemsize == 80 and FACE_QUANTA.itemsize == 24

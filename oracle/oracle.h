@@ -1,0 +1,143 @@
+// CPU oracle for the FluidiTy liquid-step hot path (P2G, G2P, MAC pressure projection).
+//
+// TEST INFRASTRUCTURE ONLY. This library is the checker: it is linked or loaded by
+// tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and by nothing
+// that ships. The product path (fluidity_amd/) never calls it.
+//
+// It is an FP64 C++ restatement of the reference algorithms, with the reference's
+// particle/cell caps lifted (callers pass the cap they want). Every function cites the
+// reference file:line it follows. Built with -ffp-contract=off -fno-fast-math so the
+// rounding sequence equals the reference's (cmake/NativeOptions.cmake:25).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+extern "C" {
+
+// Status codes equal absl::StatusCode numbering (absl/status/status.h).
+enum flo_status {
+  FLO_OK = 0,
+  FLO_INVALID_ARGUMENT = 3,
+  FLO_RESOURCE_EXHAUSTED = 8,
+  FLO_FAILED_PRECONDITION = 9,
+  FLO_ABORTED = 10,
+  FLO_OUT_OF_RANGE = 11,
+  FLO_INTERNAL = 13,
+};
+
+// Byte-for-byte mirrors of native/include/fluidity/base/mac_types_3d.h:9-72 and
+// mac_transfer_types_3d.h:7-37.
+struct flo_vec3 { double x, y, z; };
+struct flo_mat3 { flo_vec3 x_row, y_row, z_row; };
+struct flo_int3 { int32_t x, y, z; };
+struct flo_geometry { flo_vec3 world_origin; double cell_width; };
+struct flo_face_index { uint8_t axis; flo_int3 lattice; };  // 16 B, lattice at offset 4
+struct flo_particle {
+  flo_vec3 sorted_world_position;
+  flo_vec3 trial_world_position;
+  double particle_mass;
+  flo_vec3 particle_velocity;
+  flo_mat3 particle_velocity_slopes;
+};  // 152 B
+struct flo_quantization { double mass_per_quantum; double pure_momentum_per_quantum; uint64_t declared_particle_capacity; };
+struct flo_face_deposit { double deposited_mass; double deposited_pure_momentum; };
+struct flo_face_quanta { int64_t mass_quanta_sum; int64_t pure_momentum_quanta_sum; int64_t stress_force_quanta_sum; };
+struct flo_gather_result { flo_vec3 new_pic_velocity; flo_vec3 grid_velocity_increment; flo_mat3 fitted_velocity_slopes; flo_mat3 velocity_gradient; };
+struct flo_stencil_sample { uint64_t face_slot; double weight; flo_vec3 face_offset_from_particle; flo_vec3 weight_gradient; };
+struct flo_sort_entry { flo_int3 bin_index; uint32_t pad; uint64_t original_ordinal; };  // 24 B, as LiquidParticleSortEntry3D
+
+// Flattened MacPressureCell3D / MacPressureFace3D (native/include/fluidity/host/mac_pressure_3d.h:14-51).
+// link_kind: 0 sealed, 1 internal (lift = jump), 2 Dirichlet (lift = exterior pressure).
+struct flo_pressure_cell {
+  flo_int3 cell_index;
+  int32_t pad;
+  double fluid_volume;
+  double target_divergence;
+  double bulk_modulus;  // +inf selects zero compliance
+  double previous_pressure;
+  double integrated_inward_wall_flux;
+};  // 56 B
+struct flo_pressure_face {
+  int32_t axis;
+  flo_int3 lattice;
+  double open_area;
+  double inverse_density;
+  double pressure_gradient_distance;
+  double predicted_normal_velocity;
+  double wall_normal_velocity;
+  int32_t link_kind;
+  int32_t has_prescribed_wall_flux;
+  double lift;
+  double prescribed_wall_flux;
+};  // 80 B
+
+struct flo_pressure_options {
+  uint64_t max_iterations;             // PCG iteration budget
+  double cpu_residual_tolerance_scale; // reference option, [1e-12, 1]
+  double relative_divergence_target;   // 0 disables; else also require ||defect||/||defect(p=0)|| <= target
+  int32_t levels;                      // geometric levels (>= 1)
+  int32_t smoothing_sweeps;            // RBGS sweeps before and after the coarse correction
+  int32_t coarse_sweeps;               // symmetric RBGS sweeps on the coarsest level
+  int32_t fp32_vcycle;                 // 1: V-cycle and CG vectors in FP32 (mixed), 0: all FP64
+  double coarse_scale;                 // Galerkin coarse operator scale (1 = P^T A P)
+};
+
+struct flo_pressure_diagnostics {
+  uint64_t iteration_count;
+  uint64_t multigrid_level_count;
+  double true_residual_l2;
+  double max_volume_scaled_residual;
+  double max_cell_constraint_defect;
+  double rhs_l2;
+  double initial_defect_l2;
+  double defect_l2;
+  double defect_linf;
+  uint64_t restarts;
+  uint64_t gauge_count;
+};
+
+// ---- grid (sparse_mac_grid_3d.cc) ----
+int flo_compute_bin(const flo_geometry* geometry, flo_vec3 position, flo_int3* out_bin);
+int flo_grid_sort_blocks(const flo_geometry* geometry, const flo_int3* blocks, size_t count, flo_int3* out_sorted);
+int flo_face_slot(const flo_int3* sorted_blocks, size_t count, flo_face_index face, uint64_t* out_slot);
+int flo_slot_face(const flo_int3* sorted_blocks, size_t count, uint64_t slot, flo_face_index* out_face);
+int flo_face_world_anchor(const flo_geometry* geometry, flo_face_index face, flo_vec3* out);
+int flo_owned_stencil(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t count, int axis,
+                      flo_vec3 sorted_position, flo_vec3 trial_position, flo_stencil_sample* out27);
+
+// ---- transfers (transfers_3d.cc) ----
+int flo_deposit(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count,
+                const flo_particle* particles, size_t n, flo_face_deposit* out, size_t face_count);
+// max_capacity: the reference limit 2^20 (transfers_3d.cc:22) or any larger value for the uncapped form.
+int flo_deposit_quanta(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count,
+                       const flo_particle* particles, size_t n, flo_quantization quantization,
+                       uint64_t max_capacity, flo_face_quanta* out, size_t face_count, int threads);
+int flo_gather(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count,
+               const flo_particle* particles, size_t n, const double* old_faces, const double* new_faces,
+               size_t face_count, flo_gather_result* out, int threads);
+int flo_normalize(const flo_face_quanta* quanta, size_t face_count, flo_quantization quantization,
+                  double minimum_face_mass, double* out_mass, double* out_velocity, uint8_t* out_validity);
+int flo_apply_forces(const flo_int3* sorted_blocks, size_t block_count, const double* old_velocities,
+                     size_t face_count, flo_vec3 acceleration, double dt, double* out_predicted);
+
+// ---- topology (liquid_step_workspace_3d.cc:171-300) ----
+int flo_sort_particles(const flo_geometry* geometry, const flo_vec3* positions, size_t n, flo_sort_entry* out);
+// Writes the sorted unique active blocks; *io_count holds the capacity on entry, the count on exit.
+int flo_activate_blocks(const flo_geometry* geometry, const flo_vec3* positions, size_t n,
+                        const flo_int3* pressure_cells, size_t cell_count, flo_int3* out_blocks, size_t* io_count);
+
+// ---- pressure (mac_pressure_3d_arithmetic.h, mac_pressure_workspace_3d.cc) ----
+// Outputs may be null. gauges receives up to cell_count ordinals.
+int flo_project(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count, double dt,
+                const flo_pressure_cell* cells, size_t cell_count, const flo_pressure_face* faces, size_t face_count,
+                const flo_pressure_options* options, double* out_pressures, double* out_velocities,
+                double* out_rhs, double* out_residuals, double* out_defects, uint64_t* out_gauges,
+                flo_pressure_diagnostics* out_diagnostics);
+// Evaluate a supplied pressure vector with the FP64 original operator (workspace.cc:399-461).
+int flo_evaluate(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count, double dt,
+                 const flo_pressure_cell* cells, size_t cell_count, const flo_pressure_face* faces, size_t face_count,
+                 const double* pressures, double* out_velocities, double* out_residuals, double* out_defects,
+                 flo_pressure_diagnostics* out_diagnostics);
+
+}  // extern "C"

@@ -1,0 +1,795 @@
+// Oracle: MAC pressure projection (test infrastructure only).
+//
+// Preparation restates native/src/mac_pressure_3d_arithmetic.h:110-399 (FreezeCell,
+// FreezeLinkFromIncidence, AssembleIntegratedRhsInto with Neumaier sums, SelectGaugesInto)
+// and mac_pressure_workspace_3d.cc:68-166 (Prepare). Evaluate restates
+// mac_pressure_3d_arithmetic.h:404-503 and mac_pressure_workspace_3d.cc:399-461. The
+// stopping rule restates mac_pressure_3d_multigrid.cc:303-367. The outer mixed-precision
+// PCG follows mac_pressure_workspace_3d.cc:544-658 (FP32 vectors, FP64 dots, FP64 restart).
+//
+// The preconditioner is the geometric V-cycle the HIP path runs (2x2x2 aggregation with
+// piecewise-constant prolongation, restriction R = P^T, Galerkin crossing-sum coarse
+// operators, red-black Gauss-Seidel smoothing), so the oracle times and checks the same
+// algorithm. The reference's own preconditioner (aggregation + weighted Jacobi + dense
+// Cholesky, mac_pressure_workspace_3d.cc:298-397) converges to the same acceptance gates;
+// parity is judged on the accepted projection, not iterate by iterate.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <unordered_map>
+#include <vector>
+
+#include "grid_internal.h"
+
+using namespace flo;
+
+namespace {
+
+constexpr double kEps = std::numeric_limits<double>::epsilon();
+constexpr double kCompatibilityRoundoff = 32.0 * kEps;
+constexpr double kIdentityRoundoff = 64.0 * kEps;
+constexpr size_t kNone = SIZE_MAX;
+
+bool FinitePositive(double v) { return std::isfinite(v) && v > 0.0; }
+
+// mac_pressure_3d_arithmetic.h:33-49
+bool Representable(double a, double b, double p) {
+  if (!std::isfinite(p)) return false;
+  if (a == 0.0 || b == 0.0 || std::fabs(p) >= std::numeric_limits<double>::min()) return true;
+  if (p == 0.0) return false;
+  int ea = 0, eb = 0;
+  const double ma = std::frexp(a, &ea);
+  const double mb = std::frexp(b, &eb);
+  const double np = std::scalbn(p, -(ea + eb));
+  return std::fma(ma, mb, -np) == 0.0;
+}
+
+// Neumaier summation (mac_pressure_3d_arithmetic.h:57-74)
+struct Comp {
+  double sum = 0.0, corr = 0.0;
+  void Add(double t) {
+    const double n = sum + t;
+    if (std::fabs(sum) >= std::fabs(t))
+      corr += (sum - n) + t;
+    else
+      corr += (t - n) + sum;
+    sum = n;
+  }
+  double Total() const { return sum + corr; }
+};
+struct FluxRow {
+  Comp flux, abs;
+  void Add(double t) {
+    flux.Add(t);
+    abs.Add(std::fabs(t));
+  }
+};
+
+struct Cell {
+  flo_int3 index;
+  double volume, target_flux, wall_flux, compliance, previous;
+};
+struct Link {
+  uint64_t slot;
+  size_t tail, head;  // kNone when absent
+  double open_area, blocked_area, scale, conductance, predicted, wall_flux;
+  int kind;
+  double lift;
+};
+
+struct System {
+  double dt = 0;
+  std::vector<Cell> cells;
+  std::vector<Link> links;
+  std::vector<double> rhs, rhs_abs;
+  std::vector<size_t> component, gauges;
+};
+
+bool CellLess(const flo_int3& a, const flo_int3& b) { return BlockLess(a, b); }
+
+size_t FindCell(const std::vector<Cell>& cells, flo_int3 idx) {
+  auto it = std::lower_bound(cells.begin(), cells.end(), idx,
+                             [](const Cell& c, const flo_int3& k) { return CellLess(c.index, k); });
+  if (it == cells.end() || !BlockEq(it->index, idx)) return kNone;
+  return static_cast<size_t>(it - cells.begin());
+}
+
+template <typename F>
+void ForIncident(const Link& l, F f) {
+  if (l.tail != kNone) f(l.tail, 1.0);
+  if (l.head != kNone) f(l.head, -1.0);
+}
+
+int FreezeCell(const flo_pressure_cell& c, double dt, double full_volume, Cell& out) {
+  if (!FinitePositive(c.fluid_volume) || c.fluid_volume > full_volume) return FLO_INVALID_ARGUMENT;
+  if (!std::isfinite(c.target_divergence) || !std::isfinite(c.previous_pressure) ||
+      !std::isfinite(c.integrated_inward_wall_flux))
+    return FLO_INVALID_ARGUMENT;
+  double compliance = 0.0;
+  if (c.bulk_modulus != std::numeric_limits<double>::infinity()) {
+    if (!FinitePositive(c.bulk_modulus)) return FLO_INVALID_ARGUMENT;
+    const double kdt = c.bulk_modulus * dt;
+    if (!FinitePositive(kdt)) return FLO_OUT_OF_RANGE;
+    compliance = c.fluid_volume / kdt;
+    if (!FinitePositive(compliance)) return FLO_OUT_OF_RANGE;
+  }
+  const double tf = c.fluid_volume * c.target_divergence;
+  if (!Representable(c.fluid_volume, c.target_divergence, tf)) return FLO_OUT_OF_RANGE;
+  out = {c.cell_index, c.fluid_volume, tf, c.integrated_inward_wall_flux, compliance, c.previous_pressure};
+  return FLO_OK;
+}
+
+// mac_pressure_3d_arithmetic.h:174-245
+int FreezeLink(const flo_pressure_face& f, uint64_t slot, double full_area, double dt, size_t tail, size_t head,
+               Link& out) {
+  if (!std::isfinite(f.open_area) || f.open_area < 0.0 || f.open_area > full_area) return FLO_INVALID_ARGUMENT;
+  if (!std::isfinite(f.inverse_density) || !std::isfinite(f.pressure_gradient_distance) ||
+      !std::isfinite(f.predicted_normal_velocity) || !std::isfinite(f.wall_normal_velocity))
+    return FLO_INVALID_ARGUMENT;
+  const bool both = tail != kNone && head != kNone;
+  const bool one = (tail != kNone) != (head != kNone);
+  if (f.link_kind == 1) {
+    if (!both || !std::isfinite(f.lift)) return FLO_INVALID_ARGUMENT;
+  } else if (f.link_kind == 2) {
+    if (!one || f.open_area == 0.0 || !std::isfinite(f.lift)) return FLO_INVALID_ARGUMENT;
+  } else if (f.link_kind == 0) {
+    if (f.open_area != 0.0) return FLO_INVALID_ARGUMENT;
+  } else {
+    return FLO_INVALID_ARGUMENT;
+  }
+  double scale = 0.0, conductance = 0.0;
+  if (f.open_area > 0.0) {
+    if (f.inverse_density <= 0.0 || f.pressure_gradient_distance <= 0.0) return FLO_INVALID_ARGUMENT;
+    const double mobility = dt * f.inverse_density;
+    scale = mobility / f.pressure_gradient_distance;
+    conductance = f.open_area * scale;
+    if (!FinitePositive(scale) || !FinitePositive(conductance)) return FLO_OUT_OF_RANGE;
+  }
+  const double blocked = full_area - f.open_area;
+  double wall_flux = 0.0;
+  if (f.has_prescribed_wall_flux) {
+    wall_flux = f.prescribed_wall_flux;
+    if (!std::isfinite(wall_flux) || (blocked == 0 && wall_flux != 0)) return FLO_INVALID_ARGUMENT;
+  } else {
+    wall_flux = blocked * f.wall_normal_velocity;
+    if (!Representable(blocked, f.wall_normal_velocity, wall_flux)) return FLO_OUT_OF_RANGE;
+  }
+  out = {slot, tail, head, f.open_area, blocked, scale, conductance, f.predicted_normal_velocity,
+         wall_flux, f.link_kind, f.link_kind == 0 ? 0.0 : f.lift};
+  return FLO_OK;
+}
+
+size_t Root(std::vector<size_t>& parent, size_t c) {
+  while (parent[c] != c) c = parent[c];
+  return c;
+}
+
+// mac_pressure_workspace_3d.cc:68-166
+int Prepare(const flo_geometry& g, const flo_int3* blocks, size_t nb, double dt, const flo_pressure_cell* in_cells,
+            size_t nc, const flo_pressure_face* in_faces, size_t nf, System& sys) {
+  if (nc == 0 || nf == 0 || !FinitePositive(dt)) return FLO_INVALID_ARGUMENT;
+  const double h = g.cell_width;
+  const double area = h * h;
+  const double volume = area * h;
+  if (!FinitePositive(area) || !FinitePositive(volume)) return FLO_OUT_OF_RANGE;
+  sys.dt = dt;
+  sys.cells.resize(nc);
+  std::vector<uint64_t> incident;
+  incident.reserve(6 * nc);
+  for (size_t i = 0; i < nc; ++i) {
+    const auto& c = in_cells[i];
+    if (i != 0 && !CellLess(in_cells[i - 1].cell_index, c.cell_index)) return FLO_INVALID_ARGUMENT;
+    if (int s = FreezeCell(c, dt, volume, sys.cells[i])) return s;
+    for (int a = 0; a < 3; ++a) {
+      if (Get(c.cell_index, a) == kI32Max) return FLO_OUT_OF_RANGE;
+      for (int side = 0; side < 2; ++side) {
+        flo_int3 l = c.cell_index;
+        if (a == 0) l.x += side;
+        if (a == 1) l.y += side;
+        if (a == 2) l.z += side;
+        uint64_t slot;
+        if (int s = FaceSlot(blocks, nb, a, l, slot)) return s;
+        incident.push_back(slot);
+      }
+    }
+  }
+  std::sort(incident.begin(), incident.end());
+  incident.erase(std::unique(incident.begin(), incident.end()), incident.end());
+  if (incident.size() != nf) return FLO_INVALID_ARGUMENT;
+  sys.links.resize(nf);
+  for (size_t i = 0; i < nf; ++i) {
+    const auto& f = in_faces[i];
+    uint64_t slot;
+    if (int s = FaceSlot(blocks, nb, f.axis, f.lattice, slot)) return s;
+    if (slot != incident[i]) return FLO_INVALID_ARGUMENT;
+    flo_int3 tail = f.lattice;
+    const int64_t tc = static_cast<int64_t>(Get(f.lattice, f.axis)) - 1;
+    size_t tail_ordinal = kNone;
+    if (tc >= kI32Min) {
+      if (f.axis == 0) tail.x = static_cast<int32_t>(tc);
+      if (f.axis == 1) tail.y = static_cast<int32_t>(tc);
+      if (f.axis == 2) tail.z = static_cast<int32_t>(tc);
+      tail_ordinal = FindCell(sys.cells, tail);
+    }
+    if (int s = FreezeLink(f, slot, area, dt, tail_ordinal, FindCell(sys.cells, f.lattice), sys.links[i])) return s;
+  }
+  // AssembleIntegratedRhsInto (mac_pressure_3d_arithmetic.h:263-313)
+  std::vector<FluxRow> rows(nc);
+  for (size_t c = 0; c < nc; ++c) {
+    rows[c].Add(sys.cells[c].target_flux);
+    rows[c].Add(sys.cells[c].wall_flux);
+    const double pc = sys.cells[c].compliance * sys.cells[c].previous;
+    if (!Representable(sys.cells[c].compliance, sys.cells[c].previous, pc)) return FLO_OUT_OF_RANGE;
+    rows[c].Add(pc);
+  }
+  for (const auto& l : sys.links) {
+    const double pf = l.open_area * l.predicted;
+    if (!Representable(l.open_area, l.predicted, pf)) return FLO_OUT_OF_RANGE;
+    ForIncident(l, [&](size_t c, double s) {
+      rows[c].Add(-s * pf);
+      rows[c].Add(-s * l.wall_flux);
+    });
+    if (l.kind == 1) {
+      const double jf = l.conductance * l.lift;
+      if (!Representable(l.conductance, l.lift, jf)) return FLO_OUT_OF_RANGE;
+      ForIncident(l, [&](size_t c, double s) { rows[c].Add(-s * jf); });
+    } else if (l.kind == 2) {
+      const double ef = l.conductance * l.lift;
+      if (!Representable(l.conductance, l.lift, ef)) return FLO_OUT_OF_RANGE;
+      ForIncident(l, [&](size_t c, double) { rows[c].Add(ef); });
+    }
+  }
+  sys.rhs.resize(nc);
+  sys.rhs_abs.resize(nc);
+  for (size_t c = 0; c < nc; ++c) {
+    sys.rhs[c] = rows[c].flux.Total();
+    sys.rhs_abs[c] = rows[c].abs.Total();
+    if (!std::isfinite(sys.rhs[c]) || !std::isfinite(sys.rhs_abs[c])) return FLO_OUT_OF_RANGE;
+  }
+  // SelectGaugesInto (mac_pressure_3d_arithmetic.h:328-399)
+  auto& comp = sys.component;
+  comp.resize(nc);
+  std::iota(comp.begin(), comp.end(), size_t{0});
+  for (const auto& l : sys.links)
+    if (l.conductance > 0.0 && l.tail != kNone && l.head != kNone) {
+      const size_t a = Root(comp, l.tail), b = Root(comp, l.head);
+      comp[std::max(a, b)] = std::min(a, b);
+    }
+  std::vector<char> anchored(nc, 0);
+  std::vector<FluxRow> balance(nc);
+  for (size_t c = 0; c < nc; ++c) {
+    const size_t r = Root(comp, c);
+    comp[c] = r;
+    if (sys.cells[c].compliance > 0.0) anchored[r] = 1;
+  }
+  for (const auto& l : sys.links)
+    if (l.kind == 2) ForIncident(l, [&](size_t c, double) { anchored[comp[c]] = 1; });
+  for (size_t c = 0; c < nc; ++c)
+    if (!anchored[comp[c]]) {
+      balance[comp[c]].Add(sys.cells[c].target_flux);
+      balance[comp[c]].Add(sys.cells[c].wall_flux);
+    }
+  for (const auto& l : sys.links) {
+    const bool stays = l.tail != kNone && l.head != kNone && comp[l.tail] == comp[l.head];
+    if (stays) continue;
+    ForIncident(l, [&](size_t c, double s) {
+      if (!anchored[comp[c]]) balance[comp[c]].Add(-s * l.wall_flux);
+    });
+  }
+  sys.gauges.clear();
+  for (size_t c = 0; c < nc; ++c) {
+    if (comp[c] != c || anchored[c]) continue;
+    const double b = balance[c].flux.Total(), a = balance[c].abs.Total();
+    if (!std::isfinite(b) || !std::isfinite(a)) return FLO_OUT_OF_RANGE;
+    if (std::fabs(b) > kCompatibilityRoundoff * a) return FLO_FAILED_PRECONDITION;
+    sys.gauges.push_back(c);
+  }
+  return FLO_OK;
+}
+
+struct Diag {
+  double l2 = 0, max_res = 0, max_defect = 0, defect_l2 = 0;
+};
+
+// Physical defect rows only (mac_pressure_3d_arithmetic.h:441-503).
+int PhysicalRows(const System& sys, const std::vector<double>& p, std::vector<double>* vel, std::vector<FluxRow>& rows) {
+  const size_t nc = sys.cells.size();
+  rows.assign(nc, FluxRow{});
+  for (size_t c = 0; c < nc; ++c) {
+    const auto& k = sys.cells[c];
+    rows[c].Add(-k.target_flux);
+    rows[c].Add(-k.wall_flux);
+    if (k.compliance > 0.0) {
+      const double dp = p[c] - k.previous;
+      const double f = k.compliance * dp;
+      if (!Representable(k.compliance, dp, f)) return FLO_OUT_OF_RANGE;
+      rows[c].Add(f);
+    }
+  }
+  for (size_t i = 0; i < sys.links.size(); ++i) {
+    const auto& l = sys.links[i];
+    double u = l.predicted;
+    if (l.open_area > 0.0) {
+      const double ext = l.kind == 2 ? l.lift : 0.0;
+      const double jump = l.kind == 1 ? l.lift : 0.0;
+      const double pt = l.tail != kNone ? p[l.tail] : ext;
+      const double ph = l.head != kNone ? p[l.head] : ext;
+      const double dp = pt - ph + jump;
+      const double du = l.scale * dp;
+      if (!Representable(l.scale, dp, du)) return FLO_OUT_OF_RANGE;
+      u += du;
+    }
+    if (!std::isfinite(u)) return FLO_OUT_OF_RANGE;
+    if (vel) (*vel)[i] = u;
+    const double ff = l.open_area * u;
+    if (!Representable(l.open_area, u, ff)) return FLO_OUT_OF_RANGE;
+    ForIncident(l, [&](size_t c, double s) {
+      rows[c].Add(s * ff);
+      rows[c].Add(s * l.wall_flux);
+    });
+  }
+  return FLO_OK;
+}
+
+// mac_pressure_workspace_3d.cc:399-461
+int Evaluate(const System& sys, const std::vector<double>& p, std::vector<double>& vel, std::vector<double>& res,
+             std::vector<double>& def, Diag& d) {
+  const size_t nc = sys.cells.size();
+  for (double v : p)
+    if (!std::isfinite(v)) return FLO_INVALID_ARGUMENT;
+  std::vector<FluxRow> op(nc);
+  for (size_t c = 0; c < nc; ++c) {
+    const double f = sys.cells[c].compliance * p[c];
+    if (!Representable(sys.cells[c].compliance, p[c], f)) return FLO_OUT_OF_RANGE;
+    op[c].Add(f);
+  }
+  for (const auto& l : sys.links) {
+    if (l.conductance == 0.0) continue;
+    const double pt = l.tail != kNone ? p[l.tail] : 0.0;
+    const double ph = l.head != kNone ? p[l.head] : 0.0;
+    const double dp = pt - ph;
+    const double f = l.conductance * dp;
+    if (!Representable(l.conductance, dp, f)) return FLO_OUT_OF_RANGE;
+    ForIncident(l, [&](size_t c, double s) { op[c].Add(s * f); });
+  }
+  std::vector<FluxRow> phys;
+  vel.resize(sys.links.size());
+  if (int s = PhysicalRows(sys, p, &vel, phys)) return s;
+  d = {};
+  res.resize(nc);
+  def.resize(nc);
+  for (size_t c = 0; c < nc; ++c) {
+    const double r = sys.rhs[c] - op[c].flux.Total();
+    const double V = sys.cells[c].volume;
+    const double vr = std::fabs(r) / V;
+    const double df = phys[c].flux.Total() / V;
+    const double ie = V * df + r;
+    const double terms = sys.rhs_abs[c] + op[c].abs.Total() + phys[c].abs.Total();
+    if (!std::isfinite(vr) || !std::isfinite(df) || !std::isfinite(ie) || !std::isfinite(terms))
+      return FLO_OUT_OF_RANGE;
+    if (std::fabs(ie) > kIdentityRoundoff * terms) return FLO_INTERNAL;
+    res[c] = r;
+    def[c] = df;
+    d.l2 = std::hypot(d.l2, r);
+    d.defect_l2 = std::hypot(d.defect_l2, df);
+    d.max_res = std::max(d.max_res, vr);
+    d.max_defect = std::max(d.max_defect, std::fabs(df));
+  }
+  if (!std::isfinite(d.l2)) return FLO_OUT_OF_RANGE;
+  return FLO_OK;
+}
+
+// ---------------- geometric V-cycle ----------------
+
+template <typename T>
+struct Level {
+  size_t n = 0;
+  std::vector<flo_int3> coord;
+  std::vector<uint8_t> color;
+  std::vector<std::array<int32_t, 6>> nbr;  // -x,+x,-y,+y,-z,+z
+  std::vector<std::array<T, 6>> k;
+  std::vector<T> anchor, diag;
+  std::vector<int32_t> parent;  // into the next level
+  std::vector<T> x, b, r;
+};
+
+struct KeyHash {
+  size_t operator()(const flo_int3& c) const {
+    return (static_cast<size_t>(static_cast<uint32_t>(c.x)) * 73856093u) ^
+           (static_cast<size_t>(static_cast<uint32_t>(c.y)) * 19349663u) ^
+           (static_cast<size_t>(static_cast<uint32_t>(c.z)) * 83492791u);
+  }
+};
+struct KeyEq {
+  bool operator()(const flo_int3& a, const flo_int3& b) const { return BlockEq(a, b); }
+};
+using CoordMap = std::unordered_map<flo_int3, int32_t, KeyHash, KeyEq>;
+
+int32_t FloorHalf(int32_t v) { return v >= 0 ? v / 2 : -((-(v + 1)) / 2) - 1; }
+
+// Fine edges and anchors in FP64 as built from the frozen links (anchors: compliance,
+// Dirichlet conductances and conductances to zero-correction gauge cells).
+struct FineGraph {
+  std::vector<size_t> unknown_to_cell;
+  std::vector<size_t> cell_to_unknown;
+  std::vector<flo_int3> coord;
+  std::vector<double> anchor;
+  std::vector<std::array<double, 6>> k;
+  std::vector<std::array<int32_t, 6>> nbr;
+};
+
+FineGraph BuildFine(const System& sys) {
+  FineGraph g;
+  const size_t nc = sys.cells.size();
+  std::vector<char> is_gauge(nc, 0);
+  for (size_t c : sys.gauges) is_gauge[c] = 1;
+  g.cell_to_unknown.assign(nc, kNone);
+  for (size_t c = 0; c < nc; ++c)
+    if (!is_gauge[c]) {
+      g.cell_to_unknown[c] = g.unknown_to_cell.size();
+      g.unknown_to_cell.push_back(c);
+      g.coord.push_back(sys.cells[c].index);
+    }
+  const size_t n = g.unknown_to_cell.size();
+  g.anchor.assign(n, 0.0);
+  g.k.assign(n, {0, 0, 0, 0, 0, 0});
+  g.nbr.assign(n, {-1, -1, -1, -1, -1, -1});
+  for (size_t u = 0; u < n; ++u) g.anchor[u] = sys.cells[g.unknown_to_cell[u]].compliance;
+  for (const auto& l : sys.links) {
+    if (l.conductance <= 0.0) continue;
+    const size_t ut = l.tail != kNone ? g.cell_to_unknown[l.tail] : kNone;
+    const size_t uh = l.head != kNone ? g.cell_to_unknown[l.head] : kNone;
+    // Recover the face axis from the slot: (slot % 192) / 64.
+    const int axis = static_cast<int>((l.slot % 192) / 64);
+    if (ut != kNone && uh != kNone) {
+      g.k[ut][2 * axis + 1] = l.conductance;
+      g.nbr[ut][2 * axis + 1] = static_cast<int32_t>(uh);
+      g.k[uh][2 * axis] = l.conductance;
+      g.nbr[uh][2 * axis] = static_cast<int32_t>(ut);
+    } else {
+      if (ut != kNone) g.anchor[ut] += l.conductance;
+      if (uh != kNone) g.anchor[uh] += l.conductance;
+    }
+  }
+  return g;
+}
+
+template <typename T>
+void FinishLevel(Level<T>& L) {
+  L.color.resize(L.n);
+  L.diag.resize(L.n);
+  for (size_t i = 0; i < L.n; ++i) {
+    L.color[i] = static_cast<uint8_t>((L.coord[i].x + L.coord[i].y + L.coord[i].z) & 1);
+    double d = static_cast<double>(L.anchor[i]);
+    for (int f = 0; f < 6; ++f) d += static_cast<double>(L.k[i][f]);
+    L.diag[i] = static_cast<T>(d);
+  }
+  L.x.assign(L.n, T(0));
+  L.b.assign(L.n, T(0));
+  L.r.assign(L.n, T(0));
+}
+
+template <typename T>
+std::vector<Level<T>> BuildHierarchy(const FineGraph& g, int levels, double coarse_scale) {
+  std::vector<Level<T>> H(1);
+  Level<T>& L0 = H[0];
+  L0.n = g.coord.size();
+  L0.coord = g.coord;
+  L0.nbr = g.nbr;
+  L0.k.resize(L0.n);
+  L0.anchor.resize(L0.n);
+  for (size_t i = 0; i < L0.n; ++i) {
+    for (int f = 0; f < 6; ++f) L0.k[i][f] = static_cast<T>(g.k[i][f]);
+    L0.anchor[i] = static_cast<T>(g.anchor[i]);
+  }
+  FinishLevel(L0);
+  for (int lv = 1; lv < levels; ++lv) {
+    Level<T>& F = H[lv - 1];
+    CoordMap map;
+    Level<T> C;
+    F.parent.resize(F.n);
+    for (size_t i = 0; i < F.n; ++i) {
+      const flo_int3 pc{FloorHalf(F.coord[i].x), FloorHalf(F.coord[i].y), FloorHalf(F.coord[i].z)};
+      auto it = map.find(pc);
+      int32_t id;
+      if (it == map.end()) {
+        id = static_cast<int32_t>(C.coord.size());
+        map.emplace(pc, id);
+        C.coord.push_back(pc);
+      } else {
+        id = it->second;
+      }
+      F.parent[i] = id;
+    }
+    C.n = C.coord.size();
+    if (C.n == F.n || C.n == 0) break;
+    std::vector<double> anchor(C.n, 0.0);
+    std::vector<std::array<double, 6>> k(C.n, {0, 0, 0, 0, 0, 0});
+    C.nbr.assign(C.n, {-1, -1, -1, -1, -1, -1});
+    for (size_t i = 0; i < F.n; ++i) {
+      const int32_t I = F.parent[i];
+      anchor[I] += coarse_scale * static_cast<double>(F.anchor[i]);
+      for (int axis = 0; axis < 3; ++axis) {
+        const int32_t j = F.nbr[i][2 * axis + 1];
+        if (j < 0) continue;
+        const int32_t J = F.parent[j];
+        if (J == I) continue;
+        const double kk = coarse_scale * static_cast<double>(F.k[i][2 * axis + 1]);
+        k[I][2 * axis + 1] += kk;
+        k[J][2 * axis] += kk;
+        C.nbr[I][2 * axis + 1] = J;
+        C.nbr[J][2 * axis] = I;
+      }
+    }
+    C.k.resize(C.n);
+    C.anchor.resize(C.n);
+    for (size_t i = 0; i < C.n; ++i) {
+      for (int f = 0; f < 6; ++f) C.k[i][f] = static_cast<T>(k[i][f]);
+      C.anchor[i] = static_cast<T>(anchor[i]);
+    }
+    FinishLevel(C);
+    H.push_back(std::move(C));
+  }
+  return H;
+}
+
+// Edge-difference operator (mac_pressure_workspace_3d.cc:228-246).
+template <typename T>
+void Apply(const Level<T>& L, const std::vector<T>& x, std::vector<T>& y) {
+  for (size_t i = 0; i < L.n; ++i) {
+    T s = L.anchor[i] * x[i];
+    for (int f = 0; f < 6; ++f) {
+      const int32_t j = L.nbr[i][f];
+      if (j >= 0) s += L.k[i][f] * (x[i] - x[j]);
+    }
+    y[i] = s;
+  }
+}
+
+template <typename T>
+void HalfSweep(Level<T>& L, uint8_t color) {
+  for (size_t i = 0; i < L.n; ++i) {
+    if (L.color[i] != color || L.diag[i] == T(0)) continue;
+    T s = L.b[i];
+    for (int f = 0; f < 6; ++f) {
+      const int32_t j = L.nbr[i][f];
+      if (j >= 0) s += L.k[i][f] * L.x[j];
+    }
+    L.x[i] = s / L.diag[i];
+  }
+}
+
+// Symmetric V-cycle: x = M^{-1} b on level lv, starting from zero.
+template <typename T>
+void VCycle(std::vector<Level<T>>& H, size_t lv, int nu, int coarse_sweeps) {
+  Level<T>& L = H[lv];
+  std::fill(L.x.begin(), L.x.end(), T(0));
+  if (lv + 1 == H.size()) {
+    for (int s = 0; s < coarse_sweeps; ++s) {
+      HalfSweep(L, 0);
+      HalfSweep(L, 1);
+    }
+    for (int s = 0; s < coarse_sweeps; ++s) {
+      HalfSweep(L, 1);
+      HalfSweep(L, 0);
+    }
+    return;
+  }
+  for (int s = 0; s < nu; ++s) {
+    HalfSweep(L, 0);
+    HalfSweep(L, 1);
+  }
+  Apply(L, L.x, L.r);
+  Level<T>& C = H[lv + 1];
+  std::fill(C.b.begin(), C.b.end(), T(0));
+  for (size_t i = 0; i < L.n; ++i) C.b[L.parent[i]] += L.b[i] - L.r[i];
+  VCycle(H, lv + 1, nu, coarse_sweeps);
+  for (size_t i = 0; i < L.n; ++i) L.x[i] += C.x[L.parent[i]];
+  for (int s = 0; s < nu; ++s) {
+    HalfSweep(L, 1);
+    HalfSweep(L, 0);
+  }
+}
+
+template <typename T>
+double Dot(const std::vector<T>& a, const std::vector<T>& b) {
+  double s = 0.0;
+  for (size_t i = 0; i < a.size(); ++i) s += static_cast<double>(a[i]) * static_cast<double>(b[i]);
+  return s;
+}
+
+struct Criteria {
+  double integrated = 0, volume = 2e-5, relative = 0, initial_defect = 0;
+};
+
+bool Accept(const Diag& d, const Criteria& c) {
+  if (!(std::isfinite(d.l2) && std::isfinite(d.max_res) && std::isfinite(d.max_defect))) return false;
+  if (!(d.l2 <= c.integrated && d.max_res <= c.volume && d.max_defect <= c.volume)) return false;
+  if (c.relative > 0.0 && !(d.defect_l2 <= c.relative * c.initial_defect)) return false;
+  return true;
+}
+
+template <typename T>
+int Solve(const System& sys, const flo_pressure_options& opt, const Criteria& crit, std::vector<double>& p,
+          std::vector<double>& vel, std::vector<double>& res, std::vector<double>& def, Diag& diag,
+          flo_pressure_diagnostics& out) {
+  const FineGraph g = BuildFine(sys);
+  auto H = BuildHierarchy<T>(g, std::max(1, opt.levels), opt.coarse_scale);
+  out.multigrid_level_count = H.size();
+  const size_t n = g.unknown_to_cell.size();
+  std::vector<double> base = p;
+  if (int s = Evaluate(sys, p, vel, res, def, diag)) return s;
+  out.iteration_count = 0;
+  if (Accept(diag, crit)) return FLO_OK;
+  if (n == 0) return FLO_ABORTED;
+  Level<T>& L0 = H[0];
+  std::vector<T> r(n), x(n, T(0)), d(n, T(0)), q(n), z(n);
+  auto load_residual = [&]() {
+    for (size_t u = 0; u < n; ++u) r[u] = static_cast<T>(res[g.unknown_to_cell[u]]);
+  };
+  load_residual();
+  double rho_prev = 0.0;
+  bool restart = true;
+  const int refresh = 10;
+  int since_refresh = 0;
+  for (uint64_t it = 1; it <= opt.max_iterations; ++it) {
+    L0.b = r;
+    VCycle(H, 0, opt.smoothing_sweeps, opt.coarse_sweeps);
+    z = L0.x;
+    const double rho = Dot(r, z);
+    if (!(rho > 0.0)) return FLO_ABORTED;
+    if (restart) {
+      d = z;
+    } else {
+      const T beta = static_cast<T>(rho / rho_prev);
+      for (size_t u = 0; u < n; ++u) d[u] = z[u] + beta * d[u];
+    }
+    Apply(L0, d, q);
+    const double curv = Dot(d, q);
+    if (!(curv > 0.0)) return FLO_ABORTED;
+    const T alpha = static_cast<T>(rho / curv);
+    double rn = 0.0, rmax = 0.0, dn = 0.0;
+    for (size_t u = 0; u < n; ++u) {
+      x[u] += alpha * d[u];
+      r[u] -= alpha * q[u];
+      const double rv = static_cast<double>(r[u]);
+      const double V = sys.cells[g.unknown_to_cell[u]].volume;
+      rn = std::hypot(rn, rv);
+      rmax = std::max(rmax, std::fabs(rv) / V);
+      dn = std::hypot(dn, rv / V);
+    }
+    restart = false;
+    rho_prev = rho;
+    ++since_refresh;
+    const bool last = it == opt.max_iterations;
+    bool recursive = rn <= crit.integrated && rmax <= crit.volume;
+    if (crit.relative > 0.0) recursive = recursive && dn <= crit.relative * crit.initial_defect;
+    if (last || recursive || since_refresh >= refresh) {
+      p = base;
+      for (size_t u = 0; u < n; ++u) p[g.unknown_to_cell[u]] += static_cast<double>(x[u]);
+      if (int s = Evaluate(sys, p, vel, res, def, diag)) return s;
+      out.iteration_count = it;
+      if (Accept(diag, crit)) return FLO_OK;
+      if (last) break;
+      base = p;
+      std::fill(x.begin(), x.end(), T(0));
+      load_residual();
+      restart = true;
+      since_refresh = 0;
+      ++out.restarts;
+    }
+  }
+  return FLO_ABORTED;
+}
+
+int PrepareAndInit(const flo_geometry* g, const flo_int3* blocks, size_t nb, double dt, const flo_pressure_cell* cells,
+                   size_t nc, const flo_pressure_face* faces, size_t nf, System& sys, std::vector<double>& p) {
+  if (int s = Prepare(*g, blocks, nb, dt, cells, nc, faces, nf, sys)) return s;
+  // InitializeMacPressure3D (mac_pressure_3d_multigrid.cc:303-321)
+  std::vector<double> pin(nc, 0.0);
+  for (size_t c : sys.gauges) pin[sys.component[c]] = sys.cells[c].previous;
+  p.resize(nc);
+  for (size_t c = 0; c < nc; ++c) {
+    p[c] = sys.cells[c].previous - pin[sys.component[c]];
+    if (!std::isfinite(p[c])) return FLO_OUT_OF_RANGE;
+  }
+  return FLO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int flo_project(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count, double dt,
+                const flo_pressure_cell* cells, size_t cell_count, const flo_pressure_face* faces, size_t face_count,
+                const flo_pressure_options* options, double* out_pressures, double* out_velocities, double* out_rhs,
+                double* out_residuals, double* out_defects, uint64_t* out_gauges,
+                flo_pressure_diagnostics* out_diagnostics) {
+  const flo_pressure_options opt = *options;
+  if (opt.max_iterations < 1) return FLO_INVALID_ARGUMENT;
+  if (!(opt.cpu_residual_tolerance_scale >= 1e-12 && opt.cpu_residual_tolerance_scale <= 1.0))
+    return FLO_INVALID_ARGUMENT;
+  System sys;
+  std::vector<double> p;
+  if (int s = PrepareAndInit(geometry, sorted_blocks, block_count, dt, cells, cell_count, faces, face_count, sys, p))
+    return s;
+  // MacPressureStoppingCriteria (mac_pressure_3d_multigrid.cc:333-357)
+  double rhs_norm = 0.0;
+  for (double b : sys.rhs) rhs_norm = std::hypot(rhs_norm, b);
+  double limit = 2e-6 + 2e-6 * rhs_norm;
+  if (!std::isfinite(rhs_norm) || !std::isfinite(limit)) return FLO_OUT_OF_RANGE;
+  bool has_jump = false;
+  for (const auto& l : sys.links) has_jump |= (l.conductance > 0.0 && l.kind == 1 && l.lift != 0.0);
+  if (rhs_norm > 0.0 && has_jump) limit = std::min(limit, std::max(std::numeric_limits<double>::denorm_min(), 1e-3 * rhs_norm));
+  Criteria crit;
+  crit.integrated = limit * opt.cpu_residual_tolerance_scale;
+  crit.volume = 2e-5 * opt.cpu_residual_tolerance_scale;
+  crit.relative = opt.relative_divergence_target;
+  std::vector<double> zero(cell_count, 0.0);
+  {
+    std::vector<FluxRow> rows;
+    if (int s = PhysicalRows(sys, zero, nullptr, rows)) return s;
+    double n0 = 0.0;
+    for (size_t c = 0; c < cell_count; ++c) n0 = std::hypot(n0, rows[c].flux.Total() / sys.cells[c].volume);
+    crit.initial_defect = n0;
+  }
+  std::vector<double> vel, res, def;
+  Diag diag;
+  flo_pressure_diagnostics out{};
+  int s = opt.fp32_vcycle ? Solve<float>(sys, opt, crit, p, vel, res, def, diag, out)
+                          : Solve<double>(sys, opt, crit, p, vel, res, def, diag, out);
+  out.true_residual_l2 = diag.l2;
+  out.max_volume_scaled_residual = diag.max_res;
+  out.max_cell_constraint_defect = diag.max_defect;
+  out.rhs_l2 = rhs_norm;
+  out.initial_defect_l2 = crit.initial_defect;
+  out.defect_l2 = diag.defect_l2;
+  out.defect_linf = diag.max_defect;
+  out.gauge_count = sys.gauges.size();
+  if (out_diagnostics) *out_diagnostics = out;
+  if (s) return s;
+  if (out_pressures) std::copy(p.begin(), p.end(), out_pressures);
+  if (out_velocities) std::copy(vel.begin(), vel.end(), out_velocities);
+  if (out_rhs) std::copy(sys.rhs.begin(), sys.rhs.end(), out_rhs);
+  if (out_residuals) std::copy(res.begin(), res.end(), out_residuals);
+  if (out_defects) std::copy(def.begin(), def.end(), out_defects);
+  if (out_gauges) std::copy(sys.gauges.begin(), sys.gauges.end(), out_gauges);
+  return FLO_OK;
+}
+
+int flo_evaluate(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count, double dt,
+                 const flo_pressure_cell* cells, size_t cell_count, const flo_pressure_face* faces, size_t face_count,
+                 const double* pressures, double* out_velocities, double* out_residuals, double* out_defects,
+                 flo_pressure_diagnostics* out_diagnostics) {
+  System sys;
+  std::vector<double> p0;
+  if (int s = PrepareAndInit(geometry, sorted_blocks, block_count, dt, cells, cell_count, faces, face_count, sys, p0))
+    return s;
+  std::vector<double> p(pressures, pressures + cell_count), vel, res, def;
+  Diag d;
+  if (int s = Evaluate(sys, p, vel, res, def, d)) return s;
+  std::vector<double> zero(cell_count, 0.0);
+  std::vector<FluxRow> rows;
+  double n0 = 0.0;
+  if (PhysicalRows(sys, zero, nullptr, rows) == FLO_OK)
+    for (size_t c = 0; c < cell_count; ++c) n0 = std::hypot(n0, rows[c].flux.Total() / sys.cells[c].volume);
+  if (out_velocities) std::copy(vel.begin(), vel.end(), out_velocities);
+  if (out_residuals) std::copy(res.begin(), res.end(), out_residuals);
+  if (out_defects) std::copy(def.begin(), def.end(), out_defects);
+  if (out_diagnostics) {
+    flo_pressure_diagnostics o{};
+    o.true_residual_l2 = d.l2;
+    o.max_volume_scaled_residual = d.max_res;
+    o.max_cell_constraint_defect = d.max_defect;
+    o.initial_defect_l2 = n0;
+    o.defect_l2 = d.defect_l2;
+    o.defect_linf = d.max_defect;
+    o.gauge_count = sys.gauges.size();
+    *out_diagnostics = o;
+  }
+  return FLO_OK;
+}
+
+}  // extern "C"

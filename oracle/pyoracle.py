@@ -1,0 +1,243 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: imported by tests/, by __graft_entry__.smoke() and by the
+cpu_baseline leg of bench.py, as the checker. The product package never imports it.
+Record layouts are numpy structured dtypes that match oracle/oracle.h byte for byte.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+OK, INVALID_ARGUMENT, RESOURCE_EXHAUSTED, FAILED_PRECONDITION, ABORTED, OUT_OF_RANGE, INTERNAL = 0, 3, 8, 9, 10, 11, 13
+
+VEC3 = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8")])
+MAT3 = np.dtype([("x_row", VEC3), ("y_row", VEC3), ("z_row", VEC3)])
+INT3 = np.dtype([("x", "i4"), ("y", "i4"), ("z", "i4")])
+GEOMETRY = np.dtype([("world_origin", VEC3), ("cell_width", "f8")])
+FACE_INDEX = np.dtype({"names": ["axis", "lattice"], "formats": ["u1", INT3], "offsets": [0, 4], "itemsize": 16})
+PARTICLE = np.dtype([("sorted_world_position", VEC3), ("trial_world_position", VEC3), ("particle_mass", "f8"),
+                     ("particle_velocity", VEC3), ("particle_velocity_slopes", MAT3)])
+QUANTIZATION = np.dtype([("mass_per_quantum", "f8"), ("pure_momentum_per_quantum", "f8"),
+                         ("declared_particle_capacity", "u8")])
+FACE_DEPOSIT = np.dtype([("deposited_mass", "f8"), ("deposited_pure_momentum", "f8")])
+FACE_QUANTA = np.dtype([("mass_quanta_sum", "i8"), ("pure_momentum_quanta_sum", "i8"),
+                        ("stress_force_quanta_sum", "i8")])
+GATHER_RESULT = np.dtype([("new_pic_velocity", VEC3), ("grid_velocity_increment", VEC3),
+                          ("fitted_velocity_slopes", MAT3), ("velocity_gradient", MAT3)])
+STENCIL_SAMPLE = np.dtype([("face_slot", "u8"), ("weight", "f8"), ("face_offset_from_particle", VEC3),
+                           ("weight_gradient", VEC3)])
+SORT_ENTRY = np.dtype([("bin_index", INT3), ("pad", "u4"), ("original_ordinal", "u8")])
+PRESSURE_CELL = np.dtype([("cell_index", INT3), ("pad", "i4"), ("fluid_volume", "f8"), ("target_divergence", "f8"),
+                          ("bulk_modulus", "f8"), ("previous_pressure", "f8"), ("integrated_inward_wall_flux", "f8")])
+PRESSURE_FACE = np.dtype([("axis", "i4"), ("lattice", INT3), ("open_area", "f8"), ("inverse_density", "f8"),
+                          ("pressure_gradient_distance", "f8"), ("predicted_normal_velocity", "f8"),
+                          ("wall_normal_velocity", "f8"), ("link_kind", "i4"), ("has_prescribed_wall_flux", "i4"),
+                          ("lift", "f8"), ("prescribed_wall_flux", "f8")])
+PRESSURE_OPTIONS = np.dtype([("max_iterations", "u8"), ("cpu_residual_tolerance_scale", "f8"),
+                             ("relative_divergence_target", "f8"), ("levels", "i4"), ("smoothing_sweeps", "i4"),
+                             ("coarse_sweeps", "i4"), ("fp32_vcycle", "i4"), ("coarse_scale", "f8")])
+PRESSURE_DIAGNOSTICS = np.dtype([("iteration_count", "u8"), ("multigrid_level_count", "u8"),
+                                 ("true_residual_l2", "f8"), ("max_volume_scaled_residual", "f8"),
+                                 ("max_cell_constraint_defect", "f8"), ("rhs_l2", "f8"),
+                                 ("initial_defect_l2", "f8"), ("defect_l2", "f8"), ("defect_linf", "f8"),
+                                 ("restarts", "u8"), ("gauge_count", "u8")])
+
+assert PARTICLE.itemsize == 152 and GATHER_RESULT.itemsize == 192 and SORT_ENTRY.itemsize == 24
+assert PRESSURE_CELL.itemsize == 56 and PRESSURE_FACE.itemsize == 80 and FACE_QUANTA.itemsize == 24
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def geometry(origin=(0.0, 0.0, 0.0), h=1.0):
+    g = np.zeros(1, GEOMETRY)
+    g["world_origin"]["x"], g["world_origin"]["y"], g["world_origin"]["z"] = origin
+    g["cell_width"] = h
+    return g
+
+
+def int3(rows):
+    a = np.zeros(len(rows), INT3)
+    for i, (x, y, z) in enumerate(rows):
+        a[i] = (x, y, z)
+    return a
+
+
+def vec3(x, y, z):
+    v = np.zeros(1, VEC3)
+    v[0] = (x, y, z)
+    return v
+
+
+def _vec3_struct():
+    class V(C.Structure):
+        _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
+    return V
+
+
+Vec3C = _vec3_struct()
+
+
+class Int3C(C.Structure):
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32), ("z", C.c_int32)]
+
+
+class FaceIndexC(C.Structure):
+    _fields_ = [("axis", C.c_uint8), ("lattice", Int3C)]
+
+
+class QuantC(C.Structure):
+    _fields_ = [("mass_per_quantum", C.c_double), ("pure_momentum_per_quantum", C.c_double),
+                ("declared_particle_capacity", C.c_uint64)]
+
+
+def compute_bin(geom, pos):
+    out = Int3C()
+    s = lib().flo_compute_bin(_p(geom), Vec3C(*map(float, pos)), C.byref(out))
+    return s, (out.x, out.y, out.z)
+
+
+def sort_blocks(geom, blocks):
+    out = np.zeros(len(blocks), INT3)
+    s = lib().flo_grid_sort_blocks(_p(geom), _p(blocks), C.c_size_t(len(blocks)), _p(out))
+    return s, out
+
+
+def face_slot(blocks, axis, lattice):
+    out = C.c_uint64()
+    f = FaceIndexC(axis, Int3C(*lattice))
+    s = lib().flo_face_slot(_p(blocks), C.c_size_t(len(blocks)), f, C.byref(out))
+    return s, out.value
+
+
+def slot_face(blocks, slot):
+    out = FaceIndexC()
+    s = lib().flo_slot_face(_p(blocks), C.c_size_t(len(blocks)), C.c_uint64(slot), C.byref(out))
+    return s, (out.axis, (out.lattice.x, out.lattice.y, out.lattice.z))
+
+
+def face_world_anchor(geom, axis, lattice):
+    out = Vec3C()
+    s = lib().flo_face_world_anchor(_p(geom), FaceIndexC(axis, Int3C(*lattice)), C.byref(out))
+    return s, (out.x, out.y, out.z)
+
+
+def owned_stencil(geom, blocks, axis, sorted_pos, trial_pos):
+    out = np.zeros(27, STENCIL_SAMPLE)
+    s = lib().flo_owned_stencil(_p(geom), _p(blocks), C.c_size_t(len(blocks)), C.c_int(axis),
+                                Vec3C(*map(float, sorted_pos)), Vec3C(*map(float, trial_pos)), _p(out))
+    return s, out
+
+
+def deposit(geom, blocks, particles, out=None):
+    nf = 192 * len(blocks)
+    if out is None:
+        out = np.zeros(nf, FACE_DEPOSIT)
+    s = lib().flo_deposit(_p(geom), _p(blocks), C.c_size_t(len(blocks)), _p(particles), C.c_size_t(len(particles)),
+                          _p(out), C.c_size_t(len(out)))
+    return s, out
+
+
+def deposit_quanta(geom, blocks, particles, mass_q, mom_q, cap, max_capacity=1 << 20, out=None, threads=1):
+    if out is None:
+        out = np.zeros(192 * len(blocks), FACE_QUANTA)
+    s = lib().flo_deposit_quanta(_p(geom), _p(blocks), C.c_size_t(len(blocks)), _p(particles),
+                                 C.c_size_t(len(particles)), QuantC(mass_q, mom_q, cap), C.c_uint64(max_capacity),
+                                 _p(out), C.c_size_t(len(out)), C.c_int(threads))
+    return s, out
+
+
+def gather(geom, blocks, particles, old, new, out=None, threads=1):
+    if out is None:
+        out = np.zeros(len(particles), GATHER_RESULT)
+    old = np.ascontiguousarray(old, dtype=np.float64)
+    new = np.ascontiguousarray(new, dtype=np.float64)
+    s = lib().flo_gather(_p(geom), _p(blocks), C.c_size_t(len(blocks)), _p(particles), C.c_size_t(len(particles)),
+                         _p(old), _p(new), C.c_size_t(len(old)), _p(out), C.c_int(threads))
+    return s, out
+
+
+def normalize(quanta, mass_q, mom_q, min_mass=0.0):
+    nf = len(quanta)
+    m, u, v = np.zeros(nf), np.zeros(nf), np.zeros(nf, np.uint8)
+    s = lib().flo_normalize(_p(quanta), C.c_size_t(nf), QuantC(mass_q, mom_q, 0), C.c_double(min_mass), _p(m), _p(u),
+                            _p(v))
+    return s, m, u, v
+
+
+def apply_forces(blocks, old, accel, dt):
+    out = np.zeros(len(old))
+    old = np.ascontiguousarray(old, dtype=np.float64)
+    s = lib().flo_apply_forces(_p(blocks), C.c_size_t(len(blocks)), _p(old), C.c_size_t(len(old)),
+                               Vec3C(*map(float, accel)), C.c_double(dt), _p(out))
+    return s, out
+
+
+def sort_particles(geom, positions):
+    pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(len(pos), SORT_ENTRY)
+    s = lib().flo_sort_particles(_p(geom), _p(pos), C.c_size_t(len(pos)), _p(out))
+    return s, out
+
+
+def activate_blocks(geom, positions, cells=None, capacity=1 << 24):
+    pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    cells = np.zeros(0, INT3) if cells is None else cells
+    out = np.zeros(capacity, INT3)
+    cnt = C.c_size_t(capacity)
+    s = lib().flo_activate_blocks(_p(geom), _p(pos), C.c_size_t(len(pos)), _p(cells), C.c_size_t(len(cells)), _p(out),
+                                  C.byref(cnt))
+    return s, out[:cnt.value].copy()
+
+
+def pressure_options(max_iterations=256, tolerance_scale=1.0, relative_target=0.0, levels=4, sweeps=2,
+                     coarse_sweeps=16, fp32=True, coarse_scale=1.0):
+    o = np.zeros(1, PRESSURE_OPTIONS)
+    o[0] = (max_iterations, tolerance_scale, relative_target, levels, sweeps, coarse_sweeps, int(fp32), coarse_scale)
+    return o
+
+
+def project(geom, blocks, dt, cells, faces, options):
+    nc, nf = len(cells), len(faces)
+    p, u, b, r, d = (np.zeros(nc), np.zeros(nf), np.zeros(nc), np.zeros(nc), np.zeros(nc))
+    gauges = np.zeros(nc, np.uint64)
+    diag = np.zeros(1, PRESSURE_DIAGNOSTICS)
+    s = lib().flo_project(_p(geom), _p(blocks), C.c_size_t(len(blocks)), C.c_double(dt), _p(cells), C.c_size_t(nc),
+                          _p(faces), C.c_size_t(nf), _p(options), _p(p), _p(u), _p(b), _p(r), _p(d), _p(gauges),
+                          _p(diag))
+    g = gauges[: int(diag["gauge_count"][0])].copy()
+    return s, dict(pressures=p, velocities=u, rhs=b, residuals=r, defects=d, gauges=g, diagnostics=diag[0])
+
+
+def evaluate(geom, blocks, dt, cells, faces, pressures):
+    nc, nf = len(cells), len(faces)
+    u, r, d = np.zeros(nf), np.zeros(nc), np.zeros(nc)
+    diag = np.zeros(1, PRESSURE_DIAGNOSTICS)
+    pressures = np.ascontiguousarray(pressures, dtype=np.float64)
+    s = lib().flo_evaluate(_p(geom), _p(blocks), C.c_size_t(len(blocks)), C.c_double(dt), _p(cells), C.c_size_t(nc),
+                           _p(faces), C.c_size_t(nf), _p(pressures), _p(u), _p(r), _p(d), _p(diag))
+    return s, dict(velocities=u, residuals=r, defects=d, diagnostics=diag[0])

@@ -1,0 +1,206 @@
+"""Shared test inputs: the reference's known-answer cases rebuilt as record arrays.
+
+Sources: native/tests/test_transfers_3d.cc:22-85 (corner blocks, corner/tie particles),
+native/tests/mac_viscous_traction_cases.h:33-91 (affine pressure problem),
+native/tests/matched_error_pressure_3d.cc:29-35 (sealed cube p = 0.125 + 0.75 x).
+"""
+import json
+import math
+import os
+
+import numpy as np
+
+import pyoracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CORNER_BLOCKS = [(0, -1, 0), (1, -1, 0), (0, 0, 0), (1, 0, 0), (0, -1, 1), (1, -1, 1), (0, 0, 1), (1, 0, 1)]
+CORNER_POSITION = (3.75, -0.25, 3.75)
+PARTICLE_VELOCITY = (1.0, -2.0, 4.0)
+OLD_VELOCITY = (2.0, 3.0, -4.0)
+AFFINE_ROWS = ((2.0, 4.0, -1.0), (-5.0, 1.0, 5.0), (7.0, -2.0, 3.0))
+TIE_POSITION = (3.5, 0.0, 4.0)
+TIE_FACE = (0, (3, -1, 3))
+
+
+def sorted_blocks(geom, blocks):
+    s, b = O.sort_blocks(geom, O.int3(blocks))
+    assert s == O.OK
+    return b
+
+
+def particles(n):
+    return np.zeros(n, O.PARTICLE)
+
+
+def set_vec(rec, field, v):
+    rec[field]["x"], rec[field]["y"], rec[field]["z"] = v
+
+
+def set_mat(rec, field, rows):
+    for name, r in zip(("x_row", "y_row", "z_row"), rows):
+        rec[field][name]["x"], rec[field][name]["y"], rec[field][name]["z"] = r
+
+
+def corner_particle(origin=(0.0, 0.0, 0.0), h=1.0):
+    p = particles(1)
+    pos = tuple(o + h * c for o, c in zip(origin, CORNER_POSITION))
+    set_vec(p[0], "sorted_world_position", pos)
+    set_vec(p[0], "trial_world_position", pos)
+    p[0]["particle_mass"] = 2.0
+    set_vec(p[0], "particle_velocity", PARTICLE_VELOCITY)
+    set_mat(p[0], "particle_velocity_slopes", AFFINE_ROWS)
+    return p
+
+
+def tie_particle(mass, normal_velocity):
+    p = particles(1)
+    set_vec(p[0], "sorted_world_position", TIE_POSITION)
+    set_vec(p[0], "trial_world_position", TIE_POSITION)
+    p[0]["particle_mass"] = mass
+    set_vec(p[0], "particle_velocity", (normal_velocity, 0.0, 0.0))
+    return p
+
+
+def vec(a):
+    return np.array([a["x"], a["y"], a["z"]], dtype=np.float64)
+
+
+def mat(a):
+    return np.array([vec(a["x_row"]), vec(a["y_row"]), vec(a["z_row"])])
+
+
+def sample_field(geom, blocks, field):
+    """field(axis, anchor) at every owned face anchor, in slot order."""
+    values = np.zeros(192 * len(blocks))
+    h = float(geom["cell_width"][0])
+    org = np.array([geom["world_origin"][c][0] for c in "xyz"])
+    for bi, b in enumerate(blocks):
+        for axis in range(3):
+            for lz in range(4):
+                for ly in range(4):
+                    for lx in range(4):
+                        lat = (4 * int(b["x"]) + lx, 4 * int(b["y"]) + ly, 4 * int(b["z"]) + lz)
+                        s, anchor = O.face_world_anchor(geom, axis, lat)
+                        assert s == O.OK
+                        values[bi * 192 + axis * 64 + lz * 16 + ly * 4 + lx] = field(axis, np.array(anchor))
+    return values
+
+
+def load_transfer_fixture():
+    with open(os.path.join(HERE, "golden", "corner_transfer_3d.json")) as f:
+        return json.load(f)
+
+
+def load_pressure_fixture():
+    with open(os.path.join(HERE, "golden", "pressure_cases_3d.json")) as f:
+        return json.load(f)
+
+
+def pressure_case_records(case):
+    cells = np.zeros(len(case["cells"]), O.PRESSURE_CELL)
+    for i, c in enumerate(case["cells"]):
+        cells[i]["cell_index"] = tuple(c["index"])
+        cells[i]["fluid_volume"] = c["V"]
+        cells[i]["target_divergence"] = c["s"]
+        cells[i]["bulk_modulus"] = math.inf if c["K"] is None else c["K"]
+        cells[i]["previous_pressure"] = c["prev"]
+    faces = np.zeros(len(case["faces"]), O.PRESSURE_FACE)
+    for i, f in enumerate(case["faces"]):
+        faces[i]["axis"] = f["axis"]
+        faces[i]["lattice"] = tuple(f["lat"])
+        faces[i]["open_area"] = f["area"]
+        faces[i]["inverse_density"] = f["beta"]
+        faces[i]["pressure_gradient_distance"] = f["d"]
+        faces[i]["predicted_normal_velocity"] = f["u"]
+        faces[i]["wall_normal_velocity"] = f["uw"]
+        faces[i]["link_kind"] = f["kind"]
+        faces[i]["lift"] = f["lift"]
+    blocks = sorted({(f["lat"][0] // 4, f["lat"][1] // 4, f["lat"][2] // 4) for f in case["faces"]})
+    return cells, faces, blocks
+
+
+def owner(v):
+    return v // 4
+
+
+def affine_problem(minimum=-3, extents=(4, 4, 4), anchored=True, h=1.0, dt=1.0):
+    """Box of cells; interior faces carry u* = 2^axis; p = x + 2y + 4z zeroes the corrected field."""
+    def pressure(c):
+        return c[0] + 2.0 * c[1] + 4.0 * c[2]
+
+    cells_idx = [(x, y, z) for z in range(minimum, minimum + extents[2]) for y in range(minimum, minimum + extents[1])
+                 for x in range(minimum, minimum + extents[0])]
+    face_keys = set()
+    for (x, y, z) in cells_idx:
+        for a in range(3):
+            for side in range(2):
+                face_keys.add((a, x + (side if a == 0 else 0), y + (side if a == 1 else 0), z + (side if a == 2 else 0)))
+    blocks = sorted({(owner(x), owner(y), owner(z)) for (_, x, y, z) in face_keys}, key=lambda b: b[::-1])
+    cells = np.zeros(len(cells_idx), O.PRESSURE_CELL)
+    for i, c in enumerate(cells_idx):
+        cells[i]["cell_index"] = c
+        cells[i]["fluid_volume"] = h ** 3
+        cells[i]["bulk_modulus"] = math.inf
+    recs = []
+    for (a, x, y, z) in face_keys:
+        co = (x, y, z)
+        low = co[a] == minimum
+        high = co[a] == minimum + extents[a]
+        boundary = low or high
+        f = np.zeros(1, O.PRESSURE_FACE)[0]
+        f["axis"] = a
+        f["lattice"] = co
+        f["open_area"] = 0.0 if (boundary and not anchored) else h * h
+        f["inverse_density"] = 1.0
+        f["pressure_gradient_distance"] = h
+        f["predicted_normal_velocity"] = 0.0 if (boundary and not anchored) else float(1 << a) * dt / dt
+        f["link_kind"] = 1
+        if boundary and anchored:
+            ext = list(co)
+            if low:
+                ext[a] -= 1
+            f["link_kind"] = 2
+            f["lift"] = pressure(ext)
+        elif boundary:
+            f["link_kind"] = 0
+        recs.append(f)
+    faces = np.array(recs, dtype=O.PRESSURE_FACE)
+    geom = O.geometry((0.0, 0.0, 0.0), h)
+    sb = sorted_blocks(geom, blocks)
+    slots = []
+    for f in faces:
+        s, slot = O.face_slot(sb, int(f["axis"]), tuple(int(v) for v in f["lattice"]))
+        assert s == O.OK
+        slots.append(slot)
+    faces = faces[np.argsort(np.array(slots))]
+    return geom, sb, cells, faces, pressure
+
+
+QUANTUM = 2.0 ** -40
+
+
+def oracle_step_inputs(scene, threads=8):
+    """Run the oracle through activation, quantized P2G, normalize and forces for a scene.
+
+    Returns a dict with the geometry, blocks, particles, face quanta, u0, u*, and the pressure
+    cell/face records built from u* (the inputs of the projection stage).
+    """
+    geom = scene.geometry_record().view(O.GEOMETRY)
+    cells_idx = scene.liquid_cells()
+    cells3 = np.zeros(len(cells_idx), O.INT3)
+    cells3["x"], cells3["y"], cells3["z"] = cells_idx.T
+    s, blocks = O.activate_blocks(geom, scene.positions, cells3, capacity=1 << 22)
+    assert s == O.OK
+    parts = scene.particles_record().view(O.PARTICLE)
+    s, q = O.deposit_quanta(geom, blocks, parts, QUANTUM, QUANTUM, len(parts), max_capacity=1 << 40, threads=threads)
+    assert s == O.OK
+    s, mass, u0, validity = O.normalize(q, QUANTUM, QUANTUM, 0.0)
+    assert s == O.OK
+    dt = scene.dt
+    s, ustar = O.apply_forces(blocks, u0, scene.gravity, dt)
+    assert s == O.OK
+    faces = scene.pressure_faces(blocks.view(np.dtype([("x", "i4"), ("y", "i4"), ("z", "i4")])), ustar).view(
+        O.PRESSURE_FACE)
+    cells = scene.pressure_cells().view(O.PRESSURE_CELL)
+    return dict(geom=geom, blocks=blocks, particles=parts, quanta=q, mass=mass, u0=u0, validity=validity, ustar=ustar,
+                dt=dt, cells=cells, faces=faces)
