@@ -1,0 +1,520 @@
+// C-ABI entry points (include/fluidity_hip.h): context lifecycle and the span-based drop-ins for
+// the reference operators. Argument checks run in the reference's order so that every call
+// returns the same status code the CPU operator would; outputs are written only on success.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "context.h"
+
+namespace fl {
+int project_pressure(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count, double dt,
+                     const fl_pressure_cell* cells, size_t cell_count, const fl_pressure_face* faces,
+                     size_t face_count, const fl_pressure_options* options, double* out_pressures,
+                     double* out_velocities, double* out_rhs, double* out_residuals, double* out_defects,
+                     uint64_t* out_gauges, fl_pressure_diagnostics* out_diagnostics, fl_measurement* measurement);
+int normalize_forces_stress(fl_ctx* ctx, const int64_t* qs, uint64_t nf, double qm, double qp, double min_mass,
+                            const double g[3], double dt);
+void destroy_pressure(fl_ctx* ctx);
+}  // namespace fl
+
+namespace {
+
+using namespace fl;
+
+__global__ void k_split_columns(const double* in, uint64_t n, int stride, int columns, double* base, uint64_t cap) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < columns; ++c) base[c * cap + i] = in[i * stride + c];
+}
+
+__global__ void k_split_quanta(const fl_face_quanta* in, uint64_t n, int64_t* qm, int64_t* qp, int64_t* qs) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  qm[i] = in[i].mass_quanta_sum;
+  qp[i] = in[i].pure_momentum_quanta_sum;
+  qs[i] = in[i].stress_force_quanta_sum;
+}
+
+__global__ void k_pack_quanta(const int64_t* qm, const int64_t* qp, uint64_t n, fl_face_quanta* out) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  fl_face_quanta q;
+  q.mass_quanta_sum = qm[i];
+  q.pure_momentum_quanta_sum = qp[i];
+  q.stress_force_quanta_sum = 0;
+  out[i] = q;
+}
+
+__global__ void k_pack_gather(const double* cols, uint64_t cap, const uint32_t* perm, uint64_t n,
+                              fl_gather_result* out) {
+  const uint64_t k = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (k >= n) return;
+  double* r = reinterpret_cast<double*>(out + perm[k]);
+  for (int c = 0; c < kGatherDoubles; ++c) r[c] = cols[c * cap + k];
+}
+
+bool powers_of_two_ok(const fl_quantization& q) {
+  int e;
+  return std::isnormal(q.mass_per_quantum) && std::frexp(q.mass_per_quantum, &e) == 0.5 &&
+         std::isnormal(q.pure_momentum_per_quantum) && std::frexp(q.pure_momentum_per_quantum, &e) == 0.5;
+}
+
+int check_geometry(const fl_geometry* g) {
+  if (!g) return FL_INVALID_ARGUMENT;
+  if (!std::isfinite(g->world_origin.x) || !std::isfinite(g->world_origin.y) || !std::isfinite(g->world_origin.z))
+    return FL_INVALID_ARGUMENT;
+  if (!(g->cell_width > 0.0 && std::isnormal(g->cell_width))) return FL_INVALID_ARGUMENT;
+  return FL_OK;
+}
+
+// SparseMacGridView3D::Create rules (native/src/sparse_mac_grid_3d.cc:557-577).
+int check_blocks(const fl_int3* b, size_t n) {
+  const int64_t lo = -2147483648LL / 4, hi = 2147483647LL / 4;
+  for (size_t i = 0; i < n; ++i) {
+    for (int32_t c : {b[i].x, b[i].y, b[i].z})
+      if (c < lo || c > hi) return FL_OUT_OF_RANGE;
+    if (i) {
+      const fl_int3 p = b[i - 1], q = b[i];
+      const bool less = p.z != q.z ? p.z < q.z : (p.y != q.y ? p.y < q.y : p.x < q.x);
+      if (!less) return FL_INVALID_ARGUMENT;
+    }
+  }
+  return FL_OK;
+}
+
+bool overlaps(const void* a, size_t an, const void* b, size_t bn) {
+  if (!a || !b || !an || !bn) return false;
+  const auto* pa = static_cast<const unsigned char*>(a);
+  const auto* pb = static_cast<const unsigned char*>(b);
+  return pa < pb + bn && pb < pa + an;
+}
+
+void set_geometry(fl_ctx* ctx, const fl_geometry* g) {
+  ctx->geometry = {g->world_origin.x, g->world_origin.y, g->world_origin.z, g->cell_width};
+}
+
+// hipEvent interval on the context stream; each timer owns its events so timers may nest.
+struct Timer {
+  fl_ctx* ctx;
+  hipEvent_t a = nullptr, b = nullptr;
+  explicit Timer(fl_ctx* c) : ctx(c) {
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, ctx->stream);
+  }
+  ~Timer() {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+  double stop() {
+    (void)hipEventRecord(b, ctx->stream);
+    (void)hipEventSynchronize(b);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+  }
+};
+
+void fill_measurement(fl_ctx* ctx, fl_measurement* m, double kernel_ms, double copy_ms, uint32_t launches) {
+  if (!m) return;
+  m->kernel_ms = kernel_ms;
+  m->copy_ms = copy_ms;
+  m->launches = launches;
+  std::snprintf(m->architecture, sizeof(m->architecture), "%s", ctx->arch);
+}
+
+int first_error_code(fl_ctx* ctx) {
+  uint64_t w;
+  FL_TRY(read_error(ctx, &w));
+  return w == ~uint64_t(0) ? FL_OK : static_cast<int>(w & 0xff);
+}
+
+uint64_t staging_bytes(const fl_ctx* c) {
+  return std::max<uint64_t>(sizeof(fl_particle) * std::max<uint64_t>(1, c->cap.particles),
+                            24 * 192 * std::max<uint64_t>(1, c->cap.blocks));
+}
+
+template <typename T>
+int dalloc(T** p, uint64_t count) {
+  *p = nullptr;
+  if (!count) return FL_OK;
+  FL_HIP(hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)));
+  return FL_OK;
+}
+
+int allocate(fl_ctx* c) {
+  const uint64_t np = std::max<uint64_t>(1, c->cap.particles);
+  const uint64_t nb = std::max<uint64_t>(1, c->cap.blocks);
+  const uint64_t lk = std::max<uint64_t>(64, c->cap.lookup_blocks);
+  const uint64_t nf = 192 * nb;
+  c->raw.capacity = c->sorted.capacity = np;
+  FL_TRY(dalloc(&c->raw.base, kParticleDoubles * np));
+  FL_TRY(dalloc(&c->sorted.base, kParticleDoubles * np));
+  FL_TRY(dalloc(reinterpret_cast<char**>(&c->staging_aos), staging_bytes(c)));
+  FL_TRY(dalloc(&c->bin, 3 * np));
+  FL_TRY(dalloc(&c->key, np));
+  FL_TRY(dalloc(&c->rank, np));
+  FL_TRY(dalloc(&c->perm, np));
+  FL_TRY(dalloc(&c->perm_tmp, np));
+  FL_TRY(dalloc(&c->bin_count, lk));
+  FL_TRY(dalloc(&c->bin_scan, 2 * lk));
+  FL_TRY(dalloc(&c->bin_key, lk));
+  FL_TRY(dalloc(&c->bin_offset, lk));
+  FL_TRY(dalloc(&c->bin_n, lk));
+  FL_TRY(dalloc(&c->scan_partial, lk / 128 + 4096));
+  FL_TRY(dalloc(&c->blocks, 3 * nb));
+  FL_TRY(dalloc(&c->lut, lk));
+  FL_TRY(dalloc(&c->mark, lk));
+  FL_TRY(dalloc(&c->qm, nf));
+  FL_TRY(dalloc(&c->qp, nf));
+  FL_TRY(dalloc(&c->face_mass, nf));
+  FL_TRY(dalloc(&c->u0, nf));
+  FL_TRY(dalloc(&c->ustar, nf));
+  FL_TRY(dalloc(&c->unew, nf));
+  FL_TRY(dalloc(&c->validity, nf));
+  FL_TRY(dalloc(&c->quanta_aos, nf));
+  FL_TRY(dalloc(&c->gather, kGatherDoubles * np));
+  FL_TRY(dalloc(&c->gather_aos, np));
+  FL_TRY(dalloc(&c->err, 4));
+  FL_TRY(dalloc(&c->reduce_i32, 16));
+  FL_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->host_words), 64 * sizeof(uint64_t), hipHostMallocDefault));
+  return FL_OK;
+}
+
+void release(fl_ctx* c) {
+  void* ptrs[] = {c->raw.base, c->sorted.base, c->staging_aos, c->bin, c->key, c->rank, c->perm, c->perm_tmp,
+                  c->bin_count, c->bin_scan, c->bin_key, c->bin_offset, c->bin_n, c->scan_partial, c->blocks,
+                  c->lut, c->mark, c->qm, c->qp, c->face_mass, c->u0, c->ustar, c->unew, c->validity,
+                  c->quanta_aos, c->gather, c->gather_aos, c->err, c->reduce_i32};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->host_words) (void)hipHostFree(c->host_words);
+}
+
+// Uploads caller particle records, bins them (deposit or gather checks) and sorts them by bin.
+int stage_particles(fl_ctx* ctx, const fl_particle* particles, size_t n, bool deposit, double* copy_ms) {
+  Timer t(ctx);
+  if (n) FL_HIP(hipMemcpyAsync(ctx->staging_aos, particles, n * sizeof(fl_particle), hipMemcpyHostToDevice, ctx->stream));
+  *copy_ms += t.stop();
+  FL_TRY(unpack_particles(ctx, n));
+  FL_TRY(bin_particles(ctx, ctx->raw.view(), n, deposit, !deposit));
+  if (ctx->bin_box.volume() > ctx->cap.lookup_blocks) return FL_RESOURCE_EXHAUSTED;
+  return sort_particles(ctx, n, ctx->raw.view(), true);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fl_create(const fl_capacities* capacities, int device, fl_ctx** out) {
+  if (!capacities || !out) return FL_INVALID_ARGUMENT;
+  if (capacities->particles > 0xffffffffULL || capacities->lookup_blocks > 0x7fffffffULL) return FL_RESOURCE_EXHAUSTED;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) return FL_FAILED_PRECONDITION;
+  auto* c = new (std::nothrow) fl_ctx();
+  if (!c) return FL_RESOURCE_EXHAUSTED;
+  c->device = device;
+  c->cap = *capacities;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete c;
+    return FL_FAILED_PRECONDITION;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess)
+    std::snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
+  int s = FL_OK;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+    s = FL_INTERNAL;
+  for (int i = 0; i < 8 && s == FL_OK; ++i)
+    if (hipEventCreate(&c->stage[i]) != hipSuccess) s = FL_INTERNAL;
+  if (s == FL_OK) s = allocate(c);
+  if (s != FL_OK) {
+    fl_destroy(c);
+    return s == FL_INTERNAL ? FL_RESOURCE_EXHAUSTED : s;
+  }
+  *out = c;
+  return FL_OK;
+}
+
+void fl_destroy(fl_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  fl::destroy_pressure(ctx);
+  release(ctx);
+  for (hipEvent_t e : ctx->stage)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int fl_device_name(fl_ctx* ctx, char* out, size_t size) {
+  if (!ctx || !out || !size) return FL_INVALID_ARGUMENT;
+  std::snprintf(out, size, "%s", ctx->arch);
+  return FL_OK;
+}
+
+int fl_bin_sort(fl_ctx* ctx, const fl_geometry* geometry, const fl_vec3* positions, size_t n, fl_sort_entry* out,
+                fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  FL_TRY(check_geometry(geometry));
+  if (n > ctx->cap.particles) return FL_RESOURCE_EXHAUSTED;
+  if (n && (!positions || !out)) return FL_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  set_geometry(ctx, geometry);
+  double copy_ms = 0;
+  {
+    Timer t(ctx);
+    if (n) FL_HIP(hipMemcpyAsync(ctx->staging_aos, positions, n * sizeof(fl_vec3), hipMemcpyHostToDevice, ctx->stream));
+    copy_ms += t.stop();
+  }
+  Timer t(ctx);
+  if (n) {
+    k_split_columns<<<grid_for(n), kThreads, 0, ctx->stream>>>(reinterpret_cast<const double*>(ctx->staging_aos), n, 3,
+                                                              3, ctx->raw.base, ctx->raw.capacity);
+    FL_HIP(hipGetLastError());
+  }
+  ctx->raw.trial_aliases_sorted = true;
+  FL_TRY(bin_particles(ctx, ctx->raw.view(), n, false, false));
+  FL_TRY(first_error_code(ctx));
+  if (ctx->bin_box.volume() > ctx->cap.lookup_blocks) return FL_RESOURCE_EXHAUSTED;
+  FL_TRY(sort_particles(ctx, n, ctx->raw.view(), false));
+  auto* d_out = reinterpret_cast<fl_sort_entry*>(ctx->staging_aos);
+  FL_TRY(write_sort_entries(ctx, n, d_out));
+  const double kernel_ms = t.stop();
+  Timer c(ctx);
+  if (n) FL_HIP(hipMemcpyAsync(out, d_out, n * sizeof(fl_sort_entry), hipMemcpyDeviceToHost, ctx->stream));
+  copy_ms += c.stop();
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, 8);
+  return FL_OK;
+}
+
+int fl_activate(fl_ctx* ctx, const fl_geometry* geometry, const fl_vec3* positions, size_t n,
+                const fl_int3* pressure_cells, size_t cell_count, fl_int3* out_blocks, size_t* io_count,
+                fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  FL_TRY(check_geometry(geometry));
+  if (!io_count) return FL_INVALID_ARGUMENT;
+  if (n > ctx->cap.particles || cell_count > ctx->cap.pressure_cells) return FL_RESOURCE_EXHAUSTED;
+  if (cell_count * sizeof(fl_int3) > staging_bytes(ctx)) return FL_RESOURCE_EXHAUSTED;
+  (void)hipSetDevice(ctx->device);
+  set_geometry(ctx, geometry);
+  double copy_ms = 0;
+  {
+    Timer t(ctx);
+    if (n) FL_HIP(hipMemcpyAsync(ctx->staging_aos, positions, n * sizeof(fl_vec3), hipMemcpyHostToDevice, ctx->stream));
+    copy_ms += t.stop();
+  }
+  Timer t(ctx);
+  if (n) {
+    k_split_columns<<<grid_for(n), kThreads, 0, ctx->stream>>>(reinterpret_cast<const double*>(ctx->staging_aos), n, 3,
+                                                              3, ctx->raw.base, ctx->raw.capacity);
+    FL_HIP(hipGetLastError());
+  }
+  ctx->raw.trial_aliases_sorted = true;
+  FL_TRY(bin_particles(ctx, ctx->raw.view(), n, false, false));
+  FL_TRY(first_error_code(ctx));
+  // Cells: INT32_MAX coordinates are unrepresentable (liquid_step_workspace_3d.cc:225-231).
+  int32_t clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};
+  for (size_t i = 0; i < cell_count; ++i) {
+    const fl_int3 c = pressure_cells[i];
+    if (c.x == 2147483647 || c.y == 2147483647 || c.z == 2147483647) return FL_OUT_OF_RANGE;
+    const int32_t o[3] = {int32_t(floor_div4(c.x)), int32_t(floor_div4(c.y)), int32_t(floor_div4(c.z))};
+    const int32_t p[3] = {int32_t(floor_div4(int64_t(c.x) + 1)), int32_t(floor_div4(int64_t(c.y) + 1)),
+                          int32_t(floor_div4(int64_t(c.z) + 1))};
+    for (int a = 0; a < 3; ++a) {
+      if (i == 0 || o[a] < clo[a]) clo[a] = o[a];
+      if (i == 0 || p[a] > chi[a]) chi[a] = p[a];
+    }
+  }
+  if (ctx->bin_box.volume() > ctx->cap.lookup_blocks) return FL_RESOURCE_EXHAUSTED;
+  FL_TRY(sort_particles(ctx, n, ctx->raw.view(), true, 0x7u));
+  Box box;
+  bool any = false;
+  int32_t lo[3], hi[3];
+  if (n && ctx->bin_box.volume()) {
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = ctx->bin_box.lo[a];
+      hi[a] = ctx->bin_box.lo[a] + ctx->bin_box.dim[a];  // bins B..B+1 own the stencils
+    }
+    any = true;
+  }
+  if (cell_count) {
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = any ? std::min(lo[a], clo[a]) : clo[a];
+      hi[a] = any ? std::max(hi[a], chi[a]) : chi[a];
+    }
+    any = true;
+  }
+  if (any)
+    for (int a = 0; a < 3; ++a) {
+      box.lo[a] = lo[a];
+      box.dim[a] = hi[a] - lo[a] + 1;
+    }
+  if (box.volume() > ctx->cap.lookup_blocks) return FL_RESOURCE_EXHAUSTED;
+  ctx->block_box = box;
+  int32_t* d_cells = reinterpret_cast<int32_t*>(ctx->staging_aos);
+  if (cell_count)
+    FL_HIP(hipMemcpyAsync(d_cells, pressure_cells, cell_count * sizeof(fl_int3), hipMemcpyHostToDevice, ctx->stream));
+  FL_TRY(activate_blocks(ctx, ctx->nvalid, d_cells, cell_count));
+  FL_TRY(first_error_code(ctx));
+  const double kernel_ms = t.stop();
+  if (ctx->nb > *io_count) return FL_RESOURCE_EXHAUSTED;
+  Timer c(ctx);
+  if (ctx->nb) FL_HIP(hipMemcpyAsync(out_blocks, ctx->blocks, ctx->nb * sizeof(fl_int3), hipMemcpyDeviceToHost, ctx->stream));
+  copy_ms += c.stop();
+  *io_count = ctx->nb;
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, 12);
+  return FL_OK;
+}
+
+int fl_p2g_quanta(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
+                  const fl_particle* particles, size_t n, fl_quantization quantization, fl_face_quanta* out,
+                  size_t face_count, fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  FL_TRY(check_geometry(geometry));
+  FL_TRY(check_blocks(blocks, block_count));
+  if (!powers_of_two_ok(quantization)) return FL_INVALID_ARGUMENT;
+  if (quantization.declared_particle_capacity > ctx->cap.particles) return FL_RESOURCE_EXHAUSTED;
+  if (n > quantization.declared_particle_capacity) return FL_RESOURCE_EXHAUSTED;
+  if (face_count != 192 * block_count) return FL_INVALID_ARGUMENT;
+  if (overlaps(out, face_count * sizeof(fl_face_quanta), particles, n * sizeof(fl_particle))) return FL_INVALID_ARGUMENT;
+  if (block_count > ctx->cap.blocks) return FL_RESOURCE_EXHAUSTED;
+  (void)hipSetDevice(ctx->device);
+  set_geometry(ctx, geometry);
+  // Contribution limit 2^(62 - bit_width(cap - 1)) (transfers_3d.cc:164-168).
+  const uint64_t cap = std::max<uint64_t>(1, quantization.declared_particle_capacity);
+  int bits = 0;
+  for (uint64_t v = cap - 1; v; v >>= 1) ++bits;
+  const double limit = std::ldexp(1.0, 62 - bits);
+  double copy_ms = 0;
+  FL_TRY(build_lookup(ctx, reinterpret_cast<const int32_t*>(blocks), block_count));
+  Timer t(ctx);
+  FL_TRY(stage_particles(ctx, particles, n, true, &copy_ms));
+  FL_TRY(p2g_quanta(ctx, ctx->nvalid, 1.0 / quantization.mass_per_quantum,
+                    1.0 / quantization.pure_momentum_per_quantum, limit));
+  FL_TRY(first_error_code(ctx));
+  const uint64_t nf = face_count;
+  if (nf) {
+    k_pack_quanta<<<grid_for(nf), kThreads, 0, ctx->stream>>>(ctx->qm, ctx->qp, nf, ctx->quanta_aos);
+    FL_HIP(hipGetLastError());
+  }
+  const double kernel_ms = t.stop() - copy_ms;
+  Timer c(ctx);
+  if (nf) FL_HIP(hipMemcpyAsync(out, ctx->quanta_aos, nf * sizeof(fl_face_quanta), hipMemcpyDeviceToHost, ctx->stream));
+  copy_ms += c.stop();
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, 10);
+  return FL_OK;
+}
+
+int fl_normalize_forces(fl_ctx* ctx, const fl_face_quanta* quanta, size_t face_count, fl_quantization quantization,
+                        double minimum_face_mass, fl_vec3 acceleration, double dt, double* out_mass,
+                        double* out_velocity, uint8_t* out_validity, double* out_predicted,
+                        fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  if (face_count > 192 * ctx->cap.blocks) return FL_RESOURCE_EXHAUSTED;
+  if (!powers_of_two_ok(quantization)) return FL_INVALID_ARGUMENT;
+  if (!std::isfinite(minimum_face_mass) || minimum_face_mass < 0.0) return FL_INVALID_ARGUMENT;
+  if (face_count % 192 != 0) return FL_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  double copy_ms = 0;
+  Timer t0(ctx);
+  if (face_count)
+    FL_HIP(hipMemcpyAsync(ctx->quanta_aos, quanta, face_count * sizeof(fl_face_quanta), hipMemcpyHostToDevice,
+                          ctx->stream));
+  copy_ms += t0.stop();
+  Timer t(ctx);
+  FL_TRY(reset_error(ctx));
+  int64_t* qs = reinterpret_cast<int64_t*>(ctx->staging_aos);
+  if (face_count) {
+    k_split_quanta<<<grid_for(face_count), kThreads, 0, ctx->stream>>>(ctx->quanta_aos, face_count, ctx->qm, ctx->qp,
+                                                                       qs);
+    FL_HIP(hipGetLastError());
+  }
+  const double g[3] = {acceleration.x, acceleration.y, acceleration.z};
+  FL_TRY(normalize_forces_stress(ctx, qs, face_count, quantization.mass_per_quantum,
+                                 quantization.pure_momentum_per_quantum, minimum_face_mass, g, dt));
+  FL_HIP(hipMemcpyAsync(ctx->host_words, ctx->err, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  FL_HIP(hipStreamSynchronize(ctx->stream));
+  const double kernel_ms = t.stop();
+  if (ctx->host_words[0] != ~uint64_t(0)) return static_cast<int>(ctx->host_words[0] & 0xff);
+  if (ctx->host_words[1] != ~uint64_t(0)) return static_cast<int>(ctx->host_words[1] & 0xff);
+  Timer c(ctx);
+  const size_t bytes = face_count * sizeof(double);
+  if (face_count) {
+    if (out_mass) FL_HIP(hipMemcpyAsync(out_mass, ctx->face_mass, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (out_velocity) FL_HIP(hipMemcpyAsync(out_velocity, ctx->u0, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (out_validity) FL_HIP(hipMemcpyAsync(out_validity, ctx->validity, face_count, hipMemcpyDeviceToHost, ctx->stream));
+    if (out_predicted) FL_HIP(hipMemcpyAsync(out_predicted, ctx->ustar, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  copy_ms += c.stop();
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, 3);
+  return FL_OK;
+}
+
+int fl_g2p(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
+           const fl_particle* particles, size_t n, const double* old_face_velocities,
+           const double* new_face_velocities, size_t face_count, fl_gather_result* out, fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  FL_TRY(check_geometry(geometry));
+  FL_TRY(check_blocks(blocks, block_count));
+  if (face_count != 192 * block_count) return FL_INVALID_ARGUMENT;
+  if (n > ctx->cap.particles || block_count > ctx->cap.blocks) return FL_RESOURCE_EXHAUSTED;
+  const size_t out_bytes = n * sizeof(fl_gather_result);
+  if (overlaps(out, out_bytes, particles, n * sizeof(fl_particle)) ||
+      overlaps(out, out_bytes, old_face_velocities, face_count * sizeof(double)) ||
+      overlaps(out, out_bytes, new_face_velocities, face_count * sizeof(double)))
+    return FL_INVALID_ARGUMENT;
+  for (size_t f = 0; f < face_count; ++f)
+    if (!std::isfinite(old_face_velocities[f]) || !std::isfinite(new_face_velocities[f])) return FL_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  set_geometry(ctx, geometry);
+  double copy_ms = 0;
+  FL_TRY(build_lookup(ctx, reinterpret_cast<const int32_t*>(blocks), block_count));
+  {
+    Timer c(ctx);
+    if (face_count) {
+      FL_HIP(hipMemcpyAsync(ctx->u0, old_face_velocities, face_count * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+      FL_HIP(hipMemcpyAsync(ctx->unew, new_face_velocities, face_count * sizeof(double), hipMemcpyHostToDevice,
+                            ctx->stream));
+    }
+    copy_ms += c.stop();
+  }
+  Timer t(ctx);
+  FL_TRY(stage_particles(ctx, particles, n, false, &copy_ms));
+  FL_TRY(g2p(ctx, ctx->nvalid, ctx->u0, ctx->unew));
+  FL_TRY(first_error_code(ctx));
+  if (ctx->nvalid) {
+    k_pack_gather<<<grid_for(ctx->nvalid), kThreads, 0, ctx->stream>>>(ctx->gather, ctx->sorted.capacity, ctx->perm,
+                                                                       ctx->nvalid, ctx->gather_aos);
+    FL_HIP(hipGetLastError());
+  }
+  const double kernel_ms = t.stop() - copy_ms;
+  Timer c(ctx);
+  if (n) FL_HIP(hipMemcpyAsync(out, ctx->gather_aos, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
+  copy_ms += c.stop();
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, 10);
+  return FL_OK;
+}
+
+int fl_project(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count, double dt,
+               const fl_pressure_cell* cells, size_t cell_count, const fl_pressure_face* faces, size_t face_count,
+               const fl_pressure_options* options, double* out_pressures, double* out_velocities, double* out_rhs,
+               double* out_residuals, double* out_defects, uint64_t* out_gauges,
+               fl_pressure_diagnostics* out_diagnostics, fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  FL_TRY(check_geometry(geometry));
+  FL_TRY(check_blocks(blocks, block_count));
+  (void)hipSetDevice(ctx->device);
+  set_geometry(ctx, geometry);
+  return fl::project_pressure(ctx, geometry, blocks, block_count, dt, cells, cell_count, faces, face_count, options,
+                              out_pressures, out_velocities, out_rhs, out_residuals, out_defects, out_gauges,
+                              out_diagnostics, measurement);
+}
+
+}  // extern "C"

@@ -1,0 +1,185 @@
+// Host-side context of the HIP hot path: device buffers, stream, timing and launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+#include <vector>
+
+#include "../../include/fluidity_hip.h"
+#include "lattice.h"
+
+#define FL_HIP(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t fl_e_ = (expr);                                                                \
+    if (fl_e_ != hipSuccess) {                                                                \
+      std::fprintf(stderr, "fluidity_hip: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(fl_e_), \
+                   __FILE__, __LINE__);                                                       \
+      return FL_INTERNAL;                                                                     \
+    }                                                                                         \
+  } while (0)
+
+#define FL_TRY(expr)        \
+  do {                      \
+    int fl_s_ = (expr);     \
+    if (fl_s_) return fl_s_; \
+  } while (0)
+
+namespace fl {
+
+constexpr int kThreads = 256;
+constexpr int kParticleDoubles = 19;  // sorted xyz, trial xyz, mass, velocity xyz, slopes 3x3 row-major
+constexpr int kGatherDoubles = 24;    // MacParticleGatherResult3D
+
+// Structure-of-arrays particle view; trial may alias sorted.
+struct ParticleView {
+  const double* s[3];
+  const double* t[3];
+  const double* m;
+  const double* v[3];
+  const double* c[9];
+};
+
+struct ParticleBuffers {
+  double* base = nullptr;  // kParticleDoubles * capacity
+  uint64_t capacity = 0;
+  bool trial_aliases_sorted = false;
+  ParticleView view() const {
+    ParticleView p;
+    for (int a = 0; a < 3; ++a) p.s[a] = base + a * capacity;
+    for (int a = 0; a < 3; ++a) p.t[a] = trial_aliases_sorted ? p.s[a] : base + (3 + a) * capacity;
+    p.m = base + 6 * capacity;
+    for (int a = 0; a < 3; ++a) p.v[a] = base + (7 + a) * capacity;
+    for (int k = 0; k < 9; ++k) p.c[k] = base + (10 + k) * capacity;
+    return p;
+  }
+  double* column(int k) const { return base + k * capacity; }
+};
+
+// Dense ordinal table over a box of blocks (or bins), (z,y,x) linear order.
+struct Box {
+  int32_t lo[3] = {0, 0, 0};
+  int32_t dim[3] = {0, 0, 0};
+  __host__ __device__ uint64_t volume() const { return uint64_t(dim[0]) * dim[1] * dim[2]; }
+  __host__ __device__ int64_t index(int32_t x, int32_t y, int32_t z) const {
+    const int64_t rx = int64_t(x) - lo[0], ry = int64_t(y) - lo[1], rz = int64_t(z) - lo[2];
+    if (rx < 0 || ry < 0 || rz < 0 || rx >= dim[0] || ry >= dim[1] || rz >= dim[2]) return -1;
+    return (rz * dim[1] + ry) * dim[0] + rx;
+  }
+};
+
+// The sparse grid as the kernels see it: canonical blocks plus a dense lookup over their box.
+struct GridView {
+  Geometry g;
+  const int32_t* blocks;  // 3 * nb, xyz interleaved
+  const int32_t* lut;     // box.volume() ordinals, -1 if inactive
+  Box box;
+  uint64_t nb;
+  __device__ int32_t ordinal(int32_t x, int32_t y, int32_t z) const {
+    const int64_t i = box.index(x, y, z);
+    return i < 0 ? -1 : lut[i];
+  }
+};
+
+// Non-empty bins of the sorted particle order.
+struct BinList {
+  const uint32_t* key;     // linear key in the bin box
+  const uint32_t* offset;  // first sorted particle
+  const uint32_t* count;
+  uint32_t n;
+  Box box;
+};
+
+}  // namespace fl
+
+struct fl_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t stage[8] = {};
+  fl_capacities cap{};
+  char arch[32] = {};
+  bool poisoned = false;
+
+  // particles
+  fl::ParticleBuffers raw, sorted;
+  fl_particle* staging_aos = nullptr;  // device copy of caller records
+  int32_t* bin = nullptr;              // 3 * cap.particles
+  uint32_t* key = nullptr;
+  uint32_t* rank = nullptr;
+  uint32_t* perm = nullptr;
+  uint32_t* perm_tmp = nullptr;
+  // bins (capacity cap.lookup_blocks)
+  uint32_t* bin_count = nullptr;
+  uint64_t* bin_scan = nullptr;
+  uint32_t* bin_key = nullptr;
+  uint32_t* bin_offset = nullptr;
+  uint32_t* bin_n = nullptr;
+  uint64_t* scan_partial = nullptr;  // block sums of the scans
+  // grid
+  int32_t* blocks = nullptr;  // 3 * cap.blocks
+  int32_t* lut = nullptr;     // cap.lookup_blocks
+  uint8_t* mark = nullptr;    // activation bitmap, cap.lookup_blocks
+  uint64_t nb = 0;
+  fl::Box block_box;
+  fl::Geometry geometry{};
+  // faces (cap.blocks * 192)
+  int64_t* qm = nullptr;
+  int64_t* qp = nullptr;
+  double* face_mass = nullptr;
+  double* u0 = nullptr;
+  double* ustar = nullptr;
+  double* unew = nullptr;
+  uint8_t* validity = nullptr;
+  fl_face_quanta* quanta_aos = nullptr;
+  // gather output, kGatherDoubles * cap.particles (component-major)
+  double* gather = nullptr;
+  fl_gather_result* gather_aos = nullptr;
+  // scalars
+  uint64_t* err = nullptr;      // first error word
+  int32_t* reduce_i32 = nullptr;  // 16 ints: box min/max etc.
+  uint64_t* host_words = nullptr;  // pinned host scratch (64 words)
+
+  // device-resident scene
+  uint64_t scene_n = 0;
+  double scene_mass = 0;
+  bool scene_ready = false;
+  uint32_t nbins = 0;
+  uint64_t nvalid = 0;
+  fl::Box bin_box;
+
+  // pressure solver state (opaque, owned by pressure.hip)
+  void* pressure = nullptr;
+};
+
+namespace fl {
+
+inline unsigned grid_for(uint64_t n, int threads = kThreads) {
+  const uint64_t g = (n + threads - 1) / threads;
+  return static_cast<unsigned>(g == 0 ? 1 : g);
+}
+
+// Topology (topology.hip)
+int bin_particles(fl_ctx* ctx, const ParticleView& p, uint64_t n, bool deposit_checks, bool gather_checks);
+int sort_particles(fl_ctx* ctx, uint64_t n, const ParticleView& src, bool permute, uint32_t permute_columns = ~0u);
+int build_lookup(fl_ctx* ctx, const int32_t* host_blocks, uint64_t nb);
+int activate_blocks(fl_ctx* ctx, uint64_t n, const int32_t* d_cells, uint64_t cell_count);
+int write_sort_entries(fl_ctx* ctx, uint64_t n, fl_sort_entry* d_out);
+GridView grid_view(fl_ctx* ctx);
+BinList bin_list(fl_ctx* ctx);
+
+// Transfers (transfer.hip)
+int p2g_quanta(fl_ctx* ctx, uint64_t n, double inv_qm, double inv_qp, double limit);
+int normalize_forces(fl_ctx* ctx, uint64_t nf, double qm, double qp, double min_mass, const double g[3], double dt);
+int g2p(fl_ctx* ctx, uint64_t n, const double* d_old, const double* d_new);
+int unpack_particles(fl_ctx* ctx, uint64_t n);
+
+// Small helpers (topology.hip)
+int read_error(fl_ctx* ctx, uint64_t* word);
+int reset_error(fl_ctx* ctx);
+int fill_i64(fl_ctx* ctx, int64_t* p, uint64_t n, int64_t v);
+int fill_i32(fl_ctx* ctx, int32_t* p, uint64_t n, int32_t v);
+int exclusive_scan_u64(fl_ctx* ctx, const uint64_t* in, uint64_t* out, uint64_t n);
+
+}  // namespace fl

@@ -1,0 +1,202 @@
+"""ctypes binding of fluidity_amd/libfluidity_hip.so (the C-ABI of include/fluidity_hip.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback behind this module.
+Every function returns the C status code first, mirroring the reference's absl::Status returns.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import records as R
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfluidity_hip.so")
+
+OK, INVALID_ARGUMENT, RESOURCE_EXHAUSTED, FAILED_PRECONDITION, ABORTED, OUT_OF_RANGE, INTERNAL = 0, 3, 8, 9, 10, 11, 13
+
+
+class Vec3(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
+
+
+class Quantization(C.Structure):
+    _fields_ = [("mass_per_quantum", C.c_double), ("pure_momentum_per_quantum", C.c_double),
+                ("declared_particle_capacity", C.c_uint64)]
+
+
+class Measurement(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("copy_ms", C.c_double), ("launches", C.c_uint32),
+                ("architecture", C.c_char * 28)]
+
+
+class Capacities(C.Structure):
+    _fields_ = [("particles", C.c_uint64), ("blocks", C.c_uint64), ("lookup_blocks", C.c_uint64),
+                ("pressure_cells", C.c_uint64), ("pressure_faces", C.c_uint64)]
+
+
+class StepReport(C.Structure):
+    _fields_ = [("sort_ms", C.c_double), ("p2g_ms", C.c_double), ("normalize_ms", C.c_double),
+                ("projection_ms", C.c_double), ("g2p_ms", C.c_double), ("total_ms", C.c_double),
+                ("particles", C.c_uint64), ("blocks", C.c_uint64), ("faces", C.c_uint64),
+                ("pressure_cells", C.c_uint64), ("nonempty_bins", C.c_uint64),
+                ("pressure", C.c_byte * R.PRESSURE_DIAGNOSTICS.itemsize), ("status", C.c_int32), ("pad", C.c_int32)]
+
+    def diagnostics(self):
+        return np.frombuffer(bytes(self.pressure), dtype=R.PRESSURE_DIAGNOSTICS)[0]
+
+
+EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_p2g_quanta",
+           "fl_normalize_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
+           "fl_scene_download")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (no CPU fallback exists)")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.fl_destroy.restype = None
+    return _lib
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _vec3(v):
+    return Vec3(*map(float, v))
+
+
+class Context:
+    """One device context (fl_create/fl_destroy); all device memory is sized by the capacities."""
+
+    def __init__(self, particles=1 << 16, blocks=1 << 14, lookup_blocks=1 << 20, pressure_cells=1 << 18,
+                 pressure_faces=1 << 20, device=0):
+        self.handle = C.c_void_p()
+        cap = Capacities(particles, blocks, lookup_blocks, pressure_cells, pressure_faces)
+        s = lib().fl_create(C.byref(cap), C.c_int(device), C.byref(self.handle))
+        if s != OK:
+            raise RuntimeError(f"fl_create failed with status {s}")
+        self.last = Measurement()
+
+    def close(self):
+        if self.handle:
+            lib().fl_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_name(self):
+        buf = C.create_string_buffer(64)
+        lib().fl_device_name(self.handle, buf, C.c_size_t(64))
+        return buf.value.decode()
+
+    def bin_sort(self, geometry, positions):
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(len(pos), R.SORT_ENTRY)
+        s = lib().fl_bin_sort(self.handle, _p(geometry), _p(pos), C.c_size_t(len(pos)), _p(out), C.byref(self.last))
+        return s, out
+
+    def activate(self, geometry, positions, cells=None, capacity=1 << 22):
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        cells = np.zeros(0, R.INT3) if cells is None else np.ascontiguousarray(cells)
+        out = np.zeros(capacity, R.INT3)
+        cnt = C.c_size_t(capacity)
+        s = lib().fl_activate(self.handle, _p(geometry), _p(pos), C.c_size_t(len(pos)), _p(cells),
+                              C.c_size_t(len(cells)), _p(out), C.byref(cnt), C.byref(self.last))
+        return s, out[: cnt.value].copy()
+
+    def p2g_quanta(self, geometry, blocks, particles, mass_q, mom_q, cap, out=None):
+        nf = 192 * len(blocks)
+        if out is None:
+            out = np.zeros(nf, R.FACE_QUANTA)
+        s = lib().fl_p2g_quanta(self.handle, _p(geometry), _p(blocks), C.c_size_t(len(blocks)), _p(particles),
+                                C.c_size_t(len(particles)), Quantization(mass_q, mom_q, cap), _p(out),
+                                C.c_size_t(len(out)), C.byref(self.last))
+        return s, out
+
+    def normalize_forces(self, quanta, mass_q, mom_q, min_mass, accel, dt):
+        nf = len(quanta)
+        m, u, v, p = np.zeros(nf), np.zeros(nf), np.zeros(nf, np.uint8), np.zeros(nf)
+        s = lib().fl_normalize_forces(self.handle, _p(quanta), C.c_size_t(nf), Quantization(mass_q, mom_q, 0),
+                                      C.c_double(min_mass), _vec3(accel), C.c_double(dt), _p(m), _p(u), _p(v), _p(p),
+                                      C.byref(self.last))
+        return s, m, u, v, p
+
+    def g2p(self, geometry, blocks, particles, old, new, out=None):
+        if out is None:
+            out = np.zeros(len(particles), R.GATHER_RESULT)
+        old = np.ascontiguousarray(old, dtype=np.float64)
+        new = np.ascontiguousarray(new, dtype=np.float64)
+        s = lib().fl_g2p(self.handle, _p(geometry), _p(blocks), C.c_size_t(len(blocks)), _p(particles),
+                         C.c_size_t(len(particles)), _p(old), _p(new), C.c_size_t(len(old)), _p(out),
+                         C.byref(self.last))
+        return s, out
+
+    def project(self, geometry, blocks, dt, cells, faces, options):
+        nc, nf = len(cells), len(faces)
+        p, u, b, r, d = (np.zeros(nc), np.zeros(nf), np.zeros(nc), np.zeros(nc), np.zeros(nc))
+        gauges = np.zeros(nc, np.uint64)
+        diag = np.zeros(1, R.PRESSURE_DIAGNOSTICS)
+        s = lib().fl_project(self.handle, _p(geometry), _p(blocks), C.c_size_t(len(blocks)), C.c_double(dt),
+                             _p(cells), C.c_size_t(nc), _p(faces), C.c_size_t(nf), _p(options), _p(p), _p(u), _p(b),
+                             _p(r), _p(d), _p(gauges), _p(diag), C.byref(self.last))
+        g = gauges[: int(diag["gauge_count"][0])].copy()
+        return s, dict(pressures=p, velocities=u, rhs=b, residuals=r, defects=d, gauges=g, diagnostics=diag[0])
+
+    # ---- device-resident scene ----
+    def scene_upload(self, geometry, positions, velocities, slopes, mass):
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        vel = np.ascontiguousarray(velocities, dtype=np.float64)
+        c = np.ascontiguousarray(slopes, dtype=np.float64)
+        return lib().fl_scene_upload(self.handle, _p(geometry), C.c_size_t(len(pos)), _p(pos), _p(vel), _p(c),
+                                     C.c_double(mass))
+
+    def scene_prepare(self, cells, domain_cells, density):
+        return lib().fl_scene_prepare(self.handle, _p(cells), C.c_size_t(len(cells)), C.c_int32(domain_cells),
+                                      C.c_double(density))
+
+    def step(self, dt, gravity, mass_q, mom_q, cap, options):
+        rep = StepReport()
+        s = lib().fl_step(self.handle, C.c_double(dt), _vec3(gravity), Quantization(mass_q, mom_q, cap), _p(options),
+                          C.byref(rep))
+        return s, rep
+
+    def scene_download(self, n_blocks, n_particles, n_cells, what=("blocks", "quanta", "u0", "ustar", "unew", "order",
+                                                                    "gather", "pressures")):
+        nf = 192 * n_blocks
+        out = {}
+        out["blocks"] = np.zeros(n_blocks, R.INT3) if "blocks" in what else None
+        out["quanta"] = np.zeros(nf, R.FACE_QUANTA) if "quanta" in what else None
+        for k in ("u0", "ustar", "unew"):
+            out[k] = np.zeros(nf) if k in what else None
+        out["order"] = np.zeros(n_particles, R.SORT_ENTRY) if "order" in what else None
+        out["gather"] = np.zeros(n_particles, R.GATHER_RESULT) if "gather" in what else None
+        out["pressures"] = np.zeros(n_cells) if "pressures" in what else None
+        s = lib().fl_scene_download(self.handle, _p(out["blocks"]), C.c_size_t(n_blocks), _p(out["quanta"]),
+                                    _p(out["u0"]), _p(out["ustar"]), _p(out["unew"]), _p(out["order"]),
+                                    _p(out["gather"]), _p(out["pressures"]))
+        return s, out
+
+
+def pressure_options(max_iterations=256, tolerance_scale=1.0, relative_target=0.0, levels=4, sweeps=2,
+                     coarse_sweeps=16, coarse_scale=0.5):
+    o = np.zeros(1, R.PRESSURE_OPTIONS)
+    o[0] = (max_iterations, tolerance_scale, relative_target, levels, sweeps, coarse_sweeps, 1, coarse_scale)
+    return o

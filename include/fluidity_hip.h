@@ -1,0 +1,216 @@
+/* C-ABI of the MI355X-native FluidiTy liquid-step hot path (P2G, G2P, MAC pressure MGPCG).
+ *
+ * One library, fluidity_amd/libfluidity_hip.so, built for gfx950. Every entry point returns an
+ * int status whose values equal absl::StatusCode (0 OK, 3 InvalidArgument, 8 ResourceExhausted,
+ * 9 FailedPrecondition, 10 Aborted, 11 OutOfRange, 13 Internal), as the reference operators do
+ * (native/include/fluidity/host/transfers_3d.h:17-22, mac_pressure_3d.h:95-100,
+ * native/cuda/operator_runtime.h:18-40). On any error, caller outputs are left untouched.
+ *
+ * Records mirror the reference PODs byte for byte: Vec3, Mat3, BlockIndex3D
+ * (native/include/fluidity/base/mac_types_3d.h:9-72), MacTransferParticle3D,
+ * MacDepositQuantization3D, MacFaceDepositQuanta3D, MacParticleGatherResult3D
+ * (native/include/fluidity/base/mac_transfer_types_3d.h:7-37), LiquidParticleSortEntry3D
+ * (native/include/fluidity/host/liquid_step_workspace_3d.h:14-17), and MacPressureCell3D /
+ * MacPressureFace3D with the face's std::variant link and std::optional wall flux flattened
+ * (native/include/fluidity/host/mac_pressure_3d.h:14-51).
+ *
+ * Threading: a context is single-threaded and non-reentrant, like the reference workspaces
+ * (native/include/fluidity/host/mac_pressure_workspace_3d.h:65). All device memory is
+ * allocated by fl_create; calls allocate nothing on the device.
+ */
+#ifndef FLUIDITY_HIP_H_
+#define FLUIDITY_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FL_OK = 0,
+  FL_INVALID_ARGUMENT = 3,
+  FL_RESOURCE_EXHAUSTED = 8,
+  FL_FAILED_PRECONDITION = 9,
+  FL_ABORTED = 10,
+  FL_OUT_OF_RANGE = 11,
+  FL_INTERNAL = 13,
+};
+
+typedef struct fl_vec3 { double x, y, z; } fl_vec3;
+typedef struct fl_mat3 { fl_vec3 x_row, y_row, z_row; } fl_mat3;
+typedef struct fl_int3 { int32_t x, y, z; } fl_int3;
+typedef struct fl_geometry { fl_vec3 world_origin; double cell_width; } fl_geometry;
+typedef struct fl_particle {
+  fl_vec3 sorted_world_position;
+  fl_vec3 trial_world_position;
+  double particle_mass;
+  fl_vec3 particle_velocity;
+  fl_mat3 particle_velocity_slopes;
+} fl_particle;
+typedef struct fl_quantization {
+  double mass_per_quantum;
+  double pure_momentum_per_quantum;
+  uint64_t declared_particle_capacity;
+} fl_quantization;
+typedef struct fl_face_quanta {
+  int64_t mass_quanta_sum;
+  int64_t pure_momentum_quanta_sum;
+  int64_t stress_force_quanta_sum;
+} fl_face_quanta;
+typedef struct fl_gather_result {
+  fl_vec3 new_pic_velocity;
+  fl_vec3 grid_velocity_increment;
+  fl_mat3 fitted_velocity_slopes;
+  fl_mat3 velocity_gradient;
+} fl_gather_result;
+typedef struct fl_sort_entry { fl_int3 bin_index; uint32_t pad; uint64_t original_ordinal; } fl_sort_entry;
+
+/* link_kind: 0 sealed, 1 internal (lift = pressure jump), 2 Dirichlet (lift = exterior pressure). */
+typedef struct fl_pressure_cell {
+  fl_int3 cell_index;
+  int32_t pad;
+  double fluid_volume;
+  double target_divergence;
+  double bulk_modulus; /* +inf selects zero compliance */
+  double previous_pressure;
+  double integrated_inward_wall_flux;
+} fl_pressure_cell;
+typedef struct fl_pressure_face {
+  int32_t axis;
+  fl_int3 lattice;
+  double open_area;
+  double inverse_density;
+  double pressure_gradient_distance;
+  double predicted_normal_velocity;
+  double wall_normal_velocity;
+  int32_t link_kind;
+  int32_t has_prescribed_wall_flux;
+  double lift;
+  double prescribed_wall_flux;
+} fl_pressure_face;
+
+typedef struct fl_pressure_options {
+  uint64_t max_iterations;             /* PCG budget; the reference range is [1, 256] */
+  double cpu_residual_tolerance_scale; /* scales both reference gates, in [1e-12, 1] */
+  double relative_divergence_target;   /* 0 disables; else also ||defect||2 <= target ||defect(p=0)||2 */
+  int32_t levels;                      /* geometric multigrid levels, >= 1 */
+  int32_t smoothing_sweeps;            /* red-black Gauss-Seidel sweeps before and after the coarse correction */
+  int32_t coarse_sweeps;               /* symmetric sweeps on the coarsest level */
+  int32_t fp32_vcycle;                 /* must be 1: V-cycle and CG vectors in FP32, dots and residuals in FP64 */
+  double coarse_scale;                 /* coarse operator = scale * Galerkin crossing sums */
+} fl_pressure_options;
+
+typedef struct fl_pressure_diagnostics {
+  uint64_t iteration_count;
+  uint64_t multigrid_level_count;
+  double true_residual_l2;
+  double max_volume_scaled_residual;
+  double max_cell_constraint_defect;
+  double rhs_l2;
+  double initial_defect_l2;
+  double defect_l2;
+  double defect_linf;
+  uint64_t restarts;
+  uint64_t gauge_count;
+} fl_pressure_diagnostics;
+
+/* Device time of the call's kernels (hipEvents on the context stream), like the reference's
+ * CudaOperatorMeasurement (native/include/fluidity/host/cuda_measurement.h:5-10). */
+typedef struct fl_measurement {
+  double kernel_ms;   /* kernels only */
+  double copy_ms;     /* host<->device staging of the span-based call */
+  uint32_t launches;
+  char architecture[28];
+} fl_measurement;
+
+typedef struct fl_capacities {
+  uint64_t particles;       /* per call */
+  uint64_t blocks;          /* active 4^3 blocks */
+  uint64_t lookup_blocks;   /* bounding-box volume, in blocks, of the dense block and bin tables */
+  uint64_t pressure_cells;
+  uint64_t pressure_faces;
+} fl_capacities;
+
+typedef struct fl_ctx fl_ctx;
+
+/* ---- lifecycle (CudaOperatorRuntime, native/cuda/operator_runtime.h:45-213) ---- */
+int fl_create(const fl_capacities* capacities, int device, fl_ctx** out);
+void fl_destroy(fl_ctx* ctx);
+/* Architecture string of the device the context runs on (gcnArchName). */
+int fl_device_name(fl_ctx* ctx, char* out, size_t size);
+
+/* ---- topology ---- */
+/* ComputeParticleSortBin3D for each position plus the stable (z,y,x)-bin / ordinal order of
+ * LiquidStepWorkspace3D::PrepareTopology (native/src/sparse_mac_grid_3d.cc:438-449;
+ * native/src/liquid_step_workspace_3d.cc:208-268). out has n entries in sorted order. */
+int fl_bin_sort(fl_ctx* ctx, const fl_geometry* geometry, const fl_vec3* positions, size_t n, fl_sort_entry* out,
+                fl_measurement* measurement);
+/* Block activation: owners of all 81 stencil faces of every particle and of the six faces of every
+ * pressure cell, sorted (z,y,x) and unique (native/src/liquid_step_workspace_3d.cc:208-244, :272-276).
+ * *io_count holds the capacity of out_blocks on entry and the block count on return. */
+int fl_activate(fl_ctx* ctx, const fl_geometry* geometry, const fl_vec3* positions, size_t n,
+                const fl_int3* pressure_cells, size_t cell_count, fl_int3* out_blocks, size_t* io_count,
+                fl_measurement* measurement);
+
+/* ---- transfers ---- */
+/* DepositMacQuanta3D (native/include/fluidity/host/transfers_3d.h:36-41; native/src/transfers_3d.cc:147-197).
+ * blocks: canonical sorted active blocks (SparseMacGrid3D); out has 192 * block_count records in
+ * owned-slot order. The reference's 2^20 particle cap is replaced by the context capacity. */
+int fl_p2g_quanta(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
+                  const fl_particle* particles, size_t n, fl_quantization quantization, fl_face_quanta* out,
+                  size_t face_count, fl_measurement* measurement);
+/* NormalizeLiquidMacFaces3D followed by ApplyForces (native/src/liquid_boundary_transfer_3d.cc:463-516;
+ * native/src/liquid_substep_3d_cpu.cc:156-178). validity: 0 missing, 1 valid, 3 low mass.
+ * Any output pointer may be null. */
+int fl_normalize_forces(fl_ctx* ctx, const fl_face_quanta* quanta, size_t face_count, fl_quantization quantization,
+                        double minimum_face_mass, fl_vec3 acceleration, double dt, double* out_mass,
+                        double* out_velocity, uint8_t* out_validity, double* out_predicted,
+                        fl_measurement* measurement);
+/* GatherMac3D (native/include/fluidity/host/transfers_3d.h:55-61; native/src/transfers_3d.cc:199-261). */
+int fl_g2p(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
+           const fl_particle* particles, size_t n, const double* old_face_velocities,
+           const double* new_face_velocities, size_t face_count, fl_gather_result* out, fl_measurement* measurement);
+
+/* ---- pressure ---- */
+/* ProjectMacPressure3D (native/include/fluidity/host/mac_pressure_3d.h:88-102;
+ * native/src/mac_pressure_workspace_3d.cc:462-659). Cells in (z,y,x) order; faces are the exact
+ * union of incident faces in owned-slot order. Any output pointer may be null. */
+int fl_project(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count, double dt,
+               const fl_pressure_cell* cells, size_t cell_count, const fl_pressure_face* faces, size_t face_count,
+               const fl_pressure_options* options, double* out_pressures, double* out_velocities, double* out_rhs,
+               double* out_residuals, double* out_defects, uint64_t* out_gauges,
+               fl_pressure_diagnostics* out_diagnostics, fl_measurement* measurement);
+
+/* ---- device-resident liquid step (no host staging inside the timed region) ----
+ * The scene is uploaded once (SoA particles, sorted world positions = trial positions), topology and
+ * pressure structure are prepared once, and fl_step runs bin-sort -> P2G -> normalize/forces ->
+ * projection -> G2P on the device. */
+typedef struct fl_step_report {
+  double sort_ms, p2g_ms, normalize_ms, projection_ms, g2p_ms, total_ms;
+  uint64_t particles, blocks, faces, pressure_cells, nonempty_bins;
+  fl_pressure_diagnostics pressure;
+  int32_t status;
+  int32_t pad;
+} fl_step_report;
+
+int fl_scene_upload(fl_ctx* ctx, const fl_geometry* geometry, size_t n, const double* positions_xyz,
+                    const double* velocities_xyz, const double* slopes_rowmajor, double particle_mass);
+/* Activates blocks on the device and uploads the pressure cells (z,y,x order; faces are derived
+ * on the device: liquid-liquid internal, liquid-air Dirichlet p=0 at distance h, and domain
+ * walls [0, domain_cells) sealed). */
+int fl_scene_prepare(fl_ctx* ctx, const fl_pressure_cell* cells, size_t cell_count, int32_t domain_cells,
+                     double density);
+int fl_step(fl_ctx* ctx, double dt, fl_vec3 gravity, fl_quantization quantization,
+            const fl_pressure_options* options, fl_step_report* report);
+/* Copies device results of the last fl_step back (any pointer may be null): face quanta (owned
+ * slots), u0, u*, corrected face velocities, sorted order, gather results in caller order. */
+int fl_scene_download(fl_ctx* ctx, fl_int3* blocks, size_t block_capacity, fl_face_quanta* quanta, double* u0,
+                      double* ustar, double* unew, fl_sort_entry* order, fl_gather_result* gather,
+                      double* pressures);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLUIDITY_HIP_H_ */
