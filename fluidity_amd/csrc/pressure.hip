@@ -28,7 +28,6 @@ namespace fl {
 namespace {
 
 constexpr double kEpsD = 2.220446049250313080847e-16;
-constexpr int kCellsPerBlock = 64;
 constexpr int kPartials = 2048;  // reduction partial slots
 
 struct Key3 {
@@ -51,7 +50,8 @@ struct Level {
   uint64_t nb = 0;
   std::vector<Key3> keys;
   int32_t* nbr = nullptr;    // [nb][6]: -x,+x,-y,+y,-z,+z
-  int32_t* child = nullptr;  // [nb][8] blocks of the finer level (levels > 0)
+  int32_t* child = nullptr;   // [nb][8] blocks of the finer level (levels > 0)
+  int32_t* parent = nullptr;  // [nb][2] coarser block and octant (all but the coarsest level)
   float* x = nullptr;
   float* b = nullptr;
   float* r = nullptr;
@@ -97,6 +97,7 @@ struct Pressure {
   double* rhs_abs = nullptr;
   double* p = nullptr;
   double* p_base = nullptr;
+  double* p_init = nullptr;  // warm start of every solve (previous pressure minus gauge pin)
   double* res = nullptr;
   double* def = nullptr;
   double* zero = nullptr;
@@ -114,6 +115,8 @@ struct Pressure {
   // host copies
   std::vector<int32_t> h_cell_comp_root;
   std::vector<uint64_t> h_gauges;
+  std::vector<Key3> h_unknown_cells;  // level-0 unknown cell coordinates
+  int levels_requested = 0;
   bool has_jump = false;
   double full_area = 0, full_volume = 0;
   double scale_coarse = 0.5;
@@ -136,6 +139,7 @@ void free_levels(Pressure& P) {
   for (auto& L : P.levels) {
     dfree(L.nbr);
     dfree(L.child);
+    dfree(L.parent);
     dfree(L.x);
     dfree(L.b);
     dfree(L.r);
@@ -516,6 +520,28 @@ __global__ void k_residual(uint64_t n, const int32_t* nbr, const float* k, const
   r[i] = b[i] - s;
 }
 
+// y = A x in the edge-difference form.
+__global__ void k_apply(uint64_t n, const int32_t* nbr, const float* k, const float* anchor, const float* diag,
+                        const float* x, float* y) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  if (diag[i] == 0.f) {
+    y[i] = 0.f;
+    return;
+  }
+  const uint64_t blk = i >> 6;
+  const int l = static_cast<int>(i & 63);
+  const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
+  const float xi = x[i];
+  float s = anchor[i] * xi;
+  for (int dir = 0; dir < 6; ++dir) {
+    const int64_t j = neighbour(nbr, blk, lx, ly, lz, dir);
+    if (j < 0) continue;
+    s += face_k(k, n, static_cast<int64_t>(i), j, dir) * (xi - x[j]);
+  }
+  y[i] = s;
+}
+
 __global__ void k_restrict(uint64_t nc_cells, const int32_t* child, const float* fr, float* cb) {
   const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (i >= nc_cells) return;
@@ -548,7 +574,7 @@ __global__ void k_prolong(uint64_t nf_cells, const int32_t* parent, const float*
 
 // Coarsest level: `sweeps` forward (red, black) then `sweeps` backward (black, red) sweeps from zero,
 // all inside one workgroup.
-__global__ void k_coarsest(uint64_t n, const int32_t* nbr, const float* k, const float* diag, const float* b, float* x,
+__global__ __launch_bounds__(1024) void k_coarsest(uint64_t n, const int32_t* nbr, const float* k, const float* diag, const float* b, float* x,
                            int sweeps) {
   for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) x[i] = 0.f;
   __syncthreads();
@@ -574,13 +600,6 @@ __global__ void k_coarsest(uint64_t n, const int32_t* nbr, const float* k, const
 }
 
 // ------------------------------------------------------------------ PCG kernels
-
-__device__ inline void block_sum3(double a, double b, double c, double* partial, unsigned slot) {
-  double v[4] = {a, 0.0, 0.0, c};
-  // reuse block_reduce: slot 0 sums a, slot 1 max b, slot 3 sums c
-  v[1] = b;
-  block_reduce(v, partial, slot);
-}
 
 __global__ void k_dot(uint64_t n, const float* a, const float* b, double* partial) {
   double s = 0.0;
@@ -724,7 +743,7 @@ int ensure_capacity(fl_ctx* ctx, uint64_t nc, uint64_t nf, uint64_t nfine) {
                   P.face_slot, P.face_area, P.face_beta, P.face_dist, P.face_ustar, P.face_uwall, P.face_flux,
                   P.face_has_flux, P.face_lift, P.face_scale, P.face_k, P.face_wall, P.face_u, P.cell_volume,
                   P.cell_s, P.cell_K, P.cell_prev, P.cell_inwall, P.cell_comp, P.cell_target, P.rhs, P.rhs_abs,
-                  P.p, P.p_base, P.res, P.def, P.zero, P.fine_vol, P.r32, P.x32, P.d32, P.q32, P.partial, P.scal,
+                  P.p, P.p_base, P.p_init, P.res, P.def, P.zero, P.fine_vol, P.r32, P.x32, P.d32, P.q32, P.partial, P.scal,
                   P.fscal, P.perr};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -749,7 +768,7 @@ int ensure_capacity(fl_ctx* ctx, uint64_t nc, uint64_t nf, uint64_t nfine) {
   FL_TRY(dal(&P.face_wall, cf));
   FL_TRY(dal(&P.face_u, cf));
   for (double** a : {&P.cell_volume, &P.cell_s, &P.cell_K, &P.cell_prev, &P.cell_inwall, &P.cell_comp, &P.cell_target,
-                     &P.rhs, &P.rhs_abs, &P.p, &P.p_base, &P.res, &P.def, &P.zero})
+                     &P.rhs, &P.rhs_abs, &P.p, &P.p_base, &P.p_init, &P.res, &P.def, &P.zero})
     FL_TRY(dal(a, cc));
   FL_TRY(dal(&P.fine_cell, cn));
   FL_TRY(dal(&P.fine_vol, cn));
@@ -785,30 +804,47 @@ struct HostSystem {
   std::vector<Key3> blocks0;
 };
 
-// Builds the block hierarchy on the host and uploads neighbour, child and parent tables.
+uint64_t pack_key(const Key3& k) {
+  return (uint64_t(uint32_t(k.z) ^ 0x80000000u) << 42) ^ (uint64_t(uint32_t(k.y) ^ 0x80000000u) << 21) ^
+         uint64_t(uint32_t(k.x) ^ 0x80000000u);
+}
+
+// Builds the block hierarchy on the host and uploads neighbour, child and parent tables. Levels are
+// added while aggregation still shrinks the set of unknown cells (the oracle's rule), up to
+// `levels_wanted`. Level 0 keeps every block that holds a pressure cell.
 int build_levels(fl_ctx* ctx, Pressure& P, const std::vector<Key3>& blocks0, int levels_wanted) {
   free_levels(P);
+  P.levels_requested = levels_wanted;
   std::vector<std::vector<Key3>> keys{blocks0};
-  std::vector<std::vector<int32_t>> parents;  // per level (except coarsest): [nb][2] parent block, octant
-  while (static_cast<int>(keys.size()) < levels_wanted) {
-    const auto& fine = keys.back();
+  std::vector<Key3> cells = P.h_unknown_cells;
+  std::vector<std::vector<int32_t>> parents;  // per level except the coarsest: [nb][2] parent block, octant
+  while (static_cast<int>(keys.size()) < levels_wanted && !cells.empty()) {
+    std::vector<Key3> coarse_cells(cells.size());
+    for (size_t i = 0; i < cells.size(); ++i)
+      coarse_cells[i] = {floor_half(cells[i].x), floor_half(cells[i].y), floor_half(cells[i].z)};
+    std::sort(coarse_cells.begin(), coarse_cells.end(), key_less);
+    coarse_cells.erase(std::unique(coarse_cells.begin(), coarse_cells.end()), coarse_cells.end());
+    if (coarse_cells.size() == cells.size()) break;
     std::vector<Key3> coarse;
-    coarse.reserve(fine.size() / 4 + 1);
-    for (const Key3& k : fine) coarse.push_back({floor_half(k.x), floor_half(k.y), floor_half(k.z)});
+    coarse.reserve(coarse_cells.size() / 8 + 1);
+    for (const Key3& c : coarse_cells)
+      coarse.push_back({int32_t(floor_div4(c.x)), int32_t(floor_div4(c.y)), int32_t(floor_div4(c.z))});
     std::sort(coarse.begin(), coarse.end(), key_less);
     coarse.erase(std::unique(coarse.begin(), coarse.end()), coarse.end());
     std::unordered_map<Key3, int32_t, Key3Hash> index;
     for (size_t i = 0; i < coarse.size(); ++i) index[coarse[i]] = static_cast<int32_t>(i);
-    std::vector<int32_t> par(2 * fine.size());
+    const auto& fine = keys.back();
+    std::vector<int32_t> par(2 * fine.size(), -1);
     for (size_t i = 0; i < fine.size(); ++i) {
       const Key3 pk{floor_half(fine[i].x), floor_half(fine[i].y), floor_half(fine[i].z)};
-      par[2 * i] = index[pk];
+      auto it = index.find(pk);
+      if (it == index.end()) continue;  // a block without unknowns has no coarse parent
+      par[2 * i] = it->second;
       par[2 * i + 1] = ((fine[i].z - 2 * pk.z) * 2 + (fine[i].y - 2 * pk.y)) * 2 + (fine[i].x - 2 * pk.x);
     }
     parents.push_back(std::move(par));
-    const bool shrinks = coarse.size() < fine.size();
     keys.push_back(std::move(coarse));
-    if (!shrinks && keys.back().size() <= 1) break;
+    cells = std::move(coarse_cells);
   }
   P.levels.resize(keys.size());
   for (size_t l = 0; l < keys.size(); ++l) {
@@ -830,9 +866,14 @@ int build_levels(fl_ctx* ctx, Pressure& P, const std::vector<Key3>& blocks0, int
     if (l > 0) {
       std::vector<int32_t> child(8 * L.nb, -1);
       const auto& par = parents[l - 1];
-      for (size_t i = 0; i < keys[l - 1].size(); ++i) child[8 * par[2 * i] + par[2 * i + 1]] = static_cast<int32_t>(i);
+      for (size_t i = 0; i < keys[l - 1].size(); ++i)
+        if (par[2 * i] >= 0) child[8 * par[2 * i] + par[2 * i + 1]] = static_cast<int32_t>(i);
       FL_TRY(dal(&L.child, 8 * L.nb));
       FL_HIP(hipMemcpy(L.child, child.data(), child.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    if (l + 1 < keys.size()) {
+      FL_TRY(dal(&L.parent, parents[l].size()));
+      FL_HIP(hipMemcpy(L.parent, parents[l].data(), parents[l].size() * sizeof(int32_t), hipMemcpyHostToDevice));
     }
     FL_TRY(dal(&L.x, n));
     FL_TRY(dal(&L.b, n));
@@ -841,32 +882,8 @@ int build_levels(fl_ctx* ctx, Pressure& P, const std::vector<Key3>& blocks0, int
     FL_TRY(dal(&L.anchor, n));
     FL_TRY(dal(&L.diag, n));
   }
-  // Parent tables live in the fine level's `child` slot of a separate array: store in r? Use a dedicated buffer.
-  for (size_t l = 0; l + 1 < keys.size(); ++l) {
-    int32_t* par = nullptr;
-    FL_TRY(dal(&par, parents[l].size()));
-    FL_HIP(hipMemcpy(par, parents[l].data(), parents[l].size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    // The finest level has no child table; keep the parent table there for levels > 0 as well by
-    // chaining through a side vector.
-    P.levels[l].child = l == 0 ? par : P.levels[l].child;
-    if (l > 0) {
-      // Level l already owns a child table; append its parent table after it in a new allocation.
-      int32_t* both = nullptr;
-      const uint64_t nb = P.levels[l].nb;
-      FL_TRY(dal(&both, 8 * nb + 2 * nb));
-      FL_HIP(hipMemcpy(both, P.levels[l].child, 8 * nb * sizeof(int32_t), hipMemcpyDeviceToDevice));
-      FL_HIP(hipMemcpy(both + 8 * nb, par, 2 * nb * sizeof(int32_t), hipMemcpyDeviceToDevice));
-      dfree(P.levels[l].child);
-      dfree(par);
-      P.levels[l].child = both;
-    }
-  }
+  (void)pack_key;
   return FL_OK;
-}
-
-// Parent table of level l (valid for l + 1 < levels).
-const int32_t* parent_table(const Pressure& P, size_t l) {
-  return l == 0 ? P.levels[0].child : P.levels[l].child + 8 * P.levels[l].nb;
 }
 
 int setup_coefficients(fl_ctx* ctx, Pressure& P) {
@@ -917,7 +934,7 @@ int vcycle(fl_ctx* ctx, Pressure& P, size_t l, int nu, int coarse_sweeps) {
   k_restrict<<<grid_for(nc), kThreads, 0, s>>>(nc, C.child, L.r, C.b);
   FL_HIP(hipGetLastError());
   FL_TRY(vcycle(ctx, P, l + 1, nu, coarse_sweeps));
-  k_prolong<<<g, kThreads, 0, s>>>(n, parent_table(P, l), L.diag, C.x, L.x);
+  k_prolong<<<g, kThreads, 0, s>>>(n, L.parent, L.diag, C.x, L.x);
   for (int it = 0; it < nu; ++it) {
     k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, L.b, L.x, 1, 0);
     k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, L.b, L.x, 0, 0);
@@ -992,20 +1009,17 @@ int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates&
   publish(e);
   if (e.code) return e.code;
   if (accept(e, gates)) return FL_OK;
-  uint64_t unknowns = 0;
-  for (uint8_t f : std::vector<uint8_t>()) unknowns += f;  // counted on the host at setup
   k_load_residual<<<g, kThreads, 0, s>>>(n, P.fine_cell, P.cell_flag, P.res, P.r32, P.x32, P.perr);
   bool restart = true;
   int since = 0;
   for (uint64_t it = 1; it <= opt.max_iterations; ++it) {
-    hipMemcpyAsync(L0.b, P.r32, n * sizeof(float), hipMemcpyDeviceToDevice, s);
+    FL_HIP(hipMemcpyAsync(L0.b, P.r32, n * sizeof(float), hipMemcpyDeviceToDevice, s));
     FL_TRY(vcycle(ctx, P, 0, opt.smoothing_sweeps, opt.coarse_sweeps));
     k_dot<<<rg, kThreads, 0, s>>>(n, P.r32, L0.x, P.partial);
     k_finalize4<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, 20);
     k_after_rho<<<1, 64, 0, s>>>(P.scal, P.fscal, P.perr, restart ? 1 : 0);
     k_update_d<<<g, kThreads, 0, s>>>(n, L0.x, P.d32, P.fscal, restart ? 1 : 0);
-    k_residual<<<g, kThreads, 0, s>>>(n, L0.nbr, L0.k, L0.anchor, L0.diag, P.zero == nullptr ? nullptr : nullptr,
-                                      P.d32, P.q32);
+    k_apply<<<g, kThreads, 0, s>>>(n, L0.nbr, L0.k, L0.anchor, L0.diag, P.d32, P.q32);
     k_dot<<<rg, kThreads, 0, s>>>(n, P.d32, P.q32, P.partial);
     k_finalize4<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, 20);
     k_after_curv<<<1, 64, 0, s>>>(P.scal, P.fscal, P.perr);
@@ -1055,7 +1069,7 @@ void destroy_pressure(fl_ctx* ctx) {
                   P->face_slot, P->face_area, P->face_beta, P->face_dist, P->face_ustar, P->face_uwall, P->face_flux,
                   P->face_has_flux, P->face_lift, P->face_scale, P->face_k, P->face_wall, P->face_u, P->cell_volume,
                   P->cell_s, P->cell_K, P->cell_prev, P->cell_inwall, P->cell_comp, P->cell_target, P->rhs,
-                  P->rhs_abs, P->p, P->p_base, P->res, P->def, P->zero, P->fine_vol, P->r32, P->x32, P->d32, P->q32,
+                  P->rhs_abs, P->p, P->p_base, P->p_init, P->res, P->def, P->zero, P->fine_vol, P->r32, P->x32, P->d32, P->q32,
                   P->partial, P->scal, P->fscal, P->perr};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1065,3 +1079,722 @@ void destroy_pressure(fl_ctx* ctx) {
 }
 
 }  // namespace fl
+
+namespace fl {
+namespace {
+
+bool representable(double a, double b, double p) {
+  if (!std::isfinite(p)) return false;
+  if (a == 0.0 || b == 0.0 || std::fabs(p) >= std::numeric_limits<double>::min()) return true;
+  if (p == 0.0) return false;
+  int ea = 0, eb = 0;
+  const double ma = std::frexp(a, &ea), mb = std::frexp(b, &eb);
+  return std::fma(ma, mb, -std::scalbn(p, -(ea + eb))) == 0.0;
+}
+bool finite_positive(double v) { return std::isfinite(v) && v > 0.0; }
+
+struct HostComp {
+  double sum = 0.0, corr = 0.0;
+  void add(double t) {
+    const double n = sum + t;
+    if (std::fabs(sum) >= std::fabs(t)) corr += (sum - n) + t; else corr += (t - n) + sum;
+    sum = n;
+  }
+  double total() const { return sum + corr; }
+};
+
+int64_t host_floor_div4(int64_t i) { return i >= 0 ? i / 4 : -1 - (-(i + 1)) / 4; }
+
+// Validation and structure of one pressure system, in the order of Prepare
+// (mac_pressure_workspace_3d.cc:68-166) and SelectGaugesInto (mac_pressure_3d_arithmetic.h:328-399).
+// `check_values` also applies the per-value checks of FreezeLink and the RHS assembly to u*.
+int prepare_system(fl_ctx* ctx, Pressure& P, const fl_geometry* g, const fl_int3* grid_blocks, size_t nb, double dt,
+                   const fl_pressure_cell* cells, size_t nc, const fl_pressure_face* faces, size_t nf,
+                   bool check_values) {
+  if (nc == 0 || nf == 0 || !finite_positive(dt)) return FL_INVALID_ARGUMENT;
+  const double h = g->cell_width, area = h * h, volume = area * h;
+  if (!finite_positive(area) || !finite_positive(volume)) return FL_OUT_OF_RANGE;
+  if (nc > ctx->cap.pressure_cells || nf > ctx->cap.pressure_faces) return FL_RESOURCE_EXHAUSTED;
+  if (nc > 0x7fffffffULL || nf > 0x7fffffffULL) return FL_RESOURCE_EXHAUSTED;
+  P.full_area = area;
+  P.full_volume = volume;
+  std::unordered_map<Key3, int32_t, Key3Hash> grid;
+  grid.reserve(nb * 2);
+  for (size_t i = 0; i < nb; ++i) grid[{grid_blocks[i].x, grid_blocks[i].y, grid_blocks[i].z}] = int32_t(i);
+  auto slot_of = [&](int axis, int64_t x, int64_t y, int64_t z, uint64_t& slot) -> int {
+    if (axis < 0 || axis > 2) return FL_INVALID_ARGUMENT;
+    const Key3 o{int32_t(host_floor_div4(x)), int32_t(host_floor_div4(y)), int32_t(host_floor_div4(z))};
+    auto it = grid.find(o);
+    if (it == grid.end()) return FL_FAILED_PRECONDITION;
+    slot = uint64_t(it->second) * 192 + uint64_t(axis) * 64 +
+           uint64_t(((z - 4 * int64_t(o.z)) * 4 + (y - 4 * int64_t(o.y))) * 4 + (x - 4 * int64_t(o.x)));
+    return FL_OK;
+  };
+  // cells
+  std::vector<double> comp(nc), target(nc);
+  std::vector<uint64_t> incident;
+  incident.reserve(6 * nc);
+  std::unordered_map<Key3, int32_t, Key3Hash> cell_index;
+  cell_index.reserve(nc * 2);
+  for (size_t i = 0; i < nc; ++i) {
+    const fl_pressure_cell& c = cells[i];
+    if (i) {
+      const fl_int3 a = cells[i - 1].cell_index, b = c.cell_index;
+      const bool less = a.z != b.z ? a.z < b.z : (a.y != b.y ? a.y < b.y : a.x < b.x);
+      if (!less) return FL_INVALID_ARGUMENT;
+    }
+    if (!finite_positive(c.fluid_volume) || c.fluid_volume > volume) return FL_INVALID_ARGUMENT;
+    if (!std::isfinite(c.target_divergence) || !std::isfinite(c.previous_pressure) ||
+        !std::isfinite(c.integrated_inward_wall_flux))
+      return FL_INVALID_ARGUMENT;
+    double cmp = 0.0;
+    if (c.bulk_modulus != std::numeric_limits<double>::infinity()) {
+      if (!finite_positive(c.bulk_modulus)) return FL_INVALID_ARGUMENT;
+      const double kdt = c.bulk_modulus * dt;
+      if (!finite_positive(kdt)) return FL_OUT_OF_RANGE;
+      cmp = c.fluid_volume / kdt;
+      if (!finite_positive(cmp)) return FL_OUT_OF_RANGE;
+    }
+    const double tf = c.fluid_volume * c.target_divergence;
+    if (!representable(c.fluid_volume, c.target_divergence, tf)) return FL_OUT_OF_RANGE;
+    comp[i] = cmp;
+    target[i] = tf;
+    const int64_t xyz[3] = {c.cell_index.x, c.cell_index.y, c.cell_index.z};
+    for (int a = 0; a < 3; ++a) {
+      if (xyz[a] == 2147483647LL) return FL_OUT_OF_RANGE;
+      for (int side = 0; side < 2; ++side) {
+        int64_t l[3] = {xyz[0], xyz[1], xyz[2]};
+        l[a] += side;
+        uint64_t slot;
+        FL_TRY(slot_of(a, l[0], l[1], l[2], slot));
+        incident.push_back(slot);
+      }
+    }
+    cell_index[{c.cell_index.x, c.cell_index.y, c.cell_index.z}] = int32_t(i);
+  }
+  std::sort(incident.begin(), incident.end());
+  incident.erase(std::unique(incident.begin(), incident.end()), incident.end());
+  if (incident.size() != nf) return FL_INVALID_ARGUMENT;
+  // faces
+  std::vector<int32_t> tail(nf), head(nf), kind(nf);
+  std::vector<double> kcond(nf), wall(nf);
+  std::vector<uint64_t> slots(nf);
+  std::unordered_map<uint64_t, int32_t> face_of_slot;
+  face_of_slot.reserve(nf * 2);
+  bool has_jump = false;
+  for (size_t i = 0; i < nf; ++i) {
+    const fl_pressure_face& f = faces[i];
+    uint64_t slot;
+    FL_TRY(slot_of(f.axis, f.lattice.x, f.lattice.y, f.lattice.z, slot));
+    if (slot != incident[i]) return FL_INVALID_ARGUMENT;
+    slots[i] = slot;
+    face_of_slot[slot] = int32_t(i);
+    int64_t t[3] = {f.lattice.x, f.lattice.y, f.lattice.z};
+    t[f.axis] -= 1;
+    int32_t to = -1;
+    if (t[f.axis] >= -2147483648LL) {
+      auto it = cell_index.find({int32_t(t[0]), int32_t(t[1]), int32_t(t[2])});
+      if (it != cell_index.end()) to = it->second;
+    }
+    auto ih = cell_index.find({f.lattice.x, f.lattice.y, f.lattice.z});
+    const int32_t ho = ih == cell_index.end() ? -1 : ih->second;
+    // FreezeLinkFromIncidence (mac_pressure_3d_arithmetic.h:174-245)
+    if (!std::isfinite(f.open_area) || f.open_area < 0.0 || f.open_area > area) return FL_INVALID_ARGUMENT;
+    if (!std::isfinite(f.inverse_density) || !std::isfinite(f.pressure_gradient_distance) ||
+        !std::isfinite(f.predicted_normal_velocity) || !std::isfinite(f.wall_normal_velocity))
+      return FL_INVALID_ARGUMENT;
+    const bool both = to >= 0 && ho >= 0, one = (to >= 0) != (ho >= 0);
+    if (f.link_kind == 1) {
+      if (!both || !std::isfinite(f.lift)) return FL_INVALID_ARGUMENT;
+    } else if (f.link_kind == 2) {
+      if (!one || f.open_area == 0.0 || !std::isfinite(f.lift)) return FL_INVALID_ARGUMENT;
+    } else if (f.link_kind == 0) {
+      if (f.open_area != 0.0) return FL_INVALID_ARGUMENT;
+    } else {
+      return FL_INVALID_ARGUMENT;
+    }
+    double scale = 0.0, cond = 0.0;
+    if (f.open_area > 0.0) {
+      if (f.inverse_density <= 0.0 || f.pressure_gradient_distance <= 0.0) return FL_INVALID_ARGUMENT;
+      scale = (dt * f.inverse_density) / f.pressure_gradient_distance;
+      cond = f.open_area * scale;
+      if (!finite_positive(scale) || !finite_positive(cond)) return FL_OUT_OF_RANGE;
+    }
+    const double blocked = area - f.open_area;
+    double wf;
+    if (f.has_prescribed_wall_flux) {
+      wf = f.prescribed_wall_flux;
+      if (!std::isfinite(wf) || (blocked == 0 && wf != 0)) return FL_INVALID_ARGUMENT;
+    } else {
+      wf = blocked * f.wall_normal_velocity;
+      if (!representable(blocked, f.wall_normal_velocity, wf)) return FL_OUT_OF_RANGE;
+    }
+    tail[i] = to;
+    head[i] = ho;
+    kind[i] = f.link_kind;
+    kcond[i] = cond;
+    wall[i] = wf;
+    has_jump |= (cond > 0.0 && f.link_kind == 1 && f.lift != 0.0);
+  }
+  // RHS representability (AssembleIntegratedRhsInto, mac_pressure_3d_arithmetic.h:263-313)
+  if (check_values) {
+    for (size_t i = 0; i < nc; ++i)
+      if (!representable(comp[i], cells[i].previous_pressure, comp[i] * cells[i].previous_pressure))
+        return FL_OUT_OF_RANGE;
+    for (size_t i = 0; i < nf; ++i) {
+      const fl_pressure_face& f = faces[i];
+      if (!representable(f.open_area, f.predicted_normal_velocity, f.open_area * f.predicted_normal_velocity))
+        return FL_OUT_OF_RANGE;
+      if ((kind[i] == 1 || kind[i] == 2) && !representable(kcond[i], f.lift, kcond[i] * f.lift)) return FL_OUT_OF_RANGE;
+    }
+  }
+  // gauges: components over positive internal links, anchored by compliance or a Dirichlet link
+  std::vector<int32_t> parent(nc);
+  std::iota(parent.begin(), parent.end(), 0);
+  auto root = [&](int32_t c) {
+    while (parent[c] != c) c = parent[c];
+    return c;
+  };
+  for (size_t i = 0; i < nf; ++i)
+    if (kcond[i] > 0.0 && tail[i] >= 0 && head[i] >= 0) {
+      const int32_t a = root(tail[i]), b = root(head[i]);
+      parent[std::max(a, b)] = std::min(a, b);
+    }
+  std::vector<char> anchored(nc, 0);
+  std::vector<int32_t> component(nc);
+  for (size_t c = 0; c < nc; ++c) {
+    component[c] = root(int32_t(c));
+    if (comp[c] > 0.0) anchored[component[c]] = 1;
+  }
+  for (size_t i = 0; i < nf; ++i)
+    if (kind[i] == 2) {
+      if (tail[i] >= 0) anchored[component[tail[i]]] = 1;
+      if (head[i] >= 0) anchored[component[head[i]]] = 1;
+    }
+  std::vector<HostComp> bal(nc), babs(nc);
+  for (size_t c = 0; c < nc; ++c)
+    if (!anchored[component[c]]) {
+      for (double t : {target[c], cells[c].integrated_inward_wall_flux}) {
+        bal[component[c]].add(t);
+        babs[component[c]].add(std::fabs(t));
+      }
+    }
+  for (size_t i = 0; i < nf; ++i) {
+    const bool stays = tail[i] >= 0 && head[i] >= 0 && component[tail[i]] == component[head[i]];
+    if (stays) continue;
+    for (int side = 0; side < 2; ++side) {
+      const int32_t c = side == 0 ? tail[i] : head[i];
+      if (c < 0 || anchored[component[c]]) continue;
+      const double sgn = side == 0 ? 1.0 : -1.0;
+      bal[component[c]].add(-sgn * wall[i]);
+      babs[component[c]].add(std::fabs(-sgn * wall[i]));
+    }
+  }
+  std::vector<uint8_t> flag(nc, 1);
+  P.h_gauges.clear();
+  for (size_t c = 0; c < nc; ++c) {
+    if (component[c] != int32_t(c) || anchored[c]) continue;
+    const double b = bal[c].total(), a = babs[c].total();
+    if (!std::isfinite(b) || !std::isfinite(a)) return FL_OUT_OF_RANGE;
+    if (std::fabs(b) > 32.0 * 2.220446049250313e-16 * a) return FL_FAILED_PRECONDITION;
+    P.h_gauges.push_back(c);
+    flag[c] = 2;
+  }
+  P.h_cell_comp_root = component;
+  P.has_jump = has_jump;
+  P.h_unknown_cells.clear();
+  for (size_t c = 0; c < nc; ++c)
+    if (flag[c] == 1) P.h_unknown_cells.push_back({cells[c].cell_index.x, cells[c].cell_index.y, cells[c].cell_index.z});
+  // block layout of the fine level
+  std::vector<Key3> blocks0;
+  blocks0.reserve(nc / 8 + 1);
+  for (size_t c = 0; c < nc; ++c) {
+    const fl_int3 ci = cells[c].cell_index;
+    blocks0.push_back({int32_t(host_floor_div4(ci.x)), int32_t(host_floor_div4(ci.y)), int32_t(host_floor_div4(ci.z))});
+  }
+  std::sort(blocks0.begin(), blocks0.end(), key_less);
+  blocks0.erase(std::unique(blocks0.begin(), blocks0.end()), blocks0.end());
+  std::unordered_map<Key3, int32_t, Key3Hash> bidx;
+  bidx.reserve(blocks0.size() * 2);
+  for (size_t i = 0; i < blocks0.size(); ++i) bidx[blocks0[i]] = int32_t(i);
+  const uint64_t nfine = blocks0.size() * 64;
+  FL_TRY(ensure_capacity(ctx, nc, nf, nfine));
+  std::vector<int32_t> cell_fine(nc), fine_cell(nfine, -1), cell_faces(6 * nc);
+  for (size_t c = 0; c < nc; ++c) {
+    const fl_int3 ci = cells[c].cell_index;
+    const Key3 b{int32_t(host_floor_div4(ci.x)), int32_t(host_floor_div4(ci.y)), int32_t(host_floor_div4(ci.z))};
+    const int64_t fi = int64_t(bidx[b]) * 64 + ((ci.z - 4 * b.z) * 16 + (ci.y - 4 * b.y) * 4 + (ci.x - 4 * b.x));
+    cell_fine[c] = int32_t(fi);
+    fine_cell[fi] = int32_t(c);
+    const int64_t xyz[3] = {ci.x, ci.y, ci.z};
+    for (int a = 0; a < 3; ++a)
+      for (int side = 0; side < 2; ++side) {
+        int64_t l[3] = {xyz[0], xyz[1], xyz[2]};
+        l[a] += side;
+        uint64_t slot;
+        slot_of(a, l[0], l[1], l[2], slot);
+        cell_faces[6 * c + 2 * a + side] = face_of_slot[slot];
+      }
+  }
+  P.nc = nc;
+  P.nf = nf;
+  hipStream_t s = ctx->stream;
+  std::vector<double> V(nc), S(nc), K(nc), prev(nc), inw(nc);
+  for (size_t c = 0; c < nc; ++c) {
+    V[c] = cells[c].fluid_volume;
+    S[c] = cells[c].target_divergence;
+    K[c] = cells[c].bulk_modulus;
+    prev[c] = cells[c].previous_pressure;
+    inw[c] = cells[c].integrated_inward_wall_flux;
+  }
+  // Warm start (InitializeMacPressure3D, mac_pressure_3d_multigrid.cc:303-321).
+  std::vector<double> pin(nc, 0.0), p0(nc);
+  for (uint64_t gg : P.h_gauges) pin[component[gg]] = prev[gg];
+  for (size_t c = 0; c < nc; ++c) {
+    p0[c] = prev[c] - pin[component[c]];
+    if (!std::isfinite(p0[c])) return FL_OUT_OF_RANGE;
+  }
+  std::vector<double> fa(nf), fb(nf), fd(nf), fu(nf), fw(nf), ff(nf), fl(nf);
+  std::vector<int32_t> fh(nf);
+  for (size_t i = 0; i < nf; ++i) {
+    fa[i] = faces[i].open_area;
+    fb[i] = faces[i].inverse_density;
+    fd[i] = faces[i].pressure_gradient_distance;
+    fu[i] = faces[i].predicted_normal_velocity;
+    fw[i] = faces[i].wall_normal_velocity;
+    ff[i] = faces[i].prescribed_wall_flux;
+    fh[i] = faces[i].has_prescribed_wall_flux;
+    fl[i] = kind[i] == 0 ? 0.0 : faces[i].lift;
+  }
+#define UP(dst, src) FL_HIP(hipMemcpyAsync(dst, (src).data(), (src).size() * sizeof((src)[0]), hipMemcpyHostToDevice, s))
+  UP(P.cell_fine, cell_fine);
+  UP(P.cell_flag, flag);
+  UP(P.cell_faces, cell_faces);
+  UP(P.fine_cell, fine_cell);
+  UP(P.face_tail, tail);
+  UP(P.face_head, head);
+  UP(P.face_kind, kind);
+  UP(P.face_slot, slots);
+  UP(P.face_area, fa);
+  UP(P.face_beta, fb);
+  UP(P.face_dist, fd);
+  UP(P.face_ustar, fu);
+  UP(P.face_uwall, fw);
+  UP(P.face_flux, ff);
+  UP(P.face_has_flux, fh);
+  UP(P.face_lift, fl);
+  UP(P.cell_volume, V);
+  UP(P.cell_s, S);
+  UP(P.cell_K, K);
+  UP(P.cell_prev, prev);
+  UP(P.cell_inwall, inw);
+  UP(P.p_init, p0);
+#undef UP
+  FL_HIP(hipStreamSynchronize(s));
+  return build_levels(ctx, P, blocks0, 1);
+}
+
+// Freezes the system for dt on the device, assembles the RHS and the hierarchy coefficients, and
+// computes the gates. The warm start must be in P.p_base.
+int freeze_and_assemble(fl_ctx* ctx, Pressure& P, double dt, const fl_pressure_options& opt, Gates& gates,
+                        fl_pressure_diagnostics& diag) {
+  hipStream_t s = ctx->stream;
+  k_reset_err<<<1, 64, 0, s>>>(P.perr);
+  k_copy_f64<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, P.p_init, P.p_base);
+  k_freeze_faces<<<grid_for(P.nf), kThreads, 0, s>>>(P.nf, dt, P.full_area, P.face_area, P.face_beta, P.face_dist,
+                                                     P.face_uwall, P.face_has_flux, P.face_flux, P.face_scale, P.face_k,
+                                                     P.face_wall);
+  k_freeze_cells<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, dt, P.cell_volume, P.cell_s, P.cell_K, P.cell_comp,
+                                                     P.cell_target);
+  k_rhs<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, P.cell_faces, P.face_tail, P.face_kind, P.face_area, P.face_ustar,
+                                            P.face_wall, P.face_k, P.face_lift, P.cell_target, P.cell_inwall,
+                                            P.cell_comp, P.cell_prev, P.rhs, P.rhs_abs);
+  FL_HIP(hipGetLastError());
+  FL_TRY(setup_coefficients(ctx, P));
+  const unsigned g = reduce_grid(P.nc);
+  k_sum_sq<<<g, kThreads, 0, s>>>(P.nc, P.rhs, P.partial);
+  k_finalize4<<<1, kThreads, 0, s>>>(P.partial, g, P.scal, 24);
+  FL_HIP(hipMemcpyAsync(P.hscal + 24, P.scal + 24, sizeof(double), hipMemcpyDeviceToHost, s));
+  EvalResult e0{};
+  FL_TRY(evaluate(ctx, P, P.zero, e0));  // defect of the unprojected field (p = 0)
+  const double rhs_norm = std::sqrt(P.hscal[24]);
+  double limit = 2e-6 + 2e-6 * rhs_norm;
+  if (!std::isfinite(rhs_norm) || !std::isfinite(limit)) return FL_OUT_OF_RANGE;
+  if (rhs_norm > 0.0 && P.has_jump)
+    limit = std::min(limit, std::max(std::numeric_limits<double>::denorm_min(), 1e-3 * rhs_norm));
+  gates.integrated = limit * opt.cpu_residual_tolerance_scale;
+  gates.volume = 2e-5 * opt.cpu_residual_tolerance_scale;
+  gates.relative = opt.relative_divergence_target;
+  gates.initial_defect = e0.def_l2;
+  diag.rhs_l2 = rhs_norm;
+  diag.initial_defect_l2 = e0.def_l2;
+  diag.gauge_count = P.h_gauges.size();
+  k_reset_err<<<1, 64, 0, s>>>(P.perr);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+int check_options(const fl_pressure_options* o) {
+  if (!o) return FL_INVALID_ARGUMENT;
+  if (o->max_iterations < 1 || o->max_iterations > 256) return FL_INVALID_ARGUMENT;
+  if (!(o->cpu_residual_tolerance_scale >= 1e-12 && o->cpu_residual_tolerance_scale <= 1.0)) return FL_INVALID_ARGUMENT;
+  if (o->fp32_vcycle != 1 || o->levels < 1 || o->levels > 16 || o->smoothing_sweeps < 1 || o->coarse_sweeps < 1)
+    return FL_INVALID_ARGUMENT;
+  if (!(o->coarse_scale > 0.0 && o->coarse_scale <= 1.0)) return FL_INVALID_ARGUMENT;
+  if (!(o->relative_divergence_target >= 0.0)) return FL_INVALID_ARGUMENT;
+  return FL_OK;
+}
+
+}  // namespace
+
+int project_pressure(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count, double dt,
+                     const fl_pressure_cell* cells, size_t cell_count, const fl_pressure_face* faces,
+                     size_t face_count, const fl_pressure_options* options, double* out_pressures,
+                     double* out_velocities, double* out_rhs, double* out_residuals, double* out_defects,
+                     uint64_t* out_gauges, fl_pressure_diagnostics* out_diagnostics, fl_measurement* measurement) {
+  FL_TRY(check_options(options));
+  if (!ctx->pressure) ctx->pressure = new Pressure();
+  Pressure& P = *state(ctx);
+  P.scale_coarse = options->coarse_scale;
+  FL_TRY(prepare_system(ctx, P, geometry, blocks, block_count, dt, cells, cell_count, faces, face_count, true));
+  // Multigrid levels requested by the options.
+  std::vector<Key3> b0 = P.levels[0].keys;
+  FL_TRY(build_levels(ctx, P, b0, options->levels));
+  hipEvent_t a, b;
+  FL_HIP(hipEventCreate(&a));
+  FL_HIP(hipEventCreate(&b));
+  FL_HIP(hipEventRecord(a, ctx->stream));
+  Gates gates{};
+  fl_pressure_diagnostics diag{};
+  int st = freeze_and_assemble(ctx, P, dt, *options, gates, diag);
+  if (st == FL_OK) st = solve(ctx, P, *options, gates, diag);
+  if (st == FL_OK) {
+    k_face_velocity<<<grid_for(P.nf), kThreads, 0, ctx->stream>>>(P.nf, P.face_tail, P.face_head, P.face_kind,
+                                                                  P.face_area, P.face_ustar, P.face_scale,
+                                                                  P.face_lift, P.p, P.face_u);
+  }
+  (void)hipEventRecord(b, ctx->stream);
+  (void)hipEventSynchronize(b);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (out_diagnostics) *out_diagnostics = diag;
+  if (st != FL_OK) return st;
+  hipStream_t s = ctx->stream;
+  const size_t cb = cell_count * sizeof(double), fb = face_count * sizeof(double);
+  if (out_pressures) FL_HIP(hipMemcpyAsync(out_pressures, P.p, cb, hipMemcpyDeviceToHost, s));
+  if (out_velocities) FL_HIP(hipMemcpyAsync(out_velocities, P.face_u, fb, hipMemcpyDeviceToHost, s));
+  if (out_rhs) FL_HIP(hipMemcpyAsync(out_rhs, P.rhs, cb, hipMemcpyDeviceToHost, s));
+  if (out_residuals) FL_HIP(hipMemcpyAsync(out_residuals, P.res, cb, hipMemcpyDeviceToHost, s));
+  if (out_defects) FL_HIP(hipMemcpyAsync(out_defects, P.def, cb, hipMemcpyDeviceToHost, s));
+  FL_HIP(hipStreamSynchronize(s));
+  if (out_gauges) std::copy(P.h_gauges.begin(), P.h_gauges.end(), out_gauges);
+  if (measurement) {
+    measurement->kernel_ms = ms;
+    measurement->copy_ms = 0;
+    measurement->launches = 0;
+    std::snprintf(measurement->architecture, sizeof(measurement->architecture), "%s", ctx->arch);
+  }
+  return FL_OK;
+}
+
+}  // namespace fl
+
+// ---------------------------------------------------------------- device-resident liquid step
+
+namespace {
+using namespace fl;
+
+__global__ void k_split(const double* in, uint64_t n, int stride, int first, int columns, double* base, uint64_t cap) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < columns; ++c) base[(first + c) * cap + i] = in[i * stride + c];
+}
+__global__ void k_fill_f64(double* p, uint64_t n, double v) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_pack_gather_rows(const double* cols, uint64_t cap, const uint32_t* perm, uint64_t n,
+                                   fl_gather_result* out) {
+  const uint64_t k = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (k >= n) return;
+  double* r = reinterpret_cast<double*>(out + perm[k]);
+  for (int c = 0; c < kGatherDoubles; ++c) r[c] = cols[c * cap + k];
+}
+__global__ void k_pack_quanta_rows(const int64_t* qm, const int64_t* qp, uint64_t n, fl_face_quanta* out) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  out[i].mass_quanta_sum = qm[i];
+  out[i].pure_momentum_quanta_sum = qp[i];
+  out[i].stress_force_quanta_sum = 0;
+}
+
+int first_error(fl_ctx* ctx) {
+  uint64_t w;
+  FL_TRY(read_error(ctx, &w));
+  return w == ~uint64_t(0) ? FL_OK : static_cast<int>(w & 0xff);
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, a, b);
+  return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fl_scene_upload(fl_ctx* ctx, const fl_geometry* geometry, size_t n, const double* positions_xyz,
+                    const double* velocities_xyz, const double* slopes_rowmajor, double particle_mass) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  if (!geometry || !(geometry->cell_width > 0.0 && std::isnormal(geometry->cell_width))) return FL_INVALID_ARGUMENT;
+  if (n > ctx->cap.particles) return FL_RESOURCE_EXHAUSTED;
+  (void)hipSetDevice(ctx->device);
+  ctx->geometry = {geometry->world_origin.x, geometry->world_origin.y, geometry->world_origin.z, geometry->cell_width};
+  hipStream_t s = ctx->stream;
+  double* stage = reinterpret_cast<double*>(ctx->staging_aos);  // >= 19 doubles per particle
+  const struct {
+    const double* src;
+    int stride, first;
+  } parts[3] = {{positions_xyz, 3, 0}, {velocities_xyz, 3, 7}, {slopes_rowmajor, 9, 10}};
+  for (const auto& pa : parts) {
+    if (!n) break;
+    FL_HIP(hipMemcpyAsync(stage, pa.src, n * pa.stride * sizeof(double), hipMemcpyHostToDevice, s));
+    k_split<<<grid_for(n), kThreads, 0, s>>>(stage, n, pa.stride, pa.first, pa.stride, ctx->raw.base, ctx->raw.capacity);
+    FL_HIP(hipGetLastError());
+  }
+  if (n) k_fill_f64<<<grid_for(n), kThreads, 0, s>>>(ctx->raw.base + 6 * ctx->raw.capacity, n, particle_mass);
+  FL_HIP(hipGetLastError());
+  FL_HIP(hipStreamSynchronize(s));
+  ctx->raw.trial_aliases_sorted = true;
+  ctx->scene_n = n;
+  ctx->scene_mass = particle_mass;
+  ctx->scene_ready = false;
+  return FL_OK;
+}
+
+// Sort, activation, and the pressure structure of the scene. Faces of the liquid cells: both sides
+// liquid -> internal (jump 0); liquid/air inside the domain -> Dirichlet p = 0 at distance h;
+// on the domain boundary (lattice 0 or domain_cells along the normal) -> sealed, zero wall velocity.
+int fl_scene_prepare(fl_ctx* ctx, const fl_pressure_cell* cells, size_t cell_count, int32_t domain_cells,
+                     double density) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  if (!(density > 0.0) || domain_cells <= 0) return FL_INVALID_ARGUMENT;
+  if (cell_count > ctx->cap.pressure_cells) return FL_RESOURCE_EXHAUSTED;
+  (void)hipSetDevice(ctx->device);
+  const uint64_t n = ctx->scene_n;
+  FL_TRY(bin_particles(ctx, ctx->raw.view(), n, true, false));
+  FL_TRY(first_error(ctx));
+  if (ctx->bin_box.volume() > ctx->cap.lookup_blocks) return FL_RESOURCE_EXHAUSTED;
+  FL_TRY(sort_particles(ctx, n, ctx->raw.view(), true));
+  // activation box: particle windows plus cell owners
+  int32_t lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = ctx->bin_box.lo[a];
+    hi[a] = ctx->bin_box.lo[a] + ctx->bin_box.dim[a];
+  }
+  std::vector<int32_t> cxyz(3 * cell_count);
+  for (size_t i = 0; i < cell_count; ++i) {
+    const fl_int3 c = cells[i].cell_index;
+    cxyz[3 * i] = c.x;
+    cxyz[3 * i + 1] = c.y;
+    cxyz[3 * i + 2] = c.z;
+    const int32_t o[3] = {int32_t(floor_div4(c.x)), int32_t(floor_div4(c.y)), int32_t(floor_div4(c.z))};
+    const int32_t p[3] = {int32_t(floor_div4(int64_t(c.x) + 1)), int32_t(floor_div4(int64_t(c.y) + 1)),
+                          int32_t(floor_div4(int64_t(c.z) + 1))};
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], o[a]);
+      hi[a] = std::max(hi[a], p[a]);
+    }
+  }
+  Box box;
+  for (int a = 0; a < 3; ++a) {
+    box.lo[a] = lo[a];
+    box.dim[a] = hi[a] - lo[a] + 1;
+  }
+  if (box.volume() > ctx->cap.lookup_blocks) return FL_RESOURCE_EXHAUSTED;
+  ctx->block_box = box;
+  int32_t* d_cells = reinterpret_cast<int32_t*>(ctx->staging_aos);
+  if (cell_count * 12 > 152 * ctx->raw.capacity) return FL_RESOURCE_EXHAUSTED;
+  if (cell_count)
+    FL_HIP(hipMemcpyAsync(d_cells, cxyz.data(), cxyz.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+  FL_TRY(activate_blocks(ctx, ctx->nvalid, d_cells, cell_count));
+  FL_TRY(first_error(ctx));
+  std::vector<fl_int3> blocks(ctx->nb);
+  if (ctx->nb)
+    FL_HIP(hipMemcpy(blocks.data(), ctx->blocks, ctx->nb * sizeof(fl_int3), hipMemcpyDeviceToHost));
+  // faces of the liquid cells in owned-slot order
+  std::unordered_map<Key3, int32_t, Key3Hash> grid;
+  grid.reserve(2 * ctx->nb);
+  for (size_t i = 0; i < ctx->nb; ++i) grid[{blocks[i].x, blocks[i].y, blocks[i].z}] = int32_t(i);
+  const int64_t D = domain_cells;
+  std::vector<uint8_t> liquid(size_t(D) * D * D, 0);
+  for (size_t i = 0; i < cell_count; ++i) {
+    const fl_int3 c = cells[i].cell_index;
+    if (c.x < 0 || c.y < 0 || c.z < 0 || c.x >= D || c.y >= D || c.z >= D) return FL_INVALID_ARGUMENT;
+    liquid[(size_t(c.z) * D + c.y) * D + c.x] = 1;
+  }
+  auto is_liquid = [&](int64_t x, int64_t y, int64_t z) {
+    return x >= 0 && y >= 0 && z >= 0 && x < D && y < D && z < D && liquid[(size_t(z) * D + y) * D + x];
+  };
+  std::vector<std::pair<uint64_t, fl_pressure_face>> recs;
+  recs.reserve(4 * cell_count);
+  const double h = ctx->geometry.h;
+  for (size_t i = 0; i < cell_count; ++i) {
+    const fl_int3 c = cells[i].cell_index;
+    for (int a = 0; a < 3; ++a)
+      for (int side = 0; side < 2; ++side) {
+        int64_t l[3] = {c.x, c.y, c.z};
+        l[a] += side;
+        // Each face is emitted once: by its head cell, or by its tail when the head is not liquid.
+        int64_t t[3] = {l[0], l[1], l[2]};
+        t[a] -= 1;
+        const bool head_liquid = is_liquid(l[0], l[1], l[2]);
+        if (side == 1 && head_liquid) continue;
+        const Key3 o{int32_t(floor_div4(l[0])), int32_t(floor_div4(l[1])), int32_t(floor_div4(l[2]))};
+        auto it = grid.find(o);
+        if (it == grid.end()) return FL_INTERNAL;
+        const uint64_t slot = uint64_t(it->second) * 192 + uint64_t(a) * 64 +
+                              uint64_t(((l[2] - 4 * o.z) * 4 + (l[1] - 4 * o.y)) * 4 + (l[0] - 4 * o.x));
+        fl_pressure_face f{};
+        f.axis = a;
+        f.lattice = {int32_t(l[0]), int32_t(l[1]), int32_t(l[2])};
+        f.inverse_density = 1.0 / density;
+        f.pressure_gradient_distance = h;
+        const bool wall = l[a] == 0 || l[a] == D;
+        const bool both = head_liquid && is_liquid(t[0], t[1], t[2]);
+        f.link_kind = wall ? 0 : (both ? 1 : 2);
+        f.open_area = wall ? 0.0 : h * h;
+        recs.push_back({slot, f});
+      }
+  }
+  std::sort(recs.begin(), recs.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+  std::vector<fl_pressure_face> faces(recs.size());
+  for (size_t i = 0; i < recs.size(); ++i) faces[i] = recs[i].second;
+  if (!ctx->pressure) ctx->pressure = new Pressure();
+  Pressure& P = *static_cast<Pressure*>(ctx->pressure);
+  fl_geometry g{{ctx->geometry.ox, ctx->geometry.oy, ctx->geometry.oz}, h};
+  // dt only scales conductances here; each step refreezes with its own dt.
+  FL_TRY(prepare_system(ctx, P, &g, blocks.data(), blocks.size(), 1.0, cells, cell_count, faces.data(), faces.size(),
+                        false));
+  ctx->scene_ready = true;
+  return FL_OK;
+}
+
+int fl_step(fl_ctx* ctx, double dt, fl_vec3 gravity, fl_quantization quantization, const fl_pressure_options* options,
+            fl_step_report* report) {
+  if (!ctx || ctx->poisoned || !ctx->scene_ready || !ctx->pressure) return FL_FAILED_PRECONDITION;
+  if (!report) return FL_INVALID_ARGUMENT;
+  FL_TRY(check_options(options));
+  if (!(dt > 0.0) || !std::isfinite(dt)) return FL_INVALID_ARGUMENT;
+  int e;
+  if (!std::isnormal(quantization.mass_per_quantum) || std::frexp(quantization.mass_per_quantum, &e) != 0.5 ||
+      !std::isnormal(quantization.pure_momentum_per_quantum) ||
+      std::frexp(quantization.pure_momentum_per_quantum, &e) != 0.5)
+    return FL_INVALID_ARGUMENT;
+  if (quantization.declared_particle_capacity > ctx->cap.particles || ctx->scene_n > quantization.declared_particle_capacity)
+    return FL_RESOURCE_EXHAUSTED;
+  (void)hipSetDevice(ctx->device);
+  Pressure& P = *static_cast<Pressure*>(ctx->pressure);
+  if (P.levels_requested != options->levels || P.scale_coarse != options->coarse_scale) {
+    P.scale_coarse = options->coarse_scale;
+    std::vector<Key3> b0 = P.levels[0].keys;
+    FL_TRY(build_levels(ctx, P, b0, options->levels));
+  }
+  hipStream_t s = ctx->stream;
+  hipEvent_t* ev = ctx->stage;
+  *report = fl_step_report{};
+  const uint64_t n = ctx->scene_n;
+  const uint64_t cap = std::max<uint64_t>(1, quantization.declared_particle_capacity);
+  int bits = 0;
+  for (uint64_t v = cap - 1; v; v >>= 1) ++bits;
+  const double limit = std::ldexp(1.0, 62 - bits);
+  FL_HIP(hipEventRecord(ev[0], s));
+  FL_TRY(bin_particles(ctx, ctx->raw.view(), n, true, false));
+  FL_TRY(sort_particles(ctx, n, ctx->raw.view(), true));
+  FL_HIP(hipEventRecord(ev[1], s));
+  FL_TRY(p2g_quanta(ctx, ctx->nvalid, 1.0 / quantization.mass_per_quantum, 1.0 / quantization.pure_momentum_per_quantum,
+                    limit));
+  FL_HIP(hipEventRecord(ev[2], s));
+  const double g3[3] = {gravity.x, gravity.y, gravity.z};
+  const uint64_t nf_grid = ctx->nb * 192;
+  FL_TRY(normalize_forces(ctx, nf_grid, quantization.mass_per_quantum, quantization.pure_momentum_per_quantum, 0.0, g3,
+                          dt));
+  FL_HIP(hipEventRecord(ev[3], s));
+  FL_TRY(first_error(ctx));
+  // projection: u* at the pressure faces, freeze for dt, solve, corrected velocities into u_new
+  k_gather_slots<<<grid_for(P.nf), kThreads, 0, s>>>(P.nf, P.face_slot, ctx->ustar, P.face_ustar);
+  FL_HIP(hipGetLastError());
+  Gates gates{};
+  fl_pressure_diagnostics diag{};
+  int st = freeze_and_assemble(ctx, P, dt, *options, gates, diag);
+  if (st == FL_OK) st = solve(ctx, P, *options, gates, diag);
+  report->pressure = diag;
+  if (st != FL_OK) {
+    report->status = st;
+    return st;
+  }
+  k_face_velocity<<<grid_for(P.nf), kThreads, 0, s>>>(P.nf, P.face_tail, P.face_head, P.face_kind, P.face_area,
+                                                      P.face_ustar, P.face_scale, P.face_lift, P.p, P.face_u);
+  k_copy_f64<<<grid_for(nf_grid), kThreads, 0, s>>>(nf_grid, ctx->ustar, ctx->unew);
+  k_scatter_slots<<<grid_for(P.nf), kThreads, 0, s>>>(P.nf, P.face_slot, P.face_u, ctx->unew);
+  FL_HIP(hipGetLastError());
+  FL_HIP(hipEventRecord(ev[4], s));
+  FL_TRY(g2p(ctx, ctx->nvalid, ctx->u0, ctx->unew));
+  FL_HIP(hipEventRecord(ev[5], s));
+  FL_HIP(hipEventSynchronize(ev[5]));
+  st = first_error(ctx);
+  report->sort_ms = elapsed(ev[0], ev[1]);
+  report->p2g_ms = elapsed(ev[1], ev[2]);
+  report->normalize_ms = elapsed(ev[2], ev[3]);
+  report->projection_ms = elapsed(ev[3], ev[4]);
+  report->g2p_ms = elapsed(ev[4], ev[5]);
+  report->total_ms = elapsed(ev[0], ev[5]);
+  report->particles = n;
+  report->blocks = ctx->nb;
+  report->faces = P.nf;
+  report->pressure_cells = P.nc;
+  report->nonempty_bins = ctx->nbins;
+  report->status = st;
+  return st;
+}
+
+int fl_scene_download(fl_ctx* ctx, fl_int3* blocks, size_t block_capacity, fl_face_quanta* quanta, double* u0,
+                      double* ustar, double* unew, fl_sort_entry* order, fl_gather_result* gather, double* pressures) {
+  if (!ctx || ctx->poisoned || !ctx->scene_ready) return FL_FAILED_PRECONDITION;
+  (void)hipSetDevice(ctx->device);
+  hipStream_t s = ctx->stream;
+  const uint64_t nb = ctx->nb, nf = nb * 192, n = ctx->nvalid;
+  if (blocks) {
+    if (block_capacity < nb) return FL_RESOURCE_EXHAUSTED;
+    FL_HIP(hipMemcpyAsync(blocks, ctx->blocks, nb * sizeof(fl_int3), hipMemcpyDeviceToHost, s));
+  }
+  if (quanta) {
+    k_pack_quanta_rows<<<grid_for(nf), kThreads, 0, s>>>(ctx->qm, ctx->qp, nf, ctx->quanta_aos);
+    FL_HIP(hipMemcpyAsync(quanta, ctx->quanta_aos, nf * sizeof(fl_face_quanta), hipMemcpyDeviceToHost, s));
+  }
+  if (u0) FL_HIP(hipMemcpyAsync(u0, ctx->u0, nf * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (ustar) FL_HIP(hipMemcpyAsync(ustar, ctx->ustar, nf * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (unew) FL_HIP(hipMemcpyAsync(unew, ctx->unew, nf * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (order) {
+    auto* d = reinterpret_cast<fl_sort_entry*>(ctx->staging_aos);
+    FL_TRY(write_sort_entries(ctx, n, d));
+    FL_HIP(hipMemcpyAsync(order, d, n * sizeof(fl_sort_entry), hipMemcpyDeviceToHost, s));
+  }
+  if (gather) {
+    k_pack_gather_rows<<<grid_for(n), kThreads, 0, s>>>(ctx->gather, ctx->sorted.capacity, ctx->perm, n,
+                                                       ctx->gather_aos);
+    FL_HIP(hipMemcpyAsync(gather, ctx->gather_aos, n * sizeof(fl_gather_result), hipMemcpyDeviceToHost, s));
+  }
+  if (pressures && ctx->pressure) {
+    Pressure& P = *static_cast<Pressure*>(ctx->pressure);
+    FL_HIP(hipMemcpyAsync(pressures, P.p, P.nc * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  FL_HIP(hipGetLastError());
+  FL_HIP(hipStreamSynchronize(s));
+  return FL_OK;
+}
+
+}  // extern "C"

@@ -215,7 +215,7 @@ def activate_blocks(geom, positions, cells=None, capacity=1 << 24):
 
 
 def pressure_options(max_iterations=256, tolerance_scale=1.0, relative_target=0.0, levels=4, sweeps=2,
-                     coarse_sweeps=16, fp32=True, coarse_scale=1.0):
+                     coarse_sweeps=16, fp32=True, coarse_scale=0.5):
     o = np.zeros(1, PRESSURE_OPTIONS)
     o[0] = (max_iterations, tolerance_scale, relative_target, levels, sweeps, coarse_sweeps, int(fp32), coarse_scale)
     return o
