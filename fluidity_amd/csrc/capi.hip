@@ -92,7 +92,7 @@ bool overlaps(const void* a, size_t an, const void* b, size_t bn) {
 }
 
 void set_geometry(fl_ctx* ctx, const fl_geometry* g) {
-  ctx->geometry = {g->world_origin.x, g->world_origin.y, g->world_origin.z, g->cell_width};
+  ctx->geometry = make_geometry(g->world_origin.x, g->world_origin.y, g->world_origin.z, g->cell_width);
 }
 
 // hipEvent interval on the context stream; each timer owns its events so timers may nest.
@@ -163,7 +163,7 @@ int allocate(fl_ctx* c) {
   FL_TRY(dalloc(&c->bin_key, lk));
   FL_TRY(dalloc(&c->bin_offset, lk));
   FL_TRY(dalloc(&c->bin_n, lk));
-  FL_TRY(dalloc(&c->scan_partial, lk / 128 + 4096));
+  FL_TRY(dalloc(&c->scan_partial, std::max<uint64_t>(lk / 128 + 4096, 8192)));  // also the bin-box partials
   FL_TRY(dalloc(&c->blocks, 3 * nb));
   FL_TRY(dalloc(&c->lut, lk));
   FL_TRY(dalloc(&c->mark, lk));

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
 #include <cstdio>
 #include <vector>
 
@@ -155,6 +156,13 @@ struct fl_ctx {
 
 namespace fl {
 
+// Geometry scalars for the kernels, with the exact reciprocal of a power-of-two cell width.
+inline Geometry make_geometry(double ox, double oy, double oz, double h) {
+  int e = 0;
+  const bool pow2 = std::isnormal(h) && std::frexp(h, &e) == 0.5;
+  return Geometry{ox, oy, oz, h, pow2 ? 1.0 / h : 0.0};
+}
+
 inline unsigned grid_for(uint64_t n, int threads = kThreads) {
   const uint64_t g = (n + threads - 1) / threads;
   return static_cast<unsigned>(g == 0 ? 1 : g);
@@ -170,9 +178,11 @@ GridView grid_view(fl_ctx* ctx);
 BinList bin_list(fl_ctx* ctx);
 
 // Transfers (transfer.hip)
-int p2g_quanta(fl_ctx* ctx, uint64_t n, double inv_qm, double inv_qp, double limit);
+int p2g_quanta(fl_ctx* ctx, uint64_t n, double inv_qm, double inv_qp, double limit, hipEvent_t before = nullptr,
+               hipEvent_t after = nullptr);
 int normalize_forces(fl_ctx* ctx, uint64_t nf, double qm, double qp, double min_mass, const double g[3], double dt);
-int g2p(fl_ctx* ctx, uint64_t n, const double* d_old, const double* d_new);
+int g2p(fl_ctx* ctx, uint64_t n, const double* d_old, const double* d_new, hipEvent_t before = nullptr,
+        hipEvent_t after = nullptr);
 int unpack_particles(fl_ctx* ctx, uint64_t n);
 
 // Small helpers (topology.hip)

@@ -37,10 +37,13 @@ __device__ inline double sub_roundoff(double a, double b, double d) {
 }
 
 // One coordinate in cell units with the world subtraction kept for exact edge decisions.
+// invH is the exact reciprocal of H when the cell width is a power of two (then s / H and s * invH are
+// the same correctly rounded value), and 0 otherwise.
 struct Coord {
-  double X, D, E, H;
+  double X, D, E, H, invH;
   int shift;
 
+  __device__ double div_h(double s) const { return invH != 0.0 ? s * invH : s / H; }
   __device__ double scaled_offset(double edge) const { return fma(-edge, H, D) + E; }
   __device__ bool below(double edge) const {
     const double off = X - edge;
@@ -52,7 +55,7 @@ struct Coord {
     const double off = X - edge;
     if (fabs(off) >= fabs(X) && off != 0.0) return off;
     const double s = scaled_offset(edge);
-    return finite(s) ? s / H : off;
+    return finite(s) ? div_h(s) : off;
   }
   __device__ double displacement(double anchor, double h) const {
     const double n = -offset(anchor);
@@ -63,7 +66,8 @@ struct Coord {
   }
 };
 
-__device__ inline Coord normalize(double x, double o, double h) {
+// inv_h: exact 1/h when h is a power of two, else 0 (every quotient then uses a division).
+__device__ inline Coord normalize(double x, double o, double h, double inv_h = 0.0) {
   double diff = x - o;
   int e = 0;
   if (!finite(diff)) {
@@ -73,19 +77,22 @@ __device__ inline Coord normalize(double x, double o, double h) {
     e = -1;
   }
   double H = ldexp(h, e);
-  const double X = diff / H;
+  double invH = inv_h != 0.0 ? ldexp(inv_h, -e) : 0.0;
+  const double X = invH != 0.0 ? diff * invH : diff / H;
   double err = sub_roundoff(x, o, diff);
   if (finite(X) && H < 0x1p-960) {
     e += 64;
     diff = ldexp(diff, 64);
     err = ldexp(err, 64);
     H = ldexp(H, 64);
+    invH = invH != 0.0 ? ldexp(invH, -64) : 0.0;
   }
   Coord c;
   c.X = X;
   c.D = diff;
   c.E = err;
   c.H = H;
+  c.invH = invH;
   c.shift = e;
   return c;
 }
@@ -176,6 +183,7 @@ __device__ inline double physical_gradient(Factor fx, Factor fy, Factor fz, doub
 // Validated geometry scalars shared by all kernels.
 struct Geometry {
   double ox, oy, oz, h;
+  double inv_h;  // exact 1/h when h is a power of two, else 0
 };
 
 __device__ inline double comp(const Geometry& g, int a) { return a == 0 ? g.ox : (a == 1 ? g.oy : g.oz); }
@@ -187,13 +195,13 @@ __device__ inline int prepare_positions(const Geometry& g, const double s[3], co
   for (int a = 0; a < 3; ++a)
     if (!finite(s[a])) return kInvalidArgument;
   for (int a = 0; a < 3; ++a) {
-    sc[a] = normalize(s[a], comp(g, a), g.h);
+    sc[a] = normalize(s[a], comp(g, a), g.h, g.inv_h);
     if (!finite(sc[a].X)) return kOutOfRange;
   }
   for (int a = 0; a < 3; ++a)
     if (!finite(t[a])) return kInvalidArgument;
   for (int a = 0; a < 3; ++a) {
-    tc[a] = normalize(t[a], comp(g, a), g.h);
+    tc[a] = normalize(t[a], comp(g, a), g.h, g.inv_h);
     if (!finite(tc[a].X)) return kOutOfRange;
   }
   for (int a = 0; a < 3; ++a)

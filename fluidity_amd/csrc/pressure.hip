@@ -46,6 +46,12 @@ bool key_less(const Key3& a, const Key3& b) {
 }
 int32_t floor_half(int32_t v) { return v >= 0 ? v / 2 : -((-(v + 1)) / 2) - 1; }
 
+// Frozen link of one face, packed in one 64-byte line for the cell-centred gathers.
+struct alignas(64) FaceRec {
+  int32_t tail, head, kind, pad;
+  double area, ustar, wall, scale, k, lift;
+};
+
 struct Level {
   uint64_t nb = 0;
   std::vector<Key3> keys;
@@ -86,6 +92,7 @@ struct Pressure {
   double* face_k = nullptr;
   double* face_wall = nullptr;
   double* face_u = nullptr;
+  FaceRec* face_rec = nullptr;
   double* cell_volume = nullptr;
   double* cell_s = nullptr;
   double* cell_K = nullptr;
@@ -117,6 +124,7 @@ struct Pressure {
   std::vector<uint64_t> h_gauges;
   std::vector<Key3> h_unknown_cells;  // level-0 unknown cell coordinates
   int levels_requested = 0;
+  bool p_init_zero = false;  // warm start is identically zero: its evaluation equals the p = 0 one
   bool has_jump = false;
   double full_area = 0, full_volume = 0;
   double scale_coarse = 0.5;
@@ -206,9 +214,11 @@ __device__ inline float face_k(const float* k, uint64_t n, int64_t i, int64_t j,
 
 // ------------------------------------------------------------------ system kernels (FP64)
 
-__global__ void k_freeze_faces(uint64_t nf, double dt, double full_area, const double* area, const double* beta,
-                               const double* dist, const double* uwall, const int32_t* has_flux, const double* flux,
-                               double* scale, double* k, double* wall) {
+
+__global__ void k_freeze_faces(uint64_t nf, double dt, double full_area, const int32_t* tail, const int32_t* head,
+                               const int32_t* kind, const double* area, const double* beta, const double* dist,
+                               const double* ustar, const double* uwall, const int32_t* has_flux, const double* flux,
+                               const double* lift, double* scale, double* k, double* wall, FaceRec* rec) {
   const uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (f >= nf) return;
   double s = 0.0, c = 0.0;
@@ -220,7 +230,20 @@ __global__ void k_freeze_faces(uint64_t nf, double dt, double full_area, const d
   scale[f] = s;
   k[f] = c;
   const double blocked = full_area - area[f];
-  wall[f] = has_flux[f] ? flux[f] : blocked * uwall[f];
+  const double w = has_flux[f] ? flux[f] : blocked * uwall[f];
+  wall[f] = w;
+  FaceRec r;
+  r.tail = tail[f];
+  r.head = head[f];
+  r.kind = kind[f];
+  r.pad = 0;
+  r.area = area[f];
+  r.ustar = ustar[f];
+  r.wall = w;
+  r.scale = s;
+  r.k = c;
+  r.lift = lift[f];
+  rec[f] = r;
 }
 
 __global__ void k_freeze_cells(uint64_t nc, double dt, const double* V, const double* s, const double* K,
@@ -232,23 +255,25 @@ __global__ void k_freeze_cells(uint64_t nc, double dt, const double* V, const do
 }
 
 // Integrated RHS (mac_pressure_3d_arithmetic.h:263-313) with Neumaier sums per cell.
-__global__ void k_rhs(uint64_t nc, const int32_t* cell_faces, const int32_t* tail, const int32_t* kind,
-                      const double* area, const double* ustar, const double* wall, const double* k, const double* lift,
+// Cells are visited in block order (fine index) so that their faces are near each other in memory.
+__global__ void k_rhs(uint64_t nfine, const int32_t* fine_cell, const int32_t* cell_faces, const FaceRec* faces,
                       const double* target, const double* inwall, const double* comp, const double* prev, double* rhs,
                       double* rhs_abs) {
-  const uint64_t c = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  if (c >= nc) return;
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= nfine) return;
+  const int32_t c = fine_cell[i];
+  if (c < 0) return;
   Row row;
   row.add(target[c]);
   row.add(inwall[c]);
   row.add(comp[c] * prev[c]);
   for (int j = 0; j < 6; ++j) {
-    const int32_t f = cell_faces[6 * c + j];
-    const double sign = tail[f] == static_cast<int32_t>(c) ? 1.0 : -1.0;
-    row.add(-sign * (area[f] * ustar[f]));
-    row.add(-sign * wall[f]);
-    if (kind[f] == 1) row.add(-sign * (k[f] * lift[f]));
-    else if (kind[f] == 2) row.add(k[f] * lift[f]);
+    const FaceRec& f = faces[cell_faces[6 * c + j]];
+    const double sign = f.tail == c ? 1.0 : -1.0;
+    row.add(-sign * (f.area * f.ustar));
+    row.add(-sign * f.wall);
+    if (f.kind == 1) row.add(-sign * (f.k * f.lift));
+    else if (f.kind == 2) row.add(f.k * f.lift);
   }
   rhs[c] = row.flux.total();
   rhs_abs[c] = row.abs.total();
@@ -307,30 +332,37 @@ __device__ inline void block_reduce(double v[4], double* partial, unsigned slot)
 
 // Evaluate (mac_pressure_workspace_3d.cc:399-461): FP64 residual of the original operator, physical
 // defect from corrected fluxes, and the identity |V defect + r| <= 64 eps (sum of absolute terms).
-__global__ void k_evaluate(uint64_t nc, const int32_t* cell_faces, const int32_t* tail, const int32_t* head,
-                           const int32_t* kind, const double* area, const double* ustar, const double* wall,
-                           const double* scale, const double* k, const double* lift, const double* V,
-                           const double* target, const double* inwall, const double* comp, const double* prev,
-                           const double* rhs, const double* rhs_abs, const double* p, double* res, double* def,
-                           double* partial, uint64_t* err) {
+// Cells are visited in block order.
+__global__ void k_evaluate(uint64_t nfine, const int32_t* fine_cell, const int32_t* cell_faces, const FaceRec* faces,
+                           const double* V, const double* target, const double* inwall, const double* comp,
+                           const double* prev, const double* rhs, const double* rhs_abs, const double* p, double* res,
+                           double* def, double* partial, uint64_t* err) {
   double v[4] = {0.0, 0.0, 0.0, 0.0};
-  for (uint64_t c = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; c < nc; c += uint64_t(gridDim.x) * blockDim.x) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < nfine; i += uint64_t(gridDim.x) * blockDim.x) {
+    const int32_t c = fine_cell[i];
+    if (c < 0) continue;
     Row op, phys;
-    op.add(comp[c] * p[c]);
+    const double pc = p[c];
+    op.add(comp[c] * pc);
     phys.add(-target[c]);
     phys.add(-inwall[c]);
-    if (comp[c] > 0.0) phys.add(comp[c] * (p[c] - prev[c]));
+    if (comp[c] > 0.0) phys.add(comp[c] * (pc - prev[c]));
     for (int j = 0; j < 6; ++j) {
-      const int32_t f = cell_faces[6 * c + j];
-      const double sign = tail[f] == static_cast<int32_t>(c) ? 1.0 : -1.0;
-      if (k[f] != 0.0) {
-        const double pt = tail[f] >= 0 ? p[tail[f]] : 0.0;
-        const double ph = head[f] >= 0 ? p[head[f]] : 0.0;
-        op.add(sign * (k[f] * (pt - ph)));
+      const FaceRec& f = faces[cell_faces[6 * c + j]];
+      const double sign = f.tail == c ? 1.0 : -1.0;
+      const double pt_op = f.tail >= 0 ? p[f.tail] : 0.0;
+      const double ph_op = f.head >= 0 ? p[f.head] : 0.0;
+      if (f.k != 0.0) op.add(sign * (f.k * (pt_op - ph_op)));
+      double u = f.ustar;
+      if (f.area > 0.0) {
+        const double ext = f.kind == 2 ? f.lift : 0.0;
+        const double jump = f.kind == 1 ? f.lift : 0.0;
+        const double pt = f.tail >= 0 ? pt_op : ext;
+        const double ph = f.head >= 0 ? ph_op : ext;
+        u += f.scale * (pt - ph + jump);
       }
-      const double u = corrected(f, tail, head, kind, area, ustar, scale, lift, p);
-      phys.add(sign * (area[f] * u));
-      phys.add(sign * wall[f]);
+      phys.add(sign * (f.area * u));
+      phys.add(sign * f.wall);
     }
     const double r = rhs[c] - op.flux.total();
     const double d = phys.flux.total() / V[c];
@@ -388,8 +420,8 @@ __global__ void k_finalize4(const double* partial, unsigned slots, double* scal,
 // Level 0 from the frozen FP64 links: edges between unknowns keep their conductance, links to a
 // Dirichlet exterior or to a gauge cell become anchors, compliance is an anchor.
 __global__ void k_level0(uint64_t n, const int32_t* fine_cell, const uint8_t* flag, const int32_t* cell_faces,
-                         const int32_t* tail, const int32_t* head, const double* k, const double* comp, const double* V,
-                         float* kk, float* anchor, float* diag, float* vol) {
+                         const FaceRec* faces, const double* comp, const double* V, float* kk, float* anchor,
+                         float* diag, float* vol) {
   const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (i >= n) return;
   const int32_t c = fine_cell[i];
@@ -399,14 +431,14 @@ __global__ void k_level0(uint64_t n, const int32_t* fine_cell, const uint8_t* fl
   if (unknown) {
     a = comp[c];
     for (int j = 0; j < 6; ++j) {
-      const int32_t f = cell_faces[6 * c + j];
-      if (!(k[f] > 0.0)) continue;
-      const int32_t other = tail[f] == c ? head[f] : tail[f];
+      const FaceRec& f = faces[cell_faces[6 * c + j]];
+      if (!(f.k > 0.0)) continue;
+      const int32_t other = f.tail == c ? f.head : f.tail;
       if (other >= 0 && flag[other] == 1) {
-        ksum[j] = k[f];
-        if ((j & 1) == 0) kl[j >> 1] = static_cast<float>(k[f]);
+        ksum[j] = f.k;
+        if ((j & 1) == 0) kl[j >> 1] = static_cast<float>(f.k);
       } else {
-        a += k[f];
+        a += f.k;
       }
     }
   }
@@ -741,7 +773,7 @@ int ensure_capacity(fl_ctx* ctx, uint64_t nc, uint64_t nf, uint64_t nfine) {
   if (nc > ctx->cap.pressure_cells || nf > ctx->cap.pressure_faces) return FL_RESOURCE_EXHAUSTED;
   void* ptrs[] = {P.fine_cell, P.cell_fine, P.cell_flag, P.cell_faces, P.face_tail, P.face_head, P.face_kind,
                   P.face_slot, P.face_area, P.face_beta, P.face_dist, P.face_ustar, P.face_uwall, P.face_flux,
-                  P.face_has_flux, P.face_lift, P.face_scale, P.face_k, P.face_wall, P.face_u, P.cell_volume,
+                  P.face_has_flux, P.face_lift, P.face_scale, P.face_k, P.face_wall, P.face_u, P.face_rec, P.cell_volume,
                   P.cell_s, P.cell_K, P.cell_prev, P.cell_inwall, P.cell_comp, P.cell_target, P.rhs, P.rhs_abs,
                   P.p, P.p_base, P.p_init, P.res, P.def, P.zero, P.fine_vol, P.r32, P.x32, P.d32, P.q32, P.partial, P.scal,
                   P.fscal, P.perr};
@@ -767,6 +799,7 @@ int ensure_capacity(fl_ctx* ctx, uint64_t nc, uint64_t nf, uint64_t nfine) {
   FL_TRY(dal(&P.face_k, cf));
   FL_TRY(dal(&P.face_wall, cf));
   FL_TRY(dal(&P.face_u, cf));
+  FL_TRY(dal(&P.face_rec, cf));
   for (double** a : {&P.cell_volume, &P.cell_s, &P.cell_K, &P.cell_prev, &P.cell_inwall, &P.cell_comp, &P.cell_target,
                      &P.rhs, &P.rhs_abs, &P.p, &P.p_base, &P.p_init, &P.res, &P.def, &P.zero})
     FL_TRY(dal(a, cc));
@@ -890,9 +923,8 @@ int setup_coefficients(fl_ctx* ctx, Pressure& P) {
   hipStream_t s = ctx->stream;
   Level& L0 = P.levels[0];
   const uint64_t n0 = L0.nb * 64;
-  k_level0<<<grid_for(n0), kThreads, 0, s>>>(n0, P.fine_cell, P.cell_flag, P.cell_faces, P.face_tail, P.face_head,
-                                             P.face_k, P.cell_comp, P.cell_volume, L0.k, L0.anchor, L0.diag,
-                                             P.fine_vol);
+  k_level0<<<grid_for(n0), kThreads, 0, s>>>(n0, P.fine_cell, P.cell_flag, P.cell_faces, P.face_rec, P.cell_comp,
+                                             P.cell_volume, L0.k, L0.anchor, L0.diag, P.fine_vol);
   FL_HIP(hipGetLastError());
   for (size_t l = 1; l < P.levels.size(); ++l) {
     Level& F = P.levels[l - 1];
@@ -909,26 +941,28 @@ int setup_coefficients(fl_ctx* ctx, Pressure& P) {
 int vcycle(fl_ctx* ctx, Pressure& P, size_t l, int nu, int coarse_sweeps) {
   hipStream_t s = ctx->stream;
   Level& L = P.levels[l];
+  // Level 0 smooths the recursive FP32 residual of the outer PCG in place of its own right-hand side.
+  const float* b = l == 0 ? P.r32 : L.b;
   const uint64_t n = L.nb * 64;
   const unsigned g = grid_for(n);
   if (l + 1 == P.levels.size()) {
     if (L.nb <= 64) {
-      k_coarsest<<<1, 1024, 0, s>>>(n, L.nbr, L.k, L.diag, L.b, L.x, coarse_sweeps);
+      k_coarsest<<<1, 1024, 0, s>>>(n, L.nbr, L.k, L.diag, b, L.x, coarse_sweeps);
     } else {
       for (int pass = 0; pass < 4 * coarse_sweeps; ++pass) {
         const int fwd = pass < 2 * coarse_sweeps;
         const int color = fwd ? (pass & 1) : 1 - (pass & 1);
-        k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, L.b, L.x, color, pass == 0);
+        k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, b, L.x, color, pass == 0);
       }
     }
     FL_HIP(hipGetLastError());
     return FL_OK;
   }
   for (int it = 0; it < nu; ++it) {
-    k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, L.b, L.x, 0, it == 0);
-    k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, L.b, L.x, 1, 0);
+    k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, b, L.x, 0, it == 0);
+    k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, b, L.x, 1, 0);
   }
-  k_residual<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.anchor, L.diag, L.b, L.x, L.r);
+  k_residual<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.anchor, L.diag, b, L.x, L.r);
   Level& C = P.levels[l + 1];
   const uint64_t nc = C.nb * 64;
   k_restrict<<<grid_for(nc), kThreads, 0, s>>>(nc, C.child, L.r, C.b);
@@ -936,8 +970,8 @@ int vcycle(fl_ctx* ctx, Pressure& P, size_t l, int nu, int coarse_sweeps) {
   FL_TRY(vcycle(ctx, P, l + 1, nu, coarse_sweeps));
   k_prolong<<<g, kThreads, 0, s>>>(n, L.parent, L.diag, C.x, L.x);
   for (int it = 0; it < nu; ++it) {
-    k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, L.b, L.x, 1, 0);
-    k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, L.b, L.x, 0, 0);
+    k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, b, L.x, 1, 0);
+    k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, b, L.x, 0, 0);
   }
   FL_HIP(hipGetLastError());
   return FL_OK;
@@ -954,11 +988,11 @@ struct EvalResult {
 
 int evaluate(fl_ctx* ctx, Pressure& P, const double* p, EvalResult& out) {
   hipStream_t s = ctx->stream;
-  const unsigned g = reduce_grid(P.nc);
-  k_evaluate<<<g, kThreads, 0, s>>>(P.nc, P.cell_faces, P.face_tail, P.face_head, P.face_kind, P.face_area,
-                                    P.face_ustar, P.face_wall, P.face_scale, P.face_k, P.face_lift, P.cell_volume,
-                                    P.cell_target, P.cell_inwall, P.cell_comp, P.cell_prev, P.rhs, P.rhs_abs, p, P.res,
-                                    P.def, P.partial, P.perr);
+  const uint64_t nfine = P.levels[0].nb * 64;
+  const unsigned g = reduce_grid(nfine);
+  k_evaluate<<<g, kThreads, 0, s>>>(nfine, P.fine_cell, P.cell_faces, P.face_rec, P.cell_volume, P.cell_target,
+                                    P.cell_inwall, P.cell_comp, P.cell_prev, P.rhs, P.rhs_abs, p, P.res, P.def,
+                                    P.partial, P.perr);
   k_finalize4<<<1, kThreads, 0, s>>>(P.partial, g, P.scal, 8);
   FL_HIP(hipGetLastError());
   FL_HIP(hipMemcpyAsync(P.hscal, P.scal, 16 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -986,7 +1020,7 @@ __global__ void k_reset_err(uint64_t* e) {
 
 // The mixed-precision PCG with FP64 restarts. Requires the system to be frozen (faces, cells, RHS,
 // initial pressure in P.p_base) and the hierarchy coefficients set.
-int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates& gates,
+int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates& gates, const EvalResult& zero_eval,
           fl_pressure_diagnostics& diag) {
   hipStream_t s = ctx->stream;
   Level& L0 = P.levels[0];
@@ -998,7 +1032,11 @@ int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates&
   diag.restarts = 0;
   EvalResult e{};
   k_copy_f64<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, P.p_base, P.p);
-  FL_TRY(evaluate(ctx, P, P.p, e));
+  if (P.p_init_zero) {
+    e = zero_eval;  // P.res and P.def still hold the p = 0 evaluation
+  } else {
+    FL_TRY(evaluate(ctx, P, P.p, e));
+  }
   auto publish = [&](const EvalResult& r) {
     diag.true_residual_l2 = r.l2;
     diag.max_volume_scaled_residual = r.max_res;
@@ -1013,7 +1051,6 @@ int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates&
   bool restart = true;
   int since = 0;
   for (uint64_t it = 1; it <= opt.max_iterations; ++it) {
-    FL_HIP(hipMemcpyAsync(L0.b, P.r32, n * sizeof(float), hipMemcpyDeviceToDevice, s));
     FL_TRY(vcycle(ctx, P, 0, opt.smoothing_sweeps, opt.coarse_sweeps));
     k_dot<<<rg, kThreads, 0, s>>>(n, P.r32, L0.x, P.partial);
     k_finalize4<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, 20);
@@ -1067,7 +1104,7 @@ void destroy_pressure(fl_ctx* ctx) {
   free_levels(*P);
   void* ptrs[] = {P->fine_cell, P->cell_fine, P->cell_flag, P->cell_faces, P->face_tail, P->face_head, P->face_kind,
                   P->face_slot, P->face_area, P->face_beta, P->face_dist, P->face_ustar, P->face_uwall, P->face_flux,
-                  P->face_has_flux, P->face_lift, P->face_scale, P->face_k, P->face_wall, P->face_u, P->cell_volume,
+                  P->face_has_flux, P->face_lift, P->face_scale, P->face_k, P->face_wall, P->face_u, P->face_rec, P->cell_volume,
                   P->cell_s, P->cell_K, P->cell_prev, P->cell_inwall, P->cell_comp, P->cell_target, P->rhs,
                   P->rhs_abs, P->p, P->p_base, P->p_init, P->res, P->def, P->zero, P->fine_vol, P->r32, P->x32, P->d32, P->q32,
                   P->partial, P->scal, P->fscal, P->perr};
@@ -1389,6 +1426,7 @@ int prepare_system(fl_ctx* ctx, Pressure& P, const fl_geometry* g, const fl_int3
   UP(P.cell_prev, prev);
   UP(P.cell_inwall, inw);
   UP(P.p_init, p0);
+  P.p_init_zero = std::all_of(p0.begin(), p0.end(), [](double v) { return v == 0.0; });
 #undef UP
   FL_HIP(hipStreamSynchronize(s));
   return build_levels(ctx, P, blocks0, 1);
@@ -1397,25 +1435,25 @@ int prepare_system(fl_ctx* ctx, Pressure& P, const fl_geometry* g, const fl_int3
 // Freezes the system for dt on the device, assembles the RHS and the hierarchy coefficients, and
 // computes the gates. The warm start must be in P.p_base.
 int freeze_and_assemble(fl_ctx* ctx, Pressure& P, double dt, const fl_pressure_options& opt, Gates& gates,
-                        fl_pressure_diagnostics& diag) {
+                        EvalResult& e0, fl_pressure_diagnostics& diag) {
   hipStream_t s = ctx->stream;
   k_reset_err<<<1, 64, 0, s>>>(P.perr);
   k_copy_f64<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, P.p_init, P.p_base);
-  k_freeze_faces<<<grid_for(P.nf), kThreads, 0, s>>>(P.nf, dt, P.full_area, P.face_area, P.face_beta, P.face_dist,
-                                                     P.face_uwall, P.face_has_flux, P.face_flux, P.face_scale, P.face_k,
-                                                     P.face_wall);
+  k_freeze_faces<<<grid_for(P.nf), kThreads, 0, s>>>(P.nf, dt, P.full_area, P.face_tail, P.face_head, P.face_kind,
+                                                     P.face_area, P.face_beta, P.face_dist, P.face_ustar, P.face_uwall,
+                                                     P.face_has_flux, P.face_flux, P.face_lift, P.face_scale, P.face_k,
+                                                     P.face_wall, P.face_rec);
   k_freeze_cells<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, dt, P.cell_volume, P.cell_s, P.cell_K, P.cell_comp,
                                                      P.cell_target);
-  k_rhs<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, P.cell_faces, P.face_tail, P.face_kind, P.face_area, P.face_ustar,
-                                            P.face_wall, P.face_k, P.face_lift, P.cell_target, P.cell_inwall,
-                                            P.cell_comp, P.cell_prev, P.rhs, P.rhs_abs);
+  const uint64_t nfine = P.levels[0].nb * 64;
+  k_rhs<<<grid_for(nfine), kThreads, 0, s>>>(nfine, P.fine_cell, P.cell_faces, P.face_rec, P.cell_target,
+                                             P.cell_inwall, P.cell_comp, P.cell_prev, P.rhs, P.rhs_abs);
   FL_HIP(hipGetLastError());
   FL_TRY(setup_coefficients(ctx, P));
   const unsigned g = reduce_grid(P.nc);
   k_sum_sq<<<g, kThreads, 0, s>>>(P.nc, P.rhs, P.partial);
   k_finalize4<<<1, kThreads, 0, s>>>(P.partial, g, P.scal, 24);
   FL_HIP(hipMemcpyAsync(P.hscal + 24, P.scal + 24, sizeof(double), hipMemcpyDeviceToHost, s));
-  EvalResult e0{};
   FL_TRY(evaluate(ctx, P, P.zero, e0));  // defect of the unprojected field (p = 0)
   const double rhs_norm = std::sqrt(P.hscal[24]);
   double limit = 2e-6 + 2e-6 * rhs_norm;
@@ -1465,9 +1503,10 @@ int project_pressure(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* bl
   FL_HIP(hipEventCreate(&b));
   FL_HIP(hipEventRecord(a, ctx->stream));
   Gates gates{};
+  EvalResult e0{};
   fl_pressure_diagnostics diag{};
-  int st = freeze_and_assemble(ctx, P, dt, *options, gates, diag);
-  if (st == FL_OK) st = solve(ctx, P, *options, gates, diag);
+  int st = freeze_and_assemble(ctx, P, dt, *options, gates, e0, diag);
+  if (st == FL_OK) st = solve(ctx, P, *options, gates, e0, diag);
   if (st == FL_OK) {
     k_face_velocity<<<grid_for(P.nf), kThreads, 0, ctx->stream>>>(P.nf, P.face_tail, P.face_head, P.face_kind,
                                                                   P.face_area, P.face_ustar, P.face_scale,
@@ -1552,7 +1591,8 @@ int fl_scene_upload(fl_ctx* ctx, const fl_geometry* geometry, size_t n, const do
   if (!geometry || !(geometry->cell_width > 0.0 && std::isnormal(geometry->cell_width))) return FL_INVALID_ARGUMENT;
   if (n > ctx->cap.particles) return FL_RESOURCE_EXHAUSTED;
   (void)hipSetDevice(ctx->device);
-  ctx->geometry = {geometry->world_origin.x, geometry->world_origin.y, geometry->world_origin.z, geometry->cell_width};
+  ctx->geometry = make_geometry(geometry->world_origin.x, geometry->world_origin.y, geometry->world_origin.z,
+                                geometry->cell_width);
   hipStream_t s = ctx->stream;
   double* stage = reinterpret_cast<double*>(ctx->staging_aos);  // >= 19 doubles per particle
   const struct {
@@ -1716,7 +1756,7 @@ int fl_step(fl_ctx* ctx, double dt, fl_vec3 gravity, fl_quantization quantizatio
   FL_TRY(sort_particles(ctx, n, ctx->raw.view(), true));
   FL_HIP(hipEventRecord(ev[1], s));
   FL_TRY(p2g_quanta(ctx, ctx->nvalid, 1.0 / quantization.mass_per_quantum, 1.0 / quantization.pure_momentum_per_quantum,
-                    limit));
+                    limit, ev[6], ev[7]));
   FL_HIP(hipEventRecord(ev[2], s));
   const double g3[3] = {gravity.x, gravity.y, gravity.z};
   const uint64_t nf_grid = ctx->nb * 192;
@@ -1728,9 +1768,10 @@ int fl_step(fl_ctx* ctx, double dt, fl_vec3 gravity, fl_quantization quantizatio
   k_gather_slots<<<grid_for(P.nf), kThreads, 0, s>>>(P.nf, P.face_slot, ctx->ustar, P.face_ustar);
   FL_HIP(hipGetLastError());
   Gates gates{};
+  EvalResult e0{};
   fl_pressure_diagnostics diag{};
-  int st = freeze_and_assemble(ctx, P, dt, *options, gates, diag);
-  if (st == FL_OK) st = solve(ctx, P, *options, gates, diag);
+  int st = freeze_and_assemble(ctx, P, dt, *options, gates, e0, diag);
+  if (st == FL_OK) st = solve(ctx, P, *options, gates, e0, diag);
   report->pressure = diag;
   if (st != FL_OK) {
     report->status = st;
@@ -1742,10 +1783,13 @@ int fl_step(fl_ctx* ctx, double dt, fl_vec3 gravity, fl_quantization quantizatio
   k_scatter_slots<<<grid_for(P.nf), kThreads, 0, s>>>(P.nf, P.face_slot, P.face_u, ctx->unew);
   FL_HIP(hipGetLastError());
   FL_HIP(hipEventRecord(ev[4], s));
-  FL_TRY(g2p(ctx, ctx->nvalid, ctx->u0, ctx->unew));
+  const float p2g_kernel = elapsed(ev[6], ev[7]);
+  FL_TRY(g2p(ctx, ctx->nvalid, ctx->u0, ctx->unew, ev[6], ev[7]));
   FL_HIP(hipEventRecord(ev[5], s));
   FL_HIP(hipEventSynchronize(ev[5]));
   st = first_error(ctx);
+  report->p2g_kernel_ms = p2g_kernel;
+  report->g2p_kernel_ms = elapsed(ev[6], ev[7]);
   report->sort_ms = elapsed(ev[0], ev[1]);
   report->p2g_ms = elapsed(ev[1], ev[2]);
   report->normalize_ms = elapsed(ev[2], ev[3]);

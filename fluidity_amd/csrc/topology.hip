@@ -44,63 +44,74 @@ inline unsigned capped_grid(uint64_t n) {
 // First failure of one particle up to and including its bin, in the reference's order:
 // deposit-input validation (mac_deposit_3d_internal.h:25-37), sorted and trial normalisation,
 // then the bin range (sparse_mac_grid_3d.cc:463-474). Gathers skip the mass/velocity checks and
-// plain bin queries skip the trial position (sparse_mac_grid_3d.cc:438-449).
-__global__ void k_bin(ParticleView p, uint64_t n, Geometry g, int mode, int32_t* bins, uint64_t* err, int32_t* red) {
+// plain bin queries skip the trial position (sparse_mac_grid_3d.cc:438-449). Scalar code only, so
+// nothing spills to scratch.
+template <int kMode>
+__global__ void k_bin(ParticleView p, uint64_t n, Geometry g, int32_t* bins, uint64_t* err, int32_t* partial) {
   __shared__ int32_t smin[3][kThreads / 64], smax[3][kThreads / 64];
-  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  int32_t b[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
-  bool ok = false;
-  if (i < n) {
+  int32_t mn0 = 0x7fffffff, mn1 = 0x7fffffff, mn2 = 0x7fffffff;
+  int32_t mx0 = -0x7fffffff - 1, mx1 = -0x7fffffff - 1, mx2 = -0x7fffffff - 1;
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    int32_t b0 = 0x7fffffff, b1 = 0x7fffffff, b2 = 0x7fffffff;
     int st = kOk;
-    if (mode == 1) {
-      const double m = p.m[i];
-      if (!finite(m) || m < 0.0) st = kInvalidArgument;
-      for (int a = 0; a < 3 && st == kOk; ++a)
-        if (!finite(p.v[a][i])) st = kInvalidArgument;
-      for (int k = 0; k < 9 && st == kOk; ++k)
-        if (!finite(p.c[k][i])) st = kInvalidArgument;
+    if (kMode == 1) {
+      bool good = finite(p.m[i]) && p.m[i] >= 0.0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) good = good && finite(p.v[a][i]);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) good = good && finite(p.c[k][i]);
+      if (!good) st = kInvalidArgument;
     }
-    const double s[3] = {p.s[0][i], p.s[1][i], p.s[2][i]};
-    if (st == kOk) {
-      Coord sc[3], tc[3];
-      if (mode == 0) {
-        for (int a = 0; a < 3 && st == kOk; ++a)
-          if (!finite(s[a])) st = kInvalidArgument;
-        for (int a = 0; a < 3 && st == kOk; ++a) {
-          sc[a] = normalize(s[a], comp(g, a), g.h);
-          if (!finite(sc[a].X)) st = kOutOfRange;
-        }
-        for (int a = 0; a < 3 && st == kOk; ++a) st = checked_bin(sc[a], b[a]);
-      } else {
-        const double t[3] = {p.t[0][i], p.t[1][i], p.t[2][i]};
-        st = prepare_positions(g, s, t, sc, tc, b);
+    const double s0 = p.s[0][i], s1 = p.s[1][i], s2 = p.s[2][i];
+    if (st == kOk && !(finite(s0) && finite(s1) && finite(s2))) st = kInvalidArgument;
+    const Coord c0 = normalize(s0, g.ox, g.h, g.inv_h);
+    const Coord c1 = normalize(s1, g.oy, g.h, g.inv_h);
+    const Coord c2 = normalize(s2, g.oz, g.h, g.inv_h);
+    if (st == kOk && !(finite(c0.X) && finite(c1.X) && finite(c2.X))) st = kOutOfRange;
+    if (kMode != 0 && p.t[0] != p.s[0]) {
+      const double t0 = p.t[0][i], t1 = p.t[1][i], t2 = p.t[2][i];
+      if (st == kOk && !(finite(t0) && finite(t1) && finite(t2))) st = kInvalidArgument;
+      if (st == kOk) {
+        const bool fin = finite(normalize(t0, g.ox, g.h, g.inv_h).X) && finite(normalize(t1, g.oy, g.h, g.inv_h).X) &&
+                         finite(normalize(t2, g.oz, g.h, g.inv_h).X);
+        if (!fin) st = kOutOfRange;
       }
     }
+    if (st == kOk) st = checked_bin(c0, b0);
+    if (st == kOk) st = checked_bin(c1, b1);
+    if (st == kOk) st = checked_bin(c2, b2);
     if (st == kOk) {
-      ok = true;
-      bins[3 * i + 0] = b[0];
-      bins[3 * i + 1] = b[1];
-      bins[3 * i + 2] = b[2];
+      bins[3 * i + 0] = b0;
+      bins[3 * i + 1] = b1;
+      bins[3 * i + 2] = b2;
+      mn0 = min(mn0, b0);
+      mn1 = min(mn1, b1);
+      mn2 = min(mn2, b2);
+      mx0 = max(mx0, b0);
+      mx1 = max(mx1, b1);
+      mx2 = max(mx2, b2);
     } else {
       atomicMin(reinterpret_cast<unsigned long long*>(err), static_cast<unsigned long long>(error_word(i, st)));
       bins[3 * i + 0] = bins[3 * i + 1] = bins[3 * i + 2] = 0x7fffffff;
     }
   }
-  int32_t mn[3], mx[3];
-  for (int a = 0; a < 3; ++a) {
-    mn[a] = ok ? b[a] : 0x7fffffff;
-    mx[a] = ok ? b[a] : (-0x7fffffff - 1);
-    for (int o = 32; o > 0; o >>= 1) {
-      mn[a] = min(mn[a], __shfl_xor(mn[a], o));
-      mx[a] = max(mx[a], __shfl_xor(mx[a], o));
-    }
+  for (int o = 32; o > 0; o >>= 1) {
+    mn0 = min(mn0, __shfl_xor(mn0, o));
+    mn1 = min(mn1, __shfl_xor(mn1, o));
+    mn2 = min(mn2, __shfl_xor(mn2, o));
+    mx0 = max(mx0, __shfl_xor(mx0, o));
+    mx1 = max(mx1, __shfl_xor(mx1, o));
+    mx2 = max(mx2, __shfl_xor(mx2, o));
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0)
-    for (int a = 0; a < 3; ++a) {
-      smin[a][wave] = mn[a];
-      smax[a][wave] = mx[a];
-    }
+  if (lane == 0) {
+    smin[0][wave] = mn0;
+    smin[1][wave] = mn1;
+    smin[2][wave] = mn2;
+    smax[0][wave] = mx0;
+    smax[1][wave] = mx1;
+    smax[2][wave] = mx2;
+  }
   __syncthreads();
   if (threadIdx.x < 3) {
     const int a = threadIdx.x;
@@ -109,24 +120,60 @@ __global__ void k_bin(ParticleView p, uint64_t n, Geometry g, int mode, int32_t*
       lo = min(lo, smin[a][w]);
       hi = max(hi, smax[a][w]);
     }
-    if (lo <= hi) {
-      atomicMin(&red[a], lo);
-      atomicMax(&red[3 + a], hi);
-    }
+    partial[6 * blockIdx.x + a] = lo;
+    partial[6 * blockIdx.x + 3 + a] = hi;
   }
 }
 
+// Folds the per-workgroup bin boxes in a fixed order into red[0..5].
+__global__ void k_box_finalize(const int32_t* partial, unsigned groups, int32_t* red) {
+  __shared__ int32_t s[6][kThreads];
+  int32_t v[6] = {0x7fffffff, 0x7fffffff, 0x7fffffff, -0x7fffffff - 1, -0x7fffffff - 1, -0x7fffffff - 1};
+  for (unsigned g = threadIdx.x; g < groups; g += blockDim.x)
+    for (int a = 0; a < 3; ++a) {
+      v[a] = min(v[a], partial[6 * g + a]);
+      v[3 + a] = max(v[3 + a], partial[6 * g + 3 + a]);
+    }
+  for (int q = 0; q < 6; ++q) s[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int st = kThreads / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int a = 0; a < 3; ++a) {
+        s[a][threadIdx.x] = min(s[a][threadIdx.x], s[a][threadIdx.x + st]);
+        s[3 + a][threadIdx.x] = max(s[3 + a][threadIdx.x], s[3 + a][threadIdx.x + st]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) red[threadIdx.x] = s[threadIdx.x][0];
+}
+
+// Histogram over bin keys with one atomic per distinct key in each wavefront: lanes sharing a key
+// (neighbouring particles usually do) take consecutive provisional slots from a single atomicAdd.
 __global__ void k_count(const int32_t* bins, uint64_t n, Box box, uint32_t* key, uint32_t* rank, uint32_t* count) {
   const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  if (i >= n) return;
-  const int32_t bx = bins[3 * i], by = bins[3 * i + 1], bz = bins[3 * i + 2];
-  if (bx == 0x7fffffff && by == 0x7fffffff && bz == 0x7fffffff) {
-    key[i] = 0xffffffffu;
-    return;
+  const int lane = threadIdx.x & 63;
+  uint32_t k = 0xffffffffu;
+  bool valid = false;
+  if (i < n) {
+    const int32_t bx = bins[3 * i], by = bins[3 * i + 1], bz = bins[3 * i + 2];
+    if (!(bx == 0x7fffffff && by == 0x7fffffff && bz == 0x7fffffff)) {
+      k = static_cast<uint32_t>(box.index(bx, by, bz));
+      valid = true;
+    }
+    key[i] = k;
   }
-  const uint32_t k = static_cast<uint32_t>(box.index(bx, by, bz));
-  key[i] = k;
-  rank[i] = atomicAdd(&count[k], 1u);
+  uint64_t pending = __ballot(valid);
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  while (pending) {
+    const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+    const uint32_t lk = __shfl(k, leader);
+    const uint64_t same = __ballot(valid && k == lk);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&count[lk], static_cast<uint32_t>(__popcll(same)));
+    base = __shfl(base, leader);
+    if (valid && k == lk) rank[i] = base + static_cast<uint32_t>(__popcll(same & below));
+    pending &= ~same;
+  }
 }
 
 // Packs (count, nonempty) so one scan yields both the particle offset and the bin-list position.
@@ -137,6 +184,7 @@ __global__ void k_pack_counts(const uint32_t* count, uint64_t n, uint64_t* out) 
   out[i] = uint64_t(c) | (c ? (uint64_t(1) << 32) : 0);
 }
 
+constexpr unsigned kBinGroups = 2048;  // workgroups of the bin kernel (grid-stride)
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kThreads * kScanItems;
 
@@ -309,7 +357,7 @@ __global__ void k_activate_particles(const double* sx, const double* sy, const d
   Coord c[3];
   int32_t bin[3];
   for (int a = 0; a < 3; ++a) {
-    c[a] = normalize(s[a], comp(g, a), g.h);
+    c[a] = normalize(s[a], comp(g, a), g.h, g.inv_h);
     checked_bin(c[a], bin[a]);
   }
   int32_t lo[2][3], hi[2][3];  // [normal?][coordinate]
@@ -431,7 +479,11 @@ int bin_particles(fl_ctx* ctx, const ParticleView& p, uint64_t n, bool deposit_c
   FL_TRY(reset_error(ctx));
   const int mode = deposit_checks ? 1 : (gather_checks ? 2 : 0);
   if (n) {
-    k_bin<<<grid_for(n), kThreads, 0, ctx->stream>>>(p, n, ctx->geometry, mode, ctx->bin, ctx->err, ctx->reduce_i32);
+    auto* kernel = mode == 1 ? k_bin<1> : (mode == 2 ? k_bin<2> : k_bin<0>);
+    const unsigned groups = std::min<unsigned>(grid_for(n), kBinGroups);
+    int32_t* partial = reinterpret_cast<int32_t*>(ctx->scan_partial);  // 6 * kBinGroups ints
+    kernel<<<groups, kThreads, 0, ctx->stream>>>(p, n, ctx->geometry, ctx->bin, ctx->err, partial);
+    k_box_finalize<<<1, kThreads, 0, ctx->stream>>>(partial, groups, ctx->reduce_i32);
     FL_HIP(hipGetLastError());
   }
   FL_HIP(hipMemcpyAsync(ctx->host_words, ctx->reduce_i32, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));

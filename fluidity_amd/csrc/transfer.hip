@@ -45,21 +45,114 @@ __device__ inline uint32_t axis_blocks(int32_t first, int32_t tile) {
 }
 
 __device__ inline bool window_ok(const int32_t* wblock, uint32_t mx, uint32_t my, uint32_t mz) {
+  bool ok = true;
+#pragma unroll
   for (int z = 0; z < 2; ++z)
+#pragma unroll
     for (int y = 0; y < 2; ++y)
+#pragma unroll
       for (int x = 0; x < 2; ++x)
-        if (((mx >> x) & 1) && ((my >> y) & 1) && ((mz >> z) & 1) && wblock[(z * 2 + y) * 2 + x] < 0) return false;
-  return true;
+        ok = ok && !(((mx >> x) & 1) && ((my >> y) & 1) && ((mz >> z) & 1) && wblock[(z * 2 + y) * 2 + x] < 0);
+  return ok;
 }
 
+// Samples of one coordinate: lattice index of sample 0, the three weight factors, the face offsets
+// (meters) and the weight derivatives in cell units. Same arithmetic as axis_stencil; every array
+// is indexed by unrolled constants so the stencil stays in registers.
+struct Axis3 {
+  int32_t first;
+  double w[3];
+  double off[3];
+  double dw[3];
+};
+
+__device__ __forceinline__ int axis3(const Coord& c, double lattice_shift, int32_t bin, double h, Axis3& s) {
+  const double start = lattice_shift + 0.5;
+  const double first_support = static_cast<double>(kI32Min) + start;
+  const double past_last_support = static_cast<double>(kI32Max - 1) + start;
+  if (c.below(first_support) || !c.below(past_last_support)) return kOutOfRange;
+  const int64_t first = floor_shifted(c, start);
+  const int64_t tile = static_cast<int64_t>(bin) * 4;
+  if (first < tile || first + 2 >= tile + 8) return kFailedPrecondition;
+  const double a0 = static_cast<double>(first) + lattice_shift;
+  const double dl = c.offset(a0 + 0.5);
+  const double dr = fabs(c.offset(a0 + 1.5));
+  const double cc = c.offset(a0 + 1.0);
+  s.first = static_cast<int32_t>(first);
+  s.w[0] = 0.5 * dr * dr;
+  s.w[1] = 0.75 - cc * cc;
+  s.w[2] = 0.5 * dl * dl;
+  s.dw[0] = -dr;
+  s.dw[1] = -2.0 * cc;
+  s.dw[2] = dl;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) s.off[i] = c.displacement(a0 + static_cast<double>(i), h);
+  return kOk;
+}
+
+// First failure among the stencils of one face axis (coordinates x, y, z in order) and its offsets.
+__device__ __forceinline__ int axis_status(int cx, int cy, int cz, const Axis3& X, const Axis3& Y, const Axis3& Z) {
+  int code = cx ? cx : (cy ? cy : cz);
+  if (code) return code;
+  bool fin = true;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) fin = fin && finite(X.off[i]) && finite(Y.off[i]) && finite(Z.off[i]);
+  return fin ? kOk : kOutOfRange;
+}
+
+// Deposits the 27 samples of one face axis into the LDS window; returns false if a scaled
+// contribution is out of range (transfers_3d.cc:178-187). Contributions are formed exactly as in
+// AccumulateMacParticleComponent (mac_deposit_3d_internal.h:50-57).
+template <int A>
+__device__ __forceinline__ bool deposit_axis(const Axis3& X, const Axis3& Y, const Axis3& Z, const int32_t* tile,
+                                             double m, double v, double cx, double cy, double cz, double inv_qm,
+                                             double inv_qp, double limit, unsigned long long* wm,
+                                             unsigned long long* wq) {
+  bool ok = true;
+#pragma unroll
+  for (int zi = 0; zi < 3; ++zi) {
+#pragma unroll
+    for (int yi = 0; yi < 3; ++yi) {
+#pragma unroll
+      for (int xi = 0; xi < 3; ++xi) {
+        const double w = X.w[xi] * Y.w[yi] * Z.w[zi];
+        const double mass = w * m;
+        const double inc = (cx * X.off[xi] + cy * Y.off[yi]) + cz * Z.off[zi];
+        const double vel = v + inc;
+        const double mom = mass * vel;
+        const double sm = mass * inv_qm;  // equals mass / quantum: the quantum is a power of two
+        const double sp = mom * inv_qp;
+        ok = ok && finite(sm) && fabs(sm) <= limit && finite(sp) && fabs(sp) <= limit;
+        const int rx = X.first + xi - tile[0], ry = Y.first + yi - tile[1], rz = Z.first + zi - tile[2];
+        const int slot = (((rz >> 2) * 2 + (ry >> 2)) * 2 + (rx >> 2)) * 192 + A * 64 + ((rz & 3) * 16 + (ry & 3) * 4 + (rx & 3));
+        atomicAdd(&wm[slot], static_cast<unsigned long long>(static_cast<long long>(rint(sm))));
+        atomicAdd(&wq[slot], static_cast<unsigned long long>(static_cast<long long>(rint(sp))));
+      }
+    }
+  }
+  return ok;
+}
+
+__device__ inline uint32_t gcd_u32(uint32_t a, uint32_t b) {
+  while (b) {
+    const uint32_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+template <bool kTrial>
 __global__ __launch_bounds__(kThreads) void k_p2g(ParticleView p, BinList bins, GridView grid, double inv_qm,
                                                   double inv_qp, double limit, const uint32_t* perm,
                                                   unsigned long long* qm, unsigned long long* qp, uint64_t* err) {
   __shared__ unsigned long long wm[kWindow], wq[kWindow];
   __shared__ int32_t wblock[8];
+  __shared__ uint32_t stride;
   const uint32_t b = xcd_chunked(blockIdx.x, gridDim.x);
   int32_t bin[3];
   bin_coords(bins, bins.key[b], bin);
+  const uint32_t off = bins.offset[b], cnt = bins.count[b];
   for (int i = threadIdx.x; i < kWindow; i += kThreads) {
     wm[i] = 0;
     wq[i] = 0;
@@ -67,70 +160,61 @@ __global__ __launch_bounds__(kThreads) void k_p2g(ParticleView p, BinList bins, 
   if (threadIdx.x < 8)
     wblock[threadIdx.x] = grid.ordinal(bin[0] + (threadIdx.x & 1), bin[1] + ((threadIdx.x >> 1) & 1),
                                        bin[2] + (threadIdx.x >> 2));
+  if (threadIdx.x == 0) {
+    // Visit the bin with a stride coprime to its size: particles stored next to each other (often in
+    // the same cell, so with the same stencil) go to different lanes, which spreads LDS atomics.
+    uint32_t st = 9;
+    while (cnt > 1 && gcd_u32(st, cnt) != 1) st += 2;
+    stride = st;
+  }
   __syncthreads();
   const Geometry g = grid.g;
-  const uint32_t off = bins.offset[b], cnt = bins.count[b];
   const int32_t tile[3] = {4 * bin[0], 4 * bin[1], 4 * bin[2]};
-  for (uint32_t k = off + threadIdx.x; k < off + cnt; k += kThreads) {
-    const uint64_t ordinal = perm[k];
-    const double s[3] = {p.s[0][k], p.s[1][k], p.s[2][k]};
-    const double t[3] = {p.t[0][k], p.t[1][k], p.t[2][k]};
-    const double m = p.m[k];
-    bool motion = true;
-    for (int a = 0; a < 3; ++a) motion = motion && within_half_cell(s[a], t[a], g.h);
-    if (!motion) {
-      report(err, ordinal, kFailedPrecondition);
-      continue;
-    }
-    Coord tc[3];
-    for (int a = 0; a < 3; ++a) tc[a] = normalize(t[a], comp(g, a), g.h);
+  for (uint32_t v = threadIdx.x; v < cnt; v += kThreads) {
+    const uint32_t k = off + static_cast<uint32_t>((uint64_t(v) * stride) % cnt);
+    double t[3] = {p.s[0][k], p.s[1][k], p.s[2][k]};
     int code = kOk;
-    for (int axis = 0; axis < 3 && code == kOk; ++axis) {
-      AxisStencil st[3];
-      for (int c = 0; c < 3 && code == kOk; ++c) code = axis_stencil(tc[c], c == axis ? 0.0 : 0.5, bin[c], g.h, st[c]);
-      if (code) break;
-      for (int c = 0; c < 3 && code == kOk; ++c)
-        for (int i = 0; i < 3; ++i)
-          if (!finite(st[c].offset[i])) code = kOutOfRange;
-      if (code) break;
-      if (!window_ok(wblock, axis_blocks(st[0].first, tile[0]), axis_blocks(st[1].first, tile[1]),
-                     axis_blocks(st[2].first, tile[2]))) {
+    if (kTrial) {
+      const double s0 = t[0], s1 = t[1], s2 = t[2];
+      t[0] = p.t[0][k];
+      t[1] = p.t[1][k];
+      t[2] = p.t[2][k];
+      if (!(within_half_cell(s0, t[0], g.h) && within_half_cell(s1, t[1], g.h) && within_half_cell(s2, t[2], g.h)))
         code = kFailedPrecondition;
-        break;
-      }
-      const double v = p.v[axis][k];
-      const double cx = p.c[3 * axis][k], cy = p.c[3 * axis + 1][k], cz = p.c[3 * axis + 2][k];
-      for (int zi = 0; zi < 3 && code == kOk; ++zi) {
-        const int32_t rz = st[2].first + zi - tile[2];
-        const double wz = st[2].w[zi].r, ofz = st[2].offset[zi];
-        for (int yi = 0; yi < 3 && code == kOk; ++yi) {
-          const int32_t ry = st[1].first + yi - tile[1];
-          const double wy = st[1].w[yi].r, ofy = st[1].offset[yi];
-          for (int xi = 0; xi < 3; ++xi) {
-            const int32_t rx = st[0].first + xi - tile[0];
-            const double ofx = st[0].offset[xi];
-            const double w = st[0].w[xi].r * wy * wz;
-            const double mass = w * m;
-            const double inc = (cx * ofx + cy * ofy) + cz * ofz;
-            const double vel = v + inc;
-            const double mom = mass * vel;
-            const double sm = mass * inv_qm;  // equals mass / quantum: the quantum is a power of two
-            const double sp = mom * inv_qp;
-            if (!(finite(sm) && fabs(sm) <= limit && finite(sp) && fabs(sp) <= limit)) {
-              code = kOutOfRange;
-              break;
-            }
-            const int wb = ((rz >> 2) * 2 + (ry >> 2)) * 2 + (rx >> 2);
-            const int slot = wb * 192 + axis * 64 + ((rz & 3) * 16 + (ry & 3) * 4 + (rx & 3));
-            const long long im = static_cast<long long>(rint(sm));
-            const long long ip = static_cast<long long>(rint(sp));
-            if (im) atomicAdd(&wm[slot], static_cast<unsigned long long>(im));
-            if (ip) atomicAdd(&wq[slot], static_cast<unsigned long long>(ip));
-          }
-        }
-      }
     }
-    if (code) report(err, ordinal, code);
+    const Coord cx = normalize(t[0], g.ox, g.h, g.inv_h);
+    const Coord cy = normalize(t[1], g.oy, g.h, g.inv_h);
+    const Coord cz = normalize(t[2], g.oz, g.h, g.inv_h);
+    Axis3 nx, ny, nz, tx, ty, tz;  // normal (shift 0) and transverse (shift 1/2) samples per coordinate
+    const int cnx = axis3(cx, 0.0, bin[0], g.h, nx), ctx_ = axis3(cx, 0.5, bin[0], g.h, tx);
+    const int cny = axis3(cy, 0.0, bin[1], g.h, ny), cty = axis3(cy, 0.5, bin[1], g.h, ty);
+    const int cnz = axis3(cz, 0.0, bin[2], g.h, nz), ctz = axis3(cz, 0.5, bin[2], g.h, tz);
+    const double m = p.m[k];
+    // face axis x
+    if (code == kOk) code = axis_status(cnx, cty, ctz, nx, ty, tz);
+    if (code == kOk && !window_ok(wblock, axis_blocks(nx.first, tile[0]), axis_blocks(ty.first, tile[1]),
+                                  axis_blocks(tz.first, tile[2])))
+      code = kFailedPrecondition;
+    if (code == kOk && !deposit_axis<0>(nx, ty, tz, tile, m, p.v[0][k], p.c[0][k], p.c[1][k], p.c[2][k], inv_qm,
+                                        inv_qp, limit, wm, wq))
+      code = kOutOfRange;
+    // face axis y
+    if (code == kOk) code = axis_status(ctx_, cny, ctz, tx, ny, tz);
+    if (code == kOk && !window_ok(wblock, axis_blocks(tx.first, tile[0]), axis_blocks(ny.first, tile[1]),
+                                  axis_blocks(tz.first, tile[2])))
+      code = kFailedPrecondition;
+    if (code == kOk && !deposit_axis<1>(tx, ny, tz, tile, m, p.v[1][k], p.c[3][k], p.c[4][k], p.c[5][k], inv_qm,
+                                        inv_qp, limit, wm, wq))
+      code = kOutOfRange;
+    // face axis z
+    if (code == kOk) code = axis_status(ctx_, cty, cnz, tx, ty, nz);
+    if (code == kOk && !window_ok(wblock, axis_blocks(tx.first, tile[0]), axis_blocks(ty.first, tile[1]),
+                                  axis_blocks(nz.first, tile[2])))
+      code = kFailedPrecondition;
+    if (code == kOk && !deposit_axis<2>(tx, ty, nz, tile, m, p.v[2][k], p.c[6][k], p.c[7][k], p.c[8][k], inv_qm,
+                                        inv_qp, limit, wm, wq))
+      code = kOutOfRange;
+    if (code) report(err, perm[k], code);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < kWindow; i += kThreads) {
@@ -181,20 +265,73 @@ __global__ void k_normalize_forces(const int64_t* qm, const int64_t* qp, const i
   if (ustar) ustar[f] = pred;
 }
 
-// Contract a 3x3x3 tensor T[z][y][x] with one vector per axis.
-__device__ inline double contract(const double T[27], const double x[3], const double y[3], const double z[3]) {
-  double sz = 0.0;
-  for (int k = 0; k < 3; ++k) {
-    double sy = 0.0;
-    for (int j = 0; j < 3; ++j) {
-      const double* r = T + 9 * k + 3 * j;
-      sy += y[j] * ((x[0] * r[0] + x[1] * r[1]) + x[2] * r[2]);
-    }
-    sz += z[k] * sy;
+// Gathers one face axis from the LDS window: the separable contraction of the 3x3x3 velocities with
+// the weights, the weighted offsets and the weight derivatives (x, then y, then z), in FP64 with
+// fused multiply-adds. Results go straight to the component-major output columns.
+template <int A>
+__device__ __forceinline__ bool gather_axis(const Axis3& X, const Axis3& Y, const Axis3& Z, const int32_t* tile,
+                                           double inv_h, const double* wn, const double* wo, double* out,
+                                           uint64_t cap, uint64_t k) {
+  double wr[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    wr[0][i] = X.w[i] * (X.off[i] * inv_h);
+    wr[1][i] = Y.w[i] * (Y.off[i] * inv_h);
+    wr[2][i] = Z.w[i] * (Z.off[i] * inv_h);
   }
-  return sz;
+  double pic = 0.0, pold = 0.0, mx = 0.0, my = 0.0, mz = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+  for (int zi = 0; zi < 3; ++zi) {
+    double aa = 0.0, ay = 0.0, ady = 0.0, bx = 0.0, dx = 0.0, ao = 0.0;
+    const int rz = Z.first + zi - tile[2];
+#pragma unroll
+    for (int yi = 0; yi < 3; ++yi) {
+      double a = 0.0, b = 0.0, d = 0.0, o = 0.0;
+      const int ry = Y.first + yi - tile[1];
+#pragma unroll
+      for (int xi = 0; xi < 3; ++xi) {
+        const int rx = X.first + xi - tile[0];
+        const int slot = (((rz >> 2) * 2 + (ry >> 2)) * 2 + (rx >> 2)) * 192 + A * 64 + ((rz & 3) * 16 + (ry & 3) * 4 + (rx & 3));
+        const double un = wn[slot], uo = wo[slot];
+        a = fma(X.w[xi], un, a);
+        b = fma(wr[0][xi], un, b);
+        d = fma(X.dw[xi], un, d);
+        o = fma(X.w[xi], uo, o);
+      }
+      aa = fma(Y.w[yi], a, aa);
+      ay = fma(wr[1][yi], a, ay);
+      ady = fma(Y.dw[yi], a, ady);
+      bx = fma(Y.w[yi], b, bx);
+      dx = fma(Y.w[yi], d, dx);
+      ao = fma(Y.w[yi], o, ao);
+    }
+    pic = fma(Z.w[zi], aa, pic);
+    mz = fma(wr[2][zi], aa, mz);
+    gz = fma(Z.dw[zi], aa, gz);
+    my = fma(Z.w[zi], ay, my);
+    gy = fma(Z.w[zi], ady, gy);
+    mx = fma(Z.w[zi], bx, mx);
+    gx = fma(Z.w[zi], dx, gx);
+    pold = fma(Z.w[zi], ao, pold);
+  }
+  const double inc = pic - pold;
+  const double sx = (4.0 * mx) * inv_h, sy = (4.0 * my) * inv_h, sz = (4.0 * mz) * inv_h;
+  gx *= inv_h;
+  gy *= inv_h;
+  gz *= inv_h;
+  out[A * cap + k] = pic;
+  out[(3 + A) * cap + k] = inc;
+  out[(6 + 3 * A) * cap + k] = sx;
+  out[(7 + 3 * A) * cap + k] = sy;
+  out[(8 + 3 * A) * cap + k] = sz;
+  out[(15 + 3 * A) * cap + k] = gx;
+  out[(16 + 3 * A) * cap + k] = gy;
+  out[(17 + 3 * A) * cap + k] = gz;
+  return finite(pic) && finite(inc) && finite(sx) && finite(sy) && finite(sz) && finite(gx) && finite(gy) &&
+         finite(gz);
 }
 
+template <bool kTrial>
 __global__ __launch_bounds__(kThreads) void k_g2p(ParticleView p, BinList bins, GridView grid, const double* uold,
                                                   const double* unew, const uint32_t* perm, double* out, uint64_t cap,
                                                   uint64_t* err) {
@@ -215,83 +352,43 @@ __global__ __launch_bounds__(kThreads) void k_g2p(ParticleView p, BinList bins, 
   }
   __syncthreads();
   const Geometry g = grid.g;
-  const double inv_h = 1.0 / g.h;
+  const double inv_h = g.inv_h != 0.0 ? g.inv_h : 1.0 / g.h;
   const uint32_t off = bins.offset[b], cnt = bins.count[b];
   const int32_t tile[3] = {4 * bin[0], 4 * bin[1], 4 * bin[2]};
   for (uint32_t k = off + threadIdx.x; k < off + cnt; k += kThreads) {
-    const uint64_t ordinal = perm[k];
-    const double s[3] = {p.s[0][k], p.s[1][k], p.s[2][k]};
-    const double t[3] = {p.t[0][k], p.t[1][k], p.t[2][k]};
-    bool motion = true;
-    for (int a = 0; a < 3; ++a) motion = motion && within_half_cell(s[a], t[a], g.h);
-    if (!motion) {
-      report(err, ordinal, kFailedPrecondition);
-      continue;
-    }
-    Coord tc[3];
-    for (int a = 0; a < 3; ++a) tc[a] = normalize(t[a], comp(g, a), g.h);
-    double res[kGatherDoubles];
+    double t[3] = {p.s[0][k], p.s[1][k], p.s[2][k]};
     int code = kOk;
-    for (int axis = 0; axis < 3 && code == kOk; ++axis) {
-      AxisStencil st[3];
-      for (int c = 0; c < 3 && code == kOk; ++c) code = axis_stencil(tc[c], c == axis ? 0.0 : 0.5, bin[c], g.h, st[c]);
-      if (code) break;
-      for (int c = 0; c < 3 && code == kOk; ++c)
-        for (int i = 0; i < 3; ++i)
-          if (!finite(st[c].offset[i])) code = kOutOfRange;
-      if (code) break;
-      if (!window_ok(wblock, axis_blocks(st[0].first, tile[0]), axis_blocks(st[1].first, tile[1]),
-                     axis_blocks(st[2].first, tile[2]))) {
+    if (kTrial) {
+      const double s0 = t[0], s1 = t[1], s2 = t[2];
+      t[0] = p.t[0][k];
+      t[1] = p.t[1][k];
+      t[2] = p.t[2][k];
+      if (!(within_half_cell(s0, t[0], g.h) && within_half_cell(s1, t[1], g.h) && within_half_cell(s2, t[2], g.h)))
         code = kFailedPrecondition;
-        break;
-      }
-      double U[27], O[27];
-      for (int zi = 0; zi < 3; ++zi)
-        for (int yi = 0; yi < 3; ++yi)
-          for (int xi = 0; xi < 3; ++xi) {
-            const int32_t rx = st[0].first + xi - tile[0], ry = st[1].first + yi - tile[1],
-                          rz = st[2].first + zi - tile[2];
-            const int wb = ((rz >> 2) * 2 + (ry >> 2)) * 2 + (rx >> 2);
-            const int slot = wb * 192 + axis * 64 + ((rz & 3) * 16 + (ry & 3) * 4 + (rx & 3));
-            U[9 * zi + 3 * yi + xi] = wn[slot];
-            O[9 * zi + 3 * yi + xi] = wo[slot];
-          }
-      double w[3][3], wr[3][3], dw[3][3];
-      for (int c = 0; c < 3; ++c)
-        for (int i = 0; i < 3; ++i) {
-          w[c][i] = st[c].w[i].r;
-          wr[c][i] = w[c][i] * (st[c].offset[i] * inv_h);
-          dw[c][i] = st[c].dw[i];
-        }
-      const double pic = contract(U, w[0], w[1], w[2]);
-      const double pic_old = contract(O, w[0], w[1], w[2]);
-      const double mx = contract(U, wr[0], w[1], w[2]);
-      const double my = contract(U, w[0], wr[1], w[2]);
-      const double mz = contract(U, w[0], w[1], wr[2]);
-      const double gx = contract(U, dw[0], w[1], w[2]) * inv_h;
-      const double gy = contract(U, w[0], dw[1], w[2]) * inv_h;
-      const double gz = contract(U, w[0], w[1], dw[2]) * inv_h;
-      const double inc = pic - pic_old;
-      const double sx = (4.0 * mx) * inv_h, sy = (4.0 * my) * inv_h, sz = (4.0 * mz) * inv_h;
-      if (!(finite(pic) && finite(inc) && finite(sx) && finite(sy) && finite(sz) && finite(gx) && finite(gy) &&
-            finite(gz))) {
-        code = kOutOfRange;
-        break;
-      }
-      res[axis] = pic;
-      res[3 + axis] = inc;
-      res[6 + 3 * axis + 0] = sx;
-      res[6 + 3 * axis + 1] = sy;
-      res[6 + 3 * axis + 2] = sz;
-      res[15 + 3 * axis + 0] = gx;
-      res[15 + 3 * axis + 1] = gy;
-      res[15 + 3 * axis + 2] = gz;
     }
-    if (code) {
-      report(err, ordinal, code);
-      continue;
-    }
-    for (int c = 0; c < kGatherDoubles; ++c) out[c * cap + k] = res[c];
+    const Coord cx = normalize(t[0], g.ox, g.h, g.inv_h);
+    const Coord cy = normalize(t[1], g.oy, g.h, g.inv_h);
+    const Coord cz = normalize(t[2], g.oz, g.h, g.inv_h);
+    Axis3 nx, ny, nz, tx, ty, tz;
+    const int cnx = axis3(cx, 0.0, bin[0], g.h, nx), ctx_ = axis3(cx, 0.5, bin[0], g.h, tx);
+    const int cny = axis3(cy, 0.0, bin[1], g.h, ny), cty = axis3(cy, 0.5, bin[1], g.h, ty);
+    const int cnz = axis3(cz, 0.0, bin[2], g.h, nz), ctz = axis3(cz, 0.5, bin[2], g.h, tz);
+    if (code == kOk) code = axis_status(cnx, cty, ctz, nx, ty, tz);
+    if (code == kOk && !window_ok(wblock, axis_blocks(nx.first, tile[0]), axis_blocks(ty.first, tile[1]),
+                                  axis_blocks(tz.first, tile[2])))
+      code = kFailedPrecondition;
+    if (code == kOk && !gather_axis<0>(nx, ty, tz, tile, inv_h, wn, wo, out, cap, k)) code = kOutOfRange;
+    if (code == kOk) code = axis_status(ctx_, cny, ctz, tx, ny, tz);
+    if (code == kOk && !window_ok(wblock, axis_blocks(tx.first, tile[0]), axis_blocks(ny.first, tile[1]),
+                                  axis_blocks(tz.first, tile[2])))
+      code = kFailedPrecondition;
+    if (code == kOk && !gather_axis<1>(tx, ny, tz, tile, inv_h, wn, wo, out, cap, k)) code = kOutOfRange;
+    if (code == kOk) code = axis_status(ctx_, cty, cnz, tx, ty, nz);
+    if (code == kOk && !window_ok(wblock, axis_blocks(tx.first, tile[0]), axis_blocks(ty.first, tile[1]),
+                                  axis_blocks(nz.first, tile[2])))
+      code = kFailedPrecondition;
+    if (code == kOk && !gather_axis<2>(tx, ty, nz, tile, inv_h, wn, wo, out, cap, k)) code = kOutOfRange;
+    if (code) report(err, perm[k], code);
   }
 }
 
@@ -312,15 +409,20 @@ int unpack_particles(fl_ctx* ctx, uint64_t n) {
   return FL_OK;
 }
 
-int p2g_quanta(fl_ctx* ctx, uint64_t n, double inv_qm, double inv_qp, double limit) {
+int p2g_quanta(fl_ctx* ctx, uint64_t n, double inv_qm, double inv_qp, double limit, hipEvent_t before,
+               hipEvent_t after) {
   const uint64_t nf = ctx->nb * 192;
   FL_TRY(fill_i64(ctx, ctx->qm, nf, 0));
   FL_TRY(fill_i64(ctx, ctx->qp, nf, 0));
+  if (before) FL_HIP(hipEventRecord(before, ctx->stream));
+  if (after && (ctx->nbins == 0 || n == 0)) FL_HIP(hipEventRecord(after, ctx->stream));
   if (ctx->nbins == 0 || n == 0) return FL_OK;
-  k_p2g<<<ctx->nbins, kThreads, 0, ctx->stream>>>(ctx->sorted.view(), bin_list(ctx), grid_view(ctx), inv_qm, inv_qp,
-                                                  limit, ctx->perm, reinterpret_cast<unsigned long long*>(ctx->qm),
-                                                  reinterpret_cast<unsigned long long*>(ctx->qp), ctx->err);
+  auto* kernel = ctx->sorted.trial_aliases_sorted ? k_p2g<false> : k_p2g<true>;
+  kernel<<<ctx->nbins, kThreads, 0, ctx->stream>>>(ctx->sorted.view(), bin_list(ctx), grid_view(ctx), inv_qm, inv_qp,
+                                                   limit, ctx->perm, reinterpret_cast<unsigned long long*>(ctx->qm),
+                                                   reinterpret_cast<unsigned long long*>(ctx->qp), ctx->err);
   FL_HIP(hipGetLastError());
+  if (after) FL_HIP(hipEventRecord(after, ctx->stream));
   return FL_OK;
 }
 
@@ -343,11 +445,15 @@ int normalize_forces_stress(fl_ctx* ctx, const int64_t* qs, uint64_t nf, double 
   return FL_OK;
 }
 
-int g2p(fl_ctx* ctx, uint64_t n, const double* d_old, const double* d_new) {
+int g2p(fl_ctx* ctx, uint64_t n, const double* d_old, const double* d_new, hipEvent_t before, hipEvent_t after) {
+  if (before) FL_HIP(hipEventRecord(before, ctx->stream));
+  if (after && (ctx->nbins == 0 || n == 0)) FL_HIP(hipEventRecord(after, ctx->stream));
   if (ctx->nbins == 0 || n == 0) return FL_OK;
-  k_g2p<<<ctx->nbins, kThreads, 0, ctx->stream>>>(ctx->sorted.view(), bin_list(ctx), grid_view(ctx), d_old, d_new,
-                                                  ctx->perm, ctx->gather, ctx->sorted.capacity, ctx->err);
+  auto* kernel = ctx->sorted.trial_aliases_sorted ? k_g2p<false> : k_g2p<true>;
+  kernel<<<ctx->nbins, kThreads, 0, ctx->stream>>>(ctx->sorted.view(), bin_list(ctx), grid_view(ctx), d_old, d_new,
+                                                   ctx->perm, ctx->gather, ctx->sorted.capacity, ctx->err);
   FL_HIP(hipGetLastError());
+  if (after) FL_HIP(hipEventRecord(after, ctx->stream));
   return FL_OK;
 }
 

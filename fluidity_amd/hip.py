@@ -40,6 +40,7 @@ class Capacities(C.Structure):
 class StepReport(C.Structure):
     _fields_ = [("sort_ms", C.c_double), ("p2g_ms", C.c_double), ("normalize_ms", C.c_double),
                 ("projection_ms", C.c_double), ("g2p_ms", C.c_double), ("total_ms", C.c_double),
+                ("p2g_kernel_ms", C.c_double), ("g2p_kernel_ms", C.c_double),
                 ("particles", C.c_uint64), ("blocks", C.c_uint64), ("faces", C.c_uint64),
                 ("pressure_cells", C.c_uint64), ("nonempty_bins", C.c_uint64),
                 ("pressure", C.c_byte * R.PRESSURE_DIAGNOSTICS.itemsize), ("status", C.c_int32), ("pad", C.c_int32)]

@@ -189,6 +189,7 @@ int fl_project(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, 
  * projection -> G2P on the device. */
 typedef struct fl_step_report {
   double sort_ms, p2g_ms, normalize_ms, projection_ms, g2p_ms, total_ms;
+  double p2g_kernel_ms, g2p_kernel_ms; /* the P2G and G2P kernels alone (hipEvents around the launch) */
   uint64_t particles, blocks, faces, pressure_cells, nonempty_bins;
   fl_pressure_diagnostics pressure;
   int32_t status;
