@@ -48,6 +48,35 @@ def g2p_algorithmic_bytes(n_particles, n_faces):
     return 216 * n_particles + 16 * n_faces
 
 
+def copy_ceiling(nbytes=1 << 30, reps=5):
+    """Device-to-device copy of `nbytes` (a plain copy kernel): measured bandwidth in GB/s.
+
+    The same copy, of known size, calibrates the FETCH_SIZE / WRITE_SIZE counters.
+    """
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    hip.hipEventCreate(C.byref(e0))
+    hip.hipEventCreate(C.byref(e1))
+    hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3)  # warm up (hipMemcpyDeviceToDevice)
+    hip.hipDeviceSynchronize()
+    hip.hipEventRecord(e0, None)
+    for _ in range(reps):
+        hip.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), 3, None)
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = C.c_float()
+    hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+
+
 def cpu_baseline(scene, ctx, report, sample, threads=1):
     """The oracle (C++ restatement of the reference operators) timed on this host.
 
@@ -107,7 +136,11 @@ def main():
     ap.add_argument("--coarse-sweeps", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=1 << 20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true", help="only measure the plain-copy ceiling")
     args = ap.parse_args()
+    if args.copy_probe:
+        print(json.dumps({"copy_ceiling_gbs": copy_ceiling(), "bytes_per_copy": 1 << 30}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; runs on more than one GPU are not part of it",
                           "n_gpus": args.gpus}))
@@ -190,6 +223,7 @@ def main():
             "traffic": None,
             "algorithmic_bytes": p2g_bytes,
             "g2p_achieved": g2p_algorithmic_bytes(n, nf) / (g2p_ms * 1e-3) / 1e9,
+            "copy_ceiling_gbs": copy_ceiling(),
         },
     }
     if not args.no_cpu_baseline:

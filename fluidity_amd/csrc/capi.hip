@@ -457,6 +457,37 @@ int fl_normalize_forces(fl_ctx* ctx, const fl_face_quanta* quanta, size_t face_c
   return FL_OK;
 }
 
+int fl_apply_forces(fl_ctx* ctx, const double* old_face_velocities, size_t face_count, fl_vec3 acceleration,
+                    double dt, double* out_predicted, fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  if (face_count % 192 != 0) return FL_INVALID_ARGUMENT;
+  if (face_count > 192 * ctx->cap.blocks) return FL_RESOURCE_EXHAUSTED;
+  if (face_count && (!old_face_velocities || !out_predicted)) return FL_INVALID_ARGUMENT;
+  if (overlaps(out_predicted, face_count * sizeof(double), old_face_velocities, face_count * sizeof(double)))
+    return FL_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  double copy_ms = 0;
+  Timer t0(ctx);
+  if (face_count)
+    FL_HIP(hipMemcpyAsync(ctx->u0, old_face_velocities, face_count * sizeof(double), hipMemcpyHostToDevice,
+                          ctx->stream));
+  copy_ms += t0.stop();
+  Timer t(ctx);
+  FL_TRY(reset_error(ctx));
+  const double g[3] = {acceleration.x, acceleration.y, acceleration.z};
+  FL_TRY(apply_forces(ctx, ctx->u0, face_count, g, dt, ctx->ustar));
+  FL_HIP(hipMemcpyAsync(ctx->host_words, ctx->err, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  FL_HIP(hipStreamSynchronize(ctx->stream));
+  const double kernel_ms = t.stop();
+  if (ctx->host_words[0] != ~uint64_t(0)) return static_cast<int>(ctx->host_words[0] & 0xff);
+  Timer c(ctx);
+  if (face_count)
+    FL_HIP(hipMemcpyAsync(out_predicted, ctx->ustar, face_count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  copy_ms += c.stop();
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, 1);
+  return FL_OK;
+}
+
 int fl_g2p(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
            const fl_particle* particles, size_t n, const double* old_face_velocities,
            const double* new_face_velocities, size_t face_count, fl_gather_result* out, fl_measurement* measurement) {

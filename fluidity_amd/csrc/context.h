@@ -181,6 +181,7 @@ BinList bin_list(fl_ctx* ctx);
 int p2g_quanta(fl_ctx* ctx, uint64_t n, double inv_qm, double inv_qp, double limit, hipEvent_t before = nullptr,
                hipEvent_t after = nullptr);
 int normalize_forces(fl_ctx* ctx, uint64_t nf, double qm, double qp, double min_mass, const double g[3], double dt);
+int apply_forces(fl_ctx* ctx, const double* d_old, uint64_t nf, const double g[3], double dt, double* d_pred);
 int g2p(fl_ctx* ctx, uint64_t n, const double* d_old, const double* d_new, hipEvent_t before = nullptr,
         hipEvent_t after = nullptr);
 int unpack_particles(fl_ctx* ctx, uint64_t n);

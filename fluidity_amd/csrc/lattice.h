@@ -37,13 +37,17 @@ __device__ inline double sub_roundoff(double a, double b, double d) {
 }
 
 // One coordinate in cell units with the world subtraction kept for exact edge decisions.
-// invH is the exact reciprocal of H when the cell width is a power of two (then s / H and s * invH are
-// the same correctly rounded value), and 0 otherwise.
-struct Coord {
+// kPow2: the cell width is a power of two and invH is its exact reciprocal, so s / H and s * invH
+// are the same correctly rounded value; otherwise every quotient is a division.
+template <bool kPow2>
+struct CoordT {
   double X, D, E, H, invH;
   int shift;
 
-  __device__ double div_h(double s) const { return invH != 0.0 ? s * invH : s / H; }
+  __device__ double div_h(double s) const {
+    if constexpr (kPow2) return s * invH;
+    return s / H;
+  }
   __device__ double scaled_offset(double edge) const { return fma(-edge, H, D) + E; }
   __device__ bool below(double edge) const {
     const double off = X - edge;
@@ -64,10 +68,29 @@ struct Coord {
     const double s = scaled_offset(anchor);
     return s == 0.0 ? 0.0 : -ldexp(s, -shift);
   }
+
+  // Interior coordinates: 4 < |X| < 2^30 and |D| < 2^1000. Every edge and anchor of a transfer stencil
+  // lies within 1.5 cells of X, so X - edge is smaller than X in magnitude and offset() takes its
+  // compensated branch, whose scaled subtraction cannot overflow (|edge H| is about |D|); and X is far
+  // inside the int32 lattice, so the support range checks pass. The *_interior forms below skip
+  // exactly the decisions this settles and return the same values.
+  __device__ bool interior() const {
+    const double a = fabs(X);
+    return a > 4.0 && a < 0x1p30 && fabs(D) < 0x1p1000;
+  }
+  __device__ double offset_interior(double edge) const { return div_h(scaled_offset(edge)); }
+  __device__ double displacement_interior(double anchor, double h) const {
+    const double n = -offset_interior(anchor);
+    const double d = n * h;
+    if (normal(n) && finite(d)) return d;
+    const double s = scaled_offset(anchor);
+    return s == 0.0 ? 0.0 : -ldexp(s, -shift);
+  }
 };
 
-// inv_h: exact 1/h when h is a power of two, else 0 (every quotient then uses a division).
-__device__ inline Coord normalize(double x, double o, double h, double inv_h = 0.0) {
+// inv_h: exact 1/h when kPow2 (h a power of two), unused otherwise.
+template <bool kPow2>
+__device__ inline CoordT<kPow2> normalize(double x, double o, double h, double inv_h) {
   double diff = x - o;
   int e = 0;
   if (!finite(diff)) {
@@ -77,17 +100,17 @@ __device__ inline Coord normalize(double x, double o, double h, double inv_h = 0
     e = -1;
   }
   double H = ldexp(h, e);
-  double invH = inv_h != 0.0 ? ldexp(inv_h, -e) : 0.0;
-  const double X = invH != 0.0 ? diff * invH : diff / H;
+  double invH = kPow2 ? ldexp(inv_h, -e) : 0.0;
+  const double X = kPow2 ? diff * invH : diff / H;
   double err = sub_roundoff(x, o, diff);
   if (finite(X) && H < 0x1p-960) {
     e += 64;
     diff = ldexp(diff, 64);
     err = ldexp(err, 64);
     H = ldexp(H, 64);
-    invH = invH != 0.0 ? ldexp(invH, -64) : 0.0;
+    invH = kPow2 ? ldexp(invH, -64) : 0.0;
   }
-  Coord c;
+  CoordT<kPow2> c;
   c.X = X;
   c.D = diff;
   c.E = err;
@@ -97,6 +120,17 @@ __device__ inline Coord normalize(double x, double o, double h, double inv_h = 0
   return c;
 }
 
+// floor(true coordinate - shift) as an exact integer-valued double (|X| < 2^52 on every caller's path).
+template <class Coord>
+__device__ inline double floor_shifted_d(const Coord& c, double shift) {
+  const double k = floor(c.X - shift);
+  const double edge = k + shift;
+  if (c.below(edge)) return k - 1.0;
+  if (c.below(edge + 1.0)) return k;
+  return k + 1.0;
+}
+
+template <class Coord>
 __device__ inline int64_t floor_shifted(const Coord& c, double shift) {
   const int64_t k = static_cast<int64_t>(floor(c.X - shift));
   const double edge = static_cast<double>(k) + shift;
@@ -106,6 +140,7 @@ __device__ inline int64_t floor_shifted(const Coord& c, double shift) {
 }
 
 // Sort bin of one coordinate, kOutOfRange outside the int32 block lattice.
+template <class Coord>
 __device__ inline int checked_bin(const Coord& c, int32_t& bin) {
   const double first_edge = static_cast<double>(kI32Min * 4) + 1.5;
   const double past_last_edge = static_cast<double>((kI32Max + 1) * 4) + 1.5;
@@ -146,6 +181,7 @@ struct AxisStencil {
 
 // kOutOfRange for unrepresentable lattice indices, kFailedPrecondition when the support leaves the
 // sorted bin's transfer tile [4B, 4B+8).
+template <class Coord>
 __device__ inline int axis_stencil(const Coord& c, double lattice_shift, int32_t bin, double h, AxisStencil& s) {
   const double start = lattice_shift + 0.5;
   const double first_support = static_cast<double>(kI32Min) + start;
@@ -186,28 +222,10 @@ struct Geometry {
   double inv_h;  // exact 1/h when h is a power of two, else 0
 };
 
-__device__ inline double comp(const Geometry& g, int a) { return a == 0 ? g.ox : (a == 1 ? g.oy : g.oz); }
+// Kernels are instantiated for power-of-two cell widths (exact reciprocal) and for general widths;
+// the host picks the instance from Geometry::inv_h.
 
-// Normalises both positions and computes the bin. Returns the first failure in the reference's
-// order: sorted position, trial position, then the bin range (sparse_mac_grid_3d.cc:463-474).
-__device__ inline int prepare_positions(const Geometry& g, const double s[3], const double t[3], Coord sc[3],
-                                        Coord tc[3], int32_t bin[3]) {
-  for (int a = 0; a < 3; ++a)
-    if (!finite(s[a])) return kInvalidArgument;
-  for (int a = 0; a < 3; ++a) {
-    sc[a] = normalize(s[a], comp(g, a), g.h, g.inv_h);
-    if (!finite(sc[a].X)) return kOutOfRange;
-  }
-  for (int a = 0; a < 3; ++a)
-    if (!finite(t[a])) return kInvalidArgument;
-  for (int a = 0; a < 3; ++a) {
-    tc[a] = normalize(t[a], comp(g, a), g.h, g.inv_h);
-    if (!finite(tc[a].X)) return kOutOfRange;
-  }
-  for (int a = 0; a < 3; ++a)
-    if (int st = checked_bin(sc[a], bin[a])) return st;
-  return kOk;
-}
+__device__ inline double comp(const Geometry& g, int a) { return a == 0 ? g.ox : (a == 1 ? g.oy : g.oz); }
 
 // Error word ordered by particle ordinal, then by status code.
 __device__ __host__ inline uint64_t error_word(uint64_t ordinal, int code) {

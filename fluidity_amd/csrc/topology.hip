@@ -46,7 +46,7 @@ inline unsigned capped_grid(uint64_t n) {
 // then the bin range (sparse_mac_grid_3d.cc:463-474). Gathers skip the mass/velocity checks and
 // plain bin queries skip the trial position (sparse_mac_grid_3d.cc:438-449). Scalar code only, so
 // nothing spills to scratch.
-template <int kMode>
+template <int kMode, bool kPow2>
 __global__ void k_bin(ParticleView p, uint64_t n, Geometry g, int32_t* bins, uint64_t* err, int32_t* partial) {
   __shared__ int32_t smin[3][kThreads / 64], smax[3][kThreads / 64];
   int32_t mn0 = 0x7fffffff, mn1 = 0x7fffffff, mn2 = 0x7fffffff;
@@ -64,16 +64,17 @@ __global__ void k_bin(ParticleView p, uint64_t n, Geometry g, int32_t* bins, uin
     }
     const double s0 = p.s[0][i], s1 = p.s[1][i], s2 = p.s[2][i];
     if (st == kOk && !(finite(s0) && finite(s1) && finite(s2))) st = kInvalidArgument;
-    const Coord c0 = normalize(s0, g.ox, g.h, g.inv_h);
-    const Coord c1 = normalize(s1, g.oy, g.h, g.inv_h);
-    const Coord c2 = normalize(s2, g.oz, g.h, g.inv_h);
+    const auto c0 = normalize<kPow2>(s0, g.ox, g.h, g.inv_h);
+    const auto c1 = normalize<kPow2>(s1, g.oy, g.h, g.inv_h);
+    const auto c2 = normalize<kPow2>(s2, g.oz, g.h, g.inv_h);
     if (st == kOk && !(finite(c0.X) && finite(c1.X) && finite(c2.X))) st = kOutOfRange;
     if (kMode != 0 && p.t[0] != p.s[0]) {
       const double t0 = p.t[0][i], t1 = p.t[1][i], t2 = p.t[2][i];
       if (st == kOk && !(finite(t0) && finite(t1) && finite(t2))) st = kInvalidArgument;
       if (st == kOk) {
-        const bool fin = finite(normalize(t0, g.ox, g.h, g.inv_h).X) && finite(normalize(t1, g.oy, g.h, g.inv_h).X) &&
-                         finite(normalize(t2, g.oz, g.h, g.inv_h).X);
+        const bool fin = finite(normalize<kPow2>(t0, g.ox, g.h, g.inv_h).X) &&
+                         finite(normalize<kPow2>(t1, g.oy, g.h, g.inv_h).X) &&
+                         finite(normalize<kPow2>(t2, g.oz, g.h, g.inv_h).X);
         if (!fin) st = kOutOfRange;
       }
     }
@@ -349,15 +350,16 @@ __global__ void k_sort_entries(const int32_t* bins, const uint32_t* perm, uint64
 }
 
 // Owners of the geometry stencils of one particle (sorted = trial position), per face axis.
+template <bool kPow2>
 __global__ void k_activate_particles(const double* sx, const double* sy, const double* sz, const uint32_t* perm,
                                      uint64_t n, Geometry g, Box box, uint8_t* mark, uint64_t* err) {
   const uint64_t k = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (k >= n) return;
   const double s[3] = {sx[k], sy[k], sz[k]};
-  Coord c[3];
+  CoordT<kPow2> c[3];
   int32_t bin[3];
   for (int a = 0; a < 3; ++a) {
-    c[a] = normalize(s[a], comp(g, a), g.h, g.inv_h);
+    c[a] = normalize<kPow2>(s[a], comp(g, a), g.h, g.inv_h);
     checked_bin(c[a], bin[a]);
   }
   int32_t lo[2][3], hi[2][3];  // [normal?][coordinate]
@@ -479,7 +481,9 @@ int bin_particles(fl_ctx* ctx, const ParticleView& p, uint64_t n, bool deposit_c
   FL_TRY(reset_error(ctx));
   const int mode = deposit_checks ? 1 : (gather_checks ? 2 : 0);
   if (n) {
-    auto* kernel = mode == 1 ? k_bin<1> : (mode == 2 ? k_bin<2> : k_bin<0>);
+    const bool pow2 = ctx->geometry.inv_h != 0.0;
+    auto* kernel = pow2 ? (mode == 1 ? k_bin<1, true> : (mode == 2 ? k_bin<2, true> : k_bin<0, true>))
+                        : (mode == 1 ? k_bin<1, false> : (mode == 2 ? k_bin<2, false> : k_bin<0, false>));
     const unsigned groups = std::min<unsigned>(grid_for(n), kBinGroups);
     int32_t* partial = reinterpret_cast<int32_t*>(ctx->scan_partial);  // 6 * kBinGroups ints
     kernel<<<groups, kThreads, 0, ctx->stream>>>(p, n, ctx->geometry, ctx->bin, ctx->err, partial);
@@ -561,8 +565,9 @@ int activate_blocks(fl_ctx* ctx, uint64_t n, const int32_t* d_cells, uint64_t ce
   FL_HIP(hipGetLastError());
   if (n) {
     const ParticleView v = ctx->sorted.view();
-    k_activate_particles<<<grid_for(n), kThreads, 0, ctx->stream>>>(v.s[0], v.s[1], v.s[2], ctx->perm, n,
-                                                                     ctx->geometry, box, ctx->mark, ctx->err);
+    auto* kernel = ctx->geometry.inv_h != 0.0 ? k_activate_particles<true> : k_activate_particles<false>;
+    kernel<<<grid_for(n), kThreads, 0, ctx->stream>>>(v.s[0], v.s[1], v.s[2], ctx->perm, n, ctx->geometry, box,
+                                                      ctx->mark, ctx->err);
     FL_HIP(hipGetLastError());
   }
   if (cell_count) {

@@ -66,18 +66,31 @@ struct Axis3 {
   double dw[3];
 };
 
+// kInterior: every lane of the wave has an interior coordinate (CoordT::interior), so the support
+// range checks are known to pass and every offset takes the compensated branch.
+template <bool kInterior, class Coord>
 __device__ __forceinline__ int axis3(const Coord& c, double lattice_shift, int32_t bin, double h, Axis3& s) {
   const double start = lattice_shift + 0.5;
-  const double first_support = static_cast<double>(kI32Min) + start;
-  const double past_last_support = static_cast<double>(kI32Max - 1) + start;
-  if (c.below(first_support) || !c.below(past_last_support)) return kOutOfRange;
-  const int64_t first = floor_shifted(c, start);
-  const int64_t tile = static_cast<int64_t>(bin) * 4;
-  if (first < tile || first + 2 >= tile + 8) return kFailedPrecondition;
-  const double a0 = static_cast<double>(first) + lattice_shift;
-  const double dl = c.offset(a0 + 0.5);
-  const double dr = fabs(c.offset(a0 + 1.5));
-  const double cc = c.offset(a0 + 1.0);
+  if (!kInterior) {
+    const double first_support = static_cast<double>(kI32Min) + start;
+    const double past_last_support = static_cast<double>(kI32Max - 1) + start;
+    if (c.below(first_support) || !c.below(past_last_support)) return kOutOfRange;
+  }
+  // first is an integer in [-2^31, 2^31 - 2]; the tile test [4B, 4B+8) runs on exact doubles.
+  const double first = floor_shifted_d(c, start);
+  const double r = first - 4.0 * static_cast<double>(bin);
+  if (!(r >= 0.0 && r <= 5.0)) return kFailedPrecondition;
+  const double a0 = first + lattice_shift;
+  double dl, dr, cc;
+  if (kInterior) {
+    dl = c.offset_interior(a0 + 0.5);
+    dr = fabs(c.offset_interior(a0 + 1.5));
+    cc = c.offset_interior(a0 + 1.0);
+  } else {
+    dl = c.offset(a0 + 0.5);
+    dr = fabs(c.offset(a0 + 1.5));
+    cc = c.offset(a0 + 1.0);
+  }
   s.first = static_cast<int32_t>(first);
   s.w[0] = 0.5 * dr * dr;
   s.w[1] = 0.75 - cc * cc;
@@ -86,7 +99,9 @@ __device__ __forceinline__ int axis3(const Coord& c, double lattice_shift, int32
   s.dw[1] = -2.0 * cc;
   s.dw[2] = dl;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) s.off[i] = c.displacement(a0 + static_cast<double>(i), h);
+  for (int i = 0; i < 3; ++i)
+    s.off[i] = kInterior ? c.displacement_interior(a0 + static_cast<double>(i), h)
+                         : c.displacement(a0 + static_cast<double>(i), h);
   return kOk;
 }
 
@@ -100,37 +115,151 @@ __device__ __forceinline__ int axis_status(int cx, int cy, int cz, const Axis3& 
   return fin ? kOk : kOutOfRange;
 }
 
-// Deposits the 27 samples of one face axis into the LDS window; returns false if a scaled
-// contribution is out of range (transfers_3d.cc:178-187). Contributions are formed exactly as in
-// AccumulateMacParticleComponent (mac_deposit_3d_internal.h:50-57).
-template <int A>
-__device__ __forceinline__ bool deposit_axis(const Axis3& X, const Axis3& Y, const Axis3& Z, const int32_t* tile,
-                                             double m, double v, double cx, double cy, double cz, double inv_qm,
-                                             double inv_qp, double limit, unsigned long long* wm,
-                                             unsigned long long* wq) {
+// Integer quanta of one contribution x / quantum = x * scale, rounded half to even
+// (transfers_3d.cc:178-187). The scale is a power of two, so x * scale is exact.
+// kSmall: the contribution limit is at most 2^50, so every in-range scaled value s lies in
+// [-2^50, 2^50] and s + 1.5 * 2^52 lies in [2^52, 2^53), where the FP64 spacing is 1: the single
+// rounding of fma(x, scale, 1.5 * 2^52) rounds s half to even to an integer r (1.5 * 2^52 is even)
+// and the result's bit pattern is that of 1.5 * 2^52 plus r. Values beyond the limit fail the call,
+// so whatever bits they produce never reach an output.
+template <bool kSmall>
+__device__ __forceinline__ unsigned long long quanta(double x, double scale) {
+  if constexpr (kSmall) {
+    const double t = fma(x, scale, 0x1.8p52);
+    return static_cast<unsigned long long>(__double_as_longlong(t)) - 0x4338000000000000ull;
+  }
+  return static_cast<unsigned long long>(static_cast<long long>(rint(x * scale)));
+}
+
+// Range test of the 27 scaled contributions of one face axis: |mass / quantum| and
+// |momentum / quantum| must not exceed the limit (transfers_3d.cc:178-187); NaN and infinities fail.
+// A per-particle bound settles the common case: every weight is below 1, so |mass| <= |m|, and by
+// monotonic rounding |velocity| <= |v| + ((OX + OY) + OZ) with OX the rounded sum of the |cx ox_i|,
+// so |mass / qm| <= |m| / qm and |momentum / qp| <= (|m| V) / qp. NaN anywhere fails the bound and
+// falls through to the exact test of every contribution.
+__device__ __forceinline__ bool range_ok(const Axis3& X, const Axis3& Y, const Axis3& Z, double m, double v,
+                                         double cx, double cy, double cz, double inv_qm, double inv_qp,
+                                         double limit) {
+  double ox[3], oy[3], oz[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    ox[i] = cx * X.off[i];
+    oy[i] = cy * Y.off[i];
+    oz[i] = cz * Z.off[i];
+  }
+  const double OX = (fabs(ox[0]) + fabs(ox[1])) + fabs(ox[2]);
+  const double OY = (fabs(oy[0]) + fabs(oy[1])) + fabs(oy[2]);
+  const double OZ = (fabs(oz[0]) + fabs(oz[1])) + fabs(oz[2]);
+  const double V = fabs(v) + ((OX + OY) + OZ);
+  if (fabs(m) * inv_qm <= limit && (fabs(m) * V) * inv_qp <= limit) return true;
   bool ok = true;
+#pragma unroll
+  for (int zi = 0; zi < 3; ++zi)
+#pragma unroll
+    for (int yi = 0; yi < 3; ++yi)
+#pragma unroll
+      for (int xi = 0; xi < 3; ++xi) {
+        const double mass = ((X.w[xi] * Y.w[yi]) * Z.w[zi]) * m;
+        const double mom = mass * (v + ((ox[xi] + oy[yi]) + oz[zi]));
+        ok = ok && fabs(mass * inv_qm) <= limit && fabs(mom * inv_qp) <= limit;
+      }
+  return ok;
+}
+
+// Deposits the 27 samples of one face axis into the LDS window. Contributions are formed exactly as in
+// AccumulateMacParticleComponent (mac_deposit_3d_internal.h:50-57): weight (wx wy) wz, mass w m,
+// velocity v + ((cx ox + cy oy) + cz oz), momentum mass * velocity.
+template <int A, bool kSmall>
+__device__ __forceinline__ void deposit_axis(const Axis3& X, const Axis3& Y, const Axis3& Z, const int32_t* tile,
+                                             double m, double v, double cx, double cy, double cz, double inv_qm,
+                                             double inv_qp, unsigned long long* wm, unsigned long long* wq) {
+  // Window slot of sample (xi, yi, zi) = fz[zi] + fy[yi] + fx[xi]: window block (rz>>2, ry>>2, rx>>2)
+  // times 192, face axis A times 64, cell (rz&3)*16 + (ry&3)*4 + (rx&3).
+  int fx[3], fy[3], fz[3];
+  double ox[3], oy[3], oz[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int rx = X.first + i - tile[0], ry = Y.first + i - tile[1], rz = Z.first + i - tile[2];
+    fx[i] = (rx >> 2) * 192 + (rx & 3) + A * 64;
+    fy[i] = (ry >> 2) * 384 + (ry & 3) * 4;
+    fz[i] = (rz >> 2) * 768 + (rz & 3) * 16;
+    ox[i] = cx * X.off[i];
+    oy[i] = cy * Y.off[i];
+    oz[i] = cz * Z.off[i];
+  }
 #pragma unroll
   for (int zi = 0; zi < 3; ++zi) {
 #pragma unroll
     for (int yi = 0; yi < 3; ++yi) {
+      const double wxy[3] = {X.w[0] * Y.w[yi], X.w[1] * Y.w[yi], X.w[2] * Y.w[yi]};
+      const double oxy[3] = {ox[0] + oy[yi], ox[1] + oy[yi], ox[2] + oy[yi]};
 #pragma unroll
       for (int xi = 0; xi < 3; ++xi) {
-        const double w = X.w[xi] * Y.w[yi] * Z.w[zi];
-        const double mass = w * m;
-        const double inc = (cx * X.off[xi] + cy * Y.off[yi]) + cz * Z.off[zi];
-        const double vel = v + inc;
-        const double mom = mass * vel;
-        const double sm = mass * inv_qm;  // equals mass / quantum: the quantum is a power of two
-        const double sp = mom * inv_qp;
-        ok = ok && finite(sm) && fabs(sm) <= limit && finite(sp) && fabs(sp) <= limit;
-        const int rx = X.first + xi - tile[0], ry = Y.first + yi - tile[1], rz = Z.first + zi - tile[2];
-        const int slot = (((rz >> 2) * 2 + (ry >> 2)) * 2 + (rx >> 2)) * 192 + A * 64 + ((rz & 3) * 16 + (ry & 3) * 4 + (rx & 3));
-        atomicAdd(&wm[slot], static_cast<unsigned long long>(static_cast<long long>(rint(sm))));
-        atomicAdd(&wq[slot], static_cast<unsigned long long>(static_cast<long long>(rint(sp))));
+        const double mass = (wxy[xi] * Z.w[zi]) * m;
+        const double mom = mass * (v + (oxy[xi] + oz[zi]));
+        const int slot = fz[zi] + fy[yi] + fx[xi];
+        atomicAdd(&wm[slot], quanta<kSmall>(mass, inv_qm));
+        atomicAdd(&wq[slot], quanta<kSmall>(mom, inv_qp));
       }
     }
   }
-  return ok;
+}
+
+// One particle's three face axes in the reference's order (transfers_3d.cc:75-100): per axis the
+// stencil status, the window ownership, then the contribution range. Returns the first failure.
+template <bool kInterior, bool kSmall, class Coord>
+__device__ __forceinline__ int deposit_particle(const Coord& cx, const Coord& cy, const Coord& cz, const int32_t* bin,
+                                                const int32_t* tile, const int32_t* wblock, double h,
+                                                const ParticleView& p, uint32_t k, double inv_qm, double inv_qp,
+                                                double limit, unsigned long long* wm, unsigned long long* wq,
+                                                int code) {
+  const double m = p.m[k];
+  // Normal (shift 0) and transverse (shift 1/2) samples per coordinate, each computed just before
+  // its first use so that fewer stencils are live at once.
+  Axis3 ty, tz;
+  const int cty = axis3<kInterior>(cy, 0.5, bin[1], h, ty), ctz = axis3<kInterior>(cz, 0.5, bin[2], h, tz);
+  {  // face axis x
+    Axis3 nx;
+    const int cnx = axis3<kInterior>(cx, 0.0, bin[0], h, nx);
+    if (code == kOk) code = axis_status(cnx, cty, ctz, nx, ty, tz);
+    if (code == kOk && !window_ok(wblock, axis_blocks(nx.first, tile[0]), axis_blocks(ty.first, tile[1]),
+                                  axis_blocks(tz.first, tile[2])))
+      code = kFailedPrecondition;
+    if (code == kOk) {
+      const double v = p.v[0][k], c0 = p.c[0][k], c1 = p.c[1][k], c2 = p.c[2][k];
+      if (!range_ok(nx, ty, tz, m, v, c0, c1, c2, inv_qm, inv_qp, limit)) code = kOutOfRange;
+      deposit_axis<0, kSmall>(nx, ty, tz, tile, m, v, c0, c1, c2, inv_qm, inv_qp, wm, wq);
+    }
+  }
+  Axis3 tx;
+  const int ctx_ = axis3<kInterior>(cx, 0.5, bin[0], h, tx);
+  {  // face axis y
+    Axis3 ny;
+    const int cny = axis3<kInterior>(cy, 0.0, bin[1], h, ny);
+    if (code == kOk) code = axis_status(ctx_, cny, ctz, tx, ny, tz);
+    if (code == kOk && !window_ok(wblock, axis_blocks(tx.first, tile[0]), axis_blocks(ny.first, tile[1]),
+                                  axis_blocks(tz.first, tile[2])))
+      code = kFailedPrecondition;
+    if (code == kOk) {
+      const double v = p.v[1][k], c0 = p.c[3][k], c1 = p.c[4][k], c2 = p.c[5][k];
+      if (!range_ok(tx, ny, tz, m, v, c0, c1, c2, inv_qm, inv_qp, limit)) code = kOutOfRange;
+      deposit_axis<1, kSmall>(tx, ny, tz, tile, m, v, c0, c1, c2, inv_qm, inv_qp, wm, wq);
+    }
+  }
+  {  // face axis z
+    Axis3 nz;
+    const int cnz = axis3<kInterior>(cz, 0.0, bin[2], h, nz);
+    if (code == kOk) code = axis_status(ctx_, cty, cnz, tx, ty, nz);
+    if (code == kOk && !window_ok(wblock, axis_blocks(tx.first, tile[0]), axis_blocks(ty.first, tile[1]),
+                                  axis_blocks(nz.first, tile[2])))
+      code = kFailedPrecondition;
+    if (code == kOk) {
+      const double v = p.v[2][k], c0 = p.c[6][k], c1 = p.c[7][k], c2 = p.c[8][k];
+      if (!range_ok(tx, ty, nz, m, v, c0, c1, c2, inv_qm, inv_qp, limit)) code = kOutOfRange;
+      deposit_axis<2, kSmall>(tx, ty, nz, tile, m, v, c0, c1, c2, inv_qm, inv_qp, wm, wq);
+    }
+  }
+  return code;
 }
 
 __device__ inline uint32_t gcd_u32(uint32_t a, uint32_t b) {
@@ -142,7 +271,7 @@ __device__ inline uint32_t gcd_u32(uint32_t a, uint32_t b) {
   return a;
 }
 
-template <bool kTrial>
+template <bool kTrial, bool kPow2, bool kSmall>
 __global__ __launch_bounds__(kThreads) void k_p2g(ParticleView p, BinList bins, GridView grid, double inv_qm,
                                                   double inv_qp, double limit, const uint32_t* perm,
                                                   unsigned long long* qm, unsigned long long* qp, uint64_t* err) {
@@ -170,8 +299,13 @@ __global__ __launch_bounds__(kThreads) void k_p2g(ParticleView p, BinList bins, 
   __syncthreads();
   const Geometry g = grid.g;
   const int32_t tile[3] = {4 * bin[0], 4 * bin[1], 4 * bin[2]};
+  // Lane visit order: v -> (v * stride) mod cnt, advanced by (kThreads * stride) mod cnt per pass.
+  const uint32_t st = stride, step = cnt ? (kThreads * st) % cnt : 0;
+  uint32_t idx = cnt ? (threadIdx.x * st) % cnt : 0;
   for (uint32_t v = threadIdx.x; v < cnt; v += kThreads) {
-    const uint32_t k = off + static_cast<uint32_t>((uint64_t(v) * stride) % cnt);
+    const uint32_t k = off + idx;
+    idx += step;
+    idx = idx >= cnt ? idx - cnt : idx;
     double t[3] = {p.s[0][k], p.s[1][k], p.s[2][k]};
     int code = kOk;
     if (kTrial) {
@@ -182,38 +316,15 @@ __global__ __launch_bounds__(kThreads) void k_p2g(ParticleView p, BinList bins, 
       if (!(within_half_cell(s0, t[0], g.h) && within_half_cell(s1, t[1], g.h) && within_half_cell(s2, t[2], g.h)))
         code = kFailedPrecondition;
     }
-    const Coord cx = normalize(t[0], g.ox, g.h, g.inv_h);
-    const Coord cy = normalize(t[1], g.oy, g.h, g.inv_h);
-    const Coord cz = normalize(t[2], g.oz, g.h, g.inv_h);
-    Axis3 nx, ny, nz, tx, ty, tz;  // normal (shift 0) and transverse (shift 1/2) samples per coordinate
-    const int cnx = axis3(cx, 0.0, bin[0], g.h, nx), ctx_ = axis3(cx, 0.5, bin[0], g.h, tx);
-    const int cny = axis3(cy, 0.0, bin[1], g.h, ny), cty = axis3(cy, 0.5, bin[1], g.h, ty);
-    const int cnz = axis3(cz, 0.0, bin[2], g.h, nz), ctz = axis3(cz, 0.5, bin[2], g.h, tz);
-    const double m = p.m[k];
-    // face axis x
-    if (code == kOk) code = axis_status(cnx, cty, ctz, nx, ty, tz);
-    if (code == kOk && !window_ok(wblock, axis_blocks(nx.first, tile[0]), axis_blocks(ty.first, tile[1]),
-                                  axis_blocks(tz.first, tile[2])))
-      code = kFailedPrecondition;
-    if (code == kOk && !deposit_axis<0>(nx, ty, tz, tile, m, p.v[0][k], p.c[0][k], p.c[1][k], p.c[2][k], inv_qm,
-                                        inv_qp, limit, wm, wq))
-      code = kOutOfRange;
-    // face axis y
-    if (code == kOk) code = axis_status(ctx_, cny, ctz, tx, ny, tz);
-    if (code == kOk && !window_ok(wblock, axis_blocks(tx.first, tile[0]), axis_blocks(ny.first, tile[1]),
-                                  axis_blocks(tz.first, tile[2])))
-      code = kFailedPrecondition;
-    if (code == kOk && !deposit_axis<1>(tx, ny, tz, tile, m, p.v[1][k], p.c[3][k], p.c[4][k], p.c[5][k], inv_qm,
-                                        inv_qp, limit, wm, wq))
-      code = kOutOfRange;
-    // face axis z
-    if (code == kOk) code = axis_status(ctx_, cty, cnz, tx, ty, nz);
-    if (code == kOk && !window_ok(wblock, axis_blocks(tx.first, tile[0]), axis_blocks(ty.first, tile[1]),
-                                  axis_blocks(nz.first, tile[2])))
-      code = kFailedPrecondition;
-    if (code == kOk && !deposit_axis<2>(tx, ty, nz, tile, m, p.v[2][k], p.c[6][k], p.c[7][k], p.c[8][k], inv_qm,
-                                        inv_qp, limit, wm, wq))
-      code = kOutOfRange;
+    const auto cx = normalize<kPow2>(t[0], g.ox, g.h, g.inv_h);
+    const auto cy = normalize<kPow2>(t[1], g.oy, g.h, g.inv_h);
+    const auto cz = normalize<kPow2>(t[2], g.oz, g.h, g.inv_h);
+    if (__all(cx.interior() && cy.interior() && cz.interior()))
+      code = deposit_particle<true, kSmall>(cx, cy, cz, bin, tile, wblock, g.h, p, k, inv_qm, inv_qp, limit, wm, wq,
+                                            code);
+    else
+      code = deposit_particle<false, kSmall>(cx, cy, cz, bin, tile, wblock, g.h, p, k, inv_qm, inv_qp, limit, wm,
+                                             wq, code);
     if (code) report(err, perm[k], code);
   }
   __syncthreads();
@@ -263,6 +374,19 @@ __global__ void k_normalize_forces(const int64_t* qm, const int64_t* qp, const i
   if (u0) u0[f] = u;
   if (validity) validity[f] = v;
   if (ustar) ustar[f] = pred;
+}
+
+// ApplyForces (liquid_substep_3d_cpu.cc:156-178) on its own: u* = u_old + dt * g[axis(f)], where the
+// owned-slot axis is (f mod 192) / 64; err[0] holds the first face whose prediction is not finite.
+__global__ void k_apply_forces(const double* u_old, uint64_t nf, double gx, double gy, double gz, double dt,
+                               double* pred, uint64_t* err) {
+  const uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (f >= nf) return;
+  const int axis = static_cast<int>((f % 192) / 64);
+  const double ga = axis == 0 ? gx : (axis == 1 ? gy : gz);
+  const double u = u_old[f] + dt * ga;
+  if (!finite(u)) report(err, f, kOutOfRange);
+  pred[f] = u;
 }
 
 // Gathers one face axis from the LDS window: the separable contraction of the 3x3x3 velocities with
@@ -331,7 +455,47 @@ __device__ __forceinline__ bool gather_axis(const Axis3& X, const Axis3& Y, cons
          finite(gz);
 }
 
-template <bool kTrial>
+// One particle's three face axes in the reference's order (transfers_3d.cc:232-257).
+template <bool kInterior, class Coord>
+__device__ __forceinline__ int gather_particle(const Coord& cx, const Coord& cy, const Coord& cz, const int32_t* bin,
+                                               const int32_t* tile, const int32_t* wblock, double h, double inv_h,
+                                               const double* wn, const double* wo, double* out, uint64_t cap,
+                                               uint64_t k, int code) {
+  Axis3 ty, tz;
+  const int cty = axis3<kInterior>(cy, 0.5, bin[1], h, ty), ctz = axis3<kInterior>(cz, 0.5, bin[2], h, tz);
+  {
+    Axis3 nx;
+    const int cnx = axis3<kInterior>(cx, 0.0, bin[0], h, nx);
+    if (code == kOk) code = axis_status(cnx, cty, ctz, nx, ty, tz);
+    if (code == kOk && !window_ok(wblock, axis_blocks(nx.first, tile[0]), axis_blocks(ty.first, tile[1]),
+                                  axis_blocks(tz.first, tile[2])))
+      code = kFailedPrecondition;
+    if (code == kOk && !gather_axis<0>(nx, ty, tz, tile, inv_h, wn, wo, out, cap, k)) code = kOutOfRange;
+  }
+  Axis3 tx;
+  const int ctx_ = axis3<kInterior>(cx, 0.5, bin[0], h, tx);
+  {
+    Axis3 ny;
+    const int cny = axis3<kInterior>(cy, 0.0, bin[1], h, ny);
+    if (code == kOk) code = axis_status(ctx_, cny, ctz, tx, ny, tz);
+    if (code == kOk && !window_ok(wblock, axis_blocks(tx.first, tile[0]), axis_blocks(ny.first, tile[1]),
+                                  axis_blocks(tz.first, tile[2])))
+      code = kFailedPrecondition;
+    if (code == kOk && !gather_axis<1>(tx, ny, tz, tile, inv_h, wn, wo, out, cap, k)) code = kOutOfRange;
+  }
+  {
+    Axis3 nz;
+    const int cnz = axis3<kInterior>(cz, 0.0, bin[2], h, nz);
+    if (code == kOk) code = axis_status(ctx_, cty, cnz, tx, ty, nz);
+    if (code == kOk && !window_ok(wblock, axis_blocks(tx.first, tile[0]), axis_blocks(ty.first, tile[1]),
+                                  axis_blocks(nz.first, tile[2])))
+      code = kFailedPrecondition;
+    if (code == kOk && !gather_axis<2>(tx, ty, nz, tile, inv_h, wn, wo, out, cap, k)) code = kOutOfRange;
+  }
+  return code;
+}
+
+template <bool kTrial, bool kPow2>
 __global__ __launch_bounds__(kThreads) void k_g2p(ParticleView p, BinList bins, GridView grid, const double* uold,
                                                   const double* unew, const uint32_t* perm, double* out, uint64_t cap,
                                                   uint64_t* err) {
@@ -366,28 +530,13 @@ __global__ __launch_bounds__(kThreads) void k_g2p(ParticleView p, BinList bins, 
       if (!(within_half_cell(s0, t[0], g.h) && within_half_cell(s1, t[1], g.h) && within_half_cell(s2, t[2], g.h)))
         code = kFailedPrecondition;
     }
-    const Coord cx = normalize(t[0], g.ox, g.h, g.inv_h);
-    const Coord cy = normalize(t[1], g.oy, g.h, g.inv_h);
-    const Coord cz = normalize(t[2], g.oz, g.h, g.inv_h);
-    Axis3 nx, ny, nz, tx, ty, tz;
-    const int cnx = axis3(cx, 0.0, bin[0], g.h, nx), ctx_ = axis3(cx, 0.5, bin[0], g.h, tx);
-    const int cny = axis3(cy, 0.0, bin[1], g.h, ny), cty = axis3(cy, 0.5, bin[1], g.h, ty);
-    const int cnz = axis3(cz, 0.0, bin[2], g.h, nz), ctz = axis3(cz, 0.5, bin[2], g.h, tz);
-    if (code == kOk) code = axis_status(cnx, cty, ctz, nx, ty, tz);
-    if (code == kOk && !window_ok(wblock, axis_blocks(nx.first, tile[0]), axis_blocks(ty.first, tile[1]),
-                                  axis_blocks(tz.first, tile[2])))
-      code = kFailedPrecondition;
-    if (code == kOk && !gather_axis<0>(nx, ty, tz, tile, inv_h, wn, wo, out, cap, k)) code = kOutOfRange;
-    if (code == kOk) code = axis_status(ctx_, cny, ctz, tx, ny, tz);
-    if (code == kOk && !window_ok(wblock, axis_blocks(tx.first, tile[0]), axis_blocks(ny.first, tile[1]),
-                                  axis_blocks(tz.first, tile[2])))
-      code = kFailedPrecondition;
-    if (code == kOk && !gather_axis<1>(tx, ny, tz, tile, inv_h, wn, wo, out, cap, k)) code = kOutOfRange;
-    if (code == kOk) code = axis_status(ctx_, cty, cnz, tx, ty, nz);
-    if (code == kOk && !window_ok(wblock, axis_blocks(tx.first, tile[0]), axis_blocks(ty.first, tile[1]),
-                                  axis_blocks(nz.first, tile[2])))
-      code = kFailedPrecondition;
-    if (code == kOk && !gather_axis<2>(tx, ty, nz, tile, inv_h, wn, wo, out, cap, k)) code = kOutOfRange;
+    const auto cx = normalize<kPow2>(t[0], g.ox, g.h, g.inv_h);
+    const auto cy = normalize<kPow2>(t[1], g.oy, g.h, g.inv_h);
+    const auto cz = normalize<kPow2>(t[2], g.oz, g.h, g.inv_h);
+    if (__all(cx.interior() && cy.interior() && cz.interior()))
+      code = gather_particle<true>(cx, cy, cz, bin, tile, wblock, g.h, inv_h, wn, wo, out, cap, k, code);
+    else
+      code = gather_particle<false>(cx, cy, cz, bin, tile, wblock, g.h, inv_h, wn, wo, out, cap, k, code);
     if (code) report(err, perm[k], code);
   }
 }
@@ -417,7 +566,13 @@ int p2g_quanta(fl_ctx* ctx, uint64_t n, double inv_qm, double inv_qp, double lim
   if (before) FL_HIP(hipEventRecord(before, ctx->stream));
   if (after && (ctx->nbins == 0 || n == 0)) FL_HIP(hipEventRecord(after, ctx->stream));
   if (ctx->nbins == 0 || n == 0) return FL_OK;
-  auto* kernel = ctx->sorted.trial_aliases_sorted ? k_p2g<false> : k_p2g<true>;
+  // Instances: trial positions distinct from the sorted ones, power-of-two cell width, limit <= 2^50.
+  using K = decltype(&k_p2g<false, true, true>);
+  static const K table[8] = {k_p2g<false, false, false>, k_p2g<false, false, true>, k_p2g<false, true, false>,
+                             k_p2g<false, true, true>,   k_p2g<true, false, false>,  k_p2g<true, false, true>,
+                             k_p2g<true, true, false>,   k_p2g<true, true, true>};
+  const bool trial = !ctx->sorted.trial_aliases_sorted, pow2 = ctx->geometry.inv_h != 0.0, small = limit <= 0x1p50;
+  auto* kernel = table[(trial ? 4 : 0) + (pow2 ? 2 : 0) + (small ? 1 : 0)];
   kernel<<<ctx->nbins, kThreads, 0, ctx->stream>>>(ctx->sorted.view(), bin_list(ctx), grid_view(ctx), inv_qm, inv_qp,
                                                    limit, ctx->perm, reinterpret_cast<unsigned long long*>(ctx->qm),
                                                    reinterpret_cast<unsigned long long*>(ctx->qp), ctx->err);
@@ -431,6 +586,13 @@ int normalize_forces(fl_ctx* ctx, uint64_t nf, double qm, double qp, double min_
   k_normalize_forces<<<grid_for(nf), kThreads, 0, ctx->stream>>>(ctx->qm, ctx->qp, nullptr, nf, qm, qp, min_mass,
                                                                   g[0], g[1], g[2], dt, ctx->face_mass, ctx->u0,
                                                                   ctx->validity, ctx->ustar, ctx->err);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+int apply_forces(fl_ctx* ctx, const double* d_old, uint64_t nf, const double g[3], double dt, double* d_pred) {
+  if (!nf) return FL_OK;
+  k_apply_forces<<<grid_for(nf), kThreads, 0, ctx->stream>>>(d_old, nf, g[0], g[1], g[2], dt, d_pred, ctx->err);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -449,7 +611,9 @@ int g2p(fl_ctx* ctx, uint64_t n, const double* d_old, const double* d_new, hipEv
   if (before) FL_HIP(hipEventRecord(before, ctx->stream));
   if (after && (ctx->nbins == 0 || n == 0)) FL_HIP(hipEventRecord(after, ctx->stream));
   if (ctx->nbins == 0 || n == 0) return FL_OK;
-  auto* kernel = ctx->sorted.trial_aliases_sorted ? k_g2p<false> : k_g2p<true>;
+  const bool trial = !ctx->sorted.trial_aliases_sorted, pow2 = ctx->geometry.inv_h != 0.0;
+  auto* kernel = trial ? (pow2 ? k_g2p<true, true> : k_g2p<true, false>)
+                       : (pow2 ? k_g2p<false, true> : k_g2p<false, false>);
   kernel<<<ctx->nbins, kThreads, 0, ctx->stream>>>(ctx->sorted.view(), bin_list(ctx), grid_view(ctx), d_old, d_new,
                                                    ctx->perm, ctx->gather, ctx->sorted.capacity, ctx->err);
   FL_HIP(hipGetLastError());

@@ -50,7 +50,7 @@ class StepReport(C.Structure):
 
 
 EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_p2g_quanta",
-           "fl_normalize_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
+           "fl_normalize_forces", "fl_apply_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
            "fl_scene_download")
 
 _lib = None
@@ -139,6 +139,13 @@ class Context:
                                       C.c_double(min_mass), _vec3(accel), C.c_double(dt), _p(m), _p(u), _p(v), _p(p),
                                       C.byref(self.last))
         return s, m, u, v, p
+
+    def apply_forces(self, old, accel, dt):
+        old = np.ascontiguousarray(old, dtype=np.float64)
+        pred = np.zeros(len(old))
+        s = lib().fl_apply_forces(self.handle, _p(old), C.c_size_t(len(old)), _vec3(accel), C.c_double(dt), _p(pred),
+                                  C.byref(self.last))
+        return s, pred
 
     def g2p(self, geometry, blocks, particles, old, new, out=None):
         if out is None:

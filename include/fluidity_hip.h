@@ -168,6 +168,10 @@ int fl_normalize_forces(fl_ctx* ctx, const fl_face_quanta* quanta, size_t face_c
                         double minimum_face_mass, fl_vec3 acceleration, double dt, double* out_mass,
                         double* out_velocity, uint8_t* out_validity, double* out_predicted,
                         fl_measurement* measurement);
+/* ApplyForces alone (native/src/liquid_substep_3d_cpu.cc:156-178): predicted = old + dt * g[axis], the
+ * axis of owned slot f being (f mod 192) / 64. FL_OUT_OF_RANGE for a non-finite prediction. */
+int fl_apply_forces(fl_ctx* ctx, const double* old_face_velocities, size_t face_count, fl_vec3 acceleration,
+                    double dt, double* out_predicted, fl_measurement* measurement);
 /* GatherMac3D (native/include/fluidity/host/transfers_3d.h:55-61; native/src/transfers_3d.cc:199-261). */
 int fl_g2p(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
            const fl_particle* particles, size_t n, const double* old_face_velocities,

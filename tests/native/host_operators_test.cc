@@ -1,0 +1,393 @@
+// Tests of the host C++ operator layer (fluidity/host/*.h), shaped like the reference's operator
+// tests (native/tests/test_sparse_mac_grid_3d.cc, test_transfers_3d.cc, test_mac_pressure_3d.cc):
+// layout queries, status codes, publication on failure, and parity of every device operator with
+// the oracle (bit-exact quanta, sort order and forces; 1e-12 relative gathers; projection gates).
+//
+// Usage: host_operators_test cpu   (layout and status tests, no device)
+//        host_operators_test gpu   (device operators against the oracle)
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "fluidity/host/hip_liquid_substep_backend_3d.h"
+#include "fluidity/host/mac_pressure_3d.h"
+#include "fluidity/host/sparse_mac_grid_3d.h"
+#include "fluidity/host/transfers_3d.h"
+#include "oracle.h"
+
+using namespace fluidity;
+
+namespace {
+
+int g_failures = 0;
+#define EXPECT(cond)                                                                 \
+  do {                                                                               \
+    if (!(cond)) {                                                                   \
+      std::printf("  expectation failed at line %d: %s\n", __LINE__, #cond);         \
+      ++g_failures;                                                                  \
+      return;                                                                        \
+    }                                                                                \
+  } while (0)
+
+const UniformMacGeometry3D kGeometry{{0.0, 0.0, 0.0}, 1.0 / 32.0};
+
+std::vector<BlockIndex3D> DenseBlocks(int nx, int ny, int nz) {
+  std::vector<BlockIndex3D> b;
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) b.push_back({x, y, z});
+  return b;
+}
+
+const flo_int3* O(std::span<const BlockIndex3D> b) { return reinterpret_cast<const flo_int3*>(b.data()); }
+const flo_geometry* O(const UniformMacGeometry3D* g) { return reinterpret_cast<const flo_geometry*>(g); }
+
+// Particles of a liquid box of cells [lo, hi): 8 per cell, one jittered point per octant, random velocity
+// and affine slopes, trial = sorted position.
+std::vector<MacTransferParticle3D> BoxParticles(int lo, int hi, std::uint64_t seed, double speed) {
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> u(0.0, 0.5);
+  std::normal_distribution<double> n(0.0, 1.0);
+  const double h = kGeometry.cell_width;
+  std::vector<MacTransferParticle3D> p;
+  for (int z = lo; z < hi; ++z)
+    for (int y = lo; y < hi; ++y)
+      for (int x = lo; x < hi; ++x)
+        for (int o = 0; o < 8; ++o) {
+          MacTransferParticle3D q{};
+          q.sorted_world_position = {(x + 0.5 * (o & 1) + u(rng)) * h, (y + 0.5 * ((o >> 1) & 1) + u(rng)) * h,
+                                     (z + 0.5 * (o >> 2) + u(rng)) * h};
+          q.trial_world_position = q.sorted_world_position;
+          q.particle_mass = 1000.0 * h * h * h / 8.0;
+          q.particle_velocity = {speed * n(rng), speed * n(rng), speed * n(rng)};
+          q.particle_velocity_slopes = {{0.1 * n(rng), 0.1 * n(rng), 0.1 * n(rng)},
+                                        {0.1 * n(rng), 0.1 * n(rng), 0.1 * n(rng)},
+                                        {0.1 * n(rng), 0.1 * n(rng), 0.1 * n(rng)}};
+          p.push_back(q);
+        }
+  return p;
+}
+
+// ---------------------------------------------------------------- layout (no device)
+
+void GridCreateSortsAndRejects() {
+  std::vector<BlockIndex3D> keys = {{1, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 0, 0}};
+  auto grid = SparseMacGrid3D::Create(kGeometry, keys, 8);
+  EXPECT(grid.ok());
+  const auto sorted = grid->active_block_indices();
+  EXPECT(sorted.size() == 4 && sorted[0].x == 0 && sorted[1].x == 1 && sorted[2].y == 1 && sorted[3].z == 1);
+  EXPECT(grid->face_count() == 4 * 192);
+  keys.push_back({1, 0, 0});
+  EXPECT(SparseMacGrid3D::Create(kGeometry, keys, 8).status().code() == StatusCode::kInvalidArgument);
+  EXPECT(SparseMacGrid3D::Create(kGeometry, keys, 3).status().code() == StatusCode::kResourceExhausted);
+  const std::vector<BlockIndex3D> far = {{0x7fffffff / 4 + 1, 0, 0}};
+  EXPECT(SparseMacGrid3D::Create(kGeometry, far, 8).status().code() == StatusCode::kOutOfRange);
+  for (double h : {0.0, -1.0, 1e-320, std::nan("")})
+    EXPECT(SparseMacGrid3D::Create({{0, 0, 0}, h}, {}, 8).status().code() == StatusCode::kInvalidArgument);
+  EXPECT(SparseMacGrid3D::Create({{std::nan(""), 0, 0}, 1.0}, {}, 8).status().code() == StatusCode::kInvalidArgument);
+  const std::vector<BlockIndex3D> unsorted = {{1, 0, 0}, {0, 0, 0}};
+  EXPECT(SparseMacGridView3D::Create(kGeometry, unsorted).status().code() == StatusCode::kInvalidArgument);
+}
+
+void FaceSlotsMatchOracle() {
+  const std::vector<BlockIndex3D> keys = {{-1, 0, 0}, {0, 0, 0}, {0, -2, 1}, {3, 1, 1}};
+  auto grid = SparseMacGrid3D::Create(kGeometry, keys, 8);
+  EXPECT(grid.ok());
+  const auto blocks = grid->active_block_indices();
+  for (std::size_t slot = 0; slot < grid->face_count(); ++slot) {
+    auto face = grid->FaceIndex(slot);
+    EXPECT(face.ok());
+    flo_face_index of{};
+    EXPECT(flo_slot_face(O(blocks), blocks.size(), slot, &of) == 0);
+    EXPECT(of.axis == static_cast<std::uint8_t>(face->axis) && of.lattice.x == face->lattice_index.x &&
+           of.lattice.y == face->lattice_index.y && of.lattice.z == face->lattice_index.z);
+    auto back = grid->FaceSlot(*face);
+    EXPECT(back.ok() && *back == slot);
+    auto anchor = grid->FaceWorldAnchor(*face);
+    flo_vec3 oa{};
+    EXPECT(anchor.ok() && flo_face_world_anchor(O(&kGeometry), of, &oa) == 0);
+    EXPECT(anchor->x == oa.x && anchor->y == oa.y && anchor->z == oa.z);
+  }
+  EXPECT(grid->FaceIndex(grid->face_count()).status().code() == StatusCode::kOutOfRange);
+  EXPECT(grid->FaceSlot({FaceNormalAxis::kX, {100, 0, 0}}).status().code() == StatusCode::kFailedPrecondition);
+  EXPECT(grid->FaceSlot({static_cast<FaceNormalAxis>(3), {0, 0, 0}}).status().code() ==
+         StatusCode::kInvalidArgument);
+  const BlockIndex3D o = MacOwnerBlock3D({-1, -4, -5});
+  EXPECT(o.x == -1 && o.y == -1 && o.z == -2);
+  auto center = grid->CellWorldCenter({1, 2, 3});
+  EXPECT(center.ok() && center->x == 1.5 / 32 && center->y == 2.5 / 32 && center->z == 3.5 / 32);
+}
+
+void OperatorsWithoutDeviceFailPrecondition() {
+  auto rt = HipOperatorRuntime3D::Create({}, 0);
+  EXPECT(rt.status().code() == StatusCode::kFailedPrecondition);
+  auto grid = SparseMacGrid3D::Create(kGeometry, DenseBlocks(2, 2, 2), 8);
+  EXPECT(grid.ok());
+  const auto particles = BoxParticles(1, 3, 1, 1.0);
+  std::vector<MacFaceDepositQuanta3D> out(grid->face_count(), MacFaceDepositQuanta3D{7, 7, 7});
+  const Status s = DepositMacQuanta3D(*grid, particles, {0x1p-40, 0x1p-40, particles.size()}, out);
+  EXPECT(s.code() == StatusCode::kFailedPrecondition);
+  EXPECT(out[0].mass_quanta_sum == 7 && out.back().stress_force_quanta_sum == 7);
+}
+
+// ---------------------------------------------------------------- device operators
+
+void DepositMatchesOracleBitExact() {
+  auto grid = SparseMacGrid3D::Create(kGeometry, DenseBlocks(4, 4, 4), 64);
+  EXPECT(grid.ok());
+  const auto particles = BoxParticles(1, 14, 11, 2.0);
+  const MacDepositQuantization3D q{0x1p-40, 0x1p-40, particles.size()};
+  std::vector<MacFaceDepositQuanta3D> got(grid->face_count()), want(grid->face_count());
+  EXPECT(DepositMacQuanta3D(*grid, particles, q, got).ok());
+  const auto blocks = grid->active_block_indices();
+  EXPECT(flo_deposit_quanta(O(&kGeometry), O(blocks), blocks.size(),
+                            reinterpret_cast<const flo_particle*>(particles.data()), particles.size(),
+                            {q.mass_per_quantum, q.pure_momentum_per_quantum, q.declared_particle_capacity},
+                            std::uint64_t{1} << 40, reinterpret_cast<flo_face_quanta*>(want.data()), want.size(),
+                            4) == 0);
+  EXPECT(std::memcmp(got.data(), want.data(), got.size() * sizeof(got[0])) == 0);
+  // Small declared capacity: the general rounding path (limit above 2^50).
+  const std::vector<MacTransferParticle3D> few(particles.begin(), particles.begin() + 64);
+  const MacDepositQuantization3D qs{0x1p-40, 0x1p-40, 64};
+  EXPECT(DepositMacQuanta3D(*grid, few, qs, got).ok());
+  EXPECT(flo_deposit_quanta(O(&kGeometry), O(blocks), blocks.size(), reinterpret_cast<const flo_particle*>(few.data()),
+                            few.size(), {qs.mass_per_quantum, qs.pure_momentum_per_quantum, 64}, std::uint64_t{1} << 40,
+                            reinterpret_cast<flo_face_quanta*>(want.data()), want.size(), 1) == 0);
+  EXPECT(std::memcmp(got.data(), want.data(), got.size() * sizeof(got[0])) == 0);
+}
+
+void DepositFailuresPreserveOutput() {
+  auto grid = SparseMacGrid3D::Create(kGeometry, DenseBlocks(4, 4, 4), 64);
+  EXPECT(grid.ok());
+  auto particles = BoxParticles(2, 6, 12, 1.0);
+  std::vector<MacFaceDepositQuanta3D> out(grid->face_count(), MacFaceDepositQuanta3D{5, 6, 7});
+  const MacDepositQuantization3D q{0x1p-40, 0x1p-40, particles.size()};
+  auto moved = particles;
+  moved[17].trial_world_position.x += 0.75 * kGeometry.cell_width;  // more than half a cell
+  EXPECT(DepositMacQuanta3D(*grid, moved, q, out).code() == StatusCode::kFailedPrecondition);
+  EXPECT(out[0].mass_quanta_sum == 5 && out[100].pure_momentum_quanta_sum == 6);
+  auto heavy = particles;
+  heavy[3].particle_mass = 1e30;  // scaled mass beyond 2^(62 - bit_width(cap - 1))
+  EXPECT(DepositMacQuanta3D(*grid, heavy, q, out).code() == StatusCode::kOutOfRange);
+  EXPECT(out[0].mass_quanta_sum == 5);
+  auto negative = particles;
+  negative[0].particle_mass = -1.0;
+  EXPECT(DepositMacQuanta3D(*grid, negative, q, out).code() == StatusCode::kInvalidArgument);
+  EXPECT(DepositMacQuanta3D(*grid, particles, {0x1p-40, 0x1p-40, particles.size() - 1}, out).code() ==
+         StatusCode::kResourceExhausted);
+  EXPECT(DepositMacQuanta3D(*grid, particles, {3e-12, 0x1p-40, particles.size()}, out).code() ==
+         StatusCode::kInvalidArgument);
+  EXPECT(out[0].mass_quanta_sum == 5 && out.back().stress_force_quanta_sum == 7);
+}
+
+void GatherMatchesOracle() {
+  auto grid = SparseMacGrid3D::Create(kGeometry, DenseBlocks(4, 4, 4), 64);
+  EXPECT(grid.ok());
+  const auto particles = BoxParticles(1, 14, 13, 1.0);
+  std::mt19937_64 rng(5);
+  std::normal_distribution<double> n(0.0, 1.0);
+  std::vector<double> u_old(grid->face_count()), u_new(grid->face_count());
+  for (auto& v : u_old) v = n(rng);
+  for (auto& v : u_new) v = n(rng);
+  std::vector<MacParticleGatherResult3D> got(particles.size()), want(particles.size());
+  EXPECT(GatherMac3D(*grid, particles, u_old, u_new, got).ok());
+  const auto blocks = grid->active_block_indices();
+  EXPECT(flo_gather(O(&kGeometry), O(blocks), blocks.size(), reinterpret_cast<const flo_particle*>(particles.data()),
+                    particles.size(), u_old.data(), u_new.data(), u_old.size(),
+                    reinterpret_cast<flo_gather_result*>(want.data()), 4) == 0);
+  const double* a = reinterpret_cast<const double*>(got.data());
+  const double* b = reinterpret_cast<const double*>(want.data());
+  double scale = 0;
+  for (std::size_t i = 0; i < 24 * got.size(); ++i) scale = std::max(scale, std::fabs(b[i]));
+  for (std::size_t i = 0; i < 24 * got.size(); ++i) EXPECT(std::fabs(a[i] - b[i]) <= 1e-12 * std::fabs(b[i]) + 1e-12 * scale);
+  std::vector<double> bad = u_new;
+  bad[3] = std::nan("");
+  std::vector<MacParticleGatherResult3D> keep(particles.size());
+  keep[0].new_pic_velocity.x = 42.0;
+  EXPECT(GatherMac3D(*grid, particles, u_old, bad, keep).code() == StatusCode::kInvalidArgument);
+  EXPECT(keep[0].new_pic_velocity.x == 42.0);
+}
+
+void BackendStagesMatchOracle() {
+  auto rt = HipOperatorRuntime3D::Create({1 << 16, 1 << 10, 1 << 16, 1 << 12, 1 << 14}, 0);
+  EXPECT(rt.ok());
+  const HipLiquidSubstepBackend3D backend = MakeHipLiquidSubstepBackend3D(**rt);
+  auto grid = SparseMacGrid3D::Create(kGeometry, DenseBlocks(4, 4, 4), 64);
+  EXPECT(grid.ok());
+  const auto particles = BoxParticles(1, 12, 17, 1.0);
+  // topology
+  std::vector<Vec3> positions;
+  for (const auto& p : particles) positions.push_back(p.sorted_world_position);
+  std::vector<LiquidParticleSortEntry3D> order(positions.size()), want_order(positions.size());
+  EXPECT(backend.topology(backend.context, kGeometry, positions, order).ok());
+  EXPECT(flo_sort_particles(O(&kGeometry), reinterpret_cast<const flo_vec3*>(positions.data()), positions.size(),
+                            reinterpret_cast<flo_sort_entry*>(want_order.data())) == 0);
+  for (std::size_t i = 0; i < order.size(); ++i)
+    EXPECT(order[i].original_ordinal == want_order[i].original_ordinal &&
+           order[i].bin_index.x == want_order[i].bin_index.x && order[i].bin_index.z == want_order[i].bin_index.z);
+  // deposit -> normalize -> forces
+  const MacDepositQuantization3D q{0x1p-40, 0x1p-40, particles.size()};
+  std::vector<MacFaceDepositQuanta3D> quanta(grid->face_count());
+  EXPECT(backend.deposit(backend.context, *grid, particles, q, quanta).ok());
+  const std::size_t nf = grid->face_count();
+  std::vector<double> mass(nf), u0(nf), want_mass(nf), want_u0(nf), pred(nf), want_pred(nf);
+  std::vector<LiquidFaceValidity3D> validity(nf);
+  std::vector<std::uint8_t> want_validity(nf);
+  EXPECT(backend.normalize(backend.context, quanta, q, 1e-12, mass, u0, validity).ok());
+  EXPECT(flo_normalize(reinterpret_cast<const flo_face_quanta*>(quanta.data()), nf,
+                       {q.mass_per_quantum, q.pure_momentum_per_quantum, q.declared_particle_capacity}, 1e-12,
+                       want_mass.data(), want_u0.data(), want_validity.data()) == 0);
+  EXPECT(std::memcmp(mass.data(), want_mass.data(), nf * 8) == 0 && std::memcmp(u0.data(), want_u0.data(), nf * 8) == 0);
+  EXPECT(std::memcmp(validity.data(), want_validity.data(), nf) == 0);
+  const Vec3 g{0.0, -9.81, 0.0};
+  EXPECT(backend.forces(backend.context, *grid, u0, g, 1e-3, pred).ok());
+  const auto blocks = grid->active_block_indices();
+  EXPECT(flo_apply_forces(O(blocks), blocks.size(), u0.data(), nf, {g.x, g.y, g.z}, 1e-3, want_pred.data()) == 0);
+  EXPECT(std::memcmp(pred.data(), want_pred.data(), nf * 8) == 0);
+  std::vector<double> huge(nf, 1e308);
+  EXPECT(backend.forces(backend.context, *grid, huge, {0, 0, 1e308}, 10.0, pred).code() == StatusCode::kOutOfRange);
+  EXPECT(std::memcmp(pred.data(), want_pred.data(), nf * 8) == 0);
+  // gather through the table
+  std::vector<MacParticleGatherResult3D> got(particles.size());
+  EXPECT(backend.gather(backend.context, *grid, particles, u0, pred, got).ok());
+}
+
+// Liquid box [2, 10)^3 in a sealed 16^3 tank: liquid-liquid faces internal, liquid-air Dirichlet p = 0 at
+// distance h, tank walls sealed. Faces are the union of incident faces in owned-slot order.
+void ProjectionMeetsGatesAndMatchesOracle() {
+  const int n = 16, lo = 2, hi = 10;
+  const double h = 1.0 / n, dt = 1e-3, rho = 1000.0;
+  const UniformMacGeometry3D geom{{0, 0, 0}, h};
+  auto grid = SparseMacGrid3D::Create(geom, DenseBlocks(5, 5, 5), 256);
+  EXPECT(grid.ok());
+  auto liquid = [&](int x, int y, int z) { return x >= lo && x < hi && y >= lo && y < hi && z >= lo && z < hi; };
+  std::vector<MacPressureCell3D> cells;
+  for (int z = lo; z < hi; ++z)
+    for (int y = lo; y < hi; ++y)
+      for (int x = lo; x < hi; ++x) cells.push_back({{x, y, z}, h * h * h, 0.0, INFINITY, 0.0, 0.0});
+  std::mt19937_64 rng(3);
+  std::normal_distribution<double> nd(0.0, 0.5);
+  std::vector<std::pair<std::size_t, MacPressureFace3D>> faces;
+  for (const auto& c : cells)
+    for (int a = 0; a < 3; ++a)
+      for (int side = 0; side < 2; ++side) {
+        FaceLatticeIndex3D l{c.cell_index.x, c.cell_index.y, c.cell_index.z};
+        (a == 0 ? l.x : (a == 1 ? l.y : l.z)) += side;
+        const FaceIndex3D f{static_cast<FaceNormalAxis>(a), l};
+        auto slot = grid->FaceSlot(f);
+        EXPECT(slot.ok());
+        bool seen = false;
+        for (const auto& e : faces) seen = seen || e.first == *slot;
+        if (seen) continue;
+        const int t[3] = {l.x - (a == 0), l.y - (a == 1), l.z - (a == 2)};
+        const bool tail = liquid(t[0], t[1], t[2]), head = liquid(l.x, l.y, l.z);
+        const int li = a == 0 ? l.x : (a == 1 ? l.y : l.z);
+        MacPressureFace3D face{f, h * h, 1.0 / rho, h, nd(rng) - 9.81 * dt * (a == 1), 0.0, SealedMacPressureLink3D{}};
+        if (li == 0 || li == n) {
+          face.open_area = 0.0;
+        } else if (tail && head) {
+          face.pressure_link = InternalMacPressureLink3D{0.0};
+        } else {
+          face.pressure_link = DirichletMacPressureLink3D{0.0};
+        }
+        faces.push_back({*slot, face});
+      }
+  std::sort(faces.begin(), faces.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+  std::vector<MacPressureFace3D> face_list;
+  for (const auto& e : faces) face_list.push_back(e.second);
+  const MacPressureInputs3D inputs{*grid, dt, cells, face_list};
+  auto got = ProjectMacPressure3D(inputs);
+  EXPECT(got.ok());
+  const auto& d = got->diagnostics;
+  double b2 = 0;
+  for (double b : got->integrated_rhs) b2 += b * b;
+  EXPECT(d.true_residual_l2 <= 2e-6 + 2e-6 * std::sqrt(b2));
+  EXPECT(d.max_volume_scaled_residual <= 2e-5 && d.max_cell_constraint_defect <= 2e-5);
+  // Oracle projection (FP64 V-cycle) of the same system: pressures agree to the solve tolerance.
+  std::vector<flo_pressure_cell> oc(cells.size());
+  std::vector<flo_pressure_face> of(face_list.size());
+  for (std::size_t i = 0; i < cells.size(); ++i)
+    oc[i] = {{cells[i].cell_index.x, cells[i].cell_index.y, cells[i].cell_index.z}, 0, cells[i].fluid_volume,
+             0.0, cells[i].bulk_modulus, 0.0, 0.0};
+  for (std::size_t i = 0; i < face_list.size(); ++i) {
+    const auto& f = face_list[i];
+    const int kind = std::holds_alternative<InternalMacPressureLink3D>(f.pressure_link)
+                         ? 1
+                         : (std::holds_alternative<DirichletMacPressureLink3D>(f.pressure_link) ? 2 : 0);
+    of[i] = {static_cast<std::int32_t>(f.face_index.axis),
+             {f.face_index.lattice_index.x, f.face_index.lattice_index.y, f.face_index.lattice_index.z},
+             f.open_area, f.inverse_density, f.pressure_gradient_distance, f.predicted_normal_velocity,
+             f.wall_normal_velocity, kind, 0, 0.0, 0.0};
+  }
+  const flo_pressure_options opt{256, 1.0, 1e-12, 4, 2, 16, 0, 0.5};
+  std::vector<double> want(cells.size());
+  flo_pressure_diagnostics od{};
+  const auto blocks = grid->active_block_indices();
+  EXPECT(flo_project(O(&geom), O(blocks), blocks.size(), dt, oc.data(), oc.size(), of.data(), of.size(), &opt,
+                     want.data(), nullptr, nullptr, nullptr, nullptr, nullptr, &od) == 0);
+  double pmax = 0, err = 0;
+  for (std::size_t i = 0; i < want.size(); ++i) {
+    pmax = std::max(pmax, std::fabs(want[i]));
+    err = std::max(err, std::fabs(got->cell_pressures[i] - want[i]));
+  }
+  std::printf("  projection: %zu cells, %zu iterations, max |dp| / max |p| = %.3g\n", cells.size(),
+              d.iteration_count, err / pmax);
+  EXPECT(err <= 1e-5 * pmax);
+  // Invalid data: a nonpositive step duration is an argument error and publishes nothing.
+  EXPECT(ProjectMacPressure3D({*grid, -1.0, cells, face_list}).status().code() == StatusCode::kInvalidArgument);
+}
+
+void SharedRuntimeGrows() {
+  auto small = SparseMacGrid3D::Create(kGeometry, DenseBlocks(2, 2, 2), 8);
+  auto large = SparseMacGrid3D::Create(kGeometry, DenseBlocks(6, 6, 6), 512);
+  EXPECT(small.ok() && large.ok());
+  for (auto* grid : {&*small, &*large}) {
+    const int cells = static_cast<int>(grid->active_block_indices().size() == 8 ? 7 : 23);
+    const auto particles = BoxParticles(1, cells, 21, 1.0);
+    const MacDepositQuantization3D q{0x1p-40, 0x1p-40, particles.size()};
+    std::vector<MacFaceDepositQuanta3D> got(grid->face_count()), want(grid->face_count());
+    EXPECT(DepositMacQuanta3D(*grid, particles, q, got).ok());
+    const auto blocks = grid->active_block_indices();
+    EXPECT(flo_deposit_quanta(O(&kGeometry), O(blocks), blocks.size(),
+                              reinterpret_cast<const flo_particle*>(particles.data()), particles.size(),
+                              {q.mass_per_quantum, q.pure_momentum_per_quantum, q.declared_particle_capacity},
+                              std::uint64_t{1} << 40, reinterpret_cast<flo_face_quanta*>(want.data()), want.size(),
+                              4) == 0);
+    EXPECT(std::memcmp(got.data(), want.data(), got.size() * sizeof(got[0])) == 0);
+  }
+}
+
+int Run(const char* name, const std::function<void()>& test) {
+  std::printf("[ RUN      ] %s\n", name);
+  std::fflush(stdout);
+  const int before = g_failures;
+  test();
+  std::printf(g_failures == before ? "[       OK ] %s\n" : "[  FAILED  ] %s\n", name);
+  std::fflush(stdout);
+  return g_failures == before ? 0 : 1;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const std::string mode = argc > 1 ? argv[1] : "cpu";
+  int failed = 0;
+  if (mode == "cpu") {
+    failed += Run("GridCreateSortsAndRejects", GridCreateSortsAndRejects);
+    failed += Run("FaceSlotsMatchOracle", FaceSlotsMatchOracle);
+    failed += Run("OperatorsWithoutDeviceFailPrecondition", OperatorsWithoutDeviceFailPrecondition);
+  } else {
+    failed += Run("DepositMatchesOracleBitExact", DepositMatchesOracleBitExact);
+    failed += Run("DepositFailuresPreserveOutput", DepositFailuresPreserveOutput);
+    failed += Run("GatherMatchesOracle", GatherMatchesOracle);
+    failed += Run("BackendStagesMatchOracle", BackendStagesMatchOracle);
+    failed += Run("ProjectionMeetsGatesAndMatchesOracle", ProjectionMeetsGatesAndMatchesOracle);
+    failed += Run("SharedRuntimeGrows", SharedRuntimeGrows);
+  }
+  std::printf("%d test(s) failed\n", failed);
+  return failed ? 1 : 0;
+}
