@@ -48,6 +48,18 @@ def g2p_algorithmic_bytes(n_particles, n_faces):
     return 216 * n_particles + 16 * n_faces
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE passes scaled by a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"traffic_bytes_per_launch {kernel} {workload}:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def copy_ceiling(nbytes=1 << 30, reps=5):
     """Device-to-device copy of `nbytes` (a plain copy kernel): measured bandwidth in GB/s.
 
@@ -220,7 +232,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic("k_p2g", scene.name),
             "algorithmic_bytes": p2g_bytes,
             "g2p_achieved": g2p_algorithmic_bytes(n, nf) / (g2p_ms * 1e-3) / 1e9,
             "copy_ceiling_gbs": copy_ceiling(),

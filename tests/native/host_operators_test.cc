@@ -346,7 +346,7 @@ void SharedRuntimeGrows() {
   auto large = SparseMacGrid3D::Create(kGeometry, DenseBlocks(6, 6, 6), 512);
   EXPECT(small.ok() && large.ok());
   for (auto* grid : {&*small, &*large}) {
-    const int cells = static_cast<int>(grid->active_block_indices().size() == 8 ? 7 : 23);
+    const int cells = static_cast<int>(grid->active_block_indices().size() == 8 ? 6 : 22);
     const auto particles = BoxParticles(1, cells, 21, 1.0);
     const MacDepositQuantization3D q{0x1p-40, 0x1p-40, particles.size()};
     std::vector<MacFaceDepositQuanta3D> got(grid->face_count()), want(grid->face_count());
