@@ -60,7 +60,7 @@ struct Level {
   int32_t* parent = nullptr;  // [nb][2] coarser block and octant (all but the coarsest level)
   float* x = nullptr;
   float* b = nullptr;
-  float* r = nullptr;
+  float* r = nullptr;       // [nb*64] residual (red half used)
   float* k = nullptr;       // [3][nb*64] low-face conductances along x, y, z
   float* anchor = nullptr;  // [nb*64]
   float* diag = nullptr;    // [nb*64], 0 marks a non-unknown cell
@@ -149,8 +149,8 @@ void free_levels(Pressure& P) {
     dfree(L.child);
     dfree(L.parent);
     dfree(L.x);
-    dfree(L.b);
     dfree(L.r);
+    dfree(L.b);
     dfree(L.k);
     dfree(L.anchor);
     dfree(L.diag);
@@ -184,7 +184,19 @@ __device__ inline void perr_report(uint64_t* err, uint64_t ordinal, int code) {
   atomicMin(reinterpret_cast<unsigned long long*>(err), static_cast<unsigned long long>(error_word(ordinal, code)));
 }
 
-// Neighbour cell index within a level (block-local cells, (z,y,x) local order).
+// Cells of a 4^3 block are stored colour-major: the 32 cells with x+y+z even first, then the 32 odd
+// ones, each half in (z,y,x) order. l = z*16 + y*4 + x is the natural local index. A red-black half
+// sweep then reads and writes one contiguous half of every block.
+__host__ __device__ inline int cslot(int l) {
+  return ((((l & 3) + ((l >> 2) & 3) + (l >> 4)) & 1) << 5) | (l >> 1);
+}
+__host__ __device__ inline int cnat(int s) {
+  const int l0 = (s & 31) << 1;
+  const int parity = ((l0 & 3) + ((l0 >> 2) & 3) + (l0 >> 4)) & 1;
+  return l0 + (parity ^ (s >> 5));
+}
+
+// Neighbour cell index within a level.
 __device__ inline int64_t neighbour(const int32_t* nbr, uint64_t blk, int lx, int ly, int lz, int dir) {
   int nx = lx, ny = ly, nz = lz;
   int64_t b = static_cast<int64_t>(blk);
@@ -203,7 +215,7 @@ __device__ inline int64_t neighbour(const int32_t* nbr, uint64_t blk, int lx, in
     ny &= 3;
     nz &= 3;
   }
-  return b * 64 + (nz * 16 + ny * 4 + nx);
+  return b * 64 + cslot(nz * 16 + ny * 4 + nx);
 }
 
 // Conductance of face `dir` of cell i: low faces are stored on the cell, high faces on the neighbour.
@@ -458,7 +470,7 @@ __global__ void k_coarsen(uint64_t nc_cells, const int32_t* child, uint64_t nf_c
   const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (i >= nc_cells) return;
   const uint64_t blk = i >> 6;
-  const int l = static_cast<int>(i & 63);
+  const int l = cnat(static_cast<int>(i & 63));
   const int cx = l & 3, cy = (l >> 2) & 3, cz = l >> 4;
   double a = 0.0, kx = 0.0, ky = 0.0, kz = 0.0;
   bool any = false;
@@ -468,7 +480,7 @@ __global__ void k_coarsen(uint64_t nc_cells, const int32_t* child, uint64_t nf_c
         const int fx = 2 * cx + ox, fy = 2 * cy + oy, fz = 2 * cz + oz;
         const int32_t fb = child[8 * blk + ((fz >> 2) * 2 + (fy >> 2)) * 2 + (fx >> 2)];
         if (fb < 0) continue;
-        const uint64_t fi = uint64_t(fb) * 64 + ((fz & 3) * 16 + (fy & 3) * 4 + (fx & 3));
+        const uint64_t fi = uint64_t(fb) * 64 + cslot((fz & 3) * 16 + (fy & 3) * 4 + (fx & 3));
         if (fdiag[fi] == 0.f) continue;
         any = true;
         a += static_cast<double>(scale) * fanchor[fi];
@@ -488,7 +500,7 @@ __global__ void k_diag(uint64_t n, const int32_t* nbr, const float* k, const flo
   if (i >= n) return;
   if (diag[i] == 0.f) return;
   const uint64_t blk = i >> 6;
-  const int l = static_cast<int>(i & 63);
+  const int l = cnat(static_cast<int>(i & 63));
   const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
   double d = static_cast<double>(anchor[i]);
   for (int dir = 0; dir < 6; ++dir) {
@@ -501,55 +513,184 @@ __global__ void k_diag(uint64_t n, const int32_t* nbr, const float* k, const flo
 
 // ------------------------------------------------------------------ V-cycle kernels (FP32)
 
-// One red-black Gauss-Seidel half sweep. `first` means x == 0 on entry: the colour is set to b/d
-// and the other colour to zero, which saves a fill pass.
-__global__ void k_smooth(uint64_t n, const int32_t* nbr, const float* k, const float* diag, const float* b, float* x,
-                         int color, int first) {
-  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  if (i >= n) return;
-  const int l = static_cast<int>(i & 63);
-  const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
-  const float d = diag[i];
-  if (((lx + ly + lz) & 1) != color) {
-    if (first) x[i] = 0.f;
-    return;
-  }
+// Per-cell V-cycle operations, shared by the grid-wide kernels and the single-workgroup tail cycle.
+struct LevelView {
+  uint64_t n;             // cells (64 per block)
+  const int32_t* nbr;     // [nb][6]
+  const int32_t* child;   // [nb][8] finer blocks (levels > 0)
+  const int32_t* parent;  // [nb][2] coarser block and octant
+  float* x;
+  const float* b;
+  const float* k;
+  const float* anchor;
+  const float* diag;
+};
+
+// Gauss-Seidel update of the cell at colour-half position t (block t/32, slot colour*32 + t%32).
+// `first` means x == 0 on entry, so the update is b/d. Cells that are not unknowns keep x = 0.
+__device__ inline void smooth_cell(const LevelView& L, uint64_t t, int color, int first) {
+  const uint64_t blk = t >> 5;
+  const int sl = (color << 5) | static_cast<int>(t & 31);
+  const uint64_t i = blk * 64 + static_cast<uint64_t>(sl);
+  const float d = L.diag[i];
   if (d == 0.f) {
-    if (first) x[i] = 0.f;
+    L.x[i] = 0.f;
     return;
   }
-  float s = b[i];
+  float s = L.b[i];
   if (!first) {
-    const uint64_t blk = i >> 6;
+    const int l = cnat(sl);
+    const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
     for (int dir = 0; dir < 6; ++dir) {
-      const int64_t j = neighbour(nbr, blk, lx, ly, lz, dir);
+      const int64_t j = neighbour(L.nbr, blk, lx, ly, lz, dir);
       if (j < 0) continue;
-      s += face_k(k, n, static_cast<int64_t>(i), j, dir) * x[j];
+      s += face_k(L.k, L.n, static_cast<int64_t>(i), j, dir) * L.x[j];
     }
   }
-  x[i] = s / d;
+  L.x[i] = s / d;
 }
 
-// r = b - A x in the edge-difference form (mac_pressure_workspace_3d.cc:228-246).
-__global__ void k_residual(uint64_t n, const int32_t* nbr, const float* k, const float* anchor, const float* diag,
-                           const float* b, const float* x, float* r) {
-  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  if (i >= n) return;
-  if (diag[i] == 0.f) {
+// Coarse right-hand side of coarse cell ci: the sum of the fine residuals b - A x of its 2x2x2
+// children. Pre-smoothing ends with a black half sweep, which leaves every black (odd) cell's
+// equation satisfied, so only the four red children carry a residual.
+__device__ inline float restrict_residual(const LevelView& F, const int32_t* child, uint64_t ci) {
+  const uint64_t blk = ci >> 6;
+  const int l = cnat(static_cast<int>(ci & 63));
+  const int cx = l & 3, cy = (l >> 2) & 3, cz = l >> 4;
+  float sum = 0.f;
+  for (int o = 0; o < 8; ++o) {
+    const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+    if (((ox + oy + oz) & 1) != 0) continue;  // fine parity = own parity (2c is even): odd octants are black
+    const int fx = 2 * cx + ox, fy = 2 * cy + oy, fz = 2 * cz + oz;
+    const int32_t fb = child[8 * blk + ((fz >> 2) * 2 + (fy >> 2)) * 2 + (fx >> 2)];
+    if (fb < 0) continue;
+    const int lx = fx & 3, ly = fy & 3, lz = fz & 3;
+    const uint64_t fi = uint64_t(fb) * 64 + cslot(lz * 16 + ly * 4 + lx);
+    if (F.diag[fi] == 0.f) continue;
+    const float xi = F.x[fi];
+    float a = F.anchor[fi] * xi;
+    for (int dir = 0; dir < 6; ++dir) {
+      const int64_t j = neighbour(F.nbr, uint64_t(fb), lx, ly, lz, dir);
+      if (j < 0) continue;
+      a += face_k(F.k, F.n, static_cast<int64_t>(fi), j, dir) * (xi - F.x[j]);
+    }
+    sum += F.b[fi] - a;
+  }
+  return sum;
+}
+
+// x_fine += x_coarse(parent).
+__device__ inline void prolong_cell(const LevelView& F, const float* cx, uint64_t i) {
+  if (F.diag[i] == 0.f) return;
+  const uint64_t blk = i >> 6;
+  const int32_t pb = F.parent[2 * blk], oct = F.parent[2 * blk + 1];
+  const int l = cnat(static_cast<int>(i & 63));
+  const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
+  const int px = ((oct & 1) * 4 + lx) >> 1, py = (((oct >> 1) & 1) * 4 + ly) >> 1, pz = ((oct >> 2) * 4 + lz) >> 1;
+  F.x[i] += cx[uint64_t(pb) * 64 + cslot(pz * 16 + py * 4 + px)];
+}
+
+// One red-black Gauss-Seidel half sweep over the `color` half of every block.
+__global__ void k_smooth(LevelView L, int color, int first) {
+  const uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (t < L.n / 2) smooth_cell(L, t, color, first);
+}
+
+// Residual b - A x of the red (even) cells after pre-smoothing, one lane per red cell, into r's red
+// half; the black cells' residual is zero after the closing black half sweep.
+__global__ void k_residual_red(LevelView L, float* r) {
+  const uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (t >= L.n / 2) return;
+  const uint64_t blk = t >> 5;
+  const int sl = static_cast<int>(t & 31);
+  const uint64_t i = blk * 64 + static_cast<uint64_t>(sl);
+  if (L.diag[i] == 0.f) {
     r[i] = 0.f;
     return;
   }
-  const uint64_t blk = i >> 6;
-  const int l = static_cast<int>(i & 63);
+  const int l = cnat(sl);
   const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
-  const float xi = x[i];
-  float s = anchor[i] * xi;
+  const float xi = L.x[i];
+  float a = L.anchor[i] * xi;
   for (int dir = 0; dir < 6; ++dir) {
-    const int64_t j = neighbour(nbr, blk, lx, ly, lz, dir);
+    const int64_t j = neighbour(L.nbr, blk, lx, ly, lz, dir);
     if (j < 0) continue;
-    s += face_k(k, n, static_cast<int64_t>(i), j, dir) * (xi - x[j]);
+    a += face_k(L.k, L.n, static_cast<int64_t>(i), j, dir) * (xi - L.x[j]);
   }
-  r[i] = b[i] - s;
+  r[i] = L.b[i] - a;
+}
+
+// Coarse right-hand side: the sum of the red children's residuals (pull, fixed order).
+__global__ void k_restrict_red(const float* r, LevelView C) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= C.n) return;
+  const uint64_t blk = i >> 6;
+  const int l = cnat(static_cast<int>(i & 63));
+  const int cx = l & 3, cy = (l >> 2) & 3, cz = l >> 4;
+  float sum = 0.f;
+  for (int o = 0; o < 8; ++o) {
+    const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+    if (((ox + oy + oz) & 1) != 0) continue;
+    const int fx = 2 * cx + ox, fy = 2 * cy + oy, fz = 2 * cz + oz;
+    const int32_t fb = C.child[8 * blk + ((fz >> 2) * 2 + (fy >> 2)) * 2 + (fx >> 2)];
+    if (fb < 0) continue;
+    sum += r[uint64_t(fb) * 64 + cslot((fz & 3) * 16 + (fy & 3) * 4 + (fx & 3))];
+  }
+  const_cast<float*>(C.b)[i] = sum;
+}
+
+__global__ void k_prolong(LevelView F, const float* cx) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i < F.n) prolong_cell(F, cx, i);
+}
+
+constexpr int kMaxTail = 16;
+constexpr uint64_t kTailBlocks = 32;  // levels with at most this many blocks run in the tail cycle
+struct TailCycle {
+  int count, nu, coarse_sweeps;
+  LevelView lv[kMaxTail];
+};
+
+// The V-cycle below and including the first small level, in one workgroup: pre-smoothing and
+// residual restriction down to the coarsest level, symmetric sweeps there from zero, then
+// prolongation and post-smoothing back up. Barriers separate the dependent steps; every wave of the
+// workgroup shares the CU's vector cache, so each step sees the previous one's stores.
+__global__ __launch_bounds__(1024) void k_tail_vcycle(TailCycle T) {
+  const uint64_t tid = threadIdx.x, nt = blockDim.x;
+  auto sync = [] {
+    __threadfence_block();
+    __syncthreads();
+  };
+  for (int q = 0; q + 1 < T.count; ++q) {
+    const LevelView& L = T.lv[q];
+    for (int it = 0; it < T.nu; ++it) {
+      for (uint64_t t = tid; t < L.n / 2; t += nt) smooth_cell(L, t, 0, it == 0);
+      sync();
+      for (uint64_t t = tid; t < L.n / 2; t += nt) smooth_cell(L, t, 1, 0);
+      sync();
+    }
+    const LevelView& C = T.lv[q + 1];
+    for (uint64_t i = tid; i < C.n; i += nt) const_cast<float*>(C.b)[i] = restrict_residual(L, C.child, i);
+    sync();
+  }
+  const LevelView& B = T.lv[T.count - 1];
+  for (int pass = 0; pass < 4 * T.coarse_sweeps; ++pass) {
+    const int fwd = pass < 2 * T.coarse_sweeps;
+    const int color = fwd ? (pass & 1) : 1 - (pass & 1);
+    for (uint64_t t = tid; t < B.n / 2; t += nt) smooth_cell(B, t, color, pass == 0);
+    sync();
+  }
+  for (int q = T.count - 2; q >= 0; --q) {
+    const LevelView& L = T.lv[q];
+    for (uint64_t i = tid; i < L.n; i += nt) prolong_cell(L, T.lv[q + 1].x, i);
+    sync();
+    for (int it = 0; it < T.nu; ++it) {
+      for (uint64_t t = tid; t < L.n / 2; t += nt) smooth_cell(L, t, 1, 0);
+      sync();
+      for (uint64_t t = tid; t < L.n / 2; t += nt) smooth_cell(L, t, 0, 0);
+      sync();
+    }
+  }
 }
 
 // y = A x in the edge-difference form.
@@ -562,7 +703,7 @@ __global__ void k_apply(uint64_t n, const int32_t* nbr, const float* k, const fl
     return;
   }
   const uint64_t blk = i >> 6;
-  const int l = static_cast<int>(i & 63);
+  const int l = cnat(static_cast<int>(i & 63));
   const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
   const float xi = x[i];
   float s = anchor[i] * xi;
@@ -572,63 +713,6 @@ __global__ void k_apply(uint64_t n, const int32_t* nbr, const float* k, const fl
     s += face_k(k, n, static_cast<int64_t>(i), j, dir) * (xi - x[j]);
   }
   y[i] = s;
-}
-
-__global__ void k_restrict(uint64_t nc_cells, const int32_t* child, const float* fr, float* cb) {
-  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  if (i >= nc_cells) return;
-  const uint64_t blk = i >> 6;
-  const int l = static_cast<int>(i & 63);
-  const int cx = l & 3, cy = (l >> 2) & 3, cz = l >> 4;
-  float s = 0.f;
-  for (int oz = 0; oz < 2; ++oz)
-    for (int oy = 0; oy < 2; ++oy)
-      for (int ox = 0; ox < 2; ++ox) {
-        const int fx = 2 * cx + ox, fy = 2 * cy + oy, fz = 2 * cz + oz;
-        const int32_t fb = child[8 * blk + ((fz >> 2) * 2 + (fy >> 2)) * 2 + (fx >> 2)];
-        if (fb < 0) continue;
-        s += fr[uint64_t(fb) * 64 + ((fz & 3) * 16 + (fy & 3) * 4 + (fx & 3))];
-      }
-  cb[i] = s;
-}
-
-// x_fine += x_coarse(parent); parent block = child-table inverse passed as (parent block, octant).
-__global__ void k_prolong(uint64_t nf_cells, const int32_t* parent, const float* diag, const float* cx, float* fx) {
-  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  if (i >= nf_cells || diag[i] == 0.f) return;
-  const uint64_t blk = i >> 6;
-  const int32_t pb = parent[2 * blk], oct = parent[2 * blk + 1];
-  const int l = static_cast<int>(i & 63);
-  const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
-  const int px = ((oct & 1) * 4 + lx) >> 1, py = (((oct >> 1) & 1) * 4 + ly) >> 1, pz = ((oct >> 2) * 4 + lz) >> 1;
-  fx[i] += cx[uint64_t(pb) * 64 + (pz * 16 + py * 4 + px)];
-}
-
-// Coarsest level: `sweeps` forward (red, black) then `sweeps` backward (black, red) sweeps from zero,
-// all inside one workgroup.
-__global__ __launch_bounds__(1024) void k_coarsest(uint64_t n, const int32_t* nbr, const float* k, const float* diag, const float* b, float* x,
-                           int sweeps) {
-  for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) x[i] = 0.f;
-  __syncthreads();
-  for (int pass = 0; pass < 4 * sweeps; ++pass) {
-    const int fwd = pass < 2 * sweeps;
-    const int color = fwd ? (pass & 1) : 1 - (pass & 1);
-    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
-      const int l = static_cast<int>(i & 63);
-      const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
-      if (((lx + ly + lz) & 1) != color || diag[i] == 0.f) continue;
-      const uint64_t blk = i >> 6;
-      float s = b[i];
-      for (int dir = 0; dir < 6; ++dir) {
-        const int64_t j = neighbour(nbr, blk, lx, ly, lz, dir);
-        if (j < 0) continue;
-        s += face_k(k, n, static_cast<int64_t>(i), j, dir) * x[j];
-      }
-      x[i] = s / diag[i];
-    }
-    __threadfence_block();
-    __syncthreads();
-  }
 }
 
 // ------------------------------------------------------------------ PCG kernels
@@ -909,8 +993,8 @@ int build_levels(fl_ctx* ctx, Pressure& P, const std::vector<Key3>& blocks0, int
       FL_HIP(hipMemcpy(L.parent, parents[l].data(), parents[l].size() * sizeof(int32_t), hipMemcpyHostToDevice));
     }
     FL_TRY(dal(&L.x, n));
-    FL_TRY(dal(&L.b, n));
     FL_TRY(dal(&L.r, n));
+    FL_TRY(dal(&L.b, n));
     FL_TRY(dal(&L.k, 3 * n));
     FL_TRY(dal(&L.anchor, n));
     FL_TRY(dal(&L.diag, n));
@@ -938,40 +1022,51 @@ int setup_coefficients(fl_ctx* ctx, Pressure& P) {
   return FL_OK;
 }
 
+LevelView view(Pressure& P, size_t l) {
+  Level& L = P.levels[l];
+  return LevelView{L.nb * 64, L.nbr, L.child, L.parent, L.x, l == 0 ? P.r32 : L.b, L.k, L.anchor, L.diag};
+}
+
+// V-cycle from level l. Level 0 smooths the recursive FP32 residual of the outer PCG in place of its
+// own right-hand side. Levels of at most kTailBlocks blocks (and everything below them) run in one
+// single-workgroup launch.
 int vcycle(fl_ctx* ctx, Pressure& P, size_t l, int nu, int coarse_sweeps) {
   hipStream_t s = ctx->stream;
-  Level& L = P.levels[l];
-  // Level 0 smooths the recursive FP32 residual of the outer PCG in place of its own right-hand side.
-  const float* b = l == 0 ? P.r32 : L.b;
-  const uint64_t n = L.nb * 64;
-  const unsigned g = grid_for(n);
-  if (l + 1 == P.levels.size()) {
-    if (L.nb <= 64) {
-      k_coarsest<<<1, 1024, 0, s>>>(n, L.nbr, L.k, L.diag, b, L.x, coarse_sweeps);
-    } else {
-      for (int pass = 0; pass < 4 * coarse_sweeps; ++pass) {
-        const int fwd = pass < 2 * coarse_sweeps;
-        const int color = fwd ? (pass & 1) : 1 - (pass & 1);
-        k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, b, L.x, color, pass == 0);
-      }
+  const size_t last = P.levels.size() - 1;
+  if (l == last && P.levels[l].nb > kTailBlocks) {  // a large coarsest level: grid-wide sweeps from zero
+    const LevelView B = view(P, l);
+    for (int pass = 0; pass < 4 * coarse_sweeps; ++pass) {
+      const int fwd = pass < 2 * coarse_sweeps;
+      k_smooth<<<grid_for(B.n / 2), kThreads, 0, s>>>(B, fwd ? (pass & 1) : 1 - (pass & 1), pass == 0);
     }
     FL_HIP(hipGetLastError());
     return FL_OK;
   }
-  for (int it = 0; it < nu; ++it) {
-    k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, b, L.x, 0, it == 0);
-    k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, b, L.x, 1, 0);
+  if (P.levels[l].nb <= kTailBlocks) {
+    TailCycle T{};
+    T.count = static_cast<int>(last - l + 1);
+    if (T.count > kMaxTail) return FL_INTERNAL;
+    T.nu = nu;
+    T.coarse_sweeps = coarse_sweeps;
+    for (size_t q = l; q <= last; ++q) T.lv[q - l] = view(P, q);
+    k_tail_vcycle<<<1, 1024, 0, s>>>(T);
+    FL_HIP(hipGetLastError());
+    return FL_OK;
   }
-  k_residual<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.anchor, L.diag, b, L.x, L.r);
-  Level& C = P.levels[l + 1];
-  const uint64_t nc = C.nb * 64;
-  k_restrict<<<grid_for(nc), kThreads, 0, s>>>(nc, C.child, L.r, C.b);
+  const LevelView L = view(P, l), C = view(P, l + 1);
+  const unsigned gh = grid_for(L.n / 2);
+  for (int it = 0; it < nu; ++it) {
+    k_smooth<<<gh, kThreads, 0, s>>>(L, 0, it == 0);
+    k_smooth<<<gh, kThreads, 0, s>>>(L, 1, 0);
+  }
+  k_residual_red<<<gh, kThreads, 0, s>>>(L, P.levels[l].r);
+  k_restrict_red<<<grid_for(C.n), kThreads, 0, s>>>(P.levels[l].r, C);
   FL_HIP(hipGetLastError());
   FL_TRY(vcycle(ctx, P, l + 1, nu, coarse_sweeps));
-  k_prolong<<<g, kThreads, 0, s>>>(n, L.parent, L.diag, C.x, L.x);
+  k_prolong<<<grid_for(L.n), kThreads, 0, s>>>(L, C.x);
   for (int it = 0; it < nu; ++it) {
-    k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, b, L.x, 1, 0);
-    k_smooth<<<g, kThreads, 0, s>>>(n, L.nbr, L.k, L.diag, b, L.x, 0, 0);
+    k_smooth<<<gh, kThreads, 0, s>>>(L, 1, 0);
+    k_smooth<<<gh, kThreads, 0, s>>>(L, 0, 0);
   }
   FL_HIP(hipGetLastError());
   return FL_OK;
@@ -1360,7 +1455,7 @@ int prepare_system(fl_ctx* ctx, Pressure& P, const fl_geometry* g, const fl_int3
   for (size_t c = 0; c < nc; ++c) {
     const fl_int3 ci = cells[c].cell_index;
     const Key3 b{int32_t(host_floor_div4(ci.x)), int32_t(host_floor_div4(ci.y)), int32_t(host_floor_div4(ci.z))};
-    const int64_t fi = int64_t(bidx[b]) * 64 + ((ci.z - 4 * b.z) * 16 + (ci.y - 4 * b.y) * 4 + (ci.x - 4 * b.x));
+    const int64_t fi = int64_t(bidx[b]) * 64 + cslot((ci.z - 4 * b.z) * 16 + (ci.y - 4 * b.y) * 4 + (ci.x - 4 * b.x));
     cell_fine[c] = int32_t(fi);
     fine_cell[fi] = int32_t(c);
     const int64_t xyz[3] = {ci.x, ci.y, ci.z};
