@@ -137,6 +137,53 @@ def cpu_baseline(scene, ctx, report, sample, threads=1):
     }
 
 
+def learned_comparison(ctx, scene, dt, opts, args):
+    """Projection with the learned preconditioner (flexible CG, V-cycle fallback) against plain MGPCG on
+    the same step: training pairs come from the MGPCG solve of one step, the network is trained on the
+    device, then one step is timed with each preconditioner."""
+    # 8^3 tiles of the level-0 pressure blocks (blocks holding a pressure cell).
+    cells = scene.pressure_cells()
+    c = np.stack([cells["cell_index"]["x"], cells["cell_index"]["y"], cells["cell_index"]["z"]], 1)
+    tiles = len(np.unique(np.unique(np.floor_divide(c, 4), axis=0) // 2, axis=0))
+    net = H.Network(ctx, tiles=max(tiles, 1024), pairs=args.learn_pairs)
+    rng = np.random.default_rng(0)
+    init = []
+    for cin, cout in [(4, 8), (8, 8), (8, 16), (16, 16), (16, 32), (32, 32), (48, 16), (16, 16), (24, 8), (8, 1)]:
+        init += [rng.normal(0.0, np.sqrt(2.0 / (27 * cin)), cout * cin * 27), np.zeros(cout)]
+    assert net.set_params(np.concatenate(init).astype(np.float32)) == H.OK
+    assert net.collect(1, clear=True) == H.OK
+    s, rep = ctx.step(dt, scene.gravity, QUANTUM, QUANTUM, scene.count, opts)
+    assert s == H.OK, s
+    assert net.collect(0) == H.OK
+    pairs = net.pairs()
+    t0 = time.perf_counter()
+    s, losses = net.train(epochs=args.learn_epochs, batch=256, lr=2e-3, seed=1)
+    train_s = time.perf_counter() - t0
+    assert s == H.OK, s
+    learned_opts = opts.copy()
+    learned_opts["preconditioner"] = 1
+    s, lrep = ctx.step(dt, scene.gravity, QUANTUM, QUANTUM, scene.count, learned_opts)
+    assert s == H.OK, s
+    s, mrep = ctx.step(dt, scene.gravity, QUANTUM, QUANTUM, scene.count, opts)
+    assert s == H.OK, s
+    ld, md = lrep.diagnostics(), mrep.diagnostics()
+    log(f"learned: {pairs} pairs, {args.learn_epochs} epochs in {train_s:.1f}s, loss {losses[0]:.3g} -> {losses[-1]:.3g}; "
+        f"projection {lrep.projection_ms:.2f} ms vs MGPCG {mrep.projection_ms:.2f} ms")
+    return {
+        "projection_ms": lrep.projection_ms,
+        "mgpcg_projection_ms": mrep.projection_ms,
+        "iterations": int(ld["iteration_count"]),
+        "learned_iterations": int(ld["learned_iterations"]),
+        "mgpcg_iterations": int(md["iteration_count"]),
+        "relative_divergence": float(ld["defect_l2"] / ld["initial_defect_l2"]),
+        "training_pairs": int(pairs),
+        "training_epochs": args.learn_epochs,
+        "training_s": train_s,
+        "loss_first_last": [float(losses[0]), float(losses[-1])],
+        "network": "3-level 3D U-Net 8/16/32 on 8^3 tiles, FP32",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,9 +193,14 @@ def main():
                     choices=["two_spheres_256", "dam_break_128", "dam_break_128_random", "dam_break_64"])
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--coarse-sweeps", type=int, default=2)
+    ap.add_argument("--level-storage", choices=["fp32", "bf16"], default="bf16",
+                    help="V-cycle coefficient storage (BF16 keeps FP32 accumulation)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="only measure the plain-copy ceiling")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner comparison")
+    ap.add_argument("--learn-pairs", type=int, default=50000)
+    ap.add_argument("--learn-epochs", type=int, default=8)
     args = ap.parse_args()
     if args.copy_probe:
         print(json.dumps({"copy_ceiling_gbs": copy_ceiling(), "bytes_per_copy": 1 << 30}), flush=True)
@@ -169,7 +221,8 @@ def main():
                             scene.mass) == H.OK
     s = ctx.scene_prepare(cells, scene.n, scene.rho)
     assert s == H.OK, s
-    opts = H.pressure_options(relative_target=1e-6, levels=args.levels, coarse_sweeps=args.coarse_sweeps)
+    opts = H.pressure_options(relative_target=1e-6, levels=args.levels, coarse_sweeps=args.coarse_sweeps,
+                              bf16_levels=args.level_storage == "bf16")
     dt = scene.dt
     for _ in range(args.warmup):
         s, rep = ctx.step(dt, scene.gravity, QUANTUM, QUANTUM, scene.count, opts)
@@ -203,7 +256,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 transfers, f32 V-cycle with f64 dots/residuals",
+        "dtype": f"f64 transfers; {args.level_storage} V-cycle coefficients, f32 CG vectors, f64 dots/residuals",
         "data": "synthetic (seeded scene generator)",
         "config": {
             "workload": scene.name,
@@ -214,6 +267,7 @@ def main():
             "nonempty_bins": int(last.nonempty_bins),
             "grid": f"{scene.n}^3 sparse, 4^3 blocks",
             "multigrid_levels": int(d["multigrid_level_count"]),
+            "level_storage": args.level_storage,
             "relative_divergence_target": 1e-6,
             "device": device,
         },
@@ -238,6 +292,8 @@ def main():
             "copy_ceiling_gbs": copy_ceiling(),
         },
     }
+    if not args.no_learned:
+        out["learned"] = learned_comparison(ctx, scene, dt, opts, args)
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(scene, ctx, last, args.cpu_sample)

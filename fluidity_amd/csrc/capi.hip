@@ -6,6 +6,8 @@
 #include <cstring>
 #include <new>
 
+#include <limits>
+
 #include "context.h"
 
 namespace fl {
@@ -17,6 +19,7 @@ int project_pressure(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* bl
 int normalize_forces_stress(fl_ctx* ctx, const int64_t* qs, uint64_t nf, double qm, double qp, double min_mass,
                             const double g[3], double dt);
 void destroy_pressure(fl_ctx* ctx);
+void destroy_learn(fl_ctx* ctx);
 }  // namespace fl
 
 namespace {
@@ -244,6 +247,7 @@ void fl_destroy(fl_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   fl::destroy_pressure(ctx);
+  fl::destroy_learn(ctx);
   release(ctx);
   for (hipEvent_t e : ctx->stage)
     if (e) (void)hipEventDestroy(e);
@@ -407,6 +411,44 @@ int fl_p2g_quanta(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* block
   const double kernel_ms = t.stop() - copy_ms;
   Timer c(ctx);
   if (nf) FL_HIP(hipMemcpyAsync(out, ctx->quanta_aos, nf * sizeof(fl_face_quanta), hipMemcpyDeviceToHost, ctx->stream));
+  copy_ms += c.stop();
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, 10);
+  return FL_OK;
+}
+
+__global__ void k_pack_deposit(const double* m, const double* q, uint64_t n, fl_face_deposit* out) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i < n) out[i] = fl_face_deposit{m[i], q[i]};
+}
+
+int fl_p2g(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
+           const fl_particle* particles, size_t n, fl_face_deposit* out, size_t face_count,
+           fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  FL_TRY(check_geometry(geometry));
+  FL_TRY(check_blocks(blocks, block_count));
+  if (face_count != 192 * block_count) return FL_INVALID_ARGUMENT;
+  if (overlaps(out, face_count * sizeof(fl_face_deposit), particles, n * sizeof(fl_particle)))
+    return FL_INVALID_ARGUMENT;
+  if (n > ctx->cap.particles || block_count > ctx->cap.blocks) return FL_RESOURCE_EXHAUSTED;
+  (void)hipSetDevice(ctx->device);
+  set_geometry(ctx, geometry);
+  double copy_ms = 0;
+  FL_TRY(build_lookup(ctx, reinterpret_cast<const int32_t*>(blocks), block_count));
+  Timer t(ctx);
+  FL_TRY(stage_particles(ctx, particles, n, true, &copy_ms));
+  FL_TRY(p2g_quanta(ctx, ctx->nvalid, 1.0, 1.0, std::numeric_limits<double>::max(), nullptr, nullptr, true));
+  FL_TRY(first_error_code(ctx));
+  const uint64_t nf = face_count;
+  fl_face_deposit* staged = reinterpret_cast<fl_face_deposit*>(ctx->quanta_aos);
+  if (nf) {
+    k_pack_deposit<<<grid_for(nf), kThreads, 0, ctx->stream>>>(reinterpret_cast<const double*>(ctx->qm),
+                                                               reinterpret_cast<const double*>(ctx->qp), nf, staged);
+    FL_HIP(hipGetLastError());
+  }
+  const double kernel_ms = t.stop() - copy_ms;
+  Timer c(ctx);
+  if (nf) FL_HIP(hipMemcpyAsync(out, staged, nf * sizeof(fl_face_deposit), hipMemcpyDeviceToHost, ctx->stream));
   copy_ms += c.stop();
   fill_measurement(ctx, measurement, kernel_ms, copy_ms, 10);
   return FL_OK;

@@ -152,6 +152,8 @@ struct fl_ctx {
 
   // pressure solver state (opaque, owned by pressure.hip)
   void* pressure = nullptr;
+  // learned preconditioner state (fl::LearnState, owned by learn.hip)
+  void* learn = nullptr;
 };
 
 namespace fl {
@@ -179,7 +181,7 @@ BinList bin_list(fl_ctx* ctx);
 
 // Transfers (transfer.hip)
 int p2g_quanta(fl_ctx* ctx, uint64_t n, double inv_qm, double inv_qp, double limit, hipEvent_t before = nullptr,
-               hipEvent_t after = nullptr);
+               hipEvent_t after = nullptr, bool f64 = false);
 int normalize_forces(fl_ctx* ctx, uint64_t nf, double qm, double qp, double min_mass, const double g[3], double dt);
 int apply_forces(fl_ctx* ctx, const double* d_old, uint64_t nf, const double g[3], double dt, double* d_pred);
 int g2p(fl_ctx* ctx, uint64_t n, const double* d_old, const double* d_new, hipEvent_t before = nullptr,

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "context.h"
+#include "learn.h"
 
 namespace fl {
 namespace {
@@ -61,6 +62,8 @@ struct Level {
   float* x = nullptr;
   float* b = nullptr;
   float* r = nullptr;       // [nb*64] residual (red half used)
+  uint16_t* kb = nullptr;   // [3][nb*64] BF16 copies of k for the V-cycle (BF16 level storage)
+  float* diagb = nullptr;   // [nb*64] diagonal of the BF16-coefficient operator
   float* k = nullptr;       // [3][nb*64] low-face conductances along x, y, z
   float* anchor = nullptr;  // [nb*64]
   float* diag = nullptr;    // [nb*64], 0 marks a non-unknown cell
@@ -109,6 +112,10 @@ struct Pressure {
   double* def = nullptr;
   double* zero = nullptr;
   float* fine_vol = nullptr;  // [nb0*64] fluid volume (0 if none)
+  // 8^3 tiles of level 0 for the learned preconditioner: 2x2x2 blocks each, and a class per cell
+  int32_t* tile_blocks = nullptr;  // [ntiles][8] level-0 block or -1, octant (z,y,x) order
+  uint8_t* cell_class = nullptr;   // [nb0*64] 1 liquid unknown, 2 air, 3 solid
+  uint64_t ntiles = 0;
   // PCG (level-0 sized)
   float* r32 = nullptr;
   float* x32 = nullptr;
@@ -150,6 +157,8 @@ void free_levels(Pressure& P) {
     dfree(L.parent);
     dfree(L.x);
     dfree(L.r);
+    dfree(L.kb);
+    dfree(L.diagb);
     dfree(L.b);
     dfree(L.k);
     dfree(L.anchor);
@@ -224,6 +233,26 @@ __device__ inline float face_k(const float* k, uint64_t n, int64_t i, int64_t j,
   return (dir & 1) ? k[axis * n + j] : k[axis * n + i];
 }
 
+// Coefficients are FP32 or BF16 (the high half of an FP32 pattern); all arithmetic is FP32.
+struct Bf16 {
+  uint16_t bits;
+};
+__device__ inline float kload(const float* k, int64_t i) { return k[i]; }
+__device__ inline float kload(const Bf16* k, int64_t i) { return __uint_as_float(uint32_t(k[i].bits) << 16); }
+__device__ __host__ inline uint16_t to_bf16(float v) {
+  uint32_t u;
+  memcpy(&u, &v, 4);
+  return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);  // round to nearest even (finite v)
+}
+
+
+// Conductance of face `dir` of cell i (low faces on the cell, high faces on the neighbour).
+template <class KT>
+__device__ inline float face_kt(const KT* k, uint64_t n, int64_t i, int64_t j, int dir) {
+  const int axis = dir >> 1;
+  return (dir & 1) ? kload(k, axis * int64_t(n) + j) : kload(k, axis * int64_t(n) + i);
+}
+
 // ------------------------------------------------------------------ system kernels (FP64)
 
 
@@ -268,13 +297,12 @@ __global__ void k_freeze_cells(uint64_t nc, double dt, const double* V, const do
 
 // Integrated RHS (mac_pressure_3d_arithmetic.h:263-313) with Neumaier sums per cell.
 // Cells are visited in block order (fine index) so that their faces are near each other in memory.
-__global__ void k_rhs(uint64_t nfine, const int32_t* fine_cell, const int32_t* cell_faces, const FaceRec* faces,
-                      const double* target, const double* inwall, const double* comp, const double* prev, double* rhs,
-                      double* rhs_abs) {
+// One lane per cell, in input (z,y,x) cell order so that the cell arrays are read contiguously.
+__global__ void k_rhs(uint64_t nc, const int32_t* cell_faces, const FaceRec* faces, const double* target,
+                      const double* inwall, const double* comp, const double* prev, double* rhs, double* rhs_abs) {
   const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  if (i >= nfine) return;
-  const int32_t c = fine_cell[i];
-  if (c < 0) return;
+  if (i >= nc) return;
+  const int32_t c = static_cast<int32_t>(i);
   Row row;
   row.add(target[c]);
   row.add(inwall[c]);
@@ -345,14 +373,13 @@ __device__ inline void block_reduce(double v[4], double* partial, unsigned slot)
 // Evaluate (mac_pressure_workspace_3d.cc:399-461): FP64 residual of the original operator, physical
 // defect from corrected fluxes, and the identity |V defect + r| <= 64 eps (sum of absolute terms).
 // Cells are visited in block order.
-__global__ void k_evaluate(uint64_t nfine, const int32_t* fine_cell, const int32_t* cell_faces, const FaceRec* faces,
+__global__ void k_evaluate(uint64_t nc, const int32_t* cell_faces, const FaceRec* faces,
                            const double* V, const double* target, const double* inwall, const double* comp,
                            const double* prev, const double* rhs, const double* rhs_abs, const double* p, double* res,
                            double* def, double* partial, uint64_t* err) {
   double v[4] = {0.0, 0.0, 0.0, 0.0};
-  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < nfine; i += uint64_t(gridDim.x) * blockDim.x) {
-    const int32_t c = fine_cell[i];
-    if (c < 0) continue;
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < nc; i += uint64_t(gridDim.x) * blockDim.x) {
+    const int32_t c = static_cast<int32_t>(i);
     Row op, phys;
     const double pc = p[c];
     op.add(comp[c] * pc);
@@ -495,6 +522,31 @@ __global__ void k_coarsen(uint64_t nc_cells, const int32_t* child, uint64_t nf_c
   cdiag_mark[i] = any ? 1.f : 0.f;
 }
 
+// BF16 copies of a level's conductances and the diagonal of the operator they define.
+__global__ void k_to_bf16(const float* k, uint64_t n, uint16_t* kb) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i < n) kb[i] = to_bf16(k[i]);
+}
+__global__ void k_diag_bf16(uint64_t n, const int32_t* nbr, const uint16_t* kb, const float* anchor, const float* diag,
+                            float* diagb) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  if (diag[i] == 0.f) {
+    diagb[i] = 0.f;
+    return;
+  }
+  const uint64_t blk = i >> 6;
+  const int l = cnat(static_cast<int>(i & 63));
+  const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
+  float d = anchor[i];
+  for (int dir = 0; dir < 6; ++dir) {
+    const int64_t j = neighbour(nbr, blk, lx, ly, lz, dir);
+    if (j < 0) continue;
+    d += face_kt(reinterpret_cast<const Bf16*>(kb), n, static_cast<int64_t>(i), j, dir);
+  }
+  diagb[i] = d;
+}
+
 __global__ void k_diag(uint64_t n, const int32_t* nbr, const float* k, const float* anchor, float* diag) {
   const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (i >= n) return;
@@ -514,46 +566,56 @@ __global__ void k_diag(uint64_t n, const int32_t* nbr, const float* k, const flo
 // ------------------------------------------------------------------ V-cycle kernels (FP32)
 
 // Per-cell V-cycle operations, shared by the grid-wide kernels and the single-workgroup tail cycle.
-struct LevelView {
+template <class KT>
+struct LevelViewT {
   uint64_t n;             // cells (64 per block)
   const int32_t* nbr;     // [nb][6]
   const int32_t* child;   // [nb][8] finer blocks (levels > 0)
   const int32_t* parent;  // [nb][2] coarser block and octant
   float* x;
   const float* b;
-  const float* k;
+  const KT* k;
   const float* anchor;
   const float* diag;
 };
+using LevelView = LevelViewT<float>;
+
 
 // Gauss-Seidel update of the cell at colour-half position t (block t/32, slot colour*32 + t%32).
 // `first` means x == 0 on entry, so the update is b/d. Cells that are not unknowns keep x = 0.
-__device__ inline void smooth_cell(const LevelView& L, uint64_t t, int color, int first) {
+template <class KT>
+__device__ inline void smooth_cell(const LevelViewT<KT>& L, uint64_t t, int color, int first) {
   const uint64_t blk = t >> 5;
   const int sl = (color << 5) | static_cast<int>(t & 31);
   const uint64_t i = blk * 64 + static_cast<uint64_t>(sl);
+  // Every load is issued before any of them is used (no early exit on non-unknown cells), so the
+  // independent reads overlap; missing neighbours read slot i with a zero coefficient.
   const float d = L.diag[i];
-  if (d == 0.f) {
-    L.x[i] = 0.f;
-    return;
-  }
   float s = L.b[i];
   if (!first) {
     const int l = cnat(sl);
     const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
+    float kk[6], xx[6];
+#pragma unroll
     for (int dir = 0; dir < 6; ++dir) {
       const int64_t j = neighbour(L.nbr, blk, lx, ly, lz, dir);
-      if (j < 0) continue;
-      s += face_k(L.k, L.n, static_cast<int64_t>(i), j, dir) * L.x[j];
+      const int64_t jj = j < 0 ? static_cast<int64_t>(i) : j;
+      const float kv = face_kt(L.k, L.n, static_cast<int64_t>(i), jj, dir);
+      const float xv = L.x[jj];
+      kk[dir] = j < 0 ? 0.f : kv;
+      xx[dir] = j < 0 ? 0.f : xv;
     }
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) s += kk[dir] * xx[dir];
   }
-  L.x[i] = s / d;
+  L.x[i] = d == 0.f ? 0.f : s / d;
 }
 
 // Coarse right-hand side of coarse cell ci: the sum of the fine residuals b - A x of its 2x2x2
 // children. Pre-smoothing ends with a black half sweep, which leaves every black (odd) cell's
 // equation satisfied, so only the four red children carry a residual.
-__device__ inline float restrict_residual(const LevelView& F, const int32_t* child, uint64_t ci) {
+template <class KT>
+__device__ inline float restrict_residual(const LevelViewT<KT>& F, const int32_t* child, uint64_t ci) {
   const uint64_t blk = ci >> 6;
   const int l = cnat(static_cast<int>(ci & 63));
   const int cx = l & 3, cy = (l >> 2) & 3, cz = l >> 4;
@@ -572,7 +634,7 @@ __device__ inline float restrict_residual(const LevelView& F, const int32_t* chi
     for (int dir = 0; dir < 6; ++dir) {
       const int64_t j = neighbour(F.nbr, uint64_t(fb), lx, ly, lz, dir);
       if (j < 0) continue;
-      a += face_k(F.k, F.n, static_cast<int64_t>(fi), j, dir) * (xi - F.x[j]);
+      a += face_kt(F.k, F.n, static_cast<int64_t>(fi), j, dir) * (xi - F.x[j]);
     }
     sum += F.b[fi] - a;
   }
@@ -580,7 +642,8 @@ __device__ inline float restrict_residual(const LevelView& F, const int32_t* chi
 }
 
 // x_fine += x_coarse(parent).
-__device__ inline void prolong_cell(const LevelView& F, const float* cx, uint64_t i) {
+template <class KT>
+__device__ inline void prolong_cell(const LevelViewT<KT>& F, const float* cx, uint64_t i) {
   if (F.diag[i] == 0.f) return;
   const uint64_t blk = i >> 6;
   const int32_t pb = F.parent[2 * blk], oct = F.parent[2 * blk + 1];
@@ -591,14 +654,16 @@ __device__ inline void prolong_cell(const LevelView& F, const float* cx, uint64_
 }
 
 // One red-black Gauss-Seidel half sweep over the `color` half of every block.
-__global__ void k_smooth(LevelView L, int color, int first) {
+template <class KT>
+__global__ void k_smooth(LevelViewT<KT> L, int color, int first) {
   const uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (t < L.n / 2) smooth_cell(L, t, color, first);
 }
 
 // Residual b - A x of the red (even) cells after pre-smoothing, one lane per red cell, into r's red
 // half; the black cells' residual is zero after the closing black half sweep.
-__global__ void k_residual_red(LevelView L, float* r) {
+template <class KT>
+__global__ void k_residual_red(LevelViewT<KT> L, float* r) {
   const uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (t >= L.n / 2) return;
   const uint64_t blk = t >> 5;
@@ -615,13 +680,14 @@ __global__ void k_residual_red(LevelView L, float* r) {
   for (int dir = 0; dir < 6; ++dir) {
     const int64_t j = neighbour(L.nbr, blk, lx, ly, lz, dir);
     if (j < 0) continue;
-    a += face_k(L.k, L.n, static_cast<int64_t>(i), j, dir) * (xi - L.x[j]);
+    a += face_kt(L.k, L.n, static_cast<int64_t>(i), j, dir) * (xi - L.x[j]);
   }
   r[i] = L.b[i] - a;
 }
 
 // Coarse right-hand side: the sum of the red children's residuals (pull, fixed order).
-__global__ void k_restrict_red(const float* r, LevelView C) {
+template <class KT>
+__global__ void k_restrict_red(const float* r, LevelViewT<KT> C) {
   const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (i >= C.n) return;
   const uint64_t blk = i >> 6;
@@ -639,41 +705,44 @@ __global__ void k_restrict_red(const float* r, LevelView C) {
   const_cast<float*>(C.b)[i] = sum;
 }
 
-__global__ void k_prolong(LevelView F, const float* cx) {
+template <class KT>
+__global__ void k_prolong(LevelViewT<KT> F, const float* cx) {
   const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (i < F.n) prolong_cell(F, cx, i);
 }
 
 constexpr int kMaxTail = 16;
 constexpr uint64_t kTailBlocks = 32;  // levels with at most this many blocks run in the tail cycle
+template <class KT>
 struct TailCycle {
   int count, nu, coarse_sweeps;
-  LevelView lv[kMaxTail];
+  LevelViewT<KT> lv[kMaxTail];
 };
 
 // The V-cycle below and including the first small level, in one workgroup: pre-smoothing and
 // residual restriction down to the coarsest level, symmetric sweeps there from zero, then
 // prolongation and post-smoothing back up. Barriers separate the dependent steps; every wave of the
 // workgroup shares the CU's vector cache, so each step sees the previous one's stores.
-__global__ __launch_bounds__(1024) void k_tail_vcycle(TailCycle T) {
+template <class KT>
+__global__ __launch_bounds__(1024) void k_tail_vcycle(TailCycle<KT> T) {
   const uint64_t tid = threadIdx.x, nt = blockDim.x;
   auto sync = [] {
     __threadfence_block();
     __syncthreads();
   };
   for (int q = 0; q + 1 < T.count; ++q) {
-    const LevelView& L = T.lv[q];
+    const LevelViewT<KT>& L = T.lv[q];
     for (int it = 0; it < T.nu; ++it) {
       for (uint64_t t = tid; t < L.n / 2; t += nt) smooth_cell(L, t, 0, it == 0);
       sync();
       for (uint64_t t = tid; t < L.n / 2; t += nt) smooth_cell(L, t, 1, 0);
       sync();
     }
-    const LevelView& C = T.lv[q + 1];
+    const LevelViewT<KT>& C = T.lv[q + 1];
     for (uint64_t i = tid; i < C.n; i += nt) const_cast<float*>(C.b)[i] = restrict_residual(L, C.child, i);
     sync();
   }
-  const LevelView& B = T.lv[T.count - 1];
+  const LevelViewT<KT>& B = T.lv[T.count - 1];
   for (int pass = 0; pass < 4 * T.coarse_sweeps; ++pass) {
     const int fwd = pass < 2 * T.coarse_sweeps;
     const int color = fwd ? (pass & 1) : 1 - (pass & 1);
@@ -681,7 +750,7 @@ __global__ __launch_bounds__(1024) void k_tail_vcycle(TailCycle T) {
     sync();
   }
   for (int q = T.count - 2; q >= 0; --q) {
-    const LevelView& L = T.lv[q];
+    const LevelViewT<KT>& L = T.lv[q];
     for (uint64_t i = tid; i < L.n; i += nt) prolong_cell(L, T.lv[q + 1].x, i);
     sync();
     for (int it = 0; it < T.nu; ++it) {
@@ -713,6 +782,78 @@ __global__ void k_apply(uint64_t n, const int32_t* nbr, const float* k, const fl
     s += face_k(k, n, static_cast<int64_t>(i), j, dir) * (xi - x[j]);
   }
   y[i] = s;
+}
+
+// ------------------------------------------------------------------ learned preconditioner tiles
+
+// Level-0 storage index of voxel v of tile t (8^3, (z*8 + y)*8 + x), or -1 outside the level's blocks.
+__device__ inline int64_t tile_cell(const int32_t* tile_blocks, uint64_t t, int v) {
+  const int z = v >> 6, y = (v >> 3) & 7, x = v & 7;
+  const int32_t b = tile_blocks[8 * t + ((z >> 2) * 2 + (y >> 2)) * 2 + (x >> 2)];
+  if (b < 0) return -1;
+  return int64_t(b) * 64 + cslot((z & 3) * 16 + (y & 3) * 4 + (x & 3));
+}
+
+// Network inputs of every tile from the level-0 residual: the residual divided by its RMS over the
+// tile's liquid cells (the scale is kept per tile), and one-hot liquid / air / solid classes. Cells
+// outside the level's blocks are air. One workgroup of 512 lanes per tile.
+__global__ __launch_bounds__(512) void k_tile_gather(const float* r, const uint8_t* cls, const int32_t* tile_blocks,
+                                                     float* act, float* scale) {
+  __shared__ float red[512];
+  const uint64_t t = blockIdx.x;
+  const int v = threadIdx.x;
+  const int64_t i = tile_cell(tile_blocks, t, v);
+  const int c = i >= 0 ? cls[i] : 2;
+  const float rv = (i >= 0 && c == 1) ? r[i] : 0.f;
+  red[v] = rv * rv;
+  __syncthreads();
+  for (int s = 256; s > 0; s >>= 1) {
+    if (v < s) red[v] += red[v + s];
+    __syncthreads();
+  }
+  const float sum = red[0];
+  __syncthreads();
+  red[v] = c == 1 ? 1.f : 0.f;
+  __syncthreads();
+  for (int s = 256; s > 0; s >>= 1) {
+    if (v < s) red[v] += red[v + s];
+    __syncthreads();
+  }
+  const float count = red[0];
+  const float sc = (count > 0.f && sum > 0.f) ? sqrtf(sum / count) : 0.f;
+  float* in = act + t * kActFloats + kActIn;
+  in[v] = sc > 0.f ? rv / sc : 0.f;
+  in[512 + v] = c == 1 ? 1.f : 0.f;
+  in[1024 + v] = c == 2 ? 1.f : 0.f;
+  in[1536 + v] = c == 3 ? 1.f : 0.f;
+  if (v == 0) scale[t] = sc;
+}
+
+// z = network output times the tile's scale on liquid unknowns, zero elsewhere. Tiles partition the
+// level's blocks, so every cell is written once.
+__global__ void k_tile_scatter(const float* act, const float* scale, const uint8_t* cls, const int32_t* tile_blocks,
+                               uint64_t ntiles, float* z) {
+  const uint64_t gid = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (gid >= ntiles * 512) return;
+  const uint64_t t = gid >> 9;
+  const int v = static_cast<int>(gid & 511);
+  const int64_t i = tile_cell(tile_blocks, t, v);
+  if (i < 0) return;
+  const float zv = act[t * kActFloats + kActOut + v] * scale[t];
+  z[i] = (cls[i] == 1 && isfinite(zv)) ? zv : 0.f;  // a non-finite output counts as no correction
+}
+
+// Training pairs: the gathered inputs and the V-cycle's correction divided by the same tile scale.
+__global__ void k_tile_pairs(const float* act, const float* scale, const float* corr, const uint8_t* cls,
+                             const int32_t* tile_blocks, uint64_t ntiles, float* data_in, float* data_target) {
+  const uint64_t gid = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (gid >= ntiles * 512) return;
+  const uint64_t t = gid >> 9;
+  const int v = static_cast<int>(gid & 511);
+  for (int c = 0; c < 4; ++c) data_in[t * 2048 + c * 512 + v] = act[t * kActFloats + kActIn + c * 512 + v];
+  const int64_t i = tile_cell(tile_blocks, t, v);
+  const float sc = scale[t];
+  data_target[t * 512 + v] = (i >= 0 && cls[i] == 1 && sc > 0.f) ? corr[i] / sc : 0.f;
 }
 
 // ------------------------------------------------------------------ PCG kernels
@@ -752,6 +893,46 @@ __global__ void k_after_rho(double* scal, float* fscal, uint64_t* err, int resta
     }
     fscal[1] = beta;
   }
+}
+
+// Flexible CG (one-term orthogonalisation): beta = -(z . q_prev) / (d_prev . q_prev), with z . q_prev in
+// scal[28] and the previous curvature in scal[2]. The step length stays (r . z) / (d . q), which the
+// orthogonality of r to d_prev makes equal to (r . d) / (d . q); it may be of either sign.
+__global__ void k_after_rho_fcg(double* scal, float* fscal, uint64_t* err, int restart) {
+  if (threadIdx.x) return;
+  const double rho = scal[20];
+  scal[0] = rho;
+  scal[14] = 0.0;
+  if (!finite(rho)) {
+    perr_report(err + 1, 0, kOutOfRange);
+    return;
+  }
+  if (restart) {
+    fscal[1] = 0.f;
+  } else {
+    float beta;
+    if (!fine_value(-scal[28] / scal[2], beta)) {
+      perr_report(err + 1, 1, kOutOfRange);
+      return;
+    }
+    fscal[1] = beta;
+  }
+}
+
+// Flexible-CG curvature: a zero direction (a network output orthogonal to everything) is not an error
+// here; the step is skipped and scal[14] tells the host to hand over to the V-cycle.
+__global__ void k_after_curv_fcg(double* scal, float* fscal, uint64_t* err) {
+  if (threadIdx.x) return;
+  const double curv = scal[20];
+  float alpha = 0.f;
+  if (!(curv > 0.0) || !fine_value(scal[0] / curv, alpha)) {
+    scal[14] = 1.0;
+    fscal[0] = 0.f;
+    return;
+  }
+  scal[2] = curv;
+  fscal[0] = alpha;
+  scal[1] = scal[0];
 }
 
 __global__ void k_after_curv(double* scal, float* fscal, uint64_t* err) {
@@ -855,6 +1036,9 @@ int ensure_capacity(fl_ctx* ctx, uint64_t nc, uint64_t nf, uint64_t nfine) {
   const uint64_t cf = std::max<uint64_t>({nf, ctx->cap.pressure_faces, 1});
   const uint64_t cn = std::max<uint64_t>(nfine, 64);
   if (nc > ctx->cap.pressure_cells || nf > ctx->cap.pressure_faces) return FL_RESOURCE_EXHAUSTED;
+  dfree(P.tile_blocks);
+  dfree(P.cell_class);
+  P.ntiles = 0;
   void* ptrs[] = {P.fine_cell, P.cell_fine, P.cell_flag, P.cell_faces, P.face_tail, P.face_head, P.face_kind,
                   P.face_slot, P.face_area, P.face_beta, P.face_dist, P.face_ustar, P.face_uwall, P.face_flux,
                   P.face_has_flux, P.face_lift, P.face_scale, P.face_k, P.face_wall, P.face_u, P.face_rec, P.cell_volume,
@@ -889,6 +1073,8 @@ int ensure_capacity(fl_ctx* ctx, uint64_t nc, uint64_t nf, uint64_t nfine) {
     FL_TRY(dal(a, cc));
   FL_TRY(dal(&P.fine_cell, cn));
   FL_TRY(dal(&P.fine_vol, cn));
+  FL_TRY(dal(&P.cell_class, cn));
+  FL_TRY(dal(&P.tile_blocks, 8 * (cn / 64)));
   FL_TRY(dal(&P.r32, cn));
   FL_TRY(dal(&P.x32, cn));
   FL_TRY(dal(&P.d32, cn));
@@ -994,6 +1180,8 @@ int build_levels(fl_ctx* ctx, Pressure& P, const std::vector<Key3>& blocks0, int
     }
     FL_TRY(dal(&L.x, n));
     FL_TRY(dal(&L.r, n));
+    FL_TRY(dal(&L.kb, 3 * n));
+    FL_TRY(dal(&L.diagb, n));
     FL_TRY(dal(&L.b, n));
     FL_TRY(dal(&L.k, 3 * n));
     FL_TRY(dal(&L.anchor, n));
@@ -1022,19 +1210,29 @@ int setup_coefficients(fl_ctx* ctx, Pressure& P) {
   return FL_OK;
 }
 
-LevelView view(Pressure& P, size_t l) {
+template <class KT>
+LevelViewT<KT> view(Pressure& P, size_t l);
+template <>
+LevelViewT<float> view<float>(Pressure& P, size_t l) {
   Level& L = P.levels[l];
-  return LevelView{L.nb * 64, L.nbr, L.child, L.parent, L.x, l == 0 ? P.r32 : L.b, L.k, L.anchor, L.diag};
+  return {L.nb * 64, L.nbr, L.child, L.parent, L.x, l == 0 ? P.r32 : L.b, L.k, L.anchor, L.diag};
+}
+template <>
+LevelViewT<Bf16> view<Bf16>(Pressure& P, size_t l) {
+  Level& L = P.levels[l];
+  return {L.nb * 64,           L.nbr,  L.child, L.parent, L.x, l == 0 ? P.r32 : L.b, reinterpret_cast<const Bf16*>(L.kb),
+          L.anchor,            L.diagb};
 }
 
 // V-cycle from level l. Level 0 smooths the recursive FP32 residual of the outer PCG in place of its
 // own right-hand side. Levels of at most kTailBlocks blocks (and everything below them) run in one
 // single-workgroup launch.
+template <class KT>
 int vcycle(fl_ctx* ctx, Pressure& P, size_t l, int nu, int coarse_sweeps) {
   hipStream_t s = ctx->stream;
   const size_t last = P.levels.size() - 1;
   if (l == last && P.levels[l].nb > kTailBlocks) {  // a large coarsest level: grid-wide sweeps from zero
-    const LevelView B = view(P, l);
+    const LevelViewT<KT> B = view<KT>(P, l);
     for (int pass = 0; pass < 4 * coarse_sweeps; ++pass) {
       const int fwd = pass < 2 * coarse_sweeps;
       k_smooth<<<grid_for(B.n / 2), kThreads, 0, s>>>(B, fwd ? (pass & 1) : 1 - (pass & 1), pass == 0);
@@ -1043,17 +1241,17 @@ int vcycle(fl_ctx* ctx, Pressure& P, size_t l, int nu, int coarse_sweeps) {
     return FL_OK;
   }
   if (P.levels[l].nb <= kTailBlocks) {
-    TailCycle T{};
+    TailCycle<KT> T{};
     T.count = static_cast<int>(last - l + 1);
     if (T.count > kMaxTail) return FL_INTERNAL;
     T.nu = nu;
     T.coarse_sweeps = coarse_sweeps;
-    for (size_t q = l; q <= last; ++q) T.lv[q - l] = view(P, q);
+    for (size_t q = l; q <= last; ++q) T.lv[q - l] = view<KT>(P, q);
     k_tail_vcycle<<<1, 1024, 0, s>>>(T);
     FL_HIP(hipGetLastError());
     return FL_OK;
   }
-  const LevelView L = view(P, l), C = view(P, l + 1);
+  const LevelViewT<KT> L = view<KT>(P, l), C = view<KT>(P, l + 1);
   const unsigned gh = grid_for(L.n / 2);
   for (int it = 0; it < nu; ++it) {
     k_smooth<<<gh, kThreads, 0, s>>>(L, 0, it == 0);
@@ -1062,7 +1260,7 @@ int vcycle(fl_ctx* ctx, Pressure& P, size_t l, int nu, int coarse_sweeps) {
   k_residual_red<<<gh, kThreads, 0, s>>>(L, P.levels[l].r);
   k_restrict_red<<<grid_for(C.n), kThreads, 0, s>>>(P.levels[l].r, C);
   FL_HIP(hipGetLastError());
-  FL_TRY(vcycle(ctx, P, l + 1, nu, coarse_sweeps));
+  FL_TRY(vcycle<KT>(ctx, P, l + 1, nu, coarse_sweeps));
   k_prolong<<<grid_for(L.n), kThreads, 0, s>>>(L, C.x);
   for (int it = 0; it < nu; ++it) {
     k_smooth<<<gh, kThreads, 0, s>>>(L, 1, 0);
@@ -1083,9 +1281,8 @@ struct EvalResult {
 
 int evaluate(fl_ctx* ctx, Pressure& P, const double* p, EvalResult& out) {
   hipStream_t s = ctx->stream;
-  const uint64_t nfine = P.levels[0].nb * 64;
-  const unsigned g = reduce_grid(nfine);
-  k_evaluate<<<g, kThreads, 0, s>>>(nfine, P.fine_cell, P.cell_faces, P.face_rec, P.cell_volume, P.cell_target,
+  const unsigned g = reduce_grid(P.nc);
+  k_evaluate<<<g, kThreads, 0, s>>>(P.nc, P.cell_faces, P.face_rec, P.cell_volume, P.cell_target,
                                     P.cell_inwall, P.cell_comp, P.cell_prev, P.rhs, P.rhs_abs, p, P.res, P.def,
                                     P.partial, P.perr);
   k_finalize4<<<1, kThreads, 0, s>>>(P.partial, g, P.scal, 8);
@@ -1143,18 +1340,67 @@ int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates&
   if (e.code) return e.code;
   if (accept(e, gates)) return FL_OK;
   k_load_residual<<<g, kThreads, 0, s>>>(n, P.fine_cell, P.cell_flag, P.res, P.r32, P.x32, P.perr);
+  if (opt.level_storage == 2) {
+    for (auto& L : P.levels) {
+      const uint64_t nl = L.nb * 64;
+      k_to_bf16<<<grid_for(3 * nl), kThreads, 0, s>>>(L.k, 3 * nl, L.kb);
+      k_diag_bf16<<<grid_for(nl), kThreads, 0, s>>>(nl, L.nbr, L.kb, L.anchor, L.diag, L.diagb);
+    }
+    FL_HIP(hipGetLastError());
+  }
   bool restart = true;
   int since = 0;
+  // Learned preconditioner: flexible CG with the network until 3 consecutive iterations fail to reduce
+  // the recursive residual by 10 %, then the V-cycle from the current iterate (restart).
+  LearnState* net = static_cast<LearnState*>(ctx->learn);
+  bool use_net = opt.preconditioner == 1;
+  if (use_net && (!net || P.ntiles > net->tile_cap)) return FL_FAILED_PRECONDITION;
+  int stagnant = 0;
+  double rn_prev = std::numeric_limits<double>::infinity();
+  diag.learned_iterations = 0;
   for (uint64_t it = 1; it <= opt.max_iterations; ++it) {
-    FL_TRY(vcycle(ctx, P, 0, opt.smoothing_sweeps, opt.coarse_sweeps));
+    if (use_net) {
+      k_tile_gather<<<static_cast<unsigned>(P.ntiles), 512, 0, s>>>(P.r32, P.cell_class, P.tile_blocks, net->act,
+                                                                    net->scale);
+      FL_TRY(net_forward(s, net->params, net->act, P.ntiles));
+      k_tile_scatter<<<grid_for(P.ntiles * 512), kThreads, 0, s>>>(net->act, net->scale, P.cell_class, P.tile_blocks,
+                                                                    P.ntiles, L0.x);
+      ++diag.learned_iterations;
+    } else if (opt.level_storage == 2) {
+      FL_TRY(vcycle<Bf16>(ctx, P, 0, opt.smoothing_sweeps, opt.coarse_sweeps));
+    } else {
+      FL_TRY(vcycle<float>(ctx, P, 0, opt.smoothing_sweeps, opt.coarse_sweeps));
+    }
+    if (!use_net && net && net->collect_every > 0 && it % uint64_t(net->collect_every) == 0 &&
+        P.ntiles <= net->tile_cap && net->ndata + P.ntiles <= net->data_cap) {
+      // Training pairs: this iteration's residual and its V-cycle correction, per tile.
+      k_tile_gather<<<static_cast<unsigned>(P.ntiles), 512, 0, s>>>(P.r32, P.cell_class, P.tile_blocks, net->act,
+                                                                    net->scale);
+      k_tile_pairs<<<grid_for(P.ntiles * 512), kThreads, 0, s>>>(net->act, net->scale, L0.x, P.cell_class,
+                                                                  P.tile_blocks, P.ntiles,
+                                                                  net->data_in + net->ndata * 2048,
+                                                                  net->data_target + net->ndata * 512);
+      net->ndata += P.ntiles;
+    }
     k_dot<<<rg, kThreads, 0, s>>>(n, P.r32, L0.x, P.partial);
     k_finalize4<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, 20);
-    k_after_rho<<<1, 64, 0, s>>>(P.scal, P.fscal, P.perr, restart ? 1 : 0);
+    if (use_net) {
+      if (!restart) {
+        k_dot<<<rg, kThreads, 0, s>>>(n, L0.x, P.q32, P.partial);
+        k_finalize4<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, 28);
+      }
+      k_after_rho_fcg<<<1, 64, 0, s>>>(P.scal, P.fscal, P.perr, restart ? 1 : 0);
+    } else {
+      k_after_rho<<<1, 64, 0, s>>>(P.scal, P.fscal, P.perr, restart ? 1 : 0);
+    }
     k_update_d<<<g, kThreads, 0, s>>>(n, L0.x, P.d32, P.fscal, restart ? 1 : 0);
     k_apply<<<g, kThreads, 0, s>>>(n, L0.nbr, L0.k, L0.anchor, L0.diag, P.d32, P.q32);
     k_dot<<<rg, kThreads, 0, s>>>(n, P.d32, P.q32, P.partial);
     k_finalize4<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, 20);
-    k_after_curv<<<1, 64, 0, s>>>(P.scal, P.fscal, P.perr);
+    if (use_net)
+      k_after_curv_fcg<<<1, 64, 0, s>>>(P.scal, P.fscal, P.perr);
+    else
+      k_after_curv<<<1, 64, 0, s>>>(P.scal, P.fscal, P.perr);
     k_update_xr<<<rg, kThreads, 0, s>>>(n, P.d32, P.q32, L0.diag, P.fine_vol, P.x32, P.r32, P.fscal, P.partial,
                                         P.perr);
     k_finalize4<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, 8);
@@ -1169,6 +1415,14 @@ int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates&
     restart = false;
     ++since;
     const double rn = std::sqrt(P.hscal[8]), rmax = P.hscal[9], dn = std::sqrt(P.hscal[11]);
+    if (use_net) {
+      stagnant = (rn > 0.9 * rn_prev || !(P.hscal[0] > 0.0)) ? stagnant + 1 : 0;
+      if (stagnant >= 3 || P.hscal[14] != 0.0) {
+        use_net = false;
+        restart = true;
+      }
+    }
+    rn_prev = rn;
     bool recursive = rn <= gates.integrated && rmax <= gates.volume;
     if (gates.relative > 0.0) recursive = recursive && dn <= gates.relative * gates.initial_defect;
     const bool last = it == opt.max_iterations;
@@ -1197,6 +1451,8 @@ void destroy_pressure(fl_ctx* ctx) {
   Pressure* P = state(ctx);
   if (!P) return;
   free_levels(*P);
+  dfree(P->tile_blocks);
+  dfree(P->cell_class);
   void* ptrs[] = {P->fine_cell, P->cell_fine, P->cell_flag, P->cell_faces, P->face_tail, P->face_head, P->face_kind,
                   P->face_slot, P->face_area, P->face_beta, P->face_dist, P->face_ustar, P->face_uwall, P->face_flux,
                   P->face_has_flux, P->face_lift, P->face_scale, P->face_k, P->face_wall, P->face_u, P->face_rec, P->cell_volume,
@@ -1522,6 +1778,45 @@ int prepare_system(fl_ctx* ctx, Pressure& P, const fl_geometry* g, const fl_int3
   UP(P.cell_inwall, inw);
   UP(P.p_init, p0);
   P.p_init_zero = std::all_of(p0.begin(), p0.end(), [](double v) { return v == 0.0; });
+  // Learned-preconditioner tiles (2x2x2 level-0 blocks) and cell classes: unknowns are liquid; a cell
+  // across a sealed face of an unknown is solid; every other cell is air.
+  {
+    std::vector<uint8_t> cls(nfine, 2);
+    for (size_t c = 0; c < nc; ++c) {
+      if (flag[c] != 1) continue;
+      const fl_int3 ci = cells[c].cell_index;
+      cls[cell_fine[c]] = 1;
+      for (int a = 0; a < 3; ++a)
+        for (int side = 0; side < 2; ++side) {
+          if (kind[cell_faces[6 * c + 2 * a + side]] != 0) continue;
+          int64_t q[3] = {ci.x, ci.y, ci.z};
+          q[a] += side ? 1 : -1;
+          const Key3 b{int32_t(host_floor_div4(q[0])), int32_t(host_floor_div4(q[1])), int32_t(host_floor_div4(q[2]))};
+          auto it = bidx.find(b);
+          if (it == bidx.end()) continue;
+          const int64_t fi = int64_t(it->second) * 64 +
+                             cslot(int((q[2] - 4 * b.z) * 16 + (q[1] - 4 * b.y) * 4 + (q[0] - 4 * b.x)));
+          if (cls[fi] != 1) cls[fi] = 3;
+        }
+    }
+    std::vector<Key3> tiles;
+    tiles.reserve(blocks0.size());
+    for (const Key3& b : blocks0) tiles.push_back({floor_half(b.x), floor_half(b.y), floor_half(b.z)});
+    std::sort(tiles.begin(), tiles.end(), key_less);
+    tiles.erase(std::unique(tiles.begin(), tiles.end()), tiles.end());
+    std::vector<int32_t> tb(8 * tiles.size(), -1);
+    std::unordered_map<Key3, int32_t, Key3Hash> tidx;
+    for (size_t t = 0; t < tiles.size(); ++t) tidx[tiles[t]] = int32_t(t);
+    for (size_t i = 0; i < blocks0.size(); ++i) {
+      const Key3& b = blocks0[i];
+      const Key3 tk{floor_half(b.x), floor_half(b.y), floor_half(b.z)};
+      const int o = ((b.z - 2 * tk.z) * 2 + (b.y - 2 * tk.y)) * 2 + (b.x - 2 * tk.x);
+      tb[8 * tidx[tk] + o] = int32_t(i);
+    }
+    P.ntiles = tiles.size();
+    UP(P.cell_class, cls);
+    UP(P.tile_blocks, tb);
+  }
 #undef UP
   FL_HIP(hipStreamSynchronize(s));
   return build_levels(ctx, P, blocks0, 1);
@@ -1540,9 +1835,8 @@ int freeze_and_assemble(fl_ctx* ctx, Pressure& P, double dt, const fl_pressure_o
                                                      P.face_wall, P.face_rec);
   k_freeze_cells<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, dt, P.cell_volume, P.cell_s, P.cell_K, P.cell_comp,
                                                      P.cell_target);
-  const uint64_t nfine = P.levels[0].nb * 64;
-  k_rhs<<<grid_for(nfine), kThreads, 0, s>>>(nfine, P.fine_cell, P.cell_faces, P.face_rec, P.cell_target,
-                                             P.cell_inwall, P.cell_comp, P.cell_prev, P.rhs, P.rhs_abs);
+  k_rhs<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, P.cell_faces, P.face_rec, P.cell_target, P.cell_inwall,
+                                            P.cell_comp, P.cell_prev, P.rhs, P.rhs_abs);
   FL_HIP(hipGetLastError());
   FL_TRY(setup_coefficients(ctx, P));
   const unsigned g = reduce_grid(P.nc);
@@ -1571,7 +1865,8 @@ int check_options(const fl_pressure_options* o) {
   if (!o) return FL_INVALID_ARGUMENT;
   if (o->max_iterations < 1 || o->max_iterations > 256) return FL_INVALID_ARGUMENT;
   if (!(o->cpu_residual_tolerance_scale >= 1e-12 && o->cpu_residual_tolerance_scale <= 1.0)) return FL_INVALID_ARGUMENT;
-  if (o->fp32_vcycle != 1 || o->levels < 1 || o->levels > 16 || o->smoothing_sweeps < 1 || o->coarse_sweeps < 1)
+  if (o->preconditioner < 0 || o->preconditioner > 1) return FL_INVALID_ARGUMENT;
+  if (o->level_storage < 1 || o->level_storage > 2 || o->levels < 1 || o->levels > 16 || o->smoothing_sweeps < 1 || o->coarse_sweeps < 1)
     return FL_INVALID_ARGUMENT;
   if (!(o->coarse_scale > 0.0 && o->coarse_scale <= 1.0)) return FL_INVALID_ARGUMENT;
   if (!(o->relative_divergence_target >= 0.0)) return FL_INVALID_ARGUMENT;

@@ -169,7 +169,9 @@ __device__ __forceinline__ bool range_ok(const Axis3& X, const Axis3& Y, const A
 // Deposits the 27 samples of one face axis into the LDS window. Contributions are formed exactly as in
 // AccumulateMacParticleComponent (mac_deposit_3d_internal.h:50-57): weight (wx wy) wz, mass w m,
 // velocity v + ((cx ox + cy oy) + cz oz), momentum mass * velocity.
-template <int A, bool kSmall>
+// kF64: the FP64 deposit of DepositMac3D (transfers_3d.cc:139-145): mass and momentum are added to
+// the window as doubles (scales are 1); otherwise both are converted to integer quanta first.
+template <int A, bool kSmall, bool kF64>
 __device__ __forceinline__ void deposit_axis(const Axis3& X, const Axis3& Y, const Axis3& Z, const int32_t* tile,
                                              double m, double v, double cx, double cy, double cz, double inv_qm,
                                              double inv_qp, unsigned long long* wm, unsigned long long* wq) {
@@ -198,8 +200,13 @@ __device__ __forceinline__ void deposit_axis(const Axis3& X, const Axis3& Y, con
         const double mass = (wxy[xi] * Z.w[zi]) * m;
         const double mom = mass * (v + (oxy[xi] + oz[zi]));
         const int slot = fz[zi] + fy[yi] + fx[xi];
-        atomicAdd(&wm[slot], quanta<kSmall>(mass, inv_qm));
-        atomicAdd(&wq[slot], quanta<kSmall>(mom, inv_qp));
+        if constexpr (kF64) {
+          atomicAdd(reinterpret_cast<double*>(wm) + slot, mass);
+          atomicAdd(reinterpret_cast<double*>(wq) + slot, mom);
+        } else {
+          atomicAdd(&wm[slot], quanta<kSmall>(mass, inv_qm));
+          atomicAdd(&wq[slot], quanta<kSmall>(mom, inv_qp));
+        }
       }
     }
   }
@@ -207,7 +214,7 @@ __device__ __forceinline__ void deposit_axis(const Axis3& X, const Axis3& Y, con
 
 // One particle's three face axes in the reference's order (transfers_3d.cc:75-100): per axis the
 // stencil status, the window ownership, then the contribution range. Returns the first failure.
-template <bool kInterior, bool kSmall, class Coord>
+template <bool kInterior, bool kSmall, bool kF64, class Coord>
 __device__ __forceinline__ int deposit_particle(const Coord& cx, const Coord& cy, const Coord& cz, const int32_t* bin,
                                                 const int32_t* tile, const int32_t* wblock, double h,
                                                 const ParticleView& p, uint32_t k, double inv_qm, double inv_qp,
@@ -228,7 +235,7 @@ __device__ __forceinline__ int deposit_particle(const Coord& cx, const Coord& cy
     if (code == kOk) {
       const double v = p.v[0][k], c0 = p.c[0][k], c1 = p.c[1][k], c2 = p.c[2][k];
       if (!range_ok(nx, ty, tz, m, v, c0, c1, c2, inv_qm, inv_qp, limit)) code = kOutOfRange;
-      deposit_axis<0, kSmall>(nx, ty, tz, tile, m, v, c0, c1, c2, inv_qm, inv_qp, wm, wq);
+      deposit_axis<0, kSmall, kF64>(nx, ty, tz, tile, m, v, c0, c1, c2, inv_qm, inv_qp, wm, wq);
     }
   }
   Axis3 tx;
@@ -243,7 +250,7 @@ __device__ __forceinline__ int deposit_particle(const Coord& cx, const Coord& cy
     if (code == kOk) {
       const double v = p.v[1][k], c0 = p.c[3][k], c1 = p.c[4][k], c2 = p.c[5][k];
       if (!range_ok(tx, ny, tz, m, v, c0, c1, c2, inv_qm, inv_qp, limit)) code = kOutOfRange;
-      deposit_axis<1, kSmall>(tx, ny, tz, tile, m, v, c0, c1, c2, inv_qm, inv_qp, wm, wq);
+      deposit_axis<1, kSmall, kF64>(tx, ny, tz, tile, m, v, c0, c1, c2, inv_qm, inv_qp, wm, wq);
     }
   }
   {  // face axis z
@@ -256,7 +263,7 @@ __device__ __forceinline__ int deposit_particle(const Coord& cx, const Coord& cy
     if (code == kOk) {
       const double v = p.v[2][k], c0 = p.c[6][k], c1 = p.c[7][k], c2 = p.c[8][k];
       if (!range_ok(tx, ty, nz, m, v, c0, c1, c2, inv_qm, inv_qp, limit)) code = kOutOfRange;
-      deposit_axis<2, kSmall>(tx, ty, nz, tile, m, v, c0, c1, c2, inv_qm, inv_qp, wm, wq);
+      deposit_axis<2, kSmall, kF64>(tx, ty, nz, tile, m, v, c0, c1, c2, inv_qm, inv_qp, wm, wq);
     }
   }
   return code;
@@ -271,7 +278,7 @@ __device__ inline uint32_t gcd_u32(uint32_t a, uint32_t b) {
   return a;
 }
 
-template <bool kTrial, bool kPow2, bool kSmall>
+template <bool kTrial, bool kPow2, bool kSmall, bool kF64>
 __global__ __launch_bounds__(kThreads) void k_p2g(ParticleView p, BinList bins, GridView grid, double inv_qm,
                                                   double inv_qp, double limit, const uint32_t* perm,
                                                   unsigned long long* qm, unsigned long long* qp, uint64_t* err) {
@@ -320,10 +327,10 @@ __global__ __launch_bounds__(kThreads) void k_p2g(ParticleView p, BinList bins, 
     const auto cy = normalize<kPow2>(t[1], g.oy, g.h, g.inv_h);
     const auto cz = normalize<kPow2>(t[2], g.oz, g.h, g.inv_h);
     if (__all(cx.interior() && cy.interior() && cz.interior()))
-      code = deposit_particle<true, kSmall>(cx, cy, cz, bin, tile, wblock, g.h, p, k, inv_qm, inv_qp, limit, wm, wq,
+      code = deposit_particle<true, kSmall, kF64>(cx, cy, cz, bin, tile, wblock, g.h, p, k, inv_qm, inv_qp, limit, wm, wq,
                                             code);
     else
-      code = deposit_particle<false, kSmall>(cx, cy, cz, bin, tile, wblock, g.h, p, k, inv_qm, inv_qp, limit, wm,
+      code = deposit_particle<false, kSmall, kF64>(cx, cy, cz, bin, tile, wblock, g.h, p, k, inv_qm, inv_qp, limit, wm,
                                              wq, code);
     if (code) report(err, perm[k], code);
   }
@@ -332,9 +339,21 @@ __global__ __launch_bounds__(kThreads) void k_p2g(ParticleView p, BinList bins, 
     const int32_t ord = wblock[i / 192];
     if (ord < 0) continue;
     const uint64_t slot = uint64_t(ord) * 192 + (i % 192);
-    if (wm[i]) atomicAdd(&qm[slot], wm[i]);
-    if (wq[i]) atomicAdd(&qp[slot], wq[i]);
+    if constexpr (kF64) {
+      if (wm[i]) unsafeAtomicAdd(reinterpret_cast<double*>(qm) + slot, __longlong_as_double(static_cast<long long>(wm[i])));
+      if (wq[i]) unsafeAtomicAdd(reinterpret_cast<double*>(qp) + slot, __longlong_as_double(static_cast<long long>(wq[i])));
+    } else {
+      if (wm[i]) atomicAdd(&qm[slot], wm[i]);
+      if (wq[i]) atomicAdd(&qp[slot], wq[i]);
+    }
   }
+}
+
+// A face sum that overflowed from finite contributions: OutOfRange, ordered after every particle's
+// own failure (the running-sum position of the overflow is not tracked).
+__global__ void k_check_sums(const double* m, const double* q, uint64_t nf, uint64_t n, uint64_t* err) {
+  const uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (f < nf && !(finite(m[f]) && finite(q[f]))) report(err, n, kOutOfRange);
 }
 
 constexpr int64_t kMaxQuanta = int64_t(1) << 62;
@@ -559,7 +578,7 @@ int unpack_particles(fl_ctx* ctx, uint64_t n) {
 }
 
 int p2g_quanta(fl_ctx* ctx, uint64_t n, double inv_qm, double inv_qp, double limit, hipEvent_t before,
-               hipEvent_t after) {
+               hipEvent_t after, bool f64) {
   const uint64_t nf = ctx->nb * 192;
   FL_TRY(fill_i64(ctx, ctx->qm, nf, 0));
   FL_TRY(fill_i64(ctx, ctx->qp, nf, 0));
@@ -567,16 +586,26 @@ int p2g_quanta(fl_ctx* ctx, uint64_t n, double inv_qm, double inv_qp, double lim
   if (after && (ctx->nbins == 0 || n == 0)) FL_HIP(hipEventRecord(after, ctx->stream));
   if (ctx->nbins == 0 || n == 0) return FL_OK;
   // Instances: trial positions distinct from the sorted ones, power-of-two cell width, limit <= 2^50.
-  using K = decltype(&k_p2g<false, true, true>);
-  static const K table[8] = {k_p2g<false, false, false>, k_p2g<false, false, true>, k_p2g<false, true, false>,
-                             k_p2g<false, true, true>,   k_p2g<true, false, false>,  k_p2g<true, false, true>,
-                             k_p2g<true, true, false>,   k_p2g<true, true, true>};
+  using K = decltype(&k_p2g<false, true, true, false>);
+  static const K table[8] = {k_p2g<false, false, false, false>, k_p2g<false, false, true, false>,
+                             k_p2g<false, true, false, false>,  k_p2g<false, true, true, false>,
+                             k_p2g<true, false, false, false>,  k_p2g<true, false, true, false>,
+                             k_p2g<true, true, false, false>,   k_p2g<true, true, true, false>};
+  static const K table_f64[4] = {k_p2g<false, false, false, true>, k_p2g<false, true, false, true>,
+                                 k_p2g<true, false, false, true>, k_p2g<true, true, false, true>};
   const bool trial = !ctx->sorted.trial_aliases_sorted, pow2 = ctx->geometry.inv_h != 0.0, small = limit <= 0x1p50;
-  auto* kernel = table[(trial ? 4 : 0) + (pow2 ? 2 : 0) + (small ? 1 : 0)];
+  auto* kernel = f64 ? table_f64[(trial ? 2 : 0) + (pow2 ? 1 : 0)]
+                     : table[(trial ? 4 : 0) + (pow2 ? 2 : 0) + (small ? 1 : 0)];
   kernel<<<ctx->nbins, kThreads, 0, ctx->stream>>>(ctx->sorted.view(), bin_list(ctx), grid_view(ctx), inv_qm, inv_qp,
                                                    limit, ctx->perm, reinterpret_cast<unsigned long long*>(ctx->qm),
                                                    reinterpret_cast<unsigned long long*>(ctx->qp), ctx->err);
   FL_HIP(hipGetLastError());
+  if (f64) {
+    k_check_sums<<<grid_for(nf), kThreads, 0, ctx->stream>>>(reinterpret_cast<const double*>(ctx->qm),
+                                                             reinterpret_cast<const double*>(ctx->qp), nf, n,
+                                                             ctx->err);
+    FL_HIP(hipGetLastError());
+  }
   if (after) FL_HIP(hipEventRecord(after, ctx->stream));
   return FL_OK;
 }

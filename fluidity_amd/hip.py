@@ -49,9 +49,11 @@ class StepReport(C.Structure):
         return np.frombuffer(bytes(self.pressure), dtype=R.PRESSURE_DIAGNOSTICS)[0]
 
 
-EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_p2g_quanta",
+EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_p2g", "fl_p2g_quanta",
            "fl_normalize_forces", "fl_apply_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
-           "fl_scene_download")
+           "fl_scene_download", "fl_learn_param_count", "fl_learn_setup", "fl_learn_set_params", "fl_learn_get_params",
+           "fl_learn_forward", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect",
+           "fl_learn_train")
 
 _lib = None
 
@@ -63,6 +65,7 @@ def lib():
             raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (no CPU fallback exists)")
         _lib = C.CDLL(LIB_PATH)
         _lib.fl_destroy.restype = None
+        _lib.fl_learn_param_count.restype = C.c_size_t
     return _lib
 
 
@@ -122,6 +125,14 @@ class Context:
         s = lib().fl_activate(self.handle, _p(geometry), _p(pos), C.c_size_t(len(pos)), _p(cells),
                               C.c_size_t(len(cells)), _p(out), C.byref(cnt), C.byref(self.last))
         return s, out[: cnt.value].copy()
+
+    def p2g(self, geometry, blocks, particles, out=None):
+        nf = 192 * len(blocks)
+        if out is None:
+            out = np.zeros(nf, R.FACE_DEPOSIT)
+        s = lib().fl_p2g(self.handle, _p(geometry), _p(blocks), C.c_size_t(len(blocks)), _p(particles),
+                         C.c_size_t(len(particles)), _p(out), C.c_size_t(len(out)), C.byref(self.last))
+        return s, out
 
     def p2g_quanta(self, geometry, blocks, particles, mass_q, mom_q, cap, out=None):
         nf = 192 * len(blocks)
@@ -203,8 +214,63 @@ class Context:
         return s, out
 
 
+class Network:
+    """The learned preconditioner of a context (fl_learn_*): FP32 parameters, training pairs, training."""
+
+    def __init__(self, ctx, tiles, pairs):
+        self.ctx = ctx
+        s = lib().fl_learn_setup(ctx.handle, C.c_uint64(tiles), C.c_uint64(pairs))
+        if s != OK:
+            raise RuntimeError(f"fl_learn_setup failed with status {s}")
+        self.count = int(lib().fl_learn_param_count())
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        return lib().fl_learn_set_params(self.ctx.handle, _p(p), C.c_size_t(len(p)))
+
+    def params(self):
+        out = np.zeros(self.count, np.float32)
+        assert lib().fl_learn_get_params(self.ctx.handle, _p(out), C.c_size_t(self.count)) == OK
+        return out
+
+    def forward(self, inputs):
+        x = np.ascontiguousarray(inputs, dtype=np.float32).reshape(-1, 4, 512)
+        out = np.zeros((len(x), 512), np.float32)
+        s = lib().fl_learn_forward(self.ctx.handle, _p(x), C.c_size_t(len(x)), _p(out))
+        return s, out
+
+    def loss_grad(self, inputs, targets):
+        x = np.ascontiguousarray(inputs, dtype=np.float32).reshape(-1, 4, 512)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, 512)
+        loss = C.c_float()
+        gp = np.zeros(self.count, np.float32)
+        gi = np.zeros(x.shape, np.float32)
+        s = lib().fl_learn_loss_grad(self.ctx.handle, _p(x), _p(t), C.c_size_t(len(x)), C.byref(loss), _p(gp), _p(gi))
+        return s, loss.value, gp, gi
+
+    def add_pairs(self, inputs, targets):
+        x = np.ascontiguousarray(inputs, dtype=np.float32).reshape(-1, 4, 512)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, 512)
+        return lib().fl_learn_add_pairs(self.ctx.handle, _p(x), _p(t), C.c_size_t(len(x)))
+
+    def pairs(self):
+        n = C.c_size_t()
+        assert lib().fl_learn_pairs(self.ctx.handle, C.byref(n)) == OK
+        return n.value
+
+    def collect(self, every, clear=False):
+        return lib().fl_learn_collect(self.ctx.handle, C.c_int(every), C.c_int(1 if clear else 0))
+
+    def train(self, epochs, batch=256, lr=1e-3, seed=0):
+        losses = np.zeros(epochs, np.float32)
+        s = lib().fl_learn_train(self.ctx.handle, C.c_int(epochs), C.c_size_t(batch), C.c_float(lr), C.c_uint64(seed),
+                                 _p(losses))
+        return s, losses
+
+
 def pressure_options(max_iterations=256, tolerance_scale=1.0, relative_target=0.0, levels=4, sweeps=2,
-                     coarse_sweeps=16, coarse_scale=0.5):
+                     coarse_sweeps=16, coarse_scale=0.5, bf16_levels=False, learned=False):
     o = np.zeros(1, R.PRESSURE_OPTIONS)
-    o[0] = (max_iterations, tolerance_scale, relative_target, levels, sweeps, coarse_sweeps, 1, coarse_scale)
+    o[0] = (max_iterations, tolerance_scale, relative_target, levels, sweeps, coarse_sweeps, 2 if bf16_levels else 1,
+            coarse_scale, 1 if learned else 0, 0)
     return o

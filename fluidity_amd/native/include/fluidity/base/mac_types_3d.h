@@ -80,6 +80,11 @@ struct MacDepositQuantization3D {
   std::size_t declared_particle_capacity;
 };
 
+struct MacFaceDeposit3D {
+  double deposited_mass;           // kilograms
+  double deposited_pure_momentum;  // normal momentum, kilogram meters per second
+};
+
 struct MacFaceDepositQuanta3D {
   std::int64_t mass_quanta_sum;
   std::int64_t pure_momentum_quanta_sum;

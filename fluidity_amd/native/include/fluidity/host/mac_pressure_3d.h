@@ -88,6 +88,8 @@ struct HipMacPressureSolverSettings3D {
   int smoothing_sweeps = 2;                 // RBGS sweeps before and after the coarse correction
   int coarse_sweeps = 2;                    // symmetric sweeps on the coarsest level
   double coarse_scale = 0.5;                // coarse operator = scale * Galerkin crossing sums
+  bool bf16_levels = false;                 // V-cycle coefficients stored in BF16, FP32 accumulation
+  bool learned = false;  // flexible CG preconditioned by the runtime's trained network (fl_learn_*), V-cycle fallback
 };
 
 struct HipMacPressureDiagnostics3D {
@@ -96,6 +98,7 @@ struct HipMacPressureDiagnostics3D {
   double defect_l2 = 0.0;
   double defect_linf = 0.0;
   std::size_t restarts = 0;
+  std::size_t learned_iterations = 0;
 };
 
 struct HipMacPressureProjection3D {

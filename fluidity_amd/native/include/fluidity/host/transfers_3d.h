@@ -16,6 +16,14 @@
 
 namespace fluidity {
 
+// DepositMac3D (transfers_3d.cc:139-145): FP64 sums, formed in parallel, so they agree with the
+// reference's particle-order sums to rounding (1e-12 relative) rather than bit for bit.
+Status DepositMac3D(const SparseMacGrid3D& mac_grid, std::span<const MacTransferParticle3D> particles,
+                    std::span<MacFaceDeposit3D> face_deposits);
+Status HipDepositMac3D(HipOperatorRuntime3D& runtime, const SparseMacGrid3D& mac_grid,
+                       std::span<const MacTransferParticle3D> particles, std::span<MacFaceDeposit3D> face_deposits,
+                       HipOperatorMeasurement& measurement);
+
 // DepositMacQuanta3D (transfers_3d.cc:147-197): per contribution rounded half to even to int64
 // quanta, then summed; identical bits to the reference for any input.
 Status DepositMacQuanta3D(const SparseMacGrid3D& mac_grid, std::span<const MacTransferParticle3D> particles,

@@ -18,6 +18,7 @@ static_assert(sizeof(Vec3) == sizeof(fl_vec3) && sizeof(Mat3) == sizeof(fl_mat3)
 static_assert(sizeof(BlockIndex3D) == sizeof(fl_int3) && sizeof(CellIndex3D) == sizeof(fl_int3));
 static_assert(sizeof(MacTransferParticle3D) == sizeof(fl_particle) && sizeof(MacTransferParticle3D) == 152);
 static_assert(sizeof(MacFaceDepositQuanta3D) == sizeof(fl_face_quanta));
+static_assert(sizeof(MacFaceDeposit3D) == sizeof(fl_face_deposit));
 static_assert(sizeof(MacParticleGatherResult3D) == sizeof(fl_gather_result));
 static_assert(sizeof(LiquidParticleSortEntry3D) == sizeof(fl_sort_entry) &&
               offsetof(LiquidParticleSortEntry3D, original_ordinal) == offsetof(fl_sort_entry, original_ordinal));

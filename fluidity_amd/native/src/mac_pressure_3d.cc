@@ -66,7 +66,8 @@ int Project(HipOperatorRuntime3D& rt, const MacPressureInputs3D& in, MacPressure
   o.levels = settings.levels;
   o.smoothing_sweeps = settings.smoothing_sweeps;
   o.coarse_sweeps = settings.coarse_sweeps;
-  o.fp32_vcycle = 1;
+  o.level_storage = settings.bf16_levels ? 2 : 1;
+  o.preconditioner = settings.learned ? 1 : 0;
   o.coarse_scale = settings.coarse_scale;
   MacPressureProjection3D& p = out.projection;
   p.cell_pressures.assign(nc, 0.0);
@@ -87,7 +88,8 @@ int Project(HipOperatorRuntime3D& rt, const MacPressureInputs3D& in, MacPressure
   p.gauge_cell_ordinals.assign(gauges.begin(), gauges.begin() + static_cast<std::ptrdiff_t>(d.gauge_count));
   p.diagnostics = {d.iteration_count, d.multigrid_level_count, d.true_residual_l2, d.max_volume_scaled_residual,
                    d.max_cell_constraint_defect};
-  out.device_diagnostics = {d.rhs_l2, d.initial_defect_l2, d.defect_l2, d.defect_linf, d.restarts};
+  out.device_diagnostics = {d.rhs_l2, d.initial_defect_l2, d.defect_l2, d.defect_linf, d.restarts,
+                            d.learned_iterations};
   std::memcpy(out.measurement.architecture, m.architecture, sizeof(out.measurement.architecture));
   out.measurement.kernel_ms = m.kernel_ms;
   out.measurement.copy_ms = m.copy_ms;

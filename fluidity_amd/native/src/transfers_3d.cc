@@ -45,6 +45,14 @@ int Deposit(HipOperatorRuntime3D& rt, const SparseMacGrid3D& grid, std::span<con
                        reinterpret_cast<fl_face_quanta*>(out.data()), out.size(), m);
 }
 
+int DepositF64(HipOperatorRuntime3D& rt, const SparseMacGrid3D& grid, std::span<const MacTransferParticle3D> particles,
+               std::span<MacFaceDeposit3D> out, fl_measurement* m) {
+  const fl_geometry g = internal::ToC(grid.geometry());
+  return fl_p2g(rt.handle(), &g, internal::ToC(grid.active_block_indices()), grid.active_block_indices().size(),
+                reinterpret_cast<const fl_particle*>(particles.data()), particles.size(),
+                reinterpret_cast<fl_face_deposit*>(out.data()), out.size(), m);
+}
+
 int Gather(HipOperatorRuntime3D& rt, const SparseMacGrid3D& grid, std::span<const MacTransferParticle3D> particles,
            std::span<const double> u_old, std::span<const double> u_new, std::span<MacParticleGatherResult3D> out,
            fl_measurement* m) {
@@ -71,6 +79,21 @@ Status DepositMacQuanta3D(const SparseMacGrid3D& grid, std::span<const MacTransf
   if (q.declared_particle_capacity > 0xffffffffULL) return Status(StatusCode::kResourceExhausted);
   return OnShared(TransferNeed(grid, std::max(particles.size(), q.declared_particle_capacity)),
                   [&](HipOperatorRuntime3D& rt) { return Deposit(rt, grid, particles, q, out, nullptr); });
+}
+
+Status DepositMac3D(const SparseMacGrid3D& grid, std::span<const MacTransferParticle3D> particles,
+                    std::span<MacFaceDeposit3D> out) {
+  return OnShared(TransferNeed(grid, particles.size()),
+                  [&](HipOperatorRuntime3D& rt) { return DepositF64(rt, grid, particles, out, nullptr); });
+}
+
+Status HipDepositMac3D(HipOperatorRuntime3D& rt, const SparseMacGrid3D& grid,
+                       std::span<const MacTransferParticle3D> particles, std::span<MacFaceDeposit3D> out,
+                       HipOperatorMeasurement& measurement) {
+  fl_measurement m{};
+  const int code = DepositF64(rt, grid, particles, out, &m);
+  if (code == FL_OK) Publish(m, measurement);
+  return FromCode(code);
 }
 
 Status GatherMac3D(const SparseMacGrid3D& grid, std::span<const MacTransferParticle3D> particles,

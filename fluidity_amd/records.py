@@ -16,6 +16,7 @@ PARTICLE = np.dtype([("sorted_world_position", VEC3), ("trial_world_position", V
                      ("particle_velocity", VEC3), ("particle_velocity_slopes", MAT3)])
 QUANTIZATION = np.dtype([("mass_per_quantum", "f8"), ("pure_momentum_per_quantum", "f8"),
                          ("declared_particle_capacity", "u8")])
+FACE_DEPOSIT = np.dtype([("deposited_mass", "f8"), ("deposited_pure_momentum", "f8")])
 FACE_QUANTA = np.dtype([("mass_quanta_sum", "i8"), ("pure_momentum_quanta_sum", "i8"),
                         ("stress_force_quanta_sum", "i8")])
 GATHER_RESULT = np.dtype([("new_pic_velocity", VEC3), ("grid_velocity_increment", VEC3),
@@ -29,12 +30,13 @@ PRESSURE_FACE = np.dtype([("axis", "i4"), ("lattice", INT3), ("open_area", "f8")
                           ("lift", "f8"), ("prescribed_wall_flux", "f8")])
 PRESSURE_OPTIONS = np.dtype([("max_iterations", "u8"), ("cpu_residual_tolerance_scale", "f8"),
                              ("relative_divergence_target", "f8"), ("levels", "i4"), ("smoothing_sweeps", "i4"),
-                             ("coarse_sweeps", "i4"), ("fp32_vcycle", "i4"), ("coarse_scale", "f8")])
+                             ("coarse_sweeps", "i4"), ("level_storage", "i4"), ("coarse_scale", "f8"),
+                             ("preconditioner", "i4"), ("pad", "i4")])
 PRESSURE_DIAGNOSTICS = np.dtype([("iteration_count", "u8"), ("multigrid_level_count", "u8"),
                                  ("true_residual_l2", "f8"), ("max_volume_scaled_residual", "f8"),
                                  ("max_cell_constraint_defect", "f8"), ("rhs_l2", "f8"),
                                  ("initial_defect_l2", "f8"), ("defect_l2", "f8"), ("defect_linf", "f8"),
-                                 ("restarts", "u8"), ("gauge_count", "u8")])
+                                 ("restarts", "u8"), ("gauge_count", "u8"), ("learned_iterations", "u8")])
 
 SEALED, INTERNAL, DIRICHLET = 0, 1, 2
 LINK_KINDS = {"sealed": SEALED, "internal": INTERNAL, "dirichlet": DIRICHLET}

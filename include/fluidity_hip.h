@@ -54,6 +54,10 @@ typedef struct fl_quantization {
   double pure_momentum_per_quantum;
   uint64_t declared_particle_capacity;
 } fl_quantization;
+typedef struct fl_face_deposit {
+  double deposited_mass;
+  double deposited_pure_momentum;
+} fl_face_deposit;
 typedef struct fl_face_quanta {
   int64_t mass_quanta_sum;
   int64_t pure_momentum_quanta_sum;
@@ -98,8 +102,12 @@ typedef struct fl_pressure_options {
   int32_t levels;                      /* geometric multigrid levels, >= 1 */
   int32_t smoothing_sweeps;            /* red-black Gauss-Seidel sweeps before and after the coarse correction */
   int32_t coarse_sweeps;               /* symmetric sweeps on the coarsest level */
-  int32_t fp32_vcycle;                 /* must be 1: V-cycle and CG vectors in FP32, dots and residuals in FP64 */
+  int32_t level_storage;               /* V-cycle coefficients: 1 FP32, 2 BF16 (FP32 accumulation); CG vectors
+                                          are FP32, dots and residuals FP64 either way */
   double coarse_scale;                 /* coarse operator = scale * Galerkin crossing sums */
+  int32_t preconditioner;              /* 0 multigrid V-cycle (PCG); 1 learned network (flexible CG), falling back
+                                          to the V-cycle after 3 stagnating iterations (needs fl_learn_setup) */
+  int32_t pad;
 } fl_pressure_options;
 
 typedef struct fl_pressure_diagnostics {
@@ -114,6 +122,7 @@ typedef struct fl_pressure_diagnostics {
   double defect_linf;
   uint64_t restarts;
   uint64_t gauge_count;
+  uint64_t learned_iterations; /* iterations preconditioned by the learned network */
 } fl_pressure_diagnostics;
 
 /* Device time of the call's kernels (hipEvents on the context stream), like the reference's
@@ -155,6 +164,12 @@ int fl_activate(fl_ctx* ctx, const fl_geometry* geometry, const fl_vec3* positio
                 fl_measurement* measurement);
 
 /* ---- transfers ---- */
+/* DepositMac3D (native/include/fluidity/host/transfers_3d.h:24-28; native/src/transfers_3d.cc:139-145):
+ * FP64 mass and momentum sums per owned slot. The sums are formed in parallel, so they match the
+ * reference's particle-order sums to rounding (1e-12 relative), not bit for bit. */
+int fl_p2g(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
+           const fl_particle* particles, size_t n, fl_face_deposit* out, size_t face_count,
+           fl_measurement* measurement);
 /* DepositMacQuanta3D (native/include/fluidity/host/transfers_3d.h:36-41; native/src/transfers_3d.cc:147-197).
  * blocks: canonical sorted active blocks (SparseMacGrid3D); out has 192 * block_count records in
  * owned-slot order. The reference's 2^20 particle cap is replaced by the context capacity. */
@@ -214,6 +229,29 @@ int fl_step(fl_ctx* ctx, double dt, fl_vec3 gravity, fl_quantization quantizatio
 int fl_scene_download(fl_ctx* ctx, fl_int3* blocks, size_t block_capacity, fl_face_quanta* quanta, double* u0,
                       double* ustar, double* unew, fl_sort_entry* order, fl_gather_result* gather,
                       double* pressures);
+
+/* ---- learned preconditioner (no counterpart in the reference) ----
+ * A 3-level 3D U-Net (channels 8/16/32, 3x3x3 convolutions) on 8^3 tiles of the level-0 pressure grid
+ * maps (residual / its RMS, liquid, air, solid) to a correction; with fl_pressure_options.preconditioner
+ * = 1 it preconditions a flexible CG that falls back to the V-cycle after 3 stagnating iterations.
+ * Parameters: per layer, weights [cout][cin][27] then biases [cout], FP32. Tiles are [n][4][512]
+ * inputs and [n][512] outputs/targets, voxel (z*8 + y)*8 + x. fl_learn_setup allocates the network's
+ * device memory for tile_capacity tiles and pair_capacity training pairs. */
+size_t fl_learn_param_count(void);
+int fl_learn_setup(fl_ctx* ctx, uint64_t tile_capacity, uint64_t pair_capacity);
+int fl_learn_set_params(fl_ctx* ctx, const float* params, size_t count);
+int fl_learn_get_params(fl_ctx* ctx, float* out, size_t count);
+int fl_learn_forward(fl_ctx* ctx, const float* inputs, size_t n, float* outputs);
+/* Masked mean-squared loss (mask = the liquid channel) and its gradients; either output may be null. */
+int fl_learn_loss_grad(fl_ctx* ctx, const float* inputs, const float* targets, size_t n, float* loss,
+                       float* grad_params, float* grad_inputs);
+int fl_learn_add_pairs(fl_ctx* ctx, const float* inputs, const float* targets, size_t n);
+int fl_learn_pairs(fl_ctx* ctx, size_t* count);
+/* every > 0: each V-cycle-preconditioned iteration number divisible by `every` appends one
+ * (residual, correction) pair per tile of the system to the training set; clear empties it. */
+int fl_learn_collect(fl_ctx* ctx, int every, int clear);
+/* Adam over the training set, seeded shuffling; epoch_losses (may be null) receives the mean loss. */
+int fl_learn_train(fl_ctx* ctx, int epochs, size_t batch, float learning_rate, uint64_t seed, float* epoch_losses);
 
 #ifdef __cplusplus
 }

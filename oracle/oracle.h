@@ -140,4 +140,13 @@ int flo_evaluate(const flo_geometry* geometry, const flo_int3* sorted_blocks, si
                  const double* pressures, double* out_velocities, double* out_residuals, double* out_defects,
                  flo_pressure_diagnostics* out_diagnostics);
 
+// ---- learned preconditioner network (unet.cc): 3-level U-Net on 8^3 tiles, FP64 ----
+// Parameters: per convolution layer, weights [cout][cin][27] then biases [cout]. Inputs [n][4][512]
+// (normalised residual, liquid, air, solid), outputs [n][512].
+size_t flo_unet_param_count(void);
+int flo_unet_forward(const double* params, size_t n, const double* in, double* out);
+// Masked mean-squared loss (mask = input channel 1) and its gradient; grad_in may be null.
+int flo_unet_loss_grad(const double* params, size_t n, const double* in, const double* target, double* loss,
+                       double* grad_params, double* grad_in);
+
 }  // extern "C"

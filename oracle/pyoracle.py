@@ -241,3 +241,44 @@ def evaluate(geom, blocks, dt, cells, faces, pressures):
     s = lib().flo_evaluate(_p(geom), _p(blocks), C.c_size_t(len(blocks)), C.c_double(dt), _p(cells), C.c_size_t(nc),
                            _p(faces), C.c_size_t(nf), _p(pressures), _p(u), _p(r), _p(d), _p(diag))
     return s, dict(velocities=u, residuals=r, defects=d, diagnostics=diag[0])
+
+
+# ---- learned preconditioner network (unet.cc) ----
+def unet_param_count():
+    lib().flo_unet_param_count.restype = C.c_size_t
+    return int(lib().flo_unet_param_count())
+
+
+def unet_forward(params, inputs):
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    inputs = np.ascontiguousarray(inputs, dtype=np.float64).reshape(-1, 4, 512)
+    out = np.zeros((len(inputs), 512))
+    s = lib().flo_unet_forward(_p(params), C.c_size_t(len(inputs)), _p(inputs), _p(out))
+    assert s == OK
+    return out
+
+
+def unet_loss_grad(params, inputs, targets, want_input_grad=False):
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    inputs = np.ascontiguousarray(inputs, dtype=np.float64).reshape(-1, 4, 512)
+    targets = np.ascontiguousarray(targets, dtype=np.float64).reshape(-1, 512)
+    loss = C.c_double()
+    gp = np.zeros(len(params))
+    gi = np.zeros(inputs.shape) if want_input_grad else None
+    s = lib().flo_unet_loss_grad(_p(params), C.c_size_t(len(inputs)), _p(inputs), _p(targets), C.byref(loss), _p(gp),
+                                  _p(gi))
+    assert s == OK
+    return loss.value, gp, gi
+
+
+def unet_init(seed=0):
+    """He-normal weights, zero biases, in the parameter layout of unet.cc."""
+    layers = [(4, 8), (8, 8), (8, 16), (16, 16), (16, 32), (32, 32), (48, 16), (16, 16), (24, 8), (8, 1)]
+    rng = np.random.default_rng(seed)
+    parts = []
+    for cin, cout in layers:
+        parts.append(rng.normal(0.0, np.sqrt(2.0 / (27 * cin)), cout * cin * 27))
+        parts.append(np.zeros(cout))
+    p = np.concatenate(parts)
+    assert len(p) == unet_param_count()
+    return p

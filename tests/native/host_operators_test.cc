@@ -160,6 +160,26 @@ void DepositMatchesOracleBitExact() {
   EXPECT(std::memcmp(got.data(), want.data(), got.size() * sizeof(got[0])) == 0);
 }
 
+void Fp64DepositMatchesOracle() {
+  auto grid = SparseMacGrid3D::Create(kGeometry, DenseBlocks(4, 4, 4), 64);
+  EXPECT(grid.ok());
+  const auto particles = BoxParticles(1, 14, 31, 2.0);
+  std::vector<MacFaceDeposit3D> got(grid->face_count()), want(grid->face_count());
+  EXPECT(DepositMac3D(*grid, particles, got).ok());
+  const auto blocks = grid->active_block_indices();
+  EXPECT(flo_deposit(O(&kGeometry), O(blocks), blocks.size(), reinterpret_cast<const flo_particle*>(particles.data()),
+                     particles.size(), reinterpret_cast<flo_face_deposit*>(want.data()), want.size()) == 0);
+  double mscale = 0, qscale = 0;
+  for (const auto& w : want) {
+    mscale = std::max(mscale, std::fabs(w.deposited_mass));
+    qscale = std::max(qscale, std::fabs(w.deposited_pure_momentum));
+  }
+  for (std::size_t i = 0; i < got.size(); ++i) {
+    EXPECT(std::fabs(got[i].deposited_mass - want[i].deposited_mass) <= 1e-12 * mscale);
+    EXPECT(std::fabs(got[i].deposited_pure_momentum - want[i].deposited_pure_momentum) <= 1e-12 * qscale);
+  }
+}
+
 void DepositFailuresPreserveOutput() {
   auto grid = SparseMacGrid3D::Create(kGeometry, DenseBlocks(4, 4, 4), 64);
   EXPECT(grid.ok());
@@ -383,6 +403,7 @@ int main(int argc, char** argv) {
   } else {
     failed += Run("DepositMatchesOracleBitExact", DepositMatchesOracleBitExact);
     failed += Run("DepositFailuresPreserveOutput", DepositFailuresPreserveOutput);
+    failed += Run("Fp64DepositMatchesOracle", Fp64DepositMatchesOracle);
     failed += Run("GatherMatchesOracle", GatherMatchesOracle);
     failed += Run("BackendStagesMatchOracle", BackendStagesMatchOracle);
     failed += Run("ProjectionMeetsGatesAndMatchesOracle", ProjectionMeetsGatesAndMatchesOracle);

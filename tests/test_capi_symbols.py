@@ -10,7 +10,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "fluidity_hip.h")).read()
-    return sorted(set(re.findall(r"^(?:int|void)\s+(fl_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^(?:int|void|size_t)\s+(fl_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_library_exports_every_declared_entry_point():
@@ -28,7 +28,7 @@ def test_record_layouts_match_the_header():
     assert R.PARTICLE.itemsize == 152 and R.GATHER_RESULT.itemsize == 192
     assert R.FACE_QUANTA.itemsize == 24 and R.SORT_ENTRY.itemsize == 24
     assert R.PRESSURE_CELL.itemsize == 56 and R.PRESSURE_FACE.itemsize == 80
-    assert R.PRESSURE_OPTIONS.itemsize == 48 and R.PRESSURE_DIAGNOSTICS.itemsize == 88
+    assert R.PRESSURE_OPTIONS.itemsize == 56 and R.PRESSURE_DIAGNOSTICS.itemsize == 96
 
 
 def test_create_without_device_fails_cleanly():

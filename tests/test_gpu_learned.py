@@ -1,0 +1,134 @@
+"""GPU learned preconditioner: network parity with the FP64 oracle, training, and the flexible CG.
+
+Bars: the FP32 forward pass within 1e-4 of the FP64 oracle's (relative to the output range) for the
+same FP32-representable parameters; the loss within 1e-4 relative and every parameter gradient within
+2e-3 of the largest gradient of its layer (FP32 accumulation over up to 27 x 48 x 512 terms); training
+lowers the loss; a projection preconditioned by the trained network is accepted by the same FP64 gates
+and agrees with the FP64 oracle to 1e-5, and the trained network does more work per iteration than an
+untrained one (the A/B test).
+"""
+import numpy as np
+import pytest
+
+import cases as K
+import pyoracle as O
+from fluidity_amd import hip as H
+from fluidity_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+LAYERS = [(4, 8), (8, 8), (8, 16), (16, 16), (16, 32), (32, 32), (48, 16), (16, 16), (24, 8), (8, 1)]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = H.Context(particles=1 << 20, blocks=1 << 14, lookup_blocks=1 << 20, pressure_cells=1 << 18,
+                  pressure_faces=1 << 20)
+    print("device", c.device_name(), flush=True)
+    yield c
+    c.close()
+
+
+def random_tiles(n, seed):
+    rng = np.random.default_rng(seed)
+    x = np.zeros((n, 4, 512))
+    liquid = (rng.random((n, 512)) < 0.7).astype(float)
+    x[:, 0] = rng.normal(0.0, 1.0, (n, 512)) * liquid
+    x[:, 1] = liquid
+    solid = (1 - liquid) * (rng.random((n, 512)) < 0.3)
+    x[:, 3] = solid
+    x[:, 2] = 1 - liquid - solid
+    return x, rng.normal(0.0, 1.0, (n, 512))
+
+
+def layer_slices():
+    out, q = [], 0
+    for ci, co in LAYERS:
+        out.append(slice(q, q + co * ci * 27 + co))
+        q += co * ci * 27 + co
+    return out
+
+
+def test_network_forward_and_gradients_match_the_oracle(ctx):
+    net = H.Network(ctx, tiles=64, pairs=64)
+    assert net.count == O.unet_param_count()
+    p = O.unet_init(11).astype(np.float32)
+    p[np.arange(len(p)) % 97 == 0] += 0.05  # nonzero biases and offsets
+    assert net.set_params(p) == H.OK
+    assert np.array_equal(net.params(), p)
+    x, t = random_tiles(5, 12)
+    x32, t32 = x.astype(np.float32), t.astype(np.float32)
+    s, out = net.forward(x32)
+    assert s == H.OK
+    want = O.unet_forward(p.astype(np.float64), x32.astype(np.float64))
+    assert np.abs(out - want).max() <= 1e-4 * np.abs(want).max()
+    s, loss, gp, gi = net.loss_grad(x32, t32)
+    assert s == H.OK
+    wl, wg, wi = O.unet_loss_grad(p.astype(np.float64), x32.astype(np.float64), t32.astype(np.float64), True)
+    assert abs(loss - wl) <= 1e-4 * abs(wl)
+    for sl in layer_slices():
+        scale = np.abs(wg[sl]).max()
+        assert np.abs(gp[sl] - wg[sl]).max() <= 2e-3 * scale
+    assert np.abs(gi - wi).max() <= 2e-3 * np.abs(wi).max()
+
+
+def test_training_lowers_the_loss(ctx):
+    # Targets: a fixed linear smoothing of the residual channel, learnable by the network.
+    net = H.Network(ctx, tiles=256, pairs=1024)
+    assert net.set_params(O.unet_init(1).astype(np.float32)) == H.OK
+    x, _ = random_tiles(512, 2)
+    r = x[:, 0].reshape(-1, 8, 8, 8)
+    t = r.copy()
+    for a in (1, 2, 3):
+        t = t + 0.5 * (np.roll(r, 1, a) + np.roll(r, -1, a))
+    t = (t.reshape(-1, 512) * x[:, 1]).astype(np.float32)
+    assert net.add_pairs(x.astype(np.float32), t) == H.OK
+    assert net.pairs() == 512
+    s, losses = net.train(epochs=8, batch=128, lr=3e-3, seed=5)
+    assert s == H.OK
+    print("epoch losses", losses, flush=True)
+    assert np.all(np.isfinite(losses)) and losses[-1] < 0.5 * losses[0]
+
+
+def scene_problem(seed):
+    sc = scenes.dam_break(32, seed=seed, random_velocity=True)
+    st = K.oracle_step_inputs(sc)
+    return st, (st["geom"], st["blocks"], st["dt"], st["cells"], st["faces"])
+
+
+def test_learned_flexible_cg_meets_the_gates_and_beats_the_untrained_network(ctx):
+    net = H.Network(ctx, tiles=1024, pairs=40000)
+    assert net.set_params(O.unet_init(3).astype(np.float32)) == H.OK
+    # Untrained network first (A): same solver, same system.
+    st, args = scene_problem(21)
+    s, untrained = ctx.project(*args, H.pressure_options(relative_target=1e-6, levels=4, learned=True))
+    assert s == H.OK
+    # Training pairs from plain MGPCG solves of other seeds.
+    assert net.collect(1, clear=True) == H.OK
+    for seed in (1, 2, 3, 4, 5, 6):
+        _, a = scene_problem(seed)
+        assert ctx.project(*a, H.pressure_options(relative_target=1e-6, levels=4))[0] == H.OK
+    assert net.collect(0) == H.OK
+    print("pairs", net.pairs(), flush=True)
+    assert net.pairs() > 0
+    s, losses = net.train(epochs=12, batch=256, lr=2e-3, seed=9)
+    assert s == H.OK
+    print("losses", losses, flush=True)
+    assert losses[-1] < losses[0]
+    # Trained network (B).
+    s, trained = ctx.project(*args, H.pressure_options(relative_target=1e-6, levels=4, learned=True))
+    assert s == H.OK
+    s, plain = ctx.project(*args, H.pressure_options(relative_target=1e-6, levels=4))
+    assert s == H.OK
+    s, ref = O.project(*args, O.pressure_options(relative_target=1e-12, fp32=False, levels=4))
+    assert s == O.OK
+    for name, out in (("untrained", untrained), ("trained", trained), ("mgpcg", plain)):
+        d = out["diagnostics"]
+        p_err = np.abs(out["pressures"] - ref["pressures"]).max() / np.abs(ref["pressures"]).max()
+        print(name, "iterations", d["iteration_count"], "learned", d["learned_iterations"], "p_err", p_err, flush=True)
+        assert d["defect_l2"] <= 1e-6 * d["initial_defect_l2"] and p_err <= 1e-5
+    dt, du = trained["diagnostics"], untrained["diagnostics"]
+    assert dt["learned_iterations"] > 0
+    # A/B: the trained network carries the flexible CG at least as far before it hands over, and the
+    # whole solve needs no more iterations than with the untrained network.
+    assert int(dt["iteration_count"]) <= int(du["iteration_count"])

@@ -103,6 +103,26 @@ def test_scene_projection_matches_fp64_oracle(ctx, seed):
     assert abs(int(d["iteration_count"]) - int(fp32["diagnostics"]["iteration_count"])) <= 3
 
 
+def test_bf16_level_storage_meets_the_same_gates(ctx):
+    # BF16 V-cycle coefficients change the preconditioner only: the accepted projection satisfies the
+    # same FP64 gates and agrees with the FP64 oracle to the same tolerance.
+    sc = scenes.dam_break(32, seed=0)
+    st = K.oracle_step_inputs(sc)
+    args = (st["geom"], st["blocks"], st["dt"], st["cells"], st["faces"])
+    s, ref = O.project(*args, O.pressure_options(relative_target=1e-12, fp32=False, levels=4))
+    assert s == O.OK
+    s, f32 = ctx.project(*args, H.pressure_options(relative_target=1e-6, levels=4))
+    assert s == H.OK
+    s, b16 = ctx.project(*args, H.pressure_options(relative_target=1e-6, levels=4, bf16_levels=True))
+    assert s == H.OK
+    d = b16["diagnostics"]
+    print("iterations fp32", f32["diagnostics"]["iteration_count"], "bf16", d["iteration_count"], flush=True)
+    assert d["defect_l2"] <= 1e-6 * d["initial_defect_l2"]
+    p_err = np.abs(b16["pressures"] - ref["pressures"]).max() / np.abs(ref["pressures"]).max()
+    assert p_err <= 1e-5
+    assert int(d["iteration_count"]) <= int(f32["diagnostics"]["iteration_count"]) + 3
+
+
 def test_device_step_matches_oracle_pipeline(ctx):
     sc = scenes.dam_break(24, seed=2, random_velocity=True)
     st = K.oracle_step_inputs(sc)

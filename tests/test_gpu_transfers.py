@@ -196,6 +196,43 @@ def test_scene_gather_matches_oracle(ctx):
     assert ctx.g2p(st["geom"], st["blocks"], st["particles"], bad, new)[0] == H.INVALID_ARGUMENT
 
 
+def test_fp64_deposit_matches_oracle(ctx):
+    # One particle: every slot receives one contribution, so the sums are exact and bit-identical.
+    blocks = corner_blocks()
+    p = K.corner_particle()
+    s, want = O.deposit(UNIT, blocks, p)
+    assert s == O.OK
+    s, got = ctx.p2g(UNIT, blocks, p)
+    assert s == H.OK
+    assert np.array_equal(got.view(np.float64), want.view(np.float64))
+    # A scene: parallel FP64 sums agree with the particle-order sums to rounding.
+    sc = scenes.dam_break(24, seed=5, random_velocity=True)
+    st = K.oracle_step_inputs(sc)
+    s, want = O.deposit(st["geom"], st["blocks"], st["particles"])
+    assert s == O.OK
+    s, got = ctx.p2g(st["geom"], st["blocks"], st["particles"])
+    assert s == H.OK
+    a, b = got.view(np.float64).reshape(-1, 2), want.view(np.float64).reshape(-1, 2)
+    scale = np.abs(b).max(0)
+    assert np.all(np.abs(a - b) <= 1e-12 * np.abs(b) + 1e-14 * scale)
+    total = st["particles"]["particle_mass"].sum()
+    for axis in range(3):
+        m = a.reshape(-1, 3, 64, 2)[:, axis, :, 0].sum()
+        assert abs(m - total) <= 1e-12 * total
+    # Failures keep the output and report the reference's codes.
+    bad = st["particles"].copy()
+    bad[3]["particle_mass"] = -1.0
+    assert ctx.p2g(st["geom"], st["blocks"], bad)[0] == H.INVALID_ARGUMENT
+    bad = st["particles"].copy()
+    bad[5]["particle_mass"] = 1e300
+    bad[5]["particle_velocity_slopes"]["x_row"]["y"] = 1e300
+    out = np.zeros(192 * len(st["blocks"]), R.FACE_DEPOSIT)
+    out["deposited_mass"] = 3.0
+    assert ctx.p2g(st["geom"], st["blocks"], bad, out)[0] == H.OUT_OF_RANGE
+    assert O.deposit(st["geom"], st["blocks"], bad)[0] == O.OUT_OF_RANGE
+    assert np.all(out["deposited_mass"] == 3.0)
+
+
 def test_normalize_error_codes(ctx):
     q = np.zeros(192, R.FACE_QUANTA)
     q[5] = (3, 1, 0)
