@@ -61,80 +61,15 @@ def measured_traffic(kernel, workload):
 
 
 def copy_ceiling(nbytes=1 << 30, reps=5):
-    """Device-to-device copy of `nbytes` (a plain copy kernel): measured bandwidth in GB/s.
+    """Plain float4 device-to-device copy of `nbytes` (fl_probe_copy): bandwidth in GB/s.
 
     The same copy, of known size, calibrates the FETCH_SIZE / WRITE_SIZE counters.
     """
     import ctypes as C
 
-    hip = C.CDLL("libamdhip64.so")
-    src, dst = C.c_void_p(), C.c_void_p()
-    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
-    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
-    hip.hipMemset(src, 1, C.c_size_t(nbytes))
-    e0, e1 = C.c_void_p(), C.c_void_p()
-    hip.hipEventCreate(C.byref(e0))
-    hip.hipEventCreate(C.byref(e1))
-    hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3)  # warm up (hipMemcpyDeviceToDevice)
-    hip.hipDeviceSynchronize()
-    hip.hipEventRecord(e0, None)
-    for _ in range(reps):
-        hip.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), 3, None)
-    hip.hipEventRecord(e1, None)
-    hip.hipEventSynchronize(e1)
-    ms = C.c_float()
-    hip.hipEventElapsedTime(C.byref(ms), e0, e1)
-    hip.hipFree(src)
-    hip.hipFree(dst)
-    return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
-
-
-def cpu_baseline(scene, ctx, report, sample, threads=1):
-    """The oracle (C++ restatement of the reference operators) timed on this host.
-
-    Transfers are timed on `sample` particles of the scene and scaled to the full count; the
-    projection is timed on the full scene (every pressure cell). Returns particle-steps/s.
-    """
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as O
-
-    nb = int(report.blocks)
-    s, dl = ctx.scene_download(nb, 0, int(report.pressure_cells), what=("blocks", "u0", "ustar"))
-    assert s == H.OK
-    blocks = dl["blocks"].view(O.INT3)
-    geom = scene.geometry_record().view(O.GEOMETRY)
-    sel = np.arange(min(sample, scene.count))
-    parts = scene.particles_record(sel).view(O.PARTICLE)
-    t0 = time.perf_counter()
-    s, _ = O.deposit_quanta(geom, blocks, parts, QUANTUM, QUANTUM, len(parts), max_capacity=1 << 40, threads=threads)
-    t_p2g = time.perf_counter() - t0
-    assert s == O.OK, s
-    t0 = time.perf_counter()
-    s, _ = O.gather(geom, blocks, parts, dl["u0"], dl["ustar"], threads=threads)
-    t_g2p = time.perf_counter() - t0
-    assert s == O.OK, s
-    faces = scene.pressure_faces(dl["blocks"], dl["ustar"]).view(O.PRESSURE_FACE)
-    cells = scene.pressure_cells().view(O.PRESSURE_CELL)
-    t0 = time.perf_counter()
-    s, out = O.project(geom, blocks, scene.dt, cells, faces, O.pressure_options(relative_target=1e-6, levels=6, coarse_sweeps=2))
-    t_proj = time.perf_counter() - t0
-    assert s == O.OK, s
-    per_particle = (t_p2g + t_g2p) / len(parts)
-    step_s = per_particle * scene.count + t_proj
-    return {
-        "value": scene.count / step_s,
-        "unit": "particle-steps/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": (f"{scene.name}: P2G+G2P on {len(parts):,} of {scene.count:,} particles scaled to all, plus the "
-                   f"full projection of {len(cells):,} cells ({int(out['diagnostics']['iteration_count'])} MGPCG "
-                   f"iterations); oracle C++ FP64, {threads} thread"),
-        "p2g_s_per_particle": t_p2g / len(parts),
-        "g2p_s_per_particle": t_g2p / len(parts),
-        "projection_ms": t_proj * 1e3,
-        "cpu": platform.processor() or platform.machine(),
-        "nproc": os.cpu_count(),
-    }
+    gbs = C.c_double()
+    assert H.lib().fl_probe_copy(0, C.c_size_t(nbytes), reps, C.byref(gbs)) == H.OK
+    return gbs.value
 
 
 def learned_comparison(ctx, scene, dt, opts, args):
@@ -191,7 +126,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="two_spheres_256",
                     choices=["two_spheres_256", "dam_break_128", "dam_break_128_random", "dam_break_64"])
-    ap.add_argument("--levels", type=int, default=6)
+    ap.add_argument("--levels", type=int, default=5)
     ap.add_argument("--coarse-sweeps", type=int, default=2)
     ap.add_argument("--level-storage", choices=["fp32", "bf16"], default="bf16",
                     help="V-cycle coefficient storage (BF16 keeps FP32 accumulation)")

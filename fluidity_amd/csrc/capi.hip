@@ -591,3 +591,43 @@ int fl_project(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, 
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ measurement utility
+namespace {
+__global__ void k_copy_probe(const float4* src, float4* dst, uint64_t n) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" int fl_probe_copy(int device, size_t bytes, int reps, double* gbs) {
+  if (bytes < 16 || reps < 1 || !gbs) return FL_INVALID_ARGUMENT;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return FL_FAILED_PRECONDITION;
+  FL_HIP(hipSetDevice(device));
+  const uint64_t n = bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n * 16) != hipSuccess) return FL_RESOURCE_EXHAUSTED;
+  if (hipMalloc(&b, n * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return FL_RESOURCE_EXHAUSTED;
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipMemset(a, 0, n * 16);
+  const unsigned grid = 256 * 32;  // 32 workgroups per CU, grid-stride
+  k_copy_probe<<<grid, 256>>>(a, b, n);
+  (void)hipEventRecord(e0, nullptr);
+  for (int r = 0; r < reps; ++r) k_copy_probe<<<grid, 256>>>(a, b, n);
+  (void)hipEventRecord(e1, nullptr);
+  (void)hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  *gbs = 2.0 * double(n) * 16.0 * reps / (double(ms) * 1e-3) / 1e9;
+  return FL_OK;
+}

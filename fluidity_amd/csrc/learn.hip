@@ -294,6 +294,78 @@ __global__ void k_fill_f32(float* p, uint64_t n, float v) {
     p[i] = v;
 }
 
+// BF16 inference path for the 8^3 layers: an implicit GEMM per tile on the matrix cores.
+// Y[voxel][co] = sum_k X[voxel][k] W[k][co], k = ci*27 + tap, with X gathered from a zero-padded
+// 10^3 copy of the tile's input channels in LDS and W staged as [k][16] (co padded to 16); both are
+// rounded to BF16, products accumulate in FP32 (v_mfma_f32_16x16x16bf16_1k). One workgroup of four
+// waves per tile; each wave owns 8 of the 32 row blocks of 16 voxels.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline uint16_t bf16_bits(float v) {
+  const uint32_t u = __float_as_uint(v);
+  return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);  // nearest even (finite v)
+}
+
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_conv8_mfma(float* act, LayerSpec L, const float* w, const float* bias) {
+  constexpr int K = CIN * 27, KP = (K + 15) / 16 * 16;
+  __shared__ uint16_t xs[CIN * 1000];
+  __shared__ uint16_t ws[KP * 16];
+  const uint64_t t = blockIdx.x;
+  float* base = act + t * kActFloats;
+  for (int idx = threadIdx.x; idx < KP * 16; idx += 256) {
+    const int k = idx >> 4, n = idx & 15;
+    float v = 0.f;
+    if (k < K && n < COUT) v = w[(n * CIN + k / 27) * 27 + k % 27];
+    ws[idx] = bf16_bits(v);
+  }
+  for (int idx = threadIdx.x; idx < CIN * 1000; idx += 256) {
+    const int ci = idx / 1000, p = idx % 1000;
+    const int pz = p / 100, py = (p / 10) % 10, px = p % 10;
+    float v = 0.f;
+    if (pz >= 1 && pz <= 8 && py >= 1 && py <= 8 && px >= 1 && px <= 8) {
+      const float* src = ci < L.ca ? base + L.a + ci * 512 : base + L.b + (ci - L.ca) * 512;
+      v = src[((pz - 1) * 8 + (py - 1)) * 8 + (px - 1)];
+    }
+    xs[idx] = bf16_bits(v);
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, kq = lane >> 4;
+  for (int mt = wave; mt < 32; mt += 4) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int m = mt * 16 + r;  // the voxel of this lane's A row
+    const int mz = m >> 6, my = (m >> 3) & 7, mx = m & 7;
+    for (int k0 = 0; k0 < KP; k0 += 16) {
+      bf16x4 a, b;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int k = k0 + kq * 4 + e;
+        uint16_t av = 0;
+        if (k < K) {
+          const int ci = k / 27, tp = k % 27;
+          const int dz = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dx = tp % 3 - 1;
+          av = xs[ci * 1000 + ((mz + 1 + dz) * 10 + (my + 1 + dy)) * 10 + (mx + 1 + dx)];
+        }
+        a[e] = static_cast<short>(av);
+        b[e] = static_cast<short>(ws[k * 16 + r]);  // B[k][column r]
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    // D[i][j] sits in lane j + 16 (i / 4), element i % 4: column r is the output channel.
+    if (r < COUT) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int voxel = mt * 16 + kq * 4 + e;
+        float v = acc[e] + bias[r];
+        if (L.relu && v < 0.f) v = 0.f;
+        base[L.y + r * 512 + voxel] = v;
+      }
+    }
+  }
+}
+
 template <int N>
 void conv_fwd_n(hipStream_t s, float* act, uint64_t nt, const LayerSpec& L, const float* w, const float* b) {
   const uint64_t lanes = nt * uint64_t(L.side * L.side * L.side);
@@ -305,10 +377,21 @@ void conv_bwd_n(hipStream_t s, float* grad, uint64_t nt, const LayerSpec& L, con
   k_conv_bwd_data<N><<<grid_for(lanes), kThreads, 0, s>>>(grad, nt, L, w);
 }
 
-int conv_fwd(hipStream_t s, float* act, uint64_t nt, int l, const float* params) {
+int conv_fwd(hipStream_t s, float* act, uint64_t nt, int l, const float* params, bool mfma = false) {
   const LayerSpec& L = kLayers[l];
   const float* w = params + layer_offset(l);
   const float* b = w + layer_weights(l);
+  if (mfma && L.side == 8 && L.cout == 8) {
+    const unsigned g = static_cast<unsigned>(nt);
+    switch (L.ca + L.cb) {
+      case 4: k_conv8_mfma<4, 8><<<g, 256, 0, s>>>(act, L, w, b); break;
+      case 8: k_conv8_mfma<8, 8><<<g, 256, 0, s>>>(act, L, w, b); break;
+      case 24: k_conv8_mfma<24, 8><<<g, 256, 0, s>>>(act, L, w, b); break;
+      default: return FL_INTERNAL;
+    }
+    FL_HIP(hipGetLastError());
+    return FL_OK;
+  }
   switch (L.cout) {
     case 1: conv_fwd_n<1>(s, act, nt, L, w, b); break;
     case 8: conv_fwd_n<8>(s, act, nt, L, w, b); break;
@@ -340,10 +423,10 @@ int conv_bwd_data(hipStream_t s, float* grad, uint64_t nt, int l, const float* p
 
 uint64_t net_param_count() { return layer_offset(kNetLayers); }
 
-int net_forward(hipStream_t s, const float* params, float* act, uint64_t nt) {
+int net_forward(hipStream_t s, const float* params, float* act, uint64_t nt, bool mfma) {
   if (!nt) return FL_OK;
-  FL_TRY(conv_fwd(s, act, nt, 0, params));
-  FL_TRY(conv_fwd(s, act, nt, 1, params));
+  FL_TRY(conv_fwd(s, act, nt, 0, params, mfma));
+  FL_TRY(conv_fwd(s, act, nt, 1, params, mfma));
   k_pool_fwd<<<grid_for(nt * 8 * 64), kThreads, 0, s>>>(act, nt, 8, 4, kActA1, kActP1);
   FL_TRY(conv_fwd(s, act, nt, 2, params));
   FL_TRY(conv_fwd(s, act, nt, 3, params));
@@ -354,7 +437,7 @@ int net_forward(hipStream_t s, const float* params, float* act, uint64_t nt) {
   FL_TRY(conv_fwd(s, act, nt, 6, params));
   FL_TRY(conv_fwd(s, act, nt, 7, params));
   k_up_fwd<<<grid_for(nt * 16 * 512), kThreads, 0, s>>>(act, nt, 16, 4, kActA7, kActU7);
-  FL_TRY(conv_fwd(s, act, nt, 8, params));
+  FL_TRY(conv_fwd(s, act, nt, 8, params, mfma));
   FL_TRY(conv_fwd(s, act, nt, 9, params));
   FL_HIP(hipGetLastError());
   return FL_OK;
@@ -608,7 +691,7 @@ int fl_learn_get_params(fl_ctx* ctx, float* out, size_t count) {
   return FL_OK;
 }
 
-int fl_learn_forward(fl_ctx* ctx, const float* inputs, size_t n, float* outputs) {
+static int learn_forward(fl_ctx* ctx, const float* inputs, size_t n, float* outputs, bool mfma) {
   if (!ctx || !ctx->learn) return FL_FAILED_PRECONDITION;
   LearnState* L = static_cast<LearnState*>(ctx->learn);
   if (n > std::min<uint64_t>(L->tile_cap, 1024)) return FL_RESOURCE_EXHAUSTED;
@@ -616,11 +699,19 @@ int fl_learn_forward(fl_ctx* ctx, const float* inputs, size_t n, float* outputs)
   if (!n) return FL_OK;
   FL_HIP(hipMemcpyAsync(L->stage, inputs, n * 2048 * sizeof(float), hipMemcpyHostToDevice, s));
   fl::k_put_inputs<<<fl::grid_for(n * 2048), fl::kThreads, 0, s>>>(L->stage, n, L->act);
-  FL_TRY(fl::net_forward(s, L->params, L->act, n));
+  FL_TRY(fl::net_forward(s, L->params, L->act, n, mfma));
   fl::k_get_slot<<<fl::grid_for(n * 512), fl::kThreads, 0, s>>>(L->act, n, fl::kActOut, 512, L->stage);
   FL_HIP(hipMemcpyAsync(outputs, L->stage, n * 512 * sizeof(float), hipMemcpyDeviceToHost, s));
   FL_HIP(hipStreamSynchronize(s));
   return FL_OK;
+}
+
+int fl_learn_forward(fl_ctx* ctx, const float* inputs, size_t n, float* outputs) {
+  return learn_forward(ctx, inputs, n, outputs, false);
+}
+
+int fl_learn_infer(fl_ctx* ctx, const float* inputs, size_t n, float* outputs) {
+  return learn_forward(ctx, inputs, n, outputs, true);
 }
 
 int fl_learn_loss_grad(fl_ctx* ctx, const float* inputs, const float* targets, size_t n, float* loss,

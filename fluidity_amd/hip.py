@@ -51,8 +51,8 @@ class StepReport(C.Structure):
 
 EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_p2g", "fl_p2g_quanta",
            "fl_normalize_forces", "fl_apply_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
-           "fl_scene_download", "fl_learn_param_count", "fl_learn_setup", "fl_learn_set_params", "fl_learn_get_params",
-           "fl_learn_forward", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect",
+           "fl_scene_download", "fl_probe_copy", "fl_learn_param_count", "fl_learn_setup", "fl_learn_set_params", "fl_learn_get_params",
+           "fl_learn_forward", "fl_learn_infer", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect",
            "fl_learn_train")
 
 _lib = None
@@ -237,6 +237,12 @@ class Network:
         x = np.ascontiguousarray(inputs, dtype=np.float32).reshape(-1, 4, 512)
         out = np.zeros((len(x), 512), np.float32)
         s = lib().fl_learn_forward(self.ctx.handle, _p(x), C.c_size_t(len(x)), _p(out))
+        return s, out
+
+    def infer(self, inputs):
+        x = np.ascontiguousarray(inputs, dtype=np.float32).reshape(-1, 4, 512)
+        out = np.zeros((len(x), 512), np.float32)
+        s = lib().fl_learn_infer(self.ctx.handle, _p(x), C.c_size_t(len(x)), _p(out))
         return s, out
 
     def loss_grad(self, inputs, targets):

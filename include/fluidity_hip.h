@@ -15,8 +15,10 @@
  * (native/include/fluidity/host/mac_pressure_3d.h:14-51).
  *
  * Threading: a context is single-threaded and non-reentrant, like the reference workspaces
- * (native/include/fluidity/host/mac_pressure_workspace_3d.h:65). All device memory is
- * allocated by fl_create; calls allocate nothing on the device.
+ * (native/include/fluidity/host/mac_pressure_workspace_3d.h:65). Device memory: fl_create allocates
+ * the particle, grid and face arrays and the transfer calls allocate nothing; the projection allocates
+ * its arrays on first use and its multigrid levels when the system's block structure changes;
+ * fl_learn_setup allocates the network.
  */
 #ifndef FLUIDITY_HIP_H_
 #define FLUIDITY_HIP_H_
@@ -230,6 +232,12 @@ int fl_scene_download(fl_ctx* ctx, fl_int3* blocks, size_t block_capacity, fl_fa
                       double* ustar, double* unew, fl_sort_entry* order, fl_gather_result* gather,
                       double* pressures);
 
+/* ---- measurement utility (no counterpart in the reference) ----
+ * Bandwidth of a plain float4 device-to-device copy of `bytes` repeated `reps` times (read + write
+ * bytes over HIP-event time), the copy ceiling the HBM figures are compared with. Allocates and frees
+ * its two buffers. */
+int fl_probe_copy(int device, size_t bytes, int reps, double* gbs);
+
 /* ---- learned preconditioner (no counterpart in the reference) ----
  * A 3-level 3D U-Net (channels 8/16/32, 3x3x3 convolutions) on 8^3 tiles of the level-0 pressure grid
  * maps (residual / its RMS, liquid, air, solid) to a correction; with fl_pressure_options.preconditioner
@@ -241,7 +249,11 @@ size_t fl_learn_param_count(void);
 int fl_learn_setup(fl_ctx* ctx, uint64_t tile_capacity, uint64_t pair_capacity);
 int fl_learn_set_params(fl_ctx* ctx, const float* params, size_t count);
 int fl_learn_get_params(fl_ctx* ctx, float* out, size_t count);
+/* FP32 forward pass (the training precision). */
 int fl_learn_forward(fl_ctx* ctx, const float* inputs, size_t n, float* outputs);
+/* Inference precision used by the preconditioner: the 8^3 layers in BF16 on the matrix cores with
+ * FP32 accumulation, the rest FP32. */
+int fl_learn_infer(fl_ctx* ctx, const float* inputs, size_t n, float* outputs);
 /* Masked mean-squared loss (mask = the liquid channel) and its gradients; either output may be null. */
 int fl_learn_loss_grad(fl_ctx* ctx, const float* inputs, const float* targets, size_t n, float* loss,
                        float* grad_params, float* grad_inputs);

@@ -62,6 +62,10 @@ def test_network_forward_and_gradients_match_the_oracle(ctx):
     assert s == H.OK
     want = O.unet_forward(p.astype(np.float64), x32.astype(np.float64))
     assert np.abs(out - want).max() <= 1e-4 * np.abs(want).max()
+    # Inference precision: BF16 operands on the matrix cores for the 8^3 layers, FP32 accumulation.
+    s, fast = net.infer(x32)
+    assert s == H.OK
+    assert np.abs(fast - want).max() <= 3e-2 * np.abs(want).max()
     s, loss, gp, gi = net.loss_grad(x32, t32)
     assert s == H.OK
     wl, wg, wi = O.unet_loss_grad(p.astype(np.float64), x32.astype(np.float64), t32.astype(np.float64), True)
