@@ -622,6 +622,32 @@ int train_epoch(fl_ctx* ctx, uint64_t batch, float lr, uint64_t seed, float* mea
   return FL_OK;
 }
 
+// Mean loss over the training set, forward only (FP32), in order.
+int eval_pairs(fl_ctx* ctx, uint64_t batch, float* mean_loss) {
+  LearnState* L = state(ctx);
+  hipStream_t s = ctx->stream;
+  std::vector<int32_t> order(L->ndata);
+  for (uint64_t i = 0; i < L->ndata; ++i) order[i] = static_cast<int32_t>(i);
+  double total = 0.0;
+  uint64_t count = 0;
+  for (uint64_t b0 = 0; b0 < L->ndata; b0 += batch) {
+    const uint64_t nt = std::min<uint64_t>(batch, L->ndata - b0);
+    FL_HIP(hipMemcpyAsync(L->batch_index, order.data() + b0, nt * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    k_load_batch<<<grid_for(nt * 2560), kThreads, 0, s>>>(L->data_in, L->data_target, L->batch_index, nt, L->act,
+                                                          L->batch_target);
+    FL_TRY(net_forward(s, L->params, L->act, nt));
+    FL_TRY(net_clear(s, L->grad, nt * kActFloats));
+    FL_TRY(net_loss(s, L->act, L->batch_target, L->grad, nt, L->partial, L->scalars));
+    float h[2];
+    FL_HIP(hipMemcpyAsync(h, L->scalars, 2 * sizeof(float), hipMemcpyDeviceToHost, s));
+    FL_HIP(hipStreamSynchronize(s));
+    total += double(h[1]) * double(nt);
+    count += nt;
+  }
+  *mean_loss = count ? static_cast<float>(total / double(count)) : 0.f;
+  return FL_OK;
+}
+
 }  // namespace fl
 
 using fl::LearnState;
@@ -780,6 +806,15 @@ int fl_learn_train(fl_ctx* ctx, int epochs, size_t batch, float learning_rate, u
     if (epoch_losses) epoch_losses[e] = loss;
   }
   return FL_OK;
+}
+
+int fl_learn_eval(fl_ctx* ctx, size_t batch, float* mean_loss) {
+  if (!ctx || !ctx->learn || !mean_loss) return FL_FAILED_PRECONDITION;
+  LearnState* L = static_cast<LearnState*>(ctx->learn);
+  if (batch < 1) return FL_INVALID_ARGUMENT;
+  if (batch > std::min<uint64_t>(L->tile_cap, 1024)) return FL_RESOURCE_EXHAUSTED;
+  if (L->ndata == 0) return FL_FAILED_PRECONDITION;
+  return fl::eval_pairs(ctx, batch, mean_loss);
 }
 
 }  // extern "C"

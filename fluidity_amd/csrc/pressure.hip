@@ -29,7 +29,8 @@ namespace fl {
 namespace {
 
 constexpr double kEpsD = 2.220446049250313080847e-16;
-constexpr int kPartials = 2048;  // reduction partial slots
+constexpr int kPartials = 16384;  // reduction partial slots (one per workgroup of a reducing kernel)
+constexpr unsigned kStreamPartials = 2048;  // workgroups of the grid-stride streaming reductions
 
 struct Key3 {
   int32_t x, y, z;
@@ -372,7 +373,8 @@ __device__ inline void block_reduce(double v[4], double* partial, unsigned slot)
 
 // Evaluate (mac_pressure_workspace_3d.cc:399-461): FP64 residual of the original operator, physical
 // defect from corrected fluxes, and the identity |V defect + r| <= 64 eps (sum of absolute terms).
-// Cells are visited in block order.
+// Cells are visited in input order, one per lane (the per-cell work is a chain of dependent loads, so
+// the launch is as wide as the system).
 __global__ void k_evaluate(uint64_t nc, const int32_t* cell_faces, const FaceRec* faces,
                            const double* V, const double* target, const double* inwall, const double* comp,
                            const double* prev, const double* rhs, const double* rhs_abs, const double* p, double* res,
@@ -795,38 +797,37 @@ __device__ inline int64_t tile_cell(const int32_t* tile_blocks, uint64_t t, int 
 }
 
 // Network inputs of every tile from the level-0 residual: the residual divided by its RMS over the
-// tile's liquid cells (the scale is kept per tile), and one-hot liquid / air / solid classes. Cells
-// outside the level's blocks are air. One workgroup of 512 lanes per tile.
-__global__ __launch_bounds__(512) void k_tile_gather(const float* r, const uint8_t* cls, const int32_t* tile_blocks,
-                                                     float* act, float* scale) {
-  __shared__ float red[512];
+// tile's liquid cells, and one-hot liquid / air / solid classes (cells outside the level's blocks are
+// air). The output scale of the tile is RMS / (mean diagonal over its liquid cells), so the network maps
+// a dimensionless residual to a dimensionless correction. One workgroup of 512 lanes per tile.
+__global__ __launch_bounds__(512) void k_tile_gather(const float* r, const float* diag, const uint8_t* cls,
+                                                     const int32_t* tile_blocks, float* act, float* scale) {
+  __shared__ float red[3][512];
   const uint64_t t = blockIdx.x;
   const int v = threadIdx.x;
   const int64_t i = tile_cell(tile_blocks, t, v);
   const int c = i >= 0 ? cls[i] : 2;
   const float rv = (i >= 0 && c == 1) ? r[i] : 0.f;
-  red[v] = rv * rv;
+  red[0][v] = rv * rv;
+  red[1][v] = c == 1 ? 1.f : 0.f;
+  red[2][v] = (i >= 0 && c == 1) ? diag[i] : 0.f;
   __syncthreads();
   for (int s = 256; s > 0; s >>= 1) {
-    if (v < s) red[v] += red[v + s];
+    if (v < s) {
+      red[0][v] += red[0][v + s];
+      red[1][v] += red[1][v + s];
+      red[2][v] += red[2][v + s];
+    }
     __syncthreads();
   }
-  const float sum = red[0];
-  __syncthreads();
-  red[v] = c == 1 ? 1.f : 0.f;
-  __syncthreads();
-  for (int s = 256; s > 0; s >>= 1) {
-    if (v < s) red[v] += red[v + s];
-    __syncthreads();
-  }
-  const float count = red[0];
-  const float sc = (count > 0.f && sum > 0.f) ? sqrtf(sum / count) : 0.f;
+  const float sum = red[0][0], count = red[1][0], dsum = red[2][0];
+  const float rms = (count > 0.f && sum > 0.f) ? sqrtf(sum / count) : 0.f;
   float* in = act + t * kActFloats + kActIn;
-  in[v] = sc > 0.f ? rv / sc : 0.f;
+  in[v] = rms > 0.f ? rv / rms : 0.f;
   in[512 + v] = c == 1 ? 1.f : 0.f;
   in[1024 + v] = c == 2 ? 1.f : 0.f;
   in[1536 + v] = c == 3 ? 1.f : 0.f;
-  if (v == 0) scale[t] = sc;
+  if (v == 0) scale[t] = (rms > 0.f && dsum > 0.f) ? rms * count / dsum : 0.f;
 }
 
 // z = network output times the tile's scale on liquid unknowns, zero elsewhere. Tiles partition the
@@ -875,8 +876,7 @@ __device__ inline bool fine_value(double v, float& out) {
   return true;
 }
 
-__global__ void k_after_rho(double* scal, float* fscal, uint64_t* err, int restart) {
-  if (threadIdx.x) return;
+__device__ void after_rho(double* scal, float* fscal, uint64_t* err, int restart) {
   const double rho = scal[20];
   scal[0] = rho;
   if (!(rho > 0.0)) {
@@ -898,8 +898,7 @@ __global__ void k_after_rho(double* scal, float* fscal, uint64_t* err, int resta
 // Flexible CG (one-term orthogonalisation): beta = -(z . q_prev) / (d_prev . q_prev), with z . q_prev in
 // scal[28] and the previous curvature in scal[2]. The step length stays (r . z) / (d . q), which the
 // orthogonality of r to d_prev makes equal to (r . d) / (d . q); it may be of either sign.
-__global__ void k_after_rho_fcg(double* scal, float* fscal, uint64_t* err, int restart) {
-  if (threadIdx.x) return;
+__device__ void after_rho_fcg(double* scal, float* fscal, uint64_t* err, int restart) {
   const double rho = scal[20];
   scal[0] = rho;
   scal[14] = 0.0;
@@ -921,8 +920,7 @@ __global__ void k_after_rho_fcg(double* scal, float* fscal, uint64_t* err, int r
 
 // Flexible-CG curvature: a zero direction (a network output orthogonal to everything) is not an error
 // here; the step is skipped and scal[14] tells the host to hand over to the V-cycle.
-__global__ void k_after_curv_fcg(double* scal, float* fscal, uint64_t* err) {
-  if (threadIdx.x) return;
+__device__ void after_curv_fcg(double* scal, float* fscal, uint64_t* err) {
   const double curv = scal[20];
   float alpha = 0.f;
   if (!(curv > 0.0) || !fine_value(scal[0] / curv, alpha)) {
@@ -935,8 +933,7 @@ __global__ void k_after_curv_fcg(double* scal, float* fscal, uint64_t* err) {
   scal[1] = scal[0];
 }
 
-__global__ void k_after_curv(double* scal, float* fscal, uint64_t* err) {
-  if (threadIdx.x) return;
+__device__ void after_curv(double* scal, float* fscal, uint64_t* err) {
   const double curv = scal[20];
   scal[2] = curv;
   if (!(curv > 0.0)) {
@@ -950,6 +947,27 @@ __global__ void k_after_curv(double* scal, float* fscal, uint64_t* err) {
   }
   fscal[0] = alpha;
   scal[1] = scal[0];
+}
+
+// Fixed-order reduction of the dot-product partials into scal[20..23], then the scalar step of the
+// iteration on one lane: 1 rho (PCG), 2 rho (flexible CG), 3 curvature (PCG), 4 curvature (flexible CG).
+__global__ void k_finalize_step(const double* partial, unsigned slots, double* scal, float* fscal, uint64_t* err,
+                                int post, int restart) {
+  __shared__ double red[kThreads];
+  double a = 0.0;
+  for (unsigned i = threadIdx.x; i < slots; i += blockDim.x) a += partial[i];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x) return;
+  scal[20] = red[0];
+  if (post == 1) after_rho(scal, fscal, err, restart);
+  if (post == 2) after_rho_fcg(scal, fscal, err, restart);
+  if (post == 3) after_curv(scal, fscal, err);
+  if (post == 4) after_curv_fcg(scal, fscal, err);
 }
 
 __global__ void k_update_d(uint64_t n, const float* z, float* d, const float* fscal, int restart) {
@@ -1091,9 +1109,9 @@ int ensure_capacity(fl_ctx* ctx, uint64_t nc, uint64_t nf, uint64_t nfine) {
   return FL_OK;
 }
 
-unsigned reduce_grid(uint64_t n) {
+unsigned reduce_grid(uint64_t n, unsigned cap = kStreamPartials) {
   const uint64_t g = (n + kThreads - 1) / kThreads;
-  return static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(g, kPartials)));
+  return static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(g, cap)));
 }
 
 // Host-side structure for one pressure system (records in the reference's order).
@@ -1281,7 +1299,7 @@ struct EvalResult {
 
 int evaluate(fl_ctx* ctx, Pressure& P, const double* p, EvalResult& out) {
   hipStream_t s = ctx->stream;
-  const unsigned g = reduce_grid(P.nc);
+  const unsigned g = reduce_grid(P.nc, kPartials);
   k_evaluate<<<g, kThreads, 0, s>>>(P.nc, P.cell_faces, P.face_rec, P.cell_volume, P.cell_target,
                                     P.cell_inwall, P.cell_comp, P.cell_prev, P.rhs, P.rhs_abs, p, P.res, P.def,
                                     P.partial, P.perr);
@@ -1360,8 +1378,8 @@ int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates&
   diag.learned_iterations = 0;
   for (uint64_t it = 1; it <= opt.max_iterations; ++it) {
     if (use_net) {
-      k_tile_gather<<<static_cast<unsigned>(P.ntiles), 512, 0, s>>>(P.r32, P.cell_class, P.tile_blocks, net->act,
-                                                                    net->scale);
+      k_tile_gather<<<static_cast<unsigned>(P.ntiles), 512, 0, s>>>(P.r32, L0.diag, P.cell_class, P.tile_blocks,
+                                                                    net->act, net->scale);
       FL_TRY(net_forward(s, net->params, net->act, P.ntiles, true));
       k_tile_scatter<<<grid_for(P.ntiles * 512), kThreads, 0, s>>>(net->act, net->scale, P.cell_class, P.tile_blocks,
                                                                     P.ntiles, L0.x);
@@ -1374,33 +1392,24 @@ int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates&
     if (!use_net && net && net->collect_every > 0 && it % uint64_t(net->collect_every) == 0 &&
         P.ntiles <= net->tile_cap && net->ndata + P.ntiles <= net->data_cap) {
       // Training pairs: this iteration's residual and its V-cycle correction, per tile.
-      k_tile_gather<<<static_cast<unsigned>(P.ntiles), 512, 0, s>>>(P.r32, P.cell_class, P.tile_blocks, net->act,
-                                                                    net->scale);
+      k_tile_gather<<<static_cast<unsigned>(P.ntiles), 512, 0, s>>>(P.r32, L0.diag, P.cell_class, P.tile_blocks,
+                                                                    net->act, net->scale);
       k_tile_pairs<<<grid_for(P.ntiles * 512), kThreads, 0, s>>>(net->act, net->scale, L0.x, P.cell_class,
                                                                   P.tile_blocks, P.ntiles,
                                                                   net->data_in + net->ndata * 2048,
                                                                   net->data_target + net->ndata * 512);
       net->ndata += P.ntiles;
     }
-    k_dot<<<rg, kThreads, 0, s>>>(n, P.r32, L0.x, P.partial);
-    k_finalize4<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, 20);
-    if (use_net) {
-      if (!restart) {
-        k_dot<<<rg, kThreads, 0, s>>>(n, L0.x, P.q32, P.partial);
-        k_finalize4<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, 28);
-      }
-      k_after_rho_fcg<<<1, 64, 0, s>>>(P.scal, P.fscal, P.perr, restart ? 1 : 0);
-    } else {
-      k_after_rho<<<1, 64, 0, s>>>(P.scal, P.fscal, P.perr, restart ? 1 : 0);
+    if (use_net && !restart) {  // z . q_prev for the flexible-CG orthogonalisation, before q is overwritten
+      k_dot<<<rg, kThreads, 0, s>>>(n, L0.x, P.q32, P.partial);
+      k_finalize4<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, 28);
     }
+    k_dot<<<rg, kThreads, 0, s>>>(n, P.r32, L0.x, P.partial);
+    k_finalize_step<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, P.fscal, P.perr, use_net ? 2 : 1, restart ? 1 : 0);
     k_update_d<<<g, kThreads, 0, s>>>(n, L0.x, P.d32, P.fscal, restart ? 1 : 0);
     k_apply<<<g, kThreads, 0, s>>>(n, L0.nbr, L0.k, L0.anchor, L0.diag, P.d32, P.q32);
     k_dot<<<rg, kThreads, 0, s>>>(n, P.d32, P.q32, P.partial);
-    k_finalize4<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, 20);
-    if (use_net)
-      k_after_curv_fcg<<<1, 64, 0, s>>>(P.scal, P.fscal, P.perr);
-    else
-      k_after_curv<<<1, 64, 0, s>>>(P.scal, P.fscal, P.perr);
+    k_finalize_step<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, P.fscal, P.perr, use_net ? 4 : 3, 0);
     k_update_xr<<<rg, kThreads, 0, s>>>(n, P.d32, P.q32, L0.diag, P.fine_vol, P.x32, P.r32, P.fscal, P.partial,
                                         P.perr);
     k_finalize4<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, 8);

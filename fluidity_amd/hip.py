@@ -52,7 +52,7 @@ class StepReport(C.Structure):
 EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_p2g", "fl_p2g_quanta",
            "fl_normalize_forces", "fl_apply_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
            "fl_scene_download", "fl_probe_copy", "fl_learn_param_count", "fl_learn_setup", "fl_learn_set_params", "fl_learn_get_params",
-           "fl_learn_forward", "fl_learn_infer", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect",
+           "fl_learn_forward", "fl_learn_infer", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect", "fl_learn_eval",
            "fl_learn_train")
 
 _lib = None
@@ -266,6 +266,11 @@ class Network:
 
     def collect(self, every, clear=False):
         return lib().fl_learn_collect(self.ctx.handle, C.c_int(every), C.c_int(1 if clear else 0))
+
+    def eval(self, batch=256):
+        loss = C.c_float()
+        s = lib().fl_learn_eval(self.ctx.handle, C.c_size_t(batch), C.byref(loss))
+        return s, loss.value
 
     def train(self, epochs, batch=256, lr=1e-3, seed=0):
         losses = np.zeros(epochs, np.float32)

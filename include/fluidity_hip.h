@@ -262,6 +262,8 @@ int fl_learn_pairs(fl_ctx* ctx, size_t* count);
 /* every > 0: each V-cycle-preconditioned iteration number divisible by `every` appends one
  * (residual, correction) pair per tile of the system to the training set; clear empties it. */
 int fl_learn_collect(fl_ctx* ctx, int every, int clear);
+/* Mean masked loss of the current parameters over the stored pairs (forward only). */
+int fl_learn_eval(fl_ctx* ctx, size_t batch, float* mean_loss);
 /* Adam over the training set, seeded shuffling; epoch_losses (may be null) receives the mean loss. */
 int fl_learn_train(fl_ctx* ctx, int epochs, size_t batch, float learning_rate, uint64_t seed, float* epoch_losses);
 
