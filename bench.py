@@ -72,6 +72,68 @@ def copy_ceiling(nbytes=1 << 30, reps=5):
     return gbs.value
 
 
+def cpu_baseline(scene, ctx, report, sample, threads=1):
+    """The oracle (C++ restatement of the reference operators) timed on this host.
+
+    Transfers are timed on `sample` particles of the scene and scaled to the full count, on one thread
+    (the reference's semantics) and again on `threads` threads (per-thread integer accumulators, the
+    same bits); the projection is timed once on the full scene (every pressure cell, one thread: the
+    oracle's solver is serial). Returns particle-steps/s.
+    """
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+
+    nb = int(report.blocks)
+    s, dl = ctx.scene_download(nb, 0, int(report.pressure_cells), what=("blocks", "u0", "ustar"))
+    assert s == H.OK
+    blocks = dl["blocks"].view(O.INT3)
+    geom = scene.geometry_record().view(O.GEOMETRY)
+    sel = np.arange(min(sample, scene.count))
+    parts = scene.particles_record(sel).view(O.PARTICLE)
+
+    def transfers(nthreads):
+        t0 = time.perf_counter()
+        s, _ = O.deposit_quanta(geom, blocks, parts, QUANTUM, QUANTUM, len(parts), max_capacity=1 << 40,
+                                threads=nthreads)
+        t_p2g = time.perf_counter() - t0
+        assert s == O.OK, s
+        t0 = time.perf_counter()
+        s, _ = O.gather(geom, blocks, parts, dl["u0"], dl["ustar"], threads=nthreads)
+        t_g2p = time.perf_counter() - t0
+        assert s == O.OK, s
+        return t_p2g, t_g2p
+
+    t_p2g, t_g2p = transfers(1)
+    faces = scene.pressure_faces(dl["blocks"], dl["ustar"]).view(O.PRESSURE_FACE)
+    cells = scene.pressure_cells().view(O.PRESSURE_CELL)
+    t0 = time.perf_counter()
+    s, out = O.project(geom, blocks, scene.dt, cells, faces, O.pressure_options(relative_target=1e-6, levels=6, coarse_sweeps=2))
+    t_proj = time.perf_counter() - t0
+    assert s == O.OK, s
+    step_s = (t_p2g + t_g2p) / len(parts) * scene.count + t_proj
+    result = {
+        "value": scene.count / step_s,
+        "unit": "particle-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"{scene.name}: P2G+G2P on {len(parts):,} of {scene.count:,} particles scaled to all, plus the "
+                   f"full projection of {len(cells):,} cells ({int(out['diagnostics']['iteration_count'])} MGPCG "
+                   f"iterations); oracle C++ FP64, 1 thread"),
+        "p2g_s_per_particle": t_p2g / len(parts),
+        "g2p_s_per_particle": t_g2p / len(parts),
+        "projection_ms": t_proj * 1e3,
+        "cpu": platform.processor() or platform.machine(),
+        "nproc": os.cpu_count(),
+    }
+    if threads > 1:
+        m_p2g, m_g2p = transfers(threads)
+        m_step = (m_p2g + m_g2p) / len(parts) * scene.count + t_proj
+        result["all_cores"] = {"value": scene.count / m_step, "cores": threads,
+                               "p2g_s_per_particle": m_p2g / len(parts), "g2p_s_per_particle": m_g2p / len(parts),
+                               "note": "transfers threaded; projection serial (same time as above)"}
+    return result
+
+
 def learned_comparison(ctx, scene, dt, opts, args):
     """Projection with the learned preconditioner (flexible CG, V-cycle fallback) against plain MGPCG on
     the same step: training pairs come from the MGPCG solve of one step, the network is trained on the
@@ -131,6 +193,8 @@ def main():
     ap.add_argument("--level-storage", choices=["fp32", "bf16"], default="bf16",
                     help="V-cycle coefficient storage (BF16 keeps FP32 accumulation)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="threads of the all-cores CPU transfer baseline (16 = the host share of one GPU)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="only measure the plain-copy ceiling")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner comparison")
@@ -231,7 +295,7 @@ def main():
         out["learned"] = learned_comparison(ctx, scene, dt, opts, args)
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
-        out["cpu_baseline"] = cpu_baseline(scene, ctx, last, args.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(scene, ctx, last, args.cpu_sample, args.cpu_threads)
         log(f"cpu baseline took {time.perf_counter() - t0:.1f}s")
     ctx.close()
     print(json.dumps(out), flush=True)

@@ -32,6 +32,25 @@ __global__ void k_split_columns(const double* in, uint64_t n, int stride, int co
   for (int c = 0; c < columns; ++c) base[c * cap + i] = in[i * stride + c];
 }
 
+// Compatible-deposit particle records: the 19 transfer doubles into the columns, then the side tag
+// and the affine switch into their own caller-order arrays.
+__global__ void k_split_boundary(const fl_boundary_particle* in, uint64_t n, double* base, uint64_t cap,
+                                 int64_t* side, uint8_t* affine) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const double* r = reinterpret_cast<const double*>(&in[i].transfer);
+  for (int c = 0; c < kParticleDoubles; ++c) base[c * cap + i] = r[c];
+  side[i] = in[i].side_tag;
+  affine[i] = in[i].affine_enabled;
+}
+
+__global__ void k_split_tags(const fl_face_tag* in, uint64_t n, int64_t* side, uint8_t* eligible) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  side[i] = in[i].side_tag;
+  eligible[i] = in[i].structurally_eligible;
+}
+
 __global__ void k_split_quanta(const fl_face_quanta* in, uint64_t n, int64_t* qm, int64_t* qp, int64_t* qs) {
   const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (i >= n) return;
@@ -134,9 +153,11 @@ int first_error_code(fl_ctx* ctx) {
   return w == ~uint64_t(0) ? FL_OK : static_cast<int>(w & 0xff);
 }
 
+// Caller records (the 168 B compatible-deposit particles are the largest), face quanta, or one
+// particle's 81 compatible-stencil records, whichever is largest.
 uint64_t staging_bytes(const fl_ctx* c) {
-  return std::max<uint64_t>(sizeof(fl_particle) * std::max<uint64_t>(1, c->cap.particles),
-                            24 * 192 * std::max<uint64_t>(1, c->cap.blocks));
+  return std::max<uint64_t>({sizeof(fl_boundary_particle) * std::max<uint64_t>(1, c->cap.particles),
+                             24 * 192 * std::max<uint64_t>(1, c->cap.blocks), 81 * sizeof(fl_compatible_sample)});
 }
 
 template <typename T>
@@ -413,6 +434,102 @@ int fl_p2g_quanta(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* block
   if (nf) FL_HIP(hipMemcpyAsync(out, ctx->quanta_aos, nf * sizeof(fl_face_quanta), hipMemcpyDeviceToHost, ctx->stream));
   copy_ms += c.stop();
   fill_measurement(ctx, measurement, kernel_ms, copy_ms, 10);
+  return FL_OK;
+}
+
+int fl_p2g_compatible(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
+                      const fl_boundary_particle* particles, size_t n, const fl_face_tag* tags, size_t tag_count,
+                      fl_quantization quantization, fl_face_quanta* out, size_t face_count,
+                      fl_compatible_sample* stencils, fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  FL_TRY(check_geometry(geometry));
+  FL_TRY(check_blocks(blocks, block_count));
+  // Capacity, then quantization, then tag coverage (liquid_boundary_transfer_3d.cc:330-341).
+  if (n > ctx->cap.particles || block_count > ctx->cap.blocks) return FL_RESOURCE_EXHAUSTED;
+  if (!powers_of_two_ok(quantization)) return FL_INVALID_ARGUMENT;
+  if (quantization.declared_particle_capacity > ctx->cap.particles) return FL_RESOURCE_EXHAUSTED;
+  if (n > quantization.declared_particle_capacity) return FL_RESOURCE_EXHAUSTED;
+  if (face_count != 192 * block_count || tag_count != face_count) return FL_INVALID_ARGUMENT;
+  if ((n && !particles) || (face_count && (!out || !tags))) return FL_INVALID_ARGUMENT;
+  const size_t pb = n * sizeof(fl_boundary_particle), ob = face_count * sizeof(fl_face_quanta),
+               tb = tag_count * sizeof(fl_face_tag), sb = stencils ? n * 81 * sizeof(fl_compatible_sample) : 0;
+  if (overlaps(out, ob, particles, pb) || overlaps(out, ob, tags, tb) || overlaps(stencils, sb, particles, pb) ||
+      overlaps(stencils, sb, tags, tb) || overlaps(stencils, sb, out, ob))
+    return FL_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  set_geometry(ctx, geometry);
+  const uint64_t cap = std::max<uint64_t>(1, quantization.declared_particle_capacity);
+  int bits = 0;
+  for (uint64_t v = cap - 1; v; v >>= 1) ++bits;
+  const double limit = std::ldexp(1.0, 62 - bits);
+  double copy_ms = 0;
+  FL_TRY(build_lookup(ctx, reinterpret_cast<const int32_t*>(blocks), block_count));
+  // Caller-order side tags and affine switches live in the gather buffer, slot tags in u0 / validity;
+  // neither is in use during a deposit.
+  int64_t* side = reinterpret_cast<int64_t*>(ctx->gather);
+  uint8_t* affine = reinterpret_cast<uint8_t*>(ctx->gather + ctx->cap.particles);
+  int64_t* slot_side = reinterpret_cast<int64_t*>(ctx->u0);
+  uint8_t* slot_ok = ctx->validity;
+  auto* staged = reinterpret_cast<fl_boundary_particle*>(ctx->staging_aos);
+  Timer t(ctx);
+  {
+    Timer c(ctx);
+    if (n) FL_HIP(hipMemcpyAsync(staged, particles, pb, hipMemcpyHostToDevice, ctx->stream));
+    copy_ms += c.stop();
+  }
+  if (n) {
+    k_split_boundary<<<grid_for(n), kThreads, 0, ctx->stream>>>(staged, n, ctx->raw.base, ctx->raw.capacity, side,
+                                                                affine);
+    FL_HIP(hipGetLastError());
+  }
+  ctx->raw.trial_aliases_sorted = false;
+  FL_TRY(bin_particles_mode(ctx, ctx->raw.view(), n, kBinCompatible));
+  if (ctx->bin_box.volume() > ctx->cap.lookup_blocks) return FL_RESOURCE_EXHAUSTED;
+  FL_TRY(sort_particles(ctx, n, ctx->raw.view(), true));
+  {
+    Timer c(ctx);
+    if (face_count) FL_HIP(hipMemcpyAsync(ctx->staging_aos, tags, tb, hipMemcpyHostToDevice, ctx->stream));
+    copy_ms += c.stop();
+  }
+  if (face_count) {
+    k_split_tags<<<grid_for(face_count), kThreads, 0, ctx->stream>>>(
+        reinterpret_cast<const fl_face_tag*>(ctx->staging_aos), face_count, slot_side, slot_ok);
+    FL_HIP(hipGetLastError());
+  }
+  FL_TRY(p2g_compatible(ctx, ctx->nvalid, slot_side, slot_ok, side, affine, 1.0 / quantization.mass_per_quantum,
+                        1.0 / quantization.pure_momentum_per_quantum, limit));
+  FL_TRY(first_error_code(ctx));
+  const uint64_t nf = face_count;
+  if (nf) {
+    k_pack_quanta<<<grid_for(nf), kThreads, 0, ctx->stream>>>(ctx->qm, ctx->qp, nf, ctx->quanta_aos);
+    FL_HIP(hipGetLastError());
+  }
+  const double kernel_ms = t.stop() - copy_ms;
+  std::vector<fl_face_quanta> quanta_host;
+  Timer c(ctx);
+  // Nothing reaches the caller until every stage has succeeded: the quanta wait in host memory while
+  // the stencils are formed chunk by chunk in the staging buffer.
+  if (nf && stencils) {
+    quanta_host.resize(nf);
+    FL_HIP(hipMemcpyAsync(quanta_host.data(), ctx->quanta_aos, nf * sizeof(fl_face_quanta), hipMemcpyDeviceToHost,
+                          ctx->stream));
+  } else if (nf) {
+    FL_HIP(hipMemcpyAsync(out, ctx->quanta_aos, nf * sizeof(fl_face_quanta), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  if (stencils && n) {
+    auto* d = reinterpret_cast<fl_compatible_sample*>(ctx->staging_aos);
+    const uint64_t chunk = staging_bytes(ctx) / (81 * sizeof(fl_compatible_sample));
+    for (uint64_t lo = 0; lo < n; lo += chunk) {
+      const uint64_t cnt = std::min<uint64_t>(chunk, n - lo);
+      FL_TRY(compatible_stencils(ctx, lo, cnt, slot_side, slot_ok, side, affine, d));
+      FL_HIP(hipMemcpyAsync(stencils + 81 * lo, d, cnt * 81 * sizeof(fl_compatible_sample), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    }
+  }
+  FL_HIP(hipStreamSynchronize(ctx->stream));
+  if (nf && stencils) std::copy(quanta_host.begin(), quanta_host.end(), out);
+  copy_ms += c.stop();
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, 11);
   return FL_OK;
 }
 

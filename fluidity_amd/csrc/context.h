@@ -172,6 +172,10 @@ inline unsigned grid_for(uint64_t n, int threads = kThreads) {
 
 // Topology (topology.hip)
 int bin_particles(fl_ctx* ctx, const ParticleView& p, uint64_t n, bool deposit_checks, bool gather_checks);
+// Bin with the input checks of one caller: plain bin query, quantized/FP64 deposit (mass, velocity,
+// slopes), gather, or compatible deposit (mass and velocity; enabled slope rows are checked per axis).
+enum { kBinPlain = 0, kBinDeposit = 1, kBinGather = 2, kBinCompatible = 3 };
+int bin_particles_mode(fl_ctx* ctx, const ParticleView& p, uint64_t n, int mode);
 int sort_particles(fl_ctx* ctx, uint64_t n, const ParticleView& src, bool permute, uint32_t permute_columns = ~0u);
 int build_lookup(fl_ctx* ctx, const int32_t* host_blocks, uint64_t nb);
 int activate_blocks(fl_ctx* ctx, uint64_t n, const int32_t* d_cells, uint64_t cell_count);
@@ -187,6 +191,13 @@ int apply_forces(fl_ctx* ctx, const double* d_old, uint64_t nf, const double g[3
 int g2p(fl_ctx* ctx, uint64_t n, const double* d_old, const double* d_new, hipEvent_t before = nullptr,
         hipEvent_t after = nullptr);
 int unpack_particles(fl_ctx* ctx, uint64_t n);
+// Compatible deposit of the sorted particles into ctx->qm / ctx->qp; slot tags per owned slot,
+// particle side tags and affine switches in caller order (device arrays).
+int p2g_compatible(fl_ctx* ctx, uint64_t n, const int64_t* slot_side, const uint8_t* slot_ok, const int64_t* side,
+                   const uint8_t* affine, double inv_qm, double inv_qp, double limit);
+// Compatible stencils of caller particles [lo, lo + count) from ctx->raw and ctx->bin (81 per particle).
+int compatible_stencils(fl_ctx* ctx, uint64_t lo, uint64_t count, const int64_t* slot_side, const uint8_t* slot_ok,
+                        const int64_t* side, const uint8_t* affine, fl_compatible_sample* d_out);
 
 // Small helpers (topology.hip)
 int read_error(fl_ctx* ctx, uint64_t* word);

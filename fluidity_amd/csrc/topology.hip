@@ -54,12 +54,14 @@ __global__ void k_bin(ParticleView p, uint64_t n, Geometry g, int32_t* bins, uin
   for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
     int32_t b0 = 0x7fffffff, b1 = 0x7fffffff, b2 = 0x7fffffff;
     int st = kOk;
-    if (kMode == 1) {
+    if (kMode == 1 || kMode == 3) {
       bool good = finite(p.m[i]) && p.m[i] >= 0.0;
 #pragma unroll
       for (int a = 0; a < 3; ++a) good = good && finite(p.v[a][i]);
+      if (kMode == 1) {
 #pragma unroll
-      for (int k = 0; k < 9; ++k) good = good && finite(p.c[k][i]);
+        for (int k = 0; k < 9; ++k) good = good && finite(p.c[k][i]);
+      }
       if (!good) st = kInvalidArgument;
     }
     const double s0 = p.s[0][i], s1 = p.s[1][i], s2 = p.s[2][i];
@@ -478,12 +480,18 @@ int exclusive_scan_u64(fl_ctx* ctx, const uint64_t* in, uint64_t* out, uint64_t 
 // Bins every particle and reduces the bin box. mode: deposit checks (1), gather checks (2) or the
 // plain sort bin (0). Returns the first failing particle's status when mode is 0.
 int bin_particles(fl_ctx* ctx, const ParticleView& p, uint64_t n, bool deposit_checks, bool gather_checks) {
+  return bin_particles_mode(ctx, p, n, deposit_checks ? kBinDeposit : (gather_checks ? kBinGather : kBinPlain));
+}
+
+int bin_particles_mode(fl_ctx* ctx, const ParticleView& p, uint64_t n, int mode) {
   FL_TRY(reset_error(ctx));
-  const int mode = deposit_checks ? 1 : (gather_checks ? 2 : 0);
+  if (mode < kBinPlain || mode > kBinCompatible) return FL_INTERNAL;
   if (n) {
     const bool pow2 = ctx->geometry.inv_h != 0.0;
-    auto* kernel = pow2 ? (mode == 1 ? k_bin<1, true> : (mode == 2 ? k_bin<2, true> : k_bin<0, true>))
-                        : (mode == 1 ? k_bin<1, false> : (mode == 2 ? k_bin<2, false> : k_bin<0, false>));
+    using K = decltype(&k_bin<0, true>);
+    static const K table[2][4] = {{k_bin<0, false>, k_bin<1, false>, k_bin<2, false>, k_bin<3, false>},
+                                  {k_bin<0, true>, k_bin<1, true>, k_bin<2, true>, k_bin<3, true>}};
+    auto* kernel = table[pow2 ? 1 : 0][mode];
     const unsigned groups = std::min<unsigned>(grid_for(n), kBinGroups);
     int32_t* partial = reinterpret_cast<int32_t*>(ctx->scan_partial);  // 6 * kBinGroups ints
     kernel<<<groups, kThreads, 0, ctx->stream>>>(p, n, ctx->geometry, ctx->bin, ctx->err, partial);

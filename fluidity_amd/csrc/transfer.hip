@@ -560,6 +560,358 @@ __global__ __launch_bounds__(kThreads) void k_g2p(ParticleView p, BinList bins, 
   }
 }
 
+// ---- Compatible deposit (DepositCompatibleMacQuanta3D: DepositCore, liquid_boundary_transfer_3d.cc:329-460) ----
+// Each particle carries a side tag and an affine switch; each owned slot a side tag and structural
+// eligibility. Per face axis only the compatible samples (eligible, same side; :93-95) deposit, with
+// the normalised weight N = w / S (S the compatible weight sum in stencil order), the quotient-rule
+// gradient (grad w - N sum grad w) / S, and the offset recentred on the N-weighted mean offset.
+// The workgroup layout is that of k_p2g; the window's slot tags are staged in LDS next to the quanta.
+
+// A spline weight factor with the edge distance that defines an outer weight, for the exponent form
+// of small gradients (sparse_mac_grid_3d.cc:236-254): outer samples carry |dw| (their distance to the
+// support edge), the middle sample none.
+struct WFactor {
+  double r, edge;
+};
+
+__device__ __forceinline__ WFactor wfactor(const Axis3& s, int i) { return {s.w[i], i == 1 ? 0.0 : fabs(s.dw[i])}; }
+
+__device__ inline double factor_mantissa(WFactor f, int& e) {
+  if (__builtin_isnormal(f.r) || f.edge == 0.0) return frexp(f.r, &e);
+  int de;
+  const double dm = frexp(f.edge, &de);
+  e = 2 * de;
+  return 0.5 * dm * dm;
+}
+
+// PhysicalWeightGradient (sparse_mac_grid_3d.cc:312-336): (fx fy) fz / h, or, when that product is
+// not normal and no factor is exactly zero, the same quotient rebuilt from mantissas and exponents.
+// With a power-of-two width, g / h and g * (1 / h) are the same correctly rounded value.
+template <bool kPow2>
+__device__ __forceinline__ double physical_gradient(WFactor fx, WFactor fy, WFactor fz, double h, double inv_h) {
+  const double g = (fx.r * fy.r) * fz.r;
+  const bool zero = (fx.r == 0.0 && fx.edge == 0.0) || (fy.r == 0.0 && fy.edge == 0.0) ||
+                    (fz.r == 0.0 && fz.edge == 0.0);
+  if (__builtin_isnormal(g) || zero) return kPow2 ? g * inv_h : g / h;
+  int ex, ey, ez, eh;
+  const double mx = factor_mantissa(fx, ex), my = factor_mantissa(fy, ey), mz = factor_mantissa(fz, ez);
+  const double mh = frexp(h, &eh);
+  return scalbn(((mx * my) * mz) / mh, ex + ey + ez - eh);
+}
+
+// The three physical gradient components of sample (xi, yi, zi) (sparse_mac_grid_3d.cc:536-541).
+template <bool kPow2>
+__device__ __forceinline__ void sample_gradient(const Axis3& X, const Axis3& Y, const Axis3& Z, int xi, int yi, int zi,
+                                                double h, double inv_h, double g[3]) {
+  const WFactor wx = wfactor(X, xi), wy = wfactor(Y, yi), wz = wfactor(Z, zi);
+  g[0] = physical_gradient<kPow2>({X.dw[xi], 0.0}, wy, wz, h, inv_h);
+  g[1] = physical_gradient<kPow2>(wx, {Y.dw[yi], 0.0}, wz, h, inv_h);
+  g[2] = physical_gradient<kPow2>(wx, wy, {Z.dw[zi], 0.0}, h, inv_h);
+}
+
+// The exact compatible gradient sum G and quotient-rule components (g - N G) / S of one face axis, in
+// stencil order (:376-419), for the rare cases where their finiteness is not settled by a bound, and
+// for the stencil records (kOut: also writes every record's zero defaults and the components).
+// Returns OutOfRange for a non-finite S or G, FailedPrecondition for S <= 0, OutOfRange for a
+// non-finite component.
+template <bool kPow2, bool kOut>
+__device__ __noinline__ int exact_compatible_gradients(const Axis3& X, const Axis3& Y, const Axis3& Z, uint32_t mask,
+                                                      double S, double h, double inv_h, fl_compatible_sample* out) {
+  double G[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < 27; ++k) {
+    if (kOut) {
+      out[k].normalized_weight = 0.0;
+      out[k].normalized_weight_gradient = fl_vec3{0.0, 0.0, 0.0};
+      out[k].centered_face_offset = fl_vec3{0.0, 0.0, 0.0};
+    }
+    if (!((mask >> k) & 1)) continue;
+    double g[3];
+    sample_gradient<kPow2>(X, Y, Z, k % 3, (k / 3) % 3, k / 9, h, inv_h, g);
+    G[0] += g[0];
+    G[1] += g[1];
+    G[2] += g[2];
+  }
+  if (!(finite(S) && finite(G[0]) && finite(G[1]) && finite(G[2]))) return kOutOfRange;
+  if (!(S > 0.0)) return kFailedPrecondition;
+  bool ok = true;
+  for (int k = 0; k < 27; ++k) {
+    if (!((mask >> k) & 1)) continue;
+    const int xi = k % 3, yi = (k / 3) % 3, zi = k / 9;
+    const double n = ((X.w[xi] * Y.w[yi]) * Z.w[zi]) / S;
+    double g[3];
+    sample_gradient<kPow2>(X, Y, Z, xi, yi, zi, h, inv_h, g);
+    const double n0 = (g[0] - n * G[0]) / S, n1 = (g[1] - n * G[1]) / S, n2 = (g[2] - n * G[2]) / S;
+    ok = ok && finite(n0) && finite(n1) && finite(n2);
+    if (kOut) out[k].normalized_weight_gradient = fl_vec3{n0, n1, n2};
+  }
+  return ok ? kOk : kOutOfRange;
+}
+
+// One face axis of one particle. compat(window index) tells whether a sample is compatible. kOut: write
+// the 27 CompatibleMacFaceSample3D records to out (slots from the window's block ordinals) instead of
+// depositing. Returns the axis status after the stencil checks: FailedPrecondition without positive
+// compatible support, OutOfRange for any non-finite or out-of-range value (:376-456).
+template <int A, bool kSmall, bool kPow2, bool kOut, class Compat>
+__device__ __forceinline__ int compatible_axis(const Axis3& X, const Axis3& Y, const Axis3& Z, const int32_t* tile,
+                                               const int32_t* wblock, Compat compat, double m, double v, double c0,
+                                               double c1, double c2, double h, double inv_h, double inv_qm,
+                                               double inv_qp, double limit, unsigned long long* wm,
+                                               unsigned long long* wq, fl_compatible_sample* out) {
+  int fx[3], fy[3], fz[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int rx = X.first + i - tile[0], ry = Y.first + i - tile[1], rz = Z.first + i - tile[2];
+    fx[i] = (rx >> 2) * 192 + (rx & 3) + A * 64;
+    fy[i] = (ry >> 2) * 384 + (ry & 3) * 4;
+    fz[i] = (rz >> 2) * 768 + (rz & 3) * 16;
+  }
+  // Compatible weight sum in stencil order (:376-383).
+  uint32_t mask = 0;
+  double S = 0.0;
+#pragma unroll
+  for (int zi = 0; zi < 3; ++zi)
+#pragma unroll
+    for (int yi = 0; yi < 3; ++yi)
+#pragma unroll
+      for (int xi = 0; xi < 3; ++xi) {
+        if (!compat(fz[zi] + fy[yi] + fx[xi])) continue;
+        mask |= 1u << ((zi * 3 + yi) * 3 + xi);
+        S += (X.w[xi] * Y.w[yi]) * Z.w[zi];
+      }
+  // The gradient sum G and the quotient-rule components (g - N G) / S only matter through their
+  // finiteness here. Every |g| is at most T = max over components of the product of the factor maxima
+  // divided by h (monotonic rounding; an underflowed outer weight's exact value is below the middle
+  // weight, which is at least 1/2), so |G| <= 28 T and, with |N| <= 1 (w is one of the summands of S),
+  // every component is at most 30 T / S. When 64 T / S is finite nothing needs forming; otherwise the
+  // exact sums decide.
+  const double wx = fmax(X.w[0], fmax(X.w[1], X.w[2])), wy = fmax(Y.w[0], fmax(Y.w[1], Y.w[2])),
+               wz = fmax(Z.w[0], fmax(Z.w[1], Z.w[2]));
+  const double dx = fmax(fabs(X.dw[0]), fmax(fabs(X.dw[1]), fabs(X.dw[2]))),
+               dy = fmax(fabs(Y.dw[0]), fmax(fabs(Y.dw[1]), fabs(Y.dw[2]))),
+               dz = fmax(fabs(Z.dw[0]), fmax(fabs(Z.dw[1]), fabs(Z.dw[2])));
+  const double P = fmax((dx * wy) * wz, fmax((wx * dy) * wz, (wx * wy) * dz));
+  const double T = kPow2 ? P * inv_h : P / h;
+  if (kOut || !finite((64.0 * T) / S)) {
+    const int code = exact_compatible_gradients<kPow2, kOut>(X, Y, Z, mask, S, h, inv_h, out);
+    if (code != kOk) return code;
+  } else if (!(S > 0.0)) {
+    return finite(S) ? kFailedPrecondition : kOutOfRange;
+  }
+  // Normalised weights and the weighted mean offset (:397-419).
+  double N[27];
+  double M[3] = {0.0, 0.0, 0.0};
+  bool ok = true;
+#pragma unroll
+  for (int zi = 0; zi < 3; ++zi)
+#pragma unroll
+    for (int yi = 0; yi < 3; ++yi)
+#pragma unroll
+      for (int xi = 0; xi < 3; ++xi) {
+        const int k = (zi * 3 + yi) * 3 + xi;
+        N[k] = 0.0;
+        if (kOut) {
+          const int wi = fz[zi] + fy[yi] + fx[xi];
+          out[k].face_slot = uint64_t(wblock[wi / 192]) * 192 + uint64_t(wi % 192);
+          out[k].compatible = (mask >> k) & 1;
+        }
+        if (!((mask >> k) & 1)) continue;
+        N[k] = ((X.w[xi] * Y.w[yi]) * Z.w[zi]) / S;
+        if (kOut) out[k].normalized_weight = N[k];
+        M[0] += X.off[xi] * N[k];
+        M[1] += Y.off[yi] * N[k];
+        M[2] += Z.off[zi] * N[k];
+      }
+  if (!(ok && finite(M[0]) && finite(M[1]) && finite(M[2]))) return kOutOfRange;
+  // Recentred offsets and the quantized contributions (:420-456).
+#pragma unroll
+  for (int zi = 0; zi < 3; ++zi)
+#pragma unroll
+    for (int yi = 0; yi < 3; ++yi)
+#pragma unroll
+      for (int xi = 0; xi < 3; ++xi) {
+        const int k = (zi * 3 + yi) * 3 + xi;
+        if (!((mask >> k) & 1)) continue;
+        const double rx = X.off[xi] - M[0], ry = Y.off[yi] - M[1], rz = Z.off[zi] - M[2];
+        const double mass = m * N[k];
+        const double mom = mass * (v + ((c0 * rx + c1 * ry) + c2 * rz));
+        ok = ok && finite(rx) && finite(ry) && finite(rz) && fabs(mass * inv_qm) <= limit &&
+             fabs(mom * inv_qp) <= limit;
+        if constexpr (kOut) {
+          out[k].centered_face_offset = fl_vec3{rx, ry, rz};
+        } else {
+          const int wi = fz[zi] + fy[yi] + fx[xi];
+          atomicAdd(&wm[wi], quanta<kSmall>(mass, inv_qm));
+          atomicAdd(&wq[wi], quanta<kSmall>(mom, inv_qp));
+        }
+      }
+  return ok ? kOk : kOutOfRange;
+}
+
+// One particle's three face axes in the reference's order (:354-457): per axis the stencil status and
+// window ownership, the enabled affine row's finiteness (InvalidArgument), then the compatible deposit.
+// Mass and velocity were checked while binning (:346-352).
+template <bool kInterior, bool kSmall, bool kPow2, bool kOut, class Coord, class Compat>
+__device__ __forceinline__ int compatible_particle(const Coord& cx, const Coord& cy, const Coord& cz,
+                                                   const int32_t* bin, const int32_t* tile, const int32_t* wblock,
+                                                   double h, double inv_h, const ParticleView& p, uint64_t k,
+                                                   bool affine, Compat compat, double inv_qm, double inv_qp,
+                                                   double limit, unsigned long long* wm, unsigned long long* wq,
+                                                   fl_compatible_sample* out, int code) {
+  const double m = p.m[k];
+  Axis3 ty, tz;
+  const int cty = axis3<kInterior>(cy, 0.5, bin[1], h, ty), ctz = axis3<kInterior>(cz, 0.5, bin[2], h, tz);
+  const auto row = [&](int a, double r[3]) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r[j] = affine ? p.c[3 * a + j][k] : 0.0;
+    return finite(r[0]) && finite(r[1]) && finite(r[2]);
+  };
+  {
+    Axis3 nx;
+    const int cnx = axis3<kInterior>(cx, 0.0, bin[0], h, nx);
+    if (code == kOk) code = axis_status(cnx, cty, ctz, nx, ty, tz);
+    if (code == kOk && !window_ok(wblock, axis_blocks(nx.first, tile[0]), axis_blocks(ty.first, tile[1]),
+                                  axis_blocks(tz.first, tile[2])))
+      code = kFailedPrecondition;
+    double r[3];
+    if (code == kOk && !row(0, r)) code = kInvalidArgument;
+    if (code == kOk)
+      code = compatible_axis<0, kSmall, kPow2, kOut>(nx, ty, tz, tile, wblock, compat, m, p.v[0][k], r[0], r[1], r[2],
+                                                     h, inv_h, inv_qm, inv_qp, limit, wm, wq, out);
+  }
+  Axis3 tx;
+  const int ctx_ = axis3<kInterior>(cx, 0.5, bin[0], h, tx);
+  {
+    Axis3 ny;
+    const int cny = axis3<kInterior>(cy, 0.0, bin[1], h, ny);
+    if (code == kOk) code = axis_status(ctx_, cny, ctz, tx, ny, tz);
+    if (code == kOk && !window_ok(wblock, axis_blocks(tx.first, tile[0]), axis_blocks(ny.first, tile[1]),
+                                  axis_blocks(tz.first, tile[2])))
+      code = kFailedPrecondition;
+    double r[3];
+    if (code == kOk && !row(1, r)) code = kInvalidArgument;
+    if (code == kOk)
+      code = compatible_axis<1, kSmall, kPow2, kOut>(tx, ny, tz, tile, wblock, compat, m, p.v[1][k], r[0], r[1], r[2],
+                                                     h, inv_h, inv_qm, inv_qp, limit, wm, wq, out ? out + 27 : out);
+  }
+  {
+    Axis3 nz;
+    const int cnz = axis3<kInterior>(cz, 0.0, bin[2], h, nz);
+    if (code == kOk) code = axis_status(ctx_, cty, cnz, tx, ty, nz);
+    if (code == kOk && !window_ok(wblock, axis_blocks(tx.first, tile[0]), axis_blocks(ty.first, tile[1]),
+                                  axis_blocks(nz.first, tile[2])))
+      code = kFailedPrecondition;
+    double r[3];
+    if (code == kOk && !row(2, r)) code = kInvalidArgument;
+    if (code == kOk)
+      code = compatible_axis<2, kSmall, kPow2, kOut>(tx, ty, nz, tile, wblock, compat, m, p.v[2][k], r[0], r[1], r[2],
+                                                     h, inv_h, inv_qm, inv_qp, limit, wm, wq, out ? out + 54 : out);
+  }
+  return code;
+}
+
+// Per-slot tags and per-particle attributes of the compatible deposit (device arrays).
+struct CompatTags {
+  const int64_t* slot_side;   // side tag per owned slot
+  const uint8_t* slot_ok;     // structural eligibility per owned slot
+  const int64_t* side;        // particle side tag, caller order
+  const uint8_t* affine;      // particle affine switch, caller order
+};
+
+template <bool kTrial, bool kPow2, bool kSmall>
+__global__ __launch_bounds__(kThreads) void k_p2g_compatible(ParticleView p, BinList bins, GridView grid,
+                                                             CompatTags tags, double inv_qm, double inv_qp,
+                                                             double limit, const uint32_t* perm,
+                                                             unsigned long long* qm, unsigned long long* qp,
+                                                             uint64_t* err) {
+  __shared__ unsigned long long wm[kWindow], wq[kWindow];
+  __shared__ int64_t wside[kWindow];
+  __shared__ uint8_t wok[kWindow];
+  __shared__ int32_t wblock[8];
+  const uint32_t b = xcd_chunked(blockIdx.x, gridDim.x);
+  int32_t bin[3];
+  bin_coords(bins, bins.key[b], bin);
+  if (threadIdx.x < 8)
+    wblock[threadIdx.x] = grid.ordinal(bin[0] + (threadIdx.x & 1), bin[1] + ((threadIdx.x >> 1) & 1),
+                                       bin[2] + (threadIdx.x >> 2));
+  __syncthreads();
+  for (int i = threadIdx.x; i < kWindow; i += kThreads) {
+    wm[i] = 0;
+    wq[i] = 0;
+    const int32_t ord = wblock[i / 192];
+    const uint64_t slot = uint64_t(ord) * 192 + (i % 192);
+    wside[i] = ord >= 0 ? tags.slot_side[slot] : 0;
+    wok[i] = ord >= 0 ? tags.slot_ok[slot] : 0;
+  }
+  __syncthreads();
+  const Geometry g = grid.g;
+  const double inv_h = g.inv_h;
+  const int32_t tile[3] = {4 * bin[0], 4 * bin[1], 4 * bin[2]};
+  const uint32_t off = bins.offset[b], cnt = bins.count[b];
+  for (uint32_t k = off + threadIdx.x; k < off + cnt; k += kThreads) {
+    const uint32_t o = perm[k];
+    const int64_t side = tags.side[o];
+    const bool affine = tags.affine[o] != 0;
+    const auto compat = [&](int wi) { return wok[wi] != 0 && wside[wi] == side; };
+    double t[3] = {p.s[0][k], p.s[1][k], p.s[2][k]};
+    int code = kOk;
+    if (kTrial) {
+      const double s0 = t[0], s1 = t[1], s2 = t[2];
+      t[0] = p.t[0][k];
+      t[1] = p.t[1][k];
+      t[2] = p.t[2][k];
+      if (!(within_half_cell(s0, t[0], g.h) && within_half_cell(s1, t[1], g.h) && within_half_cell(s2, t[2], g.h)))
+        code = kFailedPrecondition;
+    }
+    const auto cx = normalize<kPow2>(t[0], g.ox, g.h, g.inv_h);
+    const auto cy = normalize<kPow2>(t[1], g.oy, g.h, g.inv_h);
+    const auto cz = normalize<kPow2>(t[2], g.oz, g.h, g.inv_h);
+    if (__all(cx.interior() && cy.interior() && cz.interior()))
+      code = compatible_particle<true, kSmall, kPow2, false>(cx, cy, cz, bin, tile, wblock, g.h, inv_h, p, k, affine,
+                                                             compat, inv_qm, inv_qp, limit, wm, wq, nullptr, code);
+    else
+      code = compatible_particle<false, kSmall, kPow2, false>(cx, cy, cz, bin, tile, wblock, g.h, inv_h, p, k, affine,
+                                                              compat, inv_qm, inv_qp, limit, wm, wq, nullptr, code);
+    if (code) report(err, o, code);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kWindow; i += kThreads) {
+    const int32_t ord = wblock[i / 192];
+    if (ord < 0) continue;
+    const uint64_t slot = uint64_t(ord) * 192 + (i % 192);
+    if (wm[i]) atomicAdd(&qm[slot], wm[i]);
+    if (wq[i]) atomicAdd(&qp[slot], wq[i]);
+  }
+}
+
+// The compatible stencils of caller particles [lo, lo + count), one thread per particle, from the
+// caller-order columns and bins of a call whose deposit succeeded; 81 records per particle.
+template <bool kPow2>
+__global__ __launch_bounds__(kThreads) void k_compatible_stencils(ParticleView p, const int32_t* bins, GridView grid,
+                                                                  CompatTags tags, uint64_t lo, uint64_t count,
+                                                                  fl_compatible_sample* out) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= count) return;
+  const uint64_t k = lo + i;
+  const int32_t bin[3] = {bins[3 * k], bins[3 * k + 1], bins[3 * k + 2]};
+  int32_t wblock[8];
+#pragma unroll
+  for (int w = 0; w < 8; ++w) wblock[w] = grid.ordinal(bin[0] + (w & 1), bin[1] + ((w >> 1) & 1), bin[2] + (w >> 2));
+  const int64_t side = tags.side[k];
+  const auto compat = [&](int wi) {
+    const int32_t ord = wblock[wi / 192];
+    const uint64_t slot = uint64_t(ord) * 192 + (wi % 192);
+    return ord >= 0 && tags.slot_ok[slot] != 0 && tags.slot_side[slot] == side;
+  };
+  const Geometry g = grid.g;
+  const int32_t tile[3] = {4 * bin[0], 4 * bin[1], 4 * bin[2]};
+  // Stencil weights at the trial position (the trial columns alias the sorted ones when equal).
+  const auto cx = normalize<kPow2>(p.t[0][k], g.ox, g.h, g.inv_h);
+  const auto cy = normalize<kPow2>(p.t[1][k], g.oy, g.h, g.inv_h);
+  const auto cz = normalize<kPow2>(p.t[2][k], g.oz, g.h, g.inv_h);
+  compatible_particle<false, false, kPow2, true>(cx, cy, cz, bin, tile, wblock, g.h, g.inv_h, p, k, tags.affine[k] != 0,
+                                                 compat, 1.0, 1.0, 0x1p62, nullptr, nullptr, out + 81 * i, kOk);
+}
+
 __global__ void k_unpack(const fl_particle* in, uint64_t n, double* base, uint64_t cap) {
   const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (i >= n) return;
@@ -607,6 +959,37 @@ int p2g_quanta(fl_ctx* ctx, uint64_t n, double inv_qm, double inv_qp, double lim
     FL_HIP(hipGetLastError());
   }
   if (after) FL_HIP(hipEventRecord(after, ctx->stream));
+  return FL_OK;
+}
+
+int p2g_compatible(fl_ctx* ctx, uint64_t n, const int64_t* slot_side, const uint8_t* slot_ok, const int64_t* side,
+                   const uint8_t* affine, double inv_qm, double inv_qp, double limit) {
+  const uint64_t nf = ctx->nb * 192;
+  FL_TRY(fill_i64(ctx, ctx->qm, nf, 0));
+  FL_TRY(fill_i64(ctx, ctx->qp, nf, 0));
+  if (ctx->nbins == 0 || n == 0) return FL_OK;
+  using K = decltype(&k_p2g_compatible<false, true, true>);
+  static const K table[8] = {k_p2g_compatible<false, false, false>, k_p2g_compatible<false, false, true>,
+                             k_p2g_compatible<false, true, false>,  k_p2g_compatible<false, true, true>,
+                             k_p2g_compatible<true, false, false>,  k_p2g_compatible<true, false, true>,
+                             k_p2g_compatible<true, true, false>,   k_p2g_compatible<true, true, true>};
+  const bool trial = !ctx->sorted.trial_aliases_sorted, pow2 = ctx->geometry.inv_h != 0.0, small = limit <= 0x1p50;
+  const CompatTags tags{slot_side, slot_ok, side, affine};
+  table[(trial ? 4 : 0) + (pow2 ? 2 : 0) + (small ? 1 : 0)]<<<ctx->nbins, kThreads, 0, ctx->stream>>>(
+      ctx->sorted.view(), bin_list(ctx), grid_view(ctx), tags, inv_qm, inv_qp, limit, ctx->perm,
+      reinterpret_cast<unsigned long long*>(ctx->qm), reinterpret_cast<unsigned long long*>(ctx->qp), ctx->err);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+int compatible_stencils(fl_ctx* ctx, uint64_t lo, uint64_t count, const int64_t* slot_side, const uint8_t* slot_ok,
+                        const int64_t* side, const uint8_t* affine, fl_compatible_sample* d_out) {
+  if (!count) return FL_OK;
+  const CompatTags tags{slot_side, slot_ok, side, affine};
+  auto* kernel = ctx->geometry.inv_h != 0.0 ? k_compatible_stencils<true> : k_compatible_stencils<false>;
+  kernel<<<grid_for(count), kThreads, 0, ctx->stream>>>(ctx->raw.view(), ctx->bin, grid_view(ctx), tags, lo, count,
+                                                        d_out);
+  FL_HIP(hipGetLastError());
   return FL_OK;
 }
 

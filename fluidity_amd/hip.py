@@ -49,7 +49,7 @@ class StepReport(C.Structure):
         return np.frombuffer(bytes(self.pressure), dtype=R.PRESSURE_DIAGNOSTICS)[0]
 
 
-EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_p2g", "fl_p2g_quanta",
+EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_p2g", "fl_p2g_quanta", "fl_p2g_compatible",
            "fl_normalize_forces", "fl_apply_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
            "fl_scene_download", "fl_probe_copy", "fl_learn_param_count", "fl_learn_setup", "fl_learn_set_params", "fl_learn_get_params",
            "fl_learn_forward", "fl_learn_infer", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect", "fl_learn_eval",
@@ -142,6 +142,16 @@ class Context:
                                 C.c_size_t(len(particles)), Quantization(mass_q, mom_q, cap), _p(out),
                                 C.c_size_t(len(out)), C.byref(self.last))
         return s, out
+
+    def p2g_compatible(self, geometry, blocks, particles, tags, mass_q, mom_q, cap, stencils=False):
+        """DepositCompatibleMacQuanta3D: (status, face quanta, stencils [n][3][27] or None)."""
+        out = np.zeros(192 * len(blocks), R.FACE_QUANTA)
+        st = np.zeros((len(particles), 3, 27), R.COMPATIBLE_SAMPLE) if stencils else None
+        s = lib().fl_p2g_compatible(self.handle, _p(geometry), _p(blocks), C.c_size_t(len(blocks)), _p(particles),
+                                    C.c_size_t(len(particles)), _p(tags), C.c_size_t(len(tags)),
+                                    Quantization(mass_q, mom_q, cap), _p(out), C.c_size_t(len(out)), _p(st),
+                                    C.byref(self.last))
+        return s, out, st
 
     def normalize_forces(self, quanta, mass_q, mom_q, min_mass, accel, dt):
         nf = len(quanta)

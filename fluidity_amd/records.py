@@ -21,6 +21,14 @@ FACE_QUANTA = np.dtype([("mass_quanta_sum", "i8"), ("pure_momentum_quanta_sum", 
                         ("stress_force_quanta_sum", "i8")])
 GATHER_RESULT = np.dtype([("new_pic_velocity", VEC3), ("grid_velocity_increment", VEC3),
                           ("fitted_velocity_slopes", MAT3), ("velocity_gradient", MAT3)])
+BOUNDARY_PARTICLE = np.dtype({"names": ["transfer", "side_tag", "affine_enabled"],
+                              "formats": [PARTICLE, "i8", "u1"], "offsets": [0, 152, 160], "itemsize": 168})
+FACE_TAG = np.dtype({"names": ["side_tag", "structurally_eligible"], "formats": ["i8", "u1"], "offsets": [0, 8],
+                     "itemsize": 16})
+COMPATIBLE_SAMPLE = np.dtype({"names": ["face_slot", "compatible", "normalized_weight", "normalized_weight_gradient",
+                                        "centered_face_offset"],
+                              "formats": ["u8", "u1", "f8", VEC3, VEC3], "offsets": [0, 8, 16, 24, 48],
+                              "itemsize": 72})
 SORT_ENTRY = np.dtype([("bin_index", INT3), ("pad", "u4"), ("original_ordinal", "u8")])
 PRESSURE_CELL = np.dtype([("cell_index", INT3), ("pad", "i4"), ("fluid_volume", "f8"), ("target_divergence", "f8"),
                           ("bulk_modulus", "f8"), ("previous_pressure", "f8"), ("integrated_inward_wall_flux", "f8")])

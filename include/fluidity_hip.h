@@ -72,6 +72,27 @@ typedef struct fl_gather_result {
   fl_mat3 velocity_gradient;
 } fl_gather_result;
 typedef struct fl_sort_entry { fl_int3 bin_index; uint32_t pad; uint64_t original_ordinal; } fl_sort_entry;
+/* LiquidBoundaryParticle3D, LiquidBoundaryFaceTag3D, CompatibleMacFaceSample3D
+ * (native/include/fluidity/base/liquid_boundary_transfer_types_3d.h:4-20). */
+typedef struct fl_boundary_particle {
+  fl_particle transfer;
+  int64_t side_tag;
+  uint8_t affine_enabled;
+  uint8_t pad[7];
+} fl_boundary_particle; /* 168 B */
+typedef struct fl_face_tag {
+  int64_t side_tag;
+  uint8_t structurally_eligible;
+  uint8_t pad[7];
+} fl_face_tag; /* 16 B */
+typedef struct fl_compatible_sample {
+  uint64_t face_slot;
+  uint8_t compatible;
+  uint8_t pad[7];
+  double normalized_weight;
+  fl_vec3 normalized_weight_gradient;
+  fl_vec3 centered_face_offset;
+} fl_compatible_sample; /* 72 B */
 
 /* link_kind: 0 sealed, 1 internal (lift = pressure jump), 2 Dirichlet (lift = exterior pressure). */
 typedef struct fl_pressure_cell {
@@ -178,6 +199,16 @@ int fl_p2g(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size
 int fl_p2g_quanta(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
                   const fl_particle* particles, size_t n, fl_quantization quantization, fl_face_quanta* out,
                   size_t face_count, fl_measurement* measurement);
+/* DepositCompatibleMacQuanta3D (native/include/fluidity/host/liquid_boundary_transfer_3d.h:34-48;
+ * native/src/liquid_boundary_transfer_3d.cc:329-460): the quantized deposit restricted per face axis to
+ * the samples whose slot is structurally eligible and carries the particle's side tag, with weights
+ * normalised over those samples, the quotient-rule gradient and recentred affine offsets. tags has one
+ * entry per owned slot. stencils: null, or n * 3 * 27 records (particle, component, (z,y,x) sample
+ * order). The reference's 1024-particle / 24576-face caps are replaced by the context capacities. */
+int fl_p2g_compatible(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
+                      const fl_boundary_particle* particles, size_t n, const fl_face_tag* tags, size_t tag_count,
+                      fl_quantization quantization, fl_face_quanta* out, size_t face_count,
+                      fl_compatible_sample* stencils, fl_measurement* measurement);
 /* NormalizeLiquidMacFaces3D followed by ApplyForces (native/src/liquid_boundary_transfer_3d.cc:463-516;
  * native/src/liquid_substep_3d_cpu.cc:156-178). validity: 0 missing, 1 valid, 3 low mass.
  * Any output pointer may be null. */

@@ -45,6 +45,17 @@ struct flo_face_deposit { double deposited_mass; double deposited_pure_momentum;
 struct flo_face_quanta { int64_t mass_quanta_sum; int64_t pure_momentum_quanta_sum; int64_t stress_force_quanta_sum; };
 struct flo_gather_result { flo_vec3 new_pic_velocity; flo_vec3 grid_velocity_increment; flo_mat3 fitted_velocity_slopes; flo_mat3 velocity_gradient; };
 struct flo_stencil_sample { uint64_t face_slot; double weight; flo_vec3 face_offset_from_particle; flo_vec3 weight_gradient; };
+// native/include/fluidity/base/liquid_boundary_transfer_types_3d.h:4-20
+struct flo_boundary_particle { flo_particle transfer; int64_t side_tag; uint8_t affine_enabled; uint8_t pad[7]; };  // 168 B
+struct flo_face_tag { int64_t side_tag; uint8_t structurally_eligible; uint8_t pad[7]; };                          // 16 B
+struct flo_compatible_sample {
+  uint64_t face_slot;
+  uint8_t compatible;
+  uint8_t pad[7];
+  double normalized_weight;
+  flo_vec3 normalized_weight_gradient;
+  flo_vec3 centered_face_offset;
+};  // 72 B
 struct flo_sort_entry { flo_int3 bin_index; uint32_t pad; uint64_t original_ordinal; };  // 24 B, as LiquidParticleSortEntry3D
 
 // Flattened MacPressureCell3D / MacPressureFace3D (native/include/fluidity/host/mac_pressure_3d.h:14-51).
@@ -116,6 +127,12 @@ int flo_deposit_quanta(const flo_geometry* geometry, const flo_int3* sorted_bloc
 int flo_gather(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count,
                const flo_particle* particles, size_t n, const double* old_faces, const double* new_faces,
                size_t face_count, flo_gather_result* out, int threads);
+// liquid_boundary_transfer_3d.cc:329-460 (DepositCore). stencils: null, or n * 3 * 27 samples in
+// particle, component, (z,y,x) order. max_capacity replaces the reference's 1024-particle cap.
+int flo_deposit_compatible(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count,
+                           const flo_boundary_particle* particles, size_t n, const flo_face_tag* tags,
+                           size_t tag_count, flo_quantization quantization, uint64_t max_capacity,
+                           flo_face_quanta* out, size_t face_count, flo_compatible_sample* stencils);
 int flo_normalize(const flo_face_quanta* quanta, size_t face_count, flo_quantization quantization,
                   double minimum_face_mass, double* out_mass, double* out_velocity, uint8_t* out_validity);
 int flo_apply_forces(const flo_int3* sorted_blocks, size_t block_count, const double* old_velocities,

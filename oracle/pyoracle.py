@@ -33,6 +33,14 @@ GATHER_RESULT = np.dtype([("new_pic_velocity", VEC3), ("grid_velocity_increment"
                           ("fitted_velocity_slopes", MAT3), ("velocity_gradient", MAT3)])
 STENCIL_SAMPLE = np.dtype([("face_slot", "u8"), ("weight", "f8"), ("face_offset_from_particle", VEC3),
                            ("weight_gradient", VEC3)])
+BOUNDARY_PARTICLE = np.dtype({"names": ["transfer", "side_tag", "affine_enabled"],
+                              "formats": [PARTICLE, "i8", "u1"], "offsets": [0, 152, 160], "itemsize": 168})
+FACE_TAG = np.dtype({"names": ["side_tag", "structurally_eligible"], "formats": ["i8", "u1"], "offsets": [0, 8],
+                     "itemsize": 16})
+COMPATIBLE_SAMPLE = np.dtype({"names": ["face_slot", "compatible", "normalized_weight", "normalized_weight_gradient",
+                                        "centered_face_offset"],
+                              "formats": ["u8", "u1", "f8", VEC3, VEC3], "offsets": [0, 8, 16, 24, 48],
+                              "itemsize": 72})
 SORT_ENTRY = np.dtype([("bin_index", INT3), ("pad", "u4"), ("original_ordinal", "u8")])
 PRESSURE_CELL = np.dtype([("cell_index", INT3), ("pad", "i4"), ("fluid_volume", "f8"), ("target_divergence", "f8"),
                           ("bulk_modulus", "f8"), ("previous_pressure", "f8"), ("integrated_inward_wall_flux", "f8")])
@@ -169,6 +177,17 @@ def deposit_quanta(geom, blocks, particles, mass_q, mom_q, cap, max_capacity=1 <
                                  C.c_size_t(len(particles)), QuantC(mass_q, mom_q, cap), C.c_uint64(max_capacity),
                                  _p(out), C.c_size_t(len(out)), C.c_int(threads))
     return s, out
+
+
+def deposit_compatible(geom, blocks, particles, tags, mass_q, mom_q, cap, max_capacity=1 << 20, stencils=True):
+    """DepositCore: returns (status, face quanta, compatible stencils [n][3][27] or None)."""
+    out = np.zeros(192 * len(blocks), FACE_QUANTA)
+    st = np.zeros((len(particles), 3, 27), COMPATIBLE_SAMPLE) if stencils else None
+    s = lib().flo_deposit_compatible(_p(geom), _p(blocks), C.c_size_t(len(blocks)), _p(particles),
+                                     C.c_size_t(len(particles)), _p(tags), C.c_size_t(len(tags)),
+                                     QuantC(mass_q, mom_q, cap), C.c_uint64(max_capacity), _p(out),
+                                     C.c_size_t(len(out)), _p(st) if st is not None else None)
+    return s, out, st
 
 
 def gather(geom, blocks, particles, old, new, out=None, threads=1):

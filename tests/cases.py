@@ -204,3 +204,77 @@ def oracle_step_inputs(scene, threads=8):
     cells = scene.pressure_cells().view(O.PRESSURE_CELL)
     return dict(geom=geom, blocks=blocks, particles=parts, quanta=q, mass=mass, u0=u0, validity=validity, ustar=ustar,
                 dt=dt, cells=cells, faces=faces)
+
+
+# Compatible (masked) deposit: native/tests/liquid_boundary_transfer_3d_cases.h:19-36 — one block at
+# the origin, unit cells, one particle at (1.5, 1.5, 1.5) with mass 16, velocity (3, 4, 5), side 7,
+# every slot eligible on side 7, quanta 2^-30 with declared capacity 1.
+BOUNDARY_QUANTUM = 2.0 ** -30
+
+
+def boundary_particle(position=(1.5, 1.5, 1.5), mass=16.0, velocity=(3.0, 4.0, 5.0), side=7, affine=True):
+    p = np.zeros(1, O.BOUNDARY_PARTICLE)
+    t = p[0]["transfer"]
+    set_vec(t, "sorted_world_position", position)
+    set_vec(t, "trial_world_position", position)
+    t["particle_mass"] = mass
+    set_vec(t, "particle_velocity", velocity)
+    p[0]["transfer"] = t
+    p[0]["side_tag"] = side
+    p[0]["affine_enabled"] = 1 if affine else 0
+    return p
+
+
+def boundary_tags(block_count, side=7, eligible=True):
+    tags = np.zeros(192 * block_count, O.FACE_TAG)
+    tags["side_tag"] = side
+    tags["structurally_eligible"] = 1 if eligible else 0
+    return tags
+
+
+def set_slopes(particles, i, rows):
+    t = particles[i]["transfer"]
+    set_mat(t, "particle_velocity_slopes", rows)
+    particles[i]["transfer"] = t
+
+
+def set_position(particles, i, pos):
+    t = particles[i]["transfer"]
+    set_vec(t, "sorted_world_position", pos)
+    set_vec(t, "trial_world_position", pos)
+    particles[i]["transfer"] = t
+
+
+def masked_scene(seed, n, h=1.0, blocks_per_axis=3, sides=2, eligible=0.85, affine=0.8, trial=True):
+    """Random compatible-deposit inputs inside a cube of blocks, every stencil owned.
+
+    Positions keep every normal and transverse sample inside the allocated cube; trial positions move
+    up to 0.45 h from the sorted ones. Slot and particle sides are drawn from `sides` tags, slots are
+    eligible with probability `eligible`, affine rows are enabled with probability `affine`.
+    """
+    rng = np.random.default_rng(seed)
+    geom = O.geometry((0.25 * h, -0.5 * h, 0.125 * h), h)
+    cube = [(x, y, z) for z in range(blocks_per_axis) for y in range(blocks_per_axis) for x in range(blocks_per_axis)]
+    blocks = sorted_blocks(geom, cube)
+    cells = 4 * blocks_per_axis
+    org = np.array([0.25 * h, -0.5 * h, 0.125 * h])
+    lo, hi = 2.0, cells - 2.0
+    sorted_pos = org + h * rng.uniform(lo, hi, (n, 3))
+    trial_pos = sorted_pos + (h * rng.uniform(-0.45, 0.45, (n, 3)) if trial else 0.0)
+    p = np.zeros(n, O.BOUNDARY_PARTICLE)
+    t = p["transfer"]
+    for a, c in enumerate("xyz"):
+        t["sorted_world_position"][c] = sorted_pos[:, a]
+        t["trial_world_position"][c] = trial_pos[:, a]
+        t["particle_velocity"][c] = rng.normal(0.0, 1.0, n)
+    t["particle_mass"] = rng.uniform(0.5, 2.0, n) * h ** 3
+    for r in ("x_row", "y_row", "z_row"):
+        for c in "xyz":
+            t["particle_velocity_slopes"][r][c] = rng.normal(0.0, 0.1 / h, n)
+    p["transfer"] = t
+    p["side_tag"] = rng.integers(0, sides, n)
+    p["affine_enabled"] = rng.random(n) < affine
+    tags = np.zeros(192 * len(blocks), O.FACE_TAG)
+    tags["side_tag"] = rng.integers(0, sides, len(tags))
+    tags["structurally_eligible"] = rng.random(len(tags)) < eligible
+    return geom, blocks, p, tags

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "fluidity/host/hip_liquid_substep_backend_3d.h"
+#include "fluidity/host/liquid_boundary_transfer_3d.h"
 #include "fluidity/host/mac_pressure_3d.h"
 #include "fluidity/host/sparse_mac_grid_3d.h"
 #include "fluidity/host/transfers_3d.h"
@@ -158,6 +159,54 @@ void DepositMatchesOracleBitExact() {
                             few.size(), {qs.mass_per_quantum, qs.pure_momentum_per_quantum, 64}, std::uint64_t{1} << 40,
                             reinterpret_cast<flo_face_quanta*>(want.data()), want.size(), 1) == 0);
   EXPECT(std::memcmp(got.data(), want.data(), got.size() * sizeof(got[0])) == 0);
+}
+
+// DepositCompatibleMacQuanta3D: two sides split at x = 0.5 of the box with a band of ineligible slots,
+// half the particles with affine off; quanta and stencils against the oracle, field by field.
+void CompatibleDepositMatchesOracle() {
+  auto grid = SparseMacGrid3D::Create(kGeometry, DenseBlocks(4, 4, 4), 64);
+  EXPECT(grid.ok());
+  const auto transfer = BoxParticles(1, 14, 41, 2.0);
+  std::vector<LiquidBoundaryParticle3D> particles(transfer.size());
+  for (std::size_t i = 0; i < transfer.size(); ++i)
+    particles[i] = {transfer[i], transfer[i].sorted_world_position.x < 0.25 ? 1 : 2, (i % 2) == 0};
+  std::vector<LiquidBoundaryFaceTag3D> tags(grid->face_count());
+  for (std::size_t f = 0; f < tags.size(); ++f) {
+    const auto face = grid->FaceIndex(f);
+    EXPECT(face.ok());
+    const int x = face->lattice_index.x;
+    tags[f] = {x < 8 ? 1 : 2, !(x == 8 && face->axis == FaceNormalAxis::kX)};
+  }
+  const MacDepositQuantization3D q{0x1p-40, 0x1p-40, particles.size()};
+  auto got = DepositCompatibleMacQuanta3D({*grid, particles, tags, q});
+  EXPECT(got.ok());
+  std::vector<MacFaceDepositQuanta3D> want(grid->face_count());
+  std::vector<flo_compatible_sample> want_st(81 * particles.size());
+  const auto blocks = grid->active_block_indices();
+  EXPECT(flo_deposit_compatible(O(&kGeometry), O(blocks), blocks.size(),
+                                reinterpret_cast<const flo_boundary_particle*>(particles.data()), particles.size(),
+                                reinterpret_cast<const flo_face_tag*>(tags.data()), tags.size(),
+                                {q.mass_per_quantum, q.pure_momentum_per_quantum, q.declared_particle_capacity},
+                                std::uint64_t{1} << 40, reinterpret_cast<flo_face_quanta*>(want.data()), want.size(),
+                                want_st.data()) == 0);
+  EXPECT(std::memcmp(got->face_quanta.data(), want.data(), want.size() * sizeof(want[0])) == 0);
+  for (std::size_t i = 0; i < particles.size(); ++i)
+    for (int a = 0; a < 3; ++a)
+      for (int k = 0; k < 27; ++k) {
+        const auto& g = got->compatible_stencils[i][a][k];
+        const auto& w = want_st[(i * 3 + a) * 27 + k];
+        EXPECT(g.face_slot == w.face_slot && g.compatible == (w.compatible != 0));
+        EXPECT(std::memcmp(&g.normalized_weight, &w.normalized_weight, 7 * sizeof(double)) == 0);
+      }
+  // No positive support for one particle: FailedPrecondition, nothing published through the twin.
+  particles[5].side_tag = 77;
+  std::vector<MacFaceDepositQuanta3D> out(grid->face_count(), MacFaceDepositQuanta3D{1, 2, 3});
+  HipOperatorMeasurement m{};
+  auto rt = HipOperatorRuntime3D::Create({particles.size(), 64, 1024, 16, 16});
+  EXPECT(rt.ok());
+  EXPECT(HipDepositCompatibleMacQuanta3D(**rt, {*grid, particles, tags, q}, out, {}, m).code() ==
+         StatusCode::kFailedPrecondition);
+  EXPECT(out[0].mass_quanta_sum == 1 && out.back().stress_force_quanta_sum == 3);
 }
 
 void Fp64DepositMatchesOracle() {
@@ -404,6 +453,7 @@ int main(int argc, char** argv) {
     failed += Run("DepositMatchesOracleBitExact", DepositMatchesOracleBitExact);
     failed += Run("DepositFailuresPreserveOutput", DepositFailuresPreserveOutput);
     failed += Run("Fp64DepositMatchesOracle", Fp64DepositMatchesOracle);
+    failed += Run("CompatibleDepositMatchesOracle", CompatibleDepositMatchesOracle);
     failed += Run("GatherMatchesOracle", GatherMatchesOracle);
     failed += Run("BackendStagesMatchOracle", BackendStagesMatchOracle);
     failed += Run("ProjectionMeetsGatesAndMatchesOracle", ProjectionMeetsGatesAndMatchesOracle);
