@@ -192,6 +192,9 @@ def main():
     ap.add_argument("--coarse-sweeps", type=int, default=2)
     ap.add_argument("--level-storage", choices=["fp32", "bf16"], default="bf16",
                     help="V-cycle coefficient storage (BF16 keeps FP32 accumulation)")
+    ap.add_argument("--smoother", choices=["rbgs", "block", "hybrid"], default="hybrid",
+                    help="V-cycle smoother: level-wide red-black Gauss-Seidel, red-black sweeps per 4^3 block, or "
+                         "level-wide on the finest level and per block below")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="threads of the all-cores CPU transfer baseline (16 = the host share of one GPU)")
@@ -221,7 +224,7 @@ def main():
     s = ctx.scene_prepare(cells, scene.n, scene.rho)
     assert s == H.OK, s
     opts = H.pressure_options(relative_target=1e-6, levels=args.levels, coarse_sweeps=args.coarse_sweeps,
-                              bf16_levels=args.level_storage == "bf16")
+                              bf16_levels=args.level_storage == "bf16", smoother=["rbgs", "block", "hybrid"].index(args.smoother))
     dt = scene.dt
     for _ in range(args.warmup):
         s, rep = ctx.step(dt, scene.gravity, QUANTUM, QUANTUM, scene.count, opts)
@@ -267,6 +270,7 @@ def main():
             "grid": f"{scene.n}^3 sparse, 4^3 blocks",
             "multigrid_levels": int(d["multigrid_level_count"]),
             "level_storage": args.level_storage,
+            "smoother": args.smoother,
             "relative_divergence_target": 1e-6,
             "device": device,
         },

@@ -111,7 +111,6 @@ struct Pressure {
   double* p_init = nullptr;  // warm start of every solve (previous pressure minus gauge pin)
   double* res = nullptr;
   double* def = nullptr;
-  double* zero = nullptr;
   float* fine_vol = nullptr;  // [nb0*64] fluid volume (0 if none)
   // 8^3 tiles of level 0 for the learned preconditioner: 2x2x2 blocks each, and a class per cell
   int32_t* tile_blocks = nullptr;  // [ntiles][8] level-0 block or -1, octant (z,y,x) order
@@ -121,6 +120,7 @@ struct Pressure {
   float* r32 = nullptr;
   float* x32 = nullptr;
   float* d32 = nullptr;
+  float* dn32 = nullptr;  // the next direction (swapped with d32 every iteration)
   float* q32 = nullptr;
   double* partial = nullptr;  // [8][kPartials]
   double* scal = nullptr;     // device scalars
@@ -296,29 +296,14 @@ __global__ void k_freeze_cells(uint64_t nc, double dt, const double* V, const do
   target[c] = V[c] * s[c];
 }
 
-// Integrated RHS (mac_pressure_3d_arithmetic.h:263-313) with Neumaier sums per cell.
-// Cells are visited in block order (fine index) so that their faces are near each other in memory.
-// One lane per cell, in input (z,y,x) cell order so that the cell arrays are read contiguously.
-__global__ void k_rhs(uint64_t nc, const int32_t* cell_faces, const FaceRec* faces, const double* target,
-                      const double* inwall, const double* comp, const double* prev, double* rhs, double* rhs_abs) {
-  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  if (i >= nc) return;
-  const int32_t c = static_cast<int32_t>(i);
-  Row row;
-  row.add(target[c]);
-  row.add(inwall[c]);
-  row.add(comp[c] * prev[c]);
-  for (int j = 0; j < 6; ++j) {
-    const FaceRec& f = faces[cell_faces[6 * c + j]];
-    const double sign = f.tail == c ? 1.0 : -1.0;
-    row.add(-sign * (f.area * f.ustar));
-    row.add(-sign * f.wall);
-    if (f.kind == 1) row.add(-sign * (f.k * f.lift));
-    else if (f.kind == 2) row.add(f.k * f.lift);
-  }
-  rhs[c] = row.flux.total();
-  rhs_abs[c] = row.abs.total();
-}
+// Integrated RHS (mac_pressure_3d_arithmetic.h:263-313) with Neumaier sums per cell, fused with the
+// evaluation at p = 0 (the defect of the unprojected field, which fixes the relative-divergence gate):
+// the terms of k_evaluate in the same order with every cell pressure zero, so the operator part
+// vanishes and r = b. One lane per cell, in input (z,y,x) cell order.
+__global__ void k_rhs_zero_eval(uint64_t nc, const int32_t* cell_faces, const FaceRec* faces, const double* V,
+                                const double* target, const double* inwall, const double* comp, const double* prev,
+                                double* rhs, double* rhs_abs, double* res, double* def, double* partial,
+                                uint64_t* err);
 
 // Corrected face velocity u = u* + s (p_tail - p_head + jump), exterior pressure at a missing endpoint.
 __device__ inline double corrected(int32_t f, const int32_t* tail, const int32_t* head, const int32_t* kind,
@@ -421,12 +406,83 @@ __global__ void k_evaluate(uint64_t nc, const int32_t* cell_faces, const FaceRec
   block_reduce(v, partial, blockIdx.x);
 }
 
-__global__ void k_sum_sq(uint64_t n, const double* a, double* partial) {
+// The operator residual r = b - A p of k_evaluate alone (same terms, same order, same compensated
+// sum), for the periodic restarts of the PCG that are not convergence checks: it reads the 16 B of
+// conductance and endpoints per face instead of the 64 B face record, and visits the cells in level-0
+// block order so that a wave's faces are one contiguous run of the face arrays.
+__global__ void k_true_residual(uint64_t n0, const int32_t* fine_cell, const int32_t* cell_faces,
+                                const double* face_k, const int32_t* face_tail, const int32_t* face_head,
+                                const double* comp, const double* rhs, const double* p, double* res) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= n0) return;
+  const int32_t c = fine_cell[i];
+  if (c < 0) return;
+  Row op;
+  const double pc = p[c];
+  op.add(comp[c] * pc);
+  for (int j = 0; j < 6; ++j) {
+    const int32_t f = cell_faces[6 * c + j];
+    const double fk = face_k[f];
+    const int32_t tail = face_tail[f], head = face_head[f];
+    const double sign = tail == c ? 1.0 : -1.0;
+    const double pt = tail >= 0 ? p[tail] : 0.0;
+    const double ph = head >= 0 ? p[head] : 0.0;
+    if (fk != 0.0) op.add(sign * (fk * (pt - ph)));
+  }
+  res[c] = rhs[c] - op.flux.total();
+}
+
+__global__ void k_rhs_zero_eval(uint64_t nc, const int32_t* cell_faces, const FaceRec* faces, const double* V,
+                                const double* target, const double* inwall, const double* comp, const double* prev,
+                                double* rhs, double* rhs_abs, double* res, double* def, double* partial,
+                                uint64_t* err) {
   double v[4] = {0.0, 0.0, 0.0, 0.0};
-  for (uint64_t c = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; c < n; c += uint64_t(gridDim.x) * blockDim.x)
-    v[0] += a[c] * a[c];
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < nc; i += uint64_t(gridDim.x) * blockDim.x) {
+    const int32_t c = static_cast<int32_t>(i);
+    Row row, phys;
+    row.add(target[c]);
+    row.add(inwall[c]);
+    row.add(comp[c] * prev[c]);
+    phys.add(-target[c]);
+    phys.add(-inwall[c]);
+    if (comp[c] > 0.0) phys.add(comp[c] * (0.0 - prev[c]));
+    for (int j = 0; j < 6; ++j) {
+      const FaceRec& f = faces[cell_faces[6 * c + j]];
+      const double sign = f.tail == c ? 1.0 : -1.0;
+      row.add(-sign * (f.area * f.ustar));
+      row.add(-sign * f.wall);
+      if (f.kind == 1) row.add(-sign * (f.k * f.lift));
+      else if (f.kind == 2) row.add(f.k * f.lift);
+      double u = f.ustar;
+      if (f.area > 0.0) {
+        const double ext = f.kind == 2 ? f.lift : 0.0;
+        const double jump = f.kind == 1 ? f.lift : 0.0;
+        const double pt = f.tail >= 0 ? 0.0 : ext;
+        const double ph = f.head >= 0 ? 0.0 : ext;
+        u += f.scale * (pt - ph + jump);
+      }
+      phys.add(sign * (f.area * u));
+      phys.add(sign * f.wall);
+    }
+    const double b = row.flux.total(), babs = row.abs.total();
+    rhs[c] = b;
+    rhs_abs[c] = babs;
+    const double r = b - 0.0;
+    const double d = phys.flux.total() / V[c];
+    const double ie = V[c] * d + r;
+    const double terms = babs + 0.0 + phys.abs.total();
+    if (!finite(r) || !finite(d) || !finite(ie) || !finite(terms)) perr_report(err, c, kOutOfRange);
+    else if (fabs(ie) > 64.0 * kEpsD * terms) perr_report(err, c, kInternal);
+    res[c] = r;
+    def[c] = d;
+    v[0] += r * r;
+    v[1] = fmax(v[1], fabs(r) / V[c]);
+    v[2] = fmax(v[2], fabs(d));
+    v[3] += d * d;
+  }
   block_reduce(v, partial, blockIdx.x);
 }
+
 
 // Reduces the 4 partial rows in a fixed order into scal[base..base+3].
 __global__ void k_finalize4(const double* partial, unsigned slots, double* scal, int base) {
@@ -460,9 +516,11 @@ __global__ void k_finalize4(const double* partial, unsigned slots, double* scal,
 
 // Level 0 from the frozen FP64 links: edges between unknowns keep their conductance, links to a
 // Dirichlet exterior or to a gauge cell become anchors, compliance is an anchor.
+// Reads the face conductances and endpoints from their own arrays (16 B per face instead of the 64 B
+// face record).
 __global__ void k_level0(uint64_t n, const int32_t* fine_cell, const uint8_t* flag, const int32_t* cell_faces,
-                         const FaceRec* faces, const double* comp, const double* V, float* kk, float* anchor,
-                         float* diag, float* vol) {
+                         const double* face_k, const int32_t* face_tail, const int32_t* face_head, const double* comp,
+                         const double* V, float* kk, float* anchor, float* diag, float* vol) {
   const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (i >= n) return;
   const int32_t c = fine_cell[i];
@@ -472,14 +530,16 @@ __global__ void k_level0(uint64_t n, const int32_t* fine_cell, const uint8_t* fl
   if (unknown) {
     a = comp[c];
     for (int j = 0; j < 6; ++j) {
-      const FaceRec& f = faces[cell_faces[6 * c + j]];
-      if (!(f.k > 0.0)) continue;
-      const int32_t other = f.tail == c ? f.head : f.tail;
+      const int32_t f = cell_faces[6 * c + j];
+      const double fk = face_k[f];
+      if (!(fk > 0.0)) continue;
+      const int32_t tail = face_tail[f];
+      const int32_t other = tail == c ? face_head[f] : tail;
       if (other >= 0 && flag[other] == 1) {
-        ksum[j] = f.k;
-        if ((j & 1) == 0) kl[j >> 1] = static_cast<float>(f.k);
+        ksum[j] = fk;
+        if ((j & 1) == 0) kl[j >> 1] = static_cast<float>(fk);
       } else {
-        a += f.k;
+        a += fk;
       }
     }
   }
@@ -711,6 +771,126 @@ template <class KT>
 __global__ void k_prolong(LevelViewT<KT> F, const float* cx) {
   const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (i < F.n) prolong_cell(F, cx, i);
+}
+
+// ---- Block smoother: red-black Gauss-Seidel inside each 4^3 block, block Jacobi between blocks ----
+// One wave per block, one lane per cell (colour-major slots: lanes 0-31 the even cells, 32-63 the
+// odd ones). The block's values live in LDS for the whole smoothing step; values across the block
+// faces are read once, at the start, and held fixed. A pre-smoothing step starts from x = 0 and
+// sweeps red then black `sweeps` times; a post-smoothing step first adds the coarse correction
+// (prolongation) to the block and to its halo, sweeps black then red (the adjoint order, so the
+// V-cycle stays symmetric), and writes to a second buffer so that no block reads a neighbour's
+// halo while that neighbour writes it. Per cell this reads x, b, diag and the coefficients once,
+// instead of once per half sweep.
+template <class KT>
+__device__ inline float prolonged(const LevelViewT<KT>& F, const float* xc, uint64_t blk, int l) {
+  const int32_t pb = F.parent[2 * blk], oct = F.parent[2 * blk + 1];
+  const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
+  const int px = ((oct & 1) * 4 + lx) >> 1, py = (((oct >> 1) & 1) * 4 + ly) >> 1, pz = ((oct >> 2) * 4 + lz) >> 1;
+  return xc[uint64_t(pb) * 64 + cslot(pz * 16 + py * 4 + px)];
+}
+
+template <class KT, bool kPost>
+__global__ __launch_bounds__(256) void k_block_smooth(LevelViewT<KT> L, float* xout, const float* xc, int sweeps) {
+  __shared__ float sx[4][64];
+  const int w = threadIdx.x >> 6, sl = threadIdx.x & 63;
+  const uint64_t blk = blockIdx.x * 4ull + w;
+  const bool live = blk < L.n / 64;
+  const uint64_t i = blk * 64 + sl;
+  const int l = cnat(sl);
+  const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
+  float d = 0.f, bb = 0.f, x0 = 0.f;
+  float kk[6], halo[6];
+  int inner[6];
+#pragma unroll
+  for (int dir = 0; dir < 6; ++dir) {
+    kk[dir] = 0.f;
+    halo[dir] = 0.f;
+    inner[dir] = -1;
+  }
+  if (live) {
+    d = L.diag[i];
+    bb = L.b[i];
+    if (kPost && d != 0.f) x0 = L.x[i] + prolonged(L, xc, blk, l);
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      int nx = lx, ny = ly, nz = lz;
+      if (dir == 0) nx -= 1;
+      if (dir == 1) nx += 1;
+      if (dir == 2) ny -= 1;
+      if (dir == 3) ny += 1;
+      if (dir == 4) nz -= 1;
+      if (dir == 5) nz += 1;
+      const bool outside = nx < 0 || nx > 3 || ny < 0 || ny > 3 || nz < 0 || nz > 3;
+      const int64_t nb = outside ? L.nbr[6 * blk + dir] : static_cast<int64_t>(blk);
+      if (nb < 0) continue;
+      const int nl = (nz & 3) * 16 + (ny & 3) * 4 + (nx & 3);
+      const int64_t j = nb * 64 + cslot(nl);
+      kk[dir] = face_kt(L.k, L.n, static_cast<int64_t>(i), j, dir);
+      if (!outside) {
+        inner[dir] = cslot(nl);
+      } else if (kPost) {
+        const float xj = L.x[j];
+        halo[dir] = L.diag[j] == 0.f ? xj : xj + prolonged(L, xc, static_cast<uint64_t>(nb), nl);
+      }
+    }
+  }
+  sx[w][sl] = x0;
+  __syncthreads();
+  for (int it = 0; it < sweeps; ++it) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int colour = kPost ? 1 - h : h;
+      if ((sl >> 5) == colour) {
+        float acc = bb;
+#pragma unroll
+        for (int dir = 0; dir < 6; ++dir) acc += kk[dir] * (inner[dir] >= 0 ? sx[w][inner[dir]] : halo[dir]);
+        sx[w][sl] = d == 0.f ? 0.f : acc / d;
+      }
+      __syncthreads();
+    }
+  }
+  if (live) (kPost ? xout : L.x)[i] = sx[w][sl];
+}
+
+// Residual b - A x of every cell (after block smoothing the odd cells at block faces carry a residual
+// too), then the coarse right-hand side as the sum over all eight children.
+template <class KT>
+__global__ void k_residual_all(LevelViewT<KT> L, float* r) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= L.n) return;
+  if (L.diag[i] == 0.f) {
+    r[i] = 0.f;
+    return;
+  }
+  const uint64_t blk = i >> 6;
+  const int l = cnat(static_cast<int>(i & 63));
+  const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
+  const float xi = L.x[i];
+  float a = L.anchor[i] * xi;
+  for (int dir = 0; dir < 6; ++dir) {
+    const int64_t j = neighbour(L.nbr, blk, lx, ly, lz, dir);
+    if (j < 0) continue;
+    a += face_kt(L.k, L.n, static_cast<int64_t>(i), j, dir) * (xi - L.x[j]);
+  }
+  r[i] = L.b[i] - a;
+}
+
+template <class KT>
+__global__ void k_restrict_all(const float* r, LevelViewT<KT> C) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= C.n) return;
+  const uint64_t blk = i >> 6;
+  const int l = cnat(static_cast<int>(i & 63));
+  const int cx = l & 3, cy = (l >> 2) & 3, cz = l >> 4;
+  float sum = 0.f;
+  for (int o = 0; o < 8; ++o) {
+    const int fx = 2 * cx + (o & 1), fy = 2 * cy + ((o >> 1) & 1), fz = 2 * cz + (o >> 2);
+    const int32_t fb = C.child[8 * blk + ((fz >> 2) * 2 + (fy >> 2)) * 2 + (fx >> 2)];
+    if (fb < 0) continue;
+    sum += r[uint64_t(fb) * 64 + cslot((fz & 3) * 16 + (fy & 3) * 4 + (fx & 3))];
+  }
+  const_cast<float*>(C.b)[i] = sum;
 }
 
 constexpr int kMaxTail = 16;
@@ -970,15 +1150,41 @@ __global__ void k_finalize_step(const double* partial, unsigned slots, double* s
   if (post == 4) after_curv_fcg(scal, fscal, err);
 }
 
-__global__ void k_update_d(uint64_t n, const float* z, float* d, const float* fscal, int restart) {
-  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  if (i >= n) return;
-  if (restart) {
-    d[i] = z[i];
-  } else {
-    const float prev = fscal[1] * d[i];
-    d[i] = z[i] + prev;
+// One PCG direction step in one pass: d' = z + beta d (d' = z on a restart), q = A d' in the
+// edge-difference form, and the block partials of d' . q. The neighbours' d' are formed from their z
+// and d the same way, and d' goes to a second buffer, so no lane reads a direction that another lane
+// has already replaced.
+__global__ void k_direction_apply(uint64_t n, const int32_t* nbr, const float* k, const float* anchor,
+                                  const float* diag, const float* z, const float* d, float* dn, float* q,
+                                  const float* fscal, int restart, double* partial) {
+  const float beta = restart ? 0.f : fscal[1];
+  auto direction = [&](uint64_t j) {
+    if (restart) return z[j];
+    const float prev = beta * d[j];
+    return z[j] + prev;
+  };
+  double acc = 0.0;
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    const float di = direction(i);
+    dn[i] = di;
+    if (diag[i] == 0.f) {
+      q[i] = 0.f;
+      continue;
+    }
+    const uint64_t blk = i >> 6;
+    const int l = cnat(static_cast<int>(i & 63));
+    const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
+    float sum = anchor[i] * di;
+    for (int dir = 0; dir < 6; ++dir) {
+      const int64_t j = neighbour(nbr, blk, lx, ly, lz, dir);
+      if (j < 0) continue;
+      sum += face_k(k, n, static_cast<int64_t>(i), j, dir) * (di - direction(static_cast<uint64_t>(j)));
+    }
+    q[i] = sum;
+    acc += static_cast<double>(di) * static_cast<double>(sum);
   }
+  double v[4] = {acc, 0.0, 0.0, 0.0};
+  block_reduce(v, partial, blockIdx.x);
 }
 
 __global__ void k_update_xr(uint64_t n, const float* d, const float* q, const float* diag, const float* vol, float* x,
@@ -1061,7 +1267,7 @@ int ensure_capacity(fl_ctx* ctx, uint64_t nc, uint64_t nf, uint64_t nfine) {
                   P.face_slot, P.face_area, P.face_beta, P.face_dist, P.face_ustar, P.face_uwall, P.face_flux,
                   P.face_has_flux, P.face_lift, P.face_scale, P.face_k, P.face_wall, P.face_u, P.face_rec, P.cell_volume,
                   P.cell_s, P.cell_K, P.cell_prev, P.cell_inwall, P.cell_comp, P.cell_target, P.rhs, P.rhs_abs,
-                  P.p, P.p_base, P.p_init, P.res, P.def, P.zero, P.fine_vol, P.r32, P.x32, P.d32, P.q32, P.partial, P.scal,
+                  P.p, P.p_base, P.p_init, P.res, P.def, P.fine_vol, P.r32, P.x32, P.d32, P.dn32, P.q32, P.partial, P.scal,
                   P.fscal, P.perr};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1087,7 +1293,7 @@ int ensure_capacity(fl_ctx* ctx, uint64_t nc, uint64_t nf, uint64_t nfine) {
   FL_TRY(dal(&P.face_u, cf));
   FL_TRY(dal(&P.face_rec, cf));
   for (double** a : {&P.cell_volume, &P.cell_s, &P.cell_K, &P.cell_prev, &P.cell_inwall, &P.cell_comp, &P.cell_target,
-                     &P.rhs, &P.rhs_abs, &P.p, &P.p_base, &P.p_init, &P.res, &P.def, &P.zero})
+                     &P.rhs, &P.rhs_abs, &P.p, &P.p_base, &P.p_init, &P.res, &P.def})
     FL_TRY(dal(a, cc));
   FL_TRY(dal(&P.fine_cell, cn));
   FL_TRY(dal(&P.fine_vol, cn));
@@ -1096,13 +1302,13 @@ int ensure_capacity(fl_ctx* ctx, uint64_t nc, uint64_t nf, uint64_t nfine) {
   FL_TRY(dal(&P.r32, cn));
   FL_TRY(dal(&P.x32, cn));
   FL_TRY(dal(&P.d32, cn));
+  FL_TRY(dal(&P.dn32, cn));
   FL_TRY(dal(&P.q32, cn));
   FL_TRY(dal(&P.partial, 4 * kPartials));
   FL_TRY(dal(&P.scal, 64));
   FL_TRY(dal(&P.fscal, 16));
   FL_TRY(dal(&P.perr, 4));
   FL_HIP(hipHostMalloc(reinterpret_cast<void**>(&P.hscal), 64 * sizeof(double), hipHostMallocDefault));
-  FL_HIP(hipMemsetAsync(P.zero, 0, cc * sizeof(double), ctx->stream));
   P.cap_cells = cc;
   P.cap_faces = cf;
   P.cap_fine = cn;
@@ -1213,7 +1419,8 @@ int setup_coefficients(fl_ctx* ctx, Pressure& P) {
   hipStream_t s = ctx->stream;
   Level& L0 = P.levels[0];
   const uint64_t n0 = L0.nb * 64;
-  k_level0<<<grid_for(n0), kThreads, 0, s>>>(n0, P.fine_cell, P.cell_flag, P.cell_faces, P.face_rec, P.cell_comp,
+  k_level0<<<grid_for(n0), kThreads, 0, s>>>(n0, P.fine_cell, P.cell_flag, P.cell_faces, P.face_k, P.face_tail,
+                                             P.face_head, P.cell_comp,
                                              P.cell_volume, L0.k, L0.anchor, L0.diag, P.fine_vol);
   FL_HIP(hipGetLastError());
   for (size_t l = 1; l < P.levels.size(); ++l) {
@@ -1246,7 +1453,7 @@ LevelViewT<Bf16> view<Bf16>(Pressure& P, size_t l) {
 // own right-hand side. Levels of at most kTailBlocks blocks (and everything below them) run in one
 // single-workgroup launch.
 template <class KT>
-int vcycle(fl_ctx* ctx, Pressure& P, size_t l, int nu, int coarse_sweeps) {
+int vcycle(fl_ctx* ctx, Pressure& P, size_t l, int nu, int coarse_sweeps, int smoother) {
   hipStream_t s = ctx->stream;
   const size_t last = P.levels.size() - 1;
   if (l == last && P.levels[l].nb > kTailBlocks) {  // a large coarsest level: grid-wide sweeps from zero
@@ -1269,6 +1476,20 @@ int vcycle(fl_ctx* ctx, Pressure& P, size_t l, int nu, int coarse_sweeps) {
     FL_HIP(hipGetLastError());
     return FL_OK;
   }
+  if (smoother == 1 || (smoother == 2 && l > 0)) {  // block smoother: pre-smooth, restrict, coarse cycle, post
+    const LevelViewT<KT> L = view<KT>(P, l), C = view<KT>(P, l + 1);
+    const unsigned gb = static_cast<unsigned>((P.levels[l].nb + 3) / 4);
+    k_block_smooth<KT, false><<<gb, 256, 0, s>>>(L, nullptr, nullptr, nu);
+    k_residual_all<<<grid_for(L.n), kThreads, 0, s>>>(L, P.levels[l].r);
+    k_restrict_all<<<grid_for(C.n), kThreads, 0, s>>>(P.levels[l].r, C);
+    FL_HIP(hipGetLastError());
+    FL_TRY(vcycle<KT>(ctx, P, l + 1, nu, coarse_sweeps, smoother));
+    const LevelViewT<KT> Lp = view<KT>(P, l);
+    k_block_smooth<KT, true><<<gb, 256, 0, s>>>(Lp, P.levels[l].r, P.levels[l + 1].x, nu);
+    FL_HIP(hipGetLastError());
+    std::swap(P.levels[l].x, P.levels[l].r);  // the post-smoothed values are the level's x from here on
+    return FL_OK;
+  }
   const LevelViewT<KT> L = view<KT>(P, l), C = view<KT>(P, l + 1);
   const unsigned gh = grid_for(L.n / 2);
   for (int it = 0; it < nu; ++it) {
@@ -1278,8 +1499,8 @@ int vcycle(fl_ctx* ctx, Pressure& P, size_t l, int nu, int coarse_sweeps) {
   k_residual_red<<<gh, kThreads, 0, s>>>(L, P.levels[l].r);
   k_restrict_red<<<grid_for(C.n), kThreads, 0, s>>>(P.levels[l].r, C);
   FL_HIP(hipGetLastError());
-  FL_TRY(vcycle<KT>(ctx, P, l + 1, nu, coarse_sweeps));
-  k_prolong<<<grid_for(L.n), kThreads, 0, s>>>(L, C.x);
+  FL_TRY(vcycle<KT>(ctx, P, l + 1, nu, coarse_sweeps, smoother));
+  k_prolong<<<grid_for(L.n), kThreads, 0, s>>>(L, P.levels[l + 1].x);  // the coarse level may have swapped x
   for (int it = 0; it < nu; ++it) {
     k_smooth<<<gh, kThreads, 0, s>>>(L, 1, 0);
     k_smooth<<<gh, kThreads, 0, s>>>(L, 0, 0);
@@ -1297,12 +1518,9 @@ struct EvalResult {
   int code;
 };
 
-int evaluate(fl_ctx* ctx, Pressure& P, const double* p, EvalResult& out) {
+// Reduces the evaluation partials of `groups` workgroups and reads the norms and error words back.
+int read_evaluation(fl_ctx* ctx, Pressure& P, unsigned g, EvalResult& out) {
   hipStream_t s = ctx->stream;
-  const unsigned g = reduce_grid(P.nc, kPartials);
-  k_evaluate<<<g, kThreads, 0, s>>>(P.nc, P.cell_faces, P.face_rec, P.cell_volume, P.cell_target,
-                                    P.cell_inwall, P.cell_comp, P.cell_prev, P.rhs, P.rhs_abs, p, P.res, P.def,
-                                    P.partial, P.perr);
   k_finalize4<<<1, kThreads, 0, s>>>(P.partial, g, P.scal, 8);
   FL_HIP(hipGetLastError());
   FL_HIP(hipMemcpyAsync(P.hscal, P.scal, 16 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1315,6 +1533,14 @@ int evaluate(fl_ctx* ctx, Pressure& P, const double* p, EvalResult& out) {
   out.code = ctx->host_words[0] != ~uint64_t(0) ? static_cast<int>(ctx->host_words[0] & 0xff) : FL_OK;
   if (out.code == FL_OK && ctx->host_words[1] != ~uint64_t(0)) out.code = static_cast<int>(ctx->host_words[1] & 0xff);
   return FL_OK;
+}
+
+int evaluate(fl_ctx* ctx, Pressure& P, const double* p, EvalResult& out) {
+  const unsigned g = reduce_grid(P.nc, kPartials);
+  k_evaluate<<<g, kThreads, 0, ctx->stream>>>(P.nc, P.cell_faces, P.face_rec, P.cell_volume, P.cell_target,
+                                              P.cell_inwall, P.cell_comp, P.cell_prev, P.rhs, P.rhs_abs, p, P.res,
+                                              P.def, P.partial, P.perr);
+  return read_evaluation(ctx, P, g, out);
 }
 
 bool accept(const EvalResult& e, const Gates& g) {
@@ -1337,6 +1563,7 @@ int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates&
   const uint64_t n = L0.nb * 64;
   const unsigned g = grid_for(n);
   const unsigned rg = reduce_grid(n);
+  const unsigned rgw = reduce_grid(n, kPartials);  // one cell per lane for the direction step's gathers
   diag.multigrid_level_count = P.levels.size();
   diag.iteration_count = 0;
   diag.restarts = 0;
@@ -1385,9 +1612,9 @@ int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates&
                                                                     P.ntiles, L0.x);
       ++diag.learned_iterations;
     } else if (opt.level_storage == 2) {
-      FL_TRY(vcycle<Bf16>(ctx, P, 0, opt.smoothing_sweeps, opt.coarse_sweeps));
+      FL_TRY(vcycle<Bf16>(ctx, P, 0, opt.smoothing_sweeps, opt.coarse_sweeps, opt.smoother));
     } else {
-      FL_TRY(vcycle<float>(ctx, P, 0, opt.smoothing_sweeps, opt.coarse_sweeps));
+      FL_TRY(vcycle<float>(ctx, P, 0, opt.smoothing_sweeps, opt.coarse_sweeps, opt.smoother));
     }
     if (!use_net && net && net->collect_every > 0 && it % uint64_t(net->collect_every) == 0 &&
         P.ntiles <= net->tile_cap && net->ndata + P.ntiles <= net->data_cap) {
@@ -1406,10 +1633,10 @@ int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates&
     }
     k_dot<<<rg, kThreads, 0, s>>>(n, P.r32, L0.x, P.partial);
     k_finalize_step<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, P.fscal, P.perr, use_net ? 2 : 1, restart ? 1 : 0);
-    k_update_d<<<g, kThreads, 0, s>>>(n, L0.x, P.d32, P.fscal, restart ? 1 : 0);
-    k_apply<<<g, kThreads, 0, s>>>(n, L0.nbr, L0.k, L0.anchor, L0.diag, P.d32, P.q32);
-    k_dot<<<rg, kThreads, 0, s>>>(n, P.d32, P.q32, P.partial);
-    k_finalize_step<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, P.fscal, P.perr, use_net ? 4 : 3, 0);
+    k_direction_apply<<<rgw, kThreads, 0, s>>>(n, L0.nbr, L0.k, L0.anchor, L0.diag, L0.x, P.d32, P.dn32, P.q32,
+                                               P.fscal, restart ? 1 : 0, P.partial);
+    std::swap(P.d32, P.dn32);
+    k_finalize_step<<<1, kThreads, 0, s>>>(P.partial, rgw, P.scal, P.fscal, P.perr, use_net ? 4 : 3, 0);
     k_update_xr<<<rg, kThreads, 0, s>>>(n, P.d32, P.q32, L0.diag, P.fine_vol, P.x32, P.r32, P.fscal, P.partial,
                                         P.perr);
     k_finalize4<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, 8);
@@ -1435,7 +1662,22 @@ int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates&
     bool recursive = rn <= gates.integrated && rmax <= gates.volume;
     if (gates.relative > 0.0) recursive = recursive && dn <= gates.relative * gates.initial_defect;
     const bool last = it == opt.max_iterations;
-    if (last || recursive || since >= 10) {
+    if (!last && !recursive && since >= 10) {
+      // Periodic restart from the FP64 true residual; acceptance is only tested at recursive
+      // convergence or on the last iteration.
+      k_apply_correction<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, P.cell_fine, P.cell_flag, P.p_base, P.x32, P.p,
+                                                             P.perr);
+      k_true_residual<<<g, kThreads, 0, s>>>(n, P.fine_cell, P.cell_faces, P.face_k, P.face_tail, P.face_head,
+                                              P.cell_comp, P.rhs, P.p, P.res);
+      k_copy_f64<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, P.p, P.p_base);
+      k_load_residual<<<g, kThreads, 0, s>>>(n, P.fine_cell, P.cell_flag, P.res, P.r32, P.x32, P.perr);
+      FL_HIP(hipGetLastError());
+      restart = true;
+      since = 0;
+      ++diag.restarts;
+      continue;
+    }
+    if (last || recursive) {
       k_apply_correction<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, P.cell_fine, P.cell_flag, P.p_base, P.x32, P.p,
                                                              P.perr);
       FL_TRY(evaluate(ctx, P, P.p, e));
@@ -1466,7 +1708,7 @@ void destroy_pressure(fl_ctx* ctx) {
                   P->face_slot, P->face_area, P->face_beta, P->face_dist, P->face_ustar, P->face_uwall, P->face_flux,
                   P->face_has_flux, P->face_lift, P->face_scale, P->face_k, P->face_wall, P->face_u, P->face_rec, P->cell_volume,
                   P->cell_s, P->cell_K, P->cell_prev, P->cell_inwall, P->cell_comp, P->cell_target, P->rhs,
-                  P->rhs_abs, P->p, P->p_base, P->p_init, P->res, P->def, P->zero, P->fine_vol, P->r32, P->x32, P->d32, P->q32,
+                  P->rhs_abs, P->p, P->p_base, P->p_init, P->res, P->def, P->fine_vol, P->r32, P->x32, P->d32, P->dn32, P->q32,
                   P->partial, P->scal, P->fscal, P->perr};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1844,16 +2086,15 @@ int freeze_and_assemble(fl_ctx* ctx, Pressure& P, double dt, const fl_pressure_o
                                                      P.face_wall, P.face_rec);
   k_freeze_cells<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, dt, P.cell_volume, P.cell_s, P.cell_K, P.cell_comp,
                                                      P.cell_target);
-  k_rhs<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, P.cell_faces, P.face_rec, P.cell_target, P.cell_inwall,
-                                            P.cell_comp, P.cell_prev, P.rhs, P.rhs_abs);
+  // RHS and the evaluation of the unprojected field (p = 0) in one pass; r = b there, so the residual
+  // norm is the RHS norm.
+  const unsigned g = reduce_grid(P.nc, kPartials);
+  k_rhs_zero_eval<<<g, kThreads, 0, s>>>(P.nc, P.cell_faces, P.face_rec, P.cell_volume, P.cell_target, P.cell_inwall,
+                                         P.cell_comp, P.cell_prev, P.rhs, P.rhs_abs, P.res, P.def, P.partial, P.perr);
   FL_HIP(hipGetLastError());
   FL_TRY(setup_coefficients(ctx, P));
-  const unsigned g = reduce_grid(P.nc);
-  k_sum_sq<<<g, kThreads, 0, s>>>(P.nc, P.rhs, P.partial);
-  k_finalize4<<<1, kThreads, 0, s>>>(P.partial, g, P.scal, 24);
-  FL_HIP(hipMemcpyAsync(P.hscal + 24, P.scal + 24, sizeof(double), hipMemcpyDeviceToHost, s));
-  FL_TRY(evaluate(ctx, P, P.zero, e0));  // defect of the unprojected field (p = 0)
-  const double rhs_norm = std::sqrt(P.hscal[24]);
+  FL_TRY(read_evaluation(ctx, P, g, e0));
+  const double rhs_norm = e0.l2;
   double limit = 2e-6 + 2e-6 * rhs_norm;
   if (!std::isfinite(rhs_norm) || !std::isfinite(limit)) return FL_OUT_OF_RANGE;
   if (rhs_norm > 0.0 && P.has_jump)
@@ -1874,7 +2115,7 @@ int check_options(const fl_pressure_options* o) {
   if (!o) return FL_INVALID_ARGUMENT;
   if (o->max_iterations < 1 || o->max_iterations > 256) return FL_INVALID_ARGUMENT;
   if (!(o->cpu_residual_tolerance_scale >= 1e-12 && o->cpu_residual_tolerance_scale <= 1.0)) return FL_INVALID_ARGUMENT;
-  if (o->preconditioner < 0 || o->preconditioner > 1) return FL_INVALID_ARGUMENT;
+  if (o->preconditioner < 0 || o->preconditioner > 1 || o->smoother < 0 || o->smoother > 2) return FL_INVALID_ARGUMENT;
   if (o->level_storage < 1 || o->level_storage > 2 || o->levels < 1 || o->levels > 16 || o->smoothing_sweeps < 1 || o->coarse_sweeps < 1)
     return FL_INVALID_ARGUMENT;
   if (!(o->coarse_scale > 0.0 && o->coarse_scale <= 1.0)) return FL_INVALID_ARGUMENT;

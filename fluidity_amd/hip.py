@@ -290,8 +290,10 @@ class Network:
 
 
 def pressure_options(max_iterations=256, tolerance_scale=1.0, relative_target=0.0, levels=4, sweeps=2,
-                     coarse_sweeps=16, coarse_scale=0.5, bf16_levels=False, learned=False):
+                     coarse_sweeps=16, coarse_scale=0.5, bf16_levels=False, learned=False, smoother=0):
+    """smoother: 0 level-wide red-black Gauss-Seidel, 1 block smoother on every level, 2 block smoother below
+    the finest level."""
     o = np.zeros(1, R.PRESSURE_OPTIONS)
     o[0] = (max_iterations, tolerance_scale, relative_target, levels, sweeps, coarse_sweeps, 2 if bf16_levels else 1,
-            coarse_scale, 1 if learned else 0, 0)
+            coarse_scale, 1 if learned else 0, int(smoother))
     return o

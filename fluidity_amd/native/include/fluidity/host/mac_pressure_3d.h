@@ -90,6 +90,8 @@ struct HipMacPressureSolverSettings3D {
   double coarse_scale = 0.5;                // coarse operator = scale * Galerkin crossing sums
   bool bf16_levels = false;                 // V-cycle coefficients stored in BF16, FP32 accumulation
   bool learned = false;  // flexible CG preconditioned by the runtime's trained network (fl_learn_*), V-cycle fallback
+  int smoother = 0;  // 0 level-wide red-black Gauss-Seidel; 1 red-black sweeps inside each 4^3 block with block
+                     // Jacobi between blocks; 2 level-wide on the finest level, per block below
 };
 
 struct HipMacPressureDiagnostics3D {

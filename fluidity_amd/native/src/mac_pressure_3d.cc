@@ -68,6 +68,7 @@ int Project(HipOperatorRuntime3D& rt, const MacPressureInputs3D& in, MacPressure
   o.coarse_sweeps = settings.coarse_sweeps;
   o.level_storage = settings.bf16_levels ? 2 : 1;
   o.preconditioner = settings.learned ? 1 : 0;
+  o.smoother = settings.smoother;
   o.coarse_scale = settings.coarse_scale;
   MacPressureProjection3D& p = out.projection;
   p.cell_pressures.assign(nc, 0.0);

@@ -39,7 +39,7 @@ PRESSURE_FACE = np.dtype([("axis", "i4"), ("lattice", INT3), ("open_area", "f8")
 PRESSURE_OPTIONS = np.dtype([("max_iterations", "u8"), ("cpu_residual_tolerance_scale", "f8"),
                              ("relative_divergence_target", "f8"), ("levels", "i4"), ("smoothing_sweeps", "i4"),
                              ("coarse_sweeps", "i4"), ("level_storage", "i4"), ("coarse_scale", "f8"),
-                             ("preconditioner", "i4"), ("pad", "i4")])
+                             ("preconditioner", "i4"), ("smoother", "i4")])
 PRESSURE_DIAGNOSTICS = np.dtype([("iteration_count", "u8"), ("multigrid_level_count", "u8"),
                                  ("true_residual_l2", "f8"), ("max_volume_scaled_residual", "f8"),
                                  ("max_cell_constraint_defect", "f8"), ("rhs_l2", "f8"),

@@ -130,7 +130,10 @@ typedef struct fl_pressure_options {
   double coarse_scale;                 /* coarse operator = scale * Galerkin crossing sums */
   int32_t preconditioner;              /* 0 multigrid V-cycle (PCG); 1 learned network (flexible CG), falling back
                                           to the V-cycle after 3 stagnating iterations (needs fl_learn_setup) */
-  int32_t pad;
+  int32_t smoother;                    /* 0 red-black Gauss-Seidel half sweeps over the whole level; 1 block
+                                          smoother: red-black sweeps inside each 4^3 block with the block's
+                                          neighbours held fixed (block Jacobi between blocks); 2 level-wide
+                                          sweeps on the finest level, the block smoother on coarser ones */
 } fl_pressure_options;
 
 typedef struct fl_pressure_diagnostics {

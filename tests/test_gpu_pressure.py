@@ -123,6 +123,33 @@ def test_bf16_level_storage_meets_the_same_gates(ctx):
     assert int(d["iteration_count"]) <= int(f32["diagnostics"]["iteration_count"]) + 3
 
 
+@pytest.mark.parametrize("bf16,smoother", [(False, 1), (True, 1), (True, 2)], ids=["fp32_block", "bf16_block",
+                                                                               "bf16_hybrid"])
+def test_block_smoother_meets_the_gates_deterministically(ctx, bf16, smoother):
+    # Red-black sweeps inside each 4^3 block with the block's neighbours held fixed: a different
+    # (symmetric) preconditioner, so the accepted projection meets the same FP64 gates, agrees with the
+    # FP64 oracle, and repeats bit for bit (no block reads a halo that another block is writing).
+    sc = scenes.dam_break(64, seed=1)
+    st = K.oracle_step_inputs(sc)
+    args = (st["geom"], st["blocks"], st["dt"], st["cells"], st["faces"])
+    s, ref = O.project(*args, O.pressure_options(relative_target=1e-12, fp32=False, levels=5))
+    assert s == O.OK
+    s, rb = ctx.project(*args, H.pressure_options(relative_target=1e-6, levels=5, coarse_sweeps=2, bf16_levels=bf16))
+    assert s == H.OK
+    opts = H.pressure_options(relative_target=1e-6, levels=5, coarse_sweeps=2, bf16_levels=bf16, smoother=smoother)
+    s, blk = ctx.project(*args, opts)
+    assert s == H.OK
+    s, again = ctx.project(*args, opts)
+    assert s == H.OK
+    d = blk["diagnostics"]
+    print("iterations rbgs", rb["diagnostics"]["iteration_count"], "block", d["iteration_count"], flush=True)
+    assert d["defect_l2"] <= 1e-6 * d["initial_defect_l2"]
+    p_err = np.abs(blk["pressures"] - ref["pressures"]).max() / np.abs(ref["pressures"]).max()
+    assert p_err <= 1e-5
+    assert np.array_equal(blk["pressures"], again["pressures"])
+    assert int(d["iteration_count"]) <= 2 * int(rb["diagnostics"]["iteration_count"]) + 4
+
+
 def test_device_step_matches_oracle_pipeline(ctx):
     sc = scenes.dam_break(24, seed=2, random_velocity=True)
     st = K.oracle_step_inputs(sc)
