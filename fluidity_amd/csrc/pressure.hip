@@ -1563,7 +1563,6 @@ int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates&
   const uint64_t n = L0.nb * 64;
   const unsigned g = grid_for(n);
   const unsigned rg = reduce_grid(n);
-  const unsigned rgw = reduce_grid(n, kPartials);  // one cell per lane for the direction step's gathers
   diag.multigrid_level_count = P.levels.size();
   diag.iteration_count = 0;
   diag.restarts = 0;
@@ -1633,10 +1632,10 @@ int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates&
     }
     k_dot<<<rg, kThreads, 0, s>>>(n, P.r32, L0.x, P.partial);
     k_finalize_step<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, P.fscal, P.perr, use_net ? 2 : 1, restart ? 1 : 0);
-    k_direction_apply<<<rgw, kThreads, 0, s>>>(n, L0.nbr, L0.k, L0.anchor, L0.diag, L0.x, P.d32, P.dn32, P.q32,
-                                               P.fscal, restart ? 1 : 0, P.partial);
+    k_direction_apply<<<rg, kThreads, 0, s>>>(n, L0.nbr, L0.k, L0.anchor, L0.diag, L0.x, P.d32, P.dn32, P.q32,
+                                              P.fscal, restart ? 1 : 0, P.partial);
     std::swap(P.d32, P.dn32);
-    k_finalize_step<<<1, kThreads, 0, s>>>(P.partial, rgw, P.scal, P.fscal, P.perr, use_net ? 4 : 3, 0);
+    k_finalize_step<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, P.fscal, P.perr, use_net ? 4 : 3, 0);
     k_update_xr<<<rg, kThreads, 0, s>>>(n, P.d32, P.q32, L0.diag, P.fine_vol, P.x32, P.r32, P.fscal, P.partial,
                                         P.perr);
     k_finalize4<<<1, kThreads, 0, s>>>(P.partial, rg, P.scal, 8);
