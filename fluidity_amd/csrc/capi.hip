@@ -147,16 +147,71 @@ void fill_measurement(fl_ctx* ctx, fl_measurement* m, double kernel_ms, double c
   std::snprintf(m->architecture, sizeof(m->architecture), "%s", ctx->arch);
 }
 
+// Device copies of the two velocity histories of a boundary gather, the particle side tags and affine
+// switches, and the continuation list; allocated on the first boundary gather and grown with it.
+struct BoundaryBuffers {
+  uint64_t faces = 0, particles = 0;
+  double* u[2] = {};
+  uint8_t* validity[2] = {};
+  int64_t* side[2] = {};
+  uint8_t* eligible[2] = {};
+  fl_wall_sample* walls[2] = {};
+  int64_t* pside = nullptr;
+  uint8_t* paffine = nullptr;
+  uint32_t* slow = nullptr;
+  uint32_t* slow_count = nullptr;
+  void release() {
+    void* ptrs[] = {u[0], u[1], validity[0], validity[1], side[0], side[1], eligible[0], eligible[1], walls[0], walls[1],
+                    pside, paffine, slow, slow_count};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    *this = BoundaryBuffers{};
+  }
+};
+
+void destroy_boundary(fl_ctx* ctx) {
+  auto* b = static_cast<BoundaryBuffers*>(ctx->boundary);
+  if (!b) return;
+  b->release();
+  delete b;
+  ctx->boundary = nullptr;
+}
+
+__global__ void k_split_history(const fl_face_tag* tags, uint64_t n, int64_t* side, uint8_t* eligible) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  side[i] = tags[i].side_tag;
+  eligible[i] = tags[i].structurally_eligible;
+}
+
+__global__ void k_pack_boundary_gather(const double* cols, uint64_t cap, const double* extra, uint64_t extra_cap,
+                                       const uint32_t* perm, uint64_t n, fl_boundary_gather* out) {
+  const uint64_t k = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (k >= n) return;
+  fl_boundary_gather& r = out[perm[k]];
+  double* d = reinterpret_cast<double*>(&r);
+  // new PIC (0-2), increment (3-5), then the fitted slopes and the gradient after the retention (6-8).
+  for (int c = 0; c < 6; ++c) d[c] = cols[c * cap + k];
+  for (int c = 0; c < 3; ++c) d[6 + c] = extra[c * extra_cap + k];
+  for (int c = 6; c < 24; ++c) d[3 + c] = cols[c * cap + k];
+  const int flags = static_cast<int>(extra[3 * extra_cap + k]);
+  r.affine_enabled = flags & 1;
+  r.old_pic_fallback = (flags >> 1) & 1;
+  r.new_pic_fallback = (flags >> 2) & 1;
+  r.low_mass_fallback = (flags >> 3) & 1;
+  for (int c = 0; c < 4; ++c) r.pad[c] = 0;
+}
+
 int first_error_code(fl_ctx* ctx) {
   uint64_t w;
   FL_TRY(read_error(ctx, &w));
   return w == ~uint64_t(0) ? FL_OK : static_cast<int>(w & 0xff);
 }
 
-// Caller records (the 168 B compatible-deposit particles are the largest), face quanta, or one
-// particle's 81 compatible-stencil records, whichever is largest.
+// Caller records (the 224 B boundary-gather results are the largest), face quanta, or one particle's
+// 81 compatible-stencil records, whichever is largest.
 uint64_t staging_bytes(const fl_ctx* c) {
-  return std::max<uint64_t>({sizeof(fl_boundary_particle) * std::max<uint64_t>(1, c->cap.particles),
+  return std::max<uint64_t>({sizeof(fl_boundary_gather) * std::max<uint64_t>(1, c->cap.particles),
                              24 * 192 * std::max<uint64_t>(1, c->cap.blocks), 81 * sizeof(fl_compatible_sample)});
 }
 
@@ -269,6 +324,7 @@ void fl_destroy(fl_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   fl::destroy_pressure(ctx);
   fl::destroy_learn(ctx);
+  destroy_boundary(ctx);
   release(ctx);
   for (hipEvent_t e : ctx->stage)
     if (e) (void)hipEventDestroy(e);
@@ -530,6 +586,153 @@ int fl_p2g_compatible(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* b
   if (nf && stencils) std::copy(quanta_host.begin(), quanta_host.end(), out);
   copy_ms += c.stop();
   fill_measurement(ctx, measurement, kernel_ms, copy_ms, 11);
+  return FL_OK;
+}
+
+// ValidateHistory (liquid_boundary_transfer_3d.cc:97-145), in the reference's order.
+static int validate_history(const fl_velocity_history* h, size_t face_count, double stage_time) {
+  if (h->velocity_sample_time != stage_time || h->wall_sample_time != stage_time) return FL_INVALID_ARGUMENT;
+  if (h->count != face_count) return FL_INVALID_ARGUMENT;
+  if (face_count && (!h->face_velocities || !h->validity || !h->face_tags || !h->wall_samples))
+    return FL_INVALID_ARGUMENT;
+  for (size_t s = 0; s < face_count; ++s) {
+    const uint8_t v = h->validity[s];
+    const bool usable = v == 1 || v == 2;
+    if (v != 0 && v != 3 && !usable) return FL_INVALID_ARGUMENT;
+    if (usable && !std::isfinite(h->face_velocities[s])) return FL_INVALID_ARGUMENT;
+    const fl_wall_sample& w = h->wall_samples[s];
+    if (w.available) {
+      const fl_vec3 n = w.unit_normal, u = w.wall_velocity;
+      if (!std::isfinite(u.x) || !std::isfinite(u.y) || !std::isfinite(u.z) || !std::isfinite(n.x) ||
+          !std::isfinite(n.y) || !std::isfinite(n.z))
+        return FL_INVALID_ARGUMENT;
+      if (std::fabs(((n.x * n.x + n.y * n.y) + n.z * n.z) - 1.0) > 1e-12) return FL_INVALID_ARGUMENT;
+    }
+  }
+  return FL_OK;
+}
+
+int fl_g2p_boundary(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
+                    const fl_boundary_particle* particles, size_t n, double old_time, double new_time,
+                    const fl_velocity_history* oh, const fl_velocity_history* nh, fl_boundary_gather* out,
+                    uint64_t* out_counts, fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  FL_TRY(check_geometry(geometry));
+  FL_TRY(check_blocks(blocks, block_count));
+  if (n > ctx->cap.particles || block_count > ctx->cap.blocks) return FL_RESOURCE_EXHAUSTED;
+  if (!oh || !nh || (n && (!particles || !out))) return FL_INVALID_ARGUMENT;
+  const uint64_t nf = 192 * block_count;
+  // Stage times, then each history (liquid_boundary_transfer_3d.cc:528-545).
+  if (!std::isfinite(old_time) || !std::isfinite(new_time) || new_time <= old_time) return FL_INVALID_ARGUMENT;
+  FL_TRY(validate_history(oh, nf, old_time));
+  FL_TRY(validate_history(nh, nf, new_time));
+  const size_t ob = n * sizeof(fl_boundary_gather);
+  if (overlaps(out, ob, particles, n * sizeof(fl_boundary_particle))) return FL_INVALID_ARGUMENT;
+  for (const fl_velocity_history* h : {oh, nh})
+    if (overlaps(out, ob, h->face_velocities, nf * sizeof(double)) || overlaps(out, ob, h->validity, nf) ||
+        overlaps(out, ob, h->face_tags, nf * sizeof(fl_face_tag)) ||
+        overlaps(out, ob, h->wall_samples, nf * sizeof(fl_wall_sample)))
+      return FL_INVALID_ARGUMENT;
+  const bool same = oh->face_velocities == nh->face_velocities && oh->validity == nh->validity &&
+                    oh->face_tags == nh->face_tags && oh->wall_samples == nh->wall_samples;
+  (void)hipSetDevice(ctx->device);
+  set_geometry(ctx, geometry);
+  // History and particle-attribute buffers: allocated on the first boundary gather, grown with it.
+  if (!ctx->boundary) ctx->boundary = new (std::nothrow) BoundaryBuffers();
+  auto* B = static_cast<BoundaryBuffers*>(ctx->boundary);
+  if (!B) return FL_RESOURCE_EXHAUSTED;
+  const uint64_t need_f = std::max<uint64_t>(nf, 1), need_p = std::max<uint64_t>(ctx->cap.particles, 1);
+  if (B->faces < need_f || B->particles < need_p) {
+    B->release();
+    for (int k = 0; k < 2; ++k) {
+      FL_TRY(dalloc(&B->u[k], need_f));
+      FL_TRY(dalloc(&B->validity[k], need_f));
+      FL_TRY(dalloc(&B->side[k], need_f));
+      FL_TRY(dalloc(&B->eligible[k], need_f));
+      FL_TRY(dalloc(&B->walls[k], need_f));
+    }
+    FL_TRY(dalloc(&B->pside, need_p));
+    FL_TRY(dalloc(&B->paffine, need_p));
+    FL_TRY(dalloc(&B->slow, need_p));
+    FL_TRY(dalloc(&B->slow_count, 1));
+    B->faces = need_f;
+    B->particles = need_p;
+  }
+  double copy_ms = 0;
+  FL_TRY(build_lookup(ctx, reinterpret_cast<const int32_t*>(blocks), block_count));
+  {
+    Timer c(ctx);
+    const fl_velocity_history* hs[2] = {oh, nh};
+    for (int k = 0; k < (same ? 1 : 2) && nf; ++k) {
+      FL_HIP(hipMemcpyAsync(B->u[k], hs[k]->face_velocities, nf * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+      FL_HIP(hipMemcpyAsync(B->validity[k], hs[k]->validity, nf, hipMemcpyHostToDevice, ctx->stream));
+      FL_HIP(hipMemcpyAsync(B->walls[k], hs[k]->wall_samples, nf * sizeof(fl_wall_sample), hipMemcpyHostToDevice,
+                            ctx->stream));
+      // Tags are staged in the face-quanta buffer (24 B per slot), then split into side and eligibility.
+      FL_HIP(hipMemcpyAsync(ctx->quanta_aos, hs[k]->face_tags, nf * sizeof(fl_face_tag), hipMemcpyHostToDevice,
+                            ctx->stream));
+      k_split_history<<<grid_for(nf), kThreads, 0, ctx->stream>>>(reinterpret_cast<const fl_face_tag*>(ctx->quanta_aos),
+                                                                  nf, B->side[k], B->eligible[k]);
+      FL_HIP(hipGetLastError());
+    }
+    copy_ms += c.stop();
+  }
+  Timer t(ctx);
+  auto* staged = reinterpret_cast<fl_boundary_particle*>(ctx->staging_aos);
+  {
+    Timer c(ctx);
+    if (n) FL_HIP(hipMemcpyAsync(staged, particles, n * sizeof(fl_boundary_particle), hipMemcpyHostToDevice, ctx->stream));
+    copy_ms += c.stop();
+  }
+  if (n) {
+    k_split_boundary<<<grid_for(n), kThreads, 0, ctx->stream>>>(staged, n, ctx->raw.base, ctx->raw.capacity, B->pside,
+                                                                B->paffine);
+    FL_HIP(hipGetLastError());
+  }
+  ctx->raw.trial_aliases_sorted = false;
+  FL_TRY(bin_particles_mode(ctx, ctx->raw.view(), n, kBinBoundaryGather));
+  if (ctx->bin_box.volume() > ctx->cap.lookup_blocks) return FL_RESOURCE_EXHAUSTED;
+  FL_TRY(sort_particles(ctx, n, ctx->raw.view(), true));
+  BoundaryDevice d;
+  for (int k = 0; k < 2; ++k) {
+    const int src = same ? 0 : k;
+    d.u[k] = B->u[src];
+    d.validity[k] = B->validity[src];
+    d.side[k] = B->side[src];
+    d.eligible[k] = B->eligible[src];
+    d.walls[k] = B->walls[src];
+  }
+  d.same = same ? 1 : 0;
+  d.pside = B->pside;
+  d.paffine = B->paffine;
+  d.extra = ctx->raw.base;  // the caller-order columns are free once the particles are sorted
+  d.extra_cap = ctx->raw.capacity;
+  d.slow = B->slow;
+  d.slow_count = B->slow_count;
+  uint32_t slow = 0;
+  FL_TRY(g2p_boundary(ctx, ctx->nvalid, d, &slow));
+  FL_TRY(first_error_code(ctx));
+  auto* packed = reinterpret_cast<fl_boundary_gather*>(ctx->staging_aos);
+  if (ctx->nvalid) {
+    k_pack_boundary_gather<<<grid_for(ctx->nvalid), kThreads, 0, ctx->stream>>>(
+        ctx->gather, ctx->sorted.capacity, ctx->raw.base, ctx->raw.capacity, ctx->perm, ctx->nvalid, packed);
+    FL_HIP(hipGetLastError());
+  }
+  const double kernel_ms = t.stop() - copy_ms;
+  Timer c(ctx);
+  if (n) FL_HIP(hipMemcpyAsync(out, packed, ob, hipMemcpyDeviceToHost, ctx->stream));
+  FL_HIP(hipStreamSynchronize(ctx->stream));
+  copy_ms += c.stop();
+  if (out_counts) {
+    out_counts[0] = out_counts[1] = out_counts[2] = 0;
+    for (size_t i = 0; i < n; ++i) {
+      out_counts[0] += out[i].old_pic_fallback;
+      out_counts[1] += out[i].new_pic_fallback;
+      out_counts[2] += out[i].low_mass_fallback;
+    }
+  }
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, 12);
+  (void)slow;
   return FL_OK;
 }
 

@@ -154,6 +154,8 @@ struct fl_ctx {
   void* pressure = nullptr;
   // learned preconditioner state (fl::LearnState, owned by learn.hip)
   void* learn = nullptr;
+  // boundary-gather history buffers (owned by capi.hip, allocated on first use)
+  void* boundary = nullptr;
 };
 
 namespace fl {
@@ -173,8 +175,9 @@ inline unsigned grid_for(uint64_t n, int threads = kThreads) {
 // Topology (topology.hip)
 int bin_particles(fl_ctx* ctx, const ParticleView& p, uint64_t n, bool deposit_checks, bool gather_checks);
 // Bin with the input checks of one caller: plain bin query, quantized/FP64 deposit (mass, velocity,
-// slopes), gather, or compatible deposit (mass and velocity; enabled slope rows are checked per axis).
-enum { kBinPlain = 0, kBinDeposit = 1, kBinGather = 2, kBinCompatible = 3 };
+// slopes), gather, compatible deposit (mass and velocity; enabled slope rows are checked per axis), or
+// boundary gather (velocity).
+enum { kBinPlain = 0, kBinDeposit = 1, kBinGather = 2, kBinCompatible = 3, kBinBoundaryGather = 4 };
 int bin_particles_mode(fl_ctx* ctx, const ParticleView& p, uint64_t n, int mode);
 int sort_particles(fl_ctx* ctx, uint64_t n, const ParticleView& src, bool permute, uint32_t permute_columns = ~0u);
 int build_lookup(fl_ctx* ctx, const int32_t* host_blocks, uint64_t nb);
@@ -195,6 +198,26 @@ int unpack_particles(fl_ctx* ctx, uint64_t n);
 // particle side tags and affine switches in caller order (device arrays).
 int p2g_compatible(fl_ctx* ctx, uint64_t n, const int64_t* slot_side, const uint8_t* slot_ok, const int64_t* side,
                    const uint8_t* affine, double inv_qm, double inv_qp, double limit);
+// Device arrays of one boundary gather: per history (0 old, 1 new) the face velocities, validity,
+// side tags, eligibility and walls; particle side tags and affine switches in caller order; the
+// retention and flag columns; the continuation list and its counter.
+struct BoundaryDevice {
+  const double* u[2];
+  const uint8_t* validity[2];
+  const int64_t* side[2];
+  const uint8_t* eligible[2];
+  const fl_wall_sample* walls[2];
+  int same;
+  const int64_t* pside;
+  const uint8_t* paffine;
+  double* extra;
+  uint64_t extra_cap;
+  uint32_t* slow;
+  uint32_t* slow_count;
+};
+// Boundary gather of the sorted particles into ctx->gather (24 columns) and d.extra (4 columns);
+// *slow_count receives the number of particles that needed a continuation.
+int g2p_boundary(fl_ctx* ctx, uint64_t n, const BoundaryDevice& d, uint32_t* slow_count);
 // Compatible stencils of caller particles [lo, lo + count) from ctx->raw and ctx->bin (81 per particle).
 int compatible_stencils(fl_ctx* ctx, uint64_t lo, uint64_t count, const int64_t* slot_side, const uint8_t* slot_ok,
                         const int64_t* side, const uint8_t* affine, fl_compatible_sample* d_out);

@@ -54,6 +54,9 @@ __global__ void k_bin(ParticleView p, uint64_t n, Geometry g, int32_t* bins, uin
   for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
     int32_t b0 = 0x7fffffff, b1 = 0x7fffffff, b2 = 0x7fffffff;
     int st = kOk;
+    if (kMode == 4) {  // boundary gather: the retained velocity must be finite
+      if (!(finite(p.v[0][i]) && finite(p.v[1][i]) && finite(p.v[2][i]))) st = kInvalidArgument;
+    }
     if (kMode == 1 || kMode == 3) {
       bool good = finite(p.m[i]) && p.m[i] >= 0.0;
 #pragma unroll
@@ -485,12 +488,12 @@ int bin_particles(fl_ctx* ctx, const ParticleView& p, uint64_t n, bool deposit_c
 
 int bin_particles_mode(fl_ctx* ctx, const ParticleView& p, uint64_t n, int mode) {
   FL_TRY(reset_error(ctx));
-  if (mode < kBinPlain || mode > kBinCompatible) return FL_INTERNAL;
+  if (mode < kBinPlain || mode > kBinBoundaryGather) return FL_INTERNAL;
   if (n) {
     const bool pow2 = ctx->geometry.inv_h != 0.0;
     using K = decltype(&k_bin<0, true>);
-    static const K table[2][4] = {{k_bin<0, false>, k_bin<1, false>, k_bin<2, false>, k_bin<3, false>},
-                                  {k_bin<0, true>, k_bin<1, true>, k_bin<2, true>, k_bin<3, true>}};
+    static const K table[2][5] = {{k_bin<0, false>, k_bin<1, false>, k_bin<2, false>, k_bin<3, false>, k_bin<4, false>},
+                                  {k_bin<0, true>, k_bin<1, true>, k_bin<2, true>, k_bin<3, true>, k_bin<4, true>}};
     auto* kernel = table[pow2 ? 1 : 0][mode];
     const unsigned groups = std::min<unsigned>(grid_for(n), kBinGroups);
     int32_t* partial = reinterpret_cast<int32_t*>(ctx->scan_partial);  // 6 * kBinGroups ints

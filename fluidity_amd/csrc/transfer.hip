@@ -912,6 +912,486 @@ __global__ __launch_bounds__(kThreads) void k_compatible_stencils(ParticleView p
                                                  compat, 1.0, 1.0, 0x1p62, nullptr, nullptr, out + 81 * i, kOk);
 }
 
+// ---- Boundary gather (GatherLiquidBoundaryParticles3D: GatherCore, liquid_boundary_transfer_3d.cc:519-651) ----
+// Fast path (k_g2p_boundary, one workgroup per bin): when every stencil sample of a particle is
+// structurally eligible on its side and valid or predicted in both histories, the filled fields are the
+// histories themselves, and the gather is GatherMac3D's plus the retention residual v - old PIC (old
+// PIC summed sample by sample in stencil order, as the reference does) and the low-mass flag. A particle
+// with any other sample goes to a list that k_g2p_ghost completes one particle per lane with the
+// reference's continuation: same-side donors within two cell widths in owned-slot order, the
+// rank-revealing affine fit, the nearest donor when rank-deficient, the wall, and the normal
+// replacement at oblique walls.
+
+struct HistoryView {
+  const double* u;
+  const uint8_t* validity;  // 0 missing, 1 valid, 2 predicted, 3 low mass
+  const int64_t* side;
+  const uint8_t* eligible;
+  const fl_wall_sample* walls;
+};
+
+struct BoundaryGatherArgs {
+  HistoryView old_h, new_h;
+  int same;                // one set of samples serves both histories
+  const int64_t* pside;    // particle side tags, caller order
+  const uint8_t* paffine;  // particle affine switches, caller order
+  double* extra;           // retention x, y, z and the flag word, sorted order, stride `extra_cap`
+  uint64_t extra_cap;
+  uint32_t* slow;          // sorted indices that need a continuation
+  uint32_t* slow_count;
+};
+
+__device__ inline bool usable_validity(uint8_t v) { return v == 1 || v == 2; }
+__device__ inline int32_t floor_div4_i32(int32_t v) { return v >> 2; }  // arithmetic shift: floor division
+
+// Flag word: bit 0 affine enabled, 1 old-PIC fallback, 2 new-PIC fallback, 3 low-mass fallback.
+__device__ inline void write_extra(const BoundaryGatherArgs& a, uint64_t k, const double delta[3], int flags) {
+  a.extra[0 * a.extra_cap + k] = delta[0];
+  a.extra[1 * a.extra_cap + k] = delta[1];
+  a.extra[2 * a.extra_cap + k] = delta[2];
+  a.extra[3 * a.extra_cap + k] = static_cast<double>(flags);
+}
+
+template <bool kTrial, bool kPow2>
+__global__ __launch_bounds__(kThreads) void k_g2p_boundary(ParticleView p, BinList bins, GridView grid,
+                                                           BoundaryGatherArgs a, const uint32_t* perm, double* out,
+                                                           uint64_t cap, uint64_t* err) {
+  __shared__ double wo[kWindow], wn[kWindow];
+  __shared__ int64_t so[kWindow], sn[kWindow];
+  // Per window slot: bit 0 old usable, 1 old eligible, 2 old low mass, 3-5 the same for the new history.
+  __shared__ uint8_t wflag[kWindow];
+  __shared__ int32_t wblock[8];
+  const uint32_t b = xcd_chunked(blockIdx.x, gridDim.x);
+  int32_t bin[3];
+  bin_coords(bins, bins.key[b], bin);
+  if (threadIdx.x < 8)
+    wblock[threadIdx.x] = grid.ordinal(bin[0] + (threadIdx.x & 1), bin[1] + ((threadIdx.x >> 1) & 1),
+                                       bin[2] + (threadIdx.x >> 2));
+  __syncthreads();
+  for (int i = threadIdx.x; i < kWindow; i += kThreads) {
+    const int32_t ord = wblock[i / 192];
+    const uint64_t slot = uint64_t(ord) * 192 + (i % 192);
+    double uo = 0.0, un = 0.0;
+    int64_t sdo = 0, sdn = 0;
+    int f = 0;
+    if (ord >= 0) {
+      const uint8_t vo = a.old_h.validity[slot], vn = a.new_h.validity[slot];
+      uo = a.old_h.u[slot];
+      un = a.new_h.u[slot];
+      sdo = a.old_h.side[slot];
+      sdn = a.new_h.side[slot];
+      f = (usable_validity(vo) ? 1 : 0) | (a.old_h.eligible[slot] ? 2 : 0) | (vo == 3 ? 4 : 0) |
+          (usable_validity(vn) ? 8 : 0) | (a.new_h.eligible[slot] ? 16 : 0) | (vn == 3 ? 32 : 0);
+    }
+    wo[i] = uo;
+    wn[i] = un;
+    so[i] = sdo;
+    sn[i] = sdn;
+    wflag[i] = static_cast<uint8_t>(f);
+  }
+  __syncthreads();
+  const Geometry g = grid.g;
+  const double inv_h = g.inv_h != 0.0 ? g.inv_h : 1.0 / g.h;
+  const uint32_t off = bins.offset[b], cnt = bins.count[b];
+  const int32_t tile[3] = {4 * bin[0], 4 * bin[1], 4 * bin[2]};
+  for (uint32_t k = off + threadIdx.x; k < off + cnt; k += kThreads) {
+    const uint32_t o = perm[k];
+    const int64_t side = a.pside[o];
+    double t[3] = {p.s[0][k], p.s[1][k], p.s[2][k]};
+    int code = kOk;
+    if (kTrial) {
+      const double s0 = t[0], s1 = t[1], s2 = t[2];
+      t[0] = p.t[0][k];
+      t[1] = p.t[1][k];
+      t[2] = p.t[2][k];
+      if (!(within_half_cell(s0, t[0], g.h) && within_half_cell(s1, t[1], g.h) && within_half_cell(s2, t[2], g.h)))
+        code = kFailedPrecondition;
+    }
+    const auto cx = normalize<kPow2>(t[0], g.ox, g.h, g.inv_h);
+    const auto cy = normalize<kPow2>(t[1], g.oy, g.h, g.inv_h);
+    const auto cz = normalize<kPow2>(t[2], g.oz, g.h, g.inv_h);
+    // First pass (the reference's first loop): stencil status, whether any sample needs a continuation,
+    // the low-mass flag and the old PIC value sample by sample.
+    bool fill = false, low = false;
+    double old_pic[3] = {0.0, 0.0, 0.0};
+    for (int A = 0; A < 3 && code == kOk && !fill; ++A) {
+      Axis3 X, Y, Z;
+      const int c0 = axis3<false>(cx, A == 0 ? 0.0 : 0.5, bin[0], g.h, X);
+      const int c1 = axis3<false>(cy, A == 1 ? 0.0 : 0.5, bin[1], g.h, Y);
+      const int c2 = axis3<false>(cz, A == 2 ? 0.0 : 0.5, bin[2], g.h, Z);
+      code = axis_status(c0, c1, c2, X, Y, Z);
+      if (code == kOk && !window_ok(wblock, axis_blocks(X.first, tile[0]), axis_blocks(Y.first, tile[1]),
+                                    axis_blocks(Z.first, tile[2])))
+        code = kFailedPrecondition;
+      if (code != kOk) break;
+      double sum = 0.0;
+      for (int zi = 0; zi < 3; ++zi)
+        for (int yi = 0; yi < 3; ++yi)
+          for (int xi = 0; xi < 3; ++xi) {
+            const int rx = X.first + xi - tile[0], ry = Y.first + yi - tile[1], rz = Z.first + zi - tile[2];
+            const int wi = (((rz >> 2) * 2 + (ry >> 2)) * 2 + (rx >> 2)) * 192 + A * 64 +
+                           ((rz & 3) * 16 + (ry & 3) * 4 + (rx & 3));
+            const int f = wflag[wi];
+            const bool ok_old = (f & 1) && (f & 2) && so[wi] == side;
+            const bool ok_new = a.same ? ok_old : ((f & 8) && (f & 16) && sn[wi] == side);
+            fill = fill || !ok_old || !ok_new;
+            low = low || (f & 4) || (f & 32);
+            sum += ((X.w[xi] * Y.w[yi]) * Z.w[zi]) * wo[wi];
+          }
+      old_pic[A] = sum;
+    }
+    if (code == kOk && fill) {
+      a.slow[atomicAdd(a.slow_count, 1u)] = k;
+      continue;
+    }
+    // Second pass: GatherMac3D on the (unfilled) histories.
+    if (code == kOk) {  // every stencil passed its checks in the first pass
+      Axis3 tx, ty, tz, nx, ny, nz;
+      axis3<false>(cx, 0.5, bin[0], g.h, tx);
+      axis3<false>(cy, 0.5, bin[1], g.h, ty);
+      axis3<false>(cz, 0.5, bin[2], g.h, tz);
+      axis3<false>(cx, 0.0, bin[0], g.h, nx);
+      axis3<false>(cy, 0.0, bin[1], g.h, ny);
+      axis3<false>(cz, 0.0, bin[2], g.h, nz);
+      bool ok = gather_axis<0>(nx, ty, tz, tile, inv_h, wn, wo, out, cap, k);
+      ok = gather_axis<1>(tx, ny, tz, tile, inv_h, wn, wo, out, cap, k) && ok;
+      ok = gather_axis<2>(tx, ty, nz, tile, inv_h, wn, wo, out, cap, k) && ok;
+      if (!ok) code = kOutOfRange;
+    }
+    if (code == kOk) {
+      const bool affine = a.paffine[o] != 0;
+      double delta[3] = {0.0, 0.0, 0.0};
+      if (!low)
+        for (int c = 0; c < 3; ++c) delta[c] = p.v[c][k] - old_pic[c];
+      if (!(finite(delta[0]) && finite(delta[1]) && finite(delta[2]))) code = kOutOfRange;
+      write_extra(a, k, delta, (affine && !low ? 1 : 0) | (low ? 8 : 0));
+    }
+    if (code) report(err, o, code);
+  }
+}
+
+// ---- the continuation path ----
+constexpr int kMaxGhostRows = 40;  // a radius-two sphere holds at most 36 staggered anchors
+
+__device__ inline bool compatible_slot(const HistoryView& h, uint64_t s, int64_t side) {
+  return h.eligible[s] != 0 && h.side[s] == side;
+}
+
+// Anchor of owned slot s in cell widths (BuildOwnedAnchors, liquid_boundary_transfer_3d.cc:152-177).
+__device__ inline void slot_anchor(const GridView& grid, uint64_t s, double at[3]) {
+  const uint64_t b = s / 192;
+  const int r = static_cast<int>(s % 192), axis = r / 64, l = r % 64;
+  const int lc[3] = {l & 3, (l >> 2) & 3, l >> 4};
+  for (int d = 0; d < 3; ++d) at[d] = 4.0 * grid.blocks[3 * b + d] + lc[d] + (axis == d ? 0.0 : 0.5);
+}
+
+// FitBoundedLiquidGhostAffineCore3D (base/liquid_boundary_transfer_math_3d.h:34-144): column-pivoted
+// Householder QR of rows [1, dx, dy, dz] against velocities scaled by their largest magnitude, with
+// norms recomputed by hypot accumulation, strict pivot comparison and rank loss below 1e-10 of the
+// largest pivot. *full is false for fewer than 4 rows or rank loss; returns OutOfRange for non-finite
+// arithmetic.
+__device__ int ghost_fit(int n, double (*m)[4], double* rhs, double* refl, double* at_target, bool* full) {
+  *full = false;
+  double scale = 0.0;
+  for (int r = 0; r < n; ++r) scale = fmax(scale, fabs(rhs[r]));
+  if (n < 4) return kOk;
+  if (scale == 0.0) scale = 1.0;
+  for (int r = 0; r < n; ++r) rhs[r] = rhs[r] / scale;
+  int order[4] = {0, 1, 2, 3};
+  double largest = 0.0;
+  for (int pv = 0; pv < 4; ++pv) {
+    int sel = pv;
+    double sel_norm = 0.0;
+    for (int c = pv; c < 4; ++c) {
+      double norm = 0.0;
+      for (int r = pv; r < n; ++r) norm = hypot(norm, m[r][c]);
+      if (norm > sel_norm) {
+        sel_norm = norm;
+        sel = c;
+      }
+    }
+    if (!finite(sel_norm)) return kOutOfRange;
+    largest = fmax(largest, sel_norm);
+    if (sel_norm == 0.0 || sel_norm < 1e-10 * largest) return kOk;
+    const int tmp = order[pv];
+    order[pv] = order[sel];
+    order[sel] = tmp;
+    for (int r = 0; r < n; ++r) {
+      const double v = m[r][pv];
+      m[r][pv] = m[r][sel];
+      m[r][sel] = v;
+    }
+    const double diag = -copysign(sel_norm, m[pv][pv]);
+    double rnorm = 0.0;
+    for (int r = pv; r < n; ++r) {
+      refl[r] = m[r][pv] / sel_norm;
+      if (r == pv) refl[r] -= diag / sel_norm;
+      rnorm = hypot(rnorm, refl[r]);
+    }
+    for (int r = pv; r < n; ++r) refl[r] /= rnorm;
+    for (int c = pv + 1; c < 4; ++c) {
+      double proj = 0.0;
+      for (int r = pv; r < n; ++r) proj += refl[r] * m[r][c];
+      for (int r = pv; r < n; ++r) {
+        m[r][c] -= 2.0 * refl[r] * proj;
+        if (!finite(m[r][c])) return kOutOfRange;
+      }
+    }
+    double rproj = 0.0;
+    for (int r = pv; r < n; ++r) rproj += refl[r] * rhs[r];
+    for (int r = pv; r < n; ++r) rhs[r] -= 2.0 * refl[r] * rproj;
+    m[pv][pv] = diag;
+  }
+  double pc[4] = {0.0, 0.0, 0.0, 0.0}, coef[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int pv = 3; pv >= 0; --pv) {
+    double res = rhs[pv];
+    for (int c = pv + 1; c < 4; ++c) res -= m[pv][c] * pc[c];
+    pc[pv] = res / m[pv][pv];
+    coef[order[pv]] = pc[pv] * scale;
+    if (!finite(coef[order[pv]])) return kOutOfRange;
+  }
+  *full = true;
+  *at_target = coef[0];
+  return kOk;
+}
+
+__device__ inline double vec_get(const fl_vec3& v, int c) { return c == 0 ? v.x : (c == 1 ? v.y : v.z); }
+
+// ContinueComponent (liquid_boundary_transfer_3d.cc:181-240) on the lattice neighbourhood of the target:
+// candidates are visited block by block in ordinal order and cell by cell in (z,y,x) order, which is
+// the owned-slot order the reference scans.
+__device__ __noinline__ int continue_component(const GridView& grid, const HistoryView& h, const double t[3], int c,
+                                               int64_t side,
+                                               const fl_wall_sample& wall, double* velocity, bool* fallback) {
+  double m[kMaxGhostRows][4], rhs[kMaxGhostRows], refl[kMaxGhostRows];
+  int n = 0;
+  uint64_t nearest = ~uint64_t(0);
+  double nearest_d2 = __builtin_huge_val();
+  int64_t lo[3], hi[3];
+  int32_t blo[3], bhi[3];
+  for (int d = 0; d < 3; ++d) {
+    const double sh = d == c ? 0.0 : 0.5;
+    lo[d] = static_cast<int64_t>(ceil(t[d] - sh - 2.0));
+    hi[d] = static_cast<int64_t>(floor(t[d] - sh + 2.0));
+    blo[d] = floor_div4_i32(static_cast<int32_t>(lo[d]));
+    bhi[d] = floor_div4_i32(static_cast<int32_t>(hi[d]));
+  }
+  int32_t ords[8];
+  int32_t bk[8][3];
+  int nb = 0;
+  for (int32_t bz = blo[2]; bz <= bhi[2]; ++bz)
+    for (int32_t by = blo[1]; by <= bhi[1]; ++by)
+      for (int32_t bx = blo[0]; bx <= bhi[0]; ++bx) {
+        const int32_t ord = grid.ordinal(bx, by, bz);
+        if (ord < 0 || nb == 8) continue;
+        int pos = nb++;
+        while (pos > 0 && ords[pos - 1] > ord) {  // insertion by ordinal
+          ords[pos] = ords[pos - 1];
+          bk[pos][0] = bk[pos - 1][0];
+          bk[pos][1] = bk[pos - 1][1];
+          bk[pos][2] = bk[pos - 1][2];
+          --pos;
+        }
+        ords[pos] = ord;
+        bk[pos][0] = bx;
+        bk[pos][1] = by;
+        bk[pos][2] = bz;
+      }
+  for (int q = 0; q < nb; ++q)
+    for (int lz = 0; lz < 4; ++lz)
+      for (int ly = 0; ly < 4; ++ly)
+        for (int lx = 0; lx < 4; ++lx) {
+          const int64_t L[3] = {4 * int64_t(bk[q][0]) + lx, 4 * int64_t(bk[q][1]) + ly, 4 * int64_t(bk[q][2]) + lz};
+          if (L[0] < lo[0] || L[0] > hi[0] || L[1] < lo[1] || L[1] > hi[1] || L[2] < lo[2] || L[2] > hi[2]) continue;
+          const uint64_t s = uint64_t(ords[q]) * 192 + uint64_t(c) * 64 + uint64_t(lz * 16 + ly * 4 + lx);
+          if (!compatible_slot(h, s, side) || !usable_validity(h.validity[s])) continue;
+          const double ax = 4.0 * bk[q][0] + lx + (c == 0 ? 0.0 : 0.5);
+          const double ay = 4.0 * bk[q][1] + ly + (c == 1 ? 0.0 : 0.5);
+          const double az = 4.0 * bk[q][2] + lz + (c == 2 ? 0.0 : 0.5);
+          const double ox = ax - t[0], oy = ay - t[1], oz = az - t[2];
+          const double d2 = (ox * ox + oy * oy) + oz * oz;
+          if (d2 > 4.0) continue;
+          if (n == kMaxGhostRows) return kInternal;
+          m[n][0] = 1.0;
+          m[n][1] = ox;
+          m[n][2] = oy;
+          m[n][3] = oz;
+          rhs[n] = h.u[s];
+          ++n;
+          if (d2 < nearest_d2) {
+            nearest = s;
+            nearest_d2 = d2;
+          }
+        }
+  bool full = false;
+  double at = 0.0;
+  if (int e = ghost_fit(n, m, rhs, refl, &at, &full)) return e;
+  if (full) {
+    *velocity = at;
+    *fallback = false;
+    return kOk;
+  }
+  if (nearest != ~uint64_t(0)) {
+    *velocity = h.u[nearest];
+    *fallback = true;
+    return kOk;
+  }
+  if (wall.available && wall.side_tag == side) {
+    *velocity = vec_get(wall.wall_velocity, c);
+    *fallback = true;
+    return kOk;
+  }
+  return kFailedPrecondition;
+}
+
+// FillVelocity (liquid_boundary_transfer_3d.cc:242-306).
+__device__ __noinline__ int fill_velocity(const GridView& grid, const HistoryView& h, uint64_t s, int c, int64_t side,
+                                          double* velocity, bool* fallback) {
+  if (compatible_slot(h, s, side) && usable_validity(h.validity[s])) {
+    *velocity = h.u[s];
+    *fallback = false;
+    return kOk;
+  }
+  const fl_wall_sample wall = h.walls[s];
+  double t[3];
+  slot_anchor(grid, s, t);
+  if (int e = continue_component(grid, h, t, c, side, wall, velocity, fallback)) return e;
+  const double nc = vec_get(wall.unit_normal, c);
+  if (wall.available && wall.side_tag == side && nc != 0.0) {
+    double fitted[3] = {wall.wall_velocity.x, wall.wall_velocity.y, wall.wall_velocity.z};
+    fitted[c] = *velocity;
+    for (int other = 0; other < 3; ++other) {
+      if (other == c || vec_get(wall.unit_normal, other) == 0.0) continue;
+      double v;
+      bool fb;
+      if (int e = continue_component(grid, h, t, other, side, wall, &v, &fb)) return e;
+      fitted[other] = v;
+      *fallback = *fallback || fb;
+    }
+    const double rx = fitted[0] - wall.wall_velocity.x, ry = fitted[1] - wall.wall_velocity.y,
+                 rz = fitted[2] - wall.wall_velocity.z;
+    *velocity -= ((rx * wall.unit_normal.x + ry * wall.unit_normal.y) + rz * wall.unit_normal.z) * nc;
+    if (!finite(*velocity)) return kOutOfRange;
+  }
+  return kOk;
+}
+
+// One particle per lane: GatherCore with continuations (both loops of the reference), the gather
+// sums in the reference's sample order.
+template <bool kPow2>
+__global__ __launch_bounds__(64) void k_g2p_ghost(ParticleView p, const int32_t* bins, GridView grid,
+                                                  BoundaryGatherArgs a, const uint32_t* perm, double* out,
+                                                  uint64_t cap, uint64_t* err) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= *a.slow_count) return;
+  const uint32_t k = a.slow[idx];
+  const uint32_t o = perm[k];
+  const int64_t side = a.pside[o];
+  const Geometry g = grid.g;
+  // The bin is recomputed from the sorted position (the checks already passed in the fast path).
+  int32_t bin[3];
+  const auto sx = normalize<kPow2>(p.s[0][k], g.ox, g.h, g.inv_h);
+  const auto sy = normalize<kPow2>(p.s[1][k], g.oy, g.h, g.inv_h);
+  const auto sz = normalize<kPow2>(p.s[2][k], g.oz, g.h, g.inv_h);
+  checked_bin(sx, bin[0]);
+  checked_bin(sy, bin[1]);
+  checked_bin(sz, bin[2]);
+  const auto cx = normalize<kPow2>(p.t[0][k], g.ox, g.h, g.inv_h);
+  const auto cy = normalize<kPow2>(p.t[1][k], g.oy, g.h, g.inv_h);
+  const auto cz = normalize<kPow2>(p.t[2][k], g.oz, g.h, g.inv_h);
+  double fo[3][27], fn[3][27];
+  double old_pic[3] = {0.0, 0.0, 0.0};
+  bool ofb = false, nfb = false, low = false;
+  int code = kOk;
+  for (int A = 0; A < 3 && code == kOk; ++A) {
+    Axis3 X, Y, Z;
+    const int c0 = axis3<false>(cx, A == 0 ? 0.0 : 0.5, bin[0], g.h, X);
+    const int c1 = axis3<false>(cy, A == 1 ? 0.0 : 0.5, bin[1], g.h, Y);
+    const int c2 = axis3<false>(cz, A == 2 ? 0.0 : 0.5, bin[2], g.h, Z);
+    code = axis_status(c0, c1, c2, X, Y, Z);
+    if (code != kOk) break;
+    // Ownership of every sample's block (the stencil's owned-slot resolution).
+    int32_t ord[27];
+    for (int q = 0; q < 27 && code == kOk; ++q) {
+      const int32_t lx = X.first + q % 3, ly = Y.first + (q / 3) % 3, lz = Z.first + q / 9;
+      ord[q] = grid.ordinal(floor_div4_i32(lx), floor_div4_i32(ly), floor_div4_i32(lz));
+      if (ord[q] < 0) code = kFailedPrecondition;
+    }
+    for (int q = 0; q < 27 && code == kOk; ++q) {
+      const int xi = q % 3, yi = (q / 3) % 3, zi = q / 9;
+      const int32_t lx = X.first + xi, ly = Y.first + yi, lz = Z.first + zi;
+      const uint64_t s = uint64_t(ord[q]) * 192 + uint64_t(A) * 64 +
+                         uint64_t((lz & 3) * 16 + (ly & 3) * 4 + (lx & 3));
+      double vo, vn;
+      bool bo, bn;
+      code = fill_velocity(grid, a.old_h, s, A, side, &vo, &bo);
+      if (code != kOk) break;
+      if (a.same) {
+        vn = vo;
+        bn = bo;
+      } else {
+        code = fill_velocity(grid, a.new_h, s, A, side, &vn, &bn);
+        if (code != kOk) break;
+      }
+      fo[A][q] = vo;
+      fn[A][q] = vn;
+      old_pic[A] += ((X.w[xi] * Y.w[yi]) * Z.w[zi]) * vo;
+      ofb = ofb || bo;
+      nfb = nfb || bn;
+      low = low || a.old_h.validity[s] == 3 || a.new_h.validity[s] == 3;
+    }
+  }
+  // The complete gather on the filled values (mac_gather_3d_internal.h:15-69), samples in stencil order.
+  for (int A = 0; A < 3 && code == kOk; ++A) {
+    Axis3 X, Y, Z;
+    axis3<false>(cx, A == 0 ? 0.0 : 0.5, bin[0], g.h, X);
+    axis3<false>(cy, A == 1 ? 0.0 : 0.5, bin[1], g.h, Y);
+    axis3<false>(cz, A == 2 ? 0.0 : 0.5, bin[2], g.h, Z);
+    double opic = 0.0, npic = 0.0, mom[3] = {0.0, 0.0, 0.0}, grad[3] = {0.0, 0.0, 0.0};
+    for (int q = 0; q < 27; ++q) {
+      const int xi = q % 3, yi = (q / 3) % 3, zi = q / 9;
+      const double w = (X.w[xi] * Y.w[yi]) * Z.w[zi];
+      const double uo = fo[A][q], un = fn[A][q];
+      const double wun = w * un;
+      opic += w * uo;
+      npic += wun;
+      const double r[3] = {X.off[xi], Y.off[yi], Z.off[zi]};
+      double gw[3];
+      sample_gradient<kPow2>(X, Y, Z, xi, yi, zi, g.h, g.inv_h, gw);
+      for (int d = 0; d < 3; ++d) {
+        const double rn = kPow2 ? r[d] * g.inv_h : r[d] / g.h;
+        mom[d] += rn * wun;
+        grad[d] += gw[d] * un;
+      }
+    }
+    double slope[3];
+    for (int d = 0; d < 3; ++d) slope[d] = kPow2 ? (4.0 * mom[d]) * g.inv_h : (4.0 * mom[d]) / g.h;
+    const double inc = npic - opic;
+    bool ok = finite(npic) && finite(inc);
+    for (int d = 0; d < 3; ++d) ok = ok && finite(slope[d]) && finite(grad[d]);
+    if (!ok) {
+      code = kOutOfRange;
+      break;
+    }
+    out[A * cap + k] = npic;
+    out[(3 + A) * cap + k] = inc;
+    for (int d = 0; d < 3; ++d) {
+      out[(6 + 3 * A + d) * cap + k] = slope[d];
+      out[(15 + 3 * A + d) * cap + k] = grad[d];
+    }
+  }
+  if (code == kOk) {
+    const bool fallback = ofb || nfb || low;
+    double delta[3] = {0.0, 0.0, 0.0};
+    if (!fallback)
+      for (int c = 0; c < 3; ++c) delta[c] = p.v[c][k] - old_pic[c];
+    if (!(finite(delta[0]) && finite(delta[1]) && finite(delta[2]))) code = kOutOfRange;
+    write_extra(a, k, delta,
+                (a.paffine[o] != 0 && !fallback ? 1 : 0) | (ofb ? 2 : 0) | (nfb ? 4 : 0) | (low ? 8 : 0));
+  }
+  if (code) report(err, o, code);
+}
+
 __global__ void k_unpack(const fl_particle* in, uint64_t n, double* base, uint64_t cap) {
   const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (i >= n) return;
@@ -990,6 +1470,37 @@ int compatible_stencils(fl_ctx* ctx, uint64_t lo, uint64_t count, const int64_t*
   kernel<<<grid_for(count), kThreads, 0, ctx->stream>>>(ctx->raw.view(), ctx->bin, grid_view(ctx), tags, lo, count,
                                                         d_out);
   FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+int g2p_boundary(fl_ctx* ctx, uint64_t n, const BoundaryDevice& d, uint32_t* slow_count) {
+  *slow_count = 0;
+  if (ctx->nbins == 0 || n == 0) return FL_OK;
+  BoundaryGatherArgs a;
+  a.old_h = HistoryView{d.u[0], d.validity[0], d.side[0], d.eligible[0], d.walls[0]};
+  a.new_h = HistoryView{d.u[1], d.validity[1], d.side[1], d.eligible[1], d.walls[1]};
+  a.same = d.same;
+  a.pside = d.pside;
+  a.paffine = d.paffine;
+  a.extra = d.extra;
+  a.extra_cap = d.extra_cap;
+  a.slow = d.slow;
+  a.slow_count = d.slow_count;
+  FL_HIP(hipMemsetAsync(d.slow_count, 0, sizeof(uint32_t), ctx->stream));
+  const bool trial = !ctx->sorted.trial_aliases_sorted, pow2 = ctx->geometry.inv_h != 0.0;
+  auto* fast = trial ? (pow2 ? k_g2p_boundary<true, true> : k_g2p_boundary<true, false>)
+                     : (pow2 ? k_g2p_boundary<false, true> : k_g2p_boundary<false, false>);
+  fast<<<ctx->nbins, kThreads, 0, ctx->stream>>>(ctx->sorted.view(), bin_list(ctx), grid_view(ctx), a, ctx->perm,
+                                                 ctx->gather, ctx->sorted.capacity, ctx->err);
+  FL_HIP(hipGetLastError());
+  FL_HIP(hipMemcpyAsync(slow_count, d.slow_count, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  FL_HIP(hipStreamSynchronize(ctx->stream));
+  if (*slow_count) {
+    auto* slow = pow2 ? k_g2p_ghost<true> : k_g2p_ghost<false>;
+    slow<<<(*slow_count + 63) / 64, 64, 0, ctx->stream>>>(ctx->sorted.view(), nullptr, grid_view(ctx), a, ctx->perm,
+                                                         ctx->gather, ctx->sorted.capacity, ctx->err);
+    FL_HIP(hipGetLastError());
+  }
   return FL_OK;
 }
 

@@ -49,7 +49,7 @@ class StepReport(C.Structure):
         return np.frombuffer(bytes(self.pressure), dtype=R.PRESSURE_DIAGNOSTICS)[0]
 
 
-EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_p2g", "fl_p2g_quanta", "fl_p2g_compatible",
+EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_p2g", "fl_p2g_quanta", "fl_p2g_compatible", "fl_g2p_boundary",
            "fl_normalize_forces", "fl_apply_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
            "fl_scene_download", "fl_probe_copy", "fl_learn_param_count", "fl_learn_setup", "fl_learn_set_params", "fl_learn_get_params",
            "fl_learn_forward", "fl_learn_infer", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect", "fl_learn_eval",
@@ -75,6 +75,20 @@ def _p(a):
 
 def _vec3(v):
     return Vec3(*map(float, v))
+
+
+class VelocityHistory(C.Structure):
+    _fields_ = [("velocity_sample_time", C.c_double), ("wall_sample_time", C.c_double),
+                ("face_velocities", C.c_void_p), ("validity", C.c_void_p), ("face_tags", C.c_void_p),
+                ("wall_samples", C.c_void_p), ("count", C.c_size_t)]
+
+
+def velocity_history(time, velocities, validity, tags, walls, wall_time=None):
+    """fl_velocity_history over caller arrays (kept alive by the returned object)."""
+    h = VelocityHistory(time, time if wall_time is None else wall_time, velocities.ctypes.data,
+                        validity.ctypes.data, tags.ctypes.data, walls.ctypes.data, len(velocities))
+    h._keep = (velocities, validity, tags, walls)
+    return h
 
 
 class Context:
@@ -152,6 +166,16 @@ class Context:
                                     Quantization(mass_q, mom_q, cap), _p(out), C.c_size_t(len(out)), _p(st),
                                     C.byref(self.last))
         return s, out, st
+
+    def g2p_boundary(self, geometry, blocks, particles, old_time, new_time, old_history, new_history):
+        """GatherLiquidBoundaryParticles3D: (status, records, [old-PIC, new-PIC, low-mass fallback counts])."""
+        out = np.zeros(len(particles), R.BOUNDARY_GATHER)
+        counts = np.zeros(3, np.uint64)
+        s = lib().fl_g2p_boundary(self.handle, _p(geometry), _p(blocks), C.c_size_t(len(blocks)), _p(particles),
+                                  C.c_size_t(len(particles)), C.c_double(old_time), C.c_double(new_time),
+                                  C.byref(old_history), C.byref(new_history), _p(out), _p(counts),
+                                  C.byref(self.last))
+        return s, out, counts
 
     def normalize_forces(self, quanta, mass_q, mom_q, min_mass, accel, dt):
         nf = len(quanta)

@@ -30,4 +30,30 @@ struct CompatibleMacFaceSample3D {
   Vec3 centered_face_offset;        // r minus the compatible weighted mean, m.
 };
 
+struct LiquidPrescribedWallSample3D {
+  bool available = false;
+  std::int64_t side_tag = 0;  // Wall continuation is usable only by this side.
+  Vec3 unit_normal{};         // World Cartesian unit normal when available.
+  Vec3 wall_velocity{};       // World m/s at the history's wall sample time.
+};
+
+struct LiquidBoundaryParticleGather3D {
+  Vec3 new_pic_velocity{};                 // m/s from the complete new filled stencil.
+  Vec3 grid_velocity_increment{};          // new PIC minus old PIC, m/s.
+  Vec3 particle_minus_old_pic_velocity{};  // Retained particle residual, m/s.
+  Mat3 fitted_velocity_slopes{};           // Complete second-moment fit, 1/s.
+  Mat3 velocity_gradient{};                // True original-basis derivative, 1/s.
+  bool affine_enabled = true;              // False on input affine-off or any fallback.
+  bool old_pic_fallback = false;           // An old-stage continuation fell back from the fit.
+  bool new_pic_fallback = false;           // A new-stage continuation fell back from the fit.
+  bool low_mass_fallback = false;          // Either history touches a low-mass face.
+};
+
+enum class LiquidFaceValidity3D : std::uint8_t {
+  kMissing = 0,
+  kValid = 1,
+  kPredicted = 2,  // Finite continuation usable as a donor, without mass.
+  kLowMass = 3,    // Positive mass at or below the minimum.
+};
+
 }  // namespace fluidity

@@ -47,4 +47,39 @@ Status HipDepositCompatibleMacQuanta3D(HipOperatorRuntime3D& runtime, const Liqu
                                        std::span<std::array<CompatibleMacFaceStencil3D, 3>> compatible_stencils,
                                        HipOperatorMeasurement& measurement);
 
+// One stage snapshot; every span covers all owned face slots.
+struct LiquidBoundaryVelocityHistory3D {
+  double velocity_sample_time;  // Seconds, must equal the requested stage time.
+  double wall_sample_time;      // Seconds, must equal velocity_sample_time.
+  std::span<const double> face_velocities;
+  std::span<const LiquidFaceValidity3D> validity;
+  std::span<const LiquidBoundaryFaceTag3D> face_tags;
+  std::span<const LiquidPrescribedWallSample3D> wall_samples;
+};
+
+struct LiquidBoundaryGatherInputs3D {
+  const SparseMacGrid3D& mac_grid;
+  std::span<const LiquidBoundaryParticle3D> particles;
+  double old_time;  // Finite pre-force snapshot time, seconds.
+  double new_time;  // Finite and strictly greater than old_time.
+  LiquidBoundaryVelocityHistory3D old_history;
+  LiquidBoundaryVelocityHistory3D new_history;
+};
+
+struct LiquidBoundaryGatherResults3D {
+  std::vector<LiquidBoundaryParticleGather3D> particles;
+  std::size_t old_pic_fallback_particle_count = 0;
+  std::size_t new_pic_fallback_particle_count = 0;
+  std::size_t low_mass_fallback_particle_count = 0;
+};
+
+// GatherLiquidBoundaryParticles3D (liquid_boundary_transfer_3d.cc:519-651): incompatible or unusable
+// samples are continued from same-side donors within 2h (rank-revealing affine fit, nearest donor,
+// compatible wall, oblique-wall normal replacement); retention, affine switch and fallbacks as the
+// reference. Results agree with the reference to 1e-12 relative; flags and counts exactly.
+StatusOr<LiquidBoundaryGatherResults3D> GatherLiquidBoundaryParticles3D(const LiquidBoundaryGatherInputs3D& inputs);
+Status HipGatherLiquidBoundaryParticles3D(HipOperatorRuntime3D& runtime, const LiquidBoundaryGatherInputs3D& inputs,
+                                          LiquidBoundaryGatherResults3D& results,
+                                          HipOperatorMeasurement& measurement);
+
 }  // namespace fluidity

@@ -9,6 +9,7 @@
 
 #include <span>
 
+#include "fluidity/base/liquid_boundary_transfer_types_3d.h"
 #include "fluidity/base/mac_types_3d.h"
 #include "fluidity/host/hip_operator_runtime_3d.h"
 #include "fluidity/host/sparse_mac_grid_3d.h"
@@ -46,7 +47,6 @@ Status HipGatherMac3D(HipOperatorRuntime3D& runtime, const SparseMacGrid3D& mac_
 
 // Normalized face velocities of the liquid deposit (NormalizeLiquidMacFaces3D,
 // liquid_boundary_transfer_3d.cc:463-516) followed by gravity (ApplyForces, liquid_substep_3d_cpu.cc:156-178).
-enum class LiquidFaceValidity3D : std::uint8_t { kMissing = 0, kValid = 1, kLowMass = 3 };
 struct NormalizedLiquidFaces3D {
   std::span<double> face_masses;             // kilograms, may be empty
   std::span<double> face_velocities;         // u0 = momentum / mass, m/s, may be empty

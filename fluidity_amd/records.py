@@ -29,6 +29,13 @@ COMPATIBLE_SAMPLE = np.dtype({"names": ["face_slot", "compatible", "normalized_w
                                         "centered_face_offset"],
                               "formats": ["u8", "u1", "f8", VEC3, VEC3], "offsets": [0, 8, 16, 24, 48],
                               "itemsize": 72})
+WALL_SAMPLE = np.dtype({"names": ["available", "side_tag", "unit_normal", "wall_velocity"],
+                        "formats": ["u1", "i8", VEC3, VEC3], "offsets": [0, 8, 16, 40], "itemsize": 64})
+BOUNDARY_GATHER = np.dtype({"names": ["new_pic_velocity", "grid_velocity_increment", "particle_minus_old_pic_velocity",
+                                      "fitted_velocity_slopes", "velocity_gradient", "affine_enabled",
+                                      "old_pic_fallback", "new_pic_fallback", "low_mass_fallback"],
+                            "formats": [VEC3, VEC3, VEC3, MAT3, MAT3, "u1", "u1", "u1", "u1"],
+                            "offsets": [0, 24, 48, 72, 144, 216, 217, 218, 219], "itemsize": 224})
 SORT_ENTRY = np.dtype([("bin_index", INT3), ("pad", "u4"), ("original_ordinal", "u8")])
 PRESSURE_CELL = np.dtype([("cell_index", INT3), ("pad", "i4"), ("fluid_volume", "f8"), ("target_divergence", "f8"),
                           ("bulk_modulus", "f8"), ("previous_pressure", "f8"), ("integrated_inward_wall_flux", "f8")])

@@ -93,6 +93,35 @@ typedef struct fl_compatible_sample {
   fl_vec3 normalized_weight_gradient;
   fl_vec3 centered_face_offset;
 } fl_compatible_sample; /* 72 B */
+/* LiquidPrescribedWallSample3D and LiquidBoundaryParticleGather3D
+ * (native/include/fluidity/base/liquid_boundary_transfer_types_3d.h:21-36) and the flattened spans of
+ * LiquidBoundaryVelocityHistory3D (native/include/fluidity/host/liquid_boundary_transfer_3d.h:83-90).
+ * Face validity: 0 missing, 1 valid, 2 predicted, 3 low mass. */
+typedef struct fl_wall_sample {
+  uint8_t available;
+  uint8_t pad[7];
+  int64_t side_tag;
+  fl_vec3 unit_normal;
+  fl_vec3 wall_velocity;
+} fl_wall_sample; /* 64 B */
+typedef struct fl_velocity_history {
+  double velocity_sample_time;
+  double wall_sample_time;
+  const double* face_velocities;
+  const uint8_t* validity;
+  const fl_face_tag* face_tags;
+  const fl_wall_sample* wall_samples;
+  size_t count; /* entries of each array: one per owned slot */
+} fl_velocity_history;
+typedef struct fl_boundary_gather {
+  fl_vec3 new_pic_velocity;
+  fl_vec3 grid_velocity_increment;
+  fl_vec3 particle_minus_old_pic_velocity;
+  fl_mat3 fitted_velocity_slopes;
+  fl_mat3 velocity_gradient;
+  uint8_t affine_enabled, old_pic_fallback, new_pic_fallback, low_mass_fallback;
+  uint8_t pad[4];
+} fl_boundary_gather; /* 224 B */
 
 /* link_kind: 0 sealed, 1 internal (lift = pressure jump), 2 Dirichlet (lift = exterior pressure). */
 typedef struct fl_pressure_cell {
@@ -212,6 +241,19 @@ int fl_p2g_compatible(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* b
                       const fl_boundary_particle* particles, size_t n, const fl_face_tag* tags, size_t tag_count,
                       fl_quantization quantization, fl_face_quanta* out, size_t face_count,
                       fl_compatible_sample* stencils, fl_measurement* measurement);
+/* GatherLiquidBoundaryParticles3D (native/include/fluidity/host/liquid_boundary_transfer_3d.h:100-140;
+ * native/src/liquid_boundary_transfer_3d.cc:519-651): the gather on the face velocities of two histories
+ * in which every stencil sample that is not structurally eligible on the particle's side, or whose
+ * validity is neither valid nor predicted, is continued from same-side donors within two cell widths
+ * (rank-revealing affine fit, nearest donor when rank-deficient, a compatible prescribed wall as the
+ * last resort, and the wall-normal replacement for oblique walls); particle retention, affine switch
+ * and fallback flags as the reference. The histories share their samples (one continuation serves
+ * both) when all four of their array pointers are equal. out_counts: old-PIC, new-PIC and low-mass
+ * fallback particle counts (may be null). */
+int fl_g2p_boundary(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
+                    const fl_boundary_particle* particles, size_t n, double old_time, double new_time,
+                    const fl_velocity_history* old_history, const fl_velocity_history* new_history,
+                    fl_boundary_gather* out, uint64_t* out_counts, fl_measurement* measurement);
 /* NormalizeLiquidMacFaces3D followed by ApplyForces (native/src/liquid_boundary_transfer_3d.cc:463-516;
  * native/src/liquid_substep_3d_cpu.cc:156-178). validity: 0 missing, 1 valid, 3 low mass.
  * Any output pointer may be null. */

@@ -286,6 +286,35 @@ inline int OwnedStencil(const flo_geometry& g, const flo_int3* blocks, size_t co
   return FLO_OK;
 }
 
+// One velocity component of GatherMac3D on a resolved stencil (mac_gather_3d_internal.h:15-69): new
+// PIC value, grid increment, the complete-second-moment slope row and the true gradient row.
+inline int GatherComponent(const std::array<flo_stencil_sample, 27>& st, double h, const double* old_faces,
+                           const double* new_faces, double& new_pic, double& inc, flo_vec3& slope, flo_vec3& grad) {
+  double old_pic = 0.0;
+  new_pic = 0.0;
+  flo_vec3 moment{0, 0, 0};
+  grad = {0, 0, 0};
+  for (const auto& smp : st) {
+    const double uo = old_faces[smp.face_slot];
+    const double un = new_faces[smp.face_slot];
+    const double wun = smp.weight * un;
+    old_pic += smp.weight * uo;
+    new_pic += wun;
+    const flo_vec3 rn{smp.face_offset_from_particle.x / h, smp.face_offset_from_particle.y / h,
+                      smp.face_offset_from_particle.z / h};
+    moment.x += rn.x * wun;
+    moment.y += rn.y * wun;
+    moment.z += rn.z * wun;
+    grad.x += smp.weight_gradient.x * un;
+    grad.y += smp.weight_gradient.y * un;
+    grad.z += smp.weight_gradient.z * un;
+  }
+  slope = {(4.0 * moment.x) / h, (4.0 * moment.y) / h, (4.0 * moment.z) / h};
+  inc = new_pic - old_pic;
+  if (!std::isfinite(new_pic) || !std::isfinite(inc) || !Finite3(slope) || !Finite3(grad)) return FLO_OUT_OF_RANGE;
+  return FLO_OK;
+}
+
 inline int ComputeBin(const flo_geometry& g, flo_vec3 p, flo_int3& bin) {
   if (int s = ValidateGeometry(g)) return s;
   std::array<Coord, 3> np;

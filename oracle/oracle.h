@@ -56,6 +56,28 @@ struct flo_compatible_sample {
   flo_vec3 normalized_weight_gradient;
   flo_vec3 centered_face_offset;
 };  // 72 B
+// LiquidPrescribedWallSample3D, LiquidBoundaryVelocityHistory3D (spans flattened),
+// LiquidBoundaryParticleGather3D (liquid_boundary_transfer_types_3d.h:21-36; liquid_boundary_transfer_3d.h:83-90).
+// Validity: 0 missing, 1 valid, 2 predicted, 3 low mass.
+struct flo_wall_sample { uint8_t available; uint8_t pad[7]; int64_t side_tag; flo_vec3 unit_normal; flo_vec3 wall_velocity; };  // 64 B
+struct flo_history {
+  double velocity_sample_time;
+  double wall_sample_time;
+  const double* face_velocities;
+  const uint8_t* validity;
+  const flo_face_tag* face_tags;
+  const flo_wall_sample* wall_samples;
+  size_t count;
+};
+struct flo_boundary_gather {
+  flo_vec3 new_pic_velocity;
+  flo_vec3 grid_velocity_increment;
+  flo_vec3 particle_minus_old_pic_velocity;
+  flo_mat3 fitted_velocity_slopes;
+  flo_mat3 velocity_gradient;
+  uint8_t affine_enabled, old_pic_fallback, new_pic_fallback, low_mass_fallback;
+  uint8_t pad[4];
+};  // 224 B
 struct flo_sort_entry { flo_int3 bin_index; uint32_t pad; uint64_t original_ordinal; };  // 24 B, as LiquidParticleSortEntry3D
 
 // Flattened MacPressureCell3D / MacPressureFace3D (native/include/fluidity/host/mac_pressure_3d.h:14-51).
@@ -133,6 +155,14 @@ int flo_deposit_compatible(const flo_geometry* geometry, const flo_int3* sorted_
                            const flo_boundary_particle* particles, size_t n, const flo_face_tag* tags,
                            size_t tag_count, flo_quantization quantization, uint64_t max_capacity,
                            flo_face_quanta* out, size_t face_count, flo_compatible_sample* stencils);
+// liquid_boundary_transfer_3d.cc:519-651 (GatherCore). out_counts: old-PIC, new-PIC and low-mass fallback
+// particle counts. The histories alias (same samples) when all four pointers are equal.
+int flo_fit_ghost_affine(const flo_vec3* offsets, const double* velocities, size_t n, int* full_rank,
+                         double* at_target, flo_vec3* slopes);
+int flo_gather_boundary(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count,
+                        const flo_boundary_particle* particles, size_t n, double old_time, double new_time,
+                        const flo_history* old_history, const flo_history* new_history,
+                        flo_boundary_gather* out, uint64_t* out_counts);
 int flo_normalize(const flo_face_quanta* quanta, size_t face_count, flo_quantization quantization,
                   double minimum_face_mass, double* out_mass, double* out_velocity, uint8_t* out_validity);
 int flo_apply_forces(const flo_int3* sorted_blocks, size_t block_count, const double* old_velocities,

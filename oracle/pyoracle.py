@@ -41,6 +41,13 @@ COMPATIBLE_SAMPLE = np.dtype({"names": ["face_slot", "compatible", "normalized_w
                                         "centered_face_offset"],
                               "formats": ["u8", "u1", "f8", VEC3, VEC3], "offsets": [0, 8, 16, 24, 48],
                               "itemsize": 72})
+WALL_SAMPLE = np.dtype({"names": ["available", "side_tag", "unit_normal", "wall_velocity"],
+                        "formats": ["u1", "i8", VEC3, VEC3], "offsets": [0, 8, 16, 40], "itemsize": 64})
+BOUNDARY_GATHER = np.dtype({"names": ["new_pic_velocity", "grid_velocity_increment", "particle_minus_old_pic_velocity",
+                                      "fitted_velocity_slopes", "velocity_gradient", "affine_enabled",
+                                      "old_pic_fallback", "new_pic_fallback", "low_mass_fallback"],
+                            "formats": [VEC3, VEC3, VEC3, MAT3, MAT3, "u1", "u1", "u1", "u1"],
+                            "offsets": [0, 24, 48, 72, 144, 216, 217, 218, 219], "itemsize": 224})
 SORT_ENTRY = np.dtype([("bin_index", INT3), ("pad", "u4"), ("original_ordinal", "u8")])
 PRESSURE_CELL = np.dtype([("cell_index", INT3), ("pad", "i4"), ("fluid_volume", "f8"), ("target_divergence", "f8"),
                           ("bulk_modulus", "f8"), ("previous_pressure", "f8"), ("integrated_inward_wall_flux", "f8")])
@@ -188,6 +195,40 @@ def deposit_compatible(geom, blocks, particles, tags, mass_q, mom_q, cap, max_ca
                                      QuantC(mass_q, mom_q, cap), C.c_uint64(max_capacity), _p(out),
                                      C.c_size_t(len(out)), _p(st) if st is not None else None)
     return s, out, st
+
+
+class HistoryC(C.Structure):
+    _fields_ = [("velocity_sample_time", C.c_double), ("wall_sample_time", C.c_double),
+                ("face_velocities", C.c_void_p), ("validity", C.c_void_p), ("face_tags", C.c_void_p),
+                ("wall_samples", C.c_void_p), ("count", C.c_size_t)]
+
+
+def history(time, velocities, validity, tags, walls, wall_time=None):
+    """A LiquidBoundaryVelocityHistory3D view; the arrays must outlive the call that uses it."""
+    h = HistoryC(time, time if wall_time is None else wall_time, _p(velocities), _p(validity), _p(tags), _p(walls),
+                 len(velocities))
+    h._keep = (velocities, validity, tags, walls)
+    return h
+
+
+def gather_boundary(geom, blocks, particles, old_time, new_time, old_history, new_history):
+    """GatherCore: (status, records, [old-PIC, new-PIC, low-mass fallback counts])."""
+    out = np.zeros(len(particles), BOUNDARY_GATHER)
+    counts = np.zeros(3, np.uint64)
+    s = lib().flo_gather_boundary(_p(geom), _p(blocks), C.c_size_t(len(blocks)), _p(particles),
+                                  C.c_size_t(len(particles)), C.c_double(old_time), C.c_double(new_time),
+                                  C.byref(old_history), C.byref(new_history), _p(out), _p(counts))
+    return s, out, counts
+
+
+def fit_ghost_affine(offsets, velocities):
+    """FitLiquidGhostAffine3D: (status, full_rank, value at target, slopes per cell width)."""
+    off = np.ascontiguousarray(offsets, dtype=np.float64).reshape(-1, 3)
+    vel = np.ascontiguousarray(velocities, dtype=np.float64)
+    full, at = C.c_int(0), C.c_double(0.0)
+    slopes = np.zeros(3)
+    s = lib().flo_fit_ghost_affine(_p(off), _p(vel), C.c_size_t(len(vel)), C.byref(full), C.byref(at), _p(slopes))
+    return s, bool(full.value), at.value, slopes
 
 
 def gather(geom, blocks, particles, old, new, out=None, threads=1):

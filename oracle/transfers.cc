@@ -195,27 +195,10 @@ int flo_gather(const flo_geometry* geometry, const flo_int3* sorted_blocks, size
           e = {i, s};
           return;
         }
-        double old_pic = 0.0, new_pic = 0.0;
-        flo_vec3 moment{0, 0, 0}, grad{0, 0, 0};
-        for (const auto& smp : st) {
-          const double uo = old_faces[smp.face_slot];
-          const double un = new_faces[smp.face_slot];
-          const double wun = smp.weight * un;
-          old_pic += smp.weight * uo;
-          new_pic += wun;
-          const flo_vec3 rn{smp.face_offset_from_particle.x / h, smp.face_offset_from_particle.y / h,
-                            smp.face_offset_from_particle.z / h};
-          moment.x += rn.x * wun;
-          moment.y += rn.y * wun;
-          moment.z += rn.z * wun;
-          grad.x += smp.weight_gradient.x * un;
-          grad.y += smp.weight_gradient.y * un;
-          grad.z += smp.weight_gradient.z * un;
-        }
-        const flo_vec3 slope{(4.0 * moment.x) / h, (4.0 * moment.y) / h, (4.0 * moment.z) / h};
-        const double inc = new_pic - old_pic;
-        if (!std::isfinite(new_pic) || !std::isfinite(inc) || !Finite3(slope) || !Finite3(grad)) {
-          e = {i, FLO_OUT_OF_RANGE};
+        double new_pic, inc;
+        flo_vec3 slope, grad;
+        if (int g = GatherComponent(st, h, old_faces, new_faces, new_pic, inc, slope, grad)) {
+          e = {i, g};
           return;
         }
         Set(res.new_pic_velocity, axis, new_pic);

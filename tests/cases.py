@@ -278,3 +278,164 @@ def masked_scene(seed, n, h=1.0, blocks_per_axis=3, sides=2, eligible=0.85, affi
     tags["side_tag"] = rng.integers(0, sides, len(tags))
     tags["structurally_eligible"] = rng.random(len(tags)) < eligible
     return geom, blocks, p, tags
+
+
+class BoundaryFixture:
+    """native/tests/liquid_boundary_transfer_3d_cases.h:23-50: one block at the origin, unit cells, the
+    boundary particle, every slot eligible on side 7, old face velocities 2, new 3, all valid, no walls,
+    old history at t = 0 and new at t = 1 (separate arrays)."""
+
+    def __init__(self):
+        self.geom = O.geometry((0.0, 0.0, 0.0), 1.0)
+        self.blocks = sorted_blocks(self.geom, [(0, 0, 0)])
+        self.particle = boundary_particle()
+        nf = 192 * len(self.blocks)
+        self.tags = boundary_tags(len(self.blocks))
+        self.old_values = np.full(nf, 2.0)
+        self.new_values = np.full(nf, 3.0)
+        self.validity = np.ones(nf, np.uint8)
+        self.old_walls = np.zeros(nf, O.WALL_SAMPLE)
+        self.new_walls = np.zeros(nf, O.WALL_SAMPLE)
+
+    def slot(self, axis, x, y, z):
+        s, v = O.face_slot(self.blocks, axis, (x, y, z))
+        assert s == O.OK
+        return v
+
+    def anchor(self, axis, x, y, z):
+        s, a = O.face_world_anchor(self.geom, axis, (x, y, z))
+        assert s == O.OK
+        return np.array(a)
+
+    def histories(self, old_validity=None, new_validity=None, old_wall_time=None):
+        """(time, velocities, validity, tags, walls, wall time) of the old and the new history."""
+        old = (0.0, self.old_values, self.validity if old_validity is None else old_validity, self.tags,
+               self.old_walls, 0.0 if old_wall_time is None else old_wall_time)
+        new = (1.0, self.new_values, self.validity if new_validity is None else new_validity, self.tags,
+               self.new_walls, 1.0)
+        return old, new
+
+    def gather(self, **kw):
+        old, new = self.histories(**kw)
+        return O.gather_boundary(self.geom, self.blocks, self.particle, 0.0, 1.0,
+                                 O.history(old[0], *old[1:5], wall_time=old[5]),
+                                 O.history(new[0], *new[1:5], wall_time=new[5]))
+
+
+def set_wall(walls, slot, side, normal, velocity):
+    w = walls[slot]
+    w["available"] = 1
+    w["side_tag"] = side
+    set_vec(w, "unit_normal", normal)
+    set_vec(w, "wall_velocity", velocity)
+    walls[slot] = w
+
+
+def gather_boundary_cases():
+    """The reference's boundary-gather cases (liquid_boundary_transfer_3d_cases.h:217-420) as
+    (name, fixture, keyword arguments of BoundaryFixture.histories)."""
+    out = []
+    f = BoundaryFixture()
+    for axis in range(3):
+        for z in range(4):
+            for y in range(4):
+                for x in range(4):
+                    a = f.anchor(axis, x, y, z)
+                    s = f.slot(axis, x, y, z)
+                    f.old_values[s] = 1.0 + a[0] + 2.0 * a[1] - a[2]
+                    f.new_values[s] = 4.0 + 2.0 * a[0] - a[1] + 3.0 * a[2]
+        poison = f.slot(axis, 1, 1, 1)
+        f.tags[poison]["side_tag"] = -7
+        f.old_values[poison] = -1e8
+        f.new_values[poison] = 1e9
+    out.append(("same_side_affine_ghosts", f, {}))
+    f = BoundaryFixture()
+    f.new_values = f.old_values.copy()
+    out.append(("identical_histories", f, {}))
+    f = BoundaryFixture()
+    f.validity[:] = 0
+    for slot in range(len(f.validity)):
+        set_wall(f.old_walls, slot, 7, (1.0, 0.0, 0.0), (2.0, 4.0, 6.0))
+        set_wall(f.new_walls, slot, 7, (1.0, 0.0, 0.0), (5.0, 8.0, 9.0))
+    out.append(("walls_everywhere", f, {}))
+    f = BoundaryFixture()
+    fields = {0: ((1.0, 1.0, 2.0, -1.0), (4.0, 2.0, -1.0, 3.0)), 1: ((-2.0, 3.0, -1.0, 2.0), (7.0, -1.0, 4.0, -1.0)),
+              2: ((4.0, -1.0, 1.0, 3.0), (-3.0, 3.0, 2.0, -2.0))}
+    for axis in range(3):
+        (o0, ox, oy, oz), (n0, nx, ny, nz) = fields[axis]
+        for z in range(4):
+            for y in range(4):
+                for x in range(4):
+                    a = f.anchor(axis, x, y, z)
+                    s = f.slot(axis, x, y, z)
+                    f.old_values[s] = o0 + ox * a[0] + oy * a[1] + oz * a[2]
+                    f.new_values[s] = n0 + nx * a[0] + ny * a[1] + nz * a[2]
+    nc = math.sqrt(0.5)
+    target = f.slot(0, 1, 1, 1)
+    set_wall(f.old_walls, target, 7, (nc, nc, 0.0), (2.0, -1.0, 4.0))
+    set_wall(f.new_walls, target, 7, (nc, nc, 0.0), (5.0, 1.0, -2.0))
+    f.validity[target] = 0
+    out.append(("oblique_wall", f, {}))
+    f = BoundaryFixture()
+    f.validity[:] = 0
+    for axis in range(3):
+        first = f.slot(axis, 1, 1, 1)
+        second = f.slot(axis, 2 if axis == 0 else 1, 2 if axis == 1 else 1, 2 if axis == 2 else 1)
+        f.validity[first] = 1
+        f.validity[second] = 2
+        f.old_values[first] = f.old_values[second] = 2.0
+        f.new_values[first] = f.new_values[second] = 4.0
+    out.append(("singular_donors", f, {}))
+    f = BoundaryFixture()
+    f.validity[:] = 0
+    out.append(("no_donors", f, {}))
+    f = BoundaryFixture()
+    nv = f.validity.copy()
+    nv[f.slot(0, 1, 1, 1)] = 3
+    out.append(("low_mass", f, {"old_validity": f.validity.copy(), "new_validity": nv}))
+    f = BoundaryFixture()
+    set_position(f.particle, 0, (1.75, 1.75, 1.75))
+    for axis in range(3):
+        for z in range(4):
+            for y in range(4):
+                for x in range(4):
+                    a = f.anchor(axis, x, y, z)
+                    f.new_values[f.slot(axis, x, y, z)] = (a[axis] - 1.75) ** 2
+    out.append(("nonaffine_field", f, {}))
+    f = BoundaryFixture()
+    set_position(f.particle, 0, (2.0, 1.5, 1.5))
+    nv = f.validity.copy()
+    for z in range(4):
+        for y in range(4):
+            for x in range(4):
+                nv[f.slot(0, x, y, z)] = 0
+    low, high = f.slot(0, 1, 1, 1), f.slot(0, 3, 1, 1)
+    nv[low], nv[high] = 1, 2
+    f.new_values[low], f.new_values[high] = 2.0, 8.0
+    out.append(("nearest_tie", f, {"new_validity": nv}))
+    return out
+
+
+def boundary_gather_scene(seed, n, h=0.5, blocks_per_axis=3, missing=0.05, low_mass=0.002, sides=2, eligible=0.9,
+                          walls=0.2):
+    """Random boundary-gather inputs: the masked-deposit scene plus two histories with random face
+    velocities, validity (missing with probability `missing`, low mass with `low_mass`, otherwise valid
+    or predicted), and unit-normal prescribed walls on a fraction `walls` of the slots."""
+    geom, blocks, p, tags = masked_scene(seed, n, h=h, blocks_per_axis=blocks_per_axis, sides=sides,
+                                         eligible=eligible)
+    rng = np.random.default_rng(seed + 1000)
+    nf = len(tags)
+    hist = []
+    for k in range(2):
+        u = rng.normal(0.0, 1.0, nf)
+        v = rng.choice(np.array([1, 2], np.uint8), nf, p=[0.9, 0.1])
+        v[rng.random(nf) < low_mass] = 3
+        v[rng.random(nf) < missing] = 0
+        w = np.zeros(nf, O.WALL_SAMPLE)
+        sel = np.nonzero(rng.random(nf) < walls)[0]
+        nrm = rng.normal(0.0, 1.0, (len(sel), 3))
+        nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+        for j, sl in enumerate(sel):
+            set_wall(w, sl, int(rng.integers(0, sides)), tuple(nrm[j]), tuple(rng.normal(0.0, 1.0, 3)))
+        hist.append((float(k), u, v, tags, w, float(k)))
+    return geom, blocks, p, hist[0], hist[1]

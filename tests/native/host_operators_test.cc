@@ -209,6 +209,63 @@ void CompatibleDepositMatchesOracle() {
   EXPECT(out[0].mass_quanta_sum == 1 && out.back().stress_force_quanta_sum == 3);
 }
 
+// GatherLiquidBoundaryParticles3D on the reference fixture (one block, unit cells, particle at 1.5) with one
+// missing face per axis (continued from same-side donors) and an oblique wall at one of them.
+void BoundaryGatherMatchesOracle() {
+  const UniformMacGeometry3D unit{{0.0, 0.0, 0.0}, 1.0};
+  const std::vector<BlockIndex3D> one = {{0, 0, 0}};
+  auto grid = SparseMacGrid3D::Create(unit, one, 1);
+  EXPECT(grid.ok());
+  const std::size_t nf = grid->face_count();
+  LiquidBoundaryParticle3D particle{};
+  particle.transfer.sorted_world_position = {1.5, 1.5, 1.5};
+  particle.transfer.trial_world_position = {1.5, 1.5, 1.5};
+  particle.transfer.particle_mass = 16.0;
+  particle.transfer.particle_velocity = {3.0, 4.0, 5.0};
+  particle.side_tag = 7;
+  std::vector<LiquidBoundaryFaceTag3D> tags(nf, {7, true});
+  std::vector<double> old_u(nf), new_u(nf);
+  std::vector<LiquidFaceValidity3D> validity(nf, LiquidFaceValidity3D::kValid);
+  std::vector<LiquidPrescribedWallSample3D> old_w(nf), new_w(nf);
+  for (std::size_t f = 0; f < nf; ++f) {
+    const auto face = grid->FaceIndex(f);
+    EXPECT(face.ok());
+    const auto a = face->lattice_index;
+    old_u[f] = 1.0 + a.x + 2.0 * a.y - a.z;
+    new_u[f] = 4.0 + 2.0 * a.x - a.y + 3.0 * a.z;
+  }
+  for (auto axis : {FaceNormalAxis::kX, FaceNormalAxis::kY, FaceNormalAxis::kZ}) {
+    const auto slot = grid->FaceSlot({axis, {1, 1, 1}});
+    EXPECT(slot.ok());
+    validity[*slot] = LiquidFaceValidity3D::kMissing;
+  }
+  const auto target = *grid->FaceSlot({FaceNormalAxis::kX, {1, 1, 1}});
+  old_w[target] = {true, 7, {std::sqrt(0.5), std::sqrt(0.5), 0.0}, {2.0, -1.0, 4.0}};
+  new_w[target] = {true, 7, {std::sqrt(0.5), std::sqrt(0.5), 0.0}, {5.0, 1.0, -2.0}};
+  const LiquidBoundaryGatherInputs3D in{*grid, {&particle, 1}, 0.0, 1.0, {0.0, 0.0, old_u, validity, tags, old_w},
+                                        {1.0, 1.0, new_u, validity, tags, new_w}};
+  auto got = GatherLiquidBoundaryParticles3D(in);
+  EXPECT(got.ok());
+  const auto blocks = grid->active_block_indices();
+  flo_history oh{0.0, 0.0, old_u.data(), reinterpret_cast<const std::uint8_t*>(validity.data()),
+                 reinterpret_cast<const flo_face_tag*>(tags.data()), reinterpret_cast<const flo_wall_sample*>(old_w.data()),
+                 nf};
+  flo_history nh{1.0, 1.0, new_u.data(), reinterpret_cast<const std::uint8_t*>(validity.data()),
+                 reinterpret_cast<const flo_face_tag*>(tags.data()), reinterpret_cast<const flo_wall_sample*>(new_w.data()),
+                 nf};
+  flo_boundary_gather want{};
+  std::uint64_t counts[3];
+  EXPECT(flo_gather_boundary(reinterpret_cast<const flo_geometry*>(&unit), O(blocks), blocks.size(),
+                             reinterpret_cast<const flo_boundary_particle*>(&particle), 1, 0.0, 1.0, &oh, &nh, &want,
+                             counts) == 0);
+  const auto& g = got->particles[0];
+  const double* a = &g.new_pic_velocity.x;
+  const double* b = &want.new_pic_velocity.x;
+  for (int i = 0; i < 27; ++i) EXPECT(std::fabs(a[i] - b[i]) <= 2e-12 + 1e-12 * std::fabs(b[i]));
+  EXPECT(g.affine_enabled == (want.affine_enabled != 0) && g.old_pic_fallback == (want.old_pic_fallback != 0));
+  EXPECT(got->old_pic_fallback_particle_count == counts[0] && got->low_mass_fallback_particle_count == counts[2]);
+}
+
 void Fp64DepositMatchesOracle() {
   auto grid = SparseMacGrid3D::Create(kGeometry, DenseBlocks(4, 4, 4), 64);
   EXPECT(grid.ok());
@@ -454,6 +511,7 @@ int main(int argc, char** argv) {
     failed += Run("DepositFailuresPreserveOutput", DepositFailuresPreserveOutput);
     failed += Run("Fp64DepositMatchesOracle", Fp64DepositMatchesOracle);
     failed += Run("CompatibleDepositMatchesOracle", CompatibleDepositMatchesOracle);
+    failed += Run("BoundaryGatherMatchesOracle", BoundaryGatherMatchesOracle);
     failed += Run("GatherMatchesOracle", GatherMatchesOracle);
     failed += Run("BackendStagesMatchOracle", BackendStagesMatchOracle);
     failed += Run("ProjectionMeetsGatesAndMatchesOracle", ProjectionMeetsGatesAndMatchesOracle);

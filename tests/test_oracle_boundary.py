@@ -186,3 +186,43 @@ def test_all_compatible_mask_reproduces_the_plain_deposit_to_rounding():
     assert dm.max() <= 64
     assert dp.max() <= 1e-9 * np.abs(plain["pure_momentum_quanta_sum"]).max()
     assert dm.max() > 0 or dp.max() > 0  # the two deposits are distinct computations
+
+
+def test_quantizes_each_particle_before_summing_and_includes_guard_endpoint():
+    # liquid_boundary_transfer_3d_cases.h:423-451
+    blocks = one_block()
+    tags = K.boundary_tags(len(blocks), eligible=False)
+    for axis in range(3):
+        tags[slot(blocks, axis, (1, 1, 1))]["structurally_eligible"] = 1
+    p = np.zeros(2, O.BOUNDARY_PARTICLE)  # (np.concatenate would drop the record padding)
+    p[:] = K.boundary_particle(mass=0.5, velocity=(-3.0, 3.0, 5.0))[0]
+    s, quanta, _ = deposit(blocks, p, tags, q=1.0, cap=2)
+    x = slot(blocks, 0, (1, 1, 1))
+    assert s == O.OK and quanta[x]["mass_quanta_sum"] == 0 and quanta[x]["pure_momentum_quanta_sum"] == -4
+    for i in range(2):
+        t = p[i]["transfer"]
+        t["particle_mass"] = 1.0
+        t["particle_velocity"]["x"], t["particle_velocity"]["y"], t["particle_velocity"]["z"] = 2.0 ** 61, 0.0, 0.0
+        p[i]["transfer"] = t
+    s, quanta, _ = deposit(blocks, p, tags, q=1.0, cap=2)
+    assert s == O.OK and quanta[x]["pure_momentum_quanta_sum"] == 1 << 62
+    t = p[1]["transfer"]
+    t["particle_velocity"]["x"] = np.nextafter(2.0 ** 61, np.inf)
+    p[1]["transfer"] = t
+    assert deposit(blocks, p, tags, q=1.0, cap=2)[0] == O.OUT_OF_RANGE
+
+
+def test_complete_boundary_deposit_preserves_unmasked_api_at_signed_translated_scale():
+    # liquid_boundary_transfer_3d_cases.h:452-473: one block at (-1,-1,-1), origin (11,-17,31), h = 2.
+    geom = O.geometry((11.0, -17.0, 31.0), 2.0)
+    blocks = K.sorted_blocks(geom, [(-1, -1, -1)])
+    p = K.boundary_particle(position=(6.5, -21.5, 26.5), mass=2.0, velocity=(1.0, 2.0, 3.0), side=-3)
+    K.set_slopes(p, 0, ((2.0, -1.0, 3.0), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)))
+    tags = K.boundary_tags(len(blocks), side=-3)
+    q = 2.0 ** -30
+    s, plain = O.deposit_quanta(geom, blocks, np.ascontiguousarray(p["transfer"]), q, q, 1)
+    assert s == O.OK
+    s, quanta, _ = O.deposit_compatible(geom, blocks, p, tags, q, q, 1)
+    assert s == O.OK
+    assert np.array_equal(quanta["mass_quanta_sum"], plain["mass_quanta_sum"])
+    assert np.array_equal(quanta["pure_momentum_quanta_sum"], plain["pure_momentum_quanta_sum"])
