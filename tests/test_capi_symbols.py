@@ -29,6 +29,8 @@ def test_record_layouts_match_the_header():
     assert R.FACE_QUANTA.itemsize == 24 and R.SORT_ENTRY.itemsize == 24
     assert R.PRESSURE_CELL.itemsize == 56 and R.PRESSURE_FACE.itemsize == 80
     assert R.PRESSURE_OPTIONS.itemsize == 56 and R.PRESSURE_DIAGNOSTICS.itemsize == 96
+    assert R.BOUNDARY_PARTICLE.itemsize == 168 and R.FACE_TAG.itemsize == 16 and R.COMPATIBLE_SAMPLE.itemsize == 72
+    assert R.WALL_SAMPLE.itemsize == 64 and R.BOUNDARY_GATHER.itemsize == 224
 
 
 def test_create_without_device_fails_cleanly():
