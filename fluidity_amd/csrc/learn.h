@@ -50,8 +50,9 @@ struct LearnState {
 };
 
 uint64_t net_param_count();
-// Forward pass over nt tiles whose inputs are at kActIn of each record. mfma: the 8^3 layers run as
-// BF16 implicit GEMMs on the matrix cores (inference); otherwise every layer is FP32.
+// Forward pass over nt tiles whose inputs are at kActIn of each record. mfma: every convolution but
+// the 1-channel output runs as a BF16 implicit GEMM on the matrix cores (inference); otherwise every
+// layer is FP32.
 int net_forward(hipStream_t s, const float* params, float* act, uint64_t nt, bool mfma = false);
 // Masked loss: scalars[0] = sum of the liquid mask, scalars[1] = loss; writes dL/dout into grad.
 int net_loss(hipStream_t s, const float* act, const float* target, float* grad, uint64_t nt, float* partial,

@@ -294,11 +294,13 @@ __global__ void k_fill_f32(float* p, uint64_t n, float v) {
     p[i] = v;
 }
 
-// BF16 inference path for the 8^3 layers: an implicit GEMM per tile on the matrix cores.
-// Y[voxel][co] = sum_k X[voxel][k] W[k][co], k = ci*27 + tap, with X gathered from a zero-padded
-// 10^3 copy of the tile's input channels in LDS and W staged as [k][16] (co padded to 16); both are
-// rounded to BF16, products accumulate in FP32 (v_mfma_f32_16x16x16bf16_1k). One workgroup of four
-// waves per tile; each wave owns 8 of the 32 row blocks of 16 voxels.
+// BF16 inference path: every convolution but the 1-channel output as an implicit GEMM on the matrix
+// cores, Y[m][co] = sum_k X[m][k] W[k][co] with m a voxel of one of TPW tiles and k = ci*27 + tap,
+// products in BF16, accumulation in FP32 (v_mfma_f32_16x16x16bf16_1k). The tiles' input channels are
+// staged zero-padded in LDS as BF16 ((SIDE+2)^3 per channel), the weights transposed as [co][k] (k
+// padded to 16, co to 16), and the im2col offset of every k in a table, so an A fragment is four LDS
+// loads at base(m) + off(k) with no index arithmetic. Each wave owns a run of 16-row blocks and
+// reuses every B fragment across them.
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -307,63 +309,102 @@ __device__ inline uint16_t bf16_bits(float v) {
   return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);  // nearest even (finite v)
 }
 
-template <int CIN, int COUT>
-__global__ __launch_bounds__(256) void k_conv8_mfma(float* act, LayerSpec L, const float* w, const float* bias) {
+template <int SIDE, int CIN, int COUT, int TPW>
+__global__ __launch_bounds__(256) void k_conv_mfma(float* act, uint64_t ntiles, LayerSpec L, const float* w,
+                                                   const float* bias) {
+  constexpr int P = SIDE + 2, P3 = P * P * P, V = SIDE * SIDE * SIDE;
   constexpr int K = CIN * 27, KP = (K + 15) / 16 * 16;
-  __shared__ uint16_t xs[CIN * 1000];
-  __shared__ uint16_t ws[KP * 16];
-  const uint64_t t = blockIdx.x;
-  float* base = act + t * kActFloats;
-  for (int idx = threadIdx.x; idx < KP * 16; idx += 256) {
-    const int k = idx >> 4, n = idx & 15;
+  constexpr int NP = COUT < 16 ? 16 : COUT, NB = NP / 16;
+  constexpr int M = TPW * V, RBW = M / 64;  // 16-row blocks per wave
+  static_assert(M % 64 == 0, "four waves of whole row blocks");
+  __shared__ uint16_t xs[TPW * CIN * P3];
+  __shared__ uint16_t wt[NP * KP];
+  __shared__ int32_t koff[KP];
+  const uint64_t t0 = blockIdx.x * uint64_t(TPW);
+  for (int idx = threadIdx.x; idx < NP * KP; idx += 256) {
+    const int co = idx / KP, k = idx % KP;
     float v = 0.f;
-    if (k < K && n < COUT) v = w[(n * CIN + k / 27) * 27 + k % 27];
-    ws[idx] = bf16_bits(v);
+    if (k < K && co < COUT) v = w[(co * CIN + k / 27) * 27 + k % 27];
+    wt[idx] = bf16_bits(v);
   }
-  for (int idx = threadIdx.x; idx < CIN * 1000; idx += 256) {
-    const int ci = idx / 1000, p = idx % 1000;
-    const int pz = p / 100, py = (p / 10) % 10, px = p % 10;
+  for (int k = threadIdx.x; k < KP; k += 256) {
+    int o = 0;
+    if (k < K) {
+      const int ci = k / 27, tp = k % 27;
+      o = ci * P3 + ((tp / 9 - 1) * P + ((tp / 3) % 3 - 1)) * P + (tp % 3 - 1);
+    }
+    koff[k] = o;
+  }
+  for (int idx = threadIdx.x; idx < TPW * CIN * P3; idx += 256) {
+    const int tt = idx / (CIN * P3), rem = idx % (CIN * P3), ci = rem / P3, pp = rem % P3;
+    const int pz = pp / (P * P), py = (pp / P) % P, px = pp % P;
     float v = 0.f;
-    if (pz >= 1 && pz <= 8 && py >= 1 && py <= 8 && px >= 1 && px <= 8) {
-      const float* src = ci < L.ca ? base + L.a + ci * 512 : base + L.b + (ci - L.ca) * 512;
-      v = src[((pz - 1) * 8 + (py - 1)) * 8 + (px - 1)];
+    if (t0 + tt < ntiles && pz >= 1 && pz <= SIDE && py >= 1 && py <= SIDE && px >= 1 && px <= SIDE) {
+      const float* base = act + (t0 + tt) * kActFloats;
+      const float* src = ci < L.ca ? base + L.a + ci * V : base + L.b + (ci - L.ca) * V;
+      v = src[((pz - 1) * SIDE + (py - 1)) * SIDE + (px - 1)];
     }
     xs[idx] = bf16_bits(v);
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = lane & 15, kq = lane >> 4;
-  for (int mt = wave; mt < 32; mt += 4) {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    const int m = mt * 16 + r;  // the voxel of this lane's A row
-    const int mz = m >> 6, my = (m >> 3) & 7, mx = m & 7;
-    for (int k0 = 0; k0 < KP; k0 += 16) {
-      bf16x4 a, b;
+  int base_m[RBW];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int k = k0 + kq * 4 + e;
-        uint16_t av = 0;
-        if (k < K) {
-          const int ci = k / 27, tp = k % 27;
-          const int dz = tp / 9 - 1, dy = (tp / 3) % 3 - 1, dx = tp % 3 - 1;
-          av = xs[ci * 1000 + ((mz + 1 + dz) * 10 + (my + 1 + dy)) * 10 + (mx + 1 + dx)];
-        }
-        a[e] = static_cast<short>(av);
-        b[e] = static_cast<short>(ws[k * 16 + r]);  // B[k][column r]
-      }
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  for (int j = 0; j < RBW; ++j) {
+    const int m = (wave * RBW + j) * 16 + r;
+    const int tt = m / V, v = m % V;
+    const int z = v / (SIDE * SIDE), y = (v / SIDE) % SIDE, x = v % SIDE;
+    base_m[j] = tt * CIN * P3 + ((z + 1) * P + (y + 1)) * P + (x + 1);
+  }
+  f32x4 acc[RBW][NB];
+#pragma unroll
+  for (int j = 0; j < RBW; ++j)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[j][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < KP; k0 += 16) {
+    const int kb = k0 + kq * 4;
+    const int o0 = koff[kb], o1 = koff[kb + 1], o2 = koff[kb + 2], o3 = koff[kb + 3];
+    bf16x4 bf[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const uint16_t* col = wt + (nb * 16 + r) * KP + kb;
+      bf[nb] = bf16x4{static_cast<short>(col[0]), static_cast<short>(col[1]), static_cast<short>(col[2]),
+                      static_cast<short>(col[3])};
     }
-    // D[i][j] sits in lane j + 16 (i / 4), element i % 4: column r is the output channel.
-    if (r < COUT) {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int voxel = mt * 16 + kq * 4 + e;
-        float v = acc[e] + bias[r];
-        if (L.relu && v < 0.f) v = 0.f;
-        base[L.y + r * 512 + voxel] = v;
-      }
+    for (int j = 0; j < RBW; ++j) {
+      const int bm = base_m[j];
+      const bf16x4 a = {static_cast<short>(xs[bm + o0]), static_cast<short>(xs[bm + o1]),
+                        static_cast<short>(xs[bm + o2]), static_cast<short>(xs[bm + o3])};
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) acc[j][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bf[nb], acc[j][nb], 0, 0, 0);
     }
   }
+  // D[i][j] sits in lane j + 16 (i / 4), element i % 4: column r is the output channel.
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    const int co = nb * 16 + r;
+    if (co >= COUT) continue;
+    const float bco = bias[co];
+#pragma unroll
+    for (int j = 0; j < RBW; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int m = (wave * RBW + j) * 16 + kq * 4 + e;
+        const int tt = m / V, v = m % V;
+        if (t0 + tt >= ntiles) continue;
+        float val = acc[j][nb][e] + bco;
+        if (L.relu && val < 0.f) val = 0.f;
+        act[(t0 + tt) * kActFloats + L.y + co * V + v] = val;
+      }
+  }
+}
+
+template <int SIDE, int CIN, int COUT, int TPW>
+void launch_conv_mfma(hipStream_t s, float* act, uint64_t nt, const LayerSpec& L, const float* w, const float* b) {
+  const unsigned g = static_cast<unsigned>((nt + TPW - 1) / TPW);
+  k_conv_mfma<SIDE, CIN, COUT, TPW><<<g, 256, 0, s>>>(act, nt, L, w, b);
 }
 
 template <int N>
@@ -381,12 +422,17 @@ int conv_fwd(hipStream_t s, float* act, uint64_t nt, int l, const float* params,
   const LayerSpec& L = kLayers[l];
   const float* w = params + layer_offset(l);
   const float* b = w + layer_weights(l);
-  if (mfma && L.side == 8 && L.cout == 8) {
-    const unsigned g = static_cast<unsigned>(nt);
-    switch (L.ca + L.cb) {
-      case 4: k_conv8_mfma<4, 8><<<g, 256, 0, s>>>(act, L, w, b); break;
-      case 8: k_conv8_mfma<8, 8><<<g, 256, 0, s>>>(act, L, w, b); break;
-      case 24: k_conv8_mfma<24, 8><<<g, 256, 0, s>>>(act, L, w, b); break;
+  if (mfma && L.cout > 1) {
+    switch (l) {
+      case 0: launch_conv_mfma<8, 4, 8, 1>(s, act, nt, L, w, b); break;
+      case 1: launch_conv_mfma<8, 8, 8, 1>(s, act, nt, L, w, b); break;
+      case 2: launch_conv_mfma<4, 8, 16, 8>(s, act, nt, L, w, b); break;
+      case 3: launch_conv_mfma<4, 16, 16, 8>(s, act, nt, L, w, b); break;
+      case 4: launch_conv_mfma<2, 16, 32, 16>(s, act, nt, L, w, b); break;
+      case 5: launch_conv_mfma<2, 32, 32, 16>(s, act, nt, L, w, b); break;
+      case 6: launch_conv_mfma<4, 48, 16, 4>(s, act, nt, L, w, b); break;
+      case 7: launch_conv_mfma<4, 16, 16, 8>(s, act, nt, L, w, b); break;
+      case 8: launch_conv_mfma<8, 24, 8, 1>(s, act, nt, L, w, b); break;
       default: return FL_INTERNAL;
     }
     FL_HIP(hipGetLastError());
@@ -428,14 +474,14 @@ int net_forward(hipStream_t s, const float* params, float* act, uint64_t nt, boo
   FL_TRY(conv_fwd(s, act, nt, 0, params, mfma));
   FL_TRY(conv_fwd(s, act, nt, 1, params, mfma));
   k_pool_fwd<<<grid_for(nt * 8 * 64), kThreads, 0, s>>>(act, nt, 8, 4, kActA1, kActP1);
-  FL_TRY(conv_fwd(s, act, nt, 2, params));
-  FL_TRY(conv_fwd(s, act, nt, 3, params));
+  FL_TRY(conv_fwd(s, act, nt, 2, params, mfma));
+  FL_TRY(conv_fwd(s, act, nt, 3, params, mfma));
   k_pool_fwd<<<grid_for(nt * 16 * 8), kThreads, 0, s>>>(act, nt, 16, 2, kActA3, kActP2);
-  FL_TRY(conv_fwd(s, act, nt, 4, params));
-  FL_TRY(conv_fwd(s, act, nt, 5, params));
+  FL_TRY(conv_fwd(s, act, nt, 4, params, mfma));
+  FL_TRY(conv_fwd(s, act, nt, 5, params, mfma));
   k_up_fwd<<<grid_for(nt * 32 * 64), kThreads, 0, s>>>(act, nt, 32, 2, kActA5, kActU5);
-  FL_TRY(conv_fwd(s, act, nt, 6, params));
-  FL_TRY(conv_fwd(s, act, nt, 7, params));
+  FL_TRY(conv_fwd(s, act, nt, 6, params, mfma));
+  FL_TRY(conv_fwd(s, act, nt, 7, params, mfma));
   k_up_fwd<<<grid_for(nt * 16 * 512), kThreads, 0, s>>>(act, nt, 16, 4, kActA7, kActU7);
   FL_TRY(conv_fwd(s, act, nt, 8, params, mfma));
   FL_TRY(conv_fwd(s, act, nt, 9, params));
