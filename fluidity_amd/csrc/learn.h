@@ -47,13 +47,18 @@ struct LearnState {
   float* data_target = nullptr;    // [data_cap][512]
   uint64_t data_cap = 0, ndata = 0;
   int collect_every = 0;  // > 0: the MGPCG solver appends (residual, correction) pairs every k-th iteration
+  uint8_t* frags = nullptr;  // [net_fragment_bytes()] BF16 weights in matrix-core operand order
 };
 
 uint64_t net_param_count();
-// Forward pass over nt tiles whose inputs are at kActIn of each record. mfma: every convolution but
-// the 1-channel output runs as a BF16 implicit GEMM on the matrix cores (inference); otherwise every
-// layer is FP32.
-int net_forward(hipStream_t s, const float* params, float* act, uint64_t nt, bool mfma = false);
+// FP32 forward pass over nt tiles whose inputs are at kActIn of each record (training, and the FP32
+// reference of the inference path); every activation is kept in the records for the backward pass.
+int net_forward(hipStream_t s, const float* params, float* act, uint64_t nt);
+// BF16 inference over nt tiles: reads the inputs at kActIn and writes the output at kActOut of each
+// record, with the whole network of a tile in one workgroup on the matrix cores. frags holds
+// net_fragment_bytes() bytes of device memory; the weights are repacked into it on every call.
+uint64_t net_fragment_bytes();
+int net_infer(hipStream_t s, const float* params, void* frags, float* act, uint64_t nt);
 // Masked loss: scalars[0] = sum of the liquid mask, scalars[1] = loss; writes dL/dout into grad.
 int net_loss(hipStream_t s, const float* act, const float* target, float* grad, uint64_t nt, float* partial,
              float* scalars);

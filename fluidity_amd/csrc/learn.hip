@@ -294,13 +294,17 @@ __global__ void k_fill_f32(float* p, uint64_t n, float v) {
     p[i] = v;
 }
 
-// BF16 inference path: every convolution but the 1-channel output as an implicit GEMM on the matrix
-// cores, Y[m][co] = sum_k X[m][k] W[k][co] with m a voxel of one of TPW tiles and k = ci*27 + tap,
-// products in BF16, accumulation in FP32 (v_mfma_f32_16x16x16bf16_1k). The tiles' input channels are
-// staged zero-padded in LDS as BF16 ((SIDE+2)^3 per channel), the weights transposed as [co][k] (k
-// padded to 16, co to 16), and the im2col offset of every k in a table, so an A fragment is four LDS
-// loads at base(m) + off(k) with no index arithmetic. Each wave owns a run of 16-row blocks and
-// reuses every B fragment across them.
+// BF16 inference: the whole network on one 8^3 tile per workgroup (4 waves), every activation kept in
+// LDS as BF16, so a tile reads its 4 x 512 inputs and writes its 512 outputs and nothing else goes
+// to HBM. Activations are stored channel-innermost with a one-voxel zero halo ([(S+2)^3][CS], CS the
+// channel stride), so the four channels at one tap are one 8-byte LDS read. Every convolution, the
+// 1-channel output included, is an implicit GEMM on the matrix cores with the output channels as
+// rows: Y^T[co][m] = sum_k W[co][k] X^T[k][m], k = tap * CIN + ci, m a voxel, products in BF16 and
+// accumulation in FP32 (v_mfma_f32_16x16x16bf16_1k). The weights are packed once per call into the
+// A-operand fragment order of each (16-row output block, 16-wide k step), so a wave's weight load is
+// one coalesced 512-byte read; the B operand is one LDS read per lane; each lane of the result holds
+// four consecutive output channels of one voxel, stored as one 8-byte LDS write after bias and ReLU.
+// Pooling (FP32 sum of the 8 BF16 children) and nearest upsampling run between the convolutions.
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -308,103 +312,265 @@ __device__ inline uint16_t bf16_bits(float v) {
   const uint32_t u = __float_as_uint(v);
   return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);  // nearest even (finite v)
 }
+__device__ inline float bf16_value(short h) { return __uint_as_float(uint32_t(uint16_t(h)) << 16); }
+__device__ inline bf16x4 bf16_pack(float a, float b, float c, float d) {
+  return bf16x4{static_cast<short>(bf16_bits(a)), static_cast<short>(bf16_bits(b)), static_cast<short>(bf16_bits(c)),
+                static_cast<short>(bf16_bits(d))};
+}
 
-template <int SIDE, int CIN, int COUT, int TPW>
-__global__ __launch_bounds__(256) void k_conv_mfma(float* act, uint64_t ntiles, LayerSpec L, const float* w,
-                                                   const float* bias) {
-  constexpr int P = SIDE + 2, P3 = P * P * P, V = SIDE * SIDE * SIDE;
-  constexpr int K = CIN * 27, KP = (K + 15) / 16 * 16;
-  constexpr int NP = COUT < 16 ? 16 : COUT, NB = NP / 16;
-  constexpr int M = TPW * V, RBW = M / 64;  // 16-row blocks per wave
-  static_assert(M % 64 == 0, "four waves of whole row blocks");
-  __shared__ uint16_t xs[TPW * CIN * P3];
-  __shared__ uint16_t wt[NP * KP];
-  __shared__ int32_t koff[KP];
-  const uint64_t t0 = blockIdx.x * uint64_t(TPW);
-  for (int idx = threadIdx.x; idx < NP * KP; idx += 256) {
-    const int co = idx / KP, k = idx % KP;
-    float v = 0.f;
-    if (k < K && co < COUT) v = w[(co * CIN + k / 27) * 27 + k % 27];
-    wt[idx] = bf16_bits(v);
+constexpr int frag_steps(int l) { return (27 * (kLayers[l].ca + kLayers[l].cb) + 15) / 16; }
+constexpr int frag_blocks(int l) { return (kLayers[l].cout + 15) / 16; }
+constexpr int frag_offset(int l) {
+  int o = 0;
+  for (int i = 0; i < l; ++i) o += frag_steps(i) * frag_blocks(i) * 64;
+  return o;
+}
+constexpr int param_offset(int l) {
+  int o = 0;
+  for (int i = 0; i < l; ++i) o += kLayers[i].cout * (kLayers[i].ca + kLayers[i].cb) * 27 + kLayers[i].cout;
+  return o;
+}
+
+struct FragSpec {
+  int cin[kNetLayers], cout[kNetLayers], steps[kNetLayers], first[kNetLayers + 1], param[kNetLayers];
+};
+
+// Fragment f of layer l (f = (block * steps + step) * 64 + lane): W[co][k] for co = 16 block + lane % 16
+// and k = 16 step + 4 (lane / 16) + e, e = 0..3, zero outside the layer's channels and taps.
+__global__ void k_pack_fragments(const float* params, FragSpec spec, bf16x4* frags) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= spec.first[kNetLayers]) return;
+  int l = 0;
+  while (f >= spec.first[l + 1]) ++l;
+  const int local = f - spec.first[l], lane = local & 63, step = (local >> 6) % spec.steps[l];
+  const int block = (local >> 6) / spec.steps[l];
+  const int cin = spec.cin[l], co = block * 16 + (lane & 15);
+  const float* w = params + spec.param[l];
+  float v[4];
+  for (int e = 0; e < 4; ++e) {
+    const int k = 16 * step + 4 * (lane >> 4) + e;
+    v[e] = (co < spec.cout[l] && k < 27 * cin) ? w[(co * cin + k % cin) * 27 + k / cin] : 0.f;
   }
-  for (int k = threadIdx.x; k < KP; k += 256) {
-    int o = 0;
-    if (k < K) {
-      const int ci = k / 27, tp = k % 27;
-      o = ci * P3 + ((tp / 9 - 1) * P + ((tp / 3) % 3 - 1)) * P + (tp % 3 - 1);
+  frags[f] = bf16_pack(v[0], v[1], v[2], v[3]);
+}
+
+// Padded LDS index of voxel (z, y, x) of a side-S grid.
+template <int S>
+__device__ constexpr int padded(int z, int y, int x) {
+  return ((z + 1) * (S + 2) + (y + 1)) * (S + 2) + (x + 1);
+}
+
+// Zeroes every channel slot of the halo voxels of a side-S buffer.
+template <int S, int CS>
+__device__ void zero_halo(uint16_t* buf) {
+  constexpr int P = S + 2;
+  for (int i = threadIdx.x; i < P * P * P; i += 256) {
+    const int pz = i / (P * P), py = (i / P) % P, px = i % P;
+    if (pz == 0 || pz == P - 1 || py == 0 || py == P - 1 || px == 0 || px == P - 1) {
+      bf16x4* d = reinterpret_cast<bf16x4*>(buf + i * CS);
+#pragma unroll
+      for (int c = 0; c < CS / 4; ++c) d[c] = bf16x4{0, 0, 0, 0};
     }
-    koff[k] = o;
-  }
-  for (int idx = threadIdx.x; idx < TPW * CIN * P3; idx += 256) {
-    const int tt = idx / (CIN * P3), rem = idx % (CIN * P3), ci = rem / P3, pp = rem % P3;
-    const int pz = pp / (P * P), py = (pp / P) % P, px = pp % P;
-    float v = 0.f;
-    if (t0 + tt < ntiles && pz >= 1 && pz <= SIDE && py >= 1 && py <= SIDE && px >= 1 && px <= SIDE) {
-      const float* base = act + (t0 + tt) * kActFloats;
-      const float* src = ci < L.ca ? base + L.a + ci * V : base + L.b + (ci - L.ca) * V;
-      v = src[((pz - 1) * SIDE + (py - 1)) * SIDE + (px - 1)];
-    }
-    xs[idx] = bf16_bits(v);
-  }
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int r = lane & 15, kq = lane >> 4;
-  int base_m[RBW];
-#pragma unroll
-  for (int j = 0; j < RBW; ++j) {
-    const int m = (wave * RBW + j) * 16 + r;
-    const int tt = m / V, v = m % V;
-    const int z = v / (SIDE * SIDE), y = (v / SIDE) % SIDE, x = v % SIDE;
-    base_m[j] = tt * CIN * P3 + ((z + 1) * P + (y + 1)) * P + (x + 1);
-  }
-  f32x4 acc[RBW][NB];
-#pragma unroll
-  for (int j = 0; j < RBW; ++j)
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) acc[j][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < KP; k0 += 16) {
-    const int kb = k0 + kq * 4;
-    const int o0 = koff[kb], o1 = koff[kb + 1], o2 = koff[kb + 2], o3 = koff[kb + 3];
-    bf16x4 bf[NB];
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
-      const uint16_t* col = wt + (nb * 16 + r) * KP + kb;
-      bf[nb] = bf16x4{static_cast<short>(col[0]), static_cast<short>(col[1]), static_cast<short>(col[2]),
-                      static_cast<short>(col[3])};
-    }
-#pragma unroll
-    for (int j = 0; j < RBW; ++j) {
-      const int bm = base_m[j];
-      const bf16x4 a = {static_cast<short>(xs[bm + o0]), static_cast<short>(xs[bm + o1]),
-                        static_cast<short>(xs[bm + o2]), static_cast<short>(xs[bm + o3])};
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb) acc[j][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bf[nb], acc[j][nb], 0, 0, 0);
-    }
-  }
-  // D[i][j] sits in lane j + 16 (i / 4), element i % 4: column r is the output channel.
-#pragma unroll
-  for (int nb = 0; nb < NB; ++nb) {
-    const int co = nb * 16 + r;
-    if (co >= COUT) continue;
-    const float bco = bias[co];
-#pragma unroll
-    for (int j = 0; j < RBW; ++j)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int m = (wave * RBW + j) * 16 + kq * 4 + e;
-        const int tt = m / V, v = m % V;
-        if (t0 + tt >= ntiles) continue;
-        float val = acc[j][nb][e] + bco;
-        if (L.relu && val < 0.f) val = 0.f;
-        act[(t0 + tt) * kActFloats + L.y + co * V + v] = val;
-      }
   }
 }
 
-template <int SIDE, int CIN, int COUT, int TPW>
-void launch_conv_mfma(hipStream_t s, float* act, uint64_t nt, const LayerSpec& L, const float* w, const float* b) {
-  const unsigned g = static_cast<unsigned>((nt + TPW - 1) / TPW);
-  k_conv_mfma<SIDE, CIN, COUT, TPW><<<g, 256, 0, s>>>(act, nt, L, w, b);
+// Weight fragments of k-steps s0 .. s0 + U - 1 (clamped to the last step) of CPW output blocks.
+template <int U, int CPW, int KS>
+__device__ inline void load_fragments(bf16x4 (&w)[U][CPW], const bf16x4* wf, int cb0, int s0) {
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) w[u][c] = wf[((cb0 + c) * KS + min(s0 + u, KS - 1)) * 64];
+}
+
+// k-steps s0 .. s0 + U - 1 (those below KS) of the implicit GEMM: the lane's four k of a step are
+// four channels at one tap, one 8-byte LDS read per column block.
+template <int U, int BPW, int CPW, int KS, int CIN, int P, int XCS>
+__device__ inline void conv_steps(const bf16x4 (&w)[U][CPW], const uint16_t* x, const int (&vb)[BPW], int q, int s0,
+                                  f32x4 (&acc)[BPW][CPW]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int s = s0 + u;
+    if (s >= KS) break;
+    const int k0 = 16 * s + 4 * q;
+    int off = 0;  // padding k: weights are zero, any in-range voxel will do
+    if (k0 < 27 * CIN) {
+      const int tp = k0 / CIN, ci = k0 % CIN;
+      off = (((tp / 9 - 1) * P + ((tp / 3) % 3 - 1)) * P + (tp % 3 - 1)) * XCS + ci;
+    }
+#pragma unroll
+    for (int b = 0; b < BPW; ++b) {
+      const bf16x4 xb = *reinterpret_cast<const bf16x4*>(x + vb[b] + off);
+#pragma unroll
+      for (int c = 0; c < CPW; ++c) acc[b][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w[u][c], xb, acc[b][c], 0, 0, 0);
+    }
+  }
+}
+
+// Convolution l from the LDS buffer at XOFF (channel stride XCS) into channels YCH0.. of the buffer at
+// YOFF (stride YCS), or, for the output layer, into out[512] in FP32. A side-8 layer gives each wave 8
+// column blocks of 16 voxels, a side-4 layer one; a side-2 layer (one half-used block) gives each of
+// its 16-channel output blocks to one wave.
+template <int l, int XOFF, int XCS, int YOFF, int YCS, int YCH0>
+__device__ void conv_lds(uint16_t* lds, const bf16x4* frags, const float* params, float* out) {
+  constexpr LayerSpec L = kLayers[l];
+  constexpr int S = L.side, P = S + 2, V = S * S * S, CIN = L.ca + L.cb, COUT = L.cout;
+  constexpr int KS = frag_steps(l), NCB = frag_blocks(l), NBLK = (V + 15) / 16;
+  constexpr int BPW = NBLK >= 4 ? NBLK / 4 : 1, CPW = NBLK >= 4 ? NCB : 1;
+  static_assert(NBLK >= 4 ? NBLK % 4 == 0 : NCB <= 4, "work split");
+  static_assert(CIN % 4 == 0 && XCS >= CIN && XCS % 4 == 0 && YCS % 4 == 0, "8-byte channel groups");
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, j = lane & 15, q = lane >> 4;
+  int blk0 = 0, cb0 = 0;
+  if (NBLK >= 4) {
+    blk0 = wave * BPW;
+  } else {
+    if (wave >= NCB) return;
+    cb0 = wave;
+  }
+  const uint16_t* x = lds + XOFF;
+  int vb[BPW];
+#pragma unroll
+  for (int b = 0; b < BPW; ++b) {
+    const int m = ((blk0 + b) * 16 + j) % V;  // the unused half of a side-2 block repeats real voxels
+    vb[b] = padded<S>(m / (S * S), (m / S) % S, m % S) * XCS;
+  }
+  f32x4 acc[BPW][CPW];
+#pragma unroll
+  for (int b = 0; b < BPW; ++b)
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) acc[b][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // Weight fragments are loaded one chunk of U k-steps ahead of their use (two register sets in
+  // turn), so the L2 latency of a load overlaps the matrix work of the chunk before.
+  constexpr int U = BPW >= 4 ? 4 : 16;
+  const bf16x4* wf = frags + frag_offset(l) + lane;
+  bf16x4 w0[U][CPW], w1[U][CPW];
+  load_fragments<U, CPW, KS>(w0, wf, cb0, 0);
+  for (int s0 = 0; s0 < KS; s0 += 2 * U) {
+    load_fragments<U, CPW, KS>(w1, wf, cb0, s0 + U);
+    conv_steps<U, BPW, CPW, KS, CIN, P, XCS>(w0, x, vb, q, s0, acc);
+    load_fragments<U, CPW, KS>(w0, wf, cb0, s0 + 2 * U);
+    conv_steps<U, BPW, CPW, KS, CIN, P, XCS>(w1, x, vb, q, s0 + U, acc);
+  }
+  // D[co][m] sits in lane m + 16 (co / 4), element co % 4.
+  constexpr int BOFF = param_offset(l) + COUT * CIN * 27;
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) {
+    const int co0 = (cb0 + c) * 16 + 4 * q;
+    if (co0 >= COUT) continue;
+#pragma unroll
+    for (int b = 0; b < BPW; ++b) {
+      const int m = (blk0 + b) * 16 + j;
+      if (m >= V) continue;
+      if constexpr (COUT == 1) {
+        out[m] = acc[b][c][0] + params[BOFF];
+      } else {
+        float v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          v[e] = acc[b][c][e] + params[BOFF + co0 + e];
+          if (L.relu && v[e] < 0.f) v[e] = 0.f;
+        }
+        *reinterpret_cast<bf16x4*>(lds + YOFF + padded<S>(m / (S * S), (m / S) % S, m % S) * YCS + YCH0 + co0) =
+            bf16_pack(v[0], v[1], v[2], v[3]);
+      }
+    }
+  }
+}
+
+// 2x2x2 average pooling of C channels from side S to S/2 (FP32 sum of the 8 children, in z, y, x order).
+template <int S, int C, int XOFF, int XCS, int XCH0, int YOFF, int YCS>
+__device__ void pool_lds(uint16_t* lds) {
+  constexpr int H = S / 2, G = C / 4;
+  for (int t = threadIdx.x; t < H * H * H * G; t += 256) {
+    const int v = t / G, g = t % G, z = v / (H * H), y = (v / H) % H, x = v % H;
+    float a[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const bf16x4 h = *reinterpret_cast<const bf16x4*>(
+          lds + XOFF + padded<S>(2 * z + (o >> 2), 2 * y + ((o >> 1) & 1), 2 * x + (o & 1)) * XCS + XCH0 + 4 * g);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) a[e] += bf16_value(h[e]);
+    }
+    *reinterpret_cast<bf16x4*>(lds + YOFF + padded<H>(z, y, x) * YCS + 4 * g) =
+        bf16_pack(0.125f * a[0], 0.125f * a[1], 0.125f * a[2], 0.125f * a[3]);
+  }
+}
+
+// Nearest upsampling of C channels from side S to 2S into channels YCH0.. of the target.
+template <int S, int C, int XOFF, int XCS, int YOFF, int YCS, int YCH0>
+__device__ void up_lds(uint16_t* lds) {
+  constexpr int D = 2 * S, G = C / 4;
+  for (int t = threadIdx.x; t < D * D * D * G; t += 256) {
+    const int v = t / G, g = t % G, z = v / (D * D), y = (v / D) % D, x = v % D;
+    *reinterpret_cast<bf16x4*>(lds + YOFF + padded<D>(z, y, x) * YCS + YCH0 + 4 * g) =
+        *reinterpret_cast<const bf16x4*>(lds + XOFF + padded<S>(z >> 1, y >> 1, x >> 1) * XCS + 4 * g);
+  }
+}
+
+// LDS plan, in BF16 elements. The concatenation (up(x7) ++ s1, 24 channels at side 8) has its own
+// region for the whole pass, since s1 is written by layer 1; everything else shares a scratch region
+// whose buffers are placed so that no phase reads what it writes. Channel strides of 16/48 channels
+// are padded by 4 so that 16 consecutive voxels fall on distinct LDS banks.
+namespace plan {
+constexpr int kC8 = 0, kC8Stride = 24;              // [10^3][24]: up(x7) 0..15, s1 16..23
+constexpr int kBase = 1000 * kC8Stride;
+constexpr int kX0 = kBase, kA0 = kBase + 4000;      // [10^3][4], [10^3][8]
+constexpr int kP1 = kBase + 4320, kA2 = kBase;      // [6^3][8], [6^3][20]
+constexpr int kC6 = kBase + 4320, kC6Stride = 52;   // [6^3][52]: up(x5) 0..31, s2 32..47
+constexpr int kP2 = kBase + 2048, kA4 = kBase, kA5 = kBase + 2048;  // [4^3][16], [4^3][32] x 2
+constexpr int kA6 = kBase, kA7 = kBase + 4320;      // [6^3][20] x 2
+constexpr int kA8 = kBase;                          // [10^3][8]
+constexpr int kElements = kC6 + 216 * kC6Stride;
+static_assert(kA0 + 8000 <= kElements && kA7 + 4320 <= kElements, "plan");
+}  // namespace plan
+
+__global__ __launch_bounds__(256) void k_unet_infer(float* act, const bf16x4* frags, const float* params) {
+  using namespace plan;
+  __shared__ __align__(16) uint16_t lds[kElements];
+  float* rec = act + blockIdx.x * uint64_t(kActFloats);
+  for (int v = threadIdx.x; v < 512; v += 256) {
+    const float* in = rec + kActIn + v;
+    *reinterpret_cast<bf16x4*>(lds + kX0 + padded<8>(v >> 6, (v >> 3) & 7, v & 7) * 4) =
+        bf16_pack(in[0], in[512], in[1024], in[1536]);
+  }
+  zero_halo<8, 4>(lds + kX0);
+  zero_halo<8, 8>(lds + kA0);
+  zero_halo<8, kC8Stride>(lds + kC8);
+  __syncthreads();
+  conv_lds<0, kX0, 4, kA0, 8, 0>(lds, frags, params, nullptr);
+  __syncthreads();
+  conv_lds<1, kA0, 8, kC8, kC8Stride, 16>(lds, frags, params, nullptr);
+  __syncthreads();
+  pool_lds<8, 8, kC8, kC8Stride, 16, kP1, 8>(lds);
+  zero_halo<4, 8>(lds + kP1);
+  __syncthreads();
+  conv_lds<2, kP1, 8, kA2, 20, 0>(lds, frags, params, nullptr);
+  zero_halo<4, 20>(lds + kA2);
+  __syncthreads();
+  conv_lds<3, kA2, 20, kC6, kC6Stride, 32>(lds, frags, params, nullptr);
+  zero_halo<4, kC6Stride>(lds + kC6);
+  __syncthreads();
+  pool_lds<4, 16, kC6, kC6Stride, 32, kP2, 16>(lds);
+  zero_halo<2, 16>(lds + kP2);
+  __syncthreads();
+  conv_lds<4, kP2, 16, kA4, 32, 0>(lds, frags, params, nullptr);
+  zero_halo<2, 32>(lds + kA4);
+  __syncthreads();
+  conv_lds<5, kA4, 32, kA5, 32, 0>(lds, frags, params, nullptr);
+  __syncthreads();
+  up_lds<2, 32, kA5, 32, kC6, kC6Stride, 0>(lds);
+  __syncthreads();
+  conv_lds<6, kC6, kC6Stride, kA6, 20, 0>(lds, frags, params, nullptr);
+  zero_halo<4, 20>(lds + kA6);
+  __syncthreads();
+  conv_lds<7, kA6, 20, kA7, 20, 0>(lds, frags, params, nullptr);
+  __syncthreads();
+  up_lds<4, 16, kA7, 20, kC8, kC8Stride, 0>(lds);
+  __syncthreads();
+  conv_lds<8, kC8, kC8Stride, kA8, 8, 0>(lds, frags, params, nullptr);
+  zero_halo<8, 8>(lds + kA8);
+  __syncthreads();
+  conv_lds<9, kA8, 8, 0, 4, 0>(lds, frags, params, rec + kActOut);
 }
 
 template <int N>
@@ -418,26 +584,10 @@ void conv_bwd_n(hipStream_t s, float* grad, uint64_t nt, const LayerSpec& L, con
   k_conv_bwd_data<N><<<grid_for(lanes), kThreads, 0, s>>>(grad, nt, L, w);
 }
 
-int conv_fwd(hipStream_t s, float* act, uint64_t nt, int l, const float* params, bool mfma = false) {
+int conv_fwd(hipStream_t s, float* act, uint64_t nt, int l, const float* params) {
   const LayerSpec& L = kLayers[l];
   const float* w = params + layer_offset(l);
   const float* b = w + layer_weights(l);
-  if (mfma && L.cout > 1) {
-    switch (l) {
-      case 0: launch_conv_mfma<8, 4, 8, 1>(s, act, nt, L, w, b); break;
-      case 1: launch_conv_mfma<8, 8, 8, 1>(s, act, nt, L, w, b); break;
-      case 2: launch_conv_mfma<4, 8, 16, 8>(s, act, nt, L, w, b); break;
-      case 3: launch_conv_mfma<4, 16, 16, 8>(s, act, nt, L, w, b); break;
-      case 4: launch_conv_mfma<2, 16, 32, 16>(s, act, nt, L, w, b); break;
-      case 5: launch_conv_mfma<2, 32, 32, 16>(s, act, nt, L, w, b); break;
-      case 6: launch_conv_mfma<4, 48, 16, 4>(s, act, nt, L, w, b); break;
-      case 7: launch_conv_mfma<4, 16, 16, 8>(s, act, nt, L, w, b); break;
-      case 8: launch_conv_mfma<8, 24, 8, 1>(s, act, nt, L, w, b); break;
-      default: return FL_INTERNAL;
-    }
-    FL_HIP(hipGetLastError());
-    return FL_OK;
-  }
   switch (L.cout) {
     case 1: conv_fwd_n<1>(s, act, nt, L, w, b); break;
     case 8: conv_fwd_n<8>(s, act, nt, L, w, b); break;
@@ -469,22 +619,43 @@ int conv_bwd_data(hipStream_t s, float* grad, uint64_t nt, int l, const float* p
 
 uint64_t net_param_count() { return layer_offset(kNetLayers); }
 
-int net_forward(hipStream_t s, const float* params, float* act, uint64_t nt, bool mfma) {
+int net_forward(hipStream_t s, const float* params, float* act, uint64_t nt) {
   if (!nt) return FL_OK;
-  FL_TRY(conv_fwd(s, act, nt, 0, params, mfma));
-  FL_TRY(conv_fwd(s, act, nt, 1, params, mfma));
+  FL_TRY(conv_fwd(s, act, nt, 0, params));
+  FL_TRY(conv_fwd(s, act, nt, 1, params));
   k_pool_fwd<<<grid_for(nt * 8 * 64), kThreads, 0, s>>>(act, nt, 8, 4, kActA1, kActP1);
-  FL_TRY(conv_fwd(s, act, nt, 2, params, mfma));
-  FL_TRY(conv_fwd(s, act, nt, 3, params, mfma));
+  FL_TRY(conv_fwd(s, act, nt, 2, params));
+  FL_TRY(conv_fwd(s, act, nt, 3, params));
   k_pool_fwd<<<grid_for(nt * 16 * 8), kThreads, 0, s>>>(act, nt, 16, 2, kActA3, kActP2);
-  FL_TRY(conv_fwd(s, act, nt, 4, params, mfma));
-  FL_TRY(conv_fwd(s, act, nt, 5, params, mfma));
+  FL_TRY(conv_fwd(s, act, nt, 4, params));
+  FL_TRY(conv_fwd(s, act, nt, 5, params));
   k_up_fwd<<<grid_for(nt * 32 * 64), kThreads, 0, s>>>(act, nt, 32, 2, kActA5, kActU5);
-  FL_TRY(conv_fwd(s, act, nt, 6, params, mfma));
-  FL_TRY(conv_fwd(s, act, nt, 7, params, mfma));
+  FL_TRY(conv_fwd(s, act, nt, 6, params));
+  FL_TRY(conv_fwd(s, act, nt, 7, params));
   k_up_fwd<<<grid_for(nt * 16 * 512), kThreads, 0, s>>>(act, nt, 16, 4, kActA7, kActU7);
-  FL_TRY(conv_fwd(s, act, nt, 8, params, mfma));
+  FL_TRY(conv_fwd(s, act, nt, 8, params));
   FL_TRY(conv_fwd(s, act, nt, 9, params));
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+uint64_t net_fragment_bytes() { return uint64_t(frag_offset(kNetLayers)) * sizeof(bf16x4); }
+
+int net_infer(hipStream_t s, const float* params, void* frags, float* act, uint64_t nt) {
+  if (!nt) return FL_OK;
+  if (nt > 0x7fffffffu) return FL_RESOURCE_EXHAUSTED;
+  FragSpec spec{};
+  for (int l = 0; l < kNetLayers; ++l) {
+    spec.cin[l] = kLayers[l].ca + kLayers[l].cb;
+    spec.cout[l] = kLayers[l].cout;
+    spec.steps[l] = frag_steps(l);
+    spec.first[l] = frag_offset(l);
+    spec.param[l] = param_offset(l);
+  }
+  spec.first[kNetLayers] = frag_offset(kNetLayers);
+  bf16x4* f = static_cast<bf16x4*>(frags);
+  k_pack_fragments<<<(spec.first[kNetLayers] + 255) / 256, 256, 0, s>>>(params, spec, f);
+  k_unet_infer<<<static_cast<unsigned>(nt), 256, 0, s>>>(act, f, params);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -634,6 +805,7 @@ void destroy_learn(fl_ctx* ctx) {
   lfree(L->stage);
   lfree(L->data_in);
   lfree(L->data_target);
+  lfree(L->frags);
   delete L;
   ctx->learn = nullptr;
 }
@@ -732,6 +904,7 @@ int fl_learn_setup(fl_ctx* ctx, uint64_t tile_capacity, uint64_t pair_capacity) 
   A(&L->stage, std::min<uint64_t>(tile_capacity, 1024) * 2048);
   A(&L->data_in, pair_capacity * 2048);
   A(&L->data_target, pair_capacity * 512);
+  A(&L->frags, fl::net_fragment_bytes());
   if (st != FL_OK) {
     fl::destroy_learn(ctx);
     return FL_RESOURCE_EXHAUSTED;
@@ -771,7 +944,11 @@ static int learn_forward(fl_ctx* ctx, const float* inputs, size_t n, float* outp
   if (!n) return FL_OK;
   FL_HIP(hipMemcpyAsync(L->stage, inputs, n * 2048 * sizeof(float), hipMemcpyHostToDevice, s));
   fl::k_put_inputs<<<fl::grid_for(n * 2048), fl::kThreads, 0, s>>>(L->stage, n, L->act);
-  FL_TRY(fl::net_forward(s, L->params, L->act, n, mfma));
+  if (mfma) {
+    FL_TRY(fl::net_infer(s, L->params, L->frags, L->act, n));
+  } else {
+    FL_TRY(fl::net_forward(s, L->params, L->act, n));
+  }
   fl::k_get_slot<<<fl::grid_for(n * 512), fl::kThreads, 0, s>>>(L->act, n, fl::kActOut, 512, L->stage);
   FL_HIP(hipMemcpyAsync(outputs, L->stage, n * 512 * sizeof(float), hipMemcpyDeviceToHost, s));
   FL_HIP(hipStreamSynchronize(s));

@@ -1606,7 +1606,7 @@ int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates&
     if (use_net) {
       k_tile_gather<<<static_cast<unsigned>(P.ntiles), 512, 0, s>>>(P.r32, L0.diag, P.cell_class, P.tile_blocks,
                                                                     net->act, net->scale);
-      FL_TRY(net_forward(s, net->params, net->act, P.ntiles, true));
+      FL_TRY(net_infer(s, net->params, net->frags, net->act, P.ntiles));
       k_tile_scatter<<<grid_for(P.ntiles * 512), kThreads, 0, s>>>(net->act, net->scale, P.cell_class, P.tile_blocks,
                                                                     P.ntiles, L0.x);
       ++diag.learned_iterations;
