@@ -202,6 +202,51 @@ __global__ void k_pack_boundary_gather(const double* cols, uint64_t cap, const d
   for (int c = 0; c < 4; ++c) r.pad[c] = 0;
 }
 
+// Device copies of a previous state's topology records, the pressure cells and the prepared records;
+// allocated on the first topology preparation and grown with it.
+struct TopologyBuffers {
+  fl_block_state *prev_blocks = nullptr, *blocks = nullptr;
+  fl_cell_state *prev_cells = nullptr, *cells = nullptr;
+  fl_face_state *prev_faces = nullptr, *faces = nullptr;
+  int32_t *cells_in = nullptr, *cells_unique = nullptr, *box = nullptr;
+  int64_t* old = nullptr;
+  uint64_t* words = nullptr;
+  uint64_t* count = nullptr;
+  uint64_t cap_prev_blocks = 0, cap_blocks = 0, cap_prev_cells = 0, cap_cells = 0, cap_prev_faces = 0, cap_faces = 0,
+           cap_cells_in = 0, cap_cells_unique = 0, cap_old = 0, cap_words = 0;
+  void release() {
+    void* ptrs[] = {prev_blocks, blocks, prev_cells, cells, prev_faces, faces, cells_in, cells_unique, box, old, words,
+                    count};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    *this = TopologyBuffers{};
+  }
+};
+
+void destroy_topology(fl_ctx* ctx) {
+  auto* b = static_cast<TopologyBuffers*>(ctx->topology);
+  if (!b) return;
+  b->release();
+  delete b;
+  ctx->topology = nullptr;
+}
+
+// (Re)allocates *p for at least need elements (capacity in *cap).
+template <typename T>
+int grow(T** p, uint64_t* cap, uint64_t need) {
+  if (need <= *cap && *p) return FL_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  const uint64_t n = std::max<uint64_t>(need, 1);
+  if (hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    return FL_RESOURCE_EXHAUSTED;
+  }
+  *cap = n;
+  return FL_OK;
+}
+
 int first_error_code(fl_ctx* ctx) {
   uint64_t w;
   FL_TRY(read_error(ctx, &w));
@@ -325,6 +370,7 @@ void fl_destroy(fl_ctx* ctx) {
   fl::destroy_pressure(ctx);
   fl::destroy_learn(ctx);
   destroy_boundary(ctx);
+  destroy_topology(ctx);
   release(ctx);
   for (hipEvent_t e : ctx->stage)
     if (e) (void)hipEventDestroy(e);
@@ -451,6 +497,132 @@ int fl_activate(fl_ctx* ctx, const fl_geometry* geometry, const fl_vec3* positio
   copy_ms += c.stop();
   *io_count = ctx->nb;
   fill_measurement(ctx, measurement, kernel_ms, copy_ms, 12);
+  return FL_OK;
+}
+
+int fl_prepare_topology(fl_ctx* ctx, const fl_geometry* geometry, const fl_topology_capacities* capacities,
+                        const fl_vec3* positions, size_t n, const fl_int3* pressure_cells, size_t cell_count,
+                        const fl_topology_history* previous, fl_sort_entry* out_order, fl_block_state* out_blocks,
+                        uint64_t* out_block_count, fl_cell_state* out_cells, uint64_t* out_cell_count,
+                        fl_face_state* out_faces, uint64_t* out_face_count, fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  FL_TRY(check_geometry(geometry));
+  if (!capacities || !out_block_count || !out_cell_count || !out_face_count) return FL_INVALID_ARGUMENT;
+  if ((n && (!positions || !out_order)) || (cell_count && !pressure_cells)) return FL_INVALID_ARGUMENT;
+  if (!out_blocks || !out_cells || !out_faces) return FL_INVALID_ARGUMENT;
+  if (previous && ((previous->block_count && !previous->blocks) || (previous->cell_count && !previous->cells) ||
+                   (previous->face_count && !previous->faces)))
+    return FL_INVALID_ARGUMENT;
+  // The workspace's capacities (liquid_step_workspace_3d.cc:182-184), then the context's.
+  if (n > capacities->particles || cell_count > capacities->cells) return FL_RESOURCE_EXHAUSTED;
+  if (n > ctx->cap.particles || cell_count > ctx->cap.pressure_cells) return FL_RESOURCE_EXHAUSTED;
+  if (previous && (previous->block_count > ctx->cap.blocks || previous->cell_count > ctx->cap.pressure_cells ||
+                   previous->face_count > 192 * ctx->cap.blocks))
+    return FL_RESOURCE_EXHAUSTED;
+  (void)hipSetDevice(ctx->device);
+  if (!ctx->topology) ctx->topology = new TopologyBuffers();
+  TopologyBuffers& B = *static_cast<TopologyBuffers*>(ctx->topology);
+  hipStream_t s = ctx->stream;
+  set_geometry(ctx, geometry);
+  double copy_ms = 0, kernel_ms = 0;
+  uint64_t generation = 1;
+  const uint64_t pnb = previous ? previous->block_count : 0, pnc = previous ? previous->cell_count : 0,
+                 pnf = previous ? previous->face_count : 0;
+  FL_TRY(grow(&B.prev_blocks, &B.cap_prev_blocks, pnb));
+  FL_TRY(grow(&B.prev_cells, &B.cap_prev_cells, pnc));
+  FL_TRY(grow(&B.prev_faces, &B.cap_prev_faces, pnf));
+  FL_TRY(grow(&B.cells_in, &B.cap_cells_in, 3 * cell_count));
+  FL_TRY(grow(&B.cells_unique, &B.cap_cells_unique, 3 * cell_count));
+  if (!B.box) FL_HIP(hipMalloc(reinterpret_cast<void**>(&B.box), 8 * sizeof(int32_t)));
+  if (!B.count) FL_HIP(hipMalloc(reinterpret_cast<void**>(&B.count), sizeof(uint64_t)));
+  FL_TRY(grow(&B.words, &B.cap_words, ctx->cap.lookup_blocks));
+  {
+    Timer c(ctx);
+    if (n) FL_HIP(hipMemcpyAsync(ctx->staging_aos, positions, n * sizeof(fl_vec3), hipMemcpyHostToDevice, s));
+    if (cell_count)
+      FL_HIP(hipMemcpyAsync(B.cells_in, pressure_cells, cell_count * sizeof(fl_int3), hipMemcpyHostToDevice, s));
+    if (pnb) FL_HIP(hipMemcpyAsync(B.prev_blocks, previous->blocks, pnb * sizeof(fl_block_state), hipMemcpyHostToDevice, s));
+    if (pnc) FL_HIP(hipMemcpyAsync(B.prev_cells, previous->cells, pnc * sizeof(fl_cell_state), hipMemcpyHostToDevice, s));
+    if (pnf) FL_HIP(hipMemcpyAsync(B.prev_faces, previous->faces, pnf * sizeof(fl_face_state), hipMemcpyHostToDevice, s));
+    copy_ms += c.stop();
+  }
+  Timer t(ctx);
+  if (previous) {
+    const fl_geometry& pg = previous->geometry;
+    if (!(pg.world_origin.x == geometry->world_origin.x && pg.world_origin.y == geometry->world_origin.y &&
+          pg.world_origin.z == geometry->world_origin.z && pg.cell_width == geometry->cell_width))
+      return FL_INVALID_ARGUMENT;
+    if (pnb > UINT64_MAX / 192) return FL_OUT_OF_RANGE;
+    FL_TRY(reset_error(ctx));
+    FL_TRY(check_history(ctx, B.prev_blocks, pnb, B.prev_cells, pnc, B.prev_faces, pnf));
+    FL_TRY(first_error_code(ctx));
+    if (previous->epoch == UINT64_MAX) return FL_OUT_OF_RANGE;
+    generation = previous->epoch + 1;
+  }
+  if (n) {
+    k_split_columns<<<grid_for(n), kThreads, 0, s>>>(reinterpret_cast<const double*>(ctx->staging_aos), n, 3, 3,
+                                                    ctx->raw.base, ctx->raw.capacity);
+    FL_HIP(hipGetLastError());
+  }
+  ctx->raw.trial_aliases_sorted = true;
+  FL_TRY(reset_error(ctx));
+  FL_TRY(check_topology_inputs(ctx, n, B.cells_in, cell_count));
+  FL_TRY(first_error_code(ctx));
+  FL_TRY(bin_particles(ctx, ctx->raw.view(), n, false, false));
+  FL_TRY(first_error_code(ctx));
+  if (ctx->bin_box.volume() > ctx->cap.lookup_blocks) return FL_RESOURCE_EXHAUSTED;
+  FL_TRY(sort_particles(ctx, n, ctx->raw.view(), true, 0x7u));
+  int32_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  bool any = false;
+  if (n && ctx->bin_box.volume()) {
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = ctx->bin_box.lo[a];
+      hi[a] = ctx->bin_box.lo[a] + ctx->bin_box.dim[a];  // bins B..B+1 own the stencils
+    }
+    any = true;
+  }
+  if (cell_count) {
+    int32_t clo[3], chi[3];
+    FL_TRY(cell_owner_box(ctx, B.cells_in, cell_count, B.box, clo, chi));
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = any ? std::min(lo[a], clo[a]) : clo[a];
+      hi[a] = any ? std::max(hi[a], chi[a]) : chi[a];
+    }
+    any = true;
+  }
+  Box box;
+  if (any)
+    for (int a = 0; a < 3; ++a) {
+      box.lo[a] = lo[a];
+      box.dim[a] = hi[a] - lo[a] + 1;
+    }
+  if (box.volume() > ctx->cap.lookup_blocks) return FL_RESOURCE_EXHAUSTED;
+  ctx->block_box = box;
+  FL_TRY(activate_blocks(ctx, ctx->nvalid, B.cells_in, cell_count));
+  FL_TRY(first_error_code(ctx));
+  const uint64_t nb = ctx->nb;
+  if (nb > capacities->blocks || nb > capacities->faces / 192) return FL_RESOURCE_EXHAUSTED;
+  uint64_t nc = 0;
+  FL_TRY(unique_cells(ctx, B.cells_in, cell_count, B.words, B.cells_unique, B.count, &nc));
+  FL_TRY(grow(&B.blocks, &B.cap_blocks, nb));
+  FL_TRY(grow(&B.old, &B.cap_old, nb));
+  FL_TRY(grow(&B.cells, &B.cap_cells, nc));
+  FL_TRY(grow(&B.faces, &B.cap_faces, 192 * nb));
+  FL_TRY(remap_topology(ctx, B.prev_blocks, pnb, B.prev_cells, pnc, B.prev_faces, pnf, generation, B.cells_unique, nc,
+                        B.old, B.blocks, B.cells, B.faces));
+  auto* d_order = reinterpret_cast<fl_sort_entry*>(ctx->staging_aos);
+  FL_TRY(write_sort_entries(ctx, n, d_order));
+  kernel_ms = t.stop();
+  Timer c(ctx);
+  if (n) FL_HIP(hipMemcpyAsync(out_order, d_order, n * sizeof(fl_sort_entry), hipMemcpyDeviceToHost, s));
+  if (nb) FL_HIP(hipMemcpyAsync(out_blocks, B.blocks, nb * sizeof(fl_block_state), hipMemcpyDeviceToHost, s));
+  if (nc) FL_HIP(hipMemcpyAsync(out_cells, B.cells, nc * sizeof(fl_cell_state), hipMemcpyDeviceToHost, s));
+  if (nb) FL_HIP(hipMemcpyAsync(out_faces, B.faces, 192 * nb * sizeof(fl_face_state), hipMemcpyDeviceToHost, s));
+  copy_ms += c.stop();
+  *out_block_count = nb;
+  *out_cell_count = nc;
+  *out_face_count = 192 * nb;
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, 16);
   return FL_OK;
 }
 

@@ -156,6 +156,8 @@ struct fl_ctx {
   void* learn = nullptr;
   // boundary-gather history buffers (owned by capi.hip, allocated on first use)
   void* boundary = nullptr;
+  // topology-preparation buffers (owned by capi.hip, allocated on first use)
+  void* topology = nullptr;
 };
 
 namespace fl {
@@ -221,6 +223,22 @@ int g2p_boundary(fl_ctx* ctx, uint64_t n, const BoundaryDevice& d, uint32_t* slo
 // Compatible stencils of caller particles [lo, lo + count) from ctx->raw and ctx->bin (81 per particle).
 int compatible_stencils(fl_ctx* ctx, uint64_t lo, uint64_t count, const int64_t* slot_side, const uint8_t* slot_ok,
                         const int64_t* side, const uint8_t* affine, fl_compatible_sample* d_out);
+
+// Topology preparation (history.hip). Checks write the first failing (phase, ordinal, code) into
+// ctx->err: previous blocks, cells and faces (phases 0-2), then particles and pressure cells (3-4).
+int check_history(fl_ctx* ctx, const fl_block_state* b, uint64_t nb, const fl_cell_state* c, uint64_t nc,
+                  const fl_face_state* f, uint64_t nf);
+int check_topology_inputs(fl_ctx* ctx, uint64_t n, const int32_t* cells, uint64_t nc);
+// Owner-block box of the pressure cells' faces (d_red: 6 device ints of scratch).
+int cell_owner_box(fl_ctx* ctx, const int32_t* cells, uint64_t nc, int32_t* d_red, int32_t lo[3], int32_t hi[3]);
+// Canonical unique cells of the (checked) pressure cells into out; words holds cap.lookup_blocks
+// 64-bit words and d_count one device word.
+int unique_cells(fl_ctx* ctx, const int32_t* cells, uint64_t nc, void* words, int32_t* out, uint64_t* d_count,
+                 uint64_t* count);
+// Block, cell and face records of ctx->blocks / cells with the previous state's history.
+int remap_topology(fl_ctx* ctx, const fl_block_state* pb, uint64_t pnb, const fl_cell_state* pc, uint64_t pnc,
+                   const fl_face_state* pf, uint64_t pnf, uint64_t generation, const int32_t* cells, uint64_t nc,
+                   int64_t* old, fl_block_state* ob, fl_cell_state* oc, fl_face_state* of);
 
 // Small helpers (topology.hip)
 int read_error(fl_ctx* ctx, uint64_t* word);

@@ -572,6 +572,10 @@ int activate_blocks(fl_ctx* ctx, uint64_t n, const int32_t* d_cells, uint64_t ce
   Box box = ctx->block_box;  // set by the caller
   const uint64_t vol = box.volume();
   if (vol > ctx->cap.lookup_blocks) return FL_RESOURCE_EXHAUSTED;
+  if (vol == 0) {  // nothing to activate
+    ctx->nb = 0;
+    return FL_OK;
+  }
   k_fill_u8<<<capped_grid(vol), kThreads, 0, ctx->stream>>>(ctx->mark, vol, 0);
   FL_HIP(hipGetLastError());
   if (n) {

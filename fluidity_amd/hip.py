@@ -49,7 +49,7 @@ class StepReport(C.Structure):
         return np.frombuffer(bytes(self.pressure), dtype=R.PRESSURE_DIAGNOSTICS)[0]
 
 
-EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_p2g", "fl_p2g_quanta", "fl_p2g_compatible", "fl_g2p_boundary",
+EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_prepare_topology", "fl_p2g", "fl_p2g_quanta", "fl_p2g_compatible", "fl_g2p_boundary",
            "fl_normalize_forces", "fl_apply_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
            "fl_scene_download", "fl_probe_copy", "fl_learn_param_count", "fl_learn_setup", "fl_learn_set_params", "fl_learn_get_params",
            "fl_learn_forward", "fl_learn_infer", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect", "fl_learn_eval",
@@ -75,6 +75,28 @@ def _p(a):
 
 def _vec3(v):
     return Vec3(*map(float, v))
+
+
+class TopologyHistory(C.Structure):
+    """fl_topology_history; keep the record arrays alive while it is in use (held in _keep)."""
+    _fields_ = [("origin", C.c_double * 3), ("cell_width", C.c_double), ("epoch", C.c_uint64),
+                ("blocks", C.c_void_p), ("block_count", C.c_uint64), ("cells", C.c_void_p), ("cell_count", C.c_uint64),
+                ("faces", C.c_void_p), ("face_count", C.c_uint64)]
+
+
+def topology_history(geometry, epoch, blocks, cells, faces):
+    g = geometry[0]
+    o = g["world_origin"]
+    h = TopologyHistory((C.c_double * 3)(o["x"], o["y"], o["z"]), g["cell_width"], epoch, _p(blocks), len(blocks),
+                        _p(cells), len(cells), _p(faces), len(faces))
+    h._keep = (blocks, cells, faces)
+    return h
+
+
+def topology_capacities(particles=1 << 20, blocks=1 << 16, cells=1 << 20, faces=192 << 16):
+    c = np.zeros(1, R.TOPOLOGY_CAPACITIES)
+    c[0] = (particles, blocks, cells, faces)
+    return c
 
 
 class VelocityHistory(C.Structure):
@@ -139,6 +161,26 @@ class Context:
         s = lib().fl_activate(self.handle, _p(geometry), _p(pos), C.c_size_t(len(pos)), _p(cells),
                               C.c_size_t(len(cells)), _p(out), C.byref(cnt), C.byref(self.last))
         return s, out[: cnt.value].copy()
+
+    def prepare_topology(self, geometry, positions, cells=None, previous=None, capacities=None):
+        """fl_prepare_topology: (status, order, blocks, cells, faces); previous is a TopologyHistory or None."""
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        cells = np.zeros(0, R.INT3) if cells is None else np.ascontiguousarray(cells)
+        caps = topology_capacities() if capacities is None else capacities
+        c = caps[0]
+        nb_cap = min(int(c["blocks"]), 27 * 3 * len(pos) + 4 * len(cells) + 1)
+        order = np.zeros(len(pos), R.SORT_ENTRY)
+        ob = np.zeros(nb_cap, R.BLOCK_STATE)
+        oc = np.zeros(min(int(c["cells"]), len(cells) + 1), R.CELL_STATE)
+        of = np.zeros(min(int(c["faces"]), 192 * nb_cap), R.FACE_STATE)
+        nb, nc, nf = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        s = lib().fl_prepare_topology(self.handle, _p(geometry), _p(caps), _p(pos), C.c_size_t(len(pos)), _p(cells),
+                                      C.c_size_t(len(cells)), None if previous is None else C.byref(previous),
+                                      _p(order), _p(ob), C.byref(nb), _p(oc), C.byref(nc), _p(of), C.byref(nf),
+                                      C.byref(self.last))
+        if s != OK:
+            return s, None, None, None, None
+        return s, order, ob[:nb.value].copy(), oc[:nc.value].copy(), of[:nf.value].copy()
 
     def p2g(self, geometry, blocks, particles, out=None):
         nf = 192 * len(blocks)

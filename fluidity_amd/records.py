@@ -53,8 +53,20 @@ PRESSURE_DIAGNOSTICS = np.dtype([("iteration_count", "u8"), ("multigrid_level_co
                                  ("initial_defect_l2", "f8"), ("defect_l2", "f8"), ("defect_linf", "f8"),
                                  ("restarts", "u8"), ("gauge_count", "u8"), ("learned_iterations", "u8")])
 
+# LiquidBlockState3D / LiquidCellState3D / LiquidFaceState3D (native/include/fluidity/host/liquid_state_3d.h:129-159).
+BLOCK_STATE = np.dtype([("block_index", INT3), ("pad", "u4"), ("generation", "u8"), ("activation_age", "u4"),
+                        ("inactive_age", "u4")])
+CELL_STATE = np.dtype({"names": ["cell_index", "occupancy", "previous_pressure"], "formats": [INT3, "u1", "f8"],
+                       "offsets": [0, 12, 16], "itemsize": 24})
+FACE_STATE = np.dtype({"names": ["axis", "lattice", "face_slot", "carrier_velocity", "validity", "side_tag",
+                                 "structurally_eligible"],
+                       "formats": ["u1", INT3, "u8", "f8", "u1", "i8", "u1"],
+                       "offsets": [0, 4, 16, 24, 32, 40, 48], "itemsize": 56})
+TOPOLOGY_CAPACITIES = np.dtype([("particles", "u8"), ("blocks", "u8"), ("cells", "u8"), ("faces", "u8")])
+
 SEALED, INTERNAL, DIRICHLET = 0, 1, 2
 LINK_KINDS = {"sealed": SEALED, "internal": INTERNAL, "dirichlet": DIRICHLET}
 
 assert PARTICLE.itemsize == 152 and GATHER_RESULT.itemsize == 192 and SORT_ENTRY.itemsize == 24
 assert PRESSURE_CELL.itemsize == 56 and PRESSURE_FACE.itemsize == 80 and FACE_QUANTA.itemsize == 24
+assert BLOCK_STATE.itemsize == 32 and CELL_STATE.itemsize == 24 and FACE_STATE.itemsize == 56

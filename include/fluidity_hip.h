@@ -133,6 +133,49 @@ typedef struct fl_pressure_cell {
   double previous_pressure;
   double integrated_inward_wall_flux;
 } fl_pressure_cell;
+/* Topology records of a liquid state (native/include/fluidity/host/liquid_state_3d.h:129-159):
+ * LiquidBlockState3D, LiquidCellState3D (occupancy 0 air, 1 liquid, 2 solid) and LiquidFaceState3D
+ * (axis 0..2 with the lattice index at offset 4; validity 0 missing, 1 valid, 2 predicted, 3 low mass). */
+typedef struct fl_block_state {
+  fl_int3 block_index;
+  uint32_t pad;
+  uint64_t generation;
+  uint32_t activation_age;
+  uint32_t inactive_age;
+} fl_block_state; /* 32 B */
+typedef struct fl_cell_state {
+  fl_int3 cell_index;
+  uint8_t occupancy;
+  uint8_t pad[3];
+  double previous_pressure;
+} fl_cell_state; /* 24 B */
+typedef struct fl_face_state {
+  uint8_t axis;
+  uint8_t pad0[3];
+  fl_int3 lattice;
+  uint64_t face_slot;
+  double carrier_velocity;
+  uint8_t validity;
+  uint8_t pad1[7];
+  int64_t side_tag;
+  uint8_t structurally_eligible;
+  uint8_t pad2[7];
+} fl_face_state; /* 56 B */
+/* The topology of a previous LiquidState3D: its geometry, clock epoch and block, cell and face records. */
+typedef struct fl_topology_history {
+  fl_geometry geometry;
+  uint64_t epoch;
+  const fl_block_state* blocks;
+  uint64_t block_count;
+  const fl_cell_state* cells;
+  uint64_t cell_count;
+  const fl_face_state* faces;
+  uint64_t face_count;
+} fl_topology_history;
+/* LiquidStateCapacities3D without sources. */
+typedef struct fl_topology_capacities {
+  uint64_t particles, blocks, cells, faces;
+} fl_topology_capacities;
 typedef struct fl_pressure_face {
   int32_t axis;
   fl_int3 lattice;
@@ -217,6 +260,22 @@ int fl_bin_sort(fl_ctx* ctx, const fl_geometry* geometry, const fl_vec3* positio
 int fl_activate(fl_ctx* ctx, const fl_geometry* geometry, const fl_vec3* positions, size_t n,
                 const fl_int3* pressure_cells, size_t cell_count, fl_int3* out_blocks, size_t* io_count,
                 fl_measurement* measurement);
+
+/* LiquidStepWorkspace3D::PrepareTopology without an imported order (native/src/liquid_step_workspace_3d.cc:171-341):
+ * the sorted particle order (n entries), the active blocks (a new block takes generation
+ * previous->epoch + 1, or 1; a block of the previous state keeps its record with inactive_age 0), the
+ * canonical unique pressure cells (a previous cell keeps its record, a new one is liquid at zero
+ * pressure) and one face record per owned slot (the previous record of the same face with the new
+ * slot, or a missing face with side tag 0 that is structurally eligible). capacities are the
+ * workspace's; out_blocks, out_cells and out_faces hold capacities->blocks, ->cells and ->faces
+ * records. previous may be null; its records get the checks of ValidateLiquidState3D for blocks,
+ * cells and faces (native/src/liquid_state_3d.cc:559-604), the rest of that state being the caller's
+ * to validate. Outputs are written only on success. */
+int fl_prepare_topology(fl_ctx* ctx, const fl_geometry* geometry, const fl_topology_capacities* capacities,
+                        const fl_vec3* positions, size_t n, const fl_int3* pressure_cells, size_t cell_count,
+                        const fl_topology_history* previous, fl_sort_entry* out_order, fl_block_state* out_blocks,
+                        uint64_t* out_block_count, fl_cell_state* out_cells, uint64_t* out_cell_count,
+                        fl_face_state* out_faces, uint64_t* out_face_count, fl_measurement* measurement);
 
 /* ---- transfers ---- */
 /* DepositMac3D (native/include/fluidity/host/transfers_3d.h:24-28; native/src/transfers_3d.cc:139-145):

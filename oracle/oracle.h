@@ -169,6 +169,52 @@ int flo_apply_forces(const flo_int3* sorted_blocks, size_t block_count, const do
                      size_t face_count, flo_vec3 acceleration, double dt, double* out_predicted);
 
 // ---- topology (liquid_step_workspace_3d.cc:171-300) ----
+// LiquidBlockState3D, LiquidCellState3D and LiquidFaceState3D (native/include/fluidity/host/liquid_state_3d.h:129-159).
+struct flo_block_state {
+  flo_int3 block_index;
+  uint32_t pad;
+  uint64_t generation;
+  uint32_t activation_age, inactive_age;
+};  // 32 B
+struct flo_cell_state {
+  flo_int3 cell_index;
+  uint8_t occupancy;  // 0 air, 1 liquid, 2 solid
+  uint8_t pad[3];
+  double previous_pressure;
+};  // 24 B
+struct flo_face_state {
+  flo_face_index face_index;
+  uint64_t face_slot;
+  double carrier_velocity;
+  uint8_t validity;  // 0 missing, 1 valid, 2 predicted, 3 low mass
+  uint8_t pad[7];
+  int64_t side_tag;
+  uint8_t structurally_eligible;
+  uint8_t pad2[7];
+};  // 56 B
+// The topology records of a previous LiquidState3D with its geometry and clock epoch.
+struct flo_topology_history {
+  flo_geometry geometry;
+  uint64_t epoch;
+  const flo_block_state* blocks;
+  uint64_t block_count;
+  const flo_cell_state* cells;
+  uint64_t cell_count;
+  const flo_face_state* faces;
+  uint64_t face_count;
+};
+// LiquidStateCapacities3D without sources.
+struct flo_topology_capacities { uint64_t particles, blocks, cells, faces; };
+// LiquidStepWorkspace3D::PrepareTopology without an imported order (liquid_step_workspace_3d.cc:171-341):
+// sorted particle order, active blocks (generation from the previous epoch, or the previous block
+// with its inactive age cleared), canonical unique cells and one face record per owned slot, each
+// copied from the previous state when the same identity existed there. previous may be null. Outputs
+// are written only on success; out arrays hold n, capacities->blocks, ->cells and ->faces records.
+int flo_prepare_topology(const flo_geometry* geometry, const flo_topology_capacities* capacities,
+                         const flo_vec3* positions, size_t n, const flo_int3* pressure_cells, size_t cell_count,
+                         const flo_topology_history* previous, flo_sort_entry* out_order, flo_block_state* out_blocks,
+                         uint64_t* out_block_count, flo_cell_state* out_cells, uint64_t* out_cell_count,
+                         flo_face_state* out_faces, uint64_t* out_face_count);
 int flo_sort_particles(const flo_geometry* geometry, const flo_vec3* positions, size_t n, flo_sort_entry* out);
 // Writes the sorted unique active blocks; *io_count holds the capacity on entry, the count on exit.
 int flo_activate_blocks(const flo_geometry* geometry, const flo_vec3* positions, size_t n,

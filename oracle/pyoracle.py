@@ -64,6 +64,17 @@ PRESSURE_DIAGNOSTICS = np.dtype([("iteration_count", "u8"), ("multigrid_level_co
                                  ("initial_defect_l2", "f8"), ("defect_l2", "f8"), ("defect_linf", "f8"),
                                  ("restarts", "u8"), ("gauge_count", "u8")])
 
+# LiquidBlockState3D, LiquidCellState3D, LiquidFaceState3D (native/include/fluidity/host/liquid_state_3d.h:129-159).
+BLOCK_STATE = np.dtype([("block_index", INT3), ("pad", "u4"), ("generation", "u8"), ("activation_age", "u4"),
+                        ("inactive_age", "u4")])
+CELL_STATE = np.dtype({"names": ["cell_index", "occupancy", "previous_pressure"], "formats": [INT3, "u1", "f8"],
+                       "offsets": [0, 12, 16], "itemsize": 24})
+FACE_STATE = np.dtype({"names": ["face_index", "face_slot", "carrier_velocity", "validity", "side_tag",
+                                 "structurally_eligible"],
+                       "formats": [FACE_INDEX, "u8", "f8", "u1", "i8", "u1"],
+                       "offsets": [0, 16, 24, 32, 40, 48], "itemsize": 56})
+TOPOLOGY_CAPACITIES = np.dtype([("particles", "u8"), ("blocks", "u8"), ("cells", "u8"), ("faces", "u8")])
+assert BLOCK_STATE.itemsize == 32 and CELL_STATE.itemsize == 24 and FACE_STATE.itemsize == 56
 assert PARTICLE.itemsize == 152 and GATHER_RESULT.itemsize == 192 and SORT_ENTRY.itemsize == 24
 assert PRESSURE_CELL.itemsize == 56 and PRESSURE_FACE.itemsize == 80 and FACE_QUANTA.itemsize == 24
 
@@ -272,6 +283,46 @@ def activate_blocks(geom, positions, cells=None, capacity=1 << 24):
     s = lib().flo_activate_blocks(_p(geom), _p(pos), C.c_size_t(len(pos)), _p(cells), C.c_size_t(len(cells)), _p(out),
                                   C.byref(cnt))
     return s, out[:cnt.value].copy()
+
+
+class TopologyHistoryC(C.Structure):
+    _fields_ = [("origin", C.c_double * 3), ("cell_width", C.c_double), ("epoch", C.c_uint64),
+                ("blocks", C.c_void_p), ("block_count", C.c_uint64), ("cells", C.c_void_p), ("cell_count", C.c_uint64),
+                ("faces", C.c_void_p), ("face_count", C.c_uint64)]
+
+
+def topology_history(geom, epoch, blocks, cells, faces):
+    """The previous state's topology records; the arrays must outlive the returned structure."""
+    g = geom[0]
+    o = g["world_origin"]
+    return TopologyHistoryC((C.c_double * 3)(o["x"], o["y"], o["z"]), g["cell_width"], epoch, _p(blocks), len(blocks),
+                            _p(cells), len(cells), _p(faces), len(faces))
+
+
+def topology_capacities(particles=1 << 20, blocks=1 << 16, cells=1 << 20, faces=192 << 16):
+    c = np.zeros(1, TOPOLOGY_CAPACITIES)
+    c[0] = (particles, blocks, cells, faces)
+    return c
+
+
+def prepare_topology(geom, positions, cells=None, previous=None, capacities=None):
+    """flo_prepare_topology: (status, order, blocks, cells, faces); previous is a TopologyHistoryC or None."""
+    pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    cells = np.zeros(0, INT3) if cells is None else np.ascontiguousarray(cells)
+    caps = topology_capacities() if capacities is None else capacities
+    c = caps[0]
+    nb_cap = min(int(c["blocks"]), 27 * 3 * len(pos) + 4 * len(cells) + 1)
+    order = np.zeros(len(pos), SORT_ENTRY)
+    ob = np.zeros(nb_cap, BLOCK_STATE)
+    oc = np.zeros(min(int(c["cells"]), len(cells) + 1), CELL_STATE)
+    of = np.zeros(192 * nb_cap, FACE_STATE)
+    nb, nc, nf = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    s = lib().flo_prepare_topology(_p(geom), _p(caps), _p(pos), C.c_size_t(len(pos)), _p(cells), C.c_size_t(len(cells)),
+                                   None if previous is None else C.byref(previous), _p(order), _p(ob), C.byref(nb),
+                                   _p(oc), C.byref(nc), _p(of), C.byref(nf))
+    if s != OK:
+        return s, None, None, None, None
+    return s, order, ob[:nb.value].copy(), oc[:nc.value].copy(), of[:nf.value].copy()
 
 
 def pressure_options(max_iterations=256, tolerance_scale=1.0, relative_target=0.0, levels=4, sweeps=2,

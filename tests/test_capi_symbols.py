@@ -31,6 +31,41 @@ def test_record_layouts_match_the_header():
     assert R.PRESSURE_OPTIONS.itemsize == 56 and R.PRESSURE_DIAGNOSTICS.itemsize == 96
     assert R.BOUNDARY_PARTICLE.itemsize == 168 and R.FACE_TAG.itemsize == 16 and R.COMPATIBLE_SAMPLE.itemsize == 72
     assert R.WALL_SAMPLE.itemsize == 64 and R.BOUNDARY_GATHER.itemsize == 224
+    assert R.BLOCK_STATE.itemsize == 32 and R.CELL_STATE.itemsize == 24 and R.FACE_STATE.itemsize == 56
+
+
+def test_record_offsets_match_the_c_compiler(tmp_path):
+    """sizeof/offsetof of the topology and boundary records as gcc lays out the header."""
+    import subprocess
+
+    from fluidity_amd import records as R
+
+    checks = {
+        "fl_block_state": (R.BLOCK_STATE, {"generation": "generation", "activation_age": "activation_age",
+                                           "inactive_age": "inactive_age"}),
+        "fl_cell_state": (R.CELL_STATE, {"occupancy": "occupancy", "previous_pressure": "previous_pressure"}),
+        "fl_face_state": (R.FACE_STATE, {"axis": "axis", "lattice": "lattice", "face_slot": "face_slot",
+                                         "carrier_velocity": "carrier_velocity", "validity": "validity",
+                                         "side_tag": "side_tag", "structurally_eligible": "structurally_eligible"}),
+        "fl_boundary_gather": (R.BOUNDARY_GATHER, {"affine_enabled": "affine_enabled",
+                                                   "low_mass_fallback": "low_mass_fallback"}),
+    }
+    lines = ['#include <stddef.h>', '#include <stdio.h>', f'#include "{ROOT}/include/fluidity_hip.h"', "int main(void) {"]
+    for struct, (_, fields) in checks.items():
+        lines.append(f'  printf("{struct} %zu\\n", sizeof({struct}));')
+        for cfield in fields.values():
+            lines.append(f'  printf("{struct}.{cfield} %zu\\n", offsetof({struct}, {cfield}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                           check=True).stdout.splitlines())
+    for struct, (dtype, fields) in checks.items():
+        assert int(got[struct]) == dtype.itemsize, struct
+        for name, cfield in fields.items():
+            assert int(got[f"{struct}.{cfield}"]) == dtype.fields[name][1], (struct, name)
 
 
 def test_create_without_device_fails_cleanly():
