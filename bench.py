@@ -181,6 +181,25 @@ def learned_comparison(ctx, scene, dt, opts, args):
     }
 
 
+def topology_measurement(ctx, scene, cells, dt):
+    """fl_prepare_topology at the workload's size: particles advected by one step, with the topology of
+    the unmoved particles as the previous state (blocks, cells and faces remapped by identity)."""
+    geom = scene.geometry_record()
+    cl = np.ascontiguousarray(cells["cell_index"])
+    caps = H.topology_capacities(particles=scene.count, blocks=1 << 17, cells=len(cells) + 64, faces=192 << 17)
+    s, _, b0, c0, f0 = ctx.prepare_topology(geom, scene.positions, cl, None, caps)
+    assert s == H.OK, s
+    c0["previous_pressure"] = 1.0
+    moved = scene.positions + dt * scene.velocities
+    s, _, b1, c1, f1 = ctx.prepare_topology(geom, moved, cl, H.topology_history(geom, 1, b0, c0, f0), caps)
+    assert s == H.OK, s
+    m = ctx.last
+    log(f"topology: {len(b1):,} blocks, {len(c1):,} cells, {len(f1):,} faces in {m.kernel_ms:.2f} ms on the device "
+        f"(+{m.copy_ms:.1f} ms of host copies)")
+    return {"kernel_ms": m.kernel_ms, "copy_ms": m.copy_ms, "particles": scene.count, "blocks": len(b1),
+            "cells": len(c1), "faces": len(f1), "previous_faces": len(f0)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,6 +220,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="only measure the plain-copy ceiling")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner comparison")
+    ap.add_argument("--no-topology", action="store_true", help="skip the topology-preparation measurement")
     ap.add_argument("--learn-pairs", type=int, default=50000)
     ap.add_argument("--learn-epochs", type=int, default=8)
     args = ap.parse_args()
@@ -301,6 +321,8 @@ def main():
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(scene, ctx, last, args.cpu_sample, args.cpu_threads)
         log(f"cpu baseline took {time.perf_counter() - t0:.1f}s")
+    if not args.no_topology:  # last: span-based calls reuse the resident scene's buffers
+        out["topology"] = topology_measurement(ctx, scene, cells, dt)
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -113,8 +113,12 @@ bool overlaps(const void* a, size_t an, const void* b, size_t bn) {
   return pa < pb + bn && pb < pa + an;
 }
 
+// Every span-based call starts here. Those calls reuse the particle, block and face buffers that
+// also hold a device-resident scene, so the scene must be uploaded and prepared again afterwards.
 void set_geometry(fl_ctx* ctx, const fl_geometry* g) {
   ctx->geometry = make_geometry(g->world_origin.x, g->world_origin.y, g->world_origin.z, g->cell_width);
+  ctx->scene_loaded = false;
+  ctx->scene_ready = false;
 }
 
 // hipEvent interval on the context stream; each timer owns its events so timers may nest.

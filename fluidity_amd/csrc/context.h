@@ -145,7 +145,8 @@ struct fl_ctx {
   // device-resident scene
   uint64_t scene_n = 0;
   double scene_mass = 0;
-  bool scene_ready = false;
+  bool scene_loaded = false;  // fl_scene_upload ran and no span-based call reused its buffers since
+  bool scene_ready = false;   // ... and fl_scene_prepare ran on it
   uint32_t nbins = 0;
   uint64_t nvalid = 0;
   fl::Box bin_box;

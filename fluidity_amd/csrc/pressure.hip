@@ -2250,6 +2250,7 @@ int fl_scene_upload(fl_ctx* ctx, const fl_geometry* geometry, size_t n, const do
   ctx->raw.trial_aliases_sorted = true;
   ctx->scene_n = n;
   ctx->scene_mass = particle_mass;
+  ctx->scene_loaded = true;
   ctx->scene_ready = false;
   return FL_OK;
 }
@@ -2260,6 +2261,7 @@ int fl_scene_upload(fl_ctx* ctx, const fl_geometry* geometry, size_t n, const do
 int fl_scene_prepare(fl_ctx* ctx, const fl_pressure_cell* cells, size_t cell_count, int32_t domain_cells,
                      double density) {
   if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  if (!ctx->scene_loaded) return FL_FAILED_PRECONDITION;
   if (!(density > 0.0) || domain_cells <= 0) return FL_INVALID_ARGUMENT;
   if (cell_count > ctx->cap.pressure_cells) return FL_RESOURCE_EXHAUSTED;
   (void)hipSetDevice(ctx->device);

@@ -342,7 +342,9 @@ int fl_project(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, 
 /* ---- device-resident liquid step (no host staging inside the timed region) ----
  * The scene is uploaded once (SoA particles, sorted world positions = trial positions), topology and
  * pressure structure are prepared once, and fl_step runs bin-sort -> P2G -> normalize/forces ->
- * projection -> G2P on the device. */
+ * projection -> G2P on the device. The span-based calls above reuse the context buffers that hold the
+ * scene: after any of them, fl_scene_prepare, fl_step and fl_scene_download return FailedPrecondition
+ * until fl_scene_upload runs again. */
 typedef struct fl_step_report {
   double sort_ms, p2g_ms, normalize_ms, projection_ms, g2p_ms, total_ms;
   double p2g_kernel_ms, g2p_kernel_ms; /* the P2G and G2P kernels alone (hipEvents around the launch) */

@@ -15,6 +15,7 @@
 
 #include "fluidity/host/hip_liquid_substep_backend_3d.h"
 #include "fluidity/host/liquid_boundary_transfer_3d.h"
+#include "fluidity/host/liquid_step_workspace_3d.h"
 #include "fluidity/host/mac_pressure_3d.h"
 #include "fluidity/host/sparse_mac_grid_3d.h"
 #include "fluidity/host/transfers_3d.h"
@@ -487,6 +488,115 @@ void SharedRuntimeGrows() {
   }
 }
 
+// LiquidStepWorkspace3D.RemapsHistoryByIdentityAcrossShiftedBlockOrdinals
+// (native/tests/test_liquid_step_workspace_3d.cc:142-190), then a random history against the oracle.
+void WorkspaceRemapsHistoryAndMatchesOracle() {
+  const UniformMacGeometry3D unit{{0.0, 0.0, 0.0}, 1.0};
+  const LiquidStateCapacities3D caps{16, 32, 16, 32 * 192, 0};
+  auto ws = LiquidStepWorkspace3D::Create(caps, unit);
+  EXPECT(ws.ok());
+  EXPECT(ws->prepared().status().code() == StatusCode::kFailedPrecondition);
+  const std::vector<CellIndex3D> old_cells{{1, 1, 1}, {9, 1, 1}};
+  EXPECT(ws->PrepareTrialTopology({}, old_cells).ok());
+  auto view = ws->prepared();
+  EXPECT(view.ok() && view->blocks.size() == 2 && view->faces.size() == 384);
+  std::vector<LiquidBlockState3D> blocks(view->blocks.begin(), view->blocks.end());
+  std::vector<LiquidCellState3D> cells(view->cells.begin(), view->cells.end());
+  std::vector<LiquidFaceState3D> faces(view->faces.begin(), view->faces.end());
+  blocks[0].generation = 4;
+  blocks[0].activation_age = 7;
+  blocks[0].inactive_age = 2;
+  cells[0].previous_pressure = 123.5;
+  cells[0].occupancy = LiquidCellOccupancy3D::kSolid;
+  for (auto& f : faces) {
+    f.carrier_velocity = 10.0 + static_cast<double>(f.face_slot);
+    f.validity = LiquidFaceValidity3D::kLowMass;
+    f.side_tag = -17;
+    f.structurally_eligible = false;
+  }
+  const LiquidTopologyHistory3D previous{unit, 11, blocks, cells, faces};
+  const std::vector<CellIndex3D> new_cells{{-3, 1, 1}, {1, 1, 1}};
+  EXPECT(ws->PrepareTrialTopology({}, new_cells, &previous).ok());
+  view = ws->prepared();
+  EXPECT(view.ok() && view->blocks.size() == 2);
+  EXPECT(view->blocks[0].block_index.x == -1 && view->blocks[0].generation == 12 && view->blocks[0].activation_age == 0);
+  EXPECT(view->blocks[1].generation == 4 && view->blocks[1].activation_age == 7 && view->blocks[1].inactive_age == 0);
+  EXPECT(view->cells[0].previous_pressure == 0 && view->cells[0].occupancy == LiquidCellOccupancy3D::kLiquid);
+  EXPECT(view->cells[1].previous_pressure == 123.5 && view->cells[1].occupancy == LiquidCellOccupancy3D::kSolid);
+  for (std::size_t i = 0; i < 192; ++i) {
+    EXPECT(view->faces[i].carrier_velocity == 0 && view->faces[i].validity == LiquidFaceValidity3D::kMissing);
+    LiquidFaceState3D expected = faces[i];
+    expected.face_slot += 192;
+    EXPECT(view->faces[i + 192] == expected);
+  }
+  // A failed preparation keeps the published bank.
+  const std::vector<CellIndex3D> bad{{2147483647, 0, 0}};
+  EXPECT(ws->PrepareTrialTopology({}, bad).code() == StatusCode::kOutOfRange);
+  EXPECT(ws->prepared()->blocks.size() == 2 && ws->prepared()->cells[1].previous_pressure == 123.5);
+  LiquidTopologyHistory3D stale = previous;
+  stale.epoch = UINT64_MAX;
+  EXPECT(ws->PrepareTrialTopology({}, new_cells, &stale).code() == StatusCode::kOutOfRange);
+
+  // Random particles and cells with a remapped history: records equal the oracle's.
+  std::mt19937_64 rng(7);
+  std::uniform_real_distribution<double> u(-2.0, 2.0);
+  std::uniform_int_distribution<int> ci(-10, 10);
+  const UniformMacGeometry3D g{{-0.37, 0.11, 0.5}, 0.25};
+  const LiquidStateCapacities3D big{1 << 16, 1 << 12, 1 << 16, 192 << 12, 0};
+  std::vector<Vec3> pos(2000);
+  for (auto& p : pos) p = {u(rng), u(rng), u(rng)};
+  std::vector<CellIndex3D> cl(500);
+  for (auto& c : cl) c = {ci(rng), ci(rng), ci(rng)};
+  auto w2 = LiquidStepWorkspace3D::Create(big, g);
+  EXPECT(w2.ok() && w2->PrepareTrialTopology(pos, cl).ok());
+  auto v0 = w2->prepared();
+  std::vector<LiquidBlockState3D> b0(v0->blocks.begin(), v0->blocks.end());
+  std::vector<LiquidCellState3D> c0(v0->cells.begin(), v0->cells.end());
+  std::vector<LiquidFaceState3D> f0(v0->faces.begin(), v0->faces.end());
+  for (std::size_t i = 0; i < f0.size(); ++i) f0[i].carrier_velocity = std::sin(double(i));
+  for (std::size_t i = 0; i < c0.size(); ++i) c0[i].previous_pressure = double(i);
+  for (auto& p : pos) p = {p.x + 0.3, p.y - 0.2, p.z};
+  for (auto& c : cl) c = {c.x + 1, c.y, c.z - 2};
+  const LiquidTopologyHistory3D h0{g, 3, b0, c0, f0};
+  HipOperatorCapacities3D rc;
+  rc.particles = 1 << 16;
+  rc.blocks = 1 << 12;
+  rc.lookup_blocks = 1 << 16;
+  rc.pressure_cells = 1 << 16;
+  auto rt = HipOperatorRuntime3D::Create(rc);
+  EXPECT(rt.ok());
+  HipOperatorMeasurement m;
+  EXPECT(w2->HipPrepareTrialTopology(**rt, pos, cl, &h0, m).ok());
+  auto v1 = w2->prepared();
+  const flo_topology_capacities oc{big.particles, big.blocks, big.cells, big.faces};
+  const flo_topology_history oh{*O(&g), 3, reinterpret_cast<const flo_block_state*>(b0.data()), b0.size(),
+                                reinterpret_cast<const flo_cell_state*>(c0.data()), c0.size(),
+                                reinterpret_cast<const flo_face_state*>(f0.data()), f0.size()};
+  std::vector<flo_sort_entry> wo(pos.size());
+  std::vector<flo_block_state> wb(big.blocks);
+  std::vector<flo_cell_state> wc(cl.size());
+  std::vector<flo_face_state> wf(big.faces);
+  std::uint64_t nb = 0, nc = 0, nf = 0;
+  EXPECT(flo_prepare_topology(O(&g), &oc, reinterpret_cast<const flo_vec3*>(pos.data()), pos.size(),
+                              reinterpret_cast<const flo_int3*>(cl.data()), cl.size(), &oh, wo.data(), wb.data(), &nb,
+                              wc.data(), &nc, wf.data(), &nf) == 0);
+  EXPECT(nb == v1->blocks.size() && nc == v1->cells.size() && nf == v1->faces.size());
+  for (std::size_t i = 0; i < pos.size(); ++i)
+    EXPECT(v1->particle_order[i].original_ordinal == wo[i].original_ordinal);
+  for (std::size_t i = 0; i < nb; ++i) {
+    const LiquidBlockState3D& a = v1->blocks[i];
+    EXPECT(a.block_index.x == wb[i].block_index.x && a.block_index.z == wb[i].block_index.z &&
+           a.generation == wb[i].generation && a.activation_age == wb[i].activation_age);
+  }
+  for (std::size_t i = 0; i < nc; ++i)
+    EXPECT(v1->cells[i].cell_index.x == wc[i].cell_index.x && v1->cells[i].previous_pressure == wc[i].previous_pressure);
+  for (std::size_t i = 0; i < nf; ++i)
+    EXPECT(v1->faces[i].face_slot == wf[i].face_slot && v1->faces[i].carrier_velocity == wf[i].carrier_velocity &&
+           v1->faces[i].face_index.lattice_index.y == wf[i].face_index.lattice.y);
+  std::printf("  topology: %zu blocks, %zu cells, %.3f ms on %s\n", v1->blocks.size(), v1->cells.size(), m.kernel_ms,
+              m.architecture);
+}
+
 int Run(const char* name, const std::function<void()>& test) {
   std::printf("[ RUN      ] %s\n", name);
   std::fflush(stdout);
@@ -515,6 +625,7 @@ int main(int argc, char** argv) {
     failed += Run("GatherMatchesOracle", GatherMatchesOracle);
     failed += Run("BackendStagesMatchOracle", BackendStagesMatchOracle);
     failed += Run("ProjectionMeetsGatesAndMatchesOracle", ProjectionMeetsGatesAndMatchesOracle);
+    failed += Run("WorkspaceRemapsHistoryAndMatchesOracle", WorkspaceRemapsHistoryAndMatchesOracle);
     failed += Run("SharedRuntimeGrows", SharedRuntimeGrows);
   }
   std::printf("%d test(s) failed\n", failed);

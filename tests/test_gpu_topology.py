@@ -180,3 +180,18 @@ def test_dam_break_32_stepped_twice(ctx):
     s, (_, b1, c1, f1) = run(ctx, moved, cells1, previous=(b0, c0, f0), epoch=1, geom=geom)
     assert s == O.OK and len(c1) == len(cells1)
     print("topology", len(pos), "particles", len(b1), "blocks", len(c1), "cells", ctx.last.kernel_ms, "ms", flush=True)
+
+
+def test_span_calls_invalidate_the_resident_scene(ctx):
+    sc = scenes.dam_break(16, seed=1)
+    geom = sc.geometry_record()
+    assert ctx.scene_upload(geom, sc.positions, sc.velocities, sc.slopes, sc.mass) == H.OK
+    assert ctx.scene_prepare(sc.pressure_cells(), sc.n, sc.rho) == H.OK
+    opts = H.pressure_options(relative_target=1e-6, levels=3)
+    assert ctx.step(sc.dt, sc.gravity, 2.0 ** -40, 2.0 ** -40, sc.count, opts)[0] == H.OK
+    assert run(ctx, sc.positions[:100])[0] == O.OK  # reuses the scene's buffers
+    assert ctx.step(sc.dt, sc.gravity, 2.0 ** -40, 2.0 ** -40, sc.count, opts)[0] == H.FAILED_PRECONDITION
+    assert ctx.scene_prepare(sc.pressure_cells(), sc.n, sc.rho) == H.FAILED_PRECONDITION
+    assert ctx.scene_upload(geom, sc.positions, sc.velocities, sc.slopes, sc.mass) == H.OK
+    assert ctx.scene_prepare(sc.pressure_cells(), sc.n, sc.rho) == H.OK
+    assert ctx.step(sc.dt, sc.gravity, 2.0 ** -40, 2.0 ** -40, sc.count, opts)[0] == H.OK
