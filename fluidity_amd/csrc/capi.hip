@@ -227,6 +227,28 @@ struct TopologyBuffers {
   }
 };
 
+// Device copies of one particle update's records; allocated on first use and grown with it.
+struct UpdateBuffers {
+  fl_liquid_particle* particles = nullptr;
+  fl_boundary_gather* gathered = nullptr;
+  double* distances = nullptr;
+  uint64_t cap_particles = 0, cap_gathered = 0, cap_distances = 0;
+  void release() {
+    void* ptrs[] = {particles, gathered, distances};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    *this = UpdateBuffers{};
+  }
+};
+
+void destroy_update(fl_ctx* ctx) {
+  auto* b = static_cast<UpdateBuffers*>(ctx->update);
+  if (!b) return;
+  b->release();
+  delete b;
+  ctx->update = nullptr;
+}
+
 void destroy_topology(fl_ctx* ctx) {
   auto* b = static_cast<TopologyBuffers*>(ctx->topology);
   if (!b) return;
@@ -375,6 +397,7 @@ void fl_destroy(fl_ctx* ctx) {
   fl::destroy_learn(ctx);
   destroy_boundary(ctx);
   destroy_topology(ctx);
+  destroy_update(ctx);
   release(ctx);
   for (hipEvent_t e : ctx->stage)
     if (e) (void)hipEventDestroy(e);
@@ -627,6 +650,44 @@ int fl_prepare_topology(fl_ctx* ctx, const fl_geometry* geometry, const fl_topol
   *out_cell_count = nc;
   *out_face_count = 192 * nb;
   fill_measurement(ctx, measurement, kernel_ms, copy_ms, 16);
+  return FL_OK;
+}
+
+int fl_update_particles(fl_ctx* ctx, const fl_particle_update_options* options, const fl_boundary_gather* gathered,
+                        const double* liquid_distances, size_t n, fl_liquid_particle* particles,
+                        fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  if (!options || (n && (!gathered || !liquid_distances || !particles))) return FL_INVALID_ARGUMENT;
+  if (n && (overlaps(particles, n * sizeof(fl_liquid_particle), gathered, n * sizeof(fl_boundary_gather)) ||
+            overlaps(particles, n * sizeof(fl_liquid_particle), liquid_distances, n * sizeof(double))))
+    return FL_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  if (!ctx->update) ctx->update = new UpdateBuffers();
+  UpdateBuffers& B = *static_cast<UpdateBuffers*>(ctx->update);
+  FL_TRY(grow(&B.particles, &B.cap_particles, n));
+  FL_TRY(grow(&B.gathered, &B.cap_gathered, n));
+  FL_TRY(grow(&B.distances, &B.cap_distances, n));
+  hipStream_t s = ctx->stream;
+  double copy_ms = 0;
+  {
+    Timer c(ctx);
+    if (n) {
+      FL_HIP(hipMemcpyAsync(B.particles, particles, n * sizeof(fl_liquid_particle), hipMemcpyHostToDevice, s));
+      FL_HIP(hipMemcpyAsync(B.gathered, gathered, n * sizeof(fl_boundary_gather), hipMemcpyHostToDevice, s));
+      FL_HIP(hipMemcpyAsync(B.distances, liquid_distances, n * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    copy_ms += c.stop();
+  }
+  Timer t(ctx);
+  FL_TRY(reset_error(ctx));
+  FL_TRY(update_particles(ctx, *options, B.gathered, B.distances, n, B.particles));
+  FL_TRY(first_error_code(ctx));
+  const double kernel_ms = t.stop();
+  Timer c(ctx);
+  if (n) FL_HIP(hipMemcpyAsync(particles, B.particles, n * sizeof(fl_liquid_particle), hipMemcpyDeviceToHost, s));
+  FL_HIP(hipStreamSynchronize(s));
+  copy_ms += c.stop();
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, 1);
   return FL_OK;
 }
 

@@ -159,6 +159,8 @@ struct fl_ctx {
   void* boundary = nullptr;
   // topology-preparation buffers (owned by capi.hip, allocated on first use)
   void* topology = nullptr;
+  // particle-update buffers (owned by capi.hip, allocated on first use)
+  void* update = nullptr;
 };
 
 namespace fl {
@@ -240,6 +242,10 @@ int unique_cells(fl_ctx* ctx, const int32_t* cells, uint64_t nc, void* words, in
 int remap_topology(fl_ctx* ctx, const fl_block_state* pb, uint64_t pnb, const fl_cell_state* pc, uint64_t pnc,
                    const fl_face_state* pf, uint64_t pnf, uint64_t generation, const int32_t* cells, uint64_t nc,
                    int64_t* old, fl_block_state* ob, fl_cell_state* oc, fl_face_state* of);
+
+// Particle update without an obstacle (update.hip): in place on device records, first failure into ctx->err.
+int update_particles(fl_ctx* ctx, const fl_particle_update_options& o, const fl_boundary_gather* gathered,
+                     const double* distances, uint64_t n, fl_liquid_particle* particles);
 
 // Small helpers (topology.hip)
 int read_error(fl_ctx* ctx, uint64_t* word);

@@ -49,7 +49,7 @@ class StepReport(C.Structure):
         return np.frombuffer(bytes(self.pressure), dtype=R.PRESSURE_DIAGNOSTICS)[0]
 
 
-EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_prepare_topology", "fl_p2g", "fl_p2g_quanta", "fl_p2g_compatible", "fl_g2p_boundary",
+EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_prepare_topology", "fl_update_particles", "fl_p2g", "fl_p2g_quanta", "fl_p2g_compatible", "fl_g2p_boundary",
            "fl_normalize_forces", "fl_apply_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
            "fl_scene_download", "fl_probe_copy", "fl_learn_param_count", "fl_learn_setup", "fl_learn_set_params", "fl_learn_get_params",
            "fl_learn_forward", "fl_learn_infer", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect", "fl_learn_eval",
@@ -181,6 +181,15 @@ class Context:
         if s != OK:
             return s, None, None, None, None
         return s, order, ob[:nb.value].copy(), oc[:nc.value].copy(), of[:nf.value].copy()
+
+    def update_particles(self, options, gathered, distances, particles):
+        """fl_update_particles on a copy of the particles: (status, particles after the call)."""
+        out = np.ascontiguousarray(particles).copy()
+        g = np.ascontiguousarray(gathered)
+        d = np.ascontiguousarray(distances, dtype=np.float64)
+        s = lib().fl_update_particles(self.handle, _p(options), _p(g), _p(d), C.c_size_t(len(out)), _p(out),
+                                      C.byref(self.last))
+        return s, out
 
     def p2g(self, geometry, blocks, particles, out=None):
         nf = 192 * len(blocks)

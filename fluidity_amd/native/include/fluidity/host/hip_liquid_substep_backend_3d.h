@@ -1,14 +1,15 @@
 // Liquid-step plug-in table for the HIP hot path, shaped like LiquidSubstepBackend3D
 // (native/include/fluidity/host/liquid_substep_backend_3d.h:47-260): a POD of C-compatible function
-// pointers, each taking the backend context first. The members are the six stages of the path, with
-// the reference's member names (:204-224): topology (bin + stable sort), deposit (P2G quanta),
-// normalize, forces, project and gather (G2P). The reference's boundary-workspace records are replaced
+// pointers, each taking the backend context first. The members are the stages of the path, with
+// the reference's member names (:204-225): topology (bin + stable sort), deposit (P2G quanta),
+// normalize, forces, project, gather (G2P) and update_particles (obstacle-free particle update). The reference's boundary-workspace records are replaced
 // by the standalone operator records of this layer; scene, publication and growth hooks are not
 // part of the path.
 #pragma once
 
 #include <span>
 
+#include "fluidity/host/liquid_particle_update_3d.h"
 #include "fluidity/host/mac_pressure_3d.h"
 #include "fluidity/host/transfers_3d.h"
 
@@ -32,6 +33,8 @@ struct HipLiquidSubstepBackend3D {
   using Gather = Status(void* context, const SparseMacGrid3D& grid, std::span<const MacTransferParticle3D> particles,
                         std::span<const double> old_face_velocities, std::span<const double> new_face_velocities,
                         std::span<MacParticleGatherResult3D> results);
+  using UpdateParticles = Status(void* context, const LiquidParticleUpdateInputs3D& inputs,
+                                 std::span<LiquidParticleState3D> trial_particles);
 
   void* context = nullptr;
   PrepareTopology* topology = nullptr;
@@ -40,6 +43,7 @@ struct HipLiquidSubstepBackend3D {
   ApplyForces* forces = nullptr;
   Project* project = nullptr;
   Gather* gather = nullptr;
+  UpdateParticles* update_particles = nullptr;
 };
 
 // Table whose context is the given runtime (borrowed; it must outlive the table's use). Every member

@@ -52,6 +52,12 @@ Status GatherStage(void* context, const SparseMacGrid3D& grid, std::span<const M
   return HipGatherMac3D(Rt(context), grid, particles, u_old, u_new, results, m);
 }
 
+Status UpdateStage(void* context, const LiquidParticleUpdateInputs3D& inputs,
+                   std::span<LiquidParticleState3D> particles) {
+  HipOperatorMeasurement m;
+  return HipUpdateLiquidParticles3D(Rt(context), inputs, particles, m);
+}
+
 }  // namespace
 
 HipLiquidSubstepBackend3D MakeHipLiquidSubstepBackend3D(HipOperatorRuntime3D& runtime) {
@@ -63,6 +69,7 @@ HipLiquidSubstepBackend3D MakeHipLiquidSubstepBackend3D(HipOperatorRuntime3D& ru
   b.forces = &ForcesStage;
   b.project = &ProjectStage;
   b.gather = &GatherStage;
+  b.update_particles = &UpdateStage;
   return b;
 }
 

@@ -62,6 +62,18 @@ FACE_STATE = np.dtype({"names": ["axis", "lattice", "face_slot", "carrier_veloci
                                  "structurally_eligible"],
                        "formats": ["u1", INT3, "u8", "f8", "u1", "i8", "u1"],
                        "offsets": [0, 4, 16, 24, 32, 40, 48], "itemsize": 56})
+CONSTITUENT = np.dtype([("substance_id", "u8"), ("volume_fraction", "f8")])
+# LiquidParticleState3D (native/include/fluidity/host/liquid_state_3d.h:100-120).
+LIQUID_PARTICLE = np.dtype({"names": ["id", "tag", "world_position", "world_velocity", "velocity_slopes",
+                                      "affine_enabled", "mass", "reference_volume", "volume_ratio", "side_tag",
+                                      "pic_initialized", "phase_id", "constituent_count",
+                                      "quadratic_adaptive_transfer", "constituents"],
+                            "formats": ["u8", "u4", VEC3, VEC3, MAT3, "u1", "f8", "f8", "f8", "i8", "u1", "u8", "u1",
+                                        "u1", (CONSTITUENT, 8)],
+                            "offsets": [0, 8, 16, 40, 64, 136, 144, 152, 160, 168, 176, 184, 192, 193, 200],
+                            "itemsize": 328})
+UPDATE_OPTIONS = np.dtype([("cell_width", "f8"), ("step_duration", "f8"), ("particle_energy_blend", "f8"),
+                           ("friction_coefficient", "f8"), ("penetration_tolerance", "f8")])
 TOPOLOGY_CAPACITIES = np.dtype([("particles", "u8"), ("blocks", "u8"), ("cells", "u8"), ("faces", "u8")])
 
 SEALED, INTERNAL, DIRICHLET = 0, 1, 2
@@ -70,3 +82,4 @@ LINK_KINDS = {"sealed": SEALED, "internal": INTERNAL, "dirichlet": DIRICHLET}
 assert PARTICLE.itemsize == 152 and GATHER_RESULT.itemsize == 192 and SORT_ENTRY.itemsize == 24
 assert PRESSURE_CELL.itemsize == 56 and PRESSURE_FACE.itemsize == 80 and FACE_QUANTA.itemsize == 24
 assert BLOCK_STATE.itemsize == 32 and CELL_STATE.itemsize == 24 and FACE_STATE.itemsize == 56
+assert LIQUID_PARTICLE.itemsize == 328 and UPDATE_OPTIONS.itemsize == 40

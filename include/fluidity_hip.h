@@ -133,6 +133,41 @@ typedef struct fl_pressure_cell {
   double previous_pressure;
   double integrated_inward_wall_flux;
 } fl_pressure_cell;
+/* LiquidParticleState3D (native/include/fluidity/host/liquid_state_3d.h:100-120) with its eight
+ * LiquidConstituentFraction3D slots (liquid_mixture_3d.h:33-37). */
+typedef struct fl_constituent_fraction {
+  uint64_t substance_id;
+  double volume_fraction;
+} fl_constituent_fraction;
+typedef struct fl_liquid_particle {
+  uint64_t id;
+  uint32_t tag;
+  uint32_t pad0;
+  fl_vec3 world_position;
+  fl_vec3 world_velocity;
+  fl_mat3 velocity_slopes;
+  uint8_t affine_enabled;
+  uint8_t pad1[7];
+  double mass;
+  double reference_volume;
+  double volume_ratio;
+  int64_t side_tag;
+  uint8_t pic_initialized;
+  uint8_t pad2[7];
+  uint64_t phase_id;
+  uint8_t constituent_count;
+  uint8_t quadratic_adaptive_transfer;
+  uint8_t pad3[6];
+  fl_constituent_fraction constituents[8];
+} fl_liquid_particle; /* 328 B */
+/* The scalars of LiquidParticleUpdateInputs3D (native/include/fluidity/host/liquid_substep_backend_3d.h:16-28). */
+typedef struct fl_particle_update_options {
+  double cell_width;
+  double step_duration;
+  double particle_energy_blend;
+  double friction_coefficient;
+  double penetration_tolerance;
+} fl_particle_update_options;
 /* Topology records of a liquid state (native/include/fluidity/host/liquid_state_3d.h:129-159):
  * LiquidBlockState3D, LiquidCellState3D (occupancy 0 air, 1 liquid, 2 solid) and LiquidFaceState3D
  * (axis 0..2 with the lattice index at offset 4; validity 0 missing, 1 valid, 2 predicted, 3 low mass). */
@@ -276,6 +311,22 @@ int fl_prepare_topology(fl_ctx* ctx, const fl_geometry* geometry, const fl_topol
                         const fl_topology_history* previous, fl_sort_entry* out_order, fl_block_state* out_blocks,
                         uint64_t* out_block_count, fl_cell_state* out_cells, uint64_t* out_cell_count,
                         fl_face_state* out_faces, uint64_t* out_face_count, fl_measurement* measurement);
+
+/* ---- particle update ---- */
+/* The liquid particle update of the CPU substep backend without an obstacle or regular-cell transport
+ * (native/src/liquid_substep_3d_cpu.cc:239-405, with ResolveContact's absent-obstacle branch,
+ * native/include/fluidity/base/liquid_wall_contact_math_3d.h:365-396): per particle, alpha = 0.99 e
+ * when PIC-initialised and no gather fallback, stored velocity = new PIC + alpha (v - old PIC),
+ * advecting velocity blended the same way only under surface expansion (div u > 0 and
+ * -h <= liquid distance <= 0), position advanced by fma(advecting, dt, x), a displacement above h/2
+ * Aborted, J *= exp(dt div u) (Aborted unless finite and positive), slopes and the affine switch from
+ * the gather. gathered and liquid_distances are in particle order. The first failing particle's status
+ * is returned (InvalidArgument for non-finite inputs or invalid options, OutOfRange for a
+ * non-finite advanced position, Aborted as above); particles are written only on success. J uses the
+ * device exp, which may differ from the host's in the last place. */
+int fl_update_particles(fl_ctx* ctx, const fl_particle_update_options* options, const fl_boundary_gather* gathered,
+                        const double* liquid_distances, size_t n, fl_liquid_particle* particles,
+                        fl_measurement* measurement);
 
 /* ---- transfers ---- */
 /* DepositMac3D (native/include/fluidity/host/transfers_3d.h:24-28; native/src/transfers_3d.cc:139-145):

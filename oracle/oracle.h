@@ -168,6 +168,33 @@ int flo_normalize(const flo_face_quanta* quanta, size_t face_count, flo_quantiza
 int flo_apply_forces(const flo_int3* sorted_blocks, size_t block_count, const double* old_velocities,
                      size_t face_count, flo_vec3 acceleration, double dt, double* out_predicted);
 
+// ---- particle update (liquid_substep_3d_cpu.cc:239-405) ----
+struct flo_constituent { uint64_t substance_id; double volume_fraction; };
+// LiquidParticleState3D (native/include/fluidity/host/liquid_state_3d.h:100-120), 328 B.
+struct flo_liquid_particle {
+  uint64_t id;
+  uint32_t tag, pad0;
+  flo_vec3 world_position, world_velocity;
+  flo_mat3 velocity_slopes;
+  uint8_t affine_enabled, pad1[7];
+  double mass, reference_volume, volume_ratio;
+  int64_t side_tag;
+  uint8_t pic_initialized, pad2[7];
+  uint64_t phase_id;
+  uint8_t constituent_count, quadratic_adaptive_transfer, pad3[6];
+  flo_constituent constituents[8];
+};
+// LiquidParticleUpdateInputs3D scalars (native/include/fluidity/host/liquid_substep_backend_3d.h:16-28).
+struct flo_update_options {
+  double cell_width, step_duration, particle_energy_blend, friction_coefficient, penetration_tolerance;
+};
+// The particle update without an obstacle or regular-cell transport: stored velocity
+// new PIC + 0.99 e (v - old PIC), advecting velocity with the surface-expansion blend, advection
+// by fma, the half-cell displacement guard and J *= exp(dt div u). Particles are updated in order and
+// the first failing particle's status is returned (earlier particles stay updated, as in the reference).
+int flo_update_particles(const flo_update_options* options, const flo_boundary_gather* gathered,
+                         const double* liquid_distances, size_t n, flo_liquid_particle* particles);
+
 // ---- topology (liquid_step_workspace_3d.cc:171-300) ----
 // LiquidBlockState3D, LiquidCellState3D and LiquidFaceState3D (native/include/fluidity/host/liquid_state_3d.h:129-159).
 struct flo_block_state {

@@ -73,6 +73,18 @@ FACE_STATE = np.dtype({"names": ["face_index", "face_slot", "carrier_velocity", 
                                  "structurally_eligible"],
                        "formats": [FACE_INDEX, "u8", "f8", "u1", "i8", "u1"],
                        "offsets": [0, 16, 24, 32, 40, 48], "itemsize": 56})
+CONSTITUENT = np.dtype([("substance_id", "u8"), ("volume_fraction", "f8")])
+# LiquidParticleState3D (native/include/fluidity/host/liquid_state_3d.h:100-120).
+LIQUID_PARTICLE = np.dtype({"names": ["id", "tag", "world_position", "world_velocity", "velocity_slopes",
+                                      "affine_enabled", "mass", "reference_volume", "volume_ratio", "side_tag",
+                                      "pic_initialized", "phase_id", "constituent_count",
+                                      "quadratic_adaptive_transfer", "constituents"],
+                            "formats": ["u8", "u4", VEC3, VEC3, MAT3, "u1", "f8", "f8", "f8", "i8", "u1", "u8", "u1",
+                                        "u1", (CONSTITUENT, 8)],
+                            "offsets": [0, 8, 16, 40, 64, 136, 144, 152, 160, 168, 176, 184, 192, 193, 200],
+                            "itemsize": 328})
+UPDATE_OPTIONS = np.dtype([("cell_width", "f8"), ("step_duration", "f8"), ("particle_energy_blend", "f8"),
+                           ("friction_coefficient", "f8"), ("penetration_tolerance", "f8")])
 TOPOLOGY_CAPACITIES = np.dtype([("particles", "u8"), ("blocks", "u8"), ("cells", "u8"), ("faces", "u8")])
 assert BLOCK_STATE.itemsize == 32 and CELL_STATE.itemsize == 24 and FACE_STATE.itemsize == 56
 assert PARTICLE.itemsize == 152 and GATHER_RESULT.itemsize == 192 and SORT_ENTRY.itemsize == 24
@@ -283,6 +295,21 @@ def activate_blocks(geom, positions, cells=None, capacity=1 << 24):
     s = lib().flo_activate_blocks(_p(geom), _p(pos), C.c_size_t(len(pos)), _p(cells), C.c_size_t(len(cells)), _p(out),
                                   C.byref(cnt))
     return s, out[:cnt.value].copy()
+
+
+def update_options(cell_width, step_duration, blend, friction=0.0, penetration=0.0):
+    o = np.zeros(1, UPDATE_OPTIONS)
+    o[0] = (cell_width, step_duration, blend, friction, penetration)
+    return o
+
+
+def update_particles(options, gathered, distances, particles):
+    """flo_update_particles on a copy: (status, particles after the call)."""
+    out = np.ascontiguousarray(particles).copy()
+    g = np.ascontiguousarray(gathered)
+    d = np.ascontiguousarray(distances, dtype=np.float64)
+    s = lib().flo_update_particles(_p(options), _p(g), _p(d), C.c_size_t(len(out)), _p(out))
+    return s, out
 
 
 class TopologyHistoryC(C.Structure):

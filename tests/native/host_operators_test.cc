@@ -15,6 +15,7 @@
 
 #include "fluidity/host/hip_liquid_substep_backend_3d.h"
 #include "fluidity/host/liquid_boundary_transfer_3d.h"
+#include "fluidity/host/liquid_particle_update_3d.h"
 #include "fluidity/host/liquid_step_workspace_3d.h"
 #include "fluidity/host/mac_pressure_3d.h"
 #include "fluidity/host/sparse_mac_grid_3d.h"
@@ -597,6 +598,54 @@ void WorkspaceRemapsHistoryAndMatchesOracle() {
               m.architecture);
 }
 
+// The backend table's update_particles stage against the oracle on random particles.
+void BackendUpdateMatchesOracle() {
+  std::mt19937_64 rng(3);
+  std::normal_distribution<double> n(0.0, 1.0);
+  std::uniform_real_distribution<double> u(0.0, 1.0);
+  const std::size_t count = 3000;
+  std::vector<LiquidBoundaryParticleGather3D> gathered(count);
+  std::vector<double> distances(count);
+  std::vector<LiquidParticleState3D> particles(count);
+  for (std::size_t i = 0; i < count; ++i) {
+    auto& g = gathered[i];
+    g.new_pic_velocity = {n(rng), n(rng), n(rng)};
+    g.particle_minus_old_pic_velocity = {0.3 * n(rng), 0.3 * n(rng), 0.3 * n(rng)};
+    g.velocity_gradient = {{n(rng), n(rng), n(rng)}, {n(rng), n(rng), n(rng)}, {n(rng), n(rng), n(rng)}};
+    g.fitted_velocity_slopes = {{n(rng), 0, 0}, {0, n(rng), 0}, {0, 0, n(rng)}};
+    g.affine_enabled = u(rng) < 0.7;
+    g.old_pic_fallback = u(rng) < 0.1;
+    distances[i] = -0.5 + u(rng);
+    auto& p = particles[i];
+    p.id = i + 1;
+    p.world_position = {3 * u(rng), 3 * u(rng), 3 * u(rng)};
+    p.volume_ratio = 0.5 + u(rng);
+    p.pic_initialized = u(rng) < 0.8;
+  }
+  const LiquidParticleUpdateInputs3D in{gathered, distances, 0.25, 0.01, 0.7, 0.0, 0.0};
+  std::vector<LiquidParticleState3D> want = particles;
+  const flo_update_options o{0.25, 0.01, 0.7, 0.0, 0.0};
+  EXPECT(flo_update_particles(&o, reinterpret_cast<const flo_boundary_gather*>(gathered.data()), distances.data(),
+                              count, reinterpret_cast<flo_liquid_particle*>(want.data())) == 0);
+  HipOperatorCapacities3D rc;
+  rc.particles = 1 << 12;
+  auto rt = HipOperatorRuntime3D::Create(rc);
+  EXPECT(rt.ok());
+  const auto backend = MakeHipLiquidSubstepBackend3D(**rt);
+  EXPECT(backend.update_particles(backend.context, in, particles).ok());
+  for (std::size_t i = 0; i < count; ++i) {
+    const auto& a = particles[i];
+    const auto& b = want[i];
+    EXPECT(a.world_position.x == b.world_position.x && a.world_position.y == b.world_position.y &&
+           a.world_position.z == b.world_position.z && a.world_velocity.x == b.world_velocity.x &&
+           a.velocity_slopes.y_row.y == b.velocity_slopes.y_row.y && a.affine_enabled == b.affine_enabled &&
+           a.pic_initialized && std::abs(a.volume_ratio - b.volume_ratio) <= 4.5e-16 * b.volume_ratio);
+  }
+  // A mismatched span is InvalidArgument and leaves the particles untouched.
+  const LiquidParticleUpdateInputs3D short_in{std::span(gathered).first(10), distances, 0.25, 0.01, 0.7, 0.0, 0.0};
+  EXPECT(UpdateLiquidParticles3D(short_in, particles).code() == StatusCode::kInvalidArgument);
+}
+
 int Run(const char* name, const std::function<void()>& test) {
   std::printf("[ RUN      ] %s\n", name);
   std::fflush(stdout);
@@ -626,6 +675,7 @@ int main(int argc, char** argv) {
     failed += Run("BackendStagesMatchOracle", BackendStagesMatchOracle);
     failed += Run("ProjectionMeetsGatesAndMatchesOracle", ProjectionMeetsGatesAndMatchesOracle);
     failed += Run("WorkspaceRemapsHistoryAndMatchesOracle", WorkspaceRemapsHistoryAndMatchesOracle);
+    failed += Run("BackendUpdateMatchesOracle", BackendUpdateMatchesOracle);
     failed += Run("SharedRuntimeGrows", SharedRuntimeGrows);
   }
   std::printf("%d test(s) failed\n", failed);

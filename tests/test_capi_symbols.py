@@ -32,6 +32,7 @@ def test_record_layouts_match_the_header():
     assert R.BOUNDARY_PARTICLE.itemsize == 168 and R.FACE_TAG.itemsize == 16 and R.COMPATIBLE_SAMPLE.itemsize == 72
     assert R.WALL_SAMPLE.itemsize == 64 and R.BOUNDARY_GATHER.itemsize == 224
     assert R.BLOCK_STATE.itemsize == 32 and R.CELL_STATE.itemsize == 24 and R.FACE_STATE.itemsize == 56
+    assert R.LIQUID_PARTICLE.itemsize == 328 and R.UPDATE_OPTIONS.itemsize == 40
 
 
 def test_record_offsets_match_the_c_compiler(tmp_path):
@@ -47,6 +48,7 @@ def test_record_offsets_match_the_c_compiler(tmp_path):
         "fl_face_state": (R.FACE_STATE, {"axis": "axis", "lattice": "lattice", "face_slot": "face_slot",
                                          "carrier_velocity": "carrier_velocity", "validity": "validity",
                                          "side_tag": "side_tag", "structurally_eligible": "structurally_eligible"}),
+        "fl_liquid_particle": (R.LIQUID_PARTICLE, {n: n for n in R.LIQUID_PARTICLE.names}),
         "fl_boundary_gather": (R.BOUNDARY_GATHER, {"affine_enabled": "affine_enabled",
                                                    "low_mass_fallback": "low_mass_fallback"}),
     }

@@ -1,0 +1,157 @@
+"""GPU parity of the obstacle-free particle update (fl_update_particles) against the oracle.
+
+Bar: the same status as the oracle; on success every field bit-identical except the volume ratio,
+which goes through exp and may differ from the host's exp in the last place (|dJ| <= 2 ulp). Cases:
+the closed-form cases of the oracle tests, random particles with mixed PIC initialisation,
+fallbacks, surface expansion and compression, the half-cell guard on both sides, every error kind
+and the first failing particle, and a large batch.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from fluidity_amd import hip as H
+from fluidity_amd import records as R
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = H.Context(particles=1 << 16, blocks=1 << 10, lookup_blocks=1 << 14)
+    print("device", c.device_name(), flush=True)
+    yield c
+    c.close()
+
+
+def random_case(rng, n, h=0.25, dt=0.01):
+    g = np.zeros(n, O.BOUNDARY_GATHER)
+    for k in "xyz":
+        g["new_pic_velocity"][k] = rng.normal(0, 2.0, n)
+        g["particle_minus_old_pic_velocity"][k] = rng.normal(0, 0.5, n)
+        for r in ("x_row", "y_row", "z_row"):
+            g["velocity_gradient"][r][k] = rng.normal(0, 3.0, n)
+            g["fitted_velocity_slopes"][r][k] = rng.normal(0, 1.0, n)
+    g["affine_enabled"] = rng.integers(0, 2, n)
+    g["old_pic_fallback"] = rng.random(n) < 0.1
+    g["new_pic_fallback"] = rng.random(n) < 0.1
+    g["low_mass_fallback"] = rng.random(n) < 0.1
+    p = np.zeros(n, O.LIQUID_PARTICLE)
+    p["id"] = np.arange(1, n + 1)
+    p["tag"] = rng.integers(0, 1 << 30, n)
+    for k in "xyz":
+        p["world_position"][k] = rng.uniform(-3.0, 3.0, n)
+        p["world_velocity"][k] = rng.normal(0, 1, n)
+    p["volume_ratio"] = rng.uniform(0.5, 1.5, n)
+    p["mass"] = 0.125
+    p["pic_initialized"] = rng.random(n) < 0.8
+    p["constituent_count"] = 1
+    p["constituents"][:, 0]["substance_id"] = 3
+    p["constituents"][:, 0]["volume_fraction"] = 1.0
+    d = rng.uniform(-2 * h, h, n)
+    return O.update_options(h, dt, rng.uniform(0, 1)), g, d, p
+
+
+def check(ctx, options, g, d, p):
+    s_want, want = O.update_particles(options, g, d, p)
+    s_got, got = ctx.update_particles(options.view(R.UPDATE_OPTIONS), g.view(R.BOUNDARY_GATHER), d,
+                                      p.view(R.LIQUID_PARTICLE))
+    assert s_got == s_want, (s_got, s_want)
+    got = got.view(O.LIQUID_PARTICLE)
+    if s_want != O.OK:
+        for name in O.LIQUID_PARTICLE.names:  # untouched on error
+            assert np.array_equal(got[name], p[name]), name
+        return s_want
+    for name in O.LIQUID_PARTICLE.names:
+        if name == "volume_ratio":
+            a, b = got[name], want[name]
+            assert np.all(np.abs(a - b) <= 2 * np.spacing(np.abs(b))), np.max(np.abs(a - b) / np.spacing(np.abs(b)))
+        elif got[name].dtype.names:
+            for sub in got[name].dtype.names:
+                x, y = got[name][sub], want[name][sub]
+                if x.dtype.names:
+                    for c in x.dtype.names:
+                        assert np.array_equal(x[c].view(np.uint64), y[c].view(np.uint64)), (name, sub, c)
+                else:
+                    assert np.array_equal(x, y), (name, sub)
+        else:
+            assert np.array_equal(got[name], want[name]), name
+    return s_want
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_random_particles(ctx, seed):
+    rng = np.random.default_rng(seed)
+    assert check(ctx, *random_case(rng, 5000)) == O.OK
+
+
+def test_large_batch(ctx):
+    rng = np.random.default_rng(9)
+    assert check(ctx, *random_case(rng, 60000)) == O.OK
+    print("update kernel ms", ctx.last.kernel_ms, "copies ms", ctx.last.copy_ms, flush=True)
+
+
+def one(h=0.25, dt=0.01, **kw):
+    rng = np.random.default_rng(0)
+    o, g, d, p = random_case(rng, 4, h, dt)
+    g["velocity_gradient"]["x_row"]["x"] = 0.0
+    g["velocity_gradient"]["y_row"]["y"] = 0.0
+    g["velocity_gradient"]["z_row"]["z"] = 0.0
+    for k in "xyz":
+        g["new_pic_velocity"][k] = 0.5
+    return o, g, d, p
+
+
+def test_guard_and_error_kinds(ctx):
+    h, dt = 0.25, 0.01
+    o, g, d, p = one(h, dt)
+    v = 0.5 * h / dt
+    g["new_pic_velocity"]["x"][2] = v
+    g["particle_minus_old_pic_velocity"]["x"][2] = 0.0
+    p["world_position"]["x"][2] = 0.0
+    g["old_pic_fallback"][2] = 1
+    assert check(ctx, o, g, d, p) == O.OK  # exactly h/2
+    g["new_pic_velocity"]["x"][2] = np.nextafter(v, np.inf)
+    assert check(ctx, o, g, d, p) == O.ABORTED
+    o, g, d, p = one(h, dt)
+    g["velocity_gradient"]["y_row"]["y"][1] = 1e5
+    assert check(ctx, o, g, d, p) == O.ABORTED  # J overflows
+    o, g, d, p = one(h, dt)
+    g["particle_minus_old_pic_velocity"]["z"][3] = math.nan
+    p["pic_initialized"][3] = 1
+    g["low_mass_fallback"][3] = 0
+    g["old_pic_fallback"][3] = g["new_pic_fallback"][3] = 0
+    assert check(ctx, o, g, d, p) == O.INVALID_ARGUMENT
+    o, g, d, p = one(h, dt)
+    g["new_pic_velocity"]["x"][0] = 1e308
+    p["world_position"]["x"][0] = 1.797e308
+    assert check(ctx, o, g, d, p) == O.OUT_OF_RANGE
+    o, g, d, p = one(h, dt)
+    o["friction_coefficient"] = -1.0
+    assert check(ctx, o, g, d, p) == O.INVALID_ARGUMENT
+    o, g, d, p = one(h, dt)
+    o["step_duration"] = 0.0
+    assert check(ctx, o, g, d, p) == O.INVALID_ARGUMENT
+    # the first failing particle decides: particle 1 Aborted before particle 3 InvalidArgument
+    o, g, d, p = one(h, dt)
+    g["new_pic_velocity"]["y"][1] = 1e9
+    p["world_position"]["z"][3] = math.inf
+    assert check(ctx, o, g, d, p) == O.ABORTED
+    # an empty batch checks nothing
+    assert ctx.update_particles(o.view(R.UPDATE_OPTIONS), g[:0].view(R.BOUNDARY_GATHER), d[:0],
+                                p[:0].view(R.LIQUID_PARTICLE))[0] == H.OK
+
+
+def test_surface_expansion_boundaries(ctx):
+    o, g, d, p = one()
+    for k in "xyz":
+        g["particle_minus_old_pic_velocity"][k] = 0.75
+    g["velocity_gradient"]["x_row"]["x"] = 2.0
+    g["old_pic_fallback"] = g["new_pic_fallback"] = g["low_mass_fallback"] = 0
+    p["pic_initialized"] = 1
+    h = float(o["cell_width"][0])
+    for phi in (0.0, -h, -h / 3, np.nextafter(0.0, 1.0), np.nextafter(-h, -np.inf)):
+        d[:] = phi
+        assert check(ctx, o, g, d, p) == O.OK
