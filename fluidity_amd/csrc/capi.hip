@@ -249,6 +249,38 @@ void destroy_update(fl_ctx* ctx) {
   ctx->update = nullptr;
 }
 
+// Device copies of one link build's view, fields and outputs; allocated on first use and grown with it.
+struct LinkBuffers {
+  fl_int3* blocks = nullptr;
+  fl_center_distance* centers = nullptr;
+  fl_frozen_cell* cells = nullptr;
+  fl_frozen_face* faces = nullptr;
+  fl_cell_fields* fields = nullptr;
+  double* predicted = nullptr;
+  fl_optional_flux* wetted = nullptr;
+  fl_air_pressure* air = nullptr;
+  fl_internal_interface* interfaces = nullptr;
+  fl_pressure_cell* out_cells = nullptr;
+  fl_pressure_face* out_faces = nullptr;
+  uint64_t* lower = nullptr;
+  uint64_t cap[12] = {};
+  void release() {
+    void* ptrs[] = {blocks, centers, cells, faces, fields, predicted, wetted, air, interfaces, out_cells, out_faces,
+                    lower};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    *this = LinkBuffers{};
+  }
+};
+
+void destroy_links(fl_ctx* ctx) {
+  auto* b = static_cast<LinkBuffers*>(ctx->links);
+  if (!b) return;
+  b->release();
+  delete b;
+  ctx->links = nullptr;
+}
+
 void destroy_topology(fl_ctx* ctx) {
   auto* b = static_cast<TopologyBuffers*>(ctx->topology);
   if (!b) return;
@@ -398,6 +430,7 @@ void fl_destroy(fl_ctx* ctx) {
   destroy_boundary(ctx);
   destroy_topology(ctx);
   destroy_update(ctx);
+  destroy_links(ctx);
   release(ctx);
   for (hipEvent_t e : ctx->stage)
     if (e) (void)hipEventDestroy(e);
@@ -688,6 +721,93 @@ int fl_update_particles(fl_ctx* ctx, const fl_particle_update_options* options, 
   FL_HIP(hipStreamSynchronize(s));
   copy_ms += c.stop();
   fill_measurement(ctx, measurement, kernel_ms, copy_ms, 1);
+  return FL_OK;
+}
+
+int fl_build_pressure_links(fl_ctx* ctx, const fl_frozen_view* view, const fl_pressure_fields* fields,
+                            fl_pressure_cell* out_cells, fl_pressure_face* out_faces, uint64_t* out_lower_theta_count,
+                            fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  if (!view || !fields || !out_lower_theta_count) return FL_INVALID_ARGUMENT;
+  const fl_frozen_view& g = *view;
+  const fl_pressure_fields& F = *fields;
+  if ((g.block_count && !g.blocks) || (g.center_count && !g.center_distances) || (g.cell_count && !g.cells) ||
+      (g.face_count && !g.faces) || (g.cell_count && !out_cells) || (g.face_count && !out_faces) ||
+      (F.cell_count && !F.cells) || (F.predicted_count && !F.predicted_face_velocities) ||
+      (F.wetted_count && !F.wetted_piston_face_fluxes) || (F.air_count && !F.air_cell_pressures) ||
+      (F.interface_count && !F.internal_interfaces))
+    return FL_INVALID_ARGUMENT;
+  // SparseMacGridView3D::Create of the view's keys, then ValidateFrozenView's scalar checks.
+  FL_TRY(check_geometry(&g.geometry));
+  FL_TRY(check_blocks(g.blocks, g.block_count));
+  const auto positive_finite = [](double v) { return std::isfinite(v) && v > 0; };
+  if (!positive_finite(g.step_duration) || !positive_finite(g.liquid_density) || !std::isfinite(g.surface_tension) ||
+      g.surface_tension < 0)
+    return FL_INVALID_ARGUMENT;
+  const double h = g.geometry.cell_width, full_area = h * h, full_volume = full_area * h;
+  if (!std::isnormal(full_area) || !std::isnormal(full_volume) || !std::isnormal(1 / g.liquid_density))
+    return FL_OUT_OF_RANGE;
+  (void)hipSetDevice(ctx->device);
+  if (!ctx->links) ctx->links = new LinkBuffers();
+  LinkBuffers& B = *static_cast<LinkBuffers*>(ctx->links);
+  FL_TRY(grow(&B.blocks, &B.cap[0], g.block_count));
+  FL_TRY(grow(&B.centers, &B.cap[1], g.center_count));
+  FL_TRY(grow(&B.cells, &B.cap[2], g.cell_count));
+  FL_TRY(grow(&B.faces, &B.cap[3], g.face_count));
+  FL_TRY(grow(&B.fields, &B.cap[4], F.cell_count));
+  FL_TRY(grow(&B.predicted, &B.cap[5], F.predicted_count));
+  FL_TRY(grow(&B.wetted, &B.cap[6], F.wetted_count));
+  FL_TRY(grow(&B.air, &B.cap[7], F.air_count));
+  FL_TRY(grow(&B.interfaces, &B.cap[8], F.interface_count));
+  FL_TRY(grow(&B.out_cells, &B.cap[9], g.cell_count));
+  FL_TRY(grow(&B.out_faces, &B.cap[10], g.face_count));
+  FL_TRY(grow(&B.lower, &B.cap[11], 1));
+  hipStream_t s = ctx->stream;
+  double copy_ms = 0;
+  const auto up = [&](void* dst, const void* src, size_t bytes) -> int {
+    if (bytes) FL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return FL_OK;
+  };
+  {
+    Timer c(ctx);
+    FL_TRY(up(B.blocks, g.blocks, g.block_count * sizeof(fl_int3)));
+    FL_TRY(up(B.centers, g.center_distances, g.center_count * sizeof(fl_center_distance)));
+    FL_TRY(up(B.cells, g.cells, g.cell_count * sizeof(fl_frozen_cell)));
+    FL_TRY(up(B.faces, g.faces, g.face_count * sizeof(fl_frozen_face)));
+    FL_TRY(up(B.fields, F.cells, F.cell_count * sizeof(fl_cell_fields)));
+    FL_TRY(up(B.predicted, F.predicted_face_velocities, F.predicted_count * sizeof(double)));
+    FL_TRY(up(B.wetted, F.wetted_piston_face_fluxes, F.wetted_count * sizeof(fl_optional_flux)));
+    FL_TRY(up(B.air, F.air_cell_pressures, F.air_count * sizeof(fl_air_pressure)));
+    FL_TRY(up(B.interfaces, F.internal_interfaces, F.interface_count * sizeof(fl_internal_interface)));
+    copy_ms += c.stop();
+  }
+  Timer t(ctx);
+  const LinkDevice d{B.blocks, g.block_count, B.centers, g.center_count, B.cells, g.cell_count, B.faces, g.face_count,
+                     h, g.step_duration, g.liquid_density, g.surface_tension, B.fields, B.predicted,
+                     F.ambient_pressure, B.wetted, F.wetted_count, B.air, F.air_count, B.interfaces,
+                     F.interface_count};
+  FL_TRY(reset_error(ctx));
+  FL_TRY(build_links_device(ctx, d, nullptr, nullptr, nullptr, true));
+  FL_TRY(first_error_code(ctx));
+  // LinksCore's extents (liquid_boundary_geometry_3d.cc:1085-1094).
+  if (F.cell_count != g.cell_count || F.predicted_count != g.face_count ||
+      (F.wetted_count && F.wetted_count != g.face_count) || F.air_count > g.center_count ||
+      F.interface_count > g.face_count || !std::isfinite(F.ambient_pressure))
+    return FL_INVALID_ARGUMENT;
+  FL_HIP(hipMemsetAsync(B.lower, 0, sizeof(uint64_t), s));
+  FL_TRY(build_links_device(ctx, d, B.out_cells, B.out_faces, B.lower, false));
+  FL_TRY(first_error_code(ctx));
+  const double kernel_ms = t.stop();
+  Timer c(ctx);
+  if (g.cell_count)
+    FL_HIP(hipMemcpyAsync(out_cells, B.out_cells, g.cell_count * sizeof(fl_pressure_cell), hipMemcpyDeviceToHost, s));
+  if (g.face_count)
+    FL_HIP(hipMemcpyAsync(out_faces, B.out_faces, g.face_count * sizeof(fl_pressure_face), hipMemcpyDeviceToHost, s));
+  FL_HIP(hipMemcpyAsync(ctx->host_words, B.lower, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  FL_HIP(hipStreamSynchronize(s));
+  copy_ms += c.stop();
+  *out_lower_theta_count = ctx->host_words[0];
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, 8);
   return FL_OK;
 }
 

@@ -161,6 +161,8 @@ struct fl_ctx {
   void* topology = nullptr;
   // particle-update buffers (owned by capi.hip, allocated on first use)
   void* update = nullptr;
+  // pressure-link buffers (owned by capi.hip, allocated on first use)
+  void* links = nullptr;
 };
 
 namespace fl {
@@ -242,6 +244,32 @@ int unique_cells(fl_ctx* ctx, const int32_t* cells, uint64_t nc, void* words, in
 int remap_topology(fl_ctx* ctx, const fl_block_state* pb, uint64_t pnb, const fl_cell_state* pc, uint64_t pnc,
                    const fl_face_state* pf, uint64_t pnf, uint64_t generation, const int32_t* cells, uint64_t nc,
                    int64_t* old, fl_block_state* ob, fl_cell_state* oc, fl_face_state* of);
+
+// Typed pressure links of a frozen view (links.hip): device copies of the view and the fields.
+struct LinkDevice {
+  const fl_int3* blocks;
+  uint64_t nb;
+  const fl_center_distance* centers;
+  uint64_t ncenters;
+  const fl_frozen_cell* cells;
+  uint64_t ncells;
+  const fl_frozen_face* faces;
+  uint64_t nfaces;
+  double h, step, density, sigma;
+  const fl_cell_fields* fields;
+  const double* predicted;
+  double ambient;
+  const fl_optional_flux* wetted;
+  uint64_t nwetted;
+  const fl_air_pressure* air;
+  uint64_t nair;
+  const fl_internal_interface* interfaces;
+  uint64_t ninterfaces;
+};
+// validate_only: the view's record checks (ValidateFrozenView); otherwise the field checks and the
+// links (LinksCore). Failures go to ctx->err; *d_lower (device) counts raised theta values.
+int build_links_device(fl_ctx* ctx, const LinkDevice& d, fl_pressure_cell* out_cells, fl_pressure_face* out_faces,
+                       uint64_t* d_lower, bool validate_only);
 
 // Particle update without an obstacle (update.hip): in place on device records, first failure into ctx->err.
 int update_particles(fl_ctx* ctx, const fl_particle_update_options& o, const fl_boundary_gather* gathered,

@@ -49,7 +49,7 @@ class StepReport(C.Structure):
         return np.frombuffer(bytes(self.pressure), dtype=R.PRESSURE_DIAGNOSTICS)[0]
 
 
-EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_prepare_topology", "fl_update_particles", "fl_p2g", "fl_p2g_quanta", "fl_p2g_compatible", "fl_g2p_boundary",
+EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_prepare_topology", "fl_update_particles", "fl_build_pressure_links", "fl_p2g", "fl_p2g_quanta", "fl_p2g_compatible", "fl_g2p_boundary",
            "fl_normalize_forces", "fl_apply_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
            "fl_scene_download", "fl_probe_copy", "fl_learn_param_count", "fl_learn_setup", "fl_learn_set_params", "fl_learn_get_params",
            "fl_learn_forward", "fl_learn_infer", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect", "fl_learn_eval",
@@ -97,6 +97,23 @@ def topology_capacities(particles=1 << 20, blocks=1 << 16, cells=1 << 20, faces=
     c = np.zeros(1, R.TOPOLOGY_CAPACITIES)
     c[0] = (particles, blocks, cells, faces)
     return c
+
+
+class FrozenView(C.Structure):
+    """fl_frozen_view."""
+    _fields_ = [("blocks", C.c_void_p), ("block_count", C.c_uint64), ("origin", C.c_double * 3),
+                ("cell_width", C.c_double), ("center_distances", C.c_void_p), ("center_count", C.c_uint64),
+                ("cells", C.c_void_p), ("cell_count", C.c_uint64), ("faces", C.c_void_p), ("face_count", C.c_uint64),
+                ("step_duration", C.c_double), ("liquid_density", C.c_double), ("surface_tension", C.c_double),
+                ("capillary_step_bound", C.c_double)]
+
+
+class PressureFields(C.Structure):
+    """fl_pressure_fields."""
+    _fields_ = [("cells", C.c_void_p), ("cell_count", C.c_uint64), ("predicted", C.c_void_p),
+                ("predicted_count", C.c_uint64), ("ambient_pressure", C.c_double), ("wetted", C.c_void_p),
+                ("wetted_count", C.c_uint64), ("air", C.c_void_p), ("air_count", C.c_uint64),
+                ("interfaces", C.c_void_p), ("interface_count", C.c_uint64)]
 
 
 class VelocityHistory(C.Structure):
@@ -181,6 +198,26 @@ class Context:
         if s != OK:
             return s, None, None, None, None
         return s, order, ob[:nb.value].copy(), oc[:nc.value].copy(), of[:nf.value].copy()
+
+    def build_pressure_links(self, p):
+        """fl_build_pressure_links on a problem with numpy record arrays (geom, blocks, centers, cells,
+        faces, fields, predicted, wetted, air, interfaces) and scalars (step, density, sigma, ambient):
+        (status, cells, faces, lower-theta count)."""
+        o = p.geom[0]["world_origin"]
+        arrs = [np.ascontiguousarray(a) for a in (p.blocks, p.centers, p.cells, p.faces, p.fields, p.predicted, p.wetted,
+                                                    p.air, p.interfaces)]
+        blocks, centers, cells, faces, fields, predicted, wetted, air, interfaces = arrs
+        view = FrozenView(_p(blocks), len(blocks), (C.c_double * 3)(o["x"], o["y"], o["z"]), p.geom[0]["cell_width"],
+                          _p(centers), len(centers), _p(cells), len(cells), _p(faces), len(faces), p.step, p.density,
+                          p.sigma, 0.0)
+        flds = PressureFields(_p(fields), len(fields), _p(predicted), len(predicted), p.ambient, _p(wetted), len(wetted),
+                              _p(air), len(air), _p(interfaces), len(interfaces))
+        out_c = np.zeros(len(cells), R.PRESSURE_CELL)
+        out_f = np.zeros(len(faces), R.PRESSURE_FACE)
+        lower = C.c_uint64()
+        s = lib().fl_build_pressure_links(self.handle, C.byref(view), C.byref(flds), _p(out_c), _p(out_f),
+                                          C.byref(lower), C.byref(self.last))
+        return s, out_c, out_f, lower.value
 
     def update_particles(self, options, gathered, distances, particles):
         """fl_update_particles on a copy of the particles: (status, particles after the call)."""

@@ -62,6 +62,23 @@ FACE_STATE = np.dtype({"names": ["axis", "lattice", "face_slot", "carrier_veloci
                                  "structurally_eligible"],
                        "formats": ["u1", INT3, "u8", "f8", "u1", "i8", "u1"],
                        "offsets": [0, 4, 16, 24, 32, 40, 48], "itemsize": 56})
+# Frozen boundary geometry and pressure fields (native/include/fluidity/host/liquid_boundary_geometry_3d.h).
+FACE_INDEX = np.dtype({"names": ["axis", "lattice"], "formats": ["u1", INT3], "offsets": [0, 4], "itemsize": 16})
+CENTER_DISTANCE = np.dtype({"names": ["cell_index", "signed_distance"], "formats": [INT3, "f8"], "offsets": [0, 16],
+                            "itemsize": 24})
+FROZEN_CELL = np.dtype({"names": ["cell_index", "fluid_volume", "center_distance", "curvature"],
+                        "formats": [INT3, "f8", "f8", "f8"], "offsets": [0, 16, 24, 32], "itemsize": 40})
+FROZEN_FACE = np.dtype({"names": ["face_index", "face_slot", "open_area", "blocked_area", "wall_normal_velocity",
+                                  "prescribed_wall_flux", "has_prescribed_wall_flux", "is_low_x_piston_face"],
+                        "formats": [FACE_INDEX, "u8", "f8", "f8", "f8", "f8", "u1", "u1"],
+                        "offsets": [0, 16, 24, 32, 40, 48, 56, 64], "itemsize": 72})
+CELL_FIELDS = np.dtype({"names": ["target_divergence", "bulk_modulus", "previous_pressure", "density", "has_density"],
+                        "formats": ["f8", "f8", "f8", "f8", "u1"], "offsets": [0, 8, 16, 24, 32], "itemsize": 40})
+AIR_PRESSURE = np.dtype({"names": ["cell_index", "gas_gauge_pressure"], "formats": [INT3, "f8"], "offsets": [0, 16],
+                         "itemsize": 24})
+INTERNAL_INTERFACE = np.dtype([("face_ordinal", "u8"), ("tail_fraction", "f8"), ("surface_tension", "f8"),
+                               ("curvature_tail_to_head", "f8"), ("normal_viscous_jump_tail_to_head", "f8")])
+OPTIONAL_FLUX = np.dtype({"names": ["value", "has_value"], "formats": ["f8", "u1"], "offsets": [0, 8], "itemsize": 16})
 CONSTITUENT = np.dtype([("substance_id", "u8"), ("volume_fraction", "f8")])
 # LiquidParticleState3D (native/include/fluidity/host/liquid_state_3d.h:100-120).
 LIQUID_PARTICLE = np.dtype({"names": ["id", "tag", "world_position", "world_velocity", "velocity_slopes",
@@ -83,3 +100,4 @@ assert PARTICLE.itemsize == 152 and GATHER_RESULT.itemsize == 192 and SORT_ENTRY
 assert PRESSURE_CELL.itemsize == 56 and PRESSURE_FACE.itemsize == 80 and FACE_QUANTA.itemsize == 24
 assert BLOCK_STATE.itemsize == 32 and CELL_STATE.itemsize == 24 and FACE_STATE.itemsize == 56
 assert LIQUID_PARTICLE.itemsize == 328 and UPDATE_OPTIONS.itemsize == 40
+assert FROZEN_FACE.itemsize == 72 and FROZEN_CELL.itemsize == 40 and CELL_FIELDS.itemsize == 40

@@ -133,6 +133,92 @@ typedef struct fl_pressure_cell {
   double previous_pressure;
   double integrated_inward_wall_flux;
 } fl_pressure_cell;
+/* Frozen boundary geometry and pressure-field records (native/include/fluidity/host/liquid_boundary_geometry_3d.h):
+ * LiquidCenterDistance3D, FrozenLiquidCell3D, FrozenLiquidFace3D (its std::optional wall flux as value +
+ * flag, the libstdc++ layout), LiquidPressureCellFields3D (optional density likewise),
+ * LiquidAirCellPressure3D, LiquidInternalInterface3D and an optional wetted-piston flux. */
+typedef struct fl_center_distance {
+  fl_int3 cell_index;
+  uint32_t pad;
+  double signed_distance;
+} fl_center_distance; /* 24 B */
+typedef struct fl_frozen_cell {
+  fl_int3 cell_index;
+  uint32_t pad;
+  double fluid_volume;
+  double center_distance;
+  double curvature;
+} fl_frozen_cell; /* 40 B */
+typedef struct fl_frozen_face {
+  uint8_t axis;
+  uint8_t pad0[3];
+  fl_int3 lattice;
+  uint64_t face_slot;
+  double open_area;
+  double blocked_area;
+  double wall_normal_velocity;
+  double prescribed_wall_flux;
+  uint8_t has_prescribed_wall_flux;
+  uint8_t pad1[7];
+  uint8_t is_low_x_piston_face;
+  uint8_t pad2[7];
+} fl_frozen_face; /* 72 B */
+typedef struct fl_cell_fields {
+  double target_divergence;
+  double bulk_modulus;
+  double previous_pressure;
+  double density;
+  uint8_t has_density;
+  uint8_t pad[7];
+} fl_cell_fields; /* 40 B */
+typedef struct fl_air_pressure {
+  fl_int3 cell_index;
+  uint32_t pad;
+  double gas_gauge_pressure;
+} fl_air_pressure; /* 24 B */
+typedef struct fl_internal_interface {
+  uint64_t face_ordinal;
+  double tail_fraction;
+  double surface_tension;
+  double curvature_tail_to_head;
+  double normal_viscous_jump_tail_to_head;
+} fl_internal_interface; /* 40 B */
+typedef struct fl_optional_flux {
+  double value;
+  uint8_t has_value;
+  uint8_t pad[7];
+} fl_optional_flux; /* 16 B */
+/* FrozenLiquidBoundaryGeometryView3D (native/include/fluidity/host/liquid_boundary_workspace_3d.h:34-43):
+ * the grid as its sorted block keys and geometry. */
+typedef struct fl_frozen_view {
+  const fl_int3* blocks;
+  uint64_t block_count;
+  fl_geometry geometry;
+  const fl_center_distance* center_distances;
+  uint64_t center_count;
+  const fl_frozen_cell* cells;
+  uint64_t cell_count;
+  const fl_frozen_face* faces;
+  uint64_t face_count;
+  double step_duration;
+  double liquid_density;
+  double surface_tension;
+  double capillary_step_bound;
+} fl_frozen_view;
+/* LiquidPressureFields3D (liquid_boundary_geometry_3d.h:170-185); an empty span has count 0. */
+typedef struct fl_pressure_fields {
+  const fl_cell_fields* cells;
+  uint64_t cell_count;
+  const double* predicted_face_velocities;
+  uint64_t predicted_count;
+  double ambient_pressure;
+  const fl_optional_flux* wetted_piston_face_fluxes;
+  uint64_t wetted_count;
+  const fl_air_pressure* air_cell_pressures;
+  uint64_t air_count;
+  const fl_internal_interface* internal_interfaces;
+  uint64_t interface_count;
+} fl_pressure_fields;
 /* LiquidParticleState3D (native/include/fluidity/host/liquid_state_3d.h:100-120) with its eight
  * LiquidConstituentFraction3D slots (liquid_mixture_3d.h:33-37). */
 typedef struct fl_constituent_fraction {
@@ -311,6 +397,21 @@ int fl_prepare_topology(fl_ctx* ctx, const fl_geometry* geometry, const fl_topol
                         const fl_topology_history* previous, fl_sort_entry* out_order, fl_block_state* out_blocks,
                         uint64_t* out_block_count, fl_cell_state* out_cells, uint64_t* out_cell_count,
                         fl_face_state* out_faces, uint64_t* out_face_count, fl_measurement* measurement);
+
+/* ---- pressure links ---- */
+/* BuildLiquidPressureLinks3D on a frozen view (native/src/liquid_boundary_geometry_3d.cc: ValidateFrozenView
+ * :1289-1383, then LinksCore :1077-1287): one pressure cell per frozen cell (fluid volume and the cell's
+ * fields) and one typed link per frozen face, in frozen order — internal links between two liquid cells
+ * (zero jump, or an interface crossing's fma jump with the lerp density after its capillary bound),
+ * free-surface Dirichlet links at distance clamp(theta, 0.05, 1) h with theta from the liquid and air
+ * centre distances and exterior pressure gas + sigma * curvature
+ * (base/liquid_boundary_geometry_math_3d.h:676-689), and sealed faces; the mobility from the cells'
+ * densities when present. *out_lower_theta_count receives the number of theta values raised to 0.05.
+ * The first failure of the reference's sequential checks decides the status; outputs are written only on
+ * success. The reference's bounded workspace caps (4096 cells, 24576 faces) are not applied. */
+int fl_build_pressure_links(fl_ctx* ctx, const fl_frozen_view* view, const fl_pressure_fields* fields,
+                            fl_pressure_cell* out_cells, fl_pressure_face* out_faces, uint64_t* out_lower_theta_count,
+                            fl_measurement* measurement);
 
 /* ---- particle update ---- */
 /* The liquid particle update of the CPU substep backend without an obstacle or regular-cell transport

@@ -168,6 +168,66 @@ int flo_normalize(const flo_face_quanta* quanta, size_t face_count, flo_quantiza
 int flo_apply_forces(const flo_int3* sorted_blocks, size_t block_count, const double* old_velocities,
                      size_t face_count, flo_vec3 acceleration, double dt, double* out_predicted);
 
+// ---- pressure links (liquid_boundary_geometry_3d.cc:1077-1383) ----
+// LiquidCenterDistance3D, FrozenLiquidCell3D, FrozenLiquidFace3D (with its optional wall flux as value +
+// flag, the libstdc++ layout of std::optional<double>), LiquidPressureCellFields3D,
+// LiquidAirCellPressure3D and LiquidInternalInterface3D (native/include/fluidity/host/liquid_boundary_geometry_3d.h).
+struct flo_center_distance { flo_int3 cell_index; uint32_t pad; double signed_distance; };  // 24 B
+struct flo_frozen_cell { flo_int3 cell_index; uint32_t pad; double fluid_volume, center_distance, curvature; };  // 40 B
+struct flo_frozen_face {
+  flo_face_index face_index;
+  uint64_t face_slot;
+  double open_area, blocked_area, wall_normal_velocity;
+  double prescribed_wall_flux;
+  uint8_t has_prescribed_wall_flux, pad1[7];
+  uint8_t is_low_x_piston_face, pad2[7];
+};  // 72 B
+struct flo_cell_fields {
+  double target_divergence, bulk_modulus, previous_pressure;
+  double density;
+  uint8_t has_density, pad[7];
+};  // 40 B
+struct flo_air_pressure { flo_int3 cell_index; uint32_t pad; double gas_gauge_pressure; };  // 24 B
+struct flo_internal_interface {
+  uint64_t face_ordinal;
+  double tail_fraction, surface_tension, curvature_tail_to_head, normal_viscous_jump_tail_to_head;
+};  // 40 B
+struct flo_optional_flux { double value; uint8_t has_value, pad[7]; };  // 16 B
+// FrozenLiquidBoundaryGeometryView3D (native/include/fluidity/host/liquid_boundary_workspace_3d.h:34-43).
+struct flo_frozen_view {
+  const flo_int3* blocks;
+  uint64_t block_count;
+  flo_geometry geometry;
+  const flo_center_distance* center_distances;
+  uint64_t center_count;
+  const flo_frozen_cell* cells;
+  uint64_t cell_count;
+  const flo_frozen_face* faces;
+  uint64_t face_count;
+  double step_duration, liquid_density, surface_tension, capillary_step_bound;
+};
+// LiquidPressureFields3D (liquid_boundary_geometry_3d.h:170-185); empty spans have count 0.
+struct flo_pressure_fields {
+  const flo_cell_fields* cells;
+  uint64_t cell_count;
+  const double* predicted_face_velocities;
+  uint64_t predicted_count;
+  double ambient_pressure;
+  const flo_optional_flux* wetted_piston_face_fluxes;
+  uint64_t wetted_count;
+  const flo_air_pressure* air_cell_pressures;
+  uint64_t air_count;
+  const flo_internal_interface* internal_interfaces;
+  uint64_t interface_count;
+};
+// BuildLiquidPressureLinks3D on a frozen view (ValidateFrozenView, then LinksCore): one pressure cell
+// per frozen cell and one typed link per frozen face, plus the count of theta values raised to 0.05.
+// Outputs are written only on success. The reference's 4096-cell / 24576-face workspace caps are not
+// applied.
+int flo_build_pressure_links(const flo_frozen_view* view, const flo_pressure_fields* fields,
+                             flo_pressure_cell* out_cells, flo_pressure_face* out_faces,
+                             uint64_t* out_lower_theta_count);
+
 // ---- particle update (liquid_substep_3d_cpu.cc:239-405) ----
 struct flo_constituent { uint64_t substance_id; double volume_fraction; };
 // LiquidParticleState3D (native/include/fluidity/host/liquid_state_3d.h:100-120), 328 B.

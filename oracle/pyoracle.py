@@ -73,6 +73,22 @@ FACE_STATE = np.dtype({"names": ["face_index", "face_slot", "carrier_velocity", 
                                  "structurally_eligible"],
                        "formats": [FACE_INDEX, "u8", "f8", "u1", "i8", "u1"],
                        "offsets": [0, 16, 24, 32, 40, 48], "itemsize": 56})
+# Frozen boundary geometry and pressure-field records (native/include/fluidity/host/liquid_boundary_geometry_3d.h).
+CENTER_DISTANCE = np.dtype({"names": ["cell_index", "signed_distance"], "formats": [INT3, "f8"], "offsets": [0, 16],
+                            "itemsize": 24})
+FROZEN_CELL = np.dtype({"names": ["cell_index", "fluid_volume", "center_distance", "curvature"],
+                        "formats": [INT3, "f8", "f8", "f8"], "offsets": [0, 16, 24, 32], "itemsize": 40})
+FROZEN_FACE = np.dtype({"names": ["face_index", "face_slot", "open_area", "blocked_area", "wall_normal_velocity",
+                                  "prescribed_wall_flux", "has_prescribed_wall_flux", "is_low_x_piston_face"],
+                        "formats": [FACE_INDEX, "u8", "f8", "f8", "f8", "f8", "u1", "u1"],
+                        "offsets": [0, 16, 24, 32, 40, 48, 56, 64], "itemsize": 72})
+CELL_FIELDS = np.dtype({"names": ["target_divergence", "bulk_modulus", "previous_pressure", "density", "has_density"],
+                        "formats": ["f8", "f8", "f8", "f8", "u1"], "offsets": [0, 8, 16, 24, 32], "itemsize": 40})
+AIR_PRESSURE = np.dtype({"names": ["cell_index", "gas_gauge_pressure"], "formats": [INT3, "f8"], "offsets": [0, 16],
+                         "itemsize": 24})
+INTERNAL_INTERFACE = np.dtype([("face_ordinal", "u8"), ("tail_fraction", "f8"), ("surface_tension", "f8"),
+                               ("curvature_tail_to_head", "f8"), ("normal_viscous_jump_tail_to_head", "f8")])
+OPTIONAL_FLUX = np.dtype({"names": ["value", "has_value"], "formats": ["f8", "u1"], "offsets": [0, 8], "itemsize": 16})
 CONSTITUENT = np.dtype([("substance_id", "u8"), ("volume_fraction", "f8")])
 # LiquidParticleState3D (native/include/fluidity/host/liquid_state_3d.h:100-120).
 LIQUID_PARTICLE = np.dtype({"names": ["id", "tag", "world_position", "world_velocity", "velocity_slopes",
@@ -310,6 +326,60 @@ def update_particles(options, gathered, distances, particles):
     d = np.ascontiguousarray(distances, dtype=np.float64)
     s = lib().flo_update_particles(_p(options), _p(g), _p(d), C.c_size_t(len(out)), _p(out))
     return s, out
+
+
+class FrozenViewC(C.Structure):
+    _fields_ = [("blocks", C.c_void_p), ("block_count", C.c_uint64), ("origin", C.c_double * 3),
+                ("cell_width", C.c_double), ("center_distances", C.c_void_p), ("center_count", C.c_uint64),
+                ("cells", C.c_void_p), ("cell_count", C.c_uint64), ("faces", C.c_void_p), ("face_count", C.c_uint64),
+                ("step_duration", C.c_double), ("liquid_density", C.c_double), ("surface_tension", C.c_double),
+                ("capillary_step_bound", C.c_double)]
+
+
+class PressureFieldsC(C.Structure):
+    _fields_ = [("cells", C.c_void_p), ("cell_count", C.c_uint64), ("predicted", C.c_void_p),
+                ("predicted_count", C.c_uint64), ("ambient_pressure", C.c_double), ("wetted", C.c_void_p),
+                ("wetted_count", C.c_uint64), ("air", C.c_void_p), ("air_count", C.c_uint64),
+                ("interfaces", C.c_void_p), ("interface_count", C.c_uint64)]
+
+
+class LinkProblem:
+    """A frozen view and its pressure fields as numpy arrays (the C structures point into them)."""
+
+    def __init__(self, geom, blocks, centers, cells, faces, step=0.01, density=1000.0, sigma=0.0, fields=None,
+                 predicted=None, ambient=0.0, wetted=None, air=None, interfaces=None):
+        self.geom, self.blocks = geom, np.ascontiguousarray(blocks)
+        self.centers, self.cells, self.faces = (np.ascontiguousarray(a) for a in (centers, cells, faces))
+        self.step, self.density, self.sigma, self.ambient = step, density, sigma, ambient
+        if fields is None:
+            fields = np.zeros(len(cells), CELL_FIELDS)
+            fields["bulk_modulus"] = np.inf
+        self.fields = np.ascontiguousarray(fields)
+        self.predicted = np.zeros(len(faces)) if predicted is None else np.ascontiguousarray(predicted, dtype=np.float64)
+        self.wetted = np.zeros(0, OPTIONAL_FLUX) if wetted is None else np.ascontiguousarray(wetted)
+        self.air = np.zeros(0, AIR_PRESSURE) if air is None else np.ascontiguousarray(air)
+        self.interfaces = np.zeros(0, INTERNAL_INTERFACE) if interfaces is None else np.ascontiguousarray(interfaces)
+
+    def c_view(self):
+        o = self.geom[0]["world_origin"]
+        return FrozenViewC(_p(self.blocks), len(self.blocks), (C.c_double * 3)(o["x"], o["y"], o["z"]),
+                           self.geom[0]["cell_width"], _p(self.centers), len(self.centers), _p(self.cells),
+                           len(self.cells), _p(self.faces), len(self.faces), self.step, self.density, self.sigma, 0.0)
+
+    def c_fields(self):
+        return PressureFieldsC(_p(self.fields), len(self.fields), _p(self.predicted), len(self.predicted), self.ambient,
+                               _p(self.wetted), len(self.wetted), _p(self.air), len(self.air), _p(self.interfaces),
+                               len(self.interfaces))
+
+
+def build_pressure_links(problem):
+    """flo_build_pressure_links: (status, cells, faces, lower-theta count)."""
+    cells = np.zeros(len(problem.cells), PRESSURE_CELL)
+    faces = np.zeros(len(problem.faces), PRESSURE_FACE)
+    lower = C.c_uint64()
+    v, f = problem.c_view(), problem.c_fields()
+    s = lib().flo_build_pressure_links(C.byref(v), C.byref(f), _p(cells), _p(faces), C.byref(lower))
+    return s, cells, faces, lower.value
 
 
 class TopologyHistoryC(C.Structure):

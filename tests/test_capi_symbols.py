@@ -49,6 +49,13 @@ def test_record_offsets_match_the_c_compiler(tmp_path):
                                          "carrier_velocity": "carrier_velocity", "validity": "validity",
                                          "side_tag": "side_tag", "structurally_eligible": "structurally_eligible"}),
         "fl_liquid_particle": (R.LIQUID_PARTICLE, {n: n for n in R.LIQUID_PARTICLE.names}),
+        "fl_frozen_face": (R.FROZEN_FACE, {"face_slot": "face_slot", "open_area": "open_area",
+                                           "prescribed_wall_flux": "prescribed_wall_flux",
+                                           "has_prescribed_wall_flux": "has_prescribed_wall_flux",
+                                           "is_low_x_piston_face": "is_low_x_piston_face"}),
+        "fl_frozen_cell": (R.FROZEN_CELL, {n: n for n in R.FROZEN_CELL.names}),
+        "fl_cell_fields": (R.CELL_FIELDS, {n: n for n in R.CELL_FIELDS.names}),
+        "fl_center_distance": (R.CENTER_DISTANCE, {n: n for n in R.CENTER_DISTANCE.names}),
         "fl_boundary_gather": (R.BOUNDARY_GATHER, {"affine_enabled": "affine_enabled",
                                                    "low_mass_fallback": "low_mass_fallback"}),
     }
