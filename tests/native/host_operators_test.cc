@@ -5,7 +5,9 @@
 //
 // Usage: host_operators_test cpu   (layout and status tests, no device)
 //        host_operators_test gpu   (device operators against the oracle)
+#include <array>
 #include <cmath>
+#include <limits>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -15,6 +17,7 @@
 
 #include "fluidity/host/hip_liquid_substep_backend_3d.h"
 #include "fluidity/host/liquid_boundary_transfer_3d.h"
+#include "fluidity/host/liquid_boundary_geometry_3d.h"
 #include "fluidity/host/liquid_particle_update_3d.h"
 #include "fluidity/host/liquid_step_workspace_3d.h"
 #include "fluidity/host/mac_pressure_3d.h"
@@ -646,6 +649,49 @@ void BackendUpdateMatchesOracle() {
   EXPECT(UpdateLiquidParticles3D(short_in, particles).code() == StatusCode::kInvalidArgument);
 }
 
+// LiquidBoundaryWorkspace3D.FrozenGeometryAndTypedLinksMatchOwningAndUnitCube and
+// MissingIncidentFacePreservesPublishedPressureLinks (native/tests/test_liquid_boundary_workspace_3d.cc:75-101,
+// :601-620) on a hand-frozen unit cube, then the projection of those links.
+void PressureLinksUnitCube() {
+  const UniformMacGeometry3D unit{{0.0, 0.0, 0.0}, 1.0};
+  const std::vector<BlockIndex3D> keys{{0, 0, 0}};
+  auto grid = SparseMacGrid3D::Create(unit, keys, 1);
+  EXPECT(grid.ok());
+  std::vector<LiquidCenterDistance3D> centers;
+  for (int z = 0; z <= 2; ++z)
+    for (int y = 0; y <= 2; ++y)
+      for (int x = 0; x <= 2; ++x) centers.push_back({{x, y, z}, x == 1 && y == 1 && z == 1 ? -0.5 : 0.5});
+  const std::vector<FrozenLiquidCell3D> cells{{{1, 1, 1}, 1.0, -0.5, 0.0}};
+  std::vector<FrozenLiquidFace3D> faces;
+  for (auto [axis, lx, ly, lz] : std::vector<std::array<int, 4>>{{0, 1, 1, 1}, {0, 2, 1, 1}, {1, 1, 1, 1},
+                                                                 {1, 1, 2, 1}, {2, 1, 1, 1}, {2, 1, 1, 2}}) {
+    const FaceIndex3D f{static_cast<FaceNormalAxis>(axis), {lx, ly, lz}};
+    auto slot = grid->FaceSlot(f);
+    EXPECT(slot.ok());
+    faces.push_back({f, *slot, 1.0, 0.0, 0.0});
+  }
+  const FrozenLiquidBoundaryGeometryView3D view{grid->view(), centers, cells, faces, 0.01, 1000.0, 0.0, 0.0};
+  const std::vector<LiquidPressureCellFields3D> fields{{0.0, std::numeric_limits<double>::infinity(), 7.0}};
+  const std::vector<double> velocities(6, 0.0);
+  auto links = BuildLiquidPressureLinks3D(view, {fields, velocities, 9.0});
+  EXPECT(links.ok() && links->faces.size() == 6 && links->lower_theta_regularization_count == 0);
+  EXPECT(links->cells[0].previous_pressure == 7.0);
+  for (const auto& f : links->faces) {
+    EXPECT(f.pressure_gradient_distance == 0.5 && f.inverse_density == 0.001);
+    EXPECT(std::get<DirichletMacPressureLink3D>(f.pressure_link).exterior_pressure == 9.0);
+  }
+  // With no flux the cell takes the exterior pressure.
+  auto projected = ProjectMacPressure3D({*grid, 0.01, links->cells, links->faces});
+  EXPECT(projected.ok() && std::abs(projected->cell_pressures[0] - 9.0) <= 1e-9);
+  FrozenLiquidBoundaryGeometryView3D missing = view;
+  missing.faces = std::span(faces).first(5);
+  EXPECT(BuildLiquidPressureLinks3D(missing, {fields, std::span(velocities).first(5), 9.0}).status().code() ==
+         StatusCode::kFailedPrecondition);
+  FrozenLiquidBoundaryGeometryView3D heavy = view;
+  heavy.liquid_density = std::numeric_limits<double>::max();
+  EXPECT(BuildLiquidPressureLinks3D(heavy, {fields, velocities, 9.0}).status().code() == StatusCode::kOutOfRange);
+}
+
 int Run(const char* name, const std::function<void()>& test) {
   std::printf("[ RUN      ] %s\n", name);
   std::fflush(stdout);
@@ -676,6 +722,7 @@ int main(int argc, char** argv) {
     failed += Run("ProjectionMeetsGatesAndMatchesOracle", ProjectionMeetsGatesAndMatchesOracle);
     failed += Run("WorkspaceRemapsHistoryAndMatchesOracle", WorkspaceRemapsHistoryAndMatchesOracle);
     failed += Run("BackendUpdateMatchesOracle", BackendUpdateMatchesOracle);
+    failed += Run("PressureLinksUnitCube", PressureLinksUnitCube);
     failed += Run("SharedRuntimeGrows", SharedRuntimeGrows);
   }
   std::printf("%d test(s) failed\n", failed);
