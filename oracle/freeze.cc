@@ -1,0 +1,555 @@
+// Oracle: frozen boundary geometry (test infrastructure only).
+// Restates FreezeCore (native/src/liquid_boundary_geometry_3d.cc:765-1068) with its sequential
+// geometry queries (RunGeometryQueries without a batch, :217-295) for the absent, half-space and slab
+// obstacles (ValidateObstacle :132-175, PackObstacle and LocalObstacle
+// native/include/fluidity/base/liquid_boundary_geometry_math_3d.h:625-650), and the scalar geometry
+// arithmetic of that header: polygon and polyhedron clipping (:160-260), clipped volume and area
+// (:296-417), the cell, face, curvature and capillary queries (:658-848).
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "grid_internal.h"
+
+using namespace flo;
+
+namespace {
+
+using V3 = std::array<double, 3>;
+
+V3 Add(V3 a, V3 b) { return {a[0] + b[0], a[1] + b[1], a[2] + b[2]}; }
+V3 Sub(V3 a, V3 b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+V3 Scale(V3 a, double f) { return {a[0] * f, a[1] * f, a[2] * f}; }
+double Dot(V3 a, V3 b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+V3 Cross(V3 a, V3 b) { return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]}; }
+bool Finite3(V3 a) { return std::isfinite(a[0]) && std::isfinite(a[1]) && std::isfinite(a[2]); }
+bool PositiveFinite(double v) { return std::isfinite(v) && v > 0; }
+bool NormalOrZero(double v) { return v == 0.0 || std::isnormal(v); }
+
+// Fixed-capacity list with the reference's overflow flag (base/liquid_boundary_geometry_math_3d.h:7-55).
+template <typename T, size_t N>
+struct List {
+  T v[N]{};
+  size_t n = 0;
+  bool overflow = false;
+  void push(const T& x) {
+    if (n == N) {
+      overflow = true;
+      return;
+    }
+    v[n++] = x;
+  }
+};
+struct Vertex {
+  V3 p;        // cell-width coordinates
+  double phi;  // interpolated liquid distance
+};
+using Polygon = List<Vertex, 32>;
+using Polyhedron = List<Polygon, 10>;
+struct Plane {
+  V3 normal;
+  double coordinate;
+  bool liquid;  // distance is the vertex's liquid value
+};
+
+double PlaneDistance(const Vertex& x, const Plane& p) { return p.liquid ? x.phi : Dot(p.normal, x.p) - p.coordinate; }
+
+// ClipPolygon: keeps the side with distance <= 0; crossings use both directly computed weights.
+Polygon Clip(const Polygon& in, const Plane& plane, Polygon* crossings) {
+  Polygon out;
+  if (in.overflow) {
+    out.overflow = true;
+    return out;
+  }
+  if (in.n == 0) return out;
+  Vertex prev = in.v[in.n - 1];
+  double dp = PlaneDistance(prev, plane);
+  for (size_t i = 0; i < in.n; ++i) {
+    const Vertex cur = in.v[i];
+    const double dc = PlaneDistance(cur, plane);
+    if ((dp <= 0) != (dc <= 0)) {
+      const double m = std::max(std::abs(dp), std::abs(dc));
+      const double ps = dp / m, cs = dc / m;
+      const double diff = ps - cs;
+      const double wp = -cs / diff, wc = ps / diff;
+      const Vertex x{Add(Scale(prev.p, wp), Scale(cur.p, wc)), wp * prev.phi + wc * cur.phi};
+      out.push(x);
+      if (crossings) crossings->push(x);
+    }
+    if (dc <= 0) out.push(cur);
+    prev = cur;
+    dp = dc;
+  }
+  return out;
+}
+
+// libstdc++'s three-argument std::hypot (the largest magnitude times the root of the scaled squares).
+double Hypot3(double x, double y, double z) { return std::hypot(x, y, z); }
+
+// UnitDirection: binary-exponent normalisation, then division by the length.
+V3 UnitDirection(V3 d) {
+  const double largest = std::max({std::abs(d[0]), std::abs(d[1]), std::abs(d[2])});
+  if (largest == 0) return {0, 0, 0};
+  int e = 0;
+  (void)std::frexp(largest, &e);
+  for (double& c : d) c = std::ldexp(c, -e);
+  return Scale(d, 1 / Hypot3(d[0], d[1], d[2]));
+}
+
+// ClipPolyhedron: clips every polygon and closes the cut with the deduplicated crossings, ordered by
+// angle about their centroid.
+Polyhedron ClipSolid(const Polyhedron& in, const Plane& plane) {
+  Polyhedron out;
+  if (in.overflow) {
+    out.overflow = true;
+    return out;
+  }
+  Polygon crossings;
+  for (size_t k = 0; k < in.n; ++k) {
+    Polygon c = Clip(in.v[k], plane, &crossings);
+    if (c.overflow || crossings.overflow) {
+      out.overflow = true;
+      return out;
+    }
+    if (c.n >= 3) out.push(c);
+  }
+  Polygon cap;
+  for (size_t k = 0; k < crossings.n; ++k) {
+    bool seen = false;
+    for (size_t j = 0; j < cap.n; ++j) seen = seen || cap.v[j].p == crossings.v[k].p;
+    if (!seen) cap.push(crossings.v[k]);
+  }
+  if (cap.n >= 3) {
+    V3 center{0, 0, 0};
+    for (size_t k = 0; k < cap.n; ++k) center = Add(center, Scale(cap.v[k].p, 1.0 / cap.n));
+    const V3 tangent = UnitDirection(Sub(cap.v[0].p, center));
+    V3 normal{0, 0, 0};
+    for (size_t k = 1; k < cap.n; ++k) {
+      normal = UnitDirection(Cross(tangent, UnitDirection(Sub(cap.v[k].p, center))));
+      if (Hypot3(normal[0], normal[1], normal[2]) > 0) break;
+    }
+    const V3 bitangent = Cross(normal, tangent);
+    const double tl = Hypot3(tangent[0], tangent[1], tangent[2]);
+    const double bl = Hypot3(bitangent[0], bitangent[1], bitangent[2]);
+    if (tl > 0 && bl > 0) {
+      const V3 ut = Scale(tangent, 1 / tl), ub = Scale(bitangent, 1 / bl);
+      const auto angle = [&](const Vertex& x) {
+        const V3 d = Sub(x.p, center);
+        return std::atan2(Dot(d, ub), Dot(d, ut));
+      };
+      for (size_t k = 1; k < cap.n; ++k) {  // insertion sort, as the reference
+        const Vertex x = cap.v[k];
+        size_t j = k;
+        while (j > 0 && angle(x) < angle(cap.v[j - 1])) {
+          cap.v[j] = cap.v[j - 1];
+          --j;
+        }
+        cap.v[j] = x;
+      }
+      out.push(cap);
+    }
+  }
+  return out;
+}
+
+// NormalizeCoordinateRow: scales one coordinate of every edge by the largest one's binary exponent.
+template <size_t N>
+int NormalizeRow(std::array<V3, N>& edges, int axis) {
+  double largest = 0;
+  for (const V3& e : edges) largest = std::max(largest, std::abs(e[axis]));
+  if (largest == 0) return 0;
+  int e = 0;
+  (void)std::frexp(largest, &e);
+  for (V3& x : edges) x[axis] = std::ldexp(x[axis], -e);
+  return e;
+}
+
+// PhysicalMeasure: restores the local exponent and h^dimension; NaN when a nonzero measure is not
+// representable.
+double PhysicalMeasure(double det, int local_exponent, double h, int dimension) {
+  if (det == 0) return 0;
+  int we = 0;
+  const double wm = std::frexp(h, &we);
+  double m = det;
+  for (int a = 0; a < dimension; ++a) m *= wm;
+  const double r = std::ldexp(m, local_exponent + dimension * we);
+  if (!std::isfinite(r) || r == 0) return std::numeric_limits<double>::quiet_NaN();
+  return r;
+}
+
+double Volume(const Polyhedron& s, double h) {
+  if (s.overflow) return std::numeric_limits<double>::quiet_NaN();
+  V3 apex{0, 0, 0};
+  size_t count = 0;
+  for (size_t k = 0; k < s.n; ++k) {
+    if (s.v[k].overflow) return std::numeric_limits<double>::quiet_NaN();
+    for (size_t j = 0; j < s.v[k].n; ++j) {
+      apex = Add(apex, s.v[k].v[j].p);
+      ++count;
+    }
+  }
+  if (count == 0) return 0;
+  apex = Scale(apex, 1.0 / count);
+  double total = 0;
+  for (size_t k = 0; k < s.n; ++k) {
+    const Polygon& g = s.v[k];
+    for (size_t j = 1; j + 1 < g.n; ++j) {
+      std::array<V3, 3> e{Sub(g.v[0].p, apex), Sub(g.v[j].p, apex), Sub(g.v[j + 1].p, apex)};
+      int ex = 0;
+      for (int a = 0; a < 3; ++a) ex += NormalizeRow(e, a);
+      const double scaled = std::abs(Dot(e[0], Cross(e[1], e[2]))) / 6;
+      total += PhysicalMeasure(scaled, ex, h, 3);
+    }
+  }
+  return total;
+}
+
+double Area(const Polygon& g, double h) {
+  if (g.overflow) return std::numeric_limits<double>::quiet_NaN();
+  double total = 0;
+  for (size_t j = 1; j + 1 < g.n; ++j) {
+    const std::array<V3, 2> e{Sub(g.v[j].p, g.v[0].p), Sub(g.v[j + 1].p, g.v[0].p)};
+    V3 w{0, 0, 0};
+    for (int a = 0; a < 3; ++a) {
+      auto n = e;
+      const int f = (a + 1) % 3, s = (a + 2) % 3;
+      const int fe = NormalizeRow(n, f);
+      const int se = NormalizeRow(n, s);
+      const double c = (n[0][f] * n[1][s] - n[0][s] * n[1][f]) / 2;
+      w[a] = PhysicalMeasure(c, fe + se, h, 2);
+    }
+    total += Hypot3(w[0], w[1], w[2]);
+  }
+  return total;
+}
+
+// ComputeOpenFaceArea: zero unless some vertex lies strictly on the open side.
+double OpenArea(const Polygon& g, const Plane& plane, double h) {
+  if (g.overflow) return std::numeric_limits<double>::quiet_NaN();
+  bool any = false;
+  for (size_t j = 0; j < g.n; ++j) any = any || PlaneDistance(g.v[j], plane) < 0;
+  if (!any) return 0;
+  return Area(Clip(g, plane, nullptr), h);
+}
+
+struct Planes {
+  bool present = false, slab = false;
+  V3 normal{0, 0, 0};
+  double low = 0, high = 0;
+  V3 velocity{0, 0, 0};
+};
+
+V3 ToV3(flo_vec3 v) { return {v.x, v.y, v.z}; }
+
+// ValidateObstacle.
+int ValidateObstacle(const flo_obstacle& o, Planes& out) {
+  Planes r;
+  if (o.kind == FLO_OBSTACLE_HALFSPACE) {
+    r = {true, false, ToV3(o.unit_normal), o.coordinate, o.coordinate, ToV3(o.wall_velocity)};
+  } else if (o.kind == FLO_OBSTACLE_SLAB) {
+    if (!PositiveFinite(o.half_width)) return FLO_INVALID_ARGUMENT;
+    r = {true, true, ToV3(o.unit_normal), o.coordinate - o.half_width, o.coordinate + o.half_width,
+         ToV3(o.wall_velocity)};
+    if (!std::isfinite(o.coordinate)) return FLO_INVALID_ARGUMENT;
+    if (!std::isfinite(r.low) || !std::isfinite(r.high) || r.low >= r.high) return FLO_OUT_OF_RANGE;
+  } else if (o.kind != FLO_OBSTACLE_ABSENT) {
+    return FLO_INVALID_ARGUMENT;
+  }
+  if (r.present && (!Finite3(r.normal) || !Finite3(r.velocity) || !std::isfinite(r.low) ||
+                    std::abs(Dot(r.normal, r.normal) - 1.0) > 1e-12))
+    return FLO_INVALID_ARGUMENT;
+  out = r;
+  return FLO_OK;
+}
+
+// LocalObstacle: the planes in the unit frame of the element whose low corner is lattice index c.
+int LocalPlanes(const flo_obstacle& o, const flo_geometry& g, flo_int3 c, Planes& out) {
+  Planes r;
+  r.present = o.kind == FLO_OBSTACLE_HALFSPACE || o.kind == FLO_OBSTACLE_SLAB;
+  r.slab = o.kind == FLO_OBSTACLE_SLAB;
+  r.normal = ToV3(o.unit_normal);
+  r.low = o.coordinate - (r.slab ? o.half_width : 0);
+  r.high = o.coordinate + (r.slab ? o.half_width : 0);
+  r.velocity = ToV3(o.wall_velocity);
+  if (!r.present) {
+    out = r;
+    return FLO_OK;
+  }
+  const V3 anchor{g.world_origin.x + g.cell_width * c.x, g.world_origin.y + g.cell_width * c.y,
+                  g.world_origin.z + g.cell_width * c.z};
+  const double projection = Dot(r.normal, anchor);
+  r.low = (r.low - projection) / g.cell_width;
+  r.high = (r.high - projection) / g.cell_width;
+  if (!Finite3(anchor) || !std::isfinite(r.low) || !std::isfinite(r.high) || (r.slab && r.low >= r.high))
+    return FLO_OUT_OF_RANGE;
+  out = r;
+  return FLO_OK;
+}
+
+constexpr int kTetrahedra[6][4] = {{0, 1, 3, 7}, {0, 1, 5, 7}, {0, 2, 3, 7}, {0, 2, 6, 7}, {0, 4, 5, 7}, {0, 4, 6, 7}};
+
+// kCell: liquid volume on each side of the obstacle, by six body-diagonal tetrahedra.
+int CellVolumes(const flo_freeze_inputs& in, flo_int3 c, const double corners[8], double& low, double& high) {
+  Planes P;
+  if (int s = LocalPlanes(in.obstacle, in.geometry, c, P)) return s;
+  const double h = in.geometry.cell_width;
+  Vertex x[8];
+  for (int k = 0; k < 8; ++k) {
+    for (int a = 0; a < 3; ++a) x[k].p[a] = (k >> a) & 1;
+    x[k].phi = corners[k];
+  }
+  low = high = 0;
+  for (int t = 0; t < 6; ++t) {
+    const int* q = kTetrahedra[t];
+    bool negative = false;
+    for (int k = 0; k < 4; ++k) negative = negative || x[q[k]].phi < 0;
+    if (!negative) continue;
+    const Vertex a = x[q[0]], b = x[q[1]], cc = x[q[2]], d = x[q[3]];
+    Polyhedron tet;
+    for (const auto& tri : {std::array<Vertex, 3>{a, b, cc}, std::array<Vertex, 3>{a, b, d},
+                            std::array<Vertex, 3>{a, cc, d}, std::array<Vertex, 3>{b, cc, d}}) {
+      Polygon g;
+      for (const Vertex& v : tri) g.push(v);
+      tet.push(g);
+    }
+    const Polyhedron liquid = ClipSolid(tet, {{0, 0, 0}, 0, true});
+    if (!P.present) {
+      high += Volume(liquid, h);
+    } else {
+      high += Volume(ClipSolid(liquid, {Scale(P.normal, -1), -P.high, false}), h);
+      if (P.slab) low += Volume(ClipSolid(liquid, {P.normal, P.low, false}), h);
+    }
+  }
+  return FLO_OK;
+}
+
+// kFace: open and blocked area of the face whose low corner is lattice l, normal to axis.
+int FaceAreas(const flo_freeze_inputs& in, flo_int3 l, int axis, double& open_area, double& blocked_area) {
+  Planes P;
+  if (int s = LocalPlanes(in.obstacle, in.geometry, l, P)) return s;
+  const double h = in.geometry.cell_width, full_area = h * h;
+  int tr[2], n = 0;
+  for (int a = 0; a < 3; ++a)
+    if (a != axis) tr[n++] = a;
+  Vertex x[4]{};
+  x[1].p[tr[0]] = 1;
+  x[2].p[tr[0]] = 1;
+  x[2].p[tr[1]] = 1;
+  x[3].p[tr[1]] = 1;
+  double blocked = 0, open = 0;
+  for (int t = 0; t < 2; ++t) {
+    Polygon g;
+    g.push(x[0]);
+    g.push(x[t + 1]);
+    g.push(x[t + 2]);
+    if (!P.present) {
+      open += Area(g, h);
+      continue;
+    }
+    Polygon b = Clip(g, {P.normal, P.high, false}, nullptr);
+    if (P.slab) b = Clip(b, {Scale(P.normal, -1), -P.low, false}, nullptr);
+    blocked += Area(b, h);
+    open += OpenArea(g, {Scale(P.normal, -1), -P.high, false}, h);
+    if (P.slab) open += OpenArea(g, {P.normal, P.low, false}, h);
+  }
+  open_area = std::min(open, full_area);
+  blocked_area = full_area - open_area;
+  if (!std::isfinite(open) || !std::isfinite(blocked) ||
+      open / full_area > 1 + 64 * std::numeric_limits<double>::epsilon() || !NormalOrZero(open_area) ||
+      !NormalOrZero(blocked_area) || (blocked > 0 && blocked_area == 0))
+    return FLO_OUT_OF_RANGE;
+  return FLO_OK;
+}
+
+// kCurvature: centred differences of the 19-point halo (x fastest, index 13 the centre).
+int Curvature(const double d[27], double h, double& out) {
+  V3 grad{0, 0, 0};
+  std::array<V3, 3> H{};
+  constexpr int stride[3] = {1, 3, 9};
+  for (int a = 0; a < 3; ++a) {
+    const double lo = d[13 - stride[a]], hi = d[13 + stride[a]];
+    grad[a] = (hi - lo) / (2 * h);
+    H[a][a] = ((hi - d[13]) + (lo - d[13])) / h / h;
+    for (int o = 0; o < a; ++o) {
+      double mixed = 0;
+      for (int f = -1; f <= 1; f += 2)
+        for (int s = -1; s <= 1; s += 2) mixed += f * s * (d[13 + f * stride[a] + s * stride[o]] / 4);
+      H[a][o] = mixed / h / h;
+      H[o][a] = H[a][o];
+    }
+  }
+  const double norm = Hypot3(grad[0], grad[1], grad[2]);
+  if (!Finite3(grad) || !std::isfinite(norm)) return FLO_OUT_OF_RANGE;
+  if (norm < 1e-8) return FLO_FAILED_PRECONDITION;
+  const V3 n = Scale(grad, 1 / norm);
+  double k = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (!Finite3(H[a])) return FLO_OUT_OF_RANGE;
+    k += (H[a][a] - n[a] * Dot(H[a], n)) / norm;
+  }
+  if (!std::isfinite(k)) return FLO_OUT_OF_RANGE;
+  out = k;
+  return FLO_OK;
+}
+
+int Offset(flo_int3 c, int axis, int amount, flo_int3& out) {
+  int64_t v[3] = {c.x, c.y, c.z};
+  v[axis] += amount;
+  if (v[axis] < kI32Min || v[axis] > kI32Max) return FLO_OUT_OF_RANGE;
+  out = {int32_t(v[0]), int32_t(v[1]), int32_t(v[2])};
+  return FLO_OK;
+}
+
+int ValidateSamples(const flo_center_distance* s, uint64_t n) {
+  for (uint64_t i = 0; i < n; ++i)
+    if (!std::isfinite(s[i].signed_distance) || (i > 0 && !BlockLess(s[i - 1].cell_index, s[i].cell_index)))
+      return FLO_INVALID_ARGUMENT;
+  return FLO_OK;
+}
+
+int Lookup(const flo_center_distance* s, uint64_t n, flo_int3 key, double& out) {
+  const flo_center_distance* e = s + n;
+  const flo_center_distance* f = std::lower_bound(
+      s, e, key, [](const flo_center_distance& x, const flo_int3& k) { return BlockLess(x.cell_index, k); });
+  if (f == e || !BlockEq(f->cell_index, key)) return FLO_FAILED_PRECONDITION;
+  out = f->signed_distance;
+  return FLO_OK;
+}
+
+uint64_t FindCell(const std::vector<flo_frozen_cell>& cells, flo_int3 key) {
+  const auto f = std::lower_bound(cells.begin(), cells.end(), key, [](const flo_frozen_cell& c, const flo_int3& k) {
+    return BlockLess(c.cell_index, k);
+  });
+  return (f != cells.end() && BlockEq(f->cell_index, key)) ? uint64_t(f - cells.begin()) : cells.size();
+}
+
+}  // namespace
+
+extern "C" int flo_freeze_geometry(const flo_freeze_inputs* inputs, flo_frozen_cell* out_cells,
+                                   uint64_t* out_cell_count, flo_frozen_face* out_faces, uint64_t* out_face_count,
+                                   double* out_capillary_step_bound) {
+  if (!inputs || !out_cell_count || !out_face_count || !out_capillary_step_bound) return FLO_INVALID_ARGUMENT;
+  const flo_freeze_inputs& in = *inputs;
+  if (!PositiveFinite(in.step_duration) || !PositiveFinite(in.liquid_density) || !std::isfinite(in.surface_tension) ||
+      in.surface_tension < 0)
+    return FLO_INVALID_ARGUMENT;
+  if (in.has_exact_curvature && !std::isfinite(in.exact_curvature)) return FLO_INVALID_ARGUMENT;
+  if (in.max_cells && in.cell_count > in.max_cells) return FLO_RESOURCE_EXHAUSTED;
+  Planes obstacle;
+  if (int s = ValidateObstacle(in.obstacle, obstacle)) return s;
+  const double h = in.geometry.cell_width, full_area = h * h, full_volume = full_area * h;
+  if (!std::isnormal(full_area) || !std::isnormal(full_volume) || !std::isnormal(1 / in.liquid_density))
+    return FLO_OUT_OF_RANGE;
+  if (int s = ValidateSamples(in.center_distances, in.center_count)) return s;
+  if (int s = ValidateSamples(in.corner_distances, in.corner_count)) return s;
+
+  std::vector<flo_frozen_cell> cells;
+  std::vector<flo_frozen_face> faces;
+  for (uint64_t i = 0; i < in.cell_count; ++i) {
+    const flo_int3 c = in.cell_indices[i];
+    if (i > 0 && !BlockLess(in.cell_indices[i - 1], c)) return FLO_INVALID_ARGUMENT;
+    double center = 0;
+    if (int s = Lookup(in.center_distances, in.center_count, c, center)) return s;
+    double corners[8];
+    for (int k = 0; k < 8; ++k) {
+      flo_int3 q = c;
+      for (int a = 0; a < 3; ++a)
+        if (int s = Offset(q, a, (k >> a) & 1, q)) return s;
+      if (int s = Lookup(in.corner_distances, in.corner_count, q, corners[k])) return s;
+    }
+    double low = 0, high = 0;
+    if (int s = CellVolumes(in, c, corners, low, high)) return s;
+    if (low > 0 && high > 0 && center < 0) return FLO_FAILED_PRECONDITION;
+    const double unclamped = low + high;
+    if (!std::isfinite(unclamped) || unclamped < 0 ||
+        unclamped / full_volume > 1 + 64 * std::numeric_limits<double>::epsilon())
+      return FLO_OUT_OF_RANGE;
+    const double volume = std::min(full_volume, unclamped);
+    if (!NormalOrZero(volume)) return FLO_OUT_OF_RANGE;
+    if (center >= 0 || volume == 0) continue;
+    flo_frozen_cell fc;
+    std::memset(&fc, 0, sizeof fc);
+    fc.cell_index = c;
+    fc.fluid_volume = volume;
+    fc.center_distance = center;
+    cells.push_back(fc);
+    for (int a = 0; a < 3; ++a)
+      for (int side = 0; side < 2; ++side) {
+        flo_int3 l{};
+        if (int s = Offset(c, a, side, l)) return s;
+        uint64_t slot = 0;
+        if (int s = flo_face_slot(in.blocks, in.block_count, flo_face_index{uint8_t(a), l}, &slot)) return s;
+        flo_frozen_face f;
+        std::memset(&f, 0, sizeof f);
+        f.face_index.axis = uint8_t(a);
+        f.face_index.lattice = l;
+        f.face_slot = slot;
+        faces.push_back(f);
+      }
+  }
+  std::sort(faces.begin(), faces.end(),
+            [](const flo_frozen_face& a, const flo_frozen_face& b) { return a.face_slot < b.face_slot; });
+  faces.erase(std::unique(faces.begin(), faces.end(),
+                          [](const flo_frozen_face& a, const flo_frozen_face& b) { return a.face_slot == b.face_slot; }),
+              faces.end());
+  if (in.max_faces && faces.size() > in.max_faces) return FLO_RESOURCE_EXHAUSTED;
+  for (flo_frozen_face& f : faces) {
+    if (int s = FaceAreas(in, f.face_index.lattice, f.face_index.axis, f.open_area, f.blocked_area)) return s;
+    f.wall_normal_velocity = obstacle.velocity[f.face_index.axis];
+  }
+  // Cells on an open face whose other side is air (a centre sample >= 0) need curvature.
+  std::vector<uint8_t> needs(cells.size(), 0);
+  for (const flo_frozen_face& f : faces) {
+    if (f.open_area == 0 || in.surface_tension == 0) continue;
+    const flo_int3 head = f.face_index.lattice;
+    flo_int3 tail{};
+    if (int s = Offset(head, f.face_index.axis, -1, tail)) return s;
+    const uint64_t hc = FindCell(cells, head), tc = FindCell(cells, tail);
+    const bool has_head = hc != cells.size(), has_tail = tc != cells.size();
+    if (has_head && has_tail) continue;
+    double air = 0;
+    if (int s = Lookup(in.center_distances, in.center_count, has_head ? tail : head, air)) return s;
+    if (air >= 0) needs[has_head ? hc : tc] = 1;
+  }
+  const bool capillary = std::any_of(needs.begin(), needs.end(), [](uint8_t x) { return x != 0; });
+  if (in.has_exact_curvature) {
+    if (capillary && in.exact_curvature != 0 && !std::isnormal(in.surface_tension * in.exact_curvature))
+      return FLO_OUT_OF_RANGE;
+    for (size_t i = 0; i < cells.size(); ++i)
+      if (needs[i]) cells[i].curvature = in.exact_curvature;
+  } else {
+    for (size_t i = 0; i < cells.size(); ++i) {
+      if (!needs[i]) continue;
+      double d[27] = {};
+      for (int z = -1; z <= 1; ++z)
+        for (int y = -1; y <= 1; ++y)
+          for (int x = -1; x <= 1; ++x) {
+            if (x != 0 && y != 0 && z != 0) continue;
+            flo_int3 q = cells[i].cell_index;
+            const int o[3] = {x, y, z};
+            for (int a = 0; a < 3; ++a)
+              if (int s = Offset(q, a, o[a], q)) return s;
+            if (int s = Lookup(in.center_distances, in.center_count, q, d[(z + 1) * 9 + (y + 1) * 3 + x + 1]))
+              return s;
+          }
+      if (int s = Curvature(d, h, cells[i].curvature)) return s;
+    }
+  }
+  double bound = std::numeric_limits<double>::infinity();
+  if (capillary) {
+    constexpr double kPi = 3.141592653589793238462643383279502884;
+    const double square = in.liquid_density * full_volume / (4 * kPi * in.surface_tension);
+    if (!std::isnormal(square)) return FLO_OUT_OF_RANGE;
+    bound = std::sqrt(square);
+    if (in.step_duration > bound) return FLO_FAILED_PRECONDITION;
+  }
+  if (!cells.empty()) std::memcpy(out_cells, cells.data(), cells.size() * sizeof(flo_frozen_cell));
+  if (!faces.empty()) std::memcpy(out_faces, faces.data(), faces.size() * sizeof(flo_frozen_face));
+  *out_cell_count = cells.size();
+  *out_face_count = faces.size();
+  *out_capillary_step_bound = bound;
+  return FLO_OK;
+}

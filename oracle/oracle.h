@@ -228,6 +228,41 @@ int flo_build_pressure_links(const flo_frozen_view* view, const flo_pressure_fie
                              flo_pressure_cell* out_cells, flo_pressure_face* out_faces,
                              uint64_t* out_lower_theta_count);
 
+// Frozen-geometry inputs (LiquidBoundaryGeometryViewInputs3D, native/include/fluidity/host/
+// liquid_boundary_workspace_3d.h:18-30) for the absent, half-space and slab obstacles
+// (liquid_boundary_geometry_3d.h:30-43). Corner samples use the centre-sample record.
+enum { FLO_OBSTACLE_ABSENT = 0, FLO_OBSTACLE_HALFSPACE = 1, FLO_OBSTACLE_SLAB = 2 };
+struct flo_obstacle {
+  uint32_t kind, pad;
+  flo_vec3 unit_normal;
+  double coordinate;  // half-space: start plane coordinate; slab: start centre coordinate
+  double half_width;  // slab only
+  flo_vec3 wall_velocity;
+};  // 72 B
+struct flo_freeze_inputs {
+  const flo_int3* blocks;
+  uint64_t block_count;
+  flo_geometry geometry;
+  const flo_int3* cell_indices;
+  uint64_t cell_count;
+  const flo_center_distance* center_distances;
+  uint64_t center_count;
+  const flo_center_distance* corner_distances;
+  uint64_t corner_count;
+  flo_obstacle obstacle;
+  double step_duration, liquid_density, surface_tension;
+  double exact_curvature;
+  uint8_t has_exact_curvature, pad[7];
+  uint64_t max_cells;  // ResourceExhausted above this many candidates (0: no limit)
+  uint64_t max_faces;  // ResourceExhausted above this many frozen faces (0: no limit)
+};
+// FreezeCore (liquid_boundary_geometry_3d.cc:765-1068) with the scalar geometry queries of
+// base/liquid_boundary_geometry_math_3d.h (kCell, kFace, kCurvature, kCapillary). out_cells holds
+// cell_count records and out_faces 6 * cell_count; the counts and the capillary step bound are
+// written only on success.
+int flo_freeze_geometry(const flo_freeze_inputs* inputs, flo_frozen_cell* out_cells, uint64_t* out_cell_count,
+                        flo_frozen_face* out_faces, uint64_t* out_face_count, double* out_capillary_step_bound);
+
 // ---- particle update (liquid_substep_3d_cpu.cc:239-405) ----
 struct flo_constituent { uint64_t substance_id; double volume_fraction; };
 // LiquidParticleState3D (native/include/fluidity/host/liquid_state_3d.h:100-120), 328 B.

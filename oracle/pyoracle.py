@@ -382,6 +382,76 @@ def build_pressure_links(problem):
     return s, cells, faces, lower.value
 
 
+OBSTACLE_ABSENT, OBSTACLE_HALFSPACE, OBSTACLE_SLAB = 0, 1, 2
+
+
+class ObstacleC(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("pad", C.c_uint32), ("unit_normal", C.c_double * 3), ("coordinate", C.c_double),
+                ("half_width", C.c_double), ("wall_velocity", C.c_double * 3)]
+
+
+class FreezeInputsC(C.Structure):
+    _fields_ = [("blocks", C.c_void_p), ("block_count", C.c_uint64), ("origin", C.c_double * 3),
+                ("cell_width", C.c_double), ("cell_indices", C.c_void_p), ("cell_count", C.c_uint64),
+                ("center_distances", C.c_void_p), ("center_count", C.c_uint64), ("corner_distances", C.c_void_p),
+                ("corner_count", C.c_uint64), ("obstacle", ObstacleC), ("step_duration", C.c_double),
+                ("liquid_density", C.c_double), ("surface_tension", C.c_double), ("exact_curvature", C.c_double),
+                ("has_exact_curvature", C.c_uint8), ("pad", C.c_uint8 * 7), ("max_cells", C.c_uint64),
+                ("max_faces", C.c_uint64)]
+
+
+assert C.sizeof(ObstacleC) == 72
+
+
+def halfspace(normal, plane, velocity=(0.0, 0.0, 0.0)):
+    return (OBSTACLE_HALFSPACE, tuple(normal), plane, 0.0, tuple(velocity))
+
+
+def slab(normal, center, half_width, velocity=(0.0, 0.0, 0.0)):
+    return (OBSTACLE_SLAB, tuple(normal), center, half_width, tuple(velocity))
+
+
+class FreezeProblem:
+    """Freeze inputs as numpy arrays (the C structure points into them). obstacle: None, or a tuple
+    (kind, unit normal, coordinate, half width, wall velocity) as made by halfspace() or slab()."""
+
+    def __init__(self, geom, blocks, candidates, centers, corners, obstacle=None, step=0.01, density=1000.0,
+                 sigma=0.0, exact_curvature=None, max_cells=0, max_faces=0):
+        self.geom, self.blocks = geom, np.ascontiguousarray(blocks)
+        self.candidates, self.centers, self.corners = (np.ascontiguousarray(a) for a in (candidates, centers, corners))
+        self.obstacle = obstacle
+        self.step, self.density, self.sigma, self.exact_curvature = step, density, sigma, exact_curvature
+        self.max_cells, self.max_faces = max_cells, max_faces
+
+    def c_obstacle(self):
+        if self.obstacle is None:
+            return ObstacleC()
+        kind, n, c, w, v = self.obstacle
+        return ObstacleC(kind, 0, (C.c_double * 3)(*n), c, w, (C.c_double * 3)(*v))
+
+    def c_inputs(self):
+        o = self.geom[0]["world_origin"]
+        k = self.exact_curvature
+        return FreezeInputsC(_p(self.blocks), len(self.blocks), (C.c_double * 3)(o["x"], o["y"], o["z"]),
+                             self.geom[0]["cell_width"], _p(self.candidates), len(self.candidates), _p(self.centers),
+                             len(self.centers), _p(self.corners), len(self.corners), self.c_obstacle(), self.step,
+                             self.density, self.sigma, 0.0 if k is None else k, 0 if k is None else 1,
+                             (C.c_uint8 * 7)(), self.max_cells, self.max_faces)
+
+
+def freeze_geometry(problem):
+    """flo_freeze_geometry: (status, frozen cells, frozen faces, capillary step bound)."""
+    n = len(problem.candidates)
+    cells = np.zeros(n, FROZEN_CELL)
+    faces = np.zeros(6 * n, FROZEN_FACE)
+    nc, nf, bound = C.c_uint64(), C.c_uint64(), C.c_double()
+    inputs = problem.c_inputs()
+    s = lib().flo_freeze_geometry(C.byref(inputs), _p(cells), C.byref(nc), _p(faces), C.byref(nf), C.byref(bound))
+    if s != OK:
+        return s, None, None, None
+    return s, cells[:nc.value].copy(), faces[:nf.value].copy(), bound.value
+
+
 class TopologyHistoryC(C.Structure):
     _fields_ = [("origin", C.c_double * 3), ("cell_width", C.c_double), ("epoch", C.c_uint64),
                 ("blocks", C.c_void_p), ("block_count", C.c_uint64), ("cells", C.c_void_p), ("cell_count", C.c_uint64),
