@@ -273,6 +273,39 @@ struct LinkBuffers {
   }
 };
 
+struct FreezeBuffers {
+  fl_int3* blocks = nullptr;
+  fl_int3* candidates = nullptr;
+  fl_center_distance* centers = nullptr;
+  fl_center_distance* corners = nullptr;
+  uint64_t* keep = nullptr;
+  fl_frozen_cell* staged = nullptr;
+  uint64_t* slots = nullptr;
+  unsigned long long* words = nullptr;
+  uint64_t* word_counts = nullptr;
+  uint64_t* face_slots = nullptr;
+  fl_frozen_cell* cells = nullptr;
+  fl_frozen_face* faces = nullptr;
+  uint8_t* needs = nullptr;
+  uint64_t* counts = nullptr;
+  uint64_t cap[14] = {};
+  void release() {
+    void* ptrs[] = {blocks, candidates, centers, corners, keep, staged, slots, words, word_counts, face_slots,
+                    cells, faces, needs, counts};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    *this = FreezeBuffers{};
+  }
+};
+
+void destroy_freeze(fl_ctx* ctx) {
+  auto* b = static_cast<FreezeBuffers*>(ctx->freeze);
+  if (!b) return;
+  b->release();
+  delete b;
+  ctx->freeze = nullptr;
+}
+
 void destroy_links(fl_ctx* ctx) {
   auto* b = static_cast<LinkBuffers*>(ctx->links);
   if (!b) return;
@@ -431,6 +464,7 @@ void fl_destroy(fl_ctx* ctx) {
   destroy_topology(ctx);
   destroy_update(ctx);
   destroy_links(ctx);
+  destroy_freeze(ctx);
   release(ctx);
   for (hipEvent_t e : ctx->stage)
     if (e) (void)hipEventDestroy(e);
@@ -808,6 +842,133 @@ int fl_build_pressure_links(fl_ctx* ctx, const fl_frozen_view* view, const fl_pr
   copy_ms += c.stop();
   *out_lower_theta_count = ctx->host_words[0];
   fill_measurement(ctx, measurement, kernel_ms, copy_ms, 8);
+  return FL_OK;
+}
+
+int fl_freeze_geometry(fl_ctx* ctx, const fl_freeze_inputs* inputs, fl_frozen_cell* out_cells, uint64_t* out_cell_count,
+                       fl_frozen_face* out_faces, uint64_t* out_face_count, double* out_capillary_step_bound,
+                       fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  if (!inputs || !out_cell_count || !out_face_count || !out_capillary_step_bound) return FL_INVALID_ARGUMENT;
+  const fl_freeze_inputs& I = *inputs;
+  if ((I.block_count && !I.blocks) || (I.cell_count && (!I.cell_indices || !out_cells || !out_faces)) ||
+      (I.center_count && !I.center_distances) || (I.corner_count && !I.corner_distances))
+    return FL_INVALID_ARGUMENT;
+  FL_TRY(check_geometry(&I.geometry));
+  FL_TRY(check_blocks(I.blocks, I.block_count));
+  // FreezeCore's scalar checks in its order (liquid_boundary_geometry_3d.cc:772-796), with ValidateObstacle
+  // (:132-175).
+  const auto positive_finite = [](double v) { return std::isfinite(v) && v > 0; };
+  if (!positive_finite(I.step_duration) || !positive_finite(I.liquid_density) || !std::isfinite(I.surface_tension) ||
+      I.surface_tension < 0)
+    return FL_INVALID_ARGUMENT;
+  if (I.has_exact_curvature && !std::isfinite(I.exact_curvature)) return FL_INVALID_ARGUMENT;
+  if (I.max_cells && I.cell_count > I.max_cells) return FL_RESOURCE_EXHAUSTED;
+  const fl_obstacle& o = I.obstacle;
+  if (o.kind == FL_OBSTACLE_HALFSPACE || o.kind == FL_OBSTACLE_SLAB) {
+    double low = o.coordinate, high = o.coordinate;
+    if (o.kind == FL_OBSTACLE_SLAB) {
+      if (!positive_finite(o.half_width)) return FL_INVALID_ARGUMENT;
+      low = o.coordinate - o.half_width;
+      high = o.coordinate + o.half_width;
+      if (!std::isfinite(o.coordinate)) return FL_INVALID_ARGUMENT;
+      if (!std::isfinite(low) || !std::isfinite(high) || low >= high) return FL_OUT_OF_RANGE;
+    }
+    const double n[3] = {o.unit_normal.x, o.unit_normal.y, o.unit_normal.z};
+    const double v[3] = {o.wall_velocity.x, o.wall_velocity.y, o.wall_velocity.z};
+    for (int a = 0; a < 3; ++a)
+      if (!std::isfinite(n[a]) || !std::isfinite(v[a])) return FL_INVALID_ARGUMENT;
+    if (!std::isfinite(low) || std::abs(n[0] * n[0] + n[1] * n[1] + n[2] * n[2] - 1.0) > 1e-12)
+      return FL_INVALID_ARGUMENT;
+  } else if (o.kind != FL_OBSTACLE_ABSENT) {
+    return FL_INVALID_ARGUMENT;
+  }
+  const double h = I.geometry.cell_width, full_area = h * h, full_volume = full_area * h;
+  if (!std::isnormal(full_area) || !std::isnormal(full_volume) || !std::isnormal(1 / I.liquid_density))
+    return FL_OUT_OF_RANGE;
+  // The device scans (candidates and bitmap words) share the context's scan partials.
+  const uint64_t nwords = 3 * I.block_count;
+  const uint64_t partials = std::max<uint64_t>(std::max<uint64_t>(64, ctx->cap.lookup_blocks) / 128 + 4096, 8192);
+  for (uint64_t n : {I.cell_count, nwords})
+    if (2 * (n / 1024 + 2) + 16 > partials) return FL_RESOURCE_EXHAUSTED;
+  (void)hipSetDevice(ctx->device);
+  if (!ctx->freeze) ctx->freeze = new FreezeBuffers();
+  FreezeBuffers& B = *static_cast<FreezeBuffers*>(ctx->freeze);
+  const uint64_t nc_in = I.cell_count;
+  FL_TRY(grow(&B.blocks, &B.cap[0], I.block_count));
+  FL_TRY(grow(&B.candidates, &B.cap[1], nc_in));
+  FL_TRY(grow(&B.centers, &B.cap[2], I.center_count));
+  FL_TRY(grow(&B.corners, &B.cap[3], I.corner_count));
+  FL_TRY(grow(&B.keep, &B.cap[4], 2 * nc_in));
+  FL_TRY(grow(&B.staged, &B.cap[5], nc_in));
+  FL_TRY(grow(&B.slots, &B.cap[6], 6 * nc_in));
+  FL_TRY(grow(&B.words, &B.cap[7], nwords));
+  FL_TRY(grow(&B.word_counts, &B.cap[8], 2 * nwords));
+  FL_TRY(grow(&B.face_slots, &B.cap[9], 6 * nc_in));
+  FL_TRY(grow(&B.cells, &B.cap[10], nc_in));
+  FL_TRY(grow(&B.faces, &B.cap[11], 6 * nc_in));
+  FL_TRY(grow(&B.needs, &B.cap[12], nc_in));
+  FL_TRY(grow(&B.counts, &B.cap[13], 3));
+  hipStream_t s = ctx->stream;
+  double copy_ms = 0;
+  const auto up = [&](void* dst, const void* src, size_t bytes) -> int {
+    if (bytes) FL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return FL_OK;
+  };
+  {
+    Timer c(ctx);
+    FL_TRY(up(B.blocks, I.blocks, I.block_count * sizeof(fl_int3)));
+    FL_TRY(up(B.candidates, I.cell_indices, nc_in * sizeof(fl_int3)));
+    FL_TRY(up(B.centers, I.center_distances, I.center_count * sizeof(fl_center_distance)));
+    FL_TRY(up(B.corners, I.corner_distances, I.corner_count * sizeof(fl_center_distance)));
+    copy_ms += c.stop();
+  }
+  Timer t(ctx);
+  const FreezeDevice d{B.blocks, I.block_count, I.geometry.world_origin, h, full_volume, B.candidates, nc_in,
+                       B.centers, I.center_count, B.corners, I.corner_count, o, B.keep, B.keep + nc_in, B.staged,
+                       B.slots, B.words, nwords, B.word_counts, B.word_counts + nwords, B.face_slots, B.cells,
+                       B.faces, B.needs, B.counts};
+  const auto read_counts = [&]() -> int {
+    FL_HIP(hipMemcpyAsync(ctx->host_words, B.counts, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    FL_HIP(hipStreamSynchronize(s));
+    return FL_OK;
+  };
+  FL_TRY(reset_error(ctx));
+  FL_TRY(freeze_cells_device(ctx, d));
+  FL_TRY(first_error_code(ctx));
+  FL_TRY(read_counts());
+  const uint64_t nc = nc_in ? ctx->host_words[0] : 0, nf = nc_in ? ctx->host_words[1] : 0;
+  if (I.max_faces && nf > I.max_faces) return FL_RESOURCE_EXHAUSTED;
+  FL_TRY(freeze_faces_device(ctx, d, nc, nf, I.surface_tension != 0));
+  FL_TRY(first_error_code(ctx));
+  FL_TRY(read_counts());
+  const bool capillary = ctx->host_words[2] != 0;
+  if (I.has_exact_curvature) {
+    if (capillary && I.exact_curvature != 0 && !std::isnormal(I.surface_tension * I.exact_curvature))
+      return FL_OUT_OF_RANGE;
+    FL_TRY(freeze_curvature_device(ctx, d, nc, true, I.exact_curvature));
+  } else {
+    FL_TRY(freeze_curvature_device(ctx, d, nc, false, 0.0));
+    FL_TRY(first_error_code(ctx));
+  }
+  double bound = std::numeric_limits<double>::infinity();
+  if (capillary) {
+    constexpr double kPi = 3.141592653589793238462643383279502884;
+    const double square = I.liquid_density * full_volume / (4 * kPi * I.surface_tension);
+    if (!std::isnormal(square)) return FL_OUT_OF_RANGE;
+    bound = std::sqrt(square);
+    if (I.step_duration > bound) return FL_FAILED_PRECONDITION;
+  }
+  const double kernel_ms = t.stop();
+  Timer c(ctx);
+  if (nc) FL_HIP(hipMemcpyAsync(out_cells, B.cells, nc * sizeof(fl_frozen_cell), hipMemcpyDeviceToHost, s));
+  if (nf) FL_HIP(hipMemcpyAsync(out_faces, B.faces, nf * sizeof(fl_frozen_face), hipMemcpyDeviceToHost, s));
+  FL_HIP(hipStreamSynchronize(s));
+  copy_ms += c.stop();
+  *out_cell_count = nc;
+  *out_face_count = nf;
+  *out_capillary_step_bound = bound;
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, 13);
   return FL_OK;
 }
 

@@ -163,6 +163,7 @@ struct fl_ctx {
   void* update = nullptr;
   // pressure-link buffers (owned by capi.hip, allocated on first use)
   void* links = nullptr;
+  void* freeze = nullptr;
 };
 
 namespace fl {
@@ -266,6 +267,40 @@ struct LinkDevice {
   const fl_internal_interface* interfaces;
   uint64_t ninterfaces;
 };
+// Frozen-geometry device arrays: inputs, per-candidate staging, the face-slot bitmap and the outputs.
+// counts: kept cells, unique faces, and whether any cell needs curvature.
+struct FreezeDevice {
+  const fl_int3* blocks;
+  uint64_t nb;
+  fl_vec3 origin;
+  double h, full_volume;
+  const fl_int3* candidates;
+  uint64_t ncand;
+  const fl_center_distance* centers;
+  uint64_t ncenters;
+  const fl_center_distance* corners;
+  uint64_t ncorners;
+  fl_obstacle obstacle;
+  uint64_t* keep;        // ncand
+  uint64_t* keep_scan;   // ncand
+  fl_frozen_cell* staged;  // ncand
+  uint64_t* slots;       // 6 * ncand
+  unsigned long long* words;  // nwords = 3 * nb
+  uint64_t nwords;
+  uint64_t* word_counts;  // nwords
+  uint64_t* word_scan;    // nwords
+  uint64_t* face_slots;   // 6 * ncand
+  fl_frozen_cell* cells;  // ncand
+  fl_frozen_face* faces;  // 6 * ncand
+  uint8_t* needs;         // ncand
+  uint64_t* counts;       // 3
+};
+// Phases 0-2 (sample and candidate checks, kCell, compaction, unique face slots).
+int freeze_cells_device(fl_ctx* ctx, const FreezeDevice& d);
+// Phases 3-4 over nc kept cells and nf faces (kFace; curvature needs when the surface tension is not zero).
+int freeze_faces_device(fl_ctx* ctx, const FreezeDevice& d, uint64_t nc, uint64_t nf, bool needs_curvature);
+// Phase 5: the exact curvature, or kCurvature, on cells that need it.
+int freeze_curvature_device(fl_ctx* ctx, const FreezeDevice& d, uint64_t nc, bool exact, double curvature);
 // validate_only: the view's record checks (ValidateFrozenView); otherwise the field checks and the
 // links (LinksCore). Failures go to ctx->err; *d_lower (device) counts raised theta values.
 int build_links_device(fl_ctx* ctx, const LinkDevice& d, fl_pressure_cell* out_cells, fl_pressure_face* out_faces,

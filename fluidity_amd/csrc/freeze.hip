@@ -1,0 +1,605 @@
+// Frozen boundary geometry on the device (FreezeCore, native/src/liquid_boundary_geometry_3d.cc:765-1068)
+// for the absent, half-space and slab obstacles: per candidate cell the liquid volume on each side of
+// the obstacle from six body-diagonal tetrahedra clipped by the piecewise-linear liquid distance and
+// the obstacle planes; per incident face the open and blocked area of its two triangles; the centred
+// curvature of cells on an open liquid-air face (base/liquid_boundary_geometry_math_3d.h:160-417,
+// 625-848).
+//
+// Each sequential loop of the reference is one kernel over its records; a failure writes
+// min(phase, ordinal, code) into the context's error word. Phases: 0 centre samples, 1 corner samples,
+// 2 candidate cells, 3 faces, 4 curvature needs, 5 curvature. Kept cells are compacted in candidate
+// order by a scan; their faces are made unique and sorted by slot through a bitmap over the grid's face
+// slots.
+//
+// Clipping keeps fixed-capacity polygons in registers and private memory. A liquid plane and one
+// obstacle plane cut a tetrahedron into at most six faces of at most five vertices, with at most ten
+// crossings before duplicates are removed; the capacities below hold these with room to spare, and an
+// overflow gives a NaN measure (OutOfRange), as the reference's bounded lists do.
+#include "keys.h"
+
+namespace fl {
+namespace {
+
+constexpr int kVerts = 8;     // vertices per polygon
+constexpr int kCross = 16;    // crossings collected by one polyhedron cut
+constexpr int kFaces = 8;     // polygons per polyhedron
+constexpr double kNaN = __builtin_nan("");
+
+__device__ inline bool normal_or_zero(double v) { return v == 0.0 || (finite(v) && fabs(v) >= 2.2250738585072014e-308); }
+// std::max / std::min of two values: the first unless the second compares greater (smaller).
+__device__ inline double max_std(double a, double b) { return a < b ? b : a; }
+__device__ inline double min_std(double a, double b) { return b < a ? b : a; }
+
+struct V3 {
+  double c[3];
+};
+__device__ inline V3 add(V3 a, V3 b) { return {{a.c[0] + b.c[0], a.c[1] + b.c[1], a.c[2] + b.c[2]}}; }
+__device__ inline V3 sub(V3 a, V3 b) { return {{a.c[0] - b.c[0], a.c[1] - b.c[1], a.c[2] - b.c[2]}}; }
+__device__ inline V3 scale(V3 a, double f) { return {{a.c[0] * f, a.c[1] * f, a.c[2] * f}}; }
+__device__ inline double dot(V3 a, V3 b) { return a.c[0] * b.c[0] + a.c[1] * b.c[1] + a.c[2] * b.c[2]; }
+__device__ inline V3 cross(V3 a, V3 b) {
+  return {{a.c[1] * b.c[2] - a.c[2] * b.c[1], a.c[2] * b.c[0] - a.c[0] * b.c[2], a.c[0] * b.c[1] - a.c[1] * b.c[0]}};
+}
+__device__ inline bool finite3(V3 a) { return finite(a.c[0]) && finite(a.c[1]) && finite(a.c[2]); }
+__device__ inline bool same(V3 a, V3 b) { return a.c[0] == b.c[0] && a.c[1] == b.c[1] && a.c[2] == b.c[2]; }
+
+// libstdc++'s three-argument std::hypot: the largest magnitude times the root of the scaled squares.
+__device__ inline double hypot3(double x, double y, double z) {
+  x = fabs(x);
+  y = fabs(y);
+  z = fabs(z);
+  const double a = x < y ? (y < z ? z : y) : (x < z ? z : x);
+  if (a == 0) return 0;
+  return a * sqrt((x / a) * (x / a) + (y / a) * (y / a) + (z / a) * (z / a));
+}
+
+struct Vtx {
+  V3 p;        // cell-width coordinates
+  double phi;  // interpolated liquid distance
+};
+template <int N>
+struct Poly {
+  Vtx v[N];
+  int n;
+  bool overflow;
+  __device__ void clear() {
+    n = 0;
+    overflow = false;
+  }
+  __device__ void push(const Vtx& x) {
+    if (n == N) {
+      overflow = true;
+      return;
+    }
+    v[n++] = x;
+  }
+};
+using Polygon = Poly<kVerts>;
+struct Solid {
+  Polygon f[kFaces];
+  int n;
+  bool overflow;
+};
+struct Plane {
+  V3 normal;
+  double coordinate;
+  bool liquid;
+};
+
+__device__ inline double distance(const Vtx& x, const Plane& p) {
+  return p.liquid ? x.phi : dot(p.normal, x.p) - p.coordinate;
+}
+
+// ClipPolygon: keeps distance <= 0; each crossing takes both weights computed directly.
+template <int N>
+__device__ void clip(const Polygon& in, const Plane& plane, Polygon& out, Poly<N>* crossings) {
+  out.clear();
+  if (in.overflow) {
+    out.overflow = true;
+    return;
+  }
+  if (in.n == 0) return;
+  Vtx prev = in.v[in.n - 1];
+  double dp = distance(prev, plane);
+  for (int i = 0; i < in.n; ++i) {
+    const Vtx cur = in.v[i];
+    const double dc = distance(cur, plane);
+    if ((dp <= 0) != (dc <= 0)) {
+      const double m = max_std(fabs(dp), fabs(dc));
+      const double ps = dp / m, cs = dc / m;
+      const double diff = ps - cs;
+      const double wp = -cs / diff, wc = ps / diff;
+      const Vtx x{add(scale(prev.p, wp), scale(cur.p, wc)), wp * prev.phi + wc * cur.phi};
+      out.push(x);
+      if (crossings) crossings->push(x);
+    }
+    if (dc <= 0) out.push(cur);
+    prev = cur;
+    dp = dc;
+  }
+}
+
+// UnitDirection: binary-exponent normalisation, then division by the length.
+__device__ inline V3 unit_direction(V3 d) {
+  const double largest = max_std(max_std(fabs(d.c[0]), fabs(d.c[1])), fabs(d.c[2]));
+  if (largest == 0) return {{0, 0, 0}};
+  int e = 0;
+  (void)frexp(largest, &e);
+  for (double& c : d.c) c = ldexp(c, -e);
+  return scale(d, 1 / hypot3(d.c[0], d.c[1], d.c[2]));
+}
+
+// ClipPolyhedron: clips every face and closes the cut with the distinct crossings ordered by angle
+// about their centroid (insertion sort on atan2, as the reference).
+__device__ void clip_solid(const Solid& in, const Plane& plane, Solid& out) {
+  out.n = 0;
+  out.overflow = false;
+  if (in.overflow) {
+    out.overflow = true;
+    return;
+  }
+  Poly<kCross> crossings;
+  crossings.clear();
+  for (int k = 0; k < in.n; ++k) {
+    Polygon tmp;
+    clip(in.f[k], plane, tmp, &crossings);
+    if (tmp.overflow || crossings.overflow) {
+      out.overflow = true;
+      return;
+    }
+    if (tmp.n >= 3) {
+      if (out.n == kFaces) out.overflow = true;
+      else out.f[out.n++] = tmp;
+    }
+  }
+  Polygon cap;
+  cap.clear();
+  for (int k = 0; k < crossings.n; ++k) {
+    bool seen = false;
+    for (int j = 0; j < cap.n; ++j) seen = seen || same(cap.v[j].p, crossings.v[k].p);
+    if (!seen) cap.push(crossings.v[k]);
+  }
+  if (cap.n < 3) return;
+  V3 center{{0, 0, 0}};
+  for (int k = 0; k < cap.n; ++k) center = add(center, scale(cap.v[k].p, 1.0 / cap.n));
+  const V3 tangent = unit_direction(sub(cap.v[0].p, center));
+  V3 normal{{0, 0, 0}};
+  for (int k = 1; k < cap.n; ++k) {
+    normal = unit_direction(cross(tangent, unit_direction(sub(cap.v[k].p, center))));
+    if (hypot3(normal.c[0], normal.c[1], normal.c[2]) > 0) break;
+  }
+  const V3 bitangent = cross(normal, tangent);
+  const double tl = hypot3(tangent.c[0], tangent.c[1], tangent.c[2]);
+  const double bl = hypot3(bitangent.c[0], bitangent.c[1], bitangent.c[2]);
+  if (!(tl > 0 && bl > 0)) return;
+  const V3 ut = scale(tangent, 1 / tl), ub = scale(bitangent, 1 / bl);
+  double angle[kVerts];
+  for (int k = 0; k < cap.n; ++k) {
+    const V3 d = sub(cap.v[k].p, center);
+    angle[k] = atan2(dot(d, ub), dot(d, ut));
+  }
+  for (int k = 1; k < cap.n; ++k) {
+    const Vtx x = cap.v[k];
+    const double a = angle[k];
+    int j = k;
+    while (j > 0 && a < angle[j - 1]) {
+      cap.v[j] = cap.v[j - 1];
+      angle[j] = angle[j - 1];
+      --j;
+    }
+    cap.v[j] = x;
+    angle[j] = a;
+  }
+  if (out.n == kFaces) {
+    out.overflow = true;
+    return;
+  }
+  out.f[out.n++] = cap;
+}
+
+// NormalizeCoordinateRow over n edges.
+template <int N>
+__device__ inline int normalize_row(V3 (&e)[N], int axis) {
+  double largest = 0;
+  for (int k = 0; k < N; ++k) largest = max_std(largest, fabs(e[k].c[axis]));
+  if (largest == 0) return 0;
+  int x = 0;
+  (void)frexp(largest, &x);
+  for (int k = 0; k < N; ++k) e[k].c[axis] = ldexp(e[k].c[axis], -x);
+  return x;
+}
+
+// PhysicalMeasure: restores the local exponent and h^dimension; NaN for an unrepresentable nonzero.
+__device__ inline double physical_measure(double det, int local_exponent, double h, int dimension) {
+  if (det == 0) return 0;
+  int we = 0;
+  const double wm = frexp(h, &we);
+  double m = det;
+  for (int a = 0; a < dimension; ++a) m *= wm;
+  const double r = ldexp(m, local_exponent + dimension * we);
+  if (!finite(r) || r == 0) return kNaN;
+  return r;
+}
+
+__device__ double volume(const Solid& s, double h) {
+  if (s.overflow) return kNaN;
+  V3 apex{{0, 0, 0}};
+  int count = 0;
+  for (int k = 0; k < s.n; ++k) {
+    if (s.f[k].overflow) return kNaN;
+    for (int j = 0; j < s.f[k].n; ++j) {
+      apex = add(apex, s.f[k].v[j].p);
+      ++count;
+    }
+  }
+  if (count == 0) return 0;
+  apex = scale(apex, 1.0 / count);
+  double total = 0;
+  for (int k = 0; k < s.n; ++k) {
+    const Polygon& g = s.f[k];
+    for (int j = 1; j + 1 < g.n; ++j) {
+      V3 e[3] = {sub(g.v[0].p, apex), sub(g.v[j].p, apex), sub(g.v[j + 1].p, apex)};
+      int ex = 0;
+      for (int a = 0; a < 3; ++a) ex += normalize_row(e, a);
+      const double scaled = fabs(dot(e[0], cross(e[1], e[2]))) / 6;
+      total += physical_measure(scaled, ex, h, 3);
+    }
+  }
+  return total;
+}
+
+__device__ double area(const Polygon& g, double h) {
+  if (g.overflow) return kNaN;
+  double total = 0;
+  for (int j = 1; j + 1 < g.n; ++j) {
+    const V3 e0 = sub(g.v[j].p, g.v[0].p), e1 = sub(g.v[j + 1].p, g.v[0].p);
+    double w[3];
+    for (int a = 0; a < 3; ++a) {
+      V3 n[2] = {e0, e1};
+      const int f = (a + 1) % 3, s = (a + 2) % 3;
+      const int fe = normalize_row(n, f);
+      const int se = normalize_row(n, s);
+      const double c = (n[0].c[f] * n[1].c[s] - n[0].c[s] * n[1].c[f]) / 2;
+      w[a] = physical_measure(c, fe + se, h, 2);
+    }
+    total += hypot3(w[0], w[1], w[2]);
+  }
+  return total;
+}
+
+// ComputeOpenFaceArea: zero unless a vertex lies strictly on the open side.
+__device__ double open_area(const Polygon& g, const Plane& plane, double h) {
+  if (g.overflow) return kNaN;
+  bool any = false;
+  for (int j = 0; j < g.n; ++j) any = any || distance(g.v[j], plane) < 0;
+  if (!any) return 0;
+  Polygon c;
+  clip<kCross>(g, plane, c, nullptr);
+  return area(c, h);
+}
+
+struct Planes {
+  bool present, slab;
+  V3 normal;
+  double low, high;
+};
+
+// LocalObstacle: the planes in the unit frame of the element whose low corner is lattice index c.
+__device__ inline int local_planes(const FreezeDevice& d, fl_int3 c, Planes& out) {
+  const fl_obstacle& o = d.obstacle;
+  out.present = o.kind == FL_OBSTACLE_HALFSPACE || o.kind == FL_OBSTACLE_SLAB;
+  out.slab = o.kind == FL_OBSTACLE_SLAB;
+  out.normal = {{o.unit_normal.x, o.unit_normal.y, o.unit_normal.z}};
+  out.low = o.coordinate - (out.slab ? o.half_width : 0);
+  out.high = o.coordinate + (out.slab ? o.half_width : 0);
+  if (!out.present) return kOk;
+  const V3 anchor{{d.origin.x + d.h * c.x, d.origin.y + d.h * c.y, d.origin.z + d.h * c.z}};
+  const double projection = dot(out.normal, anchor);
+  out.low = (out.low - projection) / d.h;
+  out.high = (out.high - projection) / d.h;
+  if (!finite3(anchor) || !finite(out.low) || !finite(out.high) || (out.slab && out.low >= out.high))
+    return kOutOfRange;
+  return kOk;
+}
+
+__constant__ int kTetrahedra[6][4] = {{0, 1, 3, 7}, {0, 1, 5, 7}, {0, 2, 3, 7},
+                                      {0, 2, 6, 7}, {0, 4, 5, 7}, {0, 4, 6, 7}};
+
+__device__ inline double lookup(const fl_center_distance* s, uint64_t n, fl_int3 k, bool& found) {
+  const uint64_t i = find_sorted(s, n, k, [](const fl_center_distance& x) { return x.cell_index; });
+  found = i != n;
+  return found ? s[i].signed_distance : 0.0;
+}
+
+__global__ void k_check_samples(const fl_center_distance* s, uint64_t n, uint64_t phase, uint64_t* err) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  if (!finite(s[i].signed_distance) || (i > 0 && !key_less(s[i - 1].cell_index, s[i].cell_index)))
+    fail(err, phase, i, kInvalidArgument);
+}
+
+// One lane per candidate: sample lookups, the kCell query and the consume step. keep[i] is 1 for a
+// liquid cell of positive volume, whose record goes to staged[i] and whose six face slots go to
+// slots[6 i ...].
+__global__ void __launch_bounds__(64) k_freeze_cells(FreezeDevice d, uint64_t* err) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= d.ncand) return;
+  d.keep[i] = 0;
+  const fl_int3 c = d.candidates[i];
+  if (i > 0 && !key_less(d.candidates[i - 1], c)) return fail(err, 2, i, kInvalidArgument);
+  bool found = false;
+  const double center = lookup(d.centers, d.ncenters, c, found);
+  if (!found) return fail(err, 2, i, kFailedPrecondition);
+  Vtx x[8];
+  for (int k = 0; k < 8; ++k) {
+    fl_int3 q = c;
+    for (int a = 0; a < 3; ++a)
+      if (!offset(q, a, (k >> a) & 1, q)) return fail(err, 2, i, kOutOfRange);
+    x[k].phi = lookup(d.corners, d.ncorners, q, found);
+    if (!found) return fail(err, 2, i, kFailedPrecondition);
+    for (int a = 0; a < 3; ++a) x[k].p.c[a] = (k >> a) & 1;
+  }
+  Planes P;
+  if (int code = local_planes(d, c, P)) return fail(err, 2, i, code);
+  double low = 0, high = 0;
+  Solid a_solid, liquid;  // the tetrahedron, then each obstacle part of its liquid
+  Solid& tet = a_solid;
+  Solid& part = a_solid;
+  for (int t = 0; t < 6; ++t) {
+    bool negative = false;
+    for (int k = 0; k < 4; ++k) negative = negative || x[kTetrahedra[t][k]].phi < 0;
+    if (!negative) continue;
+    const Vtx a = x[kTetrahedra[t][0]], b = x[kTetrahedra[t][1]], cc = x[kTetrahedra[t][2]], dd = x[kTetrahedra[t][3]];
+    const Vtx tri[4][3] = {{a, b, cc}, {a, b, dd}, {a, cc, dd}, {b, cc, dd}};
+    tet.n = 4;
+    tet.overflow = false;
+    for (int k = 0; k < 4; ++k) {
+      tet.f[k].clear();
+      for (int j = 0; j < 3; ++j) tet.f[k].push(tri[k][j]);
+    }
+    clip_solid(tet, Plane{{{0, 0, 0}}, 0, true}, liquid);
+    if (!P.present) {
+      high += volume(liquid, d.h);
+    } else {
+      clip_solid(liquid, Plane{scale(P.normal, -1), -P.high, false}, part);
+      high += volume(part, d.h);
+      if (P.slab) {
+        clip_solid(liquid, Plane{P.normal, P.low, false}, part);
+        low += volume(part, d.h);
+      }
+    }
+  }
+  if (low > 0 && high > 0 && center < 0) return fail(err, 2, i, kFailedPrecondition);
+  const double unclamped = low + high;
+  if (!finite(unclamped) || unclamped < 0 || unclamped / d.full_volume > 1 + 64 * 2.220446049250313e-16)
+    return fail(err, 2, i, kOutOfRange);
+  const double vol = min_std(d.full_volume, unclamped);
+  if (!normal_or_zero(vol)) return fail(err, 2, i, kOutOfRange);
+  if (center >= 0 || vol == 0) return;
+  for (int a = 0; a < 3; ++a)
+    for (int side = 0; side < 2; ++side) {
+      fl_int3 l;
+      if (!offset(c, a, side, l)) return fail(err, 2, i, kOutOfRange);
+      uint64_t slot = 0;
+      if (int code = sorted_face_slot(d.blocks, d.nb, a, l, slot)) return fail(err, 2, i, code);
+      d.slots[6 * i + 2 * a + side] = slot;
+    }
+  fl_frozen_cell r{};
+  r.cell_index = c;
+  r.fluid_volume = vol;
+  r.center_distance = center;
+  r.curvature = 0;
+  d.staged[i] = r;
+  d.keep[i] = 1;
+}
+
+// Kept cells to their scanned positions; their face slots into the slot bitmap; the kept count.
+__global__ void k_compact_cells(FreezeDevice d) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= d.ncand) return;
+  const uint64_t pos = d.keep_scan[i];
+  if (i == d.ncand - 1) d.counts[0] = pos + d.keep[i];
+  if (!d.keep[i]) return;
+  d.cells[pos] = d.staged[i];
+  d.needs[pos] = 0;
+  for (int k = 0; k < 6; ++k) {
+    const uint64_t s = d.slots[6 * i + k];
+    atomicOr(d.words + s / 64, 1ull << (s % 64));
+  }
+}
+
+__global__ void k_clear(unsigned long long* w, uint64_t n) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i < n) w[i] = 0;
+}
+
+__global__ void k_word_popcounts(const unsigned long long* w, uint64_t n, uint64_t* counts) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i < n) counts[i] = __popcll(w[i]);
+}
+
+// Every set slot in increasing order, at its word's scanned position; the face count.
+__global__ void k_compact_slots(FreezeDevice d) {
+  const uint64_t w = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (w >= d.nwords) return;
+  unsigned long long bits = d.words[w];
+  uint64_t pos = d.word_scan[w];
+  if (w == d.nwords - 1) d.counts[1] = pos + __popcll(bits);
+  while (bits) {
+    const int b = __ffsll(bits) - 1;
+    bits &= bits - 1;
+    d.face_slots[pos++] = w * 64 + uint64_t(b);
+  }
+}
+
+// kFace per unique face: open and blocked areas of the two triangles and the wall velocity.
+__global__ void __launch_bounds__(64) k_freeze_faces(FreezeDevice d, uint64_t nf, uint64_t* err) {
+  const uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (j >= nf) return;
+  const uint64_t slot = d.face_slots[j];
+  const fl_int3 b = d.blocks[slot / 192];
+  const uint64_t within = slot % 192, local = within % 64;
+  const int axis = int(within / 64);
+  const fl_int3 l{int32_t(int64_t(b.x) * 4 + int64_t(local % 4)), int32_t(int64_t(b.y) * 4 + int64_t((local / 4) % 4)),
+                  int32_t(int64_t(b.z) * 4 + int64_t(local / 16))};
+  fl_frozen_face r{};
+  r.axis = uint8_t(axis);
+  r.lattice = l;
+  r.face_slot = slot;
+  d.faces[j] = r;  // a closed placeholder until the areas are known, so later phases read a valid record
+  Planes P;
+  if (int code = local_planes(d, l, P)) return fail(err, 3, j, code);
+  const double full_area = d.h * d.h;
+  int tr[2], n = 0;
+  for (int a = 0; a < 3; ++a)
+    if (a != axis) tr[n++] = a;
+  Vtx x[4] = {};
+  x[1].p.c[tr[0]] = 1;
+  x[2].p.c[tr[0]] = 1;
+  x[2].p.c[tr[1]] = 1;
+  x[3].p.c[tr[1]] = 1;
+  double blocked = 0, open = 0;
+  for (int t = 0; t < 2; ++t) {
+    Polygon g;
+    g.clear();
+    g.push(x[0]);
+    g.push(x[t + 1]);
+    g.push(x[t + 2]);
+    if (!P.present) {
+      open += area(g, d.h);
+      continue;
+    }
+    Polygon bl, tmp;
+    clip<kCross>(g, Plane{P.normal, P.high, false}, bl, nullptr);
+    if (P.slab) {
+      clip<kCross>(bl, Plane{scale(P.normal, -1), -P.low, false}, tmp, nullptr);
+      bl = tmp;
+    }
+    blocked += area(bl, d.h);
+    open += open_area(g, Plane{scale(P.normal, -1), -P.high, false}, d.h);
+    if (P.slab) open += open_area(g, Plane{P.normal, P.low, false}, d.h);
+  }
+  r.open_area = min_std(open, full_area);
+  r.blocked_area = full_area - r.open_area;
+  if (!finite(open) || !finite(blocked) || open / full_area > 1 + 64 * 2.220446049250313e-16 ||
+      !normal_or_zero(r.open_area) || !normal_or_zero(r.blocked_area) || (blocked > 0 && r.blocked_area == 0))
+    return fail(err, 3, j, kOutOfRange);
+  const double v[3] = {d.obstacle.wall_velocity.x, d.obstacle.wall_velocity.y, d.obstacle.wall_velocity.z};
+  r.wall_normal_velocity = P.present ? v[axis] : 0.0;
+  d.faces[j] = r;
+}
+
+// Cells on an open face whose other side is air (a centre sample >= 0) need curvature.
+__global__ void k_curvature_needs(FreezeDevice d, uint64_t nc, uint64_t nf, uint64_t* err) {
+  const uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (j >= nf) return;
+  const fl_frozen_face f = d.faces[j];
+  if (f.open_area == 0) return;
+  const fl_int3 head = f.lattice;
+  fl_int3 tail;
+  if (!offset(head, f.axis, -1, tail)) return fail(err, 4, j, kOutOfRange);
+  const auto key = [](const fl_frozen_cell& c) { return c.cell_index; };
+  const uint64_t hc = find_sorted(d.cells, nc, head, key), tc = find_sorted(d.cells, nc, tail, key);
+  const bool has_head = hc != nc, has_tail = tc != nc;
+  if (has_head && has_tail) return;
+  bool found = false;
+  const double air = lookup(d.centers, d.ncenters, has_head ? tail : head, found);
+  if (!found) return fail(err, 4, j, kFailedPrecondition);
+  if (air >= 0) {
+    d.needs[has_head ? hc : tc] = 1;
+    d.counts[2] = 1;
+  }
+}
+
+__global__ void k_exact_curvature(FreezeDevice d, uint64_t nc, double curvature) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i < nc && d.needs[i]) d.cells[i].curvature = curvature;
+}
+
+// kCurvature from the 19-point centred halo (x fastest, index 13 the centre).
+__global__ void k_curvature(FreezeDevice d, uint64_t nc, uint64_t* err) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= nc || !d.needs[i]) return;
+  double s[27] = {};
+  const fl_int3 c = d.cells[i].cell_index;
+  for (int z = -1; z <= 1; ++z)
+    for (int y = -1; y <= 1; ++y)
+      for (int x = -1; x <= 1; ++x) {
+        if (x != 0 && y != 0 && z != 0) continue;
+        fl_int3 q = c;
+        const int o[3] = {x, y, z};
+        for (int a = 0; a < 3; ++a)
+          if (!offset(q, a, o[a], q)) return fail(err, 5, i, kOutOfRange);
+        bool found = false;
+        s[(z + 1) * 9 + (y + 1) * 3 + x + 1] = lookup(d.centers, d.ncenters, q, found);
+        if (!found) return fail(err, 5, i, kFailedPrecondition);
+      }
+  const double h = d.h;
+  const int stride[3] = {1, 3, 9};
+  V3 grad{{0, 0, 0}};
+  V3 H[3] = {{{0, 0, 0}}, {{0, 0, 0}}, {{0, 0, 0}}};
+  for (int a = 0; a < 3; ++a) {
+    const double lo = s[13 - stride[a]], hi = s[13 + stride[a]];
+    grad.c[a] = (hi - lo) / (2 * h);
+    H[a].c[a] = ((hi - s[13]) + (lo - s[13])) / h / h;
+    for (int o = 0; o < a; ++o) {
+      double mixed = 0;
+      for (int f = -1; f <= 1; f += 2)
+        for (int g = -1; g <= 1; g += 2) mixed += double(f * g) * (s[13 + f * stride[a] + g * stride[o]] / 4);
+      H[a].c[o] = mixed / h / h;
+      H[o].c[a] = H[a].c[o];
+    }
+  }
+  const double norm = hypot3(grad.c[0], grad.c[1], grad.c[2]);
+  if (!finite3(grad) || !finite(norm)) return fail(err, 5, i, kOutOfRange);
+  if (norm < 1e-8) return fail(err, 5, i, kFailedPrecondition);
+  const V3 n = scale(grad, 1 / norm);
+  double k = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (!finite3(H[a])) return fail(err, 5, i, kOutOfRange);
+    k += (H[a].c[a] - n.c[a] * dot(H[a], n)) / norm;
+  }
+  if (!finite(k)) return fail(err, 5, i, kOutOfRange);
+  d.cells[i].curvature = k;
+}
+
+}  // namespace
+
+int freeze_cells_device(fl_ctx* ctx, const FreezeDevice& d) {
+  hipStream_t s = ctx->stream;
+  if (d.ncenters) k_check_samples<<<grid_for(d.ncenters), kThreads, 0, s>>>(d.centers, d.ncenters, 0, ctx->err);
+  if (d.ncorners) k_check_samples<<<grid_for(d.ncorners), kThreads, 0, s>>>(d.corners, d.ncorners, 1, ctx->err);
+  k_clear<<<grid_for(d.nwords), kThreads, 0, s>>>(d.words, d.nwords);
+  k_clear<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long*>(d.counts), 3);
+  if (d.ncand) {
+    k_freeze_cells<<<grid_for(d.ncand, 64), 64, 0, s>>>(d, ctx->err);
+    FL_TRY(exclusive_scan_u64(ctx, d.keep, d.keep_scan, d.ncand));
+    k_compact_cells<<<grid_for(d.ncand), kThreads, 0, s>>>(d);
+  }
+  if (d.nwords) {
+    k_word_popcounts<<<grid_for(d.nwords), kThreads, 0, s>>>(d.words, d.nwords, d.word_counts);
+    FL_TRY(exclusive_scan_u64(ctx, d.word_counts, d.word_scan, d.nwords));
+    k_compact_slots<<<grid_for(d.nwords), kThreads, 0, s>>>(d);
+  }
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+int freeze_faces_device(fl_ctx* ctx, const FreezeDevice& d, uint64_t nc, uint64_t nf, bool needs_curvature) {
+  hipStream_t s = ctx->stream;
+  if (nf) k_freeze_faces<<<grid_for(nf, 64), 64, 0, s>>>(d, nf, ctx->err);
+  if (nf && needs_curvature) k_curvature_needs<<<grid_for(nf), kThreads, 0, s>>>(d, nc, nf, ctx->err);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+int freeze_curvature_device(fl_ctx* ctx, const FreezeDevice& d, uint64_t nc, bool exact, double curvature) {
+  hipStream_t s = ctx->stream;
+  if (!nc) return FL_OK;
+  if (exact) k_exact_curvature<<<grid_for(nc), kThreads, 0, s>>>(d, nc, curvature);
+  else k_curvature<<<grid_for(nc), kThreads, 0, s>>>(d, nc, ctx->err);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+}  // namespace fl

@@ -7,37 +7,13 @@
 // failing record of the first failing loop is reported. Lookups (blocks, cells, centre samples, faces
 // by slot, interfaces by face ordinal, air pressures) are binary searches over the sorted inputs.
 // One lane per face builds its link.
-#include "context.h"
+#include "keys.h"
 
 namespace fl {
 namespace {
 
-__device__ inline bool key_less(fl_int3 a, fl_int3 b) {
-  if (a.z != b.z) return a.z < b.z;
-  if (a.y != b.y) return a.y < b.y;
-  return a.x < b.x;
-}
-__device__ inline bool key_eq(fl_int3 a, fl_int3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
 __device__ inline bool positive_finite(double v) { return finite(v) && v > 0; }
 __device__ inline bool is_normal(double v) { return finite(v) && fabs(v) >= 2.2250738585072014e-308; }
-
-constexpr int kPhaseShift = 40;
-__device__ inline void fail(uint64_t* err, uint64_t phase, uint64_t ordinal, int code) {
-  atomicMin(reinterpret_cast<unsigned long long*>(err),
-            static_cast<unsigned long long>(error_word((phase << kPhaseShift) | ordinal, code)));
-}
-
-// Index of the record with key k among n sorted records, or n.
-template <typename R, typename Key>
-__device__ inline uint64_t find_sorted(const R* r, uint64_t n, fl_int3 k, Key key) {
-  uint64_t lo = 0, hi = n;
-  while (lo < hi) {
-    const uint64_t mid = lo + (hi - lo) / 2;
-    if (key_less(key(r[mid]), k)) lo = mid + 1;
-    else hi = mid;
-  }
-  return (lo < n && key_eq(key(r[lo]), k)) ? lo : n;
-}
 
 struct View {
   const fl_int3* blocks;
@@ -55,26 +31,8 @@ __device__ inline uint64_t find_cell(const View& v, fl_int3 k) {
   return find_sorted(v.cells, v.ncells, k, [](const fl_frozen_cell& c) { return c.cell_index; });
 }
 
-// OffsetCellIndex: false outside the int32 lattice (OutOfRange).
-__device__ inline bool offset(fl_int3 c, int axis, int amount, fl_int3& out) {
-  int64_t w[3] = {c.x, c.y, c.z};
-  w[axis] += amount;
-  if (w[axis] < kI32Min || w[axis] > kI32Max) return false;
-  out = fl_int3{int32_t(w[0]), int32_t(w[1]), int32_t(w[2])};
-  return true;
-}
-
-// SparseMacGridView3D::FaceSlot: InvalidArgument for an axis above 2, FailedPrecondition when the
-// owner block is inactive.
 __device__ inline int face_slot(const View& v, int axis, fl_int3 l, uint64_t& slot) {
-  if (axis > 2) return kInvalidArgument;
-  const fl_int3 o{int32_t(floor_div4(l.x)), int32_t(floor_div4(l.y)), int32_t(floor_div4(l.z))};
-  const uint64_t b = find_sorted(v.blocks, v.nb, o, [](const fl_int3& x) { return x; });
-  if (b == v.nb) return kFailedPrecondition;
-  const uint64_t lx = uint64_t(int64_t(l.x) - int64_t(o.x) * 4), ly = uint64_t(int64_t(l.y) - int64_t(o.y) * 4),
-                 lz = uint64_t(int64_t(l.z) - int64_t(o.z) * 4);
-  slot = b * 192 + uint64_t(axis) * 64 + lz * 16 + ly * 4 + lx;
-  return kOk;
+  return sorted_face_slot(v.blocks, v.nb, axis, l, slot);
 }
 
 __global__ void k_check_centers(View v, uint64_t* err) {

@@ -49,7 +49,7 @@ class StepReport(C.Structure):
         return np.frombuffer(bytes(self.pressure), dtype=R.PRESSURE_DIAGNOSTICS)[0]
 
 
-EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_prepare_topology", "fl_update_particles", "fl_build_pressure_links", "fl_p2g", "fl_p2g_quanta", "fl_p2g_compatible", "fl_g2p_boundary",
+EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_prepare_topology", "fl_update_particles", "fl_build_pressure_links", "fl_freeze_geometry", "fl_p2g", "fl_p2g_quanta", "fl_p2g_compatible", "fl_g2p_boundary",
            "fl_normalize_forces", "fl_apply_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
            "fl_scene_download", "fl_probe_copy", "fl_learn_param_count", "fl_learn_setup", "fl_learn_set_params", "fl_learn_get_params",
            "fl_learn_forward", "fl_learn_infer", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect", "fl_learn_eval",
@@ -97,6 +97,22 @@ def topology_capacities(particles=1 << 20, blocks=1 << 16, cells=1 << 20, faces=
     c = np.zeros(1, R.TOPOLOGY_CAPACITIES)
     c[0] = (particles, blocks, cells, faces)
     return c
+
+
+class Obstacle(C.Structure):
+    """fl_obstacle: kind 0 absent, 1 half-space, 2 slab."""
+    _fields_ = [("kind", C.c_uint32), ("pad", C.c_uint32), ("unit_normal", C.c_double * 3), ("coordinate", C.c_double),
+                ("half_width", C.c_double), ("wall_velocity", C.c_double * 3)]
+
+
+class FreezeInputs(C.Structure):
+    _fields_ = [("blocks", C.c_void_p), ("block_count", C.c_uint64), ("origin", C.c_double * 3),
+                ("cell_width", C.c_double), ("cell_indices", C.c_void_p), ("cell_count", C.c_uint64),
+                ("center_distances", C.c_void_p), ("center_count", C.c_uint64), ("corner_distances", C.c_void_p),
+                ("corner_count", C.c_uint64), ("obstacle", Obstacle), ("step_duration", C.c_double),
+                ("liquid_density", C.c_double), ("surface_tension", C.c_double), ("exact_curvature", C.c_double),
+                ("has_exact_curvature", C.c_uint8), ("pad", C.c_uint8 * 7), ("max_cells", C.c_uint64),
+                ("max_faces", C.c_uint64)]
 
 
 class FrozenView(C.Structure):
@@ -218,6 +234,32 @@ class Context:
         s = lib().fl_build_pressure_links(self.handle, C.byref(view), C.byref(flds), _p(out_c), _p(out_f),
                                           C.byref(lower), C.byref(self.last))
         return s, out_c, out_f, lower.value
+
+    def freeze_geometry(self, p):
+        """fl_freeze_geometry on a problem with numpy record arrays (geom, blocks, candidates, centers,
+        corners), an obstacle tuple (kind, unit normal, coordinate, half width, wall velocity) or None, and
+        scalars (step, density, sigma, exact_curvature or None, max_cells, max_faces):
+        (status, frozen cells, frozen faces, capillary step bound)."""
+        o = p.geom[0]["world_origin"]
+        blocks, cands, centers, corners = (np.ascontiguousarray(a) for a in (p.blocks, p.candidates, p.centers,
+                                                                               p.corners))
+        obs = Obstacle()
+        if p.obstacle is not None:
+            kind, n, c, w, v = p.obstacle
+            obs = Obstacle(kind, 0, (C.c_double * 3)(*n), c, w, (C.c_double * 3)(*v))
+        k = p.exact_curvature
+        inputs = FreezeInputs(_p(blocks), len(blocks), (C.c_double * 3)(o["x"], o["y"], o["z"]),
+                              p.geom[0]["cell_width"], _p(cands), len(cands), _p(centers), len(centers), _p(corners),
+                              len(corners), obs, p.step, p.density, p.sigma, 0.0 if k is None else k,
+                              0 if k is None else 1, (C.c_uint8 * 7)(), p.max_cells, p.max_faces)
+        cells = np.zeros(len(cands), R.FROZEN_CELL)
+        faces = np.zeros(6 * len(cands), R.FROZEN_FACE)
+        nc, nf, bound = C.c_uint64(), C.c_uint64(), C.c_double()
+        s = lib().fl_freeze_geometry(self.handle, C.byref(inputs), _p(cells), C.byref(nc), _p(faces), C.byref(nf),
+                                     C.byref(bound), C.byref(self.last))
+        if s != OK:
+            return s, None, None, None
+        return s, cells[:nc.value].copy(), faces[:nf.value].copy(), bound.value
 
     def update_particles(self, options, gathered, distances, particles):
         """fl_update_particles on a copy of the particles: (status, particles after the call)."""

@@ -205,6 +205,41 @@ typedef struct fl_frozen_view {
   double surface_tension;
   double capillary_step_bound;
 } fl_frozen_view;
+/* The frozen-geometry obstacles (native/include/fluidity/host/liquid_boundary_geometry_3d.h:30-43): absent,
+ * TranslatingHalfspace3D (solid where unit_normal . x <= coordinate) and TranslatingSlab3D (solid where
+ * |unit_normal . x - coordinate| <= half_width). */
+enum { FL_OBSTACLE_ABSENT = 0, FL_OBSTACLE_HALFSPACE = 1, FL_OBSTACLE_SLAB = 2 };
+typedef struct fl_obstacle {
+  uint32_t kind;
+  uint32_t pad;
+  fl_vec3 unit_normal;
+  double coordinate;  /* half-space: start plane coordinate; slab: start centre coordinate */
+  double half_width;  /* slab only */
+  fl_vec3 wall_velocity;
+} fl_obstacle; /* 72 B */
+/* LiquidBoundaryGeometryViewInputs3D (native/include/fluidity/host/liquid_boundary_workspace_3d.h:18-30)
+ * without a sampled solid; corner samples (LiquidCornerDistance3D) use the centre-sample record.
+ * max_cells and max_faces give the reference's capacity refusals at its points (0: none). */
+typedef struct fl_freeze_inputs {
+  const fl_int3* blocks;
+  uint64_t block_count;
+  fl_geometry geometry;
+  const fl_int3* cell_indices;
+  uint64_t cell_count;
+  const fl_center_distance* center_distances;
+  uint64_t center_count;
+  const fl_center_distance* corner_distances;
+  uint64_t corner_count;
+  fl_obstacle obstacle;
+  double step_duration;
+  double liquid_density;
+  double surface_tension;
+  double exact_curvature;
+  uint8_t has_exact_curvature;
+  uint8_t pad[7];
+  uint64_t max_cells;
+  uint64_t max_faces;
+} fl_freeze_inputs;
 /* LiquidPressureFields3D (liquid_boundary_geometry_3d.h:170-185); an empty span has count 0. */
 typedef struct fl_pressure_fields {
   const fl_cell_fields* cells;
@@ -412,6 +447,21 @@ int fl_prepare_topology(fl_ctx* ctx, const fl_geometry* geometry, const fl_topol
 int fl_build_pressure_links(fl_ctx* ctx, const fl_frozen_view* view, const fl_pressure_fields* fields,
                             fl_pressure_cell* out_cells, fl_pressure_face* out_faces, uint64_t* out_lower_theta_count,
                             fl_measurement* measurement);
+
+/* ---- frozen boundary geometry ---- */
+/* FreezeCore (native/src/liquid_boundary_geometry_3d.cc:765-1068) for the absent, half-space and slab
+ * obstacles: per candidate cell (strictly sorted) the liquid volume from six body-diagonal tetrahedra
+ * clipped by the piecewise-linear corner distances and the obstacle planes; liquid cells (centre
+ * distance < 0, volume > 0) become frozen cells in candidate order, their faces frozen faces sorted by
+ * slot with open / blocked areas and the obstacle's wall velocity; cells on an open face to air get the
+ * exact curvature or the centred one of the 19-point centre halo; the capillary step bound
+ * sqrt(rho h^3 / (4 pi sigma)) when such a cell exists (infinity otherwise), with FailedPrecondition when
+ * the step exceeds it. out_cells holds cell_count records, out_faces 6 * cell_count. The first failure of
+ * the reference's sequential checks decides the status; outputs are written only on success. The frozen
+ * records and centre samples form the fl_frozen_view that fl_build_pressure_links takes. */
+int fl_freeze_geometry(fl_ctx* ctx, const fl_freeze_inputs* inputs, fl_frozen_cell* out_cells, uint64_t* out_cell_count,
+                       fl_frozen_face* out_faces, uint64_t* out_face_count, double* out_capillary_step_bound,
+                       fl_measurement* measurement);
 
 /* ---- particle update ---- */
 /* The liquid particle update of the CPU substep backend without an obstacle or regular-cell transport
