@@ -229,6 +229,80 @@ def update_measurement(ctx, n=1 << 21, h=1.0 / 256, dt=2e-4):
             "particles_per_s": n / (m.kernel_ms * 1e-3)}
 
 
+class _Problem:
+    """Attribute bag for the freeze and link calls of the boundary measurement."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def boundary_measurement(ctx, scene, cells, dt, cpu_candidates=1 << 17):
+    """fl_freeze_geometry then fl_build_pressure_links on the workload's liquid: centre and corner samples of
+    its analytic signed distance (two spheres of radius 0.2 over the pool y < 0.1) on the whole grid, the
+    centres outside the domain taken as air, every cell with a negative centre distance a candidate (the
+    workload's liquid cells up to rounding at the surface), no obstacle and a water surface tension; the
+    oracle freeze of the first cpu_candidates candidates, on one host thread, is its CPU rate."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    n, h = scene.n, scene.h
+    c = np.arange(-1, n + 1)
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+
+    def phi(px, py, pz):
+        d1 = np.sqrt((px - 0.3) ** 2 + (py - 0.5) ** 2 + (pz - 0.5) ** 2) - 0.2
+        d2 = np.sqrt((px - 0.7) ** 2 + (py - 0.5) ** 2 + (pz - 0.5) ** 2) - 0.2
+        return np.minimum(np.minimum(d1, d2), py - 0.1)
+
+    samples = []
+    for offset in (0.5, 0.0):
+        a = np.zeros(x.size, R.CENTER_DISTANCE)
+        for k, v in zip("xyz", (x, y, z)):
+            a["cell_index"][k] = v.ravel()
+        a["signed_distance"] = phi((x + offset) * h, (y + offset) * h, (z + offset) * h).ravel()
+        samples.append(a)
+    centers, corners = samples
+    outside = ((x < 0) | (x >= n) | (y < 0) | (y >= n) | (z < 0) | (z >= n)).ravel()
+    centers["signed_distance"][outside] = h
+    cand = np.ascontiguousarray(centers["cell_index"][centers["signed_distance"] < 0])
+    del x, y, z, outside
+    nb = n // 4
+    bz, by, bx = np.meshgrid(np.arange(-1, nb + 1), np.arange(-1, nb + 1), np.arange(-1, nb + 1), indexing="ij")
+    blocks = np.zeros(bx.size, R.INT3)
+    for k, v in zip("xyz", (bx, by, bz)):
+        blocks[k] = v.ravel()
+    p = _Problem(geom=scene.geometry_record(), blocks=blocks, candidates=cand, centers=centers, corners=corners,
+                 obstacle=None, step=dt, density=scene.rho, sigma=0.072, exact_curvature=None, max_cells=0,
+                 max_faces=0)
+    for _ in range(2):  # the first call also pays the first touch of its device buffers
+        s, fc, ff, bound = ctx.freeze_geometry(p)
+        assert s == H.OK, s
+    fm = (ctx.last.kernel_ms, ctx.last.copy_ms)
+    rng = np.random.default_rng(7)
+    fields = np.zeros(len(fc), R.CELL_FIELDS)
+    fields["bulk_modulus"] = np.inf
+    lp = _Problem(geom=p.geom, blocks=blocks, centers=centers, cells=fc, faces=ff, step=dt, density=scene.rho,
+                  sigma=0.072, fields=fields, predicted=rng.normal(0.0, 1.0, len(ff)), ambient=0.0,
+                  wetted=np.zeros(0, R.OPTIONAL_FLUX), air=np.zeros(0, R.AIR_PRESSURE),
+                  interfaces=np.zeros(0, R.INTERNAL_INTERFACE))
+    s, _, _, lower = ctx.build_pressure_links(lp)
+    assert s == H.OK, s
+    lm = (ctx.last.kernel_ms, ctx.last.copy_ms)
+    log(f"boundary: {len(cand):,} candidates -> {len(fc):,} cells, {len(ff):,} faces; freeze {fm[0]:.2f} ms, links "
+        f"{lm[0]:.2f} ms on the device (+{fm[1] + lm[1]:.1f} ms of host copies)")
+    k = min(cpu_candidates, len(cand))
+    sample = O.FreezeProblem(p.geom.view(O.GEOMETRY), blocks.view(O.INT3), cand[:k].view(O.INT3),
+                             centers.view(O.CENTER_DISTANCE), corners.view(O.CENTER_DISTANCE), step=dt,
+                             density=scene.rho, sigma=0.072)
+    t0 = time.perf_counter()
+    s, _, _, _ = O.freeze_geometry(sample)
+    cpu_s = time.perf_counter() - t0
+    assert s == O.OK, s
+    return {"candidates": len(cand), "cells": len(fc), "faces": len(ff), "capillary_step_bound": bound,
+            "lower_theta_count": lower, "freeze_kernel_ms": fm[0], "freeze_copy_ms": fm[1], "links_kernel_ms": lm[0],
+            "links_copy_ms": lm[1], "candidates_per_s": len(cand) / (fm[0] * 1e-3),
+            "cpu_freeze_candidates_per_s": k / cpu_s, "cpu_sample": f"first {k} candidates, 1 thread"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -353,6 +427,7 @@ def main():
     if not args.no_topology:  # last: span-based calls reuse the resident scene's buffers
         out["topology"] = topology_measurement(ctx, scene, cells, dt)
         out["particle_update"] = update_measurement(ctx)
+        out["boundary"] = boundary_measurement(ctx, scene, cells, dt)
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -4,14 +4,19 @@
 // native/src/liquid_boundary_geometry_3d.cc:1077-1383).
 //
 // Same records, checks, status codes and publication rule as the reference's view-based builder,
-// without its bounded workspace caps (4096 cells, 24576 faces). The freeze that produces the view
-// (fluid volumes, apertures, curvature from distance samples) is not part of this layer.
+// without its bounded workspace caps (4096 cells, 24576 faces).
+//
+// The freeze that produces the view (FreezeLiquidBoundaryGeometry3D,
+// native/include/fluidity/host/liquid_boundary_geometry_3d.h:277;
+// native/src/liquid_boundary_geometry_3d.cc:765-1068) runs on the GPU too, for the absent, half-space
+// and slab obstacles; the open tank, floating boxes and sampled solids stay with the reference.
 #pragma once
 
 #include <cstddef>
 #include <cstdint>
 #include <optional>
 #include <span>
+#include <variant>
 #include <vector>
 
 #include "fluidity/host/hip_operator_runtime_3d.h"
@@ -75,6 +80,62 @@ struct LiquidPressureFields3D {
   std::span<const LiquidAirCellPressure3D> air_cell_pressures{};
   std::span<const LiquidInternalInterface3D> internal_interfaces{};
 };
+// A corner sample: corner_index names the cell whose low corner it is.
+struct LiquidCornerDistance3D {
+  CellIndex3D corner_index;
+  double signed_distance;  // Same physical field as the centre samples, m.
+};
+struct TranslatingHalfspace3D {
+  Vec3 unit_normal;               // Outward from solid; solid is n . x <= start_plane_coordinate.
+  double start_plane_coordinate;  // World meters at substep start.
+  Vec3 wall_velocity;             // m/s.
+};
+// Solid is |unit_normal . x - start_center_coordinate| <= half_width.
+struct TranslatingSlab3D {
+  Vec3 unit_normal;
+  double start_center_coordinate;
+  double half_width;  // Positive finite, m.
+  Vec3 wall_velocity;
+};
+using LiquidObstacle3D = std::variant<std::monostate, TranslatingHalfspace3D, TranslatingSlab3D>;
+struct LiquidBoundaryGeometryInputs3D {
+  const SparseMacGrid3D& mac_grid;
+  std::span<const CellIndex3D> cell_indices;  // Candidates, strictly (z,y,x) sorted.
+  std::span<const LiquidCenterDistance3D> center_distances;  // Strictly sorted.
+  std::span<const LiquidCornerDistance3D> corner_distances;  // Strictly sorted.
+  LiquidObstacle3D obstacle;
+  double step_duration;    // s.
+  double liquid_density;   // kg/m³.
+  double surface_tension;  // N/m.
+  std::optional<double> exact_outward_interface_curvature = std::nullopt;  // 1/m.
+};
+// Owns a frozen geometry and its centre samples; view() borrows them and the grid.
+class FrozenLiquidBoundaryGeometry3D {
+ public:
+  const SparseMacGrid3D& mac_grid() const { return mac_grid_; }
+  std::span<const FrozenLiquidCell3D> cells() const { return cells_; }
+  std::span<const FrozenLiquidFace3D> faces() const { return faces_; }
+  std::span<const LiquidCenterDistance3D> center_distances() const { return centers_; }
+  double step_duration() const { return step_duration_; }
+  double liquid_density() const { return liquid_density_; }
+  double surface_tension() const { return surface_tension_; }
+  // +infinity when no liquid cell has an open face to air under positive surface tension.
+  double capillary_step_bound() const { return capillary_step_bound_; }
+  FrozenLiquidBoundaryGeometryView3D view() const {
+    return {mac_grid_.view(), centers_, cells_, faces_, step_duration_, liquid_density_, surface_tension_,
+            capillary_step_bound_};
+  }
+
+ private:
+  friend struct FreezeAccess;
+  explicit FrozenLiquidBoundaryGeometry3D(const SparseMacGrid3D& grid) : mac_grid_(grid) {}
+  SparseMacGrid3D mac_grid_;
+  std::vector<LiquidCenterDistance3D> centers_;
+  std::vector<FrozenLiquidCell3D> cells_;
+  std::vector<FrozenLiquidFace3D> faces_;
+  double step_duration_ = 0, liquid_density_ = 0, surface_tension_ = 0, capillary_step_bound_ = 0;
+};
+
 struct LiquidPressureProblem3D {
   std::vector<MacPressureCell3D> cells;
   std::vector<MacPressureFace3D> faces;
@@ -93,5 +154,17 @@ StatusOr<LiquidPressureProblem3D> HipBuildLiquidPressureLinks3D(HipOperatorRunti
                                                                const FrozenLiquidBoundaryGeometryView3D& geometry,
                                                                const LiquidPressureFields3D& fields,
                                                                HipOperatorMeasurement& measurement);
+
+// Liquid cells of the candidates (centre distance < 0, clipped volume > 0) with their incident faces
+// (open / blocked areas against the obstacle, its wall velocity) and the curvature of cells on an open
+// face to air. The reference's caps apply: ResourceExhausted above 4096 candidates or 24576 faces.
+// InvalidArgument for malformed inputs, FailedPrecondition for missing samples or owners, a cell split
+// by a slab, a degenerate curvature gradient or a step above the capillary bound, OutOfRange for
+// unrepresentable measures. Errors publish nothing.
+StatusOr<FrozenLiquidBoundaryGeometry3D> FreezeLiquidBoundaryGeometry3D(const LiquidBoundaryGeometryInputs3D& inputs);
+// The same without the caps (bounded by the runtime's capacities).
+StatusOr<FrozenLiquidBoundaryGeometry3D> HipFreezeLiquidBoundaryGeometry3D(HipOperatorRuntime3D& runtime,
+                                                                          const LiquidBoundaryGeometryInputs3D& inputs,
+                                                                          HipOperatorMeasurement& measurement);
 
 }  // namespace fluidity

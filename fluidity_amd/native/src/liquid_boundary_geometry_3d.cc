@@ -1,5 +1,6 @@
-// Typed pressure links over the C-ABI (fl_build_pressure_links). The records carry std::optional
-// members, so they are converted field by field to the C-ABI records rather than reinterpreted.
+// Frozen geometry and typed pressure links over the C-ABI (fl_freeze_geometry,
+// fl_build_pressure_links). The records carry std::optional members, so they are converted field by
+// field to the C-ABI records rather than reinterpreted.
 #include "fluidity/host/liquid_boundary_geometry_3d.h"
 
 #include <cstring>
@@ -146,6 +147,103 @@ StatusOr<LiquidPressureProblem3D> HipBuildLiquidPressureLinks3D(HipOperatorRunti
   fl_measurement m{};
   LiquidPressureProblem3D out;
   const int code = Build(runtime, geometry, fields, out, &m);
+  if (code != FL_OK) return FromCode(code);
+  std::memcpy(measurement.architecture, m.architecture, sizeof(measurement.architecture));
+  measurement.kernel_ms = m.kernel_ms;
+  measurement.copy_ms = m.copy_ms;
+  measurement.launches = m.launches;
+  return out;
+}
+
+// Stages the inputs as C-ABI records, freezes on the device and converts the frozen records back.
+struct FreezeAccess {
+  static int Freeze(HipOperatorRuntime3D& rt, const LiquidBoundaryGeometryInputs3D& in, std::uint64_t max_cells,
+                    std::uint64_t max_faces, FrozenLiquidBoundaryGeometry3D& out, fl_measurement* m) {
+    std::vector<fl_int3> candidates;
+    candidates.reserve(in.cell_indices.size());
+    for (const auto& c : in.cell_indices) candidates.push_back({c.x, c.y, c.z});
+    std::vector<fl_center_distance> centers, corners;
+    centers.reserve(in.center_distances.size());
+    corners.reserve(in.corner_distances.size());
+    for (const auto& c : in.center_distances)
+      centers.push_back({{c.cell_index.x, c.cell_index.y, c.cell_index.z}, 0, c.signed_distance});
+    for (const auto& c : in.corner_distances)
+      corners.push_back({{c.corner_index.x, c.corner_index.y, c.corner_index.z}, 0, c.signed_distance});
+    fl_freeze_inputs r{};
+    const auto blocks = in.mac_grid.active_block_indices();
+    r.blocks = internal::ToC(blocks);
+    r.block_count = blocks.size();
+    r.geometry = internal::ToC(in.mac_grid.geometry());
+    r.cell_indices = candidates.data();
+    r.cell_count = candidates.size();
+    r.center_distances = centers.data();
+    r.center_count = centers.size();
+    r.corner_distances = corners.data();
+    r.corner_count = corners.size();
+    const auto c3 = [](Vec3 v) { return fl_vec3{v.x, v.y, v.z}; };
+    if (const auto* h = std::get_if<TranslatingHalfspace3D>(&in.obstacle)) {
+      r.obstacle = {FL_OBSTACLE_HALFSPACE, 0, c3(h->unit_normal), h->start_plane_coordinate, 0.0,
+                    c3(h->wall_velocity)};
+    } else if (const auto* b = std::get_if<TranslatingSlab3D>(&in.obstacle)) {
+      r.obstacle = {FL_OBSTACLE_SLAB, 0, c3(b->unit_normal), b->start_center_coordinate, b->half_width,
+                    c3(b->wall_velocity)};
+    }
+    r.step_duration = in.step_duration;
+    r.liquid_density = in.liquid_density;
+    r.surface_tension = in.surface_tension;
+    r.has_exact_curvature = in.exact_outward_interface_curvature.has_value();
+    r.exact_curvature = in.exact_outward_interface_curvature.value_or(0.0);
+    r.max_cells = max_cells;
+    r.max_faces = max_faces;
+    std::vector<fl_frozen_cell> cells(candidates.size());
+    std::vector<fl_frozen_face> faces(6 * candidates.size());
+    std::uint64_t nc = 0, nf = 0;
+    double bound = 0;
+    const int code = fl_freeze_geometry(rt.handle(), &r, cells.data(), &nc, faces.data(), &nf, &bound, m);
+    if (code != FL_OK) return code;
+    out.centers_.assign(in.center_distances.begin(), in.center_distances.end());
+    out.cells_.resize(nc);
+    for (std::size_t i = 0; i < nc; ++i) {
+      const auto& c = cells[i];
+      out.cells_[i] = {{c.cell_index.x, c.cell_index.y, c.cell_index.z}, c.fluid_volume, c.center_distance,
+                       c.curvature};
+    }
+    out.faces_.resize(nf);
+    for (std::size_t i = 0; i < nf; ++i) {
+      const auto& f = faces[i];
+      out.faces_[i] = {{static_cast<FaceNormalAxis>(f.axis), {f.lattice.x, f.lattice.y, f.lattice.z}},
+                       static_cast<std::size_t>(f.face_slot), f.open_area, f.blocked_area, f.wall_normal_velocity};
+    }
+    out.step_duration_ = in.step_duration;
+    out.liquid_density_ = in.liquid_density;
+    out.surface_tension_ = in.surface_tension;
+    out.capillary_step_bound_ = bound;
+    return FL_OK;
+  }
+  static FrozenLiquidBoundaryGeometry3D Empty(const SparseMacGrid3D& grid) { return FrozenLiquidBoundaryGeometry3D(grid); }
+};
+
+StatusOr<FrozenLiquidBoundaryGeometry3D> FreezeLiquidBoundaryGeometry3D(const LiquidBoundaryGeometryInputs3D& inputs) {
+  // The owning freeze refuses more than 4096 candidates before any other check (:1982-1984).
+  if (inputs.cell_indices.size() > 4096) return Status(StatusCode::kResourceExhausted);
+  HipOperatorCapacities3D need;
+  need.blocks = std::max<std::size_t>(inputs.mac_grid.active_block_indices().size(), 1);
+  std::unique_lock<std::mutex> lock;
+  auto rt = SharedHipRuntime3D::Acquire(need, lock);
+  if (!rt.ok()) return rt.status();
+  auto out = FreezeAccess::Empty(inputs.mac_grid);
+  // FreezeCore's candidate and face caps (liquid_boundary_geometry_3d.cc:780-782, 918-920).
+  const int code = FreezeAccess::Freeze(**rt, inputs, 4096, 24576, out, nullptr);
+  if (code != FL_OK) return FromCode(code);
+  return out;
+}
+
+StatusOr<FrozenLiquidBoundaryGeometry3D> HipFreezeLiquidBoundaryGeometry3D(HipOperatorRuntime3D& runtime,
+                                                                          const LiquidBoundaryGeometryInputs3D& inputs,
+                                                                          HipOperatorMeasurement& measurement) {
+  fl_measurement m{};
+  auto out = FreezeAccess::Empty(inputs.mac_grid);
+  const int code = FreezeAccess::Freeze(runtime, inputs, 0, 0, out, &m);
   if (code != FL_OK) return FromCode(code);
   std::memcpy(measurement.architecture, m.architecture, sizeof(measurement.architecture));
   measurement.kernel_ms = m.kernel_ms;

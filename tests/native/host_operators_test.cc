@@ -692,6 +692,97 @@ void PressureLinksUnitCube() {
   EXPECT(BuildLiquidPressureLinks3D(heavy, {fields, velocities, 9.0}).status().code() == StatusCode::kOutOfRange);
 }
 
+// The reference's freeze cases on its 2x2x2-block grid (native/tests/liquid_boundary_geometry_3d_cases.h):
+// centre samples at field(i + 2.5), corner samples at field(i + 2), candidate (-2, -2, -2).
+struct FreezeSamples {
+  std::vector<CellIndex3D> candidates{{-2, -2, -2}};
+  std::vector<LiquidCenterDistance3D> centers;
+  std::vector<LiquidCornerDistance3D> corners;
+};
+template <typename Field>
+FreezeSamples SampleField(Field field) {
+  FreezeSamples s;
+  for (int z = -4; z <= 1; ++z)
+    for (int y = -4; y <= 1; ++y)
+      for (int x = -4; x <= 1; ++x) {
+        s.centers.push_back({{x, y, z}, field(x + 2.5, y + 2.5, z + 2.5)});
+        s.corners.push_back({{x, y, z}, field(x + 2.0, y + 2.0, z + 2.0)});
+      }
+  return s;
+}
+
+void FreezeMatchesReferenceCases() {
+  std::vector<BlockIndex3D> keys;
+  for (int z = -1; z <= 0; ++z)
+    for (int y = -1; y <= 0; ++y)
+      for (int x = -1; x <= 0; ++x) keys.push_back({x, y, z});
+  auto grid = SparseMacGrid3D::Create({{2.0, 2.0, 2.0}, 1.0}, keys, keys.size());
+  EXPECT(grid.ok());
+  const auto liquid = SampleField([](double, double, double) { return -1.0; });
+  // Plane fixture: a half-space at 0.25 along x leaves an open width of 0.75.
+  auto plane = FreezeLiquidBoundaryGeometry3D(
+      {*grid, liquid.candidates, liquid.centers, liquid.corners, TranslatingHalfspace3D{{1, 0, 0}, 0.25, {2, 3, 4}},
+       0.01, 1000.0, 0.0});
+  EXPECT(plane.ok() && plane->cells().size() == 1 && plane->faces().size() == 6);
+  EXPECT(std::abs(plane->cells()[0].fluid_volume - 0.75) <= 2e-14);
+  for (const auto& f : plane->faces()) {
+    EXPECT(f.blocked_area == 1.0 - f.open_area);
+    EXPECT(f.wall_normal_velocity == 2.0 + static_cast<int>(f.face_index.axis));
+    if (f.face_index.axis != FaceNormalAxis::kX) EXPECT(std::abs(f.open_area - 0.75) <= 2e-14);
+  }
+  EXPECT(plane->capillary_step_bound() == std::numeric_limits<double>::infinity());
+  // A thin slab leaves 15/32 of a cell whose liquid lies beyond it; an all-liquid cell is split.
+  const TranslatingSlab3D slab{{1, 0, 0}, 0.5, 1.0 / 32.0, {0, 0, 0}};
+  const auto beyond = SampleField([](double x, double, double) { return x - 0.52; });
+  auto thin = FreezeLiquidBoundaryGeometry3D(
+      {*grid, beyond.candidates, beyond.centers, beyond.corners, slab, 0.01, 1000.0, 0.0});
+  EXPECT(thin.ok() && std::abs(thin->cells()[0].fluid_volume - 15.0 / 32.0) <= 2e-14);
+  EXPECT(FreezeLiquidBoundaryGeometry3D({*grid, liquid.candidates, liquid.centers, liquid.corners, slab, 0.01, 1000.0,
+                                         0.0})
+             .status()
+             .code() == StatusCode::kFailedPrecondition);
+  // The capillary bound of a plane interface.
+  const auto surface = SampleField([](double x, double, double) { return x - 0.75; });
+  auto capillary = FreezeLiquidBoundaryGeometry3D(
+      {*grid, surface.candidates, surface.centers, surface.corners, {}, 0.01, 1000.0, 0.072});
+  EXPECT(capillary.ok() && std::abs(capillary->cells()[0].curvature) <= 1e-14);
+  EXPECT(std::abs(capillary->capillary_step_bound() - std::sqrt(1000.0 / (4.0 * std::acos(-1.0) * 0.072))) <= 1e-13);
+  // More than 4096 candidates are refused first.
+  std::vector<CellIndex3D> many(4097, CellIndex3D{-2, -2, -2});
+  EXPECT(FreezeLiquidBoundaryGeometry3D({*grid, many, liquid.centers, liquid.corners, {}, 0.0, 1000.0, -1.0})
+             .status()
+             .code() == StatusCode::kResourceExhausted);
+
+  // Sealed low face at the minimum signed index: freeze, links at ambient 2 Pa, projection gives 2 Pa
+  // (SealedLowFaceAtMinimumSignedIndexNeedsNoExteriorSample).
+  constexpr int kMin = std::numeric_limits<std::int32_t>::min();
+  const std::vector<BlockIndex3D> edge{{kMin / 4, -1, -1}};
+  auto low = SparseMacGrid3D::Create({{2147483648.0, 2.0, 2.0}, 1.0}, edge, 1);
+  EXPECT(low.ok());
+  auto iso = SampleField([](double, double, double) { return 0.99; });
+  for (auto& c : iso.corners) c.signed_distance = -1.0;
+  for (auto& c : iso.centers)
+    if (c.cell_index.x == -2 && c.cell_index.y == -2 && c.cell_index.z == -2) c.signed_distance = -0.01;
+  std::erase_if(iso.centers, [](const auto& c) { return c.cell_index.x < -2; });
+  std::erase_if(iso.corners, [](const auto& c) { return c.corner_index.x < -2; });
+  for (auto& c : iso.centers) c.cell_index.x = kMin + (c.cell_index.x + 2);
+  for (auto& c : iso.corners) c.corner_index.x = kMin + (c.corner_index.x + 2);
+  iso.candidates[0].x = kMin;
+  auto frozen = FreezeLiquidBoundaryGeometry3D(
+      {*low, iso.candidates, iso.centers, iso.corners, TranslatingHalfspace3D{{1, 0, 0}, 0.25, {0, 0, 0}}, 0.01,
+       1000.0, 0.0});
+  EXPECT(frozen.ok());
+  if (!frozen.ok()) return;
+  const std::vector<LiquidPressureCellFields3D> fields(frozen->cells().size(),
+                                                       {0.0, std::numeric_limits<double>::infinity(), 0.0});
+  const std::vector<double> velocities(frozen->faces().size(), 0.0);
+  auto links = BuildLiquidPressureLinks3D(frozen->view(), {fields, velocities, 2.0});
+  EXPECT(links.ok());
+  if (!links.ok()) return;
+  auto projected = ProjectMacPressure3D({*low, 0.01, links->cells, links->faces});
+  EXPECT(projected.ok() && std::abs(projected->cell_pressures[0] - 2.0) <= 1e-6);
+}
+
 int Run(const char* name, const std::function<void()>& test) {
   std::printf("[ RUN      ] %s\n", name);
   std::fflush(stdout);
@@ -723,6 +814,7 @@ int main(int argc, char** argv) {
     failed += Run("WorkspaceRemapsHistoryAndMatchesOracle", WorkspaceRemapsHistoryAndMatchesOracle);
     failed += Run("BackendUpdateMatchesOracle", BackendUpdateMatchesOracle);
     failed += Run("PressureLinksUnitCube", PressureLinksUnitCube);
+    failed += Run("FreezeMatchesReferenceCases", FreezeMatchesReferenceCases);
     failed += Run("SharedRuntimeGrows", SharedRuntimeGrows);
   }
   std::printf("%d test(s) failed\n", failed);
