@@ -77,6 +77,42 @@ def test_record_offsets_match_the_c_compiler(tmp_path):
             assert int(got[f"{struct}.{cfield}"]) == dtype.fields[name][1], (struct, name)
 
 
+def test_ctypes_structures_match_the_c_compiler(tmp_path):
+    """The binding's argument structures (frozen geometry inputs, obstacle, frozen view, pressure fields)
+    have gcc's size and member offsets."""
+    import ctypes as C
+    import subprocess
+
+    from fluidity_amd import hip as H
+
+    checks = {"fl_freeze_inputs": (H.FreezeInputs, {"origin": "geometry", "obstacle": "obstacle",
+                                                    "step_duration": "step_duration",
+                                                    "has_exact_curvature": "has_exact_curvature",
+                                                    "max_cells": "max_cells", "max_faces": "max_faces"}),
+              "fl_obstacle": (H.Obstacle, {"unit_normal": "unit_normal", "coordinate": "coordinate",
+                                           "half_width": "half_width", "wall_velocity": "wall_velocity"}),
+              "fl_frozen_view": (H.FrozenView, {"origin": "geometry", "center_distances": "center_distances",
+                                                "capillary_step_bound": "capillary_step_bound"}),
+              "fl_pressure_fields": (H.PressureFields, {"ambient_pressure": "ambient_pressure",
+                                                        "interface_count": "interface_count"})}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', f'#include "{ROOT}/include/fluidity_hip.h"', "int main(void) {"]
+    for struct, (_, fields) in checks.items():
+        lines.append(f'  printf("{struct} %zu\\n", sizeof({struct}));')
+        for cfield in fields.values():
+            lines.append(f'  printf("{struct}.{cfield} %zu\\n", offsetof({struct}, {cfield}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "structs.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "structs"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                           check=True).stdout.splitlines())
+    for struct, (cls, fields) in checks.items():
+        assert int(got[struct]) == C.sizeof(cls), struct
+        for name, cfield in fields.items():
+            assert int(got[f"{struct}.{cfield}"]) == getattr(cls, name).offset, (struct, name)
+
+
 def test_create_without_device_fails_cleanly():
     if os.path.exists("/dev/kfd"):
         return
