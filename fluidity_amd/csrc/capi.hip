@@ -288,10 +288,13 @@ struct FreezeBuffers {
   fl_frozen_face* faces = nullptr;
   uint8_t* needs = nullptr;
   uint64_t* counts = nullptr;
-  uint64_t cap[14] = {};
+  double* phi = nullptr;
+  uint64_t* work = nullptr;
+  double* uncut = nullptr;
+  uint64_t cap[17] = {};
   void release() {
     void* ptrs[] = {blocks, candidates, centers, corners, keep, staged, slots, words, word_counts, face_slots,
-                    cells, faces, needs, counts};
+                    cells, faces, needs, counts, phi, work, uncut};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     *this = FreezeBuffers{};
@@ -908,7 +911,10 @@ int fl_freeze_geometry(fl_ctx* ctx, const fl_freeze_inputs* inputs, fl_frozen_ce
   FL_TRY(grow(&B.cells, &B.cap[10], nc_in));
   FL_TRY(grow(&B.faces, &B.cap[11], 6 * nc_in));
   FL_TRY(grow(&B.needs, &B.cap[12], nc_in));
-  FL_TRY(grow(&B.counts, &B.cap[13], 3));
+  FL_TRY(grow(&B.counts, &B.cap[13], 4));
+  FL_TRY(grow(&B.phi, &B.cap[14], 9 * nc_in));
+  FL_TRY(grow(&B.work, &B.cap[15], nc_in));
+  FL_TRY(grow(&B.uncut, &B.cap[16], 1));
   hipStream_t s = ctx->stream;
   double copy_ms = 0;
   const auto up = [&](void* dst, const void* src, size_t bytes) -> int {
@@ -927,7 +933,7 @@ int fl_freeze_geometry(fl_ctx* ctx, const fl_freeze_inputs* inputs, fl_frozen_ce
   const FreezeDevice d{B.blocks, I.block_count, I.geometry.world_origin, h, full_volume, B.candidates, nc_in,
                        B.centers, I.center_count, B.corners, I.corner_count, o, B.keep, B.keep + nc_in, B.staged,
                        B.slots, B.words, nwords, B.word_counts, B.word_counts + nwords, B.face_slots, B.cells,
-                       B.faces, B.needs, B.counts};
+                       B.faces, B.needs, B.counts, B.phi, B.work, B.uncut};
   const auto read_counts = [&]() -> int {
     FL_HIP(hipMemcpyAsync(ctx->host_words, B.counts, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     FL_HIP(hipStreamSynchronize(s));

@@ -268,7 +268,6 @@ struct LinkDevice {
   uint64_t ninterfaces;
 };
 // Frozen-geometry device arrays: inputs, per-candidate staging, the face-slot bitmap and the outputs.
-// counts: kept cells, unique faces, and whether any cell needs curvature.
 struct FreezeDevice {
   const fl_int3* blocks;
   uint64_t nb;
@@ -293,7 +292,10 @@ struct FreezeDevice {
   fl_frozen_cell* cells;  // ncand
   fl_frozen_face* faces;  // 6 * ncand
   uint8_t* needs;         // ncand
-  uint64_t* counts;       // 3
+  uint64_t* counts;       // 4: kept cells, unique faces, curvature needed, listed cut cells
+  double* phi;            // 9 * ncand: corner and centre samples of the cut cells
+  uint64_t* work;         // ncand: the cut cells
+  double* uncut_volume;   // 1
 };
 // Phases 0-2 (sample and candidate checks, kCell, compaction, unique face slots).
 int freeze_cells_device(fl_ctx* ctx, const FreezeDevice& d);

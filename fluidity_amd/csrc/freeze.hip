@@ -7,7 +7,8 @@
 //
 // Each sequential loop of the reference is one kernel over its records; a failure writes
 // min(phase, ordinal, code) into the context's error word. Phases: 0 centre samples, 1 corner samples,
-// 2 candidate cells, 3 faces, 4 curvature needs, 5 curvature. Kept cells are compacted in candidate
+// 2 candidate cells, 3 faces, 4 curvature needs, 5 curvature. Candidate cells that no plane cuts are
+// settled by a light staging kernel; only cut cells are clipped. Kept cells are compacted in candidate
 // order by a scan; their faces are made unique and sorted by slot through a bitmap over the grid's face
 // slots.
 //
@@ -318,30 +319,14 @@ __global__ void k_check_samples(const fl_center_distance* s, uint64_t n, uint64_
     fail(err, phase, i, kInvalidArgument);
 }
 
-// One lane per candidate: sample lookups, the kCell query and the consume step. keep[i] is 1 for a
-// liquid cell of positive volume, whose record goes to staged[i] and whose six face slots go to
-// slots[6 i ...].
-__global__ void __launch_bounds__(64) k_freeze_cells(FreezeDevice d, uint64_t* err) {
-  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  if (i >= d.ncand) return;
-  d.keep[i] = 0;
-  const fl_int3 c = d.candidates[i];
-  if (i > 0 && !key_less(d.candidates[i - 1], c)) return fail(err, 2, i, kInvalidArgument);
-  bool found = false;
-  const double center = lookup(d.centers, d.ncenters, c, found);
-  if (!found) return fail(err, 2, i, kFailedPrecondition);
+// The liquid volume on each side of the obstacle of a cell with corner distances phi (kCell).
+__device__ void cell_volumes(const double (&phi)[8], const Planes& P, double h, double& low, double& high) {
   Vtx x[8];
   for (int k = 0; k < 8; ++k) {
-    fl_int3 q = c;
-    for (int a = 0; a < 3; ++a)
-      if (!offset(q, a, (k >> a) & 1, q)) return fail(err, 2, i, kOutOfRange);
-    x[k].phi = lookup(d.corners, d.ncorners, q, found);
-    if (!found) return fail(err, 2, i, kFailedPrecondition);
+    x[k].phi = phi[k];
     for (int a = 0; a < 3; ++a) x[k].p.c[a] = (k >> a) & 1;
   }
-  Planes P;
-  if (int code = local_planes(d, c, P)) return fail(err, 2, i, code);
-  double low = 0, high = 0;
+  low = high = 0;
   Solid a_solid, liquid;  // the tetrahedron, then each obstacle part of its liquid
   Solid& tet = a_solid;
   Solid& part = a_solid;
@@ -359,16 +344,22 @@ __global__ void __launch_bounds__(64) k_freeze_cells(FreezeDevice d, uint64_t* e
     }
     clip_solid(tet, Plane{{{0, 0, 0}}, 0, true}, liquid);
     if (!P.present) {
-      high += volume(liquid, d.h);
+      high += volume(liquid, h);
     } else {
       clip_solid(liquid, Plane{scale(P.normal, -1), -P.high, false}, part);
-      high += volume(part, d.h);
+      high += volume(part, h);
       if (P.slab) {
         clip_solid(liquid, Plane{P.normal, P.low, false}, part);
-        low += volume(part, d.h);
+        low += volume(part, h);
       }
     }
   }
+}
+
+// The consume step of a candidate: split refusal, volume checks, and for a liquid cell its record and
+// its six face slots.
+__device__ void consume_cell(const FreezeDevice& d, uint64_t i, fl_int3 c, double center, double low, double high,
+                             uint64_t* err) {
   if (low > 0 && high > 0 && center < 0) return fail(err, 2, i, kFailedPrecondition);
   const double unclamped = low + high;
   if (!finite(unclamped) || unclamped < 0 || unclamped / d.full_volume > 1 + 64 * 2.220446049250313e-16)
@@ -391,6 +382,88 @@ __global__ void __launch_bounds__(64) k_freeze_cells(FreezeDevice d, uint64_t* e
   r.curvature = 0;
   d.staged[i] = r;
   d.keep[i] = 1;
+}
+
+// The clipped volume of an all-liquid cell that no obstacle plane cuts: it depends on h alone, so one
+// lane computes it with the general clipping for every such cell.
+__global__ void k_uncut_volume(FreezeDevice d) {
+  const double phi[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+  Planes P{};
+  P.present = false;
+  double low = 0, high = 0;
+  cell_volumes(phi, P, d.h, low, high);
+  d.uncut_volume[0] = high;
+}
+
+// Whether every corner of the unit cell lies on the kept side (distance <= 0) of the plane.
+__device__ inline bool all_kept(const Plane& plane) {
+  for (int k = 0; k < 8; ++k) {
+    const Vtx x{{{double(k & 1), double((k >> 1) & 1), double((k >> 2) & 1)}}, 0.0};
+    if (distance(x, plane) > 0) return false;
+  }
+  return true;
+}
+// Whether every corner lies strictly on the removed side of the plane (nothing kept).
+__device__ inline bool none_kept(const Plane& plane) {
+  for (int k = 0; k < 8; ++k) {
+    const Vtx x{{{double(k & 1), double((k >> 1) & 1), double((k >> 2) & 1)}}, 0.0};
+    if (distance(x, plane) <= 0) return false;
+  }
+  return true;
+}
+
+// One lane per candidate: order check and sample lookups, then the kCell query for the cells it settles
+// without clipping, which give exactly what clipping gives: no negative corner (every tetrahedron
+// skipped, no volume), or every corner negative with each obstacle plane keeping all corners or none
+// (each clip returns its input or nothing, so the volume is the uncut cell's). Other cells go to the
+// work list of k_freeze_cut with their samples in phi[9 i ...] (eight corners, then the centre).
+__global__ void k_freeze_stage(FreezeDevice d, uint64_t* err) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= d.ncand) return;
+  d.keep[i] = 0;
+  const fl_int3 c = d.candidates[i];
+  if (i > 0 && !key_less(d.candidates[i - 1], c)) return fail(err, 2, i, kInvalidArgument);
+  bool found = false;
+  const double center = lookup(d.centers, d.ncenters, c, found);
+  if (!found) return fail(err, 2, i, kFailedPrecondition);
+  double phi[8];
+  bool any_negative = false, all_negative = true;
+  for (int k = 0; k < 8; ++k) {
+    fl_int3 q = c;
+    for (int a = 0; a < 3; ++a)
+      if (!offset(q, a, (k >> a) & 1, q)) return fail(err, 2, i, kOutOfRange);
+    phi[k] = lookup(d.corners, d.ncorners, q, found);
+    if (!found) return fail(err, 2, i, kFailedPrecondition);
+    any_negative = any_negative || phi[k] < 0;
+    all_negative = all_negative && phi[k] < 0;
+  }
+  Planes P;
+  if (int code = local_planes(d, c, P)) return fail(err, 2, i, code);
+  if (!any_negative) return consume_cell(d, i, c, center, 0.0, 0.0, err);
+  if (all_negative) {
+    const bool high_whole = !P.present || all_kept(Plane{scale(P.normal, -1), -P.high, false});
+    const bool low_empty = !P.slab || none_kept(Plane{P.normal, P.low, false});
+    if (high_whole && low_empty) return consume_cell(d, i, c, center, 0.0, d.uncut_volume[0], err);
+  }
+  for (int k = 0; k < 8; ++k) d.phi[9 * i + k] = phi[k];
+  d.phi[9 * i + 8] = center;
+  const unsigned long long w = atomicAdd(reinterpret_cast<unsigned long long*>(d.counts + 3), 1ull);
+  d.work[w] = i;
+}
+
+// One lane per listed candidate: the clipped volumes and the consume step.
+__global__ void __launch_bounds__(64) k_freeze_cut(FreezeDevice d, uint64_t* err) {
+  const uint64_t w = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (w >= d.counts[3]) return;
+  const uint64_t i = d.work[w];
+  const fl_int3 c = d.candidates[i];
+  double phi[8];
+  for (int k = 0; k < 8; ++k) phi[k] = d.phi[9 * i + k];
+  Planes P;
+  (void)local_planes(d, c, P);  // succeeded in k_freeze_stage
+  double low = 0, high = 0;
+  cell_volumes(phi, P, d.h, low, high);
+  consume_cell(d, i, c, d.phi[9 * i + 8], low, high, err);
 }
 
 // Kept cells to their scanned positions; their face slots into the slot bitmap; the kept count.
@@ -570,9 +643,11 @@ int freeze_cells_device(fl_ctx* ctx, const FreezeDevice& d) {
   if (d.ncenters) k_check_samples<<<grid_for(d.ncenters), kThreads, 0, s>>>(d.centers, d.ncenters, 0, ctx->err);
   if (d.ncorners) k_check_samples<<<grid_for(d.ncorners), kThreads, 0, s>>>(d.corners, d.ncorners, 1, ctx->err);
   k_clear<<<grid_for(d.nwords), kThreads, 0, s>>>(d.words, d.nwords);
-  k_clear<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long*>(d.counts), 3);
+  k_clear<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long*>(d.counts), 4);
   if (d.ncand) {
-    k_freeze_cells<<<grid_for(d.ncand, 64), 64, 0, s>>>(d, ctx->err);
+    k_uncut_volume<<<1, 1, 0, s>>>(d);
+    k_freeze_stage<<<grid_for(d.ncand), kThreads, 0, s>>>(d, ctx->err);
+    k_freeze_cut<<<grid_for(d.ncand, 64), 64, 0, s>>>(d, ctx->err);
     FL_TRY(exclusive_scan_u64(ctx, d.keep, d.keep_scan, d.ncand));
     k_compact_cells<<<grid_for(d.ncand), kThreads, 0, s>>>(d);
   }
