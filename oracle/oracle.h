@@ -282,7 +282,8 @@ struct flo_liquid_particle {
 // LiquidParticleUpdateInputs3D scalars (native/include/fluidity/host/liquid_substep_backend_3d.h:16-28).
 struct flo_update_options {
   double cell_width, step_duration, particle_energy_blend, friction_coefficient, penetration_tolerance;
-};
+  flo_obstacle obstacle;  // absent, half-space or slab
+};  // 112 B
 // The particle update without an obstacle or regular-cell transport: stored velocity
 // new PIC + 0.99 e (v - old PIC), advecting velocity with the surface-expansion blend, advection
 // by fma, the half-cell displacement guard and J *= exp(dt div u). Particles are updated in order and

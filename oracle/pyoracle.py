@@ -99,8 +99,10 @@ LIQUID_PARTICLE = np.dtype({"names": ["id", "tag", "world_position", "world_velo
                                         "u1", (CONSTITUENT, 8)],
                             "offsets": [0, 8, 16, 40, 64, 136, 144, 152, 160, 168, 176, 184, 192, 193, 200],
                             "itemsize": 328})
+OBSTACLE = np.dtype([("kind", "u4"), ("pad", "u4"), ("unit_normal", VEC3), ("coordinate", "f8"), ("half_width", "f8"),
+                     ("wall_velocity", VEC3)])
 UPDATE_OPTIONS = np.dtype([("cell_width", "f8"), ("step_duration", "f8"), ("particle_energy_blend", "f8"),
-                           ("friction_coefficient", "f8"), ("penetration_tolerance", "f8")])
+                           ("friction_coefficient", "f8"), ("penetration_tolerance", "f8"), ("obstacle", OBSTACLE)])
 TOPOLOGY_CAPACITIES = np.dtype([("particles", "u8"), ("blocks", "u8"), ("cells", "u8"), ("faces", "u8")])
 assert BLOCK_STATE.itemsize == 32 and CELL_STATE.itemsize == 24 and FACE_STATE.itemsize == 56
 assert PARTICLE.itemsize == 152 and GATHER_RESULT.itemsize == 192 and SORT_ENTRY.itemsize == 24
@@ -313,9 +315,18 @@ def activate_blocks(geom, positions, cells=None, capacity=1 << 24):
     return s, out[:cnt.value].copy()
 
 
-def update_options(cell_width, step_duration, blend, friction=0.0, penetration=0.0):
+def update_options(cell_width, step_duration, blend, friction=0.0, penetration=0.0, obstacle=None):
+    """Particle update options; obstacle: None, or a tuple as made by halfspace() or slab()."""
     o = np.zeros(1, UPDATE_OPTIONS)
-    o[0] = (cell_width, step_duration, blend, friction, penetration)
+    o[0]["cell_width"], o[0]["step_duration"], o[0]["particle_energy_blend"] = cell_width, step_duration, blend
+    o[0]["friction_coefficient"], o[0]["penetration_tolerance"] = friction, penetration
+    if obstacle is not None:
+        kind, n, c, w, v = obstacle
+        ob = o["obstacle"]
+        ob["kind"], ob["coordinate"], ob["half_width"] = kind, c, w
+        for k, a, b in zip("xyz", n, v):
+            ob["unit_normal"][k], ob["wall_velocity"][k] = a, b
+        o["obstacle"] = ob
     return o
 
 

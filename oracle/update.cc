@@ -1,11 +1,15 @@
-// Oracle: the liquid particle update without an obstacle (test infrastructure only).
-// Restates UpdateParticle (native/src/liquid_substep_3d_cpu.cc:239-405) for the absent obstacle and
-// no regular-cell transport, with ResolveContact's absent-obstacle branch
-// (native/include/fluidity/base/liquid_wall_contact_math_3d.h:365-396) and the vector helpers of
+// Oracle: the liquid particle update (test infrastructure only).
+// Restates UpdateParticle (native/src/liquid_substep_3d_cpu.cc:239-405) without regular-cell transport
+// for the absent, half-space and slab obstacles: SolidDistance (native/src/liquid_substep_3d_staging.cc:
+// 197-205), ResolveContact with its certified swept-plane contact
+// (native/include/fluidity/base/liquid_wall_contact_math_3d.h:1-152,365-556) over the bounded projected
+// sums of native/include/fluidity/base/liquid_boundary_math.h:93-263, and the vector helpers of
 // native/src/liquid_substep_3d_staging.cc:160-171.
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <numeric>
+#include <utility>
 
 #include "grid_internal.h"
 
@@ -16,6 +20,295 @@ namespace {
 flo_vec3 Add(flo_vec3 a, flo_vec3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 flo_vec3 Scale(flo_vec3 a, double f) { return {f * a.x, f * a.y, f * a.z}; }
 double MaxMagnitude(flo_vec3 v) { return std::max({std::abs(v.x), std::abs(v.y), std::abs(v.z)}); }
+double Dot(flo_vec3 a, flo_vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+constexpr double kInf = std::numeric_limits<double>::infinity();
+
+// ---- bounded projected sums (binary64, round to nearest) ----
+struct Rounded {
+  double value, residual;
+};
+// Sum and exact residual; the larger magnitude first.
+Rounded AddWithResidual(double left, double right) {
+  if (std::abs(left) < std::abs(right)) std::swap(left, right);
+  const double sum = left + right;
+  if (!std::isfinite(sum)) return {sum, 0};
+  return {sum, right - (sum - left)};
+}
+double AddDown(double left, double right) {
+  const Rounded s = AddWithResidual(left, right);
+  return s.residual < 0 ? std::nextafter(s.value, -kInf) : s.value;
+}
+double AddUp(double left, double right) {
+  const Rounded s = AddWithResidual(left, right);
+  return s.residual > 0 ? std::nextafter(s.value, kInf) : s.value;
+}
+struct Bounds {
+  double estimate, lower, upper;
+  bool finite() const { return std::isfinite(estimate) && std::isfinite(lower) && std::isfinite(upper); }
+  bool resolved() const {
+    return finite() && upper - lower <= 16 * std::numeric_limits<double>::epsilon() * std::abs(estimate);
+  }
+  // The certified sign: 1, -1 or 0 in *out, or false when it is not certified.
+  bool sign(int* out) const {
+    if (!finite()) return false;
+    if (lower > 0) *out = 1;
+    else if (upper < 0) *out = -1;
+    else if (lower == 0 && upper == 0) *out = 0;
+    else return false;
+    return true;
+  }
+};
+// A running sum with separately bounded rounding corrections (ProjectedSum).
+class PSum {
+ public:
+  void Add(double term) {
+    if (!std::isfinite(sum_) || !std::isfinite(term)) {
+      sum_ = kInf;
+      return;
+    }
+    const Rounded s = AddWithResidual(sum_, term);
+    sum_ = s.value;
+    Correct(s.residual);
+  }
+  void AddDot(flo_vec3 n, flo_vec3 v) {
+    Scaled(n.x, v.x, 1);
+    Scaled(n.y, v.y, 1);
+    Scaled(n.z, v.z, 1);
+  }
+  void AddRelativeMotion(flo_vec3 n, flo_vec3 v, flo_vec3 wall, double duration) {
+    Relative(n.x, v.x, wall.x, duration);
+    Relative(n.y, v.y, wall.y, duration);
+    Relative(n.z, v.z, wall.z, duration);
+  }
+  void AddProduct(double left, double right) {
+    const double product = left * right;
+    Add(product);
+    if (std::isfinite(product) && left != 0 && right != 0 && std::abs(left) != 1 && std::abs(right) != 1) {
+      Correct(std::fma(left, right, -product));
+      if (std::abs(product) <= 0x1p-969) {
+        const double r = std::numeric_limits<double>::denorm_min();
+        lower_ = AddDown(lower_, -r);
+        upper_ = AddUp(upper_, r);
+      }
+    }
+  }
+  Bounds bounds() const {
+    if (!std::isfinite(sum_) || !std::isfinite(lower_) || !std::isfinite(upper_)) return {kInf, kInf, kInf};
+    return {sum_ + std::midpoint(lower_, upper_), AddDown(sum_, lower_), AddUp(sum_, upper_)};
+  }
+
+ private:
+  void Correct(double c) {
+    if (!std::isfinite(lower_) || !std::isfinite(upper_)) {
+      sum_ = kInf;
+      return;
+    }
+    lower_ = AddDown(lower_, c);
+    upper_ = AddUp(upper_, c);
+  }
+  void Scaled(double left, double right, double scale) {
+    if (left == 0 || right == 0 || scale == 0) return;
+    if (scale == 1) {
+      AddProduct(left, right);
+      return;
+    }
+    if (std::abs(left) == 1 || std::abs(right) == 1) {
+      AddProduct(left * right, scale);
+      return;
+    }
+    const double product = left * right;
+    if (!std::isfinite(product) || std::abs(product) <= 0x1p-969) {
+      sum_ = kInf;
+      return;
+    }
+    AddProduct(product, scale);
+    AddProduct(std::fma(left, right, -product), scale);
+  }
+  void Relative(double n, double v, double wall, double duration) {
+    if (n == 0 || duration == 0) return;
+    const Rounded r = AddWithResidual(v, -wall);
+    if (!std::isfinite(r.value)) {
+      sum_ = kInf;
+      return;
+    }
+    Scaled(n, r.value, duration);
+    Scaled(n, r.residual, duration);
+  }
+  double sum_ = 0, lower_ = 0, upper_ = 0;
+};
+
+// ---- the swept-plane contact ----
+struct SweepPlane {
+  flo_vec3 normal;
+  double start, half_width;
+  flo_vec3 wall;
+};
+bool Finite3v(flo_vec3 v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); }
+flo_vec3 Advance(flo_vec3 x, flo_vec3 v, double t) {
+  return {std::fma(v.x, t, x.x), std::fma(v.y, t, x.y), std::fma(v.z, t, x.z)};
+}
+Bounds DistanceAlongPath(const SweepPlane& p, flo_vec3 x, flo_vec3 v, double duration) {
+  PSum d;
+  d.AddDot(p.normal, x);
+  d.Add(-p.start);
+  d.Add(-p.half_width);
+  d.AddRelativeMotion(p.normal, v, p.wall, duration);
+  return d.bounds();
+}
+Bounds RelativeNormalVelocity(const SweepPlane& p, flo_vec3 v) {
+  PSum s;
+  s.AddRelativeMotion(p.normal, v, p.wall, 1);
+  return s.bounds();
+}
+bool RepresentablePlanes(const Bounds& c, double half_width) {
+  if (!c.finite()) return false;
+  if (half_width == 0) return true;
+  const double low = c.estimate - half_width, high = c.estimate + half_width;
+  return std::isfinite(c.lower - half_width) && std::isfinite(c.upper + half_width) && low < high;
+}
+// The next representable outward move with the largest normal progress.
+flo_vec3 NudgeOutward(flo_vec3 v, flo_vec3 n) {
+  double values[3] = {v.x, v.y, v.z};
+  const double normals[3] = {n.x, n.y, n.z};
+  int best_axis = 0;
+  double best_value = values[0], best_progress = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (normals[a] == 0) continue;
+    const double next = std::nextafter(values[a], std::copysign(kInf, normals[a]));
+    const double progress = normals[a] * (next - values[a]);
+    if (std::isfinite(next) && progress > best_progress) {
+      best_axis = a;
+      best_value = next;
+      best_progress = progress;
+    }
+  }
+  values[best_axis] = best_value;
+  return {values[0], values[1], values[2]};
+}
+template <typename Evaluate>
+bool CertifySide(flo_vec3& candidate, flo_vec3 n, Evaluate evaluate) {
+  if (!Finite3v(candidate)) return false;
+  for (int k = 0; k < 2; ++k) {
+    const Bounds b = evaluate(candidate);
+    if (!b.finite()) return false;
+    if (b.lower >= 0) return true;
+    candidate = NudgeOutward(candidate, n);
+  }
+  const Bounds b = evaluate(candidate);
+  return b.finite() && b.lower >= 0;
+}
+// ApplyOneWayResponse: OutOfRange when uncertified.
+int OneWayResponse(const SweepPlane& p, flo_vec3 v, double relative, double nls, double friction, flo_vec3& out) {
+  if (relative >= 0) {
+    out = v;
+    return FLO_OK;
+  }
+  flo_vec3 c = Advance(v, p.normal, -relative / nls);
+  const Bounds rounded = RelativeNormalVelocity(p, c);
+  if (!Finite3v(c) || !rounded.finite()) return FLO_OUT_OF_RANGE;
+  c = Advance(c, p.normal, -rounded.estimate / nls);
+  if (friction > 0) {
+    const flo_vec3 t{c.x - p.wall.x, c.y - p.wall.y, c.z - p.wall.z};
+    const double tangent_speed = std::hypot(t.x, t.y, t.z);
+    const double inward_speed = -relative / std::sqrt(nls);
+    if (!std::isfinite(tangent_speed) || inward_speed == 0) return FLO_OUT_OF_RANGE;
+    if (tangent_speed == 0 || friction >= tangent_speed / inward_speed) {
+      c = p.wall;
+    } else {
+      const double fraction = friction * inward_speed / tangent_speed;
+      c = {std::lerp(c.x, p.wall.x, fraction), std::lerp(c.y, p.wall.y, fraction), std::lerp(c.z, p.wall.z, fraction)};
+    }
+  }
+  if (!CertifySide(c, p.normal, [&](flo_vec3 u) { return RelativeNormalVelocity(p, u); })) return FLO_OUT_OF_RANGE;
+  out = c;
+  return FLO_OK;
+}
+// ResolveContact for a half-space or slab after the shared input checks: the end position and the
+// stored velocity.
+int PlaneContact(const flo_update_options& o, flo_vec3 x, flo_vec3 stored, flo_vec3 advecting, flo_vec3& position,
+                 flo_vec3& stored_out) {
+  const flo_obstacle& ob = o.obstacle;
+  const bool slab = ob.kind == FLO_OBSTACLE_SLAB;
+  if (slab && (!std::isfinite(ob.half_width) || ob.half_width <= 0)) return FLO_INVALID_ARGUMENT;
+  SweepPlane plane{ob.unit_normal, ob.coordinate, slab ? ob.half_width : 0, ob.wall_velocity};
+  const flo_vec3 n = plane.normal;
+  const double nls = n.x * n.x + n.y * n.y + n.z * n.z;
+  if (!Finite3v(n) || !Finite3v(plane.wall) || !std::isfinite(plane.start) || std::abs(nls - 1) > 1e-12)
+    return FLO_INVALID_ARGUMENT;
+  PSum translated;
+  translated.Add(plane.start);
+  if (!RepresentablePlanes(translated.bounds(), plane.half_width)) return FLO_OUT_OF_RANGE;
+  translated.AddRelativeMotion(n, plane.wall, {0, 0, 0}, o.step_duration);
+  if (!RepresentablePlanes(translated.bounds(), plane.half_width)) return FLO_OUT_OF_RANGE;
+  if (plane.half_width > 0) {
+    PSum centre;
+    centre.AddDot(n, x);
+    centre.Add(-plane.start);
+    const Bounds cb = centre.bounds();
+    if (!cb.finite()) return FLO_OUT_OF_RANGE;
+    int side = 0;
+    if (!cb.sign(&side) || side == 0) return FLO_FAILED_PRECONDITION;
+    if (side < 0) {
+      plane.normal = {-n.x, -n.y, -n.z};
+      plane.start = -plane.start;
+    }
+  }
+  const Bounds start_distance = DistanceAlongPath(plane, x, {0, 0, 0}, 0);
+  const Bounds adv_projection = RelativeNormalVelocity(plane, advecting);
+  const Bounds stored_projection = RelativeNormalVelocity(plane, stored);
+  int start_side = 0, adv_direction = 0, stored_direction = 0;
+  if (!start_distance.sign(&start_side) || !adv_projection.sign(&adv_direction) ||
+      !stored_projection.sign(&stored_direction))
+    return FLO_OUT_OF_RANGE;
+  const bool penetrating = start_side < 0;
+  if (penetrating && start_distance.lower < -o.penetration_tolerance) return FLO_FAILED_PRECONDITION;
+  double elapsed = 0;
+  bool hit = penetrating || (start_side == 0 && (adv_direction < 0 || stored_direction < 0));
+  if (start_side > 0 && adv_direction < 0) {
+    int end_side = 0;
+    if (!DistanceAlongPath(plane, x, advecting, o.step_duration).sign(&end_side)) return FLO_OUT_OF_RANGE;
+    if (end_side <= 0) {
+      if (end_side < 0 && !start_distance.resolved()) return FLO_OUT_OF_RANGE;
+      hit = true;
+      elapsed = end_side == 0 ? o.step_duration
+                              : std::min(o.step_duration, start_distance.estimate / -adv_projection.estimate);
+      if (elapsed <= 0 || elapsed / o.step_duration == 0) return FLO_OUT_OF_RANGE;
+    }
+  }
+  const auto final_distance = [&](flo_vec3 u) { return DistanceAlongPath(plane, u, {0, 0, 0}, o.step_duration); };
+  if (!hit) {
+    position = Advance(x, advecting, o.step_duration);
+    const Bounds accepted = final_distance(position);
+    if (!Finite3v(position) || !accepted.finite() || accepted.lower < 0) return FLO_OUT_OF_RANGE;
+    stored_out = stored;
+    return FLO_OK;
+  }
+  if (!stored_projection.resolved() || !adv_projection.resolved()) return FLO_OUT_OF_RANGE;
+  flo_vec3 new_stored{}, new_adv{};
+  if (int s = OneWayResponse(plane, stored, stored_projection.estimate, nls, o.friction_coefficient, new_stored))
+    return s;
+  if (int s = OneWayResponse(plane, advecting, adv_projection.estimate, nls, o.friction_coefficient, new_adv)) return s;
+  flo_vec3 hit_position = Advance(x, advecting, elapsed);
+  const auto hit_distance = [&](flo_vec3 u) { return DistanceAlongPath(plane, u, {0, 0, 0}, elapsed); };
+  const Bounds rounded_hit = hit_distance(hit_position);
+  if (!rounded_hit.finite()) return FLO_OUT_OF_RANGE;
+  hit_position = Advance(hit_position, plane.normal, -rounded_hit.estimate / nls);
+  if (!CertifySide(hit_position, plane.normal, hit_distance)) return FLO_OUT_OF_RANGE;
+  position = Advance(hit_position, new_adv, o.step_duration - elapsed);
+  if (!CertifySide(position, plane.normal, final_distance)) return FLO_OUT_OF_RANGE;
+  const double nl = std::sqrt(nls);
+  if (!std::isfinite(stored_projection.estimate / nl) || !std::isfinite(adv_projection.estimate / nl))
+    return FLO_OUT_OF_RANGE;
+  stored_out = new_stored;
+  return FLO_OK;
+}
+
+// SolidDistance: the signed distance to the obstacle at its start (+infinity without one).
+double SolidDistance(const flo_obstacle& ob, flo_vec3 x) {
+  if (ob.kind == FLO_OBSTACLE_HALFSPACE) return Dot(ob.unit_normal, x) - ob.coordinate;
+  if (ob.kind == FLO_OBSTACLE_SLAB) return std::abs(Dot(ob.unit_normal, x) - ob.coordinate) - ob.half_width;
+  return kInf;
+}
 
 int UpdateOne(const flo_update_options& o, const flo_boundary_gather& g, double liquid_distance,
               flo_liquid_particle& p) {
@@ -23,7 +316,8 @@ int UpdateOne(const flo_update_options& o, const flo_boundary_gather& g, double 
   const double alpha = p.pic_initialized && !fallback ? 0.99 * o.particle_energy_blend : 0;
   const flo_vec3 stored = Add(g.new_pic_velocity, Scale(g.particle_minus_old_pic_velocity, alpha));
   const double divergence = g.velocity_gradient.x_row.x + g.velocity_gradient.y_row.y + g.velocity_gradient.z_row.z;
-  const double solid_distance = std::numeric_limits<double>::infinity();  // no obstacle
+  const double solid_distance = SolidDistance(o.obstacle, p.world_position);
+  if (std::isnan(solid_distance)) return FLO_OUT_OF_RANGE;
   const bool surface_expansion = divergence > 0 && liquid_distance <= 0 && liquid_distance >= -o.cell_width &&
                                  solid_distance > o.cell_width;
   const double beta = surface_expansion ? o.particle_energy_blend : 0;
@@ -34,17 +328,22 @@ int UpdateOne(const flo_update_options& o, const flo_boundary_gather& g, double 
   if (!std::isfinite(o.friction_coefficient) || o.friction_coefficient < 0 ||
       !std::isfinite(o.penetration_tolerance) || o.penetration_tolerance < 0)
     return FLO_INVALID_ARGUMENT;
-  const flo_vec3 position{std::fma(advecting.x, o.step_duration, p.world_position.x),
-                          std::fma(advecting.y, o.step_duration, p.world_position.y),
-                          std::fma(advecting.z, o.step_duration, p.world_position.z)};
-  if (!Finite3(position)) return FLO_OUT_OF_RANGE;
+  flo_vec3 position{}, new_stored = stored;
+  if (o.obstacle.kind == FLO_OBSTACLE_ABSENT) {
+    position = Advance(p.world_position, advecting, o.step_duration);
+    if (!Finite3(position)) return FLO_OUT_OF_RANGE;
+  } else if (o.obstacle.kind == FLO_OBSTACLE_HALFSPACE || o.obstacle.kind == FLO_OBSTACLE_SLAB) {
+    if (int s = PlaneContact(o, p.world_position, stored, advecting, position, new_stored)) return s;
+  } else {
+    return FLO_INVALID_ARGUMENT;
+  }
   const flo_vec3 displacement{position.x - p.world_position.x, position.y - p.world_position.y,
                               position.z - p.world_position.z};
   if (MaxMagnitude(displacement) > 0.5 * o.cell_width) return FLO_ABORTED;
   const double volume_ratio = p.volume_ratio * std::exp(o.step_duration * divergence);
   if (!std::isfinite(volume_ratio) || volume_ratio <= 0) return FLO_ABORTED;
   p.world_position = position;
-  p.world_velocity = stored;
+  p.world_velocity = new_stored;
   p.volume_ratio = volume_ratio;
   p.velocity_slopes = g.fitted_velocity_slopes;
   p.affine_enabled = g.affine_enabled;
