@@ -218,8 +218,8 @@ def update_measurement(ctx, n=1 << 21, h=1.0 / 256, dt=2e-4):
     p["volume_ratio"] = 1.0
     p["pic_initialized"] = 1
     d = rng.uniform(-2 * h, h, n)
-    opts = np.zeros(1, R.UPDATE_OPTIONS)
-    opts[0] = (h, dt, 0.95, 0.0, 0.0)
+    opts = np.zeros(1, R.UPDATE_OPTIONS)  # no obstacle
+    opts["cell_width"], opts["step_duration"], opts["particle_energy_blend"] = h, dt, 0.95
     for _ in range(2):  # the first call also pays the first touch of its device buffers
         s, _ = ctx.update_particles(opts, g, d, p)
         assert s == H.OK, s

@@ -17,6 +17,7 @@
 // crossings before duplicates are removed; the capacities below hold these with room to spare, and an
 // overflow gives a NaN measure (OutOfRange), as the reference's bounded lists do.
 #include "keys.h"
+#include "stdmath.h"
 
 namespace fl {
 namespace {
@@ -43,16 +44,6 @@ __device__ inline V3 cross(V3 a, V3 b) {
 }
 __device__ inline bool finite3(V3 a) { return finite(a.c[0]) && finite(a.c[1]) && finite(a.c[2]); }
 __device__ inline bool same(V3 a, V3 b) { return a.c[0] == b.c[0] && a.c[1] == b.c[1] && a.c[2] == b.c[2]; }
-
-// libstdc++'s three-argument std::hypot: the largest magnitude times the root of the scaled squares.
-__device__ inline double hypot3(double x, double y, double z) {
-  x = fabs(x);
-  y = fabs(y);
-  z = fabs(z);
-  const double a = x < y ? (y < z ? z : y) : (x < z ? z : x);
-  if (a == 0) return 0;
-  return a * sqrt((x / a) * (x / a) + (y / a) * (y / a) + (z / a) * (z / a));
-}
 
 struct Vtx {
   V3 p;        // cell-width coordinates

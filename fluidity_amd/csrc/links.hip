@@ -8,6 +8,7 @@
 // by slot, interfaces by face ordinal, air pressures) are binary searches over the sorted inputs.
 // One lane per face builds its link.
 #include "keys.h"
+#include "stdmath.h"
 
 namespace fl {
 namespace {
@@ -162,21 +163,6 @@ __global__ void k_link_cells(View v, Fields F, fl_pressure_cell* out, uint64_t* 
 }
 
 // std::lerp and std::midpoint as libstdc++ defines them for double.
-__device__ inline double lerp_std(double a, double b, double t) {
-  if ((a <= 0 && b >= 0) || (a >= 0 && b <= 0)) return t * b + (1 - t) * a;
-  if (t == 1) return b;
-  const double x = a + t * (b - a);
-  return (t > 1) == (b > a) ? (b < x ? x : b) : (b > x ? x : b);
-}
-__device__ inline double midpoint_std(double a, double b) {
-  const double lo = 2.2250738585072014e-308 * 2, hi = 1.7976931348623157e308 / 2;
-  const double aa = a < 0 ? -a : a, ab = b < 0 ? -b : b;
-  if (aa <= hi && ab <= hi) return (a + b) / 2;
-  if (aa < lo) return a + b / 2;
-  if (ab < lo) return a / 2 + b;
-  return a / 2 + b / 2;
-}
-
 // InternalLiquidCapillaryStepBound3D (liquid_boundary_geometry_3d.cc:1669-1701).
 __device__ inline int capillary_bound(double td, double hd, double h, double sigma, double& out) {
   if (!positive_finite(td) || !positive_finite(hd) || !positive_finite(h) || !finite(sigma) || sigma < 0.0)

@@ -1,13 +1,312 @@
-// Liquid particle update without an obstacle (native/src/liquid_substep_3d_cpu.cc:239-405): one lane
-// per particle, FP64 throughout, the reference's operation order (no contraction except its explicit
-// fma advance), first failing particle by atomic min of (ordinal, code). On error the device copy of
-// the records is partly updated; the caller copies it back only on success.
+// Liquid particle update (native/src/liquid_substep_3d_cpu.cc:239-405) without regular-cell transport,
+// for the absent, half-space and slab obstacles: one lane per particle, FP64 throughout, the
+// reference's operation order (no contraction except its explicit fma), first failing particle by
+// atomic min of (ordinal, code). The obstacle gates the surface-expansion blend by its solid distance
+// (native/src/liquid_substep_3d_staging.cc:197-205) and resolves the certified swept-plane contact
+// (native/include/fluidity/base/liquid_wall_contact_math_3d.h:1-152,365-556) over bounded projected sums
+// (native/include/fluidity/base/liquid_boundary_math.h:93-263). On error the device copy of the records
+// is partly updated; the caller copies it back only on success.
 #include "context.h"
+#include "stdmath.h"
 
 namespace fl {
 namespace {
 
 __device__ inline bool finite3(const fl_vec3& v) { return finite(v.x) && finite(v.y) && finite(v.z); }
+__device__ inline fl_vec3 advance(fl_vec3 x, fl_vec3 v, double t) {
+  return fl_vec3{fma(v.x, t, x.x), fma(v.y, t, x.y), fma(v.z, t, x.z)};
+}
+constexpr double kInf = __builtin_inf();
+
+struct Rounded {
+  double value, residual;
+};
+// Sum and exact residual, the larger magnitude first.
+__device__ inline Rounded add_residual(double l, double r) {
+  if (fabs(l) < fabs(r)) {
+    const double t = l;
+    l = r;
+    r = t;
+  }
+  const double s = l + r;
+  if (!finite(s)) return {s, 0.0};
+  return {s, r - (s - l)};
+}
+__device__ inline double add_down(double l, double r) {
+  const Rounded s = add_residual(l, r);
+  return s.residual < 0 ? nextafter(s.value, -kInf) : s.value;
+}
+__device__ inline double add_up(double l, double r) {
+  const Rounded s = add_residual(l, r);
+  return s.residual > 0 ? nextafter(s.value, kInf) : s.value;
+}
+
+struct Bounds {
+  double estimate, lower, upper;
+  __device__ bool is_finite() const { return finite(estimate) && finite(lower) && finite(upper); }
+  __device__ bool resolved() const {
+    return is_finite() && upper - lower <= 16 * 2.220446049250313e-16 * fabs(estimate);
+  }
+  // The certified sign into *out, or false when it is not certified.
+  __device__ bool sign(int* out) const {
+    if (!is_finite()) return false;
+    if (lower > 0) *out = 1;
+    else if (upper < 0) *out = -1;
+    else if (lower == 0 && upper == 0) *out = 0;
+    else return false;
+    return true;
+  }
+};
+
+// A running sum with separately bounded rounding corrections.
+struct PSum {
+  double sum = 0, lo = 0, hi = 0;
+  __device__ void correct(double c) {
+    if (!finite(lo) || !finite(hi)) {
+      sum = kInf;
+      return;
+    }
+    lo = add_down(lo, c);
+    hi = add_up(hi, c);
+  }
+  __device__ void add(double term) {
+    if (!finite(sum) || !finite(term)) {
+      sum = kInf;
+      return;
+    }
+    const Rounded s = add_residual(sum, term);
+    sum = s.value;
+    correct(s.residual);
+  }
+  __device__ void product(double l, double r) {
+    const double p = l * r;
+    add(p);
+    if (finite(p) && l != 0 && r != 0 && fabs(l) != 1 && fabs(r) != 1) {
+      correct(fma(l, r, -p));
+      if (fabs(p) <= 0x1p-969) {
+        const double d = 4.9406564584124654e-324;
+        lo = add_down(lo, -d);
+        hi = add_up(hi, d);
+      }
+    }
+  }
+  __device__ void scaled(double l, double r, double scale) {
+    if (l == 0 || r == 0 || scale == 0) return;
+    if (scale == 1) {
+      product(l, r);
+      return;
+    }
+    if (fabs(l) == 1 || fabs(r) == 1) {
+      product(l * r, scale);
+      return;
+    }
+    const double p = l * r;
+    if (!finite(p) || fabs(p) <= 0x1p-969) {
+      sum = kInf;
+      return;
+    }
+    product(p, scale);
+    product(fma(l, r, -p), scale);
+  }
+  __device__ void relative(double n, double v, double wall, double duration) {
+    if (n == 0 || duration == 0) return;
+    const Rounded r = add_residual(v, -wall);
+    if (!finite(r.value)) {
+      sum = kInf;
+      return;
+    }
+    scaled(n, r.value, duration);
+    scaled(n, r.residual, duration);
+  }
+  __device__ void add_dot(fl_vec3 n, fl_vec3 v) {
+    scaled(n.x, v.x, 1);
+    scaled(n.y, v.y, 1);
+    scaled(n.z, v.z, 1);
+  }
+  __device__ void add_motion(fl_vec3 n, fl_vec3 v, fl_vec3 wall, double duration) {
+    relative(n.x, v.x, wall.x, duration);
+    relative(n.y, v.y, wall.y, duration);
+    relative(n.z, v.z, wall.z, duration);
+  }
+  __device__ Bounds bounds() const {
+    if (!finite(sum) || !finite(lo) || !finite(hi)) return {kInf, kInf, kInf};
+    return {sum + midpoint_std(lo, hi), add_down(sum, lo), add_up(sum, hi)};
+  }
+};
+
+struct Sweep {
+  fl_vec3 normal;
+  double start, half_width;
+  fl_vec3 wall;
+};
+__device__ inline Bounds distance_along(const Sweep& p, fl_vec3 x, fl_vec3 v, double duration) {
+  PSum d;
+  d.add_dot(p.normal, x);
+  d.add(-p.start);
+  d.add(-p.half_width);
+  d.add_motion(p.normal, v, p.wall, duration);
+  return d.bounds();
+}
+__device__ inline Bounds normal_velocity(const Sweep& p, fl_vec3 v) {
+  PSum s;
+  s.add_motion(p.normal, v, p.wall, 1);
+  return s.bounds();
+}
+__device__ inline bool representable_planes(const Bounds& c, double half_width) {
+  if (!c.is_finite()) return false;
+  if (half_width == 0) return true;
+  const double low = c.estimate - half_width, high = c.estimate + half_width;
+  return finite(c.lower - half_width) && finite(c.upper + half_width) && low < high;
+}
+// The next representable outward move with the largest normal progress.
+__device__ inline fl_vec3 nudge_outward(fl_vec3 v, fl_vec3 n) {
+  double values[3] = {v.x, v.y, v.z};
+  const double normals[3] = {n.x, n.y, n.z};
+  int best_axis = 0;
+  double best_value = values[0], best_progress = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (normals[a] == 0) continue;
+    const double next = nextafter(values[a], copysign(kInf, normals[a]));
+    const double progress = normals[a] * (next - values[a]);
+    if (finite(next) && progress > best_progress) {
+      best_axis = a;
+      best_value = next;
+      best_progress = progress;
+    }
+  }
+  values[best_axis] = best_value;
+  return fl_vec3{values[0], values[1], values[2]};
+}
+// At most two outward nudges to certify the kept side of the evaluated bound.
+template <typename Evaluate>
+__device__ inline bool certify_side(fl_vec3& c, fl_vec3 n, Evaluate evaluate) {
+  if (!finite3(c)) return false;
+  for (int k = 0; k < 2; ++k) {
+    const Bounds b = evaluate(c);
+    if (!b.is_finite()) return false;
+    if (b.lower >= 0) return true;
+    c = nudge_outward(c, n);
+  }
+  const Bounds b = evaluate(c);
+  return b.is_finite() && b.lower >= 0;
+}
+// ApplyOneWayResponse: OutOfRange when uncertified.
+__device__ inline int one_way(const Sweep& p, fl_vec3 v, double relative, double nls, double friction, fl_vec3& out) {
+  if (relative >= 0) {
+    out = v;
+    return kOk;
+  }
+  fl_vec3 c = advance(v, p.normal, -relative / nls);
+  const Bounds rounded = normal_velocity(p, c);
+  if (!finite3(c) || !rounded.is_finite()) return kOutOfRange;
+  c = advance(c, p.normal, -rounded.estimate / nls);
+  if (friction > 0) {
+    const fl_vec3 t{c.x - p.wall.x, c.y - p.wall.y, c.z - p.wall.z};
+    const double tangent_speed = hypot3(t.x, t.y, t.z);
+    const double inward_speed = -relative / sqrt(nls);
+    if (!finite(tangent_speed) || inward_speed == 0) return kOutOfRange;
+    if (tangent_speed == 0 || friction >= tangent_speed / inward_speed) {
+      c = p.wall;
+    } else {
+      const double fraction = friction * inward_speed / tangent_speed;
+      c = fl_vec3{lerp_std(c.x, p.wall.x, fraction), lerp_std(c.y, p.wall.y, fraction),
+                  lerp_std(c.z, p.wall.z, fraction)};
+    }
+  }
+  if (!certify_side(c, p.normal, [&](fl_vec3 u) { return normal_velocity(p, u); })) return kOutOfRange;
+  out = c;
+  return kOk;
+}
+
+// ResolveContact for a half-space or slab after the shared input checks: end position and stored
+// velocity.
+__device__ int plane_contact(const fl_particle_update_options& o, fl_vec3 x, fl_vec3 stored, fl_vec3 adv,
+                             fl_vec3& position, fl_vec3& stored_out) {
+  const fl_obstacle& ob = o.obstacle;
+  const bool slab = ob.kind == FL_OBSTACLE_SLAB;
+  if (slab && (!finite(ob.half_width) || ob.half_width <= 0)) return kInvalidArgument;
+  Sweep plane{ob.unit_normal, ob.coordinate, slab ? ob.half_width : 0.0, ob.wall_velocity};
+  const fl_vec3 n = plane.normal;
+  const double nls = n.x * n.x + n.y * n.y + n.z * n.z;
+  if (!finite3(n) || !finite3(plane.wall) || !finite(plane.start) || fabs(nls - 1) > 1e-12) return kInvalidArgument;
+  PSum translated;
+  translated.add(plane.start);
+  if (!representable_planes(translated.bounds(), plane.half_width)) return kOutOfRange;
+  translated.add_motion(n, plane.wall, fl_vec3{0, 0, 0}, o.step_duration);
+  if (!representable_planes(translated.bounds(), plane.half_width)) return kOutOfRange;
+  if (plane.half_width > 0) {
+    PSum centre;
+    centre.add_dot(n, x);
+    centre.add(-plane.start);
+    const Bounds cb = centre.bounds();
+    if (!cb.is_finite()) return kOutOfRange;
+    int side = 0;
+    if (!cb.sign(&side) || side == 0) return kFailedPrecondition;
+    if (side < 0) {
+      plane.normal = fl_vec3{-n.x, -n.y, -n.z};
+      plane.start = -plane.start;
+    }
+  }
+  const fl_vec3 zero{0, 0, 0};
+  const Bounds start_distance = distance_along(plane, x, zero, 0);
+  const Bounds adv_projection = normal_velocity(plane, adv);
+  const Bounds stored_projection = normal_velocity(plane, stored);
+  int start_side = 0, adv_direction = 0, stored_direction = 0;
+  if (!start_distance.sign(&start_side) || !adv_projection.sign(&adv_direction) ||
+      !stored_projection.sign(&stored_direction))
+    return kOutOfRange;
+  const bool penetrating = start_side < 0;
+  if (penetrating && start_distance.lower < -o.penetration_tolerance) return kFailedPrecondition;
+  double elapsed = 0;
+  bool hit = penetrating || (start_side == 0 && (adv_direction < 0 || stored_direction < 0));
+  if (start_side > 0 && adv_direction < 0) {
+    int end_side = 0;
+    if (!distance_along(plane, x, adv, o.step_duration).sign(&end_side)) return kOutOfRange;
+    if (end_side <= 0) {
+      if (end_side < 0 && !start_distance.resolved()) return kOutOfRange;
+      hit = true;
+      if (end_side == 0) {
+        elapsed = o.step_duration;
+      } else {
+        const double t = start_distance.estimate / -adv_projection.estimate;
+        elapsed = t < o.step_duration ? t : o.step_duration;
+      }
+      if (elapsed <= 0 || elapsed / o.step_duration == 0) return kOutOfRange;
+    }
+  }
+  const auto final_distance = [&](fl_vec3 u) { return distance_along(plane, u, zero, o.step_duration); };
+  if (!hit) {
+    position = advance(x, adv, o.step_duration);
+    const Bounds accepted = final_distance(position);
+    if (!finite3(position) || !accepted.is_finite() || accepted.lower < 0) return kOutOfRange;
+    stored_out = stored;
+    return kOk;
+  }
+  if (!stored_projection.resolved() || !adv_projection.resolved()) return kOutOfRange;
+  fl_vec3 new_stored{}, new_adv{};
+  if (int c = one_way(plane, stored, stored_projection.estimate, nls, o.friction_coefficient, new_stored)) return c;
+  if (int c = one_way(plane, adv, adv_projection.estimate, nls, o.friction_coefficient, new_adv)) return c;
+  fl_vec3 hit_position = advance(x, adv, elapsed);
+  const auto hit_distance = [&](fl_vec3 u) { return distance_along(plane, u, zero, elapsed); };
+  const Bounds rounded_hit = hit_distance(hit_position);
+  if (!rounded_hit.is_finite()) return kOutOfRange;
+  hit_position = advance(hit_position, plane.normal, -rounded_hit.estimate / nls);
+  if (!certify_side(hit_position, plane.normal, hit_distance)) return kOutOfRange;
+  position = advance(hit_position, new_adv, o.step_duration - elapsed);
+  if (!certify_side(position, plane.normal, final_distance)) return kOutOfRange;
+  const double nl = sqrt(nls);
+  if (!finite(stored_projection.estimate / nl) || !finite(adv_projection.estimate / nl)) return kOutOfRange;
+  stored_out = new_stored;
+  return kOk;
+}
+
+// SolidDistance: the signed distance to the obstacle at its start (+infinity without one).
+__device__ inline double solid_distance(const fl_obstacle& ob, fl_vec3 x) {
+  const fl_vec3 n = ob.unit_normal;
+  if (ob.kind == FL_OBSTACLE_HALFSPACE) return (n.x * x.x + n.y * x.y + n.z * x.z) - ob.coordinate;
+  if (ob.kind == FL_OBSTACLE_SLAB) return fabs((n.x * x.x + n.y * x.y + n.z * x.z) - ob.coordinate) - ob.half_width;
+  return kInf;
+}
 
 // One workgroup of kGroup lanes updates kGroup consecutive particles. Their 328-byte particle and
 // 224-byte gather records are contiguous, 16-byte aligned blocks (kGroup * 328 and kGroup * 224 are
@@ -17,6 +316,8 @@ constexpr int kGroup = 64;
 static_assert(kGroup * sizeof(fl_liquid_particle) % 16 == 0 && kGroup * sizeof(fl_boundary_gather) % 16 == 0,
               "16-byte blocks");
 
+// kObstacle: the half-space / slab instance; the obstacle-free instance carries no contact code.
+template <bool kObstacle>
 __global__ __launch_bounds__(kGroup) void k_update_particles(fl_particle_update_options o,
                                                              const fl_boundary_gather* gathered,
                                                              const double* distances, uint64_t n,
@@ -53,25 +354,29 @@ __global__ __launch_bounds__(kGroup) void k_update_particles(fl_particle_update_
     const fl_vec3 stored{np.x + alpha * d.x, np.y + alpha * d.y, np.z + alpha * d.z};
     const double divergence =
         g.velocity_gradient.x_row.x + g.velocity_gradient.y_row.y + g.velocity_gradient.z_row.z;
-    // Without an obstacle the solid distance is +infinity, so it always clears the cell width.
+    const double solid = kObstacle ? solid_distance(o.obstacle, p.world_position) : kInf;
     const double phi = distances[i];
-    const bool surface = divergence > 0 && phi <= 0 && phi >= -o.cell_width;
+    const bool surface = divergence > 0 && phi <= 0 && phi >= -o.cell_width && solid > o.cell_width;
     const double beta = surface ? o.particle_energy_blend : 0.0;
     const double ba = beta * alpha;
     const fl_vec3 adv{np.x + ba * d.x, np.y + ba * d.y, np.z + ba * d.z};
     const fl_vec3 x = p.world_position;
     int code = kOk;
-    if (!finite3(x) || !finite3(stored) || !finite3(adv)) code = kInvalidArgument;
+    if (isnan(solid)) code = kOutOfRange;
+    else if (!finite3(x) || !finite3(stored) || !finite3(adv)) code = kInvalidArgument;
     else if (!finite(o.step_duration) || o.step_duration <= 0) code = kInvalidArgument;
     else if (!finite(o.friction_coefficient) || o.friction_coefficient < 0 || !finite(o.penetration_tolerance) ||
              o.penetration_tolerance < 0)
       code = kInvalidArgument;
-    fl_vec3 xn{0.0, 0.0, 0.0};
+    fl_vec3 xn{0.0, 0.0, 0.0}, vn = stored;
     double j = 0.0;
     if (code == kOk) {
-      xn = fl_vec3{fma(adv.x, o.step_duration, x.x), fma(adv.y, o.step_duration, x.y),
-                   fma(adv.z, o.step_duration, x.z)};
-      if (!finite3(xn)) code = kOutOfRange;
+      if (!kObstacle) {
+        xn = advance(x, adv, o.step_duration);
+        if (!finite3(xn)) code = kOutOfRange;
+      } else {
+        code = plane_contact(o, x, stored, adv, xn, vn);
+      }
     }
     if (code == kOk) {
       const double m = fmax(fmax(fabs(xn.x - x.x), fabs(xn.y - x.y)), fabs(xn.z - x.z));
@@ -85,7 +390,7 @@ __global__ __launch_bounds__(kGroup) void k_update_particles(fl_particle_update_
       atomicMin(reinterpret_cast<unsigned long long*>(err), static_cast<unsigned long long>(error_word(i, code)));
     } else {
       p.world_position = xn;
-      p.world_velocity = stored;
+      p.world_velocity = vn;
       p.volume_ratio = j;
       p.velocity_slopes = g.fitted_velocity_slopes;
       p.affine_enabled = g.affine_enabled;
@@ -111,7 +416,12 @@ __global__ __launch_bounds__(kGroup) void k_update_particles(fl_particle_update_
 int update_particles(fl_ctx* ctx, const fl_particle_update_options& o, const fl_boundary_gather* gathered,
                      const double* distances, uint64_t n, fl_liquid_particle* particles) {
   if (!n) return FL_OK;
-  k_update_particles<<<grid_for(n, kGroup), kGroup, 0, ctx->stream>>>(o, gathered, distances, n, particles, ctx->err);
+  if (o.obstacle.kind == FL_OBSTACLE_ABSENT)
+    k_update_particles<false><<<grid_for(n, kGroup), kGroup, 0, ctx->stream>>>(o, gathered, distances, n, particles,
+                                                                             ctx->err);
+  else
+    k_update_particles<true><<<grid_for(n, kGroup), kGroup, 0, ctx->stream>>>(o, gathered, distances, n, particles,
+                                                                            ctx->err);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }

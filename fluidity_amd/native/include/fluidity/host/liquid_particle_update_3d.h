@@ -1,6 +1,6 @@
 // The liquid particle update on the GPU (the update_particles stage of LiquidSubstepBackend3D,
 // native/include/fluidity/host/liquid_substep_backend_3d.h:195-202; native/src/liquid_substep_3d_cpu.cc:239-405)
-// for a substep without an obstacle and without regular-cell transport.
+// for a substep without regular-cell transport, with no obstacle or a half-space or slab obstacle.
 #pragma once
 
 #include <array>
@@ -10,6 +10,7 @@
 #include "fluidity/base/liquid_boundary_transfer_types_3d.h"
 #include "fluidity/base/mac_types_3d.h"
 #include "fluidity/host/hip_operator_runtime_3d.h"
+#include "fluidity/host/liquid_boundary_geometry_3d.h"
 #include "fluidity/status.h"
 
 namespace fluidity {
@@ -39,10 +40,12 @@ struct LiquidParticleState3D {
   std::array<LiquidConstituentFraction3D, kMaximumLiquidParticleConstituents3D> constituents{};
 };
 
-// LiquidParticleUpdateInputs3D without the obstacle (absent) and the regular-cell transport (empty).
+// LiquidParticleUpdateInputs3D without the regular-cell transport (empty); the obstacle is absent,
+// a half-space or a slab (the open tank stays with the reference).
 struct LiquidParticleUpdateInputs3D {
   std::span<const LiquidBoundaryParticleGather3D> gathered;  // Frozen old/new gather, particle order.
   std::span<const double> liquid_distances;                   // Frozen x^n field, meters, particle order.
+  LiquidObstacle3D obstacle;     // Pose at substep start, constant translation during step_duration.
   double cell_width;             // Meters.
   double step_duration;          // Positive seconds.
   double particle_energy_blend;  // e in [0,1], alpha = 0.99 e and surface beta = e.
@@ -50,8 +53,9 @@ struct LiquidParticleUpdateInputs3D {
   double penetration_tolerance;  // Meters.
 };
 
-// ASFLIP stored velocity, surface-expansion advection, half-cell guard and J update, per particle in
-// order; the first failing particle's status (InvalidArgument, OutOfRange or Aborted). Unlike the
+// ASFLIP stored velocity, surface-expansion advection gated by the solid distance, the certified
+// swept-plane contact, half-cell guard and J update, per particle in order; the first failing
+// particle's status (InvalidArgument, FailedPrecondition, OutOfRange or Aborted). Unlike the
 // reference, which leaves earlier particles updated on error, the particles are written only on
 // success. J may differ from the host's exp in the last place.
 Status UpdateLiquidParticles3D(const LiquidParticleUpdateInputs3D& inputs,

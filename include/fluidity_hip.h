@@ -288,7 +288,8 @@ typedef struct fl_particle_update_options {
   double particle_energy_blend;
   double friction_coefficient;
   double penetration_tolerance;
-} fl_particle_update_options;
+  fl_obstacle obstacle; /* absent (all zero), half-space or slab */
+} fl_particle_update_options; /* 112 B */
 /* Topology records of a liquid state (native/include/fluidity/host/liquid_state_3d.h:129-159):
  * LiquidBlockState3D, LiquidCellState3D (occupancy 0 air, 1 liquid, 2 solid) and LiquidFaceState3D
  * (axis 0..2 with the lattice index at offset 4; validity 0 missing, 1 valid, 2 predicted, 3 low mass). */
@@ -464,17 +465,20 @@ int fl_freeze_geometry(fl_ctx* ctx, const fl_freeze_inputs* inputs, fl_frozen_ce
                        fl_measurement* measurement);
 
 /* ---- particle update ---- */
-/* The liquid particle update of the CPU substep backend without an obstacle or regular-cell transport
- * (native/src/liquid_substep_3d_cpu.cc:239-405, with ResolveContact's absent-obstacle branch,
- * native/include/fluidity/base/liquid_wall_contact_math_3d.h:365-396): per particle, alpha = 0.99 e
- * when PIC-initialised and no gather fallback, stored velocity = new PIC + alpha (v - old PIC),
- * advecting velocity blended the same way only under surface expansion (div u > 0 and
- * -h <= liquid distance <= 0), position advanced by fma(advecting, dt, x), a displacement above h/2
- * Aborted, J *= exp(dt div u) (Aborted unless finite and positive), slopes and the affine switch from
- * the gather. gathered and liquid_distances are in particle order. The first failing particle's status
- * is returned (InvalidArgument for non-finite inputs or invalid options, OutOfRange for a
- * non-finite advanced position, Aborted as above); particles are written only on success. J uses the
- * device exp, which may differ from the host's in the last place. */
+/* The liquid particle update of the CPU substep backend without regular-cell transport
+ * (native/src/liquid_substep_3d_cpu.cc:239-405) for the absent, half-space and slab obstacles: per
+ * particle, alpha = 0.99 e when PIC-initialised and no gather fallback, stored velocity = new PIC +
+ * alpha (v - old PIC), advecting velocity blended the same way only under surface expansion (div u > 0,
+ * -h <= liquid distance <= 0 and a solid distance above h), then ResolveContact
+ * (native/include/fluidity/base/liquid_wall_contact_math_3d.h:365-556): fma(advecting, dt, x) without an
+ * obstacle, else the certified swept-plane contact (first hit, one-way normal impulse with Coulomb
+ * friction on both velocities, the constrained remainder of the step); a displacement above h/2 Aborted,
+ * J *= exp(dt div u) (Aborted unless finite and positive), slopes and the affine switch from the gather.
+ * gathered and liquid_distances are in particle order. The first failing particle's status is returned
+ * (InvalidArgument for non-finite inputs or invalid options or obstacle, FailedPrecondition for deep or
+ * ambiguous penetration, OutOfRange for uncertified contact or a non-finite position, Aborted as above);
+ * particles are written only on success. J uses the device exp, which may differ from the host's in the
+ * last place. */
 int fl_update_particles(fl_ctx* ctx, const fl_particle_update_options* options, const fl_boundary_gather* gathered,
                         const double* liquid_distances, size_t n, fl_liquid_particle* particles,
                         fl_measurement* measurement);

@@ -625,7 +625,8 @@ void BackendUpdateMatchesOracle() {
     p.volume_ratio = 0.5 + u(rng);
     p.pic_initialized = u(rng) < 0.8;
   }
-  const LiquidParticleUpdateInputs3D in{gathered, distances, 0.25, 0.01, 0.7, 0.0, 0.0};
+  const LiquidParticleUpdateInputs3D in{gathered, distances, {}, 0.25, 0.01, 0.7, 0.0, 0.0};
+  const std::vector<LiquidParticleState3D> original = particles;
   std::vector<LiquidParticleState3D> want = particles;
   const flo_update_options o{0.25, 0.01, 0.7, 0.0, 0.0};
   EXPECT(flo_update_particles(&o, reinterpret_cast<const flo_boundary_gather*>(gathered.data()), distances.data(),
@@ -644,8 +645,30 @@ void BackendUpdateMatchesOracle() {
            a.velocity_slopes.y_row.y == b.velocity_slopes.y_row.y && a.affine_enabled == b.affine_enabled &&
            a.pic_initialized && std::abs(a.volume_ratio - b.volume_ratio) <= 4.5e-16 * b.volume_ratio);
   }
+  // A moving half-space at x = 0: particles near it hit it within the step.
+  std::vector<LiquidParticleState3D> start = original, walled = original;
+  const LiquidParticleUpdateInputs3D wall_in{gathered, distances, TranslatingHalfspace3D{{1, 0, 0}, 0.0, {0.3, 0, 0}},
+                                             0.25, 0.01, 0.7, 0.2, 0.0};
+  flo_update_options ow{0.25, 0.01, 0.7, 0.2, 0.0, {}};
+  ow.obstacle.kind = FLO_OBSTACLE_HALFSPACE;
+  ow.obstacle.unit_normal = {1, 0, 0};
+  ow.obstacle.wall_velocity = {0.3, 0, 0};
+  EXPECT(flo_update_particles(&ow, reinterpret_cast<const flo_boundary_gather*>(gathered.data()), distances.data(),
+                              count, reinterpret_cast<flo_liquid_particle*>(walled.data())) == 0);
+  EXPECT(backend.update_particles(backend.context, wall_in, start).ok());
+  std::size_t moved = 0;
+  for (std::size_t i = 0; i < count; ++i) {
+    const auto& a = start[i];
+    const auto& b = walled[i];
+    EXPECT(a.world_position.x == b.world_position.x && a.world_position.y == b.world_position.y &&
+           a.world_position.z == b.world_position.z && a.world_velocity.x == b.world_velocity.x &&
+           a.world_velocity.y == b.world_velocity.y && a.world_velocity.z == b.world_velocity.z);
+    moved += a.world_velocity.x != want[i].world_velocity.x;
+  }
+  std::printf("  half-space contact changed %zu of %zu stored velocities\n", moved, count);
   // A mismatched span is InvalidArgument and leaves the particles untouched.
-  const LiquidParticleUpdateInputs3D short_in{std::span(gathered).first(10), distances, 0.25, 0.01, 0.7, 0.0, 0.0};
+  const LiquidParticleUpdateInputs3D short_in{std::span(gathered).first(10), distances, {}, 0.25, 0.01, 0.7, 0.0,
+                                              0.0};
   EXPECT(UpdateLiquidParticles3D(short_in, particles).code() == StatusCode::kInvalidArgument);
 }
 

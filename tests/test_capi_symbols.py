@@ -32,7 +32,7 @@ def test_record_layouts_match_the_header():
     assert R.BOUNDARY_PARTICLE.itemsize == 168 and R.FACE_TAG.itemsize == 16 and R.COMPATIBLE_SAMPLE.itemsize == 72
     assert R.WALL_SAMPLE.itemsize == 64 and R.BOUNDARY_GATHER.itemsize == 224
     assert R.BLOCK_STATE.itemsize == 32 and R.CELL_STATE.itemsize == 24 and R.FACE_STATE.itemsize == 56
-    assert R.LIQUID_PARTICLE.itemsize == 328 and R.UPDATE_OPTIONS.itemsize == 40
+    assert R.LIQUID_PARTICLE.itemsize == 328 and R.UPDATE_OPTIONS.itemsize == 112
 
 
 def test_record_offsets_match_the_c_compiler(tmp_path):
@@ -49,6 +49,8 @@ def test_record_offsets_match_the_c_compiler(tmp_path):
                                          "carrier_velocity": "carrier_velocity", "validity": "validity",
                                          "side_tag": "side_tag", "structurally_eligible": "structurally_eligible"}),
         "fl_liquid_particle": (R.LIQUID_PARTICLE, {n: n for n in R.LIQUID_PARTICLE.names}),
+        "fl_particle_update_options": (R.UPDATE_OPTIONS, {n: n for n in R.UPDATE_OPTIONS.names}),
+        "fl_obstacle": (R.OBSTACLE, {n: n for n in R.OBSTACLE.names}),
         "fl_frozen_face": (R.FROZEN_FACE, {"face_slot": "face_slot", "open_area": "open_area",
                                            "prescribed_wall_flux": "prescribed_wall_flux",
                                            "has_prescribed_wall_flux": "has_prescribed_wall_flux",
