@@ -733,7 +733,7 @@ int fl_update_particles(fl_ctx* ctx, const fl_particle_update_options* options, 
     return FL_INVALID_ARGUMENT;
   // An unknown obstacle kind fails the first particle's contact as InvalidArgument, like its other
   // contact input checks.
-  if (n && options->obstacle.kind > FL_OBSTACLE_SLAB) return FL_INVALID_ARGUMENT;
+  if (n && options->obstacle.kind > FL_OBSTACLE_TANK) return FL_INVALID_ARGUMENT;
   (void)hipSetDevice(ctx->device);
   if (!ctx->update) ctx->update = new UpdateBuffers();
   UpdateBuffers& B = *static_cast<UpdateBuffers*>(ctx->update);

@@ -1,5 +1,5 @@
 // Liquid particle update (native/src/liquid_substep_3d_cpu.cc:239-405) without regular-cell transport,
-// for the absent, half-space and slab obstacles: one lane per particle, FP64 throughout, the
+// for the absent, half-space, slab and open-tank obstacles: one lane per particle, FP64 throughout, the
 // reference's operation order (no contraction except its explicit fma), first failing particle by
 // atomic min of (ordinal, code). The obstacle gates the surface-expansion blend by its solid distance
 // (native/src/liquid_substep_3d_staging.cc:197-205) and resolves the certified swept-plane contact
@@ -300,12 +300,132 @@ __device__ int plane_contact(const fl_particle_update_options& o, fl_vec3 x, fl_
   return kOk;
 }
 
-// SolidDistance: the signed distance to the obstacle at its start (+infinity without one).
+// SolidDistance: the signed distance to the obstacle at its start (+infinity without one); for the tank
+// the least inward clearance (SampleTankWall, liquid_boundary_math.h:319-350), NaN when one is not finite.
 __device__ inline double solid_distance(const fl_obstacle& ob, fl_vec3 x) {
   const fl_vec3 n = ob.unit_normal;
   if (ob.kind == FL_OBSTACLE_HALFSPACE) return (n.x * x.x + n.y * x.y + n.z * x.z) - ob.coordinate;
   if (ob.kind == FL_OBSTACLE_SLAB) return fabs((n.x * x.x + n.y * x.y + n.z * x.z) - ob.coordinate) - ob.half_width;
+  if (ob.kind == FL_OBSTACLE_TANK) {
+    const fl_tank& t = ob.tank;
+    const double d[5] = {x.x - t.start_low_x, t.high_x - x.x, x.y - t.floor_y, x.z - t.low_z, t.high_z - x.z};
+    double least = d[0];
+    for (int k = 0; k < 5; ++k) {
+      if (!finite(d[k])) return __builtin_nan("");
+      if (d[k] < least) least = d[k];
+    }
+    return least;
+  }
   return kInf;
+}
+
+// CheckTankStep (liquid_boundary_math.h:252-276).
+__device__ inline int check_tank_step(const fl_tank& t, double dt) {
+  if (!finite(t.start_low_x) || !finite(t.high_x) || !finite(t.floor_y) || !finite(t.low_z) || !finite(t.high_z) ||
+      !finite(t.piston_velocity_x))
+    return kInvalidArgument;
+  if (t.start_low_x >= t.high_x || t.low_z >= t.high_z) return kInvalidArgument;
+  const double displacement = t.piston_velocity_x * dt;
+  const double end_x = fma(t.piston_velocity_x, dt, t.start_low_x);
+  const double start_width = t.high_x - t.start_low_x, end_width = t.high_x - end_x, width_z = t.high_z - t.low_z;
+  if (!finite(displacement) || !finite(end_x) || !finite(start_width) || !finite(end_width) || !finite(width_z))
+    return kOutOfRange;
+  if (end_x >= t.high_x) return kFailedPrecondition;
+  return kOk;
+}
+
+// SweepTankAxis (liquid_wall_contact_math_3d.h:154-289): one Cartesian component against its walls,
+// each expressed along local x.
+__device__ int sweep_tank_axis(const fl_particle_update_options& o, const Sweep* walls, int count, double x0,
+                               double s0, double a0, double& out_x, double& out_stored) {
+  const fl_vec3 zero{0, 0, 0};
+  fl_vec3 start{x0, 0, 0}, stored{s0, 0, 0}, adv{a0, 0, 0};
+  int carrying = -1;
+  const auto impact = [&](int k) {
+    const Sweep& wall = walls[k];
+    const Bounds ss = normal_velocity(wall, stored), as = normal_velocity(wall, adv);
+    if (!ss.resolved() || !as.resolved()) return kOutOfRange;
+    if (ss.estimate < 0) stored.x = wall.wall.x;
+    if (as.estimate <= 0) {
+      adv.x = wall.wall.x;
+      carrying = k;
+    }
+    return kOk;
+  };
+  for (int k = 0; k < count; ++k) {
+    const Sweep& wall = walls[k];
+    const Bounds sd = distance_along(wall, start, zero, 0);
+    int side = 0;
+    if (!sd.sign(&side)) return kOutOfRange;
+    if (side > 0) continue;
+    if (sd.lower < -o.penetration_tolerance) return kFailedPrecondition;
+    const double wall_coordinate = wall.start * wall.normal.x;
+    const bool low_wall = wall.normal.x > 0;
+    const bool stored_in = low_wall ? stored.x < wall.wall.x : stored.x > wall.wall.x;
+    const bool adv_in = low_wall ? adv.x < wall.wall.x : adv.x > wall.wall.x;
+    if (side < 0 || stored_in || adv_in) {
+      if (int c = impact(k)) return c;
+      start.x = wall_coordinate;
+    }
+  }
+  if (carrying < 0) {
+    for (int k = 0; k < count; ++k) {
+      const Sweep& wall = walls[k];
+      const Bounds as = normal_velocity(wall, adv);
+      int direction = 0;
+      if (!as.sign(&direction)) return kOutOfRange;
+      if (direction >= 0) continue;
+      int end_side = 0;
+      if (!distance_along(wall, start, adv, o.step_duration).sign(&end_side)) return kOutOfRange;
+      if (end_side > 0) continue;
+      if (end_side < 0) {
+        const Bounds sd = distance_along(wall, start, zero, 0);
+        if (!sd.resolved() || !as.resolved()) return kOutOfRange;
+        const double q = sd.estimate / -as.estimate;
+        const double t = q < o.step_duration ? q : o.step_duration;
+        if (t <= 0 || t / o.step_duration == 0) return kOutOfRange;
+      }
+      if (int c = impact(k)) return c;
+      break;
+    }
+  }
+  fl_vec3 end{0, 0, 0};
+  if (carrying < 0) {
+    end = advance(start, adv, o.step_duration);
+  } else {
+    const Sweep& wall = walls[carrying];
+    end = advance(fl_vec3{wall.start * wall.normal.x, 0, 0}, wall.wall, o.step_duration);
+    if (!certify_side(end, wall.normal, [&](fl_vec3 u) { return distance_along(wall, u, zero, o.step_duration); }))
+      return kOutOfRange;
+  }
+  for (int k = 0; k < count; ++k) {
+    const Bounds ed = distance_along(walls[k], end, zero, o.step_duration);
+    if (!ed.is_finite() || ed.lower < 0) return kOutOfRange;
+  }
+  out_x = end.x;
+  out_stored = stored.x;
+  return kOk;
+}
+
+// ResolveTankContact (liquid_wall_contact_math_3d.h:290-363): frictionless, per axis x, y, z.
+__device__ int tank_contact(const fl_particle_update_options& o, fl_vec3 x, fl_vec3 stored, fl_vec3 adv,
+                            fl_vec3& position, fl_vec3& stored_out) {
+  if (o.friction_coefficient != 0) return kInvalidArgument;
+  const fl_tank& t = o.obstacle.tank;
+  if (int c = check_tank_step(t, o.step_duration)) return c;
+  const fl_vec3 zero{0, 0, 0};
+  const Sweep walls[3][2] = {
+      {{fl_vec3{1, 0, 0}, t.start_low_x, 0, fl_vec3{t.piston_velocity_x, 0, 0}}, {fl_vec3{-1, 0, 0}, -t.high_x, 0, zero}},
+      {{fl_vec3{1, 0, 0}, t.floor_y, 0, zero}, {zero, 0, 0, zero}},
+      {{fl_vec3{1, 0, 0}, t.low_z, 0, zero}, {fl_vec3{-1, 0, 0}, -t.high_z, 0, zero}}};
+  const int counts[3] = {2, 1, 2};
+  const double xs[3] = {x.x, x.y, x.z}, ss[3] = {stored.x, stored.y, stored.z}, as[3] = {adv.x, adv.y, adv.z};
+  double px[3], sx[3];
+  for (int a = 0; a < 3; ++a)
+    if (int c = sweep_tank_axis(o, walls[a], counts[a], xs[a], ss[a], as[a], px[a], sx[a])) return c;
+  position = fl_vec3{px[0], px[1], px[2]};
+  stored_out = fl_vec3{sx[0], sx[1], sx[2]};
+  return kOk;
 }
 
 // One workgroup of kGroup lanes updates kGroup consecutive particles. Their 328-byte particle and
@@ -316,7 +436,7 @@ constexpr int kGroup = 64;
 static_assert(kGroup * sizeof(fl_liquid_particle) % 16 == 0 && kGroup * sizeof(fl_boundary_gather) % 16 == 0,
               "16-byte blocks");
 
-// kObstacle: the half-space / slab instance; the obstacle-free instance carries no contact code.
+// kObstacle: the half-space / slab / tank instance; the obstacle-free instance carries no contact code.
 template <bool kObstacle>
 __global__ __launch_bounds__(kGroup) void k_update_particles(fl_particle_update_options o,
                                                              const fl_boundary_gather* gathered,
@@ -374,6 +494,8 @@ __global__ __launch_bounds__(kGroup) void k_update_particles(fl_particle_update_
       if (!kObstacle) {
         xn = advance(x, adv, o.step_duration);
         if (!finite3(xn)) code = kOutOfRange;
+      } else if (o.obstacle.kind == FL_OBSTACLE_TANK) {
+        code = tank_contact(o, x, stored, adv, xn, vn);
       } else {
         code = plane_contact(o, x, stored, adv, xn, vn);
       }

@@ -100,9 +100,9 @@ def topology_capacities(particles=1 << 20, blocks=1 << 16, cells=1 << 20, faces=
 
 
 class Obstacle(C.Structure):
-    """fl_obstacle: kind 0 absent, 1 half-space, 2 slab."""
+    """fl_obstacle: kind 0 absent, 1 half-space, 2 slab, 3 open tank (particle update only)."""
     _fields_ = [("kind", C.c_uint32), ("pad", C.c_uint32), ("unit_normal", C.c_double * 3), ("coordinate", C.c_double),
-                ("half_width", C.c_double), ("wall_velocity", C.c_double * 3)]
+                ("half_width", C.c_double), ("wall_velocity", C.c_double * 3), ("tank", C.c_double * 6)]
 
 
 class FreezeInputs(C.Structure):
@@ -245,8 +245,9 @@ class Context:
                                                                                p.corners))
         obs = Obstacle()
         if p.obstacle is not None:
-            kind, n, c, w, v = p.obstacle
-            obs = Obstacle(kind, 0, (C.c_double * 3)(*n), c, w, (C.c_double * 3)(*v))
+            kind, n, c, w, v = p.obstacle[:5]
+            t = p.obstacle[5] if len(p.obstacle) > 5 else (0.0,) * 6
+            obs = Obstacle(kind, 0, (C.c_double * 3)(*n), c, w, (C.c_double * 3)(*v), (C.c_double * 6)(*t))
         k = p.exact_curvature
         inputs = FreezeInputs(_p(blocks), len(blocks), (C.c_double * 3)(o["x"], o["y"], o["z"]),
                               p.geom[0]["cell_width"], _p(cands), len(cands), _p(centers), len(centers), _p(corners),

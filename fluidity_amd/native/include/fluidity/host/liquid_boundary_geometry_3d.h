@@ -97,7 +97,18 @@ struct TranslatingSlab3D {
   double half_width;  // Positive finite, m.
   Vec3 wall_velocity;
 };
-using LiquidObstacle3D = std::variant<std::monostate, TranslatingHalfspace3D, TranslatingSlab3D>;
+// An open tank (native/include/fluidity/base/liquid_boundary_math.h:22-29): the particle update resolves
+// its contact; the freeze refuses it (InvalidArgument).
+struct AxisAlignedLiquidTank3D {
+  double start_low_x;        // World x of the piston at substep start, m.
+  double high_x;             // m.
+  double floor_y;            // m.
+  double low_z;              // m.
+  double high_z;             // m.
+  double piston_velocity_x;  // m/s.
+};
+using LiquidObstacle3D =
+    std::variant<std::monostate, TranslatingHalfspace3D, TranslatingSlab3D, AxisAlignedLiquidTank3D>;
 struct LiquidBoundaryGeometryInputs3D {
   const SparseMacGrid3D& mac_grid;
   std::span<const CellIndex3D> cell_indices;  // Candidates, strictly (z,y,x) sorted.

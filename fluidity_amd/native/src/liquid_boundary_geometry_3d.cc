@@ -183,10 +183,12 @@ struct FreezeAccess {
     const auto c3 = [](Vec3 v) { return fl_vec3{v.x, v.y, v.z}; };
     if (const auto* h = std::get_if<TranslatingHalfspace3D>(&in.obstacle)) {
       r.obstacle = {FL_OBSTACLE_HALFSPACE, 0, c3(h->unit_normal), h->start_plane_coordinate, 0.0,
-                    c3(h->wall_velocity)};
+                    c3(h->wall_velocity), {}};
     } else if (const auto* b = std::get_if<TranslatingSlab3D>(&in.obstacle)) {
       r.obstacle = {FL_OBSTACLE_SLAB, 0, c3(b->unit_normal), b->start_center_coordinate, b->half_width,
-                    c3(b->wall_velocity)};
+                    c3(b->wall_velocity), {}};
+    } else if (std::holds_alternative<AxisAlignedLiquidTank3D>(in.obstacle)) {
+      r.obstacle.kind = FL_OBSTACLE_TANK;  // refused by the device freeze
     }
     r.step_duration = in.step_duration;
     r.liquid_density = in.liquid_density;

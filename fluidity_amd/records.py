@@ -89,8 +89,10 @@ LIQUID_PARTICLE = np.dtype({"names": ["id", "tag", "world_position", "world_velo
                                         "u1", (CONSTITUENT, 8)],
                             "offsets": [0, 8, 16, 40, 64, 136, 144, 152, 160, 168, 176, 184, 192, 193, 200],
                             "itemsize": 328})
+TANK = np.dtype([("start_low_x", "f8"), ("high_x", "f8"), ("floor_y", "f8"), ("low_z", "f8"), ("high_z", "f8"),
+                 ("piston_velocity_x", "f8")])
 OBSTACLE = np.dtype([("kind", "u4"), ("pad", "u4"), ("unit_normal", VEC3), ("coordinate", "f8"), ("half_width", "f8"),
-                     ("wall_velocity", VEC3)])
+                     ("wall_velocity", VEC3), ("tank", TANK)])
 UPDATE_OPTIONS = np.dtype([("cell_width", "f8"), ("step_duration", "f8"), ("particle_energy_blend", "f8"),
                            ("friction_coefficient", "f8"), ("penetration_tolerance", "f8"), ("obstacle", OBSTACLE)])
 TOPOLOGY_CAPACITIES = np.dtype([("particles", "u8"), ("blocks", "u8"), ("cells", "u8"), ("faces", "u8")])
@@ -101,5 +103,5 @@ LINK_KINDS = {"sealed": SEALED, "internal": INTERNAL, "dirichlet": DIRICHLET}
 assert PARTICLE.itemsize == 152 and GATHER_RESULT.itemsize == 192 and SORT_ENTRY.itemsize == 24
 assert PRESSURE_CELL.itemsize == 56 and PRESSURE_FACE.itemsize == 80 and FACE_QUANTA.itemsize == 24
 assert BLOCK_STATE.itemsize == 32 and CELL_STATE.itemsize == 24 and FACE_STATE.itemsize == 56
-assert LIQUID_PARTICLE.itemsize == 328 and UPDATE_OPTIONS.itemsize == 112 and OBSTACLE.itemsize == 72
+assert LIQUID_PARTICLE.itemsize == 328 and UPDATE_OPTIONS.itemsize == 160 and OBSTACLE.itemsize == 120
 assert FROZEN_FACE.itemsize == 72 and FROZEN_CELL.itemsize == 40 and CELL_FIELDS.itemsize == 40

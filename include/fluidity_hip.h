@@ -208,7 +208,17 @@ typedef struct fl_frozen_view {
 /* The frozen-geometry obstacles (native/include/fluidity/host/liquid_boundary_geometry_3d.h:30-43): absent,
  * TranslatingHalfspace3D (solid where unit_normal . x <= coordinate) and TranslatingSlab3D (solid where
  * |unit_normal . x - coordinate| <= half_width). */
-enum { FL_OBSTACLE_ABSENT = 0, FL_OBSTACLE_HALFSPACE = 1, FL_OBSTACLE_SLAB = 2 };
+enum { FL_OBSTACLE_ABSENT = 0, FL_OBSTACLE_HALFSPACE = 1, FL_OBSTACLE_SLAB = 2, FL_OBSTACLE_TANK = 3 };
+/* AxisAlignedLiquidTank3D (native/include/fluidity/base/liquid_boundary_math.h:22-29): walls at the
+ * piston x (moving at piston_velocity_x), high x, the floor and low / high z; the top is open. */
+typedef struct fl_tank {
+  double start_low_x;
+  double high_x;
+  double floor_y;
+  double low_z;
+  double high_z;
+  double piston_velocity_x;
+} fl_tank; /* 48 B */
 typedef struct fl_obstacle {
   uint32_t kind;
   uint32_t pad;
@@ -216,7 +226,8 @@ typedef struct fl_obstacle {
   double coordinate;  /* half-space: start plane coordinate; slab: start centre coordinate */
   double half_width;  /* slab only */
   fl_vec3 wall_velocity;
-} fl_obstacle; /* 72 B */
+  fl_tank tank;       /* open tank only: the particle update takes it, the freeze refuses it */
+} fl_obstacle; /* 120 B */
 /* LiquidBoundaryGeometryViewInputs3D (native/include/fluidity/host/liquid_boundary_workspace_3d.h:18-30)
  * without a sampled solid; corner samples (LiquidCornerDistance3D) use the centre-sample record.
  * max_cells and max_faces give the reference's capacity refusals at its points (0: none). */
@@ -288,8 +299,8 @@ typedef struct fl_particle_update_options {
   double particle_energy_blend;
   double friction_coefficient;
   double penetration_tolerance;
-  fl_obstacle obstacle; /* absent (all zero), half-space or slab */
-} fl_particle_update_options; /* 112 B */
+  fl_obstacle obstacle; /* absent (all zero), half-space, slab or open tank */
+} fl_particle_update_options; /* 160 B */
 /* Topology records of a liquid state (native/include/fluidity/host/liquid_state_3d.h:129-159):
  * LiquidBlockState3D, LiquidCellState3D (occupancy 0 air, 1 liquid, 2 solid) and LiquidFaceState3D
  * (axis 0..2 with the lattice index at offset 4; validity 0 missing, 1 valid, 2 predicted, 3 low mass). */

@@ -231,14 +231,17 @@ int flo_build_pressure_links(const flo_frozen_view* view, const flo_pressure_fie
 // Frozen-geometry inputs (LiquidBoundaryGeometryViewInputs3D, native/include/fluidity/host/
 // liquid_boundary_workspace_3d.h:18-30) for the absent, half-space and slab obstacles
 // (liquid_boundary_geometry_3d.h:30-43). Corner samples use the centre-sample record.
-enum { FLO_OBSTACLE_ABSENT = 0, FLO_OBSTACLE_HALFSPACE = 1, FLO_OBSTACLE_SLAB = 2 };
+enum { FLO_OBSTACLE_ABSENT = 0, FLO_OBSTACLE_HALFSPACE = 1, FLO_OBSTACLE_SLAB = 2, FLO_OBSTACLE_TANK = 3 };
+// AxisAlignedLiquidTank3D (native/include/fluidity/base/liquid_boundary_math.h:22-29).
+struct flo_tank { double start_low_x, high_x, floor_y, low_z, high_z, piston_velocity_x; };  // 48 B
 struct flo_obstacle {
   uint32_t kind, pad;
   flo_vec3 unit_normal;
   double coordinate;  // half-space: start plane coordinate; slab: start centre coordinate
   double half_width;  // slab only
   flo_vec3 wall_velocity;
-};  // 72 B
+  flo_tank tank;      // open tank only (the particle update; the freeze refuses it)
+};  // 120 B
 struct flo_freeze_inputs {
   const flo_int3* blocks;
   uint64_t block_count;

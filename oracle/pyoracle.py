@@ -99,8 +99,10 @@ LIQUID_PARTICLE = np.dtype({"names": ["id", "tag", "world_position", "world_velo
                                         "u1", (CONSTITUENT, 8)],
                             "offsets": [0, 8, 16, 40, 64, 136, 144, 152, 160, 168, 176, 184, 192, 193, 200],
                             "itemsize": 328})
+TANK = np.dtype([("start_low_x", "f8"), ("high_x", "f8"), ("floor_y", "f8"), ("low_z", "f8"), ("high_z", "f8"),
+                 ("piston_velocity_x", "f8")])
 OBSTACLE = np.dtype([("kind", "u4"), ("pad", "u4"), ("unit_normal", VEC3), ("coordinate", "f8"), ("half_width", "f8"),
-                     ("wall_velocity", VEC3)])
+                     ("wall_velocity", VEC3), ("tank", TANK)])
 UPDATE_OPTIONS = np.dtype([("cell_width", "f8"), ("step_duration", "f8"), ("particle_energy_blend", "f8"),
                            ("friction_coefficient", "f8"), ("penetration_tolerance", "f8"), ("obstacle", OBSTACLE)])
 TOPOLOGY_CAPACITIES = np.dtype([("particles", "u8"), ("blocks", "u8"), ("cells", "u8"), ("faces", "u8")])
@@ -321,11 +323,14 @@ def update_options(cell_width, step_duration, blend, friction=0.0, penetration=0
     o[0]["cell_width"], o[0]["step_duration"], o[0]["particle_energy_blend"] = cell_width, step_duration, blend
     o[0]["friction_coefficient"], o[0]["penetration_tolerance"] = friction, penetration
     if obstacle is not None:
-        kind, n, c, w, v = obstacle
+        kind, n, c, w, v = obstacle[:5]
         ob = o["obstacle"]
         ob["kind"], ob["coordinate"], ob["half_width"] = kind, c, w
         for k, a, b in zip("xyz", n, v):
             ob["unit_normal"][k], ob["wall_velocity"][k] = a, b
+        if len(obstacle) > 5:
+            for name, value in zip(TANK.names, obstacle[5]):
+                ob["tank"][name] = value
         o["obstacle"] = ob
     return o
 
@@ -393,12 +398,12 @@ def build_pressure_links(problem):
     return s, cells, faces, lower.value
 
 
-OBSTACLE_ABSENT, OBSTACLE_HALFSPACE, OBSTACLE_SLAB = 0, 1, 2
+OBSTACLE_ABSENT, OBSTACLE_HALFSPACE, OBSTACLE_SLAB, OBSTACLE_TANK = 0, 1, 2, 3
 
 
 class ObstacleC(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("pad", C.c_uint32), ("unit_normal", C.c_double * 3), ("coordinate", C.c_double),
-                ("half_width", C.c_double), ("wall_velocity", C.c_double * 3)]
+                ("half_width", C.c_double), ("wall_velocity", C.c_double * 3), ("tank", C.c_double * 6)]
 
 
 class FreezeInputsC(C.Structure):
@@ -411,7 +416,7 @@ class FreezeInputsC(C.Structure):
                 ("max_faces", C.c_uint64)]
 
 
-assert C.sizeof(ObstacleC) == 72
+assert C.sizeof(ObstacleC) == 120
 
 
 def halfspace(normal, plane, velocity=(0.0, 0.0, 0.0)):
@@ -420,6 +425,12 @@ def halfspace(normal, plane, velocity=(0.0, 0.0, 0.0)):
 
 def slab(normal, center, half_width, velocity=(0.0, 0.0, 0.0)):
     return (OBSTACLE_SLAB, tuple(normal), center, half_width, tuple(velocity))
+
+
+def tank(start_low_x, high_x, floor_y, low_z, high_z, piston_velocity_x=0.0):
+    """An open tank (particle update only); the tuple's last element holds its six fields."""
+    return (OBSTACLE_TANK, (0.0, 0.0, 0.0), 0.0, 0.0, (0.0, 0.0, 0.0),
+            (start_low_x, high_x, floor_y, low_z, high_z, piston_velocity_x))
 
 
 class FreezeProblem:
@@ -437,8 +448,9 @@ class FreezeProblem:
     def c_obstacle(self):
         if self.obstacle is None:
             return ObstacleC()
-        kind, n, c, w, v = self.obstacle
-        return ObstacleC(kind, 0, (C.c_double * 3)(*n), c, w, (C.c_double * 3)(*v))
+        kind, n, c, w, v = self.obstacle[:5]
+        t = self.obstacle[5] if len(self.obstacle) > 5 else (0.0,) * 6
+        return ObstacleC(kind, 0, (C.c_double * 3)(*n), c, w, (C.c_double * 3)(*v), (C.c_double * 6)(*t))
 
     def c_inputs(self):
         o = self.geom[0]["world_origin"]

@@ -1,6 +1,6 @@
 // Oracle: the liquid particle update (test infrastructure only).
 // Restates UpdateParticle (native/src/liquid_substep_3d_cpu.cc:239-405) without regular-cell transport
-// for the absent, half-space and slab obstacles: SolidDistance (native/src/liquid_substep_3d_staging.cc:
+// for the absent, half-space, slab and open-tank obstacles: SolidDistance (native/src/liquid_substep_3d_staging.cc:
 // 197-205), ResolveContact with its certified swept-plane contact
 // (native/include/fluidity/base/liquid_wall_contact_math_3d.h:1-152,365-556) over the bounded projected
 // sums of native/include/fluidity/base/liquid_boundary_math.h:93-263, and the vector helpers of
@@ -303,11 +303,130 @@ int PlaneContact(const flo_update_options& o, flo_vec3 x, flo_vec3 stored, flo_v
   return FLO_OK;
 }
 
-// SolidDistance: the signed distance to the obstacle at its start (+infinity without one).
+// SolidDistance: the signed distance to the obstacle at its start (+infinity without one); for the tank
+// the least inward clearance (SampleTankWall, liquid_boundary_math.h:319-350), NaN when one is not
+// finite.
 double SolidDistance(const flo_obstacle& ob, flo_vec3 x) {
   if (ob.kind == FLO_OBSTACLE_HALFSPACE) return Dot(ob.unit_normal, x) - ob.coordinate;
   if (ob.kind == FLO_OBSTACLE_SLAB) return std::abs(Dot(ob.unit_normal, x) - ob.coordinate) - ob.half_width;
+  if (ob.kind == FLO_OBSTACLE_TANK) {
+    const flo_tank& t = ob.tank;
+    const double d[5] = {x.x - t.start_low_x, t.high_x - x.x, x.y - t.floor_y, x.z - t.low_z, t.high_z - x.z};
+    double least = d[0];
+    for (double v : d) {
+      if (!std::isfinite(v)) return std::numeric_limits<double>::quiet_NaN();
+      if (v < least) least = v;
+    }
+    return least;
+  }
   return kInf;
+}
+
+// CheckTankStep (liquid_boundary_math.h:252-276).
+int CheckTankStep(const flo_tank& t, double dt) {
+  if (!std::isfinite(t.start_low_x) || !std::isfinite(t.high_x) || !std::isfinite(t.floor_y) ||
+      !std::isfinite(t.low_z) || !std::isfinite(t.high_z) || !std::isfinite(t.piston_velocity_x))
+    return FLO_INVALID_ARGUMENT;
+  if (t.start_low_x >= t.high_x || t.low_z >= t.high_z) return FLO_INVALID_ARGUMENT;
+  const double displacement = t.piston_velocity_x * dt;
+  const double end_x = std::fma(t.piston_velocity_x, dt, t.start_low_x);
+  const double start_width = t.high_x - t.start_low_x, end_width = t.high_x - end_x, width_z = t.high_z - t.low_z;
+  if (!std::isfinite(displacement) || !std::isfinite(end_x) || !std::isfinite(start_width) ||
+      !std::isfinite(end_width) || !std::isfinite(width_z))
+    return FLO_OUT_OF_RANGE;
+  if (end_x >= t.high_x) return FLO_FAILED_PRECONDITION;
+  return FLO_OK;
+}
+
+// SweepTankAxis (liquid_wall_contact_math_3d.h:154-289): one Cartesian component against its walls,
+// each expressed along local x.
+int SweepTankAxis(const flo_update_options& o, const SweepPlane* walls, int count, double x0, double s0, double a0,
+                  double& out_x, double& out_stored) {
+  flo_vec3 start{x0, 0, 0}, stored{s0, 0, 0}, adv{a0, 0, 0};
+  const SweepPlane* carrying = nullptr;
+  const auto impact = [&](const SweepPlane& wall) {
+    const Bounds ss = RelativeNormalVelocity(wall, stored), as = RelativeNormalVelocity(wall, adv);
+    if (!ss.resolved() || !as.resolved()) return FLO_OUT_OF_RANGE;
+    if (ss.estimate < 0) stored.x = wall.wall.x;
+    if (as.estimate <= 0) {
+      adv.x = wall.wall.x;
+      carrying = &wall;
+    }
+    return FLO_OK;
+  };
+  for (int k = 0; k < count; ++k) {
+    const SweepPlane& wall = walls[k];
+    const Bounds sd = DistanceAlongPath(wall, start, {0, 0, 0}, 0);
+    int side = 0;
+    if (!sd.sign(&side)) return FLO_OUT_OF_RANGE;
+    if (side > 0) continue;
+    if (sd.lower < -o.penetration_tolerance) return FLO_FAILED_PRECONDITION;
+    const double wall_coordinate = wall.start * wall.normal.x;
+    const bool low_wall = wall.normal.x > 0;
+    const bool stored_in = low_wall ? stored.x < wall.wall.x : stored.x > wall.wall.x;
+    const bool adv_in = low_wall ? adv.x < wall.wall.x : adv.x > wall.wall.x;
+    if (side < 0 || stored_in || adv_in) {
+      if (int s = impact(wall)) return s;
+      start.x = wall_coordinate;
+    }
+  }
+  if (carrying == nullptr) {
+    for (int k = 0; k < count; ++k) {
+      const SweepPlane& wall = walls[k];
+      const Bounds as = RelativeNormalVelocity(wall, adv);
+      int direction = 0;
+      if (!as.sign(&direction)) return FLO_OUT_OF_RANGE;
+      if (direction >= 0) continue;
+      int end_side = 0;
+      if (!DistanceAlongPath(wall, start, adv, o.step_duration).sign(&end_side)) return FLO_OUT_OF_RANGE;
+      if (end_side > 0) continue;
+      if (end_side < 0) {
+        const Bounds sd = DistanceAlongPath(wall, start, {0, 0, 0}, 0);
+        if (!sd.resolved() || !as.resolved()) return FLO_OUT_OF_RANGE;
+        const double t = std::min(o.step_duration, sd.estimate / -as.estimate);
+        if (t <= 0 || t / o.step_duration == 0) return FLO_OUT_OF_RANGE;
+      }
+      if (int s = impact(wall)) return s;
+      break;
+    }
+  }
+  flo_vec3 end{0, 0, 0};
+  if (carrying == nullptr) {
+    end = Advance(start, adv, o.step_duration);
+  } else {
+    const SweepPlane& wall = *carrying;
+    end = Advance({wall.start * wall.normal.x, 0, 0}, wall.wall, o.step_duration);
+    if (!CertifySide(end, wall.normal,
+                     [&](flo_vec3 u) { return DistanceAlongPath(wall, u, {0, 0, 0}, o.step_duration); }))
+      return FLO_OUT_OF_RANGE;
+  }
+  for (int k = 0; k < count; ++k) {
+    const Bounds ed = DistanceAlongPath(walls[k], end, {0, 0, 0}, o.step_duration);
+    if (!ed.finite() || ed.lower < 0) return FLO_OUT_OF_RANGE;
+  }
+  out_x = end.x;
+  out_stored = stored.x;
+  return FLO_OK;
+}
+
+// ResolveTankContact (liquid_wall_contact_math_3d.h:290-363): frictionless, per axis x, y, z.
+int TankContact(const flo_update_options& o, flo_vec3 x, flo_vec3 stored, flo_vec3 adv, flo_vec3& position,
+                flo_vec3& stored_out) {
+  if (o.friction_coefficient != 0) return FLO_INVALID_ARGUMENT;
+  const flo_tank& t = o.obstacle.tank;
+  if (int s = CheckTankStep(t, o.step_duration)) return s;
+  const SweepPlane walls[3][2] = {
+      {{{1, 0, 0}, t.start_low_x, 0, {t.piston_velocity_x, 0, 0}}, {{-1, 0, 0}, -t.high_x, 0, {0, 0, 0}}},
+      {{{1, 0, 0}, t.floor_y, 0, {0, 0, 0}}, {{0, 0, 0}, 0, 0, {0, 0, 0}}},
+      {{{1, 0, 0}, t.low_z, 0, {0, 0, 0}}, {{-1, 0, 0}, -t.high_z, 0, {0, 0, 0}}}};
+  const int counts[3] = {2, 1, 2};
+  const double xs[3] = {x.x, x.y, x.z}, ss[3] = {stored.x, stored.y, stored.z}, as[3] = {adv.x, adv.y, adv.z};
+  double px[3], sx[3];
+  for (int a = 0; a < 3; ++a)
+    if (int s = SweepTankAxis(o, walls[a], counts[a], xs[a], ss[a], as[a], px[a], sx[a])) return s;
+  position = {px[0], px[1], px[2]};
+  stored_out = {sx[0], sx[1], sx[2]};
+  return FLO_OK;
 }
 
 int UpdateOne(const flo_update_options& o, const flo_boundary_gather& g, double liquid_distance,
@@ -334,6 +453,8 @@ int UpdateOne(const flo_update_options& o, const flo_boundary_gather& g, double 
     if (!Finite3(position)) return FLO_OUT_OF_RANGE;
   } else if (o.obstacle.kind == FLO_OBSTACLE_HALFSPACE || o.obstacle.kind == FLO_OBSTACLE_SLAB) {
     if (int s = PlaneContact(o, p.world_position, stored, advecting, position, new_stored)) return s;
+  } else if (o.obstacle.kind == FLO_OBSTACLE_TANK) {
+    if (int s = TankContact(o, p.world_position, stored, advecting, position, new_stored)) return s;
   } else {
     return FLO_INVALID_ARGUMENT;
   }

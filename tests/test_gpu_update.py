@@ -221,3 +221,41 @@ def test_random_walls(ctx, seed):
         hits = int((walled["world_velocity"]["x"] != base["world_velocity"]["x"]).sum())
         print("contact hits", hits, flush=True)
         assert hits > 100
+
+
+def test_reference_tank_cases(ctx):
+    import test_oracle_contact as K
+    for args, kw, _ in K.tank_cases():
+        check(ctx, *K.one(*args, **kw))
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_random_tank(ctx, seed):
+    """20,000 particles within 0.1 of the walls of an open tank with a moving piston, some 0.01 inside
+    (within the penetration tolerance): thousands of frictionless hits."""
+    rng = np.random.default_rng(seed)
+    n = 20000
+    o, g, d, p = random_case(rng, n, 0.25, 0.01)
+    lo, hi = np.array([0.0, 0.0, 0.0]), np.array([2.0, 1.0, 1.5])
+    pos = rng.uniform(lo, hi, (n, 3))
+    axis = rng.integers(0, 3, n)
+    side = rng.integers(0, 2, n)
+    depth = rng.uniform(-0.01, 0.1, n)
+    for k in range(n):
+        a = axis[k]
+        if a == 1:
+            pos[k, 1] = lo[1] + depth[k]  # floor only: the top is open
+        else:
+            pos[k, a] = lo[a] + depth[k] if side[k] == 0 else hi[a] - depth[k]
+    for a, c in enumerate("xyz"):
+        p["world_position"][c] = pos[:, a]
+        g["new_pic_velocity"][c] = rng.normal(0, 2.0, n)
+    o = O.update_options(0.25, 0.01, float(o["particle_energy_blend"][0]), 0.0, 0.01,
+                         O.tank(lo[0], hi[0], lo[1], lo[2], hi[2], 0.4))
+    assert check(ctx, o, g, d, p) == O.OK
+    s, base = O.update_particles(O.update_options(0.25, 0.01, float(o["particle_energy_blend"][0])), g, d, p)
+    _, walled = O.update_particles(o, g, d, p)
+    hits = int(np.sum(walled["world_velocity"]["x"] != base["world_velocity"]["x"]) +
+               np.sum(walled["world_velocity"]["y"] != base["world_velocity"]["y"]))
+    print("tank hits", hits, flush=True)
+    assert hits > 1000

@@ -153,3 +153,57 @@ def test_solid_distance_gates_surface_expansion():
     assert s == O.OK
     assert out[0]["world_position"]["x"] == np.float64(dt) * 1.0
     assert abs(out[1]["world_position"]["x"] - (1.0 + 0.5 * 0.495) * dt) <= 1e-15
+
+
+TANK = (0.25, 2, 0.125, 0.125, 0.75, 0.25)  # low x (piston, +0.25 m/s), high x, floor, low z, high z
+
+
+def tank_cases():
+    """The reference's open-tank cases (liquid_wall_contact_3d_cases.h:38-145): (args, kwargs, expected
+    position and stored velocity, or a status)."""
+    t = O.tank(*TANK)
+    return [(((0.5, 0.375, 0.375), (-1, -2, 0.125), (-0.25, -0.5, 0.25), t), {}, ((0.5, 0.125, 0.625), (0.25, 0, 0.125))),
+            (((1, 0.375, 0.5), (2, 0.75, 1), (1, 0.5, 0.125), t), {}, ((2, 0.875, 0.625), (0, 0.75, 1))),
+            (((1, 0.375, 0.5), (2, 0.75, 1), (1, 0.5, 1), t), {}, ((2, 0.875, 0.75), (0, 0.75, 0))),
+            (((0.5, 0.375, 0.375), (-1, -2, 1), (0, 2, 0), t), {}, ((0.5, 2.375, 0.375), (0.25, -2, 1))),
+            (((0.125, 0.5, 0.0625), (-1, 2, -0.5), (2, 0.25, -0.25), t), dict(penetration=0.125),
+             ((2, 0.75, 0.125), (0, 2, 0))),
+            (((0.125, 0.5, 0.0625), (-1, 2, -0.5), (2, 0.25, -0.25), t), dict(penetration=0.0625), O.FAILED_PRECONDITION),
+            (((0.5, 0.375, 0.375), (0, 0, 0), (0, 0, 0), t), dict(friction=0.1), O.INVALID_ARGUMENT),
+            (((0.5, 0.375, 0.375), (0, 0, 0), (0, 0, 0), O.tank(0.25, 2, 0.125, 0.125, 0.125, 0.25)), {},
+             O.INVALID_ARGUMENT),
+            (((0.5, 0.375, 0.375), (0, 0, 0), (0, 0, 0), O.tank(0.25, 2, 0.125, 0.125, 0.75, 1.75)), {},
+             O.FAILED_PRECONDITION),
+            (((0.5, 0.375, 0.375), (0, 0, 0), (0, 0, 0), O.tank(0.25, 2, 0.125, 0.125, 0.75, 2)), {},
+             O.FAILED_PRECONDITION),
+            (((0.5, 0.375, 0.375), (0, 0, 0), (0, 0, 0), O.tank(0.25, 2, 0.125, 0.125, 0.75, -KMAX)), dict(dt=2),
+             O.OUT_OF_RANGE),
+            (((0.5, 0.375, 0.375), (0, 0, 0), (0, 0, 0), O.tank(-KMAX, KMAX, 0, -1, 1, 0)), {}, O.OUT_OF_RANGE)]
+
+
+def test_open_tank_cases():
+    for args, kw, want in tank_cases():
+        s, x, v = run(*args, **kw)
+        if isinstance(want, tuple):
+            assert s == O.OK and near(x, want[0]) and near(v, want[1]), (args, s, x, v)
+        else:
+            assert s == want, (args, s)
+
+
+def test_tank_solid_distance_gates_surface_expansion():
+    """The tank's least inward clearance is the solid distance: a particle 0.5 h from the floor drops the
+    ASFLIP term of its advecting velocity, one 5 h away keeps it."""
+    h, dt = 0.1, 0.01
+    o = O.update_options(h, dt, 0.5, 0.0, 0.0, O.tank(0.0, 10.0, 0.0, 0.0, 10.0))
+    g = np.zeros(2, O.BOUNDARY_GATHER)
+    p = np.zeros(2, O.LIQUID_PARTICLE)
+    g["new_pic_velocity"]["x"] = 1.0
+    g["particle_minus_old_pic_velocity"]["x"] = 1.0
+    g["velocity_gradient"]["x_row"]["x"] = 0.5
+    p["world_position"]["x"], p["world_position"]["z"] = 5.0, 5.0
+    p["world_position"]["y"] = (0.05, 0.5)
+    p["volume_ratio"], p["pic_initialized"] = 1.0, 1
+    s, out = O.update_particles(o, g, np.full(2, -0.5 * h), p)
+    assert s == O.OK
+    assert out[0]["world_position"]["x"] == 5.0 + dt
+    assert abs(out[1]["world_position"]["x"] - (5.0 + (1.0 + 0.5 * 0.495) * dt)) <= 1e-15
