@@ -72,24 +72,45 @@ def copy_ceiling(nbytes=1 << 30, reps=5):
     return gbs.value
 
 
-def cpu_baseline(scene, ctx, report, sample, threads=1):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def projection_inputs(scene, ctx, report):
+    """The projection's inputs exactly as the device step froze them: the step's block list and u* at the
+    liquid cells' incident faces (owned-slot order), plus the GPU's accepted pressures and face velocities."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+
+    nb, nc = int(report.blocks), int(report.pressure_cells)
+    s, dl = ctx.scene_download(nb, 0, nc, what=("blocks", "u0", "ustar", "unew", "pressures"))
+    assert s == H.OK
+    slots = scene.pressure_face_slots(dl["blocks"])[0]
+    return dict(geom=scene.geometry_record().view(O.GEOMETRY), blocks=dl["blocks"].view(O.INT3),
+                cells=scene.pressure_cells().view(O.PRESSURE_CELL),
+                faces=scene.pressure_faces(dl["blocks"], dl["ustar"]).view(O.PRESSURE_FACE), slots=slots, dl=dl)
+
+
+def cpu_baseline(scene, pin, sample, threads=1):
     """The oracle (C++ restatement of the reference operators) timed on this host.
 
     Transfers are timed on `sample` particles of the scene and scaled to the full count, on one thread
     (the reference's semantics) and again on `threads` threads (per-thread integer accumulators, the
-    same bits); the projection is timed once on the full scene (every pressure cell, one thread: the
-    oracle's solver is serial). Returns particle-steps/s.
+    same bits). The projection of every pressure cell is timed twice on one thread (the oracle's solvers
+    are serial): with the reference's own preconditioner (aggregation multigrid, weighted Jacobi, FP64
+    Cholesky blocks) and with the geometric V-cycle the GPU runs; `value` uses the reference's.
     """
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
 
-    nb = int(report.blocks)
-    s, dl = ctx.scene_download(nb, 0, int(report.pressure_cells), what=("blocks", "u0", "ustar"))
-    assert s == H.OK
-    blocks = dl["blocks"].view(O.INT3)
-    geom = scene.geometry_record().view(O.GEOMETRY)
     sel = np.arange(min(sample, scene.count))
     parts = scene.particles_record(sel).view(O.PARTICLE)
+    geom, blocks, dl = pin["geom"], pin["blocks"], pin["dl"]
 
     def transfers(nthreads):
         t0 = time.perf_counter()
@@ -104,25 +125,35 @@ def cpu_baseline(scene, ctx, report, sample, threads=1):
         return t_p2g, t_g2p
 
     t_p2g, t_g2p = transfers(1)
-    faces = scene.pressure_faces(dl["blocks"], dl["ustar"]).view(O.PRESSURE_FACE)
-    cells = scene.pressure_cells().view(O.PRESSURE_CELL)
-    t0 = time.perf_counter()
-    s, out = O.project(geom, blocks, scene.dt, cells, faces, O.pressure_options(relative_target=1e-6, levels=6, coarse_sweeps=2))
-    t_proj = time.perf_counter() - t0
-    assert s == O.OK, s
+    args = (geom, blocks, scene.dt, pin["cells"], pin["faces"])
+    legs = {}
+    for name, opts in (("reference", O.pressure_options(relative_target=1e-6, reference=True)),
+                       ("geometric", O.pressure_options(relative_target=1e-6, levels=6, coarse_sweeps=2))):
+        t0 = time.perf_counter()
+        s, out = O.project(*args, opts)
+        t = time.perf_counter() - t0
+        assert s == O.OK, (name, s)
+        d = out["diagnostics"]
+        legs[name] = {"ms": t * 1e3, "iterations": int(d["iteration_count"]), "levels": int(d["multigrid_level_count"]),
+                      "relative_divergence": float(d["defect_l2"] / d["initial_defect_l2"])}
+        log(f"cpu projection ({name} preconditioner): {t:.1f}s, {legs[name]['iterations']} iterations")
+    t_proj = legs["reference"]["ms"] * 1e-3
     step_s = (t_p2g + t_g2p) / len(parts) * scene.count + t_proj
+    nc = len(pin["cells"])
     result = {
         "value": scene.count / step_s,
         "unit": "particle-steps/s",
         "cores": 1,
         "kind": "port",
         "sample": (f"{scene.name}: P2G+G2P on {len(parts):,} of {scene.count:,} particles scaled to all, plus the "
-                   f"full projection of {len(cells):,} cells ({int(out['diagnostics']['iteration_count'])} MGPCG "
-                   f"iterations); oracle C++ FP64, 1 thread"),
+                   f"full projection of {nc:,} cells with the reference's preconditioner "
+                   f"({legs['reference']['iterations']} PCG iterations); oracle C++ FP64, 1 thread"),
         "p2g_s_per_particle": t_p2g / len(parts),
         "g2p_s_per_particle": t_g2p / len(parts),
-        "projection_ms": t_proj * 1e3,
-        "cpu": platform.processor() or platform.machine(),
+        "projection_ms": legs["reference"]["ms"],
+        "projection": legs,
+        "cpu": cpu_model(),
+        "compiler": O.compiler(),
         "nproc": os.cpu_count(),
     }
     if threads > 1:
@@ -132,6 +163,40 @@ def cpu_baseline(scene, ctx, report, sample, threads=1):
                                "p2g_s_per_particle": m_p2g / len(parts), "g2p_s_per_particle": m_g2p / len(parts),
                                "note": "transfers threaded; projection serial (same time as above)"}
     return result
+
+
+def fp64_oracle_error(scene, pin):
+    """The GPU's accepted projection against a tight FP64 oracle solve of the same frozen system
+    (MGPCG in FP64 to 1e-12 relative divergence), and the GPU pressures re-evaluated by the oracle's FP64
+    Evaluate: relative pressure and velocity errors, divergence L2 / Linf."""
+    import pyoracle as O
+
+    args = (pin["geom"], pin["blocks"], scene.dt, pin["cells"], pin["faces"])
+    t0 = time.perf_counter()
+    s, ref = O.project(*args, O.pressure_options(relative_target=1e-12, fp32=False, levels=6, coarse_sweeps=2,
+                                                 tolerance_scale=1e-6))
+    assert s == O.OK, s
+    t_ref = time.perf_counter() - t0
+    p_gpu, p_ref = pin["dl"]["pressures"], ref["pressures"]
+    u_gpu, u_ref = pin["dl"]["unew"][pin["slots"]], ref["velocities"]
+    s, ev = O.evaluate(*args, p_gpu)
+    assert s == O.OK, s
+    d, rd = ev["diagnostics"], ref["diagnostics"]
+    out = {
+        "reference_solve": "oracle MGPCG, FP64 vectors and V-cycle, 1e-12 relative divergence",
+        "reference_relative_divergence": float(rd["defect_l2"] / rd["initial_defect_l2"]),
+        "pressure_rel_linf": float(np.abs(p_gpu - p_ref).max() / np.abs(p_ref).max()),
+        "pressure_rel_l2": float(np.linalg.norm(p_gpu - p_ref) / np.linalg.norm(p_ref)),
+        "velocity_rel_linf": float(np.abs(u_gpu - u_ref).max() / np.abs(u_ref).max()),
+        "velocity_rel_l2": float(np.linalg.norm(u_gpu - u_ref) / np.linalg.norm(u_ref)),
+        "divergence_l2": float(d["defect_l2"]),
+        "divergence_linf": float(d["defect_linf"]),
+        "relative_divergence": float(d["defect_l2"] / d["initial_defect_l2"]),
+        "oracle_solve_s": t_ref,
+    }
+    log(f"fp64 oracle: pressure error {out['pressure_rel_linf']:.2e} (Linf, relative), velocity "
+        f"{out['velocity_rel_l2']:.2e} (L2, relative), divergence {out['relative_divergence']:.2e} relative")
+    return out
 
 
 def learned_comparison(ctx, scene, dt, opts, args):
@@ -177,7 +242,7 @@ def learned_comparison(ctx, scene, dt, opts, args):
         "training_epochs": args.learn_epochs,
         "training_s": train_s,
         "loss_first_last": [float(losses[0]), float(losses[-1])],
-        "network": "3-level 3D U-Net 8/16/32 on 8^3 tiles, FP32",
+        "network": "3-level 3D U-Net 8/16/32 on 8^3 tiles: BF16 MFMA inference with FP32 accumulation, FP32 training",
     }
 
 
@@ -321,6 +386,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="threads of the all-cores CPU transfer baseline (16 = the host share of one GPU)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-oracle-error", action="store_true",
+                    help="skip the comparison of the GPU projection with a tight FP64 oracle solve")
     ap.add_argument("--copy-probe", action="store_true", help="only measure the plain-copy ceiling")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner comparison")
     ap.add_argument("--no-topology", action="store_true", help="skip the topology-preparation measurement")
@@ -420,10 +487,15 @@ def main():
     }
     if not args.no_learned:
         out["learned"] = learned_comparison(ctx, scene, dt, opts, args)
-    if not args.no_cpu_baseline:
-        t0 = time.perf_counter()
-        out["cpu_baseline"] = cpu_baseline(scene, ctx, last, args.cpu_sample, args.cpu_threads)
-        log(f"cpu baseline took {time.perf_counter() - t0:.1f}s")
+    if not (args.no_cpu_baseline and args.no_oracle_error):
+        pin = projection_inputs(scene, ctx, last)
+        if not args.no_oracle_error:
+            out["fp64_oracle_error"] = fp64_oracle_error(scene, pin)
+        if not args.no_cpu_baseline:
+            t0 = time.perf_counter()
+            out["cpu_baseline"] = cpu_baseline(scene, pin, args.cpu_sample, args.cpu_threads)
+            log(f"cpu baseline took {time.perf_counter() - t0:.1f}s")
+        del pin
     if not args.no_topology:  # last: span-based calls reuse the resident scene's buffers
         out["topology"] = topology_measurement(ctx, scene, cells, dt)
         out["particle_update"] = update_measurement(ctx)

@@ -114,6 +114,18 @@ struct flo_pressure_options {
   int32_t coarse_sweeps;               // symmetric RBGS sweeps on the coarsest level
   int32_t fp32_vcycle;                 // 1: V-cycle and CG vectors in FP32 (mixed), 0: all FP64
   double coarse_scale;                 // Galerkin coarse operator scale (1 = P^T A P)
+  int32_t preconditioner;              // 0: the geometric V-cycle above; 1: the reference's aggregation
+                                       // multigrid (levels, sweeps and coarse_scale then unused)
+  int32_t pad;
+};
+
+// One level of the reference's aggregation hierarchy (mac_pressure_3d_multigrid.cc:218-301).
+struct flo_hierarchy_level {
+  uint64_t level_count;      // levels of the whole hierarchy
+  uint64_t unknowns;         // unknowns of this level
+  uint64_t edges;            // edges of this level
+  uint64_t terminal_blocks;  // Cholesky blocks (last level only)
+  uint64_t max_terminal;     // largest terminal block
 };
 
 struct flo_pressure_diagnostics {
@@ -358,6 +370,23 @@ int flo_evaluate(const flo_geometry* geometry, const flo_int3* sorted_blocks, si
                  const flo_pressure_cell* cells, size_t cell_count, const flo_pressure_face* faces, size_t face_count,
                  const double* pressures, double* out_velocities, double* out_residuals, double* out_defects,
                  flo_pressure_diagnostics* out_diagnostics);
+
+// The reference hierarchy of a prepared system, level `level`: coordinates (3 per unknown), anchor
+// and diagonal, edges (lower, higher, conductance), aggregates into the next level (SIZE_MAX on the
+// last level) and, on level 0, the cell of each unknown. Arrays hold at most `capacity` entries
+// (edges: 3 * capacity); any may be null.
+int flo_reference_hierarchy(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count, double dt,
+                            const flo_pressure_cell* cells, size_t cell_count, const flo_pressure_face* faces,
+                            size_t face_count, uint64_t level, uint64_t capacity, flo_hierarchy_level* out_level,
+                            int32_t* coords, double* anchors, double* diagonal, uint64_t* edge_lo, uint64_t* edge_hi,
+                            double* edge_k, uint64_t* aggregates, uint64_t* unknown_to_cell);
+// One application of the reference V-cycle in FP64 (z = M r over the unknowns of level 0).
+int flo_reference_vcycle(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count, double dt,
+                         const flo_pressure_cell* cells, size_t cell_count, const flo_pressure_face* faces,
+                         size_t face_count, const double* r, double* z);
+
+// Compiler that built the oracle (for the CPU-baseline record).
+const char* flo_compiler(void);
 
 // ---- learned preconditioner network (unet.cc): 3-level U-Net on 8^3 tiles, FP64 ----
 // Parameters: per convolution layer, weights [cout][cin][27] then biases [cout]. Inputs [n][4][512]

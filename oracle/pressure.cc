@@ -14,9 +14,11 @@
 // Cholesky, mac_pressure_workspace_3d.cc:298-397) converges to the same acceptance gates;
 // parity is judged on the accepted projection, not iterate by iterate.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <numeric>
 #include <unordered_map>
 #include <vector>
@@ -684,6 +686,419 @@ int Solve(const System& sys, const flo_pressure_options& opt, const Criteria& cr
   return FLO_ABORTED;
 }
 
+// ---------------- the reference's own preconditioner: aggregation multigrid ----------------
+//
+// BuildMacPressureHierarchy3D (mac_pressure_3d_multigrid.cc:218-301): fine edges and anchors
+// rounded once to FP32 (:20-31, :238-275), levels completed by CompleteLevel (:42-87), coarsened by
+// floor-half aggregation inside connected components with the linked-pair fallback on a signed
+// plateau (Coarsen, :102-165) until every component of a coarse level has at most 32 unknowns (the
+// fine level: one), and the last level's components factored by FP64 Cholesky
+// (FactorTerminalBlocks, :167-214). The V-cycle is ApplyVcycle (mac_pressure_workspace_3d.cc:298-397):
+// one 2/3-weighted Jacobi sweep before and after the piecewise-constant coarse correction, the fine
+// level in the PCG's scalar, every coarse level in FP64. The PCG is SolvePrepared's
+// (mac_pressure_workspace_3d.cc:544-658): the true residual is recomputed only at recursive
+// convergence or on the last iteration. The dense shortcut for at most 16 cells with a tightened
+// tolerance (:493-543) is not restated; this solver is the CPU baseline of large systems.
+
+struct AggEdge {
+  size_t lo, hi;
+  double k;
+};
+struct AggNeighbour {
+  size_t other, edge;
+};
+struct AggLevel {
+  std::vector<flo_int3> coord;
+  std::vector<double> anchor, diag;
+  std::vector<AggEdge> edges;
+  std::vector<size_t> row;  // CSR offsets, size n + 1
+  std::vector<AggNeighbour> nbr;
+  std::vector<size_t> comp;  // connected-component root of each unknown
+  std::vector<size_t> agg;   // aggregate in the next level
+  std::vector<size_t> term_row, term_unknowns;  // terminal blocks of the last level
+  std::vector<size_t> term_offset;              // first factor entry of each block
+  std::vector<double> term_factor;              // lower Cholesky factors, row-major per block
+  size_t size() const { return coord.size(); }
+};
+struct AggHierarchy {
+  std::vector<size_t> unknown_to_cell, cell_to_unknown;
+  std::vector<AggLevel> levels;
+};
+
+// mac_pressure_3d_multigrid.cc:20-31
+int RoundToFine(double v, double& out) {
+  if (!FinitePositive(v) || v > std::numeric_limits<float>::max()) return FLO_OUT_OF_RANGE;
+  const float f = static_cast<float>(v);
+  if (!(std::isfinite(f) && f > 0.0f)) return FLO_OUT_OF_RANGE;
+  out = static_cast<double>(f);
+  return FLO_OK;
+}
+
+size_t HalvingRoot(std::vector<size_t>& r, size_t u) {
+  while (r[u] != u) {
+    r[u] = r[r[u]];
+    u = r[u];
+  }
+  return u;
+}
+
+int32_t FloorHalfSigned(int32_t v) { return v / 2 - (v % 2 < 0 ? 1 : 0); }
+
+// Diagonal, CSR rows (neighbours in ascending ordinal) and components from the edge list.
+int CompleteAggLevel(AggLevel& L) {
+  const size_t n = L.size();
+  L.diag = L.anchor;
+  L.row.assign(n + 1, 0);
+  L.comp.resize(n);
+  std::iota(L.comp.begin(), L.comp.end(), size_t{0});
+  for (const AggEdge& e : L.edges) {
+    L.diag[e.lo] += e.k;
+    L.diag[e.hi] += e.k;
+    ++L.row[e.lo + 1];
+    ++L.row[e.hi + 1];
+    const size_t a = HalvingRoot(L.comp, e.lo), b = HalvingRoot(L.comp, e.hi);
+    L.comp[std::max(a, b)] = std::min(a, b);
+  }
+  for (size_t u = 0; u < n; ++u) {
+    if (!FinitePositive(L.diag[u]) || !std::isfinite(L.anchor[u])) return FLO_OUT_OF_RANGE;
+    L.comp[u] = HalvingRoot(L.comp, u);
+    L.row[u + 1] += L.row[u];
+  }
+  std::vector<size_t> next(L.row.begin(), L.row.end() - 1);
+  L.nbr.resize(L.row.back());
+  for (size_t i = 0; i < L.edges.size(); ++i) {
+    L.nbr[next[L.edges[i].lo]++] = {L.edges[i].hi, i};
+    L.nbr[next[L.edges[i].hi]++] = {L.edges[i].lo, i};
+  }
+  for (size_t u = 0; u < n; ++u)
+    std::sort(L.nbr.begin() + static_cast<std::ptrdiff_t>(L.row[u]), L.nbr.begin() + static_cast<std::ptrdiff_t>(L.row[u + 1]),
+              [](const AggNeighbour& a, const AggNeighbour& b) { return a.other < b.other; });
+  return FLO_OK;
+}
+
+// Connected components, ordered by root ordinal, members ascending.
+std::vector<std::vector<size_t>> AggComponents(const AggLevel& L) {
+  std::vector<std::vector<size_t>> by_root(L.size());
+  for (size_t u = 0; u < L.size(); ++u) by_root[L.comp[u]].push_back(u);
+  std::vector<std::vector<size_t>> out;
+  for (auto& c : by_root)
+    if (!c.empty()) out.push_back(std::move(c));
+  return out;
+}
+
+int CoarsenAgg(AggLevel& F, AggLevel& C) {
+  constexpr size_t kFree = SIZE_MAX;
+  F.agg.assign(F.size(), kFree);
+  for (const auto& comp : AggComponents(F)) {
+    // Spatial groups in (z, y, x) order of the halved coordinates.
+    std::map<std::array<int32_t, 3>, std::vector<size_t>> groups;
+    for (size_t u : comp) {
+      const flo_int3 c = F.coord[u];
+      groups[{FloorHalfSigned(c.z), FloorHalfSigned(c.y), FloorHalfSigned(c.x)}].push_back(u);
+    }
+    if (groups.size() < comp.size() || comp.size() == 1) {
+      for (const auto& [key, members] : groups) {
+        const size_t a = C.coord.size();
+        C.coord.push_back({key[2], key[1], key[0]});
+        for (size_t u : members) F.agg[u] = a;
+      }
+      continue;
+    }
+    // Every unknown halves to its own coordinate (a signed plateau): pair each free unknown with its
+    // lowest free neighbour so that the level still shrinks.
+    for (size_t u : comp) {
+      if (F.agg[u] != kFree) continue;
+      const size_t a = C.coord.size();
+      const flo_int3 c = F.coord[u];
+      C.coord.push_back({FloorHalfSigned(c.x), FloorHalfSigned(c.y), FloorHalfSigned(c.z)});
+      F.agg[u] = a;
+      for (size_t o = F.row[u]; o < F.row[u + 1]; ++o) {
+        const size_t v = F.nbr[o].other;
+        if (F.agg[v] == kFree) {
+          F.agg[v] = a;
+          break;
+        }
+      }
+    }
+  }
+  C.anchor.assign(C.size(), 0.0);
+  for (size_t u = 0; u < F.size(); ++u) C.anchor[F.agg[u]] += F.anchor[u];
+  // Galerkin operator of piecewise-constant prolongation: edges inside an aggregate cancel, crossing
+  // conductances add once per aggregate pair.
+  std::map<std::pair<size_t, size_t>, double> crossing;
+  for (const AggEdge& e : F.edges) {
+    const size_t a = F.agg[e.lo], b = F.agg[e.hi];
+    if (a != b) crossing[{std::min(a, b), std::max(a, b)}] += e.k;
+  }
+  C.edges.reserve(crossing.size());
+  for (const auto& [ends, k] : crossing) {
+    if (!FinitePositive(k)) return FLO_OUT_OF_RANGE;
+    C.edges.push_back({ends.first, ends.second, k});
+  }
+  return CompleteAggLevel(C);
+}
+
+// Cholesky-Banachiewicz (row by row, lower factor) of each terminal component's dense matrix.
+int FactorAggTerminal(AggLevel& L) {
+  std::vector<size_t> local(L.size());
+  L.term_row.assign(1, 0);
+  L.term_offset.clear();
+  L.term_unknowns.clear();
+  L.term_factor.clear();
+  for (const auto& comp : AggComponents(L)) {
+    const size_t m = comp.size();
+    if (m > 32) return FLO_INTERNAL;
+    for (size_t i = 0; i < m; ++i) local[comp[i]] = i;
+    std::vector<double> a(m * m, 0.0);
+    for (size_t r = 0; r < m; ++r) {
+      const size_t u = comp[r];
+      a[r * m + r] = L.diag[u];
+      for (size_t o = L.row[u]; o < L.row[u + 1]; ++o) a[r * m + local[L.nbr[o].other]] = -L.edges[L.nbr[o].edge].k;
+    }
+    for (size_t r = 0; r < m; ++r) {
+      for (size_t c = 0; c <= r; ++c) {
+        double s = a[r * m + c];
+        for (size_t q = 0; q < c; ++q) s -= a[r * m + q] * a[c * m + q];
+        if (r == c) {
+          if (!FinitePositive(s)) return FLO_OUT_OF_RANGE;
+          a[r * m + r] = std::sqrt(s);
+        } else {
+          a[r * m + c] = s / a[c * m + c];
+          if (!std::isfinite(a[r * m + c])) return FLO_OUT_OF_RANGE;
+        }
+      }
+      for (size_t c = r + 1; c < m; ++c) a[r * m + c] = 0.0;
+    }
+    L.term_unknowns.insert(L.term_unknowns.end(), comp.begin(), comp.end());
+    L.term_row.push_back(L.term_unknowns.size());
+    L.term_offset.push_back(L.term_factor.size());
+    L.term_factor.insert(L.term_factor.end(), a.begin(), a.end());
+  }
+  return FLO_OK;
+}
+
+int BuildAggHierarchy(const System& sys, AggHierarchy& H) {
+  const size_t nc = sys.cells.size();
+  std::vector<char> pinned(nc, 0);
+  for (size_t g : sys.gauges) pinned[g] = 1;
+  H.cell_to_unknown.assign(nc, kNone);
+  H.unknown_to_cell.clear();
+  AggLevel fine;
+  for (size_t c = 0; c < nc; ++c) {
+    if (pinned[c]) continue;
+    H.cell_to_unknown[c] = H.unknown_to_cell.size();
+    H.unknown_to_cell.push_back(c);
+    fine.coord.push_back(sys.cells[c].index);
+    fine.anchor.push_back(sys.cells[c].compliance);
+  }
+  for (const Link& l : sys.links) {
+    if (l.conductance == 0.0) continue;
+    const size_t t = l.tail != kNone ? H.cell_to_unknown[l.tail] : kNone;
+    const size_t h = l.head != kNone ? H.cell_to_unknown[l.head] : kNone;
+    if (t != kNone && h != kNone) {
+      double k;
+      if (int s = RoundToFine(l.conductance, k)) return s;
+      fine.edges.push_back({std::min(t, h), std::max(t, h), k});
+    } else if (t != kNone) {
+      fine.anchor[t] += l.conductance;
+    } else if (h != kNone) {
+      fine.anchor[h] += l.conductance;
+    }
+  }
+  for (double& a : fine.anchor)
+    if (a != 0.0)
+      if (int s = RoundToFine(a, a)) return s;
+  std::sort(fine.edges.begin(), fine.edges.end(),
+            [](const AggEdge& a, const AggEdge& b) { return a.lo != b.lo ? a.lo < b.lo : a.hi < b.hi; });
+  if (int s = CompleteAggLevel(fine)) return s;
+  for (double d : fine.diag) {
+    double r;
+    if (int s = RoundToFine(d, r)) return s;
+  }
+  H.levels.clear();
+  H.levels.push_back(std::move(fine));
+  for (;;) {
+    const size_t limit = H.levels.size() == 1 ? 1 : 32;
+    bool coarsen = false;
+    for (const auto& c : AggComponents(H.levels.back())) coarsen |= c.size() > limit;
+    if (!coarsen) break;
+    AggLevel coarse;
+    if (int s = CoarsenAgg(H.levels.back(), coarse)) return s;
+    H.levels.push_back(std::move(coarse));
+  }
+  return FactorAggTerminal(H.levels.back());
+}
+
+int ToFine(double v, float& out) {  // FineValue (mac_pressure_workspace_3d.cc:212-223)
+  if (!std::isfinite(v) || std::fabs(v) > std::numeric_limits<float>::max()) return FLO_OUT_OF_RANGE;
+  const float r = static_cast<float>(v);
+  if (!std::isfinite(r) || (v != 0.0 && r == 0.0f)) return FLO_OUT_OF_RANGE;
+  out = r;
+  return FLO_OK;
+}
+int ToFine(double v, double& out) {
+  out = v;
+  return FLO_OK;
+}
+
+// ApplyLevel (mac_pressure_workspace_3d.cc:228-246) in scalar S.
+template <typename S>
+int AggApply(const AggLevel& L, const std::vector<S>& x, std::vector<S>& y) {
+  y.resize(L.size());
+  for (size_t u = 0; u < L.size(); ++u) {
+    S acc = static_cast<S>(L.anchor[u]) * x[u];
+    for (size_t o = L.row[u]; o < L.row[u + 1]; ++o) {
+      const S diff = x[u] - x[L.nbr[o].other];
+      const S flux = static_cast<S>(L.edges[L.nbr[o].edge].k) * diff;
+      acc += flux;
+    }
+    if (!std::isfinite(acc)) return FLO_OUT_OF_RANGE;
+    y[u] = acc;
+  }
+  return FLO_OK;
+}
+
+template <typename S>
+int AggVcycle(const AggHierarchy& H, size_t l, const std::vector<S>& r, std::vector<S>& x) {
+  const AggLevel& L = H.levels[l];
+  x.assign(L.size(), S(0));
+  if (l + 1 == H.levels.size()) {
+    double y[32];
+    for (size_t b = 0; b + 1 < L.term_row.size(); ++b) {
+      const size_t begin = L.term_row[b], m = L.term_row[b + 1] - begin;
+      const size_t* unk = L.term_unknowns.data() + begin;
+      const double* f = L.term_factor.data() + L.term_offset[b];
+      for (size_t i = 0; i < m; ++i) {  // L y = r
+        double s = r[unk[i]];
+        for (size_t j = 0; j < i; ++j) s -= f[i * m + j] * y[j];
+        y[i] = s / f[i * m + i];
+      }
+      for (size_t k = m; k > 0; --k) {  // L^T z = y
+        const size_t i = k - 1;
+        double s = y[i];
+        for (size_t j = i + 1; j < m; ++j) s -= f[j * m + i] * y[j];
+        y[i] = s / f[i * m + i];
+        if (!std::isfinite(y[i])) return FLO_OUT_OF_RANGE;
+        if (int st = ToFine(y[i], x[unk[i]])) return st;
+      }
+    }
+    return FLO_OK;
+  }
+  const S w = S(2) / S(3);
+  for (size_t u = 0; u < L.size(); ++u) x[u] = w * (r[u] / static_cast<S>(L.diag[u]));
+  std::vector<S> flux;
+  if (int s = AggApply(L, x, flux)) return s;
+  const AggLevel& C = H.levels[l + 1];
+  std::vector<double> cr(C.size(), 0.0), cx;
+  for (size_t u = 0; u < L.size(); ++u) {
+    const S rem = r[u] - flux[u];
+    cr[L.agg[u]] += static_cast<double>(rem);
+  }
+  for (double v : cr)
+    if (!std::isfinite(v)) return FLO_OUT_OF_RANGE;
+  if (int s = AggVcycle<double>(H, l + 1, cr, cx)) return s;
+  for (size_t u = 0; u < L.size(); ++u) {
+    S c;
+    if (int s = ToFine(cx[L.agg[u]], c)) return s;
+    x[u] += c;
+  }
+  if (int s = AggApply(L, x, flux)) return s;
+  for (size_t u = 0; u < L.size(); ++u) {
+    const S rem = r[u] - flux[u];
+    x[u] += w * (rem / static_cast<S>(L.diag[u]));
+    if (!std::isfinite(x[u])) return FLO_OUT_OF_RANGE;
+  }
+  return FLO_OK;
+}
+
+template <typename T>
+int DotChecked(const std::vector<T>& a, const std::vector<T>& b, double& out) {
+  out = Dot(a, b);
+  return std::isfinite(out) ? FLO_OK : FLO_OUT_OF_RANGE;
+}
+
+// SolvePrepared's PCG (mac_pressure_workspace_3d.cc:544-658) with the aggregation V-cycle.
+template <typename T>
+int SolveAggregation(const System& sys, const flo_pressure_options& opt, const Criteria& crit, std::vector<double>& p,
+                     std::vector<double>& vel, std::vector<double>& res, std::vector<double>& def, Diag& diag,
+                     flo_pressure_diagnostics& out) {
+  AggHierarchy H;
+  if (int s = BuildAggHierarchy(sys, H)) return s;
+  out.multigrid_level_count = H.levels.size();
+  std::vector<double> base = p;
+  if (int s = Evaluate(sys, p, vel, res, def, diag)) return s;
+  out.iteration_count = 0;
+  if (Accept(diag, crit)) return FLO_OK;
+  const size_t n = H.unknown_to_cell.size();
+  if (n == 0) return FLO_ABORTED;
+  std::vector<T> r(n), x(n, T(0)), d(n, T(0)), q, z;
+  auto load_residual = [&]() -> int {
+    for (size_t u = 0; u < n; ++u)
+      if (int s = ToFine(res[H.unknown_to_cell[u]], r[u])) return s;
+    return FLO_OK;
+  };
+  if (int s = load_residual()) return s;
+  double rho_prev = 0.0;
+  bool restart = true;
+  for (uint64_t it = 1; it <= opt.max_iterations; ++it) {
+    if (int s = AggVcycle<T>(H, 0, r, z)) return s;
+    double rho;
+    if (int s = DotChecked(r, z, rho)) return s;
+    if (rho <= 0.0) return FLO_ABORTED;
+    if (restart) {
+      d = z;
+    } else {
+      T beta;
+      if (int s = ToFine(rho / rho_prev, beta)) return s;
+      for (size_t u = 0; u < n; ++u) {
+        const T prev = beta * d[u];
+        d[u] = z[u] + prev;
+      }
+    }
+    if (int s = AggApply(H.levels[0], d, q)) return s;
+    double curv;
+    if (int s = DotChecked(d, q, curv)) return s;
+    if (curv <= 0.0) return FLO_ABORTED;
+    T alpha;
+    if (int s = ToFine(rho / curv, alpha)) return s;
+    double rn = 0.0, rmax = 0.0, dn = 0.0;
+    for (size_t u = 0; u < n; ++u) {
+      const T dp = alpha * d[u];
+      const T dq = alpha * q[u];
+      x[u] += dp;
+      r[u] -= dq;
+      if (!std::isfinite(x[u]) || !std::isfinite(r[u])) return FLO_OUT_OF_RANGE;
+      const double rv = static_cast<double>(r[u]);
+      const double V = sys.cells[H.unknown_to_cell[u]].volume;
+      rn = std::hypot(rn, rv);
+      rmax = std::max(rmax, std::fabs(rv) / V);
+      dn = std::hypot(dn, rv / V);
+    }
+    restart = false;
+    rho_prev = rho;
+    const bool last = it == opt.max_iterations;
+    bool recursive = rn <= crit.integrated && rmax <= crit.volume;
+    if (crit.relative > 0.0) recursive = recursive && dn <= crit.relative * crit.initial_defect;
+    if (last || recursive) {
+      p = base;
+      for (size_t u = 0; u < n; ++u) {
+        p[H.unknown_to_cell[u]] += static_cast<double>(x[u]);
+        if (!std::isfinite(p[H.unknown_to_cell[u]])) return FLO_OUT_OF_RANGE;
+      }
+      if (int s = Evaluate(sys, p, vel, res, def, diag)) return s;
+      out.iteration_count = it;
+      if (Accept(diag, crit)) return FLO_OK;
+      if (last) break;
+      base = p;
+      std::fill(x.begin(), x.end(), T(0));
+      if (int s = load_residual()) return s;
+      restart = true;
+      ++out.restarts;
+    }
+  }
+  return FLO_ABORTED;
+}
+
 int PrepareAndInit(const flo_geometry* g, const flo_int3* blocks, size_t nb, double dt, const flo_pressure_cell* cells,
                    size_t nc, const flo_pressure_face* faces, size_t nf, System& sys, std::vector<double>& p) {
   if (int s = Prepare(*g, blocks, nb, dt, cells, nc, faces, nf, sys)) return s;
@@ -708,7 +1123,7 @@ int flo_project(const flo_geometry* geometry, const flo_int3* sorted_blocks, siz
                 double* out_residuals, double* out_defects, uint64_t* out_gauges,
                 flo_pressure_diagnostics* out_diagnostics) {
   const flo_pressure_options opt = *options;
-  if (opt.max_iterations < 1) return FLO_INVALID_ARGUMENT;
+  if (opt.max_iterations < 1 || opt.preconditioner < 0 || opt.preconditioner > 1) return FLO_INVALID_ARGUMENT;
   if (!(opt.cpu_residual_tolerance_scale >= 1e-12 && opt.cpu_residual_tolerance_scale <= 1.0))
     return FLO_INVALID_ARGUMENT;
   System sys;
@@ -738,8 +1153,13 @@ int flo_project(const flo_geometry* geometry, const flo_int3* sorted_blocks, siz
   std::vector<double> vel, res, def;
   Diag diag;
   flo_pressure_diagnostics out{};
-  int s = opt.fp32_vcycle ? Solve<float>(sys, opt, crit, p, vel, res, def, diag, out)
-                          : Solve<double>(sys, opt, crit, p, vel, res, def, diag, out);
+  int s;
+  if (opt.preconditioner == 1)
+    s = opt.fp32_vcycle ? SolveAggregation<float>(sys, opt, crit, p, vel, res, def, diag, out)
+                        : SolveAggregation<double>(sys, opt, crit, p, vel, res, def, diag, out);
+  else
+    s = opt.fp32_vcycle ? Solve<float>(sys, opt, crit, p, vel, res, def, diag, out)
+                        : Solve<double>(sys, opt, crit, p, vel, res, def, diag, out);
   out.true_residual_l2 = diag.l2;
   out.max_volume_scaled_residual = diag.max_res;
   out.max_cell_constraint_defect = diag.max_defect;
@@ -790,6 +1210,73 @@ int flo_evaluate(const flo_geometry* geometry, const flo_int3* sorted_blocks, si
     *out_diagnostics = o;
   }
   return FLO_OK;
+}
+
+int flo_reference_hierarchy(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count, double dt,
+                            const flo_pressure_cell* cells, size_t cell_count, const flo_pressure_face* faces,
+                            size_t face_count, uint64_t level, uint64_t capacity, flo_hierarchy_level* out_level,
+                            int32_t* coords, double* anchors, double* diagonal, uint64_t* edge_lo, uint64_t* edge_hi,
+                            double* edge_k, uint64_t* aggregates, uint64_t* unknown_to_cell) {
+  System sys;
+  std::vector<double> p;
+  if (int s = PrepareAndInit(geometry, sorted_blocks, block_count, dt, cells, cell_count, faces, face_count, sys, p))
+    return s;
+  AggHierarchy H;
+  if (int s = BuildAggHierarchy(sys, H)) return s;
+  if (level >= H.levels.size()) return FLO_OUT_OF_RANGE;
+  const AggLevel& L = H.levels[level];
+  if (L.size() > capacity || L.edges.size() > 3 * capacity) return FLO_RESOURCE_EXHAUSTED;
+  flo_hierarchy_level o{};
+  o.level_count = H.levels.size();
+  o.unknowns = L.size();
+  o.edges = L.edges.size();
+  if (level + 1 == H.levels.size()) {
+    o.terminal_blocks = L.term_row.size() - 1;
+    for (size_t b = 0; b + 1 < L.term_row.size(); ++b)
+      o.max_terminal = std::max<uint64_t>(o.max_terminal, L.term_row[b + 1] - L.term_row[b]);
+  }
+  if (out_level) *out_level = o;
+  for (size_t u = 0; u < L.size(); ++u) {
+    if (coords) {
+      coords[3 * u] = L.coord[u].x;
+      coords[3 * u + 1] = L.coord[u].y;
+      coords[3 * u + 2] = L.coord[u].z;
+    }
+    if (anchors) anchors[u] = L.anchor[u];
+    if (diagonal) diagonal[u] = L.diag[u];
+    if (aggregates) aggregates[u] = L.agg.empty() ? UINT64_MAX : L.agg[u];
+    if (unknown_to_cell && level == 0) unknown_to_cell[u] = H.unknown_to_cell[u];
+  }
+  for (size_t e = 0; e < L.edges.size(); ++e) {
+    if (edge_lo) edge_lo[e] = L.edges[e].lo;
+    if (edge_hi) edge_hi[e] = L.edges[e].hi;
+    if (edge_k) edge_k[e] = L.edges[e].k;
+  }
+  return FLO_OK;
+}
+
+int flo_reference_vcycle(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count, double dt,
+                         const flo_pressure_cell* cells, size_t cell_count, const flo_pressure_face* faces,
+                         size_t face_count, const double* r, double* z) {
+  System sys;
+  std::vector<double> p;
+  if (int s = PrepareAndInit(geometry, sorted_blocks, block_count, dt, cells, cell_count, faces, face_count, sys, p))
+    return s;
+  AggHierarchy H;
+  if (int s = BuildAggHierarchy(sys, H)) return s;
+  const size_t n = H.unknown_to_cell.size();
+  std::vector<double> rv(r, r + n), zv;
+  if (int s = AggVcycle<double>(H, 0, rv, zv)) return s;
+  std::copy(zv.begin(), zv.end(), z);
+  return FLO_OK;
+}
+
+const char* flo_compiler(void) {
+#if defined(__clang__)
+  return "clang " __clang_version__;
+#else
+  return "g++ " __VERSION__;
+#endif
 }
 
 }  // extern "C"

@@ -57,7 +57,8 @@ PRESSURE_FACE = np.dtype([("axis", "i4"), ("lattice", INT3), ("open_area", "f8")
                           ("lift", "f8"), ("prescribed_wall_flux", "f8")])
 PRESSURE_OPTIONS = np.dtype([("max_iterations", "u8"), ("cpu_residual_tolerance_scale", "f8"),
                              ("relative_divergence_target", "f8"), ("levels", "i4"), ("smoothing_sweeps", "i4"),
-                             ("coarse_sweeps", "i4"), ("fp32_vcycle", "i4"), ("coarse_scale", "f8")])
+                             ("coarse_sweeps", "i4"), ("fp32_vcycle", "i4"), ("coarse_scale", "f8"),
+                             ("preconditioner", "i4"), ("pad", "i4")])
 PRESSURE_DIAGNOSTICS = np.dtype([("iteration_count", "u8"), ("multigrid_level_count", "u8"),
                                  ("true_residual_l2", "f8"), ("max_volume_scaled_residual", "f8"),
                                  ("max_cell_constraint_defect", "f8"), ("rhs_l2", "f8"),
@@ -516,9 +517,12 @@ def prepare_topology(geom, positions, cells=None, previous=None, capacities=None
 
 
 def pressure_options(max_iterations=256, tolerance_scale=1.0, relative_target=0.0, levels=4, sweeps=2,
-                     coarse_sweeps=16, fp32=True, coarse_scale=0.5):
+                     coarse_sweeps=16, fp32=True, coarse_scale=0.5, reference=False):
+    """Solver options; reference=True selects the reference's aggregation multigrid (Jacobi 2/3, FP64
+    Cholesky terminal blocks, restarts only at recursive convergence) instead of the geometric V-cycle."""
     o = np.zeros(1, PRESSURE_OPTIONS)
-    o[0] = (max_iterations, tolerance_scale, relative_target, levels, sweeps, coarse_sweeps, int(fp32), coarse_scale)
+    o[0] = (max_iterations, tolerance_scale, relative_target, levels, sweeps, coarse_sweeps, int(fp32), coarse_scale,
+            int(reference), 0)
     return o
 
 
@@ -542,6 +546,46 @@ def evaluate(geom, blocks, dt, cells, faces, pressures):
     s = lib().flo_evaluate(_p(geom), _p(blocks), C.c_size_t(len(blocks)), C.c_double(dt), _p(cells), C.c_size_t(nc),
                            _p(faces), C.c_size_t(nf), _p(pressures), _p(u), _p(r), _p(d), _p(diag))
     return s, dict(velocities=u, residuals=r, defects=d, diagnostics=diag[0])
+
+
+def compiler():
+    lib().flo_compiler.restype = C.c_char_p
+    return lib().flo_compiler().decode()
+
+
+HIERARCHY_LEVEL = np.dtype([("level_count", "u8"), ("unknowns", "u8"), ("edges", "u8"), ("terminal_blocks", "u8"),
+                            ("max_terminal", "u8")])
+
+
+def reference_hierarchy(geom, blocks, dt, cells, faces, level):
+    """Level `level` of the reference aggregation hierarchy: a dict of its counts and arrays."""
+    cap = len(cells)
+    info = np.zeros(1, HIERARCHY_LEVEL)
+    coords = np.zeros((cap, 3), np.int32)
+    anchors, diag = np.zeros(cap), np.zeros(cap)
+    lo, hi, k = np.zeros(3 * cap, np.uint64), np.zeros(3 * cap, np.uint64), np.zeros(3 * cap)
+    agg, u2c = np.zeros(cap, np.uint64), np.zeros(cap, np.uint64)
+    s = lib().flo_reference_hierarchy(_p(geom), _p(blocks), C.c_size_t(len(blocks)), C.c_double(dt), _p(cells),
+                                      C.c_size_t(len(cells)), _p(faces), C.c_size_t(len(faces)), C.c_uint64(level),
+                                      C.c_uint64(cap), _p(info), _p(coords), _p(anchors), _p(diag), _p(lo), _p(hi),
+                                      _p(k), _p(agg), _p(u2c))
+    if s != OK:
+        return s, None
+    i = info[0]
+    n, ne = int(i["unknowns"]), int(i["edges"])
+    return s, dict(level_count=int(i["level_count"]), terminal_blocks=int(i["terminal_blocks"]),
+                   max_terminal=int(i["max_terminal"]), coords=coords[:n], anchors=anchors[:n], diagonal=diag[:n],
+                   edges=np.stack([lo[:ne], hi[:ne]], 1).astype(np.int64), conductances=k[:ne],
+                   aggregates=agg[:n], unknown_to_cell=u2c[:n])
+
+
+def reference_vcycle(geom, blocks, dt, cells, faces, r):
+    """z = M r for the reference V-cycle in FP64 over the level-0 unknowns."""
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    z = np.zeros_like(r)
+    s = lib().flo_reference_vcycle(_p(geom), _p(blocks), C.c_size_t(len(blocks)), C.c_double(dt), _p(cells),
+                                   C.c_size_t(len(cells)), _p(faces), C.c_size_t(len(faces)), _p(r), _p(z))
+    return s, z
 
 
 # ---- learned preconditioner network (unet.cc) ----

@@ -439,3 +439,33 @@ def boundary_gather_scene(seed, n, h=0.5, blocks_per_axis=3, missing=0.05, low_m
             set_wall(w, sl, int(rng.integers(0, sides)), tuple(nrm[j]), tuple(rng.normal(0.0, 1.0, 3)))
         hist.append((float(k), u, v, tags, w, float(k)))
     return geom, blocks, p, hist[0], hist[1]
+
+
+def device_step_system(c, sc, report):
+    """The projection a device step froze: its block list and u* at the liquid cells' incident faces, with
+    the step's accepted pressures and corrected face velocities (at the same faces)."""
+    nb, nc = int(report.blocks), int(report.pressure_cells)
+    s, dl = c.scene_download(nb, 0, nc, what=("blocks", "ustar", "unew", "pressures"))
+    assert s == 0
+    slots = sc.pressure_face_slots(dl["blocks"])[0]
+    args = (sc.geometry_record().view(O.GEOMETRY), dl["blocks"].view(O.INT3), sc.dt,
+            sc.pressure_cells().view(O.PRESSURE_CELL), sc.pressure_faces(dl["blocks"], dl["ustar"]).view(O.PRESSURE_FACE))
+    return args, dl["pressures"], dl["unew"][slots]
+
+
+def assert_matches_fp64_oracle(args, pressures, velocities, tol=1e-5):
+    """Tight FP64 oracle solve of the same system; the GPU pressures re-evaluated by the oracle's FP64 Evaluate
+    meet the reference gates and the relative-divergence target."""
+    s, ref = O.project(*args, O.pressure_options(relative_target=1e-12, fp32=False, levels=6, coarse_sweeps=2))
+    assert s == O.OK
+    p_err = np.abs(pressures - ref["pressures"]).max() / np.abs(ref["pressures"]).max()
+    u_err = np.linalg.norm(velocities - ref["velocities"]) / np.linalg.norm(ref["velocities"])
+    s, ev = O.evaluate(*args, pressures)
+    assert s == O.OK
+    d = ev["diagnostics"]
+    print("fp64 oracle: pressure error", p_err, "velocity error", u_err, "divergence", d["defect_l2"],
+          "relative", d["defect_l2"] / d["initial_defect_l2"], flush=True)
+    assert p_err <= tol and u_err <= tol
+    assert d["defect_l2"] <= 1e-6 * d["initial_defect_l2"]
+    assert d["max_cell_constraint_defect"] <= 2e-5 and d["max_volume_scaled_residual"] <= 2e-5
+    return p_err, u_err

@@ -144,3 +144,39 @@ def test_learned_flexible_cg_meets_the_gates_and_training_generalises(ctx):
         # Every accepted projection passes the same FP64 gates whatever preconditioned it.
         assert d["defect_l2"] <= 1e-6 * d["initial_defect_l2"] and p_err <= 1e-5
     assert trained["diagnostics"]["learned_iterations"] > 0
+
+
+def test_learned_flexible_cg_at_128_meets_the_gates_and_the_fp64_oracle():
+    # The learned preconditioner at a benchmark size (dam break, 128^3 sparse, 502,656 liquid cells): the
+    # network is trained on pairs logged from the plain MGPCG solve of the same step, then the flexible CG
+    # with the network (V-cycle fallback after stagnation) projects the step. The accepted projection meets
+    # the FP64 gates and agrees with a tight FP64 oracle solve of the same frozen system.
+    sc = scenes.by_name("dam_break_128")
+    cells = sc.pressure_cells()
+    with H.Context(particles=sc.count + 1024, blocks=1 << 16, lookup_blocks=1 << 21, pressure_cells=len(cells) + 64,
+                   pressure_faces=4 * len(cells) + 4096) as c:
+        assert c.scene_upload(sc.geometry_record(), sc.positions, sc.velocities, sc.slopes, sc.mass) == H.OK
+        assert c.scene_prepare(cells, sc.n, sc.rho) == H.OK
+        net = H.Network(c, tiles=4096, pairs=24000)
+        assert net.set_params(O.unet_init(4).astype(np.float32)) == H.OK
+        opts = H.pressure_options(relative_target=1e-6, levels=5, coarse_sweeps=2, bf16_levels=True, smoother=2)
+        assert net.collect(1, clear=True) == H.OK
+        s, plain = c.step(sc.dt, sc.gravity, K.QUANTUM, K.QUANTUM, sc.count, opts)
+        assert s == H.OK, s
+        assert net.collect(0) == H.OK
+        print("pairs", net.pairs(), flush=True)
+        assert net.pairs() > 0
+        s, losses = net.train(epochs=4, batch=256, lr=2e-3, seed=3)
+        assert s == H.OK and np.all(np.isfinite(losses))
+        learned_opts = opts.copy()
+        learned_opts["preconditioner"] = 1
+        s, rep = c.step(sc.dt, sc.gravity, K.QUANTUM, K.QUANTUM, sc.count, learned_opts)
+        assert s == H.OK, s
+        d = rep.diagnostics()
+        print("learned", rep.projection_ms, "ms", d["iteration_count"], "iterations (", d["learned_iterations"],
+              "learned ); mgpcg", plain.projection_ms, "ms", plain.diagnostics()["iteration_count"], flush=True)
+        assert d["learned_iterations"] > 0
+        assert d["defect_l2"] <= 1e-6 * d["initial_defect_l2"]
+        assert d["max_cell_constraint_defect"] <= 2e-5 and d["max_volume_scaled_residual"] <= 2e-5
+        args, p, u = K.device_step_system(c, sc, rep)
+        K.assert_matches_fp64_oracle(args, p, u)

@@ -89,3 +89,209 @@ def test_mgpcg_fp32_refinement_reaches_relative_divergence_target():
     d = out["diagnostics"]
     assert d["defect_l2"] <= 1e-9 * d["initial_defect_l2"]
     assert d["restarts"] >= 1
+
+
+# ---- the reference's own preconditioner (aggregation multigrid, weighted Jacobi, FP64 Cholesky) ----
+
+def matched_error_cube():
+    """The uniform-finest layout of the reference's matched-error benchmark: the sealed cube [-1, 1]^3 of
+    8^3 cells of width 0.25 m, dt = 0.5 s, inverse density 0.5 m^3/kg, interior x-faces predicted at
+    dt * beta * 0.75 = 0.1875 m/s so that p = 0.125 + 0.75 x Pa zeroes the corrected field
+    (native/tests/matched_error_pressure_3d.cc:29-35, :80-139)."""
+    h, dt, beta = 0.25, 0.5, 0.5
+    idx = [(x, y, z) for z in range(-4, 4) for y in range(-4, 4) for x in range(-4, 4)]
+    cells = np.zeros(len(idx), O.PRESSURE_CELL)
+    for i, c in enumerate(idx):
+        cells[i]["cell_index"] = c
+        cells[i]["fluid_volume"] = h ** 3
+        cells[i]["bulk_modulus"] = math.inf
+    keys = set()
+    for (x, y, z) in idx:
+        for a in range(3):
+            for side in range(2):
+                lat = [x, y, z]
+                lat[a] += side
+                keys.add((a,) + tuple(lat))
+    blocks = sorted({(K.owner(x), K.owner(y), K.owner(z)) for (_, x, y, z) in keys}, key=lambda b: b[::-1])
+    recs = []
+    for (a, x, y, z) in keys:
+        lat = (x, y, z)
+        f = np.zeros(1, O.PRESSURE_FACE)[0]
+        f["axis"], f["lattice"] = a, lat
+        f["inverse_density"], f["pressure_gradient_distance"] = beta, h
+        if lat[a] in (-4, 4):
+            f["link_kind"] = 0  # sealed domain wall: no area, no wall motion
+        else:
+            f["link_kind"], f["open_area"] = 1, h * h
+            f["predicted_normal_velocity"] = dt * beta * 0.75 if a == 0 else 0.0
+        recs.append(f)
+    faces = np.array(recs, dtype=O.PRESSURE_FACE)
+    geom = O.geometry((0.0, 0.0, 0.0), h)
+    sb = K.sorted_blocks(geom, blocks)
+    slots = [O.face_slot(sb, int(f["axis"]), tuple(int(v) for v in f["lattice"]))[1] for f in faces]
+    return geom, sb, dt, cells, faces[np.argsort(np.array(slots))]
+
+
+def test_reference_preconditioner_reproduces_the_matched_error_benchmark():
+    # docs/adaptivity/matched-error-pressure.md:36-48 (uniform finest column): 11 solver iterations, maximum
+    # pressure error 1.19209e-6 Pa, maximum corrected divergence 5.96046e-6 1/s, maximum corrected face speed
+    # 5.96046e-7 m/s. Every one of these depends on the hierarchy, the Jacobi weight, the FP32 rounding points
+    # and the restart rule, so a restatement that differs anywhere moves at least one of them.
+    geom, sb, dt, cells, faces = matched_error_cube()
+    s, out = O.project(geom, sb, dt, cells, faces, O.pressure_options(reference=True))
+    assert s == O.OK
+    d = out["diagnostics"]
+    assert int(d["iteration_count"]) == 11
+    assert list(out["gauges"]) == [0]
+    exact = 0.125 + 0.75 * (cells["cell_index"]["x"] * 0.25 + 0.125)
+    p_err = np.abs(out["pressures"] - (exact - exact[0])).max()
+    assert f"{p_err:.5e}" == "1.19209e-06"
+    # divergence recomputed from the corrected velocities and the open areas (interior faces only)
+    flux = np.zeros(len(cells))
+    lookup = {tuple(c): i for i, c in enumerate(cells["cell_index"].tolist())}
+    for f, u in zip(faces, out["velocities"]):
+        lat = tuple(int(v) for v in f["lattice"])
+        tail = list(lat)
+        tail[int(f["axis"])] -= 1
+        for cell, sign in ((tuple(tail), 1.0), (lat, -1.0)):
+            if cell in lookup:
+                flux[lookup[cell]] += sign * f["open_area"] * u
+    assert f"{np.abs(flux / 0.25 ** 3).max():.5e}" == "5.96046e-06"
+    assert f"{np.abs(out['velocities']).max():.5e}" == "5.96046e-07"
+
+
+def graph_energy(level, p):
+    e = level["edges"]
+    d = p[e[:, 0]] - p[e[:, 1]]
+    return float(np.sum(level["anchors"] * p * p) + np.sum(level["conductances"] * d * d))
+
+
+def test_reference_hierarchy_preserves_galerkin_energy_and_bounds_terminal_blocks():
+    # MacPressure3D.SharedHierarchyPreservesGalerkinEnergyAndBoundsTerminalBlocks (test_mac_pressure_3d.cc:806-854)
+    geom, sb, cells, faces, pressure = K.affine_problem(-4, (8, 8, 8), True)
+    s, L0 = O.reference_hierarchy(geom, sb, 1.0, cells, faces, 0)
+    assert s == O.OK and L0["level_count"] >= 3
+    levels = [L0] + [O.reference_hierarchy(geom, sb, 1.0, cells, faces, k)[1] for k in range(1, L0["level_count"])]
+    for fine, coarse in zip(levels, levels[1:]):
+        pc = (np.arange(len(coarse["anchors"])) % 7).astype(np.float64) - 3.0
+        pf = pc[fine["aggregates"].astype(np.int64)]
+        ef, ec = graph_energy(fine, pf), graph_energy(coarse, pc)
+        assert abs(ef - ec) <= 4 * np.finfo(float).eps * abs(ef)
+        # each edge appears in both of its rows: the diagonal is the anchor plus every incident edge once
+        inc = np.zeros(len(fine["anchors"]))
+        np.add.at(inc, fine["edges"][:, 0], fine["conductances"])
+        np.add.at(inc, fine["edges"][:, 1], fine["conductances"])
+        np.testing.assert_allclose(fine["diagonal"], fine["anchors"] + inc, rtol=1e-15)
+        assert np.all(fine["edges"][:, 0] < fine["edges"][:, 1])
+    assert 0 < levels[-1]["max_terminal"] <= 32
+    s, out = O.project(geom, sb, 1.0, cells, faces, O.pressure_options(reference=True))
+    assert s == O.OK
+    d = out["diagnostics"]
+    assert int(d["multigrid_level_count"]) == L0["level_count"] and int(d["iteration_count"]) > 0
+    want = np.array([pressure(tuple(c)) for c in cells["cell_index"].tolist()])
+    np.testing.assert_allclose(out["pressures"], want, atol=5e-5)
+
+
+def test_reference_signed_plateau_uses_linked_pair_coarsening():
+    # MacPressure3D.SignedCoordinatePlateauUsesLinkedPairCoarsening (test_mac_pressure_3d.cc:856-878): cells
+    # -1 and 0 halve to -1 and 0, so floor-half grouping cannot shrink the level and unknowns are paired with
+    # a linked neighbour instead.
+    geom, sb, cells, faces, _ = K.affine_problem(-1, (2, 2, 2), True)
+    s, L0 = O.reference_hierarchy(geom, sb, 1.0, cells, faces, 0)
+    assert s == O.OK and L0["level_count"] == 2
+    s, L1 = O.reference_hierarchy(geom, sb, 1.0, cells, faces, 1)
+    assert len(L1["anchors"]) < len(L0["anchors"])
+    agg = L0["aggregates"]
+    for u in range(len(agg)):
+        partners = [v for a, b in L0["edges"] for v in ((b,) if a == u else (a,) if b == u else ()) if agg[v] == agg[u]]
+        assert partners, f"unknown {u} has no linked partner in its aggregate"
+
+
+def test_reference_eliminated_articulation_pin_keeps_separate_singletons():
+    # MacPressure3D.EliminatedArticulationPinRetainsConductanceInSeparateSingletons (test_mac_pressure_3d.cc:880-940):
+    # cells (0,0,0), (1,0,0), (0,1,0); the lowest is the gauge and joins the two arms, which become two
+    # singleton components anchored by its conductance, each solved by a 1x1 Cholesky block.
+    geom, sb, cells, faces, _ = K.affine_problem(0, (2, 2, 2), False)
+    keep = [(0, 0, 0), (1, 0, 0), (0, 1, 0)]
+    cells = cells[[tuple(c) in keep for c in cells["cell_index"].tolist()]]
+    recs = []
+    for f in faces:
+        lat = tuple(int(v) for v in f["lattice"])
+        tail = list(lat)
+        tail[int(f["axis"])] -= 1
+        has_t, has_h = tuple(tail) in keep, lat in keep
+        if not (has_t or has_h):
+            continue
+        if has_t != has_h:
+            f = f.copy()
+            f["open_area"], f["predicted_normal_velocity"], f["link_kind"], f["lift"] = 0.0, 0.0, 0, 0.0
+        recs.append(f)
+    faces = np.array(recs, dtype=O.PRESSURE_FACE)
+    s, L0 = O.reference_hierarchy(geom, sb, 1.0, cells, faces, 0)
+    assert s == O.OK
+    assert list(L0["unknown_to_cell"]) == [1, 2]
+    assert L0["level_count"] == 1 and len(L0["edges"]) == 0
+    assert list(L0["anchors"]) == [1.0, 1.0] and L0["terminal_blocks"] == 2
+    s, out = O.project(geom, sb, 1.0, cells, faces, O.pressure_options(reference=True))
+    assert s == O.OK
+    assert list(out["pressures"]) == [0.0, 1.0, 2.0]
+
+
+def test_reference_vcycle_is_a_symmetric_positive_definite_operator():
+    # ApplyVcycle is "a fixed linear symmetric V-cycle" (mac_pressure_workspace_3d.cc:295-297): the same
+    # weighted Jacobi sweep before and after P A_c^-1 P^T. In FP64, <M a, b> = <a, M b> to rounding and
+    # <M a, a> > 0; a one-sided sweep or a transposed prolongation breaks the first.
+    geom, sb, cells, faces, _ = K.affine_problem(-4, (8, 8, 8), True)
+    n = len(cells)
+    rng = np.random.default_rng(3)
+    a, b = rng.normal(size=n), rng.normal(size=n)
+    s, ma = O.reference_vcycle(geom, sb, 1.0, cells, faces, a)
+    assert s == O.OK
+    s, mb = O.reference_vcycle(geom, sb, 1.0, cells, faces, b)
+    assert s == O.OK
+    assert abs(ma @ b - a @ mb) <= 1e-12 * np.linalg.norm(ma) * np.linalg.norm(b)
+    assert ma @ a > 0 and mb @ b > 0
+    # linearity
+    s, mab = O.reference_vcycle(geom, sb, 1.0, cells, faces, 2.0 * a - 3.0 * b)
+    np.testing.assert_allclose(mab, 2.0 * ma - 3.0 * mb, rtol=1e-12, atol=1e-12 * np.abs(mab).max())
+
+
+def test_reference_vcycle_is_exact_on_a_single_level():
+    # With every component at most one unknown the fine level is the terminal level: the V-cycle is the
+    # Cholesky solve A^-1 r (here 1x1 blocks: z = (r / sqrt(d)) / sqrt(d), r / d to rounding).
+    geom, sb, cells, faces, _ = K.affine_problem(0, (1, 1, 1), True)
+    s, L0 = O.reference_hierarchy(geom, sb, 1.0, cells, faces, 0)
+    assert s == O.OK and L0["level_count"] == 1
+    s, z = O.reference_vcycle(geom, sb, 1.0, cells, faces, np.array([3.0]))
+    assert s == O.OK and abs(z[0] - 3.0 / L0["diagonal"][0]) <= 2 * np.finfo(float).eps * z[0]
+
+
+@pytest.mark.parametrize("name", sorted(FIXTURE))
+def test_reference_preconditioner_matches_dense_solves(name):
+    case = FIXTURE[name]
+    cells, faces, blocks = K.pressure_case_records(case)
+    geom = O.geometry((0.0, 0.0, 0.0), 1.0)
+    sb = K.sorted_blocks(geom, blocks)
+    opts = O.pressure_options(max_iterations=256, tolerance_scale=1e-12, fp32=False, reference=True)
+    s, out = O.project(geom, sb, case["dt"], cells, faces, opts)
+    assert s == O.OK
+    np.testing.assert_allclose(out["pressures"], case["expected"]["pressures"], rtol=2e-12, atol=2e-12)
+    np.testing.assert_allclose(out["velocities"], case["expected"]["velocities"], rtol=2e-12, atol=2e-12)
+
+
+def test_reference_and_geometric_preconditioners_accept_the_same_projection():
+    # Parity is judged on the accepted projection: both preconditioners meet the same gates and agree with
+    # the tight FP64 solve.
+    from fluidity_amd import scenes
+
+    st = K.oracle_step_inputs(scenes.dam_break(n=24, seed=1))
+    args = (st["geom"], st["blocks"], st["dt"], st["cells"], st["faces"])
+    s, ref = O.project(*args, O.pressure_options(relative_target=1e-12, fp32=False, tolerance_scale=1e-6))
+    assert s == O.OK
+    for opts in (O.pressure_options(relative_target=1e-6, reference=True), O.pressure_options(relative_target=1e-6)):
+        s, out = O.project(*args, opts)
+        assert s == O.OK
+        d = out["diagnostics"]
+        assert d["defect_l2"] <= 1e-6 * d["initial_defect_l2"] and d["max_cell_constraint_defect"] <= 2e-5
+        err = np.abs(out["pressures"] - ref["pressures"]).max() / np.abs(ref["pressures"]).max()
+        assert err <= 1e-5
