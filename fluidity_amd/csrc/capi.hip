@@ -886,6 +886,10 @@ int fl_freeze_geometry(fl_ctx* ctx, const fl_freeze_inputs* inputs, fl_frozen_ce
       if (!std::isfinite(n[a]) || !std::isfinite(v[a])) return FL_INVALID_ARGUMENT;
     if (!std::isfinite(low) || std::abs(n[0] * n[0] + n[1] * n[1] + n[2] * n[2] - 1.0) > 1e-12)
       return FL_INVALID_ARGUMENT;
+  } else if (o.kind == FL_OBSTACLE_TANK) {
+    // The reference clips cells against the tank walls (liquid_boundary_geometry_3d.cc:162-167,319-460);
+    // that clipping is not built, so a tank is refused with its own status rather than as bad data.
+    return FL_UNIMPLEMENTED;
   } else if (o.kind != FL_OBSTACLE_ABSENT) {
     return FL_INVALID_ARGUMENT;
   }

@@ -13,8 +13,10 @@ namespace fluidity {
 
 struct LiquidBoundaryParticle3D {
   MacTransferParticle3D transfer;
-  std::int64_t side_tag = 0;   // Equality identifies the same material/body side.
-  bool affine_enabled = true;  // False deposits with a zero affine row.
+  // Material side of the particle: a face sample is compatible only when its tag holds the same value.
+  std::int64_t side_tag = 0;
+  // When false the particle deposits as if its affine matrix were zero.
+  bool affine_enabled = true;
 };
 
 struct LiquidBoundaryFaceTag3D {
@@ -22,38 +24,58 @@ struct LiquidBoundaryFaceTag3D {
   bool structurally_eligible = true;
 };
 
+// One of the 27 samples of a face axis after the compatibility mask.
 struct CompatibleMacFaceSample3D {
-  std::size_t face_slot;            // Original owned slot, kept for excluded samples too.
-  bool compatible;                  // Eligible and on the particle's side.
-  double normalized_weight;         // N / S; zero when excluded.
-  Vec3 normalized_weight_gradient;  // Quotient-rule derivative, 1/m.
-  Vec3 centered_face_offset;        // r minus the compatible weighted mean, m.
+  // Owned slot of the sample; set whether or not the sample took part.
+  std::size_t face_slot;
+  // The slot is eligible and its tag equals the particle's side tag.
+  bool compatible;
+  // Weight divided by the sum S of compatible weights; 0 for a masked sample.
+  double normalized_weight;
+  // Gradient of the normalized weight (quotient rule), per meter.
+  Vec3 normalized_weight_gradient;
+  // Face offset from the particle after subtracting the normalized-weight mean offset, meters.
+  Vec3 centered_face_offset;
 };
 
 struct LiquidPrescribedWallSample3D {
   bool available = false;
-  std::int64_t side_tag = 0;  // Wall continuation is usable only by this side.
-  Vec3 unit_normal{};         // World Cartesian unit normal when available.
-  Vec3 wall_velocity{};       // World m/s at the history's wall sample time.
+  // The one side tag allowed to continue from this wall sample.
+  std::int64_t side_tag = 0;
+  // Unit wall normal in world axes (meaningful only when available).
+  Vec3 unit_normal{};
+  // Wall velocity in world axes at the sample time of the history, meters per second.
+  Vec3 wall_velocity{};
 };
 
 struct LiquidBoundaryParticleGather3D {
-  Vec3 new_pic_velocity{};                 // m/s from the complete new filled stencil.
-  Vec3 grid_velocity_increment{};          // new PIC minus old PIC, m/s.
-  Vec3 particle_minus_old_pic_velocity{};  // Retained particle residual, m/s.
-  Mat3 fitted_velocity_slopes{};           // Complete second-moment fit, 1/s.
-  Mat3 velocity_gradient{};                // True original-basis derivative, 1/s.
-  bool affine_enabled = true;              // False on input affine-off or any fallback.
-  bool old_pic_fallback = false;           // An old-stage continuation fell back from the fit.
-  bool new_pic_fallback = false;           // A new-stage continuation fell back from the fit.
-  bool low_mass_fallback = false;          // Either history touches a low-mass face.
+  // PIC velocity gathered from the filled stencil of the new field, meters per second.
+  Vec3 new_pic_velocity{};
+  // Gathered new field minus gathered old field, meters per second.
+  Vec3 grid_velocity_increment{};
+  // Particle velocity minus the gathered old field, meters per second.
+  Vec3 particle_minus_old_pic_velocity{};
+  // Affine slopes from the second-moment fit of the new field, per second.
+  Mat3 fitted_velocity_slopes{};
+  // Gradient of the gathered new field with respect to position, per second.
+  Mat3 velocity_gradient{};
+  // Cleared when the input disabled the affine part or any sample needed a fallback.
+  bool affine_enabled = true;
+  // The old-field continuation used the nearest donor instead of the fit.
+  bool old_pic_fallback = false;
+  // The new-field continuation used the nearest donor instead of the fit.
+  bool new_pic_fallback = false;
+  // Some sample of either field lies on a face whose mass is at or below the minimum.
+  bool low_mass_fallback = false;
 };
 
 enum class LiquidFaceValidity3D : std::uint8_t {
   kMissing = 0,
   kValid = 1,
-  kPredicted = 2,  // Finite continuation usable as a donor, without mass.
-  kLowMass = 3,    // Positive mass at or below the minimum.
+  // No mass, but a finite continued value that may serve as a donor.
+  kPredicted = 2,
+  // Mass above zero but not above the minimum mass.
+  kLowMass = 3,
 };
 
 }  // namespace fluidity

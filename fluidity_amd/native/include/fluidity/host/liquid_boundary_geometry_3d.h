@@ -98,7 +98,7 @@ struct TranslatingSlab3D {
   Vec3 wall_velocity;
 };
 // An open tank (native/include/fluidity/base/liquid_boundary_math.h:22-29): the particle update resolves
-// its contact; the freeze refuses it (InvalidArgument).
+// its contact; the freeze refuses it (Unimplemented: clipping cells against the tank walls is not built).
 struct AxisAlignedLiquidTank3D {
   double start_low_x;        // World x of the piston at substep start, m.
   double high_x;             // m.

@@ -22,7 +22,7 @@ using CompatibleMacFaceStencil3D = std::array<CompatibleMacFaceSample3D, 27>;
 struct LiquidBoundaryDepositInputs3D {
   const SparseMacGrid3D& mac_grid;
   std::span<const LiquidBoundaryParticle3D> particles;
-  std::span<const LiquidBoundaryFaceTag3D> face_tags;  // One per owned slot.
+  std::span<const LiquidBoundaryFaceTag3D> face_tags;  // Indexed by owned slot of the grid.
   MacDepositQuantization3D quantization;
 };
 
@@ -60,7 +60,7 @@ struct LiquidBoundaryVelocityHistory3D {
 struct LiquidBoundaryGatherInputs3D {
   const SparseMacGrid3D& mac_grid;
   std::span<const LiquidBoundaryParticle3D> particles;
-  double old_time;  // Finite pre-force snapshot time, seconds.
+  double old_time;  // Time of the old (before forces) face snapshot, seconds; must be finite.
   double new_time;  // Finite and strictly greater than old_time.
   LiquidBoundaryVelocityHistory3D old_history;
   LiquidBoundaryVelocityHistory3D new_history;

@@ -41,7 +41,8 @@ struct LiquidParticleState3D {
 };
 
 // LiquidParticleUpdateInputs3D without the regular-cell transport (empty); the obstacle is absent,
-// a half-space or a slab (the open tank stays with the reference).
+// a half-space, a slab or the open tank (frictionless walls and a moving piston; the piston's
+// reaction force is not computed).
 struct LiquidParticleUpdateInputs3D {
   std::span<const LiquidBoundaryParticleGather3D> gathered;  // Frozen old/new gather, particle order.
   std::span<const double> liquid_distances;                   // Frozen x^n field, meters, particle order.

@@ -18,6 +18,7 @@ enum class StatusCode : int {
   kFailedPrecondition = 9,
   kAborted = 10,
   kOutOfRange = 11,
+  kUnimplemented = 12,
   kInternal = 13,
 };
 

@@ -37,6 +37,7 @@ enum {
   FL_FAILED_PRECONDITION = 9,
   FL_ABORTED = 10,
   FL_OUT_OF_RANGE = 11,
+  FL_UNIMPLEMENTED = 12,
   FL_INTERNAL = 13,
 };
 

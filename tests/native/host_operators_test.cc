@@ -666,6 +666,30 @@ void BackendUpdateMatchesOracle() {
     moved += a.world_velocity.x != want[i].world_velocity.x;
   }
   std::printf("  half-space contact changed %zu of %zu stored velocities\n", moved, count);
+  // The open tank through the host mapping (liquid_wall_contact_3d_cases.h:38-145, first case): piston at
+  // x = 0.25 moving at +0.25 m/s, walls x <= 2, floor y = 0.125, z in [0.125, 0.75]; the tied first hit
+  // takes (0.5, 0.375, 0.375) to (0.5, 0.125, 0.625) with stored velocity (0.25, 0, 0.125).
+  {
+    std::vector<LiquidBoundaryParticleGather3D> tg(1);
+    tg[0].new_pic_velocity = {-0.25, -0.5, 0.25};
+    tg[0].particle_minus_old_pic_velocity = {-1.0 + 0.25, -2.0 + 0.5, 0.125 - 0.25};
+    const std::vector<double> td(1, 0.0);
+    std::vector<LiquidParticleState3D> tp(1);
+    tp[0].world_position = {0.5, 0.375, 0.375};
+    tp[0].volume_ratio = 1.0;
+    tp[0].pic_initialized = true;
+    tp[0].mass = 1.0;
+    tp[0].constituent_count = 1;
+    tp[0].constituents[0].volume_fraction = 1.0;
+    const LiquidParticleUpdateInputs3D tank_in{tg, td, AxisAlignedLiquidTank3D{0.25, 2.0, 0.125, 0.125, 0.75, 0.25},
+                                               1e3, 1.0, 1.0 / 0.99, 0.0, 0.0};
+    EXPECT(UpdateLiquidParticles3D(tank_in, tp).ok());
+    const auto near = [](double a, double b) { return std::abs(a - b) <= 2e-14; };
+    EXPECT(near(tp[0].world_position.x, 0.5) && near(tp[0].world_position.y, 0.125) &&
+           near(tp[0].world_position.z, 0.625));
+    EXPECT(near(tp[0].world_velocity.x, 0.25) && near(tp[0].world_velocity.y, 0.0) &&
+           near(tp[0].world_velocity.z, 0.125));
+  }
   // A mismatched span is InvalidArgument and leaves the particles untouched.
   const LiquidParticleUpdateInputs3D short_in{std::span(gathered).first(10), distances, {}, 0.25, 0.01, 0.7, 0.0,
                                               0.0};
@@ -770,6 +794,12 @@ void FreezeMatchesReferenceCases() {
       {*grid, surface.candidates, surface.centers, surface.corners, {}, 0.01, 1000.0, 0.072});
   EXPECT(capillary.ok() && std::abs(capillary->cells()[0].curvature) <= 1e-14);
   EXPECT(std::abs(capillary->capillary_step_bound() - std::sqrt(1000.0 / (4.0 * std::acos(-1.0) * 0.072))) <= 1e-13);
+  // A valid open tank is refused with its own status: clipping cells against its walls is not built.
+  EXPECT(FreezeLiquidBoundaryGeometry3D({*grid, liquid.candidates, liquid.centers, liquid.corners,
+                                         AxisAlignedLiquidTank3D{-10.0, 10.0, -10.0, -10.0, 10.0, 0.0}, 0.01, 1000.0,
+                                         0.0})
+             .status()
+             .code() == StatusCode::kUnimplemented);
   // More than 4096 candidates are refused first.
   std::vector<CellIndex3D> many(4097, CellIndex3D{-2, -2, -2});
   EXPECT(FreezeLiquidBoundaryGeometry3D({*grid, many, liquid.centers, liquid.corners, {}, 0.0, 1000.0, -1.0})
