@@ -198,6 +198,10 @@ BinList bin_list(fl_ctx* ctx);
 int p2g_quanta(fl_ctx* ctx, uint64_t n, double inv_qm, double inv_qp, double limit, hipEvent_t before = nullptr,
                hipEvent_t after = nullptr, bool f64 = false);
 int normalize_forces(fl_ctx* ctx, uint64_t nf, double qm, double qp, double min_mass, const double g[3], double dt);
+// k_normalize_forces on explicit device arrays and stream (no context); err holds two words preset to ~0.
+int launch_normalize_forces(hipStream_t s, const int64_t* qm, const int64_t* qp, const int64_t* qs, uint64_t nf,
+                            double sqm, double sqp, double min_mass, const double g[3], double dt, double* mass,
+                            double* u0, uint8_t* validity, double* ustar, uint64_t* err);
 int apply_forces(fl_ctx* ctx, const double* d_old, uint64_t nf, const double g[3], double dt, double* d_pred);
 int g2p(fl_ctx* ctx, uint64_t n, const double* d_old, const double* d_new, hipEvent_t before = nullptr,
         hipEvent_t after = nullptr);

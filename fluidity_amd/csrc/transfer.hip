@@ -1522,10 +1522,16 @@ int apply_forces(fl_ctx* ctx, const double* d_old, uint64_t nf, const double g[3
 
 int normalize_forces_stress(fl_ctx* ctx, const int64_t* qs, uint64_t nf, double qm, double qp, double min_mass,
                             const double g[3], double dt) {
+  return launch_normalize_forces(ctx->stream, ctx->qm, ctx->qp, qs, nf, qm, qp, min_mass, g, dt, ctx->face_mass,
+                                 ctx->u0, ctx->validity, ctx->ustar, ctx->err);
+}
+
+int launch_normalize_forces(hipStream_t s, const int64_t* qm, const int64_t* qp, const int64_t* qs, uint64_t nf,
+                            double sqm, double sqp, double min_mass, const double g[3], double dt, double* mass,
+                            double* u0, uint8_t* validity, double* ustar, uint64_t* err) {
   if (!nf) return FL_OK;
-  k_normalize_forces<<<grid_for(nf), kThreads, 0, ctx->stream>>>(ctx->qm, ctx->qp, qs, nf, qm, qp, min_mass, g[0],
-                                                                  g[1], g[2], dt, ctx->face_mass, ctx->u0,
-                                                                  ctx->validity, ctx->ustar, ctx->err);
+  k_normalize_forces<<<grid_for(nf), kThreads, 0, s>>>(qm, qp, qs, nf, sqm, sqp, min_mass, g[0], g[1], g[2], dt, mass,
+                                                       u0, validity, ustar, err);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }

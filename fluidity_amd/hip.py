@@ -54,7 +54,7 @@ EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activ
            "fl_normalize_forces", "fl_apply_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
            "fl_scene_download", "fl_probe_copy", "fl_learn_param_count", "fl_learn_setup", "fl_learn_set_params", "fl_learn_get_params",
            "fl_learn_forward", "fl_learn_infer", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect", "fl_learn_eval",
-           "fl_learn_train")
+           "fl_learn_train", "fl_normalize_forces_static")
 
 _lib = None
 
@@ -453,3 +453,16 @@ def pressure_options(max_iterations=256, tolerance_scale=1.0, relative_target=0.
     o[0] = (max_iterations, tolerance_scale, relative_target, levels, sweeps, coarse_sweeps, 2 if bf16_levels else 1,
             coarse_scale, 1 if learned else 0, int(smoother))
     return o
+
+
+STATIC_MAX_FACES = 196608
+
+
+def normalize_forces_static(device, quanta, mass_q, mom_q, min_mass, accel, dt):
+    """fl_normalize_forces_static: the context-free, allocation-free normalize + forces on `device`."""
+    nf = len(quanta)
+    m, u, v, p = np.zeros(nf), np.zeros(nf), np.zeros(nf, np.uint8), np.zeros(nf)
+    s = lib().fl_normalize_forces_static(C.c_int(device), _p(quanta), C.c_size_t(nf), Quantization(mass_q, mom_q, 0),
+                                         C.c_double(min_mass), _vec3(accel), C.c_double(dt), _p(m), _p(u), _p(v),
+                                         _p(p))
+    return s, m, u, v, p

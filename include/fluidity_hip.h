@@ -540,6 +540,19 @@ int fl_normalize_forces(fl_ctx* ctx, const fl_face_quanta* quanta, size_t face_c
                         fl_measurement* measurement);
 /* ApplyForces alone (native/src/liquid_substep_3d_cpu.cc:156-178): predicted = old + dt * g[axis], the
  * axis of owned slot f being (f mod 192) / 64. FL_OUT_OF_RANGE for a non-finite prediction. */
+/* No counterpart in the reference: fl_normalize_forces for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It takes no
+ * context and allocates and frees nothing: its device memory is a file-scope __device__ array of
+ * FL_STATIC_SCRATCH_BYTES on each GPU it runs on, held until the process ends. Same arguments, checks,
+ * kernel, results and statuses as fl_normalize_forces, with `device` in place of the context and at
+ * most FL_STATIC_MAX_FACES faces (more is FL_RESOURCE_EXHAUSTED). Runs on the device's null stream and
+ * returns when the outputs are written. Calls on one GPU share its array: calls from different threads
+ * of one process are serialized; calls that overlap on one GPU in any other way race. */
+#define FL_STATIC_MAX_FACES 196608u /* 1024 active blocks of 192 owned face slots */
+#define FL_STATIC_SCRATCH_BYTES 14352640u
+int fl_normalize_forces_static(int device, const fl_face_quanta* quanta, size_t face_count,
+                               fl_quantization quantization, double minimum_face_mass, fl_vec3 acceleration, double dt,
+                               double* out_mass, double* out_velocity, uint8_t* out_validity, double* out_predicted);
 int fl_apply_forces(fl_ctx* ctx, const double* old_face_velocities, size_t face_count, fl_vec3 acceleration,
                     double dt, double* out_predicted, fl_measurement* measurement);
 /* GatherMac3D (native/include/fluidity/host/transfers_3d.h:55-61; native/src/transfers_3d.cc:199-261). */
