@@ -278,8 +278,10 @@ __device__ inline uint32_t gcd_u32(uint32_t a, uint32_t b) {
   return a;
 }
 
+// Three waves per SIMD: the unrolled FP64 stencils need 168 VGPRs, which the compiler fits without
+// spilling at this occupancy; unconstrained it takes 179 and only two waves fit.
 template <bool kTrial, bool kPow2, bool kSmall, bool kF64>
-__global__ __launch_bounds__(kThreads) void k_p2g(ParticleView p, BinList bins, GridView grid, double inv_qm,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_p2g(ParticleView p, BinList bins, GridView grid, double inv_qm,
                                                   double inv_qp, double limit, const uint32_t* perm,
                                                   unsigned long long* qm, unsigned long long* qp, uint64_t* err) {
   __shared__ unsigned long long wm[kWindow], wq[kWindow];
