@@ -20,6 +20,7 @@ int normalize_forces_stress(fl_ctx* ctx, const int64_t* qs, uint64_t nf, double 
                             const double g[3], double dt);
 void destroy_pressure(fl_ctx* ctx);
 void destroy_learn(fl_ctx* ctx);
+int reserve_pressure(fl_ctx* ctx);
 }  // namespace fl
 
 namespace {
@@ -449,6 +450,7 @@ int fl_create(const fl_capacities* capacities, int device, fl_ctx** out) {
   for (int i = 0; i < 8 && s == FL_OK; ++i)
     if (hipEventCreate(&c->stage[i]) != hipSuccess) s = FL_INTERNAL;
   if (s == FL_OK) s = allocate(c);
+  if (s == FL_OK) s = fl::reserve_pressure(c);
   if (s != FL_OK) {
     fl_destroy(c);
     return s == FL_INTERNAL ? FL_RESOURCE_EXHAUSTED : s;
@@ -480,6 +482,108 @@ void fl_destroy(fl_ctx* ctx) {
 int fl_device_name(fl_ctx* ctx, char* out, size_t size) {
   if (!ctx || !out || !size) return FL_INVALID_ARGUMENT;
   std::snprintf(out, size, "%s", ctx->arch);
+  return FL_OK;
+}
+
+int fl_reserve(fl_ctx* ctx) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  (void)hipSetDevice(ctx->device);
+  const fl_capacities& c = ctx->cap;
+  const uint64_t nb = c.blocks, nc = c.pressure_cells, nf = 192 * c.blocks, np = c.particles;
+  // Topology preparation: previous and prepared records, pressure cells, the cell bitmap.
+  if (!ctx->topology) ctx->topology = new (std::nothrow) TopologyBuffers();
+  if (!ctx->topology) return FL_RESOURCE_EXHAUSTED;
+  {
+    TopologyBuffers& B = *static_cast<TopologyBuffers*>(ctx->topology);
+    FL_TRY(grow(&B.prev_blocks, &B.cap_prev_blocks, nb));
+    FL_TRY(grow(&B.prev_cells, &B.cap_prev_cells, nc));
+    FL_TRY(grow(&B.prev_faces, &B.cap_prev_faces, nf));
+    FL_TRY(grow(&B.cells_in, &B.cap_cells_in, 3 * nc));
+    FL_TRY(grow(&B.cells_unique, &B.cap_cells_unique, 3 * nc));
+    if (!B.box) FL_HIP(hipMalloc(reinterpret_cast<void**>(&B.box), 8 * sizeof(int32_t)));
+    if (!B.count) FL_HIP(hipMalloc(reinterpret_cast<void**>(&B.count), sizeof(uint64_t)));
+    FL_TRY(grow(&B.words, &B.cap_words, c.lookup_blocks));
+    FL_TRY(grow(&B.blocks, &B.cap_blocks, nb));
+    FL_TRY(grow(&B.old, &B.cap_old, nb));
+    FL_TRY(grow(&B.cells, &B.cap_cells, nc));
+    FL_TRY(grow(&B.faces, &B.cap_faces, nf));
+  }
+  // Particle update records.
+  if (!ctx->update) ctx->update = new (std::nothrow) UpdateBuffers();
+  if (!ctx->update) return FL_RESOURCE_EXHAUSTED;
+  {
+    UpdateBuffers& B = *static_cast<UpdateBuffers*>(ctx->update);
+    FL_TRY(grow(&B.particles, &B.cap_particles, np));
+    FL_TRY(grow(&B.gathered, &B.cap_gathered, np));
+    FL_TRY(grow(&B.distances, &B.cap_distances, np));
+  }
+  // Boundary gather histories (two banks of every owned slot) and particle attributes.
+  if (!ctx->boundary) ctx->boundary = new (std::nothrow) BoundaryBuffers();
+  if (!ctx->boundary) return FL_RESOURCE_EXHAUSTED;
+  {
+    BoundaryBuffers& B = *static_cast<BoundaryBuffers*>(ctx->boundary);
+    const uint64_t need_f = std::max<uint64_t>(nf, 1), need_p = std::max<uint64_t>(np, 1);
+    if (B.faces < need_f || B.particles < need_p) {
+      B.release();
+      for (int k = 0; k < 2; ++k) {
+        FL_TRY(dalloc(&B.u[k], need_f));
+        FL_TRY(dalloc(&B.validity[k], need_f));
+        FL_TRY(dalloc(&B.side[k], need_f));
+        FL_TRY(dalloc(&B.eligible[k], need_f));
+        FL_TRY(dalloc(&B.walls[k], need_f));
+      }
+      FL_TRY(dalloc(&B.pside, need_p));
+      FL_TRY(dalloc(&B.paffine, need_p));
+      FL_TRY(dalloc(&B.slow, need_p));
+      FL_TRY(dalloc(&B.slow_count, 1));
+      B.faces = need_f;
+      B.particles = need_p;
+    }
+  }
+  // Frozen geometry of up to pressure_cells candidates, 27 centre and 8 corner samples each.
+  if (!ctx->freeze) ctx->freeze = new (std::nothrow) FreezeBuffers();
+  if (!ctx->freeze) return FL_RESOURCE_EXHAUSTED;
+  {
+    FreezeBuffers& B = *static_cast<FreezeBuffers*>(ctx->freeze);
+    const uint64_t nwords = 3 * nb;
+    FL_TRY(grow(&B.blocks, &B.cap[0], nb));
+    FL_TRY(grow(&B.candidates, &B.cap[1], nc));
+    FL_TRY(grow(&B.centers, &B.cap[2], 27 * nc));
+    FL_TRY(grow(&B.corners, &B.cap[3], 8 * nc));
+    FL_TRY(grow(&B.keep, &B.cap[4], 2 * nc));
+    FL_TRY(grow(&B.staged, &B.cap[5], nc));
+    FL_TRY(grow(&B.slots, &B.cap[6], 6 * nc));
+    FL_TRY(grow(&B.words, &B.cap[7], nwords));
+    FL_TRY(grow(&B.word_counts, &B.cap[8], 2 * nwords));
+    FL_TRY(grow(&B.face_slots, &B.cap[9], 6 * nc));
+    FL_TRY(grow(&B.cells, &B.cap[10], nc));
+    FL_TRY(grow(&B.faces, &B.cap[11], 6 * nc));
+    FL_TRY(grow(&B.needs, &B.cap[12], nc));
+    FL_TRY(grow(&B.counts, &B.cap[13], 4));
+    FL_TRY(grow(&B.phi, &B.cap[14], 9 * nc));
+    FL_TRY(grow(&B.work, &B.cap[15], nc));
+    FL_TRY(grow(&B.uncut, &B.cap[16], 1));
+  }
+  // Pressure links of a frozen view: blocks, 27 centre samples per cell, cells and faces, the fields.
+  if (!ctx->links) ctx->links = new (std::nothrow) LinkBuffers();
+  if (!ctx->links) return FL_RESOURCE_EXHAUSTED;
+  {
+    LinkBuffers& B = *static_cast<LinkBuffers*>(ctx->links);
+    const uint64_t pf = c.pressure_faces;
+    FL_TRY(grow(&B.blocks, &B.cap[0], nb));
+    FL_TRY(grow(&B.centers, &B.cap[1], 27 * nc));
+    FL_TRY(grow(&B.cells, &B.cap[2], nc));
+    FL_TRY(grow(&B.faces, &B.cap[3], pf));
+    FL_TRY(grow(&B.fields, &B.cap[4], nc));
+    FL_TRY(grow(&B.predicted, &B.cap[5], nf));
+    FL_TRY(grow(&B.wetted, &B.cap[6], pf));
+    FL_TRY(grow(&B.air, &B.cap[7], nc));
+    FL_TRY(grow(&B.interfaces, &B.cap[8], pf));
+    FL_TRY(grow(&B.out_cells, &B.cap[9], nc));
+    FL_TRY(grow(&B.out_faces, &B.cap[10], pf));
+    FL_TRY(grow(&B.lower, &B.cap[11], 1));
+  }
+  FL_HIP(hipStreamSynchronize(ctx->stream));
   return FL_OK;
 }
 

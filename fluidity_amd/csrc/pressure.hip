@@ -131,6 +131,9 @@ struct Pressure {
   std::vector<int32_t> h_cell_comp_root;
   std::vector<uint64_t> h_gauges;
   std::vector<Key3> h_unknown_cells;  // level-0 unknown cell coordinates
+  // Multigrid level storage: one device pool reserved with the other arrays, carved level by level.
+  unsigned char* level_pool = nullptr;
+  uint64_t pool_bytes = 0, pool_used = 0;
   int levels_requested = 0;
   bool p_init_zero = false;  // warm start is identically zero: its evaluation equals the p = 0 one
   bool has_jump = false;
@@ -151,21 +154,28 @@ void dfree(T*& p) {
   p = nullptr;
 }
 
+// Forgets the levels; their storage stays in the pool for the next hierarchy.
 void free_levels(Pressure& P) {
-  for (auto& L : P.levels) {
-    dfree(L.nbr);
-    dfree(L.child);
-    dfree(L.parent);
-    dfree(L.x);
-    dfree(L.r);
-    dfree(L.kb);
-    dfree(L.diagb);
-    dfree(L.b);
-    dfree(L.k);
-    dfree(L.anchor);
-    dfree(L.diag);
-  }
   P.levels.clear();
+  P.pool_used = 0;
+}
+
+constexpr uint64_t kPoolAlign = 256;
+
+uint64_t pool_round(uint64_t b) { return (b + kPoolAlign - 1) / kPoolAlign * kPoolAlign; }
+
+// Pool bytes of one level of nb blocks (tables and the ten per-cell arrays).
+uint64_t level_bytes(uint64_t nb) {
+  const uint64_t n = 64 * nb;
+  return pool_round(6 * nb * 4) + pool_round(8 * nb * 4) + pool_round(2 * nb * 4) + 6 * pool_round(n * 4) +
+         pool_round(3 * n * 2) + pool_round(3 * n * 4);
+}
+
+template <typename T>
+T* carve(Pressure& P, uint64_t count) {
+  T* p = reinterpret_cast<T*>(P.level_pool + P.pool_used);
+  P.pool_used += pool_round(count * sizeof(T));
+  return p;
 }
 
 // ------------------------------------------------------------------ device helpers
@@ -1258,8 +1268,11 @@ int ensure_capacity(fl_ctx* ctx, uint64_t nc, uint64_t nf, uint64_t nfine) {
   // Grow once to the context capacities (all pressure allocation happens on first use).
   const uint64_t cc = std::max<uint64_t>({nc, ctx->cap.pressure_cells, 1});
   const uint64_t cf = std::max<uint64_t>({nf, ctx->cap.pressure_faces, 1});
-  const uint64_t cn = std::max<uint64_t>(nfine, 64);
+  const uint64_t cn = std::max<uint64_t>({nfine, 64 * ctx->cap.blocks, 64});
   if (nc > ctx->cap.pressure_cells || nf > ctx->cap.pressure_faces) return FL_RESOURCE_EXHAUSTED;
+  free_levels(P);
+  dfree(P.level_pool);
+  P.pool_bytes = 0;
   dfree(P.tile_blocks);
   dfree(P.cell_class);
   P.ntiles = 0;
@@ -1309,6 +1322,11 @@ int ensure_capacity(fl_ctx* ctx, uint64_t nc, uint64_t nf, uint64_t nfine) {
   FL_TRY(dal(&P.fscal, 16));
   FL_TRY(dal(&P.perr, 4));
   FL_HIP(hipHostMalloc(reinterpret_cast<void**>(&P.hscal), 64 * sizeof(double), hipHostMallocDefault));
+  // Level pool: the fine level at its full block capacity plus as much again for the coarser levels
+  // (each coarser level has at most as many blocks as the one below; a hierarchy that would overrun
+  // the pool stops adding levels), with room for the per-level alignment.
+  P.pool_bytes = 2 * level_bytes(cn / 64) + 64 * 16 * kPoolAlign;
+  FL_TRY(dal(&P.level_pool, P.pool_bytes));
   P.cap_cells = cc;
   P.cap_faces = cf;
   P.cap_fine = cn;
@@ -1345,6 +1363,8 @@ int build_levels(fl_ctx* ctx, Pressure& P, const std::vector<Key3>& blocks0, int
   std::vector<std::vector<Key3>> keys{blocks0};
   std::vector<Key3> cells = P.h_unknown_cells;
   std::vector<std::vector<int32_t>> parents;  // per level except the coarsest: [nb][2] parent block, octant
+  uint64_t planned = level_bytes(blocks0.size());
+  if (planned > P.pool_bytes) return FL_RESOURCE_EXHAUSTED;
   while (static_cast<int>(keys.size()) < levels_wanted && !cells.empty()) {
     std::vector<Key3> coarse_cells(cells.size());
     for (size_t i = 0; i < cells.size(); ++i)
@@ -1358,6 +1378,8 @@ int build_levels(fl_ctx* ctx, Pressure& P, const std::vector<Key3>& blocks0, int
       coarse.push_back({int32_t(floor_div4(c.x)), int32_t(floor_div4(c.y)), int32_t(floor_div4(c.z))});
     std::sort(coarse.begin(), coarse.end(), key_less);
     coarse.erase(std::unique(coarse.begin(), coarse.end()), coarse.end());
+    if (planned + level_bytes(coarse.size()) > P.pool_bytes) break;  // the pool bounds the depth
+    planned += level_bytes(coarse.size());
     std::unordered_map<Key3, int32_t, Key3Hash> index;
     for (size_t i = 0; i < coarse.size(); ++i) index[coarse[i]] = static_cast<int32_t>(i);
     const auto& fine = keys.back();
@@ -1388,28 +1410,33 @@ int build_levels(fl_ctx* ctx, Pressure& P, const std::vector<Key3>& blocks0, int
         auto it = index.find({L.keys[i].x + d[dir][0], L.keys[i].y + d[dir][1], L.keys[i].z + d[dir][2]});
         if (it != index.end()) nbr[6 * i + dir] = it->second;
       }
-    FL_TRY(dal(&L.nbr, 6 * L.nb));
+    L.nbr = carve<int32_t>(P, 6 * L.nb);
     FL_HIP(hipMemcpy(L.nbr, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    L.child = carve<int32_t>(P, 8 * L.nb);
     if (l > 0) {
       std::vector<int32_t> child(8 * L.nb, -1);
       const auto& par = parents[l - 1];
       for (size_t i = 0; i < keys[l - 1].size(); ++i)
         if (par[2 * i] >= 0) child[8 * par[2 * i] + par[2 * i + 1]] = static_cast<int32_t>(i);
-      FL_TRY(dal(&L.child, 8 * L.nb));
       FL_HIP(hipMemcpy(L.child, child.data(), child.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    } else {
+      L.child = nullptr;
     }
+    L.parent = carve<int32_t>(P, 2 * L.nb);
     if (l + 1 < keys.size()) {
-      FL_TRY(dal(&L.parent, parents[l].size()));
       FL_HIP(hipMemcpy(L.parent, parents[l].data(), parents[l].size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    } else {
+      L.parent = nullptr;
     }
-    FL_TRY(dal(&L.x, n));
-    FL_TRY(dal(&L.r, n));
-    FL_TRY(dal(&L.kb, 3 * n));
-    FL_TRY(dal(&L.diagb, n));
-    FL_TRY(dal(&L.b, n));
-    FL_TRY(dal(&L.k, 3 * n));
-    FL_TRY(dal(&L.anchor, n));
-    FL_TRY(dal(&L.diag, n));
+    L.x = carve<float>(P, n);
+    L.r = carve<float>(P, n);
+    L.diagb = carve<float>(P, n);
+    L.b = carve<float>(P, n);
+    L.anchor = carve<float>(P, n);
+    L.diag = carve<float>(P, n);
+    L.kb = carve<uint16_t>(P, 3 * n);
+    L.k = carve<float>(P, 3 * n);
+    if (P.pool_used > P.pool_bytes) return FL_INTERNAL;
   }
   (void)pack_key;
   return FL_OK;
@@ -1701,6 +1728,7 @@ void destroy_pressure(fl_ctx* ctx) {
   Pressure* P = state(ctx);
   if (!P) return;
   free_levels(*P);
+  dfree(P->level_pool);
   dfree(P->tile_blocks);
   dfree(P->cell_class);
   void* ptrs[] = {P->fine_cell, P->cell_fine, P->cell_flag, P->cell_faces, P->face_tail, P->face_head, P->face_kind,
@@ -1714,6 +1742,11 @@ void destroy_pressure(fl_ctx* ctx) {
   if (P->hscal) (void)hipHostFree(P->hscal);
   delete P;
   ctx->pressure = nullptr;
+}
+
+int reserve_pressure(fl_ctx* ctx) {
+  if (ctx->cap.pressure_cells == 0 || ctx->cap.pressure_faces == 0) return FL_OK;
+  return ensure_capacity(ctx, ctx->cap.pressure_cells, ctx->cap.pressure_faces, 64 * ctx->cap.blocks);
 }
 
 }  // namespace fl
@@ -2137,9 +2170,7 @@ int project_pressure(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* bl
   // Multigrid levels requested by the options.
   std::vector<Key3> b0 = P.levels[0].keys;
   FL_TRY(build_levels(ctx, P, b0, options->levels));
-  hipEvent_t a, b;
-  FL_HIP(hipEventCreate(&a));
-  FL_HIP(hipEventCreate(&b));
+  hipEvent_t a = ctx->stage[6], b = ctx->stage[7];
   FL_HIP(hipEventRecord(a, ctx->stream));
   Gates gates{};
   EvalResult e0{};
@@ -2155,8 +2186,6 @@ int project_pressure(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* bl
   (void)hipEventSynchronize(b);
   float ms = 0;
   (void)hipEventElapsedTime(&ms, a, b);
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
   if (out_diagnostics) *out_diagnostics = diag;
   if (st != FL_OK) return st;
   hipStream_t s = ctx->stream;

@@ -50,7 +50,7 @@ class StepReport(C.Structure):
         return np.frombuffer(bytes(self.pressure), dtype=R.PRESSURE_DIAGNOSTICS)[0]
 
 
-EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_bin_sort", "fl_activate", "fl_prepare_topology", "fl_update_particles", "fl_build_pressure_links", "fl_freeze_geometry", "fl_p2g", "fl_p2g_quanta", "fl_p2g_compatible", "fl_g2p_boundary",
+EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_reserve", "fl_bin_sort", "fl_activate", "fl_prepare_topology", "fl_update_particles", "fl_build_pressure_links", "fl_freeze_geometry", "fl_p2g", "fl_p2g_quanta", "fl_p2g_compatible", "fl_g2p_boundary",
            "fl_normalize_forces", "fl_apply_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
            "fl_scene_download", "fl_probe_copy", "fl_learn_param_count", "fl_learn_setup", "fl_learn_set_params", "fl_learn_get_params",
            "fl_learn_forward", "fl_learn_infer", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect", "fl_learn_eval",

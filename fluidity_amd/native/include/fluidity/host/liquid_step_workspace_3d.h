@@ -17,42 +17,11 @@
 #include "fluidity/base/liquid_boundary_transfer_types_3d.h"
 #include "fluidity/base/mac_types_3d.h"
 #include "fluidity/host/hip_operator_runtime_3d.h"
+#include "fluidity/host/liquid_state_3d.h"
 #include "fluidity/host/sparse_mac_grid_3d.h"
 #include "fluidity/status.h"
 
 namespace fluidity {
-
-// Records of native/include/fluidity/host/liquid_state_3d.h:129-159.
-struct LiquidBlockState3D {
-  BlockIndex3D block_index{};
-  std::uint64_t generation = 0;
-  std::uint32_t activation_age = 0;  // Accepted steps since activation.
-  std::uint32_t inactive_age = 0;    // Hysteresis age in accepted steps.
-  bool operator==(const LiquidBlockState3D&) const;
-};
-enum class LiquidCellOccupancy3D : std::uint8_t { kAir, kLiquid, kSolid };
-struct LiquidCellState3D {
-  CellIndex3D cell_index{};
-  LiquidCellOccupancy3D occupancy = LiquidCellOccupancy3D::kLiquid;
-  double previous_pressure = 0.0;
-  bool operator==(const LiquidCellState3D&) const;
-};
-struct LiquidFaceState3D {
-  FaceIndex3D face_index{};
-  std::size_t face_slot = 0;
-  double carrier_velocity = 0.0;
-  LiquidFaceValidity3D validity = LiquidFaceValidity3D::kMissing;
-  std::int64_t side_tag = 0;
-  bool structurally_eligible = true;
-  bool operator==(const LiquidFaceState3D&) const;
-};
-struct LiquidStateCapacities3D {
-  std::size_t particles = 0;
-  std::size_t blocks = 0;
-  std::size_t cells = 0;
-  std::size_t faces = 0;
-  std::size_t sources = 0;
-};
 
 // The topology of a previous LiquidState3D.
 struct LiquidTopologyHistory3D {
@@ -63,9 +32,14 @@ struct LiquidTopologyHistory3D {
   std::span<const LiquidFaceState3D> faces;
 };
 
+// The topology of a committed state (its geometry, epoch and topology records).
+LiquidTopologyHistory3D TopologyHistoryOf(const LiquidState3D& state);
+
 struct LiquidPreparedTopologyView3D {
   SparseMacGridView3D grid;
   std::span<const LiquidParticleSortEntry3D> particle_order;
+  // Particle records in sorted order; empty when the preparation took world positions only.
+  std::span<const LiquidParticleState3D> particles;
   std::span<const LiquidBlockState3D> blocks;
   std::span<const LiquidCellState3D> cells;
   std::span<const LiquidFaceState3D> faces;
@@ -87,13 +61,27 @@ class LiquidStepWorkspace3D {
   Status HipPrepareTrialTopology(HipOperatorRuntime3D& runtime, std::span<const Vec3> world_positions,
                                  std::span<const CellIndex3D> pressure_cells, const LiquidTopologyHistory3D* previous,
                                  HipOperatorMeasurement& measurement);
+  // The reference's overload (liquid_step_workspace_3d.h:96-100): particle records (their world
+  // positions are sorted; every record is copied in sorted order into the prepared view) and an
+  // optional previous state whose topology records, geometry and epoch are remapped.
+  Status PrepareTrialTopology(std::span<const LiquidParticleState3D> particles,
+                              std::span<const CellIndex3D> pressure_cells, const LiquidState3D* previous_state);
+  Status HipPrepareTrialTopology(HipOperatorRuntime3D& runtime, std::span<const LiquidParticleState3D> particles,
+                                 std::span<const CellIndex3D> pressure_cells, const LiquidState3D* previous_state,
+                                 HipOperatorMeasurement& measurement);
   // kFailedPrecondition before the first successful preparation.
   StatusOr<LiquidPreparedTopologyView3D> prepared() const;
+  // Copies the sorted particles and the prepared topology into the trial state's existing reservations
+  // (liquid_step_workspace_3d.h:122-128): InvalidArgument for another geometry, ResourceExhausted when a
+  // trial vector's capacity is too small, FailedPrecondition without a prepared view; the trial is
+  // untouched on every failure.
+  Status ApplyTrialTopology(LiquidState3D& trial) const;
   LiquidStateCapacities3D capacities() const { return capacities_; }
 
  private:
   struct Bank {
     std::vector<LiquidParticleSortEntry3D> particle_order;
+    std::vector<LiquidParticleState3D> particles;
     std::vector<BlockIndex3D> block_keys;
     std::vector<LiquidBlockState3D> blocks;
     std::vector<LiquidCellState3D> cells;
@@ -103,8 +91,12 @@ class LiquidStepWorkspace3D {
       : capacities_(capacities), geometry_(geometry) {}
   int Prepare(HipOperatorRuntime3D& runtime, std::span<const Vec3> positions, std::span<const CellIndex3D> cells,
               const LiquidTopologyHistory3D* previous, Bank& bank, void* measurement);
+  int PrepareRecords(HipOperatorRuntime3D& runtime, std::span<const LiquidParticleState3D> particles,
+                     std::span<const CellIndex3D> cells, const LiquidState3D* previous, Bank& bank,
+                     void* measurement);
   LiquidStateCapacities3D capacities_;
   UniformMacGeometry3D geometry_;
+  std::vector<Vec3> positions_;  // world positions of a particle-record preparation
   Bank banks_[2];
   int published_ = -1;
 };

@@ -7,6 +7,7 @@
 #include <mutex>
 
 #include "internal.h"
+#include "records.h"
 
 namespace fluidity {
 namespace {
@@ -22,59 +23,13 @@ struct Staged {
 };
 
 void Stage(const FrozenLiquidBoundaryGeometryView3D& g, const LiquidPressureFields3D& f, Staged& s) {
-  for (const auto& c : g.center_distances)
-    s.centers.push_back({{c.cell_index.x, c.cell_index.y, c.cell_index.z}, 0, c.signed_distance});
-  for (const auto& c : g.cells)
-    s.cells.push_back(
-        {{c.cell_index.x, c.cell_index.y, c.cell_index.z}, 0, c.fluid_volume, c.center_distance, c.curvature});
-  for (const auto& x : g.faces) {
-    fl_frozen_face r{};
-    r.axis = static_cast<std::uint8_t>(x.face_index.axis);
-    r.lattice = {x.face_index.lattice_index.x, x.face_index.lattice_index.y, x.face_index.lattice_index.z};
-    r.face_slot = x.face_slot;
-    r.open_area = x.open_area;
-    r.blocked_area = x.blocked_area;
-    r.wall_normal_velocity = x.wall_normal_velocity;
-    r.has_prescribed_wall_flux = x.prescribed_wall_flux.has_value();
-    r.prescribed_wall_flux = x.prescribed_wall_flux.value_or(0.0);
-    r.is_low_x_piston_face = x.is_low_x_piston_face;
-    s.faces.push_back(r);
-  }
-  for (const auto& c : f.cells) {
-    fl_cell_fields r{};
-    r.target_divergence = c.target_divergence;
-    r.bulk_modulus = c.bulk_modulus;
-    r.previous_pressure = c.previous_pressure;
-    r.has_density = c.density.has_value();
-    r.density = c.density.value_or(0.0);
-    s.fields.push_back(r);
-  }
-  for (const auto& w : f.wetted_piston_face_fluxes) {
-    fl_optional_flux r{};
-    r.has_value = w.has_value();
-    r.value = w.value_or(0.0);
-    s.wetted.push_back(r);
-  }
-  for (const auto& a : f.air_cell_pressures)
-    s.air.push_back({{a.cell_index.x, a.cell_index.y, a.cell_index.z}, 0, a.gas_gauge_pressure});
-  for (const auto& i : f.internal_interfaces)
-    s.interfaces.push_back({i.face_ordinal, i.tail_fraction, i.surface_tension, i.curvature_tail_to_head,
-                            i.normal_viscous_jump_tail_to_head});
-}
-
-MacPressureFace3D FromC(const fl_pressure_face& r) {
-  MacPressureFace3D f{};
-  f.face_index = {static_cast<FaceNormalAxis>(r.axis), {r.lattice.x, r.lattice.y, r.lattice.z}};
-  f.open_area = r.open_area;
-  f.inverse_density = r.inverse_density;
-  f.pressure_gradient_distance = r.pressure_gradient_distance;
-  f.predicted_normal_velocity = r.predicted_normal_velocity;
-  f.wall_normal_velocity = r.wall_normal_velocity;
-  if (r.link_kind == 1) f.pressure_link = InternalMacPressureLink3D{r.lift};
-  else if (r.link_kind == 2) f.pressure_link = DirichletMacPressureLink3D{r.lift};
-  else f.pressure_link = SealedMacPressureLink3D{};
-  if (r.has_prescribed_wall_flux) f.prescribed_wall_flux = r.prescribed_wall_flux;
-  return f;
+  for (const auto& c : g.center_distances) s.centers.push_back(internal::ToC(c));
+  for (const auto& c : g.cells) s.cells.push_back(internal::ToC(c));
+  for (const auto& x : g.faces) s.faces.push_back(internal::ToC(x));
+  for (const auto& c : f.cells) s.fields.push_back(internal::ToC(c));
+  for (const auto& w : f.wetted_piston_face_fluxes) s.wetted.push_back(internal::ToC(w));
+  for (const auto& a : f.air_cell_pressures) s.air.push_back(internal::ToC(a));
+  for (const auto& i : f.internal_interfaces) s.interfaces.push_back(internal::ToC(i));
 }
 
 int Build(HipOperatorRuntime3D& rt, const FrozenLiquidBoundaryGeometryView3D& g, const LiquidPressureFields3D& f,
@@ -114,13 +69,9 @@ int Build(HipOperatorRuntime3D& rt, const FrozenLiquidBoundaryGeometryView3D& g,
   const int code = fl_build_pressure_links(rt.handle(), &v, &pf, cells.data(), faces.data(), &lower, m);
   if (code != FL_OK) return code;
   out.cells.resize(cells.size());
-  for (std::size_t i = 0; i < cells.size(); ++i) {
-    const auto& c = cells[i];
-    out.cells[i] = {{c.cell_index.x, c.cell_index.y, c.cell_index.z}, c.fluid_volume, c.target_divergence,
-                    c.bulk_modulus, c.previous_pressure, c.integrated_inward_wall_flux};
-  }
+  for (std::size_t i = 0; i < cells.size(); ++i) out.cells[i] = internal::FromC(cells[i]);
   out.faces.resize(faces.size());
-  for (std::size_t i = 0; i < faces.size(); ++i) out.faces[i] = FromC(faces[i]);
+  for (std::size_t i = 0; i < faces.size(); ++i) out.faces[i] = internal::FromC(faces[i]);
   out.lower_theta_regularization_count = lower;
   return FL_OK;
 }
@@ -161,14 +112,12 @@ struct FreezeAccess {
                     std::uint64_t max_faces, FrozenLiquidBoundaryGeometry3D& out, fl_measurement* m) {
     std::vector<fl_int3> candidates;
     candidates.reserve(in.cell_indices.size());
-    for (const auto& c : in.cell_indices) candidates.push_back({c.x, c.y, c.z});
+    for (const auto& c : in.cell_indices) candidates.push_back(internal::ToC3(c));
     std::vector<fl_center_distance> centers, corners;
     centers.reserve(in.center_distances.size());
     corners.reserve(in.corner_distances.size());
-    for (const auto& c : in.center_distances)
-      centers.push_back({{c.cell_index.x, c.cell_index.y, c.cell_index.z}, 0, c.signed_distance});
-    for (const auto& c : in.corner_distances)
-      corners.push_back({{c.corner_index.x, c.corner_index.y, c.corner_index.z}, 0, c.signed_distance});
+    for (const auto& c : in.center_distances) centers.push_back(internal::ToC(c));
+    for (const auto& c : in.corner_distances) corners.push_back(internal::ToC(c));
     fl_freeze_inputs r{};
     const auto blocks = in.mac_grid.active_block_indices();
     r.blocks = internal::ToC(blocks);
@@ -180,16 +129,7 @@ struct FreezeAccess {
     r.center_count = centers.size();
     r.corner_distances = corners.data();
     r.corner_count = corners.size();
-    const auto c3 = [](Vec3 v) { return fl_vec3{v.x, v.y, v.z}; };
-    if (const auto* h = std::get_if<TranslatingHalfspace3D>(&in.obstacle)) {
-      r.obstacle = {FL_OBSTACLE_HALFSPACE, 0, c3(h->unit_normal), h->start_plane_coordinate, 0.0,
-                    c3(h->wall_velocity), {}};
-    } else if (const auto* b = std::get_if<TranslatingSlab3D>(&in.obstacle)) {
-      r.obstacle = {FL_OBSTACLE_SLAB, 0, c3(b->unit_normal), b->start_center_coordinate, b->half_width,
-                    c3(b->wall_velocity), {}};
-    } else if (std::holds_alternative<AxisAlignedLiquidTank3D>(in.obstacle)) {
-      r.obstacle.kind = FL_OBSTACLE_TANK;  // refused by the device freeze
-    }
+    r.obstacle = internal::ToC(in.obstacle);
     r.step_duration = in.step_duration;
     r.liquid_density = in.liquid_density;
     r.surface_tension = in.surface_tension;
@@ -205,17 +145,9 @@ struct FreezeAccess {
     if (code != FL_OK) return code;
     out.centers_.assign(in.center_distances.begin(), in.center_distances.end());
     out.cells_.resize(nc);
-    for (std::size_t i = 0; i < nc; ++i) {
-      const auto& c = cells[i];
-      out.cells_[i] = {{c.cell_index.x, c.cell_index.y, c.cell_index.z}, c.fluid_volume, c.center_distance,
-                       c.curvature};
-    }
+    for (std::size_t i = 0; i < nc; ++i) out.cells_[i] = internal::FromC(cells[i]);
     out.faces_.resize(nf);
-    for (std::size_t i = 0; i < nf; ++i) {
-      const auto& f = faces[i];
-      out.faces_[i] = {{static_cast<FaceNormalAxis>(f.axis), {f.lattice.x, f.lattice.y, f.lattice.z}},
-                       static_cast<std::size_t>(f.face_slot), f.open_area, f.blocked_area, f.wall_normal_velocity};
-    }
+    for (std::size_t i = 0; i < nf; ++i) out.faces_[i] = internal::FromC(faces[i]);
     out.step_duration_ = in.step_duration;
     out.liquid_density_ = in.liquid_density;
     out.surface_tension_ = in.surface_tension;

@@ -5,6 +5,7 @@
 #include <mutex>
 
 #include "internal.h"
+#include "records.h"
 
 namespace fluidity {
 namespace {
@@ -23,17 +24,7 @@ int Update(HipOperatorRuntime3D& rt, const LiquidParticleUpdateInputs3D& in, std
   if (in.gathered.size() != p.size() || in.liquid_distances.size() != p.size()) return FL_INVALID_ARGUMENT;
   fl_particle_update_options o{in.cell_width, in.step_duration, in.particle_energy_blend, in.friction_coefficient,
                                in.penetration_tolerance, {}};
-  const auto c3 = [](Vec3 v) { return fl_vec3{v.x, v.y, v.z}; };
-  if (const auto* h = std::get_if<TranslatingHalfspace3D>(&in.obstacle)) {
-    o.obstacle = {FL_OBSTACLE_HALFSPACE, 0, c3(h->unit_normal), h->start_plane_coordinate, 0.0, c3(h->wall_velocity),
-                  {}};
-  } else if (const auto* b = std::get_if<TranslatingSlab3D>(&in.obstacle)) {
-    o.obstacle = {FL_OBSTACLE_SLAB, 0, c3(b->unit_normal), b->start_center_coordinate, b->half_width,
-                  c3(b->wall_velocity), {}};
-  } else if (const auto* k = std::get_if<AxisAlignedLiquidTank3D>(&in.obstacle)) {
-    o.obstacle = {FL_OBSTACLE_TANK, 0, {}, 0.0, 0.0, {}, {k->start_low_x, k->high_x, k->floor_y, k->low_z, k->high_z,
-                                                  k->piston_velocity_x}};
-  }
+  o.obstacle = internal::ToC(in.obstacle);
   return fl_update_particles(rt.handle(), &o, reinterpret_cast<const fl_boundary_gather*>(in.gathered.data()),
                              in.liquid_distances.data(), p.size(), reinterpret_cast<fl_liquid_particle*>(p.data()), m);
 }

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <new>
 
 #include "internal.h"
 
@@ -86,7 +87,100 @@ StatusOr<LiquidStepWorkspace3D> LiquidStepWorkspace3D::Create(LiquidStateCapacit
   if (!std::isfinite(o.x) || !std::isfinite(o.y) || !std::isfinite(o.z) || !std::isfinite(geometry.cell_width) ||
       !(geometry.cell_width > 0))
     return Status(StatusCode::kInvalidArgument);
-  return LiquidStepWorkspace3D(capacities, geometry);
+  LiquidStepWorkspace3D w(capacities, geometry);
+  // Every bank and the position staging at the capacities: preparations then resize within them.
+  try {
+    const std::size_t blocks = std::min(capacities.blocks, capacities.faces / 192);
+    for (Bank& b : w.banks_) {
+      b.particle_order.reserve(capacities.particles);
+      b.particles.reserve(capacities.particles);
+      b.block_keys.reserve(blocks);
+      b.blocks.reserve(blocks);
+      b.cells.reserve(capacities.cells);
+      b.faces.reserve(192 * blocks);
+    }
+    w.positions_.reserve(capacities.particles);
+  } catch (const std::bad_alloc&) {
+    return Status(StatusCode::kResourceExhausted);
+  }
+  return w;
+}
+
+LiquidTopologyHistory3D TopologyHistoryOf(const LiquidState3D& s) {
+  return {s.geometry, s.clock.epoch, s.blocks, s.cells, s.faces};
+}
+
+int LiquidStepWorkspace3D::PrepareRecords(HipOperatorRuntime3D& rt, std::span<const LiquidParticleState3D> particles,
+                                          std::span<const CellIndex3D> cells, const LiquidState3D* previous,
+                                          Bank& bank, void* measurement) {
+  if (particles.size() > capacities_.particles) return FL_RESOURCE_EXHAUSTED;
+  positions_.resize(particles.size());
+  for (std::size_t i = 0; i < particles.size(); ++i) positions_[i] = particles[i].world_position;
+  const LiquidTopologyHistory3D h = previous ? TopologyHistoryOf(*previous) : LiquidTopologyHistory3D{};
+  const int code = Prepare(rt, positions_, cells, previous ? &h : nullptr, bank, measurement);
+  if (code != FL_OK) return code;
+  bank.particles.resize(particles.size());
+  for (std::size_t i = 0; i < particles.size(); ++i) bank.particles[i] = particles[bank.particle_order[i].original_ordinal];
+  return FL_OK;
+}
+
+Status LiquidStepWorkspace3D::PrepareTrialTopology(std::span<const LiquidParticleState3D> particles,
+                                                   std::span<const CellIndex3D> pressure_cells,
+                                                   const LiquidState3D* previous_state) {
+  if (particles.size() > capacities_.particles) return Status(StatusCode::kResourceExhausted);
+  positions_.resize(particles.size());
+  for (std::size_t i = 0; i < particles.size(); ++i) positions_[i] = particles[i].world_position;
+  const LiquidTopologyHistory3D h = previous_state ? TopologyHistoryOf(*previous_state) : LiquidTopologyHistory3D{};
+  const int candidate = published_ == 0 ? 1 : 0;
+  HipOperatorCapacities3D need = Need(geometry_, positions_, pressure_cells, previous_state ? &h : nullptr);
+  std::unique_lock<std::mutex> lock;
+  for (int attempt = 0;; ++attempt) {
+    auto rt = SharedHipRuntime3D::Acquire(need, lock);
+    if (!rt.ok()) return rt.status();
+    const int code = PrepareRecords(**rt, particles, pressure_cells, previous_state, banks_[candidate], nullptr);
+    if (code == FL_RESOURCE_EXHAUSTED && attempt < 6 && (*rt)->capacities().lookup_blocks < (std::size_t{1} << 30)) {
+      need.lookup_blocks = (*rt)->capacities().lookup_blocks * 4;
+      lock.unlock();
+      continue;
+    }
+    if (code != FL_OK) return FromCode(code);
+    published_ = candidate;
+    return Status();
+  }
+}
+
+Status LiquidStepWorkspace3D::HipPrepareTrialTopology(HipOperatorRuntime3D& runtime,
+                                                      std::span<const LiquidParticleState3D> particles,
+                                                      std::span<const CellIndex3D> pressure_cells,
+                                                      const LiquidState3D* previous_state,
+                                                      HipOperatorMeasurement& measurement) {
+  const int candidate = published_ == 0 ? 1 : 0;
+  fl_measurement m{};
+  const int code = PrepareRecords(runtime, particles, pressure_cells, previous_state, banks_[candidate], &m);
+  if (code != FL_OK) return FromCode(code);
+  published_ = candidate;
+  std::memcpy(measurement.architecture, m.architecture, sizeof(measurement.architecture));
+  measurement.kernel_ms = m.kernel_ms;
+  measurement.copy_ms = m.copy_ms;
+  measurement.launches = m.launches;
+  return Status();
+}
+
+Status LiquidStepWorkspace3D::ApplyTrialTopology(LiquidState3D& trial) const {
+  if (published_ < 0) return Status(StatusCode::kFailedPrecondition);
+  const Bank& b = banks_[published_];
+  const auto& o = trial.geometry.world_origin;
+  if (!(o.x == geometry_.world_origin.x && o.y == geometry_.world_origin.y && o.z == geometry_.world_origin.z &&
+        trial.geometry.cell_width == geometry_.cell_width))
+    return Status(StatusCode::kInvalidArgument);
+  if (trial.particles.capacity() < b.particles.size() || trial.blocks.capacity() < b.blocks.size() ||
+      trial.cells.capacity() < b.cells.size() || trial.faces.capacity() < b.faces.size())
+    return Status(StatusCode::kResourceExhausted);
+  trial.particles.assign(b.particles.begin(), b.particles.end());
+  trial.blocks.assign(b.blocks.begin(), b.blocks.end());
+  trial.cells.assign(b.cells.begin(), b.cells.end());
+  trial.faces.assign(b.faces.begin(), b.faces.end());
+  return Status();
 }
 
 int LiquidStepWorkspace3D::Prepare(HipOperatorRuntime3D& rt, std::span<const Vec3> positions,
@@ -145,6 +239,7 @@ Status LiquidStepWorkspace3D::PrepareTrialTopology(std::span<const Vec3> world_p
       continue;
     }
     if (code != FL_OK) return FromCode(code);
+    banks_[candidate].particles.clear();
     published_ = candidate;
     return Status();
   }
@@ -159,6 +254,7 @@ Status LiquidStepWorkspace3D::HipPrepareTrialTopology(HipOperatorRuntime3D& runt
   fl_measurement m{};
   const int code = Prepare(runtime, world_positions, pressure_cells, previous, banks_[candidate], &m);
   if (code != FL_OK) return FromCode(code);
+  banks_[candidate].particles.clear();
   published_ = candidate;
   std::memcpy(measurement.architecture, m.architecture, sizeof(measurement.architecture));
   measurement.kernel_ms = m.kernel_ms;
@@ -172,7 +268,7 @@ StatusOr<LiquidPreparedTopologyView3D> LiquidStepWorkspace3D::prepared() const {
   const Bank& b = banks_[published_];
   auto grid = SparseMacGridView3D::Create(geometry_, b.block_keys);
   if (!grid.ok()) return grid.status();
-  return LiquidPreparedTopologyView3D{*grid, b.particle_order, b.blocks, b.cells, b.faces};
+  return LiquidPreparedTopologyView3D{*grid, b.particle_order, b.particles, b.blocks, b.cells, b.faces};
 }
 
 }  // namespace fluidity

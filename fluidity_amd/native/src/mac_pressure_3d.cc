@@ -6,49 +6,10 @@
 #include <type_traits>
 
 #include "internal.h"
+#include "records.h"
 
 namespace fluidity {
 namespace {
-
-fl_pressure_face ToC(const MacPressureFace3D& f) {
-  fl_pressure_face r{};
-  r.axis = static_cast<std::int32_t>(f.face_index.axis);
-  r.lattice = {f.face_index.lattice_index.x, f.face_index.lattice_index.y, f.face_index.lattice_index.z};
-  r.open_area = f.open_area;
-  r.inverse_density = f.inverse_density;
-  r.pressure_gradient_distance = f.pressure_gradient_distance;
-  r.predicted_normal_velocity = f.predicted_normal_velocity;
-  r.wall_normal_velocity = f.wall_normal_velocity;
-  std::visit(
-      [&r](const auto& link) {
-        using T = std::decay_t<decltype(link)>;
-        if constexpr (std::is_same_v<T, InternalMacPressureLink3D>) {
-          r.link_kind = 1;
-          r.lift = link.pressure_jump;
-        } else if constexpr (std::is_same_v<T, DirichletMacPressureLink3D>) {
-          r.link_kind = 2;
-          r.lift = link.exterior_pressure;
-        } else {
-          r.link_kind = 0;
-          r.lift = 0.0;
-        }
-      },
-      f.pressure_link);
-  r.has_prescribed_wall_flux = f.prescribed_wall_flux.has_value() ? 1 : 0;
-  r.prescribed_wall_flux = f.prescribed_wall_flux.value_or(0.0);
-  return r;
-}
-
-fl_pressure_cell ToC(const MacPressureCell3D& c) {
-  fl_pressure_cell r{};
-  r.cell_index = {c.cell_index.x, c.cell_index.y, c.cell_index.z};
-  r.fluid_volume = c.fluid_volume;
-  r.target_divergence = c.target_divergence;
-  r.bulk_modulus = c.bulk_modulus;
-  r.previous_pressure = c.previous_pressure;
-  r.integrated_inward_wall_flux = c.integrated_inward_wall_flux;
-  return r;
-}
 
 int Project(HipOperatorRuntime3D& rt, const MacPressureInputs3D& in, MacPressureSolverOptions3D options,
             const HipMacPressureSolverSettings3D& settings, HipMacPressureProjection3D& out) {
@@ -56,9 +17,9 @@ int Project(HipOperatorRuntime3D& rt, const MacPressureInputs3D& in, MacPressure
   std::vector<fl_pressure_cell> cells(nc);
   std::vector<fl_pressure_face> faces(nf);
   std::transform(in.pressure_cells.begin(), in.pressure_cells.end(), cells.begin(),
-                 [](const MacPressureCell3D& c) { return ToC(c); });
+                 [](const MacPressureCell3D& c) { return internal::ToC(c); });
   std::transform(in.pressure_faces.begin(), in.pressure_faces.end(), faces.begin(),
-                 [](const MacPressureFace3D& f) { return ToC(f); });
+                 [](const MacPressureFace3D& f) { return internal::ToC(f); });
   fl_pressure_options o{};
   o.max_iterations = options.max_iterations;
   o.cpu_residual_tolerance_scale = options.cpu_residual_tolerance_scale;

@@ -416,6 +416,14 @@ int fl_create(const fl_capacities* capacities, int device, fl_ctx** out);
 void fl_destroy(fl_ctx* ctx);
 /* Architecture string of the device the context runs on (gcnArchName). */
 int fl_device_name(fl_ctx* ctx, char* out, size_t size);
+/* Set-up outside physical attempts (liquid_substep_backend_3d.h:37-46: workspaces allocate only at
+ * Create; attempt operations allocate and free nothing). fl_create itself reserves the projection's
+ * arrays and its multigrid level pool from the pressure capacities. fl_reserve also reserves, from the
+ * capacities, the device buffers that topology preparation, the boundary gather, the frozen geometry,
+ * the pressure links and the particle update would otherwise allocate on first use: after it, every
+ * call whose inputs stay within the capacities (at most 27 centre and 8 corner samples per candidate
+ * cell, link fields at most one per cell or pressure face) allocates and frees no device memory. */
+int fl_reserve(fl_ctx* ctx);
 
 /* ---- topology ---- */
 /* ComputeParticleSortBin3D for each position plus the stable (z,y,x)-bin / ordinal order of

@@ -16,6 +16,8 @@
 #include <vector>
 
 #include "fluidity/host/hip_liquid_substep_backend_3d.h"
+#include "fluidity/host/liquid_boundary_workspace_3d.h"
+#include "fluidity/host/mac_pressure_workspace_3d.h"
 #include "fluidity/host/liquid_boundary_transfer_3d.h"
 #include "fluidity/host/liquid_boundary_geometry_3d.h"
 #include "fluidity/host/liquid_particle_update_3d.h"
@@ -343,48 +345,277 @@ void GatherMatchesOracle() {
   EXPECT(keep[0].new_pic_velocity.x == 42.0);
 }
 
-void BackendStagesMatchOracle() {
-  auto rt = HipOperatorRuntime3D::Create({1 << 16, 1 << 10, 1 << 16, 1 << 12, 1 << 14}, 0);
-  EXPECT(rt.ok());
-  const HipLiquidSubstepBackend3D backend = MakeHipLiquidSubstepBackend3D(**rt);
-  auto grid = SparseMacGrid3D::Create(kGeometry, DenseBlocks(4, 4, 4), 64);
-  EXPECT(grid.ok());
-  const auto particles = BoxParticles(1, 12, 17, 1.0);
+// One stage sequence of a liquid substep through the table, each stage against the oracle on the same
+// inputs: topology (sorted particles, activation, unique cells), compatible deposit, normalization,
+// forces, frozen geometry of a box level set, typed links, projection, boundary gather and the particle
+// update; then the publication hooks and a rejected stage that keeps the published view.
+void SubstepTableDrivesStagesAgainstOracle() {
+  const int lo = 4, hi = 12;  // liquid cells [4, 12)^3 of the h = 1/32 lattice
+  const double h = kGeometry.cell_width, dt = 2e-4;
+  const auto transfer = BoxParticles(lo, hi, 23, 0.5);
+  const std::size_t n = transfer.size();
+  LiquidState3D committed;
+  committed.geometry = kGeometry;
+  committed.particles.resize(n);
+  for (std::size_t i = 0; i < n; ++i) {
+    auto& p = committed.particles[i];
+    p.id = i + 1;
+    p.world_position = transfer[i].sorted_world_position;
+    p.world_velocity = transfer[i].particle_velocity;
+    p.velocity_slopes = transfer[i].particle_velocity_slopes;
+    p.mass = transfer[i].particle_mass;
+    p.volume_ratio = 1.0;
+    p.pic_initialized = true;
+    p.constituent_count = 1;
+    p.constituents[0].volume_fraction = 1.0;
+  }
+  std::vector<CellIndex3D> candidates;
+  for (int z = lo; z < hi; ++z)
+    for (int y = lo; y < hi; ++y)
+      for (int x = lo; x < hi; ++x) candidates.push_back({x, y, z});
+  const LiquidStateCapacities3D caps{8192, 512, 4096, 192 * 512, 0};
+  auto context = HipLiquidSubstepContext3D::Create(caps);
+  EXPECT(context.ok());
+  (*context)->projection_settings.relative_divergence_target = 1e-9;
+  (*context)->projection_settings.levels = 3;
+  const HipLiquidSubstepBackend3D table = MakeHipLiquidSubstepBackend3D(**context);
+  EXPECT(table.sample_field == nullptr && table.project_viscous == nullptr);
+  auto step_ws = LiquidStepWorkspace3D::Create(caps, kGeometry);
+  auto boundary_ws = LiquidBoundaryWorkspace3D::Create(caps);
+  auto pressure_ws = MacPressureWorkspace3D::Create(caps.cells, 6 * caps.cells);
+  EXPECT(step_ws.ok() && boundary_ws.ok() && pressure_ws.ok());
+  EXPECT(table.initialize(table.context, committed).ok());
+
   // topology
-  std::vector<Vec3> positions;
-  for (const auto& p : particles) positions.push_back(p.sorted_world_position);
-  std::vector<LiquidParticleSortEntry3D> order(positions.size()), want_order(positions.size());
-  EXPECT(backend.topology(backend.context, kGeometry, positions, order).ok());
-  EXPECT(flo_sort_particles(O(&kGeometry), reinterpret_cast<const flo_vec3*>(positions.data()), positions.size(),
+  EXPECT(table.topology(table.context, committed.particles, candidates, committed, *step_ws).ok());
+  auto prepared = step_ws->prepared();
+  EXPECT(prepared.ok() && prepared->particles.size() == n && prepared->cells.size() == candidates.size());
+  std::vector<Vec3> positions(n);
+  for (std::size_t i = 0; i < n; ++i) positions[i] = committed.particles[i].world_position;
+  std::vector<LiquidParticleSortEntry3D> want_order(n);
+  EXPECT(flo_sort_particles(O(&kGeometry), reinterpret_cast<const flo_vec3*>(positions.data()), n,
                             reinterpret_cast<flo_sort_entry*>(want_order.data())) == 0);
-  for (std::size_t i = 0; i < order.size(); ++i)
-    EXPECT(order[i].original_ordinal == want_order[i].original_ordinal &&
-           order[i].bin_index.x == want_order[i].bin_index.x && order[i].bin_index.z == want_order[i].bin_index.z);
-  // deposit -> normalize -> forces
-  const MacDepositQuantization3D q{0x1p-40, 0x1p-40, particles.size()};
-  std::vector<MacFaceDepositQuanta3D> quanta(grid->face_count());
-  EXPECT(backend.deposit(backend.context, *grid, particles, q, quanta).ok());
-  const std::size_t nf = grid->face_count();
-  std::vector<double> mass(nf), u0(nf), want_mass(nf), want_u0(nf), pred(nf), want_pred(nf);
-  std::vector<LiquidFaceValidity3D> validity(nf);
+  for (std::size_t i = 0; i < n; ++i)
+    EXPECT(prepared->particle_order[i].original_ordinal == want_order[i].original_ordinal &&
+           prepared->particles[i].id == committed.particles[want_order[i].original_ordinal].id);
+  const SparseMacGridView3D grid = prepared->grid;
+  const auto blocks = grid.active_block_indices();
+  const std::size_t nf = grid.face_count();
+
+  // compatible deposit with every slot eligible on side 0
+  std::vector<LiquidBoundaryParticle3D> bp(n);
+  for (std::size_t i = 0; i < n; ++i) {
+    const auto& p = prepared->particles[i];
+    bp[i].transfer = {p.world_position, p.world_position, p.mass, p.world_velocity, p.velocity_slopes};
+  }
+  const std::vector<LiquidBoundaryFaceTag3D> tags(nf);
+  const MacDepositQuantization3D q{0x1p-40, 0x1p-40, n};
+  auto deposits = table.deposit(table.context, {grid, bp, tags, q}, *boundary_ws);
+  EXPECT(deposits.ok() && deposits->face_quanta.size() == nf && deposits->compatible_stencils.size() == n);
+  std::vector<MacFaceDepositQuanta3D> want_q(nf);
+  EXPECT(flo_deposit_compatible(O(&kGeometry), O(blocks), blocks.size(),
+                                reinterpret_cast<const flo_boundary_particle*>(bp.data()), n,
+                                reinterpret_cast<const flo_face_tag*>(tags.data()), nf,
+                                {q.mass_per_quantum, q.pure_momentum_per_quantum, q.declared_particle_capacity},
+                                std::uint64_t{1} << 40, reinterpret_cast<flo_face_quanta*>(want_q.data()), nf,
+                                nullptr) == 0);
+  EXPECT(std::memcmp(deposits->face_quanta.data(), want_q.data(), nf * sizeof(MacFaceDepositQuanta3D)) == 0);
+  // a rejected deposit (tags of the wrong extent) keeps the published quanta
+  const std::vector<LiquidBoundaryFaceTag3D> short_tags(nf - 1);
+  EXPECT(table.deposit(table.context, {grid, bp, short_tags, q}, *boundary_ws).status().code() ==
+         StatusCode::kInvalidArgument);
+  EXPECT(std::memcmp(deposits->face_quanta.data(), want_q.data(), nf * sizeof(MacFaceDepositQuanta3D)) == 0);
+
+  // normalize and forces
+  auto normalized = table.normalize(table.context, deposits->face_quanta, q, 1e-12, *boundary_ws);
+  EXPECT(normalized.ok());
+  std::vector<double> want_mass(nf), want_u0(nf), pred(nf), want_pred(nf);
   std::vector<std::uint8_t> want_validity(nf);
-  EXPECT(backend.normalize(backend.context, quanta, q, 1e-12, mass, u0, validity).ok());
-  EXPECT(flo_normalize(reinterpret_cast<const flo_face_quanta*>(quanta.data()), nf,
+  EXPECT(flo_normalize(reinterpret_cast<const flo_face_quanta*>(want_q.data()), nf,
                        {q.mass_per_quantum, q.pure_momentum_per_quantum, q.declared_particle_capacity}, 1e-12,
                        want_mass.data(), want_u0.data(), want_validity.data()) == 0);
-  EXPECT(std::memcmp(mass.data(), want_mass.data(), nf * 8) == 0 && std::memcmp(u0.data(), want_u0.data(), nf * 8) == 0);
-  EXPECT(std::memcmp(validity.data(), want_validity.data(), nf) == 0);
+  EXPECT(std::memcmp(normalized->face_masses.data(), want_mass.data(), nf * 8) == 0 &&
+         std::memcmp(normalized->face_velocities.data(), want_u0.data(), nf * 8) == 0 &&
+         std::memcmp(normalized->validity.data(), want_validity.data(), nf) == 0);
   const Vec3 g{0.0, -9.81, 0.0};
-  EXPECT(backend.forces(backend.context, *grid, u0, g, 1e-3, pred).ok());
-  const auto blocks = grid->active_block_indices();
-  EXPECT(flo_apply_forces(O(blocks), blocks.size(), u0.data(), nf, {g.x, g.y, g.z}, 1e-3, want_pred.data()) == 0);
+  EXPECT(table.forces(table.context, grid, normalized->face_velocities, g, dt, pred).ok());
+  EXPECT(flo_apply_forces(O(blocks), blocks.size(), want_u0.data(), nf, {g.x, g.y, g.z}, dt, want_pred.data()) == 0);
   EXPECT(std::memcmp(pred.data(), want_pred.data(), nf * 8) == 0);
-  std::vector<double> huge(nf, 1e308);
-  EXPECT(backend.forces(backend.context, *grid, huge, {0, 0, 1e308}, 10.0, pred).code() == StatusCode::kOutOfRange);
-  EXPECT(std::memcmp(pred.data(), want_pred.data(), nf * 8) == 0);
-  // gather through the table
-  std::vector<MacParticleGatherResult3D> got(particles.size());
-  EXPECT(backend.gather(backend.context, *grid, particles, u0, pred, got).ok());
+
+  // frozen geometry of the box level set: centre samples of cells [lo - 1, hi + 1)^3, corners [lo, hi]^3
+  const auto box = [&](double x, double y, double z) {
+    const double c = 0.5 * (lo + hi) * h, half = 0.5 * (hi - lo) * h;
+    return std::max({std::fabs(x - c), std::fabs(y - c), std::fabs(z - c)}) - half;
+  };
+  std::vector<LiquidCenterDistance3D> centers;
+  std::vector<LiquidCornerDistance3D> corners;
+  for (int z = lo - 1; z <= hi; ++z)
+    for (int y = lo - 1; y <= hi; ++y)
+      for (int x = lo - 1; x <= hi; ++x) centers.push_back({{x, y, z}, box((x + 0.5) * h, (y + 0.5) * h, (z + 0.5) * h)});
+  for (int z = lo; z <= hi; ++z)
+    for (int y = lo; y <= hi; ++y)
+      for (int x = lo; x <= hi; ++x) corners.push_back({{x, y, z}, box(x * h, y * h, z * h)});
+  const double rho = 1000.0;
+  auto frozen = table.freeze(table.context, {grid, candidates, centers, corners, {}, dt, rho, 0.0}, *boundary_ws);
+  EXPECT(frozen.ok() && frozen->cells.size() == candidates.size());
+  {
+    std::vector<flo_int3> fc;
+    for (const auto& c : candidates) fc.push_back({c.x, c.y, c.z});
+    std::vector<flo_center_distance> fce, fco;
+    for (const auto& c : centers) fce.push_back({{c.cell_index.x, c.cell_index.y, c.cell_index.z}, 0, c.signed_distance});
+    for (const auto& c : corners)
+      fco.push_back({{c.corner_index.x, c.corner_index.y, c.corner_index.z}, 0, c.signed_distance});
+    flo_freeze_inputs fi{};
+    fi.blocks = O(blocks);
+    fi.block_count = blocks.size();
+    fi.geometry = *O(&kGeometry);
+    fi.cell_indices = fc.data();
+    fi.cell_count = fc.size();
+    fi.center_distances = fce.data();
+    fi.center_count = fce.size();
+    fi.corner_distances = fco.data();
+    fi.corner_count = fco.size();
+    fi.step_duration = dt;
+    fi.liquid_density = rho;
+    std::vector<flo_frozen_cell> wc(fc.size());
+    std::vector<flo_frozen_face> wf(6 * fc.size());
+    std::uint64_t wnc = 0, wnf = 0;
+    double bound = 0;
+    EXPECT(flo_freeze_geometry(&fi, wc.data(), &wnc, wf.data(), &wnf, &bound) == 0);
+    EXPECT(wnc == frozen->cells.size() && wnf == frozen->faces.size());
+    for (std::size_t i = 0; i < wnc && i < frozen->cells.size(); ++i)
+      EXPECT(frozen->cells[i].fluid_volume == wc[i].fluid_volume && frozen->cells[i].curvature == wc[i].curvature);
+    for (std::size_t i = 0; i < wnf && i < frozen->faces.size(); ++i)
+      EXPECT(frozen->faces[i].face_slot == wf[i].face_slot && frozen->faces[i].open_area == wf[i].open_area &&
+             frozen->faces[i].blocked_area == wf[i].blocked_area);
+  }
+
+  // typed links, then the projection against the FP64 oracle
+  const std::vector<LiquidPressureCellFields3D> fields(frozen->cells.size(),
+                                                       {0.0, std::numeric_limits<double>::infinity(), 0.0});
+  std::vector<double> face_pred(frozen->faces.size());
+  for (std::size_t i = 0; i < face_pred.size(); ++i) face_pred[i] = pred[frozen->faces[i].face_slot];
+  auto problem = table.links(table.context, *frozen, {fields, face_pred, 0.0}, *boundary_ws);
+  EXPECT(problem.ok() && problem->faces.size() == frozen->faces.size());
+  auto projection = table.project(table.context, grid, dt, *problem, {256, 1.0}, *pressure_ws);
+  EXPECT(projection.ok());
+  std::vector<flo_pressure_cell> oc;
+  std::vector<flo_pressure_face> of;
+  for (const auto& c : problem->cells)
+    oc.push_back({{c.cell_index.x, c.cell_index.y, c.cell_index.z}, 0, c.fluid_volume, c.target_divergence,
+                  c.bulk_modulus, c.previous_pressure, c.integrated_inward_wall_flux});
+  for (const auto& f : problem->faces) {
+    flo_pressure_face r{};
+    r.axis = static_cast<std::int32_t>(f.face_index.axis);
+    r.lattice = {f.face_index.lattice_index.x, f.face_index.lattice_index.y, f.face_index.lattice_index.z};
+    r.open_area = f.open_area;
+    r.inverse_density = f.inverse_density;
+    r.pressure_gradient_distance = f.pressure_gradient_distance;
+    r.predicted_normal_velocity = f.predicted_normal_velocity;
+    r.wall_normal_velocity = f.wall_normal_velocity;
+    if (const auto* in = std::get_if<InternalMacPressureLink3D>(&f.pressure_link)) {
+      r.link_kind = 1;
+      r.lift = in->pressure_jump;
+    } else if (const auto* d = std::get_if<DirichletMacPressureLink3D>(&f.pressure_link)) {
+      r.link_kind = 2;
+      r.lift = d->exterior_pressure;
+    }
+    of.push_back(r);
+  }
+  std::vector<double> wp(oc.size()), wu(of.size()), wr(oc.size()), wres(oc.size()), wd(oc.size());
+  std::vector<std::uint64_t> wg(oc.size());
+  flo_pressure_diagnostics wdiag{};
+  const flo_pressure_options po{256, 1.0, 1e-12, 3, 2, 16, 0, 0.5, 0, 0};
+  EXPECT(flo_project(O(&kGeometry), O(blocks), blocks.size(), dt, oc.data(), oc.size(), of.data(), of.size(), &po,
+                     wp.data(), wu.data(), wr.data(), wres.data(), wd.data(), wg.data(), &wdiag) == 0);
+  double pmax = 0, perr = 0;
+  for (std::size_t i = 0; i < wp.size(); ++i) {
+    pmax = std::max(pmax, std::fabs(wp[i]));
+    perr = std::max(perr, std::fabs(projection->cell_pressures[i] - wp[i]));
+  }
+  EXPECT(projection->diagnostics.iteration_count > 0 && perr <= 1e-6 * pmax);
+  EXPECT(std::abs(projection->frozen_measurement.integrated_rhs_l2_m3_per_s - wdiag.rhs_l2) <=
+         1e-12 * wdiag.rhs_l2);
+  std::printf("  table projection: %zu cells, %zu iterations, pressure error %.2e of %.3e Pa\n", oc.size(),
+              projection->diagnostics.iteration_count, perr, pmax);
+
+  // boundary gather: the old history is the normalized field, the new one the predicted field with the
+  // corrected pressure-face velocities
+  std::vector<double> u_new(pred.begin(), pred.end());
+  for (std::size_t i = 0; i < frozen->faces.size(); ++i)
+    u_new[frozen->faces[i].face_slot] = projection->corrected_face_velocities[i];
+  const std::vector<LiquidPrescribedWallSample3D> walls(nf);
+  const LiquidBoundaryVelocityHistory3D old_h{0.0, 0.0, normalized->face_velocities, normalized->validity, tags, walls};
+  const LiquidBoundaryVelocityHistory3D new_h{dt, dt, u_new, normalized->validity, tags, walls};
+  auto gathered = table.gather(table.context, {grid, bp, 0.0, dt, old_h, new_h}, *boundary_ws);
+  EXPECT(gathered.ok() && gathered->particles.size() == n);
+  std::vector<LiquidBoundaryParticleGather3D> want_g(n);
+  {
+    const flo_history oh{0.0, 0.0, want_u0.data(), want_validity.data(), reinterpret_cast<const flo_face_tag*>(tags.data()),
+                         reinterpret_cast<const flo_wall_sample*>(walls.data()), nf};
+    const flo_history nh{dt, dt, u_new.data(), want_validity.data(), reinterpret_cast<const flo_face_tag*>(tags.data()),
+                         reinterpret_cast<const flo_wall_sample*>(walls.data()), nf};
+    std::uint64_t counts[3] = {};
+    EXPECT(flo_gather_boundary(O(&kGeometry), O(blocks), blocks.size(),
+                               reinterpret_cast<const flo_boundary_particle*>(bp.data()), n, 0.0, dt, &oh, &nh,
+                               reinterpret_cast<flo_boundary_gather*>(want_g.data()), counts) == 0);
+    EXPECT(counts[0] == gathered->old_pic_fallback_particle_count && counts[2] == gathered->low_mass_fallback_particle_count);
+  }
+  double gscale = 0;
+  for (const auto& w : want_g) gscale = std::max({gscale, std::fabs(w.new_pic_velocity.x), std::fabs(w.new_pic_velocity.y)});
+  for (std::size_t i = 0; i < n; ++i) {
+    const auto& a = gathered->particles[i];
+    const auto& b = want_g[i];
+    EXPECT(std::fabs(a.new_pic_velocity.y - b.new_pic_velocity.y) <= 1e-12 * gscale &&
+           std::fabs(a.grid_velocity_increment.x - b.grid_velocity_increment.x) <= 1e-12 * gscale &&
+           a.affine_enabled == b.affine_enabled && a.low_mass_fallback == b.low_mass_fallback);
+  }
+
+  // particle update of the sorted trial particles against the oracle, on the same gather records
+  std::vector<double> distances(n);
+  for (std::size_t i = 0; i < n; ++i) {
+    const Vec3 x = prepared->particles[i].world_position;
+    distances[i] = box(x.x, x.y, x.z);
+  }
+  std::vector<LiquidParticleState3D> trial(prepared->particles.begin(), prepared->particles.end());
+  std::vector<LiquidParticleState3D> want_p = trial;
+  const LiquidParticleUpdateInputs3D ui{gathered->particles, distances, {}, h, dt, 1.0, 0.0, 0.0};
+  EXPECT(table.update_particles(table.context, ui, trial).ok());
+  const flo_update_options uo{h, dt, 1.0, 0.0, 0.0, {}};
+  EXPECT(flo_update_particles(&uo, reinterpret_cast<const flo_boundary_gather*>(gathered->particles.data()),
+                              distances.data(), n, reinterpret_cast<flo_liquid_particle*>(want_p.data())) == 0);
+  std::size_t moved = 0;
+  for (std::size_t i = 0; i < n; ++i) {
+    EXPECT(trial[i].world_position.x == want_p[i].world_position.x &&
+           trial[i].world_position.y == want_p[i].world_position.y &&
+           trial[i].world_velocity.y == want_p[i].world_velocity.y);
+    moved += trial[i].world_position.y != prepared->particles[i].world_position.y;
+  }
+  EXPECT(moved > n / 2);
+
+  // publication: the prepared topology into a reserved trial state, then the hooks
+  LiquidState3D next;
+  next.geometry = kGeometry;
+  next.particles.reserve(caps.particles);
+  next.blocks.reserve(caps.blocks);
+  next.cells.reserve(caps.cells);
+  next.faces.reserve(caps.faces);
+  EXPECT(step_ws->ApplyTrialTopology(next).ok());
+  EXPECT(next.particles.size() == n && next.blocks.size() == blocks.size() && next.faces.size() == nf);
+  LiquidState3D small;
+  small.geometry = kGeometry;
+  EXPECT(step_ws->ApplyTrialTopology(small).code() == StatusCode::kResourceExhausted && small.particles.empty());
+  LiquidClock3D clock;
+  clock.epoch = 1;
+  EXPECT(table.prepare_publication(table.context, next, clock).ok());
+  table.commit_publication(table.context);
+  table.discard_publication(table.context);
+  // the second step's topology with the first as history: the surviving cells keep their records
+  next.particles.assign(trial.begin(), trial.end());
+  next.clock = clock;
+  EXPECT(table.topology(table.context, next.particles, candidates, next, *step_ws).ok());
+  auto second = step_ws->prepared();
+  EXPECT(second.ok() && second->cells.size() == candidates.size());
 }
 
 // Liquid box [2, 10)^3 in a sealed 16^3 tank: liquid-liquid faces internal, liquid-air Dirichlet p = 0 at
@@ -631,11 +862,9 @@ void BackendUpdateMatchesOracle() {
   const flo_update_options o{0.25, 0.01, 0.7, 0.0, 0.0};
   EXPECT(flo_update_particles(&o, reinterpret_cast<const flo_boundary_gather*>(gathered.data()), distances.data(),
                               count, reinterpret_cast<flo_liquid_particle*>(want.data())) == 0);
-  HipOperatorCapacities3D rc;
-  rc.particles = 1 << 12;
-  auto rt = HipOperatorRuntime3D::Create(rc);
-  EXPECT(rt.ok());
-  const auto backend = MakeHipLiquidSubstepBackend3D(**rt);
+  auto context = HipLiquidSubstepContext3D::Create({1 << 12, 64, 256, 192 * 64, 0});
+  EXPECT(context.ok());
+  const auto backend = MakeHipLiquidSubstepBackend3D(**context);
   EXPECT(backend.update_particles(backend.context, in, particles).ok());
   for (std::size_t i = 0; i < count; ++i) {
     const auto& a = particles[i];
@@ -862,7 +1091,7 @@ int main(int argc, char** argv) {
     failed += Run("CompatibleDepositMatchesOracle", CompatibleDepositMatchesOracle);
     failed += Run("BoundaryGatherMatchesOracle", BoundaryGatherMatchesOracle);
     failed += Run("GatherMatchesOracle", GatherMatchesOracle);
-    failed += Run("BackendStagesMatchOracle", BackendStagesMatchOracle);
+    failed += Run("SubstepTableDrivesStagesAgainstOracle", SubstepTableDrivesStagesAgainstOracle);
     failed += Run("ProjectionMeetsGatesAndMatchesOracle", ProjectionMeetsGatesAndMatchesOracle);
     failed += Run("WorkspaceRemapsHistoryAndMatchesOracle", WorkspaceRemapsHistoryAndMatchesOracle);
     failed += Run("BackendUpdateMatchesOracle", BackendUpdateMatchesOracle);
