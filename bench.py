@@ -48,6 +48,57 @@ def g2p_algorithmic_bytes(n_particles, n_faces):
     return 216 * n_particles + 16 * n_faces
 
 
+# Algorithmic bytes per cell slot (64 per 4^3 block) of the projection kernels, each array counted once
+# per launch; cb is the byte width of a V-cycle coefficient (2 for BF16 storage, 4 for FP32). None: the
+# single-workgroup tail cycle over the smallest levels is latency-bound and not priced.
+def projection_bytes_per_slot(name, cb):
+    return {
+        # other colour's x, own b and diagonal, the three coefficient arrays, own x written
+        "k_smooth": 8 + 3 * cb,
+        # every x, three coefficient arrays, red anchor, b and diagonal, red residual written
+        "k_residual_red": 12 + 3 * cb,
+        # red residuals read, coarse right-hand side (1/8 of the slots) written
+        "k_restrict_red": 2.5,
+        # diagonal and x read, coarse x (1/8) read, x written
+        "k_prolong": 12.5,
+        # b, diagonal and coefficients read, x written (level 1)
+        "k_block_smooth_pre": 12 + 3 * cb,
+        # x, b, diagonal, coefficients and coarse x (1/8) read, second buffer written (level 1)
+        "k_block_smooth_post": 16.5 + 3 * cb,
+        # x, coefficients, anchor, b and diagonal read, residual written (level 1)
+        "k_residual_all": 20 + 3 * cb,
+        # residuals read, coarse right-hand side written (level 1 to 2)
+        "k_restrict_all": 4.5,
+        "k_tail_vcycle": None,
+        # z, d, three FP32 coefficients, anchor and diagonal read, next d and q written
+        "k_direction_apply": 36,
+        # d, q, diagonal, volume, x and r read, x and r written
+        "k_update_xr": 32,
+        # r and z read
+        "k_dot": 8,
+        # per cell: p and seven FP64 cell arrays, six face ordinals, three 64-byte face records (each
+        # face is shared by two cells), residual and defect written
+        "k_evaluate": 8 + 56 + 24 + 192 + 16,
+    }[name]
+
+
+def projection_roofline(ctx, level_storage, reps=20):
+    s, rows = ctx.time_projection_kernels(reps)
+    assert s == H.OK, s
+    cb = 2 if level_storage == "bf16" else 4
+    out = []
+    for kernel, level, slots, ms in rows:
+        name = H.KERNEL_NAMES[kernel]
+        per = projection_bytes_per_slot(name, cb)
+        row = {"kernel": name, "level": level, "slots": slots, "ms": ms}
+        if per is not None:
+            b = per * slots
+            row.update(algorithmic_bytes=b, achieved_gbs=b / (ms * 1e-3) / 1e9,
+                       frac=b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
+        out.append(row)
+    return out
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 counter summary
     (FETCH_SIZE and WRITE_SIZE passes scaled by a copy of known size), or None."""
@@ -463,6 +514,8 @@ def main():
             "smoother": args.smoother,
             "relative_divergence_target": 1e-6,
             "device": device,
+            "step": "bin + stable sort, P2G quanta, normalize + gravity, MGPCG projection, G2P on the scene's fixed "
+                    "pressure structure (first-order free-surface links built once); particles are not advected",
         },
         "projection_ms": mean("projection_ms"),
         "projection_iterations": int(d["iteration_count"]),
@@ -485,6 +538,7 @@ def main():
             "copy_ceiling_gbs": copy_ceiling(),
         },
     }
+    out["roofline"]["vcycle"] = projection_roofline(ctx, args.level_storage)
     if not args.no_learned:
         out["learned"] = learned_comparison(ctx, scene, dt, opts, args)
     if not (args.no_cpu_baseline and args.no_oracle_error):

@@ -135,6 +135,7 @@ struct Pressure {
   unsigned char* level_pool = nullptr;
   uint64_t pool_bytes = 0, pool_used = 0;
   int levels_requested = 0;
+  int last_storage = 1, last_nu = 2, last_coarse_sweeps = 2;  // V-cycle settings of the last solve
   bool p_init_zero = false;  // warm start is identically zero: its evaluation equals the p = 0 one
   bool has_jump = false;
   double full_area = 0, full_volume = 0;
@@ -1592,6 +1593,9 @@ int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates&
   const unsigned rg = reduce_grid(n);
   diag.multigrid_level_count = P.levels.size();
   diag.iteration_count = 0;
+  P.last_storage = opt.level_storage;
+  P.last_nu = opt.smoothing_sweeps;
+  P.last_coarse_sweeps = opt.coarse_sweeps;
   diag.restarts = 0;
   EvalResult e{};
   k_copy_f64<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, P.p_base, P.p);
@@ -1742,6 +1746,117 @@ void destroy_pressure(fl_ctx* ctx) {
   if (P->hscal) (void)hipHostFree(P->hscal);
   delete P;
   ctx->pressure = nullptr;
+}
+
+namespace {
+
+// Mean device time of `reps` back-to-back launches of `launch` on the context stream.
+template <class Launch>
+int mean_ms(fl_ctx* ctx, int reps, Launch&& launch, double* ms) {
+  hipStream_t s = ctx->stream;
+  launch();  // warm-up
+  FL_HIP(hipGetLastError());
+  FL_HIP(hipEventRecord(ctx->stage[6], s));
+  for (int i = 0; i < reps; ++i) launch();
+  FL_HIP(hipEventRecord(ctx->stage[7], s));
+  FL_HIP(hipEventSynchronize(ctx->stage[7]));
+  float t = 0;
+  FL_HIP(hipEventElapsedTime(&t, ctx->stage[6], ctx->stage[7]));
+  *ms = static_cast<double>(t) / reps;
+  return FL_OK;
+}
+
+template <class KT>
+int time_levels(fl_ctx* ctx, Pressure& P, int reps, fl_kernel_time* out, size_t capacity, size_t* count) {
+  hipStream_t s = ctx->stream;
+  size_t k = 0;
+  auto put = [&](uint32_t kernel, uint32_t level, uint64_t slots, double ms) {
+    if (k < capacity) out[k] = fl_kernel_time{kernel, level, slots, ms};
+    ++k;
+  };
+  double ms = 0;
+  Level& L0 = P.levels[0];
+  const uint64_t n0 = L0.nb * 64;
+  const LevelViewT<KT> V0 = view<KT>(P, 0);
+  const int nu = std::max(P.last_nu, 1);
+  // Level-0 V-cycle kernels.
+  FL_TRY(mean_ms(ctx, reps, [&] { k_smooth<KT><<<grid_for(n0 / 2), kThreads, 0, s>>>(V0, 0, 0); }, &ms));
+  put(FL_KERNEL_SMOOTH, 0, n0, ms);
+  FL_TRY(mean_ms(ctx, reps, [&] { k_residual_red<KT><<<grid_for(n0 / 2), kThreads, 0, s>>>(V0, L0.r); }, &ms));
+  put(FL_KERNEL_RESIDUAL_RED, 0, n0, ms);
+  if (P.levels.size() > 1) {
+    const LevelViewT<KT> V1 = view<KT>(P, 1);
+    FL_TRY(mean_ms(ctx, reps, [&] { k_restrict_red<KT><<<grid_for(V1.n), kThreads, 0, s>>>(L0.r, V1); }, &ms));
+    put(FL_KERNEL_RESTRICT_RED, 0, n0, ms);
+    FL_TRY(mean_ms(ctx, reps, [&] { k_prolong<KT><<<grid_for(n0), kThreads, 0, s>>>(V0, P.levels[1].x); }, &ms));
+    put(FL_KERNEL_PROLONG, 0, n0, ms);
+    if (P.levels[1].nb > kTailBlocks && P.levels.size() > 2) {
+      const LevelViewT<KT> V2 = view<KT>(P, 2);
+      const unsigned gb = static_cast<unsigned>((P.levels[1].nb + 3) / 4);
+      FL_TRY(mean_ms(ctx, reps, [&] { k_block_smooth<KT, false><<<gb, 256, 0, s>>>(V1, nullptr, nullptr, nu); }, &ms));
+      put(FL_KERNEL_BLOCK_SMOOTH_PRE, 1, V1.n, ms);
+      FL_TRY(mean_ms(ctx, reps,
+                     [&] { k_block_smooth<KT, true><<<gb, 256, 0, s>>>(V1, P.levels[1].r, P.levels[2].x, nu); },
+                     &ms));
+      put(FL_KERNEL_BLOCK_SMOOTH_POST, 1, V1.n, ms);
+      FL_TRY(mean_ms(ctx, reps, [&] { k_residual_all<KT><<<grid_for(V1.n), kThreads, 0, s>>>(V1, P.levels[1].r); },
+                     &ms));
+      put(FL_KERNEL_RESIDUAL_ALL, 1, V1.n, ms);
+      FL_TRY(mean_ms(ctx, reps, [&] { k_restrict_all<KT><<<grid_for(V2.n), kThreads, 0, s>>>(P.levels[1].r, V2); },
+                     &ms));
+      put(FL_KERNEL_RESTRICT_ALL, 1, V1.n, ms);
+    }
+  }
+  // The single-workgroup tail from the first level of at most kTailBlocks blocks.
+  size_t first = P.levels.size();
+  for (size_t l = 0; l < P.levels.size(); ++l)
+    if (P.levels[l].nb <= kTailBlocks) {
+      first = l;
+      break;
+    }
+  if (first < P.levels.size() && P.levels.size() - first <= static_cast<size_t>(kMaxTail)) {
+    TailCycle<KT> T{};
+    T.count = static_cast<int>(P.levels.size() - first);
+    T.nu = nu;
+    T.coarse_sweeps = P.last_coarse_sweeps;
+    for (size_t q = first; q < P.levels.size(); ++q) T.lv[q - first] = view<KT>(P, q);
+    FL_TRY(mean_ms(ctx, reps, [&] { k_tail_vcycle<<<1, 1024, 0, s>>>(T); }, &ms));
+    put(FL_KERNEL_TAIL_VCYCLE, static_cast<uint32_t>(first), P.levels[first].nb * 64, ms);
+  }
+  // PCG kernels on level 0 (FP32 vectors and coefficients).
+  const unsigned rg = reduce_grid(n0);
+  FL_TRY(mean_ms(ctx, reps, [&] {
+    k_direction_apply<<<rg, kThreads, 0, s>>>(n0, L0.nbr, L0.k, L0.anchor, L0.diag, L0.x, P.d32, P.dn32, P.q32,
+                                              P.fscal, 0, P.partial);
+  }, &ms));
+  put(FL_KERNEL_DIRECTION_APPLY, 0, n0, ms);
+  FL_TRY(mean_ms(ctx, reps, [&] {
+    k_update_xr<<<rg, kThreads, 0, s>>>(n0, P.d32, P.q32, L0.diag, P.fine_vol, P.x32, P.r32, P.fscal, P.partial,
+                                        P.perr);
+  }, &ms));
+  put(FL_KERNEL_UPDATE_XR, 0, n0, ms);
+  FL_TRY(mean_ms(ctx, reps, [&] { k_dot<<<rg, kThreads, 0, s>>>(n0, P.r32, L0.x, P.partial); }, &ms));
+  put(FL_KERNEL_DOT, 0, n0, ms);
+  // The FP64 evaluation of the accepted pressure (recomputes the same residuals and defects).
+  const unsigned g = reduce_grid(P.nc, kPartials);
+  FL_TRY(mean_ms(ctx, reps, [&] {
+    k_evaluate<<<g, kThreads, 0, s>>>(P.nc, P.cell_faces, P.face_rec, P.cell_volume, P.cell_target, P.cell_inwall,
+                                      P.cell_comp, P.cell_prev, P.rhs, P.rhs_abs, P.p, P.res, P.def, P.partial,
+                                      P.perr);
+  }, &ms));
+  put(FL_KERNEL_EVALUATE, 0, P.nc, ms);
+  *count = k;
+  return k <= capacity ? FL_OK : FL_RESOURCE_EXHAUSTED;
+}
+
+}  // namespace
+
+int time_projection_kernels(fl_ctx* ctx, int reps, fl_kernel_time* out, size_t capacity, size_t* count) {
+  Pressure* P = state(ctx);
+  if (!P || P->levels.empty() || P->nc == 0) return FL_FAILED_PRECONDITION;
+  (void)hipSetDevice(ctx->device);
+  if (P->last_storage == 2) return time_levels<Bf16>(ctx, *P, reps, out, capacity, count);
+  return time_levels<float>(ctx, *P, reps, out, capacity, count);
 }
 
 int reserve_pressure(fl_ctx* ctx) {
@@ -2391,6 +2506,12 @@ int fl_scene_prepare(fl_ctx* ctx, const fl_pressure_cell* cells, size_t cell_cou
                         false));
   ctx->scene_ready = true;
   return FL_OK;
+}
+
+int fl_time_projection_kernels(fl_ctx* ctx, int reps, fl_kernel_time* out, size_t capacity, size_t* count) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  if (reps < 1 || !count || (capacity && !out)) return FL_INVALID_ARGUMENT;
+  return fl::time_projection_kernels(ctx, reps, out, capacity, count);
 }
 
 int fl_step(fl_ctx* ctx, double dt, fl_vec3 gravity, fl_quantization quantization, const fl_pressure_options* options,

@@ -54,7 +54,7 @@ EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_reserve", "fl_bin_so
            "fl_normalize_forces", "fl_apply_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
            "fl_scene_download", "fl_probe_copy", "fl_learn_param_count", "fl_learn_setup", "fl_learn_set_params", "fl_learn_get_params",
            "fl_learn_forward", "fl_learn_infer", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect", "fl_learn_eval",
-           "fl_learn_train", "fl_normalize_forces_static")
+           "fl_learn_train", "fl_normalize_forces_static", "fl_time_projection_kernels")
 
 _lib = None
 
@@ -145,6 +145,15 @@ def velocity_history(time, velocities, validity, tags, walls, wall_time=None):
                         validity.ctypes.data, tags.ctypes.data, walls.ctypes.data, len(velocities))
     h._keep = (velocities, validity, tags, walls)
     return h
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("kernel", C.c_uint32), ("level", C.c_uint32), ("slots", C.c_uint64), ("ms", C.c_double)]
+
+
+KERNEL_NAMES = ("k_smooth", "k_residual_red", "k_restrict_red", "k_prolong", "k_block_smooth_pre",
+                "k_block_smooth_post", "k_residual_all", "k_restrict_all", "k_tail_vcycle", "k_direction_apply",
+                "k_update_xr", "k_dot", "k_evaluate")
 
 
 class Context:
@@ -356,6 +365,14 @@ class Context:
     def scene_prepare(self, cells, domain_cells, density):
         return lib().fl_scene_prepare(self.handle, _p(cells), C.c_size_t(len(cells)), C.c_int32(domain_cells),
                                       C.c_double(density))
+
+    def time_projection_kernels(self, reps=20):
+        """fl_time_projection_kernels: [(kernel, level, slots, ms)] for the last projected system."""
+        cap = 32
+        out = (KernelTime * cap)()
+        count = C.c_size_t(0)
+        s = lib().fl_time_projection_kernels(self.handle, C.c_int(reps), out, C.c_size_t(cap), C.byref(count))
+        return s, [(int(r.kernel), int(r.level), int(r.slots), float(r.ms)) for r in out[:count.value]]
 
     def step(self, dt, gravity, mass_q, mom_q, cap, options):
         rep = StepReport()

@@ -614,6 +614,36 @@ int fl_scene_download(fl_ctx* ctx, fl_int3* blocks, size_t block_capacity, fl_fa
  * its two buffers. */
 int fl_probe_copy(int device, size_t bytes, int reps, double* gbs);
 
+/* Measurement of the projection's kernels on the last system the context projected (fl_project or
+ * fl_step): each V-cycle and PCG kernel is launched `reps` times back to back on the context stream
+ * between HIP events, on the arrays of that system, and its mean launch time is reported with the
+ * level and the number of cell slots (64 per 4^3 block; cells for the evaluation) it covers. The
+ * launches overwrite only solver scratch (level vectors, PCG vectors) and recompute the accepted
+ * pressure's residuals and defects. FL_FAILED_PRECONDITION before any projection; *count receives the
+ * number of entries (FL_RESOURCE_EXHAUSTED when it exceeds capacity). */
+enum {
+  FL_KERNEL_SMOOTH = 0,            /* red-black Gauss-Seidel half sweep, level 0 */
+  FL_KERNEL_RESIDUAL_RED = 1,      /* residual of the red cells, level 0 */
+  FL_KERNEL_RESTRICT_RED = 2,      /* sum of the red children into the coarse right-hand side */
+  FL_KERNEL_PROLONG = 3,           /* coarse correction added to level 0 */
+  FL_KERNEL_BLOCK_SMOOTH_PRE = 4,  /* block smoother, pre-smoothing, level 1 */
+  FL_KERNEL_BLOCK_SMOOTH_POST = 5, /* block smoother with prolongation, post-smoothing, level 1 */
+  FL_KERNEL_RESIDUAL_ALL = 6,      /* residual of every cell, level 1 */
+  FL_KERNEL_RESTRICT_ALL = 7,      /* sum of all children, level 1 to 2 */
+  FL_KERNEL_TAIL_VCYCLE = 8,       /* single-workgroup cycle of the smallest levels */
+  FL_KERNEL_DIRECTION_APPLY = 9,   /* d = z + beta d, q = A d, d . q (FP32, FP64 partials) */
+  FL_KERNEL_UPDATE_XR = 10,        /* x += alpha d, r -= alpha q, residual norms */
+  FL_KERNEL_DOT = 11,              /* r . z */
+  FL_KERNEL_EVALUATE = 12          /* FP64 residual, defect and identity check */
+};
+typedef struct fl_kernel_time {
+  uint32_t kernel;
+  uint32_t level;
+  uint64_t slots;
+  double ms;
+} fl_kernel_time;
+int fl_time_projection_kernels(fl_ctx* ctx, int reps, fl_kernel_time* out, size_t capacity, size_t* count);
+
 /* ---- learned preconditioner (no counterpart in the reference) ----
  * A 3-level 3D U-Net (channels 8/16/32, 3x3x3 convolutions) on 8^3 tiles of the level-0 pressure grid
  * maps (residual / its RMS, liquid, air, solid) to a correction; with fl_pressure_options.preconditioner
