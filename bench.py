@@ -419,6 +419,56 @@ def boundary_measurement(ctx, scene, cells, dt, cpu_candidates=1 << 17):
             "cpu_freeze_candidates_per_s": k / cpu_s, "cpu_sample": f"first {k} candidates, 1 thread"}
 
 
+def viscosity_measurement(ctx, scene, dt, report, nu=1e-3, cpu_cells=1 << 16):
+    """fl_assemble_full_cell_strain then fl_solve_viscous_block on the workload's liquid cells with the
+    kinematic viscosity nu (dynamic viscosity rho * nu per cell): the backward-Euler viscous velocity block
+    with the step's face masses as inertia (massless faces eliminated) and mass * u* as right-hand side,
+    tolerance 1e-10 of the largest |rhs|. The oracle's assembly and solve of the first cpu_cells cells,
+    on one host thread, is its CPU rate."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    nb = int(report.blocks)
+    s, dl = ctx.scene_download(nb, 0, 0, what=("blocks", "quanta", "ustar"))
+    assert s == H.OK, s
+    pc = scene.pressure_cells()
+    ci = pc["cell_index"]
+    order = np.lexsort((ci["x"], ci["y"], ci["z"]))
+    cells = np.zeros(len(pc), R.VISCOSITY_CELL)
+    cells["cell_index"] = ci[order]
+    cells["dynamic_viscosity"] = scene.rho * nu
+    geom = scene.geometry_record()
+    for _ in range(2):  # the first call also pays the first touch of its device buffers
+        s, rows = ctx.assemble_full_cell_strain(geom, dl["blocks"], cells)
+        if s != H.OK:
+            log(f"viscosity: assembly status {s}")
+            return {"status": int(s)}
+    am = (ctx.last.kernel_ms, ctx.last.copy_ms)
+    mass = dl["quanta"]["mass_quanta_sum"].astype(np.float64) * QUANTUM
+    rhs = mass * dl["ustar"]
+    blocks = dl["blocks"]
+    del dl
+    for _ in range(2):
+        s, v, it, mr = ctx.solve_viscous_block(rows, mass, dt, rhs)
+        assert s == H.OK, s
+    sm = (ctx.last.kernel_ms, ctx.last.copy_ms)
+    log(f"viscosity: {len(cells):,} cells -> {len(rows):,} strain rows in {am[0]:.2f} ms; velocity block "
+        f"{it} iterations in {sm[0]:.2f} ms on the device (+{am[1] + sm[1]:.1f} ms of host copies)")
+    k = min(cpu_cells, len(cells))
+    t0 = time.perf_counter()
+    so, orows = O.assemble_full_cell_strain(geom.view(O.GEOMETRY), blocks.view(O.INT3),
+                                            cells[:k].view(O.VISCOSITY_CELL))
+    assert so == O.OK, so
+    so, _, oit, _ = O.solve_viscous_block(orows, mass, dt, rhs)
+    cpu_s = time.perf_counter() - t0
+    assert so == O.OK, so
+    return {"cells": len(cells), "strain_rows": len(rows), "faces": len(mass), "kinematic_viscosity": nu,
+            "step_duration": dt, "assemble_kernel_ms": am[0], "solve_kernel_ms": sm[0], "iterations": int(it),
+            "max_residual_over_max_rhs": mr / max(float(np.abs(rhs).max()), 1e-300),
+            "copy_ms": am[1] + sm[1], "cells_per_s": len(cells) / ((am[0] + sm[0]) * 1e-3),
+            "cpu_cells_per_s": k / cpu_s, "cpu_iterations": int(oit),
+            "cpu_sample": f"first {k} cells (z,y,x order), assembly + solve, 1 thread"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -442,6 +492,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="only measure the plain-copy ceiling")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner comparison")
     ap.add_argument("--no-topology", action="store_true", help="skip the topology-preparation measurement")
+    ap.add_argument("--no-viscosity", action="store_true", help="skip the viscosity measurement")
     ap.add_argument("--learn-pairs", type=int, default=50000)
     ap.add_argument("--learn-epochs", type=int, default=8)
     args = ap.parse_args()
@@ -550,6 +601,8 @@ def main():
             out["cpu_baseline"] = cpu_baseline(scene, pin, args.cpu_sample, args.cpu_threads)
             log(f"cpu baseline took {time.perf_counter() - t0:.1f}s")
         del pin
+    if not args.no_viscosity:
+        out["viscosity"] = viscosity_measurement(ctx, scene, dt, last)
     if not args.no_topology:  # last: span-based calls reuse the resident scene's buffers
         out["topology"] = topology_measurement(ctx, scene, cells, dt)
         out["particle_update"] = update_measurement(ctx)

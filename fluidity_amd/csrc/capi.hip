@@ -302,6 +302,32 @@ struct FreezeBuffers {
   }
 };
 
+// Device arrays of the viscosity operators; allocated on first use (or by fl_reserve) and grown with them.
+struct ViscosityBuffers {
+  fl_viscosity_cell* cells = nullptr;
+  fl_strain_row* rows = nullptr;
+  uint64_t *counts = nullptr, *offsets = nullptr, *inc_offset = nullptr, *entries = nullptr;
+  double *inertia = nullptr, *rhs = nullptr, *v = nullptr, *r = nullptr, *z = nullptr, *d = nullptr, *q = nullptr,
+         *diag = nullptr, *strain = nullptr, *partial = nullptr, *scal = nullptr;
+  uint32_t *flag = nullptr, *inc_count = nullptr, *cursor = nullptr;
+  uint64_t cap[20] = {};
+  void release() {
+    void* ptrs[] = {cells, rows, counts, offsets, inc_offset, entries, inertia, rhs, v, r, z, d, q, diag, strain,
+                    partial, scal, flag, inc_count, cursor};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    *this = ViscosityBuffers{};
+  }
+};
+
+void destroy_viscosity(fl_ctx* ctx) {
+  auto* b = static_cast<ViscosityBuffers*>(ctx->viscosity);
+  if (!b) return;
+  b->release();
+  delete b;
+  ctx->viscosity = nullptr;
+}
+
 void destroy_freeze(fl_ctx* ctx) {
   auto* b = static_cast<FreezeBuffers*>(ctx->freeze);
   if (!b) return;
@@ -340,6 +366,41 @@ int grow(T** p, uint64_t* cap, uint64_t need) {
   }
   *cap = n;
   return FL_OK;
+}
+
+// Viscosity arrays for n cells, nr strain rows and nf faces.
+int grow_viscosity(fl_ctx* ctx, uint64_t n, uint64_t nr, uint64_t nf) {
+  if (!ctx->viscosity) ctx->viscosity = new (std::nothrow) ViscosityBuffers();
+  auto* B = static_cast<ViscosityBuffers*>(ctx->viscosity);
+  if (!B) return FL_RESOURCE_EXHAUSTED;
+  FL_TRY(grow(&B->cells, &B->cap[0], n));
+  FL_TRY(grow(&B->rows, &B->cap[1], nr));
+  FL_TRY(grow(&B->counts, &B->cap[2], std::max(n, nf)));
+  FL_TRY(grow(&B->offsets, &B->cap[3], n));
+  FL_TRY(grow(&B->inc_offset, &B->cap[4], nf));
+  FL_TRY(grow(&B->entries, &B->cap[5], 8 * nr));
+  double** fv[] = {&B->inertia, &B->rhs, &B->v, &B->r, &B->z, &B->d, &B->q, &B->diag};
+  for (int i = 0; i < 8; ++i) FL_TRY(grow(fv[i], &B->cap[6 + i], nf));
+  FL_TRY(grow(&B->strain, &B->cap[14], nr));
+  FL_TRY(grow(&B->partial, &B->cap[15], 1024));
+  FL_TRY(grow(&B->scal, &B->cap[16], 8));
+  FL_TRY(grow(&B->flag, &B->cap[17], 1));
+  FL_TRY(grow(&B->inc_count, &B->cap[18], nf));
+  FL_TRY(grow(&B->cursor, &B->cap[19], nf));
+  return FL_OK;
+}
+
+// Whether the context's scan partials hold an exclusive scan of n values (the tile sums of two levels).
+bool scan_fits(fl_ctx* ctx, uint64_t n) {
+  const uint64_t partials = std::max<uint64_t>(std::max<uint64_t>(64, ctx->cap.lookup_blocks) / 128 + 4096, 8192);
+  return 2 * (n / 1024 + 2) + 16 <= partials;
+}
+
+ViscosityDevice viscosity_device(fl_ctx* ctx) {
+  auto& B = *static_cast<ViscosityBuffers*>(ctx->viscosity);
+  return ViscosityDevice{B.cells, B.rows,  B.cap[1], B.counts, B.offsets, B.inertia, B.rhs,       B.v,
+                         B.r,     B.z,     B.d,      B.q,      B.diag,    B.strain,  B.partial,   B.scal,
+                         B.flag,  B.inc_count, B.cursor, B.inc_offset, B.entries};
 }
 
 int first_error_code(fl_ctx* ctx) {
@@ -470,6 +531,7 @@ void fl_destroy(fl_ctx* ctx) {
   destroy_update(ctx);
   destroy_links(ctx);
   destroy_freeze(ctx);
+  destroy_viscosity(ctx);
   release(ctx);
   for (hipEvent_t e : ctx->stage)
     if (e) (void)hipEventDestroy(e);
@@ -583,6 +645,8 @@ int fl_reserve(fl_ctx* ctx) {
     FL_TRY(grow(&B.out_faces, &B.cap[10], pf));
     FL_TRY(grow(&B.lower, &B.cap[11], 1));
   }
+  // Viscosity of up to pressure_cells cells (six strain rows each) on the owned face slots.
+  FL_TRY(grow_viscosity(ctx, nc, 6 * nc, nf));
   FL_HIP(hipStreamSynchronize(ctx->stream));
   return FL_OK;
 }
@@ -1406,6 +1470,94 @@ int fl_p2g(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size
   if (nf) FL_HIP(hipMemcpyAsync(out, staged, nf * sizeof(fl_face_deposit), hipMemcpyDeviceToHost, ctx->stream));
   copy_ms += c.stop();
   fill_measurement(ctx, measurement, kernel_ms, copy_ms, 10);
+  return FL_OK;
+}
+
+int fl_assemble_full_cell_strain(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
+                                 const fl_viscosity_cell* cells, size_t cell_count, fl_strain_row* out,
+                                 size_t* io_count, fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  if (!io_count || (cell_count && (!cells || !out)) || (block_count && !blocks)) return FL_INVALID_ARGUMENT;
+  FL_TRY(check_geometry(geometry));
+  FL_TRY(check_blocks(blocks, block_count));
+  if (cell_count > ctx->cap.pressure_cells || !scan_fits(ctx, cell_count)) return FL_RESOURCE_EXHAUSTED;
+  (void)hipSetDevice(ctx->device);
+  set_geometry(ctx, geometry);
+  const uint64_t n = cell_count, capacity = 6 * n;
+  FL_TRY(grow_viscosity(ctx, n, capacity, 1));
+  double copy_ms = 0;
+  {
+    Timer c(ctx);
+    FL_TRY(build_lookup(ctx, reinterpret_cast<const int32_t*>(blocks), block_count));
+    auto& B = *static_cast<ViscosityBuffers*>(ctx->viscosity);
+    if (n) FL_HIP(hipMemcpyAsync(B.cells, cells, n * sizeof(fl_viscosity_cell), hipMemcpyHostToDevice, ctx->stream));
+    copy_ms += c.stop();
+  }
+  Timer t(ctx);
+  uint64_t total = 0;
+  FL_TRY(fl::assemble_strain(ctx, viscosity_device(ctx), n, geometry->cell_width, &total));
+  const double kernel_ms = t.stop();
+  if (total > *io_count) return FL_RESOURCE_EXHAUSTED;
+  Timer c(ctx);
+  auto& B = *static_cast<ViscosityBuffers*>(ctx->viscosity);
+  if (total) FL_HIP(hipMemcpyAsync(out, B.rows, total * sizeof(fl_strain_row), hipMemcpyDeviceToHost, ctx->stream));
+  FL_HIP(hipStreamSynchronize(ctx->stream));
+  copy_ms += c.stop();
+  *io_count = total;
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, 5);
+  return FL_OK;
+}
+
+int fl_solve_viscous_block(fl_ctx* ctx, const fl_strain_row* rows, size_t row_count, const double* inertia,
+                           size_t face_count, double dt, const double* rhs, uint64_t max_iterations,
+                           double relative_tolerance, double* out_velocity, uint64_t* out_iterations,
+                           double* out_max_residual, fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  if (!out_iterations || !out_max_residual || (row_count && !rows) ||
+      (face_count && (!inertia || !rhs || !out_velocity)))
+    return FL_INVALID_ARGUMENT;
+  // ValidateVelocityBlockInputs (viscosity.cc:1331-1358), in its order.
+  if (!std::isfinite(dt) || dt < 0.0) return FL_INVALID_ARGUMENT;
+  if (max_iterations == 0 || !std::isfinite(relative_tolerance) || relative_tolerance <= 0.0 ||
+      relative_tolerance >= 1.0)
+    return FL_INVALID_ARGUMENT;
+  double max_rhs = 0.0;
+  for (size_t f = 0; f < face_count; ++f) {
+    if (!std::isfinite(inertia[f]) || inertia[f] < 0.0 || !std::isfinite(rhs[f])) return FL_INVALID_ARGUMENT;
+    if (inertia[f] == 0.0 && rhs[f] != 0.0) return FL_INVALID_ARGUMENT;
+    max_rhs = std::max(max_rhs, std::fabs(rhs[f]));
+  }
+  if (face_count > 192 * std::max<uint64_t>(ctx->cap.blocks, 1) ||
+      row_count > 6 * std::max<uint64_t>(ctx->cap.pressure_cells, 1) || !scan_fits(ctx, face_count))
+    return FL_RESOURCE_EXHAUSTED;
+  (void)hipSetDevice(ctx->device);
+  FL_TRY(grow_viscosity(ctx, 1, std::max<uint64_t>(row_count, 1), std::max<uint64_t>(face_count, 1)));
+  auto& B = *static_cast<ViscosityBuffers*>(ctx->viscosity);
+  double copy_ms = 0;
+  {
+    Timer c(ctx);
+    if (row_count)
+      FL_HIP(hipMemcpyAsync(B.rows, rows, row_count * sizeof(fl_strain_row), hipMemcpyHostToDevice, ctx->stream));
+    if (face_count) {
+      FL_HIP(hipMemcpyAsync(B.inertia, inertia, face_count * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+      FL_HIP(hipMemcpyAsync(B.rhs, rhs, face_count * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
+    copy_ms += c.stop();
+  }
+  Timer t(ctx);
+  uint64_t iterations = 0;
+  double max_residual = 0.0;
+  FL_TRY(fl::solve_viscous_block(ctx, viscosity_device(ctx), row_count, face_count, dt, max_rhs, max_iterations,
+                                 relative_tolerance, &iterations, &max_residual));
+  const double kernel_ms = t.stop();
+  Timer c(ctx);
+  if (face_count)
+    FL_HIP(hipMemcpyAsync(out_velocity, B.v, face_count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  FL_HIP(hipStreamSynchronize(ctx->stream));
+  copy_ms += c.stop();
+  *out_iterations = iterations;
+  *out_max_residual = max_residual;
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, static_cast<uint32_t>(8 + 12 * iterations));
   return FL_OK;
 }
 

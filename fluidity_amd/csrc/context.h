@@ -164,6 +164,7 @@ struct fl_ctx {
   // pressure-link buffers (owned by capi.hip, allocated on first use)
   void* links = nullptr;
   void* freeze = nullptr;
+  void* viscosity = nullptr;  // ViscosityBuffers (capi.hip)
 };
 
 namespace fl {
@@ -317,6 +318,27 @@ int update_particles(fl_ctx* ctx, const fl_particle_update_options& o, const fl_
                      const double* distances, uint64_t n, fl_liquid_particle* particles);
 
 // Small helpers (topology.hip)
+// Variational MAC viscosity (viscosity.hip): device arrays of one strain assembly or velocity-block solve.
+struct ViscosityDevice {
+  const fl_viscosity_cell* cells;
+  fl_strain_row* rows;
+  uint64_t row_capacity;
+  uint64_t* counts;   // per cell shear rows, then per face incidence counts widened for the scan
+  uint64_t* offsets;  // exclusive scan of the shear counts
+  const double* inertia;
+  const double* rhs;
+  double *v, *r, *z, *d, *q, *diag, *strain, *partial, *scal;
+  uint32_t* flag;
+  uint32_t *inc_count, *cursor;
+  uint64_t *inc_offset, *entries;
+};
+// AssembleFullCellMacStrain3D of n cells on the grid of the context's lookup; *total rows written.
+int assemble_strain(fl_ctx* ctx, const ViscosityDevice& v, uint64_t n, double h, uint64_t* total);
+// SolveMacViscousVelocityBlock3D after the host's input validation (max_rhs from it); v.rows are
+// reweighted in place.
+int solve_viscous_block(fl_ctx* ctx, const ViscosityDevice& v, uint64_t nr, uint64_t nf, double dt, double max_rhs,
+                        uint64_t max_iterations, double relative_tolerance, uint64_t* iterations,
+                        double* max_residual);
 int read_error(fl_ctx* ctx, uint64_t* word);
 int reset_error(fl_ctx* ctx);
 int fill_i64(fl_ctx* ctx, int64_t* p, uint64_t n, int64_t v);

@@ -54,7 +54,7 @@ EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_reserve", "fl_bin_so
            "fl_normalize_forces", "fl_apply_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
            "fl_scene_download", "fl_probe_copy", "fl_learn_param_count", "fl_learn_setup", "fl_learn_set_params", "fl_learn_get_params",
            "fl_learn_forward", "fl_learn_infer", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect", "fl_learn_eval",
-           "fl_learn_train", "fl_normalize_forces_static", "fl_time_projection_kernels")
+           "fl_learn_train", "fl_normalize_forces_static", "fl_time_projection_kernels", "fl_assemble_full_cell_strain", "fl_solve_viscous_block")
 
 _lib = None
 
@@ -365,6 +365,29 @@ class Context:
     def scene_prepare(self, cells, domain_cells, density):
         return lib().fl_scene_prepare(self.handle, _p(cells), C.c_size_t(len(cells)), C.c_int32(domain_cells),
                                       C.c_double(density))
+
+    def assemble_full_cell_strain(self, geometry, blocks, cells):
+        """fl_assemble_full_cell_strain: (status, strain rows)."""
+        cells = np.ascontiguousarray(cells, dtype=R.VISCOSITY_CELL)
+        out = np.zeros(max(6 * len(cells), 1), R.STRAIN_ROW)
+        count = C.c_size_t(len(out))
+        s = lib().fl_assemble_full_cell_strain(self.handle, _p(geometry), _p(blocks), C.c_size_t(len(blocks)),
+                                               _p(cells), C.c_size_t(len(cells)), _p(out), C.byref(count),
+                                               C.byref(self.last))
+        return s, out[:count.value] if s == OK else None
+
+    def solve_viscous_block(self, rows, inertia, dt, rhs, max_iterations=256, relative_tolerance=1e-10):
+        """fl_solve_viscous_block: (status, velocity, iterations, max residual)."""
+        rows = np.ascontiguousarray(rows, dtype=R.STRAIN_ROW)
+        inertia = np.ascontiguousarray(inertia, dtype=np.float64)
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        v = np.zeros(len(inertia))
+        it, mr = C.c_uint64(0), C.c_double(0.0)
+        s = lib().fl_solve_viscous_block(self.handle, _p(rows), C.c_size_t(len(rows)), _p(inertia),
+                                         C.c_size_t(len(inertia)), C.c_double(dt), _p(rhs), C.c_uint64(max_iterations),
+                                         C.c_double(relative_tolerance), _p(v), C.byref(it), C.byref(mr),
+                                         C.byref(self.last))
+        return s, v, it.value, mr.value
 
     def time_projection_kernels(self, reps=20):
         """fl_time_projection_kernels: [(kernel, level, slots, ms)] for the last projected system."""

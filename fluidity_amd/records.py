@@ -105,3 +105,12 @@ assert PRESSURE_CELL.itemsize == 56 and PRESSURE_FACE.itemsize == 80 and FACE_QU
 assert BLOCK_STATE.itemsize == 32 and CELL_STATE.itemsize == 24 and FACE_STATE.itemsize == 56
 assert LIQUID_PARTICLE.itemsize == 328 and UPDATE_OPTIONS.itemsize == 160 and OBSTACLE.itemsize == 120
 assert FROZEN_FACE.itemsize == 72 and FROZEN_CELL.itemsize == 40 and CELL_FIELDS.itemsize == 40
+
+# MacViscosityCell3D, MacStrainTerm3D, MacStrainRow3D (native/include/fluidity/host/viscosity.h:47-66).
+VISCOSITY_CELL = np.dtype({"names": ["cell_index", "dynamic_viscosity"], "formats": [INT3, "f8"], "offsets": [0, 16],
+                           "itemsize": 24})
+STRAIN_TERM = np.dtype([("face_slot", "u8"), ("coefficient", "f8")])
+STRAIN_ROW = np.dtype({"names": ["low_cell", "first_axis", "second_axis", "terms", "term_count", "quadrature_weight",
+                                 "prescribed_strain"],
+                       "formats": [INT3, "u1", "u1", (STRAIN_TERM, 8), "u8", "f8", "f8"],
+                       "offsets": [0, 12, 13, 16, 144, 152, 160], "itemsize": 168})

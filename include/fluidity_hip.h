@@ -425,6 +425,43 @@ int fl_device_name(fl_ctx* ctx, char* out, size_t size);
  * cell, link fields at most one per cell or pressure face) allocates and frees no device memory. */
 int fl_reserve(fl_ctx* ctx);
 
+/* ---- variational MAC viscosity (native/src/viscosity.cc; host/viscosity.h:47-160) ---- */
+/* MacViscosityCell3D, MacStrainTerm3D and MacStrainRow3D (axes 0/1/2). */
+typedef struct fl_viscosity_cell {
+  fl_int3 cell_index;
+  uint32_t pad;
+  double dynamic_viscosity; /* Pa s, finite and nonnegative */
+} fl_viscosity_cell;       /* 24 B */
+typedef struct fl_strain_term {
+  uint64_t face_slot;
+  double coefficient; /* 1/m */
+} fl_strain_term;
+typedef struct fl_strain_row {
+  fl_int3 low_cell;
+  uint8_t first_axis, second_axis, pad[2];
+  fl_strain_term terms[8];
+  uint64_t term_count;
+  double quadrature_weight; /* kg m^2/s */
+  double prescribed_strain; /* 1/s */
+} fl_strain_row;            /* 168 B */
+/* AssembleFullCellMacStrain3D (native/include/fluidity/host/viscosity.h:77-78; viscosity.cc:762-946):
+ * three normal rows per cell in cell and axis order, then the interior shear rows (cell, +a, +b and the
+ * diagonal cell present) in cell and axis-pair order xy, xz, yz, with harmonic edge viscosity. Cells
+ * strictly (z,y,x) sorted. *io_count holds the capacity of out on entry (6 * n always suffices) and the
+ * row count on return. Statuses and their order as the reference; out is written only on success. */
+int fl_assemble_full_cell_strain(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
+                                 const fl_viscosity_cell* cells, size_t cell_count, fl_strain_row* out,
+                                 size_t* io_count, fl_measurement* measurement);
+/* SolveMacViscousVelocityBlock3D (native/include/fluidity/host/viscosity.h:128-156; viscosity.cc:1331-1650):
+ * (M + dt E^T W E) u = rhs by Jacobi PCG in FP64 on the faces of positive inertia (eliminated faces stay
+ * zero), stopping on the true residual max|rhs - Hu| <= relative_tolerance max|rhs|. InvalidArgument for
+ * malformed inputs or options, OutOfRange for unrepresentable arithmetic, Aborted for a non-positive
+ * curvature or an exhausted budget. Outputs are written only on success. */
+int fl_solve_viscous_block(fl_ctx* ctx, const fl_strain_row* rows, size_t row_count, const double* face_inertias,
+                           size_t face_count, double step_duration, const double* rhs, uint64_t max_iterations,
+                           double relative_tolerance, double* out_velocity, uint64_t* out_iterations,
+                           double* out_max_residual, fl_measurement* measurement);
+
 /* ---- topology ---- */
 /* ComputeParticleSortBin3D for each position plus the stable (z,y,x)-bin / ordinal order of
  * LiquidStepWorkspace3D::PrepareTopology (native/src/sparse_mac_grid_3d.cc:438-449;

@@ -180,6 +180,28 @@ int flo_normalize(const flo_face_quanta* quanta, size_t face_count, flo_quantiza
 int flo_apply_forces(const flo_int3* sorted_blocks, size_t block_count, const double* old_velocities,
                      size_t face_count, flo_vec3 acceleration, double dt, double* out_predicted);
 
+// ---- variational MAC viscosity (native/src/viscosity.cc; host/viscosity.h:47-160) ----
+// MacViscosityCell3D, MacStrainTerm3D and MacStrainRow3D (axes as 0/1/2; prescribed strain carried).
+struct flo_viscosity_cell { flo_int3 cell_index; uint32_t pad; double dynamic_viscosity; };  // 24 B
+struct flo_strain_term { uint64_t face_slot; double coefficient; };                            // 16 B
+struct flo_strain_row {
+  flo_int3 low_cell;
+  uint8_t first_axis, second_axis, pad[2];
+  flo_strain_term terms[8];
+  uint64_t term_count;
+  double quadrature_weight;
+  double prescribed_strain;
+};  // 168 B
+// AssembleFullCellMacStrain3D: three normal rows per cell in cell and axis order, then the interior
+// shear rows in low-cell and axis-pair order (xy, xz, yz). Cells strictly (z,y,x) sorted.
+int flo_assemble_full_cell_strain(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count,
+                                  const flo_viscosity_cell* cells, size_t n, flo_strain_row* out, size_t capacity,
+                                  size_t* out_count);
+// SolveMacViscousVelocityBlock3D: (M + dt E^T W E) u = rhs by Jacobi PCG on positive-inertia faces.
+int flo_solve_viscous_block(const flo_strain_row* rows, size_t row_count, const double* inertia, size_t face_count,
+                            double dt, const double* rhs, uint64_t max_iterations, double relative_tolerance,
+                            double* out_velocity, uint64_t* out_iterations, double* out_max_residual);
+
 // ---- pressure links (liquid_boundary_geometry_3d.cc:1077-1383) ----
 // LiquidCenterDistance3D, FrozenLiquidCell3D, FrozenLiquidFace3D (with its optional wall flux as value +
 // flag, the libstdc++ layout of std::optional<double>), LiquidPressureCellFields3D,

@@ -106,6 +106,13 @@ OBSTACLE = np.dtype([("kind", "u4"), ("pad", "u4"), ("unit_normal", VEC3), ("coo
                      ("wall_velocity", VEC3), ("tank", TANK)])
 UPDATE_OPTIONS = np.dtype([("cell_width", "f8"), ("step_duration", "f8"), ("particle_energy_blend", "f8"),
                            ("friction_coefficient", "f8"), ("penetration_tolerance", "f8"), ("obstacle", OBSTACLE)])
+VISCOSITY_CELL = np.dtype({"names": ["cell_index", "dynamic_viscosity"], "formats": [INT3, "f8"], "offsets": [0, 16],
+                           "itemsize": 24})
+STRAIN_TERM = np.dtype([("face_slot", "u8"), ("coefficient", "f8")])
+STRAIN_ROW = np.dtype({"names": ["low_cell", "first_axis", "second_axis", "terms", "term_count", "quadrature_weight",
+                                 "prescribed_strain"],
+                       "formats": [INT3, "u1", "u1", (STRAIN_TERM, 8), "u8", "f8", "f8"],
+                       "offsets": [0, 12, 13, 16, 144, 152, 160], "itemsize": 168})
 TOPOLOGY_CAPACITIES = np.dtype([("particles", "u8"), ("blocks", "u8"), ("cells", "u8"), ("faces", "u8")])
 assert BLOCK_STATE.itemsize == 32 and CELL_STATE.itemsize == 24 and FACE_STATE.itemsize == 56
 assert PARTICLE.itemsize == 152 and GATHER_RESULT.itemsize == 192 and SORT_ENTRY.itemsize == 24
@@ -627,3 +634,47 @@ def unet_init(seed=0):
     p = np.concatenate(parts)
     assert len(p) == unet_param_count()
     return p
+
+
+def viscosity_cells(indices, viscosities):
+    """Viscosity cell records from (n, 3) indices and n viscosities."""
+    c = np.zeros(len(indices), VISCOSITY_CELL)
+    idx = np.asarray(indices, dtype=np.int32).reshape(-1, 3)
+    c["cell_index"]["x"], c["cell_index"]["y"], c["cell_index"]["z"] = idx[:, 0], idx[:, 1], idx[:, 2]
+    c["dynamic_viscosity"] = viscosities
+    return c
+
+
+def assemble_full_cell_strain(geom, blocks, cells):
+    """AssembleFullCellMacStrain3D (native/src/viscosity.cc:762-946): (status, strain rows)."""
+    out = np.zeros(max(6 * len(cells), 1), STRAIN_ROW)
+    count = C.c_size_t(0)
+    s = lib().flo_assemble_full_cell_strain(_p(geom), _p(blocks), C.c_size_t(len(blocks)), _p(cells),
+                                            C.c_size_t(len(cells)), _p(out), C.c_size_t(len(out)), C.byref(count))
+    return s, out[:count.value] if s == OK else None
+
+
+def solve_viscous_block(rows, inertia, dt, rhs, max_iterations=256, relative_tolerance=1e-10):
+    """SolveMacViscousVelocityBlock3D (native/src/viscosity.cc:1331-1650): (status, velocity, iterations,
+    max residual)."""
+    rows = np.ascontiguousarray(rows, dtype=STRAIN_ROW)
+    inertia = np.ascontiguousarray(inertia, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    v = np.zeros(len(inertia))
+    it = C.c_uint64(0)
+    mr = C.c_double(0.0)
+    s = lib().flo_solve_viscous_block(_p(rows), C.c_size_t(len(rows)), _p(inertia), C.c_size_t(len(inertia)),
+                                      C.c_double(dt), _p(rhs), C.c_uint64(max_iterations),
+                                      C.c_double(relative_tolerance), _p(v), C.byref(it), C.byref(mr))
+    return s, v, it.value, mr.value
+
+
+def strain_row(terms, weight, first_axis=0, second_axis=0):
+    """One strain row from [(slot, coefficient), ...] and its quadrature weight."""
+    r = np.zeros(1, STRAIN_ROW)
+    r["first_axis"], r["second_axis"] = first_axis, second_axis
+    for i, (slot, c) in enumerate(terms):
+        r["terms"][0, i] = (slot, c)
+    r["term_count"] = len(terms)
+    r["quadrature_weight"] = weight
+    return r[0]

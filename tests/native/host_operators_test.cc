@@ -25,6 +25,7 @@
 #include "fluidity/host/mac_pressure_3d.h"
 #include "fluidity/host/sparse_mac_grid_3d.h"
 #include "fluidity/host/transfers_3d.h"
+#include "fluidity/host/viscosity.h"
 #include "oracle.h"
 
 using namespace fluidity;
@@ -1077,6 +1078,62 @@ int Run(const char* name, const std::function<void()>& test) {
 
 }  // namespace
 
+// Strain rows of a box of cells with varying viscosity, bit-exact against the oracle; the velocity block
+// of those rows against the oracle's FP64 Jacobi PCG (same iterations, 1e-10 of the largest velocity);
+// an unsorted cell list is InvalidArgument with no rows.
+void ViscosityMatchesOracle() {
+  const auto keys = DenseBlocks(4, 4, 4);
+  auto grid = SparseMacGrid3D::Create(kGeometry, keys, 64);
+  EXPECT(grid.ok());
+  const auto blocks = grid->active_block_indices();
+  std::mt19937_64 rng(7);
+  std::uniform_real_distribution<double> u(0.5, 2.0);
+  std::vector<MacViscosityCell3D> cells;
+  for (int z = 2; z < 13; ++z)
+    for (int y = 2; y < 11; ++y)
+      for (int x = 3; x < 14; ++x) cells.push_back({{x, y, z}, u(rng)});
+  auto rows = AssembleFullCellMacStrain3D(grid->view(), cells);
+  if (!rows.ok()) std::printf("  assembly status %d\n", static_cast<int>(rows.status().code()));
+  EXPECT(rows.ok());
+  std::vector<flo_strain_row> expected(6 * cells.size());
+  std::size_t count = 0;
+  EXPECT(flo_assemble_full_cell_strain(O(&kGeometry), O(blocks), blocks.size(),
+                                       reinterpret_cast<const flo_viscosity_cell*>(cells.data()), cells.size(),
+                                       expected.data(), expected.size(), &count) == 0);
+  EXPECT(rows->size() == count && count > 3 * cells.size());
+  EXPECT(std::memcmp(rows->data(), expected.data(), count * sizeof(flo_strain_row)) == 0);
+
+  std::vector<double> inertia(grid->face_count(), 0.0), rhs(grid->face_count(), 0.0);
+  for (const auto& row : *rows)
+    for (std::size_t t = 0; t < row.term_count; ++t) {
+      const std::size_t s = row.terms[t].face_slot;
+      inertia[s] = 1000.0 * std::pow(kGeometry.cell_width, 3) * (0.5 + 0.5 * std::sin(0.37 * s) * std::sin(0.37 * s));
+      rhs[s] = inertia[s] * std::cos(0.11 * s);
+    }
+  MacViscousVelocityBlockInputs3D in{*rows, inertia, 0.01};
+  auto solved = SolveMacViscousVelocityBlock3D(in, rhs, {256, 1e-10});
+  EXPECT(solved.ok());
+  std::vector<double> v(inertia.size());
+  std::uint64_t iterations = 0;
+  double max_residual = 0;
+  EXPECT(flo_solve_viscous_block(expected.data(), count, inertia.data(), inertia.size(), 0.01, rhs.data(), 256,
+                                 1e-10, v.data(), &iterations, &max_residual) == 0);
+  double scale = 0, diff = 0;
+  for (std::size_t i = 0; i < v.size(); ++i) {
+    scale = std::max(scale, std::abs(v[i]));
+    diff = std::max(diff, std::abs(v[i] - solved->velocity[i]));
+  }
+  std::printf("  viscous block: %zu rows, %llu iterations (oracle %llu), max |dv| %.3e of %.3e\n", count,
+              static_cast<unsigned long long>(solved->iterations), static_cast<unsigned long long>(iterations), diff,
+              scale);
+  EXPECT(scale > 0 && diff <= 1e-10 * scale);
+  EXPECT(solved->iterations + 1 >= iterations && solved->iterations <= iterations + 1);
+
+  std::vector<MacViscosityCell3D> unsorted = {cells[1], cells[0]};
+  auto bad = AssembleFullCellMacStrain3D(grid->view(), unsorted);
+  EXPECT(bad.status().code() == StatusCode::kInvalidArgument);
+}
+
 int main(int argc, char** argv) {
   const std::string mode = argc > 1 ? argv[1] : "cpu";
   int failed = 0;
@@ -1097,6 +1154,7 @@ int main(int argc, char** argv) {
     failed += Run("BackendUpdateMatchesOracle", BackendUpdateMatchesOracle);
     failed += Run("PressureLinksUnitCube", PressureLinksUnitCube);
     failed += Run("FreezeMatchesReferenceCases", FreezeMatchesReferenceCases);
+    failed += Run("ViscosityMatchesOracle", ViscosityMatchesOracle);
     failed += Run("SharedRuntimeGrows", SharedRuntimeGrows);
   }
   std::printf("%d test(s) failed\n", failed);

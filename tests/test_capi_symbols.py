@@ -33,6 +33,7 @@ def test_record_layouts_match_the_header():
     assert R.WALL_SAMPLE.itemsize == 64 and R.BOUNDARY_GATHER.itemsize == 224
     assert R.BLOCK_STATE.itemsize == 32 and R.CELL_STATE.itemsize == 24 and R.FACE_STATE.itemsize == 56
     assert R.LIQUID_PARTICLE.itemsize == 328 and R.UPDATE_OPTIONS.itemsize == 160
+    assert R.VISCOSITY_CELL.itemsize == 24 and R.STRAIN_ROW.itemsize == 168
 
 
 def test_record_offsets_match_the_c_compiler(tmp_path):
@@ -60,6 +61,8 @@ def test_record_offsets_match_the_c_compiler(tmp_path):
         "fl_center_distance": (R.CENTER_DISTANCE, {n: n for n in R.CENTER_DISTANCE.names}),
         "fl_boundary_gather": (R.BOUNDARY_GATHER, {"affine_enabled": "affine_enabled",
                                                    "low_mass_fallback": "low_mass_fallback"}),
+        "fl_viscosity_cell": (R.VISCOSITY_CELL, {n: n for n in R.VISCOSITY_CELL.names}),
+        "fl_strain_row": (R.STRAIN_ROW, {n: n for n in R.STRAIN_ROW.names}),
     }
     lines = ['#include <stddef.h>', '#include <stdio.h>', f'#include "{ROOT}/include/fluidity_hip.h"', "int main(void) {"]
     for struct, (_, fields) in checks.items():
