@@ -306,14 +306,17 @@ struct FreezeBuffers {
 struct ViscosityBuffers {
   fl_viscosity_cell* cells = nullptr;
   fl_strain_row* rows = nullptr;
-  uint64_t *counts = nullptr, *offsets = nullptr, *inc_offset = nullptr, *entries = nullptr;
+  uint64_t *counts = nullptr, *offsets = nullptr, *entries = nullptr, *row_nnz = nullptr, *row_ptr = nullptr,
+           *inc_ptr = nullptr;
   double *inertia = nullptr, *rhs = nullptr, *v = nullptr, *r = nullptr, *z = nullptr, *d = nullptr, *q = nullptr,
-         *diag = nullptr, *strain = nullptr, *partial = nullptr, *scal = nullptr;
-  uint32_t *flag = nullptr, *inc_count = nullptr, *cursor = nullptr;
-  uint64_t cap[20] = {};
+         *diag = nullptr, *strain = nullptr, *partial = nullptr, *scal = nullptr, *term_coef = nullptr,
+         *inc_coef = nullptr;
+  uint32_t *flag = nullptr, *inc_count = nullptr, *cursor = nullptr, *term_slot = nullptr, *inc_row = nullptr;
+  uint64_t cap[28] = {};
   void release() {
-    void* ptrs[] = {cells, rows, counts, offsets, inc_offset, entries, inertia, rhs, v, r, z, d, q, diag, strain,
-                    partial, scal, flag, inc_count, cursor};
+    void* ptrs[] = {cells, rows,   counts, offsets, entries, row_nnz,   row_ptr,  inc_ptr, inertia,
+                    rhs,   v,      r,      z,       d,       q,         diag,     strain,  partial,
+                    scal,  term_coef, inc_coef, flag, inc_count, cursor, term_slot, inc_row};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     *this = ViscosityBuffers{};
@@ -377,16 +380,28 @@ int grow_viscosity(fl_ctx* ctx, uint64_t n, uint64_t nr, uint64_t nf) {
   FL_TRY(grow(&B->rows, &B->cap[1], nr));
   FL_TRY(grow(&B->counts, &B->cap[2], std::max(n, nf)));
   FL_TRY(grow(&B->offsets, &B->cap[3], n));
-  FL_TRY(grow(&B->inc_offset, &B->cap[4], nf));
-  FL_TRY(grow(&B->entries, &B->cap[5], 8 * nr));
+  FL_TRY(grow(&B->row_nnz, &B->cap[4], nr));
+  FL_TRY(grow(&B->row_ptr, &B->cap[5], nr + 1));
+  FL_TRY(grow(&B->inc_ptr, &B->cap[6], nf + 1));
   double** fv[] = {&B->inertia, &B->rhs, &B->v, &B->r, &B->z, &B->d, &B->q, &B->diag};
-  for (int i = 0; i < 8; ++i) FL_TRY(grow(fv[i], &B->cap[6 + i], nf));
-  FL_TRY(grow(&B->strain, &B->cap[14], nr));
-  FL_TRY(grow(&B->partial, &B->cap[15], 1024));
-  FL_TRY(grow(&B->scal, &B->cap[16], 8));
-  FL_TRY(grow(&B->flag, &B->cap[17], 1));
-  FL_TRY(grow(&B->inc_count, &B->cap[18], nf));
-  FL_TRY(grow(&B->cursor, &B->cap[19], nf));
+  for (int i = 0; i < 8; ++i) FL_TRY(grow(fv[i], &B->cap[7 + i], nf));
+  FL_TRY(grow(&B->strain, &B->cap[15], nr));
+  FL_TRY(grow(&B->partial, &B->cap[16], std::max<uint64_t>(2048, (nf + 255) / 256)));
+  FL_TRY(grow(&B->scal, &B->cap[17], 8));
+  FL_TRY(grow(&B->flag, &B->cap[18], 2));
+  FL_TRY(grow(&B->inc_count, &B->cap[19], nf));
+  FL_TRY(grow(&B->cursor, &B->cap[20], nf));
+  return FL_OK;
+}
+
+// The packed operator's arrays for nt terms (contents not kept).
+int grow_viscosity_terms(fl_ctx* ctx, uint64_t nt) {
+  auto* B = static_cast<ViscosityBuffers*>(ctx->viscosity);
+  FL_TRY(grow(&B->entries, &B->cap[21], nt));
+  FL_TRY(grow(&B->term_slot, &B->cap[22], nt));
+  FL_TRY(grow(&B->term_coef, &B->cap[23], nt));
+  FL_TRY(grow(&B->inc_row, &B->cap[24], nt));
+  FL_TRY(grow(&B->inc_coef, &B->cap[25], nt));
   return FL_OK;
 }
 
@@ -398,9 +413,22 @@ bool scan_fits(fl_ctx* ctx, uint64_t n) {
 
 ViscosityDevice viscosity_device(fl_ctx* ctx) {
   auto& B = *static_cast<ViscosityBuffers*>(ctx->viscosity);
-  return ViscosityDevice{B.cells, B.rows,  B.cap[1], B.counts, B.offsets, B.inertia, B.rhs,       B.v,
-                         B.r,     B.z,     B.d,      B.q,      B.diag,    B.strain,  B.partial,   B.scal,
-                         B.flag,  B.inc_count, B.cursor, B.inc_offset, B.entries};
+  ViscosityDevice d{};
+  d.cells = B.cells;
+  d.rows = B.rows;
+  d.row_capacity = B.cap[1];
+  d.counts = B.counts;
+  d.offsets = B.offsets;
+  d.inertia = B.inertia;
+  d.rhs = B.rhs;
+  d.v = B.v, d.r = B.r, d.z = B.z, d.d = B.d, d.q = B.q, d.diag = B.diag, d.strain = B.strain;
+  d.partial = B.partial, d.partial_capacity = B.cap[16], d.scal = B.scal, d.flag = B.flag;
+  d.inc_count = B.inc_count, d.cursor = B.cursor;
+  d.row_nnz = B.row_nnz, d.row_ptr = B.row_ptr, d.inc_ptr = B.inc_ptr;
+  d.term_capacity = std::min({B.cap[21], B.cap[22], B.cap[23], B.cap[24], B.cap[25]});
+  d.term_slot = B.term_slot, d.inc_row = B.inc_row, d.term_coef = B.term_coef, d.inc_coef = B.inc_coef;
+  d.entries = B.entries;
+  return d;
 }
 
 int first_error_code(fl_ctx* ctx) {
@@ -645,8 +673,10 @@ int fl_reserve(fl_ctx* ctx) {
     FL_TRY(grow(&B.out_faces, &B.cap[10], pf));
     FL_TRY(grow(&B.lower, &B.cap[11], 1));
   }
-  // Viscosity of up to pressure_cells cells (six strain rows each) on the owned face slots.
+  // Viscosity of up to pressure_cells cells (six strain rows each, at most 18 terms per cell) on the
+  // owned face slots.
   FL_TRY(grow_viscosity(ctx, nc, 6 * nc, nf));
+  FL_TRY(grow_viscosity_terms(ctx, 18 * nc));
   FL_HIP(hipStreamSynchronize(ctx->stream));
   return FL_OK;
 }
@@ -1527,8 +1557,10 @@ int fl_solve_viscous_block(fl_ctx* ctx, const fl_strain_row* rows, size_t row_co
     if (inertia[f] == 0.0 && rhs[f] != 0.0) return FL_INVALID_ARGUMENT;
     max_rhs = std::max(max_rhs, std::fabs(rhs[f]));
   }
+  // Packed indices are 32-bit: face slots and row ordinals below 2^32.
   if (face_count > 192 * std::max<uint64_t>(ctx->cap.blocks, 1) ||
-      row_count > 6 * std::max<uint64_t>(ctx->cap.pressure_cells, 1) || !scan_fits(ctx, face_count))
+      row_count > 6 * std::max<uint64_t>(ctx->cap.pressure_cells, 1) || !scan_fits(ctx, face_count) ||
+      !scan_fits(ctx, row_count) || face_count >= (uint64_t(1) << 32) || row_count >= (uint64_t(1) << 32))
     return FL_RESOURCE_EXHAUSTED;
   (void)hipSetDevice(ctx->device);
   FL_TRY(grow_viscosity(ctx, 1, std::max<uint64_t>(row_count, 1), std::max<uint64_t>(face_count, 1)));
@@ -1545,9 +1577,11 @@ int fl_solve_viscous_block(fl_ctx* ctx, const fl_strain_row* rows, size_t row_co
     copy_ms += c.stop();
   }
   Timer t(ctx);
-  uint64_t iterations = 0;
+  uint64_t iterations = 0, terms = 0;
   double max_residual = 0.0;
-  FL_TRY(fl::solve_viscous_block(ctx, viscosity_device(ctx), row_count, face_count, dt, max_rhs, max_iterations,
+  FL_TRY(fl::weight_viscous_rows(ctx, viscosity_device(ctx), row_count, face_count, dt, &terms));
+  FL_TRY(grow_viscosity_terms(ctx, terms));
+  FL_TRY(fl::solve_viscous_block(ctx, viscosity_device(ctx), row_count, face_count, terms, dt, max_rhs, max_iterations,
                                  relative_tolerance, &iterations, &max_residual));
   const double kernel_ms = t.stop();
   Timer c(ctx);

@@ -328,15 +328,30 @@ struct ViscosityDevice {
   const double* inertia;
   const double* rhs;
   double *v, *r, *z, *d, *q, *diag, *strain, *partial, *scal;
-  uint32_t* flag;
+  uint64_t partial_capacity;
+  uint32_t* flag;  // 2 words
   uint32_t *inc_count, *cursor;
-  uint64_t *inc_offset, *entries;
+  uint64_t* row_nnz;  // nr: nonzero weighted terms per row
+  uint64_t* row_ptr;  // nr + 1: first packed term of each row, then the term count
+  uint64_t* inc_ptr;  // nf + 1: first incidence of each face, then the incidence count
+  // Packed operator, term_capacity entries each: the rows' nonzero weighted terms in row and term order
+  // (face slot, coefficient) and each face's incidences in row order (row, coefficient); entries holds
+  // the unsorted (row << 3 | term) keys while the incidence is built.
+  uint64_t term_capacity;
+  uint32_t *term_slot, *inc_row;
+  double *term_coef, *inc_coef;
+  uint64_t* entries;
 };
 // AssembleFullCellMacStrain3D of n cells on the grid of the context's lookup; *total rows written.
 int assemble_strain(fl_ctx* ctx, const ViscosityDevice& v, uint64_t n, double h, uint64_t* total);
-// SolveMacViscousVelocityBlock3D after the host's input validation (max_rhs from it); v.rows are
-// reweighted in place.
-int solve_viscous_block(fl_ctx* ctx, const ViscosityDevice& v, uint64_t nr, uint64_t nf, double dt, double max_rhs,
+// First half of SolveMacViscousVelocityBlock3D after the host's input validation: checks and weights
+// the rows in place (sqrt(dt) sqrt(w) c, zero on eliminated faces) and scans the nonzero terms per row
+// and per face; *terms is the packed operator's size, for which the caller sizes the term arrays.
+int weight_viscous_rows(fl_ctx* ctx, const ViscosityDevice& v, uint64_t nr, uint64_t nf, double dt, uint64_t* terms);
+// Second half: packs the operator, forms the Jacobi diagonal and runs the PCG (max_rhs from the host's
+// validation).
+int solve_viscous_block(fl_ctx* ctx, const ViscosityDevice& v, uint64_t nr, uint64_t nf, uint64_t nt, double dt,
+                        double max_rhs,
                         uint64_t max_iterations, double relative_tolerance, uint64_t* iterations,
                         double* max_residual);
 int read_error(fl_ctx* ctx, uint64_t* word);

@@ -9,10 +9,13 @@
 // and the smallest ordinal is the call's status.
 //
 // Velocity block (SolveMacViscousVelocityBlock3D, :1331-1650): Jacobi PCG on (M + dt E^T W E) u = rhs in
-// FP64. The operator is applied in gather form: a row pass forms every weighted strain, then a face pass
-// adds to M u the strains of the rows that hold the face, in row order, from a face -> (row, term)
-// incidence built on the device. Dots and maxima reduce per workgroup and then in one fixed-order pass,
-// so every run gives the same bits.
+// FP64. Two passes over the caller's rows (staged through LDS) pack the operator: the nonzero weighted
+// terms in row order (face slot, coefficient) and, per face, its incidence in row order (row,
+// coefficient). An apply is then a row pass forming every weighted strain and a face pass adding to M u
+// the strains of the rows that hold the face. The PCG scalars (r.z, curvature, step, beta) and the
+// iteration's checks stay on the device; the host reads them once per iteration for the stopping test.
+// Dots and maxima reduce per workgroup and then in one fixed-order workgroup pass, so every run gives
+// the same bits.
 #include "context.h"
 
 namespace fl {
@@ -222,24 +225,54 @@ __global__ void k_strain_shear(GridView g, const fl_viscosity_cell* cells, uint6
 
 // ---------------------------------------------------------------- velocity block
 
-// BuildWeightedVelocityRows (:1360-1406), per row, ordinals 9r + (0 row, 1 + t term): the row and term
-// checks, then each term's coefficient sqrt(dt) sqrt(w) c (zero on an eliminated face or a zero
-// factor); terms with a nonzero coefficient are counted against their face.
-__global__ void k_weight_rows(fl_strain_row* rows, uint64_t nr, const double* inertia, uint64_t nf, double sqrt_dt,
-                              uint32_t* inc_count, uint64_t* err) {
+// Rows of one workgroup staged through LDS with coalesced 8-byte loads (a row is 21 words), so that the
+// row passes read HBM in full lines instead of one 168-byte stride per lane.
+constexpr int kRowWords = sizeof(fl_strain_row) / sizeof(uint64_t);
+static_assert(sizeof(fl_strain_row) == 168 && kRowWords == 21);
+__device__ inline const fl_strain_row& stage_row(const fl_strain_row* rows, uint64_t nr, fl_strain_row* lds) {
+  const uint64_t base = blockIdx.x * uint64_t(blockDim.x);
+  const uint64_t count = base < nr ? min(uint64_t(blockDim.x), nr - base) : 0;
+  const uint64_t* src = reinterpret_cast<const uint64_t*>(rows + base);
+  uint64_t* dst = reinterpret_cast<uint64_t*>(lds);
+  for (uint64_t i = threadIdx.x; i < count * kRowWords; i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
+  return lds[threadIdx.x];
+}
+
+// Weighted coefficient of term t of a checked row (BuildWeightedVelocityRows, :1360-1406): sqrt(dt)
+// sqrt(w) c, zero on an eliminated face or a zero factor; an unrepresentable product reports OutOfRange
+// at the term and counts as zero.
+__device__ inline double weighted_coefficient(const fl_strain_row& row, int t, double sqrt_w, const double* inertia,
+                                              double sqrt_dt, uint64_t r, uint64_t* err) {
+  const uint64_t slot = row.terms[t].face_slot;
+  const double c = row.terms[t].coefficient;
+  if (inertia[slot] == 0.0 || sqrt_dt == 0.0 || sqrt_w == 0.0 || c == 0.0) return 0.0;
+  const double wc = scaled3(sqrt_dt, sqrt_w, c);
+  if (!normal_d(wc)) {
+    if (err) report_min(err, 9 * r + 1 + t, kOutOfRange);
+    return 0.0;
+  }
+  return wc;
+}
+
+// First row pass, per row, ordinals 9r + (0 row, 1 + t term): the row and term checks, then the nonzero
+// weighted terms counted per row and per face. The rows are not modified.
+__global__ __launch_bounds__(kThreads) void k_count_rows(const fl_strain_row* rows, uint64_t nr, const double* inertia,
+                                                         uint64_t nf, double sqrt_dt, uint32_t* inc_count,
+                                                         uint64_t* row_nnz, uint64_t* err) {
+  __shared__ fl_strain_row lds[kThreads];
+  const fl_strain_row& row = stage_row(rows, nr, lds);
   const uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (r >= nr) return;
-  fl_strain_row& row = rows[r];
+  row_nnz[r] = 0;
   if (row.term_count == 0 || row.term_count > 8 || !finite(row.quadrature_weight) || row.quadrature_weight < 0.0) {
     report_min(err, 9 * r, kInvalidArgument);
     return;
   }
-  const double sqrt_w = sqrt(row.quadrature_weight);
   const int count = static_cast<int>(row.term_count);
   for (int t = 0; t < count; ++t) {
     const uint64_t slot = row.terms[t].face_slot;
-    const double c = row.terms[t].coefficient;
-    if (slot >= nf || !finite(c)) {
+    if (slot >= nf || !finite(row.terms[t].coefficient)) {
       report_min(err, 9 * r + 1 + t, kInvalidArgument);
       return;
     }
@@ -249,47 +282,57 @@ __global__ void k_weight_rows(fl_strain_row* rows, uint64_t nr, const double* in
         return;
       }
   }
-  for (int t = 0; t < count; ++t) {
-    const uint64_t slot = row.terms[t].face_slot;
-    const double c = row.terms[t].coefficient;
-    if (inertia[slot] == 0.0 || sqrt_dt == 0.0 || sqrt_w == 0.0 || c == 0.0) {
-      row.terms[t].coefficient = 0.0;
-      continue;
+  const double sqrt_w = sqrt(row.quadrature_weight);
+  uint64_t nnz = 0;
+  for (int t = 0; t < count; ++t)
+    if (weighted_coefficient(row, t, sqrt_w, inertia, sqrt_dt, r, err) != 0.0) {
+      atomicAdd(&inc_count[row.terms[t].face_slot], 1u);
+      ++nnz;
     }
-    const double wc = scaled3(sqrt_dt, sqrt_w, c);
-    if (!normal_d(wc)) {
-      report_min(err, 9 * r + 1 + t, kOutOfRange);
-      row.terms[t].coefficient = 0.0;
-      continue;
-    }
-    row.terms[t].coefficient = wc;
-    atomicAdd(&inc_count[slot], 1u);
-  }
+  row_nnz[r] = nnz;
 }
 
-__global__ void k_fill_incidence(const fl_strain_row* rows, uint64_t nr, const uint64_t* inc_offset, uint32_t* cursor,
-                                 uint64_t* entries) {
+// ptr[n] = ptr[n - 1] + count[n - 1]: the total after an exclusive scan.
+__global__ void k_scan_tail(const uint64_t* count, uint64_t n, uint64_t* ptr) {
+  if (threadIdx.x == 0) ptr[n] = n ? ptr[n - 1] + count[n - 1] : 0;
+}
+
+// Second row pass (checked rows): the nonzero weighted terms packed in row and term order, and each
+// packed term k of row r entered into its face's incidence as (k << 32 | r), unordered.
+__global__ __launch_bounds__(kThreads) void k_pack_rows(const fl_strain_row* rows, uint64_t nr, const double* inertia,
+                                                        double sqrt_dt, const uint64_t* row_ptr,
+                                                        const uint64_t* inc_ptr, uint32_t* cursor,
+                                                        uint32_t* term_slot, double* term_coef, uint64_t* entries) {
+  __shared__ fl_strain_row lds[kThreads];
+  const fl_strain_row& row = stage_row(rows, nr, lds);
   const uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (r >= nr) return;
-  const fl_strain_row& row = rows[r];
+  const double sqrt_w = sqrt(row.quadrature_weight);
+  uint64_t k = row_ptr[r];
   const int count = static_cast<int>(row.term_count);
   for (int t = 0; t < count && t < 8; ++t) {
-    if (row.terms[t].coefficient == 0.0) continue;
+    const double c = weighted_coefficient(row, t, sqrt_w, inertia, sqrt_dt, r, nullptr);
+    if (c == 0.0) continue;
     const uint64_t slot = row.terms[t].face_slot;
-    const uint32_t k = atomicAdd(&cursor[slot], 1u);
-    entries[inc_offset[slot] + k] = (r << 3) | uint64_t(t);
+    term_slot[k] = static_cast<uint32_t>(slot);
+    term_coef[k] = c;
+    const uint32_t i = atomicAdd(&cursor[slot], 1u);
+    entries[inc_ptr[slot] + i] = (k << 32) | r;
+    ++k;
   }
 }
 
-// Sorts each face's incidence by (row, term) and forms the Jacobi diagonal inertia + sum c^2 in that
-// order; a diagonal that leaves the finite range reports OutOfRange at the term that made it so.
+// Sorts each face's incidence by packed term (row, then term order), stores it as (row, coefficient)
+// and forms the Jacobi diagonal inertia + sum c^2 in that order; a diagonal that leaves the finite range
+// reports OutOfRange at the term that made it so.
 __global__ void k_incidence_diagonal(const fl_strain_row* rows, const double* inertia, uint64_t nf,
-                                     const uint64_t* inc_offset, const uint32_t* inc_count, uint64_t* entries,
-                                     double* diag, uint64_t* err) {
+                                     const uint64_t* inc_ptr, uint64_t* entries, const double* term_coef,
+                                     uint32_t* inc_row, double* inc_coef, double* diag, uint64_t* err) {
   const uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (f >= nf) return;
-  uint64_t* e = entries + inc_offset[f];
-  const uint32_t m = inc_count[f];
+  const uint64_t base = inc_ptr[f];
+  uint64_t* e = entries + base;
+  const uint32_t m = static_cast<uint32_t>(inc_ptr[f + 1] - base);
   for (uint32_t i = 1; i < m; ++i) {
     const uint64_t key = e[i];
     uint32_t j = i;
@@ -301,65 +344,153 @@ __global__ void k_incidence_diagonal(const fl_strain_row* rows, const double* in
   }
   double d = inertia[f];
   for (uint32_t i = 0; i < m; ++i) {
-    const uint64_t r = e[i] >> 3;
-    const int t = static_cast<int>(e[i] & 7);
-    const double c = rows[r].terms[t].coefficient;
+    const uint32_t r = static_cast<uint32_t>(e[i]);
+    const double c = term_coef[e[i] >> 32];
+    inc_row[base + i] = r;
+    inc_coef[base + i] = c;
     d += c * c;
     if (!finite(d)) {
-      report_min(err, 9 * r + 1 + t, kOutOfRange);
+      int t = 0;
+      while (t < 7 && rows[r].terms[t].face_slot != f) ++t;
+      report_min(err, 9 * uint64_t(r) + 1 + t, kOutOfRange);
       break;
     }
   }
   diag[f] = d;
 }
 
-// Weighted strain of every row: sum of c x[slot] in term order.
-__global__ void k_row_strain(const fl_strain_row* rows, uint64_t nr, const double* x, double* strain) {
+// Weighted strain of every row: sum of c x[slot] over its packed terms in term order (a zero term of the
+// row would add an exact zero and is not packed).
+__global__ void k_row_strain(const uint64_t* row_ptr, const uint32_t* term_slot, const double* term_coef, uint64_t nr,
+                             const double* x, double* strain) {
   const uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (r >= nr) return;
-  const fl_strain_row& row = rows[r];
+  const uint64_t end = row_ptr[r + 1];
   double s = 0.0;
-  const int count = static_cast<int>(row.term_count);
-  for (int t = 0; t < count; ++t) s += row.terms[t].coefficient * x[row.terms[t].face_slot];
+  for (uint64_t k = row_ptr[r]; k < end; ++k) s += term_coef[k] * x[term_slot[k]];
   strain[r] = s;
-}
-
-// y = M x + E^T (strain), the strains of a face's rows added in row order; flag 1 if any y is not finite.
-__global__ void k_face_apply(const fl_strain_row* rows, const double* inertia, uint64_t nf, const uint64_t* inc_offset,
-                             const uint32_t* inc_count, const uint64_t* entries, const double* strain, const double* x,
-                             double* y, uint32_t* flag) {
-  const uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  if (f >= nf) return;
-  double m = inertia[f] * x[f];
-  const uint64_t* e = entries + inc_offset[f];
-  for (uint32_t i = 0; i < inc_count[f]; ++i) {
-    const uint64_t r = e[i] >> 3;
-    m += rows[r].terms[e[i] & 7].coefficient * strain[r];
-  }
-  y[f] = m;
-  if (!finite(m)) atomicOr(flag, 1u);
 }
 
 // Workgroup partial sums written to partial[blockIdx] (256 lanes, LDS tree).
 __device__ inline void block_sum(double v, double* partial) {
-  __shared__ double red[256];
+  __shared__ double red[kThreads];
   red[threadIdx.x] = v;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 __device__ inline void block_max(double v, double* partial) {
-  __shared__ double red[256];
+  __shared__ double red[kThreads];
   red[threadIdx.x] = v;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// y = M x + E^T (strain), the strains of a face's rows added in row order; bit 0 of flag[1] if any y is
+// not finite. With DOT, partial[blockIdx] holds the workgroup's sum of x . y (one face per lane).
+template <bool DOT>
+__global__ __launch_bounds__(kThreads) void k_face_apply(const double* inertia, uint64_t nf, const uint64_t* inc_ptr,
+                                                         const uint32_t* inc_row, const double* inc_coef,
+                                                         const double* strain, const double* x, double* y,
+                                                         uint32_t* flag, double* partial) {
+  const uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  double dot = 0.0;
+  if (f < nf) {
+    const double xf = x[f];
+    double m = inertia[f] * xf;
+    const uint64_t end = inc_ptr[f + 1];
+    for (uint64_t i = inc_ptr[f]; i < end; ++i) m += inc_coef[i] * strain[inc_row[i]];
+    y[f] = m;
+    if (!finite(m)) atomicOr(flag + 1, 1u);
+    dot = xf * m;
+  }
+  if (DOT) block_sum(dot, partial);
+}
+
+// Device scalars of the PCG: scal[0] r.z, [1] curvature d.q, [2] max |r|, [3] step, [4] beta. flag[0] is
+// the first failure status (0 none), flag[1] bit 0 a non-finite iterate since the last conversion.
+__device__ inline void fail(uint32_t* flag, uint32_t code) { atomicCAS(flag, 0u, code); }
+
+// Fixed-order reduction of g partials by one workgroup: lane i folds i, i + 256, ..., then an LDS tree.
+__device__ inline double reduce_partials(const double* partial, int g, bool max_mode) {
+  __shared__ double red[kThreads];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < g; i += kThreads) acc = max_mode ? fmax(acc, partial[i]) : acc + partial[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      red[threadIdx.x] = max_mode ? fmax(red[threadIdx.x], red[threadIdx.x + s]) : red[threadIdx.x] + red[threadIdx.x + s];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+// Sum of g partials into scal[slot].
+__global__ __launch_bounds__(kThreads) void k_finish(const double* partial, int g, int max_mode, double* scal,
+                                                     int slot) {
+  const double v = reduce_partials(partial, g, max_mode != 0);
+  if (threadIdx.x == 0) scal[slot] = v;
+}
+
+// After the direction's apply: the curvature, the iteration's checks in the reference's order (r.z
+// positive and finite with a finite apply, else OutOfRange; a positive finite curvature, else Aborted;
+// a normal step, else OutOfRange) and the step rz / curvature.
+__global__ __launch_bounds__(kThreads) void k_finish_curvature(const double* partial, int g, double* scal,
+                                                               uint32_t* flag) {
+  const double curvature = reduce_partials(partial, g, false);
+  if (threadIdx.x != 0) return;
+  scal[1] = curvature;
+  scal[3] = 0.0;
+  const double rz = scal[0];
+  if (!isfinite(rz) || rz <= 0.0 || (flag[1] & 1u)) return fail(flag, kOutOfRange);
+  if (!isfinite(curvature) || curvature <= 0.0) return fail(flag, kAborted);
+  const double step = rz / curvature;
+  if (!normal_d(step)) return fail(flag, kOutOfRange);
+  scal[3] = step;
+}
+
+// v += step d, r -= step q (OutOfRange on a non-finite iterate); then z = r / diag on positive-inertia
+// faces (0 elsewhere). Partials: max |r| in partial[0, g), r . z in partial[g, 2g).
+__global__ __launch_bounds__(kThreads) void k_step_precondition(const double* scal, const double* d, const double* q,
+                                                                const double* inertia, const double* diag,
+                                                                uint64_t nf, double* v, double* r, double* z,
+                                                                double* partial, uint32_t* flag) {
+  const double step = scal[3];
+  double mx = 0.0, rz = 0.0;
+  for (uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; f < nf; f += uint64_t(gridDim.x) * blockDim.x) {
+    const double vf = v[f] + step * d[f];
+    const double rf = r[f] - step * q[f];
+    v[f] = vf;
+    r[f] = rf;
+    if (!finite(vf) || !finite(rf)) atomicOr(flag + 1, 1u);
+    mx = fmax(mx, fabs(rf));
+    const double zf = inertia[f] == 0.0 ? 0.0 : rf / diag[f];
+    z[f] = zf;
+    rz += rf * zf;
+  }
+  block_max(mx, partial);
+  block_sum(rz, partial + gridDim.x);
+}
+
+// After the step: max |r| into scal[2], a non-finite iterate into the status, and the next r.z with
+// beta = next / previous (scal[0] becomes the next r.z).
+__global__ __launch_bounds__(kThreads) void k_finish_step(const double* partial, int g, double* scal, uint32_t* flag) {
+  const double mr = reduce_partials(partial, g, true);
+  __syncthreads();
+  const double rz = reduce_partials(partial + g, g, false);
+  if (threadIdx.x != 0) return;
+  scal[2] = mr;
+  if (flag[1] & 1u) fail(flag, kOutOfRange);
+  scal[4] = rz / scal[0];
+  scal[0] = rz;
 }
 
 // z = r / diag on positive-inertia faces (0 elsewhere); partial r . z.
@@ -374,52 +505,24 @@ __global__ void k_precondition(const double* r, const double* inertia, const dou
   block_sum(acc, partial);
 }
 
-__global__ void k_dot_f64(const double* a, const double* b, uint64_t n, double* partial) {
-  double acc = 0.0;
-  for (uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; f < n; f += uint64_t(gridDim.x) * blockDim.x)
-    acc += a[f] * b[f];
-  block_sum(acc, partial);
-}
-
-// v += step d, r -= step q; partial max |r|; flag 1 on a non-finite iterate.
-__global__ void k_step(double step, const double* d, const double* q, uint64_t nf, double* v, double* r,
-                       double* partial, uint32_t* flag) {
-  double mx = 0.0;
-  for (uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; f < nf; f += uint64_t(gridDim.x) * blockDim.x) {
-    const double vf = v[f] + step * d[f];
-    const double rf = r[f] - step * q[f];
-    v[f] = vf;
-    r[f] = rf;
-    if (!finite(vf) || !finite(rf)) atomicOr(flag, 1u);
-    mx = fmax(mx, fabs(rf));
-  }
-  block_max(mx, partial);
-}
-
-// r = rhs - q; partial max |r|; flag 1 on a non-finite residual.
+// r = rhs - q; partial max |r|; bit 0 of flag[1] on a non-finite residual.
 __global__ void k_true_residual_f64(const double* rhs, const double* q, uint64_t nf, double* r, double* partial,
                                     uint32_t* flag) {
   double mx = 0.0;
   for (uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; f < nf; f += uint64_t(gridDim.x) * blockDim.x) {
     const double rf = rhs[f] - q[f];
     r[f] = rf;
-    if (!finite(rf)) atomicOr(flag, 1u);
+    if (!finite(rf)) atomicOr(flag + 1, 1u);
     mx = fmax(mx, fabs(rf));
   }
   block_max(mx, partial);
 }
 
-__global__ void k_direction_f64(const double* z, double beta, uint64_t nf, double* d) {
+// d = z + beta d, beta = scal[4] (0 after a residual recomputation).
+__global__ void k_direction_f64(const double* z, const double* scal, int use_beta, uint64_t nf, double* d) {
   const uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  const double beta = use_beta ? scal[4] : 0.0;
   if (f < nf) d[f] = z[f] + beta * d[f];
-}
-
-// Fixed-order finish of a reduction: partial[0..g) summed (or maximised) by one lane into out.
-__global__ void k_finish(const double* partial, int g, int max_mode, double* out) {
-  if (threadIdx.x != 0) return;
-  double acc = 0.0;
-  for (int i = 0; i < g; ++i) acc = max_mode ? fmax(acc, partial[i]) : acc + partial[i];
-  *out = acc;
 }
 
 __global__ void k_copy_d(const double* a, uint64_t n, double* b) {
@@ -475,26 +578,44 @@ int assemble_strain(fl_ctx* ctx, const ViscosityDevice& V, uint64_t n, double h,
   return FL_OK;
 }
 
-int solve_viscous_block(fl_ctx* ctx, const ViscosityDevice& V, uint64_t nr, uint64_t nf, double dt, double max_rhs,
-                        uint64_t max_iterations, double relative_tolerance, uint64_t* iterations,
-                        double* max_residual) {
+int weight_viscous_rows(fl_ctx* ctx, const ViscosityDevice& V, uint64_t nr, uint64_t nf, double dt, uint64_t* terms) {
   hipStream_t s = ctx->stream;
-  *iterations = 0;
-  *max_residual = 0.0;
+  *terms = 0;
   FL_TRY(reset_error(ctx));
   k_fill_u32<<<grid_for(nf), kThreads, 0, s>>>(V.inc_count, nf);
-  k_fill_u32<<<grid_for(nf), kThreads, 0, s>>>(V.cursor, nf);
-  if (nr) k_weight_rows<<<grid_for(nr), kThreads, 0, s>>>(V.rows, nr, V.inertia, nf, std::sqrt(dt), V.inc_count, ctx->err);
+  if (nr) k_count_rows<<<grid_for(nr), kThreads, 0, s>>>(V.rows, nr, V.inertia, nf, std::sqrt(dt), V.inc_count,
+                                                         V.row_nnz, ctx->err);
   FL_HIP(hipGetLastError());
   uint64_t word = 0;
   FL_TRY(read_error(ctx, &word));
   if (word != ~uint64_t(0)) return static_cast<int>(word & 0xff);
+  FL_TRY(exclusive_scan_u64(ctx, V.row_nnz, V.row_ptr, nr));
+  k_scan_tail<<<1, 64, 0, s>>>(V.row_nnz, nr, V.row_ptr);
+  k_widen<<<grid_for(nf), kThreads, 0, s>>>(V.inc_count, nf, V.counts);
+  FL_TRY(exclusive_scan_u64(ctx, V.counts, V.inc_ptr, nf));
+  k_scan_tail<<<1, 64, 0, s>>>(V.counts, nf, V.inc_ptr);
+  FL_HIP(hipGetLastError());
+  FL_HIP(hipMemcpyAsync(terms, V.row_ptr + nr, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  FL_HIP(hipStreamSynchronize(s));
+  return FL_OK;
+}
+
+int solve_viscous_block(fl_ctx* ctx, const ViscosityDevice& V, uint64_t nr, uint64_t nf, uint64_t nt, double dt,
+                        double max_rhs, uint64_t max_iterations, double relative_tolerance, uint64_t* iterations,
+                        double* max_residual) {
+  hipStream_t s = ctx->stream;
+  *iterations = 0;
+  *max_residual = 0.0;
+  // Packed term ordinals and row ordinals share one 64-bit incidence key.
+  if (nt > V.term_capacity || nt >= (uint64_t(1) << 32)) return FL_RESOURCE_EXHAUSTED;
+  uint64_t word = 0;
   if (nf) {
-    k_widen<<<grid_for(nf), kThreads, 0, s>>>(V.inc_count, nf, V.counts);
-    FL_TRY(exclusive_scan_u64(ctx, V.counts, V.inc_offset, nf));
-    if (nr) k_fill_incidence<<<grid_for(nr), kThreads, 0, s>>>(V.rows, nr, V.inc_offset, V.cursor, V.entries);
-    k_incidence_diagonal<<<grid_for(nf), kThreads, 0, s>>>(V.rows, V.inertia, nf, V.inc_offset, V.inc_count, V.entries,
-                                                           V.diag, ctx->err);
+    k_fill_u32<<<grid_for(nf), kThreads, 0, s>>>(V.cursor, nf);
+    if (nr)
+      k_pack_rows<<<grid_for(nr), kThreads, 0, s>>>(V.rows, nr, V.inertia, std::sqrt(dt), V.row_ptr, V.inc_ptr,
+                                                    V.cursor, V.term_slot, V.term_coef, V.entries);
+    k_incidence_diagonal<<<grid_for(nf), kThreads, 0, s>>>(V.rows, V.inertia, nf, V.inc_ptr, V.entries, V.term_coef,
+                                                           V.inc_row, V.inc_coef, V.diag, ctx->err);
     FL_HIP(hipGetLastError());
     FL_TRY(read_error(ctx, &word));
     if (word != ~uint64_t(0)) return static_cast<int>(word & 0xff);
@@ -503,71 +624,62 @@ int solve_viscous_block(fl_ctx* ctx, const ViscosityDevice& V, uint64_t nr, uint
   if (max_rhs == 0.0) return FL_OK;
   const double tolerance = relative_tolerance * max_rhs;
   const unsigned rg = static_cast<unsigned>(std::min<uint64_t>(kReduceGroups, (nf + kThreads - 1) / kThreads));
-  double* scal = V.scal;  // [0] r.z, [1] curvature, [2] max |r|
-  auto read = [&](int i, double* out) -> int {
-    FL_HIP(hipMemcpyAsync(out, scal + i, sizeof(double), hipMemcpyDeviceToHost, s));
+  const int fg = static_cast<int>(grid_for(nf));  // apply partials, one per face workgroup
+  if (static_cast<uint64_t>(fg) > V.partial_capacity || 2 * uint64_t(rg) > V.partial_capacity)
+    return FL_RESOURCE_EXHAUSTED;
+  double* scal = V.scal;
+  uint32_t status[2] = {0, 0};
+  double host[5] = {};
+  // Reads the status words and the scalars; one synchronization.
+  auto sync = [&]() -> int {
+    FL_HIP(hipMemcpyAsync(status, V.flag, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    FL_HIP(hipMemcpyAsync(host, scal, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+    FL_HIP(hipStreamSynchronize(s));
     return FL_OK;
   };
-  uint32_t flag = 0;
-  auto apply = [&](const double* x, double* y) -> int {
-    if (nr) k_row_strain<<<grid_for(nr), kThreads, 0, s>>>(V.rows, nr, x, V.strain);
-    k_face_apply<<<grid_for(nf), kThreads, 0, s>>>(V.rows, V.inertia, nf, V.inc_offset, V.inc_count, V.entries,
-                                                   V.strain, x, y, V.flag);
-    FL_HIP(hipGetLastError());
-    return FL_OK;
+  auto apply = [&](const double* x, double* y, bool dot) {
+    if (nr) k_row_strain<<<grid_for(nr), kThreads, 0, s>>>(V.row_ptr, V.term_slot, V.term_coef, nr, x, V.strain);
+    if (dot)
+      k_face_apply<true><<<fg, kThreads, 0, s>>>(V.inertia, nf, V.inc_ptr, V.inc_row, V.inc_coef, V.strain, x, y,
+                                                 V.flag, V.partial);
+    else
+      k_face_apply<false><<<fg, kThreads, 0, s>>>(V.inertia, nf, V.inc_ptr, V.inc_row, V.inc_coef, V.strain, x, y,
+                                                  V.flag, V.partial);
   };
+  k_fill_u32<<<1, 64, 0, s>>>(V.flag, 2);
   k_copy_d<<<grid_for(nf), kThreads, 0, s>>>(V.rhs, nf, V.r);
   k_precondition<<<rg, kThreads, 0, s>>>(V.r, V.inertia, V.diag, nf, V.z, V.partial);
-  k_finish<<<1, 64, 0, s>>>(V.partial, static_cast<int>(rg), 0, scal + 0);
+  k_finish<<<1, kThreads, 0, s>>>(V.partial, static_cast<int>(rg), 0, scal, 0);
   k_copy_d<<<grid_for(nf), kThreads, 0, s>>>(V.z, nf, V.d);
   FL_HIP(hipGetLastError());
-  double rz = 0;
-  FL_TRY(read(0, &rz));
-  FL_HIP(hipStreamSynchronize(s));
   while (*iterations < max_iterations) {
-    k_fill_u32<<<1, 64, 0, s>>>(V.flag, 1);
-    FL_TRY(apply(V.d, V.q));
-    k_dot_f64<<<rg, kThreads, 0, s>>>(V.d, V.q, nf, V.partial);
-    k_finish<<<1, 64, 0, s>>>(V.partial, static_cast<int>(rg), 0, scal + 1);
-    double curvature = 0;
-    FL_TRY(read(1, &curvature));
-    FL_HIP(hipMemcpyAsync(&flag, V.flag, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    FL_HIP(hipStreamSynchronize(s));
-    if (!std::isfinite(rz) || rz <= 0.0 || flag) return FL_OUT_OF_RANGE;
-    if (!std::isfinite(curvature) || curvature <= 0.0) return FL_ABORTED;
-    const double step = rz / curvature;
-    if (!std::isnormal(step)) return FL_OUT_OF_RANGE;
-    k_step<<<rg, kThreads, 0, s>>>(step, V.d, V.q, nf, V.v, V.r, V.partial, V.flag);
-    k_finish<<<1, 64, 0, s>>>(V.partial, static_cast<int>(rg), 1, scal + 2);
-    double mr = 0;
-    FL_TRY(read(2, &mr));
-    FL_HIP(hipMemcpyAsync(&flag, V.flag, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    FL_HIP(hipStreamSynchronize(s));
-    if (flag) return FL_OUT_OF_RANGE;
+    apply(V.d, V.q, true);
+    k_finish_curvature<<<1, kThreads, 0, s>>>(V.partial, fg, scal, V.flag);
+    k_step_precondition<<<rg, kThreads, 0, s>>>(scal, V.d, V.q, V.inertia, V.diag, nf, V.v, V.r, V.z, V.partial,
+                                                V.flag);
+    k_finish_step<<<1, kThreads, 0, s>>>(V.partial, static_cast<int>(rg), scal, V.flag);
+    FL_HIP(hipGetLastError());
+    FL_TRY(sync());
+    if (status[0]) return static_cast<int>(status[0]);
     ++*iterations;
+    const double mr = host[2];
     const bool check = mr <= tolerance || *iterations == max_iterations;
     if (check) {
-      FL_TRY(apply(V.v, V.q));
+      apply(V.v, V.q, false);
       k_true_residual_f64<<<rg, kThreads, 0, s>>>(V.rhs, V.q, nf, V.r, V.partial, V.flag);
-      k_finish<<<1, 64, 0, s>>>(V.partial, static_cast<int>(rg), 1, scal + 2);
-      FL_TRY(read(2, &mr));
-      FL_HIP(hipMemcpyAsync(&flag, V.flag, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-      FL_HIP(hipStreamSynchronize(s));
-      if (flag) return FL_OUT_OF_RANGE;
-      if (mr <= tolerance) {
-        *max_residual = mr;
+      k_finish<<<1, kThreads, 0, s>>>(V.partial, static_cast<int>(rg), 1, scal, 2);
+      FL_HIP(hipGetLastError());
+      FL_TRY(sync());
+      if (status[1] & 1u) return FL_OUT_OF_RANGE;
+      if (host[2] <= tolerance) {
+        *max_residual = host[2];
         return FL_OK;
       }
+      k_precondition<<<rg, kThreads, 0, s>>>(V.r, V.inertia, V.diag, nf, V.z, V.partial);
+      k_finish<<<1, kThreads, 0, s>>>(V.partial, static_cast<int>(rg), 0, scal, 0);
     }
-    k_precondition<<<rg, kThreads, 0, s>>>(V.r, V.inertia, V.diag, nf, V.z, V.partial);
-    k_finish<<<1, 64, 0, s>>>(V.partial, static_cast<int>(rg), 0, scal + 0);
-    double next_rz = 0;
-    FL_TRY(read(0, &next_rz));
-    FL_HIP(hipStreamSynchronize(s));
-    const double beta = check ? 0.0 : next_rz / rz;
-    k_direction_f64<<<grid_for(nf), kThreads, 0, s>>>(V.z, beta, nf, V.d);
+    k_direction_f64<<<grid_for(nf), kThreads, 0, s>>>(V.z, scal, check ? 0 : 1, nf, V.d);
     FL_HIP(hipGetLastError());
-    rz = next_rz;
   }
   return FL_ABORTED;
 }
