@@ -386,7 +386,7 @@ int grow_viscosity(fl_ctx* ctx, uint64_t n, uint64_t nr, uint64_t nf) {
   double** fv[] = {&B->inertia, &B->rhs, &B->v, &B->r, &B->z, &B->d, &B->q, &B->diag};
   for (int i = 0; i < 8; ++i) FL_TRY(grow(fv[i], &B->cap[7 + i], nf));
   FL_TRY(grow(&B->strain, &B->cap[15], nr));
-  FL_TRY(grow(&B->partial, &B->cap[16], std::max<uint64_t>(2048, (nf + 255) / 256)));
+  FL_TRY(grow(&B->partial, &B->cap[16], 2048));  // two partial sets of up to 1024 workgroups
   FL_TRY(grow(&B->scal, &B->cap[17], 8));
   FL_TRY(grow(&B->flag, &B->cap[18], 2));
   FL_TRY(grow(&B->inc_count, &B->cap[19], nf));

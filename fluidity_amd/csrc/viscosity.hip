@@ -394,22 +394,21 @@ __device__ inline void block_max(double v, double* partial) {
 }
 
 // y = M x + E^T (strain), the strains of a face's rows added in row order; bit 0 of flag[1] if any y is
-// not finite. With DOT, partial[blockIdx] holds the workgroup's sum of x . y (one face per lane).
+// not finite. With DOT, partial[blockIdx] holds the workgroup's sum of x . y (grid-stride over faces).
 template <bool DOT>
 __global__ __launch_bounds__(kThreads) void k_face_apply(const double* inertia, uint64_t nf, const uint64_t* inc_ptr,
                                                          const uint32_t* inc_row, const double* inc_coef,
                                                          const double* strain, const double* x, double* y,
                                                          uint32_t* flag, double* partial) {
-  const uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   double dot = 0.0;
-  if (f < nf) {
+  for (uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; f < nf; f += uint64_t(gridDim.x) * blockDim.x) {
     const double xf = x[f];
     double m = inertia[f] * xf;
     const uint64_t end = inc_ptr[f + 1];
     for (uint64_t i = inc_ptr[f]; i < end; ++i) m += inc_coef[i] * strain[inc_row[i]];
     y[f] = m;
     if (!finite(m)) atomicOr(flag + 1, 1u);
-    dot = xf * m;
+    dot += xf * m;
   }
   if (DOT) block_sum(dot, partial);
 }
@@ -624,9 +623,8 @@ int solve_viscous_block(fl_ctx* ctx, const ViscosityDevice& V, uint64_t nr, uint
   if (max_rhs == 0.0) return FL_OK;
   const double tolerance = relative_tolerance * max_rhs;
   const unsigned rg = static_cast<unsigned>(std::min<uint64_t>(kReduceGroups, (nf + kThreads - 1) / kThreads));
-  const int fg = static_cast<int>(grid_for(nf));  // apply partials, one per face workgroup
-  if (static_cast<uint64_t>(fg) > V.partial_capacity || 2 * uint64_t(rg) > V.partial_capacity)
-    return FL_RESOURCE_EXHAUSTED;
+  const unsigned fg = grid_for(nf);
+  if (2 * uint64_t(rg) > V.partial_capacity) return FL_RESOURCE_EXHAUSTED;
   double* scal = V.scal;
   uint32_t status[2] = {0, 0};
   double host[5] = {};
@@ -640,7 +638,7 @@ int solve_viscous_block(fl_ctx* ctx, const ViscosityDevice& V, uint64_t nr, uint
   auto apply = [&](const double* x, double* y, bool dot) {
     if (nr) k_row_strain<<<grid_for(nr), kThreads, 0, s>>>(V.row_ptr, V.term_slot, V.term_coef, nr, x, V.strain);
     if (dot)
-      k_face_apply<true><<<fg, kThreads, 0, s>>>(V.inertia, nf, V.inc_ptr, V.inc_row, V.inc_coef, V.strain, x, y,
+      k_face_apply<true><<<rg, kThreads, 0, s>>>(V.inertia, nf, V.inc_ptr, V.inc_row, V.inc_coef, V.strain, x, y,
                                                  V.flag, V.partial);
     else
       k_face_apply<false><<<fg, kThreads, 0, s>>>(V.inertia, nf, V.inc_ptr, V.inc_row, V.inc_coef, V.strain, x, y,
@@ -654,7 +652,7 @@ int solve_viscous_block(fl_ctx* ctx, const ViscosityDevice& V, uint64_t nr, uint
   FL_HIP(hipGetLastError());
   while (*iterations < max_iterations) {
     apply(V.d, V.q, true);
-    k_finish_curvature<<<1, kThreads, 0, s>>>(V.partial, fg, scal, V.flag);
+    k_finish_curvature<<<1, kThreads, 0, s>>>(V.partial, static_cast<int>(rg), scal, V.flag);
     k_step_precondition<<<rg, kThreads, 0, s>>>(scal, V.d, V.q, V.inertia, V.diag, nf, V.v, V.r, V.z, V.partial,
                                                 V.flag);
     k_finish_step<<<1, kThreads, 0, s>>>(V.partial, static_cast<int>(rg), scal, V.flag);
