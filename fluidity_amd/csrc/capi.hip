@@ -292,10 +292,13 @@ struct FreezeBuffers {
   double* phi = nullptr;
   uint64_t* work = nullptr;
   double* uncut = nullptr;
-  uint64_t cap[17] = {};
+  fl_solid_corner* solids = nullptr;
+  double* face_velocities = nullptr;
+  double* solid_phi = nullptr;
+  uint64_t cap[20] = {};
   void release() {
-    void* ptrs[] = {blocks, candidates, centers, corners, keep, staged, slots, words, word_counts, face_slots,
-                    cells, faces, needs, counts, phi, work, uncut};
+    void* ptrs[] = {blocks, candidates, centers, corners, keep,  staged, slots, words,  word_counts,     face_slots,
+                    cells,  faces,      needs,   counts,  phi,   work,   uncut, solids, face_velocities, solid_phi};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     *this = FreezeBuffers{};
@@ -653,6 +656,10 @@ int fl_reserve(fl_ctx* ctx) {
     FL_TRY(grow(&B.phi, &B.cap[14], 9 * nc));
     FL_TRY(grow(&B.work, &B.cap[15], nc));
     FL_TRY(grow(&B.uncut, &B.cap[16], 1));
+    // A sampled solid: 8 corner samples per candidate, one face velocity per owned slot.
+    FL_TRY(grow(&B.solids, &B.cap[17], 8 * nc));
+    FL_TRY(grow(&B.face_velocities, &B.cap[18], 192 * nb));
+    FL_TRY(grow(&B.solid_phi, &B.cap[19], 8 * nc));
   }
   // Pressure links of a frozen view: blocks, 27 centre samples per cell, cells and faces, the fields.
   if (!ctx->links) ctx->links = new (std::nothrow) LinkBuffers();
@@ -1094,6 +1101,18 @@ int fl_freeze_geometry(fl_ctx* ctx, const fl_freeze_inputs* inputs, fl_frozen_ce
   const double h = I.geometry.cell_width, full_area = h * h, full_volume = full_area * h;
   if (!std::isnormal(full_area) || !std::isnormal(full_volume) || !std::isnormal(1 / I.liquid_density))
     return FL_OUT_OF_RANGE;
+  // The sampled solid's checks (:803-818): a half-space or no obstacle, dense face velocities of the
+  // grid's size and finite. The reference checks the centre and corner samples first; a failure of
+  // either is InvalidArgument too, so the order does not change the status.
+  const bool sampled = I.has_sampled_solid != 0;
+  if (sampled) {
+    if ((I.solid_corner_count && !I.solid_corners) || (I.face_velocity_count && !I.face_normal_velocities))
+      return FL_INVALID_ARGUMENT;
+    bool bad = (o.kind != FL_OBSTACLE_HALFSPACE && o.kind != FL_OBSTACLE_ABSENT) ||
+               (I.face_velocity_count != 0 && I.face_velocity_count != 192 * I.block_count);
+    for (uint64_t f = 0; !bad && f < I.face_velocity_count; ++f) bad = !std::isfinite(I.face_normal_velocities[f]);
+    if (bad) return FL_INVALID_ARGUMENT;
+  }
   // The device scans (candidates and bitmap words) share the context's scan partials.
   const uint64_t nwords = 3 * I.block_count;
   const uint64_t partials = std::max<uint64_t>(std::max<uint64_t>(64, ctx->cap.lookup_blocks) / 128 + 4096, 8192);
@@ -1120,6 +1139,11 @@ int fl_freeze_geometry(fl_ctx* ctx, const fl_freeze_inputs* inputs, fl_frozen_ce
   FL_TRY(grow(&B.phi, &B.cap[14], 9 * nc_in));
   FL_TRY(grow(&B.work, &B.cap[15], nc_in));
   FL_TRY(grow(&B.uncut, &B.cap[16], 1));
+  if (sampled) {
+    FL_TRY(grow(&B.solids, &B.cap[17], I.solid_corner_count));
+    FL_TRY(grow(&B.face_velocities, &B.cap[18], I.face_velocity_count));
+    FL_TRY(grow(&B.solid_phi, &B.cap[19], 8 * nc_in));
+  }
   hipStream_t s = ctx->stream;
   double copy_ms = 0;
   const auto up = [&](void* dst, const void* src, size_t bytes) -> int {
@@ -1132,13 +1156,30 @@ int fl_freeze_geometry(fl_ctx* ctx, const fl_freeze_inputs* inputs, fl_frozen_ce
     FL_TRY(up(B.candidates, I.cell_indices, nc_in * sizeof(fl_int3)));
     FL_TRY(up(B.centers, I.center_distances, I.center_count * sizeof(fl_center_distance)));
     FL_TRY(up(B.corners, I.corner_distances, I.corner_count * sizeof(fl_center_distance)));
+    if (sampled) {
+      FL_TRY(up(B.solids, I.solid_corners, I.solid_corner_count * sizeof(fl_solid_corner)));
+      FL_TRY(up(B.face_velocities, I.face_normal_velocities, I.face_velocity_count * sizeof(double)));
+    }
     copy_ms += c.stop();
   }
   Timer t(ctx);
-  const FreezeDevice d{B.blocks, I.block_count, I.geometry.world_origin, h, full_volume, B.candidates, nc_in,
-                       B.centers, I.center_count, B.corners, I.corner_count, o, B.keep, B.keep + nc_in, B.staged,
-                       B.slots, B.words, nwords, B.word_counts, B.word_counts + nwords, B.face_slots, B.cells,
-                       B.faces, B.needs, B.counts, B.phi, B.work, B.uncut};
+  const FreezeDevice d{B.blocks,     I.block_count,
+                       I.geometry.world_origin, h,
+                       full_volume,  B.candidates,
+                       nc_in,        B.centers,
+                       I.center_count, B.corners,
+                       I.corner_count, o,
+                       B.keep,       B.keep + nc_in,
+                       B.staged,     B.slots,
+                       B.words,      nwords,
+                       B.word_counts, B.word_counts + nwords,
+                       B.face_slots, B.cells,
+                       B.faces,      B.needs,
+                       B.counts,     B.phi,
+                       B.work,       B.uncut,
+                       sampled ? 1 : 0, sampled ? B.solids : nullptr,
+                       sampled ? I.solid_corner_count : 0, sampled ? B.face_velocities : nullptr,
+                       sampled ? I.face_velocity_count : 0, sampled ? B.solid_phi : nullptr};
   const auto read_counts = [&]() -> int {
     FL_HIP(hipMemcpyAsync(ctx->host_words, B.counts, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     FL_HIP(hipStreamSynchronize(s));

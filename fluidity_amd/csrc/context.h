@@ -301,6 +301,14 @@ struct FreezeDevice {
   double* phi;            // 9 * ncand: corner and centre samples of the cut cells
   uint64_t* work;         // ncand: the cut cells
   double* uncut_volume;   // 1
+  // Sampled solid (sampled != 0): corner samples, dense face normal velocities (count 0: none), and the
+  // eight solid corner distances of each listed cell.
+  int sampled;
+  const fl_solid_corner* solids;
+  uint64_t nsolids;
+  const double* face_velocities;
+  uint64_t nface_velocities;
+  double* solid_phi;  // 8 * ncand
 };
 // Phases 0-2 (sample and candidate checks, kCell, compaction, unique face slots).
 int freeze_cells_device(fl_ctx* ctx, const FreezeDevice& d);

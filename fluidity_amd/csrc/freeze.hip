@@ -5,9 +5,15 @@
 // curvature of cells on an open liquid-air face (base/liquid_boundary_geometry_math_3d.h:160-417,
 // 625-848).
 //
+// With a sampled solid (base/sampled_solid_liquid_geometry_3d.h) the cell query measures the liquid outside
+// the piecewise-linear solid of the corner samples (each tetrahedron clipped by the liquid plane and by
+// the affine extension of its solid samples, optionally by an analytic half-space), checks that the wet
+// pieces are connected, and the face query measures the open area and integrates the corner wall
+// velocities over the blocked part of the face.
+//
 // Each sequential loop of the reference is one kernel over its records; a failure writes
-// min(phase, ordinal, code) into the context's error word. Phases: 0 centre samples, 1 corner samples,
-// 2 candidate cells, 3 faces, 4 curvature needs, 5 curvature. Candidate cells that no plane cuts are
+// min(phase, ordinal, code) into the context's error word. Phases: 0 centre samples, 1 corner samples
+// (then the solid samples), 2 candidate cells, 3 faces, 4 curvature needs, 5 curvature. Candidate cells that no plane cuts are
 // settled by a light staging kernel; only cut cells are clipped. Kept cells are compacted in candidate
 // order by a scan; their faces are made unique and sorted by slot through a bitmap over the grid's face
 // slots.
@@ -31,6 +37,8 @@ __device__ inline bool normal_or_zero(double v) { return v == 0.0 || (finite(v) 
 // std::max / std::min of two values: the first unless the second compares greater (smaller).
 __device__ inline double max_std(double a, double b) { return a < b ? b : a; }
 __device__ inline double min_std(double a, double b) { return b < a ? b : a; }
+// std::clamp: lo if v < lo, else hi if hi < v, else v.
+__device__ inline double clamp_std(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
 
 struct V3 {
   double c[3];
@@ -297,6 +305,212 @@ __device__ inline int local_planes(const FreezeDevice& d, fl_int3 c, Planes& out
 __constant__ int kTetrahedra[6][4] = {{0, 1, 3, 7}, {0, 1, 5, 7}, {0, 2, 3, 7},
                                       {0, 2, 6, 7}, {0, 4, 5, 7}, {0, 4, 6, 7}};
 
+// ---- sampled solids ----
+
+constexpr double kMeasureSlack = 64 * 2.220446049250313e-16;
+
+// SampledSolidExteriorPlane: the affine extension of the solid samples along a chain of corners whose
+// consecutive bits differ in one axis; its distance is minus the interpolated solid distance.
+template <int N>
+__device__ inline Plane sampled_exterior(const double (&s)[8], const int (&corners)[N]) {
+  Plane p{{{0, 0, 0}}, s[corners[0]], false};
+  for (int e = 1; e < N; ++e) {
+    const int step = corners[e] ^ corners[e - 1];
+    const int axis = (step & 1) != 0 ? 0 : ((step & 2) != 0 ? 1 : 2);
+    p.normal.c[axis] = s[corners[e - 1]] - s[corners[e]];
+  }
+  return p;
+}
+
+// ClipSampledAvailableFace: the part of a polygon on the open side of the analytic boundary.
+__device__ inline void available_face(const Polygon& g, const Plane* analytic, Polygon& out) {
+  if (analytic == nullptr || g.overflow) {
+    out = g;
+    return;
+  }
+  bool any = false;
+  for (int j = 0; j < g.n; ++j) any = any || distance(g.v[j], *analytic) < 0;
+  if (!any) {
+    out.clear();
+    return;
+  }
+  clip<kCross>(g, *analytic, out, nullptr);
+}
+
+__device__ inline void triangle(const Vtx& a, const Vtx& b, const Vtx& c, Polygon& g) {
+  g.clear();
+  g.push(a);
+  g.push(b);
+  g.push(c);
+}
+
+// CheckSampledWetConnectivity: wet tetrahedra joined through positive wet openings of the triangles they
+// share around the body diagonal must form one piece.
+__device__ int wet_connectivity(const Vtx (&x)[8], const double (&s)[8], const bool (&wet)[6], double width,
+                                const Plane* analytic) {
+  const auto wet_corner = [&](int k) {
+    return x[k].phi < 0 && s[k] > 0 && (analytic == nullptr || distance(x[k], *analytic) < 0);
+  };
+  if (wet_corner(0) || wet_corner(7)) return kOk;
+  int components = 0;
+  for (int t = 0; t < 6; ++t) components += wet[t] ? 1 : 0;
+  if (components <= 1) return kOk;
+  const int ring[6] = {0, 1, 4, 5, 3, 2};
+  const int shared[6] = {1, 5, 4, 6, 2, 3};
+  int group[6] = {0, 1, 2, 3, 4, 5};
+  for (int e = 0; e < 6; ++e) {
+    const int first = ring[e], second = ring[(e + 1) % 6];
+    if (!wet[first] || !wet[second] || group[first] == group[second]) continue;
+    const int middle = shared[e];
+    if (!(x[0].phi < 0 || x[middle].phi < 0 || x[7].phi < 0)) continue;
+    const int corners[3] = {0, middle, 7};
+    if (s[0] <= 0 && s[middle] <= 0 && s[7] <= 0) continue;
+    Polygon tri, liquid, avail;
+    triangle(x[0], x[middle], x[7], tri);
+    clip<kCross>(tri, Plane{{{0, 0, 0}}, 0, true}, liquid, nullptr);
+    available_face(liquid, analytic, avail);
+    const double wet_area = open_area(avail, sampled_exterior(s, corners), width);
+    if (!normal_or_zero(wet_area)) return kOutOfRange;
+    if (wet_area > 0) {
+      const int old_group = group[second], new_group = group[first];
+      for (int k = 0; k < 6; ++k)
+        if (group[k] == old_group) group[k] = new_group;
+      --components;
+    }
+  }
+  return components == 1 ? kOk : kFailedPrecondition;
+}
+
+// MeasureSampledSolidLiquidCell: wet volume and the six open face areas (low/high x, y, z).
+__device__ int measure_sampled_cell(double width, const double (&liquid)[8], const double (&solid)[8],
+                                    const Plane* analytic, double& wet_volume, double (&open)[6]) {
+  if (!finite(width) || width <= 0) return kInvalidArgument;
+  const double full_area = width * width, full_volume = full_area * width;
+  if (!normal(full_area) || !normal(full_volume)) return kOutOfRange;
+  Vtx x[8];
+  double largest = 0;
+  for (int k = 0; k < 8; ++k) {
+    if (!finite(liquid[k]) || !finite(solid[k])) return kInvalidArgument;
+    x[k].p = {{double(k & 1), double((k >> 1) & 1), double((k >> 2) & 1)}};
+    x[k].phi = liquid[k];
+    largest = max_std(largest, fabs(solid[k]));
+  }
+  int exponent = 0;
+  (void)frexp(largest, &exponent);
+  double scaled[8];
+  for (int k = 0; k < 8; ++k) {
+    scaled[k] = ldexp(solid[k], -exponent);
+    if (solid[k] != 0 && !normal(scaled[k])) return kOutOfRange;
+  }
+  double wet = 0;
+  bool wet_tet[6] = {false, false, false, false, false, false};
+  Solid a_solid, b_solid;
+  for (int t = 0; t < 6; ++t) {
+    const int corners[4] = {0, kTetrahedra[t][1], kTetrahedra[t][2], 7};
+    bool has_liquid = false, has_exterior = false;
+    for (int k = 0; k < 4; ++k) {
+      has_liquid = has_liquid || x[corners[k]].phi < 0;
+      has_exterior = has_exterior || scaled[corners[k]] > 0;
+    }
+    if (!has_liquid || !has_exterior) continue;
+    const Vtx a = x[corners[0]], b = x[corners[1]], c = x[corners[2]], dd = x[corners[3]];
+    a_solid.n = 4;
+    a_solid.overflow = false;
+    triangle(a, b, c, a_solid.f[0]);
+    triangle(a, b, dd, a_solid.f[1]);
+    triangle(a, c, dd, a_solid.f[2]);
+    triangle(b, c, dd, a_solid.f[3]);
+    clip_solid(a_solid, Plane{{{0, 0, 0}}, 0, true}, b_solid);           // liquid part
+    clip_solid(b_solid, sampled_exterior(scaled, corners), a_solid);    // outside the solid
+    double v;
+    if (analytic == nullptr) {
+      v = volume(a_solid, width);
+    } else {
+      clip_solid(a_solid, *analytic, b_solid);
+      v = volume(b_solid, width);
+    }
+    wet_tet[t] = v > 0;
+    wet += v;
+  }
+  if (!normal_or_zero(wet) || wet / full_volume > 1 + kMeasureSlack) return kOutOfRange;
+  wet = min_std(wet, full_volume);
+  if (int code = wet_connectivity(x, scaled, wet_tet, width, analytic)) return code;
+  double areas[6];
+  for (int axis = 0; axis < 3; ++axis) {
+    const int bit = 1 << axis;
+    for (int side = 0; side < 2; ++side) {
+      const int first = side * bit, last = first | (7 ^ bit);
+      double o = 0;
+      for (int tr = 0; tr < 3; ++tr) {
+        if (tr == axis) continue;
+        const int middle = first | (1 << tr);
+        const int corners[3] = {first, middle, last};
+        Polygon tri, avail;
+        triangle(x[first], x[middle], x[last], tri);
+        available_face(tri, analytic, avail);
+        o += open_area(avail, sampled_exterior(scaled, corners), width);
+      }
+      if (!normal_or_zero(o) || o / full_area > 1 + kMeasureSlack) return kOutOfRange;
+      areas[2 * axis + side] = min_std(o, full_area);
+    }
+  }
+  wet_volume = wet;
+  for (int k = 0; k < 6; ++k) open[k] = areas[k];
+  return kOk;
+}
+
+// MeasureSampledSolidFaceWeights: the blocked part of the face's two triangles, split over the four
+// corners by the centroids of its fan pieces.
+__device__ int sampled_face_weights(double width, const double (&solid)[4], const Plane* analytic,
+                                    double (&out)[4]) {
+  if (!finite(width) || width <= 0) return kInvalidArgument;
+  const double full_area = width * width;
+  if (!normal(full_area)) return kOutOfRange;
+  double largest = 0;
+  for (int k = 0; k < 4; ++k) {
+    if (!finite(solid[k])) return kInvalidArgument;
+    largest = max_std(largest, fabs(solid[k]));
+  }
+  int exponent = 0;
+  (void)frexp(largest, &exponent);
+  double scaled[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  Vtx x[4];
+  for (int k = 0; k < 4; ++k) {
+    scaled[k] = ldexp(solid[k], -exponent);
+    if (solid[k] != 0 && !normal(scaled[k])) return kOutOfRange;
+    x[k].p = {{double(k & 1), double((k >> 1) & 1), 0.0}};
+    x[k].phi = 0;
+  }
+  double w[4] = {0, 0, 0, 0};
+  for (int axis = 0; axis < 2; ++axis) {
+    const int middle = 1 << axis;
+    const int corners[3] = {0, middle, 3};
+    Polygon tri, avail, blocked;
+    triangle(x[0], x[middle], x[3], tri);
+    const Plane exterior = sampled_exterior(scaled, corners);
+    const Plane interior{scale(exterior.normal, -1), -exterior.coordinate, false};
+    available_face(tri, analytic, avail);
+    clip<kCross>(avail, interior, blocked, nullptr);
+    if (blocked.overflow) return kOutOfRange;
+    for (int v = 1; v + 1 < blocked.n; ++v) {
+      Polygon piece;
+      triangle(blocked.v[0], blocked.v[v], blocked.v[v + 1], piece);
+      const double a = area(piece, width);
+      if (!normal_or_zero(a)) return kOutOfRange;
+      const V3 centroid = scale(add(add(piece.v[0].p, piece.v[1].p), piece.v[2].p), 1.0 / 3);
+      const double along = clamp_std(centroid.c[axis], 0.0, 1.0);
+      const double across = clamp_std(centroid.c[1 - axis], 0.0, along);
+      w[0] += a * (1 - along);
+      w[middle] += a * (along - across);
+      w[3] += a * across;
+    }
+  }
+  const double measured = w[0] + w[1] + w[2] + w[3];
+  if (!normal_or_zero(measured) || measured / full_area > 1 + kMeasureSlack) return kOutOfRange;
+  for (int k = 0; k < 4; ++k) out[k] = w[k];
+  return kOk;
+}
+
 __device__ inline double lookup(const fl_center_distance* s, uint64_t n, fl_int3 k, bool& found) {
   const uint64_t i = find_sorted(s, n, k, [](const fl_center_distance& x) { return x.cell_index; });
   found = i != n;
@@ -308,6 +522,19 @@ __global__ void k_check_samples(const fl_center_distance* s, uint64_t n, uint64_
   if (i >= n) return;
   if (!finite(s[i].signed_distance) || (i > 0 && !key_less(s[i - 1].cell_index, s[i].cell_index)))
     fail(err, phase, i, kInvalidArgument);
+}
+
+// The solid samples' checks, after the corner samples' in phase 1.
+__global__ void k_check_solids(const fl_solid_corner* s, uint64_t n, uint64_t first, uint64_t* err) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  if (!finite(s[i].signed_distance) || (i > 0 && !key_less(s[i - 1].corner_index, s[i].corner_index)))
+    fail(err, 1, first + i, kInvalidArgument);
+}
+
+__device__ inline const fl_solid_corner* lookup_solid(const fl_solid_corner* s, uint64_t n, fl_int3 k) {
+  const uint64_t i = find_sorted(s, n, k, [](const fl_solid_corner& x) { return x.corner_index; });
+  return i != n ? s + i : nullptr;
 }
 
 // The liquid volume on each side of the obstacle of a cell with corner distances phi (kCell).
@@ -425,8 +652,20 @@ __global__ void k_freeze_stage(FreezeDevice d, uint64_t* err) {
       if (!offset(q, a, (k >> a) & 1, q)) return fail(err, 2, i, kOutOfRange);
     phi[k] = lookup(d.corners, d.ncorners, q, found);
     if (!found) return fail(err, 2, i, kFailedPrecondition);
+    if (d.sampled) {
+      const fl_solid_corner* solid = lookup_solid(d.solids, d.nsolids, q);
+      if (!solid) return fail(err, 2, i, kFailedPrecondition);
+      d.solid_phi[8 * i + k] = solid->signed_distance;
+    }
     any_negative = any_negative || phi[k] < 0;
     all_negative = all_negative && phi[k] < 0;
+  }
+  if (d.sampled) {  // every cell is measured against the sampled solid
+    for (int k = 0; k < 8; ++k) d.phi[9 * i + k] = phi[k];
+    d.phi[9 * i + 8] = center;
+    const unsigned long long w = atomicAdd(reinterpret_cast<unsigned long long*>(d.counts + 3), 1ull);
+    d.work[w] = i;
+    return;
   }
   Planes P;
   if (int code = local_planes(d, c, P)) return fail(err, 2, i, code);
@@ -451,8 +690,19 @@ __global__ void __launch_bounds__(64) k_freeze_cut(FreezeDevice d, uint64_t* err
   double phi[8];
   for (int k = 0; k < 8; ++k) phi[k] = d.phi[9 * i + k];
   Planes P;
-  (void)local_planes(d, c, P);  // succeeded in k_freeze_stage
   double low = 0, high = 0;
+  if (d.sampled) {  // ComputeSampledSolidGeometry for a cell: the wet volume as the high side's
+    if (int code = local_planes(d, c, P)) return fail(err, 2, i, code);
+    if (d.obstacle.kind != FL_OBSTACLE_ABSENT && d.obstacle.kind != FL_OBSTACLE_HALFSPACE)
+      return fail(err, 2, i, kInvalidArgument);
+    double solid[8], open[6];
+    for (int k = 0; k < 8; ++k) solid[k] = d.solid_phi[8 * i + k];
+    const Plane analytic{scale(P.normal, -1), -P.high, false};
+    if (int code = measure_sampled_cell(d.h, phi, solid, P.present ? &analytic : nullptr, high, open))
+      return fail(err, 2, i, code);
+    return consume_cell(d, i, c, d.phi[9 * i + 8], 0.0, high, err);
+  }
+  (void)local_planes(d, c, P);  // succeeded in k_freeze_stage
   cell_volumes(phi, P, d.h, low, high);
   consume_cell(d, i, c, d.phi[9 * i + 8], low, high, err);
 }
@@ -496,6 +746,125 @@ __global__ void k_compact_slots(FreezeDevice d) {
   }
 }
 
+// kFace: open and blocked areas of the face whose low corner is lattice l (two triangles).
+__device__ int face_areas(double h, const Planes& P, int axis, double& open_out, double& blocked_out) {
+  const double full_area = h * h;
+  int tr[2], n = 0;
+  for (int a = 0; a < 3; ++a)
+    if (a != axis) tr[n++] = a;
+  Vtx x[4] = {};
+  x[1].p.c[tr[0]] = 1;
+  x[2].p.c[tr[0]] = 1;
+  x[2].p.c[tr[1]] = 1;
+  x[3].p.c[tr[1]] = 1;
+  double blocked = 0, open = 0;
+  for (int t = 0; t < 2; ++t) {
+    Polygon g;
+    triangle(x[0], x[t + 1], x[t + 2], g);
+    if (!P.present) {
+      open += area(g, h);
+      continue;
+    }
+    Polygon bl, tmp;
+    clip<kCross>(g, Plane{P.normal, P.high, false}, bl, nullptr);
+    if (P.slab) {
+      clip<kCross>(bl, Plane{scale(P.normal, -1), -P.low, false}, tmp, nullptr);
+      bl = tmp;
+    }
+    blocked += area(bl, h);
+    open += open_area(g, Plane{scale(P.normal, -1), -P.high, false}, h);
+    if (P.slab) open += open_area(g, Plane{P.normal, P.low, false}, h);
+  }
+  open_out = min_std(open, full_area);
+  blocked_out = full_area - open_out;
+  if (!finite(open) || !finite(blocked) || open / full_area > 1 + 64 * 2.220446049250313e-16 ||
+      !normal_or_zero(open_out) || !normal_or_zero(blocked_out) || (blocked > 0 && blocked_out == 0))
+    return kOutOfRange;
+  return kOk;
+}
+
+// ComputeSampledSolidGeometry for the face at lattice l normal to axis: open and blocked areas, the wall
+// flux of the corner normal velocities through the blocked-area weights plus an analytic wall's, and
+// their mean over the weighted area. The solid samples of the four low-side corners are looked up here.
+__device__ int sampled_face(const FreezeDevice& d, fl_int3 l, int axis, uint64_t slot, fl_frozen_face& r) {
+  double solid[8] = {0, 0, 0, 0, 0, 0, 0, 0}, velocity[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const bool halfspace = d.obstacle.kind == FL_OBSTACLE_HALFSPACE;
+  for (int k = 0; k < 8; ++k) {
+    if ((k & (1 << axis)) != 0) continue;
+    fl_int3 q = l;
+    for (int a = 0; a < 3; ++a)
+      if (!offset(q, a, (k >> a) & 1, q)) return kOutOfRange;
+    const fl_solid_corner* sample = lookup_solid(d.solids, d.nsolids, q);
+    if (!sample) return kFailedPrecondition;
+    solid[k] = sample->signed_distance;
+    const double wv[3] = {sample->wall_velocity.x, sample->wall_velocity.y, sample->wall_velocity.z};
+    if (d.nface_velocities == 0) velocity[k] = wv[axis];
+    else if (halfspace) velocity[k] = d.face_velocities[slot];
+  }
+  const double width = d.h;
+  Planes P;
+  if (int code = local_planes(d, l, P)) return code;
+  if (d.obstacle.kind != FL_OBSTACLE_ABSENT && !halfspace) return kInvalidArgument;
+  const Plane analytic3{scale(P.normal, -1), -P.high, false};
+  const double liquid[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double wet = 0, open[6];
+  if (int code = measure_sampled_cell(width, liquid, solid, P.present ? &analytic3 : nullptr, wet, open)) return code;
+  const double face_open = open[2 * axis];
+  const double face_blocked = width * width - face_open;
+  double face_solid[4], face_velocity[4];
+  int fc = 0;
+  for (int k = 0; k < 8; ++k) {
+    if ((k & (1 << axis)) != 0) continue;
+    face_solid[fc] = solid[k];
+    face_velocity[fc] = velocity[k];
+    if (!finite(face_velocity[fc])) return kInvalidArgument;
+    ++fc;
+  }
+  // MeasureSampledSolidAnalyticFace: the analytic wall's blocked area and flux, and its open side in the
+  // face's transverse coordinates.
+  Plane fluid_side{{{0, 0, 0}}, -P.high, false};
+  double analytic_blocked = 0, analytic_flux = 0;
+  if (P.present) {
+    double o = 0, b = 0;
+    if (int code = face_areas(width, P, axis, o, b)) return code;
+    const double v[3] = {d.obstacle.wall_velocity.x, d.obstacle.wall_velocity.y, d.obstacle.wall_velocity.z};
+    analytic_blocked = b;
+    analytic_flux = b * v[axis];
+    int t = 0;
+    for (int a = 0; a < 3; ++a)
+      if (a != axis) fluid_side.normal.c[t++] = -P.normal.c[a];
+    if (!normal_or_zero(analytic_flux)) return kOutOfRange;
+  }
+  double w[4];
+  if (int code = sampled_face_weights(width, face_solid, P.present ? &fluid_side : nullptr, w)) return code;
+  double weighted = 0, flux = 0;
+  for (int k = 0; k < 4; ++k) {
+    weighted += w[k];
+    flux += w[k] * face_velocity[k];
+  }
+  flux += analytic_flux;
+  weighted += analytic_blocked;
+  const double slack = 64 * 2.220446049250313e-16 * width * width;
+  if (!normal_or_zero(face_blocked) || !normal_or_zero(flux) || fabs(face_blocked - weighted) > slack ||
+      (weighted > 0 && face_blocked == 0))
+    return kOutOfRange;
+  double mean = 0;
+  if (weighted > 0) {
+    mean = flux / weighted;
+    if (!finite(mean)) return kOutOfRange;
+  }
+  r.open_area = face_open;
+  r.blocked_area = face_blocked;
+  if (halfspace || d.nface_velocities == 0) {
+    r.wall_normal_velocity = mean;
+    r.prescribed_wall_flux = flux;
+    r.has_prescribed_wall_flux = 1;
+  } else {
+    r.wall_normal_velocity = d.face_velocities[slot];
+  }
+  return kOk;
+}
+
 // kFace per unique face: open and blocked areas of the two triangles and the wall velocity.
 __global__ void __launch_bounds__(64) k_freeze_faces(FreezeDevice d, uint64_t nf, uint64_t* err) {
   const uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
@@ -511,43 +880,14 @@ __global__ void __launch_bounds__(64) k_freeze_faces(FreezeDevice d, uint64_t nf
   r.lattice = l;
   r.face_slot = slot;
   d.faces[j] = r;  // a closed placeholder until the areas are known, so later phases read a valid record
+  if (d.sampled) {
+    if (int code = sampled_face(d, l, axis, slot, r)) return fail(err, 3, j, code);
+    d.faces[j] = r;
+    return;
+  }
   Planes P;
   if (int code = local_planes(d, l, P)) return fail(err, 3, j, code);
-  const double full_area = d.h * d.h;
-  int tr[2], n = 0;
-  for (int a = 0; a < 3; ++a)
-    if (a != axis) tr[n++] = a;
-  Vtx x[4] = {};
-  x[1].p.c[tr[0]] = 1;
-  x[2].p.c[tr[0]] = 1;
-  x[2].p.c[tr[1]] = 1;
-  x[3].p.c[tr[1]] = 1;
-  double blocked = 0, open = 0;
-  for (int t = 0; t < 2; ++t) {
-    Polygon g;
-    g.clear();
-    g.push(x[0]);
-    g.push(x[t + 1]);
-    g.push(x[t + 2]);
-    if (!P.present) {
-      open += area(g, d.h);
-      continue;
-    }
-    Polygon bl, tmp;
-    clip<kCross>(g, Plane{P.normal, P.high, false}, bl, nullptr);
-    if (P.slab) {
-      clip<kCross>(bl, Plane{scale(P.normal, -1), -P.low, false}, tmp, nullptr);
-      bl = tmp;
-    }
-    blocked += area(bl, d.h);
-    open += open_area(g, Plane{scale(P.normal, -1), -P.high, false}, d.h);
-    if (P.slab) open += open_area(g, Plane{P.normal, P.low, false}, d.h);
-  }
-  r.open_area = min_std(open, full_area);
-  r.blocked_area = full_area - r.open_area;
-  if (!finite(open) || !finite(blocked) || open / full_area > 1 + 64 * 2.220446049250313e-16 ||
-      !normal_or_zero(r.open_area) || !normal_or_zero(r.blocked_area) || (blocked > 0 && r.blocked_area == 0))
-    return fail(err, 3, j, kOutOfRange);
+  if (int code = face_areas(d.h, P, axis, r.open_area, r.blocked_area)) return fail(err, 3, j, code);
   const double v[3] = {d.obstacle.wall_velocity.x, d.obstacle.wall_velocity.y, d.obstacle.wall_velocity.z};
   r.wall_normal_velocity = P.present ? v[axis] : 0.0;
   d.faces[j] = r;
@@ -633,6 +973,8 @@ int freeze_cells_device(fl_ctx* ctx, const FreezeDevice& d) {
   hipStream_t s = ctx->stream;
   if (d.ncenters) k_check_samples<<<grid_for(d.ncenters), kThreads, 0, s>>>(d.centers, d.ncenters, 0, ctx->err);
   if (d.ncorners) k_check_samples<<<grid_for(d.ncorners), kThreads, 0, s>>>(d.corners, d.ncorners, 1, ctx->err);
+  if (d.sampled && d.nsolids)
+    k_check_solids<<<grid_for(d.nsolids), kThreads, 0, s>>>(d.solids, d.nsolids, d.ncorners, ctx->err);
   k_clear<<<grid_for(d.nwords), kThreads, 0, s>>>(d.words, d.nwords);
   k_clear<<<1, 64, 0, s>>>(reinterpret_cast<unsigned long long*>(d.counts), 4);
   if (d.ncand) {

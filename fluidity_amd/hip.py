@@ -113,7 +113,9 @@ class FreezeInputs(C.Structure):
                 ("corner_count", C.c_uint64), ("obstacle", Obstacle), ("step_duration", C.c_double),
                 ("liquid_density", C.c_double), ("surface_tension", C.c_double), ("exact_curvature", C.c_double),
                 ("has_exact_curvature", C.c_uint8), ("pad", C.c_uint8 * 7), ("max_cells", C.c_uint64),
-                ("max_faces", C.c_uint64)]
+                ("max_faces", C.c_uint64), ("solid_corners", C.c_void_p), ("solid_corner_count", C.c_uint64),
+                ("face_normal_velocities", C.c_void_p), ("face_velocity_count", C.c_uint64),
+                ("has_sampled_solid", C.c_uint8), ("pad2", C.c_uint8 * 7)]
 
 
 class FrozenView(C.Structure):
@@ -247,9 +249,10 @@ class Context:
 
     def freeze_geometry(self, p):
         """fl_freeze_geometry on a problem with numpy record arrays (geom, blocks, candidates, centers,
-        corners), an obstacle tuple (kind, unit normal, coordinate, half width, wall velocity) or None, and
-        scalars (step, density, sigma, exact_curvature or None, max_cells, max_faces):
-        (status, frozen cells, frozen faces, capillary step bound)."""
+        corners), an obstacle tuple (kind, unit normal, coordinate, half width, wall velocity) or None,
+        scalars (step, density, sigma, exact_curvature or None, max_cells, max_faces) and optionally a
+        sampled solid (solid_corners: SOLID_CORNER records or None; face_normal_velocities: None or one
+        per owned slot): (status, frozen cells, frozen faces, capillary step bound)."""
         o = p.geom[0]["world_origin"]
         blocks, cands, centers, corners = (np.ascontiguousarray(a) for a in (p.blocks, p.candidates, p.centers,
                                                                                p.corners))
@@ -263,6 +266,15 @@ class Context:
                               p.geom[0]["cell_width"], _p(cands), len(cands), _p(centers), len(centers), _p(corners),
                               len(corners), obs, p.step, p.density, p.sigma, 0.0 if k is None else k,
                               0 if k is None else 1, (C.c_uint8 * 7)(), p.max_cells, p.max_faces)
+        solids = getattr(p, "solid_corners", None)
+        velocities = getattr(p, "face_normal_velocities", None)
+        if solids is not None:
+            solids = np.ascontiguousarray(solids, dtype=R.SOLID_CORNER)
+            inputs.solid_corners, inputs.solid_corner_count = _p(solids), len(solids)
+            inputs.has_sampled_solid = 1
+            if velocities is not None:
+                velocities = np.ascontiguousarray(velocities, dtype=np.float64)
+                inputs.face_normal_velocities, inputs.face_velocity_count = _p(velocities), len(velocities)
         cells = np.zeros(len(cands), R.FROZEN_CELL)
         faces = np.zeros(6 * len(cands), R.FROZEN_FACE)
         nc, nf, bound = C.c_uint64(), C.c_uint64(), C.c_double()

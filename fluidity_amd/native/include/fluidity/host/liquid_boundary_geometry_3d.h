@@ -85,6 +85,21 @@ struct LiquidCornerDistance3D {
   CellIndex3D corner_index;
   double signed_distance;  // Same physical field as the centre samples, m.
 };
+// A solid corner sample (liquid_boundary_geometry_3d.h:77-81 of the reference): the cell whose low corner
+// it is, the solid's signed distance there (positive outside the solid, m) and the wall velocity
+// integrated over blocked face parts when no dense face velocities are given (m/s).
+struct SolidCornerDistance3D {
+  CellIndex3D corner_index;
+  double signed_distance;
+  Vec3 wall_velocity{};
+};
+// A sampled solid for the freeze (:83-91): corner samples, distinct and (z,y,x) sorted, covering every
+// candidate and frozen face; face normal velocities along each face's positive axis, one per owned slot,
+// or none (then the corner velocities are integrated). A translating half-space may clip it.
+struct SampledSolidBoundary3D {
+  std::span<const SolidCornerDistance3D> corner_distances;
+  std::span<const double> face_normal_velocities;
+};
 struct TranslatingHalfspace3D {
   Vec3 unit_normal;               // Outward from solid; solid is n . x <= start_plane_coordinate.
   double start_plane_coordinate;  // World meters at substep start.
@@ -119,6 +134,7 @@ struct LiquidBoundaryGeometryInputs3D {
   double liquid_density;   // kg/m³.
   double surface_tension;  // N/m.
   std::optional<double> exact_outward_interface_curvature = std::nullopt;  // 1/m.
+  std::optional<SampledSolidBoundary3D> sampled_solid = std::nullopt;
 };
 // Owns a frozen geometry and its centre samples; view() borrows them and the grid.
 class FrozenLiquidBoundaryGeometry3D {

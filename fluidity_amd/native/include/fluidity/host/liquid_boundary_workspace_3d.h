@@ -29,7 +29,6 @@
 namespace fluidity {
 
 // Same units, sample order and geometry contract as LiquidBoundaryGeometryInputs3D, on a borrowed grid.
-// The sampled-solid input of the reference is not part of this layer.
 struct LiquidBoundaryGeometryViewInputs3D {
   SparseMacGridView3D mac_grid;
   std::span<const CellIndex3D> cell_indices;
@@ -40,6 +39,7 @@ struct LiquidBoundaryGeometryViewInputs3D {
   double liquid_density;
   double surface_tension;
   std::optional<double> exact_outward_interface_curvature = std::nullopt;
+  std::optional<SampledSolidBoundary3D> sampled_solid = std::nullopt;
 };
 // Pressure cells and typed links in frozen cell and face order.
 struct LiquidPressureProblemView3D {

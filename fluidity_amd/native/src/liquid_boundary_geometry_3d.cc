@@ -137,6 +137,7 @@ struct FreezeAccess {
     r.exact_curvature = in.exact_outward_interface_curvature.value_or(0.0);
     r.max_cells = max_cells;
     r.max_faces = max_faces;
+    internal::SetSampledSolid(in.sampled_solid, r);
     std::vector<fl_frozen_cell> cells(candidates.size());
     std::vector<fl_frozen_face> faces(6 * candidates.size());
     std::uint64_t nc = 0, nf = 0;

@@ -117,6 +117,7 @@ StatusOr<FrozenLiquidBoundaryGeometryView3D> LiquidBoundaryWorkspace3D::HipFreez
   r.exact_curvature = in.exact_outward_interface_curvature.value_or(0.0);
   r.max_cells = c.cells;
   r.max_faces = c.faces;
+  internal::SetSampledSolid(in.sampled_solid, r);
   s.frozen_cells.resize(s.candidates.size());
   s.frozen_faces.resize(6 * s.candidates.size());
   std::uint64_t nc = 0, nf = 0;

@@ -2,6 +2,8 @@
 // flat C-ABI records, shared by the owning operators and the bounded workspaces.
 #pragma once
 
+#include <cstddef>
+#include <optional>
 #include <type_traits>
 #include <variant>
 
@@ -124,6 +126,18 @@ inline fl_optional_flux ToC(const std::optional<double>& w) {
   r.has_value = w.has_value();
   r.value = w.value_or(0.0);
   return r;
+}
+static_assert(sizeof(SolidCornerDistance3D) == sizeof(fl_solid_corner) &&
+              offsetof(SolidCornerDistance3D, signed_distance) == offsetof(fl_solid_corner, signed_distance) &&
+              offsetof(SolidCornerDistance3D, wall_velocity) == offsetof(fl_solid_corner, wall_velocity));
+// The sampled-solid members of the freeze inputs (none: the plane-obstacle freeze).
+inline void SetSampledSolid(const std::optional<SampledSolidBoundary3D>& solid, fl_freeze_inputs& r) {
+  if (!solid) return;
+  r.has_sampled_solid = 1;
+  r.solid_corners = reinterpret_cast<const fl_solid_corner*>(solid->corner_distances.data());
+  r.solid_corner_count = solid->corner_distances.size();
+  r.face_normal_velocities = solid->face_normal_velocities.data();
+  r.face_velocity_count = solid->face_normal_velocities.size();
 }
 inline fl_air_pressure ToC(const LiquidAirCellPressure3D& a) { return {ToC3(a.cell_index), 0, a.gas_gauge_pressure}; }
 inline fl_internal_interface ToC(const LiquidInternalInterface3D& i) {

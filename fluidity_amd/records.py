@@ -66,6 +66,9 @@ FACE_STATE = np.dtype({"names": ["axis", "lattice", "face_slot", "carrier_veloci
 FACE_INDEX = np.dtype({"names": ["axis", "lattice"], "formats": ["u1", INT3], "offsets": [0, 4], "itemsize": 16})
 CENTER_DISTANCE = np.dtype({"names": ["cell_index", "signed_distance"], "formats": [INT3, "f8"], "offsets": [0, 16],
                             "itemsize": 24})
+# SolidCornerDistance3D (liquid_boundary_geometry_3d.h:77-81), fl_solid_corner.
+SOLID_CORNER = np.dtype({"names": ["corner_index", "signed_distance", "wall_velocity"], "formats": [INT3, "f8", VEC3],
+                         "offsets": [0, 16, 24], "itemsize": 48})
 FROZEN_CELL = np.dtype({"names": ["cell_index", "fluid_volume", "center_distance", "curvature"],
                         "formats": [INT3, "f8", "f8", "f8"], "offsets": [0, 16, 24, 32], "itemsize": 40})
 FROZEN_FACE = np.dtype({"names": ["face_index", "face_slot", "open_area", "blocked_area", "wall_normal_velocity",

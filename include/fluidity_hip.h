@@ -232,6 +232,14 @@ typedef struct fl_obstacle {
 /* LiquidBoundaryGeometryViewInputs3D (native/include/fluidity/host/liquid_boundary_workspace_3d.h:18-30)
  * without a sampled solid; corner samples (LiquidCornerDistance3D) use the centre-sample record.
  * max_cells and max_faces give the reference's capacity refusals at its points (0: none). */
+/* SolidCornerDistance3D (liquid_boundary_geometry_3d.h:77-81): corner key, signed distance in metres
+ * (positive outside the solid) and the wall velocity used when no dense face velocities are given. 48 B. */
+typedef struct fl_solid_corner {
+  fl_int3 corner_index;
+  uint32_t pad;
+  double signed_distance;
+  fl_vec3 wall_velocity;
+} fl_solid_corner;
 typedef struct fl_freeze_inputs {
   const fl_int3* blocks;
   uint64_t block_count;
@@ -251,6 +259,18 @@ typedef struct fl_freeze_inputs {
   uint8_t pad[7];
   uint64_t max_cells;
   uint64_t max_faces;
+  /* SampledSolidBoundary3D (liquid_boundary_geometry_3d.h:83-91), used when has_sampled_solid is 1: solid
+   * corner samples, distinct and (z,y,x) sorted, covering every candidate's and frozen face's corners;
+   * and either no face velocities (count 0: the corner wall velocities are integrated over the blocked
+   * part of each face) or one wall normal velocity per owned slot (192 per block). The obstacle must then
+   * be absent or a half-space (which clips the sampled geometry). Zero-initialised callers keep the
+   * plane-obstacle freeze. */
+  const fl_solid_corner* solid_corners;
+  uint64_t solid_corner_count;
+  const double* face_normal_velocities;
+  uint64_t face_velocity_count;
+  uint8_t has_sampled_solid;
+  uint8_t pad2[7];
 } fl_freeze_inputs;
 /* LiquidPressureFields3D (liquid_boundary_geometry_3d.h:170-185); an empty span has count 0. */
 typedef struct fl_pressure_fields {

@@ -4,7 +4,10 @@
 // obstacles (ValidateObstacle :132-175, PackObstacle and LocalObstacle
 // native/include/fluidity/base/liquid_boundary_geometry_math_3d.h:625-650), and the scalar geometry
 // arithmetic of that header: polygon and polyhedron clipping (:160-260), clipped volume and area
-// (:296-417), the cell, face, curvature and capillary queries (:658-848).
+// (:296-417), the cell, face, curvature and capillary queries (:658-848); and the sampled-solid mode
+// (native/include/fluidity/base/sampled_solid_liquid_geometry_3d.h: MeasureSampledSolidLiquidCell,
+// MeasureSampledSolidFaceWeights, MeasureSampledSolidAnalyticFace, ComputeSampledSolidGeometry) that
+// FreezeCore selects when the inputs carry solid corner samples.
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -364,6 +367,277 @@ int FaceAreas(const flo_freeze_inputs& in, flo_int3 l, int axis, double& open_ar
   return FLO_OK;
 }
 
+// ---- sampled solids (base/sampled_solid_liquid_geometry_3d.h) ----
+
+constexpr double kMeasureSlack = 64 * std::numeric_limits<double>::epsilon();
+
+// SampledSolidExteriorPlane (:18-31): the affine extension of the solid samples along a chain of corners
+// whose consecutive bits differ in one axis; its distance is minus the interpolated solid distance, so
+// clipping keeps the exterior (distance >= 0 outside the solid).
+template <size_t N>
+Plane SampledExterior(const double (&s)[8], const int (&corners)[N]) {
+  Plane p{{0, 0, 0}, s[corners[0]], false};
+  for (size_t e = 1; e < N; ++e) {
+    const int step = corners[e] ^ corners[e - 1];
+    const int axis = (step & 1) != 0 ? 0 : ((step & 2) != 0 ? 1 : 2);
+    p.normal[axis] = s[corners[e - 1]] - s[corners[e]];
+  }
+  return p;
+}
+
+// ClipSampledAvailableFace (:33-46): the part of a polygon on the open side of the analytic boundary.
+Polygon AvailableFace(const Polygon& g, const Plane* analytic) {
+  if (analytic == nullptr || g.overflow) return g;
+  bool any = false;
+  for (size_t j = 0; j < g.n; ++j) any = any || PlaneDistance(g.v[j], *analytic) < 0;
+  if (!any) return Polygon{};
+  return Clip(g, *analytic, nullptr);
+}
+
+// CheckSampledWetConnectivity (:48-115): wet tetrahedra joined through positive wet openings of the
+// triangles they share around the body diagonal must form one piece.
+int WetConnectivity(const Vertex (&x)[8], const double (&s)[8], const bool (&wet)[6], double width,
+                    const Plane* analytic) {
+  const auto wet_corner = [&](int k) {
+    return x[k].phi < 0 && s[k] > 0 && (analytic == nullptr || PlaneDistance(x[k], *analytic) < 0);
+  };
+  if (wet_corner(0) || wet_corner(7)) return FLO_OK;
+  int components = 0;
+  for (const bool w : wet) components += w ? 1 : 0;
+  if (components <= 1) return FLO_OK;
+  constexpr int kRing[6] = {0, 1, 4, 5, 3, 2};
+  constexpr int kShared[6] = {1, 5, 4, 6, 2, 3};
+  int group[6] = {0, 1, 2, 3, 4, 5};
+  for (int e = 0; e < 6; ++e) {
+    const int first = kRing[e], second = kRing[(e + 1) % 6];
+    if (!wet[first] || !wet[second] || group[first] == group[second]) continue;
+    const int middle = kShared[e];
+    Polygon tri;
+    tri.push(x[0]);
+    tri.push(x[middle]);
+    tri.push(x[7]);
+    bool any_liquid = false;
+    for (size_t j = 0; j < tri.n; ++j) any_liquid = any_liquid || tri.v[j].phi < 0;
+    if (!any_liquid) continue;
+    const int corners[3] = {0, middle, 7};
+    if (s[0] <= 0 && s[middle] <= 0 && s[7] <= 0) continue;
+    const Polygon liquid = Clip(tri, {{0, 0, 0}, 0, true}, nullptr);
+    const double wet_area = OpenArea(AvailableFace(liquid, analytic), SampledExterior(s, corners), width);
+    if (!NormalOrZero(wet_area)) return FLO_OUT_OF_RANGE;
+    if (wet_area > 0) {
+      const int old_group = group[second], new_group = group[first];
+      for (int& m : group)
+        if (m == old_group) m = new_group;
+      --components;
+    }
+  }
+  return components == 1 ? FLO_OK : FLO_FAILED_PRECONDITION;
+}
+
+// MeasureSampledSolidLiquidCell (:118-218).
+int MeasureSampledCell(double width, const double (&liquid)[8], const double (&solid)[8], const Plane* analytic,
+                       double& wet_volume, double (&open_area)[6]) {
+  if (!std::isfinite(width) || width <= 0) return FLO_INVALID_ARGUMENT;
+  const double full_area = width * width, full_volume = full_area * width;
+  if (!std::isnormal(full_area) || !std::isnormal(full_volume)) return FLO_OUT_OF_RANGE;
+  Vertex x[8];
+  double largest = 0;
+  for (int k = 0; k < 8; ++k) {
+    if (!std::isfinite(liquid[k]) || !std::isfinite(solid[k])) return FLO_INVALID_ARGUMENT;
+    x[k].p = {double(k & 1), double((k >> 1) & 1), double((k >> 2) & 1)};
+    x[k].phi = liquid[k];
+    largest = std::max(largest, std::abs(solid[k]));
+  }
+  int exponent = 0;
+  (void)std::frexp(largest, &exponent);
+  double scaled[8];
+  for (int k = 0; k < 8; ++k) {
+    scaled[k] = std::ldexp(solid[k], -exponent);
+    if (solid[k] != 0 && !std::isnormal(scaled[k])) return FLO_OUT_OF_RANGE;
+  }
+  double wet = 0;
+  bool wet_tet[6] = {};
+  for (int t = 0; t < 6; ++t) {
+    const int corners[4] = {0, kTetrahedra[t][1], kTetrahedra[t][2], 7};
+    bool has_liquid = false, has_exterior = false;
+    for (const int k : corners) {
+      has_liquid = has_liquid || x[k].phi < 0;
+      has_exterior = has_exterior || scaled[k] > 0;
+    }
+    if (!has_liquid || !has_exterior) continue;
+    const Vertex a = x[corners[0]], b = x[corners[1]], c = x[corners[2]], d = x[corners[3]];
+    Polyhedron tet;
+    for (const auto& tri : {std::array<Vertex, 3>{a, b, c}, std::array<Vertex, 3>{a, b, d},
+                            std::array<Vertex, 3>{a, c, d}, std::array<Vertex, 3>{b, c, d}}) {
+      Polygon g;
+      for (const Vertex& v : tri) g.push(v);
+      tet.push(g);
+    }
+    const Polyhedron liquid_part = ClipSolid(tet, {{0, 0, 0}, 0, true});
+    const Polyhedron exterior = ClipSolid(liquid_part, SampledExterior(scaled, corners));
+    const double v = analytic == nullptr ? Volume(exterior, width) : Volume(ClipSolid(exterior, *analytic), width);
+    wet_tet[t] = v > 0;
+    wet += v;
+  }
+  if (!NormalOrZero(wet) || wet / full_volume > 1 + kMeasureSlack) return FLO_OUT_OF_RANGE;
+  wet = std::min(wet, full_volume);
+  if (int c = WetConnectivity(x, scaled, wet_tet, width, analytic)) return c;
+  double open[6];
+  for (int axis = 0; axis < 3; ++axis) {
+    const int bit = 1 << axis;
+    for (int side = 0; side < 2; ++side) {
+      const int first = side * bit, last = first | (7 ^ bit);
+      double o = 0;
+      for (int tr = 0; tr < 3; ++tr) {
+        if (tr == axis) continue;
+        const int middle = first | (1 << tr);
+        const int corners[3] = {first, middle, last};
+        Polygon tri;
+        tri.push(x[first]);
+        tri.push(x[middle]);
+        tri.push(x[last]);
+        o += OpenArea(AvailableFace(tri, analytic), SampledExterior(scaled, corners), width);
+      }
+      if (!NormalOrZero(o) || o / full_area > 1 + kMeasureSlack) return FLO_OUT_OF_RANGE;
+      open[2 * axis + side] = std::min(o, full_area);
+    }
+  }
+  wet_volume = wet;
+  for (int k = 0; k < 6; ++k) open_area[k] = open[k];
+  return FLO_OK;
+}
+
+// MeasureSampledSolidFaceWeights (:220-318): the blocked part of the face's two triangles (solid side
+// of each triangle's plane, inside the analytic open side), split over the four corners by the
+// centroids of its fan pieces.
+int SampledFaceWeights(double width, const double (&solid)[4], const Plane* analytic, double (&out)[4]) {
+  if (!std::isfinite(width) || width <= 0) return FLO_INVALID_ARGUMENT;
+  const double full_area = width * width;
+  if (!std::isnormal(full_area)) return FLO_OUT_OF_RANGE;
+  double largest = 0;
+  for (const double d : solid) {
+    if (!std::isfinite(d)) return FLO_INVALID_ARGUMENT;
+    largest = std::max(largest, std::abs(d));
+  }
+  int exponent = 0;
+  (void)std::frexp(largest, &exponent);
+  double scaled[8] = {};
+  Vertex x[4]{};
+  for (int k = 0; k < 4; ++k) {
+    scaled[k] = std::ldexp(solid[k], -exponent);
+    if (solid[k] != 0 && !std::isnormal(scaled[k])) return FLO_OUT_OF_RANGE;
+    x[k].p = {double(k & 1), double((k >> 1) & 1), 0};
+  }
+  double w[4] = {};
+  for (int axis = 0; axis < 2; ++axis) {
+    const int middle = 1 << axis;
+    const int corners[3] = {0, middle, 3};
+    Polygon tri;
+    tri.push(x[0]);
+    tri.push(x[middle]);
+    tri.push(x[3]);
+    const Plane exterior = SampledExterior(scaled, corners);
+    const Plane interior{Scale(exterior.normal, -1), -exterior.coordinate, false};
+    const Polygon blocked = Clip(AvailableFace(tri, analytic), interior, nullptr);
+    if (blocked.overflow) return FLO_OUT_OF_RANGE;
+    for (size_t v = 1; v + 1 < blocked.n; ++v) {
+      Polygon piece;
+      piece.push(blocked.v[0]);
+      piece.push(blocked.v[v]);
+      piece.push(blocked.v[v + 1]);
+      const double a = Area(piece, width);
+      if (!NormalOrZero(a)) return FLO_OUT_OF_RANGE;
+      const V3 centroid = Scale(Add(Add(piece.v[0].p, piece.v[1].p), piece.v[2].p), 1.0 / 3);
+      const double along = std::clamp(centroid[axis], 0.0, 1.0);
+      const double across = std::clamp(centroid[1 - axis], 0.0, along);
+      w[0] += a * (1 - along);
+      w[middle] += a * (along - across);
+      w[3] += a * across;
+    }
+  }
+  const double measured = w[0] + w[1] + w[2] + w[3];
+  if (!NormalOrZero(measured) || measured / full_area > 1 + kMeasureSlack) return FLO_OUT_OF_RANGE;
+  for (int k = 0; k < 4; ++k) out[k] = w[k];
+  return FLO_OK;
+}
+
+// ComputeSampledSolidGeometry (:386-471) for a cell: the wet volume (liquid outside the sampled solid and
+// on the open side of an analytic half-space), reported as the high-side volume.
+int SampledCellVolume(const flo_freeze_inputs& in, flo_int3 c, const double (&liquid)[8], const double (&solid)[8],
+                      double& wet) {
+  Planes P;
+  if (int s = LocalPlanes(in.obstacle, in.geometry, c, P)) return s;
+  if (in.obstacle.kind != FLO_OBSTACLE_ABSENT && in.obstacle.kind != FLO_OBSTACLE_HALFSPACE) return FLO_INVALID_ARGUMENT;
+  const Plane analytic{Scale(P.normal, -1), -P.high, false};
+  double open[6];
+  return MeasureSampledCell(in.geometry.cell_width, liquid, solid, P.present ? &analytic : nullptr, wet, open);
+}
+
+// ComputeSampledSolidGeometry for a face (the low face of the element at lattice l): open and blocked
+// areas, the wall flux of the corner normal velocities through the blocked-area weights plus the
+// analytic wall's, and the mean normal velocity over the weighted area. solid and velocity hold the
+// four low-side corners (zeros elsewhere).
+int SampledFace(const flo_freeze_inputs& in, flo_int3 l, int axis, const double (&solid)[8], const double (&velocity)[8],
+                double& open_area, double& blocked_area, double& flux, double& piston) {
+  if (axis < 0 || axis >= 3) return FLO_INVALID_ARGUMENT;
+  const double width = in.geometry.cell_width;
+  const double liquid[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  Planes P;
+  if (int s = LocalPlanes(in.obstacle, in.geometry, l, P)) return s;
+  if (in.obstacle.kind != FLO_OBSTACLE_ABSENT && in.obstacle.kind != FLO_OBSTACLE_HALFSPACE) return FLO_INVALID_ARGUMENT;
+  const Plane analytic3{Scale(P.normal, -1), -P.high, false};
+  double wet = 0, open[6];
+  if (int s = MeasureSampledCell(width, liquid, solid, P.present ? &analytic3 : nullptr, wet, open)) return s;
+  const double face_open = open[2 * axis];
+  const double face_blocked = width * width - face_open;
+  double face_solid[4], face_velocity[4];
+  int fc = 0;
+  for (int k = 0; k < 8; ++k) {
+    if ((k & (1 << axis)) != 0) continue;
+    face_solid[fc] = solid[k];
+    face_velocity[fc] = velocity[k];
+    if (!std::isfinite(face_velocity[fc])) return FLO_INVALID_ARGUMENT;
+    ++fc;
+  }
+  // MeasureSampledSolidAnalyticFace (:362-384): the analytic wall's blocked area and flux, and its open
+  // side in the face's transverse coordinates.
+  Plane fluid_side{{0, 0, 0}, -P.high, false};
+  double analytic_blocked = 0, analytic_flux = 0;
+  if (P.present) {
+    double o = 0, b = 0;
+    if (int s = FaceAreas(in, l, axis, o, b)) return s;
+    analytic_blocked = b;
+    analytic_flux = b * P.velocity[axis];
+    int t = 0;
+    for (int a = 0; a < 3; ++a)
+      if (a != axis) fluid_side.normal[t++] = -P.normal[a];
+    if (!NormalOrZero(analytic_flux)) return FLO_OUT_OF_RANGE;
+  }
+  double w[4];
+  if (int s = SampledFaceWeights(width, face_solid, P.present ? &fluid_side : nullptr, w)) return s;
+  double weighted = 0, f = 0;
+  for (int k = 0; k < 4; ++k) {
+    weighted += w[k];
+    f += w[k] * face_velocity[k];
+  }
+  f += analytic_flux;
+  weighted += analytic_blocked;
+  const double slack = 64 * std::numeric_limits<double>::epsilon() * width * width;
+  if (!NormalOrZero(face_blocked) || !NormalOrZero(f) || std::abs(face_blocked - weighted) > slack ||
+      (weighted > 0 && face_blocked == 0))
+    return FLO_OUT_OF_RANGE;
+  double v = 0;
+  if (weighted > 0) {
+    v = f / weighted;
+    if (!std::isfinite(v)) return FLO_OUT_OF_RANGE;
+  }
+  open_area = face_open;
+  blocked_area = face_blocked;
+  flux = f;
+  piston = v;
+  return FLO_OK;
+}
+
 // kCurvature: centred differences of the 19-point halo (x fastest, index 13 the centre).
 int Curvature(const double d[27], double h, double& out) {
   V3 grad{0, 0, 0};
@@ -419,6 +693,20 @@ int Lookup(const flo_center_distance* s, uint64_t n, flo_int3 key, double& out) 
   return FLO_OK;
 }
 
+int ValidateSolidSamples(const flo_solid_corner* s, uint64_t n) {
+  for (uint64_t i = 0; i < n; ++i)
+    if (!std::isfinite(s[i].signed_distance) || (i > 0 && !BlockLess(s[i - 1].corner_index, s[i].corner_index)))
+      return FLO_INVALID_ARGUMENT;
+  return FLO_OK;
+}
+
+const flo_solid_corner* LookupSolid(const flo_solid_corner* s, uint64_t n, flo_int3 key) {
+  const flo_solid_corner* e = s + n;
+  const flo_solid_corner* f = std::lower_bound(
+      s, e, key, [](const flo_solid_corner& x, const flo_int3& k) { return BlockLess(x.corner_index, k); });
+  return (f == e || !BlockEq(f->corner_index, key)) ? nullptr : f;
+}
+
 uint64_t FindCell(const std::vector<flo_frozen_cell>& cells, flo_int3 key) {
   const auto f = std::lower_bound(cells.begin(), cells.end(), key, [](const flo_frozen_cell& c, const flo_int3& k) {
     return BlockLess(c.cell_index, k);
@@ -445,6 +733,16 @@ extern "C" int flo_freeze_geometry(const flo_freeze_inputs* inputs, flo_frozen_c
     return FLO_OUT_OF_RANGE;
   if (int s = ValidateSamples(in.center_distances, in.center_count)) return s;
   if (int s = ValidateSamples(in.corner_distances, in.corner_count)) return s;
+  const bool sampled = in.has_sampled_solid != 0;
+  const bool sampled_halfspace = sampled && in.obstacle.kind == FLO_OBSTACLE_HALFSPACE;
+  if (sampled) {
+    const uint64_t face_count = 192 * in.block_count;
+    bool bad = (!sampled_halfspace && in.obstacle.kind != FLO_OBSTACLE_ABSENT) ||
+               (in.face_velocity_count != 0 && in.face_velocity_count != face_count);
+    for (uint64_t f = 0; !bad && f < in.face_velocity_count; ++f) bad = !std::isfinite(in.face_normal_velocities[f]);
+    if (bad) return FLO_INVALID_ARGUMENT;
+    if (int s = ValidateSolidSamples(in.solid_corners, in.solid_corner_count)) return s;
+  }
 
   std::vector<flo_frozen_cell> cells;
   std::vector<flo_frozen_face> faces;
@@ -453,15 +751,24 @@ extern "C" int flo_freeze_geometry(const flo_freeze_inputs* inputs, flo_frozen_c
     if (i > 0 && !BlockLess(in.cell_indices[i - 1], c)) return FLO_INVALID_ARGUMENT;
     double center = 0;
     if (int s = Lookup(in.center_distances, in.center_count, c, center)) return s;
-    double corners[8];
+    double corners[8], solid[8];
     for (int k = 0; k < 8; ++k) {
       flo_int3 q = c;
       for (int a = 0; a < 3; ++a)
         if (int s = Offset(q, a, (k >> a) & 1, q)) return s;
       if (int s = Lookup(in.corner_distances, in.corner_count, q, corners[k])) return s;
+      if (sampled) {
+        const flo_solid_corner* f = LookupSolid(in.solid_corners, in.solid_corner_count, q);
+        if (!f) return FLO_FAILED_PRECONDITION;
+        solid[k] = f->signed_distance;
+      }
     }
     double low = 0, high = 0;
-    if (int s = CellVolumes(in, c, corners, low, high)) return s;
+    if (sampled) {
+      if (int s = SampledCellVolume(in, c, corners, solid, high)) return s;
+    } else if (int s = CellVolumes(in, c, corners, low, high)) {
+      return s;
+    }
     if (low > 0 && high > 0 && center < 0) return FLO_FAILED_PRECONDITION;
     const double unclamped = low + high;
     if (!std::isfinite(unclamped) || unclamped < 0 ||
@@ -497,8 +804,35 @@ extern "C" int flo_freeze_geometry(const flo_freeze_inputs* inputs, flo_frozen_c
               faces.end());
   if (in.max_faces && faces.size() > in.max_faces) return FLO_RESOURCE_EXHAUSTED;
   for (flo_frozen_face& f : faces) {
-    if (int s = FaceAreas(in, f.face_index.lattice, f.face_index.axis, f.open_area, f.blocked_area)) return s;
-    f.wall_normal_velocity = obstacle.velocity[f.face_index.axis];
+    if (!sampled) {
+      if (int s = FaceAreas(in, f.face_index.lattice, f.face_index.axis, f.open_area, f.blocked_area)) return s;
+      f.wall_normal_velocity = obstacle.velocity[f.face_index.axis];
+      continue;
+    }
+    const int axis = f.face_index.axis;
+    double solid[8] = {}, velocity[8] = {};
+    for (int k = 0; k < 8; ++k) {
+      if ((k & (1 << axis)) != 0) continue;
+      flo_int3 q = f.face_index.lattice;
+      for (int a = 0; a < 3; ++a)
+        if (int s = Offset(q, a, (k >> a) & 1, q)) return s;
+      const flo_solid_corner* sample = LookupSolid(in.solid_corners, in.solid_corner_count, q);
+      if (!sample) return FLO_FAILED_PRECONDITION;
+      solid[k] = sample->signed_distance;
+      const double wv[3] = {sample->wall_velocity.x, sample->wall_velocity.y, sample->wall_velocity.z};
+      if (in.face_velocity_count == 0) velocity[k] = wv[axis];
+      else if (sampled_halfspace) velocity[k] = in.face_normal_velocities[f.face_slot];
+    }
+    double flux = 0, piston = 0;
+    if (int s = SampledFace(in, f.face_index.lattice, axis, solid, velocity, f.open_area, f.blocked_area, flux, piston))
+      return s;
+    if (sampled_halfspace || in.face_velocity_count == 0) {
+      f.wall_normal_velocity = piston;
+      f.prescribed_wall_flux = flux;
+      f.has_prescribed_wall_flux = 1;
+    } else {
+      f.wall_normal_velocity = in.face_normal_velocities[f.face_slot];
+    }
   }
   // Cells on an open face whose other side is air (a centre sample >= 0) need curvature.
   std::vector<uint8_t> needs(cells.size(), 0);
@@ -551,5 +885,32 @@ extern "C" int flo_freeze_geometry(const flo_freeze_inputs* inputs, flo_frozen_c
   *out_cell_count = cells.size();
   *out_face_count = faces.size();
   *out_capillary_step_bound = bound;
+  return FLO_OK;
+}
+
+extern "C" int flo_measure_sampled_cell(double width, const double liquid[8], const double solid[8],
+                                        const double* analytic, double* out_wet_volume, double out_open_area[6]) {
+  double l[8], so[8], open[6], wet = 0;
+  for (int k = 0; k < 8; ++k) {
+    l[k] = liquid[k];
+    so[k] = solid[k];
+  }
+  Plane p{};
+  if (analytic) p = Plane{{analytic[0], analytic[1], analytic[2]}, analytic[3], false};
+  const int s = MeasureSampledCell(width, l, so, analytic ? &p : nullptr, wet, open);
+  if (s) return s;
+  *out_wet_volume = wet;
+  for (int k = 0; k < 6; ++k) out_open_area[k] = open[k];
+  return FLO_OK;
+}
+
+extern "C" int flo_sampled_face_weights(double width, const double solid[4], const double* analytic,
+                                        double out_weights[4]) {
+  double so[4] = {solid[0], solid[1], solid[2], solid[3]}, w[4];
+  Plane p{};
+  if (analytic) p = Plane{{analytic[0], analytic[1], analytic[2]}, analytic[3], false};
+  const int s = SampledFaceWeights(width, so, analytic ? &p : nullptr, w);
+  if (s) return s;
+  for (int k = 0; k < 4; ++k) out_weights[k] = w[k];
   return FLO_OK;
 }

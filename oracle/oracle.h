@@ -276,6 +276,8 @@ struct flo_obstacle {
   flo_vec3 wall_velocity;
   flo_tank tank;      // open tank only (the particle update; the freeze refuses it)
 };  // 120 B
+// SolidCornerDistance3D: corner key, signed distance (positive outside the solid), wall velocity. 48 B.
+struct flo_solid_corner { flo_int3 corner_index; uint32_t pad; double signed_distance; flo_vec3 wall_velocity; };
 struct flo_freeze_inputs {
   const flo_int3* blocks;
   uint64_t block_count;
@@ -292,6 +294,14 @@ struct flo_freeze_inputs {
   uint8_t has_exact_curvature, pad[7];
   uint64_t max_cells;  // ResourceExhausted above this many candidates (0: no limit)
   uint64_t max_faces;  // ResourceExhausted above this many frozen faces (0: no limit)
+  // SampledSolidBoundary3D (host/liquid_boundary_geometry_3d.h:77-91), used when has_sampled_solid:
+  // solid corner samples (distinct, (z,y,x) sorted) and, when face_velocity_count is not 0, one wall
+  // normal velocity per owned slot.
+  const struct flo_solid_corner* solid_corners;
+  uint64_t solid_corner_count;
+  const double* face_normal_velocities;
+  uint64_t face_velocity_count;
+  uint8_t has_sampled_solid, pad2[7];
 };
 // FreezeCore (liquid_boundary_geometry_3d.cc:765-1068) with the scalar geometry queries of
 // base/liquid_boundary_geometry_math_3d.h (kCell, kFace, kCurvature, kCapillary). out_cells holds
@@ -299,6 +309,15 @@ struct flo_freeze_inputs {
 // written only on success.
 int flo_freeze_geometry(const flo_freeze_inputs* inputs, flo_frozen_cell* out_cells, uint64_t* out_cell_count,
                         flo_frozen_face* out_faces, uint64_t* out_face_count, double* out_capillary_step_bound);
+// MeasureSampledSolidLiquidCell (base/sampled_solid_liquid_geometry_3d.h:118-218): wet volume and the six
+// open face areas (low/high x, y, z) of a cell with liquid and solid corner distances; analytic (may be
+// null) is an open half-space {normal, coordinate} in cell-width coordinates. Returns 0, or
+// FLO_INVALID_ARGUMENT / FLO_OUT_OF_RANGE / FLO_FAILED_PRECONDITION for the reference's kInvalid /
+// kRange / kPrecondition.
+int flo_measure_sampled_cell(double width, const double liquid[8], const double solid[8], const double* analytic,
+                             double* out_wet_volume, double out_open_area[6]);
+// MeasureSampledSolidFaceWeights (:220-318): corner area weights of the blocked part of a face.
+int flo_sampled_face_weights(double width, const double solid[4], const double* analytic, double out_weights[4]);
 
 // ---- particle update (liquid_substep_3d_cpu.cc:239-405) ----
 struct flo_constituent { uint64_t substance_id; double volume_fraction; };

@@ -77,6 +77,9 @@ FACE_STATE = np.dtype({"names": ["face_index", "face_slot", "carrier_velocity", 
 # Frozen boundary geometry and pressure-field records (native/include/fluidity/host/liquid_boundary_geometry_3d.h).
 CENTER_DISTANCE = np.dtype({"names": ["cell_index", "signed_distance"], "formats": [INT3, "f8"], "offsets": [0, 16],
                             "itemsize": 24})
+# SolidCornerDistance3D (host/liquid_boundary_geometry_3d.h:77-81).
+SOLID_CORNER = np.dtype({"names": ["corner_index", "signed_distance", "wall_velocity"], "formats": [INT3, "f8", VEC3],
+                         "offsets": [0, 16, 24], "itemsize": 48})
 FROZEN_CELL = np.dtype({"names": ["cell_index", "fluid_volume", "center_distance", "curvature"],
                         "formats": [INT3, "f8", "f8", "f8"], "offsets": [0, 16, 24, 32], "itemsize": 40})
 FROZEN_FACE = np.dtype({"names": ["face_index", "face_slot", "open_area", "blocked_area", "wall_normal_velocity",
@@ -421,7 +424,9 @@ class FreezeInputsC(C.Structure):
                 ("corner_count", C.c_uint64), ("obstacle", ObstacleC), ("step_duration", C.c_double),
                 ("liquid_density", C.c_double), ("surface_tension", C.c_double), ("exact_curvature", C.c_double),
                 ("has_exact_curvature", C.c_uint8), ("pad", C.c_uint8 * 7), ("max_cells", C.c_uint64),
-                ("max_faces", C.c_uint64)]
+                ("max_faces", C.c_uint64), ("solid_corners", C.c_void_p), ("solid_corner_count", C.c_uint64),
+                ("face_normal_velocities", C.c_void_p), ("face_velocity_count", C.c_uint64),
+                ("has_sampled_solid", C.c_uint8), ("pad2", C.c_uint8 * 7)]
 
 
 assert C.sizeof(ObstacleC) == 120
@@ -446,12 +451,18 @@ class FreezeProblem:
     (kind, unit normal, coordinate, half width, wall velocity) as made by halfspace() or slab()."""
 
     def __init__(self, geom, blocks, candidates, centers, corners, obstacle=None, step=0.01, density=1000.0,
-                 sigma=0.0, exact_curvature=None, max_cells=0, max_faces=0):
+                 sigma=0.0, exact_curvature=None, max_cells=0, max_faces=0, solid_corners=None,
+                 face_normal_velocities=None):
+        """solid_corners: SOLID_CORNER records selecting the sampled-solid mode (None: no sampled solid);
+        face_normal_velocities: None (empty span) or one wall normal velocity per owned slot."""
         self.geom, self.blocks = geom, np.ascontiguousarray(blocks)
         self.candidates, self.centers, self.corners = (np.ascontiguousarray(a) for a in (candidates, centers, corners))
         self.obstacle = obstacle
         self.step, self.density, self.sigma, self.exact_curvature = step, density, sigma, exact_curvature
         self.max_cells, self.max_faces = max_cells, max_faces
+        self.solid_corners = None if solid_corners is None else np.ascontiguousarray(solid_corners, dtype=SOLID_CORNER)
+        self.face_normal_velocities = (None if face_normal_velocities is None
+                                       else np.ascontiguousarray(face_normal_velocities, dtype=np.float64))
 
     def c_obstacle(self):
         if self.obstacle is None:
@@ -467,7 +478,31 @@ class FreezeProblem:
                              self.geom[0]["cell_width"], _p(self.candidates), len(self.candidates), _p(self.centers),
                              len(self.centers), _p(self.corners), len(self.corners), self.c_obstacle(), self.step,
                              self.density, self.sigma, 0.0 if k is None else k, 0 if k is None else 1,
-                             (C.c_uint8 * 7)(), self.max_cells, self.max_faces)
+                             (C.c_uint8 * 7)(), self.max_cells, self.max_faces,
+                             _p(self.solid_corners), 0 if self.solid_corners is None else len(self.solid_corners),
+                             _p(self.face_normal_velocities),
+                             0 if self.face_normal_velocities is None else len(self.face_normal_velocities),
+                             0 if self.solid_corners is None else 1, (C.c_uint8 * 7)())
+
+
+def measure_sampled_cell(width, liquid, solid, analytic=None):
+    """flo_measure_sampled_cell: (status, wet volume, six open areas); analytic = (nx, ny, nz, coordinate)."""
+    l = np.ascontiguousarray(liquid, dtype=np.float64)
+    so = np.ascontiguousarray(solid, dtype=np.float64)
+    a = None if analytic is None else np.ascontiguousarray(analytic, dtype=np.float64)
+    wet, open_area = C.c_double(), np.zeros(6)
+    s = lib().flo_measure_sampled_cell(C.c_double(width), _p(l), _p(so), _p(a), C.byref(wet), _p(open_area))
+    return s, wet.value, open_area
+
+
+def sampled_face_weights(width, solid, analytic=None):
+    """flo_sampled_face_weights: (status, four corner weights); analytic = (n0, n1, 0, coordinate) in face
+    coordinates."""
+    so = np.ascontiguousarray(solid, dtype=np.float64)
+    a = None if analytic is None else np.ascontiguousarray(analytic, dtype=np.float64)
+    w = np.zeros(4)
+    s = lib().flo_sampled_face_weights(C.c_double(width), _p(so), _p(a), _p(w))
+    return s, w
 
 
 def freeze_geometry(problem):

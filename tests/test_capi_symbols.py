@@ -33,7 +33,7 @@ def test_record_layouts_match_the_header():
     assert R.WALL_SAMPLE.itemsize == 64 and R.BOUNDARY_GATHER.itemsize == 224
     assert R.BLOCK_STATE.itemsize == 32 and R.CELL_STATE.itemsize == 24 and R.FACE_STATE.itemsize == 56
     assert R.LIQUID_PARTICLE.itemsize == 328 and R.UPDATE_OPTIONS.itemsize == 160
-    assert R.VISCOSITY_CELL.itemsize == 24 and R.STRAIN_ROW.itemsize == 168
+    assert R.VISCOSITY_CELL.itemsize == 24 and R.STRAIN_ROW.itemsize == 168 and R.SOLID_CORNER.itemsize == 48
 
 
 def test_record_offsets_match_the_c_compiler(tmp_path):
@@ -63,6 +63,7 @@ def test_record_offsets_match_the_c_compiler(tmp_path):
                                                    "low_mass_fallback": "low_mass_fallback"}),
         "fl_viscosity_cell": (R.VISCOSITY_CELL, {n: n for n in R.VISCOSITY_CELL.names}),
         "fl_strain_row": (R.STRAIN_ROW, {n: n for n in R.STRAIN_ROW.names}),
+        "fl_solid_corner": (R.SOLID_CORNER, {n: n for n in R.SOLID_CORNER.names}),
     }
     lines = ['#include <stddef.h>', '#include <stdio.h>', f'#include "{ROOT}/include/fluidity_hip.h"', "int main(void) {"]
     for struct, (_, fields) in checks.items():
@@ -93,7 +94,10 @@ def test_ctypes_structures_match_the_c_compiler(tmp_path):
     checks = {"fl_freeze_inputs": (H.FreezeInputs, {"origin": "geometry", "obstacle": "obstacle",
                                                     "step_duration": "step_duration",
                                                     "has_exact_curvature": "has_exact_curvature",
-                                                    "max_cells": "max_cells", "max_faces": "max_faces"}),
+                                                    "max_cells": "max_cells", "max_faces": "max_faces",
+                                                    "solid_corners": "solid_corners",
+                                                    "face_velocity_count": "face_velocity_count",
+                                                    "has_sampled_solid": "has_sampled_solid"}),
               "fl_obstacle": (H.Obstacle, {"unit_normal": "unit_normal", "coordinate": "coordinate",
                                            "half_width": "half_width", "wall_velocity": "wall_velocity"}),
               "fl_frozen_view": (H.FrozenView, {"origin": "geometry", "center_distances": "center_distances",
