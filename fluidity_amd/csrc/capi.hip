@@ -1092,9 +1092,19 @@ int fl_freeze_geometry(fl_ctx* ctx, const fl_freeze_inputs* inputs, fl_frozen_ce
     if (!std::isfinite(low) || std::abs(n[0] * n[0] + n[1] * n[1] + n[2] * n[2] - 1.0) > 1e-12)
       return FL_INVALID_ARGUMENT;
   } else if (o.kind == FL_OBSTACLE_TANK) {
-    // The reference clips cells against the tank walls (liquid_boundary_geometry_3d.cc:162-167,319-460);
-    // that clipping is not built, so a tank is refused with its own status rather than as bad data.
-    return FL_UNIMPLEMENTED;
+    // CheckTankStep (base/liquid_boundary_math.h:252-276): finite walls and piston, ordered x and z walls,
+    // a representable piston pose strictly inside the high-x wall at the step's end.
+    const fl_tank& t = o.tank;
+    if (!std::isfinite(t.start_low_x) || !std::isfinite(t.high_x) || !std::isfinite(t.floor_y) ||
+        !std::isfinite(t.low_z) || !std::isfinite(t.high_z) || !std::isfinite(t.piston_velocity_x))
+      return FL_INVALID_ARGUMENT;
+    if (t.start_low_x >= t.high_x || t.low_z >= t.high_z) return FL_INVALID_ARGUMENT;
+    const double end_x = std::fma(t.piston_velocity_x, I.step_duration, t.start_low_x);
+    if (!std::isfinite(t.piston_velocity_x * I.step_duration) || !std::isfinite(end_x) ||
+        !std::isfinite(t.high_x - t.start_low_x) || !std::isfinite(t.high_x - end_x) ||
+        !std::isfinite(t.high_z - t.low_z))
+      return FL_OUT_OF_RANGE;
+    if (end_x >= t.high_x) return FL_FAILED_PRECONDITION;
   } else if (o.kind != FL_OBSTACLE_ABSENT) {
     return FL_INVALID_ARGUMENT;
   }

@@ -1,9 +1,11 @@
 // Frozen boundary geometry on the device (FreezeCore, native/src/liquid_boundary_geometry_3d.cc:765-1068)
-// for the absent, half-space and slab obstacles: per candidate cell the liquid volume on each side of
+// for the absent, half-space, slab and open-tank obstacles: per candidate cell the liquid volume on each side of
 // the obstacle from six body-diagonal tetrahedra clipped by the piecewise-linear liquid distance and
 // the obstacle planes; per incident face the open and blocked area of its two triangles; the centred
 // curvature of cells on an open liquid-air face (base/liquid_boundary_geometry_math_3d.h:160-417,
-// 625-848).
+// 625-848). The open tank clips the liquid by its axis-aligned walls located in the lattice frame with
+// bounded sums; its faces carry the aperture of the open walls and the piston's flux on the low-x face
+// (:420-623).
 //
 // With a sampled solid (base/sampled_solid_liquid_geometry_3d.h) the cell query measures the liquid outside
 // the piecewise-linear solid of the corner samples (each tetrahedron clipped by the liquid plane and by
@@ -22,6 +24,7 @@
 // obstacle plane cut a tetrahedron into at most six faces of at most five vertices, with at most ten
 // crossings before duplicates are removed; the capacities below hold these with room to spare, and an
 // overflow gives a NaN measure (OutOfRange), as the reference's bounded lists do.
+#include "bounded.h"
 #include "keys.h"
 #include "stdmath.h"
 
@@ -84,6 +87,7 @@ struct Plane {
   V3 normal;
   double coordinate;
   bool liquid;
+  int snap = -1;  // an axis-aligned cut pins its crossings' coordinate on this axis to the plane's
 };
 
 __device__ inline double distance(const Vtx& x, const Plane& p) {
@@ -109,7 +113,8 @@ __device__ void clip(const Polygon& in, const Plane& plane, Polygon& out, Poly<N
       const double ps = dp / m, cs = dc / m;
       const double diff = ps - cs;
       const double wp = -cs / diff, wc = ps / diff;
-      const Vtx x{add(scale(prev.p, wp), scale(cur.p, wc)), wp * prev.phi + wc * cur.phi};
+      Vtx x{add(scale(prev.p, wp), scale(cur.p, wc)), wp * prev.phi + wc * cur.phi};
+      if (plane.snap >= 0) x.p.c[plane.snap] = plane.coordinate * plane.normal.c[plane.snap];
       out.push(x);
       if (crossings) crossings->push(x);
     }
@@ -304,6 +309,179 @@ __device__ inline int local_planes(const FreezeDevice& d, fl_int3 c, Planes& out
 
 __constant__ int kTetrahedra[6][4] = {{0, 1, 3, 7}, {0, 1, 5, 7}, {0, 2, 3, 7},
                                       {0, 2, 6, 7}, {0, 4, 5, 7}, {0, 4, 6, 7}};
+
+// ---- the open tank (base/liquid_boundary_math.h:278-292; liquid_boundary_geometry_math_3d.h:420-623) ----
+
+struct TankLattice {
+  double origin, index, width;
+};
+__device__ inline TankLattice tank_axis(const FreezeDevice& d, fl_int3 c, int axis) {
+  const double o[3] = {d.origin.x, d.origin.y, d.origin.z};
+  const int32_t i[3] = {c.x, c.y, c.z};
+  return {o[axis], double(i[axis]), d.h};
+}
+
+// TankCellHasResolvedWorldExtent: each axis's world interval is ordered, finite and resolved by the
+// spacing toward its interior at both ends.
+__device__ inline bool tank_cell_resolved(const FreezeDevice& d, fl_int3 c) {
+  const double anchor[3] = {d.origin.x + d.h * c.x, d.origin.y + d.h * c.y, d.origin.z + d.h * c.z};
+  for (int a = 0; a < 3; ++a) {
+    const TankLattice L = tank_axis(d, c, a);
+    const double low = anchor[a], high = L.origin + L.width * (L.index + 1);
+    if (!finite(low) || !finite(high) || low >= high) return false;
+    const double low_spacing = nextafter(low, high) - low;
+    const double high_spacing = high - nextafter(high, low);
+    if (low_spacing > L.width || high_spacing > L.width) return false;
+  }
+  return true;
+}
+
+// TankPlaneOffset: world - origin - h index as a bounded sum in the lattice frame.
+__device__ inline bounded::PSum tank_plane_offset(double world, const TankLattice& L) {
+  bounded::PSum o;
+  o.add(world);
+  o.add(-L.origin);
+  o.product(-L.width, L.index);
+  return o;
+}
+
+__device__ inline int tank_plane_sign(double world, const TankLattice& L, int& sign) {
+  const double translated = world - L.origin;
+  if (isinf(translated)) {
+    sign = translated > 0 ? 1 : -1;
+    return kOk;
+  }
+  return tank_plane_offset(world, L).bounds().sign(&sign) ? kOk : kOutOfRange;
+}
+
+// TankClipCoordinate: the plane's coordinate in the unit cell, 0 or 1 beyond a corner.
+__device__ inline int tank_clip_coordinate(double world, const TankLattice& L, double& out) {
+  const double translated = world - L.origin;
+  if (isinf(translated)) {
+    out = translated > 0 ? 1.0 : 0.0;
+    return kOk;
+  }
+  bounded::PSum offset = tank_plane_offset(world, L);
+  const bounded::Bounds low = offset.bounds();
+  int low_side = 0;
+  if (!low.sign(&low_side)) return kOutOfRange;
+  if (low_side <= 0) {
+    out = 0.0;
+    return kOk;
+  }
+  offset.add(-L.width);
+  int high_side = 0;
+  if (!offset.bounds().sign(&high_side)) return kOutOfRange;
+  if (high_side >= 0) {
+    out = 1.0;
+    return kOk;
+  }
+  const double cell = low.estimate / L.width;
+  if (cell <= 0 || cell >= 1) return kOutOfRange;
+  out = cell;
+  return kOk;
+}
+
+struct TankBounds {
+  double low[3], high[3];
+};
+
+// LocalTankBounds: the walls in the unit frame of the element at c (a face skips its normal axis).
+__device__ inline int local_tank_bounds(const FreezeDevice& d, fl_int3 c, int face_axis, TankBounds& b) {
+  const fl_tank& t = d.obstacle.tank;
+  const double low_world[3] = {t.start_low_x, t.floor_y, t.low_z};
+  const double high_world[3] = {t.high_x, __builtin_inf(), t.high_z};
+  for (int a = 0; a < 3; ++a) {
+    b.low[a] = 0;
+    b.high[a] = 1;
+  }
+  for (int a = 0; a < 3; ++a) {
+    if (a == face_axis) continue;
+    const TankLattice L = tank_axis(d, c, a);
+    double lo = 0, hi = 0;
+    const int s1 = tank_clip_coordinate(low_world[a], L, lo);
+    const int s2 = tank_clip_coordinate(high_world[a], L, hi);
+    if (s1 || s2) return kOutOfRange;
+    b.low[a] = lo;
+    b.high[a] = hi;
+    if (lo == hi && lo > 0 && hi < 1) return kOutOfRange;
+  }
+  return kOk;
+}
+
+// TankBoxMeasure: the product of the box's extents, scaled before multiplying.
+__device__ inline double tank_box_measure(const TankBounds& b, double h, int excluded) {
+  double mantissa = 1;
+  int exponent = 0, dimension = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (a == excluded) continue;
+    const double extent = b.high[a] - b.low[a];
+    if (extent <= 0) return 0;
+    int e = 0;
+    mantissa *= frexp(extent, &e);
+    exponent += e;
+    ++dimension;
+  }
+  return physical_measure(mantissa, exponent, h, dimension);
+}
+
+// ClipLiquidToTank: the liquid inside the walls that cut the unit cell; in and tmp swap roles, the
+// result is left in `in`.
+__device__ inline void clip_liquid_to_tank(Solid& in, Solid& tmp, const TankBounds& b) {
+  for (int a = 0; a < 3; ++a) {
+    V3 n{{0, 0, 0}};
+    if (b.low[a] > 0) {
+      n.c[a] = -1;
+      clip_solid(in, Plane{n, -b.low[a], false, a}, tmp);
+      in = tmp;
+    }
+    if (b.high[a] < 1) {
+      n.c[a] = 1;
+      clip_solid(in, Plane{n, b.high[a], false, a}, tmp);
+      in = tmp;
+    }
+  }
+}
+
+// ComputeTankFace: the aperture of an open face; on a closed low-x face the piston's rectangle and flux.
+__device__ int tank_face(const FreezeDevice& d, fl_int3 l, int axis, const TankBounds& b, fl_frozen_face& r) {
+  const fl_tank& t = d.obstacle.tank;
+  const double h = d.h, full_area = h * h;
+  const double low_world[3] = {t.start_low_x, t.floor_y, t.low_z};
+  const double high_world[3] = {t.high_x, __builtin_inf(), t.high_z};
+  const TankLattice L = tank_axis(d, l, axis);
+  int low_sign = 0, high_sign = 0;
+  const int s1 = tank_plane_sign(low_world[axis], L, low_sign);
+  const int s2 = tank_plane_sign(high_world[axis], L, high_sign);
+  if (s1 || s2) return kOutOfRange;
+  const bool piston_side = axis == 0 && low_sign >= 0;
+  const bool open = low_sign < 0 && high_sign > 0;
+  double o = 0, flux = 0, v = 0;
+  uint8_t low_x = 0;
+  if (open || piston_side) {
+    const double transverse = tank_box_measure(b, h, axis);
+    if (!normal_or_zero(transverse) || transverse > full_area) return kOutOfRange;
+    if (open) {
+      o = transverse;
+      for (int a = 0; a < 3; ++a)
+        if (a != axis && transverse == full_area && (b.low[a] > 0 || b.high[a] < 1)) return kOutOfRange;
+    } else if (transverse > 0) {
+      low_x = 1;
+      v = t.piston_velocity_x;
+      flux = transverse * v;
+      if (!normal_or_zero(flux) || (v != 0 && flux == 0)) return kOutOfRange;
+    }
+  }
+  const double blocked = full_area - o;
+  if (!normal_or_zero(blocked)) return kOutOfRange;
+  r.open_area = o;
+  r.blocked_area = blocked;
+  r.wall_normal_velocity = v;
+  r.prescribed_wall_flux = flux;
+  r.has_prescribed_wall_flux = 1;
+  r.is_low_x_piston_face = low_x;
+  return kOk;
+}
 
 // ---- sampled solids ----
 
@@ -538,7 +716,9 @@ __device__ inline const fl_solid_corner* lookup_solid(const fl_solid_corner* s, 
 }
 
 // The liquid volume on each side of the obstacle of a cell with corner distances phi (kCell).
-__device__ void cell_volumes(const double (&phi)[8], const Planes& P, double h, double& low, double& high) {
+// (with the open tank's bounds: the liquid inside its walls).
+__device__ void cell_volumes(const double (&phi)[8], const Planes& P, double h, double& low, double& high,
+                             const TankBounds* tank = nullptr) {
   Vtx x[8];
   for (int k = 0; k < 8; ++k) {
     x[k].phi = phi[k];
@@ -561,7 +741,10 @@ __device__ void cell_volumes(const double (&phi)[8], const Planes& P, double h, 
       for (int j = 0; j < 3; ++j) tet.f[k].push(tri[k][j]);
     }
     clip_solid(tet, Plane{{{0, 0, 0}}, 0, true}, liquid);
-    if (!P.present) {
+    if (tank) {
+      clip_liquid_to_tank(liquid, a_solid, *tank);
+      high += volume(liquid, h);
+    } else if (!P.present) {
       high += volume(liquid, h);
     } else {
       clip_solid(liquid, Plane{scale(P.normal, -1), -P.high, false}, part);
@@ -669,6 +852,20 @@ __global__ void k_freeze_stage(FreezeDevice d, uint64_t* err) {
   }
   Planes P;
   if (int code = local_planes(d, c, P)) return fail(err, 2, i, code);
+  if (d.obstacle.kind == FL_OBSTACLE_TANK) {  // the tank cell's box, or the liquid inside it
+    if (!tank_cell_resolved(d, c)) return fail(err, 2, i, kOutOfRange);
+    TankBounds tb;
+    if (int code = local_tank_bounds(d, c, -1, tb)) return fail(err, 2, i, code);
+    const double box = tank_box_measure(tb, d.h, -1);
+    if (!normal_or_zero(box)) return fail(err, 2, i, kOutOfRange);
+    if (box == 0 || !any_negative) return consume_cell(d, i, c, center, 0.0, 0.0, err);
+    if (all_negative) return consume_cell(d, i, c, center, 0.0, box, err);
+    for (int k = 0; k < 8; ++k) d.phi[9 * i + k] = phi[k];
+    d.phi[9 * i + 8] = center;
+    const unsigned long long w = atomicAdd(reinterpret_cast<unsigned long long*>(d.counts + 3), 1ull);
+    d.work[w] = i;
+    return;
+  }
   if (!any_negative) return consume_cell(d, i, c, center, 0.0, 0.0, err);
   if (all_negative) {
     const bool high_whole = !P.present || all_kept(Plane{scale(P.normal, -1), -P.high, false});
@@ -703,6 +900,12 @@ __global__ void __launch_bounds__(64) k_freeze_cut(FreezeDevice d, uint64_t* err
     return consume_cell(d, i, c, d.phi[9 * i + 8], 0.0, high, err);
   }
   (void)local_planes(d, c, P);  // succeeded in k_freeze_stage
+  if (d.obstacle.kind == FL_OBSTACLE_TANK) {
+    TankBounds tb;
+    (void)local_tank_bounds(d, c, -1, tb);  // succeeded in k_freeze_stage
+    cell_volumes(phi, P, d.h, low, high, &tb);
+    return consume_cell(d, i, c, d.phi[9 * i + 8], low, high, err);
+  }
   cell_volumes(phi, P, d.h, low, high);
   consume_cell(d, i, c, d.phi[9 * i + 8], low, high, err);
 }
@@ -880,6 +1083,13 @@ __global__ void __launch_bounds__(64) k_freeze_faces(FreezeDevice d, uint64_t nf
   r.lattice = l;
   r.face_slot = slot;
   d.faces[j] = r;  // a closed placeholder until the areas are known, so later phases read a valid record
+  if (d.obstacle.kind == FL_OBSTACLE_TANK) {
+    TankBounds tb;
+    if (int code = local_tank_bounds(d, l, axis, tb)) return fail(err, 3, j, code);
+    if (int code = tank_face(d, l, axis, tb, r)) return fail(err, 3, j, code);
+    d.faces[j] = r;
+    return;
+  }
   if (d.sampled) {
     if (int code = sampled_face(d, l, axis, slot, r)) return fail(err, 3, j, code);
     d.faces[j] = r;

@@ -112,8 +112,8 @@ struct TranslatingSlab3D {
   double half_width;  // Positive finite, m.
   Vec3 wall_velocity;
 };
-// An open tank (native/include/fluidity/base/liquid_boundary_math.h:22-29): the particle update resolves
-// its contact; the freeze refuses it (Unimplemented: clipping cells against the tank walls is not built).
+// An open tank (native/include/fluidity/base/liquid_boundary_math.h:22-29): the freeze clips the liquid by
+// its walls and gives the low-x face the piston's flux; the particle update resolves its contact.
 struct AxisAlignedLiquidTank3D {
   double start_low_x;        // World x of the piston at substep start, m.
   double high_x;             // m.

@@ -145,8 +145,7 @@ inline fl_internal_interface ToC(const LiquidInternalInterface3D& i) {
           i.normal_viscous_jump_tail_to_head};
 }
 
-// The frozen-geometry obstacle record; a tank keeps its kind (the device freeze refuses it as
-// Unimplemented) and its walls.
+// The frozen-geometry obstacle record; a tank keeps its kind and its walls.
 inline fl_obstacle ToC(const LiquidObstacle3D& o) {
   const auto c3 = [](Vec3 v) { return fl_vec3{v.x, v.y, v.z}; };
   fl_obstacle r{};

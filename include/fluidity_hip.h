@@ -227,11 +227,8 @@ typedef struct fl_obstacle {
   double coordinate;  /* half-space: start plane coordinate; slab: start centre coordinate */
   double half_width;  /* slab only */
   fl_vec3 wall_velocity;
-  fl_tank tank;       /* open tank only: the particle update takes it, the freeze refuses it */
+  fl_tank tank;       /* open tank only */
 } fl_obstacle; /* 120 B */
-/* LiquidBoundaryGeometryViewInputs3D (native/include/fluidity/host/liquid_boundary_workspace_3d.h:18-30)
- * without a sampled solid; corner samples (LiquidCornerDistance3D) use the centre-sample record.
- * max_cells and max_faces give the reference's capacity refusals at its points (0: none). */
 /* SolidCornerDistance3D (liquid_boundary_geometry_3d.h:77-81): corner key, signed distance in metres
  * (positive outside the solid) and the wall velocity used when no dense face velocities are given. 48 B. */
 typedef struct fl_solid_corner {
@@ -240,6 +237,9 @@ typedef struct fl_solid_corner {
   double signed_distance;
   fl_vec3 wall_velocity;
 } fl_solid_corner;
+/* LiquidBoundaryGeometryViewInputs3D (native/include/fluidity/host/liquid_boundary_workspace_3d.h:18-30);
+ * corner samples (LiquidCornerDistance3D) use the centre-sample record. max_cells and max_faces give the
+ * reference's capacity refusals at its points (0: none). */
 typedef struct fl_freeze_inputs {
   const fl_int3* blocks;
   uint64_t block_count;
@@ -527,11 +527,13 @@ int fl_build_pressure_links(fl_ctx* ctx, const fl_frozen_view* view, const fl_pr
                             fl_measurement* measurement);
 
 /* ---- frozen boundary geometry ---- */
-/* FreezeCore (native/src/liquid_boundary_geometry_3d.cc:765-1068) for the absent, half-space and slab
- * obstacles: per candidate cell (strictly sorted) the liquid volume from six body-diagonal tetrahedra
- * clipped by the piecewise-linear corner distances and the obstacle planes; liquid cells (centre
- * distance < 0, volume > 0) become frozen cells in candidate order, their faces frozen faces sorted by
- * slot with open / blocked areas and the obstacle's wall velocity; cells on an open face to air get the
+/* FreezeCore (native/src/liquid_boundary_geometry_3d.cc:765-1068) for the absent, half-space, slab and
+ * open-tank obstacles and a sampled solid: per candidate cell (strictly sorted) the liquid volume from six
+ * body-diagonal tetrahedra clipped by the piecewise-linear corner distances and by the obstacle planes,
+ * the tank walls or the sampled solid's piecewise-linear surface; liquid cells (centre distance < 0,
+ * volume > 0) become frozen cells in candidate order, their faces frozen faces sorted by slot with open /
+ * blocked areas and the wall velocity (the tank's piston flux on its low-x face, the sampled solid's
+ * integrated or dense wall velocities); cells on an open face to air get the
  * exact curvature or the centred one of the 19-point centre halo; the capillary step bound
  * sqrt(rho h^3 / (4 pi sigma)) when such a cell exists (infinity otherwise), with FailedPrecondition when
  * the step exceeds it. out_cells holds cell_count records, out_faces 6 * cell_count. The first failure of

@@ -15,6 +15,7 @@
 #include <limits>
 #include <vector>
 
+#include "bounded_sums.h"
 #include "grid_internal.h"
 
 using namespace flo;
@@ -55,7 +56,8 @@ using Polyhedron = List<Polygon, 10>;
 struct Plane {
   V3 normal;
   double coordinate;
-  bool liquid;  // distance is the vertex's liquid value
+  bool liquid;    // distance is the vertex's liquid value
+  int snap = -1;  // an axis-aligned cut pins its crossings' coordinate on this axis to the plane's
 };
 
 double PlaneDistance(const Vertex& x, const Plane& p) { return p.liquid ? x.phi : Dot(p.normal, x.p) - p.coordinate; }
@@ -78,7 +80,8 @@ Polygon Clip(const Polygon& in, const Plane& plane, Polygon* crossings) {
       const double ps = dp / m, cs = dc / m;
       const double diff = ps - cs;
       const double wp = -cs / diff, wc = ps / diff;
-      const Vertex x{Add(Scale(prev.p, wp), Scale(cur.p, wc)), wp * prev.phi + wc * cur.phi};
+      Vertex x{Add(Scale(prev.p, wp), Scale(cur.p, wc)), wp * prev.phi + wc * cur.phi};
+      if (plane.snap >= 0) x.p[plane.snap] = plane.coordinate * plane.normal[plane.snap];
       out.push(x);
       if (crossings) crossings->push(x);
     }
@@ -247,8 +250,198 @@ struct Planes {
 
 V3 ToV3(flo_vec3 v) { return {v.x, v.y, v.z}; }
 
+// ---- the open tank (base/liquid_boundary_math.h:252-292; liquid_boundary_geometry_math_3d.h:420-623) ----
+
+// CheckTankStep: finite walls and piston, ordered x and z walls, a representable piston pose that stays
+// strictly inside the high-x wall.
+int CheckTankStep(const flo_tank& t, double dt) {
+  if (!std::isfinite(t.start_low_x) || !std::isfinite(t.high_x) || !std::isfinite(t.floor_y) ||
+      !std::isfinite(t.low_z) || !std::isfinite(t.high_z) || !std::isfinite(t.piston_velocity_x))
+    return FLO_INVALID_ARGUMENT;
+  if (t.start_low_x >= t.high_x || t.low_z >= t.high_z) return FLO_INVALID_ARGUMENT;
+  const double displacement = t.piston_velocity_x * dt;
+  const double end_x = std::fma(t.piston_velocity_x, dt, t.start_low_x);
+  const double start_width = t.high_x - t.start_low_x, end_width = t.high_x - end_x, width_z = t.high_z - t.low_z;
+  if (!std::isfinite(displacement) || !std::isfinite(end_x) || !std::isfinite(start_width) ||
+      !std::isfinite(end_width) || !std::isfinite(width_z))
+    return FLO_OUT_OF_RANGE;
+  if (end_x >= t.high_x) return FLO_FAILED_PRECONDITION;
+  return FLO_OK;
+}
+
+struct TankLattice {
+  double origin, index, width;
+};
+TankLattice TankAxis(const flo_geometry& g, flo_int3 c, int axis) {
+  const double o[3] = {g.world_origin.x, g.world_origin.y, g.world_origin.z};
+  const int32_t i[3] = {c.x, c.y, c.z};
+  return {o[axis], double(i[axis]), g.cell_width};
+}
+
+// TankCellHasResolvedWorldExtent: each axis's world interval is ordered, finite and resolved by the
+// spacing toward its interior at both ends.
+bool TankCellResolved(const flo_geometry& g, flo_int3 c) {
+  const double anchor[3] = {g.world_origin.x + g.cell_width * c.x, g.world_origin.y + g.cell_width * c.y,
+                            g.world_origin.z + g.cell_width * c.z};
+  for (int a = 0; a < 3; ++a) {
+    const TankLattice L = TankAxis(g, c, a);
+    const double low = anchor[a], high = L.origin + L.width * (L.index + 1);
+    if (!std::isfinite(low) || !std::isfinite(high) || low >= high) return false;
+    const double low_spacing = std::nextafter(low, high) - low;
+    const double high_spacing = high - std::nextafter(high, low);
+    if (low_spacing > L.width || high_spacing > L.width) return false;
+  }
+  return true;
+}
+
+// TankPlaneOffset: world - origin - h index as a bounded sum (in the lattice frame).
+bounded::PSum TankPlaneOffset(double world, const TankLattice& L) {
+  bounded::PSum o;
+  o.Add(world);
+  o.Add(-L.origin);
+  o.AddProduct(-L.width, L.index);
+  return o;
+}
+
+int TankPlaneOffsetSign(double world, const TankLattice& L, int& sign) {
+  const double translated = world - L.origin;
+  if (std::isinf(translated)) {
+    sign = translated > 0 ? 1 : -1;
+    return FLO_OK;
+  }
+  if (!TankPlaneOffset(world, L).bounds().sign(&sign)) return FLO_OUT_OF_RANGE;
+  return FLO_OK;
+}
+
+// TankClipCoordinate: the plane's coordinate in the unit cell, 0 or 1 when it lies beyond a corner.
+int TankClipCoordinate(double world, const TankLattice& L, double& out) {
+  const double translated = world - L.origin;
+  if (std::isinf(translated)) {
+    out = translated > 0 ? 1.0 : 0.0;
+    return FLO_OK;
+  }
+  bounded::PSum offset = TankPlaneOffset(world, L);
+  const bounded::Bounds low = offset.bounds();
+  int low_side = 0;
+  if (!low.sign(&low_side)) return FLO_OUT_OF_RANGE;
+  if (low_side <= 0) {
+    out = 0.0;
+    return FLO_OK;
+  }
+  offset.Add(-L.width);
+  int high_side = 0;
+  if (!offset.bounds().sign(&high_side)) return FLO_OUT_OF_RANGE;
+  if (high_side >= 0) {
+    out = 1.0;
+    return FLO_OK;
+  }
+  const double cell = low.estimate / L.width;
+  if (cell <= 0 || cell >= 1) return FLO_OUT_OF_RANGE;
+  out = cell;
+  return FLO_OK;
+}
+
+struct TankBounds {
+  V3 low{0, 0, 0}, high{1, 1, 1};
+};
+
+// LocalTankBounds: the tank's walls in the unit frame of the element at c (a face skips its normal axis).
+int LocalTankBounds(const flo_tank& t, const flo_geometry& g, flo_int3 c, int face_axis, TankBounds& out) {
+  const double low_world[3] = {t.start_low_x, t.floor_y, t.low_z};
+  const double high_world[3] = {t.high_x, std::numeric_limits<double>::infinity(), t.high_z};
+  TankBounds b;
+  for (int a = 0; a < 3; ++a) {
+    if (a == face_axis) continue;
+    const TankLattice L = TankAxis(g, c, a);
+    double lo = 0, hi = 0;
+    const int s1 = TankClipCoordinate(low_world[a], L, lo);
+    const int s2 = TankClipCoordinate(high_world[a], L, hi);
+    if (s1 || s2) return FLO_OUT_OF_RANGE;
+    b.low[a] = lo;
+    b.high[a] = hi;
+    if (lo == hi && lo > 0 && hi < 1) return FLO_OUT_OF_RANGE;
+  }
+  out = b;
+  return FLO_OK;
+}
+
+double PhysicalMeasure(double det, int local_exponent, double h, int dimension);
+
+// TankBoxMeasure: the product of the box extents, scaled before multiplying.
+double TankBoxMeasure(const TankBounds& b, double h, int excluded) {
+  double mantissa = 1;
+  int exponent = 0, dimension = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (a == excluded) continue;
+    const double extent = b.high[a] - b.low[a];
+    if (extent <= 0) return 0;
+    int e = 0;
+    mantissa *= std::frexp(extent, &e);
+    exponent += e;
+    ++dimension;
+  }
+  return PhysicalMeasure(mantissa, exponent, h, dimension);
+}
+
+Polyhedron ClipSolid(const Polyhedron& in, const Plane& plane);
+
+// ClipLiquidToTank: the liquid inside the tank's walls that cut the unit cell.
+Polyhedron ClipLiquidToTank(Polyhedron liquid, const TankBounds& b) {
+  for (int a = 0; a < 3; ++a) {
+    V3 n{0, 0, 0};
+    if (b.low[a] > 0) {
+      n[a] = -1;
+      liquid = ClipSolid(liquid, Plane{n, -b.low[a], false, a});
+    }
+    if (b.high[a] < 1) {
+      n[a] = 1;
+      liquid = ClipSolid(liquid, Plane{n, b.high[a], false, a});
+    }
+  }
+  return liquid;
+}
+
+// ComputeTankFace: the aperture of an open face, the piston's rectangle and flux on a closed low-x face.
+int TankFace(const flo_tank& t, const flo_geometry& g, flo_int3 l, int axis, const TankBounds& b, double& open_area,
+             double& blocked_area, double& flux, double& piston, uint8_t& low_x) {
+  const double h = g.cell_width, full_area = h * h;
+  const double low_world[3] = {t.start_low_x, t.floor_y, t.low_z};
+  const double high_world[3] = {t.high_x, std::numeric_limits<double>::infinity(), t.high_z};
+  const TankLattice L = TankAxis(g, l, axis);
+  int low_sign = 0, high_sign = 0;
+  const int s1 = TankPlaneOffsetSign(low_world[axis], L, low_sign);
+  const int s2 = TankPlaneOffsetSign(high_world[axis], L, high_sign);
+  if (s1 || s2) return FLO_OUT_OF_RANGE;
+  const bool piston_side = axis == 0 && low_sign >= 0;
+  const bool open = low_sign < 0 && high_sign > 0;
+  double o = 0, f = 0, v = 0;
+  uint8_t lx = 0;
+  if (open || piston_side) {
+    const double transverse = TankBoxMeasure(b, h, axis);
+    if (!NormalOrZero(transverse) || transverse > full_area) return FLO_OUT_OF_RANGE;
+    if (open) {
+      o = transverse;
+      for (int a = 0; a < 3; ++a)
+        if (a != axis && transverse == full_area && (b.low[a] > 0 || b.high[a] < 1)) return FLO_OUT_OF_RANGE;
+    } else if (transverse > 0) {
+      lx = 1;
+      v = t.piston_velocity_x;
+      f = transverse * v;
+      if (!NormalOrZero(f) || (v != 0 && f == 0)) return FLO_OUT_OF_RANGE;
+    }
+  }
+  const double blocked = full_area - o;
+  if (!NormalOrZero(blocked)) return FLO_OUT_OF_RANGE;
+  open_area = o;
+  blocked_area = blocked;
+  flux = f;
+  piston = v;
+  low_x = lx;
+  return FLO_OK;
+}
+
 // ValidateObstacle.
-int ValidateObstacle(const flo_obstacle& o, Planes& out) {
+int ValidateObstacle(const flo_obstacle& o, double dt, Planes& out) {
   Planes r;
   if (o.kind == FLO_OBSTACLE_HALFSPACE) {
     r = {true, false, ToV3(o.unit_normal), o.coordinate, o.coordinate, ToV3(o.wall_velocity)};
@@ -258,6 +451,8 @@ int ValidateObstacle(const flo_obstacle& o, Planes& out) {
          ToV3(o.wall_velocity)};
     if (!std::isfinite(o.coordinate)) return FLO_INVALID_ARGUMENT;
     if (!std::isfinite(r.low) || !std::isfinite(r.high) || r.low >= r.high) return FLO_OUT_OF_RANGE;
+  } else if (o.kind == FLO_OBSTACLE_TANK) {
+    if (int s = CheckTankStep(o.tank, dt)) return s;
   } else if (o.kind != FLO_OBSTACLE_ABSENT) {
     return FLO_INVALID_ARGUMENT;
   }
@@ -296,9 +491,25 @@ constexpr int kTetrahedra[6][4] = {{0, 1, 3, 7}, {0, 1, 5, 7}, {0, 2, 3, 7}, {0,
 
 // kCell: liquid volume on each side of the obstacle, by six body-diagonal tetrahedra.
 int CellVolumes(const flo_freeze_inputs& in, flo_int3 c, const double corners[8], double& low, double& high) {
+  const double h = in.geometry.cell_width;
+  const bool tank = in.obstacle.kind == FLO_OBSTACLE_TANK;
+  TankBounds tb;
+  low = high = 0;
+  if (tank) {  // the tank cell's box, or the liquid inside it when not every corner is liquid
+    if (!TankCellResolved(in.geometry, c)) return FLO_OUT_OF_RANGE;
+    if (int s = LocalTankBounds(in.obstacle.tank, in.geometry, c, -1, tb)) return s;
+    const double box = TankBoxMeasure(tb, h, -1);
+    if (!NormalOrZero(box)) return FLO_OUT_OF_RANGE;
+    if (box == 0) return FLO_OK;
+    bool all = true;
+    for (int k = 0; k < 8; ++k) all = all && corners[k] < 0;
+    if (all) {
+      high = box;
+      return FLO_OK;
+    }
+  }
   Planes P;
   if (int s = LocalPlanes(in.obstacle, in.geometry, c, P)) return s;
-  const double h = in.geometry.cell_width;
   Vertex x[8];
   for (int k = 0; k < 8; ++k) {
     for (int a = 0; a < 3; ++a) x[k].p[a] = (k >> a) & 1;
@@ -319,7 +530,9 @@ int CellVolumes(const flo_freeze_inputs& in, flo_int3 c, const double corners[8]
       tet.push(g);
     }
     const Polyhedron liquid = ClipSolid(tet, {{0, 0, 0}, 0, true});
-    if (!P.present) {
+    if (tank) {
+      high += Volume(ClipLiquidToTank(liquid, tb), h);
+    } else if (!P.present) {
       high += Volume(liquid, h);
     } else {
       high += Volume(ClipSolid(liquid, {Scale(P.normal, -1), -P.high, false}), h);
@@ -727,7 +940,7 @@ extern "C" int flo_freeze_geometry(const flo_freeze_inputs* inputs, flo_frozen_c
   if (in.has_exact_curvature && !std::isfinite(in.exact_curvature)) return FLO_INVALID_ARGUMENT;
   if (in.max_cells && in.cell_count > in.max_cells) return FLO_RESOURCE_EXHAUSTED;
   Planes obstacle;
-  if (int s = ValidateObstacle(in.obstacle, obstacle)) return s;
+  if (int s = ValidateObstacle(in.obstacle, in.step_duration, obstacle)) return s;
   const double h = in.geometry.cell_width, full_area = h * h, full_volume = full_area * h;
   if (!std::isnormal(full_area) || !std::isnormal(full_volume) || !std::isnormal(1 / in.liquid_density))
     return FLO_OUT_OF_RANGE;
@@ -804,6 +1017,18 @@ extern "C" int flo_freeze_geometry(const flo_freeze_inputs* inputs, flo_frozen_c
               faces.end());
   if (in.max_faces && faces.size() > in.max_faces) return FLO_RESOURCE_EXHAUSTED;
   for (flo_frozen_face& f : faces) {
+    if (in.obstacle.kind == FLO_OBSTACLE_TANK) {
+      TankBounds tb;
+      if (int s = LocalTankBounds(in.obstacle.tank, in.geometry, f.face_index.lattice, f.face_index.axis, tb)) return s;
+      double flux = 0, piston = 0;
+      if (int s = TankFace(in.obstacle.tank, in.geometry, f.face_index.lattice, f.face_index.axis, tb, f.open_area,
+                           f.blocked_area, flux, piston, f.is_low_x_piston_face))
+        return s;
+      f.wall_normal_velocity = piston;
+      f.prescribed_wall_flux = flux;
+      f.has_prescribed_wall_flux = 1;
+      continue;
+    }
     if (!sampled) {
       if (int s = FaceAreas(in, f.face_index.lattice, f.face_index.axis, f.open_area, f.blocked_area)) return s;
       f.wall_normal_velocity = obstacle.velocity[f.face_index.axis];

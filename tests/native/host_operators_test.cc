@@ -1024,12 +1024,28 @@ void FreezeMatchesReferenceCases() {
       {*grid, surface.candidates, surface.centers, surface.corners, {}, 0.01, 1000.0, 0.072});
   EXPECT(capillary.ok() && std::abs(capillary->cells()[0].curvature) <= 1e-14);
   EXPECT(std::abs(capillary->capillary_step_bound() - std::sqrt(1000.0 / (4.0 * std::acos(-1.0) * 0.072))) <= 1e-13);
-  // A valid open tank is refused with its own status: clipping cells against its walls is not built.
-  EXPECT(FreezeLiquidBoundaryGeometry3D({*grid, liquid.candidates, liquid.centers, liquid.corners,
-                                         AxisAlignedLiquidTank3D{-10.0, 10.0, -10.0, -10.0, 10.0, 0.0}, 0.01, 1000.0,
+  // The open tank clips the liquid below y = 0.75 to 75/256 of the cell and puts the piston's flux 35/512
+  // on the low-x face only (OpenTankPreservesNarrowAndLiquidClippedMeasures); a piston reaching the
+  // high-x wall within the step is refused.
+  const auto pool = SampleField([](double, double y, double) { return y - 0.75; });
+  const AxisAlignedLiquidTank3D tank{0.25, 2.0, 0.125, 0.125, 0.75, 0.125};
+  auto in_tank = FreezeLiquidBoundaryGeometry3D(
+      {*grid, pool.candidates, pool.centers, pool.corners, tank, 0.01, 1000.0, 0.0});
+  EXPECT(in_tank.ok() && in_tank->cells().size() == 1 &&
+         std::abs(in_tank->cells()[0].fluid_volume - 0.29296875) <= 2e-14);
+  int pistons = 0;
+  for (const auto& face : in_tank->faces()) {
+    const bool piston = face.face_index.axis == FaceNormalAxis::kX && face.face_index.lattice_index.x == -2;
+    EXPECT(face.prescribed_wall_flux.has_value() && face.is_low_x_piston_face == piston);
+    EXPECT(*face.prescribed_wall_flux == (piston ? 0.068359375 : 0.0));
+    pistons += piston ? 1 : 0;
+  }
+  EXPECT(pistons == 1);
+  EXPECT(FreezeLiquidBoundaryGeometry3D({*grid, pool.candidates, pool.centers, pool.corners,
+                                         AxisAlignedLiquidTank3D{0.0, 1.0, 0.125, 0.125, 0.75, 1.0}, 1.0, 1000.0,
                                          0.0})
              .status()
-             .code() == StatusCode::kUnimplemented);
+             .code() == StatusCode::kFailedPrecondition);
   // More than 4096 candidates are refused first.
   std::vector<CellIndex3D> many(4097, CellIndex3D{-2, -2, -2});
   EXPECT(FreezeLiquidBoundaryGeometry3D({*grid, many, liquid.centers, liquid.corners, {}, 0.0, 1000.0, -1.0})
