@@ -563,6 +563,8 @@ void fl_destroy(fl_ctx* ctx) {
   destroy_links(ctx);
   destroy_freeze(ctx);
   destroy_viscosity(ctx);
+  fl::release_field(ctx);
+  fl::release_substep(ctx);
   release(ctx);
   for (hipEvent_t e : ctx->stage)
     if (e) (void)hipEventDestroy(e);
@@ -1780,6 +1782,102 @@ int fl_project(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, 
   return fl::project_pressure(ctx, geometry, blocks, block_count, dt, cells, cell_count, faces, face_count, options,
                               out_pressures, out_velocities, out_rhs, out_residuals, out_defects, out_gauges,
                               out_diagnostics, measurement);
+}
+
+int fl_project_device_built(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
+                            double dt, const fl_pressure_cell* cells, size_t cell_count, const fl_pressure_face* faces,
+                            size_t face_count, const fl_pressure_options* options, double* out_pressures,
+                            double* out_velocities, fl_pressure_diagnostics* out_diagnostics) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  FL_TRY(check_geometry(geometry));
+  FL_TRY(check_blocks(blocks, block_count));
+  if (!options || (cell_count && !cells) || (face_count && !faces)) return FL_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  set_geometry(ctx, geometry);
+  FL_TRY(fl::build_lookup(ctx, reinterpret_cast<const int32_t*>(blocks), block_count));
+  fl_pressure_diagnostics diag{};
+  const int st = fl::project_records_device(ctx, cells, cell_count, faces, face_count, dt, *options, out_pressures,
+                                            out_velocities, &diag);
+  if (out_diagnostics) *out_diagnostics = diag;
+  return st;
+}
+
+int fl_sample_particle_field(fl_ctx* ctx, const fl_geometry* geometry, const fl_vec3* positions, size_t n,
+                             double radius, uint64_t candidate_capacity, fl_int3* out_cells, uint64_t* cell_count,
+                             fl_center_distance* out_centres, uint64_t centre_capacity, uint64_t* centre_count,
+                             fl_center_distance* out_corners, uint64_t corner_capacity, uint64_t* corner_count,
+                             fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  FL_TRY(check_geometry(geometry));
+  if (!cell_count || !centre_count || !corner_count || (n && !positions)) return FL_INVALID_ARGUMENT;
+  if (!std::isfinite(radius) || radius <= 0) return FL_INVALID_ARGUMENT;
+  if (n > ctx->cap.particles) return FL_RESOURCE_EXHAUSTED;
+  (void)hipSetDevice(ctx->device);
+  set_geometry(ctx, geometry);
+  double copy_ms = 0;
+  {
+    Timer t(ctx);
+    if (n) FL_HIP(hipMemcpyAsync(ctx->staging_aos, positions, n * sizeof(fl_vec3), hipMemcpyHostToDevice, ctx->stream));
+    copy_ms += t.stop();
+  }
+  Timer t(ctx);
+  if (n) {
+    k_split_columns<<<grid_for(n), kThreads, 0, ctx->stream>>>(reinterpret_cast<const double*>(ctx->staging_aos), n, 3,
+                                                              3, ctx->raw.base, ctx->raw.capacity);
+    FL_HIP(hipGetLastError());
+  }
+  ctx->raw.trial_aliases_sorted = true;
+  // SampleWorldBatch rejects non-finite particles as InvalidArgument before any sample.
+  FL_TRY(bin_particles(ctx, ctx->raw.view(), n, false, false));
+  uint64_t word = 0;
+  FL_TRY(fl::read_error(ctx, &word));
+  if (word != ~uint64_t(0)) {
+    const int code = static_cast<int>(word & 0xff);
+    return code == FL_OUT_OF_RANGE ? FL_OUT_OF_RANGE : FL_INVALID_ARGUMENT;
+  }
+  if (ctx->bin_box.volume() > ctx->cap.lookup_blocks) return FL_RESOURCE_EXHAUSTED;
+  FL_TRY(sort_particles(ctx, n, ctx->raw.view(), true, 0x7u));
+  fl::FieldSamples F{};
+  FL_TRY(fl::sample_particle_field(ctx, radius, candidate_capacity, &F));
+  const double kernel_ms = t.stop();
+  *cell_count = F.cell_count;
+  *centre_count = F.centre_count;
+  *corner_count = F.corner_count;
+  if (!out_cells || !out_centres || !out_corners) {
+    fill_measurement(ctx, measurement, kernel_ms, copy_ms, 12);
+    return FL_OK;
+  }
+  if (F.cell_count > candidate_capacity || F.centre_count > centre_capacity || F.corner_count > corner_capacity)
+    return FL_RESOURCE_EXHAUSTED;
+  Timer c(ctx);
+  hipStream_t s = ctx->stream;
+  if (F.cell_count) FL_HIP(hipMemcpyAsync(out_cells, F.cells, F.cell_count * sizeof(fl_int3), hipMemcpyDeviceToHost, s));
+  if (F.centre_count)
+    FL_HIP(hipMemcpyAsync(out_centres, F.centres, F.centre_count * sizeof(fl_center_distance), hipMemcpyDeviceToHost, s));
+  if (F.corner_count)
+    FL_HIP(hipMemcpyAsync(out_corners, F.corners, F.corner_count * sizeof(fl_center_distance), hipMemcpyDeviceToHost, s));
+  FL_HIP(hipStreamSynchronize(s));
+  copy_ms += c.stop();
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, 12);
+  return FL_OK;
+}
+
+int fl_substep(fl_ctx* ctx, const fl_substep_options* options, fl_substep_report* report) {
+  if (!ctx || ctx->poisoned || !ctx->scene_loaded) return FL_FAILED_PRECONDITION;
+  if (!options || !report) return FL_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  fl_substep_report r{};
+  const int st = fl::substep(ctx, *options, r);
+  r.status = st;
+  *report = r;
+  return st;
+}
+
+int fl_substep_download(fl_ctx* ctx, double* positions_xyz, double* velocities_xyz, double* slopes_rowmajor,
+                        double* volume_ratios, uint8_t* flags, size_t n) {
+  if (!ctx || ctx->poisoned || !ctx->scene_loaded) return FL_FAILED_PRECONDITION;
+  (void)hipSetDevice(ctx->device);
+  return fl::substep_download(ctx, positions_xyz, velocities_xyz, slopes_rowmajor, volume_ratios, flags, n);
 }
 
 }  // extern "C"

@@ -165,6 +165,8 @@ struct fl_ctx {
   void* links = nullptr;
   void* freeze = nullptr;
   void* viscosity = nullptr;  // ViscosityBuffers (capi.hip)
+  void* field = nullptr;      // FieldBuffers (field.hip)
+  void* substep = nullptr;    // SubstepBuffers (substep.hip)
 };
 
 namespace fl {
@@ -321,9 +323,55 @@ int freeze_curvature_device(fl_ctx* ctx, const FreezeDevice& d, uint64_t nc, boo
 int build_links_device(fl_ctx* ctx, const LinkDevice& d, fl_pressure_cell* out_cells, fl_pressure_face* out_faces,
                        uint64_t* d_lower, bool validate_only);
 
+// The particle-union liquid field (field.hip) of the sorted particles: candidate cells, and centre and
+// corner samples with their signed distances, all (z,y,x) sorted device arrays owned by the context.
+struct FieldSamples {
+  const fl_int3* cells;
+  uint64_t cell_count;
+  fl_center_distance* centres;
+  uint64_t centre_count;
+  fl_center_distance* corners;
+  uint64_t corner_count;
+  fl_int3 cell_box_lo, cell_box_hi;  // a box holding every sample
+};
+int sample_particle_field(fl_ctx* ctx, double radius, uint64_t max_candidates, FieldSamples* out);
+void release_field(fl_ctx* ctx);
+
+// Projection of device link records on the context's grid (pressure.hip): the system and hierarchy
+// built on the device, solved with the V-cycle PCG, corrected velocities scattered into unew_slots.
+int project_links_device(fl_ctx* ctx, const fl_pressure_cell* cells, uint64_t nc, const fl_pressure_face* faces,
+                         uint64_t nf, double dt, const fl_pressure_options& opt, double* unew_slots,
+                         fl_pressure_diagnostics* diag);
+int last_pressures(fl_ctx* ctx, const double** p, uint64_t* n);
+
+// The device substep (substep.hip).
+int substep(fl_ctx* ctx, const fl_substep_options& o, fl_substep_report& rep);
+int substep_download(fl_ctx* ctx, double* positions, double* velocities, double* slopes, double* ratios,
+                     uint8_t* flags, uint64_t n);
+void release_substep(fl_ctx* ctx);
+int project_records_device(fl_ctx* ctx, const fl_pressure_cell* cells, uint64_t nc, const fl_pressure_face* faces,
+                           uint64_t nf, double dt, const fl_pressure_options& opt, double* out_pressures,
+                           double* out_velocities, fl_pressure_diagnostics* diag);
+void substep_scene_uploaded(fl_ctx* ctx);
+
 // Particle update without an obstacle (update.hip): in place on device records, first failure into ctx->err.
 int update_particles(fl_ctx* ctx, const fl_particle_update_options& o, const fl_boundary_gather* gathered,
                      const double* distances, uint64_t n, fl_liquid_particle* particles);
+// Particle columns of the device substep, in sort order; ordinal (optional) maps a row to the
+// particle's ordinal in the step's input order for error words.
+struct ParticleColumns {
+  double* x[3];
+  double* v[3];
+  double* c[9];
+  double* volume_ratio;
+  uint8_t* flags;
+  const uint32_t* ordinal;
+};
+// The update of the columns with GatherMac3D results (24 component-major columns of capacity gcap)
+// and one liquid-field value phi for every particle; first failure into ctx->err; the largest updated
+// speed's bit pattern into *max_speed (atomic max, optional).
+int update_columns(fl_ctx* ctx, const fl_particle_update_options& o, uint64_t n, const ParticleColumns& P,
+                   const double* gather, uint64_t gcap, double phi, unsigned long long* max_speed);
 
 // Small helpers (topology.hip)
 // Variational MAC viscosity (viscosity.hip): device arrays of one strain assembly or velocity-block solve.

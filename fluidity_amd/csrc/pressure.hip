@@ -140,6 +140,19 @@ struct Pressure {
   bool has_jump = false;
   double full_area = 0, full_volume = 0;
   double scale_coarse = 0.5;
+  // Scratch of device-built systems (prepare_system_device): level-0 block marks and their scan over
+  // the grid's blocks, the face of each owned slot, the union-find forest and anchors of the cells,
+  // small counters, a block lookup and unknown-cell masks per level, and the block keys of every level.
+  uint64_t* dv_mark = nullptr;       // 2 * cap_dv_boxes
+  int32_t* dv_slot_face = nullptr;   // 192 * cap_dv_grid
+  int32_t* dv_parent = nullptr;      // cap_dv_cells
+  int32_t* dv_root = nullptr;        // cap_dv_cells
+  uint8_t* dv_anchor = nullptr;      // cap_dv_cells
+  uint64_t* dv_words = nullptr;      // 8 counters
+  int32_t* dv_lut = nullptr;         // cap_dv_boxes
+  uint64_t* dv_occ[2] = {nullptr, nullptr};  // cap_dv_grid each
+  int32_t* dv_keys = nullptr;        // 3 * 2 * cap_dv_grid
+  uint64_t cap_dv_grid = 0, cap_dv_cells = 0, cap_dv_boxes = 0;
 };
 
 template <typename T>
@@ -1744,6 +1757,10 @@ void destroy_pressure(fl_ctx* ctx) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (P->hscal) (void)hipHostFree(P->hscal);
+  void* dv[] = {P->dv_mark, P->dv_slot_face, P->dv_parent, P->dv_root, P->dv_anchor, P->dv_words, P->dv_lut,
+                P->dv_occ[0], P->dv_occ[1], P->dv_keys};
+  for (void* p : dv)
+    if (p) (void)hipFree(p);
   delete P;
   ctx->pressure = nullptr;
 }
@@ -2321,6 +2338,697 @@ int project_pressure(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* bl
   return FL_OK;
 }
 
+// ---------------------------------------------------------------- device-built pressure systems
+//
+// prepare_system and build_levels on the device, for link records that are already device arrays
+// (the typed links of a frozen geometry). Each phase's checks are the host's; their failures carry the
+// phase in the error word above the ordinal (cells, incidence, faces, the cells' and then the faces'
+// right-hand-side representability), so the first failure keeps the host's phase order. Connected
+// components over the positive internal links come from a lock-free union-find that always hooks the
+// larger root under the smaller, so every component's root is its least cell, as on the host; when a
+// component has no anchor (no compliance and no Dirichlet link), its compatibility sum and gauge are
+// left to the host's code on downloaded records. The hierarchy is built level by level from per-block
+// masks of unknown cells: the coarse blocks of a level are marked in a lookup over its block box and
+// numbered by a scan in (z,y,x) order, and coarsening stops where the host's would (no fewer cells,
+// the pool, or the requested depth).
+namespace {
+
+constexpr uint64_t kSysPhase = uint64_t(1) << 40;
+
+__device__ inline void sys_error(uint64_t* err, int phase, uint64_t ordinal, int code) {
+  perr_report(err, uint64_t(phase) * kSysPhase + ordinal, code);
+}
+__device__ inline bool rep_dev(double a, double b, double p) {
+  if (!finite(p)) return false;
+  if (a == 0.0 || b == 0.0 || fabs(p) >= 2.2250738585072014e-308) return true;
+  if (p == 0.0) return false;
+  int ea = 0, eb = 0;
+  const double ma = frexp(a, &ea), mb = frexp(b, &eb);
+  return fma(ma, mb, -scalbn(p, -(ea + eb))) == 0.0;
+}
+__device__ inline bool fpos(double v) { return finite(v) && v > 0.0; }
+
+struct SysGrid {
+  const int32_t* lut;  // grid block ordinal over box, -1 inactive
+  Box box;
+  const uint64_t* scan0;  // level-0 index of a marked grid block
+  const uint64_t* mark0;  // 1 when the grid block holds a cell
+  __device__ int64_t ordinal(int64_t x, int64_t y, int64_t z) const {
+    if (x < kI32Min || x > kI32Max || y < kI32Min || y > kI32Max || z < kI32Min || z > kI32Max) return -1;
+    const int64_t i = box.index(int32_t(x), int32_t(y), int32_t(z));
+    return i < 0 ? -1 : lut[i];
+  }
+  // Input ordinal of the cell at lattice index (x, y, z), or -1.
+  __device__ int32_t cell(const int32_t* fine_cell, int64_t x, int64_t y, int64_t z) const {
+    const int64_t bx = floor_div4(x), by = floor_div4(y), bz = floor_div4(z);
+    const int64_t g = ordinal(bx, by, bz);
+    if (g < 0 || !mark0[g]) return -1;
+    const int l = int((z - 4 * bz) * 16 + (y - 4 * by) * 4 + (x - 4 * bx));
+    return fine_cell[int64_t(scan0[g]) * 64 + cslot(l)];
+  }
+};
+
+__global__ void k_sys_cells(const fl_pressure_cell* cells, uint64_t nc, SysGrid G, double volume, double dt,
+                            int check_values, double* V, double* S, double* K, double* prev, double* inw,
+                            double* comp, double* target, uint64_t* mark, uint64_t* words, uint64_t* err) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= nc) return;
+  const fl_pressure_cell c = cells[i];
+  int code = 0;
+  if (i) {
+    const fl_int3 a = cells[i - 1].cell_index, b = c.cell_index;
+    const bool less = a.z != b.z ? a.z < b.z : (a.y != b.y ? a.y < b.y : a.x < b.x);
+    if (!less) code = FL_INVALID_ARGUMENT;
+  }
+  double cmp = 0.0, tf = 0.0;
+  if (!code && (!fpos(c.fluid_volume) || c.fluid_volume > volume)) code = FL_INVALID_ARGUMENT;
+  if (!code && (!finite(c.target_divergence) || !finite(c.previous_pressure) || !finite(c.integrated_inward_wall_flux)))
+    code = FL_INVALID_ARGUMENT;
+  if (!code && c.bulk_modulus != __builtin_inf()) {
+    if (!fpos(c.bulk_modulus)) {
+      code = FL_INVALID_ARGUMENT;
+    } else {
+      const double kdt = c.bulk_modulus * dt;
+      if (!fpos(kdt)) code = FL_OUT_OF_RANGE;
+      if (!code) {
+        cmp = c.fluid_volume / kdt;
+        if (!fpos(cmp)) code = FL_OUT_OF_RANGE;
+      }
+    }
+  }
+  if (!code) {
+    tf = c.fluid_volume * c.target_divergence;
+    if (!rep_dev(c.fluid_volume, c.target_divergence, tf)) code = FL_OUT_OF_RANGE;
+  }
+  const int64_t xyz[3] = {c.cell_index.x, c.cell_index.y, c.cell_index.z};
+  for (int a = 0; a < 3 && !code; ++a) {
+    if (xyz[a] == kI32Max) {
+      code = FL_OUT_OF_RANGE;
+      break;
+    }
+    for (int side = 0; side < 2; ++side) {
+      int64_t l[3] = {xyz[0], xyz[1], xyz[2]};
+      l[a] += side;
+      if (G.ordinal(floor_div4(l[0]), floor_div4(l[1]), floor_div4(l[2])) < 0) {
+        code = FL_FAILED_PRECONDITION;
+        break;
+      }
+    }
+  }
+  if (code) {
+    sys_error(err, 0, i, code);
+    return;
+  }
+  V[i] = c.fluid_volume;
+  S[i] = c.target_divergence;
+  K[i] = c.bulk_modulus;
+  prev[i] = c.previous_pressure;
+  inw[i] = c.integrated_inward_wall_flux;
+  comp[i] = cmp;
+  target[i] = tf;
+  mark[G.ordinal(floor_div4(xyz[0]), floor_div4(xyz[1]), floor_div4(xyz[2]))] = 1;
+  if (c.previous_pressure != 0.0) atomicOr(reinterpret_cast<unsigned long long*>(words + 1), 1ull);
+  if (check_values && !rep_dev(cmp, c.previous_pressure, cmp * c.previous_pressure)) sys_error(err, 3, i, FL_OUT_OF_RANGE);
+}
+
+__global__ void k_sys_keys0(const uint64_t* mark, const uint64_t* scan, uint64_t nb, const int32_t* blocks,
+                            int32_t* keys) {
+  const uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (g >= nb || !mark[g]) return;
+  const uint64_t b = scan[g];
+  for (int a = 0; a < 3; ++a) keys[3 * b + a] = blocks[3 * g + a];
+}
+
+__global__ void k_sys_fine(const fl_pressure_cell* cells, uint64_t nc, SysGrid G, int32_t* cell_fine,
+                           int32_t* fine_cell) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= nc) return;
+  const fl_int3 ci = cells[i].cell_index;
+  const int64_t bx = floor_div4(ci.x), by = floor_div4(ci.y), bz = floor_div4(ci.z);
+  const int64_t g = G.ordinal(bx, by, bz);
+  const int l = int((ci.z - 4 * bz) * 16 + (ci.y - 4 * by) * 4 + (ci.x - 4 * bx));
+  const int64_t fi = int64_t(G.scan0[g]) * 64 + cslot(l);
+  cell_fine[i] = int32_t(fi);
+  fine_cell[fi] = int32_t(i);
+}
+
+// FreezeLinkFromIncidence (mac_pressure_3d_arithmetic.h:174-245) per face, with the face's slot, its
+// endpoints and the exact-incidence checks; cond goes to kcond for the components.
+__global__ void k_sys_faces(const fl_pressure_face* faces, uint64_t nf, SysGrid G, const int32_t* fine_cell,
+                            double area, double dt, int check_values, int32_t* tail_out, int32_t* head_out,
+                            int32_t* kind_out, uint64_t* slot_out, double* fa, double* fb, double* fd, double* fu,
+                            double* fw, double* ff, int32_t* fh, double* fl, double* kcond, int32_t* slot_face,
+                            uint64_t* words, uint64_t* err) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= nf) return;
+  const fl_pressure_face f = faces[i];
+  auto slot_of = [&](const fl_pressure_face& q, uint64_t& slot) -> int {
+    if (q.axis < 0 || q.axis > 2) return FL_INVALID_ARGUMENT;
+    const int64_t x = q.lattice.x, y = q.lattice.y, z = q.lattice.z;
+    const int64_t bx = floor_div4(x), by = floor_div4(y), bz = floor_div4(z);
+    const int64_t g = G.ordinal(bx, by, bz);
+    if (g < 0) return FL_FAILED_PRECONDITION;
+    slot = uint64_t(g) * 192 + uint64_t(q.axis) * 64 + uint64_t(((z - 4 * bz) * 4 + (y - 4 * by)) * 4 + (x - 4 * bx));
+    return 0;
+  };
+  uint64_t slot = 0;
+  int code = slot_of(f, slot);
+  if (code) {
+    sys_error(err, 1, i, code);
+    return;
+  }
+  if (i) {
+    uint64_t before = 0;
+    if (slot_of(faces[i - 1], before) || before >= slot) {
+      sys_error(err, 1, i, FL_INVALID_ARGUMENT);
+      return;
+    }
+  }
+  int64_t t[3] = {f.lattice.x, f.lattice.y, f.lattice.z};
+  t[f.axis] -= 1;
+  const int32_t to = G.cell(fine_cell, t[0], t[1], t[2]);
+  const int32_t ho = G.cell(fine_cell, f.lattice.x, f.lattice.y, f.lattice.z);
+  if (to < 0 && ho < 0) {  // not incident to any cell: the face list is not the exact incident union
+    sys_error(err, 1, i, FL_INVALID_ARGUMENT);
+    return;
+  }
+  if (!finite(f.open_area) || f.open_area < 0.0 || f.open_area > area) code = FL_INVALID_ARGUMENT;
+  if (!code && (!finite(f.inverse_density) || !finite(f.pressure_gradient_distance) ||
+                !finite(f.predicted_normal_velocity) || !finite(f.wall_normal_velocity)))
+    code = FL_INVALID_ARGUMENT;
+  const bool both = to >= 0 && ho >= 0, one = (to >= 0) != (ho >= 0);
+  if (!code) {
+    if (f.link_kind == 1) {
+      if (!both || !finite(f.lift)) code = FL_INVALID_ARGUMENT;
+    } else if (f.link_kind == 2) {
+      if (!one || f.open_area == 0.0 || !finite(f.lift)) code = FL_INVALID_ARGUMENT;
+    } else if (f.link_kind == 0) {
+      if (f.open_area != 0.0) code = FL_INVALID_ARGUMENT;
+    } else {
+      code = FL_INVALID_ARGUMENT;
+    }
+  }
+  double scale = 0.0, cond = 0.0, wf = 0.0;
+  if (!code && f.open_area > 0.0) {
+    if (f.inverse_density <= 0.0 || f.pressure_gradient_distance <= 0.0) {
+      code = FL_INVALID_ARGUMENT;
+    } else {
+      scale = (dt * f.inverse_density) / f.pressure_gradient_distance;
+      cond = f.open_area * scale;
+      if (!fpos(scale) || !fpos(cond)) code = FL_OUT_OF_RANGE;
+    }
+  }
+  if (!code) {
+    const double blocked = area - f.open_area;
+    if (f.has_prescribed_wall_flux) {
+      wf = f.prescribed_wall_flux;
+      if (!finite(wf) || (blocked == 0 && wf != 0)) code = FL_INVALID_ARGUMENT;
+    } else {
+      wf = blocked * f.wall_normal_velocity;
+      if (!rep_dev(blocked, f.wall_normal_velocity, wf)) code = FL_OUT_OF_RANGE;
+    }
+  }
+  if (code) {
+    sys_error(err, 2, i, code);
+    return;
+  }
+  tail_out[i] = to;
+  head_out[i] = ho;
+  kind_out[i] = f.link_kind;
+  slot_out[i] = slot;
+  fa[i] = f.open_area;
+  fb[i] = f.inverse_density;
+  fd[i] = f.pressure_gradient_distance;
+  fu[i] = f.predicted_normal_velocity;
+  fw[i] = f.wall_normal_velocity;
+  ff[i] = f.prescribed_wall_flux;
+  fh[i] = f.has_prescribed_wall_flux;
+  fl[i] = f.link_kind == 0 ? 0.0 : f.lift;
+  kcond[i] = cond;
+  slot_face[slot] = int32_t(i);
+  if (cond > 0.0 && f.link_kind == 1 && f.lift != 0.0) atomicOr(reinterpret_cast<unsigned long long*>(words), 1ull);
+  if (check_values) {
+    if (!rep_dev(f.open_area, f.predicted_normal_velocity, f.open_area * f.predicted_normal_velocity) ||
+        ((f.link_kind == 1 || f.link_kind == 2) && !rep_dev(cond, f.lift, cond * f.lift)))
+      sys_error(err, 4, i, FL_OUT_OF_RANGE);
+  }
+}
+
+// The six incident faces of each cell (in the host's order: -x, +x, -y, +y, -z, +z).
+__global__ void k_sys_cell_faces(const fl_pressure_cell* cells, uint64_t nc, SysGrid G, const int32_t* slot_face,
+                                 int32_t* cell_faces, uint64_t* err) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= nc) return;
+  const fl_int3 ci = cells[i].cell_index;
+  const int64_t xyz[3] = {ci.x, ci.y, ci.z};
+  for (int a = 0; a < 3; ++a)
+    for (int side = 0; side < 2; ++side) {
+      int64_t l[3] = {xyz[0], xyz[1], xyz[2]};
+      l[a] += side;
+      const int64_t bx = floor_div4(l[0]), by = floor_div4(l[1]), bz = floor_div4(l[2]);
+      const int64_t g = G.ordinal(bx, by, bz);
+      const uint64_t slot = uint64_t(g) * 192 + uint64_t(a) * 64 +
+                            uint64_t(((l[2] - 4 * bz) * 4 + (l[1] - 4 * by)) * 4 + (l[0] - 4 * bx));
+      const int32_t f = slot_face[slot];
+      if (f < 0) {
+        sys_error(err, 1, i, FL_INVALID_ARGUMENT);
+        return;
+      }
+      cell_faces[6 * i + 2 * a + side] = f;
+    }
+}
+
+__device__ inline int32_t uf_find(int32_t* parent, int32_t x) {
+  volatile int32_t* p = parent;
+  while (true) {
+    const int32_t up = p[x];
+    if (up == x) return x;
+    const int32_t next = p[up];
+    if (next != up) atomicCAS(&parent[x], up, next);  // path halving; a lost race only shortens less
+    x = up;
+  }
+}
+
+__global__ void k_uf_init(int32_t* parent, uint64_t nc) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i < nc) parent[i] = int32_t(i);
+}
+
+// Unites the endpoints of every positive internal link: the larger root is hooked under the smaller.
+__global__ void k_uf_union(const double* kcond, const int32_t* tail, const int32_t* head, uint64_t nf,
+                           int32_t* parent) {
+  const uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (f >= nf || !(kcond[f] > 0.0) || tail[f] < 0 || head[f] < 0) return;
+  int32_t a = tail[f], b = head[f];
+  while (true) {
+    a = uf_find(parent, a);
+    b = uf_find(parent, b);
+    if (a == b) return;
+    if (a < b) {
+      const int32_t t = a;
+      a = b;
+      b = t;
+    }
+    if (atomicCAS(&parent[a], a, b) == a) return;
+  }
+}
+
+__global__ void k_uf_roots(int32_t* parent, uint64_t nc, const double* comp, int32_t* root, uint8_t* anchor) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= nc) return;
+  const int32_t r = uf_find(parent, int32_t(i));
+  root[i] = r;
+  if (comp[i] > 0.0) anchor[r] = 1;
+}
+
+__global__ void k_uf_dirichlet(const int32_t* kind, const int32_t* tail, const int32_t* head, uint64_t nf,
+                               const int32_t* root, uint8_t* anchor) {
+  const uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (f >= nf || kind[f] != 2) return;
+  if (tail[f] >= 0) anchor[root[tail[f]]] = 1;
+  if (head[f] >= 0) anchor[root[head[f]]] = 1;
+}
+
+// Unanchored components into words[2]; every cell an unknown, the warm start its previous pressure.
+__global__ void k_uf_finish(const int32_t* root, const uint8_t* anchor, uint64_t nc, const double* prev,
+                            uint8_t* flag, double* p_init, uint64_t* words) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= nc) return;
+  if (root[i] == int32_t(i) && !anchor[i]) atomicAdd(reinterpret_cast<unsigned long long*>(words + 2), 1ull);
+  flag[i] = 1;
+  p_init[i] = prev[i];
+}
+
+// Unknown-cell mask of each level-0 block, bit l = z * 16 + y * 4 + x.
+__global__ void k_occ0(const int32_t* fine_cell, const uint8_t* flag, uint64_t nb, uint64_t* occ) {
+  const uint64_t b = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (b >= nb) return;
+  uint64_t m = 0;
+  for (int s = 0; s < 64; ++s) {
+    const int32_t c = fine_cell[b * 64 + s];
+    if (c >= 0 && flag[c] == 1) m |= uint64_t(1) << cnat(s);
+  }
+  occ[b] = m;
+}
+
+__host__ __device__ inline int32_t half_floor(int32_t v) { return v >= 0 ? v / 2 : -((-(v + 1)) / 2) - 1; }
+
+// Marks the coarse blocks of the blocks that hold unknown cells (mark[box index] = 1).
+__global__ void k_coarse_mark(const int32_t* keys, const uint64_t* occ, uint64_t nb, Box box, uint64_t* mark) {
+  const uint64_t b = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (b >= nb || !occ[b]) return;
+  mark[box.index(half_floor(keys[3 * b]), half_floor(keys[3 * b + 1]), half_floor(keys[3 * b + 2]))] = 1;
+}
+
+// Coarse lookup and keys from the marks and their scan.
+__global__ void k_coarse_lut(const uint64_t* mark, const uint64_t* scan, Box box, int32_t* lut, int32_t* keys) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= box.volume()) return;
+  if (!mark[i]) {
+    lut[i] = -1;
+    return;
+  }
+  const uint64_t c = scan[i];
+  lut[i] = int32_t(c);
+  const uint64_t x = i % box.dim[0], y = (i / box.dim[0]) % box.dim[1], z = i / (uint64_t(box.dim[0]) * box.dim[1]);
+  keys[3 * c] = box.lo[0] + int32_t(x);
+  keys[3 * c + 1] = box.lo[1] + int32_t(y);
+  keys[3 * c + 2] = box.lo[2] + int32_t(z);
+}
+
+// Parent (coarse block, octant) of every fine block whose coarse key exists, and the coarse masks.
+__global__ void k_coarse_link(const int32_t* keys, const uint64_t* occ, uint64_t nb, Box box, const int32_t* lut,
+                              int32_t* parent, uint64_t* occ_c) {
+  const uint64_t b = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (b >= nb) return;
+  const int32_t k[3] = {keys[3 * b], keys[3 * b + 1], keys[3 * b + 2]};
+  const int32_t pk[3] = {half_floor(k[0]), half_floor(k[1]), half_floor(k[2])};
+  const int64_t bi = box.index(pk[0], pk[1], pk[2]);
+  const int32_t c = bi < 0 ? -1 : lut[bi];
+  const int o[3] = {k[0] - 2 * pk[0], k[1] - 2 * pk[1], k[2] - 2 * pk[2]};
+  if (parent) {
+    parent[2 * b] = c;
+    parent[2 * b + 1] = c < 0 ? -1 : (o[2] * 2 + o[1]) * 2 + o[0];
+  }
+  uint64_t m = occ[b];
+  if (c < 0 || !m) return;
+  uint64_t out = 0;
+  while (m) {
+    const int l = __ffsll(static_cast<long long>(m)) - 1;
+    m &= m - 1;
+    const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
+    const int px = (o[0] * 4 + lx) >> 1, py = (o[1] * 4 + ly) >> 1, pz = (o[2] * 4 + lz) >> 1;
+    out |= uint64_t(1) << (pz * 16 + py * 4 + px);
+  }
+  atomicOr(reinterpret_cast<unsigned long long*>(occ_c + c), static_cast<unsigned long long>(out));
+}
+
+__global__ void k_popcount_sum(const uint64_t* occ, uint64_t nb, uint64_t* out) {
+  uint64_t s = 0;
+  for (uint64_t b = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; b < nb; b += uint64_t(gridDim.x) * blockDim.x)
+    s += __popcll(occ[b]);
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+  if ((threadIdx.x & 63) == 0 && s) atomicAdd(reinterpret_cast<unsigned long long*>(out), static_cast<unsigned long long>(s));
+}
+
+// Six-neighbour table of a level from its lookup.
+__global__ void k_level_nbr(const int32_t* keys, uint64_t nb, Box box, const int32_t* lut, int32_t* nbr) {
+  const uint64_t b = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (b >= nb) return;
+  const int d[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  for (int dir = 0; dir < 6; ++dir) {
+    const int64_t i = box.index(keys[3 * b] + d[dir][0], keys[3 * b + 1] + d[dir][1], keys[3 * b + 2] + d[dir][2]);
+    nbr[6 * b + dir] = i < 0 ? -1 : lut[i];
+  }
+}
+
+// Level-0 lookup over the grid's box: the level-0 index of each marked grid block.
+__global__ void k_lut0(const int32_t* grid_lut, uint64_t vol, const uint64_t* mark, const uint64_t* scan, int32_t* lut) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= vol) return;
+  const int32_t g = grid_lut[i];
+  lut[i] = g >= 0 && mark[g] ? int32_t(scan[g]) : -1;
+}
+
+__global__ void k_children(const int32_t* parent, uint64_t nb, int32_t* child) {
+  const uint64_t b = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (b >= nb || parent[2 * b] < 0) return;
+  child[8 * parent[2 * b] + parent[2 * b + 1]] = int32_t(b);
+}
+
+// Size of a marked set: last scan entry plus its mark.
+__global__ void k_totals_mark(const uint64_t* mark, const uint64_t* scan, uint64_t n, uint64_t* out) {
+  if (threadIdx.x == 0) out[0] = n ? scan[n - 1] + (mark[n - 1] ? 1 : 0) : 0;
+}
+
+__global__ void k_fill_words(uint64_t* p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = 0;
+}
+__global__ void k_fill_int(int32_t* p, uint64_t n, int32_t v) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_byte(uint8_t* p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = 0;
+}
+
+inline unsigned capped(uint64_t n) { return static_cast<unsigned>(std::min<uint64_t>(grid_for(n), 65536)); }
+
+int ensure_device_scratch(fl_ctx* ctx, Pressure& P) {
+  const uint64_t grid = std::max<uint64_t>(ctx->cap.blocks, 1), cells = std::max<uint64_t>(ctx->cap.pressure_cells, 1);
+  const uint64_t boxes = std::max<uint64_t>({ctx->cap.lookup_blocks, grid, 64});
+  if (P.dv_mark && grid <= P.cap_dv_grid && cells <= P.cap_dv_cells && boxes <= P.cap_dv_boxes) return FL_OK;
+  void* dv[] = {P.dv_mark, P.dv_slot_face, P.dv_parent, P.dv_root, P.dv_anchor, P.dv_words, P.dv_lut,
+                P.dv_occ[0], P.dv_occ[1], P.dv_keys};
+  for (void* p : dv)
+    if (p) (void)hipFree(p);
+  FL_TRY(dal(&P.dv_mark, 2 * boxes));
+  FL_TRY(dal(&P.dv_slot_face, 192 * grid));
+  FL_TRY(dal(&P.dv_parent, cells));
+  FL_TRY(dal(&P.dv_root, cells));
+  FL_TRY(dal(&P.dv_anchor, cells));
+  FL_TRY(dal(&P.dv_words, 8));
+  FL_TRY(dal(&P.dv_lut, boxes));
+  FL_TRY(dal(&P.dv_occ[0], grid));
+  FL_TRY(dal(&P.dv_occ[1], grid));
+  FL_TRY(dal(&P.dv_keys, 6 * grid));
+  P.cap_dv_grid = grid;
+  P.cap_dv_cells = cells;
+  P.cap_dv_boxes = boxes;
+  return FL_OK;
+}
+
+// Reads n device words into ctx->host_words, with the error word after them.
+int read_words(fl_ctx* ctx, const uint64_t* d, int n) {
+  FL_HIP(hipMemcpyAsync(ctx->host_words, d, n * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  FL_HIP(hipMemcpyAsync(ctx->host_words + n, ctx->err, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  FL_HIP(hipStreamSynchronize(ctx->stream));
+  return FL_OK;
+}
+
+// build_levels on the device: level 0 is the nb0 blocks whose keys are in P.dv_keys.
+int build_levels_device(fl_ctx* ctx, Pressure& P, uint64_t nb0, int levels_wanted) {
+  hipStream_t s = ctx->stream;
+  free_levels(P);
+  P.levels_requested = levels_wanted;
+  uint64_t planned = level_bytes(nb0);
+  if (planned > P.pool_bytes) return FL_RESOURCE_EXHAUSTED;
+  // Level 0: lookup over the grid box, masks of unknown cells.
+  Box box = ctx->block_box;
+  const uint64_t vol0 = box.volume();
+  if (vol0 > P.cap_dv_boxes) return FL_RESOURCE_EXHAUSTED;
+  k_lut0<<<grid_for(vol0), kThreads, 0, s>>>(ctx->lut, vol0, P.dv_mark, P.dv_mark + P.cap_dv_boxes, P.dv_lut);
+  k_occ0<<<grid_for(nb0), kThreads, 0, s>>>(P.fine_cell, P.cell_flag, nb0, P.dv_occ[0]);
+  FL_HIP(hipGetLastError());
+  auto add_level = [&](uint64_t nb, const int32_t* keys) -> int {
+    Level L;
+    L.nb = nb;
+    L.nbr = carve<int32_t>(P, 6 * nb);
+    L.child = P.levels.empty() ? nullptr : carve<int32_t>(P, 8 * nb);
+    L.parent = carve<int32_t>(P, 2 * nb);
+    const uint64_t n = nb * 64;
+    L.x = carve<float>(P, n);
+    L.r = carve<float>(P, n);
+    L.diagb = carve<float>(P, n);
+    L.b = carve<float>(P, n);
+    L.anchor = carve<float>(P, n);
+    L.diag = carve<float>(P, n);
+    L.kb = carve<uint16_t>(P, 3 * n);
+    L.k = carve<float>(P, 3 * n);
+    if (P.pool_used > P.pool_bytes) return FL_INTERNAL;
+    k_level_nbr<<<grid_for(nb), kThreads, 0, s>>>(keys, nb, box, P.dv_lut, L.nbr);
+    FL_HIP(hipGetLastError());
+    P.levels.push_back(L);
+    return FL_OK;
+  };
+  const int32_t* keys = P.dv_keys;
+  FL_TRY(add_level(nb0, keys));
+  uint64_t cells = 0;
+  {
+    k_fill_words<<<1, 64, 0, s>>>(P.dv_words + 4, 1);
+    k_popcount_sum<<<capped(nb0), kThreads, 0, s>>>(P.dv_occ[0], nb0, P.dv_words + 4);
+    FL_TRY(read_words(ctx, P.dv_words + 4, 1));
+    cells = ctx->host_words[0];
+  }
+  int cur = 0;
+  uint64_t nb = nb0;
+  int32_t* next_keys = P.dv_keys + 3 * nb0;
+  while (static_cast<int>(P.levels.size()) < levels_wanted && cells > 0) {
+    Box cbox;
+    for (int a = 0; a < 3; ++a) {
+      const int32_t lo = half_floor(box.lo[a]), hi = half_floor(box.lo[a] + box.dim[a] - 1);
+      cbox.lo[a] = lo;
+      cbox.dim[a] = hi - lo + 1;
+    }
+    const uint64_t cvol = cbox.volume();
+    if (next_keys + 3 * nb > P.dv_keys + 6 * P.cap_dv_grid) break;  // room for at most nb coarse keys
+    uint64_t* mark = P.dv_mark;
+    uint64_t* scan = P.dv_mark + P.cap_dv_boxes;
+    k_fill_words<<<capped(cvol), kThreads, 0, s>>>(mark, cvol);
+    k_coarse_mark<<<grid_for(nb), kThreads, 0, s>>>(keys, P.dv_occ[cur], nb, cbox, mark);
+    FL_HIP(hipGetLastError());
+    FL_TRY(exclusive_scan_u64(ctx, mark, scan, cvol));
+    // Coarse lookup (replaces the fine one: the fine level's neighbours are already built).
+    k_coarse_lut<<<grid_for(cvol), kThreads, 0, s>>>(mark, scan, cbox, P.dv_lut, next_keys);
+    k_fill_words<<<capped(nb), kThreads, 0, s>>>(P.dv_occ[1 - cur], std::max<uint64_t>(nb, 1));
+    k_fill_words<<<1, 64, 0, s>>>(P.dv_words + 4, 2);
+    k_totals_mark<<<1, 64, 0, s>>>(mark, scan, cvol, P.dv_words + 5);
+    k_coarse_link<<<grid_for(nb), kThreads, 0, s>>>(keys, P.dv_occ[cur], nb, cbox, P.dv_lut,
+                                                    P.levels.back().parent, P.dv_occ[1 - cur]);
+    FL_HIP(hipGetLastError());
+    // The coarse block count first, then its cell count.
+    FL_TRY(read_words(ctx, P.dv_words + 5, 1));
+    const uint64_t nbc = ctx->host_words[0];
+    k_popcount_sum<<<capped(nbc), kThreads, 0, s>>>(P.dv_occ[1 - cur], nbc, P.dv_words + 4);
+    FL_TRY(read_words(ctx, P.dv_words + 4, 1));
+    const uint64_t ccells = ctx->host_words[0];
+    if (ccells == cells) break;
+    if (planned + level_bytes(nbc) > P.pool_bytes) break;  // the pool bounds the depth
+    planned += level_bytes(nbc);
+    box = cbox;
+    FL_TRY(add_level(nbc, next_keys));
+    keys = next_keys;
+    next_keys = const_cast<int32_t*>(keys) + 3 * nbc;
+    nb = nbc;
+    cells = ccells;
+    cur = 1 - cur;
+  }
+  // The coarsest level has no parent; every coarser level's children from the parents below it.
+  P.levels.back().parent = nullptr;
+  for (size_t l = 1; l < P.levels.size(); ++l) {
+    Level& C = P.levels[l];
+    k_fill_int<<<capped(8 * C.nb), kThreads, 0, s>>>(C.child, 8 * C.nb, -1);
+    k_children<<<grid_for(P.levels[l - 1].nb), kThreads, 0, s>>>(P.levels[l - 1].parent, P.levels[l - 1].nb, C.child);
+  }
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+}  // namespace
+
+// prepare_system on device link records: the grid is the context's (ctx->blocks and its lookup).
+// Returns the system's status; a component without an anchor sends the records to the host's
+// prepare_system.
+int prepare_system_device(fl_ctx* ctx, Pressure& P, const fl_pressure_cell* cells, uint64_t nc,
+                          const fl_pressure_face* faces, uint64_t nf, double dt, int levels, bool* host_fallback) {
+  *host_fallback = false;
+  hipStream_t s = ctx->stream;
+  if (nc == 0 || nf == 0 || !finite_positive(dt)) return FL_INVALID_ARGUMENT;
+  const double h = ctx->geometry.h, area = h * h, volume = area * h;
+  if (!finite_positive(area) || !finite_positive(volume)) return FL_OUT_OF_RANGE;
+  if (nc > ctx->cap.pressure_cells || nf > ctx->cap.pressure_faces) return FL_RESOURCE_EXHAUSTED;
+  if (nc > 0x7fffffffULL || nf > 0x7fffffffULL) return FL_RESOURCE_EXHAUSTED;
+  FL_TRY(ensure_capacity(ctx, nc, nf, 64 * ctx->nb));
+  FL_TRY(ensure_device_scratch(ctx, P));
+  const uint64_t ngrid = ctx->nb;
+  if (ngrid > P.cap_dv_grid) return FL_RESOURCE_EXHAUSTED;
+  P.full_area = area;
+  P.full_volume = volume;
+  uint64_t* mark = P.dv_mark;
+  uint64_t* scan = P.dv_mark + P.cap_dv_boxes;
+  FL_TRY(reset_error(ctx));
+  k_fill_words<<<capped(ngrid), kThreads, 0, s>>>(mark, ngrid);
+  k_fill_words<<<1, 64, 0, s>>>(P.dv_words, 8);
+  SysGrid G{ctx->lut, ctx->block_box, scan, mark};
+  k_sys_cells<<<grid_for(nc), kThreads, 0, s>>>(cells, nc, G, volume, dt, 1, P.cell_volume, P.cell_s, P.cell_K,
+                                                P.cell_prev, P.cell_inwall, P.cell_comp, P.cell_target, mark,
+                                                P.dv_words, ctx->err);
+  FL_HIP(hipGetLastError());
+  FL_TRY(exclusive_scan_u64(ctx, mark, scan, ngrid));
+  k_totals_mark<<<1, 64, 0, s>>>(mark, scan, ngrid, P.dv_words + 3);
+  FL_TRY(read_words(ctx, P.dv_words + 3, 1));
+  const uint64_t nb0 = ctx->host_words[0];
+  if (ctx->host_words[1] != ~uint64_t(0) && (ctx->host_words[1] >> 8) < kSysPhase) return int(ctx->host_words[1] & 0xff);
+  const uint64_t nfine = nb0 * 64;
+  if (nfine > P.cap_fine) return FL_RESOURCE_EXHAUSTED;
+  k_sys_keys0<<<grid_for(ngrid), kThreads, 0, s>>>(mark, scan, ngrid, ctx->blocks, P.dv_keys);
+  k_fill_int<<<capped(nfine), kThreads, 0, s>>>(P.fine_cell, nfine, -1);
+  k_fill_int<<<capped(192 * ngrid), kThreads, 0, s>>>(P.dv_slot_face, 192 * ngrid, -1);
+  k_sys_fine<<<grid_for(nc), kThreads, 0, s>>>(cells, nc, G, P.cell_fine, P.fine_cell);
+  k_sys_faces<<<grid_for(nf), kThreads, 0, s>>>(faces, nf, G, P.fine_cell, area, dt, 1, P.face_tail, P.face_head,
+                                                P.face_kind, P.face_slot, P.face_area, P.face_beta, P.face_dist,
+                                                P.face_ustar, P.face_uwall, P.face_flux, P.face_has_flux, P.face_lift,
+                                                P.face_k, P.dv_slot_face, P.dv_words, ctx->err);
+  k_sys_cell_faces<<<grid_for(nc), kThreads, 0, s>>>(cells, nc, G, P.dv_slot_face, P.cell_faces, ctx->err);
+  k_uf_init<<<grid_for(nc), kThreads, 0, s>>>(P.dv_parent, nc);
+  k_fill_byte<<<capped(nc), kThreads, 0, s>>>(P.dv_anchor, nc);
+  k_uf_union<<<grid_for(nf), kThreads, 0, s>>>(P.face_k, P.face_tail, P.face_head, nf, P.dv_parent);
+  k_uf_roots<<<grid_for(nc), kThreads, 0, s>>>(P.dv_parent, nc, P.cell_comp, P.dv_root, P.dv_anchor);
+  k_uf_dirichlet<<<grid_for(nf), kThreads, 0, s>>>(P.face_kind, P.face_tail, P.face_head, nf, P.dv_root, P.dv_anchor);
+  k_uf_finish<<<grid_for(nc), kThreads, 0, s>>>(P.dv_root, P.dv_anchor, nc, P.cell_prev, P.cell_flag, P.p_init,
+                                                P.dv_words);
+  FL_HIP(hipGetLastError());
+  FL_TRY(read_words(ctx, P.dv_words, 3));
+  if (ctx->host_words[3] != ~uint64_t(0)) return int(ctx->host_words[3] & 0xff);
+  P.has_jump = ctx->host_words[0] != 0;
+  P.p_init_zero = ctx->host_words[1] == 0;
+  P.nc = nc;
+  P.nf = nf;
+  P.h_gauges.clear();
+  P.h_cell_comp_root.clear();
+  P.h_unknown_cells.clear();
+  P.ntiles = 0;  // the learned preconditioner's tiles are not built for device systems
+  if (ctx->host_words[2] != 0) {
+    *host_fallback = true;
+    return FL_OK;
+  }
+  return build_levels_device(ctx, P, nb0, levels);
+}
+
+// The projection of device link records on the context's grid (the typed links of a frozen geometry):
+// the system and its hierarchy built on the device, frozen for dt, solved, and the corrected face
+// velocities written into unew_slots at the faces' owned slots (other slots untouched).
+int project_links_device(fl_ctx* ctx, const fl_pressure_cell* cells, uint64_t nc, const fl_pressure_face* faces,
+                         uint64_t nf, double dt, const fl_pressure_options& opt, double* unew_slots,
+                         fl_pressure_diagnostics* out_diag) {
+  FL_TRY(check_options(&opt));
+  if (opt.preconditioner != 0) return FL_FAILED_PRECONDITION;
+  if (!ctx->pressure) ctx->pressure = new Pressure();
+  Pressure& P = *state(ctx);
+  P.scale_coarse = opt.coarse_scale;
+  bool fallback = false;
+  FL_TRY(prepare_system_device(ctx, P, cells, nc, faces, nf, dt, opt.levels, &fallback));
+  if (fallback) {
+    // A component without an anchor: its compatibility test and gauge on the host, over the records.
+    hipStream_t s = ctx->stream;
+    std::vector<fl_pressure_cell> hc(nc);
+    std::vector<fl_pressure_face> hf(nf);
+    std::vector<fl_int3> hb(ctx->nb);
+    FL_HIP(hipMemcpyAsync(hc.data(), cells, nc * sizeof(fl_pressure_cell), hipMemcpyDeviceToHost, s));
+    FL_HIP(hipMemcpyAsync(hf.data(), faces, nf * sizeof(fl_pressure_face), hipMemcpyDeviceToHost, s));
+    if (ctx->nb) FL_HIP(hipMemcpyAsync(hb.data(), ctx->blocks, ctx->nb * sizeof(fl_int3), hipMemcpyDeviceToHost, s));
+    FL_HIP(hipStreamSynchronize(s));
+    const fl_geometry g{{ctx->geometry.ox, ctx->geometry.oy, ctx->geometry.oz}, ctx->geometry.h};
+    FL_TRY(prepare_system(ctx, P, &g, hb.data(), hb.size(), dt, hc.data(), nc, hf.data(), nf, true));
+    std::vector<Key3> b0 = P.levels[0].keys;
+    FL_TRY(build_levels(ctx, P, b0, opt.levels));
+  }
+  Gates gates{};
+  EvalResult e0{};
+  fl_pressure_diagnostics diag{};
+  int st = freeze_and_assemble(ctx, P, dt, opt, gates, e0, diag);
+  if (st == FL_OK) st = solve(ctx, P, opt, gates, e0, diag);
+  if (out_diag) *out_diag = diag;
+  if (st != FL_OK) return st;
+  hipStream_t s = ctx->stream;
+  k_face_velocity<<<grid_for(P.nf), kThreads, 0, s>>>(P.nf, P.face_tail, P.face_head, P.face_kind, P.face_area,
+                                                      P.face_ustar, P.face_scale, P.face_lift, P.p, P.face_u);
+  k_scatter_slots<<<grid_for(P.nf), kThreads, 0, s>>>(P.nf, P.face_slot, P.face_u, unew_slots);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+// Pressures of the last device projection, in cell order (device array and count).
+int last_pressures(fl_ctx* ctx, const double** p, uint64_t* n) {
+  Pressure* P = state(ctx);
+  if (!P) return FL_FAILED_PRECONDITION;
+  *p = P->p;
+  *n = P->nc;
+  return FL_OK;
+}
+
 }  // namespace fl
 
 // ---------------------------------------------------------------- device-resident liquid step
@@ -2393,6 +3101,7 @@ int fl_scene_upload(fl_ctx* ctx, const fl_geometry* geometry, size_t n, const do
   FL_HIP(hipStreamSynchronize(s));
   ctx->raw.trial_aliases_sorted = true;
   ctx->scene_n = n;
+  substep_scene_uploaded(ctx);
   ctx->scene_mass = particle_mass;
   ctx->scene_loaded = true;
   ctx->scene_ready = false;

@@ -323,6 +323,107 @@ static_assert(kGroup * sizeof(fl_liquid_particle) % 16 == 0 && kGroup * sizeof(f
               "16-byte blocks");
 
 // kObstacle: the half-space / slab / tank instance; the obstacle-free instance carries no contact code.
+// One particle's update: ASFLIP stored velocity v = new PIC + alpha (v - old PIC), the surface-expansion
+// advecting velocity (blend beta where the velocity diverges at the liquid surface away from the solid),
+// the obstacle's contact or the plain advance, the half-cell guard and J exp(dt div u).
+struct UpdateIn {
+  fl_vec3 x, new_pic, minus_old_pic;
+  double divergence, phi, volume_ratio;
+  bool pic_initialized, fallback;
+};
+template <bool kObstacle>
+__device__ inline int update_core(const fl_particle_update_options& o, const UpdateIn& in, fl_vec3& xn, fl_vec3& vn,
+                                  double& j) {
+  const double alpha = in.pic_initialized && !in.fallback ? 0.99 * o.particle_energy_blend : 0.0;
+  const fl_vec3 np = in.new_pic, d = in.minus_old_pic;
+  const fl_vec3 stored{np.x + alpha * d.x, np.y + alpha * d.y, np.z + alpha * d.z};
+  const double divergence = in.divergence;
+  const double solid = kObstacle ? solid_distance(o.obstacle, in.x) : kInf;
+  const double phi = in.phi;
+  const bool surface = divergence > 0 && phi <= 0 && phi >= -o.cell_width && solid > o.cell_width;
+  const double beta = surface ? o.particle_energy_blend : 0.0;
+  const double ba = beta * alpha;
+  const fl_vec3 adv{np.x + ba * d.x, np.y + ba * d.y, np.z + ba * d.z};
+  const fl_vec3 x = in.x;
+  int code = kOk;
+  if (isnan(solid)) code = kOutOfRange;
+  else if (!finite3(x) || !finite3(stored) || !finite3(adv)) code = kInvalidArgument;
+  else if (!finite(o.step_duration) || o.step_duration <= 0) code = kInvalidArgument;
+  else if (!finite(o.friction_coefficient) || o.friction_coefficient < 0 || !finite(o.penetration_tolerance) ||
+           o.penetration_tolerance < 0)
+    code = kInvalidArgument;
+  xn = fl_vec3{0.0, 0.0, 0.0};
+  vn = stored;
+  j = 0.0;
+  if (code == kOk) {
+    if (!kObstacle) {
+      xn = advance(x, adv, o.step_duration);
+      if (!finite3(xn)) code = kOutOfRange;
+    } else if (o.obstacle.kind == FL_OBSTACLE_TANK) {
+      code = tank_contact(o, x, stored, adv, xn, vn);
+    } else {
+      code = plane_contact(o, x, stored, adv, xn, vn);
+    }
+  }
+  if (code == kOk) {
+    const double m = fmax(fmax(fabs(xn.x - x.x), fabs(xn.y - x.y)), fabs(xn.z - x.z));
+    if (m > 0.5 * o.cell_width) code = kAborted;
+  }
+  if (code == kOk) {
+    j = in.volume_ratio * exp(o.step_duration * divergence);
+    if (!finite(j) || j <= 0) code = kAborted;
+  }
+  return code;
+}
+
+// The same update on particle columns in sort order (the device substep): positions, velocities and
+// slopes in place, the gather results of GatherMac3D (24 component-major columns: new PIC, increment,
+// fitted slopes, gradient; old PIC = new PIC - increment), the volume ratio and a per-particle flag
+// byte (bit 0: PIC initialised). phi is the liquid field at every particle.
+template <bool kObstacle>
+__global__ void k_update_columns(fl_particle_update_options o, uint64_t n, ParticleColumns P, const double* gather,
+                                 uint64_t gcap, double phi, uint64_t* err, unsigned long long* max_speed) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  double speed = 0.0;
+  if (i < n) {
+    const fl_vec3 np{gather[i], gather[gcap + i], gather[2 * gcap + i]};
+    const fl_vec3 inc{gather[3 * gcap + i], gather[4 * gcap + i], gather[5 * gcap + i]};
+    const fl_vec3 v{P.v[0][i], P.v[1][i], P.v[2][i]};
+    const fl_vec3 old{np.x - inc.x, np.y - inc.y, np.z - inc.z};
+    const uint8_t flag = P.flags[i];
+    const UpdateIn in{fl_vec3{P.x[0][i], P.x[1][i], P.x[2][i]},
+                      np,
+                      fl_vec3{v.x - old.x, v.y - old.y, v.z - old.z},
+                      gather[15 * gcap + i] + gather[19 * gcap + i] + gather[23 * gcap + i],
+                      phi,
+                      P.volume_ratio[i],
+                      (flag & 1) != 0,
+                      false};
+    fl_vec3 xn, vn;
+    double j = 0.0;
+    const int code = update_core<kObstacle>(o, in, xn, vn, j);
+    if (code != kOk) {
+      atomicMin(reinterpret_cast<unsigned long long*>(err),
+                static_cast<unsigned long long>(error_word(P.ordinal ? P.ordinal[i] : i, code)));
+    } else {
+      P.x[0][i] = xn.x;
+      P.x[1][i] = xn.y;
+      P.x[2][i] = xn.z;
+      P.v[0][i] = vn.x;
+      P.v[1][i] = vn.y;
+      P.v[2][i] = vn.z;
+      for (int k = 0; k < 9; ++k) P.c[k][i] = gather[(6 + k) * gcap + i];
+      P.volume_ratio[i] = j;
+      P.flags[i] = static_cast<uint8_t>(flag | 1);
+      speed = sqrt(vn.x * vn.x + vn.y * vn.y + vn.z * vn.z);
+    }
+  }
+  // Largest speed of the updated particles (non-negative doubles order like their bit patterns).
+  for (int off = 32; off > 0; off >>= 1) speed = fmax(speed, __shfl_xor(speed, off));
+  if ((threadIdx.x & 63) == 0 && max_speed)
+    atomicMax(max_speed, static_cast<unsigned long long>(__double_as_longlong(speed)));
+}
+
 template <bool kObstacle>
 __global__ __launch_bounds__(kGroup) void k_update_particles(fl_particle_update_options o,
                                                              const fl_boundary_gather* gathered,
@@ -354,46 +455,17 @@ __global__ __launch_bounds__(kGroup) void k_update_particles(fl_particle_update_
     const uint64_t i = i0 + lane;
     const fl_boundary_gather& g = sg[lane];
     fl_liquid_particle& p = sp[lane];
-    const bool fallback = g.old_pic_fallback || g.new_pic_fallback || g.low_mass_fallback;
-    const double alpha = p.pic_initialized && !fallback ? 0.99 * o.particle_energy_blend : 0.0;
-    const fl_vec3 np = g.new_pic_velocity, d = g.particle_minus_old_pic_velocity;
-    const fl_vec3 stored{np.x + alpha * d.x, np.y + alpha * d.y, np.z + alpha * d.z};
-    const double divergence =
-        g.velocity_gradient.x_row.x + g.velocity_gradient.y_row.y + g.velocity_gradient.z_row.z;
-    const double solid = kObstacle ? solid_distance(o.obstacle, p.world_position) : kInf;
-    const double phi = distances[i];
-    const bool surface = divergence > 0 && phi <= 0 && phi >= -o.cell_width && solid > o.cell_width;
-    const double beta = surface ? o.particle_energy_blend : 0.0;
-    const double ba = beta * alpha;
-    const fl_vec3 adv{np.x + ba * d.x, np.y + ba * d.y, np.z + ba * d.z};
-    const fl_vec3 x = p.world_position;
-    int code = kOk;
-    if (isnan(solid)) code = kOutOfRange;
-    else if (!finite3(x) || !finite3(stored) || !finite3(adv)) code = kInvalidArgument;
-    else if (!finite(o.step_duration) || o.step_duration <= 0) code = kInvalidArgument;
-    else if (!finite(o.friction_coefficient) || o.friction_coefficient < 0 || !finite(o.penetration_tolerance) ||
-             o.penetration_tolerance < 0)
-      code = kInvalidArgument;
-    fl_vec3 xn{0.0, 0.0, 0.0}, vn = stored;
+    const UpdateIn in{p.world_position,
+                      g.new_pic_velocity,
+                      g.particle_minus_old_pic_velocity,
+                      g.velocity_gradient.x_row.x + g.velocity_gradient.y_row.y + g.velocity_gradient.z_row.z,
+                      distances[i],
+                      p.volume_ratio,
+                      p.pic_initialized != 0,
+                      g.old_pic_fallback || g.new_pic_fallback || g.low_mass_fallback};
+    fl_vec3 xn, vn;
     double j = 0.0;
-    if (code == kOk) {
-      if (!kObstacle) {
-        xn = advance(x, adv, o.step_duration);
-        if (!finite3(xn)) code = kOutOfRange;
-      } else if (o.obstacle.kind == FL_OBSTACLE_TANK) {
-        code = tank_contact(o, x, stored, adv, xn, vn);
-      } else {
-        code = plane_contact(o, x, stored, adv, xn, vn);
-      }
-    }
-    if (code == kOk) {
-      const double m = fmax(fmax(fabs(xn.x - x.x), fabs(xn.y - x.y)), fabs(xn.z - x.z));
-      if (m > 0.5 * o.cell_width) code = kAborted;
-    }
-    if (code == kOk) {
-      j = p.volume_ratio * exp(o.step_duration * divergence);
-      if (!finite(j) || j <= 0) code = kAborted;
-    }
+    const int code = update_core<kObstacle>(o, in, xn, vn, j);
     if (code != kOk) {
       atomicMin(reinterpret_cast<unsigned long long*>(err), static_cast<unsigned long long>(error_word(i, code)));
     } else {
@@ -430,6 +502,19 @@ int update_particles(fl_ctx* ctx, const fl_particle_update_options& o, const fl_
   else
     k_update_particles<true><<<grid_for(n, kGroup), kGroup, 0, ctx->stream>>>(o, gathered, distances, n, particles,
                                                                             ctx->err);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+int update_columns(fl_ctx* ctx, const fl_particle_update_options& o, uint64_t n, const ParticleColumns& P,
+                   const double* gather, uint64_t gcap, double phi, unsigned long long* max_speed) {
+  if (!n) return FL_OK;
+  if (o.obstacle.kind == FL_OBSTACLE_ABSENT)
+    k_update_columns<false><<<grid_for(n), kThreads, 0, ctx->stream>>>(o, n, P, gather, gcap, phi, ctx->err,
+                                                                      max_speed);
+  else
+    k_update_columns<true><<<grid_for(n), kThreads, 0, ctx->stream>>>(o, n, P, gather, gcap, phi, ctx->err,
+                                                                     max_speed);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }

@@ -54,7 +54,8 @@ EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_reserve", "fl_bin_so
            "fl_normalize_forces", "fl_apply_forces", "fl_g2p", "fl_project", "fl_scene_upload", "fl_scene_prepare", "fl_step",
            "fl_scene_download", "fl_probe_copy", "fl_learn_param_count", "fl_learn_setup", "fl_learn_set_params", "fl_learn_get_params",
            "fl_learn_forward", "fl_learn_infer", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect", "fl_learn_eval",
-           "fl_learn_train", "fl_normalize_forces_static", "fl_time_projection_kernels", "fl_assemble_full_cell_strain", "fl_solve_viscous_block")
+           "fl_learn_train", "fl_normalize_forces_static", "fl_time_projection_kernels", "fl_assemble_full_cell_strain", "fl_solve_viscous_block", "fl_substep",
+           "fl_substep_download", "fl_sample_particle_field", "fl_project_device_built")
 
 _lib = None
 
@@ -432,6 +433,49 @@ class Context:
         return s, out
 
 
+    def project_device_built(self, geometry, blocks, dt, cells, faces, options):
+        """fl_project_device_built: (status, pressures, face velocities, diagnostics)."""
+        p, u = np.zeros(len(cells)), np.zeros(len(faces))
+        diag = np.zeros(1, R.PRESSURE_DIAGNOSTICS)
+        s = lib().fl_project_device_built(self.handle, _p(geometry), _p(blocks), C.c_size_t(len(blocks)),
+                                          C.c_double(dt), _p(cells), C.c_size_t(len(cells)), _p(faces),
+                                          C.c_size_t(len(faces)), _p(options), _p(p), _p(u), _p(diag))
+        return s, p, u, diag[0]
+
+    def sample_particle_field(self, geometry, positions, radius, capacity=1 << 24):
+        """fl_sample_particle_field: (status, candidate cells, centre samples, corner samples)."""
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        nc, ncen, ncor = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        s = lib().fl_sample_particle_field(self.handle, _p(geometry), _p(pos), C.c_size_t(len(pos)), C.c_double(radius),
+                                           C.c_uint64(capacity), None, C.byref(nc), None, C.c_uint64(0),
+                                           C.byref(ncen), None, C.c_uint64(0), C.byref(ncor), C.byref(self.last))
+        if s != OK:
+            return s, None, None, None
+        cells = np.zeros(max(nc.value, 1), R.INT3)
+        cen = np.zeros(max(ncen.value, 1), R.CENTER_DISTANCE)
+        cor = np.zeros(max(ncor.value, 1), R.CENTER_DISTANCE)
+        s = lib().fl_sample_particle_field(self.handle, _p(geometry), _p(pos), C.c_size_t(len(pos)), C.c_double(radius),
+                                           C.c_uint64(capacity), _p(cells), C.byref(nc), _p(cen), C.c_uint64(len(cen)),
+                                           C.byref(ncen), _p(cor), C.c_uint64(len(cor)), C.byref(ncor),
+                                           C.byref(self.last))
+        if s != OK:
+            return s, None, None, None
+        return s, cells[:nc.value], cen[:ncen.value], cor[:ncor.value]
+
+    def substep(self, options):
+        """fl_substep on the uploaded scene: (status, report record)."""
+        rep = np.zeros(1, R.SUBSTEP_REPORT)
+        o = np.ascontiguousarray(options, dtype=R.SUBSTEP_OPTIONS)
+        s = lib().fl_substep(self.handle, _p(o), _p(rep))
+        return s, rep[0]
+
+    def substep_download(self, n):
+        """fl_substep_download: (status, positions, velocities, slopes, volume ratios, flags)."""
+        x, v, c, j, f = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 9)), np.zeros(n), np.zeros(n, np.uint8)
+        s = lib().fl_substep_download(self.handle, _p(x), _p(v), _p(c), _p(j), _p(f), C.c_size_t(n))
+        return s, x, v, c, j, f
+
+
 class Network:
     """The learned preconditioner of a context (fl_learn_*): FP32 parameters, training pairs, training."""
 
@@ -504,6 +548,28 @@ def pressure_options(max_iterations=256, tolerance_scale=1.0, relative_target=0.
     o = np.zeros(1, R.PRESSURE_OPTIONS)
     o[0] = (max_iterations, tolerance_scale, relative_target, levels, sweeps, coarse_sweeps, 2 if bf16_levels else 1,
             coarse_scale, 1 if learned else 0, int(smoother))
+    return o
+
+
+def substep_options(dt, gravity=(0.0, -9.81, 0.0), mass_q=2.0 ** -40, mom_q=2.0 ** -40, capacity=1 << 20,
+                    minimum_face_mass=0.0, field_radius=1.0, density=1000.0, surface_tension=0.0, ambient=0.0,
+                    blend=1.0, friction=0.0, penetration=0.0, obstacle=None, pressure=None):
+    """fl_substep_options record (obstacle: an OBSTACLE record; pressure: a pressure_options record)."""
+    o = np.zeros(1, R.SUBSTEP_OPTIONS)
+    o["step_duration"] = dt
+    o["gravity"] = tuple(gravity)
+    o["quantization"] = (mass_q, mom_q, capacity)
+    o["minimum_face_mass"] = minimum_face_mass
+    o["field_radius"] = field_radius
+    o["liquid_density"] = density
+    o["surface_tension"] = surface_tension
+    o["ambient_pressure"] = ambient
+    o["particle_energy_blend"] = blend
+    o["friction_coefficient"] = friction
+    o["penetration_tolerance"] = penetration
+    if obstacle is not None:
+        o["obstacle"] = obstacle
+    o["pressure"] = pressure_options() if pressure is None else pressure
     return o
 
 

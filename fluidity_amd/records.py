@@ -99,6 +99,19 @@ OBSTACLE = np.dtype([("kind", "u4"), ("pad", "u4"), ("unit_normal", VEC3), ("coo
 UPDATE_OPTIONS = np.dtype([("cell_width", "f8"), ("step_duration", "f8"), ("particle_energy_blend", "f8"),
                            ("friction_coefficient", "f8"), ("penetration_tolerance", "f8"), ("obstacle", OBSTACLE)])
 TOPOLOGY_CAPACITIES = np.dtype([("particles", "u8"), ("blocks", "u8"), ("cells", "u8"), ("faces", "u8")])
+# fl_substep_options / fl_substep_report (the device substep).
+SUBSTEP_OPTIONS = np.dtype([("step_duration", "f8"), ("gravity", VEC3), ("quantization", QUANTIZATION),
+                            ("minimum_face_mass", "f8"), ("field_radius", "f8"), ("liquid_density", "f8"),
+                            ("surface_tension", "f8"), ("ambient_pressure", "f8"), ("particle_energy_blend", "f8"),
+                            ("friction_coefficient", "f8"), ("penetration_tolerance", "f8"), ("obstacle", OBSTACLE),
+                            ("pressure", PRESSURE_OPTIONS)])
+SUBSTEP_REPORT = np.dtype([("status", "i4"), ("stage", "i4"), ("error_ordinal", "u8"), ("particles", "u8"), ("blocks", "u8"),
+                           ("candidates", "u8"), ("centre_samples", "u8"), ("corner_samples", "u8"),
+                           ("frozen_cells", "u8"), ("frozen_faces", "u8"), ("lower_theta_count", "u8"),
+                           ("capillary_step_bound", "f8"), ("pressure", PRESSURE_DIAGNOSTICS), ("sort_ms", "f8"),
+                           ("field_ms", "f8"), ("deposit_ms", "f8"), ("freeze_ms", "f8"), ("links_ms", "f8"),
+                           ("projection_ms", "f8"), ("gather_ms", "f8"), ("update_ms", "f8"), ("total_ms", "f8"),
+                           ("max_speed", "f8")])
 
 SEALED, INTERNAL, DIRICHLET = 0, 1, 2
 LINK_KINDS = {"sealed": SEALED, "internal": INTERNAL, "dirichlet": DIRICHLET}
@@ -108,6 +121,7 @@ assert PRESSURE_CELL.itemsize == 56 and PRESSURE_FACE.itemsize == 80 and FACE_QU
 assert BLOCK_STATE.itemsize == 32 and CELL_STATE.itemsize == 24 and FACE_STATE.itemsize == 56
 assert LIQUID_PARTICLE.itemsize == 328 and UPDATE_OPTIONS.itemsize == 160 and OBSTACLE.itemsize == 120
 assert FROZEN_FACE.itemsize == 72 and FROZEN_CELL.itemsize == 40 and CELL_FIELDS.itemsize == 40
+assert SUBSTEP_OPTIONS.itemsize == 296 and SUBSTEP_REPORT.itemsize == 264
 
 # MacViscosityCell3D, MacStrainTerm3D, MacStrainRow3D (native/include/fluidity/host/viscosity.h:47-66).
 VISCOSITY_CELL = np.dtype({"names": ["cell_index", "dynamic_viscosity"], "formats": [INT3, "f8"], "offsets": [0, 16],

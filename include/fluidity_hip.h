@@ -667,6 +667,81 @@ int fl_scene_download(fl_ctx* ctx, fl_int3* blocks, size_t block_capacity, fl_fa
                       double* ustar, double* unew, fl_sort_entry* order, fl_gather_result* gather,
                       double* pressures);
 
+/* ProjectMacPressure3D as fl_project (same records, checks, solver and results), with the system's
+ * structure, gauges and multigrid hierarchy built on the device from the uploaded records instead of
+ * on the host: the projection path of fl_substep. A component without an anchor (no compliance, no
+ * Dirichlet link) has its compatibility test and gauge computed on the host. The learned preconditioner
+ * is not available on this path (FailedPrecondition). */
+int fl_project_device_built(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
+                            double dt, const fl_pressure_cell* cells, size_t cell_count, const fl_pressure_face* faces,
+                            size_t face_count, const fl_pressure_options* options, double* out_pressures,
+                            double* out_velocities, fl_pressure_diagnostics* out_diagnostics);
+
+/* The particle-union liquid field (LiquidParticleSphereUnion3D) of PrepareField and SampleWorldBatch
+ * (native/src/liquid_substep_3d_staging.cc:759-1000; native/src/liquid_substep_3d_cpu.cc:77-120):
+ * the candidate cells (every cell of floor((x -/+ r - o) / h) on each axis around every particle),
+ * the centre samples (27 cells around each candidate) and corner samples (8 corners of each candidate),
+ * each unique and (z,y,x) sorted, with FieldDistance = min over the particles of hypot(sample - x) - r
+ * at every sample. Counts are always written; arrays only when all three are given and large enough.
+ * More candidates than candidate_capacity is ResourceExhausted. */
+int fl_sample_particle_field(fl_ctx* ctx, const fl_geometry* geometry, const fl_vec3* positions, size_t n,
+                             double radius, uint64_t candidate_capacity, fl_int3* out_cells, uint64_t* cell_count,
+                             fl_center_distance* out_centres, uint64_t centre_capacity, uint64_t* centre_count,
+                             fl_center_distance* out_corners, uint64_t corner_capacity, uint64_t* corner_count,
+                             fl_measurement* measurement);
+
+/* ---- device-resident substep over moving particles ----
+ * One liquid substep on the uploaded scene in the stage order of AttemptImpl
+ * (native/src/liquid_substep_3d.cc:1175-2820; the backend members it chains are topology, deposit,
+ * normalize, forces, freeze, links, project, gather and update_particles of LiquidSubstepBackend3D,
+ * native/include/fluidity/host/liquid_substep_backend_3d.h:47-260, and the field sampling of
+ * PrepareField, native/src/liquid_substep_3d_staging.cc:759-1000, for LiquidParticleSphereUnion3D):
+ * stable bin sort, the particle-union field (candidate cells, centre and corner samples, signed
+ * distances), activation, deposit quanta, normalize + gravity, the frozen boundary geometry with the
+ * obstacle, typed pressure links (incompressible cells of one density at rest, the predicted face
+ * velocities u*), the MGPCG projection on a device-built system and hierarchy, GatherMac3D of the old
+ * and corrected face velocities, and the particle update (ASFLIP with the energy blend, the surface
+ * gate with the field value -r at each particle, the obstacle's contact, J). The updated particles,
+ * in this substep's sort order, are the scene's particles for the next call (fl_substep_download).
+ * Statuses are the failing stage's; report.stage names it (1 sort, 2 field and activation, 3 deposit
+ * and forces, 4 freeze, 5 links, 6 projection, 7 gather, 8 update). A failed call leaves the scene's
+ * particles as they were, so a caller can retry with a shorter step (the update's Aborted on a move
+ * beyond half a cell is the reference scheduler's cue to do so). No reference counterpart exists
+ * for the options record as a whole: it gathers the scene and options fields those stages read. */
+typedef struct fl_substep_options {
+  double step_duration;
+  fl_vec3 gravity;
+  fl_quantization quantization;
+  double minimum_face_mass;
+  double field_radius; /* LiquidParticleSphereUnion3D::radius, metres */
+  double liquid_density;
+  double surface_tension;
+  double ambient_pressure;
+  double particle_energy_blend;
+  double friction_coefficient;
+  double penetration_tolerance;
+  fl_obstacle obstacle; /* absent, half-space, slab or open tank (freeze and contact) */
+  fl_pressure_options pressure; /* preconditioner 0 (V-cycle) */
+} fl_substep_options;
+typedef struct fl_substep_report {
+  int32_t status;
+  int32_t stage;
+  uint64_t error_ordinal; /* first failing element of the failing stage (a particle's ordinal in the
+                           * substep's input order for the sort, deposit and update), or ~0 */
+  uint64_t particles, blocks, candidates, centre_samples, corner_samples, frozen_cells, frozen_faces;
+  uint64_t lower_theta_count;
+  double capillary_step_bound;
+  fl_pressure_diagnostics pressure;
+  double sort_ms, field_ms, deposit_ms, freeze_ms, links_ms, projection_ms, gather_ms, update_ms, total_ms;
+  double max_speed; /* largest particle speed after the update (m/s), for the next step's CFL duration */
+} fl_substep_report;
+int fl_substep(fl_ctx* ctx, const fl_substep_options* options, fl_substep_report* report);
+/* The scene's particles after substeps, in the order of the last substep's sort (any pointer may be
+ * null): positions and velocities (3 per particle), slopes (9, row-major), volume ratios and flags
+ * (bit 0: PIC initialised). n must be the scene's particle count. */
+int fl_substep_download(fl_ctx* ctx, double* positions_xyz, double* velocities_xyz, double* slopes_rowmajor,
+                        double* volume_ratios, uint8_t* flags, size_t n);
+
 /* ---- measurement utility (no counterpart in the reference) ----
  * Bandwidth of a plain float4 device-to-device copy of `bytes` repeated `reps` times (read + write
  * bytes over HIP-event time), the copy ceiling the HBM figures are compared with. Allocates and frees

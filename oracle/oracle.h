@@ -427,6 +427,14 @@ int flo_reference_vcycle(const flo_geometry* geometry, const flo_int3* sorted_bl
                          size_t face_count, const double* r, double* z);
 
 // Compiler that built the oracle (for the CPU-baseline record).
+// The particle-union liquid field (field.cc): PrepareField's candidate cells, centre and corner samples
+// (z,y,x sorted, unique) and FieldDistance at each sample. With null output arrays only the three
+// counts are written.
+int flo_particle_field(const flo_geometry* geometry, const flo_vec3* positions, size_t n, double radius,
+                       uint64_t candidate_capacity, flo_int3* out_cells, uint64_t* cell_count,
+                       flo_center_distance* out_centres, uint64_t centre_capacity, uint64_t* centre_count,
+                       flo_center_distance* out_corners, uint64_t corner_capacity, uint64_t* corner_count);
+
 const char* flo_compiler(void);
 
 // ---- learned preconditioner network (unet.cc): 3-level U-Net on 8^3 tiles, FP64 ----

@@ -713,3 +713,24 @@ def strain_row(terms, weight, first_axis=0, second_axis=0):
     r["term_count"] = len(terms)
     r["quadrature_weight"] = weight
     return r[0]
+
+
+def particle_field(geom, positions, radius, candidate_capacity=1 << 24):
+    """flo_particle_field: (status, candidate cells, centre samples, corner samples) of the particle union."""
+    pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    n = len(pos)
+    nc, ncen, ncor = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    s = lib().flo_particle_field(_p(geom), _p(pos), C.c_size_t(n), C.c_double(radius), C.c_uint64(candidate_capacity),
+                                 None, C.byref(nc), None, C.c_uint64(0), C.byref(ncen), None, C.c_uint64(0),
+                                 C.byref(ncor))
+    if s != OK:
+        return s, None, None, None
+    cells = np.zeros(max(nc.value, 1), INT3)
+    centres = np.zeros(max(ncen.value, 1), CENTER_DISTANCE)
+    corners = np.zeros(max(ncor.value, 1), CENTER_DISTANCE)
+    s = lib().flo_particle_field(_p(geom), _p(pos), C.c_size_t(n), C.c_double(radius), C.c_uint64(candidate_capacity),
+                                 _p(cells), C.byref(nc), _p(centres), C.c_uint64(len(centres)), C.byref(ncen),
+                                 _p(corners), C.c_uint64(len(corners)), C.byref(ncor))
+    if s != OK:
+        return s, None, None, None
+    return s, cells[:nc.value].copy(), centres[:ncen.value].copy(), corners[:ncor.value].copy()

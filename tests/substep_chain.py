@@ -1,0 +1,140 @@
+"""The oracle's chain of one liquid substep, in the device substep's stage order.
+
+It chains the oracle's restatements of the reference stages on the CPU (sort, particle-union field,
+activation, deposit quanta, normalize, forces, frozen geometry, typed links, projection, GatherMac3D and
+the particle update) over the same particle state as fl_substep, so the device chain can be compared
+stage by stage and step by step. Test infrastructure: it calls the oracle only.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import pyoracle as O
+
+ROWS = ("x_row", "y_row", "z_row")
+
+
+def tank_obstacle(n_cells_high=1.0):
+    """The unit-cube open tank (walls at x = 0, 1, z = 0, 1, floor y = 0; no piston motion)."""
+    return O.tank(0.0, 1.0, 0.0, 0.0, 1.0, 0.0)
+
+
+def obstacle_record(obstacle, dtype):
+    """An OBSTACLE record of the given dtype from an oracle obstacle tuple (or absent)."""
+    rec = np.zeros(1, dtype)
+    if obstacle is None:
+        return rec
+    kind, n, c, w, v = obstacle[:5]
+    rec["kind"], rec["coordinate"], rec["half_width"] = kind, c, w
+    for k, a, b in zip("xyz", n, v):
+        rec["unit_normal"][k], rec["wall_velocity"][k] = a, b
+    if len(obstacle) > 5:
+        for name, value in zip(rec["tank"].dtype.names, obstacle[5]):
+            rec["tank"][name] = value
+    return rec
+
+
+class State:
+    """Particle state in the current order: positions, velocities, slopes (rows by component)."""
+
+    def __init__(self, x, v, c, mass, ratio=None, flags=None):
+        self.x = np.array(x, np.float64).reshape(-1, 3)
+        self.v = np.array(v, np.float64).reshape(-1, 3)
+        self.c = np.array(c, np.float64).reshape(-1, 3, 3)
+        self.mass = float(mass)
+        n = len(self.x)
+        self.ratio = np.ones(n) if ratio is None else np.array(ratio, np.float64)
+        self.flags = np.zeros(n, np.uint8) if flags is None else np.array(flags, np.uint8)
+
+    def permuted(self, order):
+        return State(self.x[order], self.v[order], self.c[order], self.mass, self.ratio[order], self.flags[order])
+
+    def transfer_records(self):
+        p = np.zeros(len(self.x), O.PARTICLE)
+        for i, k in enumerate("xyz"):
+            p["sorted_world_position"][k] = self.x[:, i]
+            p["trial_world_position"][k] = self.x[:, i]
+            p["particle_velocity"][k] = self.v[:, i]
+            for j, row in enumerate(ROWS):
+                p["particle_velocity_slopes"][row][k] = self.c[:, j, i]
+        p["particle_mass"] = self.mass
+        return p
+
+    def liquid_records(self):
+        p = np.zeros(len(self.x), O.LIQUID_PARTICLE)
+        for i, k in enumerate("xyz"):
+            p["world_position"][k] = self.x[:, i]
+            p["world_velocity"][k] = self.v[:, i]
+            for j, row in enumerate(ROWS):
+                p["velocity_slopes"][row][k] = self.c[:, j, i]
+        p["mass"] = self.mass
+        p["volume_ratio"] = self.ratio
+        p["pic_initialized"] = self.flags & 1
+        return p
+
+
+def vec(a):
+    return np.stack([a["x"], a["y"], a["z"]], -1)
+
+
+def mat(a):
+    return np.stack([vec(a[r]) for r in ROWS], -2)
+
+
+def oracle_substep(geom, state, dt, radius, obstacle, pressure_options, gravity=(0.0, -9.81, 0.0),
+                   mass_q=2.0 ** -40, mom_q=2.0 ** -40, capacity=1 << 20, density=1000.0, sigma=0.0, ambient=0.0,
+                   blend=1.0, minimum_face_mass=0.0):
+    """One substep: (new state in this step's sort order, dict of the intermediate records)."""
+    h = float(geom[0]["cell_width"])
+    s, entries = O.sort_particles(geom, state.x)
+    assert s == O.OK, ("sort", s)
+    order = entries["original_ordinal"].astype(np.int64)
+    st = state.permuted(order)
+    s, cand, cen, cor = O.particle_field(geom, st.x, radius)
+    assert s == O.OK, ("field", s)
+    s, blocks = O.activate_blocks(geom, st.x, cand)
+    assert s == O.OK, ("activate", s)
+    particles = st.transfer_records()
+    s, quanta = O.deposit_quanta(geom, blocks, particles, mass_q, mom_q, capacity)
+    assert s == O.OK, ("deposit", s)
+    s, fmass, u0, validity = O.normalize(quanta, mass_q, mom_q, minimum_face_mass)
+    assert s == O.OK, ("normalize", s)
+    s, ustar = O.apply_forces(blocks, u0, gravity, dt)
+    assert s == O.OK, ("forces", s)
+    fp = O.FreezeProblem(geom, blocks, cand, cen, cor, obstacle=obstacle, step=dt, density=density, sigma=sigma)
+    s, fcells, ffaces, bound = O.freeze_geometry(fp)
+    assert s == O.OK, ("freeze", s)
+    fields = np.zeros(len(fcells), O.CELL_FIELDS)
+    fields["bulk_modulus"] = np.inf
+    fields["density"] = density
+    fields["has_density"] = 1
+    predicted = ustar[ffaces["face_slot"].astype(np.int64)]
+    lp = O.LinkProblem(geom, blocks, cen, fcells, ffaces, step=dt, density=density, sigma=sigma, fields=fields,
+                       predicted=predicted, ambient=ambient)
+    s, pcells, pfaces, lower = O.build_pressure_links(lp)
+    assert s == O.OK, ("links", s)
+    s, proj = O.project(geom, blocks, dt, pcells, pfaces, pressure_options)
+    assert s == O.OK, ("project", s)
+    unew = ustar.copy()
+    unew[ffaces["face_slot"].astype(np.int64)] = proj["velocities"]
+    s, gathered = O.gather(geom, blocks, particles, u0, unew)
+    assert s == O.OK, ("gather", s)
+    pic = vec(gathered["new_pic_velocity"])
+    inc = vec(gathered["grid_velocity_increment"])
+    old = pic - inc
+    g = np.zeros(len(st.x), O.BOUNDARY_GATHER)
+    for i, k in enumerate("xyz"):
+        g["new_pic_velocity"][k] = pic[:, i]
+        g["grid_velocity_increment"][k] = inc[:, i]
+        g["particle_minus_old_pic_velocity"][k] = st.v[:, i] - old[:, i]
+    g["fitted_velocity_slopes"] = gathered["fitted_velocity_slopes"]
+    g["velocity_gradient"] = gathered["velocity_gradient"]
+    opts = O.update_options(h, dt, blend, obstacle=obstacle)
+    s, updated = O.update_particles(opts, g, np.full(len(st.x), -radius), st.liquid_records())
+    assert s == O.OK, ("update", s)
+    new = State(vec(updated["world_position"]), vec(updated["world_velocity"]), mat(updated["velocity_slopes"]),
+                st.mass, updated["volume_ratio"], np.ones(len(st.x), np.uint8))
+    info = dict(order=order, candidates=cand, centres=cen, corners=cor, blocks=blocks, quanta=quanta, u0=u0,
+                ustar=ustar, frozen_cells=fcells, frozen_faces=ffaces, pressure_cells=pcells, pressure_faces=pfaces,
+                lower=lower, projection=proj, unew=unew, gathered=gathered, bound=bound)
+    return new, info
