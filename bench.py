@@ -562,6 +562,8 @@ def main():
     ap.add_argument("--no-viscosity", action="store_true", help="skip the viscosity measurement")
     ap.add_argument("--substeps", type=int, default=20,
                     help="consecutive device substeps over moving particles (0: skip)")
+    ap.add_argument("--only-substep", action="store_true",
+                    help="run only the substep leg (profiling): one JSON line with the substep object")
     ap.add_argument("--learn-pairs", type=int, default=50000)
     ap.add_argument("--learn-epochs", type=int, default=8)
     args = ap.parse_args()
@@ -577,6 +579,9 @@ def main():
     scene = scenes.by_name(args.workload)
     cells = scene.pressure_cells()
     log(f"scene {scene.name}: {scene.count:,} particles, {len(cells):,} liquid cells ({time.perf_counter() - t0:.1f}s)")
+    if args.only_substep:
+        print(json.dumps({"workload": scene.name, "substep": substep_measurement(scene, cells, args)}), flush=True)
+        return
     ctx = H.Context(particles=scene.count + 1024, blocks=1 << 17, lookup_blocks=1 << 22,
                     pressure_cells=len(cells) + 64, pressure_faces=4 * len(cells) + 4096)
     device = ctx.device_name()

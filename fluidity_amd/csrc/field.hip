@@ -35,14 +35,12 @@ struct FieldBuffers {
   fl_center_distance* centres = nullptr;
   fl_center_distance* corners = nullptr;
   uint64_t cap_cells = 0, cap_centres = 0, cap_corners = 0;
-  int32_t* bin_slot = nullptr;  // bin box -> non-empty bin index, -1 empty
-  uint64_t cap_bin_slot = 0;
-  uint32_t* sub_start = nullptr;  // 65 per non-empty bin: sub-cell starts relative to the bin's first particle
+  uint2* sub_range = nullptr;    // 64 per bin of the bin box: [first, last) particles of each sub-cell
   uint64_t cap_sub = 0;
   double* sub_pos = nullptr;    // 4 doubles per particle: x, y, z of the particles in sub-cell order
   uint64_t cap_pos = 0;
   void release() {
-    void* ptrs[] = {cand, centre, corner, counts, scans, tot, cells, centres, corners, bin_slot, sub_start, sub_pos};
+    void* ptrs[] = {cand, centre, corner, counts, scans, tot, cells, centres, corners, sub_range, sub_pos};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     *this = FieldBuffers{};
@@ -120,6 +118,95 @@ __global__ void k_mark_candidates(const double* sx, const double* sy, const doub
     }
 }
 
+// The same marks, one workgroup per non-empty bin: the bin's particles set their cells in an LDS bit
+// window of W^3 cells (a bin holds lattice coordinates [4B + 1.5, 4B + 5.5), so with R = ceil(r / h)
+// every box lies in [4B - R, 4B + 6 + R] on each axis, one cell of slack included), then each window
+// row is ORed into the global bits with at most two atomics. A particle whose box leaves the window
+// (never, short of rounding beyond the slack) marks its cells directly.
+constexpr int kMaxWindow = 15;
+__global__ __launch_bounds__(256) void k_mark_candidates_bins(const double* sx, const double* sy, const double* sz,
+                                                              const uint32_t* perm, const uint32_t* bkey,
+                                                              const uint32_t* boff, const uint32_t* bn, Box bins,
+                                                              int32_t reach, double ox, double oy, double oz, double h,
+                                                              double r, BitBox box, uint64_t* bits, uint64_t* err) {
+  __shared__ unsigned int win[(kMaxWindow * kMaxWindow * kMaxWindow + 31) / 32];
+  const int W = 7 + 2 * reach;
+  const int nwords = (W * W * W + 31) / 32;
+  const uint32_t key = bkey[blockIdx.x], off = boff[blockIdx.x], cnt = bn[blockIdx.x];
+  const int32_t B[3] = {bins.lo[0] + static_cast<int32_t>(key % bins.dim[0]),
+                        bins.lo[1] + static_cast<int32_t>((key / bins.dim[0]) % bins.dim[1]),
+                        bins.lo[2] + static_cast<int32_t>(key / (uint32_t(bins.dim[0]) * bins.dim[1]))};
+  const int32_t w0[3] = {4 * B[0] - reach, 4 * B[1] - reach, 4 * B[2] - reach};
+  for (int k = threadIdx.x; k < nwords; k += blockDim.x) win[k] = 0;
+  __syncthreads();
+  const double o[3] = {ox, oy, oz};
+  for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
+    const uint32_t i = off + k;
+    const double c[3] = {sx[i], sy[i], sz[i]};
+    int32_t lo[3], hi[3];
+    bool ok = true, inside = true;
+    for (int a = 0; a < 3; ++a) {
+      const double low = floor(((c[a] - r) - o[a]) / h);
+      const double high = floor(((c[a] + r) - o[a]) / h);
+      if (!finite(low) || !finite(high) || low < double(kI32Min) + 1 || high > double(kI32Max) - 1) {
+        atomicMin(reinterpret_cast<unsigned long long*>(err),
+                  static_cast<unsigned long long>(error_word(perm[i], kOutOfRange)));
+        ok = false;
+        break;
+      }
+      lo[a] = static_cast<int32_t>(low);
+      hi[a] = static_cast<int32_t>(high);
+      inside = inside && lo[a] >= w0[a] && hi[a] < w0[a] + W;
+      if (lo[a] - box.lo[a] < 0 || hi[a] - box.lo[a] >= box.dim[a]) {
+        atomicMin(reinterpret_cast<unsigned long long*>(err),
+                  static_cast<unsigned long long>(error_word(perm[i], kInternal)));
+        ok = false;
+        break;
+      }
+    }
+    if (!ok) continue;
+    if (inside) {
+      for (int32_t z = lo[2]; z <= hi[2]; ++z)
+        for (int32_t y = lo[1]; y <= hi[1]; ++y)
+          for (int32_t x = lo[0]; x <= hi[0]; ++x) {
+            const int b = ((z - w0[2]) * W + (y - w0[1])) * W + (x - w0[0]);
+            atomicOr(&win[b >> 5], 1u << (b & 31));
+          }
+    } else {
+      for (int32_t z = lo[2]; z <= hi[2]; ++z)
+        for (int32_t y = lo[1]; y <= hi[1]; ++y) {
+          uint64_t* row = bits + (uint64_t(z - box.lo[2]) * box.dim[1] + (y - box.lo[1])) * box.row_words;
+          for (int32_t x = lo[0]; x <= hi[0]; ++x) {
+            const int32_t bx = x - box.lo[0];
+            atomicOr(reinterpret_cast<unsigned long long*>(row + (bx >> 6)),
+                     static_cast<unsigned long long>(uint64_t(1) << (bx & 63)));
+          }
+        }
+    }
+  }
+  __syncthreads();
+  for (int row = threadIdx.x; row < W * W; row += blockDim.x) {
+    uint32_t m = 0;  // W <= 15 bits of this window row
+    const int first = row * W;
+    for (int x = 0; x < W; ++x) {
+      const int b = first + x;
+      m |= ((win[b >> 5] >> (b & 31)) & 1u) << x;
+    }
+    if (!m) continue;
+    const int32_t gz = w0[2] + row / W, gy = w0[1] + row % W;
+    if (gz - box.lo[2] < 0 || gz - box.lo[2] >= box.dim[2] || gy - box.lo[1] < 0 || gy - box.lo[1] >= box.dim[1])
+      continue;  // only empty rows can fall outside (every set bit came from a checked box)
+    uint64_t* grow_ = bits + (uint64_t(gz - box.lo[2]) * box.dim[1] + (gy - box.lo[1])) * box.row_words;
+    const int32_t bx = w0[0] - box.lo[0];  // >= 0 for any row with a set bit
+    const uint64_t wide = uint64_t(m);
+    const int32_t word = bx >> 6, shift = bx & 63;
+    atomicOr(reinterpret_cast<unsigned long long*>(grow_ + word), static_cast<unsigned long long>(wide << shift));
+    if (shift && (wide >> (64 - shift)))
+      atomicOr(reinterpret_cast<unsigned long long*>(grow_ + word + 1),
+               static_cast<unsigned long long>(wide >> (64 - shift)));
+  }
+}
+
 // Centre and corner sets from the candidate bits, one output word per lane. Centres: a cell within
 // one cell of a candidate on every axis; corners: corner (x, y, z) of candidate (x - i, y - j, z - k),
 // i, j, k in {0, 1}. The box keeps two empty cells around every candidate, so nothing spills out.
@@ -191,19 +278,15 @@ __global__ void k_compact(const uint64_t* cand, const uint64_t* centre, const ui
   }
 }
 
-__global__ void k_bin_slots(const uint32_t* key, uint32_t nbins, int32_t* slot) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nbins) slot[key[i]] = static_cast<int32_t>(i);
-}
-
 __device__ inline int32_t floor_i32(double v) { return static_cast<int32_t>(floor(v)); }
 
 // Regroups the particles of every non-empty bin by unit sub-cell: sub-cell u = floor(X - 1.5) - 4 B
-// on each axis with X = (x - o) / h (clamped into the bin; the search below allows for the rounding).
+// on each axis with X = (x - o) / h (clamped into the bin; the search below allows for the rounding),
+// and writes each sub-cell's particle range into the dense table over the bin box.
 __global__ __launch_bounds__(256) void k_subcells(const double* sx, const double* sy, const double* sz,
                                                   const uint32_t* bkey, const uint32_t* boff, const uint32_t* bn,
                                                   Box bins, double ox, double oy, double oz, double h,
-                                                  uint32_t* sub_start, double* sub_pos) {
+                                                  uint2* sub_range, double* sub_pos) {
   __shared__ uint32_t count[64], start[65], fill[64];
   const uint32_t b = blockIdx.x;
   const uint32_t key = bkey[b], off = boff[b], cnt = bn[b];
@@ -236,7 +319,8 @@ __global__ __launch_bounds__(256) void k_subcells(const double* sx, const double
     start[64] = run;
   }
   __syncthreads();
-  if (threadIdx.x < 65) sub_start[uint64_t(b) * 65 + threadIdx.x] = start[threadIdx.x];
+  if (threadIdx.x < 64)
+    sub_range[uint64_t(key) * 64 + threadIdx.x] = make_uint2(off + start[threadIdx.x], off + start[threadIdx.x + 1]);
   for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
     const uint32_t i = off + k;
     const int q = sub(i);
@@ -247,38 +331,24 @@ __global__ __launch_bounds__(256) void k_subcells(const double* sx, const double
   }
 }
 
+__global__ void k_zero_ranges(uint2* r, uint64_t n) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    r[i] = make_uint2(0, 0);
+}
+
 struct SubGrid {
-  const int32_t* slot;
-  const uint32_t* boff;
-  const uint32_t* sub_start;
+  const uint2* range;  // 64 per bin of the bin box
   const double* pos;
   Box bins;
 };
 
-// Visits the particles of the sub-cells [lo, hi] (inclusive, global sub-cell coordinates).
-template <class Visit>
-__device__ inline void visit_subcells(const SubGrid& G, const int32_t lo[3], const int32_t hi[3], Visit&& visit) {
-  for (int32_t uz = lo[2]; uz <= hi[2]; ++uz)
-    for (int32_t uy = lo[1]; uy <= hi[1]; ++uy)
-      for (int32_t ux = lo[0]; ux <= hi[0]; ++ux) {
-        const int32_t b[3] = {static_cast<int32_t>(floor_div4(ux)), static_cast<int32_t>(floor_div4(uy)),
-                              static_cast<int32_t>(floor_div4(uz))};
-        const int64_t bi = G.bins.index(b[0], b[1], b[2]);
-        if (bi < 0) continue;
-        const int32_t s = G.slot[bi];
-        if (s < 0) continue;
-        const int q = ((uz - 4 * b[2]) * 4 + (uy - 4 * b[1])) * 4 + (ux - 4 * b[0]);
-        const uint64_t base = G.boff[s];
-        const uint32_t first = G.sub_start[uint64_t(s) * 65 + q], last = G.sub_start[uint64_t(s) * 65 + q + 1];
-        for (uint32_t k = first; k < last; ++k) {
-          const double* p = G.pos + 4 * (base + k);
-          visit(p[0], p[1], p[2]);
-        }
-      }
-}
-
 // FieldDistance of LiquidParticleSphereUnion3D at every sample (centres at index + 0.5, corners at
-// the index), world position origin + lattice * h as the reference rounds it.
+// the index), world position origin + lattice * h as the reference rounds it. One pass keeps the least
+// squared distance d2 and the least hypot among the particles whose d2 is within a relative 1e-10 of
+// it: a particle more than that above the least d2 is farther than the hypot minimum by far more than
+// hypot's rounding. Sub-cells are visited in rings of growing radius rho (cells); within a ring a
+// sub-cell whose box is farther than the current least distance is skipped, and the search ends when
+// every unvisited sub-cell lies beyond the least distance.
 __global__ void k_field_distance(fl_center_distance* samples, uint64_t n, int corner, SubGrid G, double ox, double oy,
                                  double oz, double h, double r, uint64_t* err) {
   const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
@@ -287,16 +357,42 @@ __global__ void k_field_distance(fl_center_distance* samples, uint64_t n, int co
   const double half = corner ? 0.0 : 0.5;
   const double L[3] = {c.x + half, c.y + half, c.z + half};
   const double w[3] = {ox + L[0] * h, oy + L[1] * h, oz + L[2] * h};
-  double best = __builtin_inf();
-  int32_t lo[3], hi[3];
-  // Sub-cells outside the bin box hold no particle: the search is clamped to it, and once the clamped
-  // cube is the whole box every particle has been visited.
+  const double h2 = h * h;
+  double best = __builtin_inf(), least = __builtin_inf();
   int32_t blo[3], bhi[3];
   for (int a = 0; a < 3; ++a) {
     blo[a] = 4 * G.bins.lo[a];
     bhi[a] = 4 * (G.bins.lo[a] + G.bins.dim[a]) - 1;
   }
+  // Lattice gap between the sample and sub-cell u on one axis (u covers [u + 1.5, u + 2.5)), less a
+  // slack for particles whose computed sub-cell rounding moved across a face.
+  auto gap = [&](int a, int32_t u) {
+    const double g = fmax(fmax((u + 1.5) - L[a], L[a] - (u + 2.5)), 0.0);
+    return fmax(g - 1e-6, 0.0);
+  };
+  auto visit = [&](int32_t ux, int32_t uy, int32_t uz) {
+    const int32_t b[3] = {static_cast<int32_t>(floor_div4(ux)), static_cast<int32_t>(floor_div4(uy)),
+                          static_cast<int32_t>(floor_div4(uz))};
+    const int64_t bi = G.bins.index(b[0], b[1], b[2]);
+    if (bi < 0) return;
+    const int q = ((uz - 4 * b[2]) * 4 + (uy - 4 * b[1])) * 4 + (ux - 4 * b[0]);
+    const uint2 rg = G.range[uint64_t(bi) * 64 + q];
+    for (uint32_t k = rg.x; k < rg.y; ++k) {
+      const double* p = G.pos + 4 * uint64_t(k);
+      const double dx = w[0] - p[0], dy = w[1] - p[1], dz = w[2] - p[2];
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 < best * (1 - 1e-10)) {
+        best = d2;
+        least = hypot3(dx, dy, dz);
+      } else if (d2 <= best * (1 + 1e-10)) {
+        best = fmin(best, d2);
+        least = fmin(least, hypot3(dx, dy, dz));
+      }
+    }
+  };
+  int32_t plo[3] = {0, 0, 0}, phi[3] = {-1, -1, -1};  // the previous ring's cube (empty at first)
   for (double rho = 1;; rho *= 2) {
+    int32_t lo[3], hi[3];
     bool all = true;
     for (int a = 0; a < 3; ++a) {
       const double l = floor(L[a] - rho - 1.5), u = floor(L[a] + rho - 1.5);
@@ -304,20 +400,33 @@ __global__ void k_field_distance(fl_center_distance* samples, uint64_t n, int co
       hi[a] = u >= bhi[a] ? bhi[a] : static_cast<int32_t>(fmax(u, double(blo[a]) - 1));
       all = all && l <= blo[a] && u >= bhi[a];
     }
-    visit_subcells(G, lo, hi, [&](double px, double py, double pz) {
-      const double dx = w[0] - px, dy = w[1] - py, dz = w[2] - pz;
-      best = fmin(best, dx * dx + dy * dy + dz * dz);
-    });
+    // Sub-cells touching the sample first (they hold the nearest particle of a liquid sample), then
+    // the rest of the ring, each skipped when its box is farther than the least distance so far.
+    for (int pass = 0; pass < 2; ++pass)
+      for (int32_t uz = lo[2]; uz <= hi[2]; ++uz) {
+        const double gz = gap(2, uz);
+        for (int32_t uy = lo[1]; uy <= hi[1]; ++uy) {
+          const double gy = gap(1, uy);
+          for (int32_t ux = lo[0]; ux <= hi[0]; ++ux) {
+            const bool old = ux >= plo[0] && ux <= phi[0] && uy >= plo[1] && uy <= phi[1] && uz >= plo[2] &&
+                             uz <= phi[2];
+            if (old) continue;
+            const double gx = gap(0, ux);
+            const double g2 = gx * gx + gy * gy + gz * gz;
+            if ((g2 == 0) != (pass == 0)) continue;
+            if (g2 * h2 > best * (1 + 1e-9)) continue;
+            visit(ux, uy, uz);
+          }
+        }
+      }
+    for (int a = 0; a < 3; ++a) {
+      plo[a] = lo[a];
+      phi[a] = hi[a];
+    }
     // Every unvisited particle lies at least (rho - rounding) cells from the sample.
     if (finite(best) && sqrt(best) * (1 + 1e-12) < (rho - 1e-6) * h) break;
     if (all) break;
   }
-  double least = __builtin_inf();
-  const double bound = best * (1 + 1e-10);
-  visit_subcells(G, lo, hi, [&](double px, double py, double pz) {
-    const double dx = w[0] - px, dy = w[1] - py, dz = w[2] - pz;
-    if (dx * dx + dy * dy + dz * dz <= bound) least = fmin(least, hypot3(dx, dy, dz));
-  });
   const double d = least - r;
   if (!finite(d)) {
     atomicMin(reinterpret_cast<unsigned long long*>(err), static_cast<unsigned long long>(error_word(i, kOutOfRange)));
@@ -384,8 +493,15 @@ int sample_particle_field(fl_ctx* ctx, double radius, uint64_t max_candidates, F
   const ParticleView v = ctx->sorted.view();
   k_zero_words<<<grid_for(nw), kThreads, 0, s>>>(B.cand, nw);
   FL_TRY(reset_error(ctx));
-  k_mark_candidates<<<grid_for(n), kThreads, 0, s>>>(v.s[0], v.s[1], v.s[2], ctx->perm, n, g.ox, g.oy, g.oz, h, radius,
-                                                     box, B.cand, ctx->err);
+  const int32_t creach = static_cast<int32_t>(reach);
+  if (7 + 2 * creach <= kMaxWindow) {
+    k_mark_candidates_bins<<<ctx->nbins, 256, 0, s>>>(v.s[0], v.s[1], v.s[2], ctx->perm, ctx->bin_key, ctx->bin_offset,
+                                                      ctx->bin_n, ctx->bin_box, creach, g.ox, g.oy, g.oz, h, radius,
+                                                      box, B.cand, ctx->err);
+  } else {
+    k_mark_candidates<<<grid_for(n), kThreads, 0, s>>>(v.s[0], v.s[1], v.s[2], ctx->perm, n, g.ox, g.oy, g.oz, h,
+                                                       radius, box, B.cand, ctx->err);
+  }
   k_dilate<<<grid_for(nw), kThreads, 0, s>>>(B.cand, box, B.centre, B.corner);
   k_popcounts<<<grid_for(nw), kThreads, 0, s>>>(B.cand, B.centre, B.corner, nw, B.counts);
   FL_HIP(hipGetLastError());
@@ -403,15 +519,13 @@ int sample_particle_field(fl_ctx* ctx, double radius, uint64_t max_candidates, F
   k_compact<<<grid_for(nw), kThreads, 0, s>>>(B.cand, B.centre, B.corner, B.scans, box, B.cells, B.centres, B.corners);
   // Sub-cell regrouping of the sorted particles for the nearest-particle search.
   const uint64_t nkeys = ctx->bin_box.volume();
-  FL_TRY(grow_buf(&B.bin_slot, &B.cap_bin_slot, nkeys));
-  FL_TRY(grow_buf(&B.sub_start, &B.cap_sub, 65 * uint64_t(ctx->nbins)));
+  FL_TRY(grow_buf(&B.sub_range, &B.cap_sub, 64 * nkeys));
   FL_TRY(grow_buf(&B.sub_pos, &B.cap_pos, 4 * n));
-  FL_TRY(fill_i32(ctx, B.bin_slot, nkeys, -1));
-  k_bin_slots<<<grid_for(ctx->nbins), kThreads, 0, s>>>(ctx->bin_key, ctx->nbins, B.bin_slot);
+  k_zero_ranges<<<std::min<uint64_t>(grid_for(64 * nkeys), 65536), kThreads, 0, s>>>(B.sub_range, 64 * nkeys);
   k_subcells<<<ctx->nbins, 256, 0, s>>>(v.s[0], v.s[1], v.s[2], ctx->bin_key, ctx->bin_offset, ctx->bin_n,
-                                        ctx->bin_box, g.ox, g.oy, g.oz, h, B.sub_start, B.sub_pos);
+                                        ctx->bin_box, g.ox, g.oy, g.oz, h, B.sub_range, B.sub_pos);
   FL_HIP(hipGetLastError());
-  const SubGrid G{B.bin_slot, ctx->bin_offset, B.sub_start, B.sub_pos, ctx->bin_box};
+  const SubGrid G{B.sub_range, B.sub_pos, ctx->bin_box};
   FL_TRY(reset_error(ctx));
   if (ncen) k_field_distance<<<grid_for(ncen), kThreads, 0, s>>>(B.centres, ncen, 0, G, g.ox, g.oy, g.oz, h, radius, ctx->err);
   if (ncor) k_field_distance<<<grid_for(ncor), kThreads, 0, s>>>(B.corners, ncor, 1, G, g.ox, g.oy, g.oz, h, radius, ctx->err);

@@ -293,6 +293,23 @@ __device__ int sweep_tank_axis(const fl_particle_update_options& o, const Sweep*
   return kOk;
 }
 
+// True when the tank step is valid and the particle stays farther from every wall than its whole
+// move (and the piston's), by a margin far above the rounding of the plain differences: every wall
+// distance that ResolveTankContact evaluates with bounded sums is then certainly positive at the start
+// and at the end of the step, so it makes no contact and returns the plain advance.
+__device__ inline bool tank_clear(const fl_particle_update_options& o, fl_vec3 x, fl_vec3 adv) {
+  const fl_tank& t = o.obstacle.tank;
+  if (o.friction_coefficient != 0 || check_tank_step(t, o.step_duration) != kOk) return false;
+  const double dt = o.step_duration;
+  const double reach = (fmax(fabs(adv.x), fmax(fabs(adv.y), fabs(adv.z))) + fabs(t.piston_velocity_x)) * dt;
+  const double scale = fabs(x.x) + fabs(x.y) + fabs(x.z) + fabs(t.start_low_x) + fabs(t.high_x) + fabs(t.floor_y) +
+                       fabs(t.low_z) + fabs(t.high_z) + 1.0;
+  const double margin = reach * (1 + 1e-9) + 1e-12 * scale;
+  if (!finite(margin)) return false;
+  return x.x - t.start_low_x > margin && t.high_x - x.x > margin && x.y - t.floor_y > margin &&
+         x.z - t.low_z > margin && t.high_z - x.z > margin;
+}
+
 // ResolveTankContact (liquid_wall_contact_math_3d.h:290-363): frictionless, per axis x, y, z.
 __device__ int tank_contact(const fl_particle_update_options& o, fl_vec3 x, fl_vec3 stored, fl_vec3 adv,
                             fl_vec3& position, fl_vec3& stored_out) {
@@ -360,7 +377,14 @@ __device__ inline int update_core(const fl_particle_update_options& o, const Upd
       xn = advance(x, adv, o.step_duration);
       if (!finite3(xn)) code = kOutOfRange;
     } else if (o.obstacle.kind == FL_OBSTACLE_TANK) {
-      code = tank_contact(o, x, stored, adv, xn, vn);
+      if (tank_clear(o, x, adv)) {
+        // Far from every wall for the whole step: the per-axis sweep of ResolveTankContact finds the
+        // particle inside every wall at both ends and advances it unchanged (fma of each component).
+        xn = advance(x, adv, o.step_duration);
+        if (!finite3(xn)) code = kOutOfRange;
+      } else {
+        code = tank_contact(o, x, stored, adv, xn, vn);
+      }
     } else {
       code = plane_contact(o, x, stored, adv, xn, vn);
     }
