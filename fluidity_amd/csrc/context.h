@@ -370,8 +370,11 @@ struct ParticleColumns {
 // The update of the columns with GatherMac3D results (24 component-major columns of capacity gcap)
 // and one liquid-field value phi for every particle; first failure into ctx->err; the largest updated
 // speed's bit pattern into *max_speed (atomic max, optional).
+// The open tank runs a clear-particle pass and a contact pass over the particles near a wall, listed
+// in list (n entries) with their number in *list_count (device).
 int update_columns(fl_ctx* ctx, const fl_particle_update_options& o, uint64_t n, const ParticleColumns& P,
-                   const double* gather, uint64_t gcap, double phi, unsigned long long* max_speed);
+                   const double* gather, uint64_t gcap, double phi, unsigned long long* max_speed, uint32_t* list,
+                   uint32_t* list_count);
 
 // Small helpers (topology.hip)
 // Variational MAC viscosity (viscosity.hip): device arrays of one strain assembly or velocity-block solve.

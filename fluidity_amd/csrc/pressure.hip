@@ -2598,14 +2598,17 @@ __global__ void k_sys_cell_faces(const fl_pressure_cell* cells, uint64_t nc, Sys
     }
 }
 
+// Root of x with path halving. Links only ever point from a root to a smaller root, so a node's
+// grandparent is always one of its ancestors and the halving store needs no atomic: a racing store
+// can only leave a longer, still valid path.
 __device__ inline int32_t uf_find(int32_t* parent, int32_t x) {
   volatile int32_t* p = parent;
   while (true) {
     const int32_t up = p[x];
     if (up == x) return x;
     const int32_t next = p[up];
-    if (next != up) atomicCAS(&parent[x], up, next);  // path halving; a lost race only shortens less
-    x = up;
+    if (next != up) p[x] = next;
+    x = next;
   }
 }
 

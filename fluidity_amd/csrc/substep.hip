@@ -31,13 +31,14 @@ struct SubstepBuffers {
   // particle state beyond the scene's columns, in the scene's order and in sort order
   double *ratio = nullptr, *ratio_sorted = nullptr;
   uint8_t *flags = nullptr, *flags_sorted = nullptr;
+  uint32_t *contact = nullptr, *contact_count = nullptr;  // particles near a wall for the contact pass
   uint64_t cap[24] = {};
   hipEvent_t ev[12] = {};
   bool fresh = true;  // the scene was uploaded since the last substep: volume ratios 1, PIC unset
   void release() {
     void* ptrs[] = {keep,   slots,  word_counts, face_slots, counts, work,      words,        staged,
                     cells,  faces,  needs,       phi,        uncut,  fields,    predicted,    pcells,
-                    pfaces, lower,  ratio,       ratio_sorted, flags, flags_sorted};
+                    pfaces, lower,  ratio,       ratio_sorted, flags, flags_sorted, contact, contact_count};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     for (hipEvent_t e : ev)
@@ -384,8 +385,10 @@ int substep(fl_ctx* ctx, const fl_substep_options& o, fl_substep_report& rep) {
     P.ordinal = ctx->perm;
     FL_TRY(reset_error(ctx));
     FL_HIP(hipMemsetAsync(B.lower, 0, sizeof(uint64_t), s));
+    FL_TRY(grow_to(&B.contact, &B.cap[22], n));
+    FL_TRY(grow_to(&B.contact_count, &B.cap[23], 1));
     STAGE(8, update_columns(ctx, uo, n, P, ctx->gather, ctx->sorted.capacity, -o.field_radius,
-                            reinterpret_cast<unsigned long long*>(B.lower)));
+                            reinterpret_cast<unsigned long long*>(B.lower), B.contact, B.contact_count));
     STAGE(8, error_code(ctx));
     FL_HIP(hipMemcpyAsync(ctx->host_words, B.lower, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     FL_HIP(hipStreamSynchronize(s));

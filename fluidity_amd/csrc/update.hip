@@ -339,16 +339,21 @@ constexpr int kGroup = 64;
 static_assert(kGroup * sizeof(fl_liquid_particle) % 16 == 0 && kGroup * sizeof(fl_boundary_gather) % 16 == 0,
               "16-byte blocks");
 
-// kObstacle: the half-space / slab / tank instance; the obstacle-free instance carries no contact code.
+// kObstacle: 0 no obstacle (no contact code), 1 the half-space / slab instance, 2 the open-tank
+// instance, 3 the open tank's clear-particle pass (no contact code: a particle near a wall returns
+// kDeferred); each instance carries only its own contact code, so the plane sweep's registers do not
+// weigh on the tank's kernels and the contact code's on the clear particles.
 // One particle's update: ASFLIP stored velocity v = new PIC + alpha (v - old PIC), the surface-expansion
 // advecting velocity (blend beta where the velocity diverges at the liquid surface away from the solid),
 // the obstacle's contact or the plain advance, the half-cell guard and J exp(dt div u).
+constexpr int kDeferred = -1;  // the clear-particle pass leaves this particle to the contact pass
+
 struct UpdateIn {
   fl_vec3 x, new_pic, minus_old_pic;
   double divergence, phi, volume_ratio;
   bool pic_initialized, fallback;
 };
-template <bool kObstacle>
+template <int kObstacle>
 __device__ inline int update_core(const fl_particle_update_options& o, const UpdateIn& in, fl_vec3& xn, fl_vec3& vn,
                                   double& j) {
   const double alpha = in.pic_initialized && !in.fallback ? 0.99 * o.particle_energy_blend : 0.0;
@@ -376,7 +381,12 @@ __device__ inline int update_core(const fl_particle_update_options& o, const Upd
     if (!kObstacle) {
       xn = advance(x, adv, o.step_duration);
       if (!finite3(xn)) code = kOutOfRange;
-    } else if (o.obstacle.kind == FL_OBSTACLE_TANK) {
+    } else if (kObstacle == 3) {
+      // The clear-particle pass of the tank: particles near a wall go to the contact pass.
+      if (!tank_clear(o, x, adv)) return kDeferred;
+      xn = advance(x, adv, o.step_duration);
+      if (!finite3(xn)) code = kOutOfRange;
+    } else if (kObstacle == 2) {
       if (tank_clear(o, x, adv)) {
         // Far from every wall for the whole step: the per-axis sweep of ResolveTankContact finds the
         // particle inside every wall at both ends and advances it unchanged (fma of each component).
@@ -404,12 +414,17 @@ __device__ inline int update_core(const fl_particle_update_options& o, const Upd
 // slopes in place, the gather results of GatherMac3D (24 component-major columns: new PIC, increment,
 // fitted slopes, gradient; old PIC = new PIC - increment), the volume ratio and a per-particle flag
 // byte (bit 0: PIC initialised). phi is the liquid field at every particle.
-template <bool kObstacle>
+// rows: when set, lane t updates particle rows[t] of *count (the contact pass); deferred: when set,
+// particles the clear pass leaves to the contact pass are appended there (count in deferred_count).
+template <int kObstacle>
 __global__ void k_update_columns(fl_particle_update_options o, uint64_t n, ParticleColumns P, const double* gather,
-                                 uint64_t gcap, double phi, uint64_t* err, unsigned long long* max_speed) {
-  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+                                 uint64_t gcap, double phi, uint64_t* err, unsigned long long* max_speed,
+                                 const uint32_t* rows, uint32_t* deferred, uint32_t* deferred_count) {
+  const uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  const uint64_t limit = rows ? uint64_t(*deferred_count) : n;
+  const uint64_t i = rows && t < limit ? rows[t] : t;
   double speed = 0.0;
-  if (i < n) {
+  if (t < limit) {
     const fl_vec3 np{gather[i], gather[gcap + i], gather[2 * gcap + i]};
     const fl_vec3 inc{gather[3 * gcap + i], gather[4 * gcap + i], gather[5 * gcap + i]};
     const fl_vec3 v{P.v[0][i], P.v[1][i], P.v[2][i]};
@@ -426,7 +441,9 @@ __global__ void k_update_columns(fl_particle_update_options o, uint64_t n, Parti
     fl_vec3 xn, vn;
     double j = 0.0;
     const int code = update_core<kObstacle>(o, in, xn, vn, j);
-    if (code != kOk) {
+    if (code == kDeferred) {
+      deferred[atomicAdd(deferred_count, 1u)] = static_cast<uint32_t>(i);
+    } else if (code != kOk) {
       atomicMin(reinterpret_cast<unsigned long long*>(err),
                 static_cast<unsigned long long>(error_word(P.ordinal ? P.ordinal[i] : i, code)));
     } else {
@@ -448,7 +465,7 @@ __global__ void k_update_columns(fl_particle_update_options o, uint64_t n, Parti
     atomicMax(max_speed, static_cast<unsigned long long>(__double_as_longlong(speed)));
 }
 
-template <bool kObstacle>
+template <int kObstacle>
 __global__ __launch_bounds__(kGroup) void k_update_particles(fl_particle_update_options o,
                                                              const fl_boundary_gather* gathered,
                                                              const double* distances, uint64_t n,
@@ -521,24 +538,41 @@ int update_particles(fl_ctx* ctx, const fl_particle_update_options& o, const fl_
                      const double* distances, uint64_t n, fl_liquid_particle* particles) {
   if (!n) return FL_OK;
   if (o.obstacle.kind == FL_OBSTACLE_ABSENT)
-    k_update_particles<false><<<grid_for(n, kGroup), kGroup, 0, ctx->stream>>>(o, gathered, distances, n, particles,
-                                                                             ctx->err);
+    k_update_particles<0><<<grid_for(n, kGroup), kGroup, 0, ctx->stream>>>(o, gathered, distances, n, particles,
+                                                                         ctx->err);
+  else if (o.obstacle.kind == FL_OBSTACLE_TANK)
+    k_update_particles<2><<<grid_for(n, kGroup), kGroup, 0, ctx->stream>>>(o, gathered, distances, n, particles,
+                                                                         ctx->err);
   else
-    k_update_particles<true><<<grid_for(n, kGroup), kGroup, 0, ctx->stream>>>(o, gathered, distances, n, particles,
-                                                                            ctx->err);
+    k_update_particles<1><<<grid_for(n, kGroup), kGroup, 0, ctx->stream>>>(o, gathered, distances, n, particles,
+                                                                         ctx->err);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
 
 int update_columns(fl_ctx* ctx, const fl_particle_update_options& o, uint64_t n, const ParticleColumns& P,
-                   const double* gather, uint64_t gcap, double phi, unsigned long long* max_speed) {
+                   const double* gather, uint64_t gcap, double phi, unsigned long long* max_speed, uint32_t* list,
+                   uint32_t* list_count) {
   if (!n) return FL_OK;
-  if (o.obstacle.kind == FL_OBSTACLE_ABSENT)
-    k_update_columns<false><<<grid_for(n), kThreads, 0, ctx->stream>>>(o, n, P, gather, gcap, phi, ctx->err,
-                                                                      max_speed);
-  else
-    k_update_columns<true><<<grid_for(n), kThreads, 0, ctx->stream>>>(o, n, P, gather, gcap, phi, ctx->err,
-                                                                     max_speed);
+  hipStream_t s = ctx->stream;
+  if (o.obstacle.kind == FL_OBSTACLE_ABSENT) {
+    k_update_columns<0><<<grid_for(n), kThreads, 0, s>>>(o, n, P, gather, gcap, phi, ctx->err, max_speed, nullptr,
+                                                         nullptr, nullptr);
+  } else if (o.obstacle.kind == FL_OBSTACLE_TANK) {
+    // Clear particles first, then the contact pass over the ones near a wall.
+    FL_HIP(hipMemsetAsync(list_count, 0, sizeof(uint32_t), s));
+    k_update_columns<3><<<grid_for(n), kThreads, 0, s>>>(o, n, P, gather, gcap, phi, ctx->err, max_speed, nullptr,
+                                                         list, list_count);
+    uint32_t count = 0;
+    FL_HIP(hipMemcpyAsync(&count, list_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    FL_HIP(hipStreamSynchronize(s));
+    if (count)
+      k_update_columns<2><<<grid_for(count), kThreads, 0, s>>>(o, n, P, gather, gcap, phi, ctx->err, max_speed,
+                                                               list, nullptr, list_count);
+  } else {
+    k_update_columns<1><<<grid_for(n), kThreads, 0, s>>>(o, n, P, gather, gcap, phi, ctx->err, max_speed, nullptr,
+                                                         nullptr, nullptr);
+  }
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
