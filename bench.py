@@ -469,6 +469,24 @@ def viscosity_measurement(ctx, scene, dt, report, nu=1e-3, cpu_cells=1 << 16):
             "cpu_sample": f"first {k} cells (z,y,x order), assembly + solve, 1 thread"}
 
 
+def fp32_transfer_measurement(ctx, scene, dt, opts, args, n, nf):
+    """The same steps with the FP32-arithmetic deposit (fl_set_transfer_precision 32): FP32 stencils and
+    contributions into the same integer quanta (deterministic; within 1e-5 of the FP64 quanta)."""
+    assert ctx.set_transfer_precision(32) == H.OK
+    reps = []
+    for k in range(args.warmup + args.steps):
+        s, rep = ctx.step(dt, scene.gravity, QUANTUM, QUANTUM, scene.count, opts)
+        assert s == H.OK, s
+        if k >= args.warmup:
+            reps.append(rep)
+    assert ctx.set_transfer_precision(64) == H.OK
+    p2g_ms = float(np.mean([r.p2g_kernel_ms for r in reps]))
+    achieved = p2g_algorithmic_bytes(n, nf) / (p2g_ms * 1e-3) / 1e9
+    return {"kernel": "k_p2g_f32", "p2g_kernel_ms": p2g_ms, "achieved_gbs": achieved,
+            "frac": achieved / HBM_PEAK_GBS, "p2g_particle_steps_per_s": n / (p2g_ms * 1e-3),
+            "step_total_ms": float(np.mean([r.total_ms for r in reps]))}
+
+
 def substep_measurement(scene, cells, args):
     """Consecutive device substeps over moving particles (fl_substep) of the workload in the unit-cube
     open tank: the particle-union field (r = 0.9 h), freeze, typed links, MGPCG on a device-built
@@ -665,6 +683,7 @@ def main():
         },
     }
     out["roofline"]["vcycle"] = projection_roofline(ctx, args.level_storage)
+    out["fp32_transfers"] = fp32_transfer_measurement(ctx, scene, dt, opts, args, n, nf)
     if not args.no_learned:
         out["learned"] = learned_comparison(ctx, scene, dt, opts, args)
     if not (args.no_cpu_baseline and args.no_oracle_error):

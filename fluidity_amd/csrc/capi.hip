@@ -1862,6 +1862,13 @@ int fl_sample_particle_field(fl_ctx* ctx, const fl_geometry* geometry, const fl_
   return FL_OK;
 }
 
+int fl_set_transfer_precision(fl_ctx* ctx, int bits) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  if (bits != 32 && bits != 64) return FL_INVALID_ARGUMENT;
+  ctx->transfer_bits = bits;
+  return FL_OK;
+}
+
 int fl_substep(fl_ctx* ctx, const fl_substep_options* options, fl_substep_report* report) {
   if (!ctx || ctx->poisoned || !ctx->scene_loaded) return FL_FAILED_PRECONDITION;
   if (!options || !report) return FL_INVALID_ARGUMENT;

@@ -143,6 +143,9 @@ struct fl_ctx {
   uint64_t* host_words = nullptr;  // pinned host scratch (64 words)
 
   // device-resident scene
+  // Transfer arithmetic of the device-resident step and substep: 64 (bit-exact FP64, the default) or
+  // 32 (FP32 stencils and contributions into the same integer quanta).
+  int transfer_bits = 64;
   uint64_t scene_n = 0;
   double scene_mass = 0;
   bool scene_loaded = false;  // fl_scene_upload ran and no span-based call reused its buffers since

@@ -329,11 +329,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
     const auto cy = normalize<kPow2>(t[1], g.oy, g.h, g.inv_h);
     const auto cz = normalize<kPow2>(t[2], g.oz, g.h, g.inv_h);
     if (__all(cx.interior() && cy.interior() && cz.interior()))
-      code = deposit_particle<true, kSmall, kF64>(cx, cy, cz, bin, tile, wblock, g.h, p, k, inv_qm, inv_qp, limit, wm, wq,
-                                            code);
+      code = deposit_particle<true, kSmall, kF64>(cx, cy, cz, bin, tile, wblock, g.h, p, k, inv_qm, inv_qp, limit,
+                                                  wm, wq, code);
     else
-      code = deposit_particle<false, kSmall, kF64>(cx, cy, cz, bin, tile, wblock, g.h, p, k, inv_qm, inv_qp, limit, wm,
-                                             wq, code);
+      code = deposit_particle<false, kSmall, kF64>(cx, cy, cz, bin, tile, wblock, g.h, p, k, inv_qm, inv_qp, limit,
+                                                   wm, wq, code);
     if (code) report(err, perm[k], code);
   }
   __syncthreads();
@@ -348,6 +348,154 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
       if (wm[i]) atomicAdd(&qm[slot], wm[i]);
       if (wq[i]) atomicAdd(&qp[slot], wq[i]);
     }
+  }
+}
+
+// ---- FP32 transfer arithmetic (the FP32 deposit of the GPU configurations) ----
+// The same bins, LDS windows, window ownership and integer quanta as k_p2g, with the stencil weights,
+// offsets and contributions in FP32: each particle's lattice coordinate is taken relative to its bin's
+// window in FP64 and rounded once to FP32 (a value in [1.5, 5.5), so within 5e-7 of a cell); weights
+// are the quadratic B-spline of that coordinate, offsets (node - x) h, mass w m and momentum
+// mass (v + c . offset) in FP32. Each contribution is rounded half to even to integer quanta (through
+// an exact FP64 add of 1.5 * 2^52) before any addition, so the sums are deterministic in any order; they
+// agree with the FP64 deposit to the FP32 rounding of the contributions (1e-6 relative).
+struct Axis3f {
+  int32_t r;  // first node relative to the window's low corner, in [0, 5]
+  float w[3];
+  float off[3];
+};
+
+__device__ __forceinline__ Axis3f axis3f(float xr, float shift, float h) {
+  // Nodes at (first + shift + i), first = floor(xr - shift - 0.5); fx = xr - (first + shift) in [0.5, 1.5].
+  const float base = floorf(xr - shift - 0.5f);
+  Axis3f s;
+  const float f = xr - (base + shift);
+  s.r = min(max(static_cast<int32_t>(base), 0), 5);
+  const float a = 1.5f - f, c = f - 1.0f, d = f - 0.5f;
+  s.w[0] = 0.5f * a * a;
+  s.w[1] = 0.75f - c * c;
+  s.w[2] = 0.5f * d * d;
+  s.off[0] = -f * h;
+  s.off[1] = (1.0f - f) * h;
+  s.off[2] = (2.0f - f) * h;
+  return s;
+}
+
+__device__ __forceinline__ unsigned long long quanta_f32(float x) {
+  const double t = static_cast<double>(x) + 0x1.8p52;
+  return static_cast<unsigned long long>(__double_as_longlong(t)) - 0x4338000000000000ull;
+}
+
+template <int A>
+__device__ __forceinline__ bool deposit_axis_f32(const Axis3f& X, const Axis3f& Y, const Axis3f& Z, float m, float v,
+                                                 float cx, float cy, float cz, float inv_qm, float inv_qp, float limit,
+                                                 unsigned long long* wm, unsigned long long* wq) {
+  int fx[3], fy[3], fz[3];
+  float ox[3], oy[3], oz[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int rx = X.r + i, ry = Y.r + i, rz = Z.r + i;
+    fx[i] = (rx >> 2) * 192 + (rx & 3) + A * 64;
+    fy[i] = (ry >> 2) * 384 + (ry & 3) * 4;
+    fz[i] = (rz >> 2) * 768 + (rz & 3) * 16;
+    ox[i] = cx * X.off[i];
+    oy[i] = cy * Y.off[i];
+    oz[i] = cz * Z.off[i];
+  }
+  bool ok = true;
+#pragma unroll
+  for (int zi = 0; zi < 3; ++zi) {
+    const float mz = Z.w[zi] * m;
+#pragma unroll
+    for (int yi = 0; yi < 3; ++yi) {
+      const float myz = Y.w[yi] * mz;
+      const float vyz = v + oy[yi] + oz[zi];
+#pragma unroll
+      for (int xi = 0; xi < 3; ++xi) {
+        const float mass = X.w[xi] * myz;
+        const float qm = mass * inv_qm, qp = mass * (vyz + ox[xi]) * inv_qp;
+        ok = ok && fabsf(qm) <= limit && fabsf(qp) <= limit;
+        const int slot = fz[zi] + fy[yi] + fx[xi];
+        atomicAdd(&wm[slot], quanta_f32(qm));
+        atomicAdd(&wq[slot], quanta_f32(qp));
+      }
+    }
+  }
+  return ok;
+}
+
+// Lattice coordinate (x - o) / h relative to the window corner 4 B, rounded once to FP32.
+template <bool kPow2>
+__device__ __forceinline__ float window_coord(double x, double o, double h, double inv_h, int32_t corner) {
+  const double X = kPow2 ? (x - o) * inv_h : (x - o) / h;
+  return static_cast<float>(X - static_cast<double>(corner));
+}
+
+template <bool kPow2>
+__global__ __launch_bounds__(kThreads) void k_p2g_f32(ParticleView p, BinList bins, GridView grid, double inv_qm,
+                                                      double inv_qp, double limit, const uint32_t* perm,
+                                                      unsigned long long* qm, unsigned long long* qp, uint64_t* err) {
+  __shared__ unsigned long long wm[kWindow], wq[kWindow];
+  __shared__ int32_t wblock[8];
+  __shared__ uint32_t stride;
+  const uint32_t b = xcd_chunked(blockIdx.x, gridDim.x);
+  int32_t bin[3];
+  bin_coords(bins, bins.key[b], bin);
+  const uint32_t off = bins.offset[b], cnt = bins.count[b];
+  for (int i = threadIdx.x; i < kWindow; i += kThreads) {
+    wm[i] = 0;
+    wq[i] = 0;
+  }
+  if (threadIdx.x < 8)
+    wblock[threadIdx.x] = grid.ordinal(bin[0] + (threadIdx.x & 1), bin[1] + ((threadIdx.x >> 1) & 1),
+                                       bin[2] + (threadIdx.x >> 2));
+  if (threadIdx.x == 0) {
+    uint32_t st = 9;
+    while (cnt > 1 && gcd_u32(st, cnt) != 1) st += 2;
+    stride = st;
+  }
+  const Geometry g = grid.g;
+  __syncthreads();
+  const float h = static_cast<float>(g.h);
+  const float fqm = static_cast<float>(inv_qm), fqp = static_cast<float>(inv_qp), flimit = static_cast<float>(limit);
+  const uint32_t st = stride, step = cnt ? (kThreads * st) % cnt : 0;
+  uint32_t idx = cnt ? (threadIdx.x * st) % cnt : 0;
+  for (uint32_t v = threadIdx.x; v < cnt; v += kThreads) {
+    const uint32_t k = off + idx;
+    idx += step;
+    idx = idx >= cnt ? idx - cnt : idx;
+    const float xr = window_coord<kPow2>(p.s[0][k], g.ox, g.h, g.inv_h, 4 * bin[0]);
+    const float yr = window_coord<kPow2>(p.s[1][k], g.oy, g.h, g.inv_h, 4 * bin[1]);
+    const float zr = window_coord<kPow2>(p.s[2][k], g.oz, g.h, g.inv_h, 4 * bin[2]);
+    const float m = static_cast<float>(p.m[k]);
+    const Axis3f nx = axis3f(xr, 0.0f, h), tx = axis3f(xr, 0.5f, h);
+    const Axis3f ny = axis3f(yr, 0.0f, h), ty = axis3f(yr, 0.5f, h);
+    const Axis3f nz = axis3f(zr, 0.0f, h), tz = axis3f(zr, 0.5f, h);
+    int code = kOk;
+    if (!window_ok(wblock, axis_blocks(nx.r, 0), axis_blocks(ty.r, 0), axis_blocks(tz.r, 0)) ||
+        !window_ok(wblock, axis_blocks(tx.r, 0), axis_blocks(ny.r, 0), axis_blocks(tz.r, 0)) ||
+        !window_ok(wblock, axis_blocks(tx.r, 0), axis_blocks(ty.r, 0), axis_blocks(nz.r, 0))) {
+      code = kFailedPrecondition;
+    } else {
+      const float v0 = static_cast<float>(p.v[0][k]), v1 = static_cast<float>(p.v[1][k]),
+                  v2 = static_cast<float>(p.v[2][k]);
+      float c[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) c[q] = static_cast<float>(p.c[q][k]);
+      bool ok = deposit_axis_f32<0>(nx, ty, tz, m, v0, c[0], c[1], c[2], fqm, fqp, flimit, wm, wq);
+      ok = deposit_axis_f32<1>(tx, ny, tz, m, v1, c[3], c[4], c[5], fqm, fqp, flimit, wm, wq) && ok;
+      ok = deposit_axis_f32<2>(tx, ty, nz, m, v2, c[6], c[7], c[8], fqm, fqp, flimit, wm, wq) && ok;
+      if (!ok) code = kOutOfRange;
+    }
+    if (code) report(err, perm[k], code);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kWindow; i += kThreads) {
+    const int32_t ord = wblock[i / 192];
+    if (ord < 0) continue;
+    const uint64_t slot = uint64_t(ord) * 192 + (i % 192);
+    if (wm[i]) atomicAdd(&qm[slot], wm[i]);
+    if (wq[i]) atomicAdd(&qp[slot], wq[i]);
   }
 }
 
@@ -1413,6 +1561,22 @@ int unpack_particles(fl_ctx* ctx, uint64_t n) {
 
 int p2g_quanta(fl_ctx* ctx, uint64_t n, double inv_qm, double inv_qp, double limit, hipEvent_t before,
                hipEvent_t after, bool f64) {
+  if (ctx->transfer_bits == 32 && !f64 && ctx->sorted.trial_aliases_sorted) {
+    const uint64_t nf = ctx->nb * 192;
+    FL_TRY(fill_i64(ctx, ctx->qm, nf, 0));
+    FL_TRY(fill_i64(ctx, ctx->qp, nf, 0));
+    if (before) FL_HIP(hipEventRecord(before, ctx->stream));
+    if (ctx->nbins && n) {
+      auto* kernel = ctx->geometry.inv_h != 0.0 ? k_p2g_f32<true> : k_p2g_f32<false>;
+      kernel<<<ctx->nbins, kThreads, 0, ctx->stream>>>(ctx->sorted.view(), bin_list(ctx), grid_view(ctx), inv_qm,
+                                                       inv_qp, limit, ctx->perm,
+                                                       reinterpret_cast<unsigned long long*>(ctx->qm),
+                                                       reinterpret_cast<unsigned long long*>(ctx->qp), ctx->err);
+      FL_HIP(hipGetLastError());
+    }
+    if (after) FL_HIP(hipEventRecord(after, ctx->stream));
+    return FL_OK;
+  }
   const uint64_t nf = ctx->nb * 192;
   FL_TRY(fill_i64(ctx, ctx->qm, nf, 0));
   FL_TRY(fill_i64(ctx, ctx->qp, nf, 0));

@@ -55,7 +55,8 @@ EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_reserve", "fl_bin_so
            "fl_scene_download", "fl_probe_copy", "fl_learn_param_count", "fl_learn_setup", "fl_learn_set_params", "fl_learn_get_params",
            "fl_learn_forward", "fl_learn_infer", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect", "fl_learn_eval",
            "fl_learn_train", "fl_normalize_forces_static", "fl_time_projection_kernels", "fl_assemble_full_cell_strain", "fl_solve_viscous_block", "fl_substep",
-           "fl_substep_download", "fl_sample_particle_field", "fl_project_device_built")
+           "fl_substep_download", "fl_sample_particle_field", "fl_project_device_built",
+           "fl_set_transfer_precision")
 
 _lib = None
 
@@ -461,6 +462,10 @@ class Context:
         if s != OK:
             return s, None, None, None
         return s, cells[:nc.value], cen[:ncen.value], cor[:ncor.value]
+
+    def set_transfer_precision(self, bits):
+        """fl_set_transfer_precision: 64 (bit-exact FP64 deposit) or 32 (FP32 arithmetic, integer quanta)."""
+        return lib().fl_set_transfer_precision(self.handle, C.c_int(bits))
 
     def substep(self, options):
         """fl_substep on the uploaded scene: (status, report record)."""

@@ -690,6 +690,13 @@ int fl_sample_particle_field(fl_ctx* ctx, const fl_geometry* geometry, const fl_
                              fl_center_distance* out_corners, uint64_t corner_capacity, uint64_t* corner_count,
                              fl_measurement* measurement);
 
+/* Transfer arithmetic of the device-resident paths (fl_step, fl_substep): 64 (the default) keeps the
+ * reference's FP64 arithmetic and bit-exact quanta; 32 computes the deposit's stencils and
+ * contributions in FP32 (the FP32 transfers of the GPU configurations) into the same integer quanta,
+ * rounded per contribution, so the sums stay deterministic and agree with FP64 to about 1e-6 relative.
+ * No reference counterpart: the reference's deposit is FP64 only. Span-based entry points are FP64. */
+int fl_set_transfer_precision(fl_ctx* ctx, int bits);
+
 /* ---- device-resident substep over moving particles ----
  * One liquid substep on the uploaded scene in the stage order of AttemptImpl
  * (native/src/liquid_substep_3d.cc:1175-2820; the backend members it chains are topology, deposit,
