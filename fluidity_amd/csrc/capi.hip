@@ -565,6 +565,7 @@ void fl_destroy(fl_ctx* ctx) {
   destroy_viscosity(ctx);
   fl::release_field(ctx);
   fl::release_substep(ctx);
+  fl::release_coupled(ctx);
   release(ctx);
   for (hipEvent_t e : ctx->stage)
     if (e) (void)hipEventDestroy(e);
@@ -1645,6 +1646,94 @@ int fl_solve_viscous_block(fl_ctx* ctx, const fl_strain_row* rows, size_t row_co
   *out_iterations = iterations;
   *out_max_residual = max_residual;
   fill_measurement(ctx, measurement, kernel_ms, copy_ms, static_cast<uint32_t>(8 + 12 * iterations));
+  return FL_OK;
+}
+
+int fl_project_viscous(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
+                       double dt, const fl_pressure_cell* cells, size_t cell_count, const fl_pressure_face* faces,
+                       size_t face_count, const fl_strain_row* rows, size_t row_count,
+                       const fl_pressure_options* options, double* out_pressures, double* out_velocities,
+                       double* out_defects, fl_viscous_diagnostics* out_diagnostics, fl_measurement* measurement) {
+  if (!ctx || ctx->poisoned) return FL_FAILED_PRECONDITION;
+  if (!options || (row_count && !rows) || !cells || !faces) return FL_INVALID_ARGUMENT;
+  if (options->max_iterations == 0 || options->max_iterations > 256) return FL_INVALID_ARGUMENT;
+  if (options->preconditioner != 0 && options->preconditioner != 2) return FL_INVALID_ARGUMENT;
+  FL_TRY(check_geometry(geometry));
+  FL_TRY(check_blocks(blocks, block_count));
+  if (face_count >= (uint64_t(1) << 32) || row_count >= (uint64_t(1) << 32) ||
+      row_count > 6 * std::max<uint64_t>(ctx->cap.pressure_cells, 1) || !scan_fits(ctx, face_count) ||
+      !scan_fits(ctx, row_count))
+    return FL_RESOURCE_EXHAUSTED;
+  (void)hipSetDevice(ctx->device);
+  const bool jacobi = options->preconditioner == 2;
+  fl_pressure_options popt = *options;
+  popt.preconditioner = 0;
+  Timer t(ctx);
+  fl::CoupledSystem S{};
+  FL_TRY(fl::coupled_prepare(ctx, geometry, blocks, block_count, dt, cells, cell_count, faces, face_count, &popt, &S));
+  FL_TRY(grow_viscosity(ctx, 1, std::max<uint64_t>(row_count, 1), std::max<uint64_t>(face_count, 1)));
+  auto& B = *static_cast<ViscosityBuffers*>(ctx->viscosity);
+  if (row_count)
+    FL_HIP(hipMemcpyAsync(B.rows, rows, row_count * sizeof(fl_strain_row), hipMemcpyHostToDevice, ctx->stream));
+  FL_TRY(fl::coupled_rows(ctx, S, viscosity_device(ctx), row_count, dt));
+  uint64_t terms = 0;
+  FL_TRY(fl::weight_viscous_rows(ctx, viscosity_device(ctx), row_count, face_count, dt, &terms));
+  FL_TRY(grow_viscosity_terms(ctx, terms));
+  fl::CoupledResult r{};
+  const int st = fl::coupled_solve(ctx, S, viscosity_device(ctx), row_count, terms, dt, popt, jacobi, &r);
+  const double kernel_ms = t.stop();
+  if (out_diagnostics) {
+    fl_viscous_diagnostics d{};
+    d.pressure_iterations = r.iterations;
+    d.constraint_residual_l2 = r.residual_l2;
+    d.initial_constraint_residual_l2 = r.initial_residual_l2;
+    d.max_cell_constraint_defect = r.max_defect;
+    d.max_momentum_residual = r.max_momentum_residual;
+    d.velocity_solves = r.velocity_solves;
+    d.velocity_iterations = r.velocity_iterations;
+    // AccumulateInwardExplicitWallFlux (mac_pressure_3d_arithmetic.h:97-108) over the input faces.
+    double sum = 0.0, corr = 0.0;
+    for (size_t f = 0; f < face_count; ++f) {
+      if (!faces[f].has_prescribed_wall_flux) continue;
+      const int a = faces[f].axis;
+      fl_int3 tail = faces[f].lattice;
+      int32_t& along = a == 0 ? tail.x : a == 1 ? tail.y : tail.z;
+      const bool tail_exists = along > std::numeric_limits<int32_t>::min();
+      if (tail_exists) along -= 1;
+      auto find = [&](const fl_int3& k) {
+        size_t lo = 0, hi = cell_count;
+        while (lo < hi) {
+          const size_t mid = (lo + hi) / 2;
+          const fl_int3& c = cells[mid].cell_index;
+          const bool less = c.z != k.z ? c.z < k.z : c.y != k.y ? c.y < k.y : c.x < k.x;
+          if (less) lo = mid + 1;
+          else hi = mid;
+        }
+        return lo < cell_count && cells[lo].cell_index.x == k.x && cells[lo].cell_index.y == k.y &&
+               cells[lo].cell_index.z == k.z;
+      };
+      const bool has_tail = tail_exists && find(tail);
+      const bool has_head = find(faces[f].lattice);
+      if (has_tail == has_head) continue;
+      const double term = (has_tail ? -1.0 : 1.0) * faces[f].prescribed_wall_flux;
+      const double n = sum + term;
+      corr += std::fabs(sum) >= std::fabs(term) ? (sum - n) + term : (term - n) + sum;
+      sum = n;
+    }
+    d.inward_prescribed_wall_flux = sum + corr;
+    *out_diagnostics = d;
+  }
+  if (st != FL_OK) return st;
+  const double *p = nullptr, *u = nullptr, *def = nullptr;
+  FL_TRY(fl::coupled_outputs(ctx, &p, &u, &def));
+  Timer c(ctx);
+  hipStream_t s = ctx->stream;
+  if (out_pressures) FL_HIP(hipMemcpyAsync(out_pressures, p, cell_count * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (out_velocities) FL_HIP(hipMemcpyAsync(out_velocities, u, face_count * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (out_defects) FL_HIP(hipMemcpyAsync(out_defects, def, cell_count * sizeof(double), hipMemcpyDeviceToHost, s));
+  FL_HIP(hipStreamSynchronize(s));
+  const double copy_ms = c.stop();
+  fill_measurement(ctx, measurement, kernel_ms, copy_ms, static_cast<uint32_t>(r.velocity_iterations));
   return FL_OK;
 }
 

@@ -170,6 +170,7 @@ struct fl_ctx {
   void* viscosity = nullptr;  // ViscosityBuffers (capi.hip)
   void* field = nullptr;      // FieldBuffers (field.hip)
   void* substep = nullptr;    // SubstepBuffers (substep.hip)
+  void* coupled = nullptr;    // CoupledBuffers (coupled.hip)
 };
 
 namespace fl {
@@ -347,6 +348,44 @@ int project_links_device(fl_ctx* ctx, const fl_pressure_cell* cells, uint64_t nc
                          fl_pressure_diagnostics* diag);
 int last_pressures(fl_ctx* ctx, const double** p, uint64_t* n);
 
+struct ViscosityDevice;
+
+// The prepared pressure system of fl_project's host path (pressure.hip), as device arrays in link and
+// cell order, for the coupled viscous pressure solve (coupled.hip).
+struct CoupledSystem {
+  uint64_t nc, nf;
+  const int32_t *tail, *head, *kind, *has_flux, *cell_faces;  // cell_faces: [nc][6] incident links
+  const uint64_t* slot;                                       // grid slot of each link, ascending
+  const double *area, *scale, *conductance, *ustar, *wall, *lift, *uwall;
+  const uint8_t* flag;  // 1 unknown, 2 gauge
+  const double *volume, *target, *comp, *prev, *p_init;
+  bool p_init_zero;
+};
+int coupled_prepare(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count, double dt,
+                    const fl_pressure_cell* cells, size_t cell_count, const fl_pressure_face* faces,
+                    size_t face_count, const fl_pressure_options* options, CoupledSystem* out);
+// Re-derives the hierarchy's coefficients from the given link conductances (the effective ones of the
+// viscous Schur complement's diagonal).
+int coupled_coefficients(fl_ctx* ctx, const double* conductance);
+// One level-0 V-cycle of the prepared system's hierarchy on the residual r (cell order): z on the
+// unknown cells, zero on gauge cells.
+int coupled_vcycle(fl_ctx* ctx, const double* r, double* z, const fl_pressure_options& options);
+// The pressure error words (residual conversion failures of coupled_vcycle).
+int coupled_error(fl_ctx* ctx);
+
+struct CoupledResult {
+  uint64_t iterations, velocity_solves, velocity_iterations;
+  double residual_l2, initial_residual_l2, max_defect, max_momentum_residual;
+};
+// SolveCoupled on a prepared system and rows already in the velocity block's arrays (v.rows, face
+// ordinals = link ordinals after coupled_rows): the GCR iteration; outputs in p (nc), u (nf) and
+// defects (nc), device arrays of coupled.hip's buffers (coupled_outputs).
+int coupled_rows(fl_ctx* ctx, const CoupledSystem& S, const ViscosityDevice& v, uint64_t nr, double dt);
+int coupled_solve(fl_ctx* ctx, const CoupledSystem& S, const ViscosityDevice& v, uint64_t nr, uint64_t nt, double dt,
+                  const fl_pressure_options& options, bool jacobi, CoupledResult* out);
+int coupled_outputs(fl_ctx* ctx, const double** p, const double** u, const double** defects);
+void release_coupled(fl_ctx* ctx);
+
 // The device substep (substep.hip).
 int substep(fl_ctx* ctx, const fl_substep_options& o, fl_substep_report& rep);
 int substep_download(fl_ctx* ctx, double* positions, double* velocities, double* slopes, double* ratios,
@@ -416,6 +455,13 @@ int solve_viscous_block(fl_ctx* ctx, const ViscosityDevice& v, uint64_t nr, uint
                         double max_rhs,
                         uint64_t max_iterations, double relative_tolerance, uint64_t* iterations,
                         double* max_residual);
+// The two halves of solve_viscous_block: pack the weighted rows and the face incidence and form the
+// Jacobi diagonal (v.diag); then the PCG from zero for the right-hand side in v.rhs (several solves may
+// follow one pack).
+int pack_viscous_block(fl_ctx* ctx, const ViscosityDevice& v, uint64_t nr, uint64_t nf, uint64_t nt, double dt);
+int pcg_viscous_block(fl_ctx* ctx, const ViscosityDevice& v, uint64_t nr, uint64_t nf, double max_rhs,
+                      uint64_t max_iterations, double relative_tolerance, uint64_t* iterations,
+                      double* max_residual);
 int read_error(fl_ctx* ctx, uint64_t* word);
 int reset_error(fl_ctx* ctx);
 int fill_i64(fl_ctx* ctx, int64_t* p, uint64_t n, int64_t v);

@@ -1456,11 +1456,14 @@ int build_levels(fl_ctx* ctx, Pressure& P, const std::vector<Key3>& blocks0, int
   return FL_OK;
 }
 
-int setup_coefficients(fl_ctx* ctx, Pressure& P) {
+// Level coefficients from the links' conductances (or from `conductance` when given: the coupled
+// viscous solve's effective conductances).
+int setup_coefficients(fl_ctx* ctx, Pressure& P, const double* conductance = nullptr) {
   hipStream_t s = ctx->stream;
   Level& L0 = P.levels[0];
   const uint64_t n0 = L0.nb * 64;
-  k_level0<<<grid_for(n0), kThreads, 0, s>>>(n0, P.fine_cell, P.cell_flag, P.cell_faces, P.face_k, P.face_tail,
+  k_level0<<<grid_for(n0), kThreads, 0, s>>>(n0, P.fine_cell, P.cell_flag, P.cell_faces,
+                                             conductance ? conductance : P.face_k, P.face_tail,
                                              P.face_head, P.cell_comp,
                                              P.cell_volume, L0.k, L0.anchor, L0.diag, P.fine_vol);
   FL_HIP(hipGetLastError());
@@ -3029,6 +3032,87 @@ int last_pressures(fl_ctx* ctx, const double** p, uint64_t* n) {
   if (!P) return FL_FAILED_PRECONDITION;
   *p = P->p;
   *n = P->nc;
+  return FL_OK;
+}
+
+// ---------------------------------------------------------------- coupled viscous pressure support
+
+namespace {
+// z = the level-0 V-cycle's output on the unknown cells, zero on gauge cells.
+__global__ void k_store_cells(uint64_t nc, const int32_t* cell_fine, const uint8_t* flag, const float* x, double* z) {
+  const uint64_t c = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (c >= nc) return;
+  z[c] = flag[c] == 1 ? static_cast<double>(x[cell_fine[c]]) : 0.0;
+}
+}  // namespace
+
+int coupled_prepare(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count, double dt,
+                    const fl_pressure_cell* cells, size_t cell_count, const fl_pressure_face* faces,
+                    size_t face_count, const fl_pressure_options* options, CoupledSystem* out) {
+  FL_TRY(check_options(options));
+  if (!ctx->pressure) ctx->pressure = new Pressure();
+  Pressure& P = *state(ctx);
+  P.scale_coarse = options->coarse_scale;
+  FL_TRY(prepare_system(ctx, P, geometry, blocks, block_count, dt, cells, cell_count, faces, face_count, true));
+  std::vector<Key3> b0 = P.levels[0].keys;
+  FL_TRY(build_levels(ctx, P, b0, options->levels));
+  Gates gates{};
+  EvalResult e0{};
+  fl_pressure_diagnostics diag{};
+  FL_TRY(freeze_and_assemble(ctx, P, dt, *options, gates, e0, diag));
+  CoupledSystem S{};
+  S.nc = P.nc;
+  S.nf = P.nf;
+  S.tail = P.face_tail;
+  S.head = P.face_head;
+  S.kind = P.face_kind;
+  S.slot = P.face_slot;
+  S.area = P.face_area;
+  S.scale = P.face_scale;
+  S.conductance = P.face_k;
+  S.ustar = P.face_ustar;
+  S.wall = P.face_wall;
+  S.lift = P.face_lift;
+  S.uwall = P.face_uwall;
+  S.has_flux = P.face_has_flux;
+  S.cell_faces = P.cell_faces;
+  S.flag = P.cell_flag;
+  S.volume = P.cell_volume;
+  S.target = P.cell_target;
+  S.comp = P.cell_comp;
+  S.prev = P.cell_prev;
+  S.p_init = P.p_init;
+  S.p_init_zero = P.p_init_zero;
+  *out = S;
+  return FL_OK;
+}
+
+int coupled_coefficients(fl_ctx* ctx, const double* conductance) {
+  Pressure* P = state(ctx);
+  if (!P || P->levels.empty()) return FL_FAILED_PRECONDITION;
+  return setup_coefficients(ctx, *P, conductance);
+}
+
+int coupled_vcycle(fl_ctx* ctx, const double* r, double* z, const fl_pressure_options& o) {
+  Pressure* P = state(ctx);
+  if (!P || P->levels.empty()) return FL_FAILED_PRECONDITION;
+  hipStream_t s = ctx->stream;
+  const uint64_t n = P->levels[0].nb * 64;
+  k_load_residual<<<grid_for(n), kThreads, 0, s>>>(n, P->fine_cell, P->cell_flag, r, P->r32, P->x32, P->perr);
+  FL_HIP(hipGetLastError());
+  FL_TRY(vcycle<float>(ctx, *P, 0, o.smoothing_sweeps, o.coarse_sweeps, o.smoother));
+  k_store_cells<<<grid_for(P->nc), kThreads, 0, s>>>(P->nc, P->cell_fine, P->cell_flag, P->levels[0].x, z);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+int coupled_error(fl_ctx* ctx) {
+  Pressure* P = state(ctx);
+  if (!P) return FL_OK;
+  FL_HIP(hipMemcpyAsync(ctx->host_words, P->perr, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  FL_HIP(hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < 2; ++i)
+    if (ctx->host_words[i] != ~uint64_t(0)) return static_cast<int>(ctx->host_words[i] & 0xff);
   return FL_OK;
 }
 

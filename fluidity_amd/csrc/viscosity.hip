@@ -599,12 +599,8 @@ int weight_viscous_rows(fl_ctx* ctx, const ViscosityDevice& V, uint64_t nr, uint
   return FL_OK;
 }
 
-int solve_viscous_block(fl_ctx* ctx, const ViscosityDevice& V, uint64_t nr, uint64_t nf, uint64_t nt, double dt,
-                        double max_rhs, uint64_t max_iterations, double relative_tolerance, uint64_t* iterations,
-                        double* max_residual) {
+int pack_viscous_block(fl_ctx* ctx, const ViscosityDevice& V, uint64_t nr, uint64_t nf, uint64_t nt, double dt) {
   hipStream_t s = ctx->stream;
-  *iterations = 0;
-  *max_residual = 0.0;
   // Packed term ordinals and row ordinals share one 64-bit incidence key.
   if (nt > V.term_capacity || nt >= (uint64_t(1) << 32)) return FL_RESOURCE_EXHAUSTED;
   uint64_t word = 0;
@@ -618,8 +614,26 @@ int solve_viscous_block(fl_ctx* ctx, const ViscosityDevice& V, uint64_t nr, uint
     FL_HIP(hipGetLastError());
     FL_TRY(read_error(ctx, &word));
     if (word != ~uint64_t(0)) return static_cast<int>(word & 0xff);
-    k_fill_d<<<grid_for(nf), kThreads, 0, s>>>(V.v, nf, 0.0);
   }
+  return FL_OK;
+}
+
+int solve_viscous_block(fl_ctx* ctx, const ViscosityDevice& V, uint64_t nr, uint64_t nf, uint64_t nt, double dt,
+                        double max_rhs, uint64_t max_iterations, double relative_tolerance, uint64_t* iterations,
+                        double* max_residual) {
+  *iterations = 0;
+  *max_residual = 0.0;
+  FL_TRY(pack_viscous_block(ctx, V, nr, nf, nt, dt));
+  return pcg_viscous_block(ctx, V, nr, nf, max_rhs, max_iterations, relative_tolerance, iterations, max_residual);
+}
+
+int pcg_viscous_block(fl_ctx* ctx, const ViscosityDevice& V, uint64_t nr, uint64_t nf, double max_rhs,
+                      uint64_t max_iterations, double relative_tolerance, uint64_t* iterations,
+                      double* max_residual) {
+  hipStream_t s = ctx->stream;
+  *iterations = 0;
+  *max_residual = 0.0;
+  if (nf) k_fill_d<<<grid_for(nf), kThreads, 0, s>>>(V.v, nf, 0.0);
   if (max_rhs == 0.0) return FL_OK;
   const double tolerance = relative_tolerance * max_rhs;
   const unsigned rg = static_cast<unsigned>(std::min<uint64_t>(kReduceGroups, (nf + kThreads - 1) / kThreads));

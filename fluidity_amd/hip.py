@@ -56,7 +56,7 @@ EXPORTS = ("fl_create", "fl_destroy", "fl_device_name", "fl_reserve", "fl_bin_so
            "fl_learn_forward", "fl_learn_infer", "fl_learn_loss_grad", "fl_learn_add_pairs", "fl_learn_pairs", "fl_learn_collect", "fl_learn_eval",
            "fl_learn_train", "fl_normalize_forces_static", "fl_time_projection_kernels", "fl_assemble_full_cell_strain", "fl_solve_viscous_block", "fl_substep",
            "fl_substep_download", "fl_sample_particle_field", "fl_project_device_built",
-           "fl_set_transfer_precision")
+           "fl_set_transfer_precision", "fl_project_viscous")
 
 _lib = None
 
@@ -367,6 +367,18 @@ class Context:
                              _p(r), _p(d), _p(gauges), _p(diag), C.byref(self.last))
         g = gauges[: int(diag["gauge_count"][0])].copy()
         return s, dict(pressures=p, velocities=u, rhs=b, residuals=r, defects=d, gauges=g, diagnostics=diag[0])
+
+    def project_viscous(self, geometry, blocks, dt, cells, faces, rows, options):
+        """fl_project_viscous: (status, dict of pressures, face velocities, cell defects, diagnostics)."""
+        nc, nf = len(cells), len(faces)
+        rows = np.ascontiguousarray(rows, dtype=R.STRAIN_ROW)
+        p, u, d = np.zeros(nc), np.zeros(nf), np.zeros(nc)
+        diag = np.zeros(1, R.VISCOUS_DIAGNOSTICS)
+        s = lib().fl_project_viscous(self.handle, _p(geometry), _p(blocks), C.c_size_t(len(blocks)), C.c_double(dt),
+                                     _p(cells), C.c_size_t(nc), _p(faces), C.c_size_t(nf), _p(rows),
+                                     C.c_size_t(len(rows)), _p(options), _p(p), _p(u), _p(d), _p(diag),
+                                     C.byref(self.last))
+        return s, dict(pressures=p, velocities=u, defects=d, diagnostics=diag[0])
 
     # ---- device-resident scene ----
     def scene_upload(self, geometry, positions, velocities, slopes, mass):

@@ -52,6 +52,11 @@ PRESSURE_DIAGNOSTICS = np.dtype([("iteration_count", "u8"), ("multigrid_level_co
                                  ("max_cell_constraint_defect", "f8"), ("rhs_l2", "f8"),
                                  ("initial_defect_l2", "f8"), ("defect_l2", "f8"), ("defect_linf", "f8"),
                                  ("restarts", "u8"), ("gauge_count", "u8"), ("learned_iterations", "u8")])
+VISCOUS_DIAGNOSTICS = np.dtype([("pressure_iterations", "u8"), ("constraint_residual_l2", "f8"),
+                                ("initial_constraint_residual_l2", "f8"), ("max_cell_constraint_defect", "f8"),
+                                ("max_momentum_residual", "f8"), ("inward_prescribed_wall_flux", "f8"),
+                                ("velocity_solves", "u8"), ("velocity_iterations", "u8")])
+assert VISCOUS_DIAGNOSTICS.itemsize == 64
 
 # LiquidBlockState3D / LiquidCellState3D / LiquidFaceState3D (native/include/fluidity/host/liquid_state_3d.h:129-159).
 BLOCK_STATE = np.dtype([("block_index", INT3), ("pad", "u4"), ("generation", "u8"), ("activation_age", "u4"),

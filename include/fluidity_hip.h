@@ -482,6 +482,33 @@ int fl_solve_viscous_block(fl_ctx* ctx, const fl_strain_row* rows, size_t row_co
                            double relative_tolerance, double* out_velocity, uint64_t* out_iterations,
                            double* out_max_residual, fl_measurement* measurement);
 
+/* ProjectMacViscousPressure3D (native/include/fluidity/host/mac_pressure_3d.h; mac_viscous_pressure_3d.cc:
+ * 151-459, 558-575, 699-705): the coupled viscous pressure solve of fl_project's cells and faces with the
+ * given strain rows (face_slot: grid slots of the faces; AssembleFullCellMacStrain3D's rows, for example).
+ * Each pressure iterate's face velocities solve (M + dt E^T W E) u = M u* + dt A (lift + p_tail - p_head)
+ * by Jacobi PCG to 1e-10 of the largest right-hand side; the pressure iteration is the reference's GCR
+ * (restart 32, budget options->max_iterations in [1, 256]) accepted at |defect| <= 2e-5 per cell and the
+ * residual L2 gate. options->preconditioner 0 preconditions the GCR with one V-cycle of the pressure
+ * hierarchy of the same links (levels, sweeps, smoother from the options; FP32 level storage); 2 uses the
+ * reference's Jacobi diagonal of the Schur complement. Outputs: pressures (cell_count), accepted face
+ * velocities (face_count; wall normal velocities on closed faces), cell constraint defects (cell_count).
+ * Statuses: the reference's input checks; Aborted when the budget is spent. */
+typedef struct fl_viscous_diagnostics {
+  uint64_t pressure_iterations;
+  double constraint_residual_l2;
+  double initial_constraint_residual_l2;
+  double max_cell_constraint_defect;
+  double max_momentum_residual;
+  double inward_prescribed_wall_flux;
+  uint64_t velocity_solves;     /* velocity block solves */
+  uint64_t velocity_iterations; /* their PCG iterations in total */
+} fl_viscous_diagnostics;        /* 64 B */
+int fl_project_viscous(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count,
+                       double step_duration, const fl_pressure_cell* cells, size_t cell_count,
+                       const fl_pressure_face* faces, size_t face_count, const fl_strain_row* rows, size_t row_count,
+                       const fl_pressure_options* options, double* out_pressures, double* out_velocities,
+                       double* out_defects, fl_viscous_diagnostics* out_diagnostics, fl_measurement* measurement);
+
 /* ---- topology ---- */
 /* ComputeParticleSortBin3D for each position plus the stable (z,y,x)-bin / ordinal order of
  * LiquidStepWorkspace3D::PrepareTopology (native/src/sparse_mac_grid_3d.cc:438-449;

@@ -202,6 +202,21 @@ int flo_solve_viscous_block(const flo_strain_row* rows, size_t row_count, const 
                             double dt, const double* rhs, uint64_t max_iterations, double relative_tolerance,
                             double* out_velocity, uint64_t* out_iterations, double* out_max_residual);
 
+// ProjectMacViscousPressure3D (mac_viscous_pressure_3d.cc:151-459, 558-575, 699-705): the coupled
+// viscous pressure solve on full or supplied strain rows (face_slot: grid slots), GCR with the Jacobi
+// diagonal of the viscous Schur complement, restart length 32, iteration budget in [1, 256].
+struct flo_viscous_diagnostics {
+  uint64_t pressure_iterations;
+  double constraint_residual_l2, initial_constraint_residual_l2, max_cell_constraint_defect;
+  double max_momentum_residual, inward_prescribed_wall_flux;
+  uint64_t velocity_solves;
+};
+int flo_project_viscous(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count, double dt,
+                        const flo_pressure_cell* cells, size_t cell_count, const flo_pressure_face* faces,
+                        size_t face_count, const flo_strain_row* rows, size_t row_count, uint64_t max_iterations,
+                        double* out_pressures, double* out_velocities, double* out_defects,
+                        flo_viscous_diagnostics* out_diagnostics);
+
 // ---- pressure links (liquid_boundary_geometry_3d.cc:1077-1383) ----
 // LiquidCenterDistance3D, FrozenLiquidCell3D, FrozenLiquidFace3D (with its optional wall flux as value +
 // flag, the libstdc++ layout of std::optional<double>), LiquidPressureCellFields3D,

@@ -1113,6 +1113,231 @@ int PrepareAndInit(const flo_geometry* g, const flo_int3* blocks, size_t nb, dou
   return FLO_OK;
 }
 
+
+// ---- coupled viscous pressure (mac_viscous_pressure_3d.cc:151-459) ----
+
+// AccumulateConstraint (mac_viscous_pressure_3d.cc:83-134): integrated D u + compliance flux per cell;
+// the target and wall fluxes and the pressure history enter only for the physical candidate.
+int CoupledConstraint(const System& sys, const std::vector<double>& p, const std::vector<double>& u, bool physical,
+                      std::vector<double>& out) {
+  const size_t nc = sys.cells.size();
+  std::vector<Comp> flux(nc);
+  for (size_t c = 0; c < nc; ++c) {
+    if (physical) flux[c].Add(-sys.cells[c].target_flux);
+    if (sys.cells[c].compliance > 0.0) {
+      const double dp = p[c] - (physical ? sys.cells[c].previous : 0.0);
+      const double f = sys.cells[c].compliance * dp;
+      if (!Representable(sys.cells[c].compliance, dp, f)) return FLO_OUT_OF_RANGE;
+      flux[c].Add(f);
+    }
+  }
+  for (size_t f = 0; f < sys.links.size(); ++f) {
+    const Link& l = sys.links[f];
+    const double open = l.open_area * u[f];
+    if (!Representable(l.open_area, u[f], open)) return FLO_OUT_OF_RANGE;
+    ForIncident(l, [&](size_t c, double sign) {
+      flux[c].Add(sign * open);
+      if (physical) flux[c].Add(sign * l.wall_flux);
+    });
+  }
+  out.resize(nc);
+  for (size_t c = 0; c < nc; ++c) {
+    out[c] = flux[c].Total();
+    if (!std::isfinite(out[c])) return FLO_OUT_OF_RANGE;
+  }
+  return FLO_OK;
+}
+
+int CoupledNorm(const std::vector<double>& v, double& norm) {  // CoupledConstraintNorm (:140-149)
+  norm = 0.0;
+  for (double x : v) norm = std::hypot(norm, x);
+  return std::isfinite(norm) ? FLO_OK : FLO_OUT_OF_RANGE;
+}
+
+double PlainDot(const std::vector<double>& a, const std::vector<double>& b) {
+  return std::inner_product(a.begin(), a.end(), b.begin(), 0.0);
+}
+
+constexpr size_t kCoupledRestart = 32;  // kCoupledPressureRestartLength3D
+
+// SolveCoupled (mac_viscous_pressure_3d.cc:151-459): GCR on the pressure with the Jacobi diagonal of
+// the viscous Schur complement, each pressure iterate's velocity from the velocity block
+// (M + dt E^T W E) u = base + dt A (p_tail - p_head) solved to 1e-10 of its largest right-hand side.
+int SolveCoupled(const System& sys, double dt, std::vector<flo_strain_row> rows, const std::vector<double>& wall_u,
+                 uint64_t max_iterations, std::vector<double>& p, std::vector<double>& velocity,
+                 std::vector<double>& defects, flo_viscous_diagnostics& out) {
+  const size_t nc = sys.cells.size(), nf = sys.links.size();
+  // RemapStrain (:50-77): grid slots to link ordinals (the links are slot-sorted).
+  for (auto& row : rows) {
+    if (row.term_count == 0 || row.term_count > 8) return FLO_INVALID_ARGUMENT;
+    if (!std::isfinite(row.quadrature_weight) || row.quadrature_weight < 0.0 || !std::isfinite(row.prescribed_strain))
+      return FLO_INVALID_ARGUMENT;
+    for (uint64_t t = 0; t < row.term_count; ++t) {
+      auto& term = row.terms[t];
+      auto it = std::lower_bound(sys.links.begin(), sys.links.end(), term.face_slot,
+                                 [](const Link& l, uint64_t slot) { return l.slot < slot; });
+      if (it == sys.links.end() || it->slot != term.face_slot || !std::isfinite(term.coefficient))
+        return FLO_INVALID_ARGUMENT;
+      term.face_slot = static_cast<uint64_t>(it - sys.links.begin());
+    }
+  }
+  std::vector<double> inertia(nf, 0.0), base(nf, 0.0), vdiag(nf, 0.0), pdiag(nc, 0.0);
+  for (size_t c = 0; c < nc; ++c) pdiag[c] = sys.cells[c].compliance;
+  for (size_t f = 0; f < nf; ++f) {
+    const Link& l = sys.links[f];
+    if (l.open_area == 0.0) continue;
+    const double step_area = dt * l.open_area;
+    inertia[f] = step_area / l.scale;
+    if (!std::isnormal(inertia[f]) || inertia[f] <= 0.0) return FLO_OUT_OF_RANGE;
+    double lift = 0.0;
+    if (l.kind == 1) lift = l.lift;
+    else if (l.kind == 2) lift = l.tail != kNone ? -l.lift : l.lift;
+    base[f] = inertia[f] * l.predicted + step_area * lift;
+  }
+  vdiag = inertia;
+  for (const auto& row : rows)
+    for (uint64_t t = 0; t < row.term_count; ++t) {
+      const auto& term = row.terms[t];
+      if (inertia[term.face_slot] > 0.0)
+        vdiag[term.face_slot] += dt * row.quadrature_weight * term.coefficient * term.coefficient;
+    }
+  for (size_t f = 0; f < nf; ++f) {
+    if (inertia[f] == 0.0) continue;
+    const Link& l = sys.links[f];
+    const double effective = l.conductance * (inertia[f] / vdiag[f]);
+    if (!FinitePositive(effective)) return FLO_OUT_OF_RANGE;
+    ForIncident(l, [&](size_t c, double) { pdiag[c] += effective; });
+  }
+  // Eliminate prescribed velocities of the strain rows (:226-246).
+  for (const auto& row : rows) {
+    if (row.quadrature_weight == 0.0) continue;
+    double prescribed = row.prescribed_strain;
+    for (uint64_t t = 0; t < row.term_count; ++t)
+      if (inertia[row.terms[t].face_slot] == 0.0) prescribed += row.terms[t].coefficient * wall_u[row.terms[t].face_slot];
+    if (prescribed == 0.0) continue;
+    for (uint64_t t = 0; t < row.term_count; ++t)
+      if (inertia[row.terms[t].face_slot] > 0.0)
+        base[row.terms[t].face_slot] -= dt * row.quadrature_weight * prescribed * row.terms[t].coefficient;
+  }
+  for (double b : base)
+    if (!std::isfinite(b)) return FLO_OUT_OF_RANGE;
+  std::vector<char> pinned(nc, 0);
+  for (size_t c : sys.gauges) pinned[c] = 1;
+  std::vector<double> momentum(nf), u(nf);
+  double momentum_residual = 0.0;
+  auto solve_velocity = [&](const std::vector<double>& pressure, bool include_base) -> int {
+    for (size_t f = 0; f < nf; ++f) {
+      const Link& l = sys.links[f];
+      const double tail = l.tail != kNone ? pressure[l.tail] : 0.0;
+      const double head = l.head != kNone ? pressure[l.head] : 0.0;
+      momentum[f] = include_base ? base[f] : 0.0;
+      if (l.open_area > 0.0) momentum[f] += dt * l.open_area * (tail - head);
+    }
+    uint64_t iterations = 0;
+    ++out.velocity_solves;
+    return flo_solve_viscous_block(rows.data(), rows.size(), inertia.data(), nf, dt, momentum.data(), 256, 1e-10,
+                                   u.data(), &iterations, &momentum_residual);
+  };
+  // InitializeMacPressure3D (mac_pressure_3d_multigrid.cc:303-321)
+  std::vector<double> pin(nc, 0.0);
+  for (size_t c : sys.gauges) pin[sys.component[c]] = sys.cells[c].previous;
+  p.assign(nc, 0.0);
+  bool history = false;
+  for (size_t c = 0; c < nc; ++c) {
+    p[c] = sys.cells[c].previous - pin[sys.component[c]];
+    if (!std::isfinite(p[c])) return FLO_OUT_OF_RANGE;
+    history |= p[c] != 0.0;
+  }
+  std::vector<double> residual, response;
+  double rhs_norm = 0.0;
+  if (history) {
+    const std::vector<double> zero(nc, 0.0);
+    if (int s = solve_velocity(zero, true)) return s;
+    if (int s = CoupledConstraint(sys, zero, u, true, residual)) return s;
+    if (int s = CoupledNorm(residual, rhs_norm)) return s;
+    if (rhs_norm == 0.0) std::fill(p.begin(), p.end(), 0.0);
+  }
+  const size_t capacity = std::min<size_t>(kCoupledRestart, max_iterations);
+  std::vector<std::vector<double>> directions(capacity, std::vector<double>(nc)), responses(capacity,
+                                                                                           std::vector<double>(nc));
+  size_t basis = 0;
+  double integrated_limit = 0.0;
+  const double volume_limit = 2e-5;
+  for (uint64_t it = 0; it <= max_iterations; ++it) {
+    if (int s = solve_velocity(p, true)) return s;
+    if (int s = CoupledConstraint(sys, p, u, true, residual)) return s;
+    double norm = 0.0;
+    if (int s = CoupledNorm(residual, norm)) return s;
+    if (it == 0) {
+      if (!history) rhs_norm = norm;
+      // MacPressureStoppingCriteria (mac_pressure_3d_multigrid.cc:341-357)
+      integrated_limit = 2e-6 + 2e-6 * rhs_norm;
+      if (!std::isfinite(rhs_norm) || rhs_norm < 0.0 || !std::isfinite(integrated_limit)) return FLO_OUT_OF_RANGE;
+      bool jump = false;
+      for (const auto& l : sys.links) jump |= l.conductance > 0.0 && l.kind == 1 && l.lift != 0.0;
+      if (rhs_norm > 0.0 && jump)
+        integrated_limit =
+            std::min(integrated_limit, std::max(std::numeric_limits<double>::denorm_min(), 1e-3 * rhs_norm));
+    }
+    double max_defect = 0.0;
+    for (size_t c = 0; c < nc; ++c) {
+      const double d = residual[c] / sys.cells[c].volume;
+      if (!std::isfinite(d)) return FLO_OUT_OF_RANGE;
+      max_defect = std::max(max_defect, std::fabs(d));
+    }
+    out.pressure_iterations = it;
+    out.constraint_residual_l2 = norm;
+    out.initial_constraint_residual_l2 = rhs_norm;
+    out.max_cell_constraint_defect = max_defect;
+    out.max_momentum_residual = momentum_residual;
+    if (max_defect <= volume_limit && norm <= integrated_limit) {
+      defects.resize(nc);
+      for (size_t c = 0; c < nc; ++c) defects[c] = residual[c] / sys.cells[c].volume;
+      velocity = u;
+      for (size_t f = 0; f < nf; ++f)
+        if (inertia[f] == 0.0) velocity[f] = wall_u[f];
+      return FLO_OK;
+    }
+    if (it == max_iterations) break;
+    if (basis == capacity) basis = 0;
+    std::vector<double>& dir = directions[basis];
+    std::vector<double>& resp = responses[basis];
+    for (size_t c = 0; c < nc; ++c) {
+      if (pinned[c]) {
+        residual[c] = 0.0;
+        dir[c] = 0.0;
+      } else {
+        if (!FinitePositive(pdiag[c])) return FLO_OUT_OF_RANGE;
+        dir[c] = residual[c] / pdiag[c];
+      }
+    }
+    if (int s = solve_velocity(dir, false)) return s;
+    if (int s = CoupledConstraint(sys, dir, u, false, resp)) return s;
+    for (size_t c = 0; c < nc; ++c)
+      if (pinned[c]) resp[c] = 0.0;
+    for (size_t q = 0; q < basis; ++q) {
+      const double overlap = PlainDot(resp, responses[q]);
+      for (size_t c = 0; c < nc; ++c) {
+        resp[c] -= overlap * responses[q][c];
+        dir[c] -= overlap * directions[q][c];
+      }
+    }
+    const double rn = std::sqrt(PlainDot(resp, resp));
+    if (!std::isfinite(rn) || rn <= 0.0) return FLO_ABORTED;
+    for (size_t c = 0; c < nc; ++c) {
+      resp[c] /= rn;
+      dir[c] /= rn;
+    }
+    const double step = PlainDot(residual, resp);
+    for (size_t c = 0; c < nc; ++c) {
+      p[c] -= step * dir[c];
+      if (!std::isfinite(p[c])) return FLO_OUT_OF_RANGE;
+    }
+    ++basis;
+  }
+  return FLO_ABORTED;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1268,6 +1493,37 @@ int flo_reference_vcycle(const flo_geometry* geometry, const flo_int3* sorted_bl
   std::vector<double> rv(r, r + n), zv;
   if (int s = AggVcycle<double>(H, 0, rv, zv)) return s;
   std::copy(zv.begin(), zv.end(), z);
+  return FLO_OK;
+}
+
+int flo_project_viscous(const flo_geometry* geometry, const flo_int3* sorted_blocks, size_t block_count, double dt,
+                        const flo_pressure_cell* cells, size_t cell_count, const flo_pressure_face* faces,
+                        size_t face_count, const flo_strain_row* rows, size_t row_count, uint64_t max_iterations,
+                        double* out_pressures, double* out_velocities, double* out_defects,
+                        flo_viscous_diagnostics* out_diagnostics) {
+  // ProjectStandaloneUniform (mac_viscous_pressure_3d.cc:558-575), ProjectUniformWithVelocityView (:461-507)
+  if (max_iterations == 0 || max_iterations > 256) return FLO_INVALID_ARGUMENT;
+  System sys;
+  if (int s = Prepare(*geometry, sorted_blocks, block_count, dt, cells, cell_count, faces, face_count, sys)) return s;
+  std::vector<double> wall_u(face_count);
+  Comp inward;
+  for (size_t f = 0; f < face_count; ++f) {
+    wall_u[f] = faces[f].wall_normal_velocity;
+    const Link& l = sys.links[f];
+    // AccumulateInwardExplicitWallFlux (mac_pressure_3d_arithmetic.h:97-108)
+    if (faces[f].has_prescribed_wall_flux && (l.tail != kNone) != (l.head != kNone))
+      ForIncident(l, [&](size_t, double sign) { inward.Add(-sign * l.wall_flux); });
+  }
+  flo_viscous_diagnostics out{};
+  std::vector<double> p, u, d;
+  const int s = SolveCoupled(sys, dt, std::vector<flo_strain_row>(rows, rows + row_count), wall_u, max_iterations, p,
+                             u, d, out);
+  out.inward_prescribed_wall_flux = inward.Total();
+  if (out_diagnostics) *out_diagnostics = out;
+  if (s) return s;
+  if (out_pressures) std::copy(p.begin(), p.end(), out_pressures);
+  if (out_velocities) std::copy(u.begin(), u.end(), out_velocities);
+  if (out_defects) std::copy(d.begin(), d.end(), out_defects);
   return FLO_OK;
 }
 

@@ -704,6 +704,25 @@ def solve_viscous_block(rows, inertia, dt, rhs, max_iterations=256, relative_tol
     return s, v, it.value, mr.value
 
 
+VISCOUS_DIAGNOSTICS = np.dtype([("pressure_iterations", "u8"), ("constraint_residual_l2", "f8"),
+                                ("initial_constraint_residual_l2", "f8"), ("max_cell_constraint_defect", "f8"),
+                                ("max_momentum_residual", "f8"), ("inward_prescribed_wall_flux", "f8"),
+                                ("velocity_solves", "u8")])
+
+
+def project_viscous(geom, blocks, dt, cells, faces, rows, max_iterations=256):
+    """ProjectMacViscousPressure3D (native/src/mac_viscous_pressure_3d.cc:151-459): (status, dict of
+    pressures, face velocities, cell constraint defects and diagnostics)."""
+    nc, nf = len(cells), len(faces)
+    rows = np.ascontiguousarray(rows, dtype=STRAIN_ROW)
+    p, u, d = np.zeros(nc), np.zeros(nf), np.zeros(nc)
+    diag = np.zeros(1, VISCOUS_DIAGNOSTICS)
+    s = lib().flo_project_viscous(_p(geom), _p(blocks), C.c_size_t(len(blocks)), C.c_double(dt), _p(cells),
+                                  C.c_size_t(nc), _p(faces), C.c_size_t(nf), _p(rows), C.c_size_t(len(rows)),
+                                  C.c_uint64(max_iterations), _p(p), _p(u), _p(d), _p(diag))
+    return s, dict(pressures=p, velocities=u, defects=d, diagnostics=diag[0])
+
+
 def strain_row(terms, weight, first_axis=0, second_axis=0):
     """One strain row from [(slot, coefficient), ...] and its quadrature weight."""
     r = np.zeros(1, STRAIN_ROW)
