@@ -517,6 +517,19 @@ int stage_particles(fl_ctx* ctx, const fl_particle* particles, size_t n, bool de
 
 }  // namespace
 
+namespace fl {
+int viscosity_reserve(fl_ctx* ctx, uint64_t n, uint64_t nr, uint64_t nf, ViscosityDevice* out) {
+  FL_TRY(grow_viscosity(ctx, std::max<uint64_t>(n, 1), std::max<uint64_t>(nr, 1), std::max<uint64_t>(nf, 1)));
+  *out = viscosity_device(ctx);
+  return FL_OK;
+}
+int viscosity_reserve_terms(fl_ctx* ctx, uint64_t nt, ViscosityDevice* out) {
+  FL_TRY(grow_viscosity_terms(ctx, nt));
+  *out = viscosity_device(ctx);
+  return FL_OK;
+}
+}  // namespace fl
+
 extern "C" {
 
 int fl_create(const fl_capacities* capacities, int device, fl_ctx** out) {
@@ -1118,6 +1131,7 @@ int fl_freeze_geometry(fl_ctx* ctx, const fl_freeze_inputs* inputs, fl_frozen_ce
   // grid's size and finite. The reference checks the centre and corner samples first; a failure of
   // either is InvalidArgument too, so the order does not change the status.
   const bool sampled = I.has_sampled_solid != 0;
+  if (I.has_sampled_solid > 3) return FL_INVALID_ARGUMENT;
   if (sampled) {
     if ((I.solid_corner_count && !I.solid_corners) || (I.face_velocity_count && !I.face_normal_velocities))
       return FL_INVALID_ARGUMENT;
@@ -1190,7 +1204,7 @@ int fl_freeze_geometry(fl_ctx* ctx, const fl_freeze_inputs* inputs, fl_frozen_ce
                        B.faces,      B.needs,
                        B.counts,     B.phi,
                        B.work,       B.uncut,
-                       sampled ? 1 : 0, sampled ? B.solids : nullptr,
+                       sampled ? static_cast<int>(I.has_sampled_solid) : 0, sampled ? B.solids : nullptr,
                        sampled ? I.solid_corner_count : 0, sampled ? B.face_velocities : nullptr,
                        sampled ? I.face_velocity_count : 0, sampled ? B.solid_phi : nullptr};
   const auto read_counts = [&]() -> int {

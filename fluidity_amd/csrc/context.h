@@ -364,6 +364,10 @@ struct CoupledSystem {
 int coupled_prepare(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count, double dt,
                     const fl_pressure_cell* cells, size_t cell_count, const fl_pressure_face* faces,
                     size_t face_count, const fl_pressure_options* options, CoupledSystem* out);
+// The same for the typed links of a frozen geometry already on the device (the substep's records), with
+// the system and hierarchy built on the device as for project_links_device.
+int coupled_prepare_device(fl_ctx* ctx, const fl_pressure_cell* cells, uint64_t nc, const fl_pressure_face* faces,
+                           uint64_t nf, double dt, const fl_pressure_options& options, CoupledSystem* out);
 // Re-derives the hierarchy's coefficients from the given link conductances (the effective ones of the
 // viscous Schur complement's diagonal).
 int coupled_coefficients(fl_ctx* ctx, const double* conductance);
@@ -385,6 +389,10 @@ int coupled_solve(fl_ctx* ctx, const CoupledSystem& S, const ViscosityDevice& v,
                   const fl_pressure_options& options, bool jacobi, CoupledResult* out);
 int coupled_outputs(fl_ctx* ctx, const double** p, const double** u, const double** defects);
 void release_coupled(fl_ctx* ctx);
+// The context's viscosity arrays (capi.hip) sized for n cells, nr rows and nf faces, then for nt packed
+// terms; *out views them.
+int viscosity_reserve(fl_ctx* ctx, uint64_t n, uint64_t nr, uint64_t nf, ViscosityDevice* out);
+int viscosity_reserve_terms(fl_ctx* ctx, uint64_t nt, ViscosityDevice* out);
 
 // The device substep (substep.hip).
 int substep(fl_ctx* ctx, const fl_substep_options& o, fl_substep_report& rep);

@@ -560,8 +560,14 @@ __device__ int wet_connectivity(const Vtx (&x)[8], const double (&s)[8], const b
 }
 
 // MeasureSampledSolidLiquidCell: wet volume and the six open face areas (low/high x, y, z).
+// split_rule: 1 the reference's wet-piece connectivity refusal; 2 only where a corner lies in the solid
+// (sampled distance <= 0 or on the analytic wall's side), so liquid that the particle field itself
+// splits inside a solid-free cell is measured as in the plane-obstacle freeze instead of refused; 3 no
+// refusal (the wet pieces are one cell).
 __device__ int measure_sampled_cell(double width, const double (&liquid)[8], const double (&solid)[8],
-                                    const Plane* analytic, double& wet_volume, double (&open)[6]) {
+                                    const Plane* analytic, double& wet_volume, double (&open)[6],
+                                    int split_rule = 1) {
+  const bool solid_split_only = split_rule == 2;
   if (!finite(width) || width <= 0) return kInvalidArgument;
   const double full_area = width * width, full_volume = full_area * width;
   if (!normal(full_area) || !normal(full_volume)) return kOutOfRange;
@@ -612,7 +618,11 @@ __device__ int measure_sampled_cell(double width, const double (&liquid)[8], con
   }
   if (!normal_or_zero(wet) || wet / full_volume > 1 + kMeasureSlack) return kOutOfRange;
   wet = min_std(wet, full_volume);
-  if (int code = wet_connectivity(x, scaled, wet_tet, width, analytic)) return code;
+  bool touched = !solid_split_only;
+  for (int k = 0; k < 8 && !touched; ++k)
+    touched = scaled[k] <= 0 || (analytic != nullptr && distance(x[k], *analytic) >= 0);
+  if (touched && split_rule != 3)
+    if (int code = wet_connectivity(x, scaled, wet_tet, width, analytic)) return code;
   double areas[6];
   for (int axis = 0; axis < 3; ++axis) {
     const int bit = 1 << axis;
@@ -895,7 +905,7 @@ __global__ void __launch_bounds__(64) k_freeze_cut(FreezeDevice d, uint64_t* err
     double solid[8], open[6];
     for (int k = 0; k < 8; ++k) solid[k] = d.solid_phi[8 * i + k];
     const Plane analytic{scale(P.normal, -1), -P.high, false};
-    if (int code = measure_sampled_cell(d.h, phi, solid, P.present ? &analytic : nullptr, high, open))
+    if (int code = measure_sampled_cell(d.h, phi, solid, P.present ? &analytic : nullptr, high, open, d.sampled))
       return fail(err, 2, i, code);
     return consume_cell(d, i, c, d.phi[9 * i + 8], 0.0, high, err);
   }
@@ -1011,7 +1021,9 @@ __device__ int sampled_face(const FreezeDevice& d, fl_int3 l, int axis, uint64_t
   const Plane analytic3{scale(P.normal, -1), -P.high, false};
   const double liquid[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   double wet = 0, open[6];
-  if (int code = measure_sampled_cell(width, liquid, solid, P.present ? &analytic3 : nullptr, wet, open)) return code;
+  if (int code = measure_sampled_cell(width, liquid, solid, P.present ? &analytic3 : nullptr, wet, open,
+                                      d.sampled))
+    return code;
   const double face_open = open[2 * axis];
   const double face_blocked = width * width - face_open;
   double face_solid[4], face_velocity[4];

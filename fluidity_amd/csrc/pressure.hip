@@ -3046,6 +3046,8 @@ __global__ void k_store_cells(uint64_t nc, const int32_t* cell_fine, const uint8
 }
 }  // namespace
 
+void coupled_view(Pressure& P, CoupledSystem* out);
+
 int coupled_prepare(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blocks, size_t block_count, double dt,
                     const fl_pressure_cell* cells, size_t cell_count, const fl_pressure_face* faces,
                     size_t face_count, const fl_pressure_options* options, CoupledSystem* out) {
@@ -3060,6 +3062,41 @@ int coupled_prepare(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blo
   EvalResult e0{};
   fl_pressure_diagnostics diag{};
   FL_TRY(freeze_and_assemble(ctx, P, dt, *options, gates, e0, diag));
+  coupled_view(P, out);
+  return FL_OK;
+}
+
+int coupled_prepare_device(fl_ctx* ctx, const fl_pressure_cell* cells, uint64_t nc, const fl_pressure_face* faces,
+                           uint64_t nf, double dt, const fl_pressure_options& opt, CoupledSystem* out) {
+  FL_TRY(check_options(&opt));
+  if (!ctx->pressure) ctx->pressure = new Pressure();
+  Pressure& P = *state(ctx);
+  P.scale_coarse = opt.coarse_scale;
+  bool fallback = false;
+  FL_TRY(prepare_system_device(ctx, P, cells, nc, faces, nf, dt, opt.levels, &fallback));
+  if (fallback) {  // a component without an anchor: its compatibility test and gauge on the host
+    hipStream_t s = ctx->stream;
+    std::vector<fl_pressure_cell> hc(nc);
+    std::vector<fl_pressure_face> hf(nf);
+    std::vector<fl_int3> hb(ctx->nb);
+    FL_HIP(hipMemcpyAsync(hc.data(), cells, nc * sizeof(fl_pressure_cell), hipMemcpyDeviceToHost, s));
+    FL_HIP(hipMemcpyAsync(hf.data(), faces, nf * sizeof(fl_pressure_face), hipMemcpyDeviceToHost, s));
+    if (ctx->nb) FL_HIP(hipMemcpyAsync(hb.data(), ctx->blocks, ctx->nb * sizeof(fl_int3), hipMemcpyDeviceToHost, s));
+    FL_HIP(hipStreamSynchronize(s));
+    const fl_geometry g{{ctx->geometry.ox, ctx->geometry.oy, ctx->geometry.oz}, ctx->geometry.h};
+    FL_TRY(prepare_system(ctx, P, &g, hb.data(), hb.size(), dt, hc.data(), nc, hf.data(), nf, true));
+    std::vector<Key3> b0 = P.levels[0].keys;
+    FL_TRY(build_levels(ctx, P, b0, opt.levels));
+  }
+  Gates gates{};
+  EvalResult e0{};
+  fl_pressure_diagnostics diag{};
+  FL_TRY(freeze_and_assemble(ctx, P, dt, opt, gates, e0, diag));
+  coupled_view(P, out);
+  return FL_OK;
+}
+
+void coupled_view(Pressure& P, CoupledSystem* out) {
   CoupledSystem S{};
   S.nc = P.nc;
   S.nf = P.nf;
@@ -3084,7 +3121,6 @@ int coupled_prepare(fl_ctx* ctx, const fl_geometry* geometry, const fl_int3* blo
   S.p_init = P.p_init;
   S.p_init_zero = P.p_init_zero;
   *out = S;
-  return FL_OK;
 }
 
 int coupled_coefficients(fl_ctx* ctx, const double* conductance) {

@@ -7,6 +7,7 @@
 // particles (ASFLIP, advection, contact, volume ratio). Every stage runs on the context's stream on
 // device arrays; the host reads only counts and error words between stages. At the end the sorted
 // particle columns become the scene's particles, so the next substep sorts the moved particles.
+#include <cmath>
 #include <cstring>
 
 #include "context.h"
@@ -32,13 +33,17 @@ struct SubstepBuffers {
   double *ratio = nullptr, *ratio_sorted = nullptr;
   uint8_t *flags = nullptr, *flags_sorted = nullptr;
   uint32_t *contact = nullptr, *contact_count = nullptr;  // particles near a wall for the contact pass
-  uint64_t cap[24] = {};
+  // sampled solid: the solid's corner samples at the field's corner keys, and the freeze's per-cell copy
+  fl_solid_corner* solids = nullptr;
+  double* solid_phi = nullptr;
+  uint64_t cap[26] = {};
   hipEvent_t ev[12] = {};
   bool fresh = true;  // the scene was uploaded since the last substep: volume ratios 1, PIC unset
   void release() {
     void* ptrs[] = {keep,   slots,  word_counts, face_slots, counts, work,      words,        staged,
                     cells,  faces,  needs,       phi,        uncut,  fields,    predicted,    pcells,
-                    pfaces, lower,  ratio,       ratio_sorted, flags, flags_sorted, contact, contact_count};
+                    pfaces, lower,  ratio,       ratio_sorted, flags, flags_sorted, contact, contact_count,
+                    solids, solid_phi};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     for (hipEvent_t e : ev)
@@ -99,6 +104,71 @@ __global__ void k_predicted(const fl_frozen_face* faces, uint64_t n, const doubl
   if (i < n) out[i] = ustar[faces[i].face_slot];
 }
 
+// The scene's level-set solids at the field's corner keys: signed distance (positive outside) of the
+// union of the selected parts and the wall velocity of the nearest part. cos_a, sin_a: the paddle's
+// turn about +y (body = R^T (x - pivot)).
+struct SolidParams {
+  uint32_t parts;
+  double sx, sy, sz, sr;
+  double px, py, pz, ex, ey, ez, cos_a, sin_a, omega;
+  double wlx, wlz, whx, whz;
+};
+
+__global__ void k_sample_solid(const fl_center_distance* corners, uint64_t n, double ox, double oy, double oz, double h,
+                               SolidParams sp, fl_solid_corner* out) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const fl_int3 k = corners[i].cell_index;
+  const double x = ox + k.x * h, y = oy + k.y * h, z = oz + k.z * h;
+  double best = __builtin_inf();
+  double vx = 0.0, vz = 0.0;
+  if (sp.parts & 1u) {
+    const double dx = x - sp.sx, dy = y - sp.sy, dz = z - sp.sz;
+    best = sqrt(dx * dx + dy * dy + dz * dz) - sp.sr;
+  }
+  if (sp.parts & 2u) {
+    const double rx = x - sp.px, ry = y - sp.py, rz = z - sp.pz;
+    const double bx = sp.cos_a * rx - sp.sin_a * rz, bz = sp.sin_a * rx + sp.cos_a * rz;
+    const double qx = fabs(bx) - sp.ex, qy = fabs(ry) - sp.ey, qz = fabs(bz) - sp.ez;
+    const double mx = fmax(qx, 0.0), my = fmax(qy, 0.0), mz = fmax(qz, 0.0);
+    const double d = sqrt(mx * mx + my * my + mz * mz) + fmin(fmax(qx, fmax(qy, qz)), 0.0);
+    if (d < best) {
+      best = d;
+      vx = sp.omega * rz;
+      vz = -sp.omega * rx;
+    }
+  }
+  if (sp.parts & 4u) {
+    const double d = fmin(fmin(x - sp.wlx, sp.whx - x), fmin(z - sp.wlz, sp.whz - z));
+    if (d < best) {
+      best = d;
+      vx = 0.0;
+      vz = 0.0;
+    }
+  }
+  fl_solid_corner c{};
+  c.corner_index = k;
+  c.signed_distance = best;
+  c.wall_velocity = fl_vec3{vx, 0.0, vz};
+  out[i] = c;
+}
+
+// MacViscosityCell3D of the pressure cells with one dynamic viscosity.
+__global__ void k_viscosity_cells(const fl_pressure_cell* cells, uint64_t n, double mu, fl_viscosity_cell* out) {
+  const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  fl_viscosity_cell c{};
+  c.cell_index = cells[i].cell_index;
+  c.dynamic_viscosity = mu;
+  out[i] = c;
+}
+
+// Accepted link velocities into their grid slots.
+__global__ void k_scatter_links(const uint64_t* slot, const double* u, uint64_t nf, double* unew) {
+  const uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (f < nf) unew[slot[f]] = u[f];
+}
+
 __global__ void k_copy_slots(const double* a, uint64_t n, double* b) {
   const uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (i < n) b[i] = a[i];
@@ -137,6 +207,26 @@ int check_substep_options(const fl_substep_options* o) {
   if (!finite(o->particle_energy_blend) || o->particle_energy_blend < 0 || o->particle_energy_blend > 1)
     return FL_INVALID_ARGUMENT;
   if (o->obstacle.kind > FL_OBSTACLE_TANK) return FL_INVALID_ARGUMENT;
+  if (!(o->dynamic_viscosity >= 0.0) || !finite(o->dynamic_viscosity)) return FL_INVALID_ARGUMENT;
+  const fl_scene_solid& q2 = o->solid;
+  if (q2.parts > 7 || q2.split_rule > 3) return FL_INVALID_ARGUMENT;
+  if (q2.parts) {
+    // The sampled-solid freeze takes an absent or half-space obstacle.
+    if (o->obstacle.kind != FL_OBSTACLE_ABSENT && o->obstacle.kind != FL_OBSTACLE_HALFSPACE) return FL_INVALID_ARGUMENT;
+    const double v[] = {q2.sphere_centre.x, q2.sphere_centre.y, q2.sphere_centre.z, q2.sphere_radius,
+                        q2.paddle_pivot.x, q2.paddle_pivot.y, q2.paddle_pivot.z, q2.paddle_half_extent.x,
+                        q2.paddle_half_extent.y, q2.paddle_half_extent.z, q2.paddle_angle,
+                        q2.paddle_angular_velocity, q2.walls_low.x, q2.walls_low.z, q2.walls_high.x,
+                        q2.walls_high.z};
+    for (double a : v)
+      if (!finite(a)) return FL_INVALID_ARGUMENT;
+    if ((q2.parts & 1u) && !(q2.sphere_radius > 0.0)) return FL_INVALID_ARGUMENT;
+    if ((q2.parts & 2u) && !(q2.paddle_half_extent.x > 0.0 && q2.paddle_half_extent.y > 0.0 &&
+                             q2.paddle_half_extent.z > 0.0))
+      return FL_INVALID_ARGUMENT;
+    if ((q2.parts & 4u) && !(q2.walls_low.x < q2.walls_high.x && q2.walls_low.z < q2.walls_high.z))
+      return FL_INVALID_ARGUMENT;
+  }
   return FL_OK;
 }
 
@@ -300,22 +390,46 @@ int substep(fl_ctx* ctx, const fl_substep_options& o, fl_substep_report& rep) {
                         nullptr,
                         0,
                         nullptr};
+  FreezeDevice fd_run = fd;
+  if (o.solid.parts) {
+    FL_TRY(grow_to(&B.solids, &B.cap[24], F.corner_count));
+    FL_TRY(grow_to(&B.solid_phi, &B.cap[25], 8 * ncand));
+    const fl_scene_solid& q2 = o.solid;
+    SolidParams sp{};
+    sp.parts = q2.parts;
+    sp.sx = q2.sphere_centre.x, sp.sy = q2.sphere_centre.y, sp.sz = q2.sphere_centre.z, sp.sr = q2.sphere_radius;
+    sp.px = q2.paddle_pivot.x, sp.py = q2.paddle_pivot.y, sp.pz = q2.paddle_pivot.z;
+    sp.ex = q2.paddle_half_extent.x, sp.ey = q2.paddle_half_extent.y, sp.ez = q2.paddle_half_extent.z;
+    sp.cos_a = std::cos(q2.paddle_angle);
+    sp.sin_a = std::sin(q2.paddle_angle);
+    sp.omega = q2.paddle_angular_velocity;
+    sp.wlx = q2.walls_low.x, sp.wlz = q2.walls_low.z, sp.whx = q2.walls_high.x, sp.whz = q2.walls_high.z;
+    if (F.corner_count)
+      k_sample_solid<<<grid_for(F.corner_count), kThreads, 0, s>>>(F.corners, F.corner_count, ctx->geometry.ox,
+                                                                   ctx->geometry.oy, ctx->geometry.oz, h, sp,
+                                                                   B.solids);
+    FL_HIP(hipGetLastError());
+    fd_run.sampled = q2.split_rule == 0 ? 1 : static_cast<int>(q2.split_rule);
+    fd_run.solids = B.solids;
+    fd_run.nsolids = F.corner_count;
+    fd_run.solid_phi = B.solid_phi;
+  }
   auto read_counts = [&]() -> int {
     FL_HIP(hipMemcpyAsync(ctx->host_words, B.counts, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     FL_HIP(hipStreamSynchronize(s));
     return FL_OK;
   };
   FL_TRY(reset_error(ctx));
-  STAGE(4, freeze_cells_device(ctx, fd));
+  STAGE(4, freeze_cells_device(ctx, fd_run));
   STAGE(4, error_code(ctx));
   FL_TRY(read_counts());
   const uint64_t nc = ncand ? ctx->host_words[0] : 0, nf = ncand ? ctx->host_words[1] : 0;
   if (nc == 0) return fail(4, FL_FAILED_PRECONDITION);
-  STAGE(4, freeze_faces_device(ctx, fd, nc, nf, o.surface_tension != 0));
+  STAGE(4, freeze_faces_device(ctx, fd_run, nc, nf, o.surface_tension != 0));
   STAGE(4, error_code(ctx));
   FL_TRY(read_counts());
   const bool capillary = ctx->host_words[2] != 0;
-  STAGE(4, freeze_curvature_device(ctx, fd, nc, false, 0.0));
+  STAGE(4, freeze_curvature_device(ctx, fd_run, nc, false, 0.0));
   STAGE(4, error_code(ctx));
   rep.capillary_step_bound = __builtin_inf();
   if (capillary) {
@@ -357,7 +471,42 @@ int substep(fl_ctx* ctx, const fl_substep_options& o, fl_substep_report& rep) {
   k_copy_slots<<<grid_for(nslots), kThreads, 0, s>>>(ctx->ustar, nslots, ctx->unew);
   FL_HIP(hipGetLastError());
   FL_TRY(reset_error(ctx));
-  STAGE(6, project_links_device(ctx, B.pcells, nc, B.pfaces, nf, dt, o.pressure, ctx->unew, &rep.pressure));
+  if (o.dynamic_viscosity > 0.0) {
+    // The coupled viscous pressure solve on the full-cell strain rows of the pressure cells.
+    // preconditioner 2: the reference's Jacobi diagonal; otherwise the V-cycle of the effective conductances.
+    const bool jacobi = o.pressure.preconditioner == 2;
+    fl_pressure_options popt = o.pressure;
+    popt.preconditioner = 0;
+    CoupledSystem S{};
+    STAGE(6, coupled_prepare_device(ctx, B.pcells, nc, B.pfaces, nf, dt, popt, &S));
+    ViscosityDevice V{};
+    STAGE(6, viscosity_reserve(ctx, nc, 6 * nc, nf, &V));
+    k_viscosity_cells<<<grid_for(nc), kThreads, 0, s>>>(B.pcells, nc, o.dynamic_viscosity,
+                                                        const_cast<fl_viscosity_cell*>(V.cells));
+    FL_HIP(hipGetLastError());
+    uint64_t nr = 0, nt = 0;
+    STAGE(6, assemble_strain(ctx, V, nc, h, &nr));
+    rep.strain_rows = nr;
+    STAGE(6, coupled_rows(ctx, S, V, nr, dt));
+    STAGE(6, weight_viscous_rows(ctx, V, nr, nf, dt, &nt));
+    STAGE(6, viscosity_reserve_terms(ctx, nt, &V));
+    CoupledResult cr{};
+    const int cst = coupled_solve(ctx, S, V, nr, nt, dt, popt, jacobi, &cr);
+    rep.viscous.pressure_iterations = cr.iterations;
+    rep.viscous.constraint_residual_l2 = cr.residual_l2;
+    rep.viscous.initial_constraint_residual_l2 = cr.initial_residual_l2;
+    rep.viscous.max_cell_constraint_defect = cr.max_defect;
+    rep.viscous.max_momentum_residual = cr.max_momentum_residual;
+    rep.viscous.velocity_solves = cr.velocity_solves;
+    rep.viscous.velocity_iterations = cr.velocity_iterations;
+    STAGE(6, cst);
+    const double *cp = nullptr, *cu = nullptr, *cd = nullptr;
+    STAGE(6, coupled_outputs(ctx, &cp, &cu, &cd));
+    k_scatter_links<<<grid_for(nf), kThreads, 0, s>>>(S.slot, cu, nf, ctx->unew);
+    FL_HIP(hipGetLastError());
+  } else {
+    STAGE(6, project_links_device(ctx, B.pcells, nc, B.pfaces, nf, dt, o.pressure, ctx->unew, &rep.pressure));
+  }
   // 7. Gather (GatherMac3D of the old and new face velocities).
   FL_HIP(hipEventRecord(ev[6], s));
   FL_TRY(reset_error(ctx));

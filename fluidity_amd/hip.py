@@ -254,7 +254,8 @@ class Context:
         corners), an obstacle tuple (kind, unit normal, coordinate, half width, wall velocity) or None,
         scalars (step, density, sigma, exact_curvature or None, max_cells, max_faces) and optionally a
         sampled solid (solid_corners: SOLID_CORNER records or None; face_normal_velocities: None or one
-        per owned slot): (status, frozen cells, frozen faces, capillary step bound)."""
+        per owned slot; split_rule: the wet-piece refusal rule, 1 by default): (status, frozen cells, frozen
+        faces, capillary step bound)."""
         o = p.geom[0]["world_origin"]
         blocks, cands, centers, corners = (np.ascontiguousarray(a) for a in (p.blocks, p.candidates, p.centers,
                                                                                p.corners))
@@ -273,7 +274,7 @@ class Context:
         if solids is not None:
             solids = np.ascontiguousarray(solids, dtype=R.SOLID_CORNER)
             inputs.solid_corners, inputs.solid_corner_count = _p(solids), len(solids)
-            inputs.has_sampled_solid = 1
+            inputs.has_sampled_solid = int(getattr(p, "split_rule", 1))
             if velocities is not None:
                 velocities = np.ascontiguousarray(velocities, dtype=np.float64)
                 inputs.face_normal_velocities, inputs.face_velocity_count = _p(velocities), len(velocities)
@@ -570,8 +571,10 @@ def pressure_options(max_iterations=256, tolerance_scale=1.0, relative_target=0.
 
 def substep_options(dt, gravity=(0.0, -9.81, 0.0), mass_q=2.0 ** -40, mom_q=2.0 ** -40, capacity=1 << 20,
                     minimum_face_mass=0.0, field_radius=1.0, density=1000.0, surface_tension=0.0, ambient=0.0,
-                    blend=1.0, friction=0.0, penetration=0.0, obstacle=None, pressure=None):
-    """fl_substep_options record (obstacle: an OBSTACLE record; pressure: a pressure_options record)."""
+                    blend=1.0, friction=0.0, penetration=0.0, obstacle=None, pressure=None, solid=None,
+                    dynamic_viscosity=0.0):
+    """fl_substep_options record (obstacle: an OBSTACLE record; pressure: a pressure_options record;
+    solid: a SCENE_SOLID record, see scene_solid)."""
     o = np.zeros(1, R.SUBSTEP_OPTIONS)
     o["step_duration"] = dt
     o["gravity"] = tuple(gravity)
@@ -587,7 +590,31 @@ def substep_options(dt, gravity=(0.0, -9.81, 0.0), mass_q=2.0 ** -40, mom_q=2.0 
     if obstacle is not None:
         o["obstacle"] = obstacle
     o["pressure"] = pressure_options() if pressure is None else pressure
+    if solid is not None:
+        o["solid"] = solid
+    o["dynamic_viscosity"] = dynamic_viscosity
     return o
+
+
+def scene_solid(sphere=None, paddle=None, walls=None, split_rule=1):
+    """fl_scene_solid record: sphere = (centre, radius); paddle = (pivot, half extent, angle, angular
+    velocity about +y); walls = (low, high) bounds of the container in x and z; split_rule: the freeze's
+    wet-piece refusal (1 the reference's, 2 only in cells a solid reaches, 3 none)."""
+    r = np.zeros(1, R.SCENE_SOLID)
+    r["split_rule"] = split_rule
+    parts = 0
+    if sphere is not None:
+        parts |= 1
+        r["sphere_centre"], r["sphere_radius"] = tuple(sphere[0]), sphere[1]
+    if paddle is not None:
+        parts |= 2
+        r["paddle_pivot"], r["paddle_half_extent"] = tuple(paddle[0]), tuple(paddle[1])
+        r["paddle_angle"], r["paddle_angular_velocity"] = paddle[2], paddle[3]
+    if walls is not None:
+        parts |= 4
+        r["walls_low"], r["walls_high"] = tuple(walls[0]), tuple(walls[1])
+    r["parts"] = parts
+    return r
 
 
 STATIC_MAX_FACES = 196608

@@ -104,19 +104,23 @@ OBSTACLE = np.dtype([("kind", "u4"), ("pad", "u4"), ("unit_normal", VEC3), ("coo
 UPDATE_OPTIONS = np.dtype([("cell_width", "f8"), ("step_duration", "f8"), ("particle_energy_blend", "f8"),
                            ("friction_coefficient", "f8"), ("penetration_tolerance", "f8"), ("obstacle", OBSTACLE)])
 TOPOLOGY_CAPACITIES = np.dtype([("particles", "u8"), ("blocks", "u8"), ("cells", "u8"), ("faces", "u8")])
+# fl_scene_solid: the level-set solids sampled by the device substep.
+SCENE_SOLID = np.dtype([("parts", "u4"), ("split_rule", "u4"), ("sphere_centre", VEC3), ("sphere_radius", "f8"),
+                        ("paddle_pivot", VEC3), ("paddle_half_extent", VEC3), ("paddle_angle", "f8"),
+                        ("paddle_angular_velocity", "f8"), ("walls_low", VEC3), ("walls_high", VEC3)])
 # fl_substep_options / fl_substep_report (the device substep).
 SUBSTEP_OPTIONS = np.dtype([("step_duration", "f8"), ("gravity", VEC3), ("quantization", QUANTIZATION),
                             ("minimum_face_mass", "f8"), ("field_radius", "f8"), ("liquid_density", "f8"),
                             ("surface_tension", "f8"), ("ambient_pressure", "f8"), ("particle_energy_blend", "f8"),
                             ("friction_coefficient", "f8"), ("penetration_tolerance", "f8"), ("obstacle", OBSTACLE),
-                            ("pressure", PRESSURE_OPTIONS)])
+                            ("pressure", PRESSURE_OPTIONS), ("solid", SCENE_SOLID), ("dynamic_viscosity", "f8")])
 SUBSTEP_REPORT = np.dtype([("status", "i4"), ("stage", "i4"), ("error_ordinal", "u8"), ("particles", "u8"), ("blocks", "u8"),
                            ("candidates", "u8"), ("centre_samples", "u8"), ("corner_samples", "u8"),
                            ("frozen_cells", "u8"), ("frozen_faces", "u8"), ("lower_theta_count", "u8"),
                            ("capillary_step_bound", "f8"), ("pressure", PRESSURE_DIAGNOSTICS), ("sort_ms", "f8"),
                            ("field_ms", "f8"), ("deposit_ms", "f8"), ("freeze_ms", "f8"), ("links_ms", "f8"),
                            ("projection_ms", "f8"), ("gather_ms", "f8"), ("update_ms", "f8"), ("total_ms", "f8"),
-                           ("max_speed", "f8")])
+                           ("max_speed", "f8"), ("viscous", VISCOUS_DIAGNOSTICS), ("strain_rows", "u8")])
 
 SEALED, INTERNAL, DIRICHLET = 0, 1, 2
 LINK_KINDS = {"sealed": SEALED, "internal": INTERNAL, "dirichlet": DIRICHLET}
@@ -126,7 +130,7 @@ assert PRESSURE_CELL.itemsize == 56 and PRESSURE_FACE.itemsize == 80 and FACE_QU
 assert BLOCK_STATE.itemsize == 32 and CELL_STATE.itemsize == 24 and FACE_STATE.itemsize == 56
 assert LIQUID_PARTICLE.itemsize == 328 and UPDATE_OPTIONS.itemsize == 160 and OBSTACLE.itemsize == 120
 assert FROZEN_FACE.itemsize == 72 and FROZEN_CELL.itemsize == 40 and CELL_FIELDS.itemsize == 40
-assert SUBSTEP_OPTIONS.itemsize == 296 and SUBSTEP_REPORT.itemsize == 264
+assert SUBSTEP_OPTIONS.itemsize == 456 and SUBSTEP_REPORT.itemsize == 336 and SCENE_SOLID.itemsize == 152
 
 # MacViscosityCell3D, MacStrainTerm3D, MacStrainRow3D (native/include/fluidity/host/viscosity.h:47-66).
 VISCOSITY_CELL = np.dtype({"names": ["cell_index", "dynamic_viscosity"], "formats": [INT3, "f8"], "offsets": [0, 16],

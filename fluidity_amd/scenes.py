@@ -15,6 +15,8 @@ from __future__ import annotations
 
 import dataclasses
 
+import math
+
 import numpy as np
 
 from . import records as R
@@ -205,6 +207,58 @@ def two_spheres(n: int = 256, seed: int = 0) -> Scene:
     vel[in1, 0] = 2.0
     vel[in2, 0] = -2.0
     return Scene(f"two_spheres_{n}", n, seed, liquid, pos, vel, np.zeros((len(pos), 3, 3)), RHO * (1.0 / n) ** 3 / 8)
+
+
+def solid_distance(points, sphere=None, paddle=None, walls=None):
+    """Signed distance (positive outside) of the union of a sphere (centre, radius), a box paddle (pivot,
+    half extent, angle about +y, angular velocity) and the side walls of a container ((low, high) in x and
+    z), and the wall velocity of the nearest part; the same operations, in the same order, as the device
+    substep's sampling of the scene solid. points: (m, 3)."""
+    x, y, z = points[:, 0], points[:, 1], points[:, 2]
+    best = np.full(len(points), np.inf)
+    vx = np.zeros(len(points))
+    vz = np.zeros(len(points))
+    if sphere is not None:
+        (cx, cy, cz), r = sphere
+        dx, dy, dz = x - cx, y - cy, z - cz
+        best = np.sqrt(dx * dx + dy * dy + dz * dz) - r
+    if paddle is not None:
+        (px, py, pz), (ex, ey, ez), angle, omega = paddle
+        ca, sa = math.cos(angle), math.sin(angle)
+        rx, ry, rz = x - px, y - py, z - pz
+        bx, bz = ca * rx - sa * rz, sa * rx + ca * rz
+        qx, qy, qz = np.abs(bx) - ex, np.abs(ry) - ey, np.abs(bz) - ez
+        mx, my, mz = np.maximum(qx, 0.0), np.maximum(qy, 0.0), np.maximum(qz, 0.0)
+        d = np.sqrt(mx * mx + my * my + mz * mz) + np.minimum(np.maximum(qx, np.maximum(qy, qz)), 0.0)
+        take = d < best
+        best = np.where(take, d, best)
+        vx = np.where(take, omega * rz, vx)
+        vz = np.where(take, -omega * rx, vz)
+    if walls is not None:
+        (lx, _, lz), (hx, _, hz) = walls
+        d = np.minimum(np.minimum(x - lx, hx - x), np.minimum(z - lz, hz - z))
+        take = d < best
+        best = np.where(take, d, best)
+        vx = np.where(take, 0.0, vx)
+        vz = np.where(take, 0.0, vz)
+    return best, vx, vz
+
+
+def stirred_dam_break(n: int = 256, seed: int = 0, paddle_half_extent=(0.18, 0.1, 0.02), paddle_pivot=(0.7, 0.12, 0.5)):
+    """Dam break (liquid where x < 0.4 and y < 0.3) in the unit box with a static sphere r = 0.15 at the
+    centre and a box paddle turning at 2 rad/s about the vertical through its pivot; the side walls
+    are part of the solid, the floor y = 0 is a half-space obstacle. Particles seeded inside the solid are
+    dropped. Returns (scene, solid description: dict of sphere, paddle, walls)."""
+    solid = dict(sphere=((0.5, 0.5, 0.5), 0.15), paddle=(tuple(paddle_pivot), tuple(paddle_half_extent), 0.0, 2.0),
+                 walls=((0.0, 0.0, 0.0), (1.0, 0.0, 1.0)))
+    x, y, z = _centres(n)
+    liquid = np.broadcast_to((x < 0.4) & (y < 0.3), (n, n, n)).copy()
+    pos, _ = _seed_particles(liquid, n, seed)
+    keep = solid_distance(pos, **solid)[0] > 0.0
+    pos = pos[keep]
+    sc = Scene(f"stirred_dam_break_{n}", n, seed, liquid, pos, np.zeros((len(pos), 3)), np.zeros((len(pos), 3, 3)),
+               RHO * (1.0 / n) ** 3 / 8)
+    return sc, solid
 
 
 def by_name(name: str, seed: int = 0) -> Scene:

@@ -259,12 +259,16 @@ typedef struct fl_freeze_inputs {
   uint8_t pad[7];
   uint64_t max_cells;
   uint64_t max_faces;
-  /* SampledSolidBoundary3D (liquid_boundary_geometry_3d.h:83-91), used when has_sampled_solid is 1: solid
+  /* SampledSolidBoundary3D (liquid_boundary_geometry_3d.h:83-91), used when has_sampled_solid is 1 or 2: solid
    * corner samples, distinct and (z,y,x) sorted, covering every candidate's and frozen face's corners;
    * and either no face velocities (count 0: the corner wall velocities are integrated over the blocked
    * part of each face) or one wall normal velocity per owned slot (192 per block). The obstacle must then
    * be absent or a half-space (which clips the sampled geometry). Zero-initialised callers keep the
-   * plane-obstacle freeze. */
+   * plane-obstacle freeze. has_sampled_solid 2 and 3 have no reference counterpart: 2 applies the
+   * wet-piece connectivity refusal (CheckSampledWetConnectivity) only to cells with a corner in the solid
+   * (sampled distance <= 0 or on the analytic wall's side), so liquid split by the particle field alone
+   * is measured as in the plane-obstacle freeze instead of refused; 3 applies it nowhere (a cell's wet
+   * pieces are one pressure cell whether or not a solid separates them). */
   const fl_solid_corner* solid_corners;
   uint64_t solid_corner_count;
   const double* face_normal_velocities;
@@ -742,6 +746,28 @@ int fl_set_transfer_precision(fl_ctx* ctx, int bits);
  * particles as they were, so a caller can retry with a shorter step (the update's Aborted on a move
  * beyond half a cell is the reference scheduler's cue to do so). No reference counterpart exists
  * for the options record as a whole: it gathers the scene and options fields those stages read. */
+/* Level-set solids of a scene, sampled by fl_substep at the particle field's corner keys at the start
+ * of each substep into a SampledSolidBoundary3D (liquid_boundary_geometry_3d.h:83-91: corner signed
+ * distances and wall velocities), which the freeze clips the liquid with. The solid is the union of
+ * the parts selected in `parts`: a sphere; a box paddle turned by paddle_angle about the vertical (y)
+ * axis through its pivot and rotating at paddle_angular_velocity (rad/s about +y); and the side walls
+ * of a container (solid where x or z lies outside [walls_low, walls_high]). Distances are positive
+ * outside the solid (the least over the parts); a corner's wall velocity is that of the part nearest
+ * to it (omega x r on the paddle, zero otherwise). The caller advances paddle_angle between substeps.
+ * No reference counterpart as a record: the reference's substep samples its solids from a snow model
+ * (liquid_substep_3d.cc:1286-1311); the sampled-solid freeze is the same. */
+typedef struct fl_scene_solid {
+  uint32_t parts; /* bit 0 sphere, bit 1 paddle, bit 2 container walls; 0: no sampled solid */
+  uint32_t split_rule; /* the freeze's wet-piece refusal, as fl_freeze_inputs' has_sampled_solid: 0 or 1 the
+                        * reference's, 2 only in cells a solid reaches, 3 none */
+  fl_vec3 sphere_centre;
+  double sphere_radius;
+  fl_vec3 paddle_pivot;
+  fl_vec3 paddle_half_extent; /* body frame */
+  double paddle_angle;
+  double paddle_angular_velocity;
+  fl_vec3 walls_low, walls_high; /* x and z bounds of the container */
+} fl_scene_solid;                /* 152 B */
 typedef struct fl_substep_options {
   double step_duration;
   fl_vec3 gravity;
@@ -754,9 +780,16 @@ typedef struct fl_substep_options {
   double particle_energy_blend;
   double friction_coefficient;
   double penetration_tolerance;
-  fl_obstacle obstacle; /* absent, half-space, slab or open tank (freeze and contact) */
+  fl_obstacle obstacle; /* absent, half-space, slab or open tank (freeze and contact); absent or
+                         * half-space with a sampled solid */
   fl_pressure_options pressure; /* preconditioner 0 (V-cycle) */
-} fl_substep_options;
+  fl_scene_solid solid;         /* parts 0: none */
+  /* Uniform dynamic viscosity (Pa s). Positive: the projection is the coupled viscous pressure solve
+   * (fl_project_viscous) on AssembleFullCellMacStrain3D's rows of the pressure cells, preconditioned by
+   * the V-cycle (pressure.preconditioner 0) or the reference's Jacobi diagonal (2); 0: the inviscid
+   * MGPCG projection. */
+  double dynamic_viscosity;
+} fl_substep_options; /* 456 B */
 typedef struct fl_substep_report {
   int32_t status;
   int32_t stage;
@@ -768,7 +801,9 @@ typedef struct fl_substep_report {
   fl_pressure_diagnostics pressure;
   double sort_ms, field_ms, deposit_ms, freeze_ms, links_ms, projection_ms, gather_ms, update_ms, total_ms;
   double max_speed; /* largest particle speed after the update (m/s), for the next step's CFL duration */
-} fl_substep_report;
+  fl_viscous_diagnostics viscous; /* the coupled viscous projection's (dynamic_viscosity > 0) */
+  uint64_t strain_rows;
+} fl_substep_report; /* 336 B */
 int fl_substep(fl_ctx* ctx, const fl_substep_options* options, fl_substep_report* report);
 /* The scene's particles after substeps, in the order of the last substep's sort (any pointer may be
  * null): positions and velocities (3 per particle), slopes (9, row-major), volume ratios and flags

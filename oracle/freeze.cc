@@ -647,9 +647,12 @@ int WetConnectivity(const Vertex (&x)[8], const double (&s)[8], const bool (&wet
   return components == 1 ? FLO_OK : FLO_FAILED_PRECONDITION;
 }
 
-// MeasureSampledSolidLiquidCell (:118-218).
+// MeasureSampledSolidLiquidCell (:118-218). split_rule (has_sampled_solid; 2 and 3 are relaxations with
+// no reference counterpart): 1 the wet-piece refusal everywhere, 2 only where some corner is in the
+// solid, 3 nowhere.
 int MeasureSampledCell(double width, const double (&liquid)[8], const double (&solid)[8], const Plane* analytic,
-                       double& wet_volume, double (&open_area)[6]) {
+                       double& wet_volume, double (&open_area)[6], int split_rule = 1) {
+  const bool solid_split_only = split_rule == 2;
   if (!std::isfinite(width) || width <= 0) return FLO_INVALID_ARGUMENT;
   const double full_area = width * width, full_volume = full_area * width;
   if (!std::isnormal(full_area) || !std::isnormal(full_volume)) return FLO_OUT_OF_RANGE;
@@ -694,7 +697,11 @@ int MeasureSampledCell(double width, const double (&liquid)[8], const double (&s
   }
   if (!NormalOrZero(wet) || wet / full_volume > 1 + kMeasureSlack) return FLO_OUT_OF_RANGE;
   wet = std::min(wet, full_volume);
-  if (int c = WetConnectivity(x, scaled, wet_tet, width, analytic)) return c;
+  bool touched = !solid_split_only;
+  for (int k = 0; k < 8 && !touched; ++k)
+    touched = scaled[k] <= 0 || (analytic != nullptr && PlaneDistance(x[k], *analytic) >= 0);
+  if (touched && split_rule != 3)
+    if (int c = WetConnectivity(x, scaled, wet_tet, width, analytic)) return c;
   double open[6];
   for (int axis = 0; axis < 3; ++axis) {
     const int bit = 1 << axis;
@@ -783,7 +790,8 @@ int SampledCellVolume(const flo_freeze_inputs& in, flo_int3 c, const double (&li
   if (in.obstacle.kind != FLO_OBSTACLE_ABSENT && in.obstacle.kind != FLO_OBSTACLE_HALFSPACE) return FLO_INVALID_ARGUMENT;
   const Plane analytic{Scale(P.normal, -1), -P.high, false};
   double open[6];
-  return MeasureSampledCell(in.geometry.cell_width, liquid, solid, P.present ? &analytic : nullptr, wet, open);
+  return MeasureSampledCell(in.geometry.cell_width, liquid, solid, P.present ? &analytic : nullptr, wet, open,
+                            in.has_sampled_solid);
 }
 
 // ComputeSampledSolidGeometry for a face (the low face of the element at lattice l): open and blocked
@@ -947,6 +955,7 @@ extern "C" int flo_freeze_geometry(const flo_freeze_inputs* inputs, flo_frozen_c
   if (int s = ValidateSamples(in.center_distances, in.center_count)) return s;
   if (int s = ValidateSamples(in.corner_distances, in.corner_count)) return s;
   const bool sampled = in.has_sampled_solid != 0;
+  if (in.has_sampled_solid > 3) return FLO_INVALID_ARGUMENT;
   const bool sampled_halfspace = sampled && in.obstacle.kind == FLO_OBSTACLE_HALFSPACE;
   if (sampled) {
     const uint64_t face_count = 192 * in.block_count;

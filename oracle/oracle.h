@@ -316,7 +316,7 @@ struct flo_freeze_inputs {
   uint64_t solid_corner_count;
   const double* face_normal_velocities;
   uint64_t face_velocity_count;
-  uint8_t has_sampled_solid, pad2[7];
+  uint8_t has_sampled_solid, pad2[7];  // 2: the wet-piece refusal only in cells with a corner in the solid, 3: none
 };
 // FreezeCore (liquid_boundary_geometry_3d.cc:765-1068) with the scalar geometry queries of
 // base/liquid_boundary_geometry_math_3d.h (kCell, kFace, kCurvature, kCapillary). out_cells holds

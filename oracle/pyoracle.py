@@ -452,9 +452,12 @@ class FreezeProblem:
 
     def __init__(self, geom, blocks, candidates, centers, corners, obstacle=None, step=0.01, density=1000.0,
                  sigma=0.0, exact_curvature=None, max_cells=0, max_faces=0, solid_corners=None,
-                 face_normal_velocities=None):
+                 face_normal_velocities=None, split_rule=1):
         """solid_corners: SOLID_CORNER records selecting the sampled-solid mode (None: no sampled solid);
-        face_normal_velocities: None (empty span) or one wall normal velocity per owned slot."""
+        face_normal_velocities: None (empty span) or one wall normal velocity per owned slot;
+        split_rule: the wet-piece refusal, 1 the reference's, 2 only in cells with a corner in the solid,
+        3 none."""
+        self.split_rule = split_rule
         self.geom, self.blocks = geom, np.ascontiguousarray(blocks)
         self.candidates, self.centers, self.corners = (np.ascontiguousarray(a) for a in (candidates, centers, corners))
         self.obstacle = obstacle
@@ -482,7 +485,8 @@ class FreezeProblem:
                              _p(self.solid_corners), 0 if self.solid_corners is None else len(self.solid_corners),
                              _p(self.face_normal_velocities),
                              0 if self.face_normal_velocities is None else len(self.face_normal_velocities),
-                             0 if self.solid_corners is None else 1, (C.c_uint8 * 7)())
+                             0 if self.solid_corners is None else self.split_rule,
+                             (C.c_uint8 * 7)())
 
 
 def measure_sampled_cell(width, liquid, solid, analytic=None):

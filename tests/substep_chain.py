@@ -81,10 +81,28 @@ def mat(a):
     return np.stack([vec(a[r]) for r in ROWS], -2)
 
 
+def solid_corners(geom, corners, solid):
+    """SOLID_CORNER records of the scene solid (scenes.solid_distance) at the field's corner keys."""
+    from fluidity_amd import scenes
+    h = float(geom[0]["cell_width"])
+    o = geom[0]["world_origin"]
+    k = np.stack([corners["cell_index"][a].astype(np.float64) for a in "xyz"], 1)
+    pts = np.stack([float(o[a]) + k[:, i] * h for i, a in enumerate("xyz")], 1)
+    d, vx, vz = scenes.solid_distance(pts, **solid)
+    out = np.zeros(len(corners), O.SOLID_CORNER)
+    out["corner_index"] = corners["cell_index"]
+    out["signed_distance"] = d
+    out["wall_velocity"]["x"], out["wall_velocity"]["z"] = vx, vz
+    return out
+
+
 def oracle_substep(geom, state, dt, radius, obstacle, pressure_options, gravity=(0.0, -9.81, 0.0),
                    mass_q=2.0 ** -40, mom_q=2.0 ** -40, capacity=1 << 20, density=1000.0, sigma=0.0, ambient=0.0,
-                   blend=1.0, minimum_face_mass=0.0):
-    """One substep: (new state in this step's sort order, dict of the intermediate records)."""
+                   blend=1.0, minimum_face_mass=0.0, solid=None, viscosity=0.0, split_rule=1):
+    """One substep: (new state in this step's sort order, dict of the intermediate records). solid: the
+    scene solid (scenes.solid_distance's parts), sampled at the corner keys for the sampled-solid freeze;
+    viscosity > 0: the coupled viscous projection (ProjectMacViscousPressure3D) on the full-cell strain
+    rows of the pressure cells instead of the inviscid projection."""
     h = float(geom[0]["cell_width"])
     s, entries = O.sort_particles(geom, state.x)
     assert s == O.OK, ("sort", s)
@@ -101,7 +119,9 @@ def oracle_substep(geom, state, dt, radius, obstacle, pressure_options, gravity=
     assert s == O.OK, ("normalize", s)
     s, ustar = O.apply_forces(blocks, u0, gravity, dt)
     assert s == O.OK, ("forces", s)
-    fp = O.FreezeProblem(geom, blocks, cand, cen, cor, obstacle=obstacle, step=dt, density=density, sigma=sigma)
+    solids = None if solid is None else solid_corners(geom, cor, solid)
+    fp = O.FreezeProblem(geom, blocks, cand, cen, cor, obstacle=obstacle, step=dt, density=density, sigma=sigma,
+                         solid_corners=solids, split_rule=split_rule)
     s, fcells, ffaces, bound = O.freeze_geometry(fp)
     assert s == O.OK, ("freeze", s)
     fields = np.zeros(len(fcells), O.CELL_FIELDS)
@@ -113,8 +133,17 @@ def oracle_substep(geom, state, dt, radius, obstacle, pressure_options, gravity=
                        predicted=predicted, ambient=ambient)
     s, pcells, pfaces, lower = O.build_pressure_links(lp)
     assert s == O.OK, ("links", s)
-    s, proj = O.project(geom, blocks, dt, pcells, pfaces, pressure_options)
-    assert s == O.OK, ("project", s)
+    rows = None
+    if viscosity > 0.0:
+        vc = O.viscosity_cells([tuple(int(v) for v in c["cell_index"].tolist()) for c in pcells],
+                               [viscosity] * len(pcells))
+        s, rows = O.assemble_full_cell_strain(geom, blocks, vc)
+        assert s == O.OK, ("strain", s)
+        s, proj = O.project_viscous(geom, blocks, dt, pcells, pfaces, rows)
+        assert s == O.OK, ("project_viscous", s)
+    else:
+        s, proj = O.project(geom, blocks, dt, pcells, pfaces, pressure_options)
+        assert s == O.OK, ("project", s)
     unew = ustar.copy()
     unew[ffaces["face_slot"].astype(np.int64)] = proj["velocities"]
     s, gathered = O.gather(geom, blocks, particles, u0, unew)
@@ -136,5 +165,5 @@ def oracle_substep(geom, state, dt, radius, obstacle, pressure_options, gravity=
                 st.mass, updated["volume_ratio"], np.ones(len(st.x), np.uint8))
     info = dict(order=order, candidates=cand, centres=cen, corners=cor, blocks=blocks, quanta=quanta, u0=u0,
                 ustar=ustar, frozen_cells=fcells, frozen_faces=ffaces, pressure_cells=pcells, pressure_faces=pfaces,
-                lower=lower, projection=proj, unew=unew, gathered=gathered, bound=bound)
+                lower=lower, projection=proj, unew=unew, gathered=gathered, bound=bound, solids=solids, rows=rows)
     return new, info

@@ -154,3 +154,22 @@ def test_sampled_freeze_then_links(ctx):
     assert s_want == O.OK and s_got == s_want and gl == wl
     same(gc.view(O.PRESSURE_CELL), wc)
     same(gf.view(O.PRESSURE_FACE), wf)
+
+
+@pytest.mark.parametrize("rule", [1, 2, 3])
+def test_split_rules(ctx, rule):
+    """The wet-piece refusal rules, bit-exact with the oracle: a cell the particle field alone splits is
+    refused by the reference's rule (1) and measured by rules 2 and 3; lobes a solid separates are
+    refused by rules 1 and 2 and measured as one cell by rule 3."""
+    centers, corners, solid = S.field_split_cell()
+    want = O.FAILED_PRECONDITION if rule == 1 else O.OK
+    assert check(ctx, F.problem(centers=centers, corners=corners, candidates=((-2, -2, -2),), solid_corners=solid,
+                                split_rule=rule)) == want
+    centers, corners, solid = S.shared_face_problem()
+    for i, c in enumerate(solid["corner_index"]):
+        idx = (int(c["x"]), int(c["y"]), int(c["z"]))
+        solid["signed_distance"][i] = 1.0 if idx in ((-1, -2, -2), (-2, -1, -1)) else -1.0
+    corners["signed_distance"] = -1.0
+    want = O.OK if rule == 3 else O.FAILED_PRECONDITION
+    assert check(ctx, F.problem(centers=centers, corners=corners, candidates=((-2, -2, -2),), solid_corners=solid,
+                                split_rule=rule)) == want

@@ -9,7 +9,8 @@ measure, the blocked-face weights of an affine wall (11/48, 1/48, 7/48, 5/48) an
 open limits, an analytic platen clipping the wet volume and the weights, the closed zero plateau, the
 two-cell freeze with its shared face (open 3/4, blocked 1/4, wall velocity 0.2; affine corner
 velocities giving the flux 5/8 and mean velocity 5/2), and the missing-sample and split-lobe
-refusals. Independent closed form: a solid sampled from an exact plane (affine, so its piecewise-
+refusals. Mode 2 (no reference counterpart): a cell the particle field alone splits, with no corner in
+the solid, is measured as the plane-obstacle freeze measures it instead of refused. Independent closed form: a solid sampled from an exact plane (affine, so its piecewise-
 linear interpolation is exact) gives the analytic half-space freeze's volumes and areas to rounding.
 """
 import math
@@ -148,6 +149,55 @@ def test_split_wet_lobes_refused():
     corners["signed_distance"] = -1.0
     p = problem(centers=centers, corners=corners, candidates=((-2, -2, -2),), solid_corners=solid)
     assert O.freeze_geometry(p)[0] == O.FAILED_PRECONDITION
+
+
+def field_split_cell():
+    """Cell (-2,-2,-2) whose liquid is two lobes at its corners (-1,-2,-2) and (-2,-1,-1) (corner ordinals
+    1 and 6), every other corner dry: the particle field alone splits it; the solid is far (+1 everywhere)."""
+    centers, corners = sample(lambda x, y, z: 1.0)
+    for i, c in enumerate(corners["cell_index"]):
+        if tuple(int(v) for v in c) in ((-1, -2, -2), (-2, -1, -1)):
+            corners["signed_distance"][i] = -0.5
+    centers["signed_distance"][:] = 1.0
+    for i, c in enumerate(centers["cell_index"]):
+        if tuple(int(v) for v in c) == (-2, -2, -2):
+            centers["signed_distance"][i] = -0.1
+    solid = solid_samples(corners, lambda c: 1.0)
+    return centers, corners, solid
+
+
+def test_split_only_where_a_solid_reaches():
+    """The reference refuses the field-split cell in sampled mode; mode 2 measures it as the plane-obstacle
+    freeze does (same kept cell, volume to rounding), and still refuses the solid-split lobes above."""
+    centers, corners, solid = field_split_cell()
+    p = problem(centers=centers, corners=corners, candidates=((-2, -2, -2),), solid_corners=solid)
+    assert O.freeze_geometry(p)[0] == O.FAILED_PRECONDITION
+    p = problem(centers=centers, corners=corners, candidates=((-2, -2, -2),), solid_corners=solid,
+                split_rule=2)
+    s, cells, faces, _ = O.freeze_geometry(p)
+    assert s == O.OK
+    s, plain, _, _ = O.freeze_geometry(problem(centers=centers, corners=corners, candidates=((-2, -2, -2),)))
+    assert s == O.OK
+    assert len(cells) == len(plain) == 1
+    assert 0.0 < cells[0]["fluid_volume"] < 0.5
+    assert abs(cells[0]["fluid_volume"] - plain[0]["fluid_volume"]) <= 1e-15
+    # a corner in the solid keeps the reference's refusal
+    centers, corners, solid = shared_face_problem()
+    for i, c in enumerate(solid["corner_index"]):
+        idx = (int(c["x"]), int(c["y"]), int(c["z"]))
+        solid["signed_distance"][i] = 1.0 if idx in ((-1, -2, -2), (-2, -1, -1)) else -1.0
+    corners["signed_distance"] = -1.0
+    p = problem(centers=centers, corners=corners, candidates=((-2, -2, -2),), solid_corners=solid,
+                split_rule=2)
+    assert O.freeze_geometry(p)[0] == O.FAILED_PRECONDITION
+    # rule 3 keeps the solid-split cell as one cell whose wet volume is the sum of its lobes
+    p = problem(centers=centers, corners=corners, candidates=((-2, -2, -2),), solid_corners=solid,
+                split_rule=3)
+    s, cells, _, _ = O.freeze_geometry(p)
+    assert s == O.OK and len(cells) == 1 and 0.0 < cells[0]["fluid_volume"] < 1.0
+    p = problem(centers=centers, corners=corners, candidates=((-2, -2, -2),), solid_corners=solid,
+                split_rule=4)
+    assert O.freeze_geometry(p)[0] == O.INVALID_ARGUMENT
 
 
 def test_inputs_rejected():
