@@ -138,6 +138,10 @@ struct Pressure {
   int last_storage = 1, last_nu = 2, last_coarse_sweeps = 2;  // V-cycle settings of the last solve
   bool p_init_zero = false;  // warm start is identically zero: its evaluation equals the p = 0 one
   bool has_jump = false;
+  // Frozen link coefficients and hierarchy coefficients of the current system for step duration
+  // coeff_dt (valid until the system, its levels or its coefficients change); BF16 level copies valid.
+  bool coeff_valid = false, bf16_valid = false;
+  double coeff_dt = 0.0;
   double full_area = 0, full_volume = 0;
   double scale_coarse = 0.5;
   // Scratch of device-built systems (prepare_system_device): level-0 block marks and their scan over
@@ -310,6 +314,12 @@ __global__ void k_freeze_faces(uint64_t nf, double dt, double full_area, const i
   r.k = c;
   r.lift = lift[f];
   rec[f] = r;
+}
+
+// The predicted normal velocity of every face record (the only record field a new u* changes).
+__global__ void k_refresh_ustar(uint64_t nf, const double* ustar, FaceRec* rec) {
+  const uint64_t f = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (f < nf) rec[f].ustar = ustar[f];
 }
 
 __global__ void k_freeze_cells(uint64_t nc, double dt, const double* V, const double* s, const double* K,
@@ -1372,6 +1382,8 @@ uint64_t pack_key(const Key3& k) {
 // added while aggregation still shrinks the set of unknown cells (the oracle's rule), up to
 // `levels_wanted`. Level 0 keeps every block that holds a pressure cell.
 int build_levels(fl_ctx* ctx, Pressure& P, const std::vector<Key3>& blocks0, int levels_wanted) {
+  P.coeff_valid = false;
+  P.bf16_valid = false;
   free_levels(P);
   P.levels_requested = levels_wanted;
   std::vector<std::vector<Key3>> keys{blocks0};
@@ -1460,6 +1472,8 @@ int build_levels(fl_ctx* ctx, Pressure& P, const std::vector<Key3>& blocks0, int
 // viscous solve's effective conductances).
 int setup_coefficients(fl_ctx* ctx, Pressure& P, const double* conductance = nullptr) {
   hipStream_t s = ctx->stream;
+  P.coeff_valid = false;
+  P.bf16_valid = false;
   Level& L0 = P.levels[0];
   const uint64_t n0 = L0.nb * 64;
   k_level0<<<grid_for(n0), kThreads, 0, s>>>(n0, P.fine_cell, P.cell_flag, P.cell_faces,
@@ -1631,13 +1645,14 @@ int solve(fl_ctx* ctx, Pressure& P, const fl_pressure_options& opt, const Gates&
   if (e.code) return e.code;
   if (accept(e, gates)) return FL_OK;
   k_load_residual<<<g, kThreads, 0, s>>>(n, P.fine_cell, P.cell_flag, P.res, P.r32, P.x32, P.perr);
-  if (opt.level_storage == 2) {
+  if (opt.level_storage == 2 && !P.bf16_valid) {
     for (auto& L : P.levels) {
       const uint64_t nl = L.nb * 64;
       k_to_bf16<<<grid_for(3 * nl), kThreads, 0, s>>>(L.k, 3 * nl, L.kb);
       k_diag_bf16<<<grid_for(nl), kThreads, 0, s>>>(nl, L.nbr, L.kb, L.anchor, L.diag, L.diagb);
     }
     FL_HIP(hipGetLastError());
+    P.bf16_valid = true;
   }
   bool restart = true;
   int since = 0;
@@ -1917,6 +1932,8 @@ int64_t host_floor_div4(int64_t i) { return i >= 0 ? i / 4 : -1 - (-(i + 1)) / 4
 int prepare_system(fl_ctx* ctx, Pressure& P, const fl_geometry* g, const fl_int3* grid_blocks, size_t nb, double dt,
                    const fl_pressure_cell* cells, size_t nc, const fl_pressure_face* faces, size_t nf,
                    bool check_values) {
+  P.coeff_valid = false;
+  P.bf16_valid = false;
   if (nc == 0 || nf == 0 || !finite_positive(dt)) return FL_INVALID_ARGUMENT;
   const double h = g->cell_width, area = h * h, volume = area * h;
   if (!finite_positive(area) || !finite_positive(volume)) return FL_OUT_OF_RANGE;
@@ -2242,24 +2259,36 @@ int prepare_system(fl_ctx* ctx, Pressure& P, const fl_geometry* g, const fl_int3
 
 // Freezes the system for dt on the device, assembles the RHS and the hierarchy coefficients, and
 // computes the gates. The warm start must be in P.p_base.
+// reuse: a caller whose system, levels and step duration may be unchanged since the last freeze (the
+// device-resident step on the scene's fixed structure): then only the predicted velocities of the face
+// records are refreshed; every coefficient is what the full freeze would write again.
 int freeze_and_assemble(fl_ctx* ctx, Pressure& P, double dt, const fl_pressure_options& opt, Gates& gates,
-                        EvalResult& e0, fl_pressure_diagnostics& diag) {
+                        EvalResult& e0, fl_pressure_diagnostics& diag, bool reuse = false) {
   hipStream_t s = ctx->stream;
   k_reset_err<<<1, 64, 0, s>>>(P.perr);
   k_copy_f64<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, P.p_init, P.p_base);
-  k_freeze_faces<<<grid_for(P.nf), kThreads, 0, s>>>(P.nf, dt, P.full_area, P.face_tail, P.face_head, P.face_kind,
-                                                     P.face_area, P.face_beta, P.face_dist, P.face_ustar, P.face_uwall,
-                                                     P.face_has_flux, P.face_flux, P.face_lift, P.face_scale, P.face_k,
-                                                     P.face_wall, P.face_rec);
-  k_freeze_cells<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, dt, P.cell_volume, P.cell_s, P.cell_K, P.cell_comp,
-                                                     P.cell_target);
+  const bool cached = reuse && P.coeff_valid && P.coeff_dt == dt;
+  if (cached) {
+    k_refresh_ustar<<<grid_for(P.nf), kThreads, 0, s>>>(P.nf, P.face_ustar, P.face_rec);
+  } else {
+    k_freeze_faces<<<grid_for(P.nf), kThreads, 0, s>>>(P.nf, dt, P.full_area, P.face_tail, P.face_head, P.face_kind,
+                                                       P.face_area, P.face_beta, P.face_dist, P.face_ustar,
+                                                       P.face_uwall, P.face_has_flux, P.face_flux, P.face_lift,
+                                                       P.face_scale, P.face_k, P.face_wall, P.face_rec);
+    k_freeze_cells<<<grid_for(P.nc), kThreads, 0, s>>>(P.nc, dt, P.cell_volume, P.cell_s, P.cell_K, P.cell_comp,
+                                                       P.cell_target);
+  }
   // RHS and the evaluation of the unprojected field (p = 0) in one pass; r = b there, so the residual
   // norm is the RHS norm.
   const unsigned g = reduce_grid(P.nc, kPartials);
   k_rhs_zero_eval<<<g, kThreads, 0, s>>>(P.nc, P.cell_faces, P.face_rec, P.cell_volume, P.cell_target, P.cell_inwall,
                                          P.cell_comp, P.cell_prev, P.rhs, P.rhs_abs, P.res, P.def, P.partial, P.perr);
   FL_HIP(hipGetLastError());
-  FL_TRY(setup_coefficients(ctx, P));
+  if (!cached) {
+    FL_TRY(setup_coefficients(ctx, P));
+    P.coeff_valid = true;
+    P.coeff_dt = dt;
+  }
   FL_TRY(read_evaluation(ctx, P, g, e0));
   const double rhs_norm = e0.l2;
   double limit = 2e-6 + 2e-6 * rhs_norm;
@@ -2816,6 +2845,8 @@ int read_words(fl_ctx* ctx, const uint64_t* d, int n) {
 
 // build_levels on the device: level 0 is the nb0 blocks whose keys are in P.dv_keys.
 int build_levels_device(fl_ctx* ctx, Pressure& P, uint64_t nb0, int levels_wanted) {
+  P.coeff_valid = false;
+  P.bf16_valid = false;
   hipStream_t s = ctx->stream;
   free_levels(P);
   P.levels_requested = levels_wanted;
@@ -2920,6 +2951,8 @@ int build_levels_device(fl_ctx* ctx, Pressure& P, uint64_t nb0, int levels_wante
 int prepare_system_device(fl_ctx* ctx, Pressure& P, const fl_pressure_cell* cells, uint64_t nc,
                           const fl_pressure_face* faces, uint64_t nf, double dt, int levels, bool* host_fallback) {
   *host_fallback = false;
+  P.coeff_valid = false;
+  P.bf16_valid = false;
   hipStream_t s = ctx->stream;
   if (nc == 0 || nf == 0 || !finite_positive(dt)) return FL_INVALID_ARGUMENT;
   const double h = ctx->geometry.h, area = h * h, volume = area * h;
@@ -3333,7 +3366,7 @@ int fl_scene_prepare(fl_ctx* ctx, const fl_pressure_cell* cells, size_t cell_cou
   if (!ctx->pressure) ctx->pressure = new Pressure();
   Pressure& P = *static_cast<Pressure*>(ctx->pressure);
   fl_geometry g{{ctx->geometry.ox, ctx->geometry.oy, ctx->geometry.oz}, h};
-  // dt only scales conductances here; each step refreezes with its own dt.
+  // dt only scales conductances here; a step freezes with its own dt (a repeated dt reuses the coefficients).
   FL_TRY(prepare_system(ctx, P, &g, blocks.data(), blocks.size(), 1.0, cells, cell_count, faces.data(), faces.size(),
                         false));
   ctx->scene_ready = true;
@@ -3393,7 +3426,7 @@ int fl_step(fl_ctx* ctx, double dt, fl_vec3 gravity, fl_quantization quantizatio
   Gates gates{};
   EvalResult e0{};
   fl_pressure_diagnostics diag{};
-  int st = freeze_and_assemble(ctx, P, dt, *options, gates, e0, diag);
+  int st = freeze_and_assemble(ctx, P, dt, *options, gates, e0, diag, true);
   if (st == FL_OK) st = solve(ctx, P, *options, gates, e0, diag);
   report->pressure = diag;
   if (st != FL_OK) {
