@@ -123,6 +123,33 @@ def copy_ceiling(nbytes=1 << 30, reps=5):
     return gbs.value
 
 
+def runtime_copy_ceiling(nbytes=1 << 30, reps=5):
+    """The HIP runtime's own device-to-device copy (hipMemcpyAsync, DeviceToDevice) of `nbytes`, timed
+    with HIP events: a cross-check of the float4 copy kernel's ceiling. GB/s counts read + write."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    hip.hipEventCreate(C.byref(e0))
+    hip.hipEventCreate(C.byref(e1))
+    hip.hipMemsetAsync(a, 0, C.c_size_t(nbytes), None)
+    hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), 3, None)
+    hip.hipEventRecord(e0, None)
+    for _ in range(reps):
+        hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), 3, None)
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = C.c_float()
+    hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -666,7 +693,8 @@ def main():
     ap.add_argument("--learn-epochs", type=int, default=8)
     args = ap.parse_args()
     if args.copy_probe:
-        print(json.dumps({"copy_ceiling_gbs": copy_ceiling(), "bytes_per_copy": 1 << 30}), flush=True)
+        print(json.dumps({"copy_ceiling_gbs": copy_ceiling(), "runtime_copy_gbs": runtime_copy_ceiling(),
+                          "bytes_per_copy": 1 << 30}), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; runs on more than one GPU are not part of it",
@@ -763,6 +791,7 @@ def main():
             "algorithmic_bytes": p2g_bytes,
             "g2p_achieved": g2p_algorithmic_bytes(n, nf) / (g2p_ms * 1e-3) / 1e9,
             "copy_ceiling_gbs": copy_ceiling(),
+            "runtime_copy_gbs": runtime_copy_ceiling(),
         },
     }
     out["roofline"]["vcycle"] = projection_roofline(ctx, args.level_storage)

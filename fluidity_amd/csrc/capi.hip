@@ -671,7 +671,7 @@ int fl_reserve(fl_ctx* ctx) {
     FL_TRY(grow(&B.counts, &B.cap[13], 4));
     FL_TRY(grow(&B.phi, &B.cap[14], 9 * nc));
     FL_TRY(grow(&B.work, &B.cap[15], nc));
-    FL_TRY(grow(&B.uncut, &B.cap[16], 1));
+    FL_TRY(grow(&B.uncut, &B.cap[16], 2));
     // A sampled solid: 8 corner samples per candidate, one face velocity per owned slot.
     FL_TRY(grow(&B.solids, &B.cap[17], 8 * nc));
     FL_TRY(grow(&B.face_velocities, &B.cap[18], 192 * nb));
@@ -1165,7 +1165,7 @@ int fl_freeze_geometry(fl_ctx* ctx, const fl_freeze_inputs* inputs, fl_frozen_ce
   FL_TRY(grow(&B.counts, &B.cap[13], 4));
   FL_TRY(grow(&B.phi, &B.cap[14], 9 * nc_in));
   FL_TRY(grow(&B.work, &B.cap[15], nc_in));
-  FL_TRY(grow(&B.uncut, &B.cap[16], 1));
+  FL_TRY(grow(&B.uncut, &B.cap[16], 2));
   if (sampled) {
     FL_TRY(grow(&B.solids, &B.cap[17], I.solid_corner_count));
     FL_TRY(grow(&B.face_velocities, &B.cap[18], I.face_velocity_count));

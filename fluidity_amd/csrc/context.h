@@ -306,7 +306,7 @@ struct FreezeDevice {
   uint64_t* counts;       // 4: kept cells, unique faces, curvature needed, listed cut cells
   double* phi;            // 9 * ncand: corner and centre samples of the cut cells
   uint64_t* work;         // ncand: the cut cells
-  double* uncut_volume;   // 1
+  double* uncut_volume;   // 2: the uncut cell's volume, plane mode then sampled mode
   // Sampled solid (sampled != 0): corner samples, dense face normal velocities (count 0: none), and the
   // eight solid corner distances of each listed cell.
   int sampled;

@@ -804,6 +804,11 @@ __global__ void k_uncut_volume(FreezeDevice d) {
   double low = 0, high = 0;
   cell_volumes(phi, P, d.h, low, high);
   d.uncut_volume[0] = high;
+  if (d.sampled) {  // the sampled measure of a cell that no plane cuts: its clips return their inputs
+    const double solid[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+    double wet = 0, open[6];
+    d.uncut_volume[1] = measure_sampled_cell(d.h, phi, solid, nullptr, wet, open, 1) == kOk ? wet : -1.0;
+  }
 }
 
 // Whether every corner of the unit cell lies on the kept side (distance <= 0) of the plane.
@@ -853,7 +858,31 @@ __global__ void k_freeze_stage(FreezeDevice d, uint64_t* err) {
     any_negative = any_negative || phi[k] < 0;
     all_negative = all_negative && phi[k] < 0;
   }
-  if (d.sampled) {  // every cell is measured against the sampled solid
+  if (d.sampled) {
+    // A cell wholly liquid, with every solid sample clearly outside the solid and every corner strictly
+    // on the fluid side of the analytic wall, is one that no clip of the sampled measure cuts (the
+    // liquid, sampled-exterior and wall planes keep every vertex), so its volume is the uncut cell's
+    // sampled measure, computed once. Every other cell is measured against the sampled solid.
+    if (all_negative && d.uncut_volume[1] >= 0) {
+      double smax = 0, smin = __builtin_inf();
+      for (int k = 0; k < 8; ++k) {
+        smax = max_std(smax, fabs(d.solid_phi[8 * i + k]));
+        smin = min_std(smin, d.solid_phi[8 * i + k]);
+      }
+      bool clear = smin > 1e-9 * smax;
+      if (clear) {
+        Planes P;
+        if (int code = local_planes(d, c, P)) return fail(err, 2, i, code);
+        if (P.present) {
+          const Plane analytic{scale(P.normal, -1), -P.high, false};
+          for (int k = 0; k < 8 && clear; ++k) {
+            const Vtx x{{{double(k & 1), double((k >> 1) & 1), double((k >> 2) & 1)}}, 0.0};
+            clear = distance(x, analytic) < 0;
+          }
+        }
+      }
+      if (clear) return consume_cell(d, i, c, center, 0.0, d.uncut_volume[1], err);
+    }
     for (int k = 0; k < 8; ++k) d.phi[9 * i + k] = phi[k];
     d.phi[9 * i + 8] = center;
     const unsigned long long w = atomicAdd(reinterpret_cast<unsigned long long*>(d.counts + 3), 1ull);

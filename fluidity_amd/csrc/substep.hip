@@ -354,7 +354,7 @@ int substep(fl_ctx* ctx, const fl_substep_options& o, fl_substep_report& rep) {
   FL_TRY(grow_to(&B.counts, &B.cap[13], 4));
   FL_TRY(grow_to(&B.phi, &B.cap[14], 9 * ncand));
   FL_TRY(grow_to(&B.work, &B.cap[15], ncand));
-  FL_TRY(grow_to(&B.uncut, &B.cap[16], 1));
+  FL_TRY(grow_to(&B.uncut, &B.cap[16], 2));
   const double full_area = h * h, full_volume = full_area * h;
   const FreezeDevice fd{reinterpret_cast<const fl_int3*>(ctx->blocks),
                         nb,
