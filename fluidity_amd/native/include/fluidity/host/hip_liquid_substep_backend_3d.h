@@ -5,8 +5,9 @@
 //
 // Members filled: the lifecycle hooks (initialize, prepare/commit/discard publication, prepare/commit/
 // discard growth), topology, deposit, normalize, forces, freeze, links, project, gather and
-// update_particles. Scene sampling (sample_field, sample_faces and their batches), missing-face
-// prediction, the wetted piston, the viscous and adaptive projections and the diagnostic phase hook
+// update_particles, and project_viscous for geometry without cut cells or faces. Scene sampling
+// (sample_field, sample_faces and their batches), missing-face prediction, the wetted piston, the
+// adaptive projection and the diagnostic phase hook
 // belong to subsystems outside this library and stay null; a scheduler that needs them must refuse the
 // table, as the reference's does (:39-41).
 //
@@ -92,6 +93,10 @@ struct HipLiquidSubstepBackend3D {
                                                          const LiquidPressureProblemView3D& problem,
                                                          MacPressureSolverOptions3D options,
                                                          MacPressureWorkspace3D& workspace);
+  using ProjectViscous = StatusOr<MacViscousPressureProjectionView3D>(
+      void* context, const LiquidBoundaryGeometryViewInputs3D& geometry, const LiquidPressureProblemView3D& problem,
+      double dynamic_viscosity, std::span<const double> cell_dynamic_viscosity, MacPressureSolverOptions3D options,
+      MacPressureWorkspace3D& workspace);
   using Gather = StatusOr<LiquidBoundaryGatherView3D>(void* context, const LiquidBoundaryGatherViewInputs3D& inputs,
                                                        LiquidBoundaryWorkspace3D& workspace);
   using UpdateParticles = Status(void* context, const LiquidParticleUpdateInputs3D& inputs,
@@ -116,7 +121,7 @@ struct HipLiquidSubstepBackend3D {
   void* wetted_piston = nullptr;  // not provided
   BuildLinks* links = nullptr;
   Project* project = nullptr;
-  void* project_viscous = nullptr;  // not provided
+  ProjectViscous* project_viscous = nullptr;  // uncut geometry (cut cells: Unimplemented)
   Gather* gather = nullptr;
   UpdateParticles* update_particles = nullptr;
 };

@@ -18,10 +18,24 @@
 #include "fluidity/host/liquid_boundary_workspace_3d.h"
 #include "fluidity/host/mac_pressure_3d.h"
 #include "fluidity/host/sparse_mac_grid_3d.h"
+#include "fluidity/host/viscosity.h"
 #include "fluidity/status.h"
 #include "fluidity_hip.h"
 
 namespace fluidity {
+
+// The coupled viscous pressure projection's borrowed result (host/viscosity.h:171-182).
+struct MacViscousPressureProjectionView3D {
+  std::span<const double> cell_pressures;           // Pa, in pressure cell order
+  std::span<const double> face_velocities;          // m/s, in incident pressure face order
+  std::span<const double> cell_constraint_defects;  // outward flux minus source plus compliance, / volume
+  std::size_t pressure_iterations = 0;
+  double constraint_residual_l2 = 0;
+  double initial_constraint_residual_l2 = 0;
+  double max_cell_constraint_defect = 0;
+  double max_momentum_residual = 0;
+  double inward_prescribed_wall_flux_m3_per_s = 0;
+};
 
 // Scalar observations of the frozen system that take no part in the solve.
 struct MacPressureFrozenMeasurement3D {
@@ -67,6 +81,9 @@ class MacPressureWorkspace3D {
                                                                        double, const LiquidPressureProblemView3D&,
                                                                        MacPressureSolverOptions3D,
                                                                        MacPressureWorkspace3D&);
+  friend StatusOr<MacViscousPressureProjectionView3D> HipProjectUniformViscosity3D(
+      HipOperatorRuntime3D&, const LiquidBoundaryGeometryViewInputs3D&, const LiquidPressureProblemView3D&, double,
+      std::span<const double>, MacPressureSolverOptions3D, MacPressureWorkspace3D&);
   MacPressureWorkspace3D() = default;
   struct Bank {
     std::vector<double> pressures, velocities, rhs, residuals, defects;
@@ -79,6 +96,8 @@ class MacPressureWorkspace3D {
   std::array<Bank, 2> banks_;
   std::vector<fl_pressure_cell> cells_;
   std::vector<fl_pressure_face> faces_;
+  std::vector<fl_viscosity_cell> material_;  // cell_capacity
+  std::vector<fl_strain_row> rows_;          // 6 * cell_capacity
   HipMacPressureSolverSettings3D settings_{};
   HipMacPressureDiagnostics3D device_diagnostics_{};
   HipOperatorMeasurement measurement_{};
@@ -96,5 +115,17 @@ StatusOr<MacPressureProjectionView3D> HipProjectMacPressure3D(HipOperatorRuntime
                                                               const LiquidPressureProblemView3D& problem,
                                                               MacPressureSolverOptions3D options,
                                                               MacPressureWorkspace3D& workspace);
+// The substep's viscous projection member (ProjectUniformViscosity, liquid_substep_3d_cpu.cc:619-695):
+// uniform (or per-cell, when cell_dynamic_viscosity is not empty) viscosity on the problem's cells,
+// full-cell strain rows (fl_assemble_full_cell_strain) and the coupled viscous pressure solve
+// (fl_project_viscous, the V-cycle-preconditioned GCR with the workspace's device settings) into the
+// candidate bank. Cut geometry (a cell's volume off the full cell by more than 64 eps, or a face
+// partly open) needs the reference's octant-weighted strain quadrature, which is not built:
+// Unimplemented. The result borrows the workspace.
+StatusOr<MacViscousPressureProjectionView3D> HipProjectUniformViscosity3D(
+    HipOperatorRuntime3D& runtime, const LiquidBoundaryGeometryViewInputs3D& geometry,
+    const LiquidPressureProblemView3D& problem, double dynamic_viscosity,
+    std::span<const double> cell_dynamic_viscosity, MacPressureSolverOptions3D options,
+    MacPressureWorkspace3D& workspace);
 
 }  // namespace fluidity

@@ -112,6 +112,17 @@ struct HipBackendAccess {
     if (r.ok()) c.last_measurement = workspace.measurement();
     return r;
   }
+  static StatusOr<MacViscousPressureProjectionView3D> ProjectViscous(
+      void* context, const LiquidBoundaryGeometryViewInputs3D& geometry, const LiquidPressureProblemView3D& problem,
+      double dynamic_viscosity, std::span<const double> cell_dynamic_viscosity, MacPressureSolverOptions3D options,
+      MacPressureWorkspace3D& workspace) {
+    auto& c = Ctx(context);
+    workspace.set_device_settings(c.projection_settings);
+    auto r = HipProjectUniformViscosity3D(c.runtime(), geometry, problem, dynamic_viscosity, cell_dynamic_viscosity,
+                                          options, workspace);
+    if (r.ok()) c.last_measurement = workspace.measurement();
+    return r;
+  }
   static StatusOr<LiquidBoundaryGatherView3D> Gather(void* context, const LiquidBoundaryGatherViewInputs3D& inputs,
                                                     LiquidBoundaryWorkspace3D& workspace) {
     return workspace.HipGather(Ctx(context).runtime(), inputs);
@@ -169,6 +180,7 @@ HipLiquidSubstepBackend3D MakeHipLiquidSubstepBackend3D(HipLiquidSubstepContext3
   b.freeze = &HipBackendAccess::Freeze;
   b.links = &HipBackendAccess::Links;
   b.project = &HipBackendAccess::Project;
+  b.project_viscous = &HipBackendAccess::ProjectViscous;
   b.gather = &HipBackendAccess::Gather;
   b.update_particles = &HipBackendAccess::Update;
   return b;

@@ -2,6 +2,8 @@
 #include "fluidity/host/mac_pressure_workspace_3d.h"
 
 #include <algorithm>
+#include <cmath>
+#include <limits>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -40,6 +42,8 @@ StatusOr<MacPressureWorkspace3D> MacPressureWorkspace3D::Create(std::size_t cell
     }
     w.cells_.reserve(cell_capacity);
     w.faces_.reserve(face_capacity);
+    w.material_.reserve(cell_capacity);
+    w.rows_.reserve(6 * cell_capacity);
   } catch (const std::bad_alloc&) {
     return Status(StatusCode::kResourceExhausted);
   }
@@ -127,6 +131,80 @@ StatusOr<MacPressureProjectionView3D> ProjectMacPressure3D(SparseMacGridView3D g
   auto rt = SharedHipRuntime3D::Acquire(need, lock);
   if (!rt.ok()) return rt.status();
   return HipProjectMacPressure3D(**rt, grid, step_duration, problem, options, workspace);
+}
+
+StatusOr<MacViscousPressureProjectionView3D> HipProjectUniformViscosity3D(
+    HipOperatorRuntime3D& runtime, const LiquidBoundaryGeometryViewInputs3D& geometry,
+    const LiquidPressureProblemView3D& problem, double dynamic_viscosity,
+    std::span<const double> cell_dynamic_viscosity, MacPressureSolverOptions3D options, MacPressureWorkspace3D& w) {
+  const std::size_t nc = problem.cells.size(), nf = problem.faces.size();
+  if (nc > w.cell_capacity_ || nf > w.face_capacity_) return Status(StatusCode::kResourceExhausted);
+  if (!cell_dynamic_viscosity.empty() && cell_dynamic_viscosity.size() != nc)
+    return Status(StatusCode::kInvalidArgument);
+  const auto grid = geometry.mac_grid;
+  const double width = grid.geometry().cell_width;
+  const double full_area = width * width, full_volume = full_area * width;
+  constexpr double kFullCellRoundoff = 64 * std::numeric_limits<double>::epsilon();
+  bool cut = false;
+  for (const auto& c : problem.cells) cut = cut || std::abs(c.fluid_volume / full_volume - 1.0) > kFullCellRoundoff;
+  for (const auto& f : problem.faces) cut = cut || (f.open_area != 0.0 && f.open_area != full_area);
+  if (cut) return Status(StatusCode::kUnimplemented);
+  w.cells_.clear();
+  w.faces_.clear();
+  w.material_.clear();
+  for (std::size_t i = 0; i < nc; ++i) {
+    const auto& c = problem.cells[i];
+    w.cells_.push_back(internal::ToC(c));
+    fl_viscosity_cell m{};
+    m.cell_index = {c.cell_index.x, c.cell_index.y, c.cell_index.z};
+    m.dynamic_viscosity = cell_dynamic_viscosity.empty() ? dynamic_viscosity : cell_dynamic_viscosity[i];
+    w.material_.push_back(m);
+  }
+  for (const auto& f : problem.faces) w.faces_.push_back(internal::ToC(f));
+  const fl_geometry g = internal::ToC(grid.geometry());
+  const auto blocks = grid.active_block_indices();
+  w.rows_.resize(6 * nc);
+  std::size_t rows = w.rows_.size();
+  fl_measurement m{};
+  int code = fl_assemble_full_cell_strain(runtime.handle(), &g, internal::ToC(blocks), blocks.size(),
+                                          w.material_.data(), nc, w.rows_.data(), &rows, &m);
+  if (code != FL_OK) return FromCode(code);
+  const HipMacPressureSolverSettings3D& st = w.settings_;
+  fl_pressure_options o{};
+  o.max_iterations = options.max_iterations;
+  o.cpu_residual_tolerance_scale = options.cpu_residual_tolerance_scale;
+  o.levels = st.levels;
+  o.smoothing_sweeps = st.smoothing_sweeps;
+  o.coarse_sweeps = st.coarse_sweeps;
+  o.level_storage = 1;
+  o.coarse_scale = st.coarse_scale;
+  o.smoother = st.smoother;
+  const int k = w.published_ == 0 ? 1 : 0;
+  auto& b = w.banks_[k];
+  b.pressures.resize(nc);
+  b.defects.resize(nc);
+  b.velocities.resize(nf);
+  fl_viscous_diagnostics d{};
+  code = fl_project_viscous(runtime.handle(), &g, internal::ToC(blocks), blocks.size(), geometry.step_duration,
+                            w.cells_.data(), nc, w.faces_.data(), nf, w.rows_.data(), rows, &o, b.pressures.data(),
+                            b.velocities.data(), b.defects.data(), &d, &m);
+  if (code != FL_OK) return FromCode(code);
+  w.measurement_.kernel_ms = m.kernel_ms;
+  w.measurement_.copy_ms = m.copy_ms;
+  w.measurement_.launches = m.launches;
+  std::memcpy(w.measurement_.architecture, m.architecture, sizeof(w.measurement_.architecture));
+  w.published_ = k;
+  MacViscousPressureProjectionView3D v;
+  v.cell_pressures = b.pressures;
+  v.face_velocities = b.velocities;
+  v.cell_constraint_defects = b.defects;
+  v.pressure_iterations = d.pressure_iterations;
+  v.constraint_residual_l2 = d.constraint_residual_l2;
+  v.initial_constraint_residual_l2 = d.initial_constraint_residual_l2;
+  v.max_cell_constraint_defect = d.max_cell_constraint_defect;
+  v.max_momentum_residual = d.max_momentum_residual;
+  v.inward_prescribed_wall_flux_m3_per_s = d.inward_prescribed_wall_flux;
+  return v;
 }
 
 }  // namespace fluidity

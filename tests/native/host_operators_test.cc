@@ -380,7 +380,7 @@ void SubstepTableDrivesStagesAgainstOracle() {
   (*context)->projection_settings.relative_divergence_target = 1e-9;
   (*context)->projection_settings.levels = 3;
   const HipLiquidSubstepBackend3D table = MakeHipLiquidSubstepBackend3D(**context);
-  EXPECT(table.sample_field == nullptr && table.project_viscous == nullptr);
+  EXPECT(table.sample_field == nullptr && table.project_viscous != nullptr);
   auto step_ws = LiquidStepWorkspace3D::Create(caps, kGeometry);
   auto boundary_ws = LiquidBoundaryWorkspace3D::Create(caps);
   auto pressure_ws = MacPressureWorkspace3D::Create(caps.cells, 6 * caps.cells);
@@ -539,6 +539,50 @@ void SubstepTableDrivesStagesAgainstOracle() {
          1e-12 * wdiag.rhs_l2);
   std::printf("  table projection: %zu cells, %zu iterations, pressure error %.2e of %.3e Pa\n", oc.size(),
               projection->diagnostics.iteration_count, perr, pmax);
+
+  // The viscous projection member (0.5 Pa s) against the oracle's coupled solve (SolveCoupled with the
+  // reference's Jacobi diagonal) on full-cell strain rows: the links' problem with every cell made full
+  // (the box's high corner cells are cut, which the member refuses as Unimplemented, below).
+  {
+    const double mu = 0.5, full = h * h * h;
+    const LiquidBoundaryGeometryViewInputs3D gin{grid, candidates, centers, corners, {}, dt, rho, 0.0};
+    std::vector<MacPressureCell3D> full_cells(problem->cells.begin(), problem->cells.end());
+    for (auto& c : full_cells) c.fluid_volume = full;
+    const LiquidPressureProblemView3D uncut{full_cells, problem->faces};
+    auto viscous = table.project_viscous(table.context, gin, uncut, mu, {}, {256, 1.0}, *pressure_ws);
+    if (!viscous.ok()) std::printf("  viscous projection status %d\n", static_cast<int>(viscous.status().code()));
+    EXPECT(viscous.ok());
+    std::vector<flo_pressure_cell> voc(oc);
+    for (auto& c : voc) c.fluid_volume = full;
+    std::vector<flo_viscosity_cell> vc;
+    for (const auto& c : problem->cells) vc.push_back({{c.cell_index.x, c.cell_index.y, c.cell_index.z}, 0, mu});
+    std::vector<flo_strain_row> rows(6 * vc.size());
+    std::size_t nr = 0;
+    EXPECT(flo_assemble_full_cell_strain(O(&kGeometry), O(blocks), blocks.size(), vc.data(), vc.size(), rows.data(),
+                                         rows.size(), &nr) == 0);
+    std::vector<double> vp(voc.size()), vu(of.size()), vd(voc.size());
+    flo_viscous_diagnostics vdiag{};
+    EXPECT(flo_project_viscous(O(&kGeometry), O(blocks), blocks.size(), dt, voc.data(), voc.size(), of.data(),
+                               of.size(), rows.data(), nr, 256, vp.data(), vu.data(), vd.data(), &vdiag) == 0);
+    if (viscous.ok()) {
+      double vmax = 0, verr = 0, umax = 0, uerr = 0;
+      for (std::size_t i = 0; i < vp.size(); ++i) {
+        vmax = std::max(vmax, std::fabs(vp[i]));
+        verr = std::max(verr, std::fabs(viscous->cell_pressures[i] - vp[i]));
+      }
+      for (std::size_t i = 0; i < vu.size(); ++i) {
+        umax = std::max(umax, std::fabs(vu[i]));
+        uerr = std::max(uerr, std::fabs(viscous->face_velocities[i] - vu[i]));
+      }
+      EXPECT(verr <= 1e-3 * vmax && uerr <= 1e-3 * umax);
+      EXPECT(viscous->max_cell_constraint_defect <= 2e-5);
+      std::printf("  table viscous projection: %zu rows, %zu GCR iterations (oracle %llu), pressure error %.2e of %.3e Pa\n",
+                  nr, viscous->pressure_iterations, static_cast<unsigned long long>(vdiag.pressure_iterations), verr,
+                  vmax);
+    }
+    EXPECT(table.project_viscous(table.context, gin, *problem, mu, {}, {256, 1.0}, *pressure_ws).status().code() ==
+           StatusCode::kUnimplemented);
+  }
 
   // boundary gather: the old history is the normalized field, the new one the predicted field with the
   // corrected pressure-face velocities
