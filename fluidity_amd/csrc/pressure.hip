@@ -362,6 +362,18 @@ __global__ void k_face_velocity(uint64_t nf, const int32_t* tail, const int32_t*
   u[f] = corrected(static_cast<int32_t>(f), tail, head, kind, area, ustar, scale, lift, p);
 }
 
+// IsRepresentablePressureProduct (mac_pressure_3d_arithmetic.h:33-49): a product of the evaluation is
+// usable when it is finite and, if it fell below the normal range with both factors nonzero, it is still
+// the exact product (checked on the factors' mantissas, with the product brought back to their scale).
+__device__ inline bool exact_product(double a, double b, double prod) {
+  if (!finite(prod)) return false;
+  if (a == 0.0 || b == 0.0 || fabs(prod) >= __DBL_MIN__) return true;
+  if (prod == 0.0) return false;
+  int ea = 0, eb = 0;
+  const double ma = frexp(a, &ea), mb = frexp(b, &eb);
+  return fma(ma, mb, -scalbn(prod, -(ea + eb))) == 0.0;
+}
+
 // Partial reductions: 0 sum r^2, 1 max |r|/V, 2 max |defect|, 3 sum defect^2 ; flag 4 identity failures.
 __device__ inline void block_reduce(double v[4], double* partial, unsigned slot) {
   __shared__ double red[4][kThreads / 64];
@@ -403,32 +415,46 @@ __global__ void k_evaluate(uint64_t nc, const int32_t* cell_faces, const FaceRec
     const int32_t c = static_cast<int32_t>(i);
     Row op, phys;
     const double pc = p[c];
-    op.add(comp[c] * pc);
+    const double cflux = comp[c] * pc;
+    bool exact = exact_product(comp[c], pc, cflux);
+    op.add(cflux);
     phys.add(-target[c]);
     phys.add(-inwall[c]);
-    if (comp[c] > 0.0) phys.add(comp[c] * (pc - prev[c]));
+    if (comp[c] > 0.0) {
+      const double dp = pc - prev[c], dflux = comp[c] * dp;
+      exact = exact && exact_product(comp[c], dp, dflux);
+      phys.add(dflux);
+    }
     for (int j = 0; j < 6; ++j) {
       const FaceRec& f = faces[cell_faces[6 * c + j]];
       const double sign = f.tail == c ? 1.0 : -1.0;
       const double pt_op = f.tail >= 0 ? p[f.tail] : 0.0;
       const double ph_op = f.head >= 0 ? p[f.head] : 0.0;
-      if (f.k != 0.0) op.add(sign * (f.k * (pt_op - ph_op)));
+      if (f.k != 0.0) {
+        const double dpf = pt_op - ph_op, kflux = f.k * dpf;
+        exact = exact && exact_product(f.k, dpf, kflux);
+        op.add(sign * kflux);
+      }
       double u = f.ustar;
       if (f.area > 0.0) {
         const double ext = f.kind == 2 ? f.lift : 0.0;
         const double jump = f.kind == 1 ? f.lift : 0.0;
         const double pt = f.tail >= 0 ? pt_op : ext;
         const double ph = f.head >= 0 ? ph_op : ext;
-        u += f.scale * (pt - ph + jump);
+        const double lifted = pt - ph + jump, du = f.scale * lifted;
+        exact = exact && exact_product(f.scale, lifted, du);
+        u += du;
       }
-      phys.add(sign * (f.area * u));
+      const double aflux = f.area * u;
+      exact = exact && finite(u) && exact_product(f.area, u, aflux);
+      phys.add(sign * aflux);
       phys.add(sign * f.wall);
     }
     const double r = rhs[c] - op.flux.total();
     const double d = phys.flux.total() / V[c];
     const double ie = V[c] * d + r;
     const double terms = rhs_abs[c] + op.abs.total() + phys.abs.total();
-    if (!finite(r) || !finite(d) || !finite(ie) || !finite(terms)) perr_report(err, c, kOutOfRange);
+    if (!exact || !finite(r) || !finite(d) || !finite(ie) || !finite(terms)) perr_report(err, c, kOutOfRange);
     else if (fabs(ie) > 64.0 * kEpsD * terms) perr_report(err, c, kInternal);
     res[c] = r;
     def[c] = d;
@@ -479,7 +505,12 @@ __global__ void k_rhs_zero_eval(uint64_t nc, const int32_t* cell_faces, const Fa
     row.add(comp[c] * prev[c]);
     phys.add(-target[c]);
     phys.add(-inwall[c]);
-    if (comp[c] > 0.0) phys.add(comp[c] * (0.0 - prev[c]));
+    bool exact = true;
+    if (comp[c] > 0.0) {
+      const double dp = 0.0 - prev[c], dflux = comp[c] * dp;
+      exact = exact_product(comp[c], dp, dflux);
+      phys.add(dflux);
+    }
     for (int j = 0; j < 6; ++j) {
       const FaceRec& f = faces[cell_faces[6 * c + j]];
       const double sign = f.tail == c ? 1.0 : -1.0;
@@ -493,9 +524,13 @@ __global__ void k_rhs_zero_eval(uint64_t nc, const int32_t* cell_faces, const Fa
         const double jump = f.kind == 1 ? f.lift : 0.0;
         const double pt = f.tail >= 0 ? 0.0 : ext;
         const double ph = f.head >= 0 ? 0.0 : ext;
-        u += f.scale * (pt - ph + jump);
+        const double lifted = pt - ph + jump, du = f.scale * lifted;
+        exact = exact && exact_product(f.scale, lifted, du);
+        u += du;
       }
-      phys.add(sign * (f.area * u));
+      const double aflux = f.area * u;
+      exact = exact && finite(u) && exact_product(f.area, u, aflux);
+      phys.add(sign * aflux);
       phys.add(sign * f.wall);
     }
     const double b = row.flux.total(), babs = row.abs.total();
@@ -505,7 +540,7 @@ __global__ void k_rhs_zero_eval(uint64_t nc, const int32_t* cell_faces, const Fa
     const double d = phys.flux.total() / V[c];
     const double ie = V[c] * d + r;
     const double terms = babs + 0.0 + phys.abs.total();
-    if (!finite(r) || !finite(d) || !finite(ie) || !finite(terms)) perr_report(err, c, kOutOfRange);
+    if (!exact || !finite(r) || !finite(d) || !finite(ie) || !finite(terms)) perr_report(err, c, kOutOfRange);
     else if (fabs(ie) > 64.0 * kEpsD * terms) perr_report(err, c, kInternal);
     res[c] = r;
     def[c] = d;
